@@ -1,0 +1,230 @@
+/* rrea_hip.h -- C-ABI of the MI355X-native RREA transport step.
+ *
+ * The drop-in boundary for the reference's per-step hook
+ *   RreaParticleInteraction::Apply(WarpX&, RreaWarpXCoupling&,
+ *       RreaInteractionTables const&, int step, Real time_s, Real dt_s)
+ *   (native_amrex_rrea/warpx_overlay/Source/Rrea/RreaParticleInteraction.H:1609-1623,
+ *    RreaParticleInteraction.cpp:12-62)
+ * and for the services it reads from the coupling and the table library.
+ * Plain C: pointers and sizes, no torch or AMReX types.  Every entry point
+ * returns an rrea_status (0 = success); device-side failures are reported
+ * through the step's error word and nothing is committed by the host glue on
+ * error, as the reference fails closed before any commit.
+ *
+ * Threading: stateless apart from the handles; one HIP stream per call; no
+ * internal threads.  Calls on one handle must not overlap.
+ */
+#ifndef RREA_HIP_H
+#define RREA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rrea_status;
+typedef struct rrea_tables_s* rrea_tables_t;
+typedef struct rrea_fields_s* rrea_fields_t;
+typedef void* rrea_stream_t; /* a hipStream_t; NULL = the default stream */
+
+enum {
+    RREA_OK = 0,
+    RREA_ERR_TABLE_RANGE = 1,    /* a certified-range table query: the reference Aborts */
+    RREA_ERR_OUTSIDE_DOMAIN = 2, /* a survivor stored outside the domain */
+    RREA_ERR_GUARD = 3,          /* strict optical-depth subcycle guard */
+    RREA_ERR_INVALID = 4,        /* any other fail-closed invariant */
+    RREA_ERR_SECONDARY_CAPACITY = 5,
+    RREA_ERR_ARGUMENT = 16,      /* host-side argument check */
+    RREA_ERR_HIP = 17,           /* a HIP runtime call failed */
+};
+
+/* ------------------------------------------------------------------------
+ * Tables.  Replaces RreaInteractionTables::Load + its accessors
+ * (native_amrex_rrea/src/RreaInteractionTables.cpp:1659-2230, :2272-3265).
+ * The host loader fills this image with the bundle's columns exactly as Load
+ * groups them; rrea_tables_create derives the node logarithms, the
+ * hard-Moller mean secondary energy (Load :2146-2179) and the brems CDF
+ * group means on the host, then uploads everything to the current device.
+ * ------------------------------------------------------------------------ */
+typedef struct rrea_rate_column {
+    const double* x;   /* energy nodes [eV], nondecreasing */
+    const double* y;   /* values */
+    int64_t n;
+    int32_t mode;      /* 0 loglog, 1 loglog with zero threshold, 2 logx-linear, 3 linear */
+} rrea_rate_column;
+
+typedef struct rrea_cdf_groups {
+    const double* primary; /* group primary energies, strictly increasing */
+    int64_t n_groups;
+    const int64_t* start;  /* n_groups + 1 row offsets into u/v */
+    const double* u;       /* cdf nodes per group, strictly increasing */
+    const double* v;       /* sampled values */
+} rrea_cdf_groups;
+
+typedef struct rrea_angle_table {
+    const double* e; int64_t n_e;  /* energy grid */
+    const double* n; int64_t n_n;  /* path n = s / lambda_el grid */
+    const double* u; int64_t n_u;  /* shared cdf nodes */
+    const double* omc;             /* one_minus_cos, (ie*n_n + ip)*n_u + iu */
+} rrea_angle_table;
+
+typedef struct rrea_elastic_table {
+    const double* e;
+    const double* mfp_m_stp;
+    const double* screening_eta;
+    int64_t n;
+} rrea_elastic_table;
+
+typedef struct rrea_table_image {
+    /* e- collision, soft radiative, hard-Moller 1/m, tracked brems 1/m, elastic transport 1/m,
+       e+ collision, soft radiative, hard-Bhabha, tracked brems, elastic transport, annihilation,
+       gamma Compton, photoelectric, pair nuclear, pair triplet */
+    rrea_rate_column rate[15];
+    rrea_cdf_groups moller, bhabha, electron_brems, positron_brems;
+    rrea_angle_table gs_electron, gs_positron, wentzel_electron, wentzel_positron;
+    rrea_elastic_table elastic_electron, elastic_positron;
+    double hard_moller_secondary_threshold_eV;
+    double brems_absolute_cutoff_eV;
+    double reference_density_kg_m3;
+    double charged_energy_range_eV[2];
+    double photon_energy_range_eV[2];
+    double gs_few_collision_threshold_n;
+    double gs_isotropic_above_path_n;
+    double gs_wentzel_transition_energy_eV;
+} rrea_table_image;
+
+rrea_status rrea_tables_create(const rrea_table_image* image, rrea_tables_t* out);
+rrea_status rrea_tables_destroy(rrea_tables_t tables);
+/* Host copy of the derived hard-Moller mean secondary energy on the rate grid
+ * (for checks); n must equal the hard-Moller rate column length. */
+rrea_status rrea_tables_hm_mean(rrea_tables_t tables, double* out, int64_t n);
+
+/* ------------------------------------------------------------------------
+ * Fields and density.  Replaces RreaWarpXCoupling::GatherFieldRZ
+ * (warpx_overlay/Source/Rrea/RreaWarpXCoupling.cpp:556-576) over the
+ * replicated Maxwell + ambient state of RreaAmrexAdvance::GatherTotalFieldRZ
+ * (native_amrex_rrea/src/RreaAmrexAdvance.cpp:532-579), and
+ * TransportDensityRatioAtZ (RreaWarpXCoupling.cpp:79-95).
+ * ------------------------------------------------------------------------ */
+typedef struct rrea_rz_geometry {
+    int32_t nr, nz;     /* physical cells */
+    double dr, dz;
+    double r_lo;        /* must be 0: the axis */
+    double z_lo;
+} rrea_rz_geometry;
+
+rrea_status rrea_fields_create(const rrea_rz_geometry* geometry, rrea_fields_t* out);
+rrea_status rrea_fields_destroy(rrea_fields_t fields);
+/* Er/Ez faces of the Maxwell state and of the ambient field, Btheta nodes, in
+ * the RreaMaxwellTMRZ layouts: er (nr+1)*nz, ez nr*(nz+1), btheta
+ * (nr+1)*(nz+1).  `device_pointers` says whether they are device memory.  The
+ * cell-centred totals the gather reads are formed on the device, in the
+ * reference's order of operations. */
+rrea_status rrea_fields_update(rrea_fields_t fields, const double* er, const double* ez,
+                               const double* amb_er, const double* amb_ez, const double* btheta,
+                               int32_t device_pointers, rrea_stream_t stream);
+rrea_status rrea_density_set_constant(rrea_fields_t fields, double density_ratio);
+/* Absolute density [kg/m^3] against altitude [m]; the ratio divides by the
+ * table set's reference density.  altitude(z) = altitude_at_z0_m + z. */
+rrea_status rrea_density_set_profile(rrea_fields_t fields, const double* altitude_m, const double* rho_kg_m3,
+                                     int64_t n, double altitude_at_z0_m, double reference_density_kg_m3);
+
+/* ------------------------------------------------------------------------
+ * One transport step of one species' tile.
+ * Replaces the electron phase of RreaParticleInteraction::Apply,
+ * TransportStep::ApplyElectronTransport
+ * (warpx_overlay/Source/Rrea/RreaElectronTransport.cpp:9-902), including the
+ * net-chord replay into the padded SegmentGridRZ
+ * (RreaWarpXCoupling.cpp:417-544).
+ * ------------------------------------------------------------------------ */
+typedef struct rrea_step_config {
+    uint64_t rng_seed;
+    int64_t step;
+    double dt_s;
+    double low_energy_cutoff_eV;
+    double photon_cutoff_eV;
+    double max_optical_depth_per_substep;
+    uint64_t max_subcycles;
+    int32_t strict_transport_guards;
+    int32_t disable_hard_moller;
+    double max_gs_collisions_per_substep;
+    double gs_total_collision_cap;
+    double max_field_impulse_fraction;
+    double field_impulse_floor_eV;
+} rrea_step_config;
+
+/* Device SoA of one tile (the ChargedSoAView of RreaTransportStep.H:454-468).
+ * prev_* is the pre-push snapshot the step reads; r..uz receive the
+ * written-back state and may alias prev_* (fused in-place mode). */
+typedef struct rrea_tile {
+    int64_t n;
+    const double* prev_r; const double* prev_z; const double* prev_theta;
+    const double* prev_ux; const double* prev_uy; const double* prev_uz;
+    const double* w;
+    int64_t* id;         /* in/out: an invalidated particle gets RREA_INVALID_ID and is
+                            skipped by later steps (particle_id.make_invalid() /
+                            is_valid(), RreaElectronTransport.cpp:60-63) */
+    const int32_t* cpu;  /* may be NULL: cpu 0 */
+    double* r; double* z; double* theta; double* ux; double* uy; double* uz;
+} rrea_tile;
+
+/* One staged secondary, before the newborn residual advance
+ * (RreaSecondaryParticle, RreaParticleInteraction.H:612-635). */
+typedef struct rrea_secondary {
+    int64_t parent_index;
+    int32_t species;        /* 0 e-, 1 e+, 2 photon */
+    int32_t process;        /* RreaSecondaryProcess id: 2 hard Moller, 4 brems, ... */
+    double x, y, z;
+    double dir_x, dir_y, dir_z;
+    double weight;
+    double energy_eV;
+    double birth_fraction;  /* elapsed fraction of the outer step at birth */
+    uint64_t root_key;      /* |id| ^ (cpu << 40) of the parent */
+    uint64_t lineage;       /* per-parent spawn counter */
+} rrea_secondary;
+
+enum {
+    RREA_SURVIVE = 0,
+    RREA_ESCAPE = 1,
+    RREA_DEMOTED = 2,
+    RREA_ZERO_WEIGHT = 3,
+    RREA_NO_WRITEBACK = 4,
+    RREA_SKIPPED_INVALID = 5,
+};
+
+#define RREA_INVALID_ID INT64_MIN
+
+#define RREA_TALLY_COUNT 16
+
+typedef struct rrea_step_out {
+    uint8_t* status;          /* per particle, RREA_SURVIVE.. (may be NULL) */
+    double* ion_pairs;        /* per particle ion-pair weight (may be NULL) */
+    int32_t* substeps;        /* per particle substep count (may be NULL) */
+    rrea_secondary* secondaries; int64_t secondary_capacity;
+    int64_t* secondary_count; /* device int64; the step adds to it */
+    double* jr; double* jz; double* rho; /* padded net-chord grid, accumulated */
+    double* tallies;          /* device double[RREA_TALLY_COUNT], accumulated */
+    int32_t* error;           /* device int32[2]: first error code, particle index */
+    int32_t order_secondaries;/* nonzero: sort the staged secondaries by (parent, lineage) */
+} rrea_step_out;
+
+rrea_status rrea_transport_step_electrons(rrea_tables_t tables, rrea_fields_t fields, const rrea_tile* tile,
+                                          const rrea_step_config* config, const rrea_step_out* out,
+                                          rrea_stream_t stream);
+
+/* Kernel timing with HIP events recorded around every transport-kernel
+ * launch made through `fields` while enabled (no counterpart in the reference:
+ * measurement support).  _read waits for the last launch, returns the summed
+ * kernel milliseconds and the launch count, and clears them. */
+rrea_status rrea_kernel_timing(rrea_fields_t fields, int32_t enable);
+rrea_status rrea_kernel_timing_read(rrea_fields_t fields, double* total_ms, int64_t* launches);
+
+/* The library version string. */
+const char* rrea_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RREA_HIP_H */

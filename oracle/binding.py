@@ -1,0 +1,312 @@
+"""ctypes binding of the CPU oracle (``oracle/build/liboracle.so``) and of
+the reference-compiled pins (``oracle/_ref/libref.so``) -- test infrastructure.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+
+P_D = C.POINTER(C.c_double)
+P_I64 = C.POINTER(C.c_int64)
+P_I32 = C.POINTER(C.c_int32)
+
+
+class Geometry(C.Structure):
+    _fields_ = [
+        ("nr", C.c_int32), ("nz", C.c_int32), ("dr", C.c_double), ("dz", C.c_double),
+        ("r_lo", C.c_double), ("z_lo", C.c_double),
+        ("er", P_D), ("ez", P_D), ("amb_er", P_D), ("amb_ez", P_D), ("btheta", P_D),
+        ("density_mode", C.c_int32), ("profile_n", C.c_int32), ("density_ratio", C.c_double),
+        ("profile_alt_m", P_D), ("profile_rho", P_D), ("altitude_at_z0_m", C.c_double),
+    ]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("rng_seed", C.c_uint64), ("step", C.c_int64), ("dt_s", C.c_double),
+        ("low_energy_cutoff_eV", C.c_double), ("photon_cutoff_eV", C.c_double),
+        ("max_optical_depth_per_substep", C.c_double), ("max_subcycles", C.c_uint64),
+        ("strict_transport_guards", C.c_int32), ("disable_hard_moller", C.c_int32),
+        ("max_gs_collisions_per_substep", C.c_double), ("gs_total_collision_cap", C.c_double),
+        ("max_field_impulse_fraction", C.c_double), ("field_impulse_floor_eV", C.c_double),
+    ]
+
+
+class Secondary(C.Structure):
+    _fields_ = [
+        ("parent_index", C.c_int64), ("species", C.c_int32), ("process", C.c_int32),
+        ("x", C.c_double), ("y", C.c_double), ("z", C.c_double),
+        ("dir_x", C.c_double), ("dir_y", C.c_double), ("dir_z", C.c_double),
+        ("weight", C.c_double), ("energy_eV", C.c_double), ("birth_fraction", C.c_double),
+        ("root_key", C.c_uint64), ("lineage", C.c_uint64),
+    ]
+
+
+SECONDARY_DTYPE = np.dtype([
+    ("parent_index", "<i8"), ("species", "<i4"), ("process", "<i4"),
+    ("x", "<f8"), ("y", "<f8"), ("z", "<f8"),
+    ("dir_x", "<f8"), ("dir_y", "<f8"), ("dir_z", "<f8"),
+    ("weight", "<f8"), ("energy_eV", "<f8"), ("birth_fraction", "<f8"),
+    ("root_key", "<u8"), ("lineage", "<u8"),
+])
+assert SECONDARY_DTYPE.itemsize == C.sizeof(Secondary)
+
+RATE_KEYS = ["e_col", "e_soft", "e_moller", "e_brems", "e_eltr",
+             "p_col", "p_soft", "p_bhabha", "p_brems", "p_eltr", "p_annih",
+             "g_compton", "g_pe", "g_pairn", "g_pairt"]
+CDF_KEYS = ["moller", "bhabha", "ebrems", "pbrems"]
+ANGLE_KEYS = ["gs_e", "gs_p", "wz_e", "wz_p"]
+N_TALLIES = 16
+
+
+def build(quiet: bool = True) -> None:
+    """Compile the oracle (and the reference pins when the checkout exists)."""
+    out = subprocess.DEVNULL if quiet else None
+    subprocess.check_call(["make", "-C", HERE, "all"], stdout=out)
+    if os.path.isdir("/root/reference/native_amrex_rrea"):
+        subprocess.check_call(["make", "-C", HERE, "ref"], stdout=out)
+
+
+def _ptr(a: np.ndarray, ctype=P_D):
+    return a.ctypes.data_as(ctype)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_rng_counter.restype = C.c_uint64
+        L.orc_rng_counter.argtypes = [C.c_uint64] * 5
+        L.orc_rng_uniform.restype = C.c_double
+        L.orc_rng_uniform.argtypes = [C.c_uint64] * 5
+        L.orc_uniform_stream.argtypes = [C.c_double, C.c_double, C.c_int64, P_D]
+        L.orc_tables_create.restype = C.c_void_p
+        L.orc_tables_destroy.argtypes = [C.c_void_p]
+        L.orc_tables_set_rate.argtypes = [C.c_void_p, C.c_int, P_D, P_D, C.c_int64, C.c_int]
+        L.orc_tables_set_cdf.argtypes = [C.c_void_p, C.c_int, P_D, C.c_int64, P_I64, P_D, P_D]
+        L.orc_tables_set_angle.argtypes = [C.c_void_p, C.c_int, P_D, C.c_int64, P_D, C.c_int64, P_D, C.c_int64, P_D]
+        L.orc_tables_set_elastic.argtypes = [C.c_void_p, C.c_int, P_D, P_D, P_D, P_D, C.c_int64]
+        L.orc_tables_finalize.argtypes = [C.c_void_p, P_D]
+        L.orc_tables_finalize.restype = C.c_int
+        L.orc_tables_hm_mean.argtypes = [C.c_void_p]
+        L.orc_tables_hm_mean.restype = P_D
+        for name, extra in [("orc_interpolate", [C.c_int, C.c_double]),
+                            ("orc_sample_moller", [C.c_double, C.c_double]),
+                            ("orc_sample_brems_energy", [C.c_double, C.c_double]),
+                            ("orc_sample_brems_cos", [C.c_double, C.c_double]),
+                            ("orc_sample_gs_cos", [C.c_int, C.c_double, C.c_double, C.c_double]),
+                            ("orc_sample_single_elastic_cos", [C.c_int, C.c_double, C.c_double]),
+                            ("orc_elastic_mfp", [C.c_int, C.c_double]),
+                            ("orc_ensemble_drag", [C.c_double, C.c_double]),
+                            ("orc_hard_moller_energy_loss", [C.c_double])]:
+            f = getattr(L, name)
+            f.restype = C.c_double
+            f.argtypes = [C.c_void_p] + extra + [P_I32]
+        L.orc_clip_path.argtypes = [C.c_double] * 10 + [P_D]
+        L.orc_clip_path.restype = C.c_int
+        L.orc_gather.argtypes = [C.POINTER(Geometry), C.c_double, C.c_double, P_D]
+        L.orc_deposit_segment.argtypes = [C.c_int, C.c_int] + [C.c_double] * 4 + [P_D] * 3 + [C.c_double] * 6
+        L.orc_deposit_escaping_segment.argtypes = ([C.c_int, C.c_int] + [C.c_double] * 4 + [P_D] * 3
+                                                   + [C.c_double] * 2 + [P_D, C.c_int, C.c_int, C.c_int, C.c_uint32])
+        L.orc_deposit_escaping_segment.restype = C.c_int
+        L.orc_electron_step.restype = C.c_int
+        L.orc_electron_step.argtypes = ([C.c_void_p, C.POINTER(Geometry), C.POINTER(Config), C.c_int64]
+                                        + [P_D] * 7 + [P_I64, P_I32] + [P_D] * 6 + [P_I32, P_D, P_I64]
+                                        + [C.c_int64, C.POINTER(Secondary), P_I64] + [P_D] * 4 + [C.c_int])
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference-compiled pins, or None where the reference is absent."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_PATH):
+            if not os.path.isdir("/root/reference/native_amrex_rrea"):
+                return None
+            subprocess.check_call(["make", "-C", HERE, "ref"], stdout=subprocess.DEVNULL)
+        R = C.CDLL(REF_PATH)
+        R.ref_rng_counter.restype = C.c_uint64
+        R.ref_rng_counter.argtypes = [C.c_uint64] * 5
+        R.ref_rng_uniform.restype = C.c_double
+        R.ref_rng_uniform.argtypes = [C.c_uint64] * 5
+        R.ref_uniform_stream.argtypes = [C.c_double, C.c_double, C.c_int64, P_D]
+        R.ref_gl2_cubic.restype = C.c_double
+        R.ref_gl2_cubic.argtypes = [C.c_double, C.c_double, P_D]
+        R.ref_deposit_segment.argtypes = [C.c_int, C.c_int] + [C.c_double] * 4 + [P_D] * 3 + [C.c_double] * 6
+        R.ref_deposit_escaping_segment.argtypes = ([C.c_int, C.c_int] + [C.c_double] * 4 + [P_D] * 3
+                                                   + [C.c_double] * 2 + [P_D, C.c_int, C.c_int, C.c_int, C.c_uint32])
+        R.ref_deposit_escaping_segment.restype = C.c_int
+        _ref = R
+    return _ref
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Tables:
+    """The oracle's own schema-6 table set, filled from the bundle arrays."""
+
+    def __init__(self, bundle: dict):
+        L = lib()
+        self._keep = []
+        self.h = L.orc_tables_create()
+        for i, key in enumerate(RATE_KEYS):
+            x = np.ascontiguousarray(bundle[key + "_x"], dtype=np.float64)
+            y = np.ascontiguousarray(bundle[key + "_y"], dtype=np.float64)
+            L.orc_tables_set_rate(self.h, i, _ptr(x), _ptr(y), len(x), int(bundle[key + "_mode"]))
+        for i, key in enumerate(CDF_KEYS):
+            prim = np.ascontiguousarray(bundle[key + "_primary"])
+            start = np.ascontiguousarray(bundle[key + "_start"], dtype=np.int64)
+            u = np.ascontiguousarray(bundle[key + "_u"])
+            v = np.ascontiguousarray(bundle[key + "_v"])
+            L.orc_tables_set_cdf(self.h, i, _ptr(prim), len(prim), _ptr(start, P_I64), _ptr(u), _ptr(v))
+        for i, key in enumerate(ANGLE_KEYS):
+            e = np.ascontiguousarray(bundle[key + "_e"])
+            n = np.ascontiguousarray(bundle[key + "_n"])
+            u = np.ascontiguousarray(bundle[key + "_u"])
+            omc = np.ascontiguousarray(bundle[key + "_omc"])
+            L.orc_tables_set_angle(self.h, i, _ptr(e), len(e), _ptr(n), len(n), _ptr(u), len(u), _ptr(omc))
+        for i, key in enumerate(["el_e", "el_p"]):
+            arrs = [np.ascontiguousarray(bundle[key + s]) for s in ("_e", "_mfp", "_eta", "_trmfp")]
+            L.orc_tables_set_elastic(self.h, i, *[_ptr(a) for a in arrs], len(arrs[0]))
+        scalars = np.ascontiguousarray(bundle["scalars"], dtype=np.float64)
+        code = L.orc_tables_finalize(self.h, _ptr(scalars))
+        if code:
+            raise OracleError(f"oracle table finalize failed: {code}")
+        self.scalars = scalars
+        self.n_moller = len(bundle["e_moller_x"])
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_tables_destroy(self.h)
+            self.h = None
+
+    def hm_mean(self) -> np.ndarray:
+        p = lib().orc_tables_hm_mean(self.h)
+        return np.ctypeslib.as_array(p, shape=(self.n_moller,)).copy()
+
+    def _call(self, name, *args):
+        err = C.c_int32(0)
+        v = getattr(lib(), name)(self.h, *args, C.byref(err))
+        if err.value:
+            raise OracleError(f"{name} failed with code {err.value}")
+        return v
+
+    def interpolate(self, table: str, e: float) -> float:
+        return self._call("orc_interpolate", RATE_KEYS.index(table), e)
+
+    def sample_moller(self, e, u):
+        return self._call("orc_sample_moller", e, u)
+
+    def sample_brems_energy(self, e, u):
+        return self._call("orc_sample_brems_energy", e, u)
+
+    def sample_brems_cos(self, e, u):
+        return self._call("orc_sample_brems_cos", e, u)
+
+    def sample_gs_cos(self, positron, e, n, u):
+        return self._call("orc_sample_gs_cos", int(positron), e, n, u)
+
+    def sample_single_elastic_cos(self, positron, e, u):
+        return self._call("orc_sample_single_elastic_cos", int(positron), e, u)
+
+    def elastic_mfp(self, positron, e):
+        return self._call("orc_elastic_mfp", int(positron), e)
+
+    def ensemble_drag(self, e, ratio):
+        return self._call("orc_ensemble_drag", e, ratio)
+
+    def hard_moller_energy_loss(self, e):
+        return self._call("orc_hard_moller_energy_loss", e)
+
+
+def make_geometry(fields) -> tuple[Geometry, list]:
+    """Pack a transport field/density description into the oracle struct."""
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _ptr(a)
+
+    g = Geometry()
+    g.nr, g.nz = fields.nr, fields.nz
+    g.dr, g.dz = fields.dr, fields.dz
+    g.r_lo, g.z_lo = 0.0, fields.z_lo
+    g.er, g.ez = arr(fields.er), arr(fields.ez)
+    g.amb_er, g.amb_ez = arr(fields.amb_er), arr(fields.amb_ez)
+    g.btheta = arr(fields.btheta)
+    g.density_mode = 1 if fields.profile_alt_m is not None else 0
+    g.density_ratio = fields.density_ratio
+    if fields.profile_alt_m is not None:
+        g.profile_n = len(fields.profile_alt_m)
+        g.profile_alt_m = arr(fields.profile_alt_m)
+        g.profile_rho = arr(fields.profile_rho)
+        g.altitude_at_z0_m = fields.altitude_at_z0_m
+    return g, keep
+
+
+def make_config(cfg) -> Config:
+    c = Config()
+    c.rng_seed = cfg.rng_seed
+    c.step = cfg.step
+    c.dt_s = cfg.dt_s
+    c.low_energy_cutoff_eV = cfg.low_energy_cutoff_eV
+    c.photon_cutoff_eV = cfg.photon_cutoff_eV
+    c.max_optical_depth_per_substep = cfg.max_optical_depth_per_substep
+    c.max_subcycles = cfg.max_subcycles
+    c.strict_transport_guards = int(cfg.strict_transport_guards)
+    c.disable_hard_moller = int(cfg.debug_disable_hard_moller_secondaries)
+    c.max_gs_collisions_per_substep = cfg.transport_max_gs_collisions_per_substep
+    c.gs_total_collision_cap = cfg.transport_gs_total_collision_cap
+    c.max_field_impulse_fraction = cfg.transport_max_field_impulse_fraction
+    c.field_impulse_floor_eV = cfg.transport_field_impulse_floor_eV
+    return c
+
+
+def electron_step(tables: Tables, fields, cfg, particles: dict, nthreads: int = 1, sec_cap: int | None = None):
+    """Run the oracle step.  ``particles`` holds prev r, z, theta, ux, uy, uz,
+    w (float64), id (int64) and cpu (int32).  Returns a dict of outputs."""
+    L = lib()
+    n = len(particles["r"])
+    g, keep = make_geometry(fields)
+    c = make_config(cfg)
+    ins = [np.ascontiguousarray(particles[k], dtype=np.float64) for k in ("r", "z", "theta", "ux", "uy", "uz", "w")]
+    pid = np.ascontiguousarray(particles["id"], dtype=np.int64)
+    cpu = np.ascontiguousarray(particles["cpu"], dtype=np.int32)
+    outs = {k: np.zeros(n) for k in ("r", "z", "theta", "ux", "uy", "uz")}
+    status = np.zeros(n, dtype=np.int32)
+    ion = np.zeros(n)
+    sub = np.zeros(n, dtype=np.int64)
+    cap = sec_cap if sec_cap is not None else max(1024, n)
+    sec = (Secondary * cap)()
+    nsec = C.c_int64(0)
+    nrp, nzp = fields.nr + 2, fields.nz + 4
+    jr = np.zeros((nrp + 1) * nzp)
+    jz = np.zeros(nrp * (nzp + 1))
+    rho = np.zeros(nrp * nzp)
+    tallies = np.zeros(N_TALLIES)
+    code = L.orc_electron_step(
+        tables.h, C.byref(g), C.byref(c), n, *[_ptr(a) for a in ins], _ptr(pid, P_I64), _ptr(cpu, P_I32),
+        *[_ptr(outs[k]) for k in ("r", "z", "theta", "ux", "uy", "uz")], _ptr(status, P_I32), _ptr(ion),
+        _ptr(sub, P_I64), cap, sec, C.byref(nsec), _ptr(jr), _ptr(jz), _ptr(rho), _ptr(tallies), nthreads)
+    del keep
+    secs = np.frombuffer(bytes(sec), dtype=SECONDARY_DTYPE, count=cap)[: nsec.value].copy()
+    return {"code": code, **outs, "status": status, "ion_pairs": ion, "substeps": sub,
+            "secondaries": secs, "jr": jr, "jz": jz, "rho": rho, "tallies": tallies}

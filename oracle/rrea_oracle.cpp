@@ -1,0 +1,2074 @@
+// CPU oracle -- TEST INFRASTRUCTURE ONLY.
+//
+// A plain C++17 restatement of the reference's per-step electron transport:
+// the algorithm of RreaParticleInteraction::Apply's electron phase
+// (native_amrex_rrea/warpx_overlay/Source/Rrea/RreaElectronTransport.cpp:9-902)
+// with every primitive it calls, each function citing the reference lines it
+// follows.  It is the checker of the HIP path, never part of it: the shipped
+// library does not link this file, and only tests/, the smoke check and the
+// CPU-baseline leg of bench.py load it.
+//
+// Abbreviations in citations:
+//   OV/ = native_amrex_rrea/warpx_overlay/Source/Rrea/
+//   NA/ = native_amrex_rrea/
+//
+// Scope notes (documented, deliberate):
+//   * non-periodic z only -- the reference's net-chord deposit refuses periodic
+//     z (OV/RreaWarpXCoupling.cpp:420-424), so the production step never runs
+//     with it;
+//   * r_lo = 0 (the RZ axis), the reference's startup contract;
+//   * C&D plane diagnostics are off (the reference default).
+//
+// Compile with -ffp-contract=off so that no multiply-add is fused.
+
+#include "rrea_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+constexpr double kMeC2 = 510998.95000;           // NA/include/rrea/RreaConstants.H:9
+constexpr double kC = 299792458.0;               // RreaConstants.H:10
+constexpr double kWAir = 33.97;                  // RreaConstants.H:30
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+struct OracleError : std::runtime_error {
+    int code;
+    OracleError(int c, const char* what) : std::runtime_error(what), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const char* what) { throw OracleError(code, what); }
+
+// ---------------------------------------------------------------------------
+// Counter RNG -- OV/RreaRng.cpp:9-44
+// ---------------------------------------------------------------------------
+inline uint64_t splitmix64(uint64_t v)
+{
+    v += 0x9e3779b97f4a7c15ULL;
+    v = (v ^ (v >> 30U)) * 0xbf58476d1ce4e5b9ULL;
+    v = (v ^ (v >> 27U)) * 0x94d049bb133111ebULL;
+    return v ^ (v >> 31U);
+}
+
+inline uint64_t rng_counter(uint64_t seed, uint64_t pid, uint64_t step, uint64_t idx, uint64_t chan)
+{
+    uint64_t s = splitmix64(seed);
+    s ^= splitmix64(pid + 0x100000001b3ULL);
+    s ^= splitmix64(step + 0x9e3779b97f4a7c15ULL);
+    s ^= splitmix64(idx + 0xbf58476d1ce4e5b9ULL);
+    s ^= splitmix64(chan + 0x94d049bb133111ebULL);
+    return splitmix64(s);
+}
+
+inline double rng_uniform(uint64_t seed, uint64_t pid, uint64_t step, uint64_t idx, uint64_t chan)
+{
+    return (static_cast<double>(rng_counter(seed, pid, step, idx, chan) >> 12U) + 0.5) * 0x1.0p-52;
+}
+
+// NA/include/rrea/RreaUniformStream.H:21-53
+struct UniformStream {
+    uint64_t state;
+    static uint64_t rotl(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+    UniformStream(double u1, double u2)
+    {
+        uint64_t b1, b2;
+        std::memcpy(&b1, &u1, 8);
+        std::memcpy(&b2, &u2, 8);
+        state = splitmix64(b1) ^ rotl(splitmix64(b2), 32);
+    }
+    double next()
+    {
+        state += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z = z ^ (z >> 31);
+        return static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0);
+    }
+};
+
+// RNG channel ids -- OV/RreaRng.H:16-98 (the subset the electron step uses)
+enum : uint64_t {
+    kChHardMollerEnergy = 22,
+    kChHardMollerAzimuth = 23,
+    kChBremsPhotonEnergy = 42,
+    kChBremsPhotonAngle = 43,
+    kChBremsPhotonAzimuth = 44,
+    kChGsAngle = 89,
+    kChGsAzimuth = 90,
+    kChGsFewCount = 91,
+    kChGsFewScatter = 92,
+    kChGsFewAzimuth = 93,
+    kChCompetingOpticalDepth = 107,
+    kChCompetingChannel = 108,
+    kChContinuationOpticalDepth = 124,
+    kChContinuationChannel = 125,
+};
+
+// OV/RreaParticleInteraction.H:642-710
+constexpr uint64_t kLineageCap = uint64_t{1} << 14;
+constexpr uint64_t kSegmentCap = uint64_t{1} << 14;
+constexpr uint64_t kEventCap = uint64_t{1} << 13;
+constexpr uint64_t kInnerCap = uint64_t{1} << 8;
+
+inline bool pack_transient_index(bool newborn, uint64_t species, uint64_t lineage, uint64_t segment,
+                                 uint64_t event, uint64_t inner, uint64_t& out)
+{
+    if (species >= 3U || lineage >= kLineageCap || segment >= kSegmentCap || event >= kEventCap
+        || inner >= kInnerCap) {
+        return false;
+    }
+    out = (static_cast<uint64_t>(newborn) << 51) | (species << 49) | (lineage << 35) | (segment << 21)
+        | (event << 8) | inner;
+    return true;
+}
+
+inline bool live_hazard_index(uint64_t species, uint64_t segment, uint64_t event, uint64_t& out)
+{
+    if (species >= 3U) return false;
+    if (event == 0U) {
+        if (segment >= kSegmentCap) return false;
+        out = segment;
+        return true;
+    }
+    return pack_transient_index(false, species, 1U, segment, event, 0U, out);
+}
+
+// ---------------------------------------------------------------------------
+// Schema-6 tables -- NA/src/RreaInteractionTables.cpp
+// ---------------------------------------------------------------------------
+enum Mode { LogLog = 0, LogLogZero = 1, LogXLinear = 2, Linear = 3 };
+
+struct RateTable {
+    std::vector<double> x, y, logx, logy;
+    int mode = LogLog;
+
+    // Finalize :1632-1657
+    void finalize()
+    {
+        if (mode == Linear) return;
+        logx.resize(x.size());
+        logy.resize(y.size());
+        for (size_t i = 0; i < x.size(); ++i) {
+            logx[i] = std::log(x[i]);
+            logy[i] = y[i] > 0.0 ? std::log(y[i]) : 0.0;
+        }
+    }
+
+    // Interpolate :1580-1630
+    double interpolate(double q) const
+    {
+        if (!std::isfinite(q) || q < x.front() || q > x.back()) {
+            fail(ORC_ERR_TABLE_RANGE, "interaction table query outside certified range");
+        }
+        auto const upper = std::lower_bound(x.begin(), x.end(), q);
+        if (upper == x.begin()) return y.front();
+        if (upper == x.end()) return y.back();
+        size_t const hi = static_cast<size_t>(upper - x.begin());
+        size_t const lo = hi - 1U;
+        if (x[hi] == q) return y[hi];
+        if (mode == Linear) {
+            double const t = (q - x[lo]) / (x[hi] - x[lo]);
+            return y[lo] + t * (y[hi] - y[lo]);
+        }
+        double const t = (std::log(q) - logx[lo]) / (logx[hi] - logx[lo]);
+        bool const positive = y[lo] > 0.0 && y[hi] > 0.0;
+        if (mode == LogXLinear || (mode == LogLogZero && !positive)) {
+            return y[lo] + t * (y[hi] - y[lo]);
+        }
+        if (!positive) fail(ORC_ERR_INVALID, "loglog_linear requires positive bracket");
+        return std::exp(logy[lo] + t * (logy[hi] - logy[lo]));
+    }
+};
+
+struct CdfGroups {
+    std::vector<double> primary;
+    std::vector<std::vector<double>> u, v;
+};
+
+struct AngleTable {
+    std::vector<double> e, n, u, omc;  // omc flattened (ie*np + ip)*nu + iu
+};
+
+struct ElasticTable {
+    std::vector<double> e, mfp, eta, trmfp;
+};
+
+enum RateId {
+    kECol = 0, kESoft, kEMoller, kEBrems, kEEltr,
+    kPCol, kPSoft, kPBhabha, kPBrems, kPEltr, kPAnnih,
+    kGCompton, kGPe, kGPairN, kGPairT, kRateCount
+};
+
+// interp_semilogx_strict :624-648
+inline double interp_semilogx_strict(std::vector<double> const& x, std::vector<double> const& y, double q)
+{
+    if (x.size() < 2) return y.empty() ? 0.0 : y.front();
+    if (!std::isfinite(q) || q < x.front() || q > x.back()) {
+        fail(ORC_ERR_TABLE_RANGE, "elastic-table query outside range");
+    }
+    if (q == x.front()) return y.front();
+    if (q == x.back()) return y.back();
+    auto const upper = std::lower_bound(x.begin(), x.end(), q);
+    size_t const hi = static_cast<size_t>(upper - x.begin());
+    size_t const lo = hi - 1U;
+    if (x[hi] == q) return y[hi];
+    double const t = (std::log(q) - std::log(x[lo])) / (std::log(x[hi]) - std::log(x[lo]));
+    return y[lo] + t * (y[hi] - y[lo]);
+}
+
+// log_bracket :807-829
+inline void log_bracket(std::vector<double> const& g, double q, int& i, double& frac)
+{
+    int const last = static_cast<int>(g.size()) - 1;
+    if (!std::isfinite(q) || q < g[0] || q > g[last]) {
+        fail(ORC_ERR_TABLE_RANGE, "elastic-angle query outside range");
+    }
+    if (q == g[0]) { i = 0; frac = 0.0; return; }
+    if (q == g[last]) { i = last - 1; frac = 1.0; return; }
+    int lo = 0, hi = last;
+    while (hi - lo > 1) { int const m = (lo + hi) / 2; if (g[m] <= q) lo = m; else hi = m; }
+    i = lo;
+    double const log_lo = std::log(g[lo]);
+    double const log_hi = std::log(g[hi]);
+    if (log_hi == log_lo) { frac = (q == g[lo]) ? 0.0 : 1.0; return; }
+    frac = (std::log(q) - log_lo) / (log_hi - log_lo);
+}
+
+// sample_gs_angle_cos :831-891
+inline double sample_gs_angle_cos(AngleTable const& t, double energy, double path_n, double u)
+{
+    int const np = static_cast<int>(t.n.size());
+    int const nu = static_cast<int>(t.u.size());
+    int ie = 0, ip = 0;
+    double te = 0.0, tn = 0.0;
+    log_bracket(t.e, energy, ie, te);
+    log_bracket(t.n, path_n, ip, tn);
+    auto cos_at_node = [&](int iu) {
+        auto val = [&](int e_i, int p_i) { return t.omc[(static_cast<size_t>(e_i) * np + p_i) * nu + iu]; };
+        double const la = std::log(val(ie, ip));
+        double const lb = std::log(val(ie + 1, ip));
+        double const lc = std::log(val(ie, ip + 1));
+        double const ld = std::log(val(ie + 1, ip + 1));
+        double const lx = (1 - te) * (1 - tn) * la + te * (1 - tn) * lb + (1 - te) * tn * lc + te * tn * ld;
+        return 1.0 - std::exp(lx);
+    };
+    auto omc_at = [&](int iu) { return std::max(1.0 - cos_at_node(iu), std::numeric_limits<double>::min()); };
+    int const first = 1;
+    int const last = nu - 2;
+    double one_minus_cos;
+    if (last <= first) {
+        one_minus_cos = omc_at(first);
+    } else if (u <= t.u[first]) {
+        double const d1 = omc_at(first);
+        double const d2 = omc_at(first + 1);
+        double const u1 = t.u[first];
+        double const u2 = t.u[first + 1];
+        double const slope = std::log(d2 / d1) / std::log(u2 / u1);
+        double const u_eff = std::max(u, u1 * 1.0e-6);
+        one_minus_cos = d1 * std::exp(slope * std::log(u_eff / u1));
+    } else if (u >= t.u[last]) {
+        one_minus_cos = omc_at(last);
+    } else {
+        int lo = first, hi = last;
+        while (hi - lo > 1) { int const m = (lo + hi) / 2; if (t.u[m] <= u) lo = m; else hi = m; }
+        double const d0 = omc_at(lo);
+        double const d1 = omc_at(hi);
+        double const fu = std::log(u / t.u[lo]) / std::log(t.u[hi] / t.u[lo]);
+        one_minus_cos = d0 * std::exp(fu * std::log(d1 / d0));
+    }
+    double const cos_val = 1.0 - std::min(2.0, one_minus_cos);
+    return std::min(1.0, std::max(-1.0, cos_val));
+}
+
+// sample_conditional_cdf :995-1069
+inline double sample_cdf_group(std::vector<double> const& gu, std::vector<double> const& gv, double cdf_u)
+{
+    auto const upper = std::lower_bound(gu.begin(), gu.end(), cdf_u);
+    if (upper == gu.begin()) return gv.front();
+    if (upper == gu.end()) return gv.back();
+    size_t const hi = static_cast<size_t>(upper - gu.begin());
+    size_t const lo = hi - 1U;
+    double const denom = gu[hi] - gu[lo];
+    double const t = denom > 0.0 ? (cdf_u - gu[lo]) / denom : 0.0;
+    return gv[lo] + t * (gv[hi] - gv[lo]);
+}
+
+inline double sample_conditional_cdf(CdfGroups const& g, double primary, double cdf_u, bool log_interp)
+{
+    cdf_u = std::max(0.0, std::min(1.0, cdf_u));
+    if (!std::isfinite(primary) || primary < g.primary.front() || primary > g.primary.back()) {
+        fail(ORC_ERR_TABLE_RANGE, "conditional-CDF primary energy outside range");
+    }
+    size_t const n = g.primary.size();
+    if (primary == g.primary.front()) return sample_cdf_group(g.u[0], g.v[0], cdf_u);
+    if (primary == g.primary.back()) return sample_cdf_group(g.u[n - 1], g.v[n - 1], cdf_u);
+    auto const upper = std::lower_bound(g.primary.begin(), g.primary.end(), primary);
+    if (upper == g.primary.begin()) return sample_cdf_group(g.u[0], g.v[0], cdf_u);
+    if (upper == g.primary.end()) return sample_cdf_group(g.u[n - 1], g.v[n - 1], cdf_u);
+    size_t const hi = static_cast<size_t>(upper - g.primary.begin());
+    size_t const lo = hi - 1U;
+    if (g.primary[hi] == primary) return sample_cdf_group(g.u[hi], g.v[hi], cdf_u);
+    double const y0 = sample_cdf_group(g.u[lo], g.v[lo], cdf_u);
+    double const y1 = sample_cdf_group(g.u[hi], g.v[hi], cdf_u);
+    double const t = (std::log(primary) - std::log(g.primary[lo])) / (std::log(g.primary[hi]) - std::log(g.primary[lo]));
+    if (log_interp && y0 > 0.0 && y1 > 0.0) return std::exp(std::log(y0) + t * (std::log(y1) - std::log(y0)));
+    return y0 + t * (y1 - y0);
+}
+
+// mean_conditional_cdf :1071-1110
+inline double cdf_group_mean(std::vector<double> const& gu, std::vector<double> const& gv)
+{
+    double mean = gu.front() * gv.front();
+    for (size_t i = 1; i < gu.size(); ++i) mean += 0.5 * (gv[i - 1] + gv[i]) * (gu[i] - gu[i - 1]);
+    mean += (1.0 - gu.back()) * gv.back();
+    return mean;
+}
+
+inline double mean_conditional_cdf(CdfGroups const& g, double primary)
+{
+    if (g.primary.empty() || primary < g.primary.front() || primary > g.primary.back()) {
+        fail(ORC_ERR_TABLE_RANGE, "conditional-CDF mean query outside range");
+    }
+    size_t const n = g.primary.size();
+    auto const upper = std::lower_bound(g.primary.begin(), g.primary.end(), primary);
+    if (upper == g.primary.begin()) return cdf_group_mean(g.u[0], g.v[0]);
+    if (upper == g.primary.end()) return cdf_group_mean(g.u[n - 1], g.v[n - 1]);
+    size_t const hi = static_cast<size_t>(upper - g.primary.begin());
+    if (g.primary[hi] == primary) return cdf_group_mean(g.u[hi], g.v[hi]);
+    size_t const lo = hi - 1U;
+    double const y0 = cdf_group_mean(g.u[lo], g.v[lo]);
+    double const y1 = cdf_group_mean(g.u[hi], g.v[hi]);
+    double const t = (std::log(primary) - std::log(g.primary[lo])) / (std::log(g.primary[hi]) - std::log(g.primary[lo]));
+    return y0 > 0.0 && y1 > 0.0 ? std::exp(std::log(y0) + t * (std::log(y1) - std::log(y0))) : y0 + t * (y1 - y0);
+}
+
+struct Tables {
+    RateTable rate[kRateCount];
+    CdfGroups moller, bhabha, ebrems, pbrems;
+    AngleTable gs_e, gs_p, wz_e, wz_p;
+    ElasticTable el_e, el_p;
+    std::vector<double> hm_mean_secondary;  // on the hard-Moller rate grid
+    double hard_moller_threshold = 0.0;
+    double brems_cutoff = 0.0;
+    double reference_density = 0.0;
+    double charged_min = 0.0, charged_max = 0.0, photon_min = 0.0, photon_max = 0.0;
+    double gs_few = 10.0, gs_iso = 1.0e5, gs_transition = 0.0;
+
+    void require_charged(double e) const
+    {
+        if (!std::isfinite(e) || e < charged_min || e > charged_max) {
+            fail(ORC_ERR_TABLE_RANGE, "schema-6 charged sampler energy outside certified range");
+        }
+    }
+
+    // SampleMollerSecondaryEnergy :2584-2653
+    double sample_moller(double primary, double cdf_u) const
+    {
+        require_charged(primary);
+        if (!std::isfinite(primary) || primary < moller.primary.front() || primary > moller.primary.back()) {
+            fail(ORC_ERR_TABLE_RANGE, "Moller CDF primary energy outside range");
+        }
+        cdf_u = std::max(0.0, std::min(1.0, cdf_u));
+        size_t const n = moller.primary.size();
+        auto const upper = std::lower_bound(moller.primary.begin(), moller.primary.end(), primary);
+        if (upper == moller.primary.begin()) return sample_cdf_group(moller.u[0], moller.v[0], cdf_u);
+        if (upper == moller.primary.end()) return sample_cdf_group(moller.u[n - 1], moller.v[n - 1], cdf_u);
+        size_t const hi = static_cast<size_t>(upper - moller.primary.begin());
+        size_t const lo = hi - 1U;
+        if (moller.primary[hi] == primary) return sample_cdf_group(moller.u[hi], moller.v[hi], cdf_u);
+        double const y0 = sample_cdf_group(moller.u[lo], moller.v[lo], cdf_u);
+        double const y1 = sample_cdf_group(moller.u[hi], moller.v[hi], cdf_u);
+        double const x0 = std::log(moller.primary[lo]);
+        double const x1 = std::log(moller.primary[hi]);
+        double const t = (std::log(primary) - x0) / (x1 - x0);
+        return std::exp(std::log(y0) + t * (std::log(y1) - std::log(y0)));
+    }
+
+    // Load :2146-2179 (hard-Moller mean secondary by 4096-point midpoint quadrature)
+    void build_hm_mean()
+    {
+        RateTable const& r = rate[kEMoller];
+        hm_mean_secondary.assign(r.x.size(), 0.0);
+        int const samples = 4096;
+        for (size_t i = 0; i < r.x.size(); ++i) {
+            if (!(r.y[i] > 0.0)) continue;
+            double mean = 0.0;
+            for (int s = 0; s < samples; ++s) {
+                mean += sample_moller(r.x[i], (static_cast<double>(s) + 0.5) / samples);
+            }
+            hm_mean_secondary[i] = mean / static_cast<double>(samples);
+        }
+    }
+
+    // ElectronHardMollerEnergyLoss :2414-2440
+    double hard_moller_energy_loss(double e, double density_ratio) const
+    {
+        RateTable const& r = rate[kEMoller];
+        double const inv = r.interpolate(e);
+        if (!(inv > 0.0)) return 0.0;
+        double const mean = interp_semilogx_strict(r.x, hm_mean_secondary, e);
+        if (!(mean > 0.0)) return 0.0;
+        return mean * inv * density_ratio;
+    }
+
+    // ElectronTrackedBremsEnergyLoss :2678-2696
+    double tracked_brems_energy_loss(double e, double density_ratio) const
+    {
+        double const inv = rate[kEBrems].interpolate(e) * density_ratio;
+        if (!(inv > 0.0)) return 0.0;
+        return inv * mean_conditional_cdf(ebrems, e);
+    }
+
+    // ElasticMeanFreePath :2477-2493
+    double elastic_mfp(bool positron, double e, double density_ratio) const
+    {
+        ElasticTable const& t = positron ? el_p : el_e;
+        return interp_semilogx_strict(t.e, t.mfp, e) / density_ratio;
+    }
+
+    // ScreeningEta :2495-2502
+    double screening_eta(bool positron, double e) const
+    {
+        ElasticTable const& t = positron ? el_p : el_e;
+        return interp_semilogx_strict(t.e, t.eta, e);
+    }
+
+    // SampleGsCosTheta :2504-2544
+    double sample_gs_cos(bool positron, double e, double path_n, double cdf_u) const
+    {
+        require_charged(e);
+        if (!std::isfinite(cdf_u) || cdf_u < 0.0 || cdf_u > 1.0) fail(ORC_ERR_INVALID, "GS cdf out of [0,1]");
+        bool const use_gs = e <= gs_transition;
+        AngleTable const& t = positron ? (use_gs ? gs_p : wz_p) : (use_gs ? gs_e : wz_e);
+        if (!std::isfinite(e) || e < t.e.front() || e > t.e.back()) fail(ORC_ERR_TABLE_RANGE, "GS energy outside branch");
+        if (!std::isfinite(path_n) || path_n < gs_few) fail(ORC_ERR_INVALID, "GS path below multiple-scattering branch");
+        if (path_n >= gs_iso) return 2.0 * cdf_u - 1.0;
+        return sample_gs_angle_cos(t, e, path_n, cdf_u);
+    }
+
+    // SampleSingleElasticCosTheta :2546-2560
+    double sample_single_elastic_cos(bool positron, double e, double cdf_u) const
+    {
+        require_charged(e);
+        double const eta = screening_eta(positron, e);
+        double const u = std::min(std::max(cdf_u, 0.0), 1.0 - 1.0e-15);
+        return 1.0 - 2.0 * eta * u / (1.0 + eta - u);
+    }
+
+    // SampleBremsstrahlungPhotonEnergy :2698-2712
+    double sample_brems_energy(double primary, double cdf_u) const
+    {
+        require_charged(primary);
+        return sample_conditional_cdf(ebrems, primary, cdf_u, true);
+    }
+
+    // SampleBremsstrahlungCosTheta :2714-2760 (bounded ModifiedTsai by bisection)
+    double sample_brems_cos(double primary, double cdf_u) const
+    {
+        require_charged(primary);
+        cdf_u = std::max(0.0, std::min(1.0, cdf_u));
+        if (cdf_u == 0.0) return 1.0;
+        if (cdf_u == 1.0) return -1.0;
+        constexpr double a = 0.625;
+        auto g2 = [](double x) {
+            if (x < 1.0e-3) {
+                return x * x * (0.5 + x * (-1.0 / 3.0 + x * (1.0 / 8.0 + x * (-1.0 / 30.0 + x * (1.0 / 144.0)))));
+            }
+            return -std::expm1(-x) - x * std::exp(-x);
+        };
+        auto mix = [&](double u) { return 0.25 * g2(a * u) + 0.75 * g2(3.0 * a * u); };
+        double const u_max = 2.0 * (1.0 + primary / kMeC2);
+        double const target = cdf_u * mix(u_max);
+        double lo = 0.0, hi = u_max;
+        for (int it = 0; it < 60; ++it) {
+            double const mid = 0.5 * (lo + hi);
+            if (mix(mid) < target) lo = mid; else hi = mid;
+        }
+        double const f = 0.5 * (lo + hi) / u_max;
+        return 1.0 - 2.0 * f * f;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Geometry: the RZ chord -- NA/src/RreaTransportPath.cpp, NA/include/rrea/RreaTransportPath.H
+// ---------------------------------------------------------------------------
+struct Piece { double begin = 0, end = 0, length = 0, shift = 0; };
+
+struct Path {
+    double sx = 0, sy = 0, sz = 0, ex = 0, ey = 0, ez = 0;
+    double full_length = 0, begin_f = 0, end_f = 0, in_length = 0;
+    bool exits = false;
+    double r_lo = 0, r_hi = 0, z_lo = 0, z_hi = 0;
+    std::vector<Piece> pieces;
+
+    // PointAtFullPathFraction :42-53 (WrapZ is the identity when not periodic)
+    void point_at_full(double f, double& x, double& y, double& z) const
+    {
+        f = std::max(0.0, std::min(1.0, f));
+        x = sx + f * (ex - sx);
+        y = sy + f * (ey - sy);
+        z = sz + f * (ez - sz);
+    }
+    // PointAtPieceFraction :55-72
+    void point_at_piece(Piece const& p, double f, double& x, double& y, double& z) const
+    {
+        f = std::max(p.begin, std::min(p.end, f));
+        x = sx + f * (ex - sx);
+        y = sy + f * (ey - sy);
+        z = sz + f * (ez - sz) + p.shift;
+        z = std::max(z_lo, std::min(z_hi, z));
+    }
+    // OutsideCertifiedDomain :28-40
+    bool outside_certified(double r, double z) const
+    {
+        if (!std::isfinite(r) || !std::isfinite(z)) return true;
+        if (r < r_lo || r >= r_hi) return true;
+        return z < z_lo || z >= z_hi;
+    }
+};
+
+// ClipRzTransportPath :74-293, non-periodic branch
+inline Path clip_rz_path(double sr, double st, double sz, double er_, double et, double ez_,
+                         double r_lo, double r_hi, double z_lo, double z_hi)
+{
+    auto finite = [](double v) { return std::isfinite(v); };
+    if (!finite(sr) || !finite(st) || !finite(sz) || !finite(er_) || !finite(et) || !finite(ez_)
+        || sr < 0.0 || er_ < 0.0 || r_lo < 0.0 || !(r_lo < r_hi) || !(z_lo < z_hi)) {
+        fail(ORC_ERR_INVALID, "ClipRzTransportPath received invalid geometry");
+    }
+    Path p;
+    p.r_lo = r_lo; p.r_hi = r_hi; p.z_lo = z_lo; p.z_hi = z_hi;
+    p.sx = sr * std::cos(st);
+    p.sy = sr * std::sin(st);
+    p.sz = sz;
+    p.ex = er_ * std::cos(et);
+    p.ey = er_ * std::sin(et);
+    p.ez = ez_;
+    double const dx = p.ex - p.sx;
+    double const dy = p.ey - p.sy;
+    double const dz = p.ez - p.sz;
+    p.full_length = std::sqrt(dx * dx + dy * dy + dz * dz);
+    p.begin_f = 0.0;
+    double const scale = std::max(1.0, std::max(r_hi, std::max(std::abs(z_lo), std::abs(z_hi))));
+    double const tol = 64.0 * std::numeric_limits<double>::epsilon() * scale;
+    if (sr < r_lo - tol || sr > r_hi + tol || sz < z_lo - tol || sz > z_hi + tol) {
+        fail(ORC_ERR_OUTSIDE_DOMAIN, "previous particle position outside the RZ domain");
+    }
+    double exit_f = 1.0;
+    auto retain = [&](double c) { if (c >= 0.0 && c < exit_f) exit_f = c; };
+    if (dz > 0.0) retain((z_hi - sz) / dz);
+    else if (dz < 0.0) retain((z_lo - sz) / dz);
+    double const a = dx * dx + dy * dy;
+    if (a > 0.0) {
+        double const b = 2.0 * (p.sx * dx + p.sy * dy);
+        auto cyl = [&](double radius, bool outer) {
+            double const c = p.sx * p.sx + p.sy * p.sy - radius * radius;
+            double const disc = b * b - 4.0 * a * c;
+            if (disc < 0.0) return;
+            double const root = std::sqrt(std::max(disc, 0.0));
+            double const inv_2a = 0.5 / a;
+            double const roots[2] = {(-b - root) * inv_2a, (-b + root) * inv_2a};
+            for (double cand : roots) {
+                if (cand <= 0.0 || cand > 1.0) continue;
+                double const deriv = 2.0 * a * cand + b;
+                bool const leaves = outer ? deriv > 0.0 : deriv < 0.0;
+                if (leaves) retain(cand);
+            }
+        };
+        cyl(r_hi, true);
+        if (r_lo > 0.0) cyl(r_lo, false);
+    }
+    p.end_f = std::max(0.0, std::min(1.0, exit_f));
+    p.in_length = p.full_length * (p.end_f - p.begin_f);
+    p.exits = p.end_f < 1.0 - 64.0 * std::numeric_limits<double>::epsilon();
+    // Non-periodic: the breakpoints are {begin, end}; one piece when end > begin
+    // (the unique() merge at :258-270 cannot fire on two distinct entries that
+    // the piece loop would keep).
+    double const b0 = p.begin_f, b1 = p.end_f;
+    bool merged = std::abs(b0 - b1) <= 128.0 * std::numeric_limits<double>::epsilon()
+                                       * std::max(1.0, std::max(std::abs(b0), std::abs(b1)));
+    if (!merged && b1 > b0) {
+        p.pieces.push_back(Piece{b0, b1, p.full_length * (b1 - b0), 0.0});
+    }
+    return p;
+}
+
+struct Chord { Path path; bool valid = false; };
+
+// RreaChargedSpeedFromKineticEnergy OV/RreaParticleInteraction.H:92-105
+inline double charged_speed(double k)
+{
+    if (!std::isfinite(k) || k < 0.0) return std::numeric_limits<double>::quiet_NaN();
+    double const gamma = 1.0 + k / kMeC2;
+    double const beta2 = std::max(0.0, 1.0 - 1.0 / (gamma * gamma));
+    return kC * std::sqrt(beta2);
+}
+
+// RreaBuildRunningChargedChord OV/RreaParticleInteraction.H:110-161
+inline Chord build_chord(double x, double y, double z, double dx, double dy, double dz, double k,
+                         double req_dt, double outer_dt, double r_lo, double r_hi, double z_lo, double z_hi)
+{
+    Chord c;
+    if (!std::isfinite(req_dt) || !std::isfinite(outer_dt) || !(req_dt >= 0.0) || !(outer_dt > 0.0)) return c;
+    double const speed = charged_speed(k);
+    double const norm = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (!std::isfinite(speed) || !std::isfinite(norm) || !(norm > 0.0)) return c;
+    double const scale = speed * req_dt / norm;
+    double const ex = x + scale * dx;
+    double const ey = y + scale * dy;
+    double const ez = z + scale * dz;
+    c.path = clip_rz_path(std::hypot(x, y), std::atan2(y, x), z, std::hypot(ex, ey), std::atan2(ey, ex), ez,
+                          r_lo, r_hi, z_lo, z_hi);
+    c.valid = std::isfinite(c.path.full_length) && c.path.begin_f == 0.0;
+    return c;
+}
+
+struct EscapePolicy { bool escapes = false; bool invalidate = false; };
+
+// RreaEndpointEscapePolicyForPath / RreaClippedTransportEscapePolicyForPath
+// OV/RreaParticleInteraction.H:409-452
+inline EscapePolicy clipped_escape_policy(Path const& p, double er_, double ez_)
+{
+    if (!std::isfinite(er_) || !std::isfinite(ez_)) return {true, true};
+    bool const outside = p.outside_certified(er_, ez_);
+    bool const escapes = p.exits || outside;
+    EscapePolicy pol{escapes, escapes};
+    return pol;
+}
+
+// RreaCartesianPositionInHalfOpenRzDomain :318-354
+inline bool in_half_open(double x, double y, double z, double r_lo, double r_hi, double z_lo, double z_hi)
+{
+    if (!std::isfinite(x) || !std::isfinite(y)) return false;
+    double const r = std::hypot(x, y);
+    return std::isfinite(r) && std::isfinite(z) && r_hi > r_lo && z_hi > z_lo && r >= r_lo && r < r_hi
+        && z >= z_lo && z < z_hi;
+}
+
+// RreaAdvanceRunningChargedPosition :368-392
+inline bool advance_running_position(Path const& p, double f, double& rx, double& ry, double& rz)
+{
+    double x, y, z;
+    p.point_at_full(f, x, y, z);
+    if (!in_half_open(x, y, z, p.r_lo, p.r_hi, p.z_lo, p.z_hi)) return false;
+    rx = x; ry = y; rz = z;
+    return true;
+}
+
+struct BoundaryCrossing {
+    bool valid = false;
+    double start_r = 0, start_z = 0, cross_r = 0, cross_z = 0;
+    uint32_t faces = 0;
+};
+
+// RreaChargedBoundaryCrossingForPath :468-540 (non-periodic)
+inline BoundaryCrossing boundary_crossing(Path const& p)
+{
+    BoundaryCrossing res;
+    double cx, cy;
+    p.point_at_full(p.end_f, cx, cy, res.cross_z);
+    res.cross_r = std::hypot(cx, cy);
+    res.start_r = std::hypot(p.sx, p.sy);
+    res.start_z = p.sz;
+    double const scale = std::max(1.0, std::max(p.r_hi, std::max(std::abs(p.z_lo), std::abs(p.z_hi))));
+    double const tol = 512.0 * std::numeric_limits<double>::epsilon() * scale;
+    bool const on_r_hi = std::abs(res.cross_r - p.r_hi) <= tol;
+    bool const on_z_lo = std::abs(res.cross_z - p.z_lo) <= tol;
+    bool const on_z_hi = std::abs(res.cross_z - p.z_hi) <= tol;
+    double const dx = p.ex - p.sx, dy = p.ey - p.sy, dz = p.ez - p.sz;
+    double const outward = cx * dx + cy * dy;
+    if (on_r_hi && outward > -tol * scale) { res.faces |= 1U; res.cross_r = p.r_hi; }
+    if (on_z_lo && dz < 0.0) { res.faces |= 2U; res.cross_z = p.z_lo; }
+    if (on_z_hi && dz > 0.0) { res.faces |= 4U; res.cross_z = p.z_hi; }
+    if (res.faces == 0U) {
+        if (on_r_hi) { res.faces |= 1U; res.cross_r = p.r_hi; }
+        if (on_z_hi) { res.faces |= 4U; res.cross_z = p.z_hi; }
+    }
+    res.valid = res.faces != 0U && std::isfinite(res.start_r) && std::isfinite(res.start_z)
+        && std::isfinite(res.cross_r) && std::isfinite(res.cross_z);
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Coupling services: density ratio and field gather
+// ---------------------------------------------------------------------------
+struct Coupling {
+    orc_geometry g;
+    double reference_density;
+    double r_hi, z_hi;
+
+    // TransportDensityRatioAtZ OV/RreaWarpXCoupling.cpp:79-95 +
+    // RreaAltitudeProfile::Interpolate NA/src/RreaAltitudeProfile.cpp:125-151
+    double density_ratio_at_z(double z) const
+    {
+        if (g.density_mode == 0) return g.density_ratio;
+        double const alt = g.altitude_at_z0_m + z;
+        int const n = g.profile_n;
+        double const* a = g.profile_alt_m;
+        if (alt < a[0] || alt > a[n - 1]) fail(ORC_ERR_TABLE_RANGE, "altitude outside profile");
+        double const* up = std::upper_bound(a, a + n, alt);
+        double v;
+        if (up == a) v = g.profile_rho[0];
+        else if (up == a + n) v = g.profile_rho[n - 1];
+        else {
+            size_t const hi = static_cast<size_t>(up - a);
+            size_t const lo = hi - 1U;
+            double const span = a[hi] - a[lo];
+            double const t = span > 0.0 ? (alt - a[lo]) / span : 0.0;
+            v = (1.0 - t) * g.profile_rho[lo] + t * g.profile_rho[hi];
+        }
+        return v / reference_density;
+    }
+
+    // GatherTotalFieldRZ NA/src/RreaAmrexAdvance.cpp:532-579 with the stencil of
+    // NA/include/rrea/RreaFieldGatherStencil.H:49-130, and MaxwellBthetaAt :511-530
+    void gather(double r, double z, double& er, double& ez, double& bt) const
+    {
+        int const nr = g.nr, nz = g.nz;
+        double const fx = (r - g.r_lo) / g.dr + 0.0 - 0.5;
+        double const fz = (z - g.z_lo) / g.dz + 0.0 - 0.5;
+        int const ir0 = static_cast<int>(std::floor(fx));
+        int const iz0 = static_cast<int>(std::floor(fz));
+        double const tx = fx - static_cast<double>(ir0);
+        double const tz = fz - static_cast<double>(iz0);
+        int const i0 = std::max(0, std::min(nr - 1, ir0));
+        int const i1 = std::max(0, std::min(nr - 1, ir0 + 1));
+        int const j0 = std::max(0, std::min(nz - 1, iz0));
+        int const j1 = std::max(0, std::min(nz - 1, iz0 + 1));
+        auto er_idx = [&](int i, int j) { return static_cast<size_t>(i) + static_cast<size_t>(nr + 1) * j; };
+        auto ez_idx = [&](int i, int j) { return static_cast<size_t>(i) + static_cast<size_t>(nr) * j; };
+        auto cell_er = [&](int i, int j) {
+            double const lo = g.er[er_idx(i, j)] + g.amb_er[er_idx(i, j)];
+            double const hi = g.er[er_idx(i + 1, j)] + g.amb_er[er_idx(i + 1, j)];
+            return 0.5 * (lo + hi);
+        };
+        auto cell_ez = [&](int i, int j) {
+            double const lo = g.ez[ez_idx(i, j)] + g.amb_ez[ez_idx(i, j)];
+            double const hi = g.ez[ez_idx(i, j + 1)] + g.amb_ez[ez_idx(i, j + 1)];
+            return 0.5 * (lo + hi);
+        };
+        auto bilinear = [](double f00, double f01, double f10, double f11, double tx_, double tz_) {
+            return (1.0 - tx_) * ((1.0 - tz_) * f00 + tz_ * f01) + tx_ * ((1.0 - tz_) * f10 + tz_ * f11);
+        };
+        double const first_center = g.r_lo + 0.5 * g.dr;
+        double const axis = r >= first_center ? 1.0 : std::max(0.0, (r - g.r_lo) / (0.5 * g.dr));
+        er = bilinear(cell_er(i0, j0), cell_er(i0, j1), cell_er(i1, j0), cell_er(i1, j1), tx, tz) * axis;
+        ez = bilinear(cell_ez(i0, j0), cell_ez(i0, j1), cell_ez(i1, j0), cell_ez(i1, j1), tx, tz);
+        // MaxwellBthetaAt
+        if (r <= 0.0) {
+            bt = 0.0;
+        } else {
+            double const frr = std::clamp(r / g.dr, 0.0, static_cast<double>(nr));
+            double const fzz = std::clamp((z - g.z_lo) / g.dz, 0.0, static_cast<double>(nz));
+            int const a0 = std::min(static_cast<int>(std::floor(frr)), nr - 1);
+            int const c0 = std::min(static_cast<int>(std::floor(fzz)), nz - 1);
+            int const a1 = a0 + 1, c1 = c0 + 1;
+            double const trr = frr - a0, tzz = fzz - c0;
+            auto B = [&](int i, int j) { return g.btheta[er_idx(i, j)]; };
+            bt = (1.0 - trr) * ((1.0 - tzz) * B(a0, c0) + tzz * B(a0, c1))
+                + trr * ((1.0 - tzz) * B(a1, c0) + tzz * B(a1, c1));
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Material path -- OV/RreaMaterialPath.H
+// ---------------------------------------------------------------------------
+struct Interval { double begin, end, length, ratio; };
+
+inline double positive_or(double v, double fallback) { return v > 0.0 ? v : fallback; }
+
+// integrate_density_piece_m :60-84 + build_material_path_intervals :86-114
+inline std::vector<Interval> build_intervals(Coupling const& cp, Path const& p, double fallback)
+{
+    std::vector<Interval> out;
+    for (auto const& pc : p.pieces) {
+        if (!(pc.length > 0.0)) continue;
+        double dl = 0.0;
+        double const b = std::max(pc.begin, pc.begin), e = std::min(pc.end, pc.end);
+        if (e > b && p.full_length > 0.0) {
+            double x, y, z;
+            p.point_at_piece(pc, 0.5 * (b + e), x, y, z);
+            dl = p.full_length * (e - b) * positive_or(cp.density_ratio_at_z(z), fallback);
+        }
+        out.push_back(Interval{pc.begin, pc.end, pc.length, positive_or(dl / pc.length, fallback)});
+    }
+    return out;
+}
+
+// density_path_m :117-137
+inline double density_path(std::vector<Interval> const& iv, Path const& p, double b, double e)
+{
+    b = std::max(p.begin_f, b);
+    e = std::min(p.end_f, e);
+    if (!(e > b)) return 0.0;
+    double s = 0.0;
+    for (auto const& it : iv) {
+        double const lb = std::max(b, it.begin), le = std::min(e, it.end);
+        if (le > lb) s += p.full_length * (le - lb) * it.ratio;
+    }
+    return s;
+}
+
+// fraction_at_density_path_m :143-172
+inline double fraction_at_density_path(std::vector<Interval> const& iv, Path const& p, double b, double e, double target)
+{
+    b = std::max(p.begin_f, b);
+    e = std::min(p.end_f, e);
+    if (!std::isfinite(target) || target < 0.0) fail(ORC_ERR_INVALID, "sampled density path invalid");
+    double rem = target;
+    for (auto const& it : iv) {
+        double const lb = std::max(b, it.begin), le = std::min(e, it.end);
+        if (!(le > lb) || !(it.ratio > 0.0)) continue;
+        double const contrib = p.full_length * (le - lb) * it.ratio;
+        if (rem <= contrib) return lb + rem / (p.full_length * it.ratio);
+        rem -= contrib;
+    }
+    return e;
+}
+
+// max_uniform_subcycle_density_path_m :177-201
+inline double max_uniform_slice(std::vector<Interval> const& iv, Path const& p, uint64_t n)
+{
+    if (n == 0U) fail(ORC_ERR_INVALID, "zero subcycles");
+    double const span = p.end_f - p.begin_f;
+    double const den = static_cast<double>(n);
+    double m = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+        m = std::max(m, density_path(iv, p, p.begin_f + static_cast<double>(i) / den * span,
+                                     p.begin_f + static_cast<double>(i + 1U) / den * span));
+    }
+    return m;
+}
+
+// RreaFrozenStpRate :216-223
+inline double frozen(double v)
+{
+    if (!std::isfinite(v) || v < 0.0) fail(ORC_ERR_INVALID, "frozen rate negative or non-finite");
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Field work -- OV/RreaPathFieldWork.H:57-150 with the GL2 rule of
+// NA/include/rrea/RreaGaussLegendre8.H (two-point node 1/sqrt(3), weights 1)
+// ---------------------------------------------------------------------------
+constexpr double kGl2Node = static_cast<double>(0.577350269189625764509148780501957L);
+
+struct FieldRange { double work = 0, ex = 0, ey = 0, ez = 0, bx = 0, by = 0, bz = 0, length = 0; };
+
+inline FieldRange path_field_work(Coupling const& cp, Path const& p, double q, double b, double e)
+{
+    FieldRange res;
+    if (!(p.full_length > 0.0)) return res;
+    b = std::max(p.begin_f, b);
+    e = std::min(p.end_f, e);
+    if (!(e > b)) return res;
+    double const pdx = (p.ex - p.sx) / p.full_length;
+    double const pdy = (p.ey - p.sy) / p.full_length;
+    double const pdz = (p.ez - p.sz) / p.full_length;
+    for (auto const& pc : p.pieces) {
+        double const pb = std::max(b, pc.begin), pe = std::min(e, pc.end);
+        if (!(pe > pb)) continue;
+        auto sample = [&](double f) {
+            double x, y, z;
+            p.point_at_piece(pc, f, x, y, z);
+            double const r = std::hypot(x, y);
+            double rad_dir = 0.0, rc = 1.0, rs = 0.0;
+            if (r > 0.0) { rad_dir = (pdx * x + pdy * y) / r; rc = x / r; rs = y / r; }
+            double er, ez, bt;
+            cp.gather(r, z, er, ez, bt);
+            double const w = q * (er * rad_dir + ez * pdz) * 1.0;
+            if (!std::isfinite(w) || !std::isfinite(er) || !std::isfinite(ez) || !std::isfinite(bt)) {
+                fail(ORC_ERR_INVALID, "substep field-impulse gather non-finite");
+            }
+            return std::array<double, 7>{w, er * rc, er * rs, ez, -bt * rs, bt * rc, 0.0};
+        };
+        std::array<double, 7> integ{};
+        double const mid = 0.5 * (pb + pe);
+        double const half = 0.5 * (pe - pb);
+        double const off = half * kGl2Node;
+        auto const lo = sample(mid - off);
+        auto const hi = sample(mid + off);
+        for (int c = 0; c < 7; ++c) integ[c] = half * (lo[c] + hi[c]);
+        res.work += p.full_length * integ[0];
+        res.ex += p.full_length * integ[1];
+        res.ey += p.full_length * integ[2];
+        res.ez += p.full_length * integ[3];
+        res.bx += p.full_length * integ[4];
+        res.by += p.full_length * integ[5];
+        res.bz += p.full_length * integ[6];
+        res.length += p.full_length * (pe - pb);
+    }
+    if (res.length > 0.0) {
+        res.ex /= res.length; res.ey /= res.length; res.ez /= res.length;
+        res.bx /= res.length; res.by /= res.length; res.bz /= res.length;
+    }
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Kinematics -- OV/RreaChargedInterleave.H, NA/include/rrea/RreaLorentzPush.H,
+// NA/include/rrea/RreaSeedSchedule.H:30-43, OV/RreaParticleInteraction.H:1494-1506
+// ---------------------------------------------------------------------------
+inline double proper_velocity_from_kinetic(double k)
+{
+    if (k <= 0.0) return 0.0;
+    double const gamma = 1.0 + k / kMeC2;
+    return kC * std::sqrt(std::max(gamma * gamma - 1.0, 0.0));
+}
+
+inline double kinetic_from_proper_velocity(double ux, double uy, double uz)
+{
+    double const u2 = ux * ux + uy * uy + uz * uz;
+    double const gamma = std::sqrt(1.0 + u2 / (kC * kC));
+    return u2 * kMeC2 / ((kC * kC) * (gamma + 1.0));
+}
+
+inline void normalize(double& x, double& y, double& z)
+{
+    double const n = std::sqrt(x * x + y * y + z * z);
+    if (n <= 0.0) { x = 0.0; y = 0.0; z = 1.0; return; }
+    x /= n; y /= n; z /= n;
+}
+
+// RreaBorisMagneticRotate NA/include/rrea/RreaLorentzPush.H:21-46
+inline void boris_rotate(double& ux, double& uy, double& uz, double qmdt, double bx, double by, double bz)
+{
+    double const gamma = std::sqrt(1.0 + (ux * ux + uy * uy + uz * uz) / (kC * kC));
+    double const tx = qmdt * bx / (2.0 * gamma);
+    double const ty = qmdt * by / (2.0 * gamma);
+    double const tz = qmdt * bz / (2.0 * gamma);
+    double const t2 = tx * tx + ty * ty + tz * tz;
+    double const sx = 2.0 * tx / (1.0 + t2);
+    double const sy = 2.0 * ty / (1.0 + t2);
+    double const sz = 2.0 * tz / (1.0 + t2);
+    double const px = ux + uy * tz - uz * ty;
+    double const py = uy + uz * tx - ux * tz;
+    double const pz = uz + ux * ty - uy * tx;
+    double const ox = ux + py * sz - pz * sy;
+    double const oy = uy + pz * sx - px * sz;
+    double const oz = uz + px * sy - py * sx;
+    ux = ox; uy = oy; uz = oz;
+}
+
+// RreaApplyChargedFieldHalfKick OV/RreaChargedInterleave.H:100-138
+inline double half_kick(FieldRange const& f, double half_dt, double q, double& k, double& dx, double& dy, double& dz)
+{
+    double const umag = proper_velocity_from_kinetic(k);
+    double const impulse = q * (kC * kC / kMeC2) * half_dt;
+    double ux = umag * dx + impulse * f.ex;
+    double uy = umag * dy + impulse * f.ey;
+    double uz = umag * dz + impulse * f.ez;
+    boris_rotate(ux, uy, uz, impulse, f.bx, f.by, f.bz);
+    double const n = std::sqrt(ux * ux + uy * uy + uz * uz);
+    if (n > 0.0 && std::isfinite(n)) { dx = ux / n; dy = uy / n; dz = uz / n; }
+    double const hw = 0.5 * f.work;
+    k = std::max(0.0, k + hw);
+    return hw;
+}
+
+// RreaTransverseBasis :309-338
+inline void transverse_basis(double dx, double dy, double dz, double& b1x, double& b1y, double& b1z,
+                             double& b2x, double& b2y, double& b2z)
+{
+    normalize(dx, dy, dz);
+    double rx = 0.0, ry = 0.0, rz = 1.0;
+    if (std::abs(dz) > 0.9) { rx = 1.0; rz = 0.0; }
+    b1x = ry * dz - rz * dy;
+    b1y = rz * dx - rx * dz;
+    b1z = rx * dy - ry * dx;
+    normalize(b1x, b1y, b1z);
+    b2x = dy * b1z - dz * b1y;
+    b2y = dz * b1x - dx * b1z;
+    b2z = dx * b1y - dy * b1x;
+    normalize(b2x, b2y, b2z);
+}
+
+// RreaSampleConeDirection :340-364
+inline void cone_direction(double ax, double ay, double az, double ct, double uphi, double& ox, double& oy, double& oz)
+{
+    normalize(ax, ay, az);
+    double b1x, b1y, b1z, b2x, b2y, b2z;
+    transverse_basis(ax, ay, az, b1x, b1y, b1z, b2x, b2y, b2z);
+    ct = std::min(1.0, std::max(-1.0, ct));
+    double const st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+    double const phi = kTwoPi * uphi;
+    double const cp = std::cos(phi), sp = std::sin(phi);
+    ox = ax * ct + st * (cp * b1x + sp * b2x);
+    oy = ay * ct + st * (cp * b1y + sp * b2y);
+    oz = az * ct + st * (cp * b1z + sp * b2z);
+    normalize(ox, oy, oz);
+}
+
+// RreaPoissonSampleCapped :371-383
+inline int poisson_capped(double mean, double u, int kmax)
+{
+    if (mean <= 0.0) return 0;
+    double p = std::exp(-mean);
+    double cum = p;
+    int k = 0;
+    while (u > cum && k < kmax) { ++k; p *= mean / static_cast<double>(k); cum += p; }
+    return k;
+}
+
+inline double unit_cycle(double u)
+{
+    u -= std::floor(u);
+    return u < 0.0 ? u + 1.0 : u;
+}
+
+// Moller two-body lab cosines :391-423
+inline double moller_secondary_cos(double kp, double ks)
+{
+    kp = std::max(kp, 0.0);
+    ks = std::max(ks, 0.0);
+    if (kp <= 0.0 || ks <= 0.0) return 1.0;
+    double const v = ks * (kp + 2.0 * kMeC2) / (kp * (ks + 2.0 * kMeC2));
+    return std::sqrt(std::min(1.0, std::max(0.0, v)));
+}
+
+inline double moller_primary_cos(double kp, double ks)
+{
+    double const sc = std::max(0.0, kp - ks);
+    if (kp <= 0.0 || sc <= 0.0) return 1.0;
+    double const v = sc * (kp + 2.0 * kMeC2) / (kp * (sc + 2.0 * kMeC2));
+    return std::sqrt(std::min(1.0, std::max(0.0, v)));
+}
+
+// RreaSampleMollerLabDirections :429-463
+inline void moller_lab_directions(double pdx, double pdy, double pdz, double kp, double ks, double az,
+                                  double& spx, double& spy, double& spz, double& sdx, double& sdy, double& sdz)
+{
+    normalize(pdx, pdy, pdz);
+    az = unit_cycle(az);
+    cone_direction(pdx, pdy, pdz, moller_secondary_cos(kp, ks), az, sdx, sdy, sdz);
+    cone_direction(pdx, pdy, pdz, moller_primary_cos(kp, ks), unit_cycle(az + 0.5), spx, spy, spz);
+}
+
+// RreaApplyPhotonRecoilToChargedDirection :465-496
+inline void photon_recoil(double k0, double eg, double gx, double gy, double gz, double& cx, double& cy, double& cz)
+{
+    eg = std::max(eg, 0.0);
+    if (k0 <= 0.0 || eg <= 0.0) return;
+    normalize(cx, cy, cz);
+    normalize(gx, gy, gz);
+    double const p0 = std::sqrt(std::max(0.0, k0 * (k0 + 2.0 * kMeC2)));
+    double const px = p0 * cx - eg * gx;
+    double const py = p0 * cy - eg * gy;
+    double const pz = p0 * cz - eg * gz;
+    double const n = std::sqrt(px * px + py * py + pz * pz);
+    if (n > 0.0) { cx = px / n; cy = py / n; cz = pz / n; normalize(cx, cy, cz); }
+}
+
+// ---------------------------------------------------------------------------
+// Competing hazard -- OV/RreaParticleInteraction.H:1357-1492
+// ---------------------------------------------------------------------------
+struct Hazard { bool valid = false, event = false; int channel = -1; double sampled = 0, total = 0; };
+
+inline Hazard select_hazard(double const taus[3], int nch, double u1, double u2)
+{
+    Hazard h;
+    if (nch < 1 || nch > 3 || !std::isfinite(u1) || !std::isfinite(u2)) return h;
+    for (int i = 0; i < nch; ++i) {
+        if (!std::isfinite(taus[i]) || taus[i] < 0.0) return h;
+        h.total += taus[i];
+    }
+    h.valid = std::isfinite(h.total);
+    if (!h.valid) return h;
+    u1 = std::max(0.0, std::min(1.0, u1));
+    h.sampled = -std::log(std::max(1.0e-30, 1.0 - u1));
+    if (!(h.sampled < h.total)) return h;
+    h.event = true;
+    u2 = std::max(0.0, std::min(1.0, u2));
+    double const target = u2 * h.total;
+    double cum = 0.0;
+    int last_open = -1;
+    for (int i = 0; i < nch; ++i) {
+        if (taus[i] > 0.0) last_open = i;
+        cum += taus[i];
+        if (target < cum) { h.channel = i; return h; }
+    }
+    h.channel = last_open;
+    h.valid = h.channel >= 0;
+    h.event = h.valid;
+    return h;
+}
+
+// ---------------------------------------------------------------------------
+// Net-chord deposit -- NA/include/rrea/RreaSegmentDepositionRZ.H
+// ---------------------------------------------------------------------------
+struct Grid {
+    int nr = 0, nz = 0;
+    double dr = 0, dz = 0, r_lo = 0, z_lo = 0;
+    double* rho = nullptr;
+    double* jr = nullptr;
+    double* jz = nullptr;
+};
+
+// DepositChargeCIC :68-90
+inline void deposit_cic(Grid& g, double q, double r, double z)
+{
+    double const xr = (r - g.r_lo) / g.dr - 0.5;
+    double const xz = (z - g.z_lo) / g.dz - 0.5;
+    int const i0 = static_cast<int>(std::floor(xr));
+    int const j0 = static_cast<int>(std::floor(xz));
+    double const fr = xr - i0, fz = xz - j0;
+    double const wr[2] = {1.0 - fr, fr};
+    double const wz[2] = {1.0 - fz, fz};
+    for (int a = 0; a < 2; ++a) {
+        for (int b = 0; b < 2; ++b) {
+            int i = i0 + a, j = j0 + b;
+            if (i < 0) i = -1 - i;
+            if (i >= g.nr) i = g.nr - 1;
+            if (j < 0) j = 0;
+            if (j >= g.nz) j = g.nz - 1;
+            g.rho[static_cast<size_t>(i) + static_cast<size_t>(g.nr) * j] += q * wr[a] * wz[b];
+        }
+    }
+}
+
+// detail::deposit_piece :96-152
+inline void deposit_piece(Grid& g, double qdt, double xr0, double xz0, double xr1, double xz1)
+{
+    double fr_flux = qdt * (xr1 - xr0);
+    double const fz_flux = qdt * (xz1 - xz0);
+    double const mr = 0.5 * (xr0 + xr1);
+    double const mz = 0.5 * (xz0 + xz1);
+    {
+        int const jf = static_cast<int>(std::floor(mz));
+        double const fz = mz - jf;
+        int const fi = static_cast<int>(std::floor(mr)) + 1;
+        if (fi <= 0) fr_flux = 0.0;
+        double const wz0 = 1.0 - fz;
+        for (int b = 0; b < 2; ++b) {
+            int j = jf + b;
+            if (j < 0) j = 0;
+            if (j >= g.nz) j = g.nz - 1;
+            int i = fi;
+            if (i <= 0) continue;
+            if (i > g.nr) i = g.nr;
+            g.jr[static_cast<size_t>(i) + static_cast<size_t>(g.nr + 1) * j] += fr_flux * (b == 0 ? wz0 : fz);
+        }
+    }
+    {
+        int const ifc = static_cast<int>(std::floor(mr));
+        double const fr = mr - ifc;
+        int const fj = static_cast<int>(std::floor(mz)) + 1;
+        double const wr0 = 1.0 - fr;
+        for (int a = 0; a < 2; ++a) {
+            int i = ifc + a;
+            if (i < 0) i = -1 - i;
+            if (i >= g.nr) i = g.nr - 1;
+            int j = fj;
+            if (j < 0) j = 0;
+            if (j > g.nz) j = g.nz;
+            g.jz[static_cast<size_t>(i) + static_cast<size_t>(g.nr) * j] += fz_flux * (a == 0 ? wr0 : fr);
+        }
+    }
+}
+
+// DepositSegmentRZ :159-192
+inline void deposit_segment(Grid& g, double q, double dt, double r0, double z0, double r1, double z1)
+{
+    double const qdt = q / dt;
+    double const xr0 = (r0 - g.r_lo) / g.dr - 0.5;
+    double const xz0 = (z0 - g.z_lo) / g.dz - 0.5;
+    double const xr1 = (r1 - g.r_lo) / g.dr - 0.5;
+    double const xz1 = (z1 - g.z_lo) / g.dz - 0.5;
+    std::vector<double> cuts = {0.0, 1.0};
+    auto add = [&](double a, double b) {
+        if (a == b) return;
+        double const lo = std::min(a, b), hi = std::max(a, b);
+        for (double k = std::ceil(lo); k <= std::floor(hi); ++k) {
+            double const s = (k - a) / (b - a);
+            if (s > 0.0 && s < 1.0) cuts.push_back(s);
+        }
+    };
+    add(xr0, xr1);
+    add(xz0, xz1);
+    std::sort(cuts.begin(), cuts.end());
+    for (size_t p = 0; p + 1 < cuts.size(); ++p) {
+        double const s0 = cuts[p], s1 = cuts[p + 1];
+        if (s1 <= s0) continue;
+        deposit_piece(g, qdt, xr0 + s0 * (xr1 - xr0), xz0 + s0 * (xz1 - xz0),
+                      xr0 + s1 * (xr1 - xr0), xz0 + s1 * (xz1 - xz0));
+    }
+}
+
+// DepositEscapingSegmentRZ :229-389
+inline bool deposit_escaping_segment(Grid& g, double q, double dt, double sr, double sz, double fr_, double fz_,
+                                     double cr, double cz, int pnr, int pzb, int pnz, uint32_t faces)
+{
+    uint32_t constexpr known = 1U | 2U | 4U;
+    if (!(dt > 0.0) || !std::isfinite(q) || !std::isfinite(dt) || !std::isfinite(sr) || !std::isfinite(sz)
+        || !std::isfinite(fr_) || !std::isfinite(fz_) || !std::isfinite(cr) || !std::isfinite(cz)
+        || pnr <= 0 || pnr >= g.nr || pnz <= 0 || pzb <= 0 || pzb + pnz >= g.nz || faces == 0U
+        || (faces & ~known) != 0U || ((faces & 2U) != 0U && (faces & 4U) != 0U)) {
+        return false;
+    }
+    if (faces & 1U) cr = g.r_lo + pnr * g.dr;
+    if (faces & 2U) cz = g.z_lo + pzb * g.dz;
+    else if (faces & 4U) cz = g.z_lo + (pzb + pnz) * g.dz;
+    struct Move { size_t from, to, face; double charge; bool radial; int orient; };
+    std::vector<Move> moves;
+    double const xr = (cr - g.r_lo) / g.dr - 0.5;
+    double const xz = (cz - g.z_lo) / g.dz - 0.5;
+    int const i0 = static_cast<int>(std::floor(xr));
+    int const j0 = static_cast<int>(std::floor(xz));
+    double const frc = xr - i0, fzc = xz - j0;
+    double const wr[2] = {1.0 - frc, frc};
+    double const wz[2] = {1.0 - fzc, fzc};
+    for (int a = 0; a < 2; ++a) {
+        for (int b = 0; b < 2; ++b) {
+            int i = i0 + a, j = j0 + b;
+            if (i < 0) i = -1 - i;
+            if (i < 0 || i >= g.nr || j < 0 || j >= g.nz) return false;
+            bool const in_phys = i < pnr && j >= pzb && j < pzb + pnz;
+            if (!in_phys) continue;
+            uint32_t cf[3] = {0, 0, 0};
+            int cc = 0;
+            if ((faces & 1U) && i == pnr - 1) cf[cc++] = 1U;
+            if ((faces & 2U) && j == pzb) cf[cc++] = 2U;
+            if ((faces & 4U) && j == pzb + pnz - 1) cf[cc++] = 4U;
+            if (cc == 0) return false;
+            double const cell_charge = q * wr[a] * wz[b];
+            double const share = cell_charge / cc;
+            size_t const from = static_cast<size_t>(i) + static_cast<size_t>(g.nr) * j;
+            for (int c = 0; c < cc; ++c) {
+                if (cf[c] == 1U) {
+                    moves.push_back(Move{from, static_cast<size_t>(i + 1) + static_cast<size_t>(g.nr) * j,
+                                         static_cast<size_t>(pnr) + static_cast<size_t>(g.nr + 1) * j, share, true, 1});
+                } else {
+                    int const face_j = cf[c] == 2U ? pzb : pzb + pnz;
+                    int const to_j = cf[c] == 2U ? j - 1 : j + 1;
+                    moves.push_back(Move{from, static_cast<size_t>(i) + static_cast<size_t>(g.nr) * to_j,
+                                         static_cast<size_t>(i) + static_cast<size_t>(g.nr) * face_j, share, false,
+                                         cf[c] == 2U ? -1 : 1});
+                }
+            }
+        }
+    }
+    if (moves.empty()) return false;
+    deposit_cic(g, -q, sr, sz);
+    deposit_cic(g, q, cr, cz);
+    deposit_segment(g, q, dt, sr, sz, fr_, fz_);
+    deposit_segment(g, q, dt, fr_, fz_, cr, cz);
+    for (auto const& m : moves) {
+        double const flux = m.orient * m.charge / dt;
+        if (m.radial) g.jr[m.face] += flux; else g.jz[m.face] += flux;
+        g.rho[m.from] -= m.charge;
+        g.rho[m.to] += m.charge;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Per-particle side effects (the oracle's per-chunk log)
+// ---------------------------------------------------------------------------
+struct DepositOp {
+    int kind;  // 0 segment, 1 escape
+    double a[7];
+    uint32_t faces;
+};
+
+struct ChunkLog {
+    std::vector<orc_secondary> secondaries;
+    std::vector<DepositOp> deposits;
+    double tallies[ORC_T_COUNT] = {0};
+    int error = 0;
+};
+
+struct StepContext {
+    Tables const* t;
+    Coupling cp;
+    orc_config cfg;
+    double r_lo, r_hi, z_lo, z_hi;
+    double density_ratio;  // reference-z fallback (RreaParticleInteraction.cpp:33-35)
+};
+
+// substep_driver_count OV/RreaTransportStep.H:106-110
+inline uint64_t driver_count(double amount, double per)
+{
+    return static_cast<uint64_t>(std::max(1.0, std::ceil(amount / per)));
+}
+
+// selected_subcycles :125-141 + selected_material_subcycles :148-184
+inline uint64_t select_material_subcycles(double max_tau, std::vector<Interval> const& iv, Path const& p,
+                                          orc_config const& cfg, bool& violation, double total_stp)
+{
+    if (!(std::isfinite(cfg.max_optical_depth_per_substep) && cfg.max_optical_depth_per_substep > 0.0
+          && cfg.max_subcycles > 0U)) {
+        fail(ORC_ERR_INVALID, "invalid optical-depth subcycle controls");
+    }
+    double const target = cfg.max_optical_depth_per_substep;
+    uint64_t const cap = cfg.max_subcycles;
+    uint64_t const requested = static_cast<uint64_t>(std::max(1.0, std::ceil(max_tau / target)));
+    violation = requested > cap;
+    uint64_t n = std::min(requested, cap);
+    for (;;) {
+        double const maximum = total_stp * max_uniform_slice(iv, p, n);
+        double const tol = 64.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, target);
+        if (maximum <= target + tol) return n;
+        if (n >= cap) { violation = true; return n; }
+        long double const est = std::ceil(static_cast<long double>(n) * static_cast<long double>(maximum / target));
+        uint64_t next = n + 1U;
+        if (est >= static_cast<long double>(cap)) next = cap;
+        else if (est > static_cast<long double>(next)) next = static_cast<uint64_t>(est);
+        n = std::min(next, cap);
+    }
+}
+
+// RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:107-152 (electron)
+inline void electron_stp_rates(Tables const& t, double thr, double gcut, double e, bool hm_on, double out[3])
+{
+    bool const hm_open = hm_on && thr > 0.0 && e > 2.0 * thr;
+    double hm = 0.0;
+    if (hm_open) {
+        // ElectronHardMollerMeanFreePath :2394-2412 then 1/mfp
+        double const inv = t.rate[kEMoller].interpolate(e);
+        double mfp;
+        if (thr > 0.0 && e <= 2.0 * thr) mfp = 1.0e99;
+        else {
+            double const v = inv * 1.0;
+            if (!std::isfinite(v) || v < 0.0) fail(ORC_ERR_INVALID, "inverse length invalid");
+            mfp = v > 0.0 ? 1.0 / v : 1.0e99;
+        }
+        hm = mfp > 0.0 ? 1.0 / mfp : 0.0;
+    }
+    double br = 0.0;
+    if (std::isfinite(e) && std::isfinite(gcut) && e > gcut) {
+        // ElectronBremsstrahlungMeanFreePath :2666-2676
+        double const v = t.rate[kEBrems].interpolate(e) * 1.0;
+        if (!std::isfinite(v) || v < 0.0) fail(ORC_ERR_INVALID, "inverse length invalid");
+        double const mfp = v > 0.0 ? 1.0 / v : 1.0e99;
+        br = mfp > 0.0 ? 1.0 / mfp : 0.0;
+    }
+    out[0] = frozen(hm);
+    out[1] = frozen(br);
+    out[2] = 0.0;
+}
+
+// RreaChargedChannelOpenAt OV/RreaChargedFinalStates.H:56-96 (electron)
+inline bool electron_channel_open(Tables const& t, double thr, double gcut, int ch, double e, bool hm_on)
+{
+    if (!std::isfinite(e) || e < t.charged_min || e > t.charged_max) {
+        fail(ORC_ERR_TABLE_RANGE, "schema-6 electron energy outside certified range");
+    }
+    // ElectronTransportCoefficients :2298-2324 evaluates all five rates.
+    double const hm_inv = t.rate[kEMoller].interpolate(e) * 1.0;
+    double const br_inv = t.rate[kEBrems].interpolate(e) * 1.0;
+    (void)t.rate[kECol].interpolate(e);
+    (void)t.rate[kESoft].interpolate(e);
+    (void)t.rate[kEEltr].interpolate(e);
+    if (ch == 0) return hm_on && thr > 0.0 && e > 2.0 * thr && hm_inv > 0.0;
+    if (ch == 1) return std::isfinite(e) && std::isfinite(gcut) && e > gcut && br_inv > 0.0;
+    return false;
+}
+
+// elastic_inverse_length_stp OV/RreaTransportStep.H:199-208
+inline double elastic_inverse_length(Tables const& t, bool positron, double e)
+{
+    double const mfp = t.elastic_mfp(positron, e, 1.0);
+    return frozen(mfp > 0.0 ? 1.0 / mfp : 0.0);
+}
+
+// electron_realized_continuous_loss_ev_per_m :210-224
+inline double realized_loss(Tables const& t, double e, double ratio)
+{
+    if (!(e > 0.0) || !(ratio > 0.0)) return 0.0;
+    return t.rate[kECol].interpolate(e) * ratio + t.rate[kESoft].interpolate(e) * ratio;
+}
+
+// electron_ensemble_mean_drag_ev_per_m :226-247
+inline double ensemble_drag(Tables const& t, double e, double ratio, bool hm_on)
+{
+    if (!(e > 0.0) || !(ratio > 0.0)) return 0.0;
+    double const dd = hm_on ? t.hard_moller_energy_loss(e, ratio) : 0.0;
+    return realized_loss(t, e, ratio) + dd + t.tracked_brems_energy_loss(e, ratio);
+}
+
+// One electron, the body of the particle loop in
+// OV/RreaElectronTransport.cpp:59-889, with the shared prologue
+// BeginRunningChargedParticle OV/RreaTransportStep.H:720-820.
+struct ElectronOut {
+    double r, z, th, ux, uy, uz, ion_pairs;
+    int status;
+    uint64_t substeps;
+};
+
+inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, double pr, double pz, double pth,
+                              double pux, double puy, double puz, double w, int64_t id, int32_t cpu,
+                              ElectronOut& out)
+{
+    Tables const& t = *ctx.t;
+    orc_config const& cfg = ctx.cfg;
+    double const cutoff = cfg.low_energy_cutoff_eV;
+    double const gcut = cfg.photon_cutoff_eV;
+    double const thr = t.hard_moller_threshold;
+    bool const hm_on = cfg.disable_hard_moller == 0;
+    double const dt = cfg.dt_s;
+    uint64_t const step = static_cast<uint64_t>(std::max<int64_t>(cfg.step, 0));
+    uint64_t const seed = cfg.rng_seed;
+
+    out.ion_pairs = 0.0;
+    out.substeps = 0;
+    uint64_t const pkey = static_cast<uint64_t>(std::llabs(id)) ^ (static_cast<uint64_t>(static_cast<uint32_t>(cpu)) << 40);
+    if (w <= 0.0) { out.status = ORC_ZERO_WEIGHT; return; }
+    if (pr > ctx.r_hi + 1.0e-9 * ctx.r_hi) fail(ORC_ERR_OUTSIDE_DOMAIN, "electron survived outside the domain");
+
+    double dx = pux, dy = puy, dz = puz;
+    double k = kinetic_from_proper_velocity(dx, dy, dz);
+    normalize(dx, dy, dz);
+    double run_x = pr * std::cos(pth);
+    double run_y = pr * std::sin(pth);
+    double run_z = pz;
+    double const r_particle = pr, z_particle = pz;
+
+    Chord const driver = build_chord(run_x, run_y, run_z, dx, dy, dz, k, dt, dt, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
+    Path const& dpath = driver.path;
+    EscapePolicy escape = clipped_escape_policy(dpath, std::hypot(dpath.ex, dpath.ey), dpath.ez);
+    double const ds = driver.valid ? dpath.in_length : 0.0;
+    std::vector<Interval> const div = build_intervals(ctx.cp, dpath, ctx.density_ratio);
+    double const span = std::max(dpath.end_f, 1.0e-6);
+    double const field_work = path_field_work(ctx.cp, dpath, -1.0, 0.0, 1.0).work / span;
+
+    double const chord_dp = density_path(div, dpath, dpath.begin_f, dpath.end_f);
+    double particle_ratio = ds > 0.0 ? chord_dp / ds : positive_or(ctx.cp.density_ratio_at_z(z_particle), ctx.density_ratio);
+    double tr_dp = 0.0, tr_len = 0.0;
+    uint64_t lineage = 0;  // spawn_transient_lineage, OV/RreaTransportLog.H:179,212,445
+    Path path = dpath;
+    std::vector<Interval> iv = div;
+
+    auto ion_event = [&](double ev_w) { out.ion_pairs += ev_w; fx.tallies[ORC_T_ION_PAIRS] += ev_w; };
+    auto deposit_local = [&](double energy) {
+        if (w <= 0.0 || energy <= 0.0) return;
+        double const pw = w * energy / std::max(kWAir, 1.0);
+        if (pw <= 0.0) return;
+        ion_event(pw);
+    };
+    auto segment = [&](double r0, double z0, double r1, double z1) {
+        DepositOp op{0, {r0, z0, r1, z1, -w, 0.0, 0.0}, 0U};
+        fx.deposits.push_back(op);
+    };
+    auto commit_escape = [&]() {
+        double const um = proper_velocity_from_kinetic(k);
+        out.ux = um * dx; out.uy = um * dy; out.uz = um * dz;
+        out.r = pr; out.z = pz; out.th = pth;
+        BoundaryCrossing const bc = boundary_crossing(path);
+        if (!bc.valid) fail(ORC_ERR_INVALID, "charged escape has no certified absorbing-face crossing");
+        DepositOp op{1, {r_particle, z_particle, bc.start_r, bc.start_z, bc.cross_r, bc.cross_z, -w}, bc.faces};
+        fx.deposits.push_back(op);
+        out.status = ORC_ESCAPE;
+        fx.tallies[ORC_T_ESCAPES] += 1.0;
+    };
+    auto demote = [&](double rr, double zz) {
+        deposit_local(k);
+        segment(r_particle, z_particle, rr, zz);
+        out.r = pr; out.z = pz; out.th = pth; out.ux = pux; out.uy = puy; out.uz = puz;
+        out.status = ORC_DEMOTED;
+        fx.tallies[ORC_T_DEMOTIONS] += 1.0;
+    };
+
+    // Initial terminal disposition :129-175
+    bool const below0 = k < cutoff;
+    if (escape.escapes && below0) { commit_escape(); return; }
+    if (!escape.escapes && below0) { demote(r_particle, z_particle); return; }
+    if (ds <= 0.0) {
+        if (escape.escapes) { commit_escape(); return; }
+        out.r = pr; out.z = pz; out.th = pth; out.ux = pux; out.uy = puy; out.uz = puz;
+        out.status = ORC_NO_WRITEBACK;
+        return;
+    }
+    if (div.empty()) fail(ORC_ERR_INVALID, "positive-length path without material segments");
+
+    // Substep sizing :187-279
+    double stp0[3];
+    electron_stp_rates(t, thr, gcut, k, hm_on, stp0);
+    double const ddp = chord_dp / span;
+    bool violation = false;
+    uint64_t subcycles = select_material_subcycles(stp0[0] * ddp + stp0[1] * ddp, div, dpath, cfg, violation,
+                                                   stp0[0] + stp0[1]);
+    {
+        double const gs_n_total = elastic_inverse_length(t, false, k) * ddp;
+        double const drag_est = realized_loss(t, std::max(k, cutoff), 1.0) * ddp;
+        double const impulse_scale = std::max(cfg.max_field_impulse_fraction * k, std::max(cfg.field_impulse_floor_eV, 1.0));
+        uint64_t const gsd = driver_count(std::min(gs_n_total, cfg.gs_total_collision_cap),
+                                          std::max(cfg.max_gs_collisions_per_substep, 1.0));
+        uint64_t const imd = driver_count(std::abs(field_work) + drag_est, impulse_scale);
+        subcycles = std::min<uint64_t>(std::max({subcycles, gsd, imd}), cfg.max_subcycles);
+    }
+    if (cfg.strict_transport_guards && violation) fail(ORC_ERR_GUARD, "strict transport guard: optical-depth cap exceeded");
+    out.substeps = subcycles;
+    fx.tallies[ORC_T_SUBSTEPS] += static_cast<double>(subcycles);
+
+    bool exited = false;
+    bool terminated = false;
+    double mid_r = r_particle, mid_z = z_particle;
+    uint64_t completed = 0;
+    auto elapsed_at = [&](double f) {
+        double const within = std::min(1.0, std::max(0.0, f));
+        return std::min(1.0, (static_cast<double>(completed) + within) / static_cast<double>(subcycles));
+    };
+    auto mark_stopped = [&](double f) {
+        if (path.exits && f >= path.end_f) { exited = true; return; }
+        double x, y, z;
+        path.point_at_full(f, x, y, z);
+        double const r = std::hypot(x, y);
+        if (!in_half_open(x, y, z, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi)) { exited = true; return; }
+        mid_z = z; mid_r = r; terminated = true;
+    };
+
+    // apply_electron_continuous_segment :294-400
+    struct SegLimit { double end; bool cutoff; };
+    auto continuous = [&](double b, double e, double stop_e) -> SegLimit {
+        if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return {b, false};
+        double const col = frozen(t.rate[kECol].interpolate(stop_e) * 1.0);
+        double const soft = frozen(t.rate[kESoft].interpolate(stop_e) * 1.0);
+        double const total = col + soft;
+        // RreaLimitContinuousSegmentAtCutoff OV/RreaParticleInteraction.H:203-265
+        auto loss_to = [&](double f) { return total * density_path(iv, path, b, f); };
+        double eff = b;
+        bool reaches = false;
+        {
+            if (!std::isfinite(b) || !std::isfinite(e) || !std::isfinite(k) || !std::isfinite(cutoff) || !(e >= b)
+                || !(k >= cutoff) || !(cutoff >= 0.0)) {
+                fail(ORC_ERR_INVALID, "continuous stopping could not locate a cutoff point");
+            }
+            double const to_cut = k - cutoff;
+            double const req = loss_to(e);
+            if (!std::isfinite(req) || req < 0.0) fail(ORC_ERR_INVALID, "continuous stopping invalid");
+            if (req < to_cut) {
+                eff = e;
+            } else {
+                reaches = true;
+                if (to_cut != 0.0) {
+                    double lo = b, hi = e;
+                    for (int it = 0; it < 56; ++it) {
+                        double const mid = 0.5 * (lo + hi);
+                        double const l = loss_to(mid);
+                        if (!std::isfinite(l) || l < 0.0) fail(ORC_ERR_INVALID, "continuous stopping invalid");
+                        if (l < to_cut) lo = mid; else hi = mid;
+                    }
+                    eff = 0.5 * (lo + hi);
+                }
+            }
+        }
+        double const pm = density_path(iv, path, b, eff);
+        double const raw_col = col * pm;
+        double const raw_soft = soft * pm;
+        double const avail = std::max(0.0, k - cutoff);
+        double const drag = reaches ? avail : std::min(raw_col + raw_soft, avail);
+        double const col_loss = std::min(raw_col, drag);
+        // RreaPartitionContinuousIonPairs OV/RreaChargedInterleave.H:157-196
+        if (col_loss > 0.0 && raw_col > 0.0) {
+            for (auto const& it : iv) {
+                double const lb = std::max(b, it.begin), le = std::min(eff, it.end);
+                if (!(le > lb)) continue;
+                double const raw_i = path.full_length * (le - lb) * it.ratio * col;
+                double const loss_i = col_loss * raw_i / raw_col;
+                double const pw = w * loss_i / std::max(kWAir, 1.0);
+                if (!(pw > 0.0)) continue;
+                ion_event(pw);
+            }
+        }
+        fx.tallies[ORC_T_DRAG_LOSS_WEV] += w * drag;
+        k = std::max(0.0, k - drag);
+        if (reaches) k = cutoff;
+        return {eff, reaches};
+    };
+
+    double const sub_dt = dt / static_cast<double>(subcycles);
+    FieldRange sfield;
+    double substep_dt = 0.0;
+    double ex_ = 0.0, ey_ = 0.0, ez_ = 0.0, er_ = 0.0;
+    (void)er_;
+    uint64_t idx = 0;
+    double stp[3] = {0, 0, 0};
+
+    // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:686-732
+    for (uint64_t sc = 0; sc < subcycles; ++sc) {
+        if (k < cutoff) break;
+        double sb = 0.0, se = 0.0;
+        // build_substep :535-598
+        {
+            Chord const ch = build_chord(run_x, run_y, run_z, dx, dy, dz, k, sub_dt, dt, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
+            if (!ch.valid) break;
+            path = ch.path;
+            if (!(path.full_length > 0.0) || !(path.in_length > 0.0)) { exited = path.exits; break; }
+            iv = build_intervals(ctx.cp, path, ctx.density_ratio);
+            if (iv.empty()) fail(ORC_ERR_INVALID, "microchord without material intervals");
+            for (auto const& it : iv) { tr_dp += it.length * it.ratio; tr_len += it.length; }
+            sb = path.begin_f;
+            se = path.end_f;
+            sfield = path_field_work(ctx.cp, path, -1.0, sb, se);
+            substep_dt = sub_dt * (se - sb);
+            fx.tallies[ORC_T_FIELD_WORK_WEV] += w * half_kick(sfield, 0.5 * substep_dt, -1.0, k, dx, dy, dz);
+            if (k < cutoff) { mark_stopped(sb); break; }
+        }
+        // run_events :599-752 via RreaRunChargedEventInterleave :546-664
+        {
+            double rem = sb;
+            uint64_t ord = 0;
+            while (rem < se && k >= cutoff) {
+                if (ord >= kEventCap) fail(ORC_ERR_INVALID, "event ordinal overflow");
+                double const frozen_e = k;
+                electron_stp_rates(t, thr, gcut, frozen_e, hm_on, stp);
+                double const L = density_path(iv, path, rem, se);
+                double const taus[3] = {stp[0] * L, stp[1] * L, 0.0};
+                if (!live_hazard_index(0U, sc, ord, idx)) fail(ORC_ERR_INVALID, "hazard RNG index overflow");
+                double const u1 = rng_uniform(seed, pkey, step, idx, ord == 0U ? kChCompetingOpticalDepth : kChContinuationOpticalDepth);
+                double const u2 = rng_uniform(seed, pkey, step, idx, ord == 0U ? kChCompetingChannel : kChContinuationChannel);
+                Hazard const h = select_hazard(taus, 2, u1, u2);
+                if (!h.valid) fail(ORC_ERR_INVALID, "competing-hazard selection invalid");
+                if (!h.event) {
+                    SegLimit const sl = continuous(rem, se, frozen_e);
+                    if (sl.cutoff) mark_stopped(sl.end);
+                    break;
+                }
+                double const ef = fraction_at_density_path(iv, path, rem, se, h.sampled / (stp[0] + stp[1]));
+                if (!(std::isfinite(rem) && std::isfinite(ef) && std::isfinite(se) && se > rem && ef >= rem && ef <= se)) {
+                    fail(ORC_ERR_INVALID, "sampled event depth outside its subcycle");
+                }
+                path.point_at_full(ef, ex_, ey_, ez_);
+                er_ = std::hypot(ex_, ey_);
+                SegLimit const sl = continuous(rem, ef, frozen_e);
+                if (sl.cutoff) { mark_stopped(sl.end); break; }
+                bool const open_ev = k >= cutoff && electron_channel_open(t, thr, gcut, h.channel, k, hm_on);
+                if (!open_ev) {
+                    bool const open_fr = frozen_e >= cutoff && electron_channel_open(t, thr, gcut, h.channel, frozen_e, hm_on);
+                    if (!open_fr) fail(ORC_ERR_INVALID, "selected channel closed before its sampled depth");
+                    rem = ef;
+                    ++ord;
+                    continue;
+                }
+                // final_state :719-751
+                double const efrac = elapsed_at(ef);
+                if (h.channel == 0) {
+                    // RreaApplyHardMollerFinalState OV/RreaChargedFinalStates.H:164-225
+                    double const before = k;
+                    double T = t.sample_moller(before, rng_uniform(seed, pkey, step, idx, kChHardMollerEnergy));
+                    T = std::max(thr, std::min(T, 0.5 * before));
+                    double pdx, pdy, pdz, sdx, sdy, sdz;
+                    moller_lab_directions(dx, dy, dz, before, T, rng_uniform(seed, pkey, step, idx, kChHardMollerAzimuth),
+                                          pdx, pdy, pdz, sdx, sdy, sdz);
+                    k = std::max(0.0, before - T);
+                    dx = pdx; dy = pdy; dz = pdz;
+                    orc_secondary s{ip, 0, 2, ex_, ey_, ez_, sdx, sdy, sdz, w, T, efrac, pkey, lineage++};
+                    fx.secondaries.push_back(s);
+                    ion_event(w);
+                    fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
+                } else if (h.channel == 1) {
+                    // RreaApplyBremsFinalState OV/RreaChargedFinalStates.H:329-394
+                    double const before = k;
+                    double const eu = rng_uniform(seed, pkey, step, idx, kChBremsPhotonEnergy);
+                    double const ks = t.sample_brems_energy(before, eu);
+                    // RreaClassifyBremsEnergyDisposition OV/RreaParticleInteraction.H:1552-1574
+                    double const pe = std::min(std::max(ks, 0.0), std::max(before, 0.0));
+                    double const after = std::max(0.0, std::max(before, 0.0) - pe);
+                    bool const tracked = std::isfinite(gcut) && gcut >= 0.0 && pe >= gcut;
+                    double const local = tracked ? 0.0 : pe;
+                    double gx, gy, gz;
+                    cone_direction(dx, dy, dz, t.sample_brems_cos(before, rng_uniform(seed, pkey, step, idx, kChBremsPhotonAngle)),
+                                   rng_uniform(seed, pkey, step, idx, kChBremsPhotonAzimuth), gx, gy, gz);
+                    photon_recoil(before, pe, gx, gy, gz, dx, dy, dz);
+                    k = after;
+                    if (tracked) {
+                        orc_secondary s{ip, 2, 4, ex_, ey_, ez_, gx, gy, gz, w, pe, efrac, pkey, lineage++};
+                        fx.secondaries.push_back(s);
+                        fx.tallies[ORC_T_BREMS_PHOTONS] += 1.0;
+                    } else if (local > 0.0) {
+                        deposit_local(local);
+                    }
+                    fx.tallies[ORC_T_BREMS_EVENTS] += 1.0;
+                }
+                if (k <= cutoff) { mark_stopped(ef); break; }
+                rem = ef;
+                ++ord;
+            }
+        }
+        if (terminated) break;
+        // finish_interleaved_substep :502-524
+        {
+            if (k >= cutoff) {
+                // apply_gs_elastic_scatter OV/RreaTransportStep.H:372-423
+                double const gs_n = elastic_inverse_length(t, false, k) * density_path(iv, path, sb, se);
+                if (gs_n > 0.0) {
+                    auto U = [&](uint64_t chan, uint64_t inner) {
+                        return rng_uniform(seed, pkey, step, (sc << 8) | inner, chan);
+                    };
+                    auto deflect = [&](double ct, double uphi) {
+                        double nx, ny, nz;
+                        cone_direction(dx, dy, dz, ct, uphi, nx, ny, nz);
+                        dx = nx; dy = ny; dz = nz;
+                    };
+                    if (gs_n < t.gs_few) {
+                        int const cap = static_cast<int>(std::ceil(3.0 * t.gs_few));
+                        if (cap >= 255) fail(ORC_ERR_INVALID, "GS few-collision cap does not pack");
+                        int const cnt = poisson_capped(gs_n, U(kChGsFewCount, 0), cap);
+                        for (int q = 0; q < cnt; ++q) {
+                            double const ct = t.sample_single_elastic_cos(false, k, U(kChGsFewScatter, static_cast<uint64_t>(q + 1)));
+                            deflect(ct, U(kChGsFewAzimuth, static_cast<uint64_t>(q + 1)));
+                        }
+                    } else {
+                        double const ct = t.sample_gs_cos(false, k, gs_n, U(kChGsAngle, 0));
+                        deflect(ct, U(kChGsAzimuth, 0));
+                    }
+                }
+            }
+            fx.tallies[ORC_T_FIELD_WORK_WEV] += w * half_kick(sfield, 0.5 * substep_dt, -1.0, k, dx, dy, dz);
+            if (k < cutoff) { mark_stopped(se); break; }
+        }
+        if (!advance_running_position(path, se, run_x, run_y, run_z)) { exited = true; break; }
+        if (path.exits) { exited = true; break; }
+        completed = sc + 1U;
+    }
+
+    if (exited) escape = clipped_escape_policy(path, std::hypot(path.ex, path.ey), path.ez);
+    if (tr_len > 0.0) particle_ratio = tr_dp / tr_len;
+    double const end_r = std::hypot(run_x, run_y);
+    double fer, fez, fbt;
+    ctx.cp.gather(end_r, run_z, fer, fez, fbt);
+    // RreaEvaluateElectronStatePolicy OV/RreaParticleInteraction.H:1279-1317.  The
+    // ensemble drag is read only when K <= the hard-Moller threshold; it has no
+    // side effect, so evaluating it only there is exact.
+    bool const below = k < cutoff;
+    bool nonrunaway = false;
+    if (thr > 0.0 && k <= thr) {
+        double const thr_v = ensemble_drag(t, std::max(k, cutoff), particle_ratio, hm_on);
+        nonrunaway = std::hypot(fer, fez) <= thr_v;
+    }
+    // RreaResolveChargedTransportTerminal :292-312
+    if (exited) { commit_escape(); return; }
+    if (terminated || below || nonrunaway) {
+        double const dr_ = terminated ? mid_r : end_r;
+        double const dz_ = terminated ? mid_z : run_z;
+        demote(dr_, dz_);
+        return;
+    }
+    // CommitSurvivorChargedWriteback OV/RreaTransportStep.H:874-905
+    double const um = proper_velocity_from_kinetic(k);
+    out.ux = um * dx; out.uy = um * dy; out.uz = um * dz;
+    out.r = end_r;
+    out.th = std::atan2(run_y, run_x);
+    out.z = run_z;
+    segment(r_particle, z_particle, end_r, run_z);
+    out.status = ORC_SURVIVE;
+    fx.tallies[ORC_T_SURVIVORS] += 1.0;
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C entry points (ctypes)
+// ---------------------------------------------------------------------------
+using namespace orc;
+
+extern "C" {
+
+uint64_t orc_rng_counter(uint64_t seed, uint64_t pid, uint64_t step, uint64_t idx, uint64_t chan)
+{
+    return rng_counter(seed, pid, step, idx, chan);
+}
+
+double orc_rng_uniform(uint64_t seed, uint64_t pid, uint64_t step, uint64_t idx, uint64_t chan)
+{
+    return rng_uniform(seed, pid, step, idx, chan);
+}
+
+void orc_uniform_stream(double u1, double u2, int64_t n, double* out)
+{
+    UniformStream s(u1, u2);
+    for (int64_t i = 0; i < n; ++i) out[i] = s.next();
+}
+
+void* orc_tables_create() { return new Tables(); }
+void orc_tables_destroy(void* h) { delete static_cast<Tables*>(h); }
+
+// Rate table slot `id` (RateId order), x/y arrays of length n, interpolation mode.
+void orc_tables_set_rate(void* h, int id, const double* x, const double* y, int64_t n, int mode)
+{
+    auto* t = static_cast<Tables*>(h);
+    RateTable& r = t->rate[id];
+    r.x.assign(x, x + n);
+    r.y.assign(y, y + n);
+    r.mode = mode;
+    r.finalize();
+}
+
+// Conditional CDF `which`: 0 moller, 1 bhabha, 2 e- brems, 3 e+ brems.
+void orc_tables_set_cdf(void* h, int which, const double* primary, int64_t ngroups, const int64_t* start,
+                        const double* u, const double* v)
+{
+    auto* t = static_cast<Tables*>(h);
+    CdfGroups* g = which == 0 ? &t->moller : which == 1 ? &t->bhabha : which == 2 ? &t->ebrems : &t->pbrems;
+    g->primary.assign(primary, primary + ngroups);
+    g->u.clear();
+    g->v.clear();
+    for (int64_t i = 0; i < ngroups; ++i) {
+        g->u.emplace_back(u + start[i], u + start[i + 1]);
+        g->v.emplace_back(v + start[i], v + start[i + 1]);
+    }
+}
+
+// Angle table `which`: 0 GS e-, 1 GS e+, 2 Wentzel e-, 3 Wentzel e+.
+void orc_tables_set_angle(void* h, int which, const double* e, int64_t ne, const double* n, int64_t nn,
+                          const double* u, int64_t nu, const double* omc)
+{
+    auto* t = static_cast<Tables*>(h);
+    AngleTable* a = which == 0 ? &t->gs_e : which == 1 ? &t->gs_p : which == 2 ? &t->wz_e : &t->wz_p;
+    a->e.assign(e, e + ne);
+    a->n.assign(n, n + nn);
+    a->u.assign(u, u + nu);
+    a->omc.assign(omc, omc + ne * nn * nu);
+}
+
+void orc_tables_set_elastic(void* h, int positron, const double* e, const double* mfp, const double* eta,
+                            const double* trmfp, int64_t n)
+{
+    auto* t = static_cast<Tables*>(h);
+    ElasticTable& el = positron ? t->el_p : t->el_e;
+    el.e.assign(e, e + n);
+    el.mfp.assign(mfp, mfp + n);
+    el.eta.assign(eta, eta + n);
+    el.trmfp.assign(trmfp, trmfp + n);
+}
+
+// scalars: hard-Moller threshold, brems cutoff, reference density, charged
+// min/max, photon min/max, GS few threshold, isotropic n, transition energy.
+int orc_tables_finalize(void* h, const double* s)
+{
+    auto* t = static_cast<Tables*>(h);
+    t->hard_moller_threshold = s[0];
+    t->brems_cutoff = s[1];
+    t->reference_density = s[2];
+    t->charged_min = s[3];
+    t->charged_max = s[4];
+    t->photon_min = s[5];
+    t->photon_max = s[6];
+    t->gs_few = s[7];
+    t->gs_iso = s[8];
+    t->gs_transition = s[9];
+    try {
+        t->build_hm_mean();
+    } catch (OracleError const& e) {
+        return e.code;
+    }
+    return 0;
+}
+
+const double* orc_tables_hm_mean(void* h) { return static_cast<Tables*>(h)->hm_mean_secondary.data(); }
+
+#define ORC_GUARD(expr) \
+    try { expr; } catch (OracleError const& e) { if (err) *err = e.code; return 0.0; }
+
+double orc_interpolate(void* h, int id, double e, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return static_cast<Tables*>(h)->rate[id].interpolate(e));
+}
+
+double orc_sample_moller(void* h, double e, double u, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return static_cast<Tables*>(h)->sample_moller(e, u));
+}
+
+double orc_sample_brems_energy(void* h, double e, double u, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return static_cast<Tables*>(h)->sample_brems_energy(e, u));
+}
+
+double orc_sample_brems_cos(void* h, double e, double u, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return static_cast<Tables*>(h)->sample_brems_cos(e, u));
+}
+
+double orc_sample_gs_cos(void* h, int positron, double e, double n, double u, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return static_cast<Tables*>(h)->sample_gs_cos(positron != 0, e, n, u));
+}
+
+double orc_sample_single_elastic_cos(void* h, int positron, double e, double u, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return static_cast<Tables*>(h)->sample_single_elastic_cos(positron != 0, e, u));
+}
+
+double orc_elastic_mfp(void* h, int positron, double e, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return static_cast<Tables*>(h)->elastic_mfp(positron != 0, e, 1.0));
+}
+
+double orc_ensemble_drag(void* h, double e, double ratio, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return ensemble_drag(*static_cast<Tables*>(h), e, ratio, true));
+}
+
+double orc_hard_moller_energy_loss(void* h, double e, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return static_cast<Tables*>(h)->hard_moller_energy_loss(e, 1.0));
+}
+
+// Chord clip: out = [sx,sy,sz,ex,ey,ez,full,begin,end,in_len,exits,npieces]
+int orc_clip_path(double sr, double st, double sz, double er_, double et, double ez_, double r_lo, double r_hi,
+                  double z_lo, double z_hi, double* out)
+{
+    try {
+        Path const p = clip_rz_path(sr, st, sz, er_, et, ez_, r_lo, r_hi, z_lo, z_hi);
+        double const v[12] = {p.sx, p.sy, p.sz, p.ex, p.ey, p.ez, p.full_length, p.begin_f, p.end_f,
+                              p.in_length, p.exits ? 1.0 : 0.0, static_cast<double>(p.pieces.size())};
+        std::memcpy(out, v, sizeof(v));
+        return 0;
+    } catch (OracleError const& e) {
+        return e.code;
+    }
+}
+
+void orc_gather(const orc_geometry* g, double r, double z, double* out3)
+{
+    Coupling cp{*g, 1.0, 0.0, 0.0};
+    cp.gather(r, z, out3[0], out3[1], out3[2]);
+}
+
+// Index-space deposit of one segment on a grid of nr x nz cells.
+void orc_deposit_segment(int nr, int nz, double dr, double dz, double r_lo, double z_lo, double* rho, double* jr,
+                         double* jz, double q, double dt, double r0, double z0, double r1, double z1)
+{
+    Grid g{nr, nz, dr, dz, r_lo, z_lo, rho, jr, jz};
+    deposit_segment(g, q, dt, r0, z0, r1, z1);
+}
+
+int orc_deposit_escaping_segment(int nr, int nz, double dr, double dz, double r_lo, double z_lo, double* rho,
+                                 double* jr, double* jz, double q, double dt, const double* a7, int pnr, int pzb,
+                                 int pnz, uint32_t faces)
+{
+    Grid g{nr, nz, dr, dz, r_lo, z_lo, rho, jr, jz};
+    return deposit_escaping_segment(g, q, dt, a7[0], a7[1], a7[2], a7[3], a7[4], a7[5], pnr, pzb, pnz, faces) ? 1 : 0;
+}
+
+// One electron transport step over n particles.  prev_* is the pre-push
+// snapshot the reference reads; out_* receives the written-back state.  The
+// padded net-chord grid is (nr+2) x (nz+4) with two low-z pad cells
+// (OV/RreaWarpXCoupling.cpp:417-444); jr/jz/rho are accumulated (+=).
+int orc_electron_step(void* h, const orc_geometry* g, const orc_config* cfg, int64_t n, const double* pr,
+                      const double* pz, const double* pth, const double* pux, const double* puy, const double* puz,
+                      const double* w, const int64_t* id, const int32_t* cpu, double* out_r, double* out_z,
+                      double* out_th, double* out_ux, double* out_uy, double* out_uz, int32_t* status,
+                      double* ion_pairs, int64_t* substeps, int64_t sec_cap, orc_secondary* sec, int64_t* n_sec,
+                      double* jr, double* jz, double* rho, double* tallies, int nthreads)
+{
+    auto const* t = static_cast<Tables const*>(h);
+    if (g->r_lo != 0.0) return ORC_ERR_INVALID;
+    StepContext ctx;
+    ctx.t = t;
+    ctx.cp = Coupling{*g, t->reference_density, 0.0, 0.0};
+    ctx.cfg = *cfg;
+    ctx.r_lo = g->r_lo;
+    ctx.r_hi = g->r_lo + g->nr * g->dr;
+    ctx.z_lo = g->z_lo;
+    ctx.z_hi = g->z_lo + g->nz * g->dz;
+    ctx.cp.r_hi = ctx.r_hi;
+    ctx.cp.z_hi = ctx.z_hi;
+    // RreaParticleInteraction::Apply :29-35: ratio at the domain's mid z, or 1.
+    try {
+        ctx.density_ratio = positive_or(ctx.cp.density_ratio_at_z(0.5 * (ctx.z_lo + ctx.z_hi)), 1.0);
+    } catch (OracleError const& e) {
+        *n_sec = 0;
+        return e.code;
+    }
+    if (!(cfg->dt_s > 0.0)) { *n_sec = 0; return 0; }
+
+    constexpr int64_t chunk = 64;
+    int64_t const nchunks = (n + chunk - 1) / chunk;
+    std::vector<ChunkLog> logs(static_cast<size_t>(std::max<int64_t>(nchunks, 0)));
+    std::atomic<int> error{0};
+#ifdef _OPENMP
+    int const threads = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) if (threads > 1 && nchunks > 1)
+#endif
+    for (int64_t c = 0; c < nchunks; ++c) {
+        ChunkLog& fx = logs[static_cast<size_t>(c)];
+        int64_t const b = c * chunk, e = std::min(n, b + chunk);
+        for (int64_t i = b; i < e; ++i) {
+            if (error.load(std::memory_order_relaxed) != 0) break;
+            ElectronOut o{};
+            if (id[i] == std::numeric_limits<int64_t>::min()) {
+                // an invalidated particle: the reference loop `continue`s (RreaElectronTransport.cpp:60-63)
+                out_r[i] = pr[i]; out_z[i] = pz[i]; out_th[i] = pth[i];
+                out_ux[i] = pux[i]; out_uy[i] = puy[i]; out_uz[i] = puz[i];
+                status[i] = 5;
+                ion_pairs[i] = 0.0;
+                substeps[i] = 0;
+                continue;
+            }
+            try {
+                electron_particle(ctx, fx, i, pr[i], pz[i], pth[i], pux[i], puy[i], puz[i], w[i], id[i], cpu[i], o);
+            } catch (OracleError const& err) {
+                int expected = 0;
+                error.compare_exchange_strong(expected, err.code);
+                break;
+            }
+            if (o.status == ORC_ZERO_WEIGHT) {
+                o.r = pr[i]; o.z = pz[i]; o.th = pth[i]; o.ux = pux[i]; o.uy = puy[i]; o.uz = puz[i];
+            }
+            out_r[i] = o.r; out_z[i] = o.z; out_th[i] = o.th;
+            out_ux[i] = o.ux; out_uy[i] = o.uy; out_uz[i] = o.uz;
+            status[i] = o.status;
+            ion_pairs[i] = o.ion_pairs;
+            substeps[i] = static_cast<int64_t>(o.substeps);
+        }
+    }
+    (void)nthreads;
+    if (error.load() != 0) { *n_sec = 0; return error.load(); }
+
+    // Deterministic merge in chunk order, then replay of the deposits
+    // (OV/RreaElectronTransport.cpp:892-897, OV/RreaTransportLog.H:603-653).
+    int const pnr = g->nr, pnz = g->nz, pad = 2;
+    Grid grid{pnr + pad, pnz + 2 * pad, g->dr, g->dz, g->r_lo, g->z_lo - pad * g->dz, rho, jr, jz};
+    int64_t ns = 0;
+    for (auto const& fx : logs) {
+        for (auto const& s : fx.secondaries) {
+            if (ns >= sec_cap) { *n_sec = ns; return ORC_ERR_SECONDARY_CAPACITY; }
+            sec[ns++] = s;
+        }
+        for (int q = 0; q < ORC_T_COUNT; ++q) tallies[q] += fx.tallies[q];
+        for (auto const& op : fx.deposits) {
+            if (op.kind == 0) {
+                if (op.a[4] != 0.0) deposit_segment(grid, op.a[4], cfg->dt_s, op.a[0], op.a[1], op.a[2], op.a[3]);
+            } else {
+                if (op.a[6] != 0.0
+                    && !deposit_escaping_segment(grid, op.a[6], cfg->dt_s, op.a[0], op.a[1], op.a[2], op.a[3],
+                                                 op.a[4], op.a[5], pnr, pad, pnz, op.faces)) {
+                    *n_sec = ns;
+                    return ORC_ERR_INVALID;
+                }
+            }
+        }
+    }
+    *n_sec = ns;
+    return 0;
+}
+
+}  // extern "C"

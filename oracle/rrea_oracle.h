@@ -1,0 +1,110 @@
+/* CPU oracle for the RREA per-step charged transport -- TEST INFRASTRUCTURE.
+ *
+ * A plain C++17 restatement of the reference's electron transport step
+ * (native_amrex_rrea/warpx_overlay/Source/Rrea/RreaElectronTransport.cpp and
+ * the headers it includes).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library; the shipped HIP path
+ * never links or calls it.
+ *
+ * Every struct here is the oracle's own; the product's C-ABI declares its
+ * own equivalents in include/rrea_hip.h.  Tests feed both the same inputs.
+ */
+#ifndef RREA_ORACLE_H
+#define RREA_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_geometry {
+    int32_t nr;          /* physical cells in r */
+    int32_t nz;          /* physical cells in z */
+    double dr;
+    double dz;
+    double r_lo;         /* must be 0 (the RZ axis) */
+    double z_lo;
+    const double* er;    /* Maxwell Er on r-faces, (nr+1)*nz, f = i + (nr+1)*j */
+    const double* ez;    /* Maxwell Ez on z-faces, nr*(nz+1), f = i + nr*j */
+    const double* amb_er;/* ambient Er, same layout as er */
+    const double* amb_ez;/* ambient Ez, same layout as ez */
+    const double* btheta;/* Btheta on nodes, (nr+1)*(nz+1), c = i + (nr+1)*j */
+    /* density: mode 0 = constant ratio; 1 = altitude profile (absolute kg/m^3) */
+    int32_t density_mode;
+    int32_t profile_n;
+    double density_ratio;
+    const double* profile_alt_m;
+    const double* profile_rho;
+    double altitude_at_z0_m;
+} orc_geometry;
+
+typedef struct orc_config {
+    uint64_t rng_seed;
+    int64_t step;
+    double dt_s;
+    double low_energy_cutoff_eV;
+    double photon_cutoff_eV;
+    double max_optical_depth_per_substep;
+    uint64_t max_subcycles;
+    int32_t strict_transport_guards;
+    int32_t disable_hard_moller;
+    double max_gs_collisions_per_substep;
+    double gs_total_collision_cap;
+    double max_field_impulse_fraction;
+    double field_impulse_floor_eV;
+} orc_config;
+
+typedef struct orc_secondary {
+    int64_t parent_index;
+    int32_t species;      /* 0 e-, 1 e+, 2 photon */
+    int32_t process;      /* RreaSecondaryProcess id */
+    double x, y, z;
+    double dir_x, dir_y, dir_z;
+    double weight;
+    double energy_eV;
+    double birth_fraction;
+    uint64_t root_key;
+    uint64_t lineage;
+} orc_secondary;
+
+/* Per-particle status written by the step. */
+enum {
+    ORC_SURVIVE = 0,
+    ORC_ESCAPE = 1,
+    ORC_DEMOTED = 2,
+    ORC_ZERO_WEIGHT = 3,
+    ORC_NO_WRITEBACK = 4,
+    ORC_SKIPPED_INVALID = 5,
+};
+
+/* Tally slots (sums over the step). */
+enum {
+    ORC_T_SURVIVORS = 0,
+    ORC_T_ESCAPES,
+    ORC_T_DEMOTIONS,
+    ORC_T_MOLLER_EVENTS,
+    ORC_T_BREMS_EVENTS,
+    ORC_T_BREMS_PHOTONS,
+    ORC_T_SUBSTEPS,
+    ORC_T_DRAG_LOSS_WEV,
+    ORC_T_ION_PAIRS,
+    ORC_T_FIELD_WORK_WEV,
+    ORC_T_COUNT = 16,
+};
+
+/* Error codes; 0 means success. */
+enum {
+    ORC_OK = 0,
+    ORC_ERR_TABLE_RANGE = 1,
+    ORC_ERR_OUTSIDE_DOMAIN = 2,
+    ORC_ERR_GUARD = 3,
+    ORC_ERR_INVALID = 4,
+    ORC_ERR_SECONDARY_CAPACITY = 5,
+};
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,148 @@
+"""ctypes binding of ``librrea_hip.so`` -- the C-ABI declared in
+``include/rrea_hip.h``.
+
+The structs below mirror the header field by field; a CPU test compiles the
+header with the host compiler and checks every size and offset against them.
+There is no fallback: if the library is missing the import of the transport
+module fails with the build command to run.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librrea_hip.so")
+
+P_D = C.POINTER(C.c_double)
+P_I64 = C.POINTER(C.c_int64)
+
+
+class RateColumn(C.Structure):
+    _fields_ = [("x", P_D), ("y", P_D), ("n", C.c_int64), ("mode", C.c_int32)]
+
+
+class CdfGroups(C.Structure):
+    _fields_ = [("primary", P_D), ("n_groups", C.c_int64), ("start", P_I64), ("u", P_D), ("v", P_D)]
+
+
+class AngleTable(C.Structure):
+    _fields_ = [("e", P_D), ("n_e", C.c_int64), ("n", P_D), ("n_n", C.c_int64), ("u", P_D), ("n_u", C.c_int64),
+                ("omc", P_D)]
+
+
+class ElasticTable(C.Structure):
+    _fields_ = [("e", P_D), ("mfp_m_stp", P_D), ("screening_eta", P_D), ("n", C.c_int64)]
+
+
+class TableImage(C.Structure):
+    _fields_ = [
+        ("rate", RateColumn * 15),
+        ("moller", CdfGroups), ("bhabha", CdfGroups), ("electron_brems", CdfGroups), ("positron_brems", CdfGroups),
+        ("gs_electron", AngleTable), ("gs_positron", AngleTable),
+        ("wentzel_electron", AngleTable), ("wentzel_positron", AngleTable),
+        ("elastic_electron", ElasticTable), ("elastic_positron", ElasticTable),
+        ("hard_moller_secondary_threshold_eV", C.c_double),
+        ("brems_absolute_cutoff_eV", C.c_double),
+        ("reference_density_kg_m3", C.c_double),
+        ("charged_energy_range_eV", C.c_double * 2),
+        ("photon_energy_range_eV", C.c_double * 2),
+        ("gs_few_collision_threshold_n", C.c_double),
+        ("gs_isotropic_above_path_n", C.c_double),
+        ("gs_wentzel_transition_energy_eV", C.c_double),
+    ]
+
+
+class RzGeometry(C.Structure):
+    _fields_ = [("nr", C.c_int32), ("nz", C.c_int32), ("dr", C.c_double), ("dz", C.c_double),
+                ("r_lo", C.c_double), ("z_lo", C.c_double)]
+
+
+class StepConfigC(C.Structure):
+    _fields_ = [
+        ("rng_seed", C.c_uint64), ("step", C.c_int64), ("dt_s", C.c_double),
+        ("low_energy_cutoff_eV", C.c_double), ("photon_cutoff_eV", C.c_double),
+        ("max_optical_depth_per_substep", C.c_double), ("max_subcycles", C.c_uint64),
+        ("strict_transport_guards", C.c_int32), ("disable_hard_moller", C.c_int32),
+        ("max_gs_collisions_per_substep", C.c_double), ("gs_total_collision_cap", C.c_double),
+        ("max_field_impulse_fraction", C.c_double), ("field_impulse_floor_eV", C.c_double),
+    ]
+
+
+class Tile(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in (
+        "prev_r", "prev_z", "prev_theta", "prev_ux", "prev_uy", "prev_uz", "w", "id", "cpu",
+        "r", "z", "theta", "ux", "uy", "uz")]
+
+
+class Secondary(C.Structure):
+    _fields_ = [
+        ("parent_index", C.c_int64), ("species", C.c_int32), ("process", C.c_int32),
+        ("x", C.c_double), ("y", C.c_double), ("z", C.c_double),
+        ("dir_x", C.c_double), ("dir_y", C.c_double), ("dir_z", C.c_double),
+        ("weight", C.c_double), ("energy_eV", C.c_double), ("birth_fraction", C.c_double),
+        ("root_key", C.c_uint64), ("lineage", C.c_uint64),
+    ]
+
+
+class StepOut(C.Structure):
+    _fields_ = [
+        ("status", C.c_void_p), ("ion_pairs", C.c_void_p), ("substeps", C.c_void_p),
+        ("secondaries", C.c_void_p), ("secondary_capacity", C.c_int64), ("secondary_count", C.c_void_p),
+        ("jr", C.c_void_p), ("jz", C.c_void_p), ("rho", C.c_void_p), ("tallies", C.c_void_p),
+        ("error", C.c_void_p), ("order_secondaries", C.c_int32),
+    ]
+
+
+TALLY_COUNT = 16
+STATUS = {0: "survive", 1: "escape", 2: "demoted", 3: "zero_weight", 4: "no_writeback", 5: "skipped_invalid"}
+INVALID_ID = -(2**63)
+ERRORS = {0: "ok", 1: "table range", 2: "outside domain", 3: "strict guard", 4: "invalid", 5: "secondary capacity",
+          16: "argument", 17: "hip"}
+
+# (symbol, restype, argtypes) for every entry point of include/rrea_hip.h
+SIGNATURES = [
+    ("rrea_tables_create", C.c_int32, [C.POINTER(TableImage), C.POINTER(C.c_void_p)]),
+    ("rrea_tables_destroy", C.c_int32, [C.c_void_p]),
+    ("rrea_tables_hm_mean", C.c_int32, [C.c_void_p, P_D, C.c_int64]),
+    ("rrea_fields_create", C.c_int32, [C.POINTER(RzGeometry), C.POINTER(C.c_void_p)]),
+    ("rrea_fields_destroy", C.c_int32, [C.c_void_p]),
+    ("rrea_fields_update", C.c_int32, [C.c_void_p, P_D, P_D, P_D, P_D, P_D, C.c_int32, C.c_void_p]),
+    ("rrea_density_set_constant", C.c_int32, [C.c_void_p, C.c_double]),
+    ("rrea_density_set_profile", C.c_int32, [C.c_void_p, P_D, P_D, C.c_int64, C.c_double, C.c_double]),
+    ("rrea_transport_step_electrons", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
+    ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, P_D, P_I64]),
+    ("rrea_version", C.c_char_p, []),
+]
+
+
+class RreaHipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree library; raise if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RreaHipError(
+                f"{LIB_PATH} is missing; build it with "
+                "`python -m rrea_feedback_warpx_amrex_geant4_amd.build` (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code: int, what: str) -> None:
+    if code != 0:
+        raise RreaHipError(f"{what} failed: {ERRORS.get(code, code)} ({code})")

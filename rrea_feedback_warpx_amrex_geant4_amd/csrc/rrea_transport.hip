@@ -1,0 +1,1255 @@
+// MI355X-native RREA transport step: kernels and the C-ABI of include/rrea_hip.h.
+//
+// The electron kernel is the per-particle body of
+// TransportStep::ApplyElectronTransport
+// (native_amrex_rrea/warpx_overlay/Source/Rrea/RreaElectronTransport.cpp:59-889)
+// with the shared prologue BeginRunningChargedParticle
+// (warpx_overlay/Source/Rrea/RreaTransportStep.H:720-820), run one lane per
+// particle.  Side effects the reference logs per 64-particle chunk and
+// replays later are applied directly: net-chord and escape deposits go to the
+// padded J grid with FP64 atomics, secondaries are appended to a staging
+// buffer and then ordered by (parent index, spawn lineage) -- the reference's
+// chunk-order merge -- and tallies are reduced per workgroup.
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -ffp-contract=off (no fused
+// multiply-add anywhere: the step must agree with the reference to 1e-12).
+
+#include "rrea_device.h"
+#include "../../include/rrea_hip.h"
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <vector>
+
+using namespace rrea_dev;
+
+namespace {
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t const e_ = (expr);                                              \
+        if (e_ != hipSuccess) {                                                    \
+            std::fprintf(stderr, "rrea_hip: %s failed: %s\n", #expr, hipGetErrorString(e_)); \
+            return RREA_ERR_HIP;                                                   \
+        }                                                                          \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Host-side table derivation (the parts of RreaInteractionTables::Load that
+// compute: NA/src/RreaInteractionTables.cpp:1632-1657, 2146-2179, 1071-1110)
+// ---------------------------------------------------------------------------
+struct HostCdf {
+    std::vector<double> primary, log_primary, u, v, mean, log_mean;
+    std::vector<int32_t> start;
+};
+
+HostCdf host_cdf(rrea_cdf_groups const& g)
+{
+    HostCdf h;
+    h.primary.assign(g.primary, g.primary + g.n_groups);
+    for (double p : h.primary) h.log_primary.push_back(std::log(p));
+    int64_t const rows = g.start[g.n_groups];
+    h.u.assign(g.u, g.u + rows);
+    h.v.assign(g.v, g.v + rows);
+    for (int64_t i = 0; i <= g.n_groups; ++i) h.start.push_back(static_cast<int32_t>(g.start[i]));
+    for (int64_t k = 0; k < g.n_groups; ++k) {
+        // group_mean of mean_conditional_cdf, same summation order
+        int64_t const s = g.start[k], e = g.start[k + 1];
+        double mean = h.u[s] * h.v[s];
+        for (int64_t i = s + 1; i < e; ++i) mean += 0.5 * (h.v[i - 1] + h.v[i]) * (h.u[i] - h.u[i - 1]);
+        mean += (1.0 - h.u[e - 1]) * h.v[e - 1];
+        h.mean.push_back(mean);
+        h.log_mean.push_back(mean > 0.0 ? std::log(mean) : 0.0);
+    }
+    return h;
+}
+
+// SampleMollerSecondaryEnergy :2584-2653 on the host, for the Load-time mean.
+double host_sample_cdf_group(HostCdf const& h, int64_t g, double cu)
+{
+    int64_t const s = h.start[g], e = h.start[g + 1];
+    auto const b = h.u.begin() + s, en = h.u.begin() + e;
+    auto const up = std::lower_bound(b, en, cu);
+    if (up == b) return h.v[s];
+    if (up == en) return h.v[e - 1];
+    int64_t const hi = up - h.u.begin();
+    int64_t const lo = hi - 1;
+    double const denom = h.u[hi] - h.u[lo];
+    double const t = denom > 0.0 ? (cu - h.u[lo]) / denom : 0.0;
+    return h.v[lo] + t * (h.v[hi] - h.v[lo]);
+}
+
+double host_sample_moller(HostCdf const& h, double primary, double cu)
+{
+    cu = std::max(0.0, std::min(1.0, cu));
+    int64_t const n = static_cast<int64_t>(h.primary.size());
+    auto const up = std::lower_bound(h.primary.begin(), h.primary.end(), primary);
+    if (up == h.primary.begin()) return host_sample_cdf_group(h, 0, cu);
+    if (up == h.primary.end()) return host_sample_cdf_group(h, n - 1, cu);
+    int64_t const hi = up - h.primary.begin();
+    int64_t const lo = hi - 1;
+    if (h.primary[hi] == primary) return host_sample_cdf_group(h, hi, cu);
+    double const y0 = host_sample_cdf_group(h, lo, cu);
+    double const y1 = host_sample_cdf_group(h, hi, cu);
+    double const x0 = std::log(h.primary[lo]);
+    double const x1 = std::log(h.primary[hi]);
+    double const t = (std::log(primary) - x0) / (x1 - x0);
+    return std::exp(std::log(y0) + t * (std::log(y1) - std::log(y0)));
+}
+
+struct DeviceBlob {
+    std::vector<double> f64;
+    std::vector<int32_t> i32;
+    size_t add(std::vector<double> const& v)
+    {
+        size_t const off = f64.size();
+        f64.insert(f64.end(), v.begin(), v.end());
+        while (f64.size() % 32) f64.push_back(0.0);  // 256-byte alignment of every array
+        return off;
+    }
+    size_t add_i(std::vector<int32_t> const& v)
+    {
+        size_t const off = i32.size();
+        i32.insert(i32.end(), v.begin(), v.end());
+        while (i32.size() % 64) i32.push_back(0);
+        return off;
+    }
+};
+
+}  // namespace
+
+struct rrea_tables_s {
+    double* d_f64 = nullptr;
+    int32_t* d_i32 = nullptr;
+    DevTables view{};
+    std::vector<double> hm_mean;
+    double reference_density = 0.0;
+    double brems_cutoff = 0.0;
+    int device = 0;
+};
+
+struct rrea_fields_s {
+    rrea_rz_geometry geom{};
+    double* d_cell_er = nullptr;
+    double* d_cell_ez = nullptr;
+    double* d_btheta = nullptr;
+    double* d_stage = nullptr;   // staging for host-pointer updates
+    size_t stage_len = 0;
+    double* d_prof = nullptr;    // altitude | rho
+    int64_t prof_n = 0;
+    int density_mode = 0;
+    double density_ratio = 1.0;
+    double altitude_at_z0 = 0.0;
+    double reference_density = 1.0;
+    bool fields_set = false;
+    // secondary ordering workspace
+    rrea_secondary* d_stage_sec = nullptr;
+    int64_t stage_sec_cap = 0;
+    uint64_t* d_keys = nullptr;
+    uint64_t* d_keys_out = nullptr;
+    int64_t* d_idx = nullptr;
+    int64_t* d_idx_out = nullptr;
+    void* d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int64_t* d_count = nullptr;
+    void* d_args = nullptr;      // device copy of the kernel's argument block
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    size_t events_used = 0;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Field preparation: the cell-centred totals GatherTotalFieldRZ forms per
+// query (RreaAmrexAdvance.cpp:556-567), formed once per update, same order.
+// ---------------------------------------------------------------------------
+__global__ void cell_fields_kernel(int32_t nr, int32_t nz, const double* __restrict__ er,
+                                   const double* __restrict__ ez, const double* __restrict__ aer,
+                                   const double* __restrict__ aez, double* __restrict__ cer, double* __restrict__ cez)
+{
+    int64_t const c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= static_cast<int64_t>(nr) * nz) return;
+    int32_t const i = static_cast<int32_t>(c % nr);
+    int32_t const j = static_cast<int32_t>(c / nr);
+    int64_t const fr0 = i + static_cast<int64_t>(nr + 1) * j;
+    double const lo_r = er[fr0] + aer[fr0];
+    double const hi_r = er[fr0 + 1] + aer[fr0 + 1];
+    cer[c] = 0.5 * (lo_r + hi_r);
+    int64_t const fz0 = i + static_cast<int64_t>(nr) * j;
+    double const lo_z = ez[fz0] + aez[fz0];
+    double const hi_z = ez[fz0 + nr] + aez[fz0 + nr];
+    cez[c] = 0.5 * (lo_z + hi_z);
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup reduction of the per-particle tallies
+// ---------------------------------------------------------------------------
+constexpr int kBlock = 256;
+constexpr int kTallyUsed = 10;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// The electron step kernel
+// ---------------------------------------------------------------------------
+struct StepArgs {
+    DevTables T;
+    DevFields F;
+    Domain D;
+    DevGrid G;
+    rrea_step_config cfg;
+    double density_ratio;     // reference-z fallback ratio
+    int64_t n;
+    const double* prev_r; const double* prev_z; const double* prev_th;
+    const double* prev_ux; const double* prev_uy; const double* prev_uz;
+    const double* w; int64_t* id; const int32_t* cpu;
+    double* r; double* z; double* th; double* ux; double* uy; double* uz;
+    uint8_t* status;
+    double* ion_pairs;
+    int32_t* substeps;
+    rrea_secondary* sec; int64_t sec_cap; int64_t* sec_count;
+    double* tallies;
+    int32_t* error;
+    int32_t pnr, pzb, pnz;
+};
+
+__device__ __forceinline__ void report(int32_t* error, int32_t code, int64_t ip)
+{
+    if (atomicCAS(&error[0], 0, code) == 0) error[1] = static_cast<int32_t>(ip);
+}
+
+__device__ __forceinline__ void push_secondary(StepArgs const& A, rrea_secondary const& s, int32_t& err)
+{
+    int64_t const slot = static_cast<int64_t>(atomicAdd(reinterpret_cast<unsigned long long*>(A.sec_count), 1ULL));
+    if (slot >= A.sec_cap) { if (!err) err = kErrCapacity; return; }
+    A.sec[slot] = s;
+}
+
+// RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:107-152 (electron)
+__device__ __forceinline__ void electron_stp_rates(DevTables const& T, Bracket const& b, double thr, double gcut,
+                                                   double e, bool hm_on, double& hm, double& br, int32_t& err)
+{
+    hm = 0.0;
+    br = 0.0;
+    if (hm_on && thr > 0.0 && e > 2.0 * thr) {
+        // ElectronHardMollerMeanFreePath :2394-2412, then 1/mfp
+        double const v = charged_value(T, kEMoller, b, err) * 1.0;
+        double const mfp = v > 0.0 ? 1.0 / v : 1.0e99;
+        hm = mfp > 0.0 ? 1.0 / mfp : 0.0;
+    }
+    if (finite(e) && finite(gcut) && e > gcut) {
+        double const v = charged_value(T, kEBrems, b, err) * 1.0;
+        double const mfp = v > 0.0 ? 1.0 / v : 1.0e99;
+        br = mfp > 0.0 ? 1.0 / mfp : 0.0;
+    }
+}
+
+// RreaChargedChannelOpenAt OV/RreaChargedFinalStates.H:56-96 (electron)
+__device__ __forceinline__ bool electron_channel_open(DevTables const& T, double thr, double gcut, int ch, double e,
+                                                      bool hm_on, int32_t& err)
+{
+    Bracket b;
+    if (!(finite(e) && e >= T.charged_min && e <= T.charged_max) || !charged_bracket(T, e, b)) {
+        if (!err) err = kErrRange;
+        return false;
+    }
+    if (ch == 0) return hm_on && thr > 0.0 && e > 2.0 * thr && charged_value(T, kEMoller, b, err) * 1.0 > 0.0;
+    if (ch == 1) return finite(e) && finite(gcut) && e > gcut && charged_value(T, kEBrems, b, err) * 1.0 > 0.0;
+    return false;
+}
+
+// elastic_inverse_length_stp OV/RreaTransportStep.H:199-208
+__device__ __forceinline__ double elastic_inverse_length(DevTables const& T, double e, int32_t& err)
+{
+    double const mfp = elastic_semilog(T.el_e, T.el_e.mfp, e, err) / 1.0;
+    return mfp > 0.0 ? 1.0 / mfp : 0.0;
+}
+
+// electron_ensemble_mean_drag_ev_per_m OV/RreaTransportStep.H:226-247
+__device__ double ensemble_drag(DevTables const& T, double e, double ratio, bool hm_on, int32_t& err)
+{
+    if (!(e > 0.0) || !(ratio > 0.0)) return 0.0;
+    Bracket b;
+    if (!charged_bracket(T, e, b)) { if (!err) err = kErrRange; return 0.0; }
+    double dd = 0.0;
+    if (hm_on) {
+        // ElectronHardMollerEnergyLoss :2414-2440
+        double const inv = charged_value(T, kEMoller, b, err);
+        if (inv > 0.0) {
+            double const mean = charged_semilog(T, T.hm_mean, b, e);
+            if (mean > 0.0) dd = mean * inv * ratio;
+        }
+    }
+    double const realized = charged_value(T, kECol, b, err) * ratio + charged_value(T, kESoft, b, err) * ratio;
+    // ElectronTrackedBremsEnergyLoss :2678-2696
+    double tb = 0.0;
+    double const inv_b = charged_value(T, kEBrems, b, err) * ratio;
+    if (inv_b > 0.0) tb = inv_b * cdf_mean(T.ebrems, e, err);
+    return realized + dd + tb;
+}
+
+__global__ void __launch_bounds__(kBlock) electron_step_kernel(const StepArgs* __restrict__ args)
+{
+    StepArgs const& A = *args;
+    __shared__ double red[kTallyUsed][kBlock / 64];
+    double tally[kTallyUsed];
+    for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
+
+    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    int32_t err = kOk;
+    if (ip < A.n) {
+        DevTables const& T = A.T;
+        DevFields const& F = A.F;
+        Domain const& D = A.D;
+        rrea_step_config const& cfg = A.cfg;
+        double const cutoff = cfg.low_energy_cutoff_eV;
+        double const gcut = cfg.photon_cutoff_eV;
+        double const thr = T.hard_moller_threshold;
+        bool const hm_on = cfg.disable_hard_moller == 0;
+        double const dt = cfg.dt_s;
+        uint64_t const step = static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0);
+
+        int64_t const id = A.id[ip];
+        if (id == RREA_INVALID_ID) {
+            // not a valid particle: the reference loop `continue`s
+            if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SKIPPED_INVALID);
+            if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
+            if (A.substeps) A.substeps[ip] = 0;
+            goto reduce;
+        }
+        {
+        double const pr = A.prev_r[ip], pz = A.prev_z[ip], pth = A.prev_th[ip];
+        double const pux = A.prev_ux[ip], puy = A.prev_uy[ip], puz = A.prev_uz[ip];
+        double const w = A.w[ip];
+        int32_t const cpu = A.cpu ? A.cpu[ip] : 0;
+        uint64_t const pkey = static_cast<uint64_t>(id < 0 ? -id : id)
+            ^ (static_cast<uint64_t>(static_cast<uint32_t>(cpu)) << 40);
+
+        double o_r = pr, o_z = pz, o_th = pth, o_ux = pux, o_uy = puy, o_uz = puz;
+        int status = RREA_NO_WRITEBACK;
+        double ion = 0.0;
+        uint64_t subcycles = 0;
+        uint64_t lineage = 0;
+
+        // BeginRunningChargedParticle :720-820
+        if (w <= 0.0) {
+            status = RREA_ZERO_WEIGHT;
+        } else if (pr > D.r_hi + 1.0e-9 * D.r_hi) {
+            err = kErrDomain;
+        } else {
+            uint64_t const rng = rng_prefix(cfg.rng_seed, pkey, step);
+            double dx = pux, dy = puy, dz = puz;
+            double k = kinetic_from_u(dx, dy, dz);
+            normalize(dx, dy, dz);
+            double run_x = pr * cos(pth);
+            double run_y = pr * sin(pth);
+            double run_z = pz;
+            double const r_particle = pr, z_particle = pz;
+
+            Path path;
+            bool const driver_valid = build_chord(run_x, run_y, run_z, dx, dy, dz, k, dt, dt, D, path, err);
+            // An invalid driver chord is the reference's default-constructed path,
+            // whose zero domain puts every endpoint outside: it escapes.
+            bool escapes = driver_valid ? clipped_escapes(path, D) : true;
+            double const ds = driver_valid ? path.in_len : 0.0;
+            Interval iv = build_interval(F, path, D, A.density_ratio, err);
+            double const span = dmax(path.end_f, 1.0e-6);
+            double const field_work = path_field_work(F, path, D, -1.0, 0.0, 1.0, err).work / span;
+            double const chord_dp = density_path(iv, path, 0.0, path.end_f);
+            double particle_ratio = ds > 0.0 ? chord_dp / ds : positive_or(density_ratio_at_z(F, z_particle, err), A.density_ratio);
+            double tr_dp = 0.0, tr_len = 0.0;
+
+            auto deposit_local = [&](double energy) {
+                if (w <= 0.0 || energy <= 0.0) return;
+                double const pw = w * energy / dmax(kWAir, 1.0);
+                if (pw <= 0.0) return;
+                ion += pw;
+            };
+            enum { kNone, kEscape, kDemote, kSurvive } terminal = kNone;
+            double demote_r = r_particle, demote_z = z_particle;
+
+            bool const below0 = k < cutoff;
+            if (err) {
+                // fall through to the error report
+            } else if (escapes && below0) {
+                terminal = kEscape;
+            } else if (!escapes && below0) {
+                terminal = kDemote;
+            } else if (ds <= 0.0) {
+                terminal = escapes ? kEscape : kNone;
+            } else if (!iv.valid) {
+                err = kErrInvalid;
+            } else {
+                // Substep sizing, OV/RreaElectronTransport.cpp:187-279
+                Bracket b0;
+                if (!charged_bracket(T, k, b0)) err = kErrRange;
+                double hm0, br0;
+                electron_stp_rates(T, b0, thr, gcut, k, hm_on, hm0, br0, err);
+                double const ddp = chord_dp / span;
+                bool violation = false;
+                {
+                    // selected_subcycles :125-141 + selected_material_subcycles :148-184
+                    double const target = cfg.max_optical_depth_per_substep;
+                    uint64_t const cap = cfg.max_subcycles;
+                    double const total_stp = hm0 + br0;
+                    uint64_t const requested =
+                        static_cast<uint64_t>(dmax(1.0, ceil((hm0 * ddp + br0 * ddp) / target)));
+                    violation = requested > cap;
+                    uint64_t n = requested < cap ? requested : cap;
+                    for (;;) {
+                        double const maximum = total_stp * max_uniform_slice(iv, path, n);
+                        double const tol = 64.0 * kEps * dmax(1.0, target);
+                        if (maximum <= target + tol) break;
+                        // a non-finite rate can only come from a corrupt table: stop at once
+                        // rather than walk every count up to the cap
+                        if (!finite(maximum)) { violation = true; n = cap; break; }
+                        if (n >= cap) { violation = true; break; }
+                        // the reference estimates in long double; the estimate only picks the next
+                        // trial count and the loop re-tests it, so double is used here
+                        double const est = ceil(static_cast<double>(n) * (maximum / target));
+                        uint64_t next = n + 1U;
+                        if (est >= static_cast<double>(cap)) next = cap;
+                        else if (est > static_cast<double>(next)) next = static_cast<uint64_t>(est);
+                        n = next < cap ? next : cap;
+                    }
+                    subcycles = n;
+                }
+                {
+                    double const gs_n_total = elastic_inverse_length(T, k, err) * ddp;
+                    double const kc = dmax(k, cutoff);
+                    Bracket bc;
+                    if (!charged_bracket(T, kc, bc)) err = kErrRange;
+                    double const drag_est = (kc > 0.0 ? charged_value(T, kECol, bc, err) * 1.0
+                                                          + charged_value(T, kESoft, bc, err) * 1.0 : 0.0) * ddp;
+                    double const impulse_scale = dmax(cfg.max_field_impulse_fraction * k,
+                                                      dmax(cfg.field_impulse_floor_eV, 1.0));
+                    uint64_t const gsd = static_cast<uint64_t>(dmax(1.0, ceil(dmin(gs_n_total, cfg.gs_total_collision_cap)
+                                                                            / dmax(cfg.max_gs_collisions_per_substep, 1.0))));
+                    uint64_t const imd = static_cast<uint64_t>(dmax(1.0, ceil((fabs(field_work) + drag_est) / impulse_scale)));
+                    uint64_t m = subcycles;
+                    if (gsd > m) m = gsd;
+                    if (imd > m) m = imd;
+                    subcycles = m < cfg.max_subcycles ? m : cfg.max_subcycles;
+                }
+                if (cfg.strict_transport_guards && violation && !err) err = kErrGuard;
+                tally[6] += static_cast<double>(subcycles);
+
+                bool exited = false, terminated = false;
+                double mid_r = r_particle, mid_z = z_particle;
+                uint64_t completed = 0;
+                double const sub_dt = dt / static_cast<double>(subcycles);
+                double ex_ = 0.0, ey_ = 0.0, ez_ = 0.0;
+
+                auto mark_stopped = [&](double f) {
+                    if (path.exits && f >= path.end_f) { exited = true; return; }
+                    double x, y, z;
+                    point_at_full(path, f, x, y, z);
+                    double const r = hypot(x, y);
+                    if (!in_half_open(x, y, z, D)) { exited = true; return; }
+                    mid_z = z; mid_r = r; terminated = true;
+                };
+                // apply_electron_continuous_segment :294-400 via
+                // RreaPlanContinuousDragSegment OV/RreaChargedInterleave.H:239-292
+                auto continuous = [&](double b, double e, double stop_e, double& eff) -> bool {
+                    eff = b;
+                    if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return false;
+                    Bracket bs;
+                    if (!charged_bracket(T, stop_e, bs)) { if (!err) err = kErrRange; return false; }
+                    double const col = charged_value(T, kECol, bs, err) * 1.0;
+                    double const soft = charged_value(T, kESoft, bs, err) * 1.0;
+                    double const total = col + soft;
+                    bool reaches = false;
+                    double const to_cut = k - cutoff;
+                    double const req = total * density_path(iv, path, b, e);
+                    if (!finite(req) || req < 0.0) { if (!err) err = kErrInvalid; return false; }
+                    if (req < to_cut) {
+                        eff = e;
+                    } else {
+                        reaches = true;
+                        if (to_cut != 0.0) {
+                            double lo = b, hi = e;
+                            for (int it = 0; it < 56; ++it) {
+                                double const mid = 0.5 * (lo + hi);
+                                if (total * density_path(iv, path, b, mid) < to_cut) lo = mid; else hi = mid;
+                            }
+                            eff = 0.5 * (lo + hi);
+                        }
+                    }
+                    double const pm = density_path(iv, path, b, eff);
+                    double const raw_col = col * pm;
+                    double const raw_soft = soft * pm;
+                    double const avail = dmax(0.0, k - cutoff);
+                    double const drag = reaches ? avail : dmin(raw_col + raw_soft, avail);
+                    double const col_loss = dmin(raw_col, drag);
+                    // RreaPartitionContinuousIonPairs OV/RreaChargedInterleave.H:157-196
+                    if (col_loss > 0.0 && raw_col > 0.0 && iv.valid) {
+                        double const lb = dmax(b, 0.0), le = dmin(eff, iv.end);
+                        if (le > lb) {
+                            double const raw_i = path.full * (le - lb) * iv.ratio * col;
+                            double const loss_i = col_loss * raw_i / raw_col;
+                            double const pw = w * loss_i / dmax(kWAir, 1.0);
+                            if (pw > 0.0) ion += pw;
+                        }
+                    }
+                    tally[7] += w * drag;
+                    k = dmax(0.0, k - drag);
+                    if (reaches) k = cutoff;
+                    return reaches;
+                };
+
+                FieldRange sfield{0, 0, 0, 0, 0, 0, 0, 0};
+                double substep_dt = 0.0;
+                // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:686-732
+                for (uint64_t sc = 0; sc < subcycles && !err; ++sc) {
+                    if (k < cutoff) break;
+                    double sb = 0.0, se = 0.0;
+                    {
+                        // build_substep, OV/RreaElectronTransport.cpp:535-598
+                        {
+                            // an invalid chord leaves the previous microchord in place
+                            Path next;
+                            if (!build_chord(run_x, run_y, run_z, dx, dy, dz, k, sub_dt, dt, D, next, err)) break;
+                            path = next;
+                        }
+                        if (!(path.full > 0.0) || !(path.in_len > 0.0)) { exited = path.exits; break; }
+                        iv = build_interval(F, path, D, A.density_ratio, err);
+                        if (!iv.valid) { if (!err) err = kErrInvalid; break; }
+                        tr_dp += iv.length * iv.ratio;
+                        tr_len += iv.length;
+                        sb = 0.0;
+                        se = path.end_f;
+                        sfield = path_field_work(F, path, D, -1.0, sb, se, err);
+                        substep_dt = sub_dt * (se - sb);
+                        tally[9] += w * half_kick(sfield, 0.5 * substep_dt, -1.0, k, dx, dy, dz);
+                        if (k < cutoff) { mark_stopped(sb); break; }
+                    }
+                    {
+                        // RreaRunChargedEventInterleave OV/RreaChargedInterleave.H:546-664
+                        double rem = sb;
+                        uint64_t ord = 0;
+                        while (rem < se && k >= cutoff && !err) {
+                            if (ord >= kEventCap) { err = kErrInvalid; break; }
+                            double const frozen_e = k;
+                            Bracket bf;
+                            if (!charged_bracket(T, frozen_e, bf)) { err = kErrRange; break; }
+                            double s0, s1;
+                            electron_stp_rates(T, bf, thr, gcut, frozen_e, hm_on, s0, s1, err);
+                            double const L = density_path(iv, path, rem, se);
+                            uint64_t idx = 0;
+                            if (!live_hazard_index(sc, ord, idx)) { err = kErrInvalid; break; }
+                            double const u1 = rng_uniform(rng, idx, ord == 0U ? kChCompetingOpticalDepth : kChContinuationOpticalDepth);
+                            double const u2 = rng_uniform(rng, idx, ord == 0U ? kChCompetingChannel : kChContinuationChannel);
+                            Hazard const h = select_hazard(s0 * L, s1 * L, u1, u2);
+                            if (!h.valid) { err = kErrInvalid; break; }
+                            if (!h.event) {
+                                double eff;
+                                if (continuous(rem, se, frozen_e, eff)) mark_stopped(eff);
+                                break;
+                            }
+                            double const ef = fraction_at_density_path(iv, path, rem, se, h.sampled / (s0 + s1), err);
+                            if (!(finite(rem) && finite(ef) && finite(se) && se > rem && ef >= rem && ef <= se)) {
+                                if (!err) err = kErrInvalid;
+                                break;
+                            }
+                            point_at_full(path, ef, ex_, ey_, ez_);
+                            {
+                                double eff;
+                                if (continuous(rem, ef, frozen_e, eff)) { mark_stopped(eff); break; }
+                            }
+                            bool const open_ev = k >= cutoff && electron_channel_open(T, thr, gcut, h.channel, k, hm_on, err);
+                            if (!open_ev) {
+                                bool const open_fr = frozen_e >= cutoff
+                                    && electron_channel_open(T, thr, gcut, h.channel, frozen_e, hm_on, err);
+                                if (!open_fr) { if (!err) err = kErrInvalid; break; }
+                                rem = ef;
+                                ++ord;
+                                continue;
+                            }
+                            double const within = dmin(1.0, dmax(0.0, ef));
+                            double const efrac = dmin(1.0, (static_cast<double>(completed) + within) / static_cast<double>(subcycles));
+                            if (h.channel == 0) {
+                                // RreaApplyHardMollerFinalState OV/RreaChargedFinalStates.H:164-225
+                                double const before = k;
+                                if (!(finite(before) && before >= T.charged_min && before <= T.charged_max)) { err = kErrRange; break; }
+                                double T2 = cdf_sample(T.moller, before, rng_uniform(rng, idx, kChHardMollerEnergy), true, err);
+                                T2 = dmax(thr, dmin(T2, 0.5 * before));
+                                // RreaSampleMollerLabDirections :429-463
+                                double pdx = dx, pdy = dy, pdz = dz;
+                                normalize(pdx, pdy, pdz);
+                                double const az = unit_cycle(rng_uniform(rng, idx, kChHardMollerAzimuth));
+                                double const kp = dmax(before, 0.0), ks = dmax(T2, 0.0);
+                                double const cos_s = (kp <= 0.0 || ks <= 0.0) ? 1.0 : moller_cos(kp, ks);
+                                double const sc_e = dmax(0.0, before - T2);
+                                double const cos_p = (before <= 0.0 || sc_e <= 0.0) ? 1.0 : moller_cos(before, sc_e);
+                                double sdx, sdy, sdz, ndx, ndy, ndz;
+                                cone_direction(pdx, pdy, pdz, cos_s, az, sdx, sdy, sdz);
+                                cone_direction(pdx, pdy, pdz, cos_p, unit_cycle(az + 0.5), ndx, ndy, ndz);
+                                k = dmax(0.0, before - T2);
+                                dx = ndx; dy = ndy; dz = ndz;
+                                rrea_secondary s;
+                                s.parent_index = ip; s.species = 0; s.process = 2;
+                                s.x = ex_; s.y = ey_; s.z = ez_;
+                                s.dir_x = sdx; s.dir_y = sdy; s.dir_z = sdz;
+                                s.weight = w; s.energy_eV = T2; s.birth_fraction = efrac;
+                                s.root_key = pkey; s.lineage = lineage++;
+                                push_secondary(A, s, err);
+                                ion += w;
+                                tally[3] += 1.0;
+                            } else if (h.channel == 1) {
+                                // RreaApplyBremsFinalState OV/RreaChargedFinalStates.H:329-394
+                                double const before = k;
+                                if (!(finite(before) && before >= T.charged_min && before <= T.charged_max)) { err = kErrRange; break; }
+                                double const ks = cdf_sample(T.ebrems, before, rng_uniform(rng, idx, kChBremsPhotonEnergy), false, err);
+                                double const pe = dmin(dmax(ks, 0.0), dmax(before, 0.0));
+                                double const after = dmax(0.0, dmax(before, 0.0) - pe);
+                                bool const tracked = finite(gcut) && gcut >= 0.0 && pe >= gcut;
+                                // SampleBremsstrahlungCosTheta :2714-2760
+                                double ct;
+                                {
+                                    double cu = dmax(0.0, dmin(1.0, rng_uniform(rng, idx, kChBremsPhotonAngle)));
+                                    if (cu == 0.0) ct = 1.0;
+                                    else if (cu == 1.0) ct = -1.0;
+                                    else {
+                                        constexpr double a = 0.625;
+                                        auto g2 = [](double x) {
+                                            if (x < 1.0e-3) {
+                                                return x * x * (0.5 + x * (-1.0 / 3.0 + x * (1.0 / 8.0 + x * (-1.0 / 30.0 + x * (1.0 / 144.0)))));
+                                            }
+                                            return -expm1(-x) - x * exp(-x);
+                                        };
+                                        auto mix = [&](double u) { return 0.25 * g2(a * u) + 0.75 * g2(3.0 * a * u); };
+                                        double const u_max = 2.0 * (1.0 + before / kMeC2);
+                                        double const target = cu * mix(u_max);
+                                        double lo = 0.0, hi = u_max;
+                                        for (int it = 0; it < 60; ++it) {
+                                            double const mid = 0.5 * (lo + hi);
+                                            if (mix(mid) < target) lo = mid; else hi = mid;
+                                        }
+                                        double const f = 0.5 * (lo + hi) / u_max;
+                                        ct = 1.0 - 2.0 * f * f;
+                                    }
+                                }
+                                double gx, gy, gz;
+                                cone_direction(dx, dy, dz, ct, rng_uniform(rng, idx, kChBremsPhotonAzimuth), gx, gy, gz);
+                                // RreaApplyPhotonRecoilToChargedDirection :465-496
+                                {
+                                    double const eg = dmax(pe, 0.0);
+                                    if (before > 0.0 && eg > 0.0) {
+                                        normalize(dx, dy, dz);
+                                        double hx = gx, hy = gy, hz = gz;
+                                        normalize(hx, hy, hz);
+                                        double const p0 = sqrt(dmax(0.0, before * (before + 2.0 * kMeC2)));
+                                        double const px = p0 * dx - eg * hx;
+                                        double const py = p0 * dy - eg * hy;
+                                        double const pz2 = p0 * dz - eg * hz;
+                                        double const nn = sqrt(px * px + py * py + pz2 * pz2);
+                                        if (nn > 0.0) { dx = px / nn; dy = py / nn; dz = pz2 / nn; normalize(dx, dy, dz); }
+                                    }
+                                }
+                                k = after;
+                                if (tracked) {
+                                    rrea_secondary s;
+                                    s.parent_index = ip; s.species = 2; s.process = 4;
+                                    s.x = ex_; s.y = ey_; s.z = ez_;
+                                    s.dir_x = gx; s.dir_y = gy; s.dir_z = gz;
+                                    s.weight = w; s.energy_eV = pe; s.birth_fraction = efrac;
+                                    s.root_key = pkey; s.lineage = lineage++;
+                                    push_secondary(A, s, err);
+                                    tally[5] += 1.0;
+                                } else if (pe > 0.0) {
+                                    deposit_local(pe);
+                                }
+                                tally[4] += 1.0;
+                            }
+                            if (k <= cutoff) { mark_stopped(ef); break; }
+                            rem = ef;
+                            ++ord;
+                        }
+                    }
+                    if (terminated || err) break;
+                    {
+                        // finish_interleaved_substep, OV/RreaElectronTransport.cpp:502-524
+                        if (k >= cutoff) {
+                            // apply_gs_elastic_scatter OV/RreaTransportStep.H:372-423
+                            double const gs_n = elastic_inverse_length(T, k, err) * density_path(iv, path, sb, se);
+                            if (gs_n > 0.0) {
+                                if (gs_n < T.gs_few) {
+                                    int const cap = static_cast<int>(ceil(3.0 * T.gs_few));
+                                    if (cap >= 255) { err = kErrInvalid; break; }
+                                    int const cnt = poisson_capped(gs_n, rng_uniform(rng, (sc << 8) | 0U, kChGsFewCount), cap);
+                                    for (int q = 0; q < cnt; ++q) {
+                                        uint64_t const inner = static_cast<uint64_t>(q + 1);
+                                        // SampleSingleElasticCosTheta :2546-2560
+                                        if (!(finite(k) && k >= T.charged_min && k <= T.charged_max)) { err = kErrRange; break; }
+                                        double const eta = elastic_semilog(T.el_e, T.el_e.eta, k, err);
+                                        double const u = dmin(dmax(rng_uniform(rng, (sc << 8) | inner, kChGsFewScatter), 0.0), 1.0 - 1.0e-15);
+                                        double const ct = 1.0 - 2.0 * eta * u / (1.0 + eta - u);
+                                        double nx, ny, nz;
+                                        cone_direction(dx, dy, dz, ct, rng_uniform(rng, (sc << 8) | inner, kChGsFewAzimuth), nx, ny, nz);
+                                        dx = nx; dy = ny; dz = nz;
+                                    }
+                                } else {
+                                    // SampleGsCosTheta :2504-2544
+                                    double const u = rng_uniform(rng, (sc << 8) | 0U, kChGsAngle);
+                                    double ct;
+                                    if (!(finite(k) && k >= T.charged_min && k <= T.charged_max)) { err = kErrRange; break; }
+                                    DevAngle const& A2 = k <= T.gs_transition ? T.gs_e : T.wz_e;
+                                    if (!(k >= A2.e[0] && k <= A2.e[A2.n_e - 1])) { err = kErrRange; break; }
+                                    if (gs_n >= T.gs_iso) ct = 2.0 * u - 1.0;
+                                    else ct = sample_gs_angle_cos(A2, k, gs_n, u, err);
+                                    double nx, ny, nz;
+                                    cone_direction(dx, dy, dz, ct, rng_uniform(rng, (sc << 8) | 0U, kChGsAzimuth), nx, ny, nz);
+                                    dx = nx; dy = ny; dz = nz;
+                                }
+                            }
+                        }
+                        if (err) break;
+                        tally[9] += w * half_kick(sfield, 0.5 * substep_dt, -1.0, k, dx, dy, dz);
+                        if (k < cutoff) { mark_stopped(se); break; }
+                    }
+                    {
+                        // RreaAdvanceRunningChargedPosition OV/RreaParticleInteraction.H:368-392
+                        double x, y, z;
+                        point_at_full(path, se, x, y, z);
+                        if (!in_half_open(x, y, z, D)) { exited = true; break; }
+                        run_x = x; run_y = y; run_z = z;
+                    }
+                    if (path.exits) { exited = true; break; }
+                    completed = sc + 1U;
+                }
+
+                if (!err) {
+                    if (exited) escapes = clipped_escapes(path, D);
+                    if (tr_len > 0.0) particle_ratio = tr_dp / tr_len;
+                    double const end_r = hypot(run_x, run_y);
+                    bool const below = k < cutoff;
+                    bool nonrunaway = false;
+                    if (!exited && !terminated && !below && thr > 0.0 && k <= thr) {
+                        // RreaEvaluateElectronStatePolicy OV/RreaParticleInteraction.H:1279-1317; the
+                        // ensemble drag has no side effect, so it is read only where the gate uses it.
+                        double fer, fez, fbt;
+                        gather(F, end_r, run_z, fer, fez, fbt);
+                        double const thr_v = ensemble_drag(T, dmax(k, cutoff), particle_ratio, hm_on, err);
+                        nonrunaway = hypot(fer, fez) <= thr_v;
+                    }
+                    // RreaResolveChargedTransportTerminal :292-312
+                    if (exited) {
+                        terminal = kEscape;
+                    } else if (terminated || below || nonrunaway) {
+                        terminal = kDemote;
+                        demote_r = terminated ? mid_r : end_r;
+                        demote_z = terminated ? mid_z : run_z;
+                    } else {
+                        // CommitSurvivorChargedWriteback OV/RreaTransportStep.H:874-905
+                        double const um = proper_velocity(k);
+                        o_ux = um * dx; o_uy = um * dy; o_uz = um * dz;
+                        o_r = end_r;
+                        o_th = atan2(run_y, run_x);
+                        o_z = run_z;
+                        terminal = kSurvive;
+                        if (w != 0.0 && !deposit_segment(A.G, -w, dt, r_particle, z_particle, end_r, run_z)) err = kErrInvalid;
+                    }
+                }
+            }
+            if (!err) {
+                if (terminal == kEscape) {
+                    // CommitChargedEscape OV/RreaTransportStep.H:829-863
+                    double const um = proper_velocity(k);
+                    o_ux = um * dx; o_uy = um * dy; o_uz = um * dz;
+                    Crossing const bc = boundary_crossing(path, D);
+                    if (!bc.valid) {
+                        err = kErrInvalid;
+                    } else if (-w != 0.0 && !deposit_escaping_segment(A.G, -w, dt, r_particle, z_particle, bc.start_r, bc.start_z,
+                                                                       bc.cross_r, bc.cross_z, A.pnr, A.pzb, A.pnz, bc.faces)) {
+                        err = kErrInvalid;
+                    }
+                    status = RREA_ESCAPE;
+                    tally[1] += 1.0;
+                } else if (terminal == kDemote) {
+                    deposit_local(k);
+                    if (w != 0.0 && !deposit_segment(A.G, -w, dt, r_particle, z_particle, demote_r, demote_z)) err = kErrInvalid;
+                    status = RREA_DEMOTED;
+                    tally[2] += 1.0;
+                } else if (terminal == kSurvive) {
+                    status = RREA_SURVIVE;
+                    tally[0] += 1.0;
+                }
+            }
+        }
+        if (err) {
+            report(A.error, err, ip);
+        } else {
+            if (status == RREA_SURVIVE || status == RREA_ESCAPE) {
+                A.r[ip] = o_r; A.z[ip] = o_z; A.th[ip] = o_th;
+                A.ux[ip] = o_ux; A.uy[ip] = o_uy; A.uz[ip] = o_uz;
+            }
+            if (status == RREA_ESCAPE || status == RREA_DEMOTED || status == RREA_ZERO_WEIGHT) {
+                A.id[ip] = RREA_INVALID_ID;  // particle_id.make_invalid()
+            }
+            if (A.status) A.status[ip] = static_cast<uint8_t>(status);
+            if (A.ion_pairs) A.ion_pairs[ip] = ion;
+            if (A.substeps) A.substeps[ip] = static_cast<int32_t>(subcycles);
+            tally[8] += ion;
+        }
+        }
+    }
+reduce:
+    // workgroup reduction of the tallies, one atomic per tally per workgroup
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    for (int q = 0; q < kTallyUsed; ++q) {
+        double const s = wave_sum(tally[q]);
+        if (lane == 0) red[q][wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kTallyUsed) {
+        double s = 0.0;
+        for (int wv = 0; wv < kBlock / 64; ++wv) s += red[threadIdx.x][wv];
+        if (s != 0.0) atomic_add(&A.tallies[threadIdx.x], s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic secondary order: key = (parent index, lineage), the order of
+// the reference's chunk-ordered merge (RreaElectronTransport.cpp:892-897).
+// ---------------------------------------------------------------------------
+__global__ void secondary_keys_kernel(const rrea_secondary* __restrict__ s, const int64_t* __restrict__ count,
+                                      int64_t cap, uint64_t* __restrict__ keys, int64_t* __restrict__ idx)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t const n = *count < cap ? *count : cap;
+    if (i >= cap) return;
+    if (i < n) {
+        keys[i] = (static_cast<uint64_t>(s[i].parent_index) << 16) | (s[i].lineage & 0xffffULL);
+    } else {
+        keys[i] = ~0ULL;
+    }
+    idx[i] = i;
+}
+
+__global__ void secondary_gather_kernel(const rrea_secondary* __restrict__ in, const int64_t* __restrict__ idx,
+                                        const int64_t* __restrict__ count, int64_t cap, rrea_secondary* __restrict__ out)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t const n = *count < cap ? *count : cap;
+    if (i < n) out[i] = in[idx[i]];
+}
+
+int64_t blocks_for(int64_t n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* rrea_version(void) { return "rrea-mi355x 0.1 (gfx950, fp64)"; }
+
+rrea_status rrea_tables_create(const rrea_table_image* im, rrea_tables_t* out)
+{
+    if (!im || !out) return RREA_ERR_ARGUMENT;
+    *out = nullptr;
+    // Every charged column shares one energy grid in the production bundle;
+    // the shared bracket relies on it.
+    for (int t = 1; t < 11; ++t) {
+        if (im->rate[t].n != im->rate[0].n
+            || std::memcmp(im->rate[t].x, im->rate[0].x, sizeof(double) * im->rate[0].n) != 0) {
+            return RREA_ERR_ARGUMENT;
+        }
+    }
+    std::unique_ptr<rrea_tables_s> T(new (std::nothrow) rrea_tables_s());
+    if (!T) return RREA_ERR_ARGUMENT;
+    HIP_TRY(hipGetDevice(&T->device));
+    DeviceBlob blob;
+    int64_t const nc = im->rate[0].n;
+    std::vector<double> cx(im->rate[0].x, im->rate[0].x + nc), clogx(nc);
+    for (int64_t i = 0; i < nc; ++i) clogx[i] = std::log(cx[i]);  // Finalize :1650-1656
+    size_t const o_cx = blob.add(cx), o_clogx = blob.add(clogx);
+    size_t o_cy[11], o_clogy[11];
+    for (int t = 0; t < 11; ++t) {
+        std::vector<double> y(im->rate[t].y, im->rate[t].y + nc), ly(nc);
+        for (int64_t i = 0; i < nc; ++i) ly[i] = y[i] > 0.0 ? std::log(y[i]) : 0.0;
+        o_cy[t] = blob.add(y);
+        o_clogy[t] = blob.add(ly);
+    }
+    // Load :2146-2179 -- mean hard-Moller secondary by 4096-point midpoint rule
+    HostCdf const moller = host_cdf(im->moller);
+    std::vector<double> hm_mean(nc, 0.0);
+    for (int64_t i = 0; i < nc; ++i) {
+        if (!(im->rate[2].y[i] > 0.0)) continue;
+        if (cx[i] < moller.primary.front() || cx[i] > moller.primary.back()) return RREA_ERR_ARGUMENT;
+        double mean = 0.0;
+        for (int s = 0; s < 4096; ++s) mean += host_sample_moller(moller, cx[i], (static_cast<double>(s) + 0.5) / 4096);
+        hm_mean[i] = mean / 4096.0;
+    }
+    T->hm_mean = hm_mean;
+    size_t const o_hm = blob.add(hm_mean);
+    HostCdf const ebrems = host_cdf(im->electron_brems);
+    struct CdfOff { size_t p, lp, u, v, m, lm, s; int32_t n; };
+    auto add_cdf = [&](HostCdf const& h) {
+        CdfOff o;
+        o.p = blob.add(h.primary); o.lp = blob.add(h.log_primary); o.u = blob.add(h.u); o.v = blob.add(h.v);
+        o.m = blob.add(h.mean); o.lm = blob.add(h.log_mean); o.s = blob.add_i(h.start);
+        o.n = static_cast<int32_t>(h.primary.size());
+        return o;
+    };
+    CdfOff const om = add_cdf(moller), ob = add_cdf(ebrems);
+    struct AngOff { size_t e, le, n, ln, u, lur, lomc; int32_t ne, nn, nu; };
+    auto add_angle = [&](rrea_angle_table const& a) {
+        AngOff o;
+        std::vector<double> e(a.e, a.e + a.n_e), n(a.n, a.n + a.n_n), u(a.u, a.u + a.n_u);
+        std::vector<double> le(a.n_e), ln(a.n_n), lur(a.n_u, 0.0), lomc(a.n_e * a.n_n * a.n_u);
+        for (int64_t i = 0; i < a.n_e; ++i) le[i] = std::log(e[i]);
+        for (int64_t i = 0; i < a.n_n; ++i) ln[i] = std::log(n[i]);
+        for (int64_t i = 0; i + 1 < a.n_u; ++i) lur[i] = std::log(u[i + 1] / u[i]);
+        for (int64_t i = 0; i < a.n_e * a.n_n * a.n_u; ++i) lomc[i] = std::log(a.omc[i]);
+        o.e = blob.add(e); o.le = blob.add(le); o.n = blob.add(n); o.ln = blob.add(ln);
+        o.u = blob.add(u); o.lur = blob.add(lur); o.lomc = blob.add(lomc);
+        o.ne = static_cast<int32_t>(a.n_e); o.nn = static_cast<int32_t>(a.n_n); o.nu = static_cast<int32_t>(a.n_u);
+        return o;
+    };
+    AngOff const og = add_angle(im->gs_electron), ow = add_angle(im->wentzel_electron);
+    std::vector<double> ele(im->elastic_electron.e, im->elastic_electron.e + im->elastic_electron.n);
+    std::vector<double> ellog(ele.size());
+    for (size_t i = 0; i < ele.size(); ++i) ellog[i] = std::log(ele[i]);
+    size_t const o_ele = blob.add(ele), o_ellog = blob.add(ellog);
+    size_t const o_elmfp = blob.add(std::vector<double>(im->elastic_electron.mfp_m_stp, im->elastic_electron.mfp_m_stp + ele.size()));
+    size_t const o_eleta = blob.add(std::vector<double>(im->elastic_electron.screening_eta, im->elastic_electron.screening_eta + ele.size()));
+
+    HIP_TRY(hipMalloc(&T->d_f64, blob.f64.size() * sizeof(double)));
+    HIP_TRY(hipMemcpy(T->d_f64, blob.f64.data(), blob.f64.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&T->d_i32, std::max<size_t>(blob.i32.size(), 1) * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(T->d_i32, blob.i32.data(), blob.i32.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    double* const f = T->d_f64;
+    int32_t* const ii = T->d_i32;
+    DevTables& V = T->view;
+    V.cx = f + o_cx; V.clogx = f + o_clogx; V.nc = static_cast<int32_t>(nc);
+    for (int t = 0; t < 11; ++t) { V.cy[t] = f + o_cy[t]; V.clogy[t] = f + o_clogy[t]; V.cmode[t] = im->rate[t].mode; }
+    V.hm_mean = f + o_hm;
+    auto set_cdf = [&](DevCdf& d, CdfOff const& o) {
+        d.primary = f + o.p; d.log_primary = f + o.lp; d.u = f + o.u; d.v = f + o.v;
+        d.mean = f + o.m; d.log_mean = f + o.lm; d.start = ii + o.s; d.n_groups = o.n;
+    };
+    set_cdf(V.moller, om);
+    set_cdf(V.ebrems, ob);
+    auto set_angle = [&](DevAngle& d, AngOff const& o) {
+        d.e = f + o.e; d.log_e = f + o.le; d.n = f + o.n; d.log_n = f + o.ln; d.u = f + o.u;
+        d.log_u_ratio = f + o.lur; d.log_omc = f + o.lomc; d.n_e = o.ne; d.n_n = o.nn; d.n_u = o.nu;
+    };
+    set_angle(V.gs_e, og);
+    set_angle(V.wz_e, ow);
+    V.el_e.e = f + o_ele; V.el_e.log_e = f + o_ellog; V.el_e.mfp = f + o_elmfp; V.el_e.eta = f + o_eleta;
+    V.el_e.n = static_cast<int32_t>(ele.size());
+    V.hard_moller_threshold = im->hard_moller_secondary_threshold_eV;
+    V.charged_min = im->charged_energy_range_eV[0];
+    V.charged_max = im->charged_energy_range_eV[1];
+    V.gs_few = im->gs_few_collision_threshold_n;
+    V.gs_iso = im->gs_isotropic_above_path_n;
+    V.gs_transition = im->gs_wentzel_transition_energy_eV;
+    T->reference_density = im->reference_density_kg_m3;
+    T->brems_cutoff = im->brems_absolute_cutoff_eV;
+    *out = T.release();
+    return RREA_OK;
+}
+
+rrea_status rrea_tables_destroy(rrea_tables_t t)
+{
+    if (!t) return RREA_OK;
+    if (t->d_f64) (void)hipFree(t->d_f64);
+    if (t->d_i32) (void)hipFree(t->d_i32);
+    delete t;
+    return RREA_OK;
+}
+
+rrea_status rrea_tables_hm_mean(rrea_tables_t t, double* out, int64_t n)
+{
+    if (!t || !out || n != static_cast<int64_t>(t->hm_mean.size())) return RREA_ERR_ARGUMENT;
+    std::memcpy(out, t->hm_mean.data(), sizeof(double) * n);
+    return RREA_OK;
+}
+
+rrea_status rrea_fields_create(const rrea_rz_geometry* g, rrea_fields_t* out)
+{
+    if (!g || !out || g->nr <= 0 || g->nz <= 0 || !(g->dr > 0.0) || !(g->dz > 0.0) || g->r_lo != 0.0) {
+        return RREA_ERR_ARGUMENT;
+    }
+    auto* F = new (std::nothrow) rrea_fields_s();
+    if (!F) return RREA_ERR_ARGUMENT;
+    F->geom = *g;
+    size_t const cells = static_cast<size_t>(g->nr) * g->nz;
+    size_t const nodes = static_cast<size_t>(g->nr + 1) * (g->nz + 1);
+    if (hipMalloc(&F->d_cell_er, cells * sizeof(double)) != hipSuccess
+        || hipMalloc(&F->d_cell_ez, cells * sizeof(double)) != hipSuccess
+        || hipMalloc(&F->d_btheta, nodes * sizeof(double)) != hipSuccess
+        || hipMalloc(&F->d_count, sizeof(int64_t)) != hipSuccess
+        || hipMalloc(&F->d_args, sizeof(StepArgs)) != hipSuccess) {
+        rrea_fields_destroy(F);
+        return RREA_ERR_HIP;
+    }
+    *out = F;
+    return RREA_OK;
+}
+
+rrea_status rrea_fields_destroy(rrea_fields_t F)
+{
+    if (!F) return RREA_OK;
+    void* ptrs[] = {F->d_cell_er, F->d_cell_ez, F->d_btheta, F->d_stage, F->d_prof, F->d_stage_sec, F->d_keys,
+                    F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args};
+    for (void* p : ptrs) if (p) (void)hipFree(p);
+    for (auto& ev : F->events) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    delete F;
+    return RREA_OK;
+}
+
+rrea_status rrea_kernel_timing(rrea_fields_t F, int32_t enable)
+{
+    if (!F) return RREA_ERR_ARGUMENT;
+    F->timing = enable != 0;
+    return RREA_OK;
+}
+
+rrea_status rrea_kernel_timing_read(rrea_fields_t F, double* total_ms, int64_t* launches)
+{
+    if (!F || !total_ms || !launches) return RREA_ERR_ARGUMENT;
+    double sum = 0.0;
+    for (size_t i = 0; i < F->events_used; ++i) {
+        HIP_TRY(hipEventSynchronize(F->events[i].second));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, F->events[i].first, F->events[i].second));
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = static_cast<int64_t>(F->events_used);
+    F->events_used = 0;
+    return RREA_OK;
+}
+
+rrea_status rrea_fields_update(rrea_fields_t F, const double* er, const double* ez, const double* amb_er,
+                               const double* amb_ez, const double* btheta, int32_t device_pointers,
+                               rrea_stream_t stream)
+{
+    if (!F || !er || !ez || !amb_er || !amb_ez || !btheta) return RREA_ERR_ARGUMENT;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    int32_t const nr = F->geom.nr, nz = F->geom.nz;
+    size_t const n_er = static_cast<size_t>(nr + 1) * nz, n_ez = static_cast<size_t>(nr) * (nz + 1);
+    size_t const n_bt = static_cast<size_t>(nr + 1) * (nz + 1);
+    const double *d_er = er, *d_ez = ez, *d_aer = amb_er, *d_aez = amb_ez;
+    if (!device_pointers) {
+        size_t const need = 2 * n_er + 2 * n_ez;
+        if (F->stage_len < need) {
+            if (F->d_stage) (void)hipFree(F->d_stage);
+            F->d_stage = nullptr;
+            HIP_TRY(hipMalloc(&F->d_stage, need * sizeof(double)));
+            F->stage_len = need;
+        }
+        double* st = F->d_stage;
+        HIP_TRY(hipMemcpyAsync(st, er, n_er * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(st + n_er, amb_er, n_er * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(st + 2 * n_er, ez, n_ez * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(st + 2 * n_er + n_ez, amb_ez, n_ez * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(F->d_btheta, btheta, n_bt * sizeof(double), hipMemcpyHostToDevice, s));
+        d_er = st; d_aer = st + n_er; d_ez = st + 2 * n_er; d_aez = st + 2 * n_er + n_ez;
+    } else {
+        HIP_TRY(hipMemcpyAsync(F->d_btheta, btheta, n_bt * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    int64_t const cells = static_cast<int64_t>(nr) * nz;
+    cell_fields_kernel<<<dim3(static_cast<unsigned>(blocks_for(cells, 256))), dim3(256), 0, s>>>(
+        nr, nz, d_er, d_ez, d_aer, d_aez, F->d_cell_er, F->d_cell_ez);
+    HIP_TRY(hipGetLastError());
+    if (!device_pointers) HIP_TRY(hipStreamSynchronize(s));
+    F->fields_set = true;
+    return RREA_OK;
+}
+
+rrea_status rrea_density_set_constant(rrea_fields_t F, double ratio)
+{
+    if (!F || !std::isfinite(ratio)) return RREA_ERR_ARGUMENT;
+    F->density_mode = 0;
+    F->density_ratio = ratio;
+    return RREA_OK;
+}
+
+rrea_status rrea_density_set_profile(rrea_fields_t F, const double* alt, const double* rho, int64_t n, double z0_alt,
+                                     double reference_density)
+{
+    if (!F || !alt || !rho || n < 2 || !(reference_density > 0.0)) return RREA_ERR_ARGUMENT;
+    for (int64_t i = 1; i < n; ++i) if (!(alt[i] >= alt[i - 1])) return RREA_ERR_ARGUMENT;
+    if (F->d_prof) (void)hipFree(F->d_prof);
+    F->d_prof = nullptr;
+    HIP_TRY(hipMalloc(&F->d_prof, 2 * n * sizeof(double)));
+    HIP_TRY(hipMemcpy(F->d_prof, alt, n * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(F->d_prof + n, rho, n * sizeof(double), hipMemcpyHostToDevice));
+    F->prof_n = n;
+    F->density_mode = 1;
+    F->altitude_at_z0 = z0_alt;
+    F->reference_density = reference_density;
+    return RREA_OK;
+}
+
+rrea_status rrea_transport_step_electrons(rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+                                          const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
+{
+    if (!T || !F || !tile || !cfg || !out || !F->fields_set) return RREA_ERR_ARGUMENT;
+    if (!out->secondaries || !out->secondary_count || !out->jr || !out->jz || !out->rho || !out->tallies
+        || !out->error || out->secondary_capacity < 0) {
+        return RREA_ERR_ARGUMENT;
+    }
+    if (tile->n < 0 || (tile->n > 0 && (!tile->prev_r || !tile->prev_z || !tile->prev_theta || !tile->prev_ux
+                                         || !tile->prev_uy || !tile->prev_uz || !tile->w || !tile->id || !tile->r
+                                         || !tile->z || !tile->theta || !tile->ux || !tile->uy || !tile->uz))) {
+        return RREA_ERR_ARGUMENT;
+    }
+    // RreaParticleInteraction::Apply :38-40 -- a non-positive dt does nothing.
+    if (!(cfg->dt_s > 0.0) || tile->n == 0) return RREA_OK;
+    if (!(cfg->max_optical_depth_per_substep > 0.0) || cfg->max_subcycles == 0 || cfg->max_subcycles > 16383) {
+        return RREA_ERR_ARGUMENT;
+    }
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != T->device) return RREA_ERR_ARGUMENT;
+
+    StepArgs A;
+    A.T = T->view;
+    rrea_rz_geometry const& g = F->geom;
+    A.F.nr = g.nr; A.F.nz = g.nz; A.F.dr = g.dr; A.F.dz = g.dz; A.F.r_lo = g.r_lo; A.F.z_lo = g.z_lo;
+    A.F.cell_er = F->d_cell_er; A.F.cell_ez = F->d_cell_ez; A.F.btheta = F->d_btheta;
+    A.F.density_mode = F->density_mode; A.F.density_ratio = F->density_ratio;
+    A.F.prof_alt = F->d_prof; A.F.prof_rho = F->d_prof ? F->d_prof + F->prof_n : nullptr;
+    A.F.prof_n = static_cast<int32_t>(F->prof_n); A.F.altitude_at_z0 = F->altitude_at_z0;
+    A.F.reference_density = F->reference_density;
+    A.D = Domain{g.r_lo, g.r_lo + g.nr * g.dr, g.z_lo, g.z_lo + g.nz * g.dz};
+    // RreaParticleInteraction::Apply :29-35: the ratio at the domain's mid z.
+    {
+        double ratio = F->density_ratio;
+        if (F->density_mode == 1) {
+            std::vector<double> alt(F->prof_n), rho(F->prof_n);
+            HIP_TRY(hipMemcpy(alt.data(), F->d_prof, F->prof_n * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(rho.data(), F->d_prof + F->prof_n, F->prof_n * sizeof(double), hipMemcpyDeviceToHost));
+            double const a = F->altitude_at_z0 + 0.5 * (A.D.z_lo + A.D.z_hi);
+            if (a < alt.front() || a > alt.back()) return RREA_ERR_TABLE_RANGE;
+            auto const up = std::upper_bound(alt.begin(), alt.end(), a);
+            double v;
+            if (up == alt.begin()) v = rho.front();
+            else if (up == alt.end()) v = rho.back();
+            else {
+                size_t const hi = up - alt.begin(), lo = hi - 1;
+                double const span = alt[hi] - alt[lo];
+                double const t = span > 0.0 ? (a - alt[lo]) / span : 0.0;
+                v = (1.0 - t) * rho[lo] + t * rho[hi];
+            }
+            ratio = v / F->reference_density;
+        }
+        A.density_ratio = ratio > 0.0 ? ratio : 1.0;
+    }
+    int32_t const pad = 2;
+    A.G = DevGrid{g.nr + pad, g.nz + 2 * pad, g.dr, g.dz, g.r_lo, g.z_lo - pad * g.dz, out->rho, out->jr, out->jz};
+    A.pnr = g.nr; A.pzb = pad; A.pnz = g.nz;
+    A.cfg = *cfg;
+    A.n = tile->n;
+    A.prev_r = tile->prev_r; A.prev_z = tile->prev_z; A.prev_th = tile->prev_theta;
+    A.prev_ux = tile->prev_ux; A.prev_uy = tile->prev_uy; A.prev_uz = tile->prev_uz;
+    A.w = tile->w; A.id = tile->id; A.cpu = tile->cpu;
+    A.r = tile->r; A.z = tile->z; A.th = tile->theta; A.ux = tile->ux; A.uy = tile->uy; A.uz = tile->uz;
+    A.status = out->status; A.ion_pairs = out->ion_pairs; A.substeps = out->substeps;
+    A.tallies = out->tallies; A.error = out->error;
+
+    bool const order = out->order_secondaries != 0 && out->secondary_capacity > 0;
+    int64_t const cap = out->secondary_capacity;
+    if (order) {
+        // stage into the library's workspace, then sort into the caller's buffer
+        if (F->stage_sec_cap < cap) {
+            void* ptrs[] = {F->d_stage_sec, F->d_keys, F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp};
+            for (void* p : ptrs) if (p) (void)hipFree(p);
+            F->d_stage_sec = nullptr; F->d_keys = nullptr; F->d_keys_out = nullptr;
+            F->d_idx = nullptr; F->d_idx_out = nullptr; F->d_sort_tmp = nullptr;
+            HIP_TRY(hipMalloc(&F->d_stage_sec, cap * sizeof(rrea_secondary)));
+            HIP_TRY(hipMalloc(&F->d_keys, cap * sizeof(uint64_t)));
+            HIP_TRY(hipMalloc(&F->d_keys_out, cap * sizeof(uint64_t)));
+            HIP_TRY(hipMalloc(&F->d_idx, cap * sizeof(int64_t)));
+            HIP_TRY(hipMalloc(&F->d_idx_out, cap * sizeof(int64_t)));
+            size_t bytes = 0;
+            HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, F->d_keys, F->d_keys_out, F->d_idx, F->d_idx_out,
+                                                       static_cast<int>(cap)));
+            HIP_TRY(hipMalloc(&F->d_sort_tmp, bytes));
+            F->sort_tmp_bytes = bytes;
+            F->stage_sec_cap = cap;
+        }
+        HIP_TRY(hipMemcpyAsync(F->d_count, out->secondary_count, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+        A.sec = F->d_stage_sec;
+        A.sec_count = F->d_count;
+    } else {
+        A.sec = out->secondaries;
+        A.sec_count = out->secondary_count;
+    }
+    A.sec_cap = cap;
+    // The staged region starts at the caller's current count.
+    int64_t base = 0;
+    if (order) {
+        HIP_TRY(hipMemcpyAsync(&base, out->secondary_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        int64_t const zero = 0;
+        HIP_TRY(hipMemcpyAsync(F->d_count, &zero, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        A.sec_cap = cap - base;
+        if (A.sec_cap < 0) A.sec_cap = 0;
+    }
+
+    int64_t const nb = blocks_for(tile->n, kBlock);
+    // pageable source: the copy has consumed A when hipMemcpyAsync returns
+    HIP_TRY(hipMemcpyAsync(F->d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
+    hipEvent_t ev_stop = nullptr;
+    if (F->timing) {
+        if (F->events_used == F->events.size()) {
+            hipEvent_t a, b;
+            HIP_TRY(hipEventCreate(&a));
+            HIP_TRY(hipEventCreate(&b));
+            F->events.emplace_back(a, b);
+        }
+        HIP_TRY(hipEventRecord(F->events[F->events_used].first, s));
+        ev_stop = F->events[F->events_used].second;
+        ++F->events_used;
+    }
+    electron_step_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
+        static_cast<const StepArgs*>(F->d_args));
+    HIP_TRY(hipGetLastError());
+    if (ev_stop) HIP_TRY(hipEventRecord(ev_stop, s));
+
+    if (order) {
+        int64_t nstaged = 0;
+        HIP_TRY(hipMemcpyAsync(&nstaged, F->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        int64_t const room = A.sec_cap;
+        int64_t const nsort = nstaged < room ? nstaged : room;
+        if (nsort > 0) {
+            secondary_keys_kernel<<<dim3(static_cast<unsigned>(blocks_for(nsort, 256))), dim3(256), 0, s>>>(
+                F->d_stage_sec, F->d_count, nsort, F->d_keys, F->d_idx);
+            HIP_TRY(hipGetLastError());
+            size_t bytes = F->sort_tmp_bytes;
+            HIP_TRY(hipcub::DeviceRadixSort::SortPairs(F->d_sort_tmp, bytes, F->d_keys, F->d_keys_out, F->d_idx,
+                                                       F->d_idx_out, static_cast<int>(nsort), 0, 64, s));
+            secondary_gather_kernel<<<dim3(static_cast<unsigned>(blocks_for(nsort, 256))), dim3(256), 0, s>>>(
+                F->d_stage_sec, F->d_idx_out, F->d_count, nsort, out->secondaries + base);
+            HIP_TRY(hipGetLastError());
+        }
+        int64_t const total = base + nstaged;
+        HIP_TRY(hipMemcpyAsync(out->secondary_count, &total, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    return RREA_OK;
+}
+
+}  // extern "C"

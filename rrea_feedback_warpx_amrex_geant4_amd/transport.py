@@ -1,0 +1,288 @@
+"""Host-side mirror of the reference's per-step transport interface over the
+HIP C-ABI.
+
+``RreaParticleInteraction(config).apply_electrons(...)`` plays the role of
+``RreaParticleInteraction::Apply``'s electron phase
+(native_amrex_rrea/warpx_overlay/Source/Rrea/RreaParticleInteraction.cpp:12-62,
+RreaElectronTransport.cpp:9-902): same configuration names, same meaning of
+the pre-push snapshot, same fail-closed behaviour (a device error raises and
+the caller commits nothing).  ``InteractionTables`` plays
+``RreaInteractionTables::Load`` from the converted bundle and ``Coupling``
+holds the field and density state of ``RreaWarpXCoupling``.
+
+PyTorch provides device memory and the stream; all transport arithmetic is in
+``librrea_hip.so``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import RreaParticleInteractionConfig, RzFields, StepConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUNDLE_PATH = os.path.join(HERE, "data", "schema6_air.npz")
+
+RATE_KEYS = ["e_col", "e_soft", "e_moller", "e_brems", "e_eltr",
+             "p_col", "p_soft", "p_bhabha", "p_brems", "p_eltr", "p_annih",
+             "g_compton", "g_pe", "g_pairn", "g_pairt"]
+
+SECONDARY_DTYPE = np.dtype([
+    ("parent_index", "<i8"), ("species", "<i4"), ("process", "<i4"),
+    ("x", "<f8"), ("y", "<f8"), ("z", "<f8"),
+    ("dir_x", "<f8"), ("dir_y", "<f8"), ("dir_z", "<f8"),
+    ("weight", "<f8"), ("energy_eV", "<f8"), ("birth_fraction", "<f8"),
+    ("root_key", "<u8"), ("lineage", "<u8"),
+])
+assert SECONDARY_DTYPE.itemsize == C.sizeof(_lib.Secondary)
+
+
+def load_bundle(path: str = BUNDLE_PATH) -> dict:
+    """The schema-6 production bundle as converted by ``tables.convert``."""
+    with np.load(path, allow_pickle=False) as data:
+        return {k: data[k] for k in data.files}
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_lib.P_D)
+
+
+class InteractionTables:
+    """Device-resident schema-6 tables (RreaInteractionTables)."""
+
+    def __init__(self, bundle: dict | None = None):
+        bundle = load_bundle() if bundle is None else bundle
+        self._keep = []
+        L = _lib.lib()
+        im = _lib.TableImage()
+
+        def arr(a, dtype=np.float64):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            self._keep.append(a)
+            return a
+
+        for i, key in enumerate(RATE_KEYS):
+            x, y = arr(bundle[key + "_x"]), arr(bundle[key + "_y"])
+            im.rate[i] = _lib.RateColumn(_dptr(x), _dptr(y), len(x), int(bundle[key + "_mode"]))
+        for field, key in [("moller", "moller"), ("bhabha", "bhabha"), ("electron_brems", "ebrems"),
+                           ("positron_brems", "pbrems")]:
+            prim, start = arr(bundle[key + "_primary"]), arr(bundle[key + "_start"], np.int64)
+            u, v = arr(bundle[key + "_u"]), arr(bundle[key + "_v"])
+            setattr(im, field, _lib.CdfGroups(_dptr(prim), len(prim), start.ctypes.data_as(_lib.P_I64),
+                                              _dptr(u), _dptr(v)))
+        for field, key in [("gs_electron", "gs_e"), ("gs_positron", "gs_p"), ("wentzel_electron", "wz_e"),
+                           ("wentzel_positron", "wz_p")]:
+            e, n, u, omc = (arr(bundle[key + s]) for s in ("_e", "_n", "_u", "_omc"))
+            setattr(im, field, _lib.AngleTable(_dptr(e), len(e), _dptr(n), len(n), _dptr(u), len(u), _dptr(omc)))
+        for field, key in [("elastic_electron", "el_e"), ("elastic_positron", "el_p")]:
+            e, mfp, eta = arr(bundle[key + "_e"]), arr(bundle[key + "_mfp"]), arr(bundle[key + "_eta"])
+            setattr(im, field, _lib.ElasticTable(_dptr(e), _dptr(mfp), _dptr(eta), len(e)))
+        s = np.asarray(bundle["scalars"], dtype=np.float64)
+        im.hard_moller_secondary_threshold_eV = s[0]
+        im.brems_absolute_cutoff_eV = s[1]
+        im.reference_density_kg_m3 = s[2]
+        im.charged_energy_range_eV[:] = [s[3], s[4]]
+        im.photon_energy_range_eV[:] = [s[5], s[6]]
+        im.gs_few_collision_threshold_n = s[7]
+        im.gs_isotropic_above_path_n = s[8]
+        im.gs_wentzel_transition_energy_eV = s[9]
+        self.scalars = s
+        self.n_charged = len(bundle["e_col_x"])
+        h = C.c_void_p()
+        _lib.check(L.rrea_tables_create(C.byref(im), C.byref(h)), "rrea_tables_create")
+        self.h = h
+        self._keep = []
+
+    @property
+    def hard_moller_threshold_eV(self) -> float:
+        return float(self.scalars[0])
+
+    @property
+    def brems_cutoff_eV(self) -> float:
+        return float(self.scalars[1])
+
+    @property
+    def reference_density(self) -> float:
+        return float(self.scalars[2])
+
+    def hm_mean(self) -> np.ndarray:
+        out = np.zeros(self.n_charged)
+        _lib.check(_lib.lib().rrea_tables_hm_mean(self.h, _dptr(out), len(out)), "rrea_tables_hm_mean")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().rrea_tables_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Coupling:
+    """Field and density state the step reads (RreaWarpXCoupling services)."""
+
+    def __init__(self, fields: RzFields, tables: InteractionTables, stream: int | None = None):
+        L = _lib.lib()
+        self.fields = fields
+        g = _lib.RzGeometry(fields.nr, fields.nz, fields.dr, fields.dz, 0.0, fields.z_lo)
+        h = C.c_void_p()
+        _lib.check(L.rrea_fields_create(C.byref(g), C.byref(h)), "rrea_fields_create")
+        self.h = h
+        self.update(fields, stream)
+        if fields.profile_alt_m is not None:
+            alt = np.ascontiguousarray(fields.profile_alt_m, dtype=np.float64)
+            rho = np.ascontiguousarray(fields.profile_rho, dtype=np.float64)
+            _lib.check(L.rrea_density_set_profile(self.h, _dptr(alt), _dptr(rho), len(alt), fields.altitude_at_z0_m,
+                                                  tables.reference_density), "rrea_density_set_profile")
+        else:
+            _lib.check(L.rrea_density_set_constant(self.h, fields.density_ratio), "rrea_density_set_constant")
+
+    def update(self, fields: RzFields, stream: int | None = None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64)
+                for a in (fields.er, fields.ez, fields.amb_er, fields.amb_ez, fields.btheta)]
+        _lib.check(_lib.lib().rrea_fields_update(self.h, *[_dptr(a) for a in arrs], 0, stream),
+                   "rrea_fields_update")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().rrea_fields_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class DeviceTile:
+    """Device SoA of one species (float64, int64 ids, int32 cpu)."""
+
+    r: torch.Tensor
+    z: torch.Tensor
+    theta: torch.Tensor
+    ux: torch.Tensor
+    uy: torch.Tensor
+    uz: torch.Tensor
+    w: torch.Tensor
+    id: torch.Tensor
+    cpu: torch.Tensor
+
+    @staticmethod
+    def from_host(tile, device="cuda") -> "DeviceTile":
+        f = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)  # noqa: E731
+        return DeviceTile(*(f(getattr(tile, k), torch.float64) for k in ("r", "z", "theta", "ux", "uy", "uz", "w")),
+                          f(tile.id, torch.int64), f(tile.cpu, torch.int32))
+
+    def __len__(self):
+        return self.r.numel()
+
+    def clone(self) -> "DeviceTile":
+        return DeviceTile(*(getattr(self, k).clone() for k in
+                            ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")))
+
+
+class StepOutputs:
+    """Device outputs of one step (allocated once, reused)."""
+
+    def __init__(self, n: int, fields: RzFields, secondary_capacity: int, device="cuda", per_particle: bool = True):
+        nrp, nzp = fields.padded_shape()
+        self.status = torch.zeros(n, dtype=torch.uint8, device=device) if per_particle else None
+        self.ion_pairs = torch.zeros(n, dtype=torch.float64, device=device) if per_particle else None
+        self.substeps = torch.zeros(n, dtype=torch.int32, device=device) if per_particle else None
+        self.secondaries = torch.zeros(max(secondary_capacity, 1) * SECONDARY_DTYPE.itemsize, dtype=torch.uint8,
+                                       device=device)
+        self.secondary_capacity = secondary_capacity
+        self.secondary_count = torch.zeros(1, dtype=torch.int64, device=device)
+        self.jr = torch.zeros((nrp + 1) * nzp, dtype=torch.float64, device=device)
+        self.jz = torch.zeros(nrp * (nzp + 1), dtype=torch.float64, device=device)
+        self.rho = torch.zeros(nrp * nzp, dtype=torch.float64, device=device)
+        self.tallies = torch.zeros(_lib.TALLY_COUNT, dtype=torch.float64, device=device)
+        self.error = torch.zeros(2, dtype=torch.int32, device=device)
+
+    def reset(self):
+        for t in (self.secondary_count, self.jr, self.jz, self.rho, self.tallies, self.error):
+            t.zero_()
+
+    def struct(self, order: bool) -> _lib.StepOut:
+        p = lambda t: None if t is None else t.data_ptr()  # noqa: E731
+        return _lib.StepOut(p(self.status), p(self.ion_pairs), p(self.substeps), p(self.secondaries),
+                            self.secondary_capacity, p(self.secondary_count), p(self.jr), p(self.jz), p(self.rho),
+                            p(self.tallies), p(self.error), 1 if order else 0)
+
+    def secondaries_host(self) -> np.ndarray:
+        n = min(int(self.secondary_count.item()), self.secondary_capacity)
+        raw = self.secondaries[: n * SECONDARY_DTYPE.itemsize].cpu().numpy()
+        return np.frombuffer(raw.tobytes(), dtype=SECONDARY_DTYPE, count=n)
+
+    def check_error(self):
+        code, index = (int(v) for v in self.error.cpu().tolist())
+        if code:
+            raise _lib.RreaHipError(f"device transport error: {_lib.ERRORS.get(code, code)} at particle {index}")
+
+
+def _step_config(cfg: StepConfig) -> _lib.StepConfigC:
+    return _lib.StepConfigC(
+        cfg.rng_seed, cfg.step, cfg.dt_s, cfg.low_energy_cutoff_eV, cfg.photon_cutoff_eV,
+        cfg.max_optical_depth_per_substep, cfg.max_subcycles, int(cfg.strict_transport_guards),
+        int(cfg.debug_disable_hard_moller_secondaries), cfg.transport_max_gs_collisions_per_substep,
+        cfg.transport_gs_total_collision_cap, cfg.transport_max_field_impulse_fraction,
+        cfg.transport_field_impulse_floor_eV)
+
+
+class RreaParticleInteraction:
+    """Per-step transport hook (RreaParticleInteraction)."""
+
+    def __init__(self, config: RreaParticleInteractionConfig | None = None):
+        self.config = config if config is not None else RreaParticleInteractionConfig()
+
+    def step_config(self, step: int, dt_s: float, low_energy_cutoff_eV: float, photon_cutoff_eV: float) -> StepConfig:
+        c = self.config
+        return StepConfig(
+            density_ratio=c.density_ratio, max_optical_depth_per_substep=c.max_optical_depth_per_substep,
+            max_subcycles=c.max_subcycles, strict_transport_guards=c.strict_transport_guards,
+            debug_disable_hard_moller_secondaries=c.debug_disable_hard_moller_secondaries,
+            transport_max_gs_collisions_per_substep=c.transport_max_gs_collisions_per_substep,
+            transport_gs_total_collision_cap=c.transport_gs_total_collision_cap,
+            transport_max_field_impulse_fraction=c.transport_max_field_impulse_fraction,
+            transport_field_impulse_floor_eV=c.transport_field_impulse_floor_eV,
+            rng_seed=c.rng_seed, step=step, dt_s=dt_s, low_energy_cutoff_eV=low_energy_cutoff_eV,
+            photon_cutoff_eV=photon_cutoff_eV)
+
+    @staticmethod
+    def apply_electrons(tables: InteractionTables, coupling: Coupling, prev: DeviceTile, cfg: StepConfig,
+                        out: StepOutputs, current: DeviceTile | None = None, order_secondaries: bool = True,
+                        stream: int | None = None, check: bool = True) -> None:
+        """One electron step.  ``prev`` is the pre-push snapshot; the written-back
+        state goes to ``current`` (default: in place over ``prev``)."""
+        cur = prev if current is None else current
+        n = len(prev)
+        if n and not all(t.is_contiguous() and t.numel() == n for t in (
+                prev.r, prev.z, prev.theta, prev.ux, prev.uy, prev.uz, prev.w, prev.id, prev.cpu,
+                cur.r, cur.z, cur.theta, cur.ux, cur.uy, cur.uz)):
+            raise ValueError("tile tensors must be contiguous and of one length")
+        if n and (prev.r.dtype != torch.float64 or prev.id.dtype != torch.int64 or prev.cpu.dtype != torch.int32):
+            raise ValueError("tile dtypes must be float64 / int64 id / int32 cpu")
+        tile = _lib.Tile(n, prev.r.data_ptr(), prev.z.data_ptr(), prev.theta.data_ptr(), prev.ux.data_ptr(),
+                         prev.uy.data_ptr(), prev.uz.data_ptr(), prev.w.data_ptr(), prev.id.data_ptr(),
+                         prev.cpu.data_ptr(), cur.r.data_ptr(), cur.z.data_ptr(), cur.theta.data_ptr(),
+                         cur.ux.data_ptr(), cur.uy.data_ptr(), cur.uz.data_ptr())
+        c = _step_config(cfg)
+        o = out.struct(order_secondaries)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().rrea_transport_step_electrons(tables.h, coupling.h, C.byref(tile), C.byref(c),
+                                                             C.byref(o), stream), "rrea_transport_step_electrons")
+        if check:
+            out.check_error()

@@ -1,0 +1,75 @@
+"""The C-ABI library loads, exports every symbol include/rrea_hip.h declares,
+and the ctypes mirrors match the header's layout (CPU only, no device call)."""
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from rrea_feedback_warpx_amrex_geant4_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rrea_hip.h")
+
+STRUCTS = {
+    "rrea_rate_column": _lib.RateColumn,
+    "rrea_cdf_groups": _lib.CdfGroups,
+    "rrea_angle_table": _lib.AngleTable,
+    "rrea_elastic_table": _lib.ElasticTable,
+    "rrea_table_image": _lib.TableImage,
+    "rrea_rz_geometry": _lib.RzGeometry,
+    "rrea_step_config": _lib.StepConfigC,
+    "rrea_tile": _lib.Tile,
+    "rrea_secondary": _lib.Secondary,
+    "rrea_step_out": _lib.StepOut,
+}
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:rrea_status|const char\*)\s+(rrea_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 9
+    lib = _lib.lib()
+    for name in names:
+        assert hasattr(lib, name), name
+    assert {n for n, _, _ in _lib.SIGNATURES} == set(names)
+    assert lib.rrea_version().startswith(b"rrea-mi355x")
+
+
+def test_ctypes_layout_matches_header():
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {", 'printf("{");']
+    first = True
+    for cname, cls in STRUCTS.items():
+        for fname, _ in cls._fields_:
+            sep = "" if first else ","
+            first = False
+            lines.append(f'printf("{sep}\\"{cname}.{fname}\\": %zu", offsetof({cname}, {fname}));')
+        lines.append(f'printf(",\\"{cname}\\": %zu", sizeof({cname}));')
+    lines += ['printf("}");', "return 0; }"]
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        open(src, "w").write("\n".join(lines))
+        subprocess.check_call(["gcc", "-std=c11", "-o", exe, src])
+        layout = json.loads(subprocess.check_output([exe]))
+    for cname, cls in STRUCTS.items():
+        assert layout[cname] == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert layout[f"{cname}.{fname}"] == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_argument_checks_need_no_device():
+    lib = _lib.lib()
+    assert lib.rrea_tables_create(None, None) == 16
+    assert lib.rrea_fields_create(None, None) == 16
+    bad = _lib.RzGeometry(0, 10, 1.0, 1.0, 0.0, 0.0)
+    h = C.c_void_p()
+    assert lib.rrea_fields_create(C.byref(bad), C.byref(h)) == 16
+    assert lib.rrea_transport_step_electrons(None, None, None, None, None, None) == 16

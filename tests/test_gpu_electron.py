@@ -1,0 +1,140 @@
+"""GPU parity of the HIP electron step against the CPU oracle.
+
+Per particle-step: status and substep counts exact, continuous state to 1e-12
+relative (FP64), secondaries bit-exact in count, parent index, lineage and
+order, with payloads to 1e-12; the net-chord J grid to round-off of the atomic
+summation order.  Sizes are those the oracle finishes in seconds; the full
+2^24 workload is covered by size-independent checks in test_gpu_scale.py.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from parity import run_and_compare  # noqa: E402
+from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl  # noqa: E402
+from rrea_feedback_warpx_amrex_geant4_amd.config import ParticleTile, StepConfig  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tables(bundle):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import InteractionTables
+    return InteractionTables(bundle)
+
+
+def coupling_for(fields, tables):
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import Coupling
+    return Coupling(fields, tables)
+
+
+def test_hm_mean_matches_oracle(tables, otables):
+    assert np.array_equal(tables.hm_mean(), otables.hm_mean())
+
+
+def test_c1_plain_parity(tables, otables):
+    fields, tile = wl.c1_plain(1000)
+    cfg = StepConfig(rng_seed=20240623, dt_s=1e-12, low_energy_cutoff_eV=1e4)
+    cp = coupling_for(fields, tables)
+    print("case c1_plain 1000 e- x 20 steps", flush=True)
+    run_and_compare(otables, tables, cp, fields, tile, cfg, steps=20, label="c1")
+
+
+def test_c2_sample_parity(tables, otables):
+    fields = wl.c2_fields()
+    tile = wl.c2_electrons(1 << 15, fields, seed=21)
+    cfg = StepConfig(rng_seed=77, dt_s=1e-12)
+    cp = coupling_for(fields, tables)
+    print("case c2 2^15 e- x 3 steps", flush=True)
+    run_and_compare(otables, tables, cp, fields, tile, cfg, steps=3, label="c2")
+
+
+def test_long_step_events_and_multiple_scattering(tables, otables):
+    """dt = 1 ns: several substeps, Moller and brems events, secondaries, the
+    GS multiple-scattering branch and cutoff stops all occur."""
+    fields = wl.c2_fields()
+    tile = wl.c2_electrons(1 << 13, fields, seed=5)
+    cfg = StepConfig(rng_seed=3, dt_s=1e-9)
+    cp = coupling_for(fields, tables)
+    print("case c2 2^13 e- dt=1ns x 2 steps", flush=True)
+    run_and_compare(otables, tables, cp, fields, tile, cfg, steps=2, label="long")
+
+
+def test_edge_cases(tables, otables):
+    """Zero weight, sub-cutoff demotion, escapes through r_hi / z_hi / z_lo,
+    particles on the axis, a negative id, and low energies near the gate."""
+    fields = wl.c2_fields()
+    n = 12
+    k = np.array([7e6, 5e3, 2e4, 1e6, 1e6, 1e6, 3e4, 4.9e4, 1e8, 2e6, 1e9, 1.5e4])
+    u = wl.proper_velocity(k)
+    dirs = np.array([[0, 0, 1], [0, 0, 1], [1, 0, 0], [1, 0, 0], [0, 0, 1], [0, 0, -1], [0, 1, 0],
+                     [0.6, 0, 0.8], [0, 0, 1], [0, 0, 1], [1, 0, 0], [0, 0, -1]], dtype=float)
+    r = np.array([100.0, 5.0, 10.0, fields.r_hi - 1e-5, 0.0, 0.0, 1.0, 3.0, 0.0, 7.0, fields.r_hi * 0.5, 2.0])
+    z = np.array([500.0, 50.0, 5.0, 10.0, fields.z_hi - 1e-5, 1e-6, 100.0, 200.0, 5000.0, 0.0, 20.0, 9000.0])
+    w = np.ones(n)
+    w[0] = 0.0
+    tile = ParticleTile(r=r, z=z, theta=np.linspace(-3, 3, n), ux=u * dirs[:, 0], uy=u * dirs[:, 1],
+                        uz=u * dirs[:, 2], w=w, id=np.arange(1, n + 1, dtype=np.int64) * np.where(
+                            np.arange(n) == 3, -1, 1), cpu=np.arange(n, dtype=np.int32))
+    cfg = StepConfig(rng_seed=9, dt_s=1e-9)
+    cp = coupling_for(fields, tables)
+    print("case edge 12 particles dt=1ns", flush=True)
+    run_and_compare(otables, tables, cp, fields, tile, cfg, steps=1, label="edge")
+
+
+def test_empty_tile_and_nonpositive_dt(tables):
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import (DeviceTile, RreaParticleInteraction,
+                                                                StepOutputs)
+    fields, tile = wl.c1_plain(4)
+    cp = coupling_for(fields, tables)
+    empty = DeviceTile.from_host(tile.take(np.arange(0)))
+    out = StepOutputs(0, fields, 16)
+    RreaParticleInteraction.apply_electrons(tables, cp, empty, StepConfig(dt_s=1e-12), out)
+    dev = DeviceTile.from_host(tile)
+    before = dev.uz.clone()
+    out = StepOutputs(4, fields, 16)
+    RreaParticleInteraction.apply_electrons(tables, cp, dev, StepConfig(dt_s=0.0), out)
+    torch.cuda.synchronize()
+    assert torch.equal(dev.uz, before)
+    assert int(out.secondary_count.item()) == 0
+
+
+def test_out_of_range_energy_fails_closed(tables, otables):
+    """A charged energy above the certified 10 GeV: the reference Aborts in the
+    table query; the oracle and the device both report a range error."""
+    from oracle import binding as ob
+    from rrea_feedback_warpx_amrex_geant4_amd import _lib
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile, RreaParticleInteraction, StepOutputs
+    fields, tile = wl.c1_plain(3)
+    tile.uz[1] = wl.proper_velocity(np.array([2.0e10]))[0]
+    cfg = StepConfig(dt_s=1e-12)
+    o = ob.electron_step(otables, fields, cfg, tile.as_dict())
+    assert o["code"] == 1
+    cp = coupling_for(fields, tables)
+    dev = DeviceTile.from_host(tile)
+    out = StepOutputs(3, fields, 16)
+    with pytest.raises(_lib.RreaHipError, match="table range"):
+        RreaParticleInteraction.apply_electrons(tables, cp, dev, cfg, out)
+
+
+def test_determinism_across_runs(tables):
+    """Two runs of the same step: identical particles and secondaries, J equal
+    to round-off (atomic order)."""
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile, RreaParticleInteraction, StepOutputs
+    fields = wl.c2_fields()
+    tile = wl.c2_electrons(1 << 16, fields, seed=8)
+    cp = coupling_for(fields, tables)
+    cfg = StepConfig(rng_seed=1, dt_s=5e-10)
+    results = []
+    for _ in range(2):
+        dev = DeviceTile.from_host(tile)
+        out = StepOutputs(len(tile), fields, 1 << 16)
+        RreaParticleInteraction.apply_electrons(tables, cp, dev, cfg, out)
+        torch.cuda.synchronize()
+        results.append((dev.uz.cpu().numpy(), out.secondaries_host(), out.jz.cpu().numpy()))
+    assert np.array_equal(results[0][0], results[1][0])
+    assert results[0][1].tobytes() == results[1][1].tobytes()
+    assert np.allclose(results[0][2], results[1][2], rtol=1e-12, atol=1e-12 * np.abs(results[0][2]).max())
