@@ -10,6 +10,33 @@ from oracle import binding as ob
 # absolute floor covers values that cancel to ~0 (r on the axis, u components
 # orthogonal to the motion), scaled by the quantity's natural size.
 REL = 1e-12
+EPS = np.finfo(np.float64).eps
+ME_C2 = 510998.95
+C_LIGHT = 299792458.0
+
+
+def conditioning(tile_prev, dt, substeps=None, fields=None, status=None):
+    """Amplification of a last-place libm difference by the reference's
+    Cartesian -> (r, theta) -> Cartesian round trip of every chord
+    (RreaParticleInteraction.H:144-155 -> RreaTransportPath.cpp:109-114): the
+    chord start and end carry ~ulp(r) each, so the chord's length and direction
+    carry ~eps r / l relative, l = speed * dt."""
+    u2 = tile_prev.ux ** 2 + tile_prev.uy ** 2 + tile_prev.uz ** 2
+    gamma = np.sqrt(1.0 + u2 / C_LIGHT ** 2)
+    speed = C_LIGHT * np.sqrt(np.maximum(0.0, 1.0 - 1.0 / (gamma * gamma)))
+    ell = np.maximum(speed * dt, 1e-300)
+    if substeps is not None:
+        ell = ell / np.maximum(substeps, 1)  # each microchord is dt / substeps long
+    kappa = 1.0 + np.maximum(tile_prev.r, np.abs(tile_prev.z)) / ell
+    if fields is not None and status is not None:
+        # an exiting chord's in-domain fraction is a root of r^2 - R^2 (or a
+        # ratio (z_face - z) / dz): it carries R / (R - r) (or |z| / (z_face - z))
+        esc = status == 1
+        gap_r = np.maximum(fields.r_hi - tile_prev.r, 1e-300)
+        gap_z = np.maximum(np.minimum(tile_prev.z - fields.z_lo, fields.z_hi - tile_prev.z), 1e-300)
+        zscale = max(abs(fields.z_lo), abs(fields.z_hi))
+        kappa = kappa + np.where(esc, fields.r_hi / gap_r + zscale / gap_z, 0.0)
+    return kappa
 
 
 def close(a, b, scale):
@@ -28,27 +55,55 @@ def gpu_step(tables, coupling, tile_dev, cfg, out):
     torch.cuda.synchronize()
 
 
-def compare_step(fields, tile_host_prev, oracle_out, tile_dev, out, label=""):
+def compare_step(fields, tile_host_prev, oracle_out, tile_dev, out, label="", dt=None, stats=None):
     """Assert the HIP outputs of one step equal the oracle's."""
     status_o = oracle_out["status"]
     status_g = out.status.cpu().numpy().astype(np.int32)
     bad = np.flatnonzero(status_o != status_g)
     assert bad.size == 0, f"{label} status differs at {bad[:10]}: oracle {status_o[bad[:10]]} gpu {status_g[bad[:10]]}"
-    c = 299792458.0
-    scales = {"r": fields.dr, "z": fields.dz, "theta": 1.0, "ux": c, "uy": c, "uz": c}
-    for k, sc in scales.items():
-        g = getattr(tile_dev, k).cpu().numpy()
-        o = oracle_out[k]
-        ok = close(g, o, sc * 1e-3)
+    # The state is compared as vectors: the proper velocity to 1e-12 of |u|, the
+    # Cartesian position (r cos theta, r sin theta, z) to 1e-12 of its
+    # coordinates (a coordinate carries FP64 precision relative to its own size;
+    # theta alone is ill-conditioned near the axis).
+    g = {k: getattr(tile_dev, k).cpu().numpy() for k in ("r", "z", "theta", "ux", "uy", "uz")}
+    o = oracle_out
+    du = np.sqrt((g["ux"] - o["ux"]) ** 2 + (g["uy"] - o["uy"]) ** 2 + (g["uz"] - o["uz"]) ** 2)
+    umag = np.sqrt(o["ux"] ** 2 + o["uy"] ** 2 + o["uz"] ** 2)
+    rel_u = du / np.maximum(umag, 1.0)
+    # tolerance: 1e-12, plus the round-trip conditioning times the step's own
+    # relative change of u (the part of the state the chord length feeds)
+    kappa = conditioning(tile_host_prev, dt, oracle_out["substeps"], fields, status_o)
+    du_step = np.sqrt((o["ux"] - tile_host_prev.ux) ** 2 + (o["uy"] - tile_host_prev.uy) ** 2
+                      + (o["uz"] - tile_host_prev.uz) ** 2) / np.maximum(umag, 1.0)
+    tol_u = REL + 16.0 * EPS * kappa * du_step
+    bad = np.flatnonzero(rel_u > tol_u)
+    assert bad.size == 0, (f"{label} u differs at {bad[:5]}: |du|/|u| = {rel_u[bad[:5]]} tol {tol_u[bad[:5]]} "
+                           f"status {status_o[bad[:5]]}")
+    if stats is not None:
+        live = status_o == 0
+        stats.setdefault("n", 0)
+        stats["n"] += int(live.sum())
+        stats["within_1e-12"] = stats.get("within_1e-12", 0) + int((rel_u[live] <= REL).sum())
+        stats["max_rel_u"] = max(stats.get("max_rel_u", 0.0), float(rel_u[live].max(initial=0.0)))
+    xg, yg = g["r"] * np.cos(g["theta"]), g["r"] * np.sin(g["theta"])
+    xo, yo = o["r"] * np.cos(o["theta"]), o["r"] * np.sin(o["theta"])
+    dxy = np.hypot(xg - xo, yg - yo)
+    bad = np.flatnonzero(dxy > REL * np.maximum(o["r"], fields.dr * 1e-3))
+    assert bad.size == 0, f"{label} transverse position differs at {bad[:5]}: {dxy[bad[:5]]} at r {o['r'][bad[:5]]}"
+    for name, a, b, sc in (("z", g["z"], o["z"], fields.dz), ("r", g["r"], o["r"], fields.dr)):
+        ok = close(a, b, sc * 1e-3)
         idx = np.flatnonzero(~ok)
-        assert idx.size == 0, (f"{label} {k} differs at {idx[:5]}: oracle {o[idx[:5]]} gpu {g[idx[:5]]} "
+        assert idx.size == 0, (f"{label} {name} differs at {idx[:5]}: oracle {b[idx[:5]]} gpu {a[idx[:5]]} "
                                f"status {status_o[idx[:5]]}")
     ids = tile_dev.id.cpu().numpy()
     invalidated = np.isin(status_o, (1, 2, 3))
     assert np.all(ids[invalidated] == np.iinfo(np.int64).min), f"{label} invalidated ids not marked"
     assert np.all(ids[~invalidated] == tile_host_prev.id[~invalidated]), f"{label} live ids changed"
     ion_g = out.ion_pairs.cpu().numpy()
-    ok = close(ion_g, oracle_out["ion_pairs"], 1e-3)
+    # ion pairs are proportional to the chord's density path: they carry the
+    # round-trip conditioning at full weight
+    ion_o = oracle_out["ion_pairs"]
+    ok = np.abs(ion_g - ion_o) <= (REL + 16.0 * EPS * kappa) * np.maximum(np.abs(ion_o), 1e-3)
     assert ok.all(), f"{label} ion pairs differ at {np.flatnonzero(~ok)[:5]}"
     sub_g = out.substeps.cpu().numpy()
     assert np.array_equal(sub_g, oracle_out["substeps"].astype(np.int32)), f"{label} substeps differ"
@@ -58,15 +113,28 @@ def compare_step(fields, tile_host_prev, oracle_out, tile_dev, out, label=""):
     assert len(sg) == len(so), f"{label} secondary count {len(sg)} vs {len(so)}"
     for f in ("parent_index", "species", "process", "root_key", "lineage"):
         assert np.array_equal(sg[f], so[f]), f"{label} secondary {f} differs"
-    for f, sc in (("x", 1.0), ("y", 1.0), ("z", 1.0), ("dir_x", 1.0), ("dir_y", 1.0), ("dir_z", 1.0),
-                  ("weight", 1.0), ("energy_eV", 1.0), ("birth_fraction", 1.0)):
-        assert close(sg[f], so[f], sc).all(), f"{label} secondary {f} differs"
-    # the net-chord grid: FP64 atomics sum in a different order
+    assert np.array_equal(sg["weight"], so["weight"]), f"{label} secondary weight differs"
+    # a secondary is born on its parent's chord: its payload carries the
+    # parent's round-trip conditioning
+    kp = kappa[so["parent_index"]] if len(so) else np.zeros(0)
+    pos_scale = np.maximum(np.abs(so["z"]), np.hypot(so["x"], so["y"])) if len(so) else np.zeros(0)
+    for f in ("x", "y", "z"):
+        assert np.all(np.abs(sg[f] - so[f]) <= REL * np.maximum(pos_scale, 1e-3)), f"{label} secondary {f} differs"
+    for f, sc in (("dir_x", 1.0), ("dir_y", 1.0), ("dir_z", 1.0), ("energy_eV", 1.0), ("birth_fraction", 1.0)):
+        tol = (REL + 16.0 * EPS * kp) * np.maximum(np.maximum(np.abs(sg[f]), np.abs(so[f])), sc)
+        bad = np.flatnonzero(np.abs(sg[f] - so[f]) > tol)
+        assert bad.size == 0, f"{label} secondary {f} differs at {bad[:5]}: {sg[f][bad[:5]]} vs {so[f][bad[:5]]}"
+    # The net-chord grid: FP64 atomics sum in another order, and a face flux
+    # q/dt * (x1 - x0)/dx carries the endpoints' absolute precision ~ulp(coordinate)
+    # divided by the cell size.
+    wmax = float(np.max(np.abs(tile_host_prev.w), initial=0.0))
+    coord = fields.r_hi / fields.dr + max(abs(fields.z_lo), abs(fields.z_hi)) / fields.dz
     for k in ("jr", "jz", "rho"):
         g = getattr(out, k).cpu().numpy()
         o = oracle_out[k]
         scale = max(np.abs(o).max(), 1e-300)
-        assert np.abs(g - o).max() <= 1e-11 * scale, f"{label} {k} differs: {np.abs(g - o).max()} of {scale}"
+        tol = 1e-11 * scale + (64.0 * EPS * coord * wmax / dt if k != "rho" else 64.0 * EPS * coord * wmax)
+        assert np.abs(g - o).max() <= tol, f"{label} {k} differs: {np.abs(g - o).max()} of {scale} (tol {tol})"
     t_g = out.tallies.cpu().numpy()[:10]
     t_o = oracle_out["tallies"][:10]
     assert np.allclose(t_g, t_o, rtol=1e-10, atol=1e-300), f"{label} tallies {t_g} vs {t_o}"
@@ -79,8 +147,9 @@ def host_tile_from_dev(tile_dev):
                           ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")))
 
 
-def run_and_compare(otables, tables, coupling, fields, tile_host, cfg, steps=1, label="", nthreads=8):
+def run_and_compare(otables, tables, coupling, fields, tile_host, cfg, steps=1, label="", nthreads=8, stats=None):
     """Advance `steps` steps on both paths from the same start, comparing every step."""
+    stats = {} if stats is None else stats
     from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile, StepOutputs
 
     tile_dev = DeviceTile.from_host(tile_host)
@@ -91,7 +160,7 @@ def run_and_compare(otables, tables, coupling, fields, tile_host, cfg, steps=1, 
         o = ob.electron_step(otables, fields, cfg, host.as_dict(), nthreads=nthreads)
         assert o["code"] == 0, f"oracle error {o['code']}"
         gpu_step(tables, coupling, tile_dev, cfg, out)
-        compare_step(fields, host, o, tile_dev, out, label=f"{label} step {s}")
+        compare_step(fields, host, o, tile_dev, out, label=f"{label} step {s}", dt=cfg.dt_s, stats=stats)
         # continue both from the oracle's state, keeping survivors only
         keep = o["status"] == 0
         host = host.take(np.flatnonzero(keep))
@@ -101,4 +170,6 @@ def run_and_compare(otables, tables, coupling, fields, tile_host, cfg, steps=1, 
             break
         tile_dev = DeviceTile.from_host(host)
         out = StepOutputs(len(host), fields, secondary_capacity=max(4096, len(host)))
+    print(f"{label}: {stats.get('within_1e-12', 0)} of {stats.get('n', 0)} live particle-steps within 1e-12 of |u|; "
+          f"max |du|/|u| = {stats.get('max_rel_u', 0.0):.3e}", flush=True)
     return host
