@@ -75,9 +75,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--particles", type=int, default=None, help="electrons (default: 70%% of 2^24)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-particles", type=int, default=1 << 19)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-particles", type=int, default=1 << 20)
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run only a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_calibration_mib:
+        import torch
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are out of scope "

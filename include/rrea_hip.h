@@ -152,6 +152,9 @@ typedef struct rrea_step_config {
     double gs_total_collision_cap;
     double max_field_impulse_fraction;
     double field_impulse_floor_eV;
+    int32_t complete_path_only; /* nonzero: every particle through the complete
+                                   step, no fast-path specialisation (testing) */
+    int32_t reserved;
 } rrea_step_config;
 
 /* Device SoA of one tile (the ChargedSoAView of RreaTransportStep.H:454-468).

@@ -67,6 +67,7 @@ class StepConfigC(C.Structure):
         ("strict_transport_guards", C.c_int32), ("disable_hard_moller", C.c_int32),
         ("max_gs_collisions_per_substep", C.c_double), ("gs_total_collision_cap", C.c_double),
         ("max_field_impulse_fraction", C.c_double), ("field_impulse_floor_eV", C.c_double),
+        ("complete_path_only", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

@@ -44,6 +44,9 @@ class StepConfig(RreaParticleInteractionConfig):
     dt_s: float = 1.0e-12
     low_energy_cutoff_eV: float = 1.0e4
     photon_cutoff_eV: float = 5.0e4
+    # launch choice, not physics: route every particle through the complete
+    # step instead of the fast-path specialisation (results are identical)
+    complete_path_only: bool = False
 
     def validate(self, hard_moller_threshold_eV: float, brems_cutoff_eV: float) -> None:
         # RreaCouplingConfig.cpp:1331-1390: the charged cutoff must sit below the

@@ -221,6 +221,33 @@ __device__ __forceinline__ double elastic_semilog(DevElastic const& E, const dou
     return y[lo] + t * (y[hi] - y[lo]);
 }
 
+// interp_semilogx_strict on the elastic grid as a reusable bracket: the
+// collision mfp and the screening eta of one energy share it.
+struct EBracket { int32_t lo, hi, kind; double t; };  // kind 0 interior, 1 first, 2 last, 3 exact node
+
+__device__ __forceinline__ bool elastic_bracket(DevElastic const& E, double q, EBracket& b)
+{
+    b.lo = 0; b.hi = 0; b.kind = 1; b.t = 0.0;
+    if (!finite(q) || q < E.e[0] || q > E.e[E.n - 1]) return false;
+    if (q == E.e[0]) return true;
+    if (q == E.e[E.n - 1]) { b.kind = 2; return true; }
+    int32_t const hi = lower_bound(E.e, E.n, q);
+    b.hi = hi;
+    b.lo = hi - 1;
+    if (E.e[hi] == q) { b.kind = 3; return true; }
+    b.kind = 0;
+    b.t = (log(q) - E.log_e[b.lo]) / (E.log_e[b.hi] - E.log_e[b.lo]);
+    return true;
+}
+
+__device__ __forceinline__ double elastic_value(DevElastic const& E, const double* y, EBracket const& b)
+{
+    if (b.kind == 1) return y[0];
+    if (b.kind == 2) return y[E.n - 1];
+    if (b.kind == 3) return y[b.hi];
+    return y[b.lo] + b.t * (y[b.hi] - y[b.lo]);
+}
+
 // sample_conditional_cdf :995-1069 -- one group
 __device__ __forceinline__ double cdf_group_sample(DevCdf const& G, int32_t g, double cu)
 {
@@ -996,6 +1023,18 @@ __device__ bool deposit_segment(DevGrid const& g, double q, double dt, double r0
         if (a >= 2.0 && b >= 2.0) break;
     }
     return true;
+}
+
+// The range test of deposit_segment alone, so that a caller can decide
+// before committing anything.
+__device__ __forceinline__ bool deposit_segment_check(DevGrid const& g, double r0, double z0, double r1, double z1)
+{
+    double const xr0 = (r0 - g.r_lo) / g.dr - 0.5;
+    double const xz0 = (z0 - g.z_lo) / g.dz - 0.5;
+    double const xr1 = (r1 - g.r_lo) / g.dr - 0.5;
+    double const xz1 = (z1 - g.z_lo) / g.dz - 0.5;
+    double const lim_r = static_cast<double>(g.nr) + 2.0, lim_z = static_cast<double>(g.nz) + 2.0;
+    return fabs(xr0) <= lim_r && fabs(xr1) <= lim_r && fabs(xz0) <= lim_z && fabs(xz1) <= lim_z;
 }
 
 // DepositEscapingSegmentRZ :229-389

@@ -161,6 +161,9 @@ struct rrea_fields_s {
     size_t sort_tmp_bytes = 0;
     int64_t* d_count = nullptr;
     void* d_args = nullptr;      // device copy of the kernel's argument block
+    int32_t* d_slow_list = nullptr;  // fast-path hand-over list
+    int32_t* d_slow_count = nullptr;
+    int64_t slow_cap = 0;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     size_t events_used = 0;
@@ -301,16 +304,11 @@ __device__ double ensemble_drag(DevTables const& T, double e, double ratio, bool
     return realized + dd + tb;
 }
 
-__global__ void __launch_bounds__(kBlock) electron_step_kernel(const StepArgs* __restrict__ args)
+// The complete per-particle step (every branch of the reference loop body).
+__device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t ip, double* tally)
 {
-    StepArgs const& A = *args;
-    __shared__ double red[kTallyUsed][kBlock / 64];
-    double tally[kTallyUsed];
-    for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
-
-    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     int32_t err = kOk;
-    if (ip < A.n) {
+    {
         DevTables const& T = A.T;
         DevFields const& F = A.F;
         Domain const& D = A.D;
@@ -328,7 +326,7 @@ __global__ void __launch_bounds__(kBlock) electron_step_kernel(const StepArgs* _
             if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SKIPPED_INVALID);
             if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
             if (A.substeps) A.substeps[ip] = 0;
-            goto reduce;
+            return;
         }
         {
         double const pr = A.prev_r[ip], pz = A.prev_z[ip], pth = A.prev_th[ip];
@@ -806,8 +804,12 @@ __global__ void __launch_bounds__(kBlock) electron_step_kernel(const StepArgs* _
         }
         }
     }
-reduce:
-    // workgroup reduction of the tallies, one atomic per tally per workgroup
+}
+
+// workgroup reduction of the tallies, one atomic per tally per workgroup
+__device__ __forceinline__ void reduce_tallies(double* tally, double* out)
+{
+    __shared__ double red[kTallyUsed][kBlock / 64];
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
     for (int q = 0; q < kTallyUsed; ++q) {
@@ -818,8 +820,222 @@ reduce:
     if (threadIdx.x < kTallyUsed) {
         double s = 0.0;
         for (int wv = 0; wv < kBlock / 64; ++wv) s += red[threadIdx.x][wv];
-        if (s != 0.0) atomic_add(&A.tallies[threadIdx.x], s);
+        if (s != 0.0) atomic_add(&out[threadIdx.x], s);
     }
+}
+
+// The particles the fast path handed over, strided over a fixed grid.
+__global__ void __launch_bounds__(kBlock) electron_slow_kernel(const StepArgs* __restrict__ args,
+                                                             const int32_t* __restrict__ list,
+                                                             const int32_t* __restrict__ count)
+{
+    StepArgs const& A = *args;
+    double tally[kTallyUsed];
+    for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
+    int64_t const n = *count;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock) {
+        electron_full_particle(A, list[i], tally);
+    }
+    reduce_tallies(tally, A.tallies);
+}
+
+// ---------------------------------------------------------------------------
+// The fast path.  The dominant particle-step of the benchmark configurations
+// -- one substep, no discrete event, at most two single elastic scatters, no
+// boundary, survivor above the demotion gate -- is the same arithmetic as the
+// full path restricted to that branch; with one substep the substep chord is
+// bit-identical to the driver chord (same state, dt / 1 == dt), so its
+// geometry, density and field work are formed once.  Any other outcome is
+// detected before a single side effect and the particle index is appended to
+// the slow list, which the complete path then finishes from the unchanged
+// inputs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void append_slow(bool slow, int64_t ip, int32_t* list, int32_t* count)
+{
+    uint64_t const mask = __ballot(slow);
+    if (mask == 0) return;
+    int const lane = threadIdx.x & 63;
+    int const leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, static_cast<int32_t>(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    if (slow) {
+        uint64_t const below = mask & ((uint64_t{1} << lane) - 1U);
+        list[base + __popcll(below)] = static_cast<int32_t>(ip);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) electron_fast_kernel(const StepArgs* __restrict__ args,
+                                                             int32_t* __restrict__ slow_list,
+                                                             int32_t* __restrict__ slow_count)
+{
+    StepArgs const& A = *args;
+    double tally[kTallyUsed];
+    for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
+    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    bool slow = false;
+    if (ip < A.n) {
+        slow = true;  // cleared only when the whole fast branch completes
+        DevTables const& T = A.T;
+        DevFields const& F = A.F;
+        Domain const& D = A.D;
+        rrea_step_config const& cfg = A.cfg;
+        double const cutoff = cfg.low_energy_cutoff_eV;
+        double const gcut = cfg.photon_cutoff_eV;
+        double const thr = T.hard_moller_threshold;
+        bool const hm_on = cfg.disable_hard_moller == 0;
+        double const dt = cfg.dt_s;
+        int32_t err = kOk;
+        int64_t const id = A.id[ip];
+        if (id == RREA_INVALID_ID) {
+            if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SKIPPED_INVALID);
+            if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
+            if (A.substeps) A.substeps[ip] = 0;
+            slow = false;
+        } else do {
+            double const w = A.w[ip];
+            double const pr = A.prev_r[ip];
+            if (!(w > 0.0) || pr > D.r_hi + 1.0e-9 * D.r_hi) break;
+            double const pz = A.prev_z[ip], pth = A.prev_th[ip];
+            double dx = A.prev_ux[ip], dy = A.prev_uy[ip], dz = A.prev_uz[ip];
+            double k = kinetic_from_u(dx, dy, dz);
+            normalize(dx, dy, dz);
+            if (!(k >= cutoff)) break;
+            double run_x = pr * cos(pth);
+            double run_y = pr * sin(pth);
+            double run_z = pz;
+            Path path;
+            if (!build_chord(run_x, run_y, run_z, dx, dy, dz, k, dt, dt, D, path, err) || err) break;
+            if (clipped_escapes(path, D) || !(path.in_len > 0.0)) break;
+            Interval const iv = build_interval(F, path, D, A.density_ratio, err);
+            if (!iv.valid || err) break;
+            double const span = dmax(path.end_f, 1.0e-6);
+            FieldRange const fr = path_field_work(F, path, D, -1.0, 0.0, 1.0, err);
+            if (err) break;
+            double const field_work = fr.work / span;
+            double const chord_dp = density_path(iv, path, 0.0, path.end_f);
+            // substep sizing: all three drivers must ask for one substep
+            Bracket b0;
+            if (!charged_bracket(T, k, b0)) break;
+            double hm0, br0;
+            electron_stp_rates(T, b0, thr, gcut, k, hm_on, hm0, br0, err);
+            double const ddp = chord_dp / span;
+            double const target = cfg.max_optical_depth_per_substep;
+            if (dmax(1.0, ceil((hm0 * ddp + br0 * ddp) / target)) != 1.0) break;
+            if (!((hm0 + br0) * max_uniform_slice(iv, path, 1) <= target + 64.0 * kEps * dmax(1.0, target))) break;
+            EBracket eb0;
+            if (!elastic_bracket(T.el_e, k, eb0)) break;
+            double const mfp0 = elastic_value(T.el_e, T.el_e.mfp, eb0) / 1.0;
+            double const gs_n_total = (mfp0 > 0.0 ? 1.0 / mfp0 : 0.0) * ddp;
+            double const drag_est = (charged_value(T, kECol, b0, err) * 1.0 + charged_value(T, kESoft, b0, err) * 1.0) * ddp;
+            double const impulse_scale = dmax(cfg.max_field_impulse_fraction * k, dmax(cfg.field_impulse_floor_eV, 1.0));
+            if (dmax(1.0, ceil(dmin(gs_n_total, cfg.gs_total_collision_cap) / dmax(cfg.max_gs_collisions_per_substep, 1.0))) != 1.0) break;
+            if (dmax(1.0, ceil((fabs(field_work) + drag_est) / impulse_scale)) != 1.0) break;
+            if (err) break;
+            // the single substep: the driver chord again (sub_dt = dt / 1)
+            double const se = path.end_f;
+            double const substep_dt = dt * (se - 0.0);
+            double k1 = k, d1x = dx, d1y = dy, d1z = dz;
+            double const hw1 = half_kick(fr, 0.5 * substep_dt, -1.0, k1, d1x, d1y, d1z);
+            if (!(k1 >= cutoff) || !(0.0 < se)) break;
+            // the first hazard attempt: no event
+            Bracket b1;
+            if (!charged_bracket(T, k1, b1)) break;
+            double s0, s1;
+            electron_stp_rates(T, b1, thr, gcut, k1, hm_on, s0, s1, err);
+            double const L = density_path(iv, path, 0.0, se);
+            uint64_t const rng = rng_prefix(cfg.rng_seed, static_cast<uint64_t>(id < 0 ? -id : id)
+                                                ^ (static_cast<uint64_t>(static_cast<uint32_t>(A.cpu ? A.cpu[ip] : 0)) << 40),
+                                            static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0));
+            Hazard const h = select_hazard(s0 * L, s1 * L, rng_uniform(rng, 0, kChCompetingOpticalDepth),
+                                           rng_uniform(rng, 0, kChCompetingChannel));
+            if (!h.valid || h.event || err) break;
+            // continuous drag over [0, se] at the frozen energy k1, not reaching the cutoff
+            double const col = charged_value(T, kECol, b1, err) * 1.0;
+            double const soft = charged_value(T, kESoft, b1, err) * 1.0;
+            double const total = col + soft;
+            double const to_cut = k1 - cutoff;
+            double const req = total * L;
+            if (!finite(req) || req < 0.0 || !(req < to_cut) || !(k1 > 0.0) || err) break;
+            double const raw_col = col * L;
+            double const raw_soft = soft * L;
+            double const avail = dmax(0.0, k1 - cutoff);
+            double const drag = dmin(raw_col + raw_soft, avail);
+            double const col_loss = dmin(raw_col, drag);
+            double ion = 0.0;
+            if (col_loss > 0.0 && raw_col > 0.0) {
+                double const le = dmin(se, iv.end);
+                if (le > 0.0) {
+                    double const raw_i = path.full * (le - 0.0) * iv.ratio * col;
+                    double const loss_i = col_loss * raw_i / raw_col;
+                    double const pw = w * loss_i / dmax(kWAir, 1.0);
+                    if (pw > 0.0) ion += pw;
+                }
+            }
+            double k2 = dmax(0.0, k1 - drag);
+            if (!(k2 >= cutoff)) break;
+            // elastic scattering: the few-collision branch with at most two scatters
+            EBracket eb2;
+            if (!elastic_bracket(T.el_e, k2, eb2)) break;
+            double const mfp2 = elastic_value(T.el_e, T.el_e.mfp, eb2) / 1.0;
+            double const gs_n = (mfp2 > 0.0 ? 1.0 / mfp2 : 0.0) * L;
+            if (gs_n > 0.0) {
+                if (!(gs_n < T.gs_few) || ceil(3.0 * T.gs_few) >= 255.0) break;
+                int const cnt = poisson_capped(gs_n, rng_uniform(rng, 0U, kChGsFewCount), static_cast<int>(ceil(3.0 * T.gs_few)));
+                if (cnt > 2) break;
+                if (cnt > 0 && !(k2 >= T.charged_min && k2 <= T.charged_max)) break;
+                for (int q = 0; q < cnt; ++q) {
+                    uint64_t const inner = static_cast<uint64_t>(q + 1);
+                    double const eta = elastic_value(T.el_e, T.el_e.eta, eb2);
+                    double const u = dmin(dmax(rng_uniform(rng, inner, kChGsFewScatter), 0.0), 1.0 - 1.0e-15);
+                    double const ct = 1.0 - 2.0 * eta * u / (1.0 + eta - u);
+                    double nx, ny, nz;
+                    cone_direction(d1x, d1y, d1z, ct, rng_uniform(rng, inner, kChGsFewAzimuth), nx, ny, nz);
+                    d1x = nx; d1y = ny; d1z = nz;
+                }
+            }
+            double const hw2 = half_kick(fr, 0.5 * substep_dt, -1.0, k2, d1x, d1y, d1z);
+            if (!(k2 >= cutoff)) break;
+            double x, y, z;
+            point_at_full(path, se, x, y, z);
+            if (!in_half_open(x, y, z, D) || path.exits) break;
+            // terminal: a survivor above the demotion gate's energy
+            if (thr > 0.0 && k2 <= thr) break;
+            double const end_r = hypot(x, y);
+            if (!deposit_segment_check(A.G, pr, pz, end_r, z)) break;
+            // ---- commit: nothing above has written anything ----
+            slow = false;
+            deposit_segment(A.G, -w, dt, pr, pz, end_r, z);
+            double const um = proper_velocity(k2);
+            A.ux[ip] = um * d1x; A.uy[ip] = um * d1y; A.uz[ip] = um * d1z;
+            A.r[ip] = end_r;
+            A.th[ip] = atan2(y, x);
+            A.z[ip] = z;
+            if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SURVIVE);
+            if (A.ion_pairs) A.ion_pairs[ip] = ion;
+            if (A.substeps) A.substeps[ip] = 1;
+            tally[0] += 1.0;
+            tally[6] += 1.0;
+            tally[7] += w * drag;
+            tally[8] += ion;
+            tally[9] += w * hw1;
+            tally[9] += w * hw2;
+        } while (false);
+    }
+    append_slow(slow, ip, slow_list, slow_count);
+    reduce_tallies(tally, A.tallies);
+}
+
+// Every particle through the complete path (used when the fast path is off).
+__global__ void __launch_bounds__(kBlock) electron_full_kernel(const StepArgs* __restrict__ args)
+{
+    StepArgs const& A = *args;
+    double tally[kTallyUsed];
+    for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
+    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (ip < A.n) electron_full_particle(A, ip, tally);
+    reduce_tallies(tally, A.tallies);
 }
 
 // ---------------------------------------------------------------------------
@@ -1008,7 +1224,8 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
 {
     if (!F) return RREA_OK;
     void* ptrs[] = {F->d_cell_er, F->d_cell_ez, F->d_btheta, F->d_stage, F->d_prof, F->d_stage_sec, F->d_keys,
-                    F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args};
+                    F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args,
+                    F->d_slow_list, F->d_slow_count};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : F->events) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
     delete F;
@@ -1223,9 +1440,28 @@ rrea_status rrea_transport_step_electrons(rrea_tables_t T, rrea_fields_t F, cons
         ev_stop = F->events[F->events_used].second;
         ++F->events_used;
     }
-    electron_step_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-        static_cast<const StepArgs*>(F->d_args));
-    HIP_TRY(hipGetLastError());
+    const StepArgs* const d_args = static_cast<const StepArgs*>(F->d_args);
+    if (cfg->complete_path_only) {
+        electron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
+        HIP_TRY(hipGetLastError());
+    } else {
+        if (F->slow_cap < tile->n) {
+            if (F->d_slow_list) (void)hipFree(F->d_slow_list);
+            F->d_slow_list = nullptr;
+            if (tile->n > INT32_MAX) return RREA_ERR_ARGUMENT;
+            HIP_TRY(hipMalloc(&F->d_slow_list, tile->n * sizeof(int32_t)));
+            F->slow_cap = tile->n;
+        }
+        if (!F->d_slow_count) HIP_TRY(hipMalloc(&F->d_slow_count, sizeof(int32_t)));
+        HIP_TRY(hipMemsetAsync(F->d_slow_count, 0, sizeof(int32_t), s));
+        electron_fast_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args, F->d_slow_list,
+                                                                                     F->d_slow_count);
+        HIP_TRY(hipGetLastError());
+        int64_t const slow_blocks = nb < 2048 ? nb : 2048;
+        electron_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
+            d_args, F->d_slow_list, F->d_slow_count);
+        HIP_TRY(hipGetLastError());
+    }
     if (ev_stop) HIP_TRY(hipEventRecord(ev_stop, s));
 
     if (order) {

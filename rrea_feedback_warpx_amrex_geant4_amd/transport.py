@@ -238,7 +238,7 @@ def _step_config(cfg: StepConfig) -> _lib.StepConfigC:
         cfg.max_optical_depth_per_substep, cfg.max_subcycles, int(cfg.strict_transport_guards),
         int(cfg.debug_disable_hard_moller_secondaries), cfg.transport_max_gs_collisions_per_substep,
         cfg.transport_gs_total_collision_cap, cfg.transport_max_field_impulse_fraction,
-        cfg.transport_field_impulse_floor_eV)
+        cfg.transport_field_impulse_floor_eV, int(cfg.complete_path_only), 0)
 
 
 class RreaParticleInteraction:

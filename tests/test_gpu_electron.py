@@ -52,6 +52,17 @@ def test_c2_sample_parity(tables, otables):
     run_and_compare(otables, tables, cp, fields, tile, cfg, steps=3, label="c2")
 
 
+def test_c2_sample_complete_path_only(tables, otables):
+    """The same sample with every particle through the complete step (the
+    fast-path specialisation off): both launch shapes give the oracle's results."""
+    fields = wl.c2_fields()
+    tile = wl.c2_electrons(1 << 14, fields, seed=22)
+    cfg = StepConfig(rng_seed=78, dt_s=1e-12, complete_path_only=True)
+    cp = coupling_for(fields, tables)
+    print("case c2 2^14 e- complete path only x 2 steps", flush=True)
+    run_and_compare(otables, tables, cp, fields, tile, cfg, steps=2, label="c2-full")
+
+
 def test_long_step_events_and_multiple_scattering(tables, otables):
     """dt = 1 ns: several substeps, Moller and brems events, secondaries, the
     GS multiple-scattering branch and cutoff stops all occur."""
