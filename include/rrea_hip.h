@@ -177,7 +177,8 @@ typedef struct rrea_tile {
 typedef struct rrea_secondary {
     int64_t parent_index;
     int32_t species;        /* 0 e-, 1 e+, 2 photon */
-    int32_t process;        /* RreaSecondaryProcess id: 2 hard Moller, 4 brems, ... */
+    int32_t process;        /* RreaSecondaryProcess id: 2 hard Moller, 3 hard Bhabha,
+                               4 brems, 9 positron annihilation */
     double x, y, z;
     double dir_x, dir_y, dir_z;
     double weight;
@@ -194,11 +195,16 @@ enum {
     RREA_ZERO_WEIGHT = 3,
     RREA_NO_WRITEBACK = 4,
     RREA_SKIPPED_INVALID = 5,
+    RREA_ANNIHILATED = 6,   /* positron consumed: in flight, stopped mid-chord, or at rest */
 };
 
 #define RREA_INVALID_ID INT64_MIN
 
 #define RREA_TALLY_COUNT 16
+/* Tally slots: 0 survivors, 1 escapes, 2 demotions, 3 hard-Moller events,
+ * 4 brems events, 5 tracked brems photons, 6 substeps, 7 continuous drag
+ * (weight x eV), 8 ion pairs, 9 field work (weight x eV), 10 annihilations,
+ * 11 hard-Bhabha events, 12 positive-ion-only sources (weight). */
 
 typedef struct rrea_step_out {
     uint8_t* status;          /* per particle, RREA_SURVIVE.. (may be NULL) */
@@ -213,6 +219,17 @@ typedef struct rrea_step_out {
 } rrea_step_out;
 
 rrea_status rrea_transport_step_electrons(rrea_tables_t tables, rrea_fields_t fields, const rrea_tile* tile,
+                                          const rrea_step_config* config, const rrea_step_out* out,
+                                          rrea_stream_t stream);
+
+/* One positron transport step over a tile: the particle loop of
+ * TransportStep::ApplyPositronTransport
+ * (warpx_overlay/Source/Rrea/RreaPositronTransport.cpp:10-871) -- hard
+ * Bhabha, tracked brems and in-flight annihilation as competing hazards,
+ * at-rest annihilation of a stopped or sub-cutoff positron.  Same arguments
+ * and error behaviour as rrea_transport_step_electrons; complete_path_only is
+ * ignored (positrons always take the complete path). */
+rrea_status rrea_transport_step_positrons(rrea_tables_t tables, rrea_fields_t fields, const rrea_tile* tile,
                                           const rrea_step_config* config, const rrea_step_out* out,
                                           rrea_stream_t stream);
 

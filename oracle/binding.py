@@ -123,10 +123,17 @@ def lib():
         L.orc_deposit_escaping_segment.argtypes = ([C.c_int, C.c_int] + [C.c_double] * 4 + [P_D] * 3
                                                    + [C.c_double] * 2 + [P_D, C.c_int, C.c_int, C.c_int, C.c_uint32])
         L.orc_deposit_escaping_segment.restype = C.c_int
-        L.orc_electron_step.restype = C.c_int
-        L.orc_electron_step.argtypes = ([C.c_void_p, C.POINTER(Geometry), C.POINTER(Config), C.c_int64]
-                                        + [P_D] * 7 + [P_I64, P_I32] + [P_D] * 6 + [P_I32, P_D, P_I64]
-                                        + [C.c_int64, C.POINTER(Secondary), P_I64] + [P_D] * 4 + [C.c_int])
+        for name in ("orc_electron_step", "orc_positron_step"):
+            f = getattr(L, name)
+            f.restype = C.c_int
+            f.argtypes = ([C.c_void_p, C.POINTER(Geometry), C.POINTER(Config), C.c_int64]
+                          + [P_D] * 7 + [P_I64, P_I32] + [P_D] * 6 + [P_I32, P_D, P_I64]
+                          + [C.c_int64, C.POINTER(Secondary), P_I64] + [P_D] * 4 + [C.c_int])
+        L.orc_sample_annihilation.argtypes = [C.c_double, C.c_double, C.c_double, P_D]
+        for name in ("orc_sample_bhabha", "orc_sample_positron_brems_energy"):
+            f = getattr(L, name)
+            f.restype = C.c_double
+            f.argtypes = [C.c_void_p, C.c_double, C.c_double, P_I32]
         _lib = L
     return _lib
 
@@ -217,6 +224,12 @@ class Tables:
     def sample_brems_energy(self, e, u):
         return self._call("orc_sample_brems_energy", e, u)
 
+    def sample_bhabha(self, e, u):
+        return self._call("orc_sample_bhabha", e, u)
+
+    def sample_positron_brems_energy(self, e, u):
+        return self._call("orc_sample_positron_brems_energy", e, u)
+
     def sample_brems_cos(self, e, u):
         return self._call("orc_sample_brems_cos", e, u)
 
@@ -280,9 +293,27 @@ def make_config(cfg) -> Config:
     return c
 
 
+def sample_annihilation(k: float, draw_energy: float, draw_azimuth: float) -> dict:
+    """SamplePositronAnnihilationFinalState of the oracle."""
+    out = np.zeros(6)
+    lib().orc_sample_annihilation(k, draw_energy, draw_azimuth, _ptr(out))
+    return {"e1": out[0], "e2": out[1], "cos1": out[2], "azimuth_u": out[3], "valid": bool(out[4]),
+            "at_rest": bool(out[5])}
+
+
 def electron_step(tables: Tables, fields, cfg, particles: dict, nthreads: int = 1, sec_cap: int | None = None):
-    """Run the oracle step.  ``particles`` holds prev r, z, theta, ux, uy, uz,
-    w (float64), id (int64) and cpu (int32).  Returns a dict of outputs."""
+    """Run the oracle electron step.  ``particles`` holds prev r, z, theta, ux,
+    uy, uz, w (float64), id (int64) and cpu (int32).  Returns a dict of outputs."""
+    return charged_step("electron", tables, fields, cfg, particles, nthreads, sec_cap)
+
+
+def positron_step(tables: Tables, fields, cfg, particles: dict, nthreads: int = 1, sec_cap: int | None = None):
+    """Run the oracle positron step (same inputs and outputs as electron_step)."""
+    return charged_step("positron", tables, fields, cfg, particles, nthreads, sec_cap)
+
+
+def charged_step(species: str, tables: Tables, fields, cfg, particles: dict, nthreads: int = 1,
+                 sec_cap: int | None = None):
     L = lib()
     n = len(particles["r"])
     g, keep = make_geometry(fields)
@@ -302,7 +333,7 @@ def electron_step(tables: Tables, fields, cfg, particles: dict, nthreads: int = 
     jz = np.zeros(nrp * (nzp + 1))
     rho = np.zeros(nrp * nzp)
     tallies = np.zeros(N_TALLIES)
-    code = L.orc_electron_step(
+    code = getattr(L, f"orc_{species}_step")(
         tables.h, C.byref(g), C.byref(c), n, *[_ptr(a) for a in ins], _ptr(pid, P_I64), _ptr(cpu, P_I32),
         *[_ptr(outs[k]) for k in ("r", "z", "theta", "ux", "uy", "uz")], _ptr(status, P_I32), _ptr(ion),
         _ptr(sub, P_I64), cap, sec, C.byref(nsec), _ptr(jr), _ptr(jz), _ptr(rho), _ptr(tallies), nthreads)

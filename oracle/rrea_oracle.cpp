@@ -102,7 +102,7 @@ struct UniformStream {
     }
 };
 
-// RNG channel ids -- OV/RreaRng.H:16-98 (the subset the electron step uses)
+// RNG channel ids -- OV/RreaRng.H:16-98 (the subset the charged steps use)
 enum : uint64_t {
     kChHardMollerEnergy = 22,
     kChHardMollerAzimuth = 23,
@@ -118,7 +118,26 @@ enum : uint64_t {
     kChCompetingChannel = 108,
     kChContinuationOpticalDepth = 124,
     kChContinuationChannel = 125,
+    kChPositronBhabhaEnergy = 74,
+    kChPositronBremsPhotonEnergy = 76,
+    kChPositronBremsPhotonAngle = 77,
+    kChPositronBremsPhotonAzimuth = 78,
+    kChPositronAnnihilationPolar = 80,
+    kChPositronAnnihilationAzimuth = 81,
+    kChPositronGsAngle = 94,
+    kChPositronGsAzimuth = 95,
+    kChPositronGsFewCount = 96,
+    kChPositronGsFewScatter = 97,
+    kChPositronGsFewAzimuth = 98,
+    kChPositronCompetingOpticalDepth = 109,
+    kChPositronCompetingChannel = 110,
+    kChPositronContinuationOpticalDepth = 126,
+    kChPositronContinuationChannel = 127,
+    kChPositronBhabhaAzimuth = 130,
 };
+
+// RreaSecondaryProcess ids, OV/RreaParticleInteraction.H:34-48
+enum : int32_t { kProcHardMoller = 2, kProcHardBhabha = 3, kProcBrems = 4, kProcAnnihilation = 9 };
 
 // OV/RreaParticleInteraction.H:642-710
 constexpr uint64_t kLineageCap = uint64_t{1} << 14;
@@ -476,6 +495,20 @@ struct Tables {
     {
         require_charged(primary);
         return sample_conditional_cdf(ebrems, primary, cdf_u, true);
+    }
+
+    // SamplePositronBremsstrahlungPhotonEnergy :2562-2575
+    double sample_positron_brems_energy(double primary, double cdf_u) const
+    {
+        require_charged(primary);
+        return sample_conditional_cdf(pbrems, primary, cdf_u, true);
+    }
+
+    // SamplePositronBhabhaSecondaryEnergy :3180-3194
+    double sample_bhabha(double primary, double cdf_u) const
+    {
+        require_charged(primary);
+        return sample_conditional_cdf(bhabha, primary, cdf_u, true);
     }
 
     // SampleBremsstrahlungCosTheta :2714-2760 (bounded ModifiedTsai by bisection)
@@ -1077,6 +1110,86 @@ inline void photon_recoil(double k0, double eg, double gx, double gy, double gz,
 }
 
 // ---------------------------------------------------------------------------
+// Two-photon positron annihilation -- NA/src/RreaInteractionTables.cpp
+// ---------------------------------------------------------------------------
+// clamp_unit / wrap_unit_interval :43-61
+inline double clamp_unit(double v)
+{
+    if (!std::isfinite(v)) return 0.5;
+    return std::max(0.0, std::min(1.0, v));
+}
+
+inline double wrap_unit_interval(double v)
+{
+    if (!std::isfinite(v)) return 0.0;
+    v = v - std::floor(v);
+    if (v < 0.0) v += 1.0;
+    return v;
+}
+
+// sample_annihilation_photon_energy_fraction :63-108: the Heitler spectrum
+// inverted by 64 bisections of its antiderivative a ln e - e + 1/((g+1)^2 e)
+// over the centre-of-mass support [(1 - b_cm)/2, (1 + b_cm)/2].
+inline double annihilation_energy_fraction(double gamma, double draw)
+{
+    if (gamma <= 1.0 + 1.0e-12) return 0.5;
+    double const beta_cm = std::sqrt(std::max(0.0, (gamma - 1.0) / (gamma + 1.0)));
+    double const eps_min = 0.5 * (1.0 - beta_cm);
+    double const eps_max = 0.5 * (1.0 + beta_cm);
+    double const u = clamp_unit(draw);
+    double const gp1 = gamma + 1.0;
+    double const a = 1.0 + 2.0 * gamma / (gp1 * gp1);
+    auto anti = [a, gp1](double e) { return a * std::log(e) - e + 1.0 / (gp1 * gp1 * e); };
+    double const f_lo = anti(eps_min);
+    double const f_hi = anti(eps_max);
+    double const span = f_hi - f_lo;
+    if (!(span > 0.0)) return 0.5;
+    double const target = f_lo + u * span;
+    double lo = eps_min, hi = eps_max;
+    for (int it = 0; it < 64; ++it) {
+        double const mid = 0.5 * (lo + hi);
+        if (anti(mid) < target) lo = mid; else hi = mid;
+    }
+    return std::max(eps_min, std::min(eps_max, 0.5 * (lo + hi)));
+}
+
+struct AnnihilationState {
+    double e1 = 0.0, e2 = 0.0, cos1 = 1.0, az1 = 0.0;
+    bool valid = false, at_rest = false;
+};
+
+// SamplePositronAnnihilationFinalState :3208-3253 (the energy range check,
+// which admits an exact zero, is the caller's)
+inline AnnihilationState sample_annihilation(double k, double draw_energy, double draw_azimuth)
+{
+    AnnihilationState s;
+    k = std::max(0.0, k);
+    double const gamma = 1.0 + k / kMeC2;
+    double const total = 2.0 * kMeC2 + k;
+    if (gamma <= 1.0 + 1.0e-10) {
+        s.e1 = kMeC2;
+        s.e2 = kMeC2;
+        s.cos1 = 2.0 * clamp_unit(draw_energy) - 1.0;
+        s.az1 = wrap_unit_interval(draw_azimuth);
+        s.valid = true;
+        s.at_rest = true;
+        return s;
+    }
+    double const beta = std::sqrt(std::max(0.0, 1.0 - 1.0 / (gamma * gamma)));
+    double const eps = annihilation_energy_fraction(gamma, draw_energy);
+    s.e1 = eps * total;
+    s.e2 = std::max(0.0, total - s.e1);
+    if (s.e1 <= 0.0 || s.e2 <= 0.0) return s;
+    double const denom = eps * gamma * beta;
+    if (denom <= 0.0) return s;
+    s.cos1 = std::max(-1.0, std::min(1.0, (eps * (gamma + 1.0) - 1.0) / denom));
+    s.az1 = wrap_unit_interval(draw_azimuth);
+    s.valid = true;
+    s.at_rest = false;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
 // Competing hazard -- OV/RreaParticleInteraction.H:1357-1492
 // ---------------------------------------------------------------------------
 struct Hazard { bool valid = false, event = false; int channel = -1; double sampled = 0, total = 0; };
@@ -1334,43 +1447,77 @@ inline uint64_t select_material_subcycles(double max_tau, std::vector<Interval> 
     }
 }
 
+// mean_free_path_from_inverse_length NA/src/RreaInteractionTables.cpp:110-119
+// followed by RreaChannelInverseLength OV/RreaChargedInterleave.H:69-78
+inline double channel_inverse_length(double inverse_per_m)
+{
+    if (!std::isfinite(inverse_per_m) || inverse_per_m < 0.0) fail(ORC_ERR_INVALID, "inverse length invalid");
+    double const mfp = inverse_per_m > 0.0 ? 1.0 / inverse_per_m : 1.0e99;
+    return mfp > 0.0 ? 1.0 / mfp : 0.0;
+}
+
 // RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:107-152 (electron)
 inline void electron_stp_rates(Tables const& t, double thr, double gcut, double e, bool hm_on, double out[3])
 {
     bool const hm_open = hm_on && thr > 0.0 && e > 2.0 * thr;
     double hm = 0.0;
     if (hm_open) {
-        // ElectronHardMollerMeanFreePath :2394-2412 then 1/mfp
-        double const inv = t.rate[kEMoller].interpolate(e);
-        double mfp;
-        if (thr > 0.0 && e <= 2.0 * thr) mfp = 1.0e99;
-        else {
-            double const v = inv * 1.0;
-            if (!std::isfinite(v) || v < 0.0) fail(ORC_ERR_INVALID, "inverse length invalid");
-            mfp = v > 0.0 ? 1.0 / v : 1.0e99;
-        }
-        hm = mfp > 0.0 ? 1.0 / mfp : 0.0;
+        // ElectronHardMollerMeanFreePath :2394-2412 (its K <= 2 T_cut branch is
+        // behind the closed gate)
+        hm = channel_inverse_length(t.rate[kEMoller].interpolate(e) * 1.0);
     }
     double br = 0.0;
     if (std::isfinite(e) && std::isfinite(gcut) && e > gcut) {
         // ElectronBremsstrahlungMeanFreePath :2666-2676
-        double const v = t.rate[kEBrems].interpolate(e) * 1.0;
-        if (!std::isfinite(v) || v < 0.0) fail(ORC_ERR_INVALID, "inverse length invalid");
-        double const mfp = v > 0.0 ? 1.0 / v : 1.0e99;
-        br = mfp > 0.0 ? 1.0 / mfp : 0.0;
+        br = channel_inverse_length(t.rate[kEBrems].interpolate(e) * 1.0);
     }
     out[0] = frozen(hm);
     out[1] = frozen(br);
     out[2] = 0.0;
 }
 
-// RreaChargedChannelOpenAt OV/RreaChargedFinalStates.H:56-96 (electron)
-inline bool electron_channel_open(Tables const& t, double thr, double gcut, int ch, double e, bool hm_on)
+// RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:114-133 (positron):
+// hard Bhabha (open unless K <= T_cut), tracked brems (K > photon cutoff),
+// in-flight annihilation (always open).
+inline void positron_stp_rates(Tables const& t, double thr, double gcut, double e, double out[3])
+{
+    double bh = 0.0;
+    if (!(thr > 0.0 && e <= thr)) {
+        // PositronHardBhabhaMeanFreePath :3163-3178
+        bh = channel_inverse_length(t.rate[kPBhabha].interpolate(e) * 1.0);
+    }
+    double br = 0.0;
+    if (e > gcut) {
+        // PositronBremsstrahlungMeanFreePath :3151-3161
+        br = channel_inverse_length(t.rate[kPBrems].interpolate(e) * 1.0);
+    }
+    // PositronAnnihilationMeanFreePath :3196-3206
+    double const an = channel_inverse_length(t.rate[kPAnnih].interpolate(e) * 1.0);
+    out[0] = frozen(bh);
+    out[1] = frozen(br);
+    out[2] = frozen(an);
+}
+
+// RreaChargedChannelOpenAt OV/RreaChargedFinalStates.H:56-96; the
+// Electron/PositronTransportCoefficients call range-checks and evaluates
+// every rate of the species.
+inline bool charged_channel_open(Tables const& t, bool positron, double thr, double gcut, int ch, double e, bool hm_on)
 {
     if (!std::isfinite(e) || e < t.charged_min || e > t.charged_max) {
-        fail(ORC_ERR_TABLE_RANGE, "schema-6 electron energy outside certified range");
+        fail(ORC_ERR_TABLE_RANGE, "schema-6 charged energy outside certified range");
     }
-    // ElectronTransportCoefficients :2298-2324 evaluates all five rates.
+    if (positron) {
+        (void)t.rate[kPCol].interpolate(e);
+        (void)t.rate[kPSoft].interpolate(e);
+        double const bh_inv = t.rate[kPBhabha].interpolate(e) * 1.0;
+        double const br_inv = t.rate[kPBrems].interpolate(e) * 1.0;
+        (void)t.rate[kPEltr].interpolate(e);
+        double const an_inv = t.rate[kPAnnih].interpolate(e) * 1.0;
+        if (ch == 0) return !(thr > 0.0 && e <= thr) && bh_inv > 0.0;
+        if (ch == 1) return e > gcut && br_inv > 0.0;
+        if (ch == 2) return an_inv > 0.0;
+        return false;
+    }
     double const hm_inv = t.rate[kEMoller].interpolate(e) * 1.0;
     double const br_inv = t.rate[kEBrems].interpolate(e) * 1.0;
     (void)t.rate[kECol].interpolate(e);
@@ -1403,18 +1550,21 @@ inline double ensemble_drag(Tables const& t, double e, double ratio, bool hm_on)
     return realized_loss(t, e, ratio) + dd + t.tracked_brems_energy_loss(e, ratio);
 }
 
-// One electron, the body of the particle loop in
-// OV/RreaElectronTransport.cpp:59-889, with the shared prologue
-// BeginRunningChargedParticle OV/RreaTransportStep.H:720-820.
-struct ElectronOut {
+// One charged particle: the body of the particle loop of
+// OV/RreaElectronTransport.cpp:59-889 (positron = false) or
+// OV/RreaPositronTransport.cpp:58-862 (positron = true), with the shared
+// prologue BeginRunningChargedParticle OV/RreaTransportStep.H:720-820, the
+// substep skeleton RreaRunChargedSubstepInterleave and the event skeleton
+// RreaRunChargedEventInterleave (OV/RreaChargedInterleave.H:546-732).
+struct ChargedOut {
     double r, z, th, ux, uy, uz, ion_pairs;
     int status;
     uint64_t substeps;
 };
 
-inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, double pr, double pz, double pth,
-                              double pux, double puy, double puz, double w, int64_t id, int32_t cpu,
-                              ElectronOut& out)
+inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const pos, int64_t ip, double pr, double pz,
+                             double pth, double pux, double puy, double puz, double w, int64_t id, int32_t cpu,
+                             ChargedOut& out)
 {
     Tables const& t = *ctx.t;
     orc_config const& cfg = ctx.cfg;
@@ -1423,6 +1573,8 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
     double const thr = t.hard_moller_threshold;
     bool const hm_on = cfg.disable_hard_moller == 0;
     double const dt = cfg.dt_s;
+    double const charge = pos ? 1.0 : -1.0;
+    uint64_t const species = pos ? 1U : 0U;
     uint64_t const step = static_cast<uint64_t>(std::max<int64_t>(cfg.step, 0));
     uint64_t const seed = cfg.rng_seed;
 
@@ -1430,7 +1582,7 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
     out.substeps = 0;
     uint64_t const pkey = static_cast<uint64_t>(std::llabs(id)) ^ (static_cast<uint64_t>(static_cast<uint32_t>(cpu)) << 40);
     if (w <= 0.0) { out.status = ORC_ZERO_WEIGHT; return; }
-    if (pr > ctx.r_hi + 1.0e-9 * ctx.r_hi) fail(ORC_ERR_OUTSIDE_DOMAIN, "electron survived outside the domain");
+    if (pr > ctx.r_hi + 1.0e-9 * ctx.r_hi) fail(ORC_ERR_OUTSIDE_DOMAIN, "charged particle survived outside the domain");
 
     double dx = pux, dy = puy, dz = puz;
     double k = kinetic_from_proper_velocity(dx, dy, dz);
@@ -1438,6 +1590,7 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
     double run_x = pr * std::cos(pth);
     double run_y = pr * std::sin(pth);
     double run_z = pz;
+    double const x_particle = run_x, y_particle = run_y;
     double const r_particle = pr, z_particle = pz;
 
     Chord const driver = build_chord(run_x, run_y, run_z, dx, dy, dz, k, dt, dt, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
@@ -1446,33 +1599,38 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
     double const ds = driver.valid ? dpath.in_length : 0.0;
     std::vector<Interval> const div = build_intervals(ctx.cp, dpath, ctx.density_ratio);
     double const span = std::max(dpath.end_f, 1.0e-6);
-    double const field_work = path_field_work(ctx.cp, dpath, -1.0, 0.0, 1.0).work / span;
+    double const field_work = path_field_work(ctx.cp, dpath, charge, 0.0, 1.0).work / span;
 
     double const chord_dp = density_path(div, dpath, dpath.begin_f, dpath.end_f);
-    double particle_ratio = ds > 0.0 ? chord_dp / ds : positive_or(ctx.cp.density_ratio_at_z(z_particle), ctx.density_ratio);
+    // the electron demotion gate's density, OV/RreaElectronTransport.cpp:102-105
+    double particle_ratio = 0.0;
+    if (!pos) particle_ratio = ds > 0.0 ? chord_dp / ds : positive_or(ctx.cp.density_ratio_at_z(z_particle), ctx.density_ratio);
     double tr_dp = 0.0, tr_len = 0.0;
     uint64_t lineage = 0;  // spawn_transient_lineage, OV/RreaTransportLog.H:179,212,445
     Path path = dpath;
     std::vector<Interval> iv = div;
 
     auto ion_event = [&](double ev_w) { out.ion_pairs += ev_w; fx.tallies[ORC_T_ION_PAIRS] += ev_w; };
+    // deposit_local_energy_as_ion_pairs OV/RreaTransportStep.H:537-561
     auto deposit_local = [&](double energy) {
         if (w <= 0.0 || energy <= 0.0) return;
         double const pw = w * energy / std::max(kWAir, 1.0);
         if (pw <= 0.0) return;
         ion_event(pw);
     };
+    // the net chord carries the species' signed charge (-w for e-, +w for e+)
     auto segment = [&](double r0, double z0, double r1, double z1) {
-        DepositOp op{0, {r0, z0, r1, z1, -w, 0.0, 0.0}, 0U};
+        DepositOp op{0, {r0, z0, r1, z1, charge * w, 0.0, 0.0}, 0U};
         fx.deposits.push_back(op);
     };
     auto commit_escape = [&]() {
+        // CommitChargedEscape OV/RreaTransportStep.H:829-863
         double const um = proper_velocity_from_kinetic(k);
         out.ux = um * dx; out.uy = um * dy; out.uz = um * dz;
         out.r = pr; out.z = pz; out.th = pth;
         BoundaryCrossing const bc = boundary_crossing(path);
         if (!bc.valid) fail(ORC_ERR_INVALID, "charged escape has no certified absorbing-face crossing");
-        DepositOp op{1, {r_particle, z_particle, bc.start_r, bc.start_z, bc.cross_r, bc.cross_z, -w}, bc.faces};
+        DepositOp op{1, {r_particle, z_particle, bc.start_r, bc.start_z, bc.cross_r, bc.cross_z, charge * w}, bc.faces};
         fx.deposits.push_back(op);
         out.status = ORC_ESCAPE;
         fx.tallies[ORC_T_ESCAPES] += 1.0;
@@ -1484,11 +1642,67 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
         out.status = ORC_DEMOTED;
         fx.tallies[ORC_T_DEMOTIONS] += 1.0;
     };
+    auto push_secondary = [&](int sp, int process, double x, double y, double z, double sdx, double sdy, double sdz,
+                              double energy, double birth) {
+        orc_secondary s{ip, sp, process, x, y, z, sdx, sdy, sdz, w, energy, birth, pkey, lineage++};
+        fx.secondaries.push_back(s);
+    };
+    // spawn_positron_annihilation OV/RreaTransportStep.H:563-700: two photons
+    // (tracked at or above the photon cutoff, else deposited locally), the
+    // residual kinetic energy of an at-rest annihilation deposited as ion pairs,
+    // and the positive ion the annihilated target electron leaves behind.
+    auto annihilate = [&](double x, double y, double z, bool at_rest, uint64_t index, double birth) {
+        if (w <= 0.0) return;
+        double ax = dx, ay = dy, az = dz;
+        normalize(ax, ay, az);
+        double const ka = at_rest ? 0.0 : std::max(k, 0.0);
+        if (!(ka == 0.0) && (!std::isfinite(ka) || ka < t.charged_min || ka > t.charged_max)) {
+            fail(ORC_ERR_TABLE_RANGE, "annihilation energy outside certified range");
+        }
+        AnnihilationState const st = sample_annihilation(ka, rng_uniform(seed, pkey, step, index, kChPositronAnnihilationPolar),
+                                                         rng_uniform(seed, pkey, step, index, kChPositronAnnihilationAzimuth));
+        double g1x, g1y, g1z;
+        cone_direction(ax, ay, az, st.cos1, st.az1, g1x, g1y, g1z);
+        double const e1 = std::max(st.e1, 0.0);
+        double const e2 = std::max(st.e2, 0.0);
+        double g2x = -g1x, g2y = -g1y, g2z = -g1z;
+        if (!st.at_rest && e2 > 0.0) {
+            double const p = std::sqrt(std::max(0.0, ka * (ka + 2.0 * kMeC2)));
+            g2x = (p * ax - e1 * g1x) / e2;
+            g2y = (p * ay - e1 * g1y) / e2;
+            g2z = (p * az - e1 * g1z) / e2;
+            normalize(g2x, g2y, g2z);
+        }
+        double local = 0.0;
+        if (e1 >= gcut) push_secondary(2, kProcAnnihilation, x, y, z, g1x, g1y, g1z, e1, birth);
+        else local += e1;
+        if (e2 >= gcut) push_secondary(2, kProcAnnihilation, x, y, z, g2x, g2y, g2z, e2, birth);
+        else local += e2;
+        if (at_rest && k > 0.0) deposit_local(k);
+        if (local > 0.0) deposit_local(local);
+        fx.tallies[ORC_T_POSITIVE_IONS] += w;
+        fx.tallies[ORC_T_ANNIHILATIONS] += 1.0;
+    };
+    auto annihilated_at = [&](double r, double z) {
+        segment(r_particle, z_particle, r, z);
+        out.r = pr; out.z = pz; out.th = pth; out.ux = pux; out.uy = puy; out.uz = puz;
+        out.status = ORC_ANNIHILATED;
+    };
 
-    // Initial terminal disposition :129-175
+    // Initial terminal disposition (RreaResolveChargedTerminalDisposition
+    // OV/RreaParticleInteraction.H:267-285)
     bool const below0 = k < cutoff;
     if (escape.escapes && below0) { commit_escape(); return; }
-    if (!escape.escapes && below0) { demote(r_particle, z_particle); return; }
+    if (!escape.escapes && below0) {
+        if (pos) {
+            // at rest where it stands, before any transport: a zero-length net chord
+            annihilate(x_particle, y_particle, z_particle, true, 0U, 1.0);
+            annihilated_at(r_particle, z_particle);
+        } else {
+            demote(r_particle, z_particle);
+        }
+        return;
+    }
     if (ds <= 0.0) {
         if (escape.escapes) { commit_escape(); return; }
         out.r = pr; out.z = pz; out.th = pth; out.ux = pux; out.uy = puy; out.uz = puz;
@@ -1497,16 +1711,24 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
     }
     if (div.empty()) fail(ORC_ERR_INVALID, "positive-length path without material segments");
 
-    // Substep sizing :187-279
+    // Substep sizing, OV/RreaElectronTransport.cpp:187-279 / OV/RreaPositronTransport.cpp:177-245
     double stp0[3];
-    electron_stp_rates(t, thr, gcut, k, hm_on, stp0);
+    if (pos) positron_stp_rates(t, thr, gcut, k, stp0);
+    else electron_stp_rates(t, thr, gcut, k, hm_on, stp0);
     double const ddp = chord_dp / span;
     bool violation = false;
-    uint64_t subcycles = select_material_subcycles(stp0[0] * ddp + stp0[1] * ddp, div, dpath, cfg, violation,
-                                                   stp0[0] + stp0[1]);
+    uint64_t subcycles;
+    if (pos) {
+        double const total = stp0[0] + stp0[1] + stp0[2];
+        subcycles = select_material_subcycles(total * ddp, div, dpath, cfg, violation, total);
+    } else {
+        subcycles = select_material_subcycles(stp0[0] * ddp + stp0[1] * ddp, div, dpath, cfg, violation,
+                                              stp0[0] + stp0[1]);
+    }
     {
-        double const gs_n_total = elastic_inverse_length(t, false, k) * ddp;
-        double const drag_est = realized_loss(t, std::max(k, cutoff), 1.0) * ddp;
+        double const gs_n_total = elastic_inverse_length(t, pos, k) * ddp;
+        double const drag_est = pos ? t.rate[kPCol].interpolate(std::max(k, cutoff)) * 1.0 * ddp
+                                    : realized_loss(t, std::max(k, cutoff), 1.0) * ddp;
         double const impulse_scale = std::max(cfg.max_field_impulse_fraction * k, std::max(cfg.field_impulse_floor_eV, 1.0));
         uint64_t const gsd = driver_count(std::min(gs_n_total, cfg.gs_total_collision_cap),
                                           std::max(cfg.max_gs_collisions_per_substep, 1.0));
@@ -1519,7 +1741,9 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
 
     bool exited = false;
     bool terminated = false;
-    double mid_r = r_particle, mid_z = z_particle;
+    bool annihilated = false;
+    double mid_x = x_particle, mid_y = y_particle, mid_r = r_particle, mid_z = z_particle;
+    double stop_elapsed = 1.0;
     uint64_t completed = 0;
     auto elapsed_at = [&](double f) {
         double const within = std::min(1.0, std::max(0.0, f));
@@ -1531,15 +1755,18 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
         path.point_at_full(f, x, y, z);
         double const r = std::hypot(x, y);
         if (!in_half_open(x, y, z, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi)) { exited = true; return; }
-        mid_z = z; mid_r = r; terminated = true;
+        mid_x = x; mid_y = y; mid_z = z; mid_r = r; terminated = true;
+        stop_elapsed = elapsed_at(f);
     };
 
-    // apply_electron_continuous_segment :294-400
+    // apply_electron_continuous_segment OV/RreaElectronTransport.cpp:294-400 /
+    // apply_positron_continuous_segment OV/RreaPositronTransport.cpp:278-360:
+    // collision + soft radiative stopping frozen at the segment's energy.
     struct SegLimit { double end; bool cutoff; };
     auto continuous = [&](double b, double e, double stop_e) -> SegLimit {
         if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return {b, false};
-        double const col = frozen(t.rate[kECol].interpolate(stop_e) * 1.0);
-        double const soft = frozen(t.rate[kESoft].interpolate(stop_e) * 1.0);
+        double const col = frozen(t.rate[pos ? kPCol : kECol].interpolate(stop_e) * 1.0);
+        double const soft = frozen(t.rate[pos ? kPSoft : kESoft].interpolate(stop_e) * 1.0);
         double const total = col + soft;
         // RreaLimitContinuousSegmentAtCutoff OV/RreaParticleInteraction.H:203-265
         auto loss_to = [&](double f) { return total * density_path(iv, path, b, f); };
@@ -1597,15 +1824,19 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
     FieldRange sfield;
     double substep_dt = 0.0;
     double ex_ = 0.0, ey_ = 0.0, ez_ = 0.0, er_ = 0.0;
-    (void)er_;
     uint64_t idx = 0;
     double stp[3] = {0, 0, 0};
+    int const nch = pos ? 3 : 2;
+    uint64_t const ch_depth0 = pos ? kChPositronCompetingOpticalDepth : kChCompetingOpticalDepth;
+    uint64_t const ch_pick0 = pos ? kChPositronCompetingChannel : kChCompetingChannel;
+    uint64_t const ch_depth1 = pos ? kChPositronContinuationOpticalDepth : kChContinuationOpticalDepth;
+    uint64_t const ch_pick1 = pos ? kChPositronContinuationChannel : kChContinuationChannel;
 
     // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:686-732
     for (uint64_t sc = 0; sc < subcycles; ++sc) {
         if (k < cutoff) break;
         double sb = 0.0, se = 0.0;
-        // build_substep :535-598
+        // build_substep
         {
             Chord const ch = build_chord(run_x, run_y, run_z, dx, dy, dz, k, sub_dt, dt, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
             if (!ch.valid) break;
@@ -1616,32 +1847,34 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
             for (auto const& it : iv) { tr_dp += it.length * it.ratio; tr_len += it.length; }
             sb = path.begin_f;
             se = path.end_f;
-            sfield = path_field_work(ctx.cp, path, -1.0, sb, se);
+            sfield = path_field_work(ctx.cp, path, charge, sb, se);
             substep_dt = sub_dt * (se - sb);
-            fx.tallies[ORC_T_FIELD_WORK_WEV] += w * half_kick(sfield, 0.5 * substep_dt, -1.0, k, dx, dy, dz);
+            fx.tallies[ORC_T_FIELD_WORK_WEV] += w * half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
             if (k < cutoff) { mark_stopped(sb); break; }
         }
-        // run_events :599-752 via RreaRunChargedEventInterleave :546-664
+        // run_events via RreaRunChargedEventInterleave :546-664
         {
             double rem = sb;
             uint64_t ord = 0;
             while (rem < se && k >= cutoff) {
                 if (ord >= kEventCap) fail(ORC_ERR_INVALID, "event ordinal overflow");
                 double const frozen_e = k;
-                electron_stp_rates(t, thr, gcut, frozen_e, hm_on, stp);
+                if (pos) positron_stp_rates(t, thr, gcut, frozen_e, stp);
+                else electron_stp_rates(t, thr, gcut, frozen_e, hm_on, stp);
                 double const L = density_path(iv, path, rem, se);
-                double const taus[3] = {stp[0] * L, stp[1] * L, 0.0};
-                if (!live_hazard_index(0U, sc, ord, idx)) fail(ORC_ERR_INVALID, "hazard RNG index overflow");
-                double const u1 = rng_uniform(seed, pkey, step, idx, ord == 0U ? kChCompetingOpticalDepth : kChContinuationOpticalDepth);
-                double const u2 = rng_uniform(seed, pkey, step, idx, ord == 0U ? kChCompetingChannel : kChContinuationChannel);
-                Hazard const h = select_hazard(taus, 2, u1, u2);
+                double const taus[3] = {stp[0] * L, stp[1] * L, stp[2] * L};
+                if (!live_hazard_index(species, sc, ord, idx)) fail(ORC_ERR_INVALID, "hazard RNG index overflow");
+                double const u1 = rng_uniform(seed, pkey, step, idx, ord == 0U ? ch_depth0 : ch_depth1);
+                double const u2 = rng_uniform(seed, pkey, step, idx, ord == 0U ? ch_pick0 : ch_pick1);
+                Hazard const h = select_hazard(taus, nch, u1, u2);
                 if (!h.valid) fail(ORC_ERR_INVALID, "competing-hazard selection invalid");
                 if (!h.event) {
                     SegLimit const sl = continuous(rem, se, frozen_e);
                     if (sl.cutoff) mark_stopped(sl.end);
                     break;
                 }
-                double const ef = fraction_at_density_path(iv, path, rem, se, h.sampled / (stp[0] + stp[1]));
+                double const total_stp = pos ? stp[0] + stp[1] + stp[2] : stp[0] + stp[1];
+                double const ef = fraction_at_density_path(iv, path, rem, se, h.sampled / total_stp);
                 if (!(std::isfinite(rem) && std::isfinite(ef) && std::isfinite(se) && se > rem && ef >= rem && ef <= se)) {
                     fail(ORC_ERR_INVALID, "sampled event depth outside its subcycle");
                 }
@@ -1649,17 +1882,16 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
                 er_ = std::hypot(ex_, ey_);
                 SegLimit const sl = continuous(rem, ef, frozen_e);
                 if (sl.cutoff) { mark_stopped(sl.end); break; }
-                bool const open_ev = k >= cutoff && electron_channel_open(t, thr, gcut, h.channel, k, hm_on);
+                bool const open_ev = k >= cutoff && charged_channel_open(t, pos, thr, gcut, h.channel, k, hm_on);
                 if (!open_ev) {
-                    bool const open_fr = frozen_e >= cutoff && electron_channel_open(t, thr, gcut, h.channel, frozen_e, hm_on);
+                    bool const open_fr = frozen_e >= cutoff && charged_channel_open(t, pos, thr, gcut, h.channel, frozen_e, hm_on);
                     if (!open_fr) fail(ORC_ERR_INVALID, "selected channel closed before its sampled depth");
                     rem = ef;
                     ++ord;
                     continue;
                 }
-                // final_state :719-751
                 double const efrac = elapsed_at(ef);
-                if (h.channel == 0) {
+                if (!pos && h.channel == 0) {
                     // RreaApplyHardMollerFinalState OV/RreaChargedFinalStates.H:164-225
                     double const before = k;
                     double T = t.sample_moller(before, rng_uniform(seed, pkey, step, idx, kChHardMollerEnergy));
@@ -1669,45 +1901,70 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
                                           pdx, pdy, pdz, sdx, sdy, sdz);
                     k = std::max(0.0, before - T);
                     dx = pdx; dy = pdy; dz = pdz;
-                    orc_secondary s{ip, 0, 2, ex_, ey_, ez_, sdx, sdy, sdz, w, T, efrac, pkey, lineage++};
-                    fx.secondaries.push_back(s);
+                    push_secondary(0, kProcHardMoller, ex_, ey_, ez_, sdx, sdy, sdz, T, efrac);
                     ion_event(w);
                     fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
+                } else if (pos && h.channel == 0) {
+                    // RreaApplyBhabhaFinalState OV/RreaChargedFinalStates.H:238-297: a
+                    // secondary below the production threshold discards this event only
+                    double const before = k;
+                    double const T = std::min(t.sample_bhabha(before, rng_uniform(seed, pkey, step, idx, kChPositronBhabhaEnergy)),
+                                              std::max(0.0, before));
+                    if (T > 0.0 && !(thr > 0.0 && T < thr)) {
+                        double pdx, pdy, pdz, sdx, sdy, sdz;
+                        moller_lab_directions(dx, dy, dz, before, T,
+                                              rng_uniform(seed, pkey, step, idx, kChPositronBhabhaAzimuth),
+                                              pdx, pdy, pdz, sdx, sdy, sdz);
+                        k = std::max(0.0, before - T);
+                        dx = pdx; dy = pdy; dz = pdz;
+                        push_secondary(0, kProcHardBhabha, ex_, ey_, ez_, sdx, sdy, sdz, T, efrac);
+                        fx.tallies[ORC_T_POSITIVE_IONS] += w;
+                        fx.tallies[ORC_T_BHABHA_EVENTS] += 1.0;
+                    }
                 } else if (h.channel == 1) {
                     // RreaApplyBremsFinalState OV/RreaChargedFinalStates.H:329-394
                     double const before = k;
-                    double const eu = rng_uniform(seed, pkey, step, idx, kChBremsPhotonEnergy);
-                    double const ks = t.sample_brems_energy(before, eu);
+                    double const eu = rng_uniform(seed, pkey, step, idx, pos ? kChPositronBremsPhotonEnergy : kChBremsPhotonEnergy);
+                    double const ks = pos ? t.sample_positron_brems_energy(before, eu) : t.sample_brems_energy(before, eu);
                     // RreaClassifyBremsEnergyDisposition OV/RreaParticleInteraction.H:1552-1574
                     double const pe = std::min(std::max(ks, 0.0), std::max(before, 0.0));
                     double const after = std::max(0.0, std::max(before, 0.0) - pe);
                     bool const tracked = std::isfinite(gcut) && gcut >= 0.0 && pe >= gcut;
                     double const local = tracked ? 0.0 : pe;
                     double gx, gy, gz;
-                    cone_direction(dx, dy, dz, t.sample_brems_cos(before, rng_uniform(seed, pkey, step, idx, kChBremsPhotonAngle)),
-                                   rng_uniform(seed, pkey, step, idx, kChBremsPhotonAzimuth), gx, gy, gz);
+                    cone_direction(dx, dy, dz,
+                                   t.sample_brems_cos(before, rng_uniform(seed, pkey, step, idx,
+                                                                          pos ? kChPositronBremsPhotonAngle : kChBremsPhotonAngle)),
+                                   rng_uniform(seed, pkey, step, idx, pos ? kChPositronBremsPhotonAzimuth : kChBremsPhotonAzimuth),
+                                   gx, gy, gz);
                     photon_recoil(before, pe, gx, gy, gz, dx, dy, dz);
                     k = after;
                     if (tracked) {
-                        orc_secondary s{ip, 2, 4, ex_, ey_, ez_, gx, gy, gz, w, pe, efrac, pkey, lineage++};
-                        fx.secondaries.push_back(s);
+                        push_secondary(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
                         fx.tallies[ORC_T_BREMS_PHOTONS] += 1.0;
                     } else if (local > 0.0) {
                         deposit_local(local);
                     }
                     fx.tallies[ORC_T_BREMS_EVENTS] += 1.0;
+                } else if (pos && h.channel == 2) {
+                    // in-flight annihilation, OV/RreaPositronTransport.cpp:710-745: the
+                    // parent is consumed at the event point
+                    annihilate(ex_, ey_, ez_, false, idx, efrac);
+                    segment(r_particle, z_particle, er_, ez_);
+                    annihilated = true;
+                    break;
                 }
                 if (k <= cutoff) { mark_stopped(ef); break; }
                 rem = ef;
                 ++ord;
             }
         }
-        if (terminated) break;
-        // finish_interleaved_substep :502-524
+        if (terminated || annihilated) break;
+        // finish_interleaved_substep
         {
             if (k >= cutoff) {
                 // apply_gs_elastic_scatter OV/RreaTransportStep.H:372-423
-                double const gs_n = elastic_inverse_length(t, false, k) * density_path(iv, path, sb, se);
+                double const gs_n = elastic_inverse_length(t, pos, k) * density_path(iv, path, sb, se);
                 if (gs_n > 0.0) {
                     auto U = [&](uint64_t chan, uint64_t inner) {
                         return rng_uniform(seed, pkey, step, (sc << 8) | inner, chan);
@@ -1720,18 +1977,20 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
                     if (gs_n < t.gs_few) {
                         int const cap = static_cast<int>(std::ceil(3.0 * t.gs_few));
                         if (cap >= 255) fail(ORC_ERR_INVALID, "GS few-collision cap does not pack");
-                        int const cnt = poisson_capped(gs_n, U(kChGsFewCount, 0), cap);
+                        int const cnt = poisson_capped(gs_n, U(pos ? kChPositronGsFewCount : kChGsFewCount, 0), cap);
                         for (int q = 0; q < cnt; ++q) {
-                            double const ct = t.sample_single_elastic_cos(false, k, U(kChGsFewScatter, static_cast<uint64_t>(q + 1)));
-                            deflect(ct, U(kChGsFewAzimuth, static_cast<uint64_t>(q + 1)));
+                            uint64_t const inner = static_cast<uint64_t>(q + 1);
+                            double const ct = t.sample_single_elastic_cos(
+                                pos, k, U(pos ? kChPositronGsFewScatter : kChGsFewScatter, inner));
+                            deflect(ct, U(pos ? kChPositronGsFewAzimuth : kChGsFewAzimuth, inner));
                         }
                     } else {
-                        double const ct = t.sample_gs_cos(false, k, gs_n, U(kChGsAngle, 0));
-                        deflect(ct, U(kChGsAzimuth, 0));
+                        double const ct = t.sample_gs_cos(pos, k, gs_n, U(pos ? kChPositronGsAngle : kChGsAngle, 0));
+                        deflect(ct, U(pos ? kChPositronGsAzimuth : kChGsAzimuth, 0));
                     }
                 }
             }
-            fx.tallies[ORC_T_FIELD_WORK_WEV] += w * half_kick(sfield, 0.5 * substep_dt, -1.0, k, dx, dy, dz);
+            fx.tallies[ORC_T_FIELD_WORK_WEV] += w * half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
             if (k < cutoff) { mark_stopped(se); break; }
         }
         if (!advance_running_position(path, se, run_x, run_y, run_z)) { exited = true; break; }
@@ -1740,28 +1999,48 @@ inline void electron_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, 
     }
 
     if (exited) escape = clipped_escape_policy(path, std::hypot(path.ex, path.ey), path.ez);
-    if (tr_len > 0.0) particle_ratio = tr_dp / tr_len;
-    double const end_r = std::hypot(run_x, run_y);
-    double fer, fez, fbt;
-    ctx.cp.gather(end_r, run_z, fer, fez, fbt);
-    // RreaEvaluateElectronStatePolicy OV/RreaParticleInteraction.H:1279-1317.  The
-    // ensemble drag is read only when K <= the hard-Moller threshold; it has no
-    // side effect, so evaluating it only there is exact.
-    bool const below = k < cutoff;
-    bool nonrunaway = false;
-    if (thr > 0.0 && k <= thr) {
-        double const thr_v = ensemble_drag(t, std::max(k, cutoff), particle_ratio, hm_on);
-        nonrunaway = std::hypot(fer, fez) <= thr_v;
-    }
-    // RreaResolveChargedTransportTerminal :292-312
-    if (exited) { commit_escape(); return; }
-    if (terminated || below || nonrunaway) {
-        double const dr_ = terminated ? mid_r : end_r;
-        double const dz_ = terminated ? mid_z : run_z;
-        demote(dr_, dz_);
-        return;
+    if (pos) {
+        // OV/RreaPositronTransport.cpp:764-846: a consumed parent outranks every
+        // other outcome; then escape; then a stop or a below-cutoff endpoint
+        // annihilates at rest where the positron ended up.
+        if (annihilated) {
+            out.r = pr; out.z = pz; out.th = pth; out.ux = pux; out.uy = puy; out.uz = puz;
+            out.status = ORC_ANNIHILATED;
+            return;
+        }
+        if (exited) { commit_escape(); return; }
+        if (terminated || k < cutoff) {
+            double const ax = terminated ? mid_x : run_x;
+            double const ay = terminated ? mid_y : run_y;
+            double const az = terminated ? mid_z : run_z;
+            double const ar = terminated ? mid_r : std::hypot(run_x, run_y);
+            annihilate(ax, ay, az, true, subcycles, terminated ? stop_elapsed : 1.0);
+            annihilated_at(ar, az);
+            return;
+        }
+    } else {
+        if (tr_len > 0.0) particle_ratio = tr_dp / tr_len;
+        double const end_r = std::hypot(run_x, run_y);
+        double fer, fez, fbt;
+        ctx.cp.gather(end_r, run_z, fer, fez, fbt);
+        // RreaEvaluateElectronStatePolicy OV/RreaParticleInteraction.H:1279-1317.  The
+        // ensemble drag is read only when K <= the hard-Moller threshold; it has no
+        // side effect, so evaluating it only there is exact.
+        bool const below = k < cutoff;
+        bool nonrunaway = false;
+        if (thr > 0.0 && k <= thr) {
+            double const thr_v = ensemble_drag(t, std::max(k, cutoff), particle_ratio, hm_on);
+            nonrunaway = std::hypot(fer, fez) <= thr_v;
+        }
+        // RreaResolveChargedTransportTerminal :292-312
+        if (exited) { commit_escape(); return; }
+        if (terminated || below || nonrunaway) {
+            demote(terminated ? mid_r : end_r, terminated ? mid_z : run_z);
+            return;
+        }
     }
     // CommitSurvivorChargedWriteback OV/RreaTransportStep.H:874-905
+    double const end_r = std::hypot(run_x, run_y);
     double const um = proper_velocity_from_kinetic(k);
     out.ux = um * dx; out.uy = um * dy; out.uz = um * dz;
     out.r = end_r;
@@ -1968,16 +2247,16 @@ int orc_deposit_escaping_segment(int nr, int nz, double dr, double dz, double r_
     return deposit_escaping_segment(g, q, dt, a7[0], a7[1], a7[2], a7[3], a7[4], a7[5], pnr, pzb, pnz, faces) ? 1 : 0;
 }
 
-// One electron transport step over n particles.  prev_* is the pre-push
-// snapshot the reference reads; out_* receives the written-back state.  The
-// padded net-chord grid is (nr+2) x (nz+4) with two low-z pad cells
-// (OV/RreaWarpXCoupling.cpp:417-444); jr/jz/rho are accumulated (+=).
-int orc_electron_step(void* h, const orc_geometry* g, const orc_config* cfg, int64_t n, const double* pr,
-                      const double* pz, const double* pth, const double* pux, const double* puy, const double* puz,
-                      const double* w, const int64_t* id, const int32_t* cpu, double* out_r, double* out_z,
-                      double* out_th, double* out_ux, double* out_uy, double* out_uz, int32_t* status,
-                      double* ion_pairs, int64_t* substeps, int64_t sec_cap, orc_secondary* sec, int64_t* n_sec,
-                      double* jr, double* jz, double* rho, double* tallies, int nthreads)
+// One charged transport step over n particles of one species.  prev_* is the
+// pre-push snapshot the reference reads; out_* receives the written-back
+// state.  The padded net-chord grid is (nr+2) x (nz+4) with two low-z pad
+// cells (OV/RreaWarpXCoupling.cpp:417-444); jr/jz/rho are accumulated (+=).
+static int charged_step(bool positron, void* h, const orc_geometry* g, const orc_config* cfg, int64_t n,
+                        const double* pr, const double* pz, const double* pth, const double* pux, const double* puy,
+                        const double* puz, const double* w, const int64_t* id, const int32_t* cpu, double* out_r,
+                        double* out_z, double* out_th, double* out_ux, double* out_uy, double* out_uz,
+                        int32_t* status, double* ion_pairs, int64_t* substeps, int64_t sec_cap, orc_secondary* sec,
+                        int64_t* n_sec, double* jr, double* jz, double* rho, double* tallies, int nthreads)
 {
     auto const* t = static_cast<Tables const*>(h);
     if (g->r_lo != 0.0) return ORC_ERR_INVALID;
@@ -2013,9 +2292,10 @@ int orc_electron_step(void* h, const orc_geometry* g, const orc_config* cfg, int
         int64_t const b = c * chunk, e = std::min(n, b + chunk);
         for (int64_t i = b; i < e; ++i) {
             if (error.load(std::memory_order_relaxed) != 0) break;
-            ElectronOut o{};
+            ChargedOut o{};
             if (id[i] == std::numeric_limits<int64_t>::min()) {
-                // an invalidated particle: the reference loop `continue`s (RreaElectronTransport.cpp:60-63)
+                // an invalidated particle: the reference loop `continue`s
+                // (RreaElectronTransport.cpp:60-63, RreaPositronTransport.cpp:59-62)
                 out_r[i] = pr[i]; out_z[i] = pz[i]; out_th[i] = pth[i];
                 out_ux[i] = pux[i]; out_uy[i] = puy[i]; out_uz[i] = puz[i];
                 status[i] = 5;
@@ -2024,7 +2304,7 @@ int orc_electron_step(void* h, const orc_geometry* g, const orc_config* cfg, int
                 continue;
             }
             try {
-                electron_particle(ctx, fx, i, pr[i], pz[i], pth[i], pux[i], puy[i], puz[i], w[i], id[i], cpu[i], o);
+                charged_particle(ctx, fx, positron, i, pr[i], pz[i], pth[i], pux[i], puy[i], puz[i], w[i], id[i], cpu[i], o);
             } catch (OracleError const& err) {
                 int expected = 0;
                 error.compare_exchange_strong(expected, err.code);
@@ -2069,6 +2349,51 @@ int orc_electron_step(void* h, const orc_geometry* g, const orc_config* cfg, int
     }
     *n_sec = ns;
     return 0;
+}
+
+int orc_electron_step(void* h, const orc_geometry* g, const orc_config* cfg, int64_t n, const double* pr,
+                      const double* pz, const double* pth, const double* pux, const double* puy, const double* puz,
+                      const double* w, const int64_t* id, const int32_t* cpu, double* out_r, double* out_z,
+                      double* out_th, double* out_ux, double* out_uy, double* out_uz, int32_t* status,
+                      double* ion_pairs, int64_t* substeps, int64_t sec_cap, orc_secondary* sec, int64_t* n_sec,
+                      double* jr, double* jz, double* rho, double* tallies, int nthreads)
+{
+    return charged_step(false, h, g, cfg, n, pr, pz, pth, pux, puy, puz, w, id, cpu, out_r, out_z, out_th, out_ux,
+                        out_uy, out_uz, status, ion_pairs, substeps, sec_cap, sec, n_sec, jr, jz, rho, tallies,
+                        nthreads);
+}
+
+int orc_positron_step(void* h, const orc_geometry* g, const orc_config* cfg, int64_t n, const double* pr,
+                      const double* pz, const double* pth, const double* pux, const double* puy, const double* puz,
+                      const double* w, const int64_t* id, const int32_t* cpu, double* out_r, double* out_z,
+                      double* out_th, double* out_ux, double* out_uy, double* out_uz, int32_t* status,
+                      double* ion_pairs, int64_t* substeps, int64_t sec_cap, orc_secondary* sec, int64_t* n_sec,
+                      double* jr, double* jz, double* rho, double* tallies, int nthreads)
+{
+    return charged_step(true, h, g, cfg, n, pr, pz, pth, pux, puy, puz, w, id, cpu, out_r, out_z, out_th, out_ux,
+                        out_uy, out_uz, status, ion_pairs, substeps, sec_cap, sec, n_sec, jr, jz, rho, tallies,
+                        nthreads);
+}
+
+// SamplePositronAnnihilationFinalState as a standalone sampler:
+// out = [E1, E2, cos1, azimuth_u, valid, at_rest].
+void orc_sample_annihilation(double k, double draw_energy, double draw_azimuth, double* out6)
+{
+    AnnihilationState const s = sample_annihilation(k, draw_energy, draw_azimuth);
+    out6[0] = s.e1; out6[1] = s.e2; out6[2] = s.cos1; out6[3] = s.az1;
+    out6[4] = s.valid ? 1.0 : 0.0; out6[5] = s.at_rest ? 1.0 : 0.0;
+}
+
+double orc_sample_bhabha(void* h, double e, double u, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return static_cast<Tables*>(h)->sample_bhabha(e, u));
+}
+
+double orc_sample_positron_brems_energy(void* h, double e, double u, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return static_cast<Tables*>(h)->sample_positron_brems_energy(e, u));
 }
 
 }  // extern "C"

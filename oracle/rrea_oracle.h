@@ -1,8 +1,9 @@
 /* CPU oracle for the RREA per-step charged transport -- TEST INFRASTRUCTURE.
  *
- * A plain C++17 restatement of the reference's electron transport step
- * (native_amrex_rrea/warpx_overlay/Source/Rrea/RreaElectronTransport.cpp and
- * the headers it includes).  Only tests/, __graft_entry__.smoke() and the
+ * A plain C++17 restatement of the reference's electron and positron
+ * transport steps (native_amrex_rrea/warpx_overlay/Source/Rrea/
+ * RreaElectronTransport.cpp, RreaPositronTransport.cpp and the headers they
+ * include).  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load this library; the shipped HIP path
  * never links or calls it.
  *
@@ -76,6 +77,7 @@ enum {
     ORC_ZERO_WEIGHT = 3,
     ORC_NO_WRITEBACK = 4,
     ORC_SKIPPED_INVALID = 5,
+    ORC_ANNIHILATED = 6,   /* positron consumed: in flight, stopped, or at rest */
 };
 
 /* Tally slots (sums over the step). */
@@ -90,6 +92,9 @@ enum {
     ORC_T_DRAG_LOSS_WEV,
     ORC_T_ION_PAIRS,
     ORC_T_FIELD_WORK_WEV,
+    ORC_T_ANNIHILATIONS,
+    ORC_T_BHABHA_EVENTS,
+    ORC_T_POSITIVE_IONS,   /* weight of positive-ion-only sources (Bhabha, annihilation) */
     ORC_T_COUNT = 16,
 };
 

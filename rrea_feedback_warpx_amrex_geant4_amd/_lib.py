@@ -97,7 +97,8 @@ class StepOut(C.Structure):
 
 
 TALLY_COUNT = 16
-STATUS = {0: "survive", 1: "escape", 2: "demoted", 3: "zero_weight", 4: "no_writeback", 5: "skipped_invalid"}
+STATUS = {0: "survive", 1: "escape", 2: "demoted", 3: "zero_weight", 4: "no_writeback", 5: "skipped_invalid",
+          6: "annihilated"}
 INVALID_ID = -(2**63)
 ERRORS = {0: "ok", 1: "table range", 2: "outside domain", 3: "strict guard", 4: "invalid", 5: "secondary capacity",
           16: "argument", 17: "hip"}
@@ -113,6 +114,8 @@ SIGNATURES = [
     ("rrea_density_set_constant", C.c_int32, [C.c_void_p, C.c_double]),
     ("rrea_density_set_profile", C.c_int32, [C.c_void_p, P_D, P_D, C.c_int64, C.c_double, C.c_double]),
     ("rrea_transport_step_electrons", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
+    ("rrea_transport_step_positrons", C.c_int32,
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, P_D, P_I64]),

@@ -70,13 +70,25 @@ enum : uint64_t {
     kChGsAngle = 89, kChGsAzimuth = 90, kChGsFewCount = 91, kChGsFewScatter = 92, kChGsFewAzimuth = 93,
     kChCompetingOpticalDepth = 107, kChCompetingChannel = 108,
     kChContinuationOpticalDepth = 124, kChContinuationChannel = 125,
+    kChPositronBhabhaEnergy = 74, kChPositronBhabhaAzimuth = 130,
+    kChPositronBremsPhotonEnergy = 76, kChPositronBremsPhotonAngle = 77, kChPositronBremsPhotonAzimuth = 78,
+    kChPositronAnnihilationPolar = 80, kChPositronAnnihilationAzimuth = 81,
+    kChPositronGsAngle = 94, kChPositronGsAzimuth = 95, kChPositronGsFewCount = 96, kChPositronGsFewScatter = 97,
+    kChPositronGsFewAzimuth = 98,
+    kChPositronCompetingOpticalDepth = 109, kChPositronCompetingChannel = 110,
+    kChPositronContinuationOpticalDepth = 126, kChPositronContinuationChannel = 127,
 };
+
+// RreaSecondaryProcess ids, OV/RreaParticleInteraction.H:34-48
+enum : int32_t { kProcHardMoller = 2, kProcHardBhabha = 3, kProcBrems = 4, kProcAnnihilation = 9 };
 
 constexpr uint64_t kSegmentCap = uint64_t{1} << 14;  // OV/RreaParticleInteraction.H:642-650
 constexpr uint64_t kEventCap = uint64_t{1} << 13;
 
-// RreaTryLiveHazardRngIndex OV/RreaParticleInteraction.H:664-710 (electron root, species 0)
-__device__ __forceinline__ bool live_hazard_index(uint64_t segment, uint64_t event, uint64_t& out)
+// RreaTryLiveHazardRngIndex OV/RreaParticleInteraction.H:664-710 for a live
+// root (lineage 1) of species 0 (e-) or 1 (e+)
+__device__ __forceinline__ bool live_hazard_index(uint64_t segment, uint64_t event, uint64_t& out,
+                                                  uint64_t species = 0U)
 {
     if (event == 0U) {
         if (segment >= kSegmentCap) return false;
@@ -84,7 +96,7 @@ __device__ __forceinline__ bool live_hazard_index(uint64_t segment, uint64_t eve
         return true;
     }
     if (segment >= kSegmentCap || event >= kEventCap) return false;
-    out = (uint64_t{1} << 35) | (segment << 21) | (event << 8);
+    out = (species << 49) | (uint64_t{1} << 35) | (segment << 21) | (event << 8);
     return true;
 }
 
@@ -129,9 +141,9 @@ struct DevTables {
     const double* clogy[kChargedCount];
     int32_t cmode[kChargedCount];
     const double* hm_mean;          // hard-Moller mean secondary on the grid
-    DevCdf moller, ebrems;
-    DevAngle gs_e, wz_e;
-    DevElastic el_e;
+    DevCdf moller, ebrems, bhabha, pbrems;
+    DevAngle gs_e, wz_e, gs_p, wz_p;
+    DevElastic el_e, el_p;
     double hard_moller_threshold;
     double charged_min, charged_max;
     double gs_few, gs_iso, gs_transition;
@@ -889,6 +901,114 @@ __device__ __forceinline__ Hazard select_hazard(double t0, double t1, double u1,
     h.valid = h.channel >= 0;
     h.event = h.valid;
     return h;
+}
+
+// The same selection over three channels (the positron's Bhabha, brems,
+// annihilation); the running sums are formed in the same order.
+__device__ __forceinline__ Hazard select_hazard3(double t0, double t1, double t2, double u1, double u2)
+{
+    Hazard h{false, false, -1, 0.0};
+    if (!finite(u1) || !finite(u2)) return h;
+    if (!finite(t0) || t0 < 0.0 || !finite(t1) || t1 < 0.0 || !finite(t2) || t2 < 0.0) return h;
+    double total = 0.0;
+    total += t0;
+    total += t1;
+    total += t2;
+    h.valid = finite(total);
+    if (!h.valid) return h;
+    u1 = dmax(0.0, dmin(1.0, u1));
+    h.sampled = -log(dmax(1.0e-30, 1.0 - u1));
+    if (!(h.sampled < total)) return h;
+    h.event = true;
+    u2 = dmax(0.0, dmin(1.0, u2));
+    double const target = u2 * total;
+    double cum = 0.0;
+    int last_open = -1;
+    if (t0 > 0.0) last_open = 0;
+    cum += t0;
+    if (target < cum) { h.channel = 0; return h; }
+    if (t1 > 0.0) last_open = 1;
+    cum += t1;
+    if (target < cum) { h.channel = 1; return h; }
+    if (t2 > 0.0) last_open = 2;
+    cum += t2;
+    if (target < cum) { h.channel = 2; return h; }
+    h.channel = last_open;
+    h.valid = h.channel >= 0;
+    h.event = h.valid;
+    return h;
+}
+
+// ---------------------------------------------------------------------------
+// Two-photon annihilation -- NA/src/RreaInteractionTables.cpp:43-108, 3208-3253
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double clamp_unit(double v)
+{
+    if (!finite(v)) return 0.5;
+    return dmax(0.0, dmin(1.0, v));
+}
+
+__device__ __forceinline__ double wrap_unit_interval(double v)
+{
+    if (!finite(v)) return 0.0;
+    v = v - floor(v);
+    if (v < 0.0) v += 1.0;
+    return v;
+}
+
+// Heitler photon-energy fraction: 64 bisections of the antiderivative
+// a ln e - e + 1/((g+1)^2 e) over the centre-of-mass support.
+__device__ double annihilation_energy_fraction(double gamma, double draw)
+{
+    if (gamma <= 1.0 + 1.0e-12) return 0.5;
+    double const beta_cm = sqrt(dmax(0.0, (gamma - 1.0) / (gamma + 1.0)));
+    double const eps_min = 0.5 * (1.0 - beta_cm);
+    double const eps_max = 0.5 * (1.0 + beta_cm);
+    double const u = clamp_unit(draw);
+    double const gp1 = gamma + 1.0;
+    double const a = 1.0 + 2.0 * gamma / (gp1 * gp1);
+    double const f_lo = a * log(eps_min) - eps_min + 1.0 / (gp1 * gp1 * eps_min);
+    double const f_hi = a * log(eps_max) - eps_max + 1.0 / (gp1 * gp1 * eps_max);
+    double const span = f_hi - f_lo;
+    if (!(span > 0.0)) return 0.5;
+    double const target = f_lo + u * span;
+    double lo = eps_min, hi = eps_max;
+    for (int it = 0; it < 64; ++it) {
+        double const mid = 0.5 * (lo + hi);
+        if (a * log(mid) - mid + 1.0 / (gp1 * gp1 * mid) < target) lo = mid; else hi = mid;
+    }
+    return dmax(eps_min, dmin(eps_max, 0.5 * (lo + hi)));
+}
+
+struct Annihilation { double e1, e2, cos1, az1; bool valid, at_rest; };
+
+__device__ __forceinline__ Annihilation sample_annihilation(double k, double draw_energy, double draw_azimuth)
+{
+    Annihilation s{0.0, 0.0, 1.0, 0.0, false, false};
+    k = dmax(0.0, k);
+    double const gamma = 1.0 + k / kMeC2;
+    double const total = 2.0 * kMeC2 + k;
+    if (gamma <= 1.0 + 1.0e-10) {
+        s.e1 = kMeC2;
+        s.e2 = kMeC2;
+        s.cos1 = 2.0 * clamp_unit(draw_energy) - 1.0;
+        s.az1 = wrap_unit_interval(draw_azimuth);
+        s.valid = true;
+        s.at_rest = true;
+        return s;
+    }
+    double const beta = sqrt(dmax(0.0, 1.0 - 1.0 / (gamma * gamma)));
+    double const eps = annihilation_energy_fraction(gamma, draw_energy);
+    s.e1 = eps * total;
+    s.e2 = dmax(0.0, total - s.e1);
+    if (s.e1 <= 0.0 || s.e2 <= 0.0) return s;
+    double const denom = eps * gamma * beta;
+    if (denom <= 0.0) return s;
+    s.cos1 = dmax(-1.0, dmin(1.0, (eps * (gamma + 1.0) - 1.0) / denom));
+    s.az1 = wrap_unit_interval(draw_azimuth);
+    s.valid = true;
+    s.at_rest = false;
+    return s;
 }
 
 // ---------------------------------------------------------------------------

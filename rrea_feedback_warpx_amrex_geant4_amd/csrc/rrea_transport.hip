@@ -197,7 +197,7 @@ __global__ void cell_fields_kernel(int32_t nr, int32_t nz, const double* __restr
 // Workgroup reduction of the per-particle tallies
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
-constexpr int kTallyUsed = 10;
+constexpr int kTallyUsed = 13;
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -275,10 +275,46 @@ __device__ __forceinline__ bool electron_channel_open(DevTables const& T, double
 }
 
 // elastic_inverse_length_stp OV/RreaTransportStep.H:199-208
-__device__ __forceinline__ double elastic_inverse_length(DevTables const& T, double e, int32_t& err)
+__device__ __forceinline__ double elastic_inverse_length(DevElastic const& E, double e, int32_t& err)
 {
-    double const mfp = elastic_semilog(T.el_e, T.el_e.mfp, e, err) / 1.0;
+    double const mfp = elastic_semilog(E, E.mfp, e, err) / 1.0;
     return mfp > 0.0 ? 1.0 / mfp : 0.0;
+}
+
+// mean_free_path_from_inverse_length NA/src/RreaInteractionTables.cpp:110-119,
+// then RreaChannelInverseLength OV/RreaChargedInterleave.H:69-78
+__device__ __forceinline__ double channel_inverse_length(double v)
+{
+    double const mfp = v > 0.0 ? 1.0 / v : 1.0e99;
+    return mfp > 0.0 ? 1.0 / mfp : 0.0;
+}
+
+// RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:114-133 (positron):
+// hard Bhabha (open unless K <= T_cut), tracked brems (K > photon cutoff),
+// in-flight annihilation (always open)
+__device__ __forceinline__ void positron_stp_rates(DevTables const& T, Bracket const& b, double thr, double gcut,
+                                                   double e, double& bh, double& br, double& an, int32_t& err)
+{
+    bh = 0.0;
+    br = 0.0;
+    if (!(thr > 0.0 && e <= thr)) bh = channel_inverse_length(charged_value(T, kPBhabha, b, err) * 1.0);
+    if (e > gcut) br = channel_inverse_length(charged_value(T, kPBrems, b, err) * 1.0);
+    an = channel_inverse_length(charged_value(T, kPAnnih, b, err) * 1.0);
+}
+
+// RreaChargedChannelOpenAt OV/RreaChargedFinalStates.H:56-96 (positron)
+__device__ __forceinline__ bool positron_channel_open(DevTables const& T, double thr, double gcut, int ch, double e,
+                                                      int32_t& err)
+{
+    Bracket b;
+    if (!(finite(e) && e >= T.charged_min && e <= T.charged_max) || !charged_bracket(T, e, b)) {
+        if (!err) err = kErrRange;
+        return false;
+    }
+    if (ch == 0) return !(thr > 0.0 && e <= thr) && charged_value(T, kPBhabha, b, err) * 1.0 > 0.0;
+    if (ch == 1) return e > gcut && charged_value(T, kPBrems, b, err) * 1.0 > 0.0;
+    if (ch == 2) return charged_value(T, kPAnnih, b, err) * 1.0 > 0.0;
+    return false;
 }
 
 // electron_ensemble_mean_drag_ev_per_m OV/RreaTransportStep.H:226-247
@@ -304,9 +340,85 @@ __device__ double ensemble_drag(DevTables const& T, double e, double ratio, bool
     return realized + dd + tb;
 }
 
-// The complete per-particle step (every branch of the reference loop body).
-__device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t ip, double* tally)
+// RreaSampleMollerLabDirections OV/RreaChargedInterleave.H:429-463: the
+// secondary on a cone of the two-body cosine, the primary on the opposite azimuth
+__device__ __forceinline__ void moller_lab_directions(double dx, double dy, double dz, double before, double t2,
+                                                      double u_az, double& sdx, double& sdy, double& sdz,
+                                                      double& ndx, double& ndy, double& ndz)
 {
+    double pdx = dx, pdy = dy, pdz = dz;
+    normalize(pdx, pdy, pdz);
+    double const az = unit_cycle(u_az);
+    double const kp = dmax(before, 0.0), ks = dmax(t2, 0.0);
+    double const cos_s = (kp <= 0.0 || ks <= 0.0) ? 1.0 : moller_cos(kp, ks);
+    double const sc_e = dmax(0.0, before - t2);
+    double const cos_p = (before <= 0.0 || sc_e <= 0.0) ? 1.0 : moller_cos(before, sc_e);
+    cone_direction(pdx, pdy, pdz, cos_s, az, sdx, sdy, sdz);
+    cone_direction(pdx, pdy, pdz, cos_p, unit_cycle(az + 0.5), ndx, ndy, ndz);
+}
+
+// SampleBremsstrahlungCosTheta NA/src/RreaInteractionTables.cpp:2714-2760:
+// the bounded ModifiedTsai polar angle by 60 bisections
+__device__ double brems_cos(double before, double u)
+{
+    double const cu = dmax(0.0, dmin(1.0, u));
+    if (cu == 0.0) return 1.0;
+    if (cu == 1.0) return -1.0;
+    constexpr double a = 0.625;
+    auto g2 = [](double x) {
+        if (x < 1.0e-3) {
+            return x * x * (0.5 + x * (-1.0 / 3.0 + x * (1.0 / 8.0 + x * (-1.0 / 30.0 + x * (1.0 / 144.0)))));
+        }
+        return -expm1(-x) - x * exp(-x);
+    };
+    auto mix = [&](double v) { return 0.25 * g2(a * v) + 0.75 * g2(3.0 * a * v); };
+    double const u_max = 2.0 * (1.0 + before / kMeC2);
+    double const target = cu * mix(u_max);
+    double lo = 0.0, hi = u_max;
+    for (int it = 0; it < 60; ++it) {
+        double const mid = 0.5 * (lo + hi);
+        if (mix(mid) < target) lo = mid; else hi = mid;
+    }
+    double const f = 0.5 * (lo + hi) / u_max;
+    return 1.0 - 2.0 * f * f;
+}
+
+// Per-species constants of the charged step.
+template <bool kPos> struct Species;
+template <> struct Species<false> {
+    static constexpr double kCharge = -1.0;
+    static constexpr uint64_t kId = 0;
+    static constexpr int kCol = kECol, kSoft = kESoft;
+    static constexpr uint64_t kDepth0 = kChCompetingOpticalDepth, kPick0 = kChCompetingChannel;
+    static constexpr uint64_t kDepth1 = kChContinuationOpticalDepth, kPick1 = kChContinuationChannel;
+    static constexpr uint64_t kBremsE = kChBremsPhotonEnergy, kBremsA = kChBremsPhotonAngle;
+    static constexpr uint64_t kBremsPhi = kChBremsPhotonAzimuth;
+    static constexpr uint64_t kGsAngle = kChGsAngle, kGsPhi = kChGsAzimuth, kFewCount = kChGsFewCount;
+    static constexpr uint64_t kFewScatter = kChGsFewScatter, kFewPhi = kChGsFewAzimuth;
+};
+template <> struct Species<true> {
+    static constexpr double kCharge = 1.0;
+    static constexpr uint64_t kId = 1;
+    static constexpr int kCol = kPCol, kSoft = kPSoft;
+    static constexpr uint64_t kDepth0 = kChPositronCompetingOpticalDepth, kPick0 = kChPositronCompetingChannel;
+    static constexpr uint64_t kDepth1 = kChPositronContinuationOpticalDepth;
+    static constexpr uint64_t kPick1 = kChPositronContinuationChannel;
+    static constexpr uint64_t kBremsE = kChPositronBremsPhotonEnergy, kBremsA = kChPositronBremsPhotonAngle;
+    static constexpr uint64_t kBremsPhi = kChPositronBremsPhotonAzimuth;
+    static constexpr uint64_t kGsAngle = kChPositronGsAngle, kGsPhi = kChPositronGsAzimuth;
+    static constexpr uint64_t kFewCount = kChPositronGsFewCount, kFewScatter = kChPositronGsFewScatter;
+    static constexpr uint64_t kFewPhi = kChPositronGsFewAzimuth;
+};
+
+// The complete per-particle step (every branch of the reference loop body):
+// OV/RreaElectronTransport.cpp:59-889 for electrons, OV/RreaPositronTransport.cpp:58-862
+// for positrons, with the shared prologue BeginRunningChargedParticle
+// (OV/RreaTransportStep.H:720-820) and the substep and event skeletons of
+// OV/RreaChargedInterleave.H:546-732.
+template <bool kPos>
+__device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip, double* tally)
+{
+    using S = Species<kPos>;
     int32_t err = kOk;
     {
         DevTables const& T = A.T;
@@ -319,6 +431,7 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
         bool const hm_on = cfg.disable_hard_moller == 0;
         double const dt = cfg.dt_s;
         uint64_t const step = static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0);
+        DevElastic const& EL = kPos ? T.el_p : T.el_e;
 
         int64_t const id = A.id[ip];
         if (id == RREA_INVALID_ID) {
@@ -355,6 +468,7 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
             double run_x = pr * cos(pth);
             double run_y = pr * sin(pth);
             double run_z = pz;
+            double const x_particle = run_x, y_particle = run_y;
             double const r_particle = pr, z_particle = pz;
 
             Path path;
@@ -365,18 +479,69 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
             double const ds = driver_valid ? path.in_len : 0.0;
             Interval iv = build_interval(F, path, D, A.density_ratio, err);
             double const span = dmax(path.end_f, 1.0e-6);
-            double const field_work = path_field_work(F, path, D, -1.0, 0.0, 1.0, err).work / span;
+            double const field_work = path_field_work(F, path, D, S::kCharge, 0.0, 1.0, err).work / span;
             double const chord_dp = density_path(iv, path, 0.0, path.end_f);
-            double particle_ratio = ds > 0.0 ? chord_dp / ds : positive_or(density_ratio_at_z(F, z_particle, err), A.density_ratio);
+            double particle_ratio = 0.0;
+            if constexpr (!kPos) {
+                particle_ratio = ds > 0.0 ? chord_dp / ds : positive_or(density_ratio_at_z(F, z_particle, err), A.density_ratio);
+            }
             double tr_dp = 0.0, tr_len = 0.0;
 
+            // deposit_local_energy_as_ion_pairs OV/RreaTransportStep.H:537-561
             auto deposit_local = [&](double energy) {
                 if (w <= 0.0 || energy <= 0.0) return;
                 double const pw = w * energy / dmax(kWAir, 1.0);
                 if (pw <= 0.0) return;
                 ion += pw;
             };
-            enum { kNone, kEscape, kDemote, kSurvive } terminal = kNone;
+            auto push = [&](int32_t species, int32_t process, double x, double y, double z, double sdx, double sdy,
+                            double sdz, double energy, double birth) {
+                rrea_secondary s;
+                s.parent_index = ip; s.species = species; s.process = process;
+                s.x = x; s.y = y; s.z = z;
+                s.dir_x = sdx; s.dir_y = sdy; s.dir_z = sdz;
+                s.weight = w; s.energy_eV = energy; s.birth_fraction = birth;
+                s.root_key = pkey; s.lineage = lineage++;
+                push_secondary(A, s, err);
+            };
+            // spawn_positron_annihilation OV/RreaTransportStep.H:563-700: two photons
+            // (tracked at or above the photon cutoff, else deposited locally), the
+            // residual energy of an at-rest annihilation as ion pairs, and the
+            // positive ion the annihilated target electron leaves
+            auto annihilate = [&](double x, double y, double z, bool at_rest, uint64_t index, double birth) {
+                if (w <= 0.0) return;
+                double ax = dx, ay = dy, az = dz;
+                normalize(ax, ay, az);
+                double const ka = at_rest ? 0.0 : dmax(k, 0.0);
+                if (!(ka == 0.0) && !(finite(ka) && ka >= T.charged_min && ka <= T.charged_max)) {
+                    if (!err) err = kErrRange;
+                    return;
+                }
+                Annihilation const st = sample_annihilation(ka, rng_uniform(rng, index, kChPositronAnnihilationPolar),
+                                                            rng_uniform(rng, index, kChPositronAnnihilationAzimuth));
+                double g1x, g1y, g1z;
+                cone_direction(ax, ay, az, st.cos1, st.az1, g1x, g1y, g1z);
+                double const e1 = dmax(st.e1, 0.0);
+                double const e2 = dmax(st.e2, 0.0);
+                double g2x = -g1x, g2y = -g1y, g2z = -g1z;
+                if (!st.at_rest && e2 > 0.0) {
+                    double const p = sqrt(dmax(0.0, ka * (ka + 2.0 * kMeC2)));
+                    g2x = (p * ax - e1 * g1x) / e2;
+                    g2y = (p * ay - e1 * g1y) / e2;
+                    g2z = (p * az - e1 * g1z) / e2;
+                    normalize(g2x, g2y, g2z);
+                }
+                double local = 0.0;
+                if (e1 >= gcut) push(2, kProcAnnihilation, x, y, z, g1x, g1y, g1z, e1, birth);
+                else local += e1;
+                if (e2 >= gcut) push(2, kProcAnnihilation, x, y, z, g2x, g2y, g2z, e2, birth);
+                else local += e2;
+                if (at_rest && k > 0.0) deposit_local(k);
+                if (local > 0.0) deposit_local(local);
+                tally[12] += w;
+                tally[10] += 1.0;
+            };
+            enum { kNone, kEscape, kDemote, kAnnihilated, kSurvive } terminal = kNone;
             double demote_r = r_particle, demote_z = z_particle;
 
             bool const below0 = k < cutoff;
@@ -385,26 +550,41 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
             } else if (escapes && below0) {
                 terminal = kEscape;
             } else if (!escapes && below0) {
-                terminal = kDemote;
+                if constexpr (kPos) {
+                    // at rest where it stands, before any transport: a zero-length net chord
+                    annihilate(x_particle, y_particle, z_particle, true, 0U, 1.0);
+                    terminal = kAnnihilated;
+                } else {
+                    terminal = kDemote;
+                }
             } else if (ds <= 0.0) {
                 terminal = escapes ? kEscape : kNone;
             } else if (!iv.valid) {
                 err = kErrInvalid;
             } else {
-                // Substep sizing, OV/RreaElectronTransport.cpp:187-279
+                // Substep sizing, OV/RreaElectronTransport.cpp:187-279 /
+                // OV/RreaPositronTransport.cpp:177-245
                 Bracket b0;
                 if (!charged_bracket(T, k, b0)) err = kErrRange;
-                double hm0, br0;
-                electron_stp_rates(T, b0, thr, gcut, k, hm_on, hm0, br0, err);
+                double s00, s01, s02 = 0.0;
+                if constexpr (kPos) positron_stp_rates(T, b0, thr, gcut, k, s00, s01, s02, err);
+                else electron_stp_rates(T, b0, thr, gcut, k, hm_on, s00, s01, err);
                 double const ddp = chord_dp / span;
                 bool violation = false;
                 {
                     // selected_subcycles :125-141 + selected_material_subcycles :148-184
                     double const target = cfg.max_optical_depth_per_substep;
                     uint64_t const cap = cfg.max_subcycles;
-                    double const total_stp = hm0 + br0;
-                    uint64_t const requested =
-                        static_cast<uint64_t>(dmax(1.0, ceil((hm0 * ddp + br0 * ddp) / target)));
+                    double total_stp;
+                    double tau;
+                    if constexpr (kPos) {
+                        total_stp = s00 + s01 + s02;
+                        tau = total_stp * ddp;
+                    } else {
+                        total_stp = s00 + s01;
+                        tau = s00 * ddp + s01 * ddp;
+                    }
+                    uint64_t const requested = static_cast<uint64_t>(dmax(1.0, ceil(tau / target)));
                     violation = requested > cap;
                     uint64_t n = requested < cap ? requested : cap;
                     for (;;) {
@@ -426,12 +606,18 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                     subcycles = n;
                 }
                 {
-                    double const gs_n_total = elastic_inverse_length(T, k, err) * ddp;
+                    double const gs_n_total = elastic_inverse_length(EL, k, err) * ddp;
                     double const kc = dmax(k, cutoff);
                     Bracket bc;
                     if (!charged_bracket(T, kc, bc)) err = kErrRange;
-                    double const drag_est = (kc > 0.0 ? charged_value(T, kECol, bc, err) * 1.0
-                                                          + charged_value(T, kESoft, bc, err) * 1.0 : 0.0) * ddp;
+                    double drag_est;
+                    if constexpr (kPos) {
+                        // PositronCollisionLoss only
+                        drag_est = charged_value(T, kPCol, bc, err) * 1.0 * ddp;
+                    } else {
+                        drag_est = (kc > 0.0 ? charged_value(T, kECol, bc, err) * 1.0
+                                                   + charged_value(T, kESoft, bc, err) * 1.0 : 0.0) * ddp;
+                    }
                     double const impulse_scale = dmax(cfg.max_field_impulse_fraction * k,
                                                       dmax(cfg.field_impulse_floor_eV, 1.0));
                     uint64_t const gsd = static_cast<uint64_t>(dmax(1.0, ceil(dmin(gs_n_total, cfg.gs_total_collision_cap)
@@ -445,29 +631,36 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                 if (cfg.strict_transport_guards && violation && !err) err = kErrGuard;
                 tally[6] += static_cast<double>(subcycles);
 
-                bool exited = false, terminated = false;
-                double mid_r = r_particle, mid_z = z_particle;
+                bool exited = false, terminated = false, annihilated = false;
+                double mid_x = x_particle, mid_y = y_particle, mid_r = r_particle, mid_z = z_particle;
+                double stop_elapsed = 1.0;
                 uint64_t completed = 0;
                 double const sub_dt = dt / static_cast<double>(subcycles);
                 double ex_ = 0.0, ey_ = 0.0, ez_ = 0.0;
 
+                auto elapsed_at = [&](double f) {
+                    double const within = dmin(1.0, dmax(0.0, f));
+                    return dmin(1.0, (static_cast<double>(completed) + within) / static_cast<double>(subcycles));
+                };
                 auto mark_stopped = [&](double f) {
                     if (path.exits && f >= path.end_f) { exited = true; return; }
                     double x, y, z;
                     point_at_full(path, f, x, y, z);
                     double const r = hypot(x, y);
                     if (!in_half_open(x, y, z, D)) { exited = true; return; }
-                    mid_z = z; mid_r = r; terminated = true;
+                    mid_x = x; mid_y = y; mid_z = z; mid_r = r; terminated = true;
+                    if constexpr (kPos) stop_elapsed = elapsed_at(f);
                 };
-                // apply_electron_continuous_segment :294-400 via
+                // apply_electron_continuous_segment OV/RreaElectronTransport.cpp:294-400 /
+                // apply_positron_continuous_segment OV/RreaPositronTransport.cpp:278-360 via
                 // RreaPlanContinuousDragSegment OV/RreaChargedInterleave.H:239-292
                 auto continuous = [&](double b, double e, double stop_e, double& eff) -> bool {
                     eff = b;
                     if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return false;
                     Bracket bs;
                     if (!charged_bracket(T, stop_e, bs)) { if (!err) err = kErrRange; return false; }
-                    double const col = charged_value(T, kECol, bs, err) * 1.0;
-                    double const soft = charged_value(T, kESoft, bs, err) * 1.0;
+                    double const col = charged_value(T, S::kCol, bs, err) * 1.0;
+                    double const soft = charged_value(T, S::kSoft, bs, err) * 1.0;
                     double const total = col + soft;
                     bool reaches = false;
                     double const to_cut = k - cutoff;
@@ -515,7 +708,7 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                     if (k < cutoff) break;
                     double sb = 0.0, se = 0.0;
                     {
-                        // build_substep, OV/RreaElectronTransport.cpp:535-598
+                        // build_substep
                         {
                             // an invalid chord leaves the previous microchord in place
                             Path next;
@@ -529,9 +722,9 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                         tr_len += iv.length;
                         sb = 0.0;
                         se = path.end_f;
-                        sfield = path_field_work(F, path, D, -1.0, sb, se, err);
+                        sfield = path_field_work(F, path, D, S::kCharge, sb, se, err);
                         substep_dt = sub_dt * (se - sb);
-                        tally[9] += w * half_kick(sfield, 0.5 * substep_dt, -1.0, k, dx, dy, dz);
+                        tally[9] += w * half_kick(sfield, 0.5 * substep_dt, S::kCharge, k, dx, dy, dz);
                         if (k < cutoff) { mark_stopped(sb); break; }
                     }
                     {
@@ -543,21 +736,30 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                             double const frozen_e = k;
                             Bracket bf;
                             if (!charged_bracket(T, frozen_e, bf)) { err = kErrRange; break; }
-                            double s0, s1;
-                            electron_stp_rates(T, bf, thr, gcut, frozen_e, hm_on, s0, s1, err);
+                            double s0, s1, s2 = 0.0;
+                            if constexpr (kPos) positron_stp_rates(T, bf, thr, gcut, frozen_e, s0, s1, s2, err);
+                            else electron_stp_rates(T, bf, thr, gcut, frozen_e, hm_on, s0, s1, err);
                             double const L = density_path(iv, path, rem, se);
                             uint64_t idx = 0;
-                            if (!live_hazard_index(sc, ord, idx)) { err = kErrInvalid; break; }
-                            double const u1 = rng_uniform(rng, idx, ord == 0U ? kChCompetingOpticalDepth : kChContinuationOpticalDepth);
-                            double const u2 = rng_uniform(rng, idx, ord == 0U ? kChCompetingChannel : kChContinuationChannel);
-                            Hazard const h = select_hazard(s0 * L, s1 * L, u1, u2);
+                            if (!live_hazard_index(sc, ord, idx, S::kId)) { err = kErrInvalid; break; }
+                            double const u1 = rng_uniform(rng, idx, ord == 0U ? S::kDepth0 : S::kDepth1);
+                            double const u2 = rng_uniform(rng, idx, ord == 0U ? S::kPick0 : S::kPick1);
+                            Hazard h;
+                            double total_stp;
+                            if constexpr (kPos) {
+                                h = select_hazard3(s0 * L, s1 * L, s2 * L, u1, u2);
+                                total_stp = s0 + s1 + s2;
+                            } else {
+                                h = select_hazard(s0 * L, s1 * L, u1, u2);
+                                total_stp = s0 + s1;
+                            }
                             if (!h.valid) { err = kErrInvalid; break; }
                             if (!h.event) {
                                 double eff;
                                 if (continuous(rem, se, frozen_e, eff)) mark_stopped(eff);
                                 break;
                             }
-                            double const ef = fraction_at_density_path(iv, path, rem, se, h.sampled / (s0 + s1), err);
+                            double const ef = fraction_at_density_path(iv, path, rem, se, h.sampled / total_stp, err);
                             if (!(finite(rem) && finite(ef) && finite(se) && se > rem && ef >= rem && ef <= se)) {
                                 if (!err) err = kErrInvalid;
                                 break;
@@ -567,81 +769,72 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                                 double eff;
                                 if (continuous(rem, ef, frozen_e, eff)) { mark_stopped(eff); break; }
                             }
-                            bool const open_ev = k >= cutoff && electron_channel_open(T, thr, gcut, h.channel, k, hm_on, err);
+                            bool open_ev, open_fr = false;
+                            if constexpr (kPos) {
+                                open_ev = k >= cutoff && positron_channel_open(T, thr, gcut, h.channel, k, err);
+                                if (!open_ev) {
+                                    open_fr = frozen_e >= cutoff
+                                        && positron_channel_open(T, thr, gcut, h.channel, frozen_e, err);
+                                }
+                            } else {
+                                open_ev = k >= cutoff && electron_channel_open(T, thr, gcut, h.channel, k, hm_on, err);
+                                if (!open_ev) {
+                                    open_fr = frozen_e >= cutoff
+                                        && electron_channel_open(T, thr, gcut, h.channel, frozen_e, hm_on, err);
+                                }
+                            }
                             if (!open_ev) {
-                                bool const open_fr = frozen_e >= cutoff
-                                    && electron_channel_open(T, thr, gcut, h.channel, frozen_e, hm_on, err);
                                 if (!open_fr) { if (!err) err = kErrInvalid; break; }
                                 rem = ef;
                                 ++ord;
                                 continue;
                             }
-                            double const within = dmin(1.0, dmax(0.0, ef));
-                            double const efrac = dmin(1.0, (static_cast<double>(completed) + within) / static_cast<double>(subcycles));
+                            double const efrac = elapsed_at(ef);
                             if (h.channel == 0) {
-                                // RreaApplyHardMollerFinalState OV/RreaChargedFinalStates.H:164-225
                                 double const before = k;
                                 if (!(finite(before) && before >= T.charged_min && before <= T.charged_max)) { err = kErrRange; break; }
-                                double T2 = cdf_sample(T.moller, before, rng_uniform(rng, idx, kChHardMollerEnergy), true, err);
-                                T2 = dmax(thr, dmin(T2, 0.5 * before));
-                                // RreaSampleMollerLabDirections :429-463
-                                double pdx = dx, pdy = dy, pdz = dz;
-                                normalize(pdx, pdy, pdz);
-                                double const az = unit_cycle(rng_uniform(rng, idx, kChHardMollerAzimuth));
-                                double const kp = dmax(before, 0.0), ks = dmax(T2, 0.0);
-                                double const cos_s = (kp <= 0.0 || ks <= 0.0) ? 1.0 : moller_cos(kp, ks);
-                                double const sc_e = dmax(0.0, before - T2);
-                                double const cos_p = (before <= 0.0 || sc_e <= 0.0) ? 1.0 : moller_cos(before, sc_e);
-                                double sdx, sdy, sdz, ndx, ndy, ndz;
-                                cone_direction(pdx, pdy, pdz, cos_s, az, sdx, sdy, sdz);
-                                cone_direction(pdx, pdy, pdz, cos_p, unit_cycle(az + 0.5), ndx, ndy, ndz);
-                                k = dmax(0.0, before - T2);
-                                dx = ndx; dy = ndy; dz = ndz;
-                                rrea_secondary s;
-                                s.parent_index = ip; s.species = 0; s.process = 2;
-                                s.x = ex_; s.y = ey_; s.z = ez_;
-                                s.dir_x = sdx; s.dir_y = sdy; s.dir_z = sdz;
-                                s.weight = w; s.energy_eV = T2; s.birth_fraction = efrac;
-                                s.root_key = pkey; s.lineage = lineage++;
-                                push_secondary(A, s, err);
-                                ion += w;
-                                tally[3] += 1.0;
+                                if constexpr (kPos) {
+                                    // RreaApplyBhabhaFinalState OV/RreaChargedFinalStates.H:238-297: a
+                                    // secondary below the production threshold discards this event only
+                                    double const T2 = dmin(cdf_sample(T.bhabha, before, rng_uniform(rng, idx, kChPositronBhabhaEnergy),
+                                                                      false, err),
+                                                           dmax(0.0, before));
+                                    if (T2 > 0.0 && !(thr > 0.0 && T2 < thr)) {
+                                        double sdx, sdy, sdz, ndx, ndy, ndz;
+                                        moller_lab_directions(dx, dy, dz, before, T2,
+                                                              rng_uniform(rng, idx, kChPositronBhabhaAzimuth),
+                                                              sdx, sdy, sdz, ndx, ndy, ndz);
+                                        k = dmax(0.0, before - T2);
+                                        dx = ndx; dy = ndy; dz = ndz;
+                                        push(0, kProcHardBhabha, ex_, ey_, ez_, sdx, sdy, sdz, T2, efrac);
+                                        tally[12] += w;
+                                        tally[11] += 1.0;
+                                    }
+                                } else {
+                                    // RreaApplyHardMollerFinalState OV/RreaChargedFinalStates.H:164-225
+                                    double T2 = cdf_sample(T.moller, before, rng_uniform(rng, idx, kChHardMollerEnergy), true, err);
+                                    T2 = dmax(thr, dmin(T2, 0.5 * before));
+                                    double sdx, sdy, sdz, ndx, ndy, ndz;
+                                    moller_lab_directions(dx, dy, dz, before, T2, rng_uniform(rng, idx, kChHardMollerAzimuth),
+                                                          sdx, sdy, sdz, ndx, ndy, ndz);
+                                    k = dmax(0.0, before - T2);
+                                    dx = ndx; dy = ndy; dz = ndz;
+                                    push(0, kProcHardMoller, ex_, ey_, ez_, sdx, sdy, sdz, T2, efrac);
+                                    ion += w;
+                                    tally[3] += 1.0;
+                                }
                             } else if (h.channel == 1) {
                                 // RreaApplyBremsFinalState OV/RreaChargedFinalStates.H:329-394
                                 double const before = k;
                                 if (!(finite(before) && before >= T.charged_min && before <= T.charged_max)) { err = kErrRange; break; }
-                                double const ks = cdf_sample(T.ebrems, before, rng_uniform(rng, idx, kChBremsPhotonEnergy), false, err);
+                                double const ks = cdf_sample(kPos ? T.pbrems : T.ebrems, before,
+                                                             rng_uniform(rng, idx, S::kBremsE), false, err);
                                 double const pe = dmin(dmax(ks, 0.0), dmax(before, 0.0));
                                 double const after = dmax(0.0, dmax(before, 0.0) - pe);
                                 bool const tracked = finite(gcut) && gcut >= 0.0 && pe >= gcut;
-                                // SampleBremsstrahlungCosTheta :2714-2760
-                                double ct;
-                                {
-                                    double cu = dmax(0.0, dmin(1.0, rng_uniform(rng, idx, kChBremsPhotonAngle)));
-                                    if (cu == 0.0) ct = 1.0;
-                                    else if (cu == 1.0) ct = -1.0;
-                                    else {
-                                        constexpr double a = 0.625;
-                                        auto g2 = [](double x) {
-                                            if (x < 1.0e-3) {
-                                                return x * x * (0.5 + x * (-1.0 / 3.0 + x * (1.0 / 8.0 + x * (-1.0 / 30.0 + x * (1.0 / 144.0)))));
-                                            }
-                                            return -expm1(-x) - x * exp(-x);
-                                        };
-                                        auto mix = [&](double u) { return 0.25 * g2(a * u) + 0.75 * g2(3.0 * a * u); };
-                                        double const u_max = 2.0 * (1.0 + before / kMeC2);
-                                        double const target = cu * mix(u_max);
-                                        double lo = 0.0, hi = u_max;
-                                        for (int it = 0; it < 60; ++it) {
-                                            double const mid = 0.5 * (lo + hi);
-                                            if (mix(mid) < target) lo = mid; else hi = mid;
-                                        }
-                                        double const f = 0.5 * (lo + hi) / u_max;
-                                        ct = 1.0 - 2.0 * f * f;
-                                    }
-                                }
+                                double const ct = brems_cos(before, rng_uniform(rng, idx, S::kBremsA));
                                 double gx, gy, gz;
-                                cone_direction(dx, dy, dz, ct, rng_uniform(rng, idx, kChBremsPhotonAzimuth), gx, gy, gz);
+                                cone_direction(dx, dy, dz, ct, rng_uniform(rng, idx, S::kBremsPhi), gx, gy, gz);
                                 // RreaApplyPhotonRecoilToChargedDirection :465-496
                                 {
                                     double const eg = dmax(pe, 0.0);
@@ -659,63 +852,67 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                                 }
                                 k = after;
                                 if (tracked) {
-                                    rrea_secondary s;
-                                    s.parent_index = ip; s.species = 2; s.process = 4;
-                                    s.x = ex_; s.y = ey_; s.z = ez_;
-                                    s.dir_x = gx; s.dir_y = gy; s.dir_z = gz;
-                                    s.weight = w; s.energy_eV = pe; s.birth_fraction = efrac;
-                                    s.root_key = pkey; s.lineage = lineage++;
-                                    push_secondary(A, s, err);
+                                    push(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
                                     tally[5] += 1.0;
                                 } else if (pe > 0.0) {
                                     deposit_local(pe);
                                 }
                                 tally[4] += 1.0;
+                            } else if (kPos && h.channel == 2) {
+                                // in-flight annihilation, OV/RreaPositronTransport.cpp:710-745:
+                                // the parent is consumed at the event point
+                                annihilate(ex_, ey_, ez_, false, idx, efrac);
+                                demote_r = hypot(ex_, ey_);
+                                demote_z = ez_;
+                                annihilated = true;
+                                break;
                             }
                             if (k <= cutoff) { mark_stopped(ef); break; }
                             rem = ef;
                             ++ord;
                         }
                     }
-                    if (terminated || err) break;
+                    if (terminated || annihilated || err) break;
                     {
-                        // finish_interleaved_substep, OV/RreaElectronTransport.cpp:502-524
+                        // finish_interleaved_substep
                         if (k >= cutoff) {
                             // apply_gs_elastic_scatter OV/RreaTransportStep.H:372-423
-                            double const gs_n = elastic_inverse_length(T, k, err) * density_path(iv, path, sb, se);
+                            double const gs_n = elastic_inverse_length(EL, k, err) * density_path(iv, path, sb, se);
                             if (gs_n > 0.0) {
                                 if (gs_n < T.gs_few) {
                                     int const cap = static_cast<int>(ceil(3.0 * T.gs_few));
                                     if (cap >= 255) { err = kErrInvalid; break; }
-                                    int const cnt = poisson_capped(gs_n, rng_uniform(rng, (sc << 8) | 0U, kChGsFewCount), cap);
+                                    int const cnt = poisson_capped(gs_n, rng_uniform(rng, (sc << 8) | 0U, S::kFewCount), cap);
                                     for (int q = 0; q < cnt; ++q) {
                                         uint64_t const inner = static_cast<uint64_t>(q + 1);
                                         // SampleSingleElasticCosTheta :2546-2560
                                         if (!(finite(k) && k >= T.charged_min && k <= T.charged_max)) { err = kErrRange; break; }
-                                        double const eta = elastic_semilog(T.el_e, T.el_e.eta, k, err);
-                                        double const u = dmin(dmax(rng_uniform(rng, (sc << 8) | inner, kChGsFewScatter), 0.0), 1.0 - 1.0e-15);
+                                        double const eta = elastic_semilog(EL, EL.eta, k, err);
+                                        double const u = dmin(dmax(rng_uniform(rng, (sc << 8) | inner, S::kFewScatter), 0.0),
+                                                              1.0 - 1.0e-15);
                                         double const ct = 1.0 - 2.0 * eta * u / (1.0 + eta - u);
                                         double nx, ny, nz;
-                                        cone_direction(dx, dy, dz, ct, rng_uniform(rng, (sc << 8) | inner, kChGsFewAzimuth), nx, ny, nz);
+                                        cone_direction(dx, dy, dz, ct, rng_uniform(rng, (sc << 8) | inner, S::kFewPhi), nx, ny, nz);
                                         dx = nx; dy = ny; dz = nz;
                                     }
                                 } else {
                                     // SampleGsCosTheta :2504-2544
-                                    double const u = rng_uniform(rng, (sc << 8) | 0U, kChGsAngle);
+                                    double const u = rng_uniform(rng, (sc << 8) | 0U, S::kGsAngle);
                                     double ct;
                                     if (!(finite(k) && k >= T.charged_min && k <= T.charged_max)) { err = kErrRange; break; }
-                                    DevAngle const& A2 = k <= T.gs_transition ? T.gs_e : T.wz_e;
+                                    DevAngle const& A2 = k <= T.gs_transition ? (kPos ? T.gs_p : T.gs_e)
+                                                                               : (kPos ? T.wz_p : T.wz_e);
                                     if (!(k >= A2.e[0] && k <= A2.e[A2.n_e - 1])) { err = kErrRange; break; }
                                     if (gs_n >= T.gs_iso) ct = 2.0 * u - 1.0;
                                     else ct = sample_gs_angle_cos(A2, k, gs_n, u, err);
                                     double nx, ny, nz;
-                                    cone_direction(dx, dy, dz, ct, rng_uniform(rng, (sc << 8) | 0U, kChGsAzimuth), nx, ny, nz);
+                                    cone_direction(dx, dy, dz, ct, rng_uniform(rng, (sc << 8) | 0U, S::kGsPhi), nx, ny, nz);
                                     dx = nx; dy = ny; dz = nz;
                                 }
                             }
                         }
                         if (err) break;
-                        tally[9] += w * half_kick(sfield, 0.5 * substep_dt, -1.0, k, dx, dy, dz);
+                        tally[9] += w * half_kick(sfield, 0.5 * substep_dt, S::kCharge, k, dx, dy, dz);
                         if (k < cutoff) { mark_stopped(se); break; }
                     }
                     {
@@ -731,26 +928,53 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
 
                 if (!err) {
                     if (exited) escapes = clipped_escapes(path, D);
-                    if (tr_len > 0.0) particle_ratio = tr_dp / tr_len;
                     double const end_r = hypot(run_x, run_y);
                     bool const below = k < cutoff;
-                    bool nonrunaway = false;
-                    if (!exited && !terminated && !below && thr > 0.0 && k <= thr) {
-                        // RreaEvaluateElectronStatePolicy OV/RreaParticleInteraction.H:1279-1317; the
-                        // ensemble drag has no side effect, so it is read only where the gate uses it.
-                        double fer, fez, fbt;
-                        gather(F, end_r, run_z, fer, fez, fbt);
-                        double const thr_v = ensemble_drag(T, dmax(k, cutoff), particle_ratio, hm_on, err);
-                        nonrunaway = hypot(fer, fez) <= thr_v;
-                    }
-                    // RreaResolveChargedTransportTerminal :292-312
-                    if (exited) {
-                        terminal = kEscape;
-                    } else if (terminated || below || nonrunaway) {
-                        terminal = kDemote;
-                        demote_r = terminated ? mid_r : end_r;
-                        demote_z = terminated ? mid_z : run_z;
+                    bool resolved = false;
+                    if constexpr (kPos) {
+                        // OV/RreaPositronTransport.cpp:764-846: a consumed parent outranks
+                        // every other outcome; then escape; then a stop or a below-cutoff
+                        // endpoint annihilates at rest where the positron ended up
+                        resolved = true;
+                        if (annihilated) {
+                            terminal = kAnnihilated;
+                        } else if (exited) {
+                            terminal = kEscape;
+                        } else if (terminated || below) {
+                            double const ax = terminated ? mid_x : run_x;
+                            double const ay = terminated ? mid_y : run_y;
+                            double const az = terminated ? mid_z : run_z;
+                            annihilate(ax, ay, az, true, subcycles, terminated ? stop_elapsed : 1.0);
+                            demote_r = terminated ? mid_r : end_r;
+                            demote_z = az;
+                            terminal = kAnnihilated;
+                        } else {
+                            resolved = false;
+                        }
                     } else {
+                        if (tr_len > 0.0) particle_ratio = tr_dp / tr_len;
+                        bool nonrunaway = false;
+                        if (!exited && !terminated && !below && thr > 0.0 && k <= thr) {
+                            // RreaEvaluateElectronStatePolicy OV/RreaParticleInteraction.H:1279-1317; the
+                            // ensemble drag has no side effect, so it is read only where the gate uses it.
+                            double fer, fez, fbt;
+                            gather(F, end_r, run_z, fer, fez, fbt);
+                            double const thr_v = ensemble_drag(T, dmax(k, cutoff), particle_ratio, hm_on, err);
+                            nonrunaway = hypot(fer, fez) <= thr_v;
+                        }
+                        // RreaResolveChargedTransportTerminal :292-312
+                        resolved = true;
+                        if (exited) {
+                            terminal = kEscape;
+                        } else if (terminated || below || nonrunaway) {
+                            terminal = kDemote;
+                            demote_r = terminated ? mid_r : end_r;
+                            demote_z = terminated ? mid_z : run_z;
+                        } else {
+                            resolved = false;
+                        }
+                    }
+                    if (!resolved) {
                         // CommitSurvivorChargedWriteback OV/RreaTransportStep.H:874-905
                         double const um = proper_velocity(k);
                         o_ux = um * dx; o_uy = um * dy; o_uz = um * dz;
@@ -758,7 +982,9 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                         o_th = atan2(run_y, run_x);
                         o_z = run_z;
                         terminal = kSurvive;
-                        if (w != 0.0 && !deposit_segment(A.G, -w, dt, r_particle, z_particle, end_r, run_z)) err = kErrInvalid;
+                        if (w != 0.0 && !deposit_segment(A.G, S::kCharge * w, dt, r_particle, z_particle, end_r, run_z)) {
+                            err = kErrInvalid;
+                        }
                     }
                 }
             }
@@ -768,10 +994,12 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                     double const um = proper_velocity(k);
                     o_ux = um * dx; o_uy = um * dy; o_uz = um * dz;
                     Crossing const bc = boundary_crossing(path, D);
+                    double const q = S::kCharge * w;
                     if (!bc.valid) {
                         err = kErrInvalid;
-                    } else if (-w != 0.0 && !deposit_escaping_segment(A.G, -w, dt, r_particle, z_particle, bc.start_r, bc.start_z,
-                                                                       bc.cross_r, bc.cross_z, A.pnr, A.pzb, A.pnz, bc.faces)) {
+                    } else if (q != 0.0 && !deposit_escaping_segment(A.G, q, dt, r_particle, z_particle, bc.start_r,
+                                                                     bc.start_z, bc.cross_r, bc.cross_z, A.pnr, A.pzb,
+                                                                     A.pnz, bc.faces)) {
                         err = kErrInvalid;
                     }
                     status = RREA_ESCAPE;
@@ -781,6 +1009,10 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                     if (w != 0.0 && !deposit_segment(A.G, -w, dt, r_particle, z_particle, demote_r, demote_z)) err = kErrInvalid;
                     status = RREA_DEMOTED;
                     tally[2] += 1.0;
+                } else if (terminal == kAnnihilated) {
+                    // net chord from the captured start to the annihilation point
+                    if (w != 0.0 && !deposit_segment(A.G, w, dt, r_particle, z_particle, demote_r, demote_z)) err = kErrInvalid;
+                    status = RREA_ANNIHILATED;
                 } else if (terminal == kSurvive) {
                     status = RREA_SURVIVE;
                     tally[0] += 1.0;
@@ -794,7 +1026,8 @@ __device__ __noinline__ void electron_full_particle(StepArgs const& A, int64_t i
                 A.r[ip] = o_r; A.z[ip] = o_z; A.th[ip] = o_th;
                 A.ux[ip] = o_ux; A.uy[ip] = o_uy; A.uz[ip] = o_uz;
             }
-            if (status == RREA_ESCAPE || status == RREA_DEMOTED || status == RREA_ZERO_WEIGHT) {
+            if (status == RREA_ESCAPE || status == RREA_DEMOTED || status == RREA_ZERO_WEIGHT
+                || status == RREA_ANNIHILATED) {
                 A.id[ip] = RREA_INVALID_ID;  // particle_id.make_invalid()
             }
             if (A.status) A.status[ip] = static_cast<uint8_t>(status);
@@ -835,7 +1068,7 @@ __global__ void __launch_bounds__(kBlock) electron_slow_kernel(const StepArgs* _
     int64_t const n = *count;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kBlock) {
-        electron_full_particle(A, list[i], tally);
+        charged_full_particle<false>(A, list[i], tally);
     }
     reduce_tallies(tally, A.tallies);
 }
@@ -1034,7 +1267,19 @@ __global__ void __launch_bounds__(kBlock) electron_full_kernel(const StepArgs* _
     double tally[kTallyUsed];
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (ip < A.n) electron_full_particle(A, ip, tally);
+    if (ip < A.n) charged_full_particle<false>(A, ip, tally);
+    reduce_tallies(tally, A.tallies);
+}
+
+// The positron step: every particle through the complete path (positrons are
+// a few per cent of the charged population).
+__global__ void __launch_bounds__(kBlock) positron_full_kernel(const StepArgs* __restrict__ args)
+{
+    StepArgs const& A = *args;
+    double tally[kTallyUsed];
+    for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
+    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (ip < A.n) charged_full_particle<true>(A, ip, tally);
     reduce_tallies(tally, A.tallies);
 }
 
@@ -1123,7 +1368,8 @@ rrea_status rrea_tables_create(const rrea_table_image* im, rrea_tables_t* out)
         o.n = static_cast<int32_t>(h.primary.size());
         return o;
     };
-    CdfOff const om = add_cdf(moller), ob = add_cdf(ebrems);
+    HostCdf const bhabha = host_cdf(im->bhabha), pbrems = host_cdf(im->positron_brems);
+    CdfOff const om = add_cdf(moller), ob = add_cdf(ebrems), obh = add_cdf(bhabha), opb = add_cdf(pbrems);
     struct AngOff { size_t e, le, n, ln, u, lur, lomc; int32_t ne, nn, nu; };
     auto add_angle = [&](rrea_angle_table const& a) {
         AngOff o;
@@ -1139,12 +1385,19 @@ rrea_status rrea_tables_create(const rrea_table_image* im, rrea_tables_t* out)
         return o;
     };
     AngOff const og = add_angle(im->gs_electron), ow = add_angle(im->wentzel_electron);
-    std::vector<double> ele(im->elastic_electron.e, im->elastic_electron.e + im->elastic_electron.n);
-    std::vector<double> ellog(ele.size());
-    for (size_t i = 0; i < ele.size(); ++i) ellog[i] = std::log(ele[i]);
-    size_t const o_ele = blob.add(ele), o_ellog = blob.add(ellog);
-    size_t const o_elmfp = blob.add(std::vector<double>(im->elastic_electron.mfp_m_stp, im->elastic_electron.mfp_m_stp + ele.size()));
-    size_t const o_eleta = blob.add(std::vector<double>(im->elastic_electron.screening_eta, im->elastic_electron.screening_eta + ele.size()));
+    AngOff const ogp = add_angle(im->gs_positron), owp = add_angle(im->wentzel_positron);
+    struct ElOff { size_t e, le, mfp, eta; int32_t n; };
+    auto add_elastic = [&](rrea_elastic_table const& el) {
+        ElOff o;
+        std::vector<double> e(el.e, el.e + el.n), le(el.n);
+        for (int64_t i = 0; i < el.n; ++i) le[i] = std::log(e[i]);
+        o.e = blob.add(e); o.le = blob.add(le);
+        o.mfp = blob.add(std::vector<double>(el.mfp_m_stp, el.mfp_m_stp + el.n));
+        o.eta = blob.add(std::vector<double>(el.screening_eta, el.screening_eta + el.n));
+        o.n = static_cast<int32_t>(el.n);
+        return o;
+    };
+    ElOff const oel = add_elastic(im->elastic_electron), oelp = add_elastic(im->elastic_positron);
 
     HIP_TRY(hipMalloc(&T->d_f64, blob.f64.size() * sizeof(double)));
     HIP_TRY(hipMemcpy(T->d_f64, blob.f64.data(), blob.f64.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -1162,14 +1415,21 @@ rrea_status rrea_tables_create(const rrea_table_image* im, rrea_tables_t* out)
     };
     set_cdf(V.moller, om);
     set_cdf(V.ebrems, ob);
+    set_cdf(V.bhabha, obh);
+    set_cdf(V.pbrems, opb);
     auto set_angle = [&](DevAngle& d, AngOff const& o) {
         d.e = f + o.e; d.log_e = f + o.le; d.n = f + o.n; d.log_n = f + o.ln; d.u = f + o.u;
         d.log_u_ratio = f + o.lur; d.log_omc = f + o.lomc; d.n_e = o.ne; d.n_n = o.nn; d.n_u = o.nu;
     };
     set_angle(V.gs_e, og);
     set_angle(V.wz_e, ow);
-    V.el_e.e = f + o_ele; V.el_e.log_e = f + o_ellog; V.el_e.mfp = f + o_elmfp; V.el_e.eta = f + o_eleta;
-    V.el_e.n = static_cast<int32_t>(ele.size());
+    set_angle(V.gs_p, ogp);
+    set_angle(V.wz_p, owp);
+    auto set_elastic = [&](DevElastic& d, ElOff const& o) {
+        d.e = f + o.e; d.log_e = f + o.le; d.mfp = f + o.mfp; d.eta = f + o.eta; d.n = o.n;
+    };
+    set_elastic(V.el_e, oel);
+    set_elastic(V.el_p, oelp);
     V.hard_moller_threshold = im->hard_moller_secondary_threshold_eV;
     V.charged_min = im->charged_energy_range_eV[0];
     V.charged_max = im->charged_energy_range_eV[1];
@@ -1317,8 +1577,14 @@ rrea_status rrea_density_set_profile(rrea_fields_t F, const double* alt, const d
     return RREA_OK;
 }
 
-rrea_status rrea_transport_step_electrons(rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
-                                          const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
+}  // extern "C"
+
+namespace {
+
+// One charged step (either species): argument checks, the argument block,
+// the launch, and the deterministic secondary ordering.
+rrea_status charged_step(bool positron, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+                         const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
 {
     if (!T || !F || !tile || !cfg || !out || !F->fields_set) return RREA_ERR_ARGUMENT;
     if (!out->secondaries || !out->secondary_count || !out->jr || !out->jz || !out->rho || !out->tallies
@@ -1441,7 +1707,10 @@ rrea_status rrea_transport_step_electrons(rrea_tables_t T, rrea_fields_t F, cons
         ++F->events_used;
     }
     const StepArgs* const d_args = static_cast<const StepArgs*>(F->d_args);
-    if (cfg->complete_path_only) {
+    if (positron) {
+        positron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
+        HIP_TRY(hipGetLastError());
+    } else if (cfg->complete_path_only) {
         electron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
         HIP_TRY(hipGetLastError());
     } else {
@@ -1486,6 +1755,22 @@ rrea_status rrea_transport_step_electrons(rrea_tables_t T, rrea_fields_t F, cons
         HIP_TRY(hipStreamSynchronize(s));
     }
     return RREA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rrea_status rrea_transport_step_electrons(rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+                                          const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
+{
+    return charged_step(false, T, F, tile, cfg, out, stream);
+}
+
+rrea_status rrea_transport_step_positrons(rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+                                          const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
+{
+    return charged_step(true, T, F, tile, cfg, out, stream);
 }
 
 }  // extern "C"

@@ -264,8 +264,25 @@ class RreaParticleInteraction:
     def apply_electrons(tables: InteractionTables, coupling: Coupling, prev: DeviceTile, cfg: StepConfig,
                         out: StepOutputs, current: DeviceTile | None = None, order_secondaries: bool = True,
                         stream: int | None = None, check: bool = True) -> None:
-        """One electron step.  ``prev`` is the pre-push snapshot; the written-back
-        state goes to ``current`` (default: in place over ``prev``)."""
+        """One electron step (TransportStep::ApplyElectronTransport).  ``prev`` is
+        the pre-push snapshot; the written-back state goes to ``current``
+        (default: in place over ``prev``)."""
+        RreaParticleInteraction._apply("rrea_transport_step_electrons", tables, coupling, prev, cfg, out, current,
+                                       order_secondaries, stream, check)
+
+    @staticmethod
+    def apply_positrons(tables: InteractionTables, coupling: Coupling, prev: DeviceTile, cfg: StepConfig,
+                        out: StepOutputs, current: DeviceTile | None = None, order_secondaries: bool = True,
+                        stream: int | None = None, check: bool = True) -> None:
+        """One positron step (TransportStep::ApplyPositronTransport), same
+        arguments as ``apply_electrons``."""
+        RreaParticleInteraction._apply("rrea_transport_step_positrons", tables, coupling, prev, cfg, out, current,
+                                       order_secondaries, stream, check)
+
+    @staticmethod
+    def _apply(entry: str, tables: InteractionTables, coupling: Coupling, prev: DeviceTile, cfg: StepConfig,
+               out: StepOutputs, current: DeviceTile | None, order_secondaries: bool, stream: int | None,
+               check: bool) -> None:
         cur = prev if current is None else current
         n = len(prev)
         if n and not all(t.is_contiguous() and t.numel() == n for t in (
@@ -282,7 +299,7 @@ class RreaParticleInteraction:
         o = out.struct(order_secondaries)
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
-        _lib.check(_lib.lib().rrea_transport_step_electrons(tables.h, coupling.h, C.byref(tile), C.byref(c),
-                                                             C.byref(o), stream), "rrea_transport_step_electrons")
+        _lib.check(getattr(_lib.lib(), entry)(tables.h, coupling.h, C.byref(tile), C.byref(c), C.byref(o), stream),
+                   entry)
         if check:
             out.check_error()

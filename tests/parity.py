@@ -46,12 +46,13 @@ def close(a, b, scale):
     return np.abs(a - b) <= tol
 
 
-def gpu_step(tables, coupling, tile_dev, cfg, out):
+def gpu_step(tables, coupling, tile_dev, cfg, out, species="electron"):
     from rrea_feedback_warpx_amrex_geant4_amd.transport import RreaParticleInteraction
     import torch
 
     out.reset()
-    RreaParticleInteraction.apply_electrons(tables, coupling, tile_dev, cfg, out)  # fused in-place mode
+    apply = RreaParticleInteraction.apply_positrons if species == "positron" else RreaParticleInteraction.apply_electrons
+    apply(tables, coupling, tile_dev, cfg, out)  # fused in-place mode
     torch.cuda.synchronize()
 
 
@@ -96,7 +97,7 @@ def compare_step(fields, tile_host_prev, oracle_out, tile_dev, out, label="", dt
         assert idx.size == 0, (f"{label} {name} differs at {idx[:5]}: oracle {b[idx[:5]]} gpu {a[idx[:5]]} "
                                f"status {status_o[idx[:5]]}")
     ids = tile_dev.id.cpu().numpy()
-    invalidated = np.isin(status_o, (1, 2, 3))
+    invalidated = np.isin(status_o, (1, 2, 3, 6))
     assert np.all(ids[invalidated] == np.iinfo(np.int64).min), f"{label} invalidated ids not marked"
     assert np.all(ids[~invalidated] == tile_host_prev.id[~invalidated]), f"{label} live ids changed"
     ion_g = out.ion_pairs.cpu().numpy()
@@ -135,8 +136,8 @@ def compare_step(fields, tile_host_prev, oracle_out, tile_dev, out, label="", dt
         scale = max(np.abs(o).max(), 1e-300)
         tol = 1e-11 * scale + (64.0 * EPS * coord * wmax / dt if k != "rho" else 64.0 * EPS * coord * wmax)
         assert np.abs(g - o).max() <= tol, f"{label} {k} differs: {np.abs(g - o).max()} of {scale} (tol {tol})"
-    t_g = out.tallies.cpu().numpy()[:10]
-    t_o = oracle_out["tallies"][:10]
+    t_g = out.tallies.cpu().numpy()[:13]
+    t_o = oracle_out["tallies"][:13]
     assert np.allclose(t_g, t_o, rtol=1e-10, atol=1e-300), f"{label} tallies {t_g} vs {t_o}"
 
 
@@ -147,7 +148,8 @@ def host_tile_from_dev(tile_dev):
                           ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")))
 
 
-def run_and_compare(otables, tables, coupling, fields, tile_host, cfg, steps=1, label="", nthreads=8, stats=None):
+def run_and_compare(otables, tables, coupling, fields, tile_host, cfg, steps=1, label="", nthreads=8, stats=None,
+                    species="electron"):
     """Advance `steps` steps on both paths from the same start, comparing every step."""
     stats = {} if stats is None else stats
     from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile, StepOutputs
@@ -157,9 +159,10 @@ def run_and_compare(otables, tables, coupling, fields, tile_host, cfg, steps=1, 
     host = tile_host
     for s in range(steps):
         cfg.step = s
-        o = ob.electron_step(otables, fields, cfg, host.as_dict(), nthreads=nthreads)
+        step = ob.positron_step if species == "positron" else ob.electron_step
+        o = step(otables, fields, cfg, host.as_dict(), nthreads=nthreads)
         assert o["code"] == 0, f"oracle error {o['code']}"
-        gpu_step(tables, coupling, tile_dev, cfg, out)
+        gpu_step(tables, coupling, tile_dev, cfg, out, species)
         compare_step(fields, host, o, tile_dev, out, label=f"{label} step {s}", dt=cfg.dt_s, stats=stats)
         # continue both from the oracle's state, keeping survivors only
         keep = o["status"] == 0

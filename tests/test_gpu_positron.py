@@ -1,0 +1,151 @@
+"""GPU parity of the HIP positron step against the CPU oracle.
+
+The same bar as the electron step: status and substep counts exact, the
+continuous state to 1e-12 of |u| (with the reference's chord round-trip
+conditioning), secondaries -- hard-Bhabha electrons, brems photons and the
+two annihilation photons -- exact in count, order, species, process and
+weight with payloads to 1e-12, the net-chord grid to atomic-order round-off.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from parity import run_and_compare  # noqa: E402
+from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl  # noqa: E402
+from rrea_feedback_warpx_amrex_geant4_amd.config import ParticleTile, StepConfig  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tables(bundle):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import InteractionTables
+    return InteractionTables(bundle)
+
+
+def coupling_for(fields, tables):
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import Coupling
+    return Coupling(fields, tables)
+
+
+def positrons(n, fields, seed):
+    return wl.c2_electrons(n, fields, seed=seed, charge_sign=1.0)
+
+
+def test_c2_positron_sample_parity(tables, otables):
+    fields = wl.c2_fields()
+    tile = positrons(1 << 14, fields, seed=31)
+    cfg = StepConfig(rng_seed=91, dt_s=1e-12)
+    print("case c2 2^14 e+ x 3 steps", flush=True)
+    run_and_compare(otables, tables, coupling_for(fields, tables), fields, tile, cfg, steps=3, label="e+ c2",
+                    species="positron")
+
+
+def test_positron_long_step_events(tables, otables):
+    """dt = 1 ns: substeps, hard Bhabha, brems, in-flight annihilation, stops
+    annihilating at rest mid-chord, GS multiple scattering."""
+    fields = wl.c2_fields()
+    tile = positrons(1 << 13, fields, seed=32)
+    cfg = StepConfig(rng_seed=92, dt_s=1e-9)
+    print("case c2 2^13 e+ dt=1ns x 2 steps", flush=True)
+    run_and_compare(otables, tables, coupling_for(fields, tables), fields, tile, cfg, steps=2, label="e+ long",
+                    species="positron")
+
+
+def test_positron_very_long_step(tables, otables):
+    """dt = 10 ns: many events per particle and the event-ordinal continuation
+    draws of the repeated hazard."""
+    fields = wl.c2_fields()
+    tile = positrons(1 << 11, fields, seed=33)
+    cfg = StepConfig(rng_seed=93, dt_s=1e-8)
+    print("case c2 2^11 e+ dt=10ns x 1 step", flush=True)
+    run_and_compare(otables, tables, coupling_for(fields, tables), fields, tile, cfg, steps=1, label="e+ 10ns",
+                    species="positron")
+
+
+def test_positron_edge_cases(tables, otables):
+    """Below-cutoff positrons annihilating at rest where they stand, a
+    below-cutoff positron on an exiting chord (escape wins), zero weight, a
+    negative id, exits through r_hi / z_hi / z_lo, the axis, energies at the
+    cutoff and at the top of the tables."""
+    fields = wl.c2_fields()
+    k = np.array([5e3, 9.9e3, 1.0e4, 1.0001e4, 2e4, 1e6, 1e6, 1e6, 3e4, 1e9, 5e3, 2e6, 1e8, 6e4])
+    n = len(k)
+    u = wl.proper_velocity(k)
+    dirs = np.array([[0, 0, 1], [0, 0, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 0], [0, 0, 1], [0, 0, -1],
+                     [0.6, 0, 0.8], [0, 0, -1], [0, 0, 1], [0, 0, 1], [0, 0, -1], [0, 0, 1]], dtype=float)
+    r = np.array([100.0, 5.0, 10.0, 0.0, 3.0, fields.r_hi - 1e-5, 0.0, 0.0, 1.0, 50.0, 20.0, 7.0, 1000.0, 4.0])
+    z = np.array([500.0, 50.0, 5.0, 10.0, 300.0, 10.0, fields.z_hi - 1e-5, 1e-6, 100.0, 9000.0,
+                  fields.z_hi - 1e-9, 0.0, 2000.0, 600.0])
+    w = np.ones(n)
+    w[7] = 0.0
+    ids = np.arange(1, n + 1, dtype=np.int64)
+    ids[5] = -6
+    tile = ParticleTile(r=r, z=z, theta=np.linspace(-3, 3, n), ux=u * dirs[:, 0], uy=u * dirs[:, 1],
+                        uz=u * dirs[:, 2], w=w, id=ids, cpu=np.arange(n, dtype=np.int32))
+    print("case e+ edge 14 particles dt=1ns", flush=True)
+    run_and_compare(otables, tables, coupling_for(fields, tables), fields, tile, StepConfig(rng_seed=19, dt_s=1e-9),
+                    steps=1, label="e+ edge", species="positron")
+
+
+def test_positron_annihilation_photons(tables, otables):
+    """Every positron below the cutoff annihilates at rest before transport:
+    two back-to-back 511 keV photons at the start point, the residual kinetic
+    energy as ion pairs, the positive ion tallied, and a zero-length net chord."""
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile, RreaParticleInteraction, StepOutputs
+    fields = wl.c2_fields()
+    n = 256
+    rng = np.random.default_rng(4)
+    k = rng.uniform(1e3, 9.9e3, n)
+    tile = positrons(n, fields, seed=34)
+    scale = wl.proper_velocity(k) / np.sqrt(tile.ux ** 2 + tile.uy ** 2 + tile.uz ** 2)
+    tile.ux, tile.uy, tile.uz = tile.ux * scale, tile.uy * scale, tile.uz * scale
+    cp = coupling_for(fields, tables)
+    dev = DeviceTile.from_host(tile)
+    out = StepOutputs(n, fields, 4 * n)
+    RreaParticleInteraction.apply_positrons(tables, cp, dev, StepConfig(rng_seed=5, dt_s=1e-9), out)
+    torch.cuda.synchronize()
+    status = out.status.cpu().numpy()
+    assert np.all(status == 6)
+    s = out.secondaries_host()
+    assert len(s) == 2 * n
+    assert np.all(s["species"] == 2) and np.all(s["process"] == 9)
+    assert np.all(s["energy_eV"] == 510998.95)
+    d = np.stack([s["dir_x"], s["dir_y"], s["dir_z"]], 1).reshape(n, 2, 3)
+    assert np.allclose(d[:, 0], -d[:, 1], atol=0, rtol=0)
+    x = s["x"].reshape(n, 2)[:, 0]
+    y = s["y"].reshape(n, 2)[:, 0]
+    assert np.allclose(np.hypot(x, y), tile.r, rtol=1e-15, atol=1e-12)
+    assert np.all(s["birth_fraction"] == 1.0)
+    ion = out.ion_pairs.cpu().numpy()
+    assert np.allclose(ion, k / 33.97, rtol=1e-10)
+    t = out.tallies.cpu().numpy()
+    assert t[10] == n and t[12] == n
+    assert np.abs(out.jr.cpu().numpy()).max() == 0.0 and np.abs(out.jz.cpu().numpy()).max() == 0.0
+    from oracle import binding as ob
+    o = ob.positron_step(otables, fields, StepConfig(rng_seed=5, dt_s=1e-9), tile.as_dict())
+    so = o["secondaries"]
+    assert np.array_equal(so["parent_index"], s["parent_index"]) and np.array_equal(so["lineage"], s["lineage"])
+    for f in ("x", "y", "z", "dir_x", "dir_y", "dir_z", "energy_eV"):
+        assert np.allclose(s[f], so[f], rtol=1e-12, atol=1e-12 * max(1.0, np.abs(so[f]).max())), f
+
+
+def test_positron_determinism(tables):
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile, RreaParticleInteraction, StepOutputs
+    fields = wl.c2_fields()
+    tile = positrons(1 << 15, fields, seed=35)
+    cp = coupling_for(fields, tables)
+    cfg = StepConfig(rng_seed=7, dt_s=1e-9)
+    results = []
+    for _ in range(2):
+        dev = DeviceTile.from_host(tile)
+        out = StepOutputs(len(tile), fields, 1 << 16)
+        RreaParticleInteraction.apply_positrons(tables, cp, dev, cfg, out)
+        torch.cuda.synchronize()
+        results.append((dev.uz.cpu().numpy(), out.secondaries_host(), out.jz.cpu().numpy()))
+    assert np.array_equal(results[0][0], results[1][0])
+    assert results[0][1].tobytes() == results[1][1].tobytes()
+    assert np.allclose(results[0][2], results[1][2], rtol=1e-12, atol=1e-12 * np.abs(results[0][2]).max())
