@@ -1,20 +1,24 @@
 """Benchmark of the RREA transport step on one MI355X.
 
-Workload: the charged electron share of the at-scale configuration -- 2^24
-macroparticles, of which 70 % (11,744,051) are electrons with K ~ Exp(7.3 MeV)
-clipped to [10 keV, 100 MeV] in a 30-degree cone about the force, on a
-256 x 1024 RZ mesh over 2.56 km x 10 km with rho = 1.225 exp(-h / 8.4 km),
-h = 4 km + z, and |E_z| piecewise-linear 0 -> 500 kV/m; dt = 1 ps; FP64;
-net-chord current deposition on; secondaries staged and ordered each step.
+Workload: the at-scale configuration -- 2^24 macroparticles on a 256 x 1024 RZ
+mesh over 2.56 km x 10 km, rho = 1.225 exp(-h / 8.4 km) with h = 4 km + z,
+|E_z| piecewise-linear 0 -> 500 kV/m, dt = 1 ps, FP64, net-chord current
+deposition on, secondaries staged and ordered each step.  The charged shares
+are transported: 70 % electrons (11,744,051) and 5 % positrons (838,861),
+K ~ Exp(7.3 MeV) clipped to [10 keV, 100 MeV] in a 30-degree cone about the
+force.  (The 25 % photon share is not transported by this build yet.)
 
-One "step" is one transport pass of the HIP electron kernel over the whole
-tile, in place (the fused pre-push snapshot), followed by the deterministic
-secondary ordering.  Particles the step invalidates stay in the tile with the
-invalid id and are skipped by later steps, as WarpX keeps them until its
-redistribute.
+One "step" is one transport pass over both charged tiles, in place (the fused
+pre-push snapshot): the electron kernels, then the positron kernel, each
+followed by the deterministic secondary ordering, all depositing into one J
+grid.  Particles a step invalidates stay in their tile with the invalid id and
+are skipped by later steps, as WarpX keeps them until its redistribute.
 
-Prints one JSON line (rank 0, N=1):
-  metric / value (particle-steps/s of live electrons) / roofline / cpu_baseline ...
+Prints one JSON line (rank 0, N=1).  `roofline` is for the dominant kernel,
+the electron kernel: algorithmic bytes per launch (112 B per live electron)
+over its mean launch time from HIP events on its own stream; `traffic` is that
+kernel's measured HBM bytes per launch from the rocprofv3 FETCH_SIZE /
+WRITE_SIZE record in profiles/electron_kernel_traffic.json when present.
 """
 
 from __future__ import annotations
@@ -32,6 +36,8 @@ sys.path.insert(0, ROOT)
 
 B_ALG = 112.0  # bytes per particle-step: read r,z,theta,ux,uy,uz,w,id (64), write r,z,theta,ux,uy,uz (48)
 HBM_PEAK = 8.0e12
+TRAFFIC_RECORD = os.path.join(ROOT, "profiles", "electron_kernel_traffic.json")
+INVALID = np.iinfo(np.int64).min
 
 
 def log(msg):
@@ -40,7 +46,8 @@ def log(msg):
 
 def cpu_baseline(n_particles: int, steps: int, threads: int):
     """The CPU oracle (a restatement of the reference step, OpenMP over
-    64-particle chunks) timed on this host on a bounded sample."""
+    64-particle chunks) timed on this host on a bounded sample of the same
+    workload: electrons and positrons in the 70:5 ratio."""
     from oracle import binding as ob
     from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
     from rrea_feedback_warpx_amrex_geant4_amd.config import StepConfig
@@ -49,23 +56,35 @@ def cpu_baseline(n_particles: int, steps: int, threads: int):
     ob.build()
     tables = ob.Tables(load_bundle())
     fields = wl.c2_fields()
-    tile = wl.c2_electrons(n_particles, fields, seed=2)
-    p = tile.as_dict()
+    n_p = max(1, n_particles // 15)
+    tiles = {"electron": wl.c2_electrons(n_particles - n_p, fields, seed=2).as_dict(),
+             "positron": wl.c2_electrons(n_p, fields, seed=3, first_id=1 << 40, charge_sign=1.0).as_dict()}
     done = 0
     t0 = time.perf_counter()
     for s in range(steps):
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
-        o = ob.electron_step(tables, fields, cfg, p, nthreads=threads, sec_cap=1 << 20)
-        if o["code"] != 0:
-            raise RuntimeError(f"oracle error {o['code']}")
-        live = p["id"] != np.iinfo(np.int64).min
-        done += int(live.sum())
-        p = dict(p)
-        for k in ("r", "z", "theta", "ux", "uy", "uz"):
-            p[k] = o[k]
-        p["id"] = np.where(np.isin(o["status"], (1, 2, 3)), np.iinfo(np.int64).min, p["id"])
+        for species, p in tiles.items():
+            o = ob.charged_step(species, tables, fields, cfg, p, nthreads=threads, sec_cap=1 << 20)
+            if o["code"] != 0:
+                raise RuntimeError(f"oracle error {o['code']}")
+            done += int((p["id"] != INVALID).sum())
+            p = dict(p)
+            for k in ("r", "z", "theta", "ux", "uy", "uz"):
+                p[k] = o[k]
+            p["id"] = np.where(np.isin(o["status"], (1, 2, 3, 6)), INVALID, p["id"])
+            tiles[species] = p
     dt = time.perf_counter() - t0
     return done / dt, dt, done
+
+
+def traffic_record():
+    """Measured HBM bytes per launch of the electron kernel, or None."""
+    try:
+        with open(TRAFFIC_RECORD) as f:
+            rec = json.load(f)
+        return float(rec["bytes_per_launch"]), rec
+    except (OSError, KeyError, ValueError):
+        return None, None
 
 
 def main():
@@ -73,10 +92,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--particles", type=int, default=None, help="electrons (default: 70%% of 2^24)")
+    ap.add_argument("--particles", type=int, default=None, help="total macroparticles (default 2^24)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run only a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -97,6 +116,8 @@ def main():
               "(the J-grid allreduce design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    import ctypes as C
+
     import torch
 
     from rrea_feedback_warpx_amrex_geant4_amd import _lib
@@ -107,67 +128,78 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
-    total = 1 << 24
+    total = args.particles if args.particles else 1 << 24
     n_e, n_g, n_p = wl.c2_species_counts(total)
-    n = args.particles if args.particles else n_e
-    log(f"setup: {n} electrons (C2 electron share), building tables")
+    log(f"setup: {n_e} e- + {n_p} e+ of {total} (photon share {n_g} not transported), building tables")
     tables = InteractionTables()
     fields = wl.c2_fields()
     coupling = Coupling(fields, tables)
-    tile = wl.c2_electrons(n, fields, seed=2)
-    dev = DeviceTile.from_host(tile)
-    del tile
-    out = StepOutputs(n, fields, secondary_capacity=1 << 20, per_particle=False)
+    dev_e = DeviceTile.from_host(wl.c2_electrons(n_e, fields, seed=2))
+    dev_p = DeviceTile.from_host(wl.c2_electrons(n_p, fields, seed=3, first_id=1 << 40, charge_sign=1.0))
+    out_e = StepOutputs(n_e, fields, secondary_capacity=1 << 20, per_particle=False)
+    out_p = StepOutputs(n_p, fields, secondary_capacity=1 << 18, per_particle=False, share_grid=out_e)
     L = _lib.lib()
+    apply = RreaParticleInteraction
 
     def run_step(s):
-        out.secondary_count.zero_()
-        out.jr.zero_()
-        out.jz.zero_()
-        out.rho.zero_()
+        out_e.secondary_count.zero_()
+        out_p.secondary_count.zero_()
+        out_e.jr.zero_()
+        out_e.jz.zero_()
+        out_e.rho.zero_()
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
-        RreaParticleInteraction.apply_electrons(tables, coupling, dev, cfg, out, order_secondaries=True, check=False)
+        apply.apply_electrons(tables, coupling, dev_e, cfg, out_e, order_secondaries=True, check=False)
+        apply.apply_positrons(tables, coupling, dev_p, cfg, out_p, order_secondaries=True, check=False)
+
+    def live():
+        return (int((dev_e.id != _lib.INVALID_ID).sum().item()), int((dev_p.id != _lib.INVALID_ID).sum().item()))
 
     log(f"warmup {args.warmup}")
     for s in range(args.warmup):
         run_step(s)
     torch.cuda.synchronize()
-    out.check_error()
-    live_before = int((dev.id != _lib.INVALID_ID).sum().item())
+    out_e.check_error()
+    e0, p0 = live()
 
-    import ctypes as C
     L.rrea_kernel_timing(coupling.h, 1)
     log(f"timed {args.steps} steps")
-    live_counts = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        live_counts.append(None)
         run_step(args.warmup + s)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    kms = C.c_double(0.0)
-    launches = C.c_int64(0)
-    _lib.check(L.rrea_kernel_timing_read(coupling.h, C.byref(kms), C.byref(launches)), "timing read")
+    kms = {}
+    for name, kid in (("electron", 0), ("electron_slow", 1), ("positron", 2)):
+        ms = C.c_double(0.0)
+        cnt = C.c_int64(0)
+        _lib.check(L.rrea_kernel_timing_read(coupling.h, kid, C.byref(ms), C.byref(cnt)), "timing read")
+        kms[name] = (ms.value, cnt.value)
     L.rrea_kernel_timing(coupling.h, 0)
-    out.check_error()
-    live_after = int((dev.id != _lib.INVALID_ID).sum().item())
-    # live particles decline monotonically; the mean of the ends bounds the
-    # processed count within the (tiny) per-step loss
-    particle_steps = 0.5 * (live_before + live_after) * args.steps
-    value = particle_steps / elapsed
-    kernel_s = kms.value * 1e-3
-    achieved = particle_steps * B_ALG / kernel_s
-    log(f"live {live_before} -> {live_after}; wall {elapsed:.3f}s; kernel {kernel_s:.3f}s over {launches.value}")
+    out_e.check_error()
+    e1, p1 = live()
+    # live counts decline monotonically and by a tiny fraction per step; the
+    # mean of the ends bounds the processed count
+    steps_e = 0.5 * (e0 + e1) * args.steps
+    steps_p = 0.5 * (p0 + p1) * args.steps
+    value = (steps_e + steps_p) / elapsed
+    e_ms, e_launches = kms["electron"]
+    mean_launch_s = e_ms * 1e-3 / max(e_launches, 1)
+    alg_bytes = 0.5 * (e0 + e1) * B_ALG
+    achieved = alg_bytes / mean_launch_s
+    log(f"live e- {e0} -> {e1}, e+ {p0} -> {p1}; wall {elapsed:.3f}s; kernels " +
+        ", ".join(f"{k} {v[0]:.2f} ms / {v[1]}" for k, v in kms.items()))
+    traffic, rec = traffic_record()
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle, {threads} threads, {args.cpu_particles} e- x {args.cpu_steps} steps")
+        log(f"cpu baseline: oracle, {threads} threads, {args.cpu_particles} charged x {args.cpu_steps} steps")
         rate, secs, done = cpu_baseline(args.cpu_particles, args.cpu_steps, threads)
         cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_particles} C2 electrons x {args.cpu_steps} steps ({done} particle-steps, "
-                         f"{secs:.1f} s), oracle restatement, OpenMP dynamic over 64-particle chunks"}
+               "sample": f"{args.cpu_particles} C2 charged particles (e-:e+ = 14:1) x {args.cpu_steps} steps "
+                         f"({done} particle-steps, {secs:.1f} s), oracle restatement, OpenMP dynamic over "
+                         f"64-particle chunks"}
 
     line = {
         "metric": "particle-steps/s on 1 MI355X, plain vs batched, at 1e-12 step match; % HBM peak",
@@ -182,10 +214,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "C2 plain at scale, electron share: 11,744,051 e- of 2^24, dt=1 ps, RZ 256x1024, "
-                               "deposit on, secondaries ordered", "particles": n, "path": "plain"},
+        "config": {"workload": f"C2 plain at scale, charged shares of {total}: {n_e} e- + {n_p} e+, dt=1 ps, "
+                               "RZ 256x1024, deposit on, secondaries ordered; photon share not transported",
+                   "electrons": n_e, "positrons": n_p, "path": "plain",
+                   "kernel_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in kms.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None},
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "kernel": "electron_fast_kernel", "alg_bytes_per_launch": alg_bytes,
+                     "mean_launch_ms": mean_launch_s * 1e3},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

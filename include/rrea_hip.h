@@ -235,10 +235,14 @@ rrea_status rrea_transport_step_positrons(rrea_tables_t tables, rrea_fields_t fi
 
 /* Kernel timing with HIP events recorded around every transport-kernel
  * launch made through `fields` while enabled (no counterpart in the reference:
- * measurement support).  _read waits for the last launch, returns the summed
- * kernel milliseconds and the launch count, and clears them. */
+ * measurement support).  Enabling or disabling clears the record.  _read
+ * waits for the recorded launches of one kernel -- 0 the electron kernel
+ * (fast path, or the complete path when complete_path_only is set), 1 the
+ * electron complete-path kernel over the fast path's hand-over list, 2 the
+ * positron kernel, -1 all -- and returns their summed milliseconds and count. */
+enum { RREA_KERNEL_ALL = -1, RREA_KERNEL_ELECTRON = 0, RREA_KERNEL_ELECTRON_SLOW = 1, RREA_KERNEL_POSITRON = 2 };
 rrea_status rrea_kernel_timing(rrea_fields_t fields, int32_t enable);
-rrea_status rrea_kernel_timing_read(rrea_fields_t fields, double* total_ms, int64_t* launches);
+rrea_status rrea_kernel_timing_read(rrea_fields_t fields, int32_t kernel, double* total_ms, int64_t* launches);
 
 /* The library version string. */
 const char* rrea_version(void);

@@ -118,7 +118,7 @@ SIGNATURES = [
     ("rrea_transport_step_positrons", C.c_int32,
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
-    ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, P_D, P_I64]),
+    ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),
 ]
 

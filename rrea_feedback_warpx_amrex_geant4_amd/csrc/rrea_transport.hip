@@ -165,7 +165,8 @@ struct rrea_fields_s {
     int32_t* d_slow_count = nullptr;
     int64_t slow_cap = 0;
     bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    struct Timed { hipEvent_t start, stop; int32_t kernel; };
+    std::vector<Timed> events;   // pool; the first events_used are recorded launches
     size_t events_used = 0;
 };
 
@@ -1487,7 +1488,7 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
                     F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args,
                     F->d_slow_list, F->d_slow_count};
     for (void* p : ptrs) if (p) (void)hipFree(p);
-    for (auto& ev : F->events) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    for (auto& ev : F->events) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
     delete F;
     return RREA_OK;
 }
@@ -1496,22 +1497,25 @@ rrea_status rrea_kernel_timing(rrea_fields_t F, int32_t enable)
 {
     if (!F) return RREA_ERR_ARGUMENT;
     F->timing = enable != 0;
+    F->events_used = 0;
     return RREA_OK;
 }
 
-rrea_status rrea_kernel_timing_read(rrea_fields_t F, double* total_ms, int64_t* launches)
+rrea_status rrea_kernel_timing_read(rrea_fields_t F, int32_t kernel, double* total_ms, int64_t* launches)
 {
-    if (!F || !total_ms || !launches) return RREA_ERR_ARGUMENT;
+    if (!F || !total_ms || !launches || kernel < -1 || kernel > 2) return RREA_ERR_ARGUMENT;
     double sum = 0.0;
+    int64_t count = 0;
     for (size_t i = 0; i < F->events_used; ++i) {
-        HIP_TRY(hipEventSynchronize(F->events[i].second));
+        if (kernel >= 0 && F->events[i].kernel != kernel) continue;
+        HIP_TRY(hipEventSynchronize(F->events[i].stop));
         float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, F->events[i].first, F->events[i].second));
+        HIP_TRY(hipEventElapsedTime(&ms, F->events[i].start, F->events[i].stop));
         sum += ms;
+        ++count;
     }
     *total_ms = sum;
-    *launches = static_cast<int64_t>(F->events_used);
-    F->events_used = 0;
+    *launches = count;
     return RREA_OK;
 }
 
@@ -1694,25 +1698,32 @@ rrea_status charged_step(bool positron, rrea_tables_t T, rrea_fields_t F, const 
     int64_t const nb = blocks_for(tile->n, kBlock);
     // pageable source: the copy has consumed A when hipMemcpyAsync returns
     HIP_TRY(hipMemcpyAsync(F->d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
-    hipEvent_t ev_stop = nullptr;
-    if (F->timing) {
-        if (F->events_used == F->events.size()) {
-            hipEvent_t a, b;
-            HIP_TRY(hipEventCreate(&a));
-            HIP_TRY(hipEventCreate(&b));
-            F->events.emplace_back(a, b);
+    // HIP events around each transport kernel while timing is enabled
+    auto timed = [&](int32_t kernel, auto&& launch) -> rrea_status {
+        hipEvent_t stop = nullptr;
+        if (F->timing) {
+            if (F->events_used == F->events.size()) {
+                hipEvent_t a, b;
+                HIP_TRY(hipEventCreate(&a));
+                HIP_TRY(hipEventCreate(&b));
+                F->events.push_back({a, b, 0});
+            }
+            rrea_fields_s::Timed& t = F->events[F->events_used++];
+            t.kernel = kernel;
+            HIP_TRY(hipEventRecord(t.start, s));
+            stop = t.stop;
         }
-        HIP_TRY(hipEventRecord(F->events[F->events_used].first, s));
-        ev_stop = F->events[F->events_used].second;
-        ++F->events_used;
-    }
+        launch();
+        HIP_TRY(hipGetLastError());
+        if (stop) HIP_TRY(hipEventRecord(stop, s));
+        return RREA_OK;
+    };
     const StepArgs* const d_args = static_cast<const StepArgs*>(F->d_args);
+    rrea_status st = RREA_OK;
     if (positron) {
-        positron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
-        HIP_TRY(hipGetLastError());
+        st = timed(2, [&] { positron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args); });
     } else if (cfg->complete_path_only) {
-        electron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
-        HIP_TRY(hipGetLastError());
+        st = timed(0, [&] { electron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args); });
     } else {
         if (F->slow_cap < tile->n) {
             if (F->d_slow_list) (void)hipFree(F->d_slow_list);
@@ -1723,15 +1734,18 @@ rrea_status charged_step(bool positron, rrea_tables_t T, rrea_fields_t F, const 
         }
         if (!F->d_slow_count) HIP_TRY(hipMalloc(&F->d_slow_count, sizeof(int32_t)));
         HIP_TRY(hipMemsetAsync(F->d_slow_count, 0, sizeof(int32_t), s));
-        electron_fast_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args, F->d_slow_list,
-                                                                                     F->d_slow_count);
-        HIP_TRY(hipGetLastError());
+        st = timed(0, [&] {
+            electron_fast_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args, F->d_slow_list,
+                                                                                         F->d_slow_count);
+        });
+        if (st != RREA_OK) return st;
         int64_t const slow_blocks = nb < 2048 ? nb : 2048;
-        electron_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
-            d_args, F->d_slow_list, F->d_slow_count);
-        HIP_TRY(hipGetLastError());
+        st = timed(1, [&] {
+            electron_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
+                d_args, F->d_slow_list, F->d_slow_count);
+        });
     }
-    if (ev_stop) HIP_TRY(hipEventRecord(ev_stop, s));
+    if (st != RREA_OK) return st;
 
     if (order) {
         int64_t nstaged = 0;

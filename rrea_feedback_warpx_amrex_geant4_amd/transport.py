@@ -196,7 +196,11 @@ class DeviceTile:
 class StepOutputs:
     """Device outputs of one step (allocated once, reused)."""
 
-    def __init__(self, n: int, fields: RzFields, secondary_capacity: int, device="cuda", per_particle: bool = True):
+    def __init__(self, n: int, fields: RzFields, secondary_capacity: int, device="cuda", per_particle: bool = True,
+                 share_grid: "StepOutputs | None" = None):
+        """``share_grid``: deposit into another output's J grid and tallies, as
+        every species of one step deposits into the coupling's one net-chord
+        grid (RreaWarpXCoupling.cpp:417-444)."""
         nrp, nzp = fields.padded_shape()
         self.status = torch.zeros(n, dtype=torch.uint8, device=device) if per_particle else None
         self.ion_pairs = torch.zeros(n, dtype=torch.float64, device=device) if per_particle else None
@@ -205,11 +209,15 @@ class StepOutputs:
                                        device=device)
         self.secondary_capacity = secondary_capacity
         self.secondary_count = torch.zeros(1, dtype=torch.int64, device=device)
-        self.jr = torch.zeros((nrp + 1) * nzp, dtype=torch.float64, device=device)
-        self.jz = torch.zeros(nrp * (nzp + 1), dtype=torch.float64, device=device)
-        self.rho = torch.zeros(nrp * nzp, dtype=torch.float64, device=device)
-        self.tallies = torch.zeros(_lib.TALLY_COUNT, dtype=torch.float64, device=device)
-        self.error = torch.zeros(2, dtype=torch.int32, device=device)
+        if share_grid is not None:
+            self.jr, self.jz, self.rho = share_grid.jr, share_grid.jz, share_grid.rho
+            self.tallies, self.error = share_grid.tallies, share_grid.error
+        else:
+            self.jr = torch.zeros((nrp + 1) * nzp, dtype=torch.float64, device=device)
+            self.jz = torch.zeros(nrp * (nzp + 1), dtype=torch.float64, device=device)
+            self.rho = torch.zeros(nrp * nzp, dtype=torch.float64, device=device)
+            self.tallies = torch.zeros(_lib.TALLY_COUNT, dtype=torch.float64, device=device)
+            self.error = torch.zeros(2, dtype=torch.int32, device=device)
 
     def reset(self):
         for t in (self.secondary_count, self.jr, self.jz, self.rho, self.tallies, self.error):
