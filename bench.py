@@ -170,7 +170,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     kms = {}
-    for name, kid in (("electron", 0), ("electron_slow", 1), ("positron", 2)):
+    for name, kid in (("electron", 0), ("electron_handover", 1), ("positron", 2), ("positron_handover", 3)):
         ms = C.c_double(0.0)
         cnt = C.c_int64(0)
         _lib.check(L.rrea_kernel_timing_read(coupling.h, kid, C.byref(ms), C.byref(cnt)), "timing read")

@@ -227,8 +227,7 @@ rrea_status rrea_transport_step_electrons(rrea_tables_t tables, rrea_fields_t fi
  * (warpx_overlay/Source/Rrea/RreaPositronTransport.cpp:10-871) -- hard
  * Bhabha, tracked brems and in-flight annihilation as competing hazards,
  * at-rest annihilation of a stopped or sub-cutoff positron.  Same arguments
- * and error behaviour as rrea_transport_step_electrons; complete_path_only is
- * ignored (positrons always take the complete path). */
+ * and error behaviour as rrea_transport_step_electrons. */
 rrea_status rrea_transport_step_positrons(rrea_tables_t tables, rrea_fields_t fields, const rrea_tile* tile,
                                           const rrea_step_config* config, const rrea_step_out* out,
                                           rrea_stream_t stream);
@@ -236,11 +235,17 @@ rrea_status rrea_transport_step_positrons(rrea_tables_t tables, rrea_fields_t fi
 /* Kernel timing with HIP events recorded around every transport-kernel
  * launch made through `fields` while enabled (no counterpart in the reference:
  * measurement support).  Enabling or disabling clears the record.  _read
- * waits for the recorded launches of one kernel -- 0 the electron kernel
- * (fast path, or the complete path when complete_path_only is set), 1 the
- * electron complete-path kernel over the fast path's hand-over list, 2 the
- * positron kernel, -1 all -- and returns their summed milliseconds and count. */
-enum { RREA_KERNEL_ALL = -1, RREA_KERNEL_ELECTRON = 0, RREA_KERNEL_ELECTRON_SLOW = 1, RREA_KERNEL_POSITRON = 2 };
+ * waits for the recorded launches of one kernel and returns their summed
+ * milliseconds and count: per species, the main kernel (the fast path, or the
+ * complete path when complete_path_only is set) and the complete-path kernel
+ * over the fast path's hand-over list; -1 = all. */
+enum {
+    RREA_KERNEL_ALL = -1,
+    RREA_KERNEL_ELECTRON = 0,
+    RREA_KERNEL_ELECTRON_HANDOVER = 1,
+    RREA_KERNEL_POSITRON = 2,
+    RREA_KERNEL_POSITRON_HANDOVER = 3,
+};
 rrea_status rrea_kernel_timing(rrea_fields_t fields, int32_t enable);
 rrea_status rrea_kernel_timing_read(rrea_fields_t fields, int32_t kernel, double* total_ms, int64_t* launches);
 

@@ -127,10 +127,22 @@ struct DevAngle {
     int32_t n_e, n_n, n_u;
 };
 
+// A bin table of starting indices for a search over a nondecreasing node
+// array: bin b covers keys [lo + b/scale, lo + (b+1)/scale) and holds the
+// lower bound of its left edge.  It only shortens the search; the scans in
+// guided_lower_bound / guided_upper_bound return exactly std::lower_bound /
+// std::upper_bound from any start.
+struct Guide {
+    const int32_t* idx;
+    int32_t nbins;
+    double lo, scale;
+};
+
 struct DevElastic {
     const double* e; const double* log_e;
     const double* mfp; const double* eta;
     int32_t n;
+    Guide guide;                // over log e
 };
 
 struct DevTables {
@@ -141,6 +153,7 @@ struct DevTables {
     const double* clogy[kChargedCount];
     int32_t cmode[kChargedCount];
     const double* hm_mean;          // hard-Moller mean secondary on the grid
+    Guide cguide;                   // over log of the charged grid
     DevCdf moller, ebrems, bhabha, pbrems;
     DevAngle gs_e, wz_e, gs_p, wz_p;
     DevElastic el_e, el_p;
@@ -158,6 +171,32 @@ struct Bracket {
     double t;       // (log q - log x_lo) / (log x_hi - log x_lo)
     double lin_t;   // (q - x_lo) / (x_hi - x_lo), for linear tables
 };
+
+__device__ __forceinline__ int32_t guide_start(Guide const& g, double key)
+{
+    double const f = (key - g.lo) * g.scale;
+    int32_t b = 0;
+    if (f > 0.0) b = f < static_cast<double>(g.nbins - 1) ? static_cast<int32_t>(f) : g.nbins - 1;
+    return g.idx[b];
+}
+
+// std::lower_bound (first x[i] >= q) from the guide's start for `key`.
+__device__ __forceinline__ int32_t guided_lower_bound(const double* x, int32_t n, double q, Guide const& g, double key)
+{
+    int32_t i = guide_start(g, key);
+    while (i > 0 && !(x[i - 1] < q)) --i;
+    while (i < n && x[i] < q) ++i;
+    return i;
+}
+
+// std::upper_bound (first x[i] > q) from the guide's start for `key`.
+__device__ __forceinline__ int32_t guided_upper_bound(const double* x, int32_t n, double q, Guide const& g, double key)
+{
+    int32_t i = guide_start(g, key);
+    while (i > 0 && q < x[i - 1]) --i;
+    while (i < n && !(q < x[i])) ++i;
+    return i;
+}
 
 __device__ __forceinline__ int32_t lower_bound(const double* x, int32_t n, double q)
 {
@@ -178,7 +217,8 @@ __device__ __forceinline__ bool charged_bracket(DevTables const& T, double q, Br
         b.lo = 0; b.hi = 0; b.kind = 1; b.t = 0.0; b.lin_t = 0.0;
         return false;
     }
-    int32_t const up = lower_bound(T.cx, T.nc, q);
+    double const lq = log(q);
+    int32_t const up = guided_lower_bound(T.cx, T.nc, q, T.cguide, lq);
     b.lo = up - 1;
     b.hi = up;
     b.t = 0.0;
@@ -187,7 +227,7 @@ __device__ __forceinline__ bool charged_bracket(DevTables const& T, double q, Br
     if (up == T.nc) { b.kind = 2; return true; }
     if (T.cx[up] == q) { b.kind = 3; return true; }
     b.kind = 0;
-    b.t = (log(q) - T.clogx[b.lo]) / (T.clogx[b.hi] - T.clogx[b.lo]);
+    b.t = (lq - T.clogx[b.lo]) / (T.clogx[b.hi] - T.clogx[b.lo]);
     b.lin_t = (q - T.cx[b.lo]) / (T.cx[b.hi] - T.cx[b.lo]);
     return true;
 }
@@ -226,10 +266,11 @@ __device__ __forceinline__ double elastic_semilog(DevElastic const& E, const dou
     if (!finite(q) || q < E.e[0] || q > E.e[E.n - 1]) { if (!err) err = kErrRange; return 0.0; }
     if (q == E.e[0]) return y[0];
     if (q == E.e[E.n - 1]) return y[E.n - 1];
-    int32_t const hi = lower_bound(E.e, E.n, q);
+    double const lq = log(q);
+    int32_t const hi = guided_lower_bound(E.e, E.n, q, E.guide, lq);
     int32_t const lo = hi - 1;
     if (E.e[hi] == q) return y[hi];
-    double const t = (log(q) - E.log_e[lo]) / (E.log_e[hi] - E.log_e[lo]);
+    double const t = (lq - E.log_e[lo]) / (E.log_e[hi] - E.log_e[lo]);
     return y[lo] + t * (y[hi] - y[lo]);
 }
 
@@ -243,12 +284,13 @@ __device__ __forceinline__ bool elastic_bracket(DevElastic const& E, double q, E
     if (!finite(q) || q < E.e[0] || q > E.e[E.n - 1]) return false;
     if (q == E.e[0]) return true;
     if (q == E.e[E.n - 1]) { b.kind = 2; return true; }
-    int32_t const hi = lower_bound(E.e, E.n, q);
+    double const lq = log(q);
+    int32_t const hi = guided_lower_bound(E.e, E.n, q, E.guide, lq);
     b.hi = hi;
     b.lo = hi - 1;
     if (E.e[hi] == q) { b.kind = 3; return true; }
     b.kind = 0;
-    b.t = (log(q) - E.log_e[b.lo]) / (E.log_e[b.hi] - E.log_e[b.lo]);
+    b.t = (lq - E.log_e[b.lo]) / (E.log_e[b.hi] - E.log_e[b.lo]);
     return true;
 }
 
@@ -561,8 +603,8 @@ struct DevFields {
     const double* prof_rho;
     int32_t prof_n;
     double altitude_at_z0;
-    double inv_reference_density_unused;
     double reference_density;
+    Guide prof_guide;        // over altitude
 };
 
 // TransportDensityRatioAtZ OV/RreaWarpXCoupling.cpp:79-95 +
@@ -574,13 +616,7 @@ __device__ __forceinline__ double density_ratio_at_z(DevFields const& F, double 
     int32_t const n = F.prof_n;
     const double* a = F.prof_alt;
     if (!(alt >= a[0]) || !(alt <= a[n - 1])) { if (!err) err = kErrRange; return 1.0; }
-    // std::upper_bound
-    int32_t first = 0, count = n;
-    while (count > 0) {
-        int32_t const step = count >> 1;
-        int32_t const it = first + step;
-        if (!(alt < a[it])) { first = it + 1; count -= step + 1; } else { count = step; }
-    }
+    int32_t const first = guided_upper_bound(a, n, alt, F.prof_guide, alt);
     double v;
     if (first == 0) v = F.prof_rho[0];
     else if (first == n) v = F.prof_rho[n - 1];

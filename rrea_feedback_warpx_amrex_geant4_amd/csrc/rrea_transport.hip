@@ -105,6 +105,32 @@ double host_sample_moller(HostCdf const& h, double primary, double cu)
     return std::exp(std::log(y0) + t * (std::log(y1) - std::log(y0)));
 }
 
+// A bin table of lower-bound starts over nondecreasing keys (the Guide of
+// rrea_device.h): any start is corrected by the device scans, so the table
+// only has to be close.
+struct HostGuide {
+    std::vector<int32_t> idx;
+    double lo = 0.0, scale = 0.0;
+};
+
+HostGuide host_guide(std::vector<double> const& key, int32_t nbins)
+{
+    HostGuide g;
+    g.lo = key.front();
+    double const span = key.back() - key.front();
+    g.scale = span > 0.0 ? static_cast<double>(nbins) / span : 0.0;
+    g.idx.resize(nbins);
+    for (int32_t b = 0; b < nbins; ++b) {
+        double const edge = g.scale > 0.0 ? g.lo + static_cast<double>(b) / g.scale : g.lo;
+        g.idx[b] = static_cast<int32_t>(std::lower_bound(key.begin(), key.end(), edge) - key.begin());
+    }
+    return g;
+}
+
+constexpr int32_t kChargedGuideBins = 8192;
+constexpr int32_t kElasticGuideBins = 1024;
+constexpr int32_t kProfileGuideBins = 4096;
+
 struct DeviceBlob {
     std::vector<double> f64;
     std::vector<int32_t> i32;
@@ -145,6 +171,8 @@ struct rrea_fields_s {
     size_t stage_len = 0;
     double* d_prof = nullptr;    // altitude | rho
     int64_t prof_n = 0;
+    int32_t* d_prof_guide = nullptr;
+    double prof_guide_lo = 0.0, prof_guide_scale = 0.0;
     int density_mode = 0;
     double density_ratio = 1.0;
     double altitude_at_z0 = 0.0;
@@ -1059,9 +1087,10 @@ __device__ __forceinline__ void reduce_tallies(double* tally, double* out)
 }
 
 // The particles the fast path handed over, strided over a fixed grid.
-__global__ void __launch_bounds__(kBlock) electron_slow_kernel(const StepArgs* __restrict__ args,
-                                                             const int32_t* __restrict__ list,
-                                                             const int32_t* __restrict__ count)
+template <bool kPos>
+__global__ void __launch_bounds__(kBlock) charged_slow_kernel(const StepArgs* __restrict__ args,
+                                                            const int32_t* __restrict__ list,
+                                                            const int32_t* __restrict__ count)
 {
     StepArgs const& A = *args;
     double tally[kTallyUsed];
@@ -1069,7 +1098,7 @@ __global__ void __launch_bounds__(kBlock) electron_slow_kernel(const StepArgs* _
     int64_t const n = *count;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kBlock) {
-        charged_full_particle<false>(A, list[i], tally);
+        charged_full_particle<kPos>(A, list[i], tally);
     }
     reduce_tallies(tally, A.tallies);
 }
@@ -1100,10 +1129,12 @@ __device__ __forceinline__ void append_slow(bool slow, int64_t ip, int32_t* list
     }
 }
 
-__global__ void __launch_bounds__(kBlock) electron_fast_kernel(const StepArgs* __restrict__ args,
-                                                             int32_t* __restrict__ slow_list,
-                                                             int32_t* __restrict__ slow_count)
+template <bool kPos>
+__global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __restrict__ args,
+                                                            int32_t* __restrict__ slow_list,
+                                                            int32_t* __restrict__ slow_count)
 {
+    using S = Species<kPos>;
     StepArgs const& A = *args;
     double tally[kTallyUsed];
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
@@ -1120,6 +1151,7 @@ __global__ void __launch_bounds__(kBlock) electron_fast_kernel(const StepArgs* _
         double const thr = T.hard_moller_threshold;
         bool const hm_on = cfg.disable_hard_moller == 0;
         double const dt = cfg.dt_s;
+        DevElastic const& EL = kPos ? T.el_p : T.el_e;
         int32_t err = kOk;
         int64_t const id = A.id[ip];
         if (id == RREA_INVALID_ID) {
@@ -1145,24 +1177,37 @@ __global__ void __launch_bounds__(kBlock) electron_fast_kernel(const StepArgs* _
             Interval const iv = build_interval(F, path, D, A.density_ratio, err);
             if (!iv.valid || err) break;
             double const span = dmax(path.end_f, 1.0e-6);
-            FieldRange const fr = path_field_work(F, path, D, -1.0, 0.0, 1.0, err);
+            FieldRange const fr = path_field_work(F, path, D, S::kCharge, 0.0, 1.0, err);
             if (err) break;
             double const field_work = fr.work / span;
             double const chord_dp = density_path(iv, path, 0.0, path.end_f);
             // substep sizing: all three drivers must ask for one substep
             Bracket b0;
             if (!charged_bracket(T, k, b0)) break;
-            double hm0, br0;
-            electron_stp_rates(T, b0, thr, gcut, k, hm_on, hm0, br0, err);
             double const ddp = chord_dp / span;
             double const target = cfg.max_optical_depth_per_substep;
-            if (dmax(1.0, ceil((hm0 * ddp + br0 * ddp) / target)) != 1.0) break;
-            if (!((hm0 + br0) * max_uniform_slice(iv, path, 1) <= target + 64.0 * kEps * dmax(1.0, target))) break;
+            double tau0, total0;
+            if constexpr (kPos) {
+                double bh0, br0, an0;
+                positron_stp_rates(T, b0, thr, gcut, k, bh0, br0, an0, err);
+                total0 = bh0 + br0 + an0;
+                tau0 = total0 * ddp;
+            } else {
+                double hm0, br0;
+                electron_stp_rates(T, b0, thr, gcut, k, hm_on, hm0, br0, err);
+                total0 = hm0 + br0;
+                tau0 = hm0 * ddp + br0 * ddp;
+            }
+            if (dmax(1.0, ceil(tau0 / target)) != 1.0) break;
+            if (!(total0 * max_uniform_slice(iv, path, 1) <= target + 64.0 * kEps * dmax(1.0, target))) break;
             EBracket eb0;
-            if (!elastic_bracket(T.el_e, k, eb0)) break;
-            double const mfp0 = elastic_value(T.el_e, T.el_e.mfp, eb0) / 1.0;
+            if (!elastic_bracket(EL, k, eb0)) break;
+            double const mfp0 = elastic_value(EL, EL.mfp, eb0) / 1.0;
             double const gs_n_total = (mfp0 > 0.0 ? 1.0 / mfp0 : 0.0) * ddp;
-            double const drag_est = (charged_value(T, kECol, b0, err) * 1.0 + charged_value(T, kESoft, b0, err) * 1.0) * ddp;
+            // the driver's drag estimate: collision + soft (e-), collision only (e+)
+            double const drag_est = kPos ? charged_value(T, kPCol, b0, err) * 1.0 * ddp
+                                         : (charged_value(T, kECol, b0, err) * 1.0
+                                            + charged_value(T, kESoft, b0, err) * 1.0) * ddp;
             double const impulse_scale = dmax(cfg.max_field_impulse_fraction * k, dmax(cfg.field_impulse_floor_eV, 1.0));
             if (dmax(1.0, ceil(dmin(gs_n_total, cfg.gs_total_collision_cap) / dmax(cfg.max_gs_collisions_per_substep, 1.0))) != 1.0) break;
             if (dmax(1.0, ceil((fabs(field_work) + drag_est) / impulse_scale)) != 1.0) break;
@@ -1171,23 +1216,29 @@ __global__ void __launch_bounds__(kBlock) electron_fast_kernel(const StepArgs* _
             double const se = path.end_f;
             double const substep_dt = dt * (se - 0.0);
             double k1 = k, d1x = dx, d1y = dy, d1z = dz;
-            double const hw1 = half_kick(fr, 0.5 * substep_dt, -1.0, k1, d1x, d1y, d1z);
+            double const hw1 = half_kick(fr, 0.5 * substep_dt, S::kCharge, k1, d1x, d1y, d1z);
             if (!(k1 >= cutoff) || !(0.0 < se)) break;
             // the first hazard attempt: no event
             Bracket b1;
             if (!charged_bracket(T, k1, b1)) break;
-            double s0, s1;
-            electron_stp_rates(T, b1, thr, gcut, k1, hm_on, s0, s1, err);
             double const L = density_path(iv, path, 0.0, se);
             uint64_t const rng = rng_prefix(cfg.rng_seed, static_cast<uint64_t>(id < 0 ? -id : id)
                                                 ^ (static_cast<uint64_t>(static_cast<uint32_t>(A.cpu ? A.cpu[ip] : 0)) << 40),
                                             static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0));
-            Hazard const h = select_hazard(s0 * L, s1 * L, rng_uniform(rng, 0, kChCompetingOpticalDepth),
-                                           rng_uniform(rng, 0, kChCompetingChannel));
+            Hazard h;
+            if constexpr (kPos) {
+                double s0, s1, s2;
+                positron_stp_rates(T, b1, thr, gcut, k1, s0, s1, s2, err);
+                h = select_hazard3(s0 * L, s1 * L, s2 * L, rng_uniform(rng, 0, S::kDepth0), rng_uniform(rng, 0, S::kPick0));
+            } else {
+                double s0, s1;
+                electron_stp_rates(T, b1, thr, gcut, k1, hm_on, s0, s1, err);
+                h = select_hazard(s0 * L, s1 * L, rng_uniform(rng, 0, S::kDepth0), rng_uniform(rng, 0, S::kPick0));
+            }
             if (!h.valid || h.event || err) break;
             // continuous drag over [0, se] at the frozen energy k1, not reaching the cutoff
-            double const col = charged_value(T, kECol, b1, err) * 1.0;
-            double const soft = charged_value(T, kESoft, b1, err) * 1.0;
+            double const col = charged_value(T, S::kCol, b1, err) * 1.0;
+            double const soft = charged_value(T, S::kSoft, b1, err) * 1.0;
             double const total = col + soft;
             double const to_cut = k1 - cutoff;
             double const req = total * L;
@@ -1211,36 +1262,36 @@ __global__ void __launch_bounds__(kBlock) electron_fast_kernel(const StepArgs* _
             if (!(k2 >= cutoff)) break;
             // elastic scattering: the few-collision branch with at most two scatters
             EBracket eb2;
-            if (!elastic_bracket(T.el_e, k2, eb2)) break;
-            double const mfp2 = elastic_value(T.el_e, T.el_e.mfp, eb2) / 1.0;
+            if (!elastic_bracket(EL, k2, eb2)) break;
+            double const mfp2 = elastic_value(EL, EL.mfp, eb2) / 1.0;
             double const gs_n = (mfp2 > 0.0 ? 1.0 / mfp2 : 0.0) * L;
             if (gs_n > 0.0) {
                 if (!(gs_n < T.gs_few) || ceil(3.0 * T.gs_few) >= 255.0) break;
-                int const cnt = poisson_capped(gs_n, rng_uniform(rng, 0U, kChGsFewCount), static_cast<int>(ceil(3.0 * T.gs_few)));
+                int const cnt = poisson_capped(gs_n, rng_uniform(rng, 0U, S::kFewCount), static_cast<int>(ceil(3.0 * T.gs_few)));
                 if (cnt > 2) break;
                 if (cnt > 0 && !(k2 >= T.charged_min && k2 <= T.charged_max)) break;
                 for (int q = 0; q < cnt; ++q) {
                     uint64_t const inner = static_cast<uint64_t>(q + 1);
-                    double const eta = elastic_value(T.el_e, T.el_e.eta, eb2);
-                    double const u = dmin(dmax(rng_uniform(rng, inner, kChGsFewScatter), 0.0), 1.0 - 1.0e-15);
+                    double const eta = elastic_value(EL, EL.eta, eb2);
+                    double const u = dmin(dmax(rng_uniform(rng, inner, S::kFewScatter), 0.0), 1.0 - 1.0e-15);
                     double const ct = 1.0 - 2.0 * eta * u / (1.0 + eta - u);
                     double nx, ny, nz;
-                    cone_direction(d1x, d1y, d1z, ct, rng_uniform(rng, inner, kChGsFewAzimuth), nx, ny, nz);
+                    cone_direction(d1x, d1y, d1z, ct, rng_uniform(rng, inner, S::kFewPhi), nx, ny, nz);
                     d1x = nx; d1y = ny; d1z = nz;
                 }
             }
-            double const hw2 = half_kick(fr, 0.5 * substep_dt, -1.0, k2, d1x, d1y, d1z);
+            double const hw2 = half_kick(fr, 0.5 * substep_dt, S::kCharge, k2, d1x, d1y, d1z);
             if (!(k2 >= cutoff)) break;
             double x, y, z;
             point_at_full(path, se, x, y, z);
             if (!in_half_open(x, y, z, D) || path.exits) break;
-            // terminal: a survivor above the demotion gate's energy
-            if (thr > 0.0 && k2 <= thr) break;
+            // terminal: a survivor (for e-, above the demotion gate's energy)
+            if (!kPos && thr > 0.0 && k2 <= thr) break;
             double const end_r = hypot(x, y);
             if (!deposit_segment_check(A.G, pr, pz, end_r, z)) break;
             // ---- commit: nothing above has written anything ----
             slow = false;
-            deposit_segment(A.G, -w, dt, pr, pz, end_r, z);
+            deposit_segment(A.G, S::kCharge * w, dt, pr, pz, end_r, z);
             double const um = proper_velocity(k2);
             A.ux[ip] = um * d1x; A.uy[ip] = um * d1y; A.uz[ip] = um * d1z;
             A.r[ip] = end_r;
@@ -1272,8 +1323,7 @@ __global__ void __launch_bounds__(kBlock) electron_full_kernel(const StepArgs* _
     reduce_tallies(tally, A.tallies);
 }
 
-// The positron step: every particle through the complete path (positrons are
-// a few per cent of the charged population).
+// Every positron through the complete path (used when the fast path is off).
 __global__ void __launch_bounds__(kBlock) positron_full_kernel(const StepArgs* __restrict__ args)
 {
     StepArgs const& A = *args;
@@ -1341,6 +1391,8 @@ rrea_status rrea_tables_create(const rrea_table_image* im, rrea_tables_t* out)
     std::vector<double> cx(im->rate[0].x, im->rate[0].x + nc), clogx(nc);
     for (int64_t i = 0; i < nc; ++i) clogx[i] = std::log(cx[i]);  // Finalize :1650-1656
     size_t const o_cx = blob.add(cx), o_clogx = blob.add(clogx);
+    HostGuide const cguide = host_guide(clogx, kChargedGuideBins);
+    size_t const o_cguide = blob.add_i(cguide.idx);
     size_t o_cy[11], o_clogy[11];
     for (int t = 0; t < 11; ++t) {
         std::vector<double> y(im->rate[t].y, im->rate[t].y + nc), ly(nc);
@@ -1387,12 +1439,14 @@ rrea_status rrea_tables_create(const rrea_table_image* im, rrea_tables_t* out)
     };
     AngOff const og = add_angle(im->gs_electron), ow = add_angle(im->wentzel_electron);
     AngOff const ogp = add_angle(im->gs_positron), owp = add_angle(im->wentzel_positron);
-    struct ElOff { size_t e, le, mfp, eta; int32_t n; };
+    struct ElOff { size_t e, le, mfp, eta, guide; int32_t n; double glo, gscale; };
     auto add_elastic = [&](rrea_elastic_table const& el) {
         ElOff o;
         std::vector<double> e(el.e, el.e + el.n), le(el.n);
         for (int64_t i = 0; i < el.n; ++i) le[i] = std::log(e[i]);
         o.e = blob.add(e); o.le = blob.add(le);
+        HostGuide const g = host_guide(le, kElasticGuideBins);
+        o.guide = blob.add_i(g.idx); o.glo = g.lo; o.gscale = g.scale;
         o.mfp = blob.add(std::vector<double>(el.mfp_m_stp, el.mfp_m_stp + el.n));
         o.eta = blob.add(std::vector<double>(el.screening_eta, el.screening_eta + el.n));
         o.n = static_cast<int32_t>(el.n);
@@ -1408,6 +1462,7 @@ rrea_status rrea_tables_create(const rrea_table_image* im, rrea_tables_t* out)
     int32_t* const ii = T->d_i32;
     DevTables& V = T->view;
     V.cx = f + o_cx; V.clogx = f + o_clogx; V.nc = static_cast<int32_t>(nc);
+    V.cguide = Guide{ii + o_cguide, kChargedGuideBins, cguide.lo, cguide.scale};
     for (int t = 0; t < 11; ++t) { V.cy[t] = f + o_cy[t]; V.clogy[t] = f + o_clogy[t]; V.cmode[t] = im->rate[t].mode; }
     V.hm_mean = f + o_hm;
     auto set_cdf = [&](DevCdf& d, CdfOff const& o) {
@@ -1428,6 +1483,7 @@ rrea_status rrea_tables_create(const rrea_table_image* im, rrea_tables_t* out)
     set_angle(V.wz_p, owp);
     auto set_elastic = [&](DevElastic& d, ElOff const& o) {
         d.e = f + o.e; d.log_e = f + o.le; d.mfp = f + o.mfp; d.eta = f + o.eta; d.n = o.n;
+        d.guide = Guide{ii + o.guide, kElasticGuideBins, o.glo, o.gscale};
     };
     set_elastic(V.el_e, oel);
     set_elastic(V.el_p, oelp);
@@ -1486,7 +1542,7 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
     if (!F) return RREA_OK;
     void* ptrs[] = {F->d_cell_er, F->d_cell_ez, F->d_btheta, F->d_stage, F->d_prof, F->d_stage_sec, F->d_keys,
                     F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args,
-                    F->d_slow_list, F->d_slow_count};
+                    F->d_slow_list, F->d_slow_count, F->d_prof_guide};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : F->events) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
     delete F;
@@ -1503,7 +1559,7 @@ rrea_status rrea_kernel_timing(rrea_fields_t F, int32_t enable)
 
 rrea_status rrea_kernel_timing_read(rrea_fields_t F, int32_t kernel, double* total_ms, int64_t* launches)
 {
-    if (!F || !total_ms || !launches || kernel < -1 || kernel > 2) return RREA_ERR_ARGUMENT;
+    if (!F || !total_ms || !launches || kernel < -1 || kernel > 3) return RREA_ERR_ARGUMENT;
     double sum = 0.0;
     int64_t count = 0;
     for (size_t i = 0; i < F->events_used; ++i) {
@@ -1574,6 +1630,11 @@ rrea_status rrea_density_set_profile(rrea_fields_t F, const double* alt, const d
     HIP_TRY(hipMalloc(&F->d_prof, 2 * n * sizeof(double)));
     HIP_TRY(hipMemcpy(F->d_prof, alt, n * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(F->d_prof + n, rho, n * sizeof(double), hipMemcpyHostToDevice));
+    HostGuide const g = host_guide(std::vector<double>(alt, alt + n), kProfileGuideBins);
+    if (!F->d_prof_guide) HIP_TRY(hipMalloc(&F->d_prof_guide, kProfileGuideBins * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(F->d_prof_guide, g.idx.data(), kProfileGuideBins * sizeof(int32_t), hipMemcpyHostToDevice));
+    F->prof_guide_lo = g.lo;
+    F->prof_guide_scale = g.scale;
     F->prof_n = n;
     F->density_mode = 1;
     F->altitude_at_z0 = z0_alt;
@@ -1618,6 +1679,7 @@ rrea_status charged_step(bool positron, rrea_tables_t T, rrea_fields_t F, const 
     A.F.density_mode = F->density_mode; A.F.density_ratio = F->density_ratio;
     A.F.prof_alt = F->d_prof; A.F.prof_rho = F->d_prof ? F->d_prof + F->prof_n : nullptr;
     A.F.prof_n = static_cast<int32_t>(F->prof_n); A.F.altitude_at_z0 = F->altitude_at_z0;
+    A.F.prof_guide = Guide{F->d_prof_guide, kProfileGuideBins, F->prof_guide_lo, F->prof_guide_scale};
     A.F.reference_density = F->reference_density;
     A.D = Domain{g.r_lo, g.r_lo + g.nr * g.dr, g.z_lo, g.z_lo + g.nz * g.dz};
     // RreaParticleInteraction::Apply :29-35: the ratio at the domain's mid z.
@@ -1720,10 +1782,12 @@ rrea_status charged_step(bool positron, rrea_tables_t T, rrea_fields_t F, const 
     };
     const StepArgs* const d_args = static_cast<const StepArgs*>(F->d_args);
     rrea_status st = RREA_OK;
-    if (positron) {
-        st = timed(2, [&] { positron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args); });
-    } else if (cfg->complete_path_only) {
-        st = timed(0, [&] { electron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args); });
+    int32_t const kid = positron ? 2 : 0;   // timing ids: main kernel, then its hand-over kernel
+    if (cfg->complete_path_only) {
+        st = timed(kid, [&] {
+            if (positron) positron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
+            else electron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
+        });
     } else {
         if (F->slow_cap < tile->n) {
             if (F->d_slow_list) (void)hipFree(F->d_slow_list);
@@ -1734,15 +1798,25 @@ rrea_status charged_step(bool positron, rrea_tables_t T, rrea_fields_t F, const 
         }
         if (!F->d_slow_count) HIP_TRY(hipMalloc(&F->d_slow_count, sizeof(int32_t)));
         HIP_TRY(hipMemsetAsync(F->d_slow_count, 0, sizeof(int32_t), s));
-        st = timed(0, [&] {
-            electron_fast_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args, F->d_slow_list,
-                                                                                         F->d_slow_count);
+        st = timed(kid, [&] {
+            if (positron) {
+                charged_fast_kernel<true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
+                    d_args, F->d_slow_list, F->d_slow_count);
+            } else {
+                charged_fast_kernel<false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
+                    d_args, F->d_slow_list, F->d_slow_count);
+            }
         });
         if (st != RREA_OK) return st;
         int64_t const slow_blocks = nb < 2048 ? nb : 2048;
-        st = timed(1, [&] {
-            electron_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
-                d_args, F->d_slow_list, F->d_slow_count);
+        st = timed(kid + 1, [&] {
+            if (positron) {
+                charged_slow_kernel<true><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
+                    d_args, F->d_slow_list, F->d_slow_count);
+            } else {
+                charged_slow_kernel<false><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
+                    d_args, F->d_slow_list, F->d_slow_count);
+            }
         });
     }
     if (st != RREA_OK) return st;

@@ -44,6 +44,17 @@ def test_c2_positron_sample_parity(tables, otables):
                     species="positron")
 
 
+def test_positron_complete_path_only(tables, otables):
+    """The same kind of sample with every positron through the complete step:
+    the fast-path specialisation and the complete path give the oracle's results."""
+    fields = wl.c2_fields()
+    tile = positrons(1 << 13, fields, seed=36)
+    cfg = StepConfig(rng_seed=94, dt_s=1e-12, complete_path_only=True)
+    print("case c2 2^13 e+ complete path only x 2 steps", flush=True)
+    run_and_compare(otables, tables, coupling_for(fields, tables), fields, tile, cfg, steps=2, label="e+ full",
+                    species="positron")
+
+
 def test_positron_long_step_events(tables, otables):
     """dt = 1 ns: substeps, hard Bhabha, brems, in-flight annihilation, stops
     annihilating at rest mid-chord, GS multiple scattering."""
