@@ -64,7 +64,7 @@ RATE_KEYS = ["e_col", "e_soft", "e_moller", "e_brems", "e_eltr",
              "g_compton", "g_pe", "g_pairn", "g_pairt"]
 CDF_KEYS = ["moller", "bhabha", "ebrems", "pbrems"]
 ANGLE_KEYS = ["gs_e", "gs_p", "wz_e", "wz_p"]
-N_TALLIES = 16
+N_TALLIES = 24
 
 
 def build(quiet: bool = True) -> None:
@@ -130,6 +130,22 @@ def lib():
                           + [P_D] * 7 + [P_I64, P_I32] + [P_D] * 6 + [P_I32, P_D, P_I64]
                           + [C.c_int64, C.POINTER(Secondary), P_I64] + [P_D] * 4 + [C.c_int])
         L.orc_sample_annihilation.argtypes = [C.c_double, C.c_double, C.c_double, P_D]
+        L.orc_photon_step.restype = C.c_int
+        L.orc_photon_step.argtypes = ([C.c_void_p, C.POINTER(Geometry), C.POINTER(Config), C.c_int64]
+                                      + [P_D] * 10 + [P_I64, P_I32] + [P_D] * 6 + [P_I32, P_D, P_I64]
+                                      + [C.c_int64, C.POINTER(Secondary), P_I64, P_D, C.c_int])
+        L.orc_tables_set_photoelectric.argtypes = [C.c_void_p, P_D, C.c_int64, P_I64, P_D, P_D]
+        L.orc_tables_set_pair.argtypes = [C.c_void_p, P_I64, P_D, C.c_int64, P_I64, P_D, P_D]
+        L.orc_sample_compton.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, P_D]
+        L.orc_sample_compton.restype = C.c_int
+        L.orc_sample_photoelectric.argtypes = [C.c_void_p, C.c_double, C.c_double, P_D]
+        L.orc_sample_photoelectric.restype = C.c_int
+        L.orc_sample_photoelectron_cos.argtypes = [C.c_void_p, C.c_double, C.c_double, P_I32]
+        L.orc_sample_photoelectron_cos.restype = C.c_double
+        L.orc_sample_photon_channel.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_sample_photon_channel.restype = C.c_int
+        L.orc_sample_pair.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double, P_D]
+        L.orc_sample_pair.restype = C.c_int
         for name in ("orc_sample_bhabha", "orc_sample_positron_brems_energy"):
             f = getattr(L, name)
             f.restype = C.c_double
@@ -192,6 +208,14 @@ class Tables:
         for i, key in enumerate(["el_e", "el_p"]):
             arrs = [np.ascontiguousarray(bundle[key + s]) for s in ("_e", "_mfp", "_eta", "_trmfp")]
             L.orc_tables_set_elastic(self.h, i, *[_ptr(a) for a in arrs], len(arrs[0]))
+        pe = [np.ascontiguousarray(bundle[k]) for k in ("pe_primary", "pe_start", "pe_u", "pe_binding")]
+        self._keep += pe
+        L.orc_tables_set_photoelectric(self.h, _ptr(pe[0]), len(pe[0]), _ptr(pe[1], P_I64), _ptr(pe[2]), _ptr(pe[3]))
+        pb = [np.ascontiguousarray(bundle[k]) for k in ("pair_block_channel", "pair_block_energy", "pair_block_start",
+                                                        "pair_frac", "pair_dir")]
+        self._keep += pb
+        L.orc_tables_set_pair(self.h, _ptr(pb[0], P_I64), _ptr(pb[1]), len(pb[1]), _ptr(pb[2], P_I64), _ptr(pb[3]),
+                              _ptr(pb[4]))
         scalars = np.ascontiguousarray(bundle["scalars"], dtype=np.float64)
         code = L.orc_tables_finalize(self.h, _ptr(scalars))
         if code:
@@ -229,6 +253,37 @@ class Tables:
 
     def sample_positron_brems_energy(self, e, u):
         return self._call("orc_sample_positron_brems_energy", e, u)
+
+    def sample_compton(self, e, u1, u2):
+        out = np.zeros(3)
+        code = lib().orc_sample_compton(self.h, e, u1, u2, _ptr(out))
+        if code:
+            raise OracleError(f"orc_sample_compton failed with code {code}")
+        return {"scattered": out[0], "recoil": out[1], "cos": out[2]}
+
+    def sample_photoelectric(self, e, u):
+        out = np.zeros(2)
+        code = lib().orc_sample_photoelectric(self.h, e, u, _ptr(out))
+        if code:
+            raise OracleError(f"orc_sample_photoelectric failed with code {code}")
+        return {"binding": out[0], "electron": out[1]}
+
+    def sample_photoelectron_cos(self, ke, u):
+        return self._call("orc_sample_photoelectron_cos", ke, u)
+
+    def sample_photon_channel(self, e, u):
+        v = lib().orc_sample_photon_channel(self.h, e, u)
+        if v < 0:
+            raise OracleError(f"orc_sample_photon_channel failed with code {-v}")
+        return v
+
+    def sample_pair(self, e, channel, u_inv, u_block, u_az):
+        out = np.zeros(14)
+        code = lib().orc_sample_pair(self.h, e, channel, u_inv, u_block, u_az, _ptr(out))
+        if code:
+            raise OracleError(f"orc_sample_pair failed with code {code}")
+        return {"valid": bool(out[0]), "electron": out[1], "positron": out[2], "recoil": out[3], "local": out[4],
+                "e_dir": out[5:8].copy(), "p_dir": out[8:11].copy(), "r_dir": out[11:14].copy()}
 
     def sample_brems_cos(self, e, u):
         return self._call("orc_sample_brems_cos", e, u)
@@ -291,6 +346,36 @@ def make_config(cfg) -> Config:
     c.max_field_impulse_fraction = cfg.transport_max_field_impulse_fraction
     c.field_impulse_floor_eV = cfg.transport_field_impulse_floor_eV
     return c
+
+
+def photon_step(tables: Tables, fields, cfg, particles: dict, nthreads: int = 1, sec_cap: int | None = None):
+    """Run the oracle photon step.  ``particles`` holds prev r, z, theta, the
+    pushed (current) r_cur, z_cur, theta_cur, u = E dir (ux, uy, uz), w, id,
+    cpu.  Returns a dict of outputs (r, z, theta, ux, uy, uz written back)."""
+    L = lib()
+    n = len(particles["r"])
+    g, keep = make_geometry(fields)
+    c = make_config(cfg)
+    ins = [np.ascontiguousarray(particles[k], dtype=np.float64)
+           for k in ("r", "z", "theta", "r_cur", "z_cur", "theta_cur", "ux", "uy", "uz", "w")]
+    pid = np.ascontiguousarray(particles["id"], dtype=np.int64)
+    cpu = np.ascontiguousarray(particles["cpu"], dtype=np.int32)
+    outs = {k: np.zeros(n) for k in ("r", "z", "theta", "ux", "uy", "uz")}
+    status = np.zeros(n, dtype=np.int32)
+    ion = np.zeros(n)
+    events = np.zeros(n, dtype=np.int64)
+    cap = sec_cap if sec_cap is not None else max(1024, 4 * n)
+    sec = (Secondary * cap)()
+    nsec = C.c_int64(0)
+    tallies = np.zeros(N_TALLIES)
+    code = L.orc_photon_step(
+        tables.h, C.byref(g), C.byref(c), n, *[_ptr(a) for a in ins], _ptr(pid, P_I64), _ptr(cpu, P_I32),
+        *[_ptr(outs[k]) for k in ("r", "z", "theta", "ux", "uy", "uz")], _ptr(status, P_I32), _ptr(ion),
+        _ptr(events, P_I64), cap, sec, C.byref(nsec), _ptr(tallies), nthreads)
+    del keep
+    secs = np.frombuffer(bytes(sec), dtype=SECONDARY_DTYPE, count=cap)[: nsec.value].copy()
+    return {"code": code, **outs, "status": status, "ion_pairs": ion, "substeps": events, "secondaries": secs,
+            "tallies": tallies}
 
 
 def sample_annihilation(k: float, draw_energy: float, draw_azimuth: float) -> dict:

@@ -134,10 +134,28 @@ enum : uint64_t {
     kChPositronContinuationOpticalDepth = 126,
     kChPositronContinuationChannel = 127,
     kChPositronBhabhaAzimuth = 130,
+    kChPhotonInteraction = 51,
+    kChPhotonChannel = 52,
+    kChComptonRecoil = 53,
+    kChComptonScatter = 54,
+    kChPhotoelectric = 55,
+    kChPairAngle = 57,
+    kChComptonScatterAzimuth = 58,
+    kChPairAzimuth = 59,
+    kChPhotoelectricAngle = 60,
+    kChPhotoelectricAzimuth = 65,
+    kChPairInvariant = 66,
+    kChPairLeptonPolar = 67,
+    kChPairLeptonAzimuth = 68,
+    kChPhotonContinuationOpticalDepth = 128,
+    kChPhotonContinuationChannel = 129,
 };
 
 // RreaSecondaryProcess ids, OV/RreaParticleInteraction.H:34-48
-enum : int32_t { kProcHardMoller = 2, kProcHardBhabha = 3, kProcBrems = 4, kProcAnnihilation = 9 };
+enum : int32_t {
+    kProcHardMoller = 2, kProcHardBhabha = 3, kProcBrems = 4, kProcCompton = 5, kProcPhotoelectric = 6,
+    kProcPairNuclear = 7, kProcPairTriplet = 8, kProcAnnihilation = 9,
+};
 
 // OV/RreaParticleInteraction.H:642-710
 constexpr uint64_t kLineageCap = uint64_t{1} << 14;
@@ -388,6 +406,17 @@ struct Tables {
     double reference_density = 0.0;
     double charged_min = 0.0, charged_max = 0.0, photon_min = 0.0, photon_max = 0.0;
     double gs_few = 10.0, gs_iso = 1.0e5, gs_transition = 0.0;
+    // photoelectric subshell CDF groups (load_photoelectric_subshell_cdf :893-990)
+    std::vector<double> pe_primary;
+    std::vector<int64_t> pe_start;
+    std::vector<double> pe_u, pe_binding;
+    // pair final-state bank, training blocks by (channel, energy); records hold
+    // the pair electron, positron and triplet recoil (fraction, direction)
+    std::vector<int32_t> pair_channel;   // 1 nuclear, 2 triplet
+    std::vector<double> pair_energy;
+    std::vector<int64_t> pair_start;
+    std::vector<double> pair_frac;       // 3 per record
+    std::vector<double> pair_dir;        // 9 per record
 
     void require_charged(double e) const
     {
@@ -1447,6 +1476,82 @@ inline uint64_t select_material_subcycles(double max_tau, std::vector<Interval> 
     }
 }
 
+// The per-particle side-effect sink of one transport transaction: ion-pair
+// events (AccumulateIonizationEvent weights, OV/RreaTransportLog.H:449-462),
+// local deposits as E / W_air pairs (deposit_local_energy_as_ion_pairs
+// OV/RreaTransportStep.H:537-561) and secondaries with the parent's spawn
+// lineage counter (AddSecondary OV/RreaTransportLog.H:437-447).
+struct Emitter {
+    ChunkLog& fx;
+    int64_t ip;
+    uint64_t pkey;
+    double w;
+    double& ion;
+    uint64_t lineage = 0;
+
+    void ion_event(double ev_w)
+    {
+        ion += ev_w;
+        fx.tallies[ORC_T_ION_PAIRS] += ev_w;
+    }
+    void deposit_local(double energy)
+    {
+        if (w <= 0.0 || energy <= 0.0) return;
+        double const pw = w * energy / std::max(kWAir, 1.0);
+        if (pw <= 0.0) return;
+        ion_event(pw);
+    }
+    void secondary(int sp, int process, double x, double y, double z, double sdx, double sdy, double sdz,
+                   double energy, double birth)
+    {
+        orc_secondary s{ip, sp, process, x, y, z, sdx, sdy, sdz, w, energy, birth, pkey, lineage++};
+        fx.secondaries.push_back(s);
+    }
+};
+
+// spawn_positron_annihilation OV/RreaTransportStep.H:563-700: two photons
+// (tracked at or above the photon cutoff, else deposited locally), the
+// residual kinetic energy of an at-rest annihilation deposited as ion pairs,
+// and the positive ion the annihilated target electron leaves behind.  The
+// photons join the enclosing group, so they carry its process id.
+inline void spawn_annihilation(StepContext const& ctx, Emitter& em, int32_t process, double x, double y, double z,
+                               double ax, double ay, double az, double k, bool at_rest, uint64_t index, double birth)
+{
+    Tables const& t = *ctx.t;
+    double const gcut = ctx.cfg.photon_cutoff_eV;
+    uint64_t const step = static_cast<uint64_t>(std::max<int64_t>(ctx.cfg.step, 0));
+    if (em.w <= 0.0) return;
+    normalize(ax, ay, az);
+    double const ka = at_rest ? 0.0 : std::max(k, 0.0);
+    if (!(ka == 0.0) && (!std::isfinite(ka) || ka < t.charged_min || ka > t.charged_max)) {
+        fail(ORC_ERR_TABLE_RANGE, "annihilation energy outside certified range");
+    }
+    AnnihilationState const st = sample_annihilation(
+        ka, rng_uniform(ctx.cfg.rng_seed, em.pkey, step, index, kChPositronAnnihilationPolar),
+        rng_uniform(ctx.cfg.rng_seed, em.pkey, step, index, kChPositronAnnihilationAzimuth));
+    double g1x, g1y, g1z;
+    cone_direction(ax, ay, az, st.cos1, st.az1, g1x, g1y, g1z);
+    double const e1 = std::max(st.e1, 0.0);
+    double const e2 = std::max(st.e2, 0.0);
+    double g2x = -g1x, g2y = -g1y, g2z = -g1z;
+    if (!st.at_rest && e2 > 0.0) {
+        double const p = std::sqrt(std::max(0.0, ka * (ka + 2.0 * kMeC2)));
+        g2x = (p * ax - e1 * g1x) / e2;
+        g2y = (p * ay - e1 * g1y) / e2;
+        g2z = (p * az - e1 * g1z) / e2;
+        normalize(g2x, g2y, g2z);
+    }
+    double local = 0.0;
+    if (e1 >= gcut) em.secondary(2, process, x, y, z, g1x, g1y, g1z, e1, birth);
+    else local += e1;
+    if (e2 >= gcut) em.secondary(2, process, x, y, z, g2x, g2y, g2z, e2, birth);
+    else local += e2;
+    if (at_rest && k > 0.0) em.deposit_local(k);
+    if (local > 0.0) em.deposit_local(local);
+    em.fx.tallies[ORC_T_POSITIVE_IONS] += em.w;
+    em.fx.tallies[ORC_T_ANNIHILATIONS] += 1.0;
+}
+
 // mean_free_path_from_inverse_length NA/src/RreaInteractionTables.cpp:110-119
 // followed by RreaChannelInverseLength OV/RreaChargedInterleave.H:69-78
 inline double channel_inverse_length(double inverse_per_m)
@@ -1606,18 +1711,12 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
     double particle_ratio = 0.0;
     if (!pos) particle_ratio = ds > 0.0 ? chord_dp / ds : positive_or(ctx.cp.density_ratio_at_z(z_particle), ctx.density_ratio);
     double tr_dp = 0.0, tr_len = 0.0;
-    uint64_t lineage = 0;  // spawn_transient_lineage, OV/RreaTransportLog.H:179,212,445
     Path path = dpath;
     std::vector<Interval> iv = div;
 
-    auto ion_event = [&](double ev_w) { out.ion_pairs += ev_w; fx.tallies[ORC_T_ION_PAIRS] += ev_w; };
-    // deposit_local_energy_as_ion_pairs OV/RreaTransportStep.H:537-561
-    auto deposit_local = [&](double energy) {
-        if (w <= 0.0 || energy <= 0.0) return;
-        double const pw = w * energy / std::max(kWAir, 1.0);
-        if (pw <= 0.0) return;
-        ion_event(pw);
-    };
+    Emitter em{fx, ip, pkey, w, out.ion_pairs};
+    auto ion_event = [&](double ev_w) { em.ion_event(ev_w); };
+    auto deposit_local = [&](double energy) { em.deposit_local(energy); };
     // the net chord carries the species' signed charge (-w for e-, +w for e+)
     auto segment = [&](double r0, double z0, double r1, double z1) {
         DepositOp op{0, {r0, z0, r1, z1, charge * w, 0.0, 0.0}, 0U};
@@ -1644,44 +1743,10 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
     };
     auto push_secondary = [&](int sp, int process, double x, double y, double z, double sdx, double sdy, double sdz,
                               double energy, double birth) {
-        orc_secondary s{ip, sp, process, x, y, z, sdx, sdy, sdz, w, energy, birth, pkey, lineage++};
-        fx.secondaries.push_back(s);
+        em.secondary(sp, process, x, y, z, sdx, sdy, sdz, energy, birth);
     };
-    // spawn_positron_annihilation OV/RreaTransportStep.H:563-700: two photons
-    // (tracked at or above the photon cutoff, else deposited locally), the
-    // residual kinetic energy of an at-rest annihilation deposited as ion pairs,
-    // and the positive ion the annihilated target electron leaves behind.
     auto annihilate = [&](double x, double y, double z, bool at_rest, uint64_t index, double birth) {
-        if (w <= 0.0) return;
-        double ax = dx, ay = dy, az = dz;
-        normalize(ax, ay, az);
-        double const ka = at_rest ? 0.0 : std::max(k, 0.0);
-        if (!(ka == 0.0) && (!std::isfinite(ka) || ka < t.charged_min || ka > t.charged_max)) {
-            fail(ORC_ERR_TABLE_RANGE, "annihilation energy outside certified range");
-        }
-        AnnihilationState const st = sample_annihilation(ka, rng_uniform(seed, pkey, step, index, kChPositronAnnihilationPolar),
-                                                         rng_uniform(seed, pkey, step, index, kChPositronAnnihilationAzimuth));
-        double g1x, g1y, g1z;
-        cone_direction(ax, ay, az, st.cos1, st.az1, g1x, g1y, g1z);
-        double const e1 = std::max(st.e1, 0.0);
-        double const e2 = std::max(st.e2, 0.0);
-        double g2x = -g1x, g2y = -g1y, g2z = -g1z;
-        if (!st.at_rest && e2 > 0.0) {
-            double const p = std::sqrt(std::max(0.0, ka * (ka + 2.0 * kMeC2)));
-            g2x = (p * ax - e1 * g1x) / e2;
-            g2y = (p * ay - e1 * g1y) / e2;
-            g2z = (p * az - e1 * g1z) / e2;
-            normalize(g2x, g2y, g2z);
-        }
-        double local = 0.0;
-        if (e1 >= gcut) push_secondary(2, kProcAnnihilation, x, y, z, g1x, g1y, g1z, e1, birth);
-        else local += e1;
-        if (e2 >= gcut) push_secondary(2, kProcAnnihilation, x, y, z, g2x, g2y, g2z, e2, birth);
-        else local += e2;
-        if (at_rest && k > 0.0) deposit_local(k);
-        if (local > 0.0) deposit_local(local);
-        fx.tallies[ORC_T_POSITIVE_IONS] += w;
-        fx.tallies[ORC_T_ANNIHILATIONS] += 1.0;
+        spawn_annihilation(ctx, em, kProcAnnihilation, x, y, z, dx, dy, dz, k, at_rest, index, birth);
     };
     auto annihilated_at = [&](double r, double z) {
         segment(r_particle, z_particle, r, z);
@@ -2051,6 +2116,476 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
     fx.tallies[ORC_T_SURVIVORS] += 1.0;
 }
 
+// ---------------------------------------------------------------------------
+// Photon step -- OV/RreaPhotonTransport.cpp:9-461, OV/RreaPhotonFinalStates.H,
+// samplers NA/src/RreaInteractionTables.cpp:2272-2296, 2762-3127
+// ---------------------------------------------------------------------------
+enum PhotonChannel { kCompton = 0, kPhotoelectric = 1, kPairNuclear = 2, kPairTriplet = 3 };
+
+struct PhotonRates { double compton, pe, pair_n, pair_t; };
+
+// PhotonProcessInverseLengths :2272-2296 (density ratio 1)
+inline PhotonRates photon_rates(Tables const& t, double e)
+{
+    if (!std::isfinite(e) || e < t.photon_min || e > t.photon_max) {
+        fail(ORC_ERR_TABLE_RANGE, "schema-6 photon energy outside certified range");
+    }
+    return {t.rate[kGCompton].interpolate(e) * 1.0, t.rate[kGPe].interpolate(e) * 1.0,
+            t.rate[kGPairN].interpolate(e) * 1.0, t.rate[kGPairT].interpolate(e) * 1.0};
+}
+
+inline double photon_total(PhotonRates const& r) { return r.compton + r.pe + r.pair_n + r.pair_t; }
+
+// PhotonFeedbackMeanFreePath :2762-2771
+inline double photon_mfp(Tables const& t, double e)
+{
+    double const v = photon_total(photon_rates(t, e));
+    if (!std::isfinite(v) || v < 0.0) fail(ORC_ERR_INVALID, "photon inverse length invalid");
+    return v > 0.0 ? 1.0 / v : 1.0e99;
+}
+
+// SamplePhotonChannel :2778-2804
+inline int sample_photon_channel(Tables const& t, double e, double u)
+{
+    if (!std::isfinite(e) || e < t.photon_min || e > t.photon_max) {
+        fail(ORC_ERR_TABLE_RANGE, "photon channel energy outside certified range");
+    }
+    u = std::max(0.0, std::min(1.0, u));
+    PhotonRates const r = photon_rates(t, e);
+    double const total = photon_total(r);
+    if (!(total > 0.0) || !std::isfinite(total)) fail(ORC_ERR_INVALID, "photon inverse lengths sum to zero");
+    double const target = u * total;
+    if (target < r.compton) return kCompton;
+    if (target < r.compton + r.pe) return kPhotoelectric;
+    if (target < r.compton + r.pe + r.pair_n) return kPairNuclear;
+    return kPairTriplet;
+}
+
+struct ComptonState { double scattered, recoil, cos_theta; };
+
+// SampleComptonKleinNishinaFinalState :2806-2871: Butcher-Messel rejection,
+// round 0 on the two drawn uniforms, later draws from the uniform stream
+// seeded by them
+inline ComptonState sample_compton(Tables const& t, double e, double draw_mu, double draw_accept)
+{
+    if (!std::isfinite(e) || e < t.photon_min || e > t.photon_max) {
+        fail(ORC_ERR_TABLE_RANGE, "Compton energy outside certified range");
+    }
+    if (e <= 0.0) fail(ORC_ERR_INVALID, "Compton sampler requires positive photon energy");
+    draw_mu = std::max(0.0, std::min(1.0, draw_mu));
+    draw_accept = std::max(0.0, std::min(1.0, draw_accept));
+    UniformStream stream(draw_mu, draw_accept);
+    double const alpha = e / kMeC2;
+    double const eps0 = 1.0 / (1.0 + 2.0 * alpha);
+    double const eps0sq = eps0 * eps0;
+    double const a1 = -std::log(eps0);
+    double const a2 = 0.5 * (1.0 - eps0sq);
+    double epsilon = eps0;
+    double onecost = 0.0;
+    for (int iter = 0; iter < 128; ++iter) {
+        double const u1 = iter == 0 ? draw_mu : stream.next();
+        double const u2 = iter == 0 ? draw_accept : stream.next();
+        double epsilonsq;
+        if (a1 > (a1 + a2) * u1) {
+            epsilon = std::exp(-a1 * u2);
+            epsilonsq = epsilon * epsilon;
+        } else {
+            epsilonsq = eps0sq + (1.0 - eps0sq) * u2;
+            epsilon = std::sqrt(epsilonsq);
+        }
+        onecost = (1.0 - epsilon) / (epsilon * alpha);
+        double const sint2 = onecost * (2.0 - onecost);
+        double const greject = 1.0 - epsilon * sint2 / (1.0 + epsilonsq);
+        if (greject >= stream.next()) break;
+    }
+    double mu = 1.0 - onecost;
+    mu = std::max(-1.0, std::min(1.0, mu));
+    ComptonState st;
+    st.scattered = std::max(0.0, epsilon * e);
+    st.recoil = std::max(0.0, e - st.scattered);
+    st.cos_theta = mu;
+    return st;
+}
+
+struct PhotoelectricState { double binding, electron; };
+
+// SamplePhotoelectricFinalState :2878-2950
+inline PhotoelectricState sample_photoelectric(Tables const& t, double e, double u)
+{
+    if (!std::isfinite(e) || e < t.photon_min || e > t.photon_max) {
+        fail(ORC_ERR_TABLE_RANGE, "photoelectric energy outside certified range");
+    }
+    if (e <= 0.0) fail(ORC_ERR_INVALID, "photoelectric sampler requires positive photon energy");
+    if (t.pe_primary.empty() || e < t.pe_primary.front() || e > t.pe_primary.back()) {
+        fail(ORC_ERR_TABLE_RANGE, "photoelectric subshell primary energy out of range");
+    }
+    u = std::max(0.0, std::min(1.0, u));
+    size_t const n = t.pe_primary.size();
+    size_t const up = static_cast<size_t>(std::lower_bound(t.pe_primary.begin(), t.pe_primary.end(), e)
+                                          - t.pe_primary.begin());
+    size_t g;
+    if (up == 0) g = 0;
+    else if (up == n) g = n - 1;
+    else if (t.pe_primary[up] == e) g = up;  // the low side at a duplicated edge
+    else g = up - 1;
+    auto const b = t.pe_u.begin() + t.pe_start[g], en = t.pe_u.begin() + t.pe_start[g + 1];
+    auto const row_it = std::lower_bound(b, en, u);
+    size_t const row = static_cast<size_t>((row_it == en ? en - 1 : row_it) - t.pe_u.begin());
+    double const binding = t.pe_binding[row];
+    if (binding > e) fail(ORC_ERR_INVALID, "photoelectric binding exceeds photon energy");
+    return {binding, std::max(0.0, e - binding)};
+}
+
+// SamplePhotoelectronCosTheta :2957-3044: Sauter-Gavrila K-shell polar angle
+// by bisection of its analytic primitive in t = 1 - beta mu, in long double
+inline double sample_photoelectron_cos(Tables const& t, double ke, double draw)
+{
+    if (!(ke == 0.0) && (!std::isfinite(ke) || ke < t.charged_min || ke > t.charged_max)) {
+        fail(ORC_ERR_TABLE_RANGE, "photoelectron energy outside certified range");
+    }
+    long double const u = static_cast<long double>(clamp_unit(draw));
+    if (u <= 0.0L) return -1.0;
+    if (u >= 1.0L) return 1.0;
+    long double const gamma = 1.0L + static_cast<long double>(ke) / static_cast<long double>(kMeC2);
+    long double const inv_gamma_sq = 1.0L / (gamma * gamma);
+    long double const beta = std::sqrt(std::max(0.0L, 1.0L - inv_gamma_sq));
+    if (beta < 1.0e-4L) {
+        long double lo = -1.0L, hi = 1.0L;
+        for (int it = 0; it < 96; ++it) {
+            long double const mid = 0.5L * (lo + hi);
+            if (mid == lo || mid == hi) break;
+            long double const cdf = (2.0L + mid * (3.0L - mid * mid)) / 4.0L;
+            if (cdf < u) lo = mid; else hi = mid;
+        }
+        return static_cast<double>(0.5L * (lo + hi));
+    }
+    long double const b = 0.5L * gamma * (gamma - 1.0L) * (gamma - 2.0L);
+    long double const c = -inv_gamma_sq;
+    long double const t_lo = inv_gamma_sq / (1.0L + beta);
+    long double const t_hi = 1.0L + beta;
+    auto primitive = [c, b](long double x) {
+        return -c / (3.0L * x * x * x) - (2.0L + b * c) / (2.0L * x * x) - (2.0L * b - 1.0L) / x - b * std::log(x);
+    };
+    long double const f_lo = primitive(t_lo);
+    long double const f_hi = primitive(t_hi);
+    long double const total = f_hi - f_lo;
+    if (!(total > 0.0L) || !std::isfinite(total)) fail(ORC_ERR_INVALID, "Sauter-Gavrila normalization invalid");
+    long double const target = (1.0L - u) * f_hi + u * f_lo;
+    long double lo = t_lo, hi = t_hi;
+    for (int it = 0; it < 128; ++it) {
+        long double const mid = 0.5L * (lo + hi);
+        if (mid == lo || mid == hi) break;
+        if (primitive(mid) < target) lo = mid; else hi = mid;
+    }
+    long double const tt = 0.5L * (lo + hi);
+    long double const mu = std::max(-1.0L, std::min(1.0L, (1.0L - tt) / beta));
+    return static_cast<double>(mu);
+}
+
+struct PairState {
+    bool valid = false;
+    double electron = 0, positron = 0, recoil = 0, local = 0;
+    double edx = 0, edy = 0, edz = 0, pdx = 0, pdy = 0, pdz = 0, rdx = 0, rdy = 0, rdz = 0;
+    bool has_recoil = false;
+};
+
+// normalize_local_direction NA/src/RreaInteractionTables.cpp:121-133
+inline void normalize_local(double& x, double& y, double& z)
+{
+    double const n = std::sqrt(x * x + y * y + z * z);
+    if (n <= 0.0 || !std::isfinite(n)) { x = 0.0; y = 0.0; z = 1.0; return; }
+    x /= n; y /= n; z /= n;
+}
+
+// SamplePairFinalState :3053-3127 + BuildRreaPairEmpiricalFinalState
+// :1384-1525: replay of one training record of the bracketing energy block
+inline PairState sample_pair(Tables const& t, double e, int channel, double draw_invariant, double draw_block,
+                             double draw_azimuth)
+{
+    if (!std::isfinite(e) || e < t.photon_min || e > t.photon_max) {
+        fail(ORC_ERR_TABLE_RANGE, "pair energy outside certified range");
+    }
+    PairState st;
+    bool const triplet = channel == kPairTriplet;
+    double const threshold = triplet ? 4.0 * kMeC2 : 2.0 * kMeC2;  // RreaPairChannelHasPhaseSpace :1373-1382
+    if (!(std::isfinite(e) && e > threshold)) return st;
+    int32_t const code = triplet ? 2 : 1;
+    std::vector<size_t> blocks;
+    for (size_t i = 0; i < t.pair_channel.size(); ++i) if (t.pair_channel[i] == code) blocks.push_back(i);
+    if (blocks.empty()) fail(ORC_ERR_INVALID, "pair channel has no training blocks");
+    size_t sel = blocks.front();
+    if (e >= t.pair_energy[blocks.back()]) {
+        sel = blocks.back();
+    } else if (e > t.pair_energy[blocks.front()]) {
+        size_t up = 0;
+        while (t.pair_energy[blocks[up]] < e) ++up;  // lower_bound over the channel's blocks
+        if (t.pair_energy[blocks[up]] == e) {
+            sel = blocks[up];
+        } else {
+            double const lo_e = t.pair_energy[blocks[up - 1]], hi_e = t.pair_energy[blocks[up]];
+            double const tt = (std::log(e) - std::log(lo_e)) / (std::log(hi_e) - std::log(lo_e));
+            sel = clamp_unit(draw_block) < tt ? blocks[up] : blocks[up - 1];
+        }
+    }
+    int64_t const size = t.pair_start[sel + 1] - t.pair_start[sel];
+    if (size <= 0) fail(ORC_ERR_INVALID, "selected pair block is empty");
+    size_t const rec_in_block = std::min(static_cast<size_t>(size - 1),
+                                         static_cast<size_t>(clamp_unit(draw_invariant) * static_cast<double>(size)));
+    size_t const rec = static_cast<size_t>(t.pair_start[sel]) + rec_in_block;
+    double const* f = &t.pair_frac[3 * rec];
+    double const* d = &t.pair_dir[9 * rec];
+    double const available = e - 2.0 * kMeC2;
+    double const rotation = kTwoPi * clamp_unit(draw_azimuth);
+    double const cr = std::cos(rotation), sr = std::sin(rotation);
+    auto rotated = [&](double const* v, double& x, double& y, double& z) {
+        x = cr * v[0] - sr * v[1];
+        y = sr * v[0] + cr * v[1];
+        z = v[2];
+        normalize_local(x, y, z);
+    };
+    st.valid = true;
+    st.electron = available * f[0];
+    st.positron = available * f[1];
+    rotated(d + 0, st.edx, st.edy, st.edz);
+    rotated(d + 3, st.pdx, st.pdy, st.pdz);
+    if (triplet) {
+        st.has_recoil = true;
+        st.recoil = available * f[2];
+        rotated(d + 6, st.rdx, st.rdy, st.rdz);
+    }
+    double const tol = 64.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, available);
+    auto sum = [&]() { return st.electron + st.positron + st.recoil; };
+    double expl = sum();
+    if (!std::isfinite(expl) || st.electron < 0.0 || st.positron < 0.0 || st.recoil < 0.0) {
+        fail(ORC_ERR_INVALID, "pair record produced a negative/nonfinite energy");
+    }
+    if (expl > available) {
+        if (expl - available > tol) fail(ORC_ERR_INVALID, "pair record failed energy closure");
+        double const scale = available / expl;
+        st.electron *= scale;
+        st.positron *= scale;
+        st.recoil *= scale;
+        expl = sum();
+    }
+    st.local = available - expl;
+    auto reduce_largest = [&](double correction) {
+        double* e3[3] = {&st.electron, &st.positron, &st.recoil};
+        double** largest = std::max_element(e3, e3 + 3, [](double const* a, double const* b) { return *a < *b; });
+        if (**largest < correction) fail(ORC_ERR_INVALID, "pair record cannot close nonnegative remainder");
+        **largest -= correction;
+    };
+    if (st.local < 0.0) {
+        double const correction = -st.local;
+        if (correction > tol) fail(ORC_ERR_INVALID, "pair record remainder is negative");
+        reduce_largest(correction);
+        st.local = 0.0;
+    }
+    double accounted = sum() + st.local;
+    double residual = available - accounted;
+    if (std::abs(residual) > tol) fail(ORC_ERR_INVALID, "pair record failed energy closure");
+    if (residual >= 0.0) st.local += residual;
+    else if (st.local >= -residual) st.local += residual;
+    else reduce_largest(-residual);
+    accounted = sum() + st.local;
+    if (st.electron < 0.0 || st.positron < 0.0 || st.recoil < 0.0 || st.local < 0.0
+        || std::abs(available - accounted) > tol) {
+        fail(ORC_ERR_INVALID, "pair record closure correction failed");
+    }
+    return st;
+}
+
+// RreaRotateLocalDirectionToAxis OV/RreaChargedInterleave.H:498-517
+inline void rotate_to_axis(double ax, double ay, double az, double lx, double ly, double lz, double& ox, double& oy,
+                           double& oz)
+{
+    normalize(ax, ay, az);
+    normalize(lx, ly, lz);
+    double b1x, b1y, b1z, b2x, b2y, b2z;
+    transverse_basis(ax, ay, az, b1x, b1y, b1z, b2x, b2y, b2z);
+    ox = lx * b1x + ly * b2x + lz * ax;
+    oy = lx * b1y + ly * b2y + lz * ay;
+    oz = lx * b1z + ly * b2z + lz * az;
+    normalize(ox, oy, oz);
+}
+
+// One photon: the body of the particle loop of OV/RreaPhotonTransport.cpp:56-438.
+// The chord is the captured pre-push point to WarpX's pushed point; the loop
+// samples one exponential depth over the remaining chord per pass, and a
+// Compton survivor continues from its interaction point along the scattered
+// direction for the unused path length.
+inline void photon_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, double pr, double pth, double pz,
+                            double cr, double cth, double cz, double ux, double uy, double uz, double w, int64_t id,
+                            int32_t cpu, ChargedOut& out)
+{
+    Tables const& t = *ctx.t;
+    orc_config const& cfg = ctx.cfg;
+    double const cutoff = cfg.low_energy_cutoff_eV;
+    double const gcut = cfg.photon_cutoff_eV;
+    double const dt = cfg.dt_s;
+    uint64_t const step = static_cast<uint64_t>(std::max<int64_t>(cfg.step, 0));
+    uint64_t const seed = cfg.rng_seed;
+    uint64_t const pkey = static_cast<uint64_t>(std::llabs(id)) ^ (static_cast<uint64_t>(static_cast<uint32_t>(cpu)) << 40);
+    out.ion_pairs = 0.0;
+    out.substeps = 0;
+    out.r = cr; out.z = cz; out.th = cth; out.ux = ux; out.uy = uy; out.uz = uz;
+    Emitter em{fx, ip, pkey, w, out.ion_pairs};
+
+    double energy = std::sqrt(ux * ux + uy * uy + uz * uz);
+    double dx = ux, dy = uy, dz = uz;
+    normalize(dx, dy, dz);
+    if (pr > ctx.r_hi + 1.0e-9 * ctx.r_hi || pz < ctx.z_lo - 1.0e-9 || pz > ctx.z_hi + 1.0e-9 * ctx.z_hi) {
+        fail(ORC_ERR_OUTSIDE_DOMAIN, "photon survived outside the domain");
+    }
+    Path path = clip_rz_path(pr, pth, pz, cr, cth, cz, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
+    uint64_t ordinal = 0;
+    double elapsed_start = 0.0;
+    for (;;) {
+        if (ordinal >= kEventCap) fail(ORC_ERR_INVALID, "photon event ordinal overflow");
+        EscapePolicy const escape = clipped_escape_policy(path, std::hypot(path.ex, path.ey), path.ez);
+        double const ds = path.in_length;
+        std::vector<Interval> const iv = build_intervals(ctx.cp, path, ctx.density_ratio);
+        double const chord_dp = density_path(iv, path, path.begin_f, path.end_f);
+        if (w <= 0.0 || energy < gcut) {
+            // a photon below the production cutoff deposits at its path start
+            double x, y, z;
+            path.point_at_full(path.begin_f, x, y, z);
+            em.deposit_local(energy);
+            out.status = w <= 0.0 ? ORC_ZERO_WEIGHT : ORC_DEMOTED;
+            if (w > 0.0) fx.tallies[ORC_T_DEMOTIONS] += 1.0;
+            return;
+        }
+        auto commit_no_event = [&]() {
+            double x, y, z;
+            path.point_at_full(path.end_f, x, y, z);
+            out.r = std::hypot(x, y);
+            out.th = std::atan2(y, x);
+            out.z = z;
+            if (escape.escapes) {
+                out.status = ORC_ESCAPE;
+                fx.tallies[ORC_T_ESCAPES] += 1.0;
+            } else {
+                out.status = ORC_SURVIVE;
+                fx.tallies[ORC_T_SURVIVORS] += 1.0;
+            }
+        };
+        if (ds <= 0.0) { commit_no_event(); return; }
+        double const mfp = photon_mfp(t, energy);
+        double const tau = chord_dp / mfp;
+        uint64_t idx = 0;
+        if (!live_hazard_index(2U, 0U, ordinal, idx)) fail(ORC_ERR_INVALID, "photon hazard RNG index overflow");
+        double const draw = rng_uniform(seed, pkey, step, idx,
+                                        ordinal == 0U ? kChPhotonInteraction : kChPhotonContinuationOpticalDepth);
+        double const taus[3] = {tau, 0.0, 0.0};
+        Hazard const h = select_hazard(taus, 1, draw, 0.0);
+        if (!h.event) { commit_no_event(); return; }
+        double const f = fraction_at_density_path(iv, path, path.begin_f, path.end_f, h.sampled * mfp);
+        double const dist = path.full_length * (f - path.begin_f);
+        double x, y, z;
+        path.point_at_full(f, x, y, z);
+        double const remaining = std::max(0.0, path.full_length - dist);
+        double const birth = std::min(1.0, elapsed_start + dist / std::max(kC * dt, std::numeric_limits<double>::min()));
+        double const r_site = std::hypot(x, y);
+        int const channel = sample_photon_channel(
+            t, energy, rng_uniform(seed, pkey, step, idx, ordinal == 0U ? kChPhotonChannel : kChPhotonContinuationChannel));
+        auto U = [&](uint64_t chan) { return rng_uniform(seed, pkey, step, idx, chan); };
+        out.substeps = ordinal + 1U;
+        if (channel == kCompton) {
+            // RreaApplyComptonFinalState OV/RreaPhotonFinalStates.H:54-140
+            double const incident = energy;
+            ComptonState const st = sample_compton(t, incident, U(kChComptonRecoil), U(kChComptonScatter));
+            double sx, sy, sz;
+            cone_direction(dx, dy, dz, st.cos_theta, U(kChComptonScatterAzimuth), sx, sy, sz);
+            double rx = incident * dx - st.scattered * sx;
+            double ry = incident * dy - st.scattered * sy;
+            double rz = incident * dz - st.scattered * sz;
+            normalize(rx, ry, rz);
+            if (st.recoil >= cutoff) {
+                em.secondary(0, kProcCompton, x, y, z, rx, ry, rz, st.recoil, birth);
+                em.ion_event(w);
+            } else {
+                em.ion_event(w);
+                em.deposit_local(st.recoil);
+            }
+            fx.tallies[ORC_T_COMPTON_EVENTS] += 1.0;
+            energy = st.scattered;
+            if (energy < gcut) {
+                em.deposit_local(energy);
+                out.status = ORC_ABSORBED;
+                return;
+            }
+            dx = sx; dy = sy; dz = sz;
+            out.ux = energy * dx; out.uy = energy * dy; out.uz = energy * dz;
+            double const xt = x + dx * remaining, yt = y + dy * remaining, zt = z + dz * remaining;
+            Path const cont = clip_rz_path(r_site, std::atan2(y, x), z, std::hypot(xt, yt), std::atan2(yt, xt), zt,
+                                           ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
+            double xn, yn, zn;
+            cont.point_at_full(cont.end_f, xn, yn, zn);
+            out.r = std::hypot(xn, yn);
+            out.th = std::atan2(yn, xn);
+            out.z = zn;
+            if (!(remaining > 0.0)) {
+                out.status = ORC_SURVIVE;
+                fx.tallies[ORC_T_SURVIVORS] += 1.0;
+                return;
+            }
+            path = cont;
+            elapsed_start = birth;
+            ++ordinal;
+            continue;
+        }
+        if (channel == kPhotoelectric) {
+            // RreaApplyPhotoelectricFinalState OV/RreaPhotonFinalStates.H:153-224
+            PhotoelectricState const st = sample_photoelectric(t, energy, U(kChPhotoelectric));
+            double const ke = std::min(st.electron, energy);
+            if (ke >= cutoff) {
+                double ex, ey, ez;
+                cone_direction(dx, dy, dz, sample_photoelectron_cos(t, ke, U(kChPhotoelectricAngle)),
+                               U(kChPhotoelectricAzimuth), ex, ey, ez);
+                em.secondary(0, kProcPhotoelectric, x, y, z, ex, ey, ez, ke, birth);
+                em.ion_event(w);
+            } else {
+                em.ion_event(w);
+                em.deposit_local(ke);
+            }
+            if (st.binding > 0.0) em.deposit_local(st.binding);
+            fx.tallies[ORC_T_PHOTOELECTRIC_EVENTS] += 1.0;
+            out.status = ORC_ABSORBED;
+            return;
+        }
+        // RreaApplyPairFinalState OV/RreaPhotonFinalStates.H:239-363
+        PairState const pair = sample_pair(t, energy, channel, U(kChPairInvariant), U(kChPairAngle), U(kChPairAzimuth));
+        if (!pair.valid) fail(ORC_ERR_INVALID, "pair channel without valid final-state phase space");
+        bool const triplet = channel == kPairTriplet;
+        if (triplet && !pair.has_recoil) fail(ORC_ERR_INVALID, "triplet final state without its recoil");
+        int32_t const process = triplet ? kProcPairTriplet : kProcPairNuclear;
+        double ex, ey, ez, px, py, pz2;
+        rotate_to_axis(dx, dy, dz, pair.edx, pair.edy, pair.edz, ex, ey, ez);
+        rotate_to_axis(dx, dy, dz, pair.pdx, pair.pdy, pair.pdz, px, py, pz2);
+        if (pair.electron >= cutoff) {
+            em.secondary(0, process, x, y, z, ex, ey, ez, pair.electron, birth);
+        } else {
+            fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
+            em.deposit_local(pair.electron);
+        }
+        if (pair.positron >= cutoff) {
+            em.secondary(1, process, x, y, z, px, py, pz2, pair.positron, birth);
+        } else {
+            spawn_annihilation(ctx, em, process, x, y, z, px, py, pz2, pair.positron, true, idx, birth);
+        }
+        if (pair.has_recoil) {
+            double qx, qy, qz;
+            rotate_to_axis(dx, dy, dz, pair.rdx, pair.rdy, pair.rdz, qx, qy, qz);
+            if (pair.recoil >= cutoff) em.secondary(0, process, x, y, z, qx, qy, qz, pair.recoil, birth);
+            em.ion_event(w);
+            if (pair.recoil < cutoff) em.deposit_local(pair.recoil);
+        }
+        if (pair.local > 0.0) em.deposit_local(pair.local);
+        fx.tallies[ORC_T_PAIR_EVENTS] += 1.0;
+        out.status = ORC_ABSORBED;
+        return;
+    }
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------
@@ -2373,6 +2908,160 @@ int orc_positron_step(void* h, const orc_geometry* g, const orc_config* cfg, int
     return charged_step(true, h, g, cfg, n, pr, pz, pth, pux, puy, puz, w, id, cpu, out_r, out_z, out_th, out_ux,
                         out_uy, out_uz, status, ion_pairs, substeps, sec_cap, sec, n_sec, jr, jz, rho, tallies,
                         nthreads);
+}
+
+// One photon transport step over n photons.  prev_* is the captured
+// pre-push point, cur_* WarpX's pushed point (the chord end), u = E * dir.
+// out_* receives the written-back state (cur_* and u where nothing is written).
+int orc_photon_step(void* h, const orc_geometry* g, const orc_config* cfg, int64_t n, const double* pr,
+                    const double* pz, const double* pth, const double* cr, const double* cz, const double* cth,
+                    const double* ux, const double* uy, const double* uz, const double* w, const int64_t* id,
+                    const int32_t* cpu, double* out_r, double* out_z, double* out_th, double* out_ux,
+                    double* out_uy, double* out_uz, int32_t* status, double* ion_pairs, int64_t* events,
+                    int64_t sec_cap, orc_secondary* sec, int64_t* n_sec, double* tallies, int nthreads)
+{
+    auto const* t = static_cast<Tables const*>(h);
+    if (g->r_lo != 0.0) return ORC_ERR_INVALID;
+    StepContext ctx;
+    ctx.t = t;
+    ctx.cp = Coupling{*g, t->reference_density, 0.0, 0.0};
+    ctx.cfg = *cfg;
+    ctx.r_lo = g->r_lo;
+    ctx.r_hi = g->r_lo + g->nr * g->dr;
+    ctx.z_lo = g->z_lo;
+    ctx.z_hi = g->z_lo + g->nz * g->dz;
+    ctx.cp.r_hi = ctx.r_hi;
+    ctx.cp.z_hi = ctx.z_hi;
+    try {
+        ctx.density_ratio = positive_or(ctx.cp.density_ratio_at_z(0.5 * (ctx.z_lo + ctx.z_hi)), 1.0);
+    } catch (OracleError const& e) {
+        *n_sec = 0;
+        return e.code;
+    }
+    if (!(cfg->dt_s > 0.0)) { *n_sec = 0; return 0; }
+    constexpr int64_t chunk = 64;
+    int64_t const nchunks = (n + chunk - 1) / chunk;
+    std::vector<ChunkLog> logs(static_cast<size_t>(std::max<int64_t>(nchunks, 0)));
+    std::atomic<int> error{0};
+#ifdef _OPENMP
+    int const threads = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) if (threads > 1 && nchunks > 1)
+#endif
+    for (int64_t c = 0; c < nchunks; ++c) {
+        ChunkLog& fx = logs[static_cast<size_t>(c)];
+        int64_t const b = c * chunk, e = std::min(n, b + chunk);
+        for (int64_t i = b; i < e; ++i) {
+            if (error.load(std::memory_order_relaxed) != 0) break;
+            ChargedOut o{};
+            if (id[i] == std::numeric_limits<int64_t>::min()) {
+                // an invalidated photon: the reference loop `continue`s (RreaPhotonTransport.cpp:52-55)
+                out_r[i] = cr[i]; out_z[i] = cz[i]; out_th[i] = cth[i];
+                out_ux[i] = ux[i]; out_uy[i] = uy[i]; out_uz[i] = uz[i];
+                status[i] = ORC_SKIPPED_INVALID;
+                ion_pairs[i] = 0.0;
+                events[i] = 0;
+                continue;
+            }
+            try {
+                photon_particle(ctx, fx, i, pr[i], pth[i], pz[i], cr[i], cth[i], cz[i], ux[i], uy[i], uz[i], w[i], id[i],
+                                cpu[i], o);
+            } catch (OracleError const& err) {
+                int expected = 0;
+                error.compare_exchange_strong(expected, err.code);
+                break;
+            }
+            out_r[i] = o.r; out_z[i] = o.z; out_th[i] = o.th;
+            out_ux[i] = o.ux; out_uy[i] = o.uy; out_uz[i] = o.uz;
+            status[i] = o.status;
+            ion_pairs[i] = o.ion_pairs;
+            events[i] = static_cast<int64_t>(o.substeps);
+        }
+    }
+    (void)nthreads;
+    if (error.load() != 0) { *n_sec = 0; return error.load(); }
+    int64_t ns = 0;
+    for (auto const& fx : logs) {
+        for (auto const& s : fx.secondaries) {
+            if (ns >= sec_cap) { *n_sec = ns; return ORC_ERR_SECONDARY_CAPACITY; }
+            sec[ns++] = s;
+        }
+        for (int q = 0; q < ORC_T_COUNT; ++q) tallies[q] += fx.tallies[q];
+    }
+    *n_sec = ns;
+    return 0;
+}
+
+void orc_tables_set_photoelectric(void* h, const double* primary, int64_t ngroups, const int64_t* start,
+                                  const double* u, const double* binding)
+{
+    auto* t = static_cast<Tables*>(h);
+    t->pe_primary.assign(primary, primary + ngroups);
+    t->pe_start.assign(start, start + ngroups + 1);
+    t->pe_u.assign(u, u + start[ngroups]);
+    t->pe_binding.assign(binding, binding + start[ngroups]);
+}
+
+void orc_tables_set_pair(void* h, const int64_t* channel, const double* energy, int64_t nblocks, const int64_t* start,
+                         const double* frac, const double* dir)
+{
+    auto* t = static_cast<Tables*>(h);
+    t->pair_channel.assign(channel, channel + nblocks);
+    t->pair_energy.assign(energy, energy + nblocks);
+    t->pair_start.assign(start, start + nblocks + 1);
+    t->pair_frac.assign(frac, frac + 3 * start[nblocks]);
+    t->pair_dir.assign(dir, dir + 9 * start[nblocks]);
+}
+
+// Standalone photon samplers: out = [scattered, recoil, cos] / [binding, electron] / cos.
+int orc_sample_compton(void* h, double e, double u1, double u2, double* out3)
+{
+    try {
+        ComptonState const s = sample_compton(*static_cast<Tables*>(h), e, u1, u2);
+        out3[0] = s.scattered; out3[1] = s.recoil; out3[2] = s.cos_theta;
+        return 0;
+    } catch (OracleError const& err) {
+        return err.code;
+    }
+}
+
+int orc_sample_photoelectric(void* h, double e, double u, double* out2)
+{
+    try {
+        PhotoelectricState const s = sample_photoelectric(*static_cast<Tables*>(h), e, u);
+        out2[0] = s.binding; out2[1] = s.electron;
+        return 0;
+    } catch (OracleError const& err) {
+        return err.code;
+    }
+}
+
+double orc_sample_photoelectron_cos(void* h, double ke, double u, int* err)
+{
+    if (err) *err = 0;
+    ORC_GUARD(return sample_photoelectron_cos(*static_cast<Tables*>(h), ke, u));
+}
+
+int orc_sample_photon_channel(void* h, double e, double u)
+{
+    try {
+        return sample_photon_channel(*static_cast<Tables*>(h), e, u);
+    } catch (OracleError const& err) {
+        return -err.code;
+    }
+}
+
+// out = [valid, electron, positron, recoil, local, e dir(3), p dir(3), r dir(3)]
+int orc_sample_pair(void* h, double e, int channel, double u_inv, double u_block, double u_az, double* out14)
+{
+    try {
+        PairState const s = sample_pair(*static_cast<Tables*>(h), e, channel, u_inv, u_block, u_az);
+        double const v[14] = {s.valid ? 1.0 : 0.0, s.electron, s.positron, s.recoil, s.local, s.edx, s.edy, s.edz,
+                              s.pdx, s.pdy, s.pdz, s.rdx, s.rdy, s.rdz};
+        std::memcpy(out14, v, sizeof(v));
+        return 0;
+    } catch (OracleError const& err) {
+        return err.code;
+    }
 }
 
 // SamplePositronAnnihilationFinalState as a standalone sampler:

@@ -1,9 +1,9 @@
 /* CPU oracle for the RREA per-step charged transport -- TEST INFRASTRUCTURE.
  *
- * A plain C++17 restatement of the reference's electron and positron
- * transport steps (native_amrex_rrea/warpx_overlay/Source/Rrea/
- * RreaElectronTransport.cpp, RreaPositronTransport.cpp and the headers they
- * include).  Only tests/, __graft_entry__.smoke() and the
+ * A plain C++17 restatement of the reference's electron, positron and
+ * photon transport steps (native_amrex_rrea/warpx_overlay/Source/Rrea/
+ * RreaElectronTransport.cpp, RreaPositronTransport.cpp,
+ * RreaPhotonTransport.cpp and the headers they include).  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load this library; the shipped HIP path
  * never links or calls it.
  *
@@ -78,6 +78,7 @@ enum {
     ORC_NO_WRITEBACK = 4,
     ORC_SKIPPED_INVALID = 5,
     ORC_ANNIHILATED = 6,   /* positron consumed: in flight, stopped, or at rest */
+    ORC_ABSORBED = 7,      /* photon consumed: photoelectric, pair, or Compton below the cutoff */
 };
 
 /* Tally slots (sums over the step). */
@@ -95,7 +96,11 @@ enum {
     ORC_T_ANNIHILATIONS,
     ORC_T_BHABHA_EVENTS,
     ORC_T_POSITIVE_IONS,   /* weight of positive-ion-only sources (Bhabha, annihilation) */
-    ORC_T_COUNT = 16,
+    ORC_T_COMPTON_EVENTS,
+    ORC_T_PHOTOELECTRIC_EVENTS,
+    ORC_T_PAIR_EVENTS,
+    ORC_T_LOW_ENERGY_ELECTRONS, /* weight of sub-cutoff pair electrons booked as low-energy sources */
+    ORC_T_COUNT = 24,
 };
 
 /* Error codes; 0 means success. */

@@ -154,6 +154,24 @@ def convert(bundle_dir: str, out_path: str) -> dict:
         out[key + "_eta"] = c.column("screening_eta")
         out[key + "_trmfp"] = c.column("transport_mfp_m_stp")
 
+    # Photoelectric subshell CDF (load_photoelectric_subshell_cdf :893-990): a
+    # new group starts at a new primary energy or where the cdf resets at a
+    # duplicated absorption-edge energy.
+    spec = tables["photoelectric_subshell_cdf_air"]
+    c = rreatbl.read(os.path.join(bundle_dir, spec["path"]))
+    primary = c.column("primary_energy_eV")
+    cdf = c.column("cdf_u")
+    starts = [0]
+    for i in range(1, len(primary)):
+        if primary[i] != primary[i - 1] or cdf[i] <= cdf[i - 1]:
+            starts.append(i)
+    starts = np.array(starts, dtype=np.int64)
+    out["pe_primary"] = primary[starts]
+    out["pe_start"] = np.r_[starts, len(primary)].astype(np.int64)
+    out["pe_u"] = cdf
+    out["pe_binding"] = c.column("binding_energy_eV")
+    out["pe_relaxation"] = c.column("relaxation_energy_eV")
+
     scattering = config["elastic_scattering_model"]
     out["scalars"] = np.array([
         config["hard_moller_secondary_threshold_eV"],
@@ -167,6 +185,51 @@ def convert(bundle_dir: str, out_path: str) -> dict:
         scattering["isotropic_above_path_n"],
         scattering["transition_energy_eV"],
     ], dtype=np.float64)
+    np.savez_compressed(out_path, **out)
+    return out
+
+
+def convert_pair_bank(bundle_dir: str, out_path: str) -> dict:
+    """The training blocks of the pair final-state record bank
+    (load_pair_empirical_blocks NA/src/RreaInteractionTables.cpp:1187-1369),
+    each record's leptons in canonical order: the first pdg 11 slot (pair
+    electron), the pdg -11 slot (positron), the second pdg 11 slot (triplet
+    recoil; zero for nuclear records) -- the slot choice of
+    BuildRreaPairEmpiricalFinalState :1384-1420, made once here."""
+    bank = rreatbl.read(os.path.join(bundle_dir, "final_state/pair_final_state_samples_air.bin.rtb"))
+    with open(os.path.join(bundle_dir, "final_state/pair_final_state_samples_air.index.json")) as handle:
+        index = json.load(handle)
+    if index.get("failures"):
+        raise ValueError("pair bank index reports producer failures")
+    pdg = np.stack([bank.column(f"p{s}_pdg") for s in range(3)], 1)
+    frac = np.stack([bank.column(f"p{s}_kinetic_fraction") for s in range(3)], 1)
+    dirs = np.stack([np.stack([bank.column(f"p{s}_dir_{a}") for a in "xyz"], 1) for s in range(3)], 1)
+    channel_code = bank.column("channel_code")
+    blocks = [b for b in index["blocks"] if b["split"] == "training" and int(b["count"]) > 0]
+    code = {"nuclear": 1, "triplet": 2}
+    blocks.sort(key=lambda b: (code[b["channel"]], float(b["primary_energy_eV"])))
+    rec_frac, rec_dir, starts, energies, channels = [], [], [0], [], []
+    for b in blocks:
+        lo, n = int(b["record_offset"]), int(b["count"])
+        rows = np.arange(lo, lo + n)
+        if np.any(channel_code[rows] != code[b["channel"]]):
+            raise ValueError("pair record channel differs from its block")
+        for row in rows:
+            electrons = [s for s in range(3) if pdg[row, s] == 11]
+            positrons = [s for s in range(3) if pdg[row, s] == -11]
+            order = [electrons[0], positrons[0], electrons[1] if len(electrons) > 1 else -1]
+            rec_frac.append([frac[row, s] if s >= 0 else 0.0 for s in order])
+            rec_dir.append(np.concatenate([dirs[row, s] if s >= 0 else np.zeros(3) for s in order]))
+        starts.append(starts[-1] + n)
+        energies.append(float(b["primary_energy_eV"]))
+        channels.append(code[b["channel"]])
+    out = {
+        "pair_block_channel": np.array(channels, dtype=np.int64),
+        "pair_block_energy": np.array(energies, dtype=np.float64),
+        "pair_block_start": np.array(starts, dtype=np.int64),
+        "pair_frac": np.array(rec_frac, dtype=np.float64),
+        "pair_dir": np.array(rec_dir, dtype=np.float64),
+    }
     np.savez_compressed(out_path, **out)
     return out
 
@@ -194,3 +257,4 @@ if __name__ == "__main__":
     bundle = sys.argv[1]
     here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     convert(bundle, os.path.join(here, "data", "schema6_air.npz"))
+    convert_pair_bank(bundle, os.path.join(here, "data", "pair_bank_air.npz"))

@@ -28,6 +28,7 @@ from .config import RreaParticleInteractionConfig, RzFields, StepConfig
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 BUNDLE_PATH = os.path.join(HERE, "data", "schema6_air.npz")
+PAIR_BANK_PATH = os.path.join(HERE, "data", "pair_bank_air.npz")
 
 RATE_KEYS = ["e_col", "e_soft", "e_moller", "e_brems", "e_eltr",
              "p_col", "p_soft", "p_bhabha", "p_brems", "p_eltr", "p_annih",
@@ -43,10 +44,14 @@ SECONDARY_DTYPE = np.dtype([
 assert SECONDARY_DTYPE.itemsize == C.sizeof(_lib.Secondary)
 
 
-def load_bundle(path: str = BUNDLE_PATH) -> dict:
-    """The schema-6 production bundle as converted by ``tables.convert``."""
-    with np.load(path, allow_pickle=False) as data:
-        return {k: data[k] for k in data.files}
+def load_bundle(path: str = BUNDLE_PATH, pair_bank_path: str = PAIR_BANK_PATH) -> dict:
+    """The schema-6 production bundle as converted by ``tables.convert``,
+    with the pair final-state bank's training records."""
+    out = {}
+    for p in (path, pair_bank_path):
+        with np.load(p, allow_pickle=False) as data:
+            out.update({k: data[k] for k in data.files})
+    return out
 
 
 def _dptr(a: np.ndarray):

@@ -102,3 +102,33 @@ def c3_instances(n_instances: int = 256, seed: int = 3):
     rho = rng.uniform(0.3, 1.225, n_instances)
     n0 = np.floor(10.0 ** rng.uniform(3.0, 6.0, n_instances)).astype(np.int64)
     return e_mag, rho, n0
+
+
+def push_photons(r, theta, z, ux, uy, uz, dt_s):
+    """Straight-line photon push at c along u / |u| in Cartesian, back to
+    (r, theta, z): the pushed position the photon step reads as its chord end
+    (WarpX's own push for the photon species)."""
+    n = np.sqrt(ux * ux + uy * uy + uz * uz)
+    s = C_M_PER_S * dt_s / np.where(n > 0.0, n, 1.0)
+    x = r * np.cos(theta) + ux * s
+    y = r * np.sin(theta) + uy * s
+    return np.hypot(x, y), np.arctan2(y, x), z + uz * s
+
+
+def c2_photons(n: int, fields: RzFields, dt_s: float = 1.0e-12, seed: int = 4, first_id: int = 1):
+    """Photons of config 2: exponential energies clipped to [10 keV, 100 MeV],
+    isotropic directions, u = E * dir; ``extra`` holds the pushed position."""
+    rng = np.random.default_rng(seed)
+    k = np.clip(rng.exponential(7.3e6, n), 1.0e4, 1.0e8)
+    mu = 2.0 * rng.random(n) - 1.0
+    phi = 2.0 * np.pi * rng.random(n)
+    st = np.sqrt(1.0 - mu * mu)
+    ux, uy, uz = k * st * np.cos(phi), k * st * np.sin(phi), k * mu
+    r = fields.r_hi * np.sqrt(rng.random(n)) * (1.0 - 1e-9)
+    z = fields.z_lo + (fields.z_hi - fields.z_lo) * rng.random(n) * (1.0 - 1e-12)
+    theta = 2.0 * np.pi * rng.random(n) - np.pi
+    tile = ParticleTile(r=r, z=z, theta=theta, ux=ux, uy=uy, uz=uz, w=np.ones(n),
+                        id=np.arange(first_id, first_id + n, dtype=np.int64), cpu=np.zeros(n, dtype=np.int32))
+    rc, thc, zc = push_photons(r, theta, z, ux, uy, uz, dt_s)
+    tile.extra = {"r_cur": rc, "theta_cur": thc, "z_cur": zc}
+    return tile
