@@ -76,6 +76,30 @@ typedef struct rrea_elastic_table {
     int64_t n;
 } rrea_elastic_table;
 
+/* Photoelectric subshell CDF (load_photoelectric_subshell_cdf,
+ * NA/src/RreaInteractionTables.cpp:893-990): groups by primary energy (a
+ * duplicated absorption-edge energy opens a second group), each a strictly
+ * increasing cdf over the subshell rows with their binding energies. */
+typedef struct rrea_photoelectric_table {
+    const double* primary;   /* n_groups, nondecreasing */
+    int64_t n_groups;
+    const int64_t* start;    /* n_groups + 1 row offsets */
+    const double* cdf_u;
+    const double* binding_eV;
+} rrea_photoelectric_table;
+
+/* Pair final-state bank, training blocks only (load_pair_empirical_blocks
+ * :1187-1369), sorted by (channel, energy); each record's leptons in the
+ * order pair electron, positron, triplet recoil (zero for nuclear). */
+typedef struct rrea_pair_bank {
+    const int64_t* channel;  /* per block: 1 nuclear, 2 triplet */
+    const double* energy_eV; /* per block */
+    int64_t n_blocks;
+    const int64_t* start;    /* n_blocks + 1 record offsets */
+    const double* fraction;  /* 3 per record: kinetic fraction of (E - 2 m c^2) */
+    const double* direction; /* 9 per record: unit directions in the photon frame */
+} rrea_pair_bank;
+
 typedef struct rrea_table_image {
     /* e- collision, soft radiative, hard-Moller 1/m, tracked brems 1/m, elastic transport 1/m,
        e+ collision, soft radiative, hard-Bhabha, tracked brems, elastic transport, annihilation,
@@ -92,6 +116,8 @@ typedef struct rrea_table_image {
     double gs_few_collision_threshold_n;
     double gs_isotropic_above_path_n;
     double gs_wentzel_transition_energy_eV;
+    rrea_photoelectric_table photoelectric;
+    rrea_pair_bank pair;
 } rrea_table_image;
 
 rrea_status rrea_tables_create(const rrea_table_image* image, rrea_tables_t* out);
@@ -196,15 +222,18 @@ enum {
     RREA_NO_WRITEBACK = 4,
     RREA_SKIPPED_INVALID = 5,
     RREA_ANNIHILATED = 6,   /* positron consumed: in flight, stopped mid-chord, or at rest */
+    RREA_ABSORBED = 7,      /* photon consumed: photoelectric, pair, or Compton below the cutoff */
 };
 
 #define RREA_INVALID_ID INT64_MIN
 
-#define RREA_TALLY_COUNT 16
+#define RREA_TALLY_COUNT 24
 /* Tally slots: 0 survivors, 1 escapes, 2 demotions, 3 hard-Moller events,
  * 4 brems events, 5 tracked brems photons, 6 substeps, 7 continuous drag
  * (weight x eV), 8 ion pairs, 9 field work (weight x eV), 10 annihilations,
- * 11 hard-Bhabha events, 12 positive-ion-only sources (weight). */
+ * 11 hard-Bhabha events, 12 positive-ion-only sources (weight), 13 Compton
+ * events, 14 photoelectric events, 15 pair events, 16 sub-cutoff pair
+ * electrons booked as low-energy sources (weight); 17-23 reserved. */
 
 typedef struct rrea_step_out {
     uint8_t* status;          /* per particle, RREA_SURVIVE.. (may be NULL) */
@@ -232,6 +261,30 @@ rrea_status rrea_transport_step_positrons(rrea_tables_t tables, rrea_fields_t fi
                                           const rrea_step_config* config, const rrea_step_out* out,
                                           rrea_stream_t stream);
 
+/* One photon transport step over a tile: the particle loop of
+ * TransportStep::ApplyPhotonTransport
+ * (warpx_overlay/Source/Rrea/RreaPhotonTransport.cpp:9-461).  The chord runs
+ * from the captured pre-push point prev_(r, theta, z) to WarpX's pushed point
+ * (r, theta, z), both read from the tile; u = E * dir in ux, uy, uz.  One
+ * exponential depth is sampled over the remaining chord per pass: Compton
+ * (a survivor continues from the interaction point along the scattered
+ * direction for its unused path), photoelectric absorption, or pair
+ * production (nuclear or triplet, a replay of the pair bank).  Written back:
+ * r, theta, z and ux, uy, uz; the tile's prev_u* are not read.  Photons carry
+ * no charge: jr, jz and rho of `out` may be NULL and are not touched;
+ * `substeps` receives the number of interactions. */
+rrea_status rrea_transport_step_photons(rrea_tables_t tables, rrea_fields_t fields, const rrea_tile* tile,
+                                        const rrea_step_config* config, const rrea_step_out* out,
+                                        rrea_stream_t stream);
+
+/* The straight-line photon push at c along u / |u| that WarpX applies before
+ * the hook (no counterpart in the transport path: a stand-in for the caller's
+ * own push so that a photon tile arrives as rrea_transport_step_photons
+ * expects).  For every valid photon: prev_(r, theta, z) <- (r, theta, z), then
+ * (r, theta, z) advance by c dt in Cartesian.  prev_* are written through the
+ * tile's const pointers. */
+rrea_status rrea_photon_push(const rrea_tile* tile, double dt_s, rrea_stream_t stream);
+
 /* Kernel timing with HIP events recorded around every transport-kernel
  * launch made through `fields` while enabled (no counterpart in the reference:
  * measurement support).  Enabling or disabling clears the record.  _read
@@ -245,6 +298,7 @@ enum {
     RREA_KERNEL_ELECTRON_HANDOVER = 1,
     RREA_KERNEL_POSITRON = 2,
     RREA_KERNEL_POSITRON_HANDOVER = 3,
+    RREA_KERNEL_PHOTON = 4,
 };
 rrea_status rrea_kernel_timing(rrea_fields_t fields, int32_t enable);
 rrea_status rrea_kernel_timing_read(rrea_fields_t fields, int32_t kernel, double* total_ms, int64_t* launches);

@@ -36,6 +36,15 @@ class ElasticTable(C.Structure):
     _fields_ = [("e", P_D), ("mfp_m_stp", P_D), ("screening_eta", P_D), ("n", C.c_int64)]
 
 
+class PhotoelectricTable(C.Structure):
+    _fields_ = [("primary", P_D), ("n_groups", C.c_int64), ("start", P_I64), ("cdf_u", P_D), ("binding_eV", P_D)]
+
+
+class PairBank(C.Structure):
+    _fields_ = [("channel", P_I64), ("energy_eV", P_D), ("n_blocks", C.c_int64), ("start", P_I64),
+                ("fraction", P_D), ("direction", P_D)]
+
+
 class TableImage(C.Structure):
     _fields_ = [
         ("rate", RateColumn * 15),
@@ -51,6 +60,8 @@ class TableImage(C.Structure):
         ("gs_few_collision_threshold_n", C.c_double),
         ("gs_isotropic_above_path_n", C.c_double),
         ("gs_wentzel_transition_energy_eV", C.c_double),
+        ("photoelectric", PhotoelectricTable),
+        ("pair", PairBank),
     ]
 
 
@@ -96,9 +107,9 @@ class StepOut(C.Structure):
     ]
 
 
-TALLY_COUNT = 16
+TALLY_COUNT = 24
 STATUS = {0: "survive", 1: "escape", 2: "demoted", 3: "zero_weight", 4: "no_writeback", 5: "skipped_invalid",
-          6: "annihilated"}
+          6: "annihilated", 7: "absorbed"}
 INVALID_ID = -(2**63)
 ERRORS = {0: "ok", 1: "table range", 2: "outside domain", 3: "strict guard", 4: "invalid", 5: "secondary capacity",
           16: "argument", 17: "hip"}
@@ -117,6 +128,9 @@ SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
     ("rrea_transport_step_positrons", C.c_int32,
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
+    ("rrea_transport_step_photons", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
+    ("rrea_photon_push", C.c_int32, [C.POINTER(Tile), C.c_double, C.c_void_p]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),

@@ -77,10 +77,17 @@ enum : uint64_t {
     kChPositronGsFewAzimuth = 98,
     kChPositronCompetingOpticalDepth = 109, kChPositronCompetingChannel = 110,
     kChPositronContinuationOpticalDepth = 126, kChPositronContinuationChannel = 127,
+    kChPhotonInteraction = 51, kChPhotonChannel = 52, kChComptonRecoil = 53, kChComptonScatter = 54,
+    kChPhotoelectric = 55, kChPairAngle = 57, kChComptonScatterAzimuth = 58, kChPairAzimuth = 59,
+    kChPhotoelectricAngle = 60, kChPhotoelectricAzimuth = 65, kChPairInvariant = 66,
+    kChPhotonContinuationOpticalDepth = 128, kChPhotonContinuationChannel = 129,
 };
 
 // RreaSecondaryProcess ids, OV/RreaParticleInteraction.H:34-48
-enum : int32_t { kProcHardMoller = 2, kProcHardBhabha = 3, kProcBrems = 4, kProcAnnihilation = 9 };
+enum : int32_t {
+    kProcHardMoller = 2, kProcHardBhabha = 3, kProcBrems = 4, kProcCompton = 5, kProcPhotoelectric = 6,
+    kProcPairNuclear = 7, kProcPairTriplet = 8, kProcAnnihilation = 9,
+};
 
 constexpr uint64_t kSegmentCap = uint64_t{1} << 14;  // OV/RreaParticleInteraction.H:642-650
 constexpr uint64_t kEventCap = uint64_t{1} << 13;
@@ -160,6 +167,30 @@ struct DevTables {
     double hard_moller_threshold;
     double charged_min, charged_max;
     double gs_few, gs_iso, gs_transition;
+    // photon process inverse lengths: one grid (with duplicated edge nodes)
+    // shared by the Compton, photoelectric, pair-nuclear and pair-triplet columns
+    const double* gx;
+    const double* glogx;
+    int32_t ng;
+    Guide gguide;
+    const double* gy[4];
+    const double* glogy[4];
+    int32_t gmode[4];
+    double photon_min, photon_max;
+    // photoelectric subshell CDF groups
+    const double* pe_primary;
+    const int32_t* pe_start;
+    const double* pe_u;
+    const double* pe_binding;
+    int32_t pe_n;
+    // pair final-state bank: training blocks by (channel, energy); records of
+    // (pair electron, positron, triplet recoil) fractions and directions
+    const int32_t* pair_channel;
+    const double* pair_energy;
+    const int32_t* pair_start;
+    const double* pair_frac;
+    const double* pair_dir;
+    int32_t pair_nblocks;
 };
 
 // One bracket of the shared charged grid for one query energy: the
@@ -230,6 +261,46 @@ __device__ __forceinline__ bool charged_bracket(DevTables const& T, double q, Br
     b.t = (lq - T.clogx[b.lo]) / (T.clogx[b.hi] - T.clogx[b.lo]);
     b.lin_t = (q - T.cx[b.lo]) / (T.cx[b.hi] - T.cx[b.lo]);
     return true;
+}
+
+// The same bracket on the photon rate grid, whose duplicated absorption-edge
+// nodes resolve to the low side at an exact edge and to the high side just
+// above it (lower_bound + node-exact test).
+__device__ __forceinline__ bool photon_bracket(DevTables const& T, double q, Bracket& b)
+{
+    if (!finite(q) || q < T.gx[0] || q > T.gx[T.ng - 1]) {
+        b.lo = 0; b.hi = 0; b.kind = 1; b.t = 0.0; b.lin_t = 0.0;
+        return false;
+    }
+    double const lq = log(q);
+    int32_t const up = guided_lower_bound(T.gx, T.ng, q, T.gguide, lq);
+    b.lo = up - 1;
+    b.hi = up;
+    b.t = 0.0;
+    b.lin_t = 0.0;
+    if (up == 0) { b.kind = 1; return true; }
+    if (up == T.ng) { b.kind = 2; return true; }
+    if (T.gx[up] == q) { b.kind = 3; return true; }
+    b.kind = 0;
+    b.t = (lq - T.glogx[b.lo]) / (T.glogx[b.hi] - T.glogx[b.lo]);
+    b.lin_t = (q - T.gx[b.lo]) / (T.gx[b.hi] - T.gx[b.lo]);
+    return true;
+}
+
+__device__ __forceinline__ double photon_value(DevTables const& T, int col, Bracket const& b, int32_t& err)
+{
+    const double* y = T.gy[col];
+    if (b.kind == 1) return y[0];
+    if (b.kind == 2) return y[T.ng - 1];
+    if (b.kind == 3) return y[b.hi];
+    int32_t const mode = T.gmode[col];
+    double const ylo = y[b.lo], yhi = y[b.hi];
+    if (mode == 3) return ylo + b.lin_t * (yhi - ylo);
+    bool const positive = ylo > 0.0 && yhi > 0.0;
+    if (mode == 2 || (mode == 1 && !positive)) return ylo + b.t * (yhi - ylo);
+    if (!positive) { if (!err) err = kErrInvalid; return 0.0; }
+    const double* ly = T.glogy[col];
+    return exp(ly[b.lo] + b.t * (ly[b.hi] - ly[b.lo]));
 }
 
 // RreaInteractionTable::Interpolate on a precomputed bracket.
@@ -1260,6 +1331,300 @@ __device__ bool deposit_escaping_segment(DevGrid const& g, double q, double dt, 
         atomic_add(&g.rho[mto[m]], mcharge[m]);
     }
     return true;
+}
+
+// ---------------------------------------------------------------------------
+// Photon samplers -- NA/src/RreaInteractionTables.cpp:2272-2296, 2762-3127,
+// OV/RreaChargedInterleave.H:309-338, 498-517
+// ---------------------------------------------------------------------------
+enum PhotonChannel : int { kCompton = 0, kPhotoelectric = 1, kPairNuclear = 2, kPairTriplet = 3 };
+
+// RreaUniformStream NA/include/rrea/RreaUniformStream.H:21-53: a splitmix64
+// sequence seeded from the bit patterns of two uniforms
+struct UniformStream {
+    uint64_t state;
+    __device__ UniformStream(double u1, double u2)
+    {
+        uint64_t const b1 = static_cast<uint64_t>(__double_as_longlong(u1));
+        uint64_t const b2 = static_cast<uint64_t>(__double_as_longlong(u2));
+        uint64_t const m2 = splitmix64(b2);
+        state = splitmix64(b1) ^ ((m2 << 32) | (m2 >> 32));
+    }
+    __device__ double next()
+    {
+        state += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z = z ^ (z >> 31);
+        return static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0);
+    }
+};
+
+// RreaTransverseBasis OV/RreaChargedInterleave.H:309-338
+__device__ __forceinline__ void transverse_basis(double dx, double dy, double dz, double& b1x, double& b1y,
+                                                 double& b1z, double& b2x, double& b2y, double& b2z)
+{
+    normalize(dx, dy, dz);
+    double rx = 0.0, ry = 0.0, rz = 1.0;
+    if (fabs(dz) > 0.9) { rx = 1.0; rz = 0.0; }
+    b1x = ry * dz - rz * dy;
+    b1y = rz * dx - rx * dz;
+    b1z = rx * dy - ry * dx;
+    normalize(b1x, b1y, b1z);
+    b2x = dy * b1z - dz * b1y;
+    b2y = dz * b1x - dx * b1z;
+    b2z = dx * b1y - dy * b1x;
+    normalize(b2x, b2y, b2z);
+}
+
+// RreaRotateLocalDirectionToAxis OV/RreaChargedInterleave.H:498-517
+__device__ __forceinline__ void rotate_to_axis(double ax, double ay, double az, double lx, double ly, double lz,
+                                               double& ox, double& oy, double& oz)
+{
+    normalize(ax, ay, az);
+    normalize(lx, ly, lz);
+    double b1x, b1y, b1z, b2x, b2y, b2z;
+    transverse_basis(ax, ay, az, b1x, b1y, b1z, b2x, b2y, b2z);
+    ox = lx * b1x + ly * b2x + lz * ax;
+    oy = lx * b1y + ly * b2y + lz * ay;
+    oz = lx * b1z + ly * b2z + lz * az;
+    normalize(ox, oy, oz);
+}
+
+struct PhotonRates { double compton, pe, pair_n, pair_t; };
+
+// PhotonProcessInverseLengths :2272-2296 at density ratio 1
+__device__ __forceinline__ bool photon_rates(DevTables const& T, double e, PhotonRates& r, int32_t& err)
+{
+    Bracket b;
+    if (!(finite(e) && e >= T.photon_min && e <= T.photon_max) || !photon_bracket(T, e, b)) {
+        if (!err) err = kErrRange;
+        r = PhotonRates{0.0, 0.0, 0.0, 0.0};
+        return false;
+    }
+    r.compton = photon_value(T, 0, b, err) * 1.0;
+    r.pe = photon_value(T, 1, b, err) * 1.0;
+    r.pair_n = photon_value(T, 2, b, err) * 1.0;
+    r.pair_t = photon_value(T, 3, b, err) * 1.0;
+    return true;
+}
+
+// SampleComptonKleinNishinaFinalState :2806-2871 (Butcher-Messel rejection;
+// round 0 on the two drawn uniforms, later draws from their uniform stream)
+__device__ void sample_compton(double e, double draw_mu, double draw_accept, double& scattered, double& recoil,
+                               double& cos_theta)
+{
+    draw_mu = dmax(0.0, dmin(1.0, draw_mu));
+    draw_accept = dmax(0.0, dmin(1.0, draw_accept));
+    UniformStream stream(draw_mu, draw_accept);
+    double const alpha = e / kMeC2;
+    double const eps0 = 1.0 / (1.0 + 2.0 * alpha);
+    double const eps0sq = eps0 * eps0;
+    double const a1 = -log(eps0);
+    double const a2 = 0.5 * (1.0 - eps0sq);
+    double epsilon = eps0;
+    double onecost = 0.0;
+    for (int iter = 0; iter < 128; ++iter) {
+        double const u1 = iter == 0 ? draw_mu : stream.next();
+        double const u2 = iter == 0 ? draw_accept : stream.next();
+        double epsilonsq;
+        if (a1 > (a1 + a2) * u1) {
+            epsilon = exp(-a1 * u2);
+            epsilonsq = epsilon * epsilon;
+        } else {
+            epsilonsq = eps0sq + (1.0 - eps0sq) * u2;
+            epsilon = sqrt(epsilonsq);
+        }
+        onecost = (1.0 - epsilon) / (epsilon * alpha);
+        double const sint2 = onecost * (2.0 - onecost);
+        double const greject = 1.0 - epsilon * sint2 / (1.0 + epsilonsq);
+        if (greject >= stream.next()) break;
+    }
+    double const mu = dmax(-1.0, dmin(1.0, 1.0 - onecost));
+    scattered = dmax(0.0, epsilon * e);
+    recoil = dmax(0.0, e - scattered);
+    cos_theta = mu;
+}
+
+// SamplePhotoelectricFinalState :2878-2950: the subshell row and its binding
+__device__ bool sample_photoelectric(DevTables const& T, double e, double u, double& binding, int32_t& err)
+{
+    if (T.pe_n <= 0 || !(e >= T.pe_primary[0] && e <= T.pe_primary[T.pe_n - 1])) {
+        if (!err) err = kErrRange;
+        return false;
+    }
+    u = dmax(0.0, dmin(1.0, u));
+    int32_t const up = lower_bound(T.pe_primary, T.pe_n, e);
+    int32_t g;
+    if (up == 0) g = 0;
+    else if (up == T.pe_n) g = T.pe_n - 1;
+    else if (T.pe_primary[up] == e) g = up;  // the low side at a duplicated edge
+    else g = up - 1;
+    int32_t const s = T.pe_start[g], n = T.pe_start[g + 1] - s;
+    int32_t row = lower_bound(T.pe_u + s, n, u);
+    if (row == n) row = n - 1;
+    binding = T.pe_binding[s + row];
+    if (binding > e) { if (!err) err = kErrInvalid; return false; }
+    return true;
+}
+
+// SamplePhotoelectronCosTheta :2957-3044: the Sauter-Gavrila K-shell polar
+// angle by bisection of its analytic primitive in t = 1 - beta mu.  The
+// reference bisects in long double; the device has no wider type, so it
+// bisects in double to the same stopping rule (agreement to a few ulp of mu).
+__device__ double sample_photoelectron_cos(double ke, double draw)
+{
+    double const u = clamp_unit(draw);
+    if (u <= 0.0) return -1.0;
+    if (u >= 1.0) return 1.0;
+    double const gamma = 1.0 + ke / kMeC2;
+    double const inv_gamma_sq = 1.0 / (gamma * gamma);
+    double const beta = sqrt(dmax(0.0, 1.0 - inv_gamma_sq));
+    if (beta < 1.0e-4) {
+        double lo = -1.0, hi = 1.0;
+        for (int it = 0; it < 96; ++it) {
+            double const mid = 0.5 * (lo + hi);
+            if (mid == lo || mid == hi) break;
+            double const cdf = (2.0 + mid * (3.0 - mid * mid)) / 4.0;
+            if (cdf < u) lo = mid; else hi = mid;
+        }
+        return 0.5 * (lo + hi);
+    }
+    double const b = 0.5 * gamma * (gamma - 1.0) * (gamma - 2.0);
+    double const c = -inv_gamma_sq;
+    double const t_lo = inv_gamma_sq / (1.0 + beta);
+    double const t_hi = 1.0 + beta;
+    auto primitive = [c, b](double x) {
+        return -c / (3.0 * x * x * x) - (2.0 + b * c) / (2.0 * x * x) - (2.0 * b - 1.0) / x - b * log(x);
+    };
+    double const f_lo = primitive(t_lo);
+    double const f_hi = primitive(t_hi);
+    double const target = (1.0 - u) * f_hi + u * f_lo;
+    double lo = t_lo, hi = t_hi;
+    for (int it = 0; it < 128; ++it) {
+        double const mid = 0.5 * (lo + hi);
+        if (mid == lo || mid == hi) break;
+        if (primitive(mid) < target) lo = mid; else hi = mid;
+    }
+    double const tt = 0.5 * (lo + hi);
+    return dmax(-1.0, dmin(1.0, (1.0 - tt) / beta));
+}
+
+struct PairState {
+    double electron, positron, recoil, local;
+    double edx, edy, edz, pdx, pdy, pdz, rdx, rdy, rdz;
+    bool valid, has_recoil;
+};
+
+__device__ __forceinline__ void normalize_local(double& x, double& y, double& z)
+{
+    double const n = sqrt(x * x + y * y + z * z);
+    if (n <= 0.0 || !finite(n)) { x = 0.0; y = 0.0; z = 1.0; return; }
+    x /= n; y /= n; z /= n;
+}
+
+// SamplePairFinalState :3053-3127 + BuildRreaPairEmpiricalFinalState
+// :1384-1525: replay of one training record of the bracketing energy block,
+// rotated about the pair axis, with the record's exact energy closure
+__device__ PairState sample_pair(DevTables const& T, double e, int channel, double draw_invariant, double draw_block,
+                                 double draw_azimuth, int32_t& err)
+{
+    PairState st{};
+    st.valid = false;
+    bool const triplet = channel == kPairTriplet;
+    double const threshold = triplet ? 4.0 * kMeC2 : 2.0 * kMeC2;
+    if (!(finite(e) && e > threshold)) return st;
+    int32_t const code = triplet ? 2 : 1;
+    int32_t first = -1, last = -1;
+    for (int32_t i = 0; i < T.pair_nblocks; ++i) {
+        if (T.pair_channel[i] == code) { if (first < 0) first = i; last = i; }
+    }
+    if (first < 0) { if (!err) err = kErrInvalid; return st; }
+    int32_t sel = first;
+    if (e >= T.pair_energy[last]) {
+        sel = last;
+    } else if (e > T.pair_energy[first]) {
+        int32_t up = first;
+        while (T.pair_energy[up] < e) ++up;  // blocks of one channel are contiguous and increasing
+        if (T.pair_energy[up] == e) {
+            sel = up;
+        } else {
+            double const lo_e = T.pair_energy[up - 1], hi_e = T.pair_energy[up];
+            double const tt = (log(e) - log(lo_e)) / (log(hi_e) - log(lo_e));
+            sel = clamp_unit(draw_block) < tt ? up : up - 1;
+        }
+    }
+    int32_t const size = T.pair_start[sel + 1] - T.pair_start[sel];
+    if (size <= 0) { if (!err) err = kErrInvalid; return st; }
+    double const scaled = clamp_unit(draw_invariant) * static_cast<double>(size);
+    int32_t idx = scaled < static_cast<double>(size - 1) ? static_cast<int32_t>(scaled) : size - 1;
+    int64_t const rec = static_cast<int64_t>(T.pair_start[sel]) + idx;
+    const double* f = T.pair_frac + 3 * rec;
+    const double* d = T.pair_dir + 9 * rec;
+    double const available = e - 2.0 * kMeC2;
+    double const rotation = kTwoPi * clamp_unit(draw_azimuth);
+    double const cr = cos(rotation), sr = sin(rotation);
+    auto rotated = [&](const double* v, double& x, double& y, double& z) {
+        x = cr * v[0] - sr * v[1];
+        y = sr * v[0] + cr * v[1];
+        z = v[2];
+        normalize_local(x, y, z);
+    };
+    st.valid = true;
+    st.has_recoil = false;
+    st.electron = available * f[0];
+    st.positron = available * f[1];
+    st.recoil = 0.0;
+    rotated(d + 0, st.edx, st.edy, st.edz);
+    rotated(d + 3, st.pdx, st.pdy, st.pdz);
+    st.rdx = 0.0; st.rdy = 0.0; st.rdz = 0.0;
+    if (triplet) {
+        st.has_recoil = true;
+        st.recoil = available * f[2];
+        rotated(d + 6, st.rdx, st.rdy, st.rdz);
+    }
+    double const tol = 64.0 * kEps * dmax(1.0, available);
+    double expl = st.electron + st.positron + st.recoil;
+    if (!finite(expl) || st.electron < 0.0 || st.positron < 0.0 || st.recoil < 0.0) {
+        if (!err) err = kErrInvalid;
+        return st;
+    }
+    if (expl > available) {
+        if (expl - available > tol) { if (!err) err = kErrInvalid; return st; }
+        double const scale = available / expl;
+        st.electron *= scale;
+        st.positron *= scale;
+        st.recoil *= scale;
+        expl = st.electron + st.positron + st.recoil;
+    }
+    st.local = available - expl;
+    // std::max_element with `<`: the first of the largest, in (e-, e+, recoil) order
+    auto reduce_largest = [&](double correction) {
+        double* largest = &st.electron;
+        if (*largest < st.positron) largest = &st.positron;
+        if (*largest < st.recoil) largest = &st.recoil;
+        if (*largest < correction) { if (!err) err = kErrInvalid; return; }
+        *largest -= correction;
+    };
+    if (st.local < 0.0) {
+        double const correction = -st.local;
+        if (correction > tol) { if (!err) err = kErrInvalid; return st; }
+        reduce_largest(correction);
+        st.local = 0.0;
+    }
+    double accounted = st.electron + st.positron + st.recoil + st.local;
+    double const residual = available - accounted;
+    if (fabs(residual) > tol) { if (!err) err = kErrInvalid; return st; }
+    if (residual >= 0.0) st.local += residual;
+    else if (st.local >= -residual) st.local += residual;
+    else reduce_largest(-residual);
+    accounted = st.electron + st.positron + st.recoil + st.local;
+    if (st.electron < 0.0 || st.positron < 0.0 || st.recoil < 0.0 || st.local < 0.0
+        || fabs(available - accounted) > tol) {
+        if (!err) err = kErrInvalid;
+    }
+    return st;
 }
 
 }  // namespace rrea_dev

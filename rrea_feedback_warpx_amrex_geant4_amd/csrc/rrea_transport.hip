@@ -226,7 +226,7 @@ __global__ void cell_fields_kernel(int32_t nr, int32_t nz, const double* __restr
 // Workgroup reduction of the per-particle tallies
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
-constexpr int kTallyUsed = 13;
+constexpr int kTallyUsed = 17;
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -268,6 +268,80 @@ __device__ __forceinline__ void push_secondary(StepArgs const& A, rrea_secondary
     int64_t const slot = static_cast<int64_t>(atomicAdd(reinterpret_cast<unsigned long long*>(A.sec_count), 1ULL));
     if (slot >= A.sec_cap) { if (!err) err = kErrCapacity; return; }
     A.sec[slot] = s;
+}
+
+// The per-particle side-effect sink of one transaction: local deposits as
+// E / W_air ion pairs (deposit_local_energy_as_ion_pairs
+// OV/RreaTransportStep.H:537-561), ion-pair events, and secondaries with the
+// parent's spawn lineage counter (AddSecondary OV/RreaTransportLog.H:437-447).
+struct Emitter {
+    StepArgs const& A;
+    int64_t ip;
+    uint64_t pkey;
+    double w;
+    double& ion;
+    uint64_t& lineage;
+    int32_t& err;
+
+    __device__ void deposit_local(double energy)
+    {
+        if (w <= 0.0 || energy <= 0.0) return;
+        double const pw = w * energy / dmax(kWAir, 1.0);
+        if (pw <= 0.0) return;
+        ion += pw;
+    }
+    __device__ void secondary(int32_t species, int32_t process, double x, double y, double z, double sdx, double sdy,
+                              double sdz, double energy, double birth)
+    {
+        rrea_secondary s;
+        s.parent_index = ip; s.species = species; s.process = process;
+        s.x = x; s.y = y; s.z = z;
+        s.dir_x = sdx; s.dir_y = sdy; s.dir_z = sdz;
+        s.weight = w; s.energy_eV = energy; s.birth_fraction = birth;
+        s.root_key = pkey; s.lineage = lineage++;
+        push_secondary(A, s, err);
+    }
+};
+
+// spawn_positron_annihilation OV/RreaTransportStep.H:563-700: two photons
+// (tracked at or above the photon cutoff, else deposited locally), the
+// residual energy of an at-rest annihilation as ion pairs, and the positive
+// ion the annihilated target electron leaves.  The photons join the enclosing
+// group, so they carry its process id.
+__device__ void spawn_annihilation(DevTables const& T, double gcut, uint64_t rng, Emitter& em, int32_t process,
+                                   double x, double y, double z, double ax, double ay, double az, double k,
+                                   bool at_rest, uint64_t index, double birth, double* tally)
+{
+    if (em.w <= 0.0) return;
+    normalize(ax, ay, az);
+    double const ka = at_rest ? 0.0 : dmax(k, 0.0);
+    if (!(ka == 0.0) && !(finite(ka) && ka >= T.charged_min && ka <= T.charged_max)) {
+        if (!em.err) em.err = kErrRange;
+        return;
+    }
+    Annihilation const st = sample_annihilation(ka, rng_uniform(rng, index, kChPositronAnnihilationPolar),
+                                                rng_uniform(rng, index, kChPositronAnnihilationAzimuth));
+    double g1x, g1y, g1z;
+    cone_direction(ax, ay, az, st.cos1, st.az1, g1x, g1y, g1z);
+    double const e1 = dmax(st.e1, 0.0);
+    double const e2 = dmax(st.e2, 0.0);
+    double g2x = -g1x, g2y = -g1y, g2z = -g1z;
+    if (!st.at_rest && e2 > 0.0) {
+        double const p = sqrt(dmax(0.0, ka * (ka + 2.0 * kMeC2)));
+        g2x = (p * ax - e1 * g1x) / e2;
+        g2y = (p * ay - e1 * g1y) / e2;
+        g2z = (p * az - e1 * g1z) / e2;
+        normalize(g2x, g2y, g2z);
+    }
+    double local = 0.0;
+    if (e1 >= gcut) em.secondary(2, process, x, y, z, g1x, g1y, g1z, e1, birth);
+    else local += e1;
+    if (e2 >= gcut) em.secondary(2, process, x, y, z, g2x, g2y, g2z, e2, birth);
+    else local += e2;
+    if (at_rest && k > 0.0) em.deposit_local(k);
+    if (local > 0.0) em.deposit_local(local);
+    tally[12] += em.w;
+    tally[10] += 1.0;
 }
 
 // RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:107-152 (electron)
@@ -516,59 +590,15 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
             }
             double tr_dp = 0.0, tr_len = 0.0;
 
-            // deposit_local_energy_as_ion_pairs OV/RreaTransportStep.H:537-561
-            auto deposit_local = [&](double energy) {
-                if (w <= 0.0 || energy <= 0.0) return;
-                double const pw = w * energy / dmax(kWAir, 1.0);
-                if (pw <= 0.0) return;
-                ion += pw;
-            };
+            Emitter em{A, ip, pkey, w, ion, lineage, err};
+            auto deposit_local = [&](double energy) { em.deposit_local(energy); };
             auto push = [&](int32_t species, int32_t process, double x, double y, double z, double sdx, double sdy,
                             double sdz, double energy, double birth) {
-                rrea_secondary s;
-                s.parent_index = ip; s.species = species; s.process = process;
-                s.x = x; s.y = y; s.z = z;
-                s.dir_x = sdx; s.dir_y = sdy; s.dir_z = sdz;
-                s.weight = w; s.energy_eV = energy; s.birth_fraction = birth;
-                s.root_key = pkey; s.lineage = lineage++;
-                push_secondary(A, s, err);
+                em.secondary(species, process, x, y, z, sdx, sdy, sdz, energy, birth);
             };
-            // spawn_positron_annihilation OV/RreaTransportStep.H:563-700: two photons
-            // (tracked at or above the photon cutoff, else deposited locally), the
-            // residual energy of an at-rest annihilation as ion pairs, and the
-            // positive ion the annihilated target electron leaves
             auto annihilate = [&](double x, double y, double z, bool at_rest, uint64_t index, double birth) {
-                if (w <= 0.0) return;
-                double ax = dx, ay = dy, az = dz;
-                normalize(ax, ay, az);
-                double const ka = at_rest ? 0.0 : dmax(k, 0.0);
-                if (!(ka == 0.0) && !(finite(ka) && ka >= T.charged_min && ka <= T.charged_max)) {
-                    if (!err) err = kErrRange;
-                    return;
-                }
-                Annihilation const st = sample_annihilation(ka, rng_uniform(rng, index, kChPositronAnnihilationPolar),
-                                                            rng_uniform(rng, index, kChPositronAnnihilationAzimuth));
-                double g1x, g1y, g1z;
-                cone_direction(ax, ay, az, st.cos1, st.az1, g1x, g1y, g1z);
-                double const e1 = dmax(st.e1, 0.0);
-                double const e2 = dmax(st.e2, 0.0);
-                double g2x = -g1x, g2y = -g1y, g2z = -g1z;
-                if (!st.at_rest && e2 > 0.0) {
-                    double const p = sqrt(dmax(0.0, ka * (ka + 2.0 * kMeC2)));
-                    g2x = (p * ax - e1 * g1x) / e2;
-                    g2y = (p * ay - e1 * g1y) / e2;
-                    g2z = (p * az - e1 * g1z) / e2;
-                    normalize(g2x, g2y, g2z);
-                }
-                double local = 0.0;
-                if (e1 >= gcut) push(2, kProcAnnihilation, x, y, z, g1x, g1y, g1z, e1, birth);
-                else local += e1;
-                if (e2 >= gcut) push(2, kProcAnnihilation, x, y, z, g2x, g2y, g2z, e2, birth);
-                else local += e2;
-                if (at_rest && k > 0.0) deposit_local(k);
-                if (local > 0.0) deposit_local(local);
-                tally[12] += w;
-                tally[10] += 1.0;
+                spawn_annihilation(T, gcut, rng, em, kProcAnnihilation, x, y, z, dx, dy, dz, k, at_rest, index, birth,
+                                   tally);
             };
             enum { kNone, kEscape, kDemote, kAnnihilated, kSurvive } terminal = kNone;
             double demote_r = r_particle, demote_z = z_particle;
@@ -1068,6 +1098,238 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
     }
 }
 
+// One photon: the body of the particle loop of OV/RreaPhotonTransport.cpp:56-438.
+// The chord runs from the captured pre-push point to WarpX's pushed point; one
+// exponential depth is sampled over the remaining chord per pass, and a
+// Compton survivor continues from its interaction point along the scattered
+// direction for the unused path length.
+__device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, double* tally)
+{
+    int32_t err = kOk;
+    DevTables const& T = A.T;
+    DevFields const& F = A.F;
+    Domain const& D = A.D;
+    rrea_step_config const& cfg = A.cfg;
+    double const cutoff = cfg.low_energy_cutoff_eV;
+    double const gcut = cfg.photon_cutoff_eV;
+    double const dt = cfg.dt_s;
+    uint64_t const step = static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0);
+
+    int64_t const id = A.id[ip];
+    if (id == RREA_INVALID_ID) {
+        // not a valid photon: the reference loop `continue`s
+        if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SKIPPED_INVALID);
+        if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
+        if (A.substeps) A.substeps[ip] = 0;
+        return;
+    }
+    double const pr = A.prev_r[ip], pth = A.prev_th[ip], pz = A.prev_z[ip];
+    double const cr = A.r[ip], cth = A.th[ip], cz = A.z[ip];
+    double const ux = A.ux[ip], uy = A.uy[ip], uz = A.uz[ip];
+    double const w = A.w[ip];
+    int32_t const cpu = A.cpu ? A.cpu[ip] : 0;
+    uint64_t const pkey = static_cast<uint64_t>(id < 0 ? -id : id) ^ (static_cast<uint64_t>(static_cast<uint32_t>(cpu)) << 40);
+    uint64_t const rng = rng_prefix(cfg.rng_seed, pkey, step);
+
+    double o_r = cr, o_th = cth, o_z = cz, o_ux = ux, o_uy = uy, o_uz = uz;
+    bool write_pos = false, write_u = false;
+    int status = RREA_SURVIVE;
+    double ion = 0.0;
+    uint64_t lineage = 0;
+    uint64_t events = 0;
+    Emitter em{A, ip, pkey, w, ion, lineage, err};
+
+    double energy = sqrt(ux * ux + uy * uy + uz * uz);
+    double dx = ux, dy = uy, dz = uz;
+    normalize(dx, dy, dz);
+    if (pr > D.r_hi + 1.0e-9 * D.r_hi || pz < D.z_lo - 1.0e-9 || pz > D.z_hi + 1.0e-9 * D.z_hi) {
+        err = kErrDomain;
+    }
+    Path path;
+    if (!err) clip_rz_path(pr, pth, pz, cr, cth, cz, D, path, err);
+    uint64_t ordinal = 0;
+    double elapsed_start = 0.0;
+    while (!err) {
+        if (ordinal >= kEventCap) { err = kErrInvalid; break; }
+        bool const escapes = clipped_escapes(path, D);
+        double const ds = path.in_len;
+        Interval const iv = build_interval(F, path, D, A.density_ratio, err);
+        double const chord_dp = density_path(iv, path, 0.0, path.end_f);
+        if (err) break;
+        if (w <= 0.0 || energy < gcut) {
+            // below the production cutoff: deposited where the path starts
+            em.deposit_local(energy);
+            status = w <= 0.0 ? RREA_ZERO_WEIGHT : RREA_DEMOTED;
+            if (w > 0.0) tally[2] += 1.0;
+            break;
+        }
+        auto commit_no_event = [&]() {
+            double x, y, z;
+            point_at_full(path, path.end_f, x, y, z);
+            o_r = hypot(x, y);
+            o_th = atan2(y, x);
+            o_z = z;
+            write_pos = true;
+            if (escapes) {
+                status = RREA_ESCAPE;
+                tally[1] += 1.0;
+            } else {
+                status = RREA_SURVIVE;
+                tally[0] += 1.0;
+            }
+        };
+        if (ds <= 0.0) { commit_no_event(); break; }
+        PhotonRates rates;
+        if (!photon_rates(T, energy, rates, err)) break;
+        double const total = rates.compton + rates.pe + rates.pair_n + rates.pair_t;
+        // PhotonFeedbackMeanFreePath :2762-2771 via mean_free_path_from_inverse_length :110-119
+        if (!finite(total) || total < 0.0) { err = kErrInvalid; break; }
+        double const mfp = total > 0.0 ? 1.0 / total : 1.0e99;
+        double const tau = chord_dp / mfp;
+        uint64_t idx = 0;
+        if (!live_hazard_index(0U, ordinal, idx, 2U)) { err = kErrInvalid; break; }
+        double const draw = rng_uniform(rng, idx, ordinal == 0U ? kChPhotonInteraction : kChPhotonContinuationOpticalDepth);
+        Hazard const h = select_hazard(tau, 0.0, draw, 0.0);
+        if (!h.event) { commit_no_event(); break; }
+        double const f = fraction_at_density_path(iv, path, 0.0, path.end_f, h.sampled * mfp, err);
+        if (err) break;
+        double const dist = path.full * (f - 0.0);
+        double x, y, z;
+        point_at_full(path, f, x, y, z);
+        double const remaining = dmax(0.0, path.full - dist);
+        double const birth = dmin(1.0, elapsed_start + dist / dmax(kC * dt, kDblMin));
+        double const r_site = hypot(x, y);
+        // SamplePhotonChannel :2778-2804
+        int channel;
+        {
+            double const cu = dmax(0.0, dmin(1.0, rng_uniform(rng, idx, ordinal == 0U ? kChPhotonChannel
+                                                                                     : kChPhotonContinuationChannel)));
+            if (!(total > 0.0) || !finite(total)) { err = kErrInvalid; break; }
+            double const target = cu * total;
+            if (target < rates.compton) channel = kCompton;
+            else if (target < rates.compton + rates.pe) channel = kPhotoelectric;
+            else if (target < rates.compton + rates.pe + rates.pair_n) channel = kPairNuclear;
+            else channel = kPairTriplet;
+        }
+        events = ordinal + 1U;
+        if (channel == kCompton) {
+            // RreaApplyComptonFinalState OV/RreaPhotonFinalStates.H:54-140
+            double const incident = energy;
+            double scattered, recoil, ct;
+            sample_compton(incident, rng_uniform(rng, idx, kChComptonRecoil), rng_uniform(rng, idx, kChComptonScatter),
+                           scattered, recoil, ct);
+            double sx, sy, sz;
+            cone_direction(dx, dy, dz, ct, rng_uniform(rng, idx, kChComptonScatterAzimuth), sx, sy, sz);
+            double rx = incident * dx - scattered * sx;
+            double ry = incident * dy - scattered * sy;
+            double rz = incident * dz - scattered * sz;
+            normalize(rx, ry, rz);
+            if (recoil >= cutoff) {
+                em.secondary(0, kProcCompton, x, y, z, rx, ry, rz, recoil, birth);
+                ion += w;
+            } else {
+                ion += w;
+                em.deposit_local(recoil);
+            }
+            tally[13] += 1.0;
+            energy = scattered;
+            if (energy < gcut) {
+                em.deposit_local(energy);
+                status = RREA_ABSORBED;
+                break;
+            }
+            dx = sx; dy = sy; dz = sz;
+            o_ux = energy * dx; o_uy = energy * dy; o_uz = energy * dz;
+            write_u = true;
+            double const xt = x + dx * remaining, yt = y + dy * remaining, zt = z + dz * remaining;
+            Path cont;
+            if (!clip_rz_path(r_site, atan2(y, x), z, hypot(xt, yt), atan2(yt, xt), zt, D, cont, err)) break;
+            double xn, yn, zn;
+            point_at_full(cont, cont.end_f, xn, yn, zn);
+            o_r = hypot(xn, yn);
+            o_th = atan2(yn, xn);
+            o_z = zn;
+            write_pos = true;
+            if (!(remaining > 0.0)) {
+                status = RREA_SURVIVE;
+                tally[0] += 1.0;
+                break;
+            }
+            path = cont;
+            elapsed_start = birth;
+            ++ordinal;
+            continue;
+        }
+        if (channel == kPhotoelectric) {
+            // RreaApplyPhotoelectricFinalState OV/RreaPhotonFinalStates.H:153-224
+            double binding;
+            if (!sample_photoelectric(T, energy, rng_uniform(rng, idx, kChPhotoelectric), binding, err)) break;
+            double const ke = dmin(dmax(0.0, energy - binding), energy);
+            if (ke >= cutoff) {
+                if (!(ke == 0.0) && !(finite(ke) && ke >= T.charged_min && ke <= T.charged_max)) { err = kErrRange; break; }
+                double ex, ey, ez;
+                cone_direction(dx, dy, dz, sample_photoelectron_cos(ke, rng_uniform(rng, idx, kChPhotoelectricAngle)),
+                               rng_uniform(rng, idx, kChPhotoelectricAzimuth), ex, ey, ez);
+                em.secondary(0, kProcPhotoelectric, x, y, z, ex, ey, ez, ke, birth);
+                ion += w;
+            } else {
+                ion += w;
+                em.deposit_local(ke);
+            }
+            if (binding > 0.0) em.deposit_local(binding);
+            tally[14] += 1.0;
+            status = RREA_ABSORBED;
+            break;
+        }
+        // RreaApplyPairFinalState OV/RreaPhotonFinalStates.H:239-363
+        PairState const pair = sample_pair(T, energy, channel, rng_uniform(rng, idx, kChPairInvariant),
+                                           rng_uniform(rng, idx, kChPairAngle), rng_uniform(rng, idx, kChPairAzimuth), err);
+        if (err) break;
+        if (!pair.valid) { err = kErrInvalid; break; }
+        bool const triplet = channel == kPairTriplet;
+        if (triplet && !pair.has_recoil) { err = kErrInvalid; break; }
+        int32_t const process = triplet ? kProcPairTriplet : kProcPairNuclear;
+        double ex, ey, ez, px, py, pz2;
+        rotate_to_axis(dx, dy, dz, pair.edx, pair.edy, pair.edz, ex, ey, ez);
+        rotate_to_axis(dx, dy, dz, pair.pdx, pair.pdy, pair.pdz, px, py, pz2);
+        if (pair.electron >= cutoff) {
+            em.secondary(0, process, x, y, z, ex, ey, ez, pair.electron, birth);
+        } else {
+            tally[16] += w;
+            em.deposit_local(pair.electron);
+        }
+        if (pair.positron >= cutoff) {
+            em.secondary(1, process, x, y, z, px, py, pz2, pair.positron, birth);
+        } else {
+            spawn_annihilation(T, gcut, rng, em, process, x, y, z, px, py, pz2, pair.positron, true, idx, birth, tally);
+        }
+        if (pair.has_recoil) {
+            double qx, qy, qz;
+            rotate_to_axis(dx, dy, dz, pair.rdx, pair.rdy, pair.rdz, qx, qy, qz);
+            if (pair.recoil >= cutoff) em.secondary(0, process, x, y, z, qx, qy, qz, pair.recoil, birth);
+            ion += w;
+            if (pair.recoil < cutoff) em.deposit_local(pair.recoil);
+        }
+        if (pair.local > 0.0) em.deposit_local(pair.local);
+        tally[15] += 1.0;
+        status = RREA_ABSORBED;
+        break;
+    }
+    if (err) {
+        report(A.error, err, ip);
+        return;
+    }
+    if (write_pos) { A.r[ip] = o_r; A.th[ip] = o_th; A.z[ip] = o_z; }
+    if (write_u) { A.ux[ip] = o_ux; A.uy[ip] = o_uy; A.uz[ip] = o_uz; }
+    if (status == RREA_ESCAPE || status == RREA_DEMOTED || status == RREA_ZERO_WEIGHT || status == RREA_ABSORBED) {
+        A.id[ip] = RREA_INVALID_ID;
+    }
+    if (A.status) A.status[ip] = static_cast<uint8_t>(status);
+    if (A.ion_pairs) A.ion_pairs[ip] = ion;
+    if (A.substeps) A.substeps[ip] = static_cast<int32_t>(events);
+    tally[8] += ion;
+}
+
 // workgroup reduction of the tallies, one atomic per tally per workgroup
 __device__ __forceinline__ void reduce_tallies(double* tally, double* out)
 {
@@ -1323,6 +1585,38 @@ __global__ void __launch_bounds__(kBlock) electron_full_kernel(const StepArgs* _
     reduce_tallies(tally, A.tallies);
 }
 
+// The straight-line photon push at c along u / |u| that WarpX applies before
+// the hook: the pushed point becomes the chord end, the old point the
+// captured pre-push snapshot.
+__global__ void photon_push_kernel(int64_t n, const int64_t* __restrict__ id, double* __restrict__ prev_r,
+                                   double* __restrict__ prev_z, double* __restrict__ prev_th, double* __restrict__ r,
+                                   double* __restrict__ z, double* __restrict__ th, const double* __restrict__ ux,
+                                   const double* __restrict__ uy, const double* __restrict__ uz, double c_dt)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || id[i] == RREA_INVALID_ID) return;
+    double const r0 = r[i], t0 = th[i], z0 = z[i];
+    prev_r[i] = r0; prev_th[i] = t0; prev_z[i] = z0;
+    double const un = sqrt(ux[i] * ux[i] + uy[i] * uy[i] + uz[i] * uz[i]);
+    double const s = c_dt / (un > 0.0 ? un : 1.0);
+    double const x = r0 * cos(t0) + ux[i] * s;
+    double const y = r0 * sin(t0) + uy[i] * s;
+    r[i] = hypot(x, y);
+    th[i] = atan2(y, x);
+    z[i] = z0 + uz[i] * s;
+}
+
+// One lane per photon.
+__global__ void __launch_bounds__(kBlock) photon_kernel(const StepArgs* __restrict__ args)
+{
+    StepArgs const& A = *args;
+    double tally[kTallyUsed];
+    for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
+    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (ip < A.n) photon_particle(A, ip, tally);
+    reduce_tallies(tally, A.tallies);
+}
+
 // Every positron through the complete path (used when the fast path is off).
 __global__ void __launch_bounds__(kBlock) positron_full_kernel(const StepArgs* __restrict__ args)
 {
@@ -1412,6 +1706,47 @@ rrea_status rrea_tables_create(const rrea_table_image* im, rrea_tables_t* out)
     }
     T->hm_mean = hm_mean;
     size_t const o_hm = blob.add(hm_mean);
+    // photon rates: the four columns share one grid (duplicated edge nodes kept)
+    int64_t const ng = im->rate[11].n;
+    for (int t = 12; t < 15; ++t) {
+        if (im->rate[t].n != ng || std::memcmp(im->rate[t].x, im->rate[11].x, sizeof(double) * ng) != 0) {
+            return RREA_ERR_ARGUMENT;
+        }
+    }
+    std::vector<double> gx(im->rate[11].x, im->rate[11].x + ng), glogx(ng);
+    for (int64_t i = 0; i < ng; ++i) glogx[i] = std::log(gx[i]);
+    size_t const o_gx = blob.add(gx), o_glogx = blob.add(glogx);
+    HostGuide const gguide = host_guide(glogx, kElasticGuideBins);
+    size_t const o_gguide = blob.add_i(gguide.idx);
+    size_t o_gy[4], o_glogy[4];
+    for (int t = 0; t < 4; ++t) {
+        rrea_rate_column const& c = im->rate[11 + t];
+        std::vector<double> y(c.y, c.y + ng), ly(ng);
+        for (int64_t i = 0; i < ng; ++i) ly[i] = y[i] > 0.0 ? std::log(y[i]) : 0.0;
+        o_gy[t] = blob.add(y);
+        o_glogy[t] = blob.add(ly);
+    }
+    // photoelectric subshell groups
+    rrea_photoelectric_table const& pe = im->photoelectric;
+    if (pe.n_groups < 1 || !pe.primary || !pe.start || !pe.cdf_u || !pe.binding_eV) return RREA_ERR_ARGUMENT;
+    int64_t const pe_rows = pe.start[pe.n_groups];
+    size_t const o_pep = blob.add(std::vector<double>(pe.primary, pe.primary + pe.n_groups));
+    size_t const o_peu = blob.add(std::vector<double>(pe.cdf_u, pe.cdf_u + pe_rows));
+    size_t const o_peb = blob.add(std::vector<double>(pe.binding_eV, pe.binding_eV + pe_rows));
+    std::vector<int32_t> pe_start(pe.start, pe.start + pe.n_groups + 1);
+    size_t const o_pes = blob.add_i(pe_start);
+    // pair bank
+    rrea_pair_bank const& pb = im->pair;
+    if (pb.n_blocks < 1 || !pb.channel || !pb.energy_eV || !pb.start || !pb.fraction || !pb.direction) {
+        return RREA_ERR_ARGUMENT;
+    }
+    int64_t const pair_rec = pb.start[pb.n_blocks];
+    if (pair_rec > INT32_MAX) return RREA_ERR_ARGUMENT;
+    size_t const o_pbe = blob.add(std::vector<double>(pb.energy_eV, pb.energy_eV + pb.n_blocks));
+    size_t const o_pbf = blob.add(std::vector<double>(pb.fraction, pb.fraction + 3 * pair_rec));
+    size_t const o_pbd = blob.add(std::vector<double>(pb.direction, pb.direction + 9 * pair_rec));
+    size_t const o_pbc = blob.add_i(std::vector<int32_t>(pb.channel, pb.channel + pb.n_blocks));
+    size_t const o_pbs = blob.add_i(std::vector<int32_t>(pb.start, pb.start + pb.n_blocks + 1));
     HostCdf const ebrems = host_cdf(im->electron_brems);
     struct CdfOff { size_t p, lp, u, v, m, lm, s; int32_t n; };
     auto add_cdf = [&](HostCdf const& h) {
@@ -1465,6 +1800,15 @@ rrea_status rrea_tables_create(const rrea_table_image* im, rrea_tables_t* out)
     V.cguide = Guide{ii + o_cguide, kChargedGuideBins, cguide.lo, cguide.scale};
     for (int t = 0; t < 11; ++t) { V.cy[t] = f + o_cy[t]; V.clogy[t] = f + o_clogy[t]; V.cmode[t] = im->rate[t].mode; }
     V.hm_mean = f + o_hm;
+    V.gx = f + o_gx; V.glogx = f + o_glogx; V.ng = static_cast<int32_t>(ng);
+    V.gguide = Guide{ii + o_gguide, kElasticGuideBins, gguide.lo, gguide.scale};
+    for (int t = 0; t < 4; ++t) { V.gy[t] = f + o_gy[t]; V.glogy[t] = f + o_glogy[t]; V.gmode[t] = im->rate[11 + t].mode; }
+    V.photon_min = im->photon_energy_range_eV[0];
+    V.photon_max = im->photon_energy_range_eV[1];
+    V.pe_primary = f + o_pep; V.pe_u = f + o_peu; V.pe_binding = f + o_peb; V.pe_start = ii + o_pes;
+    V.pe_n = static_cast<int32_t>(pe.n_groups);
+    V.pair_energy = f + o_pbe; V.pair_frac = f + o_pbf; V.pair_dir = f + o_pbd;
+    V.pair_channel = ii + o_pbc; V.pair_start = ii + o_pbs; V.pair_nblocks = static_cast<int32_t>(pb.n_blocks);
     auto set_cdf = [&](DevCdf& d, CdfOff const& o) {
         d.primary = f + o.p; d.log_primary = f + o.lp; d.u = f + o.u; d.v = f + o.v;
         d.mean = f + o.m; d.log_mean = f + o.lm; d.start = ii + o.s; d.n_groups = o.n;
@@ -1559,7 +1903,7 @@ rrea_status rrea_kernel_timing(rrea_fields_t F, int32_t enable)
 
 rrea_status rrea_kernel_timing_read(rrea_fields_t F, int32_t kernel, double* total_ms, int64_t* launches)
 {
-    if (!F || !total_ms || !launches || kernel < -1 || kernel > 3) return RREA_ERR_ARGUMENT;
+    if (!F || !total_ms || !launches || kernel < -1 || kernel > 4) return RREA_ERR_ARGUMENT;
     double sum = 0.0;
     int64_t count = 0;
     for (size_t i = 0; i < F->events_used; ++i) {
@@ -1648,17 +1992,26 @@ namespace {
 
 // One charged step (either species): argument checks, the argument block,
 // the launch, and the deterministic secondary ordering.
-rrea_status charged_step(bool positron, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+enum SpeciesId : int { kElectron = 0, kPositron = 1, kPhoton = 2 };
+
+rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
                          const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
 {
+    bool const positron = species == kPositron;
+    bool const photon = species == kPhoton;
     if (!T || !F || !tile || !cfg || !out || !F->fields_set) return RREA_ERR_ARGUMENT;
-    if (!out->secondaries || !out->secondary_count || !out->jr || !out->jz || !out->rho || !out->tallies
-        || !out->error || out->secondary_capacity < 0) {
+    if (!out->secondaries || !out->secondary_count || !out->tallies || !out->error || out->secondary_capacity < 0) {
         return RREA_ERR_ARGUMENT;
     }
-    if (tile->n < 0 || (tile->n > 0 && (!tile->prev_r || !tile->prev_z || !tile->prev_theta || !tile->prev_ux
-                                         || !tile->prev_uy || !tile->prev_uz || !tile->w || !tile->id || !tile->r
-                                         || !tile->z || !tile->theta || !tile->ux || !tile->uy || !tile->uz))) {
+    // the photon step deposits no current
+    if (!photon && (!out->jr || !out->jz || !out->rho)) return RREA_ERR_ARGUMENT;
+    if (tile->n < 0 || (tile->n > 0 && (!tile->prev_r || !tile->prev_z || !tile->prev_theta || !tile->w || !tile->id
+                                         || !tile->r || !tile->z || !tile->theta || !tile->ux || !tile->uy || !tile->uz
+                                         || (!photon && (!tile->prev_ux || !tile->prev_uy || !tile->prev_uz))))) {
+        return RREA_ERR_ARGUMENT;
+    }
+    // a photon's chord end is the tile's pushed point, read before it is written
+    if (photon && tile->n > 0 && (tile->prev_r == tile->r || tile->prev_z == tile->z || tile->prev_theta == tile->theta)) {
         return RREA_ERR_ARGUMENT;
     }
     // RreaParticleInteraction::Apply :38-40 -- a non-positive dt does nothing.
@@ -1783,7 +2136,9 @@ rrea_status charged_step(bool positron, rrea_tables_t T, rrea_fields_t F, const 
     const StepArgs* const d_args = static_cast<const StepArgs*>(F->d_args);
     rrea_status st = RREA_OK;
     int32_t const kid = positron ? 2 : 0;   // timing ids: main kernel, then its hand-over kernel
-    if (cfg->complete_path_only) {
+    if (photon) {
+        st = timed(4, [&] { photon_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args); });
+    } else if (cfg->complete_path_only) {
         st = timed(kid, [&] {
             if (positron) positron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
             else electron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
@@ -1852,13 +2207,36 @@ extern "C" {
 rrea_status rrea_transport_step_electrons(rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
                                           const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
 {
-    return charged_step(false, T, F, tile, cfg, out, stream);
+    return species_step(kElectron, T, F, tile, cfg, out, stream);
 }
 
 rrea_status rrea_transport_step_positrons(rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
                                           const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
 {
-    return charged_step(true, T, F, tile, cfg, out, stream);
+    return species_step(kPositron, T, F, tile, cfg, out, stream);
+}
+
+rrea_status rrea_transport_step_photons(rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+                                        const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
+{
+    return species_step(kPhoton, T, F, tile, cfg, out, stream);
+}
+
+rrea_status rrea_photon_push(const rrea_tile* tile, double dt_s, rrea_stream_t stream)
+{
+    if (!tile || tile->n < 0 || !(dt_s >= 0.0)) return RREA_ERR_ARGUMENT;
+    if (tile->n == 0) return RREA_OK;
+    if (!tile->prev_r || !tile->prev_z || !tile->prev_theta || !tile->r || !tile->z || !tile->theta || !tile->ux
+        || !tile->uy || !tile->uz || !tile->id) {
+        return RREA_ERR_ARGUMENT;
+    }
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    photon_push_kernel<<<dim3(static_cast<unsigned>(blocks_for(tile->n, 256))), dim3(256), 0, s>>>(
+        tile->n, tile->id, const_cast<double*>(tile->prev_r), const_cast<double*>(tile->prev_z),
+        const_cast<double*>(tile->prev_theta), tile->r, tile->z, tile->theta, tile->ux, tile->uy, tile->uz,
+        kC * dt_s);
+    HIP_TRY(hipGetLastError());
+    return RREA_OK;
 }
 
 }  // extern "C"

@@ -97,6 +97,14 @@ class InteractionTables:
         im.gs_few_collision_threshold_n = s[7]
         im.gs_isotropic_above_path_n = s[8]
         im.gs_wentzel_transition_energy_eV = s[9]
+        pp, ps = arr(bundle["pe_primary"]), arr(bundle["pe_start"], np.int64)
+        pu, pb = arr(bundle["pe_u"]), arr(bundle["pe_binding"])
+        im.photoelectric = _lib.PhotoelectricTable(_dptr(pp), len(pp), ps.ctypes.data_as(_lib.P_I64), _dptr(pu),
+                                                   _dptr(pb))
+        bc, be = arr(bundle["pair_block_channel"], np.int64), arr(bundle["pair_block_energy"])
+        bs, bf, bd = arr(bundle["pair_block_start"], np.int64), arr(bundle["pair_frac"]), arr(bundle["pair_dir"])
+        im.pair = _lib.PairBank(bc.ctypes.data_as(_lib.P_I64), _dptr(be), len(be), bs.ctypes.data_as(_lib.P_I64),
+                                _dptr(bf), _dptr(bd))
         self.scalars = s
         self.n_charged = len(bundle["e_col_x"])
         h = C.c_void_p()
@@ -198,6 +206,43 @@ class DeviceTile:
                             ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")))
 
 
+@dataclass
+class PhotonTile:
+    """Device SoA of the photon species: the captured pre-push point
+    (prev_r, prev_z, prev_theta) and the tile after WarpX's push (``cur``:
+    pushed r, z, theta, u = E dir, w, id, cpu)."""
+
+    prev_r: torch.Tensor
+    prev_z: torch.Tensor
+    prev_theta: torch.Tensor
+    cur: DeviceTile
+
+    @staticmethod
+    def from_host(tile, device="cuda") -> "PhotonTile":
+        """``tile`` holds the pre-push point in r/z/theta and the pushed point in
+        extra["r_cur"/"z_cur"/"theta_cur"]."""
+        f = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)  # noqa: E731
+        cur = DeviceTile.from_host(tile, device)
+        cur.r, cur.z, cur.theta = f(tile.extra["r_cur"]), f(tile.extra["z_cur"]), f(tile.extra["theta_cur"])
+        return PhotonTile(f(tile.r), f(tile.z), f(tile.theta), cur)
+
+    def __len__(self):
+        return len(self.cur)
+
+    def struct(self) -> _lib.Tile:
+        c = self.cur
+        return _lib.Tile(len(c), self.prev_r.data_ptr(), self.prev_z.data_ptr(), self.prev_theta.data_ptr(), None,
+                         None, None, c.w.data_ptr(), c.id.data_ptr(), c.cpu.data_ptr(), c.r.data_ptr(), c.z.data_ptr(),
+                         c.theta.data_ptr(), c.ux.data_ptr(), c.uy.data_ptr(), c.uz.data_ptr())
+
+    def push(self, dt_s: float, stream: int | None = None) -> None:
+        """WarpX's photon push (rrea_photon_push): prev <- current, current += c dt u/|u|."""
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        t = self.struct()
+        _lib.check(_lib.lib().rrea_photon_push(C.byref(t), dt_s, stream), "rrea_photon_push")
+
+
 class StepOutputs:
     """Device outputs of one step (allocated once, reused)."""
 
@@ -291,6 +336,27 @@ class RreaParticleInteraction:
         arguments as ``apply_electrons``."""
         RreaParticleInteraction._apply("rrea_transport_step_positrons", tables, coupling, prev, cfg, out, current,
                                        order_secondaries, stream, check)
+
+    @staticmethod
+    def apply_photons(tables: InteractionTables, coupling: Coupling, tile: PhotonTile, cfg: StepConfig,
+                      out: StepOutputs, order_secondaries: bool = True, stream: int | None = None,
+                      check: bool = True) -> None:
+        """One photon step (TransportStep::ApplyPhotonTransport) over a pushed
+        photon tile; writes back the photon tile's current state in place."""
+        n = len(tile)
+        c = tile.cur
+        if n and not all(t.is_contiguous() and t.numel() == n and t.dtype == torch.float64 for t in (
+                tile.prev_r, tile.prev_z, tile.prev_theta, c.r, c.z, c.theta, c.ux, c.uy, c.uz, c.w)):
+            raise ValueError("photon tile tensors must be contiguous float64 of one length")
+        t = tile.struct()
+        cfgc = _step_config(cfg)
+        o = out.struct(order_secondaries)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().rrea_transport_step_photons(tables.h, coupling.h, C.byref(t), C.byref(cfgc),
+                                                           C.byref(o), stream), "rrea_transport_step_photons")
+        if check:
+            out.check_error()
 
     @staticmethod
     def _apply(entry: str, tables: InteractionTables, coupling: Coupling, prev: DeviceTile, cfg: StepConfig,

@@ -19,6 +19,8 @@ STRUCTS = {
     "rrea_cdf_groups": _lib.CdfGroups,
     "rrea_angle_table": _lib.AngleTable,
     "rrea_elastic_table": _lib.ElasticTable,
+    "rrea_photoelectric_table": _lib.PhotoelectricTable,
+    "rrea_pair_bank": _lib.PairBank,
     "rrea_table_image": _lib.TableImage,
     "rrea_rz_geometry": _lib.RzGeometry,
     "rrea_step_config": _lib.StepConfigC,
