@@ -290,8 +290,12 @@ struct Emitter {
         if (pw <= 0.0) return;
         ion += pw;
     }
-    __device__ void secondary(int32_t species, int32_t process, double x, double y, double z, double sdx, double sdy,
-                              double sdz, double energy, double birth)
+    // Out of line: inlined into the photon step, the species word of the
+    // record was taken from a register shared across the call sites and came
+    // out wrong on some paths (non-deterministic pair-electron records); one
+    // out-of-line store sequence per record keeps every field its own value.
+    __device__ __noinline__ void secondary(int32_t species, int32_t process, double x, double y, double z, double sdx,
+                                           double sdy, double sdz, double energy, double birth)
     {
         rrea_secondary s;
         s.parent_index = ip; s.species = species; s.process = process;
