@@ -6,13 +6,18 @@ mesh over 2.56 km x 10 km, rho = 1.225 exp(-h / 8.4 km) with h = 4 km + z,
 deposition on, secondaries staged and ordered each step.  The charged shares
 are transported: 70 % electrons (11,744,051) and 5 % positrons (838,861),
 K ~ Exp(7.3 MeV) clipped to [10 keV, 100 MeV] in a 30-degree cone about the
-force.  (The 25 % photon share is not transported by this build yet.)
+force; 25 % photons (4,194,304), isotropic, same energy law.
 
-One "step" is one transport pass over both charged tiles, in place (the fused
-pre-push snapshot): the electron kernels, then the positron kernel, each
-followed by the deterministic secondary ordering, all depositing into one J
-grid.  Particles a step invalidates stay in their tile with the invalid id and
-are skipped by later steps, as WarpX keeps them until its redistribute.
+One "step" is one transport pass over the three tiles in the order of
+RreaParticleInteraction::Apply (RreaParticleInteraction.cpp:57-59): the
+electron kernels, the photon kernel, the positron kernels, each followed by
+the deterministic secondary ordering, the charged species depositing into one
+J grid.  Charged tiles step in place (the fused pre-push snapshot); the photon
+tile is pushed on the device after its step (prev <- current, current += c dt
+u/|u|, WarpX's photon push), so each step reads a freshly pushed chord.  The
+push is inside the timed region.  Particles a step invalidates stay in their
+tile with the invalid id and are skipped by later steps, as WarpX keeps them
+until its redistribute.
 
 Prints one JSON line (rank 0, N=1).  `roofline` is for the dominant kernel,
 the electron kernel: algorithmic bytes per launch (112 B per live electron)
@@ -47,7 +52,9 @@ def log(msg):
 def cpu_baseline(n_particles: int, steps: int, threads: int):
     """The CPU oracle (a restatement of the reference step, OpenMP over
     64-particle chunks) timed on this host on a bounded sample of the same
-    workload: electrons and positrons in the 70:5 ratio."""
+    workload: electrons, photons and positrons in the 70:25:5 ratio.  Only
+    the oracle's step calls are timed (not the numpy photon push between
+    steps)."""
     from oracle import binding as ob
     from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
     from rrea_feedback_warpx_amrex_geant4_amd.config import StepConfig
@@ -56,24 +63,34 @@ def cpu_baseline(n_particles: int, steps: int, threads: int):
     ob.build()
     tables = ob.Tables(load_bundle())
     fields = wl.c2_fields()
-    n_p = max(1, n_particles // 15)
-    tiles = {"electron": wl.c2_electrons(n_particles - n_p, fields, seed=2).as_dict(),
+    n_e, n_g, n_p = wl.c2_species_counts(n_particles)
+    g = wl.c2_photons(n_g, fields, dt_s=1e-12, seed=4, first_id=1 << 41)
+    photons = g.as_dict()
+    photons.update(g.extra)
+    tiles = {"electron": wl.c2_electrons(n_e, fields, seed=2).as_dict(), "photon": photons,
              "positron": wl.c2_electrons(n_p, fields, seed=3, first_id=1 << 40, charge_sign=1.0).as_dict()}
     done = 0
-    t0 = time.perf_counter()
+    dt = 0.0
     for s in range(steps):
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
         for species, p in tiles.items():
-            o = ob.charged_step(species, tables, fields, cfg, p, nthreads=threads, sec_cap=1 << 20)
+            t0 = time.perf_counter()
+            if species == "photon":
+                o = ob.photon_step(tables, fields, cfg, p, nthreads=threads, sec_cap=1 << 16)
+            else:
+                o = ob.charged_step(species, tables, fields, cfg, p, nthreads=threads, sec_cap=1 << 16)
+            dt += time.perf_counter() - t0
             if o["code"] != 0:
                 raise RuntimeError(f"oracle error {o['code']}")
             done += int((p["id"] != INVALID).sum())
             p = dict(p)
             for k in ("r", "z", "theta", "ux", "uy", "uz"):
                 p[k] = o[k]
-            p["id"] = np.where(np.isin(o["status"], (1, 2, 3, 6)), INVALID, p["id"])
+            p["id"] = np.where(np.isin(o["status"], (1, 2, 3, 6, 7)), INVALID, p["id"])
+            if species == "photon":
+                p["r_cur"], p["theta_cur"], p["z_cur"] = wl.push_photons(p["r"], p["theta"], p["z"], p["ux"],
+                                                                         p["uy"], p["uz"], cfg.dt_s)
             tiles[species] = p
-    dt = time.perf_counter() - t0
     return done / dt, dt, done
 
 
@@ -95,7 +112,7 @@ def main():
     ap.add_argument("--particles", type=int, default=None, help="total macroparticles (default 2^24)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run only a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -123,43 +140,47 @@ def main():
     from rrea_feedback_warpx_amrex_geant4_amd import _lib
     from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
     from rrea_feedback_warpx_amrex_geant4_amd.config import StepConfig
-    from rrea_feedback_warpx_amrex_geant4_amd.transport import (Coupling, DeviceTile, InteractionTables,
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import (Coupling, DeviceTile, InteractionTables, PhotonTile,
                                                                 RreaParticleInteraction, StepOutputs)
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
     total = args.particles if args.particles else 1 << 24
     n_e, n_g, n_p = wl.c2_species_counts(total)
-    log(f"setup: {n_e} e- + {n_p} e+ of {total} (photon share {n_g} not transported), building tables")
+    log(f"setup: {n_e} e- + {n_g} gamma + {n_p} e+ = {total}, building tables")
     tables = InteractionTables()
     fields = wl.c2_fields()
     coupling = Coupling(fields, tables)
     dev_e = DeviceTile.from_host(wl.c2_electrons(n_e, fields, seed=2))
     dev_p = DeviceTile.from_host(wl.c2_electrons(n_p, fields, seed=3, first_id=1 << 40, charge_sign=1.0))
+    dev_g = PhotonTile.from_host(wl.c2_photons(n_g, fields, dt_s=1e-12, seed=4, first_id=1 << 41))
     out_e = StepOutputs(n_e, fields, secondary_capacity=1 << 20, per_particle=False)
     out_p = StepOutputs(n_p, fields, secondary_capacity=1 << 18, per_particle=False, share_grid=out_e)
+    out_g = StepOutputs(n_g, fields, secondary_capacity=1 << 18, per_particle=False, share_grid=out_e)
     L = _lib.lib()
     apply = RreaParticleInteraction
 
     def run_step(s):
-        out_e.secondary_count.zero_()
-        out_p.secondary_count.zero_()
+        for o in (out_e, out_g, out_p):
+            o.secondary_count.zero_()
         out_e.jr.zero_()
         out_e.jz.zero_()
         out_e.rho.zero_()
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
         apply.apply_electrons(tables, coupling, dev_e, cfg, out_e, order_secondaries=True, check=False)
+        apply.apply_photons(tables, coupling, dev_g, cfg, out_g, order_secondaries=True, check=False)
         apply.apply_positrons(tables, coupling, dev_p, cfg, out_p, order_secondaries=True, check=False)
+        dev_g.push(cfg.dt_s)
 
     def live():
-        return (int((dev_e.id != _lib.INVALID_ID).sum().item()), int((dev_p.id != _lib.INVALID_ID).sum().item()))
+        return tuple(int((t.id != _lib.INVALID_ID).sum().item()) for t in (dev_e, dev_g.cur, dev_p))
 
     log(f"warmup {args.warmup}")
     for s in range(args.warmup):
         run_step(s)
     torch.cuda.synchronize()
     out_e.check_error()
-    e0, p0 = live()
+    e0, g0, p0 = live()
 
     L.rrea_kernel_timing(coupling.h, 1)
     log(f"timed {args.steps} steps")
@@ -170,34 +191,34 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     kms = {}
-    for name, kid in (("electron", 0), ("electron_handover", 1), ("positron", 2), ("positron_handover", 3)):
+    for name, kid in (("electron", 0), ("electron_handover", 1), ("positron", 2), ("positron_handover", 3),
+                      ("photon", 4)):
         ms = C.c_double(0.0)
         cnt = C.c_int64(0)
         _lib.check(L.rrea_kernel_timing_read(coupling.h, kid, C.byref(ms), C.byref(cnt)), "timing read")
         kms[name] = (ms.value, cnt.value)
     L.rrea_kernel_timing(coupling.h, 0)
     out_e.check_error()
-    e1, p1 = live()
+    e1, g1, p1 = live()
     # live counts decline monotonically and by a tiny fraction per step; the
     # mean of the ends bounds the processed count
-    steps_e = 0.5 * (e0 + e1) * args.steps
-    steps_p = 0.5 * (p0 + p1) * args.steps
-    value = (steps_e + steps_p) / elapsed
+    processed = 0.5 * ((e0 + e1) + (g0 + g1) + (p0 + p1)) * args.steps
+    value = processed / elapsed
     e_ms, e_launches = kms["electron"]
     mean_launch_s = e_ms * 1e-3 / max(e_launches, 1)
     alg_bytes = 0.5 * (e0 + e1) * B_ALG
     achieved = alg_bytes / mean_launch_s
-    log(f"live e- {e0} -> {e1}, e+ {p0} -> {p1}; wall {elapsed:.3f}s; kernels " +
+    log(f"live e- {e0} -> {e1}, gamma {g0} -> {g1}, e+ {p0} -> {p1}; wall {elapsed:.3f}s; kernels " +
         ", ".join(f"{k} {v[0]:.2f} ms / {v[1]}" for k, v in kms.items()))
     traffic, rec = traffic_record()
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle, {threads} threads, {args.cpu_particles} charged x {args.cpu_steps} steps")
+        log(f"cpu baseline: oracle, {threads} threads, {args.cpu_particles} particles x {args.cpu_steps} steps")
         rate, secs, done = cpu_baseline(args.cpu_particles, args.cpu_steps, threads)
         cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_particles} C2 charged particles (e-:e+ = 14:1) x {args.cpu_steps} steps "
+               "sample": f"{args.cpu_particles} C2 particles (e-:gamma:e+ = 70:25:5) x {args.cpu_steps} steps "
                          f"({done} particle-steps, {secs:.1f} s), oracle restatement, OpenMP dynamic over "
                          f"64-particle chunks"}
 
@@ -214,9 +235,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"C2 plain at scale, charged shares of {total}: {n_e} e- + {n_p} e+, dt=1 ps, "
-                               "RZ 256x1024, deposit on, secondaries ordered; photon share not transported",
-                   "electrons": n_e, "positrons": n_p, "path": "plain",
+        "config": {"workload": f"C2 plain at scale, {total} macroparticles: {n_e} e- + {n_g} gamma + {n_p} e+, "
+                               "dt=1 ps, RZ 256x1024, deposit on, secondaries ordered, photon push on device",
+                   "electrons": n_e, "photons": n_g, "positrons": n_p, "path": "plain",
                    "kernel_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in kms.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
