@@ -210,8 +210,11 @@ typedef struct rrea_secondary {
     double weight;
     double energy_eV;
     double birth_fraction;  /* elapsed fraction of the outer step at birth */
-    uint64_t root_key;      /* |id| ^ (cpu << 40) of the parent */
-    uint64_t lineage;       /* per-parent spawn counter */
+    uint64_t root_key;      /* |id| ^ (cpu << 40) of the tracked root particle */
+    uint64_t lineage;       /* transient lineage: the parent's spawn counter for a live parent,
+                               the root's next free lineage for a newborn's child */
+    int32_t root_species;   /* species of the tracked root: 0 e-, 1 e+, 2 photon */
+    int32_t generation;     /* 0: staged by a live step; g: made by a generation-g newborn */
 } rrea_secondary;
 
 enum {
@@ -232,8 +235,11 @@ enum {
  * 4 brems events, 5 tracked brems photons, 6 substeps, 7 continuous drag
  * (weight x eV), 8 ion pairs, 9 field work (weight x eV), 10 annihilations,
  * 11 hard-Bhabha events, 12 positive-ion-only sources (weight), 13 Compton
- * events, 14 photoelectric events, 15 pair events, 16 sub-cutoff pair
- * electrons booked as low-energy sources (weight); 17-23 reserved. */
+ * events, 14 photoelectric events, 15 pair events, 16 low-energy electron
+ * sources (weight: demoted electrons and sub-cutoff pair electrons);
+ * rrea_newborn_drain adds 17 newborns advanced, 18 newborn survivors,
+ * 19 newborn escapes, 20 newborns demoted, annihilated or absorbed;
+ * 21-23 reserved. */
 
 typedef struct rrea_step_out {
     uint8_t* status;          /* per particle, RREA_SURVIVE.. (may be NULL) */
@@ -284,6 +290,42 @@ rrea_status rrea_transport_step_photons(rrea_tables_t tables, rrea_fields_t fiel
  * (r, theta, z) advance by c dt in Cartesian.  prev_* are written through the
  * tile's const pointers. */
 rrea_status rrea_photon_push(const rrea_tile* tile, double dt_s, rrea_stream_t stream);
+
+/* The newborn residual advance of one step.  Replaces the transient FIFO of
+ * TransportStep::DrainAndCommitSecondaries
+ * (warpx_overlay/Source/Rrea/RreaSecondaryCommit.cpp:338-468) with its
+ * per-entry advance TransportStep::AdvanceChargedNewborn /
+ * AdvancePhotonNewborn (RreaNewbornAdvance.cpp:50-994): every secondary born
+ * at elapsed fraction f < 1 is flown through the remaining (1 - f) dt with
+ * the live loops' physics, its draws taken from the root particle's stream
+ * at its transient lineage; the children it makes join the back of the FIFO
+ * with the root's next free lineage.  Validation failures that the reference
+ * aborts on are reported through `error` before anything is returned.
+ *
+ * `work[0, n_initial)` holds the step's staged secondaries in Apply's order
+ * (the electron, photon and positron steps' ordered outputs, concatenated);
+ * each root's records must be contiguous, as an ordered step output is.  The
+ * call appends every later generation behind them, so work_capacity bounds
+ * the whole FIFO.  Synchronises `stream` once per generation. */
+typedef struct rrea_drain_io {
+    rrea_secondary* work;        /* device FIFO: in [0, n_initial), out the whole FIFO */
+    int64_t work_capacity;
+    int64_t n_initial;
+    int64_t* work_count;         /* device int64: FIFO length after the drain */
+    uint8_t* status;             /* device, per FIFO entry (may be NULL): RREA_SURVIVE, RREA_ESCAPE,
+                                    RREA_DEMOTED, RREA_ANNIHILATED or RREA_ABSORBED */
+    double* ion_pairs;           /* device, per FIFO entry ion-pair weight (may be NULL) */
+    rrea_secondary* survivors;   /* device: the entries alive at the end of the step, in FIFO order,
+                                    at their end-of-step state with birth_fraction 1 */
+    int64_t survivor_capacity;
+    int64_t* survivor_count;     /* device int64 */
+    double* jr; double* jz; double* rho;  /* the step's padded net-chord grid, accumulated */
+    double* tallies;             /* device double[RREA_TALLY_COUNT], accumulated */
+    int32_t* error;              /* device int32[2]: first error code, FIFO index */
+} rrea_drain_io;
+
+rrea_status rrea_newborn_drain(rrea_tables_t tables, rrea_fields_t fields, const rrea_step_config* config,
+                               const rrea_drain_io* io, rrea_stream_t stream);
 
 /* Kernel timing with HIP events recorded around every transport-kernel
  * launch made through `fields` while enabled (no counterpart in the reference:

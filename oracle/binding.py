@@ -47,6 +47,7 @@ class Secondary(C.Structure):
         ("dir_x", C.c_double), ("dir_y", C.c_double), ("dir_z", C.c_double),
         ("weight", C.c_double), ("energy_eV", C.c_double), ("birth_fraction", C.c_double),
         ("root_key", C.c_uint64), ("lineage", C.c_uint64),
+        ("root_species", C.c_int32), ("generation", C.c_int32),
     ]
 
 
@@ -55,7 +56,7 @@ SECONDARY_DTYPE = np.dtype([
     ("x", "<f8"), ("y", "<f8"), ("z", "<f8"),
     ("dir_x", "<f8"), ("dir_y", "<f8"), ("dir_z", "<f8"),
     ("weight", "<f8"), ("energy_eV", "<f8"), ("birth_fraction", "<f8"),
-    ("root_key", "<u8"), ("lineage", "<u8"),
+    ("root_key", "<u8"), ("lineage", "<u8"), ("root_species", "<i4"), ("generation", "<i4"),
 ])
 assert SECONDARY_DTYPE.itemsize == C.sizeof(Secondary)
 
@@ -134,6 +135,18 @@ def lib():
         L.orc_photon_step.argtypes = ([C.c_void_p, C.POINTER(Geometry), C.POINTER(Config), C.c_int64]
                                       + [P_D] * 10 + [P_I64, P_I32] + [P_D] * 6 + [P_I32, P_D, P_I64]
                                       + [C.c_int64, C.POINTER(Secondary), P_I64, P_D, C.c_int])
+        L.orc_newborn_drain.restype = C.c_int
+        L.orc_newborn_drain.argtypes = [C.c_void_p, C.POINTER(Geometry), C.POINTER(Config), C.POINTER(Secondary),
+                                        C.c_int64, C.c_int64, C.POINTER(Secondary), P_I64, P_I32, P_D,
+                                        C.POINTER(Secondary), P_I64, P_D, P_D, P_D, P_D]
+        L.orc_pack_transient_index.argtypes = [C.c_int] + [C.c_uint64] * 5 + [C.POINTER(C.c_uint64)]
+        L.orc_pack_transient_index.restype = C.c_int
+        L.orc_live_hazard_index.argtypes = [C.c_uint64] * 3 + [C.POINTER(C.c_uint64)]
+        L.orc_live_hazard_index.restype = C.c_int
+        L.orc_electron_state_demotes.argtypes = [C.c_double] * 5
+        L.orc_electron_state_demotes.restype = C.c_int
+        L.orc_newborn_elapsed.argtypes = [C.c_double, C.c_uint64, C.c_uint64, C.c_double]
+        L.orc_newborn_elapsed.restype = C.c_double
         L.orc_tables_set_photoelectric.argtypes = [C.c_void_p, P_D, C.c_int64, P_I64, P_D, P_D]
         L.orc_tables_set_pair.argtypes = [C.c_void_p, P_I64, P_D, C.c_int64, P_I64, P_D, P_D]
         L.orc_sample_compton.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, P_D]
@@ -376,6 +389,59 @@ def photon_step(tables: Tables, fields, cfg, particles: dict, nthreads: int = 1,
     secs = np.frombuffer(bytes(sec), dtype=SECONDARY_DTYPE, count=cap)[: nsec.value].copy()
     return {"code": code, **outs, "status": status, "ion_pairs": ion, "substeps": events, "secondaries": secs,
             "tallies": tallies}
+
+
+def newborn_drain(tables: Tables, fields, cfg, initial: np.ndarray, cap: int | None = None):
+    """Run the oracle's transient FIFO over the step's staged secondaries
+    (SECONDARY_DTYPE records, Apply's order).  Returns the whole FIFO, one
+    status and ion-pair weight per entry, the survivors, and the J grid and
+    tallies the newborns add."""
+    L = lib()
+    initial = np.ascontiguousarray(initial, dtype=SECONDARY_DTYPE)
+    n0 = len(initial)
+    cap = cap if cap is not None else max(1024, 8 * n0)
+    g, keep = make_geometry(fields)
+    c = make_config(cfg)
+    src = (Secondary * max(n0, 1)).from_buffer_copy(initial.tobytes() + bytes(SECONDARY_DTYPE.itemsize * (n0 == 0)))
+    work = (Secondary * cap)()
+    surv = (Secondary * cap)()
+    status = np.zeros(cap, dtype=np.int32)
+    ion = np.zeros(cap)
+    nw = C.c_int64(0)
+    ns = C.c_int64(0)
+    nrp, nzp = fields.padded_shape()
+    jr = np.zeros((nrp + 1) * nzp)
+    jz = np.zeros(nrp * (nzp + 1))
+    rho = np.zeros(nrp * nzp)
+    tallies = np.zeros(N_TALLIES)
+    code = L.orc_newborn_drain(tables.h, C.byref(g), C.byref(c), src, n0, cap, work, C.byref(nw), _ptr(status, P_I32),
+                               _ptr(ion), surv, C.byref(ns), _ptr(jr), _ptr(jz), _ptr(rho), _ptr(tallies))
+    del keep
+    w = np.frombuffer(bytes(work), dtype=SECONDARY_DTYPE, count=cap)[: nw.value].copy()
+    sv = np.frombuffer(bytes(surv), dtype=SECONDARY_DTYPE, count=cap)[: ns.value].copy()
+    return {"code": code, "work": w, "status": status[: nw.value].copy(), "ion_pairs": ion[: nw.value].copy(),
+            "survivors": sv, "jr": jr, "jz": jz, "rho": rho, "tallies": tallies}
+
+
+def pack_transient_index(newborn: bool, species: int, lineage: int, segment: int, event: int, inner: int):
+    """(valid, value) of the packed transient RNG index."""
+    out = C.c_uint64(0)
+    ok = lib().orc_pack_transient_index(int(newborn), species, lineage, segment, event, inner, C.byref(out))
+    return bool(ok), int(out.value)
+
+
+def live_hazard_index(species: int, segment: int, event: int):
+    out = C.c_uint64(0)
+    ok = lib().orc_live_hazard_index(species, segment, event, C.byref(out))
+    return bool(ok), int(out.value)
+
+
+def electron_state_demotes(energy, cutoff, threshold, field, drag) -> bool:
+    return bool(lib().orc_electron_state_demotes(energy, cutoff, threshold, field, drag))
+
+
+def newborn_elapsed(birth: float, completed: int, substeps: int, within: float) -> float:
+    return lib().orc_newborn_elapsed(birth, completed, substeps, within)
 
 
 def sample_annihilation(k: float, draw_energy: float, draw_azimuth: float) -> dict:

@@ -1487,6 +1487,8 @@ struct Emitter {
     uint64_t pkey;
     double w;
     double& ion;
+    int32_t root_species;     // RreaSecondaryParticle::root_species of every record it makes
+    int32_t generation = 0;   // FIFO generation of the records it makes
     uint64_t lineage = 0;
 
     void ion_event(double ev_w)
@@ -1504,7 +1506,8 @@ struct Emitter {
     void secondary(int sp, int process, double x, double y, double z, double sdx, double sdy, double sdz,
                    double energy, double birth)
     {
-        orc_secondary s{ip, sp, process, x, y, z, sdx, sdy, sdz, w, energy, birth, pkey, lineage++};
+        orc_secondary s{ip, sp, process, x, y, z, sdx, sdy, sdz, w, energy, birth, pkey, lineage++,
+                        root_species, generation};
         fx.secondaries.push_back(s);
     }
 };
@@ -1714,7 +1717,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
     Path path = dpath;
     std::vector<Interval> iv = div;
 
-    Emitter em{fx, ip, pkey, w, out.ion_pairs};
+    Emitter em{fx, ip, pkey, w, out.ion_pairs, pos ? 1 : 0};
     auto ion_event = [&](double ev_w) { em.ion_event(ev_w); };
     auto deposit_local = [&](double energy) { em.deposit_local(energy); };
     // the net chord carries the species' signed charge (-w for e-, +w for e+)
@@ -1735,6 +1738,9 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
         fx.tallies[ORC_T_ESCAPES] += 1.0;
     };
     auto demote = [&](double rr, double zz) {
+        // a low-energy electron source of the parent's weight
+        // (AccumulateLowEnergyElectron, OV/RreaElectronTransport.cpp:152, 857)
+        fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
         deposit_local(k);
         segment(r_particle, z_particle, rr, zz);
         out.r = pr; out.z = pz; out.th = pth; out.ux = pux; out.uy = puy; out.uz = puz;
@@ -2408,6 +2414,97 @@ inline void rotate_to_axis(double ax, double ay, double az, double lx, double ly
     normalize(ox, oy, oz);
 }
 
+// The photon final states at an interaction site, for the live loop and the
+// newborn residual advance alike: Compton (RreaApplyComptonFinalState
+// OV/RreaPhotonFinalStates.H:54-140), photoelectric absorption
+// (RreaApplyPhotoelectricFinalState :153-224) and pair production
+// (RreaApplyPairFinalState :239-363).  Draws are keyed by the emitter's RNG
+// key and the pass's interaction index.  Returns true when the photon
+// survives (Compton at or above the photon cutoff), with its new energy and
+// direction; false when it is absorbed.
+inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel, uint64_t idx, double x, double y,
+                               double z, double birth, double& energy, double& dx, double& dy, double& dz)
+{
+    Tables const& t = *ctx.t;
+    ChunkLog& fx = em.fx;
+    double const cutoff = ctx.cfg.low_energy_cutoff_eV;
+    double const gcut = ctx.cfg.photon_cutoff_eV;
+    double const w = em.w;
+    uint64_t const step = static_cast<uint64_t>(std::max<int64_t>(ctx.cfg.step, 0));
+    auto U = [&](uint64_t chan) { return rng_uniform(ctx.cfg.rng_seed, em.pkey, step, idx, chan); };
+    if (channel == kCompton) {
+        double const incident = energy;
+        ComptonState const st = sample_compton(t, incident, U(kChComptonRecoil), U(kChComptonScatter));
+        double sx, sy, sz;
+        cone_direction(dx, dy, dz, st.cos_theta, U(kChComptonScatterAzimuth), sx, sy, sz);
+        double rx = incident * dx - st.scattered * sx;
+        double ry = incident * dy - st.scattered * sy;
+        double rz = incident * dz - st.scattered * sz;
+        normalize(rx, ry, rz);
+        if (st.recoil >= cutoff) {
+            em.secondary(0, kProcCompton, x, y, z, rx, ry, rz, st.recoil, birth);
+            em.ion_event(w);
+        } else {
+            em.ion_event(w);
+            em.deposit_local(st.recoil);
+        }
+        fx.tallies[ORC_T_COMPTON_EVENTS] += 1.0;
+        energy = st.scattered;
+        if (energy < gcut) {
+            em.deposit_local(energy);
+            return false;
+        }
+        dx = sx; dy = sy; dz = sz;
+        return true;
+    }
+    if (channel == kPhotoelectric) {
+        PhotoelectricState const st = sample_photoelectric(t, energy, U(kChPhotoelectric));
+        double const ke = std::min(st.electron, energy);
+        if (ke >= cutoff) {
+            double ex, ey, ez;
+            cone_direction(dx, dy, dz, sample_photoelectron_cos(t, ke, U(kChPhotoelectricAngle)),
+                           U(kChPhotoelectricAzimuth), ex, ey, ez);
+            em.secondary(0, kProcPhotoelectric, x, y, z, ex, ey, ez, ke, birth);
+            em.ion_event(w);
+        } else {
+            em.ion_event(w);
+            em.deposit_local(ke);
+        }
+        if (st.binding > 0.0) em.deposit_local(st.binding);
+        fx.tallies[ORC_T_PHOTOELECTRIC_EVENTS] += 1.0;
+        return false;
+    }
+    PairState const pair = sample_pair(t, energy, channel, U(kChPairInvariant), U(kChPairAngle), U(kChPairAzimuth));
+    if (!pair.valid) fail(ORC_ERR_INVALID, "pair channel without valid final-state phase space");
+    bool const triplet = channel == kPairTriplet;
+    if (triplet && !pair.has_recoil) fail(ORC_ERR_INVALID, "triplet final state without its recoil");
+    int32_t const process = triplet ? kProcPairTriplet : kProcPairNuclear;
+    double ex, ey, ez, px, py, pz2;
+    rotate_to_axis(dx, dy, dz, pair.edx, pair.edy, pair.edz, ex, ey, ez);
+    rotate_to_axis(dx, dy, dz, pair.pdx, pair.pdy, pair.pdz, px, py, pz2);
+    if (pair.electron >= cutoff) {
+        em.secondary(0, process, x, y, z, ex, ey, ez, pair.electron, birth);
+    } else {
+        fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
+        em.deposit_local(pair.electron);
+    }
+    if (pair.positron >= cutoff) {
+        em.secondary(1, process, x, y, z, px, py, pz2, pair.positron, birth);
+    } else {
+        spawn_annihilation(ctx, em, process, x, y, z, px, py, pz2, pair.positron, true, idx, birth);
+    }
+    if (pair.has_recoil) {
+        double qx, qy, qz;
+        rotate_to_axis(dx, dy, dz, pair.rdx, pair.rdy, pair.rdz, qx, qy, qz);
+        if (pair.recoil >= cutoff) em.secondary(0, process, x, y, z, qx, qy, qz, pair.recoil, birth);
+        em.ion_event(w);
+        if (pair.recoil < cutoff) em.deposit_local(pair.recoil);
+    }
+    if (pair.local > 0.0) em.deposit_local(pair.local);
+    fx.tallies[ORC_T_PAIR_EVENTS] += 1.0;
+    return false;
+}
+
 // One photon: the body of the particle loop of OV/RreaPhotonTransport.cpp:56-438.
 // The chord is the captured pre-push point to WarpX's pushed point; the loop
 // samples one exponential depth over the remaining chord per pass, and a
@@ -2419,7 +2516,6 @@ inline void photon_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, do
 {
     Tables const& t = *ctx.t;
     orc_config const& cfg = ctx.cfg;
-    double const cutoff = cfg.low_energy_cutoff_eV;
     double const gcut = cfg.photon_cutoff_eV;
     double const dt = cfg.dt_s;
     uint64_t const step = static_cast<uint64_t>(std::max<int64_t>(cfg.step, 0));
@@ -2428,7 +2524,7 @@ inline void photon_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, do
     out.ion_pairs = 0.0;
     out.substeps = 0;
     out.r = cr; out.z = cz; out.th = cth; out.ux = ux; out.uy = uy; out.uz = uz;
-    Emitter em{fx, ip, pkey, w, out.ion_pairs};
+    Emitter em{fx, ip, pkey, w, out.ion_pairs, 2};
 
     double energy = std::sqrt(ux * ux + uy * uy + uz * uz);
     double dx = ux, dy = uy, dz = uz;
@@ -2487,102 +2583,589 @@ inline void photon_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, do
         double const r_site = std::hypot(x, y);
         int const channel = sample_photon_channel(
             t, energy, rng_uniform(seed, pkey, step, idx, ordinal == 0U ? kChPhotonChannel : kChPhotonContinuationChannel));
-        auto U = [&](uint64_t chan) { return rng_uniform(seed, pkey, step, idx, chan); };
         out.substeps = ordinal + 1U;
-        if (channel == kCompton) {
-            // RreaApplyComptonFinalState OV/RreaPhotonFinalStates.H:54-140
-            double const incident = energy;
-            ComptonState const st = sample_compton(t, incident, U(kChComptonRecoil), U(kChComptonScatter));
-            double sx, sy, sz;
-            cone_direction(dx, dy, dz, st.cos_theta, U(kChComptonScatterAzimuth), sx, sy, sz);
-            double rx = incident * dx - st.scattered * sx;
-            double ry = incident * dy - st.scattered * sy;
-            double rz = incident * dz - st.scattered * sz;
-            normalize(rx, ry, rz);
-            if (st.recoil >= cutoff) {
-                em.secondary(0, kProcCompton, x, y, z, rx, ry, rz, st.recoil, birth);
-                em.ion_event(w);
-            } else {
-                em.ion_event(w);
-                em.deposit_local(st.recoil);
-            }
-            fx.tallies[ORC_T_COMPTON_EVENTS] += 1.0;
-            energy = st.scattered;
-            if (energy < gcut) {
-                em.deposit_local(energy);
-                out.status = ORC_ABSORBED;
-                return;
-            }
-            dx = sx; dy = sy; dz = sz;
-            out.ux = energy * dx; out.uy = energy * dy; out.uz = energy * dz;
-            double const xt = x + dx * remaining, yt = y + dy * remaining, zt = z + dz * remaining;
-            Path const cont = clip_rz_path(r_site, std::atan2(y, x), z, std::hypot(xt, yt), std::atan2(yt, xt), zt,
-                                           ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
-            double xn, yn, zn;
-            cont.point_at_full(cont.end_f, xn, yn, zn);
-            out.r = std::hypot(xn, yn);
-            out.th = std::atan2(yn, xn);
-            out.z = zn;
-            if (!(remaining > 0.0)) {
-                out.status = ORC_SURVIVE;
-                fx.tallies[ORC_T_SURVIVORS] += 1.0;
-                return;
-            }
-            path = cont;
-            elapsed_start = birth;
-            ++ordinal;
-            continue;
-        }
-        if (channel == kPhotoelectric) {
-            // RreaApplyPhotoelectricFinalState OV/RreaPhotonFinalStates.H:153-224
-            PhotoelectricState const st = sample_photoelectric(t, energy, U(kChPhotoelectric));
-            double const ke = std::min(st.electron, energy);
-            if (ke >= cutoff) {
-                double ex, ey, ez;
-                cone_direction(dx, dy, dz, sample_photoelectron_cos(t, ke, U(kChPhotoelectricAngle)),
-                               U(kChPhotoelectricAzimuth), ex, ey, ez);
-                em.secondary(0, kProcPhotoelectric, x, y, z, ex, ey, ez, ke, birth);
-                em.ion_event(w);
-            } else {
-                em.ion_event(w);
-                em.deposit_local(ke);
-            }
-            if (st.binding > 0.0) em.deposit_local(st.binding);
-            fx.tallies[ORC_T_PHOTOELECTRIC_EVENTS] += 1.0;
+        if (!photon_final_state(ctx, em, channel, idx, x, y, z, birth, energy, dx, dy, dz)) {
             out.status = ORC_ABSORBED;
             return;
         }
-        // RreaApplyPairFinalState OV/RreaPhotonFinalStates.H:239-363
-        PairState const pair = sample_pair(t, energy, channel, U(kChPairInvariant), U(kChPairAngle), U(kChPairAzimuth));
-        if (!pair.valid) fail(ORC_ERR_INVALID, "pair channel without valid final-state phase space");
-        bool const triplet = channel == kPairTriplet;
-        if (triplet && !pair.has_recoil) fail(ORC_ERR_INVALID, "triplet final state without its recoil");
-        int32_t const process = triplet ? kProcPairTriplet : kProcPairNuclear;
-        double ex, ey, ez, px, py, pz2;
-        rotate_to_axis(dx, dy, dz, pair.edx, pair.edy, pair.edz, ex, ey, ez);
-        rotate_to_axis(dx, dy, dz, pair.pdx, pair.pdy, pair.pdz, px, py, pz2);
-        if (pair.electron >= cutoff) {
-            em.secondary(0, process, x, y, z, ex, ey, ez, pair.electron, birth);
+        // a Compton survivor continues from the interaction point along its
+        // scattered direction for the unused path length
+        out.ux = energy * dx; out.uy = energy * dy; out.uz = energy * dz;
+        double const xt = x + dx * remaining, yt = y + dy * remaining, zt = z + dz * remaining;
+        Path const cont = clip_rz_path(r_site, std::atan2(y, x), z, std::hypot(xt, yt), std::atan2(yt, xt), zt,
+                                       ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
+        double xn, yn, zn;
+        cont.point_at_full(cont.end_f, xn, yn, zn);
+        out.r = std::hypot(xn, yn);
+        out.th = std::atan2(yn, xn);
+        out.z = zn;
+        if (!(remaining > 0.0)) {
+            out.status = ORC_SURVIVE;
+            fx.tallies[ORC_T_SURVIVORS] += 1.0;
+            return;
+        }
+        path = cont;
+        elapsed_start = birth;
+        ++ordinal;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Newborn residual advance -- OV/RreaNewbornAdvance.cpp -- and the transient
+// FIFO of DrainAndCommitSecondaries (OV/RreaSecondaryCommit.cpp:338-468).
+// A secondary born at elapsed fraction f < 1 is flown through the remaining
+// (1 - f) dt with the live loops' physics; its draws come from the ROOT
+// particle's stream, indexed by the packed transient lineage.
+// ---------------------------------------------------------------------------
+enum : uint64_t {
+    kChNewbornElectronGsAngle = 112,
+    kChNewbornElectronGsAzimuth = 113,
+    kChNewbornElectronGsFewCount = 114,
+    kChNewbornElectronGsFewScatter = 115,
+    kChNewbornElectronGsFewAzimuth = 116,
+    kChNewbornPositronGsAngle = 117,
+    kChNewbornPositronGsAzimuth = 118,
+    kChNewbornPositronGsFewCount = 119,
+    kChNewbornPositronGsFewScatter = 120,
+    kChNewbornPositronGsFewAzimuth = 121,
+    kChNewbornCompetingOpticalDepth = 122,
+    kChNewbornCompetingChannel = 123,
+};
+
+// RreaNewbornResidualFraction OV/RreaParticleInteraction.H:171-179
+inline double newborn_residual_fraction(double birth)
+{
+    return std::min(1.0, std::max(0.0, 1.0 - birth));
+}
+
+// RreaNewbornElapsedStepFraction OV/RreaParticleInteraction.H:181-197
+inline double newborn_elapsed(double birth, double residual, uint64_t completed, uint64_t substeps, double f)
+{
+    double const within = std::min(1.0, std::max(0.0, f));
+    double const consumed = (static_cast<double>(completed) + within) / static_cast<double>(substeps);
+    return std::min(1.0, birth + residual * consumed);
+}
+
+// TransportStep::NewbornFinalPoint OV/RreaNewbornAdvance.cpp:20-48: the point
+// at `fraction`, pulled back along the chord (1e-9, 1e-7, 1e-5, 1e-3 of the
+// in-domain span) until it lies in the half-open domain.
+inline bool newborn_final_point(Path const& path, double fraction, double& ox, double& oy, double& oz)
+{
+    double const span = fraction - path.begin_f;
+    double pullback = 1.0e-9;
+    for (int attempt = 0; attempt < 5; ++attempt) {
+        double const trial = attempt == 0 ? fraction
+                                          : std::max(path.begin_f, fraction - pullback * std::max(span, 1.0e-12));
+        path.point_at_full(trial, ox, oy, oz);
+        if (in_half_open(ox, oy, oz, path.r_lo, path.r_hi, path.z_lo, path.z_hi)) return true;
+        if (attempt > 0) pullback *= 100.0;
+    }
+    return false;
+}
+
+// The newborn's electron_state_policy_at (OV/RreaNewbornAdvance.cpp:79-96):
+// RreaEvaluateElectronStatePolicy (OV/RreaParticleInteraction.H:1296-1317)
+// with the field gathered at the point and the density ratio at its z.
+inline bool newborn_electron_demotes(StepContext const& ctx, double x, double y, double z, double k, bool hm_on)
+{
+    double const cutoff = ctx.cfg.low_energy_cutoff_eV;
+    double const thr = ctx.t->hard_moller_threshold;
+    double er, ez, bt;
+    ctx.cp.gather(std::hypot(x, y), z, er, ez, bt);
+    double const ratio = positive_or(ctx.cp.density_ratio_at_z(z), 1.0);
+    double const drag = ensemble_drag(*ctx.t, std::max(k, cutoff), ratio, hm_on);
+    bool const below = k < cutoff;
+    bool const nonrunaway = thr > 0.0 && k <= thr && std::hypot(er, ez) <= drag;
+    return below || nonrunaway;
+}
+
+struct NewbornOut {
+    int status = ORC_SURVIVE;
+    double ion_pairs = 0.0;
+};
+
+// TransportStep::AdvanceChargedNewborn OV/RreaNewbornAdvance.cpp:50-777.  On
+// survival `p` holds the state at the end of the step (birth fraction 1).
+inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary& p, int64_t ip, NewbornOut& out)
+{
+    Tables const& t = *ctx.t;
+    orc_config const& cfg = ctx.cfg;
+    bool const pos = p.species == 1;
+    double const cutoff = cfg.low_energy_cutoff_eV;
+    double const gcut = cfg.photon_cutoff_eV;
+    double const thr = t.hard_moller_threshold;
+    bool const hm_on = cfg.disable_hard_moller == 0;
+    double const dt = cfg.dt_s;
+    double const charge = pos ? 1.0 : -1.0;
+    uint64_t const step = static_cast<uint64_t>(std::max<int64_t>(cfg.step, 0));
+    uint64_t const seed = cfg.rng_seed;
+    uint64_t const root = p.root_key;
+    uint64_t const rsp = static_cast<uint64_t>(p.root_species);
+    uint64_t const lin = p.lineage;
+    double const w = p.weight;
+    Emitter em{fx, ip, root, w, out.ion_pairs, p.root_species, p.generation + 1};
+    auto pack = [&](uint64_t segment, uint64_t ordinal, uint64_t inner) {
+        uint64_t v = 0;
+        if (!pack_transient_index(true, rsp, lin, segment, ordinal, inner, v)) {
+            fail(ORC_ERR_INVALID, "newborn RNG index overflow");
+        }
+        return v;
+    };
+    double const birth_r = std::hypot(p.x, p.y);
+    double const birth_z = p.z;
+    double const residual = newborn_residual_fraction(p.birth_fraction);
+    double k = p.energy_eV;
+    if (!in_half_open(p.x, p.y, p.z, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi)) {
+        fail(ORC_ERR_OUTSIDE_DOMAIN, "charged transient born outside the half-open domain");
+    }
+    double dx = p.dir_x, dy = p.dir_y, dz = p.dir_z;
+    normalize(dx, dy, dz);
+    // the net chord from the birth point
+    auto segment = [&](double r1, double z1) {
+        fx.deposits.push_back(DepositOp{0, {birth_r, birth_z, r1, z1, charge * w, 0.0, 0.0}, 0U});
+    };
+    auto escape_along = [&](Path const& path) {
+        BoundaryCrossing const bc = boundary_crossing(path);
+        if (!bc.valid) fail(ORC_ERR_INVALID, "newborn escape has no certified absorbing-face crossing");
+        fx.deposits.push_back(
+            DepositOp{1, {birth_r, birth_z, bc.start_r, bc.start_z, bc.cross_r, bc.cross_z, charge * w}, bc.faces});
+        out.status = ORC_ESCAPE;
+        fx.tallies[ORC_T_NEWBORN_ESCAPES] += 1.0;
+    };
+    auto in_range = [&]() {
+        if (!std::isfinite(k) || k < t.charged_min || k > t.charged_max) {
+            fail(ORC_ERR_TABLE_RANGE, "newborn charged energy left the certified range");
+        }
+    };
+
+    bool const birth_demote = pos ? k < cutoff : newborn_electron_demotes(ctx, p.x, p.y, p.z, k, hm_on);
+    if (birth_demote) {
+        if (pos) {
+            spawn_annihilation(ctx, em, kProcAnnihilation, p.x, p.y, p.z, dx, dy, dz, k, true, pack(0, 0, 0),
+                               p.birth_fraction);
+            out.status = ORC_ANNIHILATED;
         } else {
             fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
-            em.deposit_local(pair.electron);
+            em.deposit_local(k);
+            out.status = ORC_DEMOTED;
         }
-        if (pair.positron >= cutoff) {
-            em.secondary(1, process, x, y, z, px, py, pz2, pair.positron, birth);
-        } else {
-            spawn_annihilation(ctx, em, process, x, y, z, px, py, pz2, pair.positron, true, idx, birth);
-        }
-        if (pair.has_recoil) {
-            double qx, qy, qz;
-            rotate_to_axis(dx, dy, dz, pair.rdx, pair.rdy, pair.rdz, qx, qy, qz);
-            if (pair.recoil >= cutoff) em.secondary(0, process, x, y, z, qx, qy, qz, pair.recoil, birth);
-            em.ion_event(w);
-            if (pair.recoil < cutoff) em.deposit_local(pair.recoil);
-        }
-        if (pair.local > 0.0) em.deposit_local(pair.local);
-        fx.tallies[ORC_T_PAIR_EVENTS] += 1.0;
-        out.status = ORC_ABSORBED;
+        fx.tallies[ORC_T_NEWBORN_TERMINATED] += 1.0;
         return;
+    }
+    if (!(residual > 0.0)) return;
+    double const birth_u = proper_velocity_from_kinetic(k);
+    double const birth_gamma = std::sqrt(1.0 + birth_u * birth_u / (kC * kC));
+    double const residual_dt = dt * residual;
+    double const chord = (birth_u / birth_gamma) * residual_dt;
+    if (!(chord > 0.0)) {
+        if (k < cutoff) out.status = ORC_DEMOTED;
+        return;
+    }
+    double const end_x = p.x + chord * dx;
+    double const end_y = p.y + chord * dy;
+    double const end_z = p.z + chord * dz;
+    // provisional whole-residual chord, used only to size the substeps
+    Path const driver = clip_rz_path(std::hypot(p.x, p.y), std::atan2(p.y, p.x), p.z, std::hypot(end_x, end_y),
+                                     std::atan2(end_y, end_x), end_z, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
+    if (!(driver.full_length > 0.0) || !(driver.in_length > 0.0)) {
+        if (driver.exits) escape_along(driver);
+        return;
+    }
+    std::vector<Interval> const div = build_intervals(ctx.cp, driver, ctx.density_ratio);
+    if (div.empty()) fail(ORC_ERR_INVALID, "newborn path without material intervals");
+    Path path = driver;
+    std::vector<Interval> iv = div;
+    double const driver_dp = density_path(div, driver, driver.begin_f, driver.end_f);
+    double const gs_n_total = elastic_inverse_length(t, pos, k) * driver_dp;
+    double const drag_est = (pos ? t.rate[kPCol].interpolate(k) * 1.0 : realized_loss(t, k, 1.0)) * driver_dp;
+    double const whole_work = path_field_work(ctx.cp, driver, charge, 0.0, 1.0).work;
+    double const impulse = std::max(cfg.max_field_impulse_fraction * k, std::max(cfg.field_impulse_floor_eV, 1.0));
+    double const span = std::max(driver.end_f, 1.0e-6);
+    uint64_t const substeps = std::min<uint64_t>(
+        std::max(driver_count(std::min(gs_n_total / span, cfg.gs_total_collision_cap),
+                              std::max(cfg.max_gs_collisions_per_substep, 1.0)),
+                 driver_count((std::abs(whole_work) + drag_est) / span, impulse)),
+        cfg.max_subcycles);
+    if (substeps >= kSegmentCap) fail(ORC_ERR_INVALID, "newborn substep count outside its transient RNG budget");
+
+    double run_x = p.x, run_y = p.y, run_z = p.z;
+    double const sub_dt = residual_dt / static_cast<double>(substeps);
+    double final_f = 0.0;
+    bool terminal_interaction = false;
+    bool terminal_at_point = false;
+    bool exited = false;
+    uint64_t completed = 0;
+    auto elapsed_at = [&](double f) { return newborn_elapsed(p.birth_fraction, residual, completed, substeps, f); };
+    auto mark_stopped = [&](double f) {
+        final_f = f;
+        terminal_at_point = true;
+    };
+    // RreaPlanContinuousDragSegment OV/RreaChargedInterleave.H:239-292 with the
+    // cutoff limiter :203-265 and the ion-pair partition :157-196
+    struct SegLimit { double end; bool cutoff; };
+    auto continuous = [&](double b, double e, double stop_e) -> SegLimit {
+        if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return {b, false};
+        double const col = frozen(t.rate[pos ? kPCol : kECol].interpolate(stop_e) * 1.0);
+        double const soft = frozen(t.rate[pos ? kPSoft : kESoft].interpolate(stop_e) * 1.0);
+        double const total = col + soft;
+        auto loss_to = [&](double f) { return total * density_path(iv, path, b, f); };
+        double eff = b;
+        bool reaches = false;
+        {
+            if (!std::isfinite(b) || !std::isfinite(e) || !std::isfinite(k) || !std::isfinite(cutoff) || !(e >= b)
+                || !(k >= cutoff) || !(cutoff >= 0.0)) {
+                fail(ORC_ERR_INVALID, "newborn continuous stopping could not locate a cutoff point");
+            }
+            double const to_cut = k - cutoff;
+            double const req = loss_to(e);
+            if (!std::isfinite(req) || req < 0.0) fail(ORC_ERR_INVALID, "newborn continuous stopping invalid");
+            if (req < to_cut) {
+                eff = e;
+            } else {
+                reaches = true;
+                if (to_cut != 0.0) {
+                    double lo = b, hi = e;
+                    for (int it = 0; it < 56; ++it) {
+                        double const mid = 0.5 * (lo + hi);
+                        double const l = loss_to(mid);
+                        if (!std::isfinite(l) || l < 0.0) fail(ORC_ERR_INVALID, "newborn continuous stopping invalid");
+                        if (l < to_cut) lo = mid; else hi = mid;
+                    }
+                    eff = 0.5 * (lo + hi);
+                }
+            }
+        }
+        double const pm = density_path(iv, path, b, eff);
+        double const raw_col = col * pm;
+        double const raw_soft = soft * pm;
+        double const avail = std::max(0.0, k - cutoff);
+        double const drag = reaches ? avail : std::min(raw_col + raw_soft, avail);
+        double const col_loss = std::min(raw_col, drag);
+        if (col_loss > 0.0 && raw_col > 0.0) {
+            for (auto const& it : iv) {
+                double const lb = std::max(b, it.begin), le = std::min(eff, it.end);
+                if (!(le > lb)) continue;
+                double const raw_i = path.full_length * (le - lb) * it.ratio * col;
+                double const loss_i = col_loss * raw_i / raw_col;
+                double const pw = w * loss_i / std::max(kWAir, 1.0);
+                if (!(pw > 0.0)) continue;
+                em.ion_event(pw);
+            }
+        }
+        fx.tallies[ORC_T_DRAG_LOSS_WEV] += w * drag;
+        k = std::max(0.0, k - drag);
+        if (reaches) k = cutoff;
+        return {eff, reaches};
+    };
+
+    FieldRange sfield;
+    double substep_dt = 0.0;
+    int const nch = pos ? 3 : 2;
+    // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:693-732
+    for (uint64_t sc = 0; sc < substeps; ++sc) {
+        if (k < cutoff) break;
+        double sb = 0.0, se = 0.0;
+        {
+            Chord const ch = build_chord(run_x, run_y, run_z, dx, dy, dz, k, sub_dt, residual_dt, ctx.r_lo, ctx.r_hi,
+                                         ctx.z_lo, ctx.z_hi);
+            if (!ch.valid) break;
+            path = ch.path;
+            if (!(path.full_length > 0.0) || !(path.in_length > 0.0)) {
+                exited = path.exits;
+                final_f = 0.0;
+                break;
+            }
+            iv = build_intervals(ctx.cp, path, ctx.density_ratio);
+            if (iv.empty()) fail(ORC_ERR_INVALID, "newborn microchord without material intervals");
+            sb = path.begin_f;
+            se = path.end_f;
+            final_f = se;
+            sfield = path_field_work(ctx.cp, path, charge, sb, se);
+            substep_dt = sub_dt * (se - sb);
+            (void)half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
+            if (k < cutoff) {
+                final_f = sb;
+                terminal_at_point = true;
+                break;
+            }
+            in_range();
+        }
+        // RreaRunChargedEventInterleave OV/RreaChargedInterleave.H:546-664
+        {
+            double rem = sb;
+            uint64_t ord = 0;
+            double ex_ = 0.0, ey_ = 0.0, ez_ = 0.0, er_ = 0.0, efrac = 1.0;
+            while (rem < se && k >= cutoff) {
+                if (ord >= kEventCap) fail(ORC_ERR_INVALID, "newborn event ordinal overflow");
+                double const frozen_e = k;
+                double stp[3];
+                if (pos) positron_stp_rates(t, thr, gcut, frozen_e, stp);
+                else electron_stp_rates(t, thr, gcut, frozen_e, hm_on, stp);
+                double const total_stp = stp[0] + stp[1] + stp[2];
+                double const L = density_path(iv, path, rem, se);
+                double const taus[3] = {stp[0] * L, stp[1] * L, stp[2] * L};
+                uint64_t const idx = pack(sc, ord, 0U);
+                double const u1 = rng_uniform(seed, root, step, idx, kChNewbornCompetingOpticalDepth);
+                double const u2 = rng_uniform(seed, root, step, idx, kChNewbornCompetingChannel);
+                Hazard const h = select_hazard(taus, nch, u1, u2);
+                if (!h.valid) fail(ORC_ERR_INVALID, "newborn competing-hazard selection invalid");
+                if (!h.event) {
+                    SegLimit const sl = continuous(rem, se, frozen_e);
+                    if (sl.cutoff) mark_stopped(sl.end);
+                    break;
+                }
+                double const ef = fraction_at_density_path(iv, path, rem, se, h.sampled / total_stp);
+                if (!(std::isfinite(rem) && std::isfinite(ef) && std::isfinite(se) && se > rem && ef >= rem && ef <= se)) {
+                    fail(ORC_ERR_INVALID, "newborn sampled event depth outside its subcycle");
+                }
+                path.point_at_full(ef, ex_, ey_, ez_);
+                er_ = std::hypot(ex_, ey_);
+                efrac = elapsed_at(ef);
+                SegLimit const sl = continuous(rem, ef, frozen_e);
+                if (sl.cutoff) { mark_stopped(sl.end); break; }
+                bool const open_ev = k >= cutoff && charged_channel_open(t, pos, thr, gcut, h.channel, k, hm_on);
+                if (!open_ev) {
+                    bool const open_fr = frozen_e >= cutoff && charged_channel_open(t, pos, thr, gcut, h.channel, frozen_e, hm_on);
+                    if (!open_fr) fail(ORC_ERR_INVALID, "newborn selected channel closed before its sampled depth");
+                    rem = ef;
+                    ++ord;
+                    continue;
+                }
+                if (!pos && h.channel == 0) {
+                    double const before = k;
+                    double T = t.sample_moller(before, rng_uniform(seed, root, step, idx, kChHardMollerEnergy));
+                    T = std::max(thr, std::min(T, 0.5 * before));
+                    double pdx, pdy, pdz, sdx, sdy, sdz;
+                    moller_lab_directions(dx, dy, dz, before, T, rng_uniform(seed, root, step, idx, kChHardMollerAzimuth),
+                                          pdx, pdy, pdz, sdx, sdy, sdz);
+                    k = std::max(0.0, before - T);
+                    dx = pdx; dy = pdy; dz = pdz;
+                    em.secondary(0, kProcHardMoller, ex_, ey_, ez_, sdx, sdy, sdz, T, efrac);
+                    em.ion_event(w);
+                    fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
+                } else if (pos && h.channel == 0) {
+                    double const before = k;
+                    double const T = std::min(t.sample_bhabha(before, rng_uniform(seed, root, step, idx, kChPositronBhabhaEnergy)),
+                                              std::max(0.0, before));
+                    if (T > 0.0 && !(thr > 0.0 && T < thr)) {
+                        double pdx, pdy, pdz, sdx, sdy, sdz;
+                        moller_lab_directions(dx, dy, dz, before, T,
+                                              rng_uniform(seed, root, step, idx, kChPositronBhabhaAzimuth),
+                                              pdx, pdy, pdz, sdx, sdy, sdz);
+                        k = std::max(0.0, before - T);
+                        dx = pdx; dy = pdy; dz = pdz;
+                        em.secondary(0, kProcHardBhabha, ex_, ey_, ez_, sdx, sdy, sdz, T, efrac);
+                        fx.tallies[ORC_T_POSITIVE_IONS] += w;
+                        fx.tallies[ORC_T_BHABHA_EVENTS] += 1.0;
+                    }
+                } else if (h.channel == 1) {
+                    double const before = k;
+                    double const eu = rng_uniform(seed, root, step, idx, pos ? kChPositronBremsPhotonEnergy : kChBremsPhotonEnergy);
+                    double const ks = pos ? t.sample_positron_brems_energy(before, eu) : t.sample_brems_energy(before, eu);
+                    double const pe = std::min(std::max(ks, 0.0), std::max(before, 0.0));
+                    double const after = std::max(0.0, std::max(before, 0.0) - pe);
+                    bool const tracked = std::isfinite(gcut) && gcut >= 0.0 && pe >= gcut;
+                    double const local = tracked ? 0.0 : pe;
+                    double gx, gy, gz;
+                    cone_direction(dx, dy, dz,
+                                   t.sample_brems_cos(before, rng_uniform(seed, root, step, idx,
+                                                                          pos ? kChPositronBremsPhotonAngle : kChBremsPhotonAngle)),
+                                   rng_uniform(seed, root, step, idx, pos ? kChPositronBremsPhotonAzimuth : kChBremsPhotonAzimuth),
+                                   gx, gy, gz);
+                    photon_recoil(before, pe, gx, gy, gz, dx, dy, dz);
+                    k = after;
+                    if (tracked) {
+                        em.secondary(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
+                        fx.tallies[ORC_T_BREMS_PHOTONS] += 1.0;
+                    } else if (local > 0.0) {
+                        em.deposit_local(local);
+                    }
+                    fx.tallies[ORC_T_BREMS_EVENTS] += 1.0;
+                } else if (pos && h.channel == 2) {
+                    // in flight: the positron is consumed at the event point
+                    spawn_annihilation(ctx, em, kProcAnnihilation, ex_, ey_, ez_, dx, dy, dz, k, false, idx, efrac);
+                    segment(er_, ez_);
+                    terminal_interaction = true;
+                    final_f = ef;
+                    break;
+                }
+                if (k <= cutoff) { mark_stopped(ef); break; }
+                rem = ef;
+                ++ord;
+            }
+        }
+        if (terminal_interaction || terminal_at_point) break;
+        {
+            if (k >= cutoff) {
+                // apply_gs_elastic_scatter OV/RreaTransportStep.H:372-423 on the newborn channels
+                double const gs_n = elastic_inverse_length(t, pos, k) * density_path(iv, path, sb, se);
+                if (gs_n > 0.0) {
+                    auto U = [&](uint64_t chan, uint64_t inner) {
+                        return rng_uniform(seed, root, step, pack(sc, 0U, inner), chan);
+                    };
+                    auto deflect = [&](double ct, double uphi) {
+                        double nx, ny, nz;
+                        cone_direction(dx, dy, dz, ct, uphi, nx, ny, nz);
+                        dx = nx; dy = ny; dz = nz;
+                    };
+                    if (gs_n < t.gs_few) {
+                        int const cap = static_cast<int>(std::ceil(3.0 * t.gs_few));
+                        if (cap >= 255) fail(ORC_ERR_INVALID, "GS few-collision cap does not pack");
+                        int const cnt = poisson_capped(
+                            gs_n, U(pos ? kChNewbornPositronGsFewCount : kChNewbornElectronGsFewCount, 0), cap);
+                        for (int q = 0; q < cnt; ++q) {
+                            uint64_t const inner = static_cast<uint64_t>(q + 1);
+                            double const ct = t.sample_single_elastic_cos(
+                                pos, k, U(pos ? kChNewbornPositronGsFewScatter : kChNewbornElectronGsFewScatter, inner));
+                            deflect(ct, U(pos ? kChNewbornPositronGsFewAzimuth : kChNewbornElectronGsFewAzimuth, inner));
+                        }
+                    } else {
+                        double const ct = t.sample_gs_cos(pos, k, gs_n,
+                                                          U(pos ? kChNewbornPositronGsAngle : kChNewbornElectronGsAngle, 0));
+                        deflect(ct, U(pos ? kChNewbornPositronGsAzimuth : kChNewbornElectronGsAzimuth, 0));
+                    }
+                }
+            }
+            (void)half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
+            if (k < cutoff) {
+                final_f = se;
+                terminal_at_point = true;
+                break;
+            }
+            in_range();
+        }
+        if (!advance_running_position(path, se, run_x, run_y, run_z)) { exited = true; break; }
+        if (path.exits) { exited = true; break; }
+        completed = sc + 1U;
+    }
+    if (terminal_interaction) {
+        out.status = ORC_ANNIHILATED;
+        fx.tallies[ORC_T_NEWBORN_TERMINATED] += 1.0;
+        return;
+    }
+    double fx_ = 0.0, fy_ = 0.0, fz_ = 0.0;
+    if (!newborn_final_point(path, final_f, fx_, fy_, fz_)) {
+        fail(ORC_ERR_INVALID, "newborn could not place its final point in the half-open domain");
+    }
+    double const final_r = std::hypot(fx_, fy_);
+    double const final_el = elapsed_at(final_f);
+    bool const demote = pos ? k < cutoff : newborn_electron_demotes(ctx, fx_, fy_, fz_, k, hm_on);
+    if (terminal_at_point || demote) {
+        if (pos) {
+            spawn_annihilation(ctx, em, kProcAnnihilation, fx_, fy_, fz_, dx, dy, dz, k, true,
+                               pack(std::min<uint64_t>(substeps, kSegmentCap - 1U), 0U, 0U), final_el);
+            out.status = ORC_ANNIHILATED;
+        } else {
+            fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
+            em.deposit_local(k);
+            out.status = ORC_DEMOTED;
+        }
+        segment(final_r, fz_);
+        fx.tallies[ORC_T_NEWBORN_TERMINATED] += 1.0;
+        return;
+    }
+    if (exited) {
+        escape_along(path);
+        return;
+    }
+    p.x = fx_; p.y = fy_; p.z = fz_;
+    p.dir_x = dx; p.dir_y = dy; p.dir_z = dz;
+    p.energy_eV = k;
+    p.birth_fraction = 1.0;
+    segment(final_r, fz_);
+}
+
+// TransportStep::AdvancePhotonNewborn OV/RreaNewbornAdvance.cpp:779-994.
+inline void photon_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary& p, int64_t ip, NewbornOut& out)
+{
+    Tables const& t = *ctx.t;
+    orc_config const& cfg = ctx.cfg;
+    double const gcut = cfg.photon_cutoff_eV;
+    double const dt = cfg.dt_s;
+    uint64_t const step = static_cast<uint64_t>(std::max<int64_t>(cfg.step, 0));
+    uint64_t const seed = cfg.rng_seed;
+    uint64_t const root = p.root_key;
+    uint64_t const rsp = static_cast<uint64_t>(p.root_species);
+    double const w = p.weight;
+    Emitter em{fx, ip, root, w, out.ion_pairs, p.root_species, p.generation + 1};
+    double const residual = newborn_residual_fraction(p.birth_fraction);
+    if (!(residual > 0.0)) {
+        if (!(p.energy_eV >= gcut)) out.status = ORC_DEMOTED;
+        return;
+    }
+    if (!in_half_open(p.x, p.y, p.z, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi)) {
+        fail(ORC_ERR_OUTSIDE_DOMAIN, "photon transient born outside the half-open domain");
+    }
+    double dx = p.dir_x, dy = p.dir_y, dz = p.dir_z;
+    normalize(dx, dy, dz);
+    double energy = p.energy_eV;
+    if (energy < gcut) {
+        em.deposit_local(energy);
+        out.status = ORC_DEMOTED;
+        fx.tallies[ORC_T_NEWBORN_TERMINATED] += 1.0;
+        return;
+    }
+    auto escape = [&]() {
+        out.status = ORC_ESCAPE;
+        fx.tallies[ORC_T_NEWBORN_ESCAPES] += 1.0;
+    };
+    auto survive_at = [&](double x, double y, double z) {
+        p.x = x; p.y = y; p.z = z;
+        p.dir_x = dx; p.dir_y = dy; p.dir_z = dz;
+        p.energy_eV = energy;
+        p.birth_fraction = 1.0;
+    };
+    double const chord = kC * dt * residual;
+    double const end_x = p.x + chord * dx;
+    double const end_y = p.y + chord * dy;
+    double const end_z = p.z + chord * dz;
+    Path path = clip_rz_path(std::hypot(p.x, p.y), std::atan2(p.y, p.x), p.z, std::hypot(end_x, end_y),
+                             std::atan2(end_y, end_x), end_z, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
+    if (!(path.full_length > 0.0) || !(path.in_length > 0.0)) {
+        if (path.exits) escape();
+        return;
+    }
+    double elapsed_start = p.birth_fraction;
+    uint64_t ord = 0;
+    for (;;) {
+        uint64_t idx = 0;
+        if (!pack_transient_index(true, rsp, p.lineage, 0U, ord, 0U, idx)) {
+            fail(ORC_ERR_INVALID, "newborn photon RNG index overflow");
+        }
+        std::vector<Interval> const iv = build_intervals(ctx.cp, path, ctx.density_ratio);
+        double const mfp = photon_mfp(t, energy);
+        double const tau = density_path(iv, path, path.begin_f, path.end_f) / mfp;
+        double const taus[3] = {tau, 0.0, 0.0};
+        Hazard const h = select_hazard(taus, 1, rng_uniform(seed, root, step, idx, kChNewbornCompetingOpticalDepth), 0.0);
+        if (!h.event) {
+            double x, y, z;
+            if (!newborn_final_point(path, path.end_f, x, y, z)) {
+                fail(ORC_ERR_INVALID, "newborn photon could not place its final point");
+            }
+            if (path.exits) { escape(); return; }
+            survive_at(x, y, z);
+            return;
+        }
+        double const f = fraction_at_density_path(iv, path, path.begin_f, path.end_f, h.sampled * mfp);
+        double const dist = path.full_length * (f - path.begin_f);
+        double const remaining = std::max(0.0, path.full_length - dist);
+        double x, y, z;
+        path.point_at_full(f, x, y, z);
+        double const r_site = std::hypot(x, y);
+        double const el = std::min(1.0, elapsed_start + dist / std::max(kC * dt, std::numeric_limits<double>::min()));
+        int const channel = sample_photon_channel(t, energy, rng_uniform(seed, root, step, idx, kChNewbornCompetingChannel));
+        if (!photon_final_state(ctx, em, channel, idx, x, y, z, el, energy, dx, dy, dz)) {
+            out.status = ORC_ABSORBED;
+            fx.tallies[ORC_T_NEWBORN_TERMINATED] += 1.0;
+            return;
+        }
+        if (!(remaining > 0.0)) { survive_at(x, y, z); return; }
+        double const tx = x + remaining * dx, ty = y + remaining * dy, tz = z + remaining * dz;
+        path = clip_rz_path(r_site, std::atan2(y, x), z, std::hypot(tx, ty), std::atan2(ty, tx), tz, ctx.r_lo, ctx.r_hi,
+                            ctx.z_lo, ctx.z_hi);
+        elapsed_start = el;
+        ++ord;
     }
 }
 
@@ -3086,3 +3669,142 @@ double orc_sample_positron_brems_energy(void* h, double e, double u, int* err)
 }
 
 }  // extern "C"
+
+// The transient FIFO of DrainAndCommitSecondaries
+// (OV/RreaSecondaryCommit.cpp:338-468): the step's staged secondaries in
+// Apply's order (electron, photon, then positron loop, each merged in chunk
+// order) seed a FIFO; every entry born before the end of the step is flown
+// through its residual time, and the children it makes join the back of the
+// FIFO with the root's next free transient lineage.  `work` receives the
+// whole FIFO (initial entries first), `status` and `ion_pairs` one value per
+// FIFO entry, and `survivors` the entries that reach the end of the step, in
+// FIFO order, at their transported state with birth fraction 1.  Net chords
+// deposit into jr/jz/rho; tallies accumulate.
+extern "C" int orc_newborn_drain(void* h, const orc_geometry* g, const orc_config* cfg, const orc_secondary* initial,
+                                 int64_t n_initial, int64_t cap, orc_secondary* work, int64_t* n_work,
+                                 int32_t* status, double* ion_pairs, orc_secondary* survivors, int64_t* n_surv,
+                                 double* jr, double* jz, double* rho, double* tallies)
+{
+    *n_work = 0;
+    *n_surv = 0;
+    auto const* t = static_cast<Tables const*>(h);
+    if (g->r_lo != 0.0) return ORC_ERR_INVALID;
+    StepContext ctx;
+    ctx.t = t;
+    ctx.cp = Coupling{*g, t->reference_density, 0.0, 0.0};
+    ctx.cfg = *cfg;
+    ctx.r_lo = g->r_lo;
+    ctx.r_hi = g->r_lo + g->nr * g->dr;
+    ctx.z_lo = g->z_lo;
+    ctx.z_hi = g->z_lo + g->nz * g->dz;
+    ctx.cp.r_hi = ctx.r_hi;
+    ctx.cp.z_hi = ctx.z_hi;
+    int const pnr = g->nr, pnz = g->nz, pad = 2;
+    Grid grid{pnr + pad, pnz + 2 * pad, g->dr, g->dz, g->r_lo, g->z_lo - pad * g->dz, rho, jr, jz};
+    try {
+        ctx.density_ratio = positive_or(ctx.cp.density_ratio_at_z(0.5 * (ctx.z_lo + ctx.z_hi)), 1.0);
+        if (!(cfg->dt_s > 0.0)) return 0;
+        std::vector<orc_secondary> fifo(initial, initial + std::max<int64_t>(n_initial, 0));
+        // the FIFO's birth-payload validation, OV/RreaSecondaryCommit.cpp:338-376
+        for (auto const& p : fifo) {
+            bool const finite = std::isfinite(p.x) && std::isfinite(p.y) && std::isfinite(p.z) && std::isfinite(p.dir_x)
+                && std::isfinite(p.dir_y) && std::isfinite(p.dir_z) && std::isfinite(p.weight)
+                && std::isfinite(p.energy_eV) && std::isfinite(p.birth_fraction);
+            bool const energy_ok = p.species == 2 ? p.energy_eV > 0.0 : p.energy_eV >= 0.0;
+            if (p.species < 0 || p.species > 2 || p.root_species < 0 || p.root_species > 2 || !finite
+                || !(p.weight > 0.0) || !energy_ok || p.birth_fraction < 0.0 || p.birth_fraction > 1.0
+                || p.lineage >= kLineageCap) {
+                fail(ORC_ERR_INVALID, "transient FIFO received an invalid birth payload");
+            }
+        }
+        // next free lineage per (root key, root species), :390-406
+        std::map<std::pair<uint64_t, int32_t>, uint64_t> next_lineage;
+        for (auto const& p : fifo) {
+            auto& nx = next_lineage[{p.root_key, p.root_species}];
+            nx = std::max(nx, p.lineage + 1U);
+        }
+        std::vector<int32_t> st;
+        std::vector<double> ion;
+        std::vector<orc_secondary> surv;
+        for (size_t i = 0; i < fifo.size(); ++i) {
+            orc_secondary p = fifo[i];
+            // transient_energy_is_certified :148-180
+            bool const photon = p.species == 2;
+            double const lo = photon ? t->photon_min : t->charged_min;
+            double const hi = photon ? t->photon_max : t->charged_max;
+            if (!std::isfinite(p.energy_eV) || p.energy_eV < lo || p.energy_eV > hi) {
+                fail(ORC_ERR_TABLE_RANGE, "transient FIFO particle outside its certified energy range");
+            }
+            ChunkLog fx;
+            NewbornOut o;
+            if (p.birth_fraction < 1.0) {
+                fx.tallies[ORC_T_NEWBORN_ADVANCED] += 1.0;
+                if (photon) photon_newborn(ctx, fx, p, static_cast<int64_t>(i), o);
+                else charged_newborn(ctx, fx, p, static_cast<int64_t>(i), o);
+            }
+            for (auto c : fx.secondaries) {
+                auto& nx = next_lineage[{c.root_key, c.root_species}];
+                if (nx >= kLineageCap) fail(ORC_ERR_INVALID, "transient FIFO lineage overflow");
+                c.lineage = nx++;
+                fifo.push_back(c);
+            }
+            for (int q = 0; q < ORC_T_COUNT; ++q) tallies[q] += fx.tallies[q];
+            for (auto const& op : fx.deposits) {
+                if (op.kind == 0) {
+                    if (op.a[4] != 0.0) deposit_segment(grid, op.a[4], cfg->dt_s, op.a[0], op.a[1], op.a[2], op.a[3]);
+                } else if (op.a[6] != 0.0
+                           && !deposit_escaping_segment(grid, op.a[6], cfg->dt_s, op.a[0], op.a[1], op.a[2], op.a[3],
+                                                        op.a[4], op.a[5], pnr, pad, pnz, op.faces)) {
+                    fail(ORC_ERR_INVALID, "newborn escape deposit rejected");
+                }
+            }
+            st.push_back(o.status);
+            ion.push_back(o.ion_pairs);
+            if (o.status == ORC_SURVIVE) {
+                p.birth_fraction = 1.0;
+                surv.push_back(p);
+                if (fifo[i].birth_fraction < 1.0) tallies[ORC_T_NEWBORN_SURVIVORS] += 1.0;
+            }
+        }
+        if (static_cast<int64_t>(fifo.size()) > cap) return ORC_ERR_SECONDARY_CAPACITY;
+        for (size_t i = 0; i < fifo.size(); ++i) {
+            work[i] = fifo[i];
+            status[i] = st[i];
+            ion_pairs[i] = ion[i];
+        }
+        for (size_t i = 0; i < surv.size(); ++i) survivors[i] = surv[i];
+        *n_work = static_cast<int64_t>(fifo.size());
+        *n_surv = static_cast<int64_t>(surv.size());
+    } catch (OracleError const& e) {
+        return e.code;
+    }
+    return 0;
+}
+
+// RreaTryPackTransientRngIndex / RreaTryLiveHazardRngIndex
+// (OV/RreaParticleInteraction.H:660-710): 1 and the packed value, or 0.
+extern "C" int orc_pack_transient_index(int newborn, uint64_t species, uint64_t lineage, uint64_t segment,
+                                        uint64_t event, uint64_t inner, uint64_t* out)
+{
+    return pack_transient_index(newborn != 0, species, lineage, segment, event, inner, *out) ? 1 : 0;
+}
+
+extern "C" int orc_live_hazard_index(uint64_t species, uint64_t segment, uint64_t event, uint64_t* out)
+{
+    return live_hazard_index(species, segment, event, *out) ? 1 : 0;
+}
+
+// RreaEvaluateElectronStatePolicy OV/RreaParticleInteraction.H:1296-1317 as
+// the newborn evaluates it: 1 when the state demotes.
+extern "C" int orc_electron_state_demotes(double energy, double cutoff, double threshold, double field, double drag)
+{
+    bool const below = energy < cutoff;
+    bool const nonrunaway = threshold > 0.0 && energy <= threshold && field <= drag;
+    return below || nonrunaway ? 1 : 0;
+}
+
+// RreaNewbornElapsedStepFraction OV/RreaParticleInteraction.H:181-197
+extern "C" double orc_newborn_elapsed(double birth, uint64_t completed, uint64_t substeps, double within)
+{
+    return newborn_elapsed(birth, newborn_residual_fraction(birth), completed, substeps, within);
+}

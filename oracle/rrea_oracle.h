@@ -67,6 +67,8 @@ typedef struct orc_secondary {
     double birth_fraction;
     uint64_t root_key;
     uint64_t lineage;
+    int32_t root_species; /* species of the tracked root particle: 0 e-, 1 e+, 2 photon */
+    int32_t generation;   /* 0: staged by a live step; g: created by a generation-g newborn */
 } orc_secondary;
 
 /* Per-particle status written by the step. */
@@ -99,7 +101,12 @@ enum {
     ORC_T_COMPTON_EVENTS,
     ORC_T_PHOTOELECTRIC_EVENTS,
     ORC_T_PAIR_EVENTS,
-    ORC_T_LOW_ENERGY_ELECTRONS, /* weight of sub-cutoff pair electrons booked as low-energy sources */
+    ORC_T_LOW_ENERGY_ELECTRONS, /* weight booked as low-energy electron sources: demoted electrons and
+                                   sub-cutoff pair electrons */
+    ORC_T_NEWBORN_ADVANCED,     /* newborns flown through their residual (1 - f) dt */
+    ORC_T_NEWBORN_SURVIVORS,    /* newborns that reach the end of the step */
+    ORC_T_NEWBORN_ESCAPES,      /* newborns that leave the domain */
+    ORC_T_NEWBORN_TERMINATED,   /* newborns demoted, annihilated or absorbed */
     ORC_T_COUNT = 24,
 };
 

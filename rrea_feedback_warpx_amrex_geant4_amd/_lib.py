@@ -95,6 +95,7 @@ class Secondary(C.Structure):
         ("dir_x", C.c_double), ("dir_y", C.c_double), ("dir_z", C.c_double),
         ("weight", C.c_double), ("energy_eV", C.c_double), ("birth_fraction", C.c_double),
         ("root_key", C.c_uint64), ("lineage", C.c_uint64),
+        ("root_species", C.c_int32), ("generation", C.c_int32),
     ]
 
 
@@ -104,6 +105,16 @@ class StepOut(C.Structure):
         ("secondaries", C.c_void_p), ("secondary_capacity", C.c_int64), ("secondary_count", C.c_void_p),
         ("jr", C.c_void_p), ("jz", C.c_void_p), ("rho", C.c_void_p), ("tallies", C.c_void_p),
         ("error", C.c_void_p), ("order_secondaries", C.c_int32),
+    ]
+
+
+class DrainIO(C.Structure):
+    _fields_ = [
+        ("work", C.c_void_p), ("work_capacity", C.c_int64), ("n_initial", C.c_int64), ("work_count", C.c_void_p),
+        ("status", C.c_void_p), ("ion_pairs", C.c_void_p), ("survivors", C.c_void_p),
+        ("survivor_capacity", C.c_int64), ("survivor_count", C.c_void_p),
+        ("jr", C.c_void_p), ("jz", C.c_void_p), ("rho", C.c_void_p), ("tallies", C.c_void_p),
+        ("error", C.c_void_p),
     ]
 
 
@@ -131,6 +142,8 @@ SIGNATURES = [
     ("rrea_transport_step_photons", C.c_int32,
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
     ("rrea_photon_push", C.c_int32, [C.POINTER(Tile), C.c_double, C.c_void_p]),
+    ("rrea_newborn_drain", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.POINTER(StepConfigC), C.POINTER(DrainIO), C.c_void_p]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),

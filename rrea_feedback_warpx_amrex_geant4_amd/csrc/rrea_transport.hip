@@ -170,6 +170,7 @@ struct rrea_fields_s {
     double* d_stage = nullptr;   // staging for host-pointer updates
     size_t stage_len = 0;
     double* d_prof = nullptr;    // altitude | rho
+    std::vector<double> h_prof_alt, h_prof_rho;  // host copy, for the reference-z ratio
     int64_t prof_n = 0;
     int32_t* d_prof_guide = nullptr;
     double prof_guide_lo = 0.0, prof_guide_scale = 0.0;
@@ -192,6 +193,18 @@ struct rrea_fields_s {
     int32_t* d_slow_list = nullptr;  // fast-path hand-over list
     int32_t* d_slow_count = nullptr;
     int64_t slow_cap = 0;
+    // newborn FIFO workspace
+    void* d_nargs = nullptr;
+    rrea_secondary* d_fin = nullptr;
+    uint8_t* d_fst = nullptr;
+    double* d_fion = nullptr;
+    int64_t* d_anchor = nullptr;
+    unsigned long long* d_next = nullptr;
+    int32_t* d_flag = nullptr;
+    int64_t* d_nsel = nullptr;
+    void* d_sel_tmp = nullptr;
+    size_t sel_tmp_bytes = 0;
+    int64_t fifo_cap = 0;
     bool timing = false;
     struct Timed { hipEvent_t start, stop; int32_t kernel; };
     std::vector<Timed> events;   // pool; the first events_used are recorded launches
@@ -282,6 +295,8 @@ struct Emitter {
     double& ion;
     uint64_t& lineage;
     int32_t& err;
+    int32_t root_species;   // species of the tracked root of every record made here
+    int32_t generation;     // FIFO generation of the records made here
 
     __device__ void deposit_local(double energy)
     {
@@ -303,6 +318,7 @@ struct Emitter {
         s.dir_x = sdx; s.dir_y = sdy; s.dir_z = sdz;
         s.weight = w; s.energy_eV = energy; s.birth_fraction = birth;
         s.root_key = pkey; s.lineage = lineage++;
+        s.root_species = root_species; s.generation = generation;
         push_secondary(A, s, err);
     }
 };
@@ -594,7 +610,7 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
             }
             double tr_dp = 0.0, tr_len = 0.0;
 
-            Emitter em{A, ip, pkey, w, ion, lineage, err};
+            Emitter em{A, ip, pkey, w, ion, lineage, err, kPos ? 1 : 0, 0};
             auto deposit_local = [&](double energy) { em.deposit_local(energy); };
             auto push = [&](int32_t species, int32_t process, double x, double y, double z, double sdx, double sdy,
                             double sdz, double energy, double birth) {
@@ -1068,6 +1084,9 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                     status = RREA_ESCAPE;
                     tally[1] += 1.0;
                 } else if (terminal == kDemote) {
+                    // a low-energy electron source of the parent's weight
+                    // (AccumulateLowEnergyElectron, OV/RreaElectronTransport.cpp:152, 857)
+                    tally[16] += w;
                     deposit_local(k);
                     if (w != 0.0 && !deposit_segment(A.G, -w, dt, r_particle, z_particle, demote_r, demote_z)) err = kErrInvalid;
                     status = RREA_DEMOTED;
@@ -1100,6 +1119,104 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
         }
         }
     }
+}
+
+// The photon final states at an interaction site, shared by the live loop
+// and the newborn residual advance: Compton (RreaApplyComptonFinalState
+// OV/RreaPhotonFinalStates.H:54-140), photoelectric absorption
+// (RreaApplyPhotoelectricFinalState :153-224) and pair production
+// (RreaApplyPairFinalState :239-363).  `rng` is the emitter's key prefix and
+// `idx` the pass's interaction index.  Returns true when the photon survives
+// (Compton at or above the photon cutoff) with its new energy and direction;
+// false when it is absorbed or on error (em.err).
+__device__ __noinline__ bool photon_final_state(DevTables const& T, double cutoff, double gcut, uint64_t rng,
+                                                Emitter& em, int channel, uint64_t idx, double x, double y, double z,
+                                                double birth, double& energy, double& dx, double& dy, double& dz,
+                                                double* tally)
+{
+    int32_t& err = em.err;
+    double const w = em.w;
+    double& ion = em.ion;
+    if (channel == kCompton) {
+        double const incident = energy;
+        double scattered, recoil, ct;
+        sample_compton(incident, rng_uniform(rng, idx, kChComptonRecoil), rng_uniform(rng, idx, kChComptonScatter),
+                       scattered, recoil, ct);
+        double sx, sy, sz;
+        cone_direction(dx, dy, dz, ct, rng_uniform(rng, idx, kChComptonScatterAzimuth), sx, sy, sz);
+        double rx = incident * dx - scattered * sx;
+        double ry = incident * dy - scattered * sy;
+        double rz = incident * dz - scattered * sz;
+        normalize(rx, ry, rz);
+        if (recoil >= cutoff) {
+            em.secondary(0, kProcCompton, x, y, z, rx, ry, rz, recoil, birth);
+            ion += w;
+        } else {
+            ion += w;
+            em.deposit_local(recoil);
+        }
+        tally[13] += 1.0;
+        energy = scattered;
+        if (energy < gcut) {
+            em.deposit_local(energy);
+            return false;
+        }
+        dx = sx; dy = sy; dz = sz;
+        return true;
+    }
+    if (channel == kPhotoelectric) {
+        double binding;
+        if (!sample_photoelectric(T, energy, rng_uniform(rng, idx, kChPhotoelectric), binding, err)) return false;
+        double const ke = dmin(dmax(0.0, energy - binding), energy);
+        if (ke >= cutoff) {
+            if (!(ke == 0.0) && !(finite(ke) && ke >= T.charged_min && ke <= T.charged_max)) {
+                err = kErrRange;
+                return false;
+            }
+            double ex, ey, ez;
+            cone_direction(dx, dy, dz, sample_photoelectron_cos(ke, rng_uniform(rng, idx, kChPhotoelectricAngle)),
+                           rng_uniform(rng, idx, kChPhotoelectricAzimuth), ex, ey, ez);
+            em.secondary(0, kProcPhotoelectric, x, y, z, ex, ey, ez, ke, birth);
+            ion += w;
+        } else {
+            ion += w;
+            em.deposit_local(ke);
+        }
+        if (binding > 0.0) em.deposit_local(binding);
+        tally[14] += 1.0;
+        return false;
+    }
+    PairState const pair = sample_pair(T, energy, channel, rng_uniform(rng, idx, kChPairInvariant),
+                                       rng_uniform(rng, idx, kChPairAngle), rng_uniform(rng, idx, kChPairAzimuth), err);
+    if (err) return false;
+    if (!pair.valid) { err = kErrInvalid; return false; }
+    bool const triplet = channel == kPairTriplet;
+    if (triplet && !pair.has_recoil) { err = kErrInvalid; return false; }
+    int32_t const process = triplet ? kProcPairTriplet : kProcPairNuclear;
+    double ex, ey, ez, px, py, pz2;
+    rotate_to_axis(dx, dy, dz, pair.edx, pair.edy, pair.edz, ex, ey, ez);
+    rotate_to_axis(dx, dy, dz, pair.pdx, pair.pdy, pair.pdz, px, py, pz2);
+    if (pair.electron >= cutoff) {
+        em.secondary(0, process, x, y, z, ex, ey, ez, pair.electron, birth);
+    } else {
+        tally[16] += w;
+        em.deposit_local(pair.electron);
+    }
+    if (pair.positron >= cutoff) {
+        em.secondary(1, process, x, y, z, px, py, pz2, pair.positron, birth);
+    } else {
+        spawn_annihilation(T, gcut, rng, em, process, x, y, z, px, py, pz2, pair.positron, true, idx, birth, tally);
+    }
+    if (pair.has_recoil) {
+        double qx, qy, qz;
+        rotate_to_axis(dx, dy, dz, pair.rdx, pair.rdy, pair.rdz, qx, qy, qz);
+        if (pair.recoil >= cutoff) em.secondary(0, process, x, y, z, qx, qy, qz, pair.recoil, birth);
+        ion += w;
+        if (pair.recoil < cutoff) em.deposit_local(pair.recoil);
+    }
+    if (pair.local > 0.0) em.deposit_local(pair.local);
+    tally[15] += 1.0;
+    return false;
 }
 
 // One photon: the body of the particle loop of OV/RreaPhotonTransport.cpp:56-438.
@@ -1141,7 +1258,7 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
     double ion = 0.0;
     uint64_t lineage = 0;
     uint64_t events = 0;
-    Emitter em{A, ip, pkey, w, ion, lineage, err};
+    Emitter em{A, ip, pkey, w, ion, lineage, err, 2, 0};
 
     double energy = sqrt(ux * ux + uy * uy + uz * uz);
     double dx = ux, dy = uy, dz = uz;
@@ -1216,108 +1333,34 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
             else channel = kPairTriplet;
         }
         events = ordinal + 1U;
-        if (channel == kCompton) {
-            // RreaApplyComptonFinalState OV/RreaPhotonFinalStates.H:54-140
-            double const incident = energy;
-            double scattered, recoil, ct;
-            sample_compton(incident, rng_uniform(rng, idx, kChComptonRecoil), rng_uniform(rng, idx, kChComptonScatter),
-                           scattered, recoil, ct);
-            double sx, sy, sz;
-            cone_direction(dx, dy, dz, ct, rng_uniform(rng, idx, kChComptonScatterAzimuth), sx, sy, sz);
-            double rx = incident * dx - scattered * sx;
-            double ry = incident * dy - scattered * sy;
-            double rz = incident * dz - scattered * sz;
-            normalize(rx, ry, rz);
-            if (recoil >= cutoff) {
-                em.secondary(0, kProcCompton, x, y, z, rx, ry, rz, recoil, birth);
-                ion += w;
-            } else {
-                ion += w;
-                em.deposit_local(recoil);
-            }
-            tally[13] += 1.0;
-            energy = scattered;
-            if (energy < gcut) {
-                em.deposit_local(energy);
-                status = RREA_ABSORBED;
-                break;
-            }
-            dx = sx; dy = sy; dz = sz;
-            o_ux = energy * dx; o_uy = energy * dy; o_uz = energy * dz;
-            write_u = true;
-            double const xt = x + dx * remaining, yt = y + dy * remaining, zt = z + dz * remaining;
-            Path cont;
-            if (!clip_rz_path(r_site, atan2(y, x), z, hypot(xt, yt), atan2(yt, xt), zt, D, cont, err)) break;
-            double xn, yn, zn;
-            point_at_full(cont, cont.end_f, xn, yn, zn);
-            o_r = hypot(xn, yn);
-            o_th = atan2(yn, xn);
-            o_z = zn;
-            write_pos = true;
-            if (!(remaining > 0.0)) {
-                status = RREA_SURVIVE;
-                tally[0] += 1.0;
-                break;
-            }
-            path = cont;
-            elapsed_start = birth;
-            ++ordinal;
-            continue;
-        }
-        if (channel == kPhotoelectric) {
-            // RreaApplyPhotoelectricFinalState OV/RreaPhotonFinalStates.H:153-224
-            double binding;
-            if (!sample_photoelectric(T, energy, rng_uniform(rng, idx, kChPhotoelectric), binding, err)) break;
-            double const ke = dmin(dmax(0.0, energy - binding), energy);
-            if (ke >= cutoff) {
-                if (!(ke == 0.0) && !(finite(ke) && ke >= T.charged_min && ke <= T.charged_max)) { err = kErrRange; break; }
-                double ex, ey, ez;
-                cone_direction(dx, dy, dz, sample_photoelectron_cos(ke, rng_uniform(rng, idx, kChPhotoelectricAngle)),
-                               rng_uniform(rng, idx, kChPhotoelectricAzimuth), ex, ey, ez);
-                em.secondary(0, kProcPhotoelectric, x, y, z, ex, ey, ez, ke, birth);
-                ion += w;
-            } else {
-                ion += w;
-                em.deposit_local(ke);
-            }
-            if (binding > 0.0) em.deposit_local(binding);
-            tally[14] += 1.0;
+        bool const survives = photon_final_state(T, cutoff, gcut, rng, em, channel, idx, x, y, z, birth, energy, dx, dy,
+                                                 dz, tally);
+        if (err) break;
+        if (!survives) {
             status = RREA_ABSORBED;
             break;
         }
-        // RreaApplyPairFinalState OV/RreaPhotonFinalStates.H:239-363
-        PairState const pair = sample_pair(T, energy, channel, rng_uniform(rng, idx, kChPairInvariant),
-                                           rng_uniform(rng, idx, kChPairAngle), rng_uniform(rng, idx, kChPairAzimuth), err);
-        if (err) break;
-        if (!pair.valid) { err = kErrInvalid; break; }
-        bool const triplet = channel == kPairTriplet;
-        if (triplet && !pair.has_recoil) { err = kErrInvalid; break; }
-        int32_t const process = triplet ? kProcPairTriplet : kProcPairNuclear;
-        double ex, ey, ez, px, py, pz2;
-        rotate_to_axis(dx, dy, dz, pair.edx, pair.edy, pair.edz, ex, ey, ez);
-        rotate_to_axis(dx, dy, dz, pair.pdx, pair.pdy, pair.pdz, px, py, pz2);
-        if (pair.electron >= cutoff) {
-            em.secondary(0, process, x, y, z, ex, ey, ez, pair.electron, birth);
-        } else {
-            tally[16] += w;
-            em.deposit_local(pair.electron);
+        // a Compton survivor continues from the interaction point along its
+        // scattered direction for the unused path length
+        o_ux = energy * dx; o_uy = energy * dy; o_uz = energy * dz;
+        write_u = true;
+        double const xt = x + dx * remaining, yt = y + dy * remaining, zt = z + dz * remaining;
+        Path cont;
+        if (!clip_rz_path(r_site, atan2(y, x), z, hypot(xt, yt), atan2(yt, xt), zt, D, cont, err)) break;
+        double xn, yn, zn;
+        point_at_full(cont, cont.end_f, xn, yn, zn);
+        o_r = hypot(xn, yn);
+        o_th = atan2(yn, xn);
+        o_z = zn;
+        write_pos = true;
+        if (!(remaining > 0.0)) {
+            status = RREA_SURVIVE;
+            tally[0] += 1.0;
+            break;
         }
-        if (pair.positron >= cutoff) {
-            em.secondary(1, process, x, y, z, px, py, pz2, pair.positron, birth);
-        } else {
-            spawn_annihilation(T, gcut, rng, em, process, x, y, z, px, py, pz2, pair.positron, true, idx, birth, tally);
-        }
-        if (pair.has_recoil) {
-            double qx, qy, qz;
-            rotate_to_axis(dx, dy, dz, pair.rdx, pair.rdy, pair.rdz, qx, qy, qz);
-            if (pair.recoil >= cutoff) em.secondary(0, process, x, y, z, qx, qy, qz, pair.recoil, birth);
-            ion += w;
-            if (pair.recoil < cutoff) em.deposit_local(pair.recoil);
-        }
-        if (pair.local > 0.0) em.deposit_local(pair.local);
-        tally[15] += 1.0;
-        status = RREA_ABSORBED;
-        break;
+        path = cont;
+        elapsed_start = birth;
+        ++ordinal;
     }
     if (err) {
         report(A.error, err, ip);
@@ -1335,17 +1378,18 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
 }
 
 // workgroup reduction of the tallies, one atomic per tally per workgroup
+template <int kSlots = kTallyUsed>
 __device__ __forceinline__ void reduce_tallies(double* tally, double* out)
 {
-    __shared__ double red[kTallyUsed][kBlock / 64];
+    __shared__ double red[kSlots][kBlock / 64];
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
-    for (int q = 0; q < kTallyUsed; ++q) {
+    for (int q = 0; q < kSlots; ++q) {
         double const s = wave_sum(tally[q]);
         if (lane == 0) red[q][wave] = s;
     }
     __syncthreads();
-    if (threadIdx.x < kTallyUsed) {
+    if (threadIdx.x < kSlots) {
         double s = 0.0;
         for (int wv = 0; wv < kBlock / 64; ++wv) s += red[threadIdx.x][wv];
         if (s != 0.0) atomic_add(&out[threadIdx.x], s);
@@ -1636,6 +1680,777 @@ __global__ void __launch_bounds__(kBlock) positron_full_kernel(const StepArgs* _
 // Deterministic secondary order: key = (parent index, lineage), the order of
 // the reference's chunk-ordered merge (RreaElectronTransport.cpp:892-897).
 // ---------------------------------------------------------------------------
+// ===========================================================================
+// Newborn residual advance (OV/RreaNewbornAdvance.cpp) and the transient FIFO
+// of DrainAndCommitSecondaries (OV/RreaSecondaryCommit.cpp:338-468).  A
+// secondary born at elapsed fraction f < 1 is flown through the remaining
+// (1 - f) dt with the live loops' physics; its draws come from the ROOT
+// particle's stream at the packed transient lineage.  The FIFO runs one
+// generation per launch: within a generation every root's entries are
+// contiguous (the step's output is ordered by parent, and children are
+// ordered by parent), which is what lets lineages be allocated per run.
+// ===========================================================================
+constexpr int kTallyNewborn = 21;
+constexpr uint64_t kLineageCap = uint64_t{1} << 14;
+constexpr uint64_t kInnerCap = uint64_t{1} << 8;
+
+// RNG channels of the residual advance, OV/RreaRng.H:65-81
+enum : uint64_t {
+    kChNewbornElectronGsAngle = 112,
+    kChNewbornElectronGsAzimuth = 113,
+    kChNewbornElectronGsFewCount = 114,
+    kChNewbornElectronGsFewScatter = 115,
+    kChNewbornElectronGsFewAzimuth = 116,
+    kChNewbornPositronGsAngle = 117,
+    kChNewbornPositronGsAzimuth = 118,
+    kChNewbornPositronGsFewCount = 119,
+    kChNewbornPositronGsFewScatter = 120,
+    kChNewbornPositronGsFewAzimuth = 121,
+    kChNewbornCompetingOpticalDepth = 122,
+    kChNewbornCompetingChannel = 123,
+};
+
+// One generation of the FIFO: entries [gb, ge) of `work`.  The children the
+// generation makes are staged through S.sec / S.sec_count.
+struct NewbornArgs {
+    StepArgs S;
+    const rrea_secondary* work;
+    int64_t gb, ge;
+    rrea_secondary* fin;   // per entry: the state at the end of the step
+    uint8_t* status;       // per entry
+    double* ion;           // per entry: ion-pair weight
+};
+
+// RreaTryPackTransientRngIndex OV/RreaParticleInteraction.H:660-686 with the
+// newborn marker (bit 51)
+__device__ __forceinline__ uint64_t newborn_index(uint64_t rsp, uint64_t lineage, uint64_t segment, uint64_t event,
+                                                  uint64_t inner, int32_t& err)
+{
+    if (rsp >= 3U || lineage >= kLineageCap || segment >= kSegmentCap || event >= kEventCap || inner >= kInnerCap) {
+        if (!err) err = kErrInvalid;
+        return 0;
+    }
+    return (uint64_t{1} << 51) | (rsp << 49) | (lineage << 35) | (segment << 21) | (event << 8) | inner;
+}
+
+// RreaNewbornResidualFraction / RreaNewbornElapsedStepFraction
+// OV/RreaParticleInteraction.H:171-197
+__device__ __forceinline__ double newborn_residual(double birth) { return dmin(1.0, dmax(0.0, 1.0 - birth)); }
+
+__device__ __forceinline__ double newborn_elapsed(double birth, double residual, uint64_t completed,
+                                                  uint64_t substeps, double f)
+{
+    double const within = dmin(1.0, dmax(0.0, f));
+    double const consumed = (static_cast<double>(completed) + within) / static_cast<double>(substeps);
+    return dmin(1.0, birth + residual * consumed);
+}
+
+// TransportStep::NewbornFinalPoint OV/RreaNewbornAdvance.cpp:20-48: the point
+// at `fraction`, pulled back along the chord until it is in the half-open domain
+__device__ bool newborn_final_point(Path const& p, Domain const& D, double fraction, double& x, double& y, double& z)
+{
+    double const span = fraction - 0.0;
+    double pullback = 1.0e-9;
+    for (int attempt = 0; attempt < 5; ++attempt) {
+        double const trial = attempt == 0 ? fraction : dmax(0.0, fraction - pullback * dmax(span, 1.0e-12));
+        point_at_full(p, trial, x, y, z);
+        if (in_half_open(x, y, z, D)) return true;
+        if (attempt > 0) pullback *= 100.0;
+    }
+    return false;
+}
+
+// The newborn's electron_state_policy_at (OV/RreaNewbornAdvance.cpp:79-96):
+// the demotion gate with the field gathered at the point and the density
+// ratio at its z
+__device__ bool newborn_electron_demotes(StepArgs const& A, double x, double y, double z, double k, int32_t& err)
+{
+    double const cutoff = A.cfg.low_energy_cutoff_eV;
+    double const thr = A.T.hard_moller_threshold;
+    double er, ez, bt;
+    gather(A.F, hypot(x, y), z, er, ez, bt);
+    double const ratio = positive_or(density_ratio_at_z(A.F, z, err), 1.0);
+    double const drag = ensemble_drag(A.T, dmax(k, cutoff), ratio, A.cfg.disable_hard_moller == 0, err);
+    return k < cutoff || (thr > 0.0 && k <= thr && hypot(er, ez) <= drag);
+}
+
+// TransportStep::AdvanceChargedNewborn OV/RreaNewbornAdvance.cpp:50-777
+template <bool kPos>
+__device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, double* tally)
+{
+    using S = Species<kPos>;
+    StepArgs const& A = N.S;
+    DevTables const& T = A.T;
+    DevFields const& F = A.F;
+    Domain const& D = A.D;
+    rrea_step_config const& cfg = A.cfg;
+    double const cutoff = cfg.low_energy_cutoff_eV;
+    double const gcut = cfg.photon_cutoff_eV;
+    double const thr = T.hard_moller_threshold;
+    bool const hm_on = cfg.disable_hard_moller == 0;
+    double const dt = cfg.dt_s;
+    uint64_t const step = static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0);
+    DevElastic const& EL = kPos ? T.el_p : T.el_e;
+    constexpr uint64_t kFewCount = kPos ? kChNewbornPositronGsFewCount : kChNewbornElectronGsFewCount;
+    constexpr uint64_t kFewScatter = kPos ? kChNewbornPositronGsFewScatter : kChNewbornElectronGsFewScatter;
+    constexpr uint64_t kFewPhi = kPos ? kChNewbornPositronGsFewAzimuth : kChNewbornElectronGsFewAzimuth;
+    constexpr uint64_t kGsAngle = kPos ? kChNewbornPositronGsAngle : kChNewbornElectronGsAngle;
+    constexpr uint64_t kGsPhi = kPos ? kChNewbornPositronGsAzimuth : kChNewbornElectronGsAzimuth;
+
+    rrea_secondary p = N.work[i];
+    int32_t err = kOk;
+    double ion = 0.0;
+    uint64_t lineage = 0;
+    int status = RREA_SURVIVE;
+    uint64_t const rsp = static_cast<uint64_t>(p.root_species);
+    uint64_t const lin = p.lineage;
+    double const w = p.weight;
+    uint64_t const rng = rng_prefix(cfg.rng_seed, p.root_key, step);
+    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1};
+    double const birth_r = hypot(p.x, p.y), birth_z = p.z;
+    double const residual = newborn_residual(p.birth_fraction);
+    double k = p.energy_eV;
+    double dx = p.dir_x, dy = p.dir_y, dz = p.dir_z;
+    normalize(dx, dy, dz);
+    auto segment_to = [&](double r1, double z1) {
+        if (w != 0.0 && !deposit_segment(A.G, S::kCharge * w, dt, birth_r, birth_z, r1, z1) && !err) err = kErrInvalid;
+    };
+    auto escape_along = [&](Path const& path) {
+        Crossing const bc = boundary_crossing(path, D);
+        double const q = S::kCharge * w;
+        if (!bc.valid) {
+            if (!err) err = kErrInvalid;
+        } else if (q != 0.0 && !deposit_escaping_segment(A.G, q, dt, birth_r, birth_z, bc.start_r, bc.start_z,
+                                                         bc.cross_r, bc.cross_z, A.pnr, A.pzb, A.pnz, bc.faces)) {
+            if (!err) err = kErrInvalid;
+        }
+        status = RREA_ESCAPE;
+        tally[19] += 1.0;
+    };
+    auto terminate_at = [&](double x, double y, double z, uint64_t index, double elapsed) {
+        if constexpr (kPos) {
+            spawn_annihilation(T, gcut, rng, em, kProcAnnihilation, x, y, z, dx, dy, dz, k, true, index, elapsed,
+                               tally);
+            status = RREA_ANNIHILATED;
+        } else {
+            tally[16] += w;
+            em.deposit_local(k);
+            status = RREA_DEMOTED;
+        }
+        tally[20] += 1.0;
+    };
+
+    if (!in_half_open(p.x, p.y, p.z, D)) {
+        err = kErrDomain;
+    } else {
+        bool const birth_demote = kPos ? k < cutoff : newborn_electron_demotes(A, p.x, p.y, p.z, k, err);
+        if (err) {
+        } else if (birth_demote) {
+            terminate_at(p.x, p.y, p.z, newborn_index(rsp, lin, 0U, 0U, 0U, err), p.birth_fraction);
+        } else if (residual > 0.0) {
+            double const birth_u = proper_velocity(k);
+            double const birth_gamma = sqrt(1.0 + birth_u * birth_u / (kC * kC));
+            double const residual_dt = dt * residual;
+            double const chord = (birth_u / birth_gamma) * residual_dt;
+            Path driver;
+            bool driver_ok = false;
+            if (!(chord > 0.0)) {
+                if (k < cutoff) status = RREA_DEMOTED;
+            } else {
+                double const end_x = p.x + chord * dx, end_y = p.y + chord * dy, end_z = p.z + chord * dz;
+                if (clip_rz_path(hypot(p.x, p.y), atan2(p.y, p.x), p.z, hypot(end_x, end_y), atan2(end_y, end_x), end_z,
+                                 D, driver, err)) {
+                    if (!(driver.full > 0.0) || !(driver.in_len > 0.0)) {
+                        if (driver.exits) escape_along(driver);
+                    } else {
+                        driver_ok = true;
+                    }
+                }
+            }
+            if (driver_ok && !err) {
+                Interval iv = build_interval(F, driver, D, A.density_ratio, err);
+                if (!iv.valid && !err) err = kErrInvalid;
+                Path path = driver;
+                uint64_t substeps = 1;
+                if (!err) {
+                    double const driver_dp = density_path(iv, driver, 0.0, driver.end_f);
+                    double const gs_n_total = elastic_inverse_length(EL, k, err) * driver_dp;
+                    Bracket bk;
+                    if (!charged_bracket(T, k, bk)) err = kErrRange;
+                    double drag_est;
+                    if constexpr (kPos) {
+                        drag_est = charged_value(T, kPCol, bk, err) * 1.0 * driver_dp;
+                    } else {
+                        drag_est = (k > 0.0 ? charged_value(T, kECol, bk, err) * 1.0 + charged_value(T, kESoft, bk, err) * 1.0
+                                            : 0.0) * driver_dp;
+                    }
+                    double const whole_work = path_field_work(F, driver, D, S::kCharge, 0.0, 1.0, err).work;
+                    double const impulse = dmax(cfg.max_field_impulse_fraction * k, dmax(cfg.field_impulse_floor_eV, 1.0));
+                    double const span = dmax(driver.end_f, 1.0e-6);
+                    uint64_t const gsd = static_cast<uint64_t>(dmax(1.0, ceil(dmin(gs_n_total / span, cfg.gs_total_collision_cap)
+                                                                            / dmax(cfg.max_gs_collisions_per_substep, 1.0))));
+                    uint64_t const imd = static_cast<uint64_t>(dmax(1.0, ceil(((fabs(whole_work) + drag_est) / span) / impulse)));
+                    substeps = gsd > imd ? gsd : imd;
+                    if (substeps > cfg.max_subcycles) substeps = cfg.max_subcycles;
+                    if (substeps >= kSegmentCap && !err) err = kErrInvalid;
+                }
+                double run_x = p.x, run_y = p.y, run_z = p.z;
+                double const sub_dt = residual_dt / static_cast<double>(substeps);
+                double final_f = 0.0;
+                bool terminal_interaction = false, terminal_at_point = false, exited = false;
+                uint64_t completed = 0;
+                auto elapsed_at = [&](double f) {
+                    return newborn_elapsed(p.birth_fraction, residual, completed, substeps, f);
+                };
+                auto mark_stopped = [&](double f) { final_f = f; terminal_at_point = true; };
+                auto in_range = [&]() {
+                    if (!(finite(k) && k >= T.charged_min && k <= T.charged_max) && !err) err = kErrRange;
+                };
+                // RreaPlanContinuousDragSegment OV/RreaChargedInterleave.H:239-292
+                auto continuous = [&](double b, double e, double stop_e, double& eff) -> bool {
+                    eff = b;
+                    if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return false;
+                    Bracket bs;
+                    if (!charged_bracket(T, stop_e, bs)) { if (!err) err = kErrRange; return false; }
+                    double const col = charged_value(T, S::kCol, bs, err) * 1.0;
+                    double const soft = charged_value(T, S::kSoft, bs, err) * 1.0;
+                    double const total = col + soft;
+                    bool reaches = false;
+                    double const to_cut = k - cutoff;
+                    double const req = total * density_path(iv, path, b, e);
+                    if (!finite(req) || req < 0.0) { if (!err) err = kErrInvalid; return false; }
+                    if (req < to_cut) {
+                        eff = e;
+                    } else {
+                        reaches = true;
+                        if (to_cut != 0.0) {
+                            double lo = b, hi = e;
+                            for (int it = 0; it < 56; ++it) {
+                                double const mid = 0.5 * (lo + hi);
+                                if (total * density_path(iv, path, b, mid) < to_cut) lo = mid; else hi = mid;
+                            }
+                            eff = 0.5 * (lo + hi);
+                        }
+                    }
+                    double const pm = density_path(iv, path, b, eff);
+                    double const raw_col = col * pm;
+                    double const raw_soft = soft * pm;
+                    double const avail = dmax(0.0, k - cutoff);
+                    double const drag = reaches ? avail : dmin(raw_col + raw_soft, avail);
+                    double const col_loss = dmin(raw_col, drag);
+                    // RreaPartitionContinuousIonPairs OV/RreaChargedInterleave.H:157-196
+                    if (col_loss > 0.0 && raw_col > 0.0 && iv.valid) {
+                        double const lb = dmax(b, 0.0), le = dmin(eff, iv.end);
+                        if (le > lb) {
+                            double const raw_i = path.full * (le - lb) * iv.ratio * col;
+                            double const loss_i = col_loss * raw_i / raw_col;
+                            double const pw = w * loss_i / dmax(kWAir, 1.0);
+                            if (pw > 0.0) ion += pw;
+                        }
+                    }
+                    tally[7] += w * drag;
+                    k = dmax(0.0, k - drag);
+                    if (reaches) k = cutoff;
+                    return reaches;
+                };
+                FieldRange sfield{0, 0, 0, 0, 0, 0, 0, 0};
+                double substep_dt = 0.0;
+                // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:693-732
+                for (uint64_t sc = 0; sc < substeps && !err; ++sc) {
+                    if (k < cutoff) break;
+                    double sb = 0.0, se = 0.0;
+                    {
+                        Path next;
+                        if (!build_chord(run_x, run_y, run_z, dx, dy, dz, k, sub_dt, residual_dt, D, next, err)) break;
+                        path = next;
+                        if (!(path.full > 0.0) || !(path.in_len > 0.0)) {
+                            exited = path.exits;
+                            final_f = 0.0;
+                            break;
+                        }
+                        iv = build_interval(F, path, D, A.density_ratio, err);
+                        if (!iv.valid) { if (!err) err = kErrInvalid; break; }
+                        sb = 0.0;
+                        se = path.end_f;
+                        final_f = se;
+                        sfield = path_field_work(F, path, D, S::kCharge, sb, se, err);
+                        substep_dt = sub_dt * (se - sb);
+                        (void)half_kick(sfield, 0.5 * substep_dt, S::kCharge, k, dx, dy, dz);
+                        if (k < cutoff) {
+                            final_f = sb;
+                            terminal_at_point = true;
+                            break;
+                        }
+                        in_range();
+                    }
+                    {
+                        // RreaRunChargedEventInterleave OV/RreaChargedInterleave.H:546-664
+                        double rem = sb;
+                        uint64_t ord = 0;
+                        double ex_ = 0.0, ey_ = 0.0, ez_ = 0.0;
+                        while (rem < se && k >= cutoff && !err) {
+                            if (ord >= kEventCap) { err = kErrInvalid; break; }
+                            double const frozen_e = k;
+                            Bracket bf;
+                            if (!charged_bracket(T, frozen_e, bf)) { err = kErrRange; break; }
+                            double s0, s1, s2 = 0.0;
+                            if constexpr (kPos) positron_stp_rates(T, bf, thr, gcut, frozen_e, s0, s1, s2, err);
+                            else electron_stp_rates(T, bf, thr, gcut, frozen_e, hm_on, s0, s1, err);
+                            double const total_stp = s0 + s1 + s2;
+                            double const L = density_path(iv, path, rem, se);
+                            uint64_t const idx = newborn_index(rsp, lin, sc, ord, 0U, err);
+                            if (err) break;
+                            double const u1 = rng_uniform(rng, idx, kChNewbornCompetingOpticalDepth);
+                            double const u2 = rng_uniform(rng, idx, kChNewbornCompetingChannel);
+                            Hazard h;
+                            if constexpr (kPos) h = select_hazard3(s0 * L, s1 * L, s2 * L, u1, u2);
+                            else h = select_hazard(s0 * L, s1 * L, u1, u2);
+                            if (!h.valid) { err = kErrInvalid; break; }
+                            if (!h.event) {
+                                double eff;
+                                if (continuous(rem, se, frozen_e, eff)) mark_stopped(eff);
+                                break;
+                            }
+                            double const ef = fraction_at_density_path(iv, path, rem, se, h.sampled / total_stp, err);
+                            if (!(finite(rem) && finite(ef) && finite(se) && se > rem && ef >= rem && ef <= se)) {
+                                if (!err) err = kErrInvalid;
+                                break;
+                            }
+                            point_at_full(path, ef, ex_, ey_, ez_);
+                            double const efrac = elapsed_at(ef);
+                            {
+                                double eff;
+                                if (continuous(rem, ef, frozen_e, eff)) { mark_stopped(eff); break; }
+                            }
+                            bool open_ev, open_fr = false;
+                            if constexpr (kPos) {
+                                open_ev = k >= cutoff && positron_channel_open(T, thr, gcut, h.channel, k, err);
+                                if (!open_ev) {
+                                    open_fr = frozen_e >= cutoff
+                                        && positron_channel_open(T, thr, gcut, h.channel, frozen_e, err);
+                                }
+                            } else {
+                                open_ev = k >= cutoff && electron_channel_open(T, thr, gcut, h.channel, k, hm_on, err);
+                                if (!open_ev) {
+                                    open_fr = frozen_e >= cutoff
+                                        && electron_channel_open(T, thr, gcut, h.channel, frozen_e, hm_on, err);
+                                }
+                            }
+                            if (!open_ev) {
+                                if (!open_fr) { if (!err) err = kErrInvalid; break; }
+                                rem = ef;
+                                ++ord;
+                                continue;
+                            }
+                            if (h.channel == 0) {
+                                double const before = k;
+                                if (!(finite(before) && before >= T.charged_min && before <= T.charged_max)) { err = kErrRange; break; }
+                                if constexpr (kPos) {
+                                    // RreaApplyBhabhaFinalState OV/RreaChargedFinalStates.H:238-297
+                                    double const T2 = dmin(cdf_sample(T.bhabha, before, rng_uniform(rng, idx, kChPositronBhabhaEnergy),
+                                                                      false, err),
+                                                           dmax(0.0, before));
+                                    if (T2 > 0.0 && !(thr > 0.0 && T2 < thr)) {
+                                        double sdx, sdy, sdz, ndx, ndy, ndz;
+                                        moller_lab_directions(dx, dy, dz, before, T2,
+                                                              rng_uniform(rng, idx, kChPositronBhabhaAzimuth),
+                                                              sdx, sdy, sdz, ndx, ndy, ndz);
+                                        k = dmax(0.0, before - T2);
+                                        dx = ndx; dy = ndy; dz = ndz;
+                                        em.secondary(0, kProcHardBhabha, ex_, ey_, ez_, sdx, sdy, sdz, T2, efrac);
+                                        tally[12] += w;
+                                        tally[11] += 1.0;
+                                    }
+                                } else {
+                                    // RreaApplyHardMollerFinalState OV/RreaChargedFinalStates.H:164-225
+                                    double T2 = cdf_sample(T.moller, before, rng_uniform(rng, idx, kChHardMollerEnergy), true, err);
+                                    T2 = dmax(thr, dmin(T2, 0.5 * before));
+                                    double sdx, sdy, sdz, ndx, ndy, ndz;
+                                    moller_lab_directions(dx, dy, dz, before, T2, rng_uniform(rng, idx, kChHardMollerAzimuth),
+                                                          sdx, sdy, sdz, ndx, ndy, ndz);
+                                    k = dmax(0.0, before - T2);
+                                    dx = ndx; dy = ndy; dz = ndz;
+                                    em.secondary(0, kProcHardMoller, ex_, ey_, ez_, sdx, sdy, sdz, T2, efrac);
+                                    ion += w;
+                                    tally[3] += 1.0;
+                                }
+                            } else if (h.channel == 1) {
+                                // RreaApplyBremsFinalState OV/RreaChargedFinalStates.H:329-394
+                                double const before = k;
+                                if (!(finite(before) && before >= T.charged_min && before <= T.charged_max)) { err = kErrRange; break; }
+                                double const ks = cdf_sample(kPos ? T.pbrems : T.ebrems, before,
+                                                             rng_uniform(rng, idx, S::kBremsE), false, err);
+                                double const pe = dmin(dmax(ks, 0.0), dmax(before, 0.0));
+                                double const after = dmax(0.0, dmax(before, 0.0) - pe);
+                                bool const tracked = finite(gcut) && gcut >= 0.0 && pe >= gcut;
+                                double const ct = brems_cos(before, rng_uniform(rng, idx, S::kBremsA));
+                                double gx, gy, gz;
+                                cone_direction(dx, dy, dz, ct, rng_uniform(rng, idx, S::kBremsPhi), gx, gy, gz);
+                                // RreaApplyPhotonRecoilToChargedDirection :465-496
+                                {
+                                    double const eg = dmax(pe, 0.0);
+                                    if (before > 0.0 && eg > 0.0) {
+                                        normalize(dx, dy, dz);
+                                        double hx = gx, hy = gy, hz = gz;
+                                        normalize(hx, hy, hz);
+                                        double const p0 = sqrt(dmax(0.0, before * (before + 2.0 * kMeC2)));
+                                        double const px = p0 * dx - eg * hx;
+                                        double const py = p0 * dy - eg * hy;
+                                        double const pz2 = p0 * dz - eg * hz;
+                                        double const nn = sqrt(px * px + py * py + pz2 * pz2);
+                                        if (nn > 0.0) { dx = px / nn; dy = py / nn; dz = pz2 / nn; normalize(dx, dy, dz); }
+                                    }
+                                }
+                                k = after;
+                                if (tracked) {
+                                    em.secondary(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
+                                    tally[5] += 1.0;
+                                } else if (pe > 0.0) {
+                                    em.deposit_local(pe);
+                                }
+                                tally[4] += 1.0;
+                            } else if (kPos && h.channel == 2) {
+                                // in flight: the positron is consumed at the event point
+                                spawn_annihilation(T, gcut, rng, em, kProcAnnihilation, ex_, ey_, ez_, dx, dy, dz, k, false,
+                                                   idx, efrac, tally);
+                                segment_to(hypot(ex_, ey_), ez_);
+                                terminal_interaction = true;
+                                final_f = ef;
+                                break;
+                            }
+                            if (k <= cutoff) { mark_stopped(ef); break; }
+                            rem = ef;
+                            ++ord;
+                        }
+                    }
+                    if (terminal_interaction || terminal_at_point || err) break;
+                    {
+                        if (k >= cutoff) {
+                            // apply_gs_elastic_scatter OV/RreaTransportStep.H:372-423 on the newborn channels
+                            double const gs_n = elastic_inverse_length(EL, k, err) * density_path(iv, path, sb, se);
+                            if (gs_n > 0.0) {
+                                if (gs_n < T.gs_few) {
+                                    int const cap = static_cast<int>(ceil(3.0 * T.gs_few));
+                                    if (cap >= 255) { err = kErrInvalid; break; }
+                                    int const cnt = poisson_capped(gs_n, rng_uniform(rng, newborn_index(rsp, lin, sc, 0U, 0U, err),
+                                                                                     kFewCount), cap);
+                                    for (int q = 0; q < cnt; ++q) {
+                                        uint64_t const ix = newborn_index(rsp, lin, sc, 0U, static_cast<uint64_t>(q + 1), err);
+                                        if (!(finite(k) && k >= T.charged_min && k <= T.charged_max)) { err = kErrRange; break; }
+                                        double const eta = elastic_semilog(EL, EL.eta, k, err);
+                                        double const u = dmin(dmax(rng_uniform(rng, ix, kFewScatter), 0.0), 1.0 - 1.0e-15);
+                                        double const ct = 1.0 - 2.0 * eta * u / (1.0 + eta - u);
+                                        double nx, ny, nz;
+                                        cone_direction(dx, dy, dz, ct, rng_uniform(rng, ix, kFewPhi), nx, ny, nz);
+                                        dx = nx; dy = ny; dz = nz;
+                                    }
+                                } else {
+                                    uint64_t const ix = newborn_index(rsp, lin, sc, 0U, 0U, err);
+                                    double const u = rng_uniform(rng, ix, kGsAngle);
+                                    double ct;
+                                    if (!(finite(k) && k >= T.charged_min && k <= T.charged_max)) { err = kErrRange; break; }
+                                    DevAngle const& A2 = k <= T.gs_transition ? (kPos ? T.gs_p : T.gs_e)
+                                                                               : (kPos ? T.wz_p : T.wz_e);
+                                    if (!(k >= A2.e[0] && k <= A2.e[A2.n_e - 1])) { err = kErrRange; break; }
+                                    if (gs_n >= T.gs_iso) ct = 2.0 * u - 1.0;
+                                    else ct = sample_gs_angle_cos(A2, k, gs_n, u, err);
+                                    double nx, ny, nz;
+                                    cone_direction(dx, dy, dz, ct, rng_uniform(rng, ix, kGsPhi), nx, ny, nz);
+                                    dx = nx; dy = ny; dz = nz;
+                                }
+                            }
+                        }
+                        if (err) break;
+                        (void)half_kick(sfield, 0.5 * substep_dt, S::kCharge, k, dx, dy, dz);
+                        if (k < cutoff) {
+                            final_f = se;
+                            terminal_at_point = true;
+                            break;
+                        }
+                        in_range();
+                    }
+                    {
+                        double x, y, z;
+                        point_at_full(path, se, x, y, z);
+                        if (!in_half_open(x, y, z, D)) { exited = true; break; }
+                        run_x = x; run_y = y; run_z = z;
+                    }
+                    if (path.exits) { exited = true; break; }
+                    completed = sc + 1U;
+                }
+                if (!err) {
+                    if (terminal_interaction) {
+                        status = RREA_ANNIHILATED;
+                        tally[20] += 1.0;
+                    } else {
+                        double fx_ = 0.0, fy_ = 0.0, fz_ = 0.0;
+                        if (!newborn_final_point(path, D, final_f, fx_, fy_, fz_)) {
+                            err = kErrInvalid;
+                        } else {
+                            double const final_r = hypot(fx_, fy_);
+                            double const final_el = elapsed_at(final_f);
+                            bool const demote = kPos ? k < cutoff : newborn_electron_demotes(A, fx_, fy_, fz_, k, err);
+                            if (err) {
+                            } else if (terminal_at_point || demote) {
+                                terminate_at(fx_, fy_, fz_,
+                                             newborn_index(rsp, lin, substeps < kSegmentCap - 1U ? substeps : kSegmentCap - 1U,
+                                                           0U, 0U, err),
+                                             final_el);
+                                segment_to(final_r, fz_);
+                            } else if (exited) {
+                                escape_along(path);
+                            } else {
+                                p.x = fx_; p.y = fy_; p.z = fz_;
+                                p.dir_x = dx; p.dir_y = dy; p.dir_z = dz;
+                                p.energy_eV = k;
+                                segment_to(final_r, fz_);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (err) {
+        report(A.error, err, i);
+        return;
+    }
+    p.birth_fraction = 1.0;
+    N.fin[i] = p;
+    N.status[i] = static_cast<uint8_t>(status);
+    N.ion[i] = ion;
+    tally[8] += ion;
+    if (status == RREA_SURVIVE) tally[18] += 1.0;
+}
+
+// TransportStep::AdvancePhotonNewborn OV/RreaNewbornAdvance.cpp:779-994
+__device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, double* tally)
+{
+    StepArgs const& A = N.S;
+    DevTables const& T = A.T;
+    DevFields const& F = A.F;
+    Domain const& D = A.D;
+    rrea_step_config const& cfg = A.cfg;
+    double const cutoff = cfg.low_energy_cutoff_eV;
+    double const gcut = cfg.photon_cutoff_eV;
+    double const dt = cfg.dt_s;
+    uint64_t const step = static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0);
+
+    rrea_secondary p = N.work[i];
+    int32_t err = kOk;
+    double ion = 0.0;
+    uint64_t lineage = 0;
+    int status = RREA_SURVIVE;
+    uint64_t const rsp = static_cast<uint64_t>(p.root_species);
+    double const w = p.weight;
+    uint64_t const rng = rng_prefix(cfg.rng_seed, p.root_key, step);
+    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1};
+    double const residual = newborn_residual(p.birth_fraction);
+    double dx = p.dir_x, dy = p.dir_y, dz = p.dir_z;
+    normalize(dx, dy, dz);
+    double energy = p.energy_eV;
+    auto escape = [&]() {
+        status = RREA_ESCAPE;
+        tally[19] += 1.0;
+    };
+    auto survive_at = [&](double x, double y, double z) {
+        p.x = x; p.y = y; p.z = z;
+        p.dir_x = dx; p.dir_y = dy; p.dir_z = dz;
+        p.energy_eV = energy;
+    };
+    if (!(residual > 0.0)) {
+        if (!(energy >= gcut)) status = RREA_DEMOTED;
+    } else if (!in_half_open(p.x, p.y, p.z, D)) {
+        err = kErrDomain;
+    } else if (energy < gcut) {
+        em.deposit_local(energy);
+        status = RREA_DEMOTED;
+        tally[20] += 1.0;
+    } else {
+        double const chord = kC * dt * residual;
+        double const end_x = p.x + chord * dx, end_y = p.y + chord * dy, end_z = p.z + chord * dz;
+        Path path;
+        if (clip_rz_path(hypot(p.x, p.y), atan2(p.y, p.x), p.z, hypot(end_x, end_y), atan2(end_y, end_x), end_z, D,
+                         path, err)) {
+            if (!(path.full > 0.0) || !(path.in_len > 0.0)) {
+                if (path.exits) escape();
+            } else {
+                double elapsed_start = p.birth_fraction;
+                uint64_t ord = 0;
+                while (!err) {
+                    uint64_t const idx = newborn_index(rsp, p.lineage, 0U, ord, 0U, err);
+                    if (err) break;
+                    Interval const iv = build_interval(F, path, D, A.density_ratio, err);
+                    PhotonRates rates;
+                    if (!photon_rates(T, energy, rates, err)) break;
+                    double const total = rates.compton + rates.pe + rates.pair_n + rates.pair_t;
+                    if (!finite(total) || total < 0.0) { err = kErrInvalid; break; }
+                    double const mfp = total > 0.0 ? 1.0 / total : 1.0e99;
+                    double const tau = density_path(iv, path, 0.0, path.end_f) / mfp;
+                    if (err) break;
+                    Hazard const h = select_hazard(tau, 0.0, rng_uniform(rng, idx, kChNewbornCompetingOpticalDepth), 0.0);
+                    if (!h.event) {
+                        double x, y, z;
+                        if (!newborn_final_point(path, D, path.end_f, x, y, z)) { err = kErrInvalid; break; }
+                        if (path.exits) escape();
+                        else survive_at(x, y, z);
+                        break;
+                    }
+                    double const f = fraction_at_density_path(iv, path, 0.0, path.end_f, h.sampled * mfp, err);
+                    if (err) break;
+                    double const dist = path.full * (f - 0.0);
+                    double const remaining = dmax(0.0, path.full - dist);
+                    double x, y, z;
+                    point_at_full(path, f, x, y, z);
+                    double const r_site = hypot(x, y);
+                    double const el = dmin(1.0, elapsed_start + dist / dmax(kC * dt, kDblMin));
+                    // SamplePhotonChannel NA/src/RreaInteractionTables.cpp:2778-2804
+                    int channel;
+                    {
+                        double const cu = dmax(0.0, dmin(1.0, rng_uniform(rng, idx, kChNewbornCompetingChannel)));
+                        if (!(total > 0.0)) { err = kErrInvalid; break; }
+                        double const target = cu * total;
+                        if (target < rates.compton) channel = kCompton;
+                        else if (target < rates.compton + rates.pe) channel = kPhotoelectric;
+                        else if (target < rates.compton + rates.pe + rates.pair_n) channel = kPairNuclear;
+                        else channel = kPairTriplet;
+                    }
+                    bool const survives = photon_final_state(T, cutoff, gcut, rng, em, channel, idx, x, y, z, el, energy,
+                                                             dx, dy, dz, tally);
+                    if (err) break;
+                    if (!survives) {
+                        status = RREA_ABSORBED;
+                        tally[20] += 1.0;
+                        break;
+                    }
+                    if (!(remaining > 0.0)) { survive_at(x, y, z); break; }
+                    double const tx = x + remaining * dx, ty = y + remaining * dy, tz = z + remaining * dz;
+                    Path cont;
+                    if (!clip_rz_path(r_site, atan2(y, x), z, hypot(tx, ty), atan2(ty, tx), tz, D, cont, err)) break;
+                    path = cont;
+                    elapsed_start = el;
+                    ++ord;
+                }
+            }
+        }
+    }
+    if (err) {
+        report(A.error, err, i);
+        return;
+    }
+    p.birth_fraction = 1.0;
+    N.fin[i] = p;
+    N.status[i] = static_cast<uint8_t>(status);
+    N.ion[i] = ion;
+    tally[8] += ion;
+    if (status == RREA_SURVIVE) tally[18] += 1.0;
+}
+
+// One lane per FIFO entry of the generation.  Entries already at the end of
+// the step (birth fraction 1) survive untouched.
+__global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __restrict__ args)
+{
+    NewbornArgs const& N = *args;
+    double tally[kTallyNewborn];
+    for (int q = 0; q < kTallyNewborn; ++q) tally[q] = 0.0;
+    int64_t const i = N.gb + static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < N.ge) {
+        rrea_secondary const& p = N.work[i];
+        bool const photon = p.species == 2;
+        // transient_energy_is_certified OV/RreaSecondaryCommit.cpp:148-180
+        double const lo = photon ? N.S.T.photon_min : N.S.T.charged_min;
+        double const hi = photon ? N.S.T.photon_max : N.S.T.charged_max;
+        if (!(finite(p.energy_eV) && p.energy_eV >= lo && p.energy_eV <= hi)) {
+            report(N.S.error, kErrRange, i);
+        } else if (p.birth_fraction < 1.0) {
+            tally[17] += 1.0;
+            if (photon) photon_newborn(N, i, tally);
+            else if (p.species == 1) charged_newborn<true>(N, i, tally);
+            else charged_newborn<false>(N, i, tally);
+        } else {
+            N.fin[i] = p;
+            N.status[i] = static_cast<uint8_t>(RREA_SURVIVE);
+            N.ion[i] = 0.0;
+        }
+    }
+    reduce_tallies<kTallyNewborn>(tally, N.S.tallies);
+}
+
+// The FIFO's birth-payload validation (OV/RreaSecondaryCommit.cpp:338-376),
+// each entry's root anchor (the first entry of its root's contiguous run) and
+// the root's next free lineage (:390-394).
+__global__ void fifo_init_kernel(const rrea_secondary* __restrict__ work, int64_t n, int64_t* __restrict__ anchor,
+                                 unsigned long long* __restrict__ next, int32_t* __restrict__ error)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rrea_secondary const& p = work[i];
+    bool const finite_payload = finite(p.x) && finite(p.y) && finite(p.z) && finite(p.dir_x) && finite(p.dir_y)
+        && finite(p.dir_z) && finite(p.weight) && finite(p.energy_eV) && finite(p.birth_fraction);
+    bool const energy_ok = p.species == 2 ? p.energy_eV > 0.0 : p.energy_eV >= 0.0;
+    if (p.species < 0 || p.species > 2 || p.root_species < 0 || p.root_species > 2 || !finite_payload
+        || !(p.weight > 0.0) || !energy_ok || p.birth_fraction < 0.0 || p.birth_fraction > 1.0
+        || p.lineage >= kLineageCap) {
+        report(error, kErrInvalid, i);
+    }
+    int64_t a = i;
+    while (a > 0 && work[a - 1].root_key == p.root_key && work[a - 1].root_species == p.root_species) --a;
+    anchor[i] = a;
+    atomicMax(&next[a], static_cast<unsigned long long>(p.lineage + 1U));
+}
+
+// Children of one generation, already in FIFO order: inherit the parent's
+// anchor
+__global__ void fifo_anchor_kernel(const rrea_secondary* __restrict__ work, int64_t b, int64_t e,
+                                   int64_t* __restrict__ anchor)
+{
+    int64_t const j = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= e) return;
+    anchor[j] = anchor[work[j].parent_index];
+}
+
+// allocate_lineage (OV/RreaSecondaryCommit.cpp:395-406) for one generation:
+// the root's next free lineage plus the child's rank within its root's run
+__global__ void fifo_lineage_kernel(rrea_secondary* __restrict__ work, int64_t b, int64_t e,
+                                    const int64_t* __restrict__ anchor, const unsigned long long* __restrict__ next,
+                                    int32_t* __restrict__ error)
+{
+    int64_t const j = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= e) return;
+    int64_t const a = anchor[j];
+    int64_t s = j;
+    while (s > b && anchor[s - 1] == a) --s;
+    uint64_t const lin = static_cast<uint64_t>(next[a]) + static_cast<uint64_t>(j - s);
+    if (lin >= kLineageCap) report(error, kErrInvalid, j);
+    work[j].lineage = lin;
+}
+
+// The last child of each root's run advances the root's next free lineage.
+__global__ void fifo_advance_kernel(int64_t b, int64_t e, const int64_t* __restrict__ anchor,
+                                    unsigned long long* __restrict__ next)
+{
+    int64_t const j = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= e) return;
+    int64_t const a = anchor[j];
+    if (j + 1 < e && anchor[j + 1] == a) return;
+    int64_t s = j;
+    while (s > b && anchor[s - 1] == a) --s;
+    next[a] += static_cast<unsigned long long>(j - s + 1);
+}
+
+__global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t n, unsigned long long v)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Survivors in FIFO order: the end-of-step states of the entries that survive.
+__global__ void survivor_flag_kernel(const uint8_t* __restrict__ status, int64_t n, int32_t* __restrict__ flag)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = status[i] == RREA_SURVIVE ? 1 : 0;
+}
+
 __global__ void secondary_keys_kernel(const rrea_secondary* __restrict__ s, const int64_t* __restrict__ count,
                                       int64_t cap, uint64_t* __restrict__ keys, int64_t* __restrict__ idx)
 {
@@ -1890,7 +2705,8 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
     if (!F) return RREA_OK;
     void* ptrs[] = {F->d_cell_er, F->d_cell_ez, F->d_btheta, F->d_stage, F->d_prof, F->d_stage_sec, F->d_keys,
                     F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args,
-                    F->d_slow_list, F->d_slow_count, F->d_prof_guide};
+                    F->d_slow_list, F->d_slow_count, F->d_prof_guide, F->d_nargs, F->d_fin, F->d_fst,
+                    F->d_fion, F->d_anchor, F->d_next, F->d_flag, F->d_nsel, F->d_sel_tmp};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : F->events) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
     delete F;
@@ -1983,6 +2799,8 @@ rrea_status rrea_density_set_profile(rrea_fields_t F, const double* alt, const d
     HIP_TRY(hipMemcpy(F->d_prof_guide, g.idx.data(), kProfileGuideBins * sizeof(int32_t), hipMemcpyHostToDevice));
     F->prof_guide_lo = g.lo;
     F->prof_guide_scale = g.scale;
+    F->h_prof_alt.assign(alt, alt + n);
+    F->h_prof_rho.assign(rho, rho + n);
     F->prof_n = n;
     F->density_mode = 1;
     F->altitude_at_z0 = z0_alt;
@@ -1997,6 +2815,93 @@ namespace {
 // One charged step (either species): argument checks, the argument block,
 // the launch, and the deterministic secondary ordering.
 enum SpeciesId : int { kElectron = 0, kPositron = 1, kPhoton = 2 };
+
+// The secondary-ordering workspace (staging, keys, indices, sort scratch),
+// grown to `cap` records.
+rrea_status order_workspace(rrea_fields_t F, int64_t cap)
+{
+    if (F->stage_sec_cap >= cap) return RREA_OK;
+    if (cap > INT32_MAX) return RREA_ERR_ARGUMENT;
+    void* ptrs[] = {F->d_stage_sec, F->d_keys, F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp};
+    for (void* p : ptrs) if (p) (void)hipFree(p);
+    F->d_stage_sec = nullptr; F->d_keys = nullptr; F->d_keys_out = nullptr;
+    F->d_idx = nullptr; F->d_idx_out = nullptr; F->d_sort_tmp = nullptr;
+    F->stage_sec_cap = 0;
+    HIP_TRY(hipMalloc(&F->d_stage_sec, cap * sizeof(rrea_secondary)));
+    HIP_TRY(hipMalloc(&F->d_keys, cap * sizeof(uint64_t)));
+    HIP_TRY(hipMalloc(&F->d_keys_out, cap * sizeof(uint64_t)));
+    HIP_TRY(hipMalloc(&F->d_idx, cap * sizeof(int64_t)));
+    HIP_TRY(hipMalloc(&F->d_idx_out, cap * sizeof(int64_t)));
+    size_t bytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, F->d_keys, F->d_keys_out, F->d_idx, F->d_idx_out,
+                                               static_cast<int>(cap)));
+    HIP_TRY(hipMalloc(&F->d_sort_tmp, bytes));
+    F->sort_tmp_bytes = bytes;
+    F->stage_sec_cap = cap;
+    return RREA_OK;
+}
+
+// Sorts the n records staged in F->d_stage_sec (count at F->d_count) by
+// (parent index, lineage) into `dst`.
+rrea_status order_staged(rrea_fields_t F, int64_t n, rrea_secondary* dst, hipStream_t s)
+{
+    if (n <= 0) return RREA_OK;
+    secondary_keys_kernel<<<dim3(static_cast<unsigned>(blocks_for(n, 256))), dim3(256), 0, s>>>(
+        F->d_stage_sec, F->d_count, n, F->d_keys, F->d_idx);
+    HIP_TRY(hipGetLastError());
+    size_t bytes = F->sort_tmp_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(F->d_sort_tmp, bytes, F->d_keys, F->d_keys_out, F->d_idx, F->d_idx_out,
+                                               static_cast<int>(n), 0, 64, s));
+    secondary_gather_kernel<<<dim3(static_cast<unsigned>(blocks_for(n, 256))), dim3(256), 0, s>>>(
+        F->d_stage_sec, F->d_idx_out, F->d_count, n, dst);
+    HIP_TRY(hipGetLastError());
+    return RREA_OK;
+}
+
+// The argument block every transport kernel shares: tables, fields, domain,
+// the reference-z density ratio, the J grid and the step configuration.
+rrea_status common_args(rrea_tables_t T, rrea_fields_t F, const rrea_step_config* cfg, double* jr, double* jz,
+                        double* rho, StepArgs& A)
+{
+    A.T = T->view;
+    rrea_rz_geometry const& g = F->geom;
+    A.F.nr = g.nr; A.F.nz = g.nz; A.F.dr = g.dr; A.F.dz = g.dz; A.F.r_lo = g.r_lo; A.F.z_lo = g.z_lo;
+    A.F.cell_er = F->d_cell_er; A.F.cell_ez = F->d_cell_ez; A.F.btheta = F->d_btheta;
+    A.F.density_mode = F->density_mode; A.F.density_ratio = F->density_ratio;
+    A.F.prof_alt = F->d_prof; A.F.prof_rho = F->d_prof ? F->d_prof + F->prof_n : nullptr;
+    A.F.prof_n = static_cast<int32_t>(F->prof_n); A.F.altitude_at_z0 = F->altitude_at_z0;
+    A.F.prof_guide = Guide{F->d_prof_guide, kProfileGuideBins, F->prof_guide_lo, F->prof_guide_scale};
+    A.F.reference_density = F->reference_density;
+    A.D = Domain{g.r_lo, g.r_lo + g.nr * g.dr, g.z_lo, g.z_lo + g.nz * g.dz};
+    // RreaParticleInteraction::Apply :29-35: the ratio at the domain's mid z.
+    {
+        double ratio = F->density_ratio;
+        if (F->density_mode == 1) {
+            // the host copy of the profile kept by rrea_density_set_profile
+            std::vector<double> const& alt = F->h_prof_alt;
+            std::vector<double> const& rhov = F->h_prof_rho;
+            double const a = F->altitude_at_z0 + 0.5 * (A.D.z_lo + A.D.z_hi);
+            if (a < alt.front() || a > alt.back()) return RREA_ERR_TABLE_RANGE;
+            auto const up = std::upper_bound(alt.begin(), alt.end(), a);
+            double v;
+            if (up == alt.begin()) v = rhov.front();
+            else if (up == alt.end()) v = rhov.back();
+            else {
+                size_t const hi = up - alt.begin(), lo = hi - 1;
+                double const span = alt[hi] - alt[lo];
+                double const t = span > 0.0 ? (a - alt[lo]) / span : 0.0;
+                v = (1.0 - t) * rhov[lo] + t * rhov[hi];
+            }
+            ratio = v / F->reference_density;
+        }
+        A.density_ratio = ratio > 0.0 ? ratio : 1.0;
+    }
+    int32_t const pad = 2;
+    A.G = DevGrid{g.nr + pad, g.nz + 2 * pad, g.dr, g.dz, g.r_lo, g.z_lo - pad * g.dz, rho, jr, jz};
+    A.pnr = g.nr; A.pzb = pad; A.pnz = g.nz;
+    A.cfg = *cfg;
+    return RREA_OK;
+}
 
 rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
                          const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
@@ -2029,43 +2934,10 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     if (dev != T->device) return RREA_ERR_ARGUMENT;
 
     StepArgs A;
-    A.T = T->view;
-    rrea_rz_geometry const& g = F->geom;
-    A.F.nr = g.nr; A.F.nz = g.nz; A.F.dr = g.dr; A.F.dz = g.dz; A.F.r_lo = g.r_lo; A.F.z_lo = g.z_lo;
-    A.F.cell_er = F->d_cell_er; A.F.cell_ez = F->d_cell_ez; A.F.btheta = F->d_btheta;
-    A.F.density_mode = F->density_mode; A.F.density_ratio = F->density_ratio;
-    A.F.prof_alt = F->d_prof; A.F.prof_rho = F->d_prof ? F->d_prof + F->prof_n : nullptr;
-    A.F.prof_n = static_cast<int32_t>(F->prof_n); A.F.altitude_at_z0 = F->altitude_at_z0;
-    A.F.prof_guide = Guide{F->d_prof_guide, kProfileGuideBins, F->prof_guide_lo, F->prof_guide_scale};
-    A.F.reference_density = F->reference_density;
-    A.D = Domain{g.r_lo, g.r_lo + g.nr * g.dr, g.z_lo, g.z_lo + g.nz * g.dz};
-    // RreaParticleInteraction::Apply :29-35: the ratio at the domain's mid z.
     {
-        double ratio = F->density_ratio;
-        if (F->density_mode == 1) {
-            std::vector<double> alt(F->prof_n), rho(F->prof_n);
-            HIP_TRY(hipMemcpy(alt.data(), F->d_prof, F->prof_n * sizeof(double), hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(rho.data(), F->d_prof + F->prof_n, F->prof_n * sizeof(double), hipMemcpyDeviceToHost));
-            double const a = F->altitude_at_z0 + 0.5 * (A.D.z_lo + A.D.z_hi);
-            if (a < alt.front() || a > alt.back()) return RREA_ERR_TABLE_RANGE;
-            auto const up = std::upper_bound(alt.begin(), alt.end(), a);
-            double v;
-            if (up == alt.begin()) v = rho.front();
-            else if (up == alt.end()) v = rho.back();
-            else {
-                size_t const hi = up - alt.begin(), lo = hi - 1;
-                double const span = alt[hi] - alt[lo];
-                double const t = span > 0.0 ? (a - alt[lo]) / span : 0.0;
-                v = (1.0 - t) * rho[lo] + t * rho[hi];
-            }
-            ratio = v / F->reference_density;
-        }
-        A.density_ratio = ratio > 0.0 ? ratio : 1.0;
+        rrea_status const cs = common_args(T, F, cfg, out->jr, out->jz, out->rho, A);
+        if (cs != RREA_OK) return cs;
     }
-    int32_t const pad = 2;
-    A.G = DevGrid{g.nr + pad, g.nz + 2 * pad, g.dr, g.dz, g.r_lo, g.z_lo - pad * g.dz, out->rho, out->jr, out->jz};
-    A.pnr = g.nr; A.pzb = pad; A.pnz = g.nz;
-    A.cfg = *cfg;
     A.n = tile->n;
     A.prev_r = tile->prev_r; A.prev_z = tile->prev_z; A.prev_th = tile->prev_theta;
     A.prev_ux = tile->prev_ux; A.prev_uy = tile->prev_uy; A.prev_uz = tile->prev_uz;
@@ -2078,22 +2950,9 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     int64_t const cap = out->secondary_capacity;
     if (order) {
         // stage into the library's workspace, then sort into the caller's buffer
-        if (F->stage_sec_cap < cap) {
-            void* ptrs[] = {F->d_stage_sec, F->d_keys, F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp};
-            for (void* p : ptrs) if (p) (void)hipFree(p);
-            F->d_stage_sec = nullptr; F->d_keys = nullptr; F->d_keys_out = nullptr;
-            F->d_idx = nullptr; F->d_idx_out = nullptr; F->d_sort_tmp = nullptr;
-            HIP_TRY(hipMalloc(&F->d_stage_sec, cap * sizeof(rrea_secondary)));
-            HIP_TRY(hipMalloc(&F->d_keys, cap * sizeof(uint64_t)));
-            HIP_TRY(hipMalloc(&F->d_keys_out, cap * sizeof(uint64_t)));
-            HIP_TRY(hipMalloc(&F->d_idx, cap * sizeof(int64_t)));
-            HIP_TRY(hipMalloc(&F->d_idx_out, cap * sizeof(int64_t)));
-            size_t bytes = 0;
-            HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, F->d_keys, F->d_keys_out, F->d_idx, F->d_idx_out,
-                                                       static_cast<int>(cap)));
-            HIP_TRY(hipMalloc(&F->d_sort_tmp, bytes));
-            F->sort_tmp_bytes = bytes;
-            F->stage_sec_cap = cap;
+        {
+            rrea_status const ws = order_workspace(F, cap);
+            if (ws != RREA_OK) return ws;
         }
         HIP_TRY(hipMemcpyAsync(F->d_count, out->secondary_count, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
         A.sec = F->d_stage_sec;
@@ -2186,16 +3045,9 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
         HIP_TRY(hipStreamSynchronize(s));
         int64_t const room = A.sec_cap;
         int64_t const nsort = nstaged < room ? nstaged : room;
-        if (nsort > 0) {
-            secondary_keys_kernel<<<dim3(static_cast<unsigned>(blocks_for(nsort, 256))), dim3(256), 0, s>>>(
-                F->d_stage_sec, F->d_count, nsort, F->d_keys, F->d_idx);
-            HIP_TRY(hipGetLastError());
-            size_t bytes = F->sort_tmp_bytes;
-            HIP_TRY(hipcub::DeviceRadixSort::SortPairs(F->d_sort_tmp, bytes, F->d_keys, F->d_keys_out, F->d_idx,
-                                                       F->d_idx_out, static_cast<int>(nsort), 0, 64, s));
-            secondary_gather_kernel<<<dim3(static_cast<unsigned>(blocks_for(nsort, 256))), dim3(256), 0, s>>>(
-                F->d_stage_sec, F->d_idx_out, F->d_count, nsort, out->secondaries + base);
-            HIP_TRY(hipGetLastError());
+        {
+            rrea_status const os = order_staged(F, nsort, out->secondaries + base, s);
+            if (os != RREA_OK) return os;
         }
         int64_t const total = base + nstaged;
         HIP_TRY(hipMemcpyAsync(out->secondary_count, &total, sizeof(int64_t), hipMemcpyHostToDevice, s));
@@ -2224,6 +3076,140 @@ rrea_status rrea_transport_step_photons(rrea_tables_t T, rrea_fields_t F, const 
                                         const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
 {
     return species_step(kPhoton, T, F, tile, cfg, out, stream);
+}
+
+rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step_config* cfg, const rrea_drain_io* io,
+                               rrea_stream_t stream)
+{
+    if (!T || !F || !cfg || !io || !F->fields_set) return RREA_ERR_ARGUMENT;
+    if (!io->work || !io->work_count || !io->survivors || !io->survivor_count || !io->jr || !io->jz || !io->rho
+        || !io->tallies || !io->error) {
+        return RREA_ERR_ARGUMENT;
+    }
+    int64_t const cap = io->work_capacity;
+    if (cap < 0 || cap > INT32_MAX || io->n_initial < 0 || io->n_initial > cap || io->survivor_capacity < 0) {
+        return RREA_ERR_ARGUMENT;
+    }
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != T->device) return RREA_ERR_ARGUMENT;
+    static int64_t const kZero = 0;
+    // RreaParticleInteraction::Apply :38-40 -- a non-positive dt does nothing.
+    if (!(cfg->dt_s > 0.0) || io->n_initial == 0) {
+        HIP_TRY(hipMemcpyAsync(io->work_count, &kZero, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(io->survivor_count, &kZero, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return RREA_OK;
+    }
+    if (!(cfg->max_optical_depth_per_substep > 0.0) || cfg->max_subcycles == 0 || cfg->max_subcycles > 16383) {
+        return RREA_ERR_ARGUMENT;
+    }
+    StepArgs A{};
+    {
+        rrea_status const cs = common_args(T, F, cfg, io->jr, io->jz, io->rho, A);
+        if (cs != RREA_OK) return cs;
+    }
+    A.tallies = io->tallies;
+    A.error = io->error;
+    {
+        rrea_status const ws = order_workspace(F, cap);
+        if (ws != RREA_OK) return ws;
+    }
+    if (F->fifo_cap < cap) {
+        void* ptrs[] = {F->d_fin, F->d_fst, F->d_fion, F->d_anchor, F->d_next, F->d_flag, F->d_sel_tmp};
+        for (void* p : ptrs) if (p) (void)hipFree(p);
+        F->d_fin = nullptr; F->d_fst = nullptr; F->d_fion = nullptr; F->d_anchor = nullptr; F->d_next = nullptr;
+        F->d_flag = nullptr; F->d_sel_tmp = nullptr;
+        F->fifo_cap = 0;
+        HIP_TRY(hipMalloc(&F->d_fin, cap * sizeof(rrea_secondary)));
+        HIP_TRY(hipMalloc(&F->d_fst, cap * sizeof(uint8_t)));
+        HIP_TRY(hipMalloc(&F->d_fion, cap * sizeof(double)));
+        HIP_TRY(hipMalloc(&F->d_anchor, cap * sizeof(int64_t)));
+        HIP_TRY(hipMalloc(&F->d_next, cap * sizeof(unsigned long long)));
+        HIP_TRY(hipMalloc(&F->d_flag, cap * sizeof(int32_t)));
+        size_t bytes = 0;
+        HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, F->d_fin, F->d_flag, F->d_stage_sec, F->d_nsel,
+                                              static_cast<int>(cap), s));
+        HIP_TRY(hipMalloc(&F->d_sel_tmp, bytes));
+        F->sel_tmp_bytes = bytes;
+        F->fifo_cap = cap;
+    }
+    if (!F->d_nsel) HIP_TRY(hipMalloc(&F->d_nsel, sizeof(int64_t)));
+    if (!F->d_nargs) HIP_TRY(hipMalloc(&F->d_nargs, sizeof(NewbornArgs)));
+    int64_t const n0 = io->n_initial;
+    fill_u64_kernel<<<dim3(static_cast<unsigned>(blocks_for(n0, 256))), dim3(256), 0, s>>>(F->d_next, n0, 0ULL);
+    HIP_TRY(hipGetLastError());
+    fifo_init_kernel<<<dim3(static_cast<unsigned>(blocks_for(n0, 256))), dim3(256), 0, s>>>(
+        io->work, n0, F->d_anchor, F->d_next, io->error);
+    HIP_TRY(hipGetLastError());
+    int32_t err_host[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(err_host, io->error, sizeof(err_host), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // One launch per generation; the host reads the child count in between.
+    // A rejected birth payload advances nothing.
+    int64_t gb = 0, ge = err_host[0] != 0 ? 0 : n0;
+    while (gb < ge) {
+        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(F->d_count), 1, 0ULL);
+        HIP_TRY(hipGetLastError());
+        A.sec = F->d_stage_sec;
+        A.sec_cap = cap - ge;
+        A.sec_count = F->d_count;
+        NewbornArgs N{A, io->work, gb, ge, F->d_fin, F->d_fst, F->d_fion};
+        // pageable source: the copy has consumed N when hipMemcpyAsync returns
+        HIP_TRY(hipMemcpyAsync(F->d_nargs, &N, sizeof(NewbornArgs), hipMemcpyHostToDevice, s));
+        newborn_kernel<<<dim3(static_cast<unsigned>(blocks_for(ge - gb, kBlock))), dim3(kBlock), 0, s>>>(
+            static_cast<const NewbornArgs*>(F->d_nargs));
+        HIP_TRY(hipGetLastError());
+        int64_t c = 0;
+        HIP_TRY(hipMemcpyAsync(&c, F->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(err_host, io->error, sizeof(err_host), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (err_host[0] != 0 || c <= 0) break;
+        if (c > cap - ge) break;  // the kernel reported the capacity error
+        {
+            rrea_status const os = order_staged(F, c, io->work + ge, s);
+            if (os != RREA_OK) return os;
+        }
+        unsigned const cb = static_cast<unsigned>(blocks_for(c, 256));
+        fifo_anchor_kernel<<<dim3(cb), dim3(256), 0, s>>>(io->work, ge, ge + c, F->d_anchor);
+        HIP_TRY(hipGetLastError());
+        fifo_lineage_kernel<<<dim3(cb), dim3(256), 0, s>>>(io->work, ge, ge + c, F->d_anchor, F->d_next, io->error);
+        HIP_TRY(hipGetLastError());
+        fifo_advance_kernel<<<dim3(cb), dim3(256), 0, s>>>(ge, ge + c, F->d_anchor, F->d_next);
+        HIP_TRY(hipGetLastError());
+        gb = ge;
+        ge += c;
+    }
+    int64_t const total = ge;
+    int64_t nsurv = 0;
+    if (err_host[0] == 0) {
+        survivor_flag_kernel<<<dim3(static_cast<unsigned>(blocks_for(total, 256))), dim3(256), 0, s>>>(F->d_fst, total,
+                                                                                                      F->d_flag);
+        HIP_TRY(hipGetLastError());
+        size_t bytes = F->sel_tmp_bytes;
+        HIP_TRY(hipcub::DeviceSelect::Flagged(F->d_sel_tmp, bytes, F->d_fin, F->d_flag, F->d_stage_sec, F->d_nsel,
+                                              static_cast<int>(total), s));
+        HIP_TRY(hipMemcpyAsync(&nsurv, F->d_nsel, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (nsurv > io->survivor_capacity) {
+            int32_t const e2[2] = {kErrCapacity, -1};
+            HIP_TRY(hipMemcpyAsync(io->error, e2, sizeof(e2), hipMemcpyHostToDevice, s));
+            nsurv = io->survivor_capacity;
+        }
+        if (nsurv > 0) {
+            HIP_TRY(hipMemcpyAsync(io->survivors, F->d_stage_sec, nsurv * sizeof(rrea_secondary),
+                                   hipMemcpyDeviceToDevice, s));
+        }
+        if (io->status) HIP_TRY(hipMemcpyAsync(io->status, F->d_fst, total, hipMemcpyDeviceToDevice, s));
+        if (io->ion_pairs) {
+            HIP_TRY(hipMemcpyAsync(io->ion_pairs, F->d_fion, total * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(io->work_count, &total, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(io->survivor_count, &nsurv, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return RREA_OK;
 }
 
 rrea_status rrea_photon_push(const rrea_tile* tile, double dt_s, rrea_stream_t stream)

@@ -39,7 +39,7 @@ SECONDARY_DTYPE = np.dtype([
     ("x", "<f8"), ("y", "<f8"), ("z", "<f8"),
     ("dir_x", "<f8"), ("dir_y", "<f8"), ("dir_z", "<f8"),
     ("weight", "<f8"), ("energy_eV", "<f8"), ("birth_fraction", "<f8"),
-    ("root_key", "<u8"), ("lineage", "<u8"),
+    ("root_key", "<u8"), ("lineage", "<u8"), ("root_species", "<i4"), ("generation", "<i4"),
 ])
 assert SECONDARY_DTYPE.itemsize == C.sizeof(_lib.Secondary)
 
@@ -290,6 +290,65 @@ class StepOutputs:
             raise _lib.RreaHipError(f"device transport error: {_lib.ERRORS.get(code, code)} at particle {index}")
 
 
+class DrainOutputs:
+    """Device buffers of the newborn residual advance (rrea_newborn_drain):
+    the FIFO, its per-entry status and ion-pair weight, and the survivors.
+    The J grid, tallies and error word are those of ``grid`` (the step's
+    StepOutputs), as the reference books newborn chords into the same
+    coupling scratch (RreaSecondaryCommit.cpp:627)."""
+
+    def __init__(self, capacity: int, grid: StepOutputs, device="cuda"):
+        self.capacity = capacity
+        self.work = torch.zeros(max(capacity, 1) * SECONDARY_DTYPE.itemsize, dtype=torch.uint8, device=device)
+        self.work_count = torch.zeros(1, dtype=torch.int64, device=device)
+        self.status = torch.zeros(max(capacity, 1), dtype=torch.uint8, device=device)
+        self.ion_pairs = torch.zeros(max(capacity, 1), dtype=torch.float64, device=device)
+        self.survivors = torch.zeros(max(capacity, 1) * SECONDARY_DTYPE.itemsize, dtype=torch.uint8, device=device)
+        self.survivor_count = torch.zeros(1, dtype=torch.int64, device=device)
+        self.grid = grid
+
+    def stage(self, staged) -> int:
+        """Concatenate the steps' ordered secondaries into the FIFO, in the
+        order given (Apply's: electrons, photons, positrons).  ``staged`` is a
+        list of StepOutputs, or host records (SECONDARY_DTYPE) already in
+        that order."""
+        size = SECONDARY_DTYPE.itemsize
+        if isinstance(staged, np.ndarray):
+            n = len(staged)
+            if n > self.capacity:
+                raise ValueError("the step's secondaries exceed the FIFO capacity")
+            if n:
+                raw = np.frombuffer(np.ascontiguousarray(staged, dtype=SECONDARY_DTYPE).tobytes(), dtype=np.uint8)
+                self.work[: n * size].copy_(torch.from_numpy(raw.copy()))
+            return n
+        at = 0
+        for out in staged:
+            n = min(int(out.secondary_count.item()), out.secondary_capacity)
+            if at + n > self.capacity:
+                raise ValueError("the step's secondaries exceed the FIFO capacity")
+            if n:
+                self.work[at * size:(at + n) * size].copy_(out.secondaries[: n * size])
+            at += n
+        return at
+
+    def struct(self, n_initial: int) -> _lib.DrainIO:
+        g = self.grid
+        return _lib.DrainIO(self.work.data_ptr(), self.capacity, n_initial, self.work_count.data_ptr(),
+                            self.status.data_ptr(), self.ion_pairs.data_ptr(), self.survivors.data_ptr(),
+                            self.capacity, self.survivor_count.data_ptr(), g.jr.data_ptr(), g.jz.data_ptr(),
+                            g.rho.data_ptr(), g.tallies.data_ptr(), g.error.data_ptr())
+
+    def work_host(self) -> np.ndarray:
+        n = int(self.work_count.item())
+        raw = self.work[: n * SECONDARY_DTYPE.itemsize].cpu().numpy()
+        return np.frombuffer(raw.tobytes(), dtype=SECONDARY_DTYPE, count=n)
+
+    def survivors_host(self) -> np.ndarray:
+        n = int(self.survivor_count.item())
+        raw = self.survivors[: n * SECONDARY_DTYPE.itemsize].cpu().numpy()
+        return np.frombuffer(raw.tobytes(), dtype=SECONDARY_DTYPE, count=n)
+
+
 def _step_config(cfg: StepConfig) -> _lib.StepConfigC:
     return _lib.StepConfigC(
         cfg.rng_seed, cfg.step, cfg.dt_s, cfg.low_energy_cutoff_eV, cfg.photon_cutoff_eV,
@@ -357,6 +416,24 @@ class RreaParticleInteraction:
                                                            C.byref(o), stream), "rrea_transport_step_photons")
         if check:
             out.check_error()
+
+    @staticmethod
+    def drain_secondaries(tables: InteractionTables, coupling: Coupling, cfg: StepConfig, staged, out: DrainOutputs,
+                          stream: int | None = None, check: bool = True) -> None:
+        """The newborn residual advance of TransportStep::DrainAndCommitSecondaries
+        (RreaSecondaryCommit.cpp:338-468): stage the species' ordered
+        secondaries (Apply's order) into the FIFO and fly every newborn
+        through the rest of the step.  Survivors are left in ``out.survivors``
+        for the caller's AddNParticles."""
+        n0 = out.stage(staged)
+        io = out.struct(n0)
+        c = _step_config(cfg)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().rrea_newborn_drain(tables.h, coupling.h, C.byref(c), C.byref(io), stream),
+                   "rrea_newborn_drain")
+        if check:
+            out.grid.check_error()
 
     @staticmethod
     def _apply(entry: str, tables: InteractionTables, coupling: Coupling, prev: DeviceTile, cfg: StepConfig,

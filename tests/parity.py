@@ -112,7 +112,7 @@ def compare_step(fields, tile_host_prev, oracle_out, tile_dev, out, label="", dt
     sg = out.secondaries_host()
     so = oracle_out["secondaries"]
     assert len(sg) == len(so), f"{label} secondary count {len(sg)} vs {len(so)}"
-    for f in ("parent_index", "species", "process", "root_key", "lineage"):
+    for f in ("parent_index", "species", "process", "root_key", "lineage", "root_species", "generation"):
         assert np.array_equal(sg[f], so[f]), f"{label} secondary {f} differs"
     assert np.array_equal(sg["weight"], so["weight"]), f"{label} secondary weight differs"
     # a secondary is born on its parent's chord: its payload carries the
@@ -136,8 +136,8 @@ def compare_step(fields, tile_host_prev, oracle_out, tile_dev, out, label="", dt
         scale = max(np.abs(o).max(), 1e-300)
         tol = 1e-11 * scale + (64.0 * EPS * coord * wmax / dt if k != "rho" else 64.0 * EPS * coord * wmax)
         assert np.abs(g - o).max() <= tol, f"{label} {k} differs: {np.abs(g - o).max()} of {scale} (tol {tol})"
-    t_g = out.tallies.cpu().numpy()[:13]
-    t_o = oracle_out["tallies"][:13]
+    t_g = out.tallies.cpu().numpy()[:17]
+    t_o = oracle_out["tallies"][:17]
     assert np.allclose(t_g, t_o, rtol=1e-10, atol=1e-300), f"{label} tallies {t_g} vs {t_o}"
 
 
@@ -212,7 +212,7 @@ def compare_photon_step(fields, p_in, oracle_out, tile_dev, out, label="", dt=No
     sg = out.secondaries_host()
     so = o["secondaries"]
     assert len(sg) == len(so), f"{label} secondary count {len(sg)} vs {len(so)}"
-    for f in ("parent_index", "species", "process", "root_key", "lineage"):
+    for f in ("parent_index", "species", "process", "root_key", "lineage", "root_species", "generation"):
         assert np.array_equal(sg[f], so[f]), f"{label} secondary {f} differs"
     assert np.array_equal(sg["weight"], so["weight"]), f"{label} secondary weight differs"
     kp = kappa[so["parent_index"]] if len(so) else np.zeros(0)
@@ -264,3 +264,91 @@ def run_photons_and_compare(otables, tables, coupling, fields, tile_host, cfg, s
         p = nxt
         tile_host = tile_host.take(keep)
     print(f"{label}: {steps} photon steps compared", flush=True)
+
+
+def drain_conditioning(work, dt):
+    """Round-trip conditioning of each FIFO entry's residual chord,
+    kappa = 1 + max(r, |z|) / l with l the residual flight length
+    (v (1 - f) dt, c for a photon), carried down to the entry's children."""
+    k = work["energy_eV"]
+    gamma = 1.0 + k / ME_C2
+    speed = np.where(work["species"] == 2, C_LIGHT,
+                     C_LIGHT * np.sqrt(np.maximum(0.0, 1.0 - 1.0 / (gamma * gamma))))
+    ell = np.maximum(speed * np.maximum(1.0 - work["birth_fraction"], 1e-6) * dt, 1e-300)
+    kappa = 1.0 + np.maximum(np.hypot(work["x"], work["y"]), np.abs(work["z"])) / ell
+    gen = work["generation"]
+    n0 = int(np.sum(gen == 0))
+    # a child's payload inherits its parent's (and so its ancestors') conditioning
+    for i in range(n0, len(work)):
+        kappa[i] = max(kappa[i], kappa[work["parent_index"][i]])
+    return kappa
+
+
+def compare_drain(fields, cfg, oracle_out, out, label="", stats=None):
+    """Assert the HIP newborn drain equals the oracle's FIFO: the same FIFO
+    (order, lineage, generation, parent, species, process exact), the same
+    statuses, payloads and survivors to 1e-12 plus the residual chords'
+    conditioning, tallies, and the J grid."""
+    wg = out.work_host()
+    wo = oracle_out["work"]
+    assert len(wg) == len(wo), f"{label} FIFO length {len(wg)} vs {len(wo)}"
+    n = len(wo)
+    for f in ("parent_index", "species", "process", "root_key", "lineage", "root_species", "generation"):
+        bad = np.flatnonzero(wg[f] != wo[f])
+        assert bad.size == 0, f"{label} FIFO {f} differs at {bad[:5]}: {wg[f][bad[:5]]} vs {wo[f][bad[:5]]}"
+    assert np.array_equal(wg["weight"], wo["weight"]), f"{label} FIFO weight differs"
+    st_g = out.status[:n].cpu().numpy().astype(np.int32)
+    bad = np.flatnonzero(st_g != oracle_out["status"])
+    assert bad.size == 0, f"{label} status differs at {bad[:10]}: {st_g[bad[:10]]} vs {oracle_out['status'][bad[:10]]}"
+    kappa = drain_conditioning(wo, cfg.dt_s)
+    kp = np.where(wo["generation"] > 0, kappa[np.minimum(wo["parent_index"], n - 1)], 1.0)
+    scale = np.maximum(np.maximum(np.hypot(wo["x"], wo["y"]), np.abs(wo["z"])), 1e-3)
+    for f in ("x", "y", "z"):
+        d = np.abs(wg[f] - wo[f])
+        bad = np.flatnonzero(d > (REL + 64.0 * EPS) * scale)
+        assert bad.size == 0, f"{label} FIFO {f} differs at {bad[:5]}: {d[bad[:5]]}"
+    for f in ("dir_x", "dir_y", "dir_z", "energy_eV", "birth_fraction"):
+        tol = (REL + 16.0 * EPS * kp) * np.maximum(np.maximum(np.abs(wg[f]), np.abs(wo[f])), 1.0)
+        bad = np.flatnonzero(np.abs(wg[f] - wo[f]) > tol)
+        assert bad.size == 0, f"{label} FIFO {f} differs at {bad[:5]}: {wg[f][bad[:5]]} vs {wo[f][bad[:5]]}"
+    ion_g = out.ion_pairs[:n].cpu().numpy()
+    ion_o = oracle_out["ion_pairs"]
+    bad = np.flatnonzero(np.abs(ion_g - ion_o) > (REL + 16.0 * EPS * kappa) * np.maximum(np.abs(ion_o), 1e-3))
+    assert bad.size == 0, f"{label} ion pairs differ at {bad[:5]}: {ion_g[bad[:5]]} vs {ion_o[bad[:5]]}"
+    sg = out.survivors_host()
+    so = oracle_out["survivors"]
+    assert len(sg) == len(so), f"{label} survivor count {len(sg)} vs {len(so)}"
+    alive = np.flatnonzero(oracle_out["status"] == 0)
+    ks = kappa[alive]
+    for f in ("species", "root_key", "lineage", "generation"):
+        assert np.array_equal(sg[f], so[f]), f"{label} survivor {f} differs"
+    sscale = np.maximum(np.maximum(np.hypot(so["x"], so["y"]), np.abs(so["z"])), 1e-3)
+    for f in ("x", "y", "z"):
+        d = np.abs(sg[f] - so[f])
+        bad = np.flatnonzero(d > (REL + 64.0 * EPS) * sscale)
+        assert bad.size == 0, f"{label} survivor {f} differs at {bad[:5]}: {d[bad[:5]]}"
+    rel_e = np.abs(sg["energy_eV"] - so["energy_eV"]) / np.maximum(so["energy_eV"], 1.0)
+    de = np.abs(so["energy_eV"] - wo["energy_eV"][alive]) / np.maximum(so["energy_eV"], 1.0)
+    bad = np.flatnonzero(rel_e > REL + 16.0 * EPS * ks * np.maximum(de, 1e-3))
+    assert bad.size == 0, f"{label} survivor energy differs at {bad[:5]}: {rel_e[bad[:5]]}"
+    dd = np.sqrt(sum((sg[f] - so[f]) ** 2 for f in ("dir_x", "dir_y", "dir_z")))
+    ddir = np.sqrt(sum((so[f] - wo[f][alive]) ** 2 for f in ("dir_x", "dir_y", "dir_z")))
+    bad = np.flatnonzero(dd > REL + 16.0 * EPS * ks * np.maximum(ddir, 1e-3))
+    assert bad.size == 0, f"{label} survivor direction differs at {bad[:5]}: {dd[bad[:5]]}"
+    if stats is not None:
+        stats["entries"] = stats.get("entries", 0) + n
+        stats["survivors"] = stats.get("survivors", 0) + len(so)
+        stats["within_1e-12"] = stats.get("within_1e-12", 0) + int((rel_e <= REL).sum())
+        stats["max_rel_e"] = max(stats.get("max_rel_e", 0.0), float(rel_e.max(initial=0.0)))
+    t_g = out.grid.tallies.cpu().numpy()
+    t_o = oracle_out["tallies"]
+    assert np.array_equal(t_g[17:21], t_o[17:21]), f"{label} newborn counts {t_g[17:21]} vs {t_o[17:21]}"
+    assert np.allclose(t_g[:17], t_o[:17], rtol=1e-9, atol=1e-300), f"{label} tallies {t_g[:17]} vs {t_o[:17]}"
+    wmax = float(np.max(wo["weight"], initial=0.0))
+    coord = fields.r_hi / fields.dr + max(abs(fields.z_lo), abs(fields.z_hi)) / fields.dz
+    for k in ("jr", "jz", "rho"):
+        g = getattr(out.grid, k).cpu().numpy()
+        o = oracle_out[k]
+        sc = max(np.abs(o).max(), 1e-300)
+        tol = 1e-11 * sc + (64.0 * EPS * coord * wmax / cfg.dt_s if k != "rho" else 64.0 * EPS * coord * wmax)
+        assert np.abs(g - o).max() <= tol, f"{label} {k} differs: {np.abs(g - o).max()} of {sc} (tol {tol})"
