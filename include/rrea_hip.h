@@ -312,9 +312,11 @@ typedef struct rrea_drain_io {
     int64_t work_capacity;
     int64_t n_initial;
     int64_t* work_count;         /* device int64: FIFO length after the drain */
-    uint8_t* status;             /* device, per FIFO entry (may be NULL): RREA_SURVIVE, RREA_ESCAPE,
-                                    RREA_DEMOTED, RREA_ANNIHILATED or RREA_ABSORBED */
+    uint8_t* status;             /* device, per FIFO entry (may be NULL; work_capacity entries):
+                                    RREA_SURVIVE, RREA_ESCAPE, RREA_DEMOTED, RREA_ANNIHILATED or RREA_ABSORBED */
     double* ion_pairs;           /* device, per FIFO entry ion-pair weight (may be NULL) */
+    int32_t* substeps;           /* device, per FIFO entry (may be NULL): residual substeps of a charged
+                                    newborn, interactions of a photon newborn */
     rrea_secondary* survivors;   /* device: the entries alive at the end of the step, in FIFO order,
                                     at their end-of-step state with birth_fraction 1 */
     int64_t survivor_capacity;

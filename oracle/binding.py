@@ -137,7 +137,7 @@ def lib():
                                       + [C.c_int64, C.POINTER(Secondary), P_I64, P_D, C.c_int])
         L.orc_newborn_drain.restype = C.c_int
         L.orc_newborn_drain.argtypes = [C.c_void_p, C.POINTER(Geometry), C.POINTER(Config), C.POINTER(Secondary),
-                                        C.c_int64, C.c_int64, C.POINTER(Secondary), P_I64, P_I32, P_D,
+                                        C.c_int64, C.c_int64, C.POINTER(Secondary), P_I64, P_I32, P_D, P_I64,
                                         C.POINTER(Secondary), P_I64, P_D, P_D, P_D, P_D]
         L.orc_pack_transient_index.argtypes = [C.c_int] + [C.c_uint64] * 5 + [C.POINTER(C.c_uint64)]
         L.orc_pack_transient_index.restype = C.c_int
@@ -407,6 +407,7 @@ def newborn_drain(tables: Tables, fields, cfg, initial: np.ndarray, cap: int | N
     surv = (Secondary * cap)()
     status = np.zeros(cap, dtype=np.int32)
     ion = np.zeros(cap)
+    sub = np.zeros(cap, dtype=np.int64)
     nw = C.c_int64(0)
     ns = C.c_int64(0)
     nrp, nzp = fields.padded_shape()
@@ -415,12 +416,13 @@ def newborn_drain(tables: Tables, fields, cfg, initial: np.ndarray, cap: int | N
     rho = np.zeros(nrp * nzp)
     tallies = np.zeros(N_TALLIES)
     code = L.orc_newborn_drain(tables.h, C.byref(g), C.byref(c), src, n0, cap, work, C.byref(nw), _ptr(status, P_I32),
-                               _ptr(ion), surv, C.byref(ns), _ptr(jr), _ptr(jz), _ptr(rho), _ptr(tallies))
+                               _ptr(ion), _ptr(sub, P_I64), surv, C.byref(ns), _ptr(jr), _ptr(jz), _ptr(rho),
+                               _ptr(tallies))
     del keep
     w = np.frombuffer(bytes(work), dtype=SECONDARY_DTYPE, count=cap)[: nw.value].copy()
     sv = np.frombuffer(bytes(surv), dtype=SECONDARY_DTYPE, count=cap)[: ns.value].copy()
     return {"code": code, "work": w, "status": status[: nw.value].copy(), "ion_pairs": ion[: nw.value].copy(),
-            "survivors": sv, "jr": jr, "jz": jz, "rho": rho, "tallies": tallies}
+            "substeps": sub[: nw.value].copy(), "survivors": sv, "jr": jr, "jz": jz, "rho": rho, "tallies": tallies}
 
 
 def pack_transient_index(newborn: bool, species: int, lineage: int, segment: int, event: int, inner: int):

@@ -2683,6 +2683,7 @@ inline bool newborn_electron_demotes(StepContext const& ctx, double x, double y,
 struct NewbornOut {
     int status = ORC_SURVIVE;
     double ion_pairs = 0.0;
+    uint64_t substeps = 0;  // charged: the residual substep count; photon: interactions
 };
 
 // TransportStep::AdvanceChargedNewborn OV/RreaNewbornAdvance.cpp:50-777.  On
@@ -2788,6 +2789,7 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                  driver_count((std::abs(whole_work) + drag_est) / span, impulse)),
         cfg.max_subcycles);
     if (substeps >= kSegmentCap) fail(ORC_ERR_INVALID, "newborn substep count outside its transient RNG budget");
+    out.substeps = substeps;
 
     double run_x = p.x, run_y = p.y, run_z = p.z;
     double const sub_dt = residual_dt / static_cast<double>(substeps);
@@ -3155,6 +3157,7 @@ inline void photon_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary& 
         double const r_site = std::hypot(x, y);
         double const el = std::min(1.0, elapsed_start + dist / std::max(kC * dt, std::numeric_limits<double>::min()));
         int const channel = sample_photon_channel(t, energy, rng_uniform(seed, root, step, idx, kChNewbornCompetingChannel));
+        out.substeps = ord + 1U;
         if (!photon_final_state(ctx, em, channel, idx, x, y, z, el, energy, dx, dy, dz)) {
             out.status = ORC_ABSORBED;
             fx.tallies[ORC_T_NEWBORN_TERMINATED] += 1.0;
@@ -3682,7 +3685,8 @@ double orc_sample_positron_brems_energy(void* h, double e, double u, int* err)
 // deposit into jr/jz/rho; tallies accumulate.
 extern "C" int orc_newborn_drain(void* h, const orc_geometry* g, const orc_config* cfg, const orc_secondary* initial,
                                  int64_t n_initial, int64_t cap, orc_secondary* work, int64_t* n_work,
-                                 int32_t* status, double* ion_pairs, orc_secondary* survivors, int64_t* n_surv,
+                                 int32_t* status, double* ion_pairs, int64_t* substeps, orc_secondary* survivors,
+                                 int64_t* n_surv,
                                  double* jr, double* jz, double* rho, double* tallies)
 {
     *n_work = 0;
@@ -3725,6 +3729,7 @@ extern "C" int orc_newborn_drain(void* h, const orc_geometry* g, const orc_confi
         }
         std::vector<int32_t> st;
         std::vector<double> ion;
+        std::vector<int64_t> sub;
         std::vector<orc_secondary> surv;
         for (size_t i = 0; i < fifo.size(); ++i) {
             orc_secondary p = fifo[i];
@@ -3760,6 +3765,7 @@ extern "C" int orc_newborn_drain(void* h, const orc_geometry* g, const orc_confi
             }
             st.push_back(o.status);
             ion.push_back(o.ion_pairs);
+            sub.push_back(static_cast<int64_t>(o.substeps));
             if (o.status == ORC_SURVIVE) {
                 p.birth_fraction = 1.0;
                 surv.push_back(p);
@@ -3771,6 +3777,7 @@ extern "C" int orc_newborn_drain(void* h, const orc_geometry* g, const orc_confi
             work[i] = fifo[i];
             status[i] = st[i];
             ion_pairs[i] = ion[i];
+            substeps[i] = sub[i];
         }
         for (size_t i = 0; i < surv.size(); ++i) survivors[i] = surv[i];
         *n_work = static_cast<int64_t>(fifo.size());

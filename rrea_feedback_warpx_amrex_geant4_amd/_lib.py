@@ -111,7 +111,7 @@ class StepOut(C.Structure):
 class DrainIO(C.Structure):
     _fields_ = [
         ("work", C.c_void_p), ("work_capacity", C.c_int64), ("n_initial", C.c_int64), ("work_count", C.c_void_p),
-        ("status", C.c_void_p), ("ion_pairs", C.c_void_p), ("survivors", C.c_void_p),
+        ("status", C.c_void_p), ("ion_pairs", C.c_void_p), ("substeps", C.c_void_p), ("survivors", C.c_void_p),
         ("survivor_capacity", C.c_int64), ("survivor_count", C.c_void_p),
         ("jr", C.c_void_p), ("jz", C.c_void_p), ("rho", C.c_void_p), ("tallies", C.c_void_p),
         ("error", C.c_void_p),

@@ -198,6 +198,7 @@ struct rrea_fields_s {
     rrea_secondary* d_fin = nullptr;
     uint8_t* d_fst = nullptr;
     double* d_fion = nullptr;
+    int32_t* d_fsub = nullptr;
     int64_t* d_anchor = nullptr;
     unsigned long long* d_next = nullptr;
     int32_t* d_flag = nullptr;
@@ -1719,6 +1720,7 @@ struct NewbornArgs {
     rrea_secondary* fin;   // per entry: the state at the end of the step
     uint8_t* status;       // per entry
     double* ion;           // per entry: ion-pair weight
+    int32_t* substeps;     // per entry: residual substeps (charged) or interactions (photon)
 };
 
 // RreaTryPackTransientRngIndex OV/RreaParticleInteraction.H:660-686 with the
@@ -1801,6 +1803,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
     int32_t err = kOk;
     double ion = 0.0;
     uint64_t lineage = 0;
+    uint64_t nsub = 0;
     int status = RREA_SURVIVE;
     uint64_t const rsp = static_cast<uint64_t>(p.root_species);
     uint64_t const lin = p.lineage;
@@ -1893,6 +1896,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                     substeps = gsd > imd ? gsd : imd;
                     if (substeps > cfg.max_subcycles) substeps = cfg.max_subcycles;
                     if (substeps >= kSegmentCap && !err) err = kErrInvalid;
+                    nsub = substeps;
                 }
                 double run_x = p.x, run_y = p.y, run_z = p.z;
                 double const sub_dt = residual_dt / static_cast<double>(substeps);
@@ -2219,6 +2223,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
     N.fin[i] = p;
     N.status[i] = static_cast<uint8_t>(status);
     N.ion[i] = ion;
+    N.substeps[i] = static_cast<int32_t>(nsub);
     tally[8] += ion;
     if (status == RREA_SURVIVE) tally[18] += 1.0;
 }
@@ -2240,6 +2245,7 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
     int32_t err = kOk;
     double ion = 0.0;
     uint64_t lineage = 0;
+    uint64_t nsub = 0;
     int status = RREA_SURVIVE;
     uint64_t const rsp = static_cast<uint64_t>(p.root_species);
     double const w = p.weight;
@@ -2315,6 +2321,7 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
                         else if (target < rates.compton + rates.pe + rates.pair_n) channel = kPairNuclear;
                         else channel = kPairTriplet;
                     }
+                    nsub = ord + 1U;
                     bool const survives = photon_final_state(T, cutoff, gcut, rng, em, channel, idx, x, y, z, el, energy,
                                                              dx, dy, dz, tally);
                     if (err) break;
@@ -2342,6 +2349,7 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
     N.fin[i] = p;
     N.status[i] = static_cast<uint8_t>(status);
     N.ion[i] = ion;
+    N.substeps[i] = static_cast<int32_t>(nsub);
     tally[8] += ion;
     if (status == RREA_SURVIVE) tally[18] += 1.0;
 }
@@ -2371,6 +2379,7 @@ __global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __re
             N.fin[i] = p;
             N.status[i] = static_cast<uint8_t>(RREA_SURVIVE);
             N.ion[i] = 0.0;
+            N.substeps[i] = 0;
         }
     }
     reduce_tallies<kTallyNewborn>(tally, N.S.tallies);
@@ -2706,7 +2715,7 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
     void* ptrs[] = {F->d_cell_er, F->d_cell_ez, F->d_btheta, F->d_stage, F->d_prof, F->d_stage_sec, F->d_keys,
                     F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args,
                     F->d_slow_list, F->d_slow_count, F->d_prof_guide, F->d_nargs, F->d_fin, F->d_fst,
-                    F->d_fion, F->d_anchor, F->d_next, F->d_flag, F->d_nsel, F->d_sel_tmp};
+                    F->d_fion, F->d_fsub, F->d_anchor, F->d_next, F->d_flag, F->d_nsel, F->d_sel_tmp};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : F->events) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
     delete F;
@@ -3117,14 +3126,16 @@ rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step
         if (ws != RREA_OK) return ws;
     }
     if (F->fifo_cap < cap) {
-        void* ptrs[] = {F->d_fin, F->d_fst, F->d_fion, F->d_anchor, F->d_next, F->d_flag, F->d_sel_tmp};
+        void* ptrs[] = {F->d_fin, F->d_fst, F->d_fion, F->d_fsub, F->d_anchor, F->d_next, F->d_flag, F->d_sel_tmp};
         for (void* p : ptrs) if (p) (void)hipFree(p);
-        F->d_fin = nullptr; F->d_fst = nullptr; F->d_fion = nullptr; F->d_anchor = nullptr; F->d_next = nullptr;
+        F->d_fin = nullptr; F->d_fst = nullptr; F->d_fion = nullptr; F->d_fsub = nullptr;
+        F->d_anchor = nullptr; F->d_next = nullptr;
         F->d_flag = nullptr; F->d_sel_tmp = nullptr;
         F->fifo_cap = 0;
         HIP_TRY(hipMalloc(&F->d_fin, cap * sizeof(rrea_secondary)));
         HIP_TRY(hipMalloc(&F->d_fst, cap * sizeof(uint8_t)));
         HIP_TRY(hipMalloc(&F->d_fion, cap * sizeof(double)));
+        HIP_TRY(hipMalloc(&F->d_fsub, cap * sizeof(int32_t)));
         HIP_TRY(hipMalloc(&F->d_anchor, cap * sizeof(int64_t)));
         HIP_TRY(hipMalloc(&F->d_next, cap * sizeof(unsigned long long)));
         HIP_TRY(hipMalloc(&F->d_flag, cap * sizeof(int32_t)));
@@ -3155,7 +3166,7 @@ rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step
         A.sec = F->d_stage_sec;
         A.sec_cap = cap - ge;
         A.sec_count = F->d_count;
-        NewbornArgs N{A, io->work, gb, ge, F->d_fin, F->d_fst, F->d_fion};
+        NewbornArgs N{A, io->work, gb, ge, F->d_fin, F->d_fst, F->d_fion, F->d_fsub};
         // pageable source: the copy has consumed N when hipMemcpyAsync returns
         HIP_TRY(hipMemcpyAsync(F->d_nargs, &N, sizeof(NewbornArgs), hipMemcpyHostToDevice, s));
         newborn_kernel<<<dim3(static_cast<unsigned>(blocks_for(ge - gb, kBlock))), dim3(kBlock), 0, s>>>(
@@ -3204,6 +3215,9 @@ rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step
         if (io->status) HIP_TRY(hipMemcpyAsync(io->status, F->d_fst, total, hipMemcpyDeviceToDevice, s));
         if (io->ion_pairs) {
             HIP_TRY(hipMemcpyAsync(io->ion_pairs, F->d_fion, total * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+        if (io->substeps) {
+            HIP_TRY(hipMemcpyAsync(io->substeps, F->d_fsub, total * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
         }
     }
     HIP_TRY(hipMemcpyAsync(io->work_count, &total, sizeof(int64_t), hipMemcpyHostToDevice, s));

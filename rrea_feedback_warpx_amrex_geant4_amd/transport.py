@@ -303,6 +303,7 @@ class DrainOutputs:
         self.work_count = torch.zeros(1, dtype=torch.int64, device=device)
         self.status = torch.zeros(max(capacity, 1), dtype=torch.uint8, device=device)
         self.ion_pairs = torch.zeros(max(capacity, 1), dtype=torch.float64, device=device)
+        self.substeps = torch.zeros(max(capacity, 1), dtype=torch.int32, device=device)
         self.survivors = torch.zeros(max(capacity, 1) * SECONDARY_DTYPE.itemsize, dtype=torch.uint8, device=device)
         self.survivor_count = torch.zeros(1, dtype=torch.int64, device=device)
         self.grid = grid
@@ -334,7 +335,8 @@ class DrainOutputs:
     def struct(self, n_initial: int) -> _lib.DrainIO:
         g = self.grid
         return _lib.DrainIO(self.work.data_ptr(), self.capacity, n_initial, self.work_count.data_ptr(),
-                            self.status.data_ptr(), self.ion_pairs.data_ptr(), self.survivors.data_ptr(),
+                            self.status.data_ptr(), self.ion_pairs.data_ptr(), self.substeps.data_ptr(),
+                            self.survivors.data_ptr(),
                             self.capacity, self.survivor_count.data_ptr(), g.jr.data_ptr(), g.jz.data_ptr(),
                             g.rho.data_ptr(), g.tallies.data_ptr(), g.error.data_ptr())
 

@@ -266,15 +266,17 @@ def run_photons_and_compare(otables, tables, coupling, fields, tile_host, cfg, s
     print(f"{label}: {steps} photon steps compared", flush=True)
 
 
-def drain_conditioning(work, dt):
-    """Round-trip conditioning of each FIFO entry's residual chord,
-    kappa = 1 + max(r, |z|) / l with l the residual flight length
-    (v (1 - f) dt, c for a photon), carried down to the entry's children."""
+def drain_conditioning(work, dt, substeps):
+    """Round-trip conditioning of each FIFO entry's residual chords,
+    kappa = 1 + max(r, |z|) / l with l the microchord length (v (1 - f) dt
+    over the entry's substeps, c (1 - f) dt for a photon), carried down to
+    the entry's children."""
     k = work["energy_eV"]
     gamma = 1.0 + k / ME_C2
     speed = np.where(work["species"] == 2, C_LIGHT,
                      C_LIGHT * np.sqrt(np.maximum(0.0, 1.0 - 1.0 / (gamma * gamma))))
-    ell = np.maximum(speed * np.maximum(1.0 - work["birth_fraction"], 1e-6) * dt, 1e-300)
+    nsub = np.where(work["species"] == 2, 1, np.maximum(substeps, 1))
+    ell = np.maximum(speed * np.maximum(1.0 - work["birth_fraction"], 1e-6) * dt / nsub, 1e-300)
     kappa = 1.0 + np.maximum(np.hypot(work["x"], work["y"]), np.abs(work["z"])) / ell
     gen = work["generation"]
     n0 = int(np.sum(gen == 0))
@@ -300,7 +302,10 @@ def compare_drain(fields, cfg, oracle_out, out, label="", stats=None):
     st_g = out.status[:n].cpu().numpy().astype(np.int32)
     bad = np.flatnonzero(st_g != oracle_out["status"])
     assert bad.size == 0, f"{label} status differs at {bad[:10]}: {st_g[bad[:10]]} vs {oracle_out['status'][bad[:10]]}"
-    kappa = drain_conditioning(wo, cfg.dt_s)
+    sub_g = out.substeps[:n].cpu().numpy().astype(np.int64)
+    bad = np.flatnonzero(sub_g != oracle_out["substeps"])
+    assert bad.size == 0, f"{label} substeps differ at {bad[:10]}: {sub_g[bad[:10]]} vs {oracle_out['substeps'][bad[:10]]}"
+    kappa = drain_conditioning(wo, cfg.dt_s, oracle_out["substeps"])
     kp = np.where(wo["generation"] > 0, kappa[np.minimum(wo["parent_index"], n - 1)], 1.0)
     scale = np.maximum(np.maximum(np.hypot(wo["x"], wo["y"]), np.abs(wo["z"])), 1e-3)
     for f in ("x", "y", "z"):
