@@ -268,16 +268,19 @@ def run_photons_and_compare(otables, tables, coupling, fields, tile_host, cfg, s
 
 def drain_conditioning(work, dt, substeps):
     """Round-trip conditioning of each FIFO entry's residual chords,
-    kappa = 1 + max(r, |z|) / l with l the microchord length (v (1 - f) dt
-    over the entry's substeps, c (1 - f) dt for a photon), carried down to
-    the entry's children."""
+    kappa = (1 + max(r, |z|) / l) n with l the microchord length (v (1 - f)
+    dt over the entry's n substeps, c (1 - f) dt for a photon, n = 1), carried
+    down to the entry's children.  The factor n: each substep's multiple
+    scattering feeds the previous substeps' last-place differences into the
+    next chord's direction and field work, so they add up over the substeps
+    rather than staying at one chord's worth."""
     k = work["energy_eV"]
     gamma = 1.0 + k / ME_C2
     speed = np.where(work["species"] == 2, C_LIGHT,
                      C_LIGHT * np.sqrt(np.maximum(0.0, 1.0 - 1.0 / (gamma * gamma))))
     nsub = np.where(work["species"] == 2, 1, np.maximum(substeps, 1))
     ell = np.maximum(speed * np.maximum(1.0 - work["birth_fraction"], 1e-6) * dt / nsub, 1e-300)
-    kappa = 1.0 + np.maximum(np.hypot(work["x"], work["y"]), np.abs(work["z"])) / ell
+    kappa = (1.0 + np.maximum(np.hypot(work["x"], work["y"]), np.abs(work["z"])) / ell) * nsub
     gen = work["generation"]
     n0 = int(np.sum(gen == 0))
     # a child's payload inherits its parent's (and so its ancestors') conditioning
@@ -333,12 +336,17 @@ def compare_drain(fields, cfg, oracle_out, out, label="", stats=None):
         bad = np.flatnonzero(d > (REL + 64.0 * EPS) * sscale)
         assert bad.size == 0, f"{label} survivor {f} differs at {bad[:5]}: {d[bad[:5]]}"
     rel_e = np.abs(sg["energy_eV"] - so["energy_eV"]) / np.maximum(so["energy_eV"], 1.0)
-    de = np.abs(so["energy_eV"] - wo["energy_eV"][alive]) / np.maximum(so["energy_eV"], 1.0)
-    bad = np.flatnonzero(rel_e > REL + 16.0 * EPS * ks * np.maximum(de, 1e-3))
-    assert bad.size == 0, f"{label} survivor energy differs at {bad[:5]}: {rel_e[bad[:5]]}"
+    bad = np.flatnonzero(rel_e > REL + 16.0 * EPS * ks)
+    if bad.size:
+        w_bad = wo[alive[bad[:5]]]
+        detail = [(int(a), int(w["species"]), int(w["generation"]), float(w["birth_fraction"]), float(w["energy_eV"]),
+                   float(so["energy_eV"][b]), int(oracle_out["substeps"][a]), float(ks[b]),
+                   float(np.hypot(w["x"], w["y"])), float(w["z"]))
+                  for a, b, w in zip(alive[bad[:5]], bad[:5], w_bad)]
+        raise AssertionError(f"{label} survivor energy differs at {bad[:5]}: {rel_e[bad[:5]]}; "
+                             f"(entry, species, generation, birth, E0, E1, substeps, kappa, r, z) = {detail}")
     dd = np.sqrt(sum((sg[f] - so[f]) ** 2 for f in ("dir_x", "dir_y", "dir_z")))
-    ddir = np.sqrt(sum((so[f] - wo[f][alive]) ** 2 for f in ("dir_x", "dir_y", "dir_z")))
-    bad = np.flatnonzero(dd > REL + 16.0 * EPS * ks * np.maximum(ddir, 1e-3))
+    bad = np.flatnonzero(dd > REL + 16.0 * EPS * ks)
     assert bad.size == 0, f"{label} survivor direction differs at {bad[:5]}: {dd[bad[:5]]}"
     if stats is not None:
         stats["entries"] = stats.get("entries", 0) + n
@@ -349,11 +357,15 @@ def compare_drain(fields, cfg, oracle_out, out, label="", stats=None):
     t_o = oracle_out["tallies"]
     assert np.array_equal(t_g[17:21], t_o[17:21]), f"{label} newborn counts {t_g[17:21]} vs {t_o[17:21]}"
     assert np.allclose(t_g[:17], t_o[:17], rtol=1e-9, atol=1e-300), f"{label} tallies {t_g[:17]} vs {t_o[:17]}"
+    # a newborn's chord ends where its transport ended, which the comparison
+    # above allows to differ by (1e-12 + 64 eps) of the coordinates; a face
+    # flux q/dt (x1 - x0)/dx inherits that in index units
     wmax = float(np.max(wo["weight"], initial=0.0))
     coord = fields.r_hi / fields.dr + max(abs(fields.z_lo), abs(fields.z_hi)) / fields.dz
     for k in ("jr", "jz", "rho"):
         g = getattr(out.grid, k).cpu().numpy()
         o = oracle_out[k]
         sc = max(np.abs(o).max(), 1e-300)
-        tol = 1e-11 * sc + (64.0 * EPS * coord * wmax / cfg.dt_s if k != "rho" else 64.0 * EPS * coord * wmax)
+        edge = (REL + 64.0 * EPS) * coord * wmax
+        tol = 1e-11 * sc + (edge / cfg.dt_s if k != "rho" else edge)
         assert np.abs(g - o).max() <= tol, f"{label} {k} differs: {np.abs(g - o).max()} of {sc} (tol {tol})"
