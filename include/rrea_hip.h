@@ -343,6 +343,8 @@ enum {
     RREA_KERNEL_POSITRON = 2,
     RREA_KERNEL_POSITRON_HANDOVER = 3,
     RREA_KERNEL_PHOTON = 4,
+    RREA_KERNEL_PHOTON_HANDOVER = 5,
+    RREA_KERNEL_NEWBORN = 6,
 };
 rrea_status rrea_kernel_timing(rrea_fields_t fields, int32_t enable);
 rrea_status rrea_kernel_timing_read(rrea_fields_t fields, int32_t kernel, double* total_ms, int64_t* launches);

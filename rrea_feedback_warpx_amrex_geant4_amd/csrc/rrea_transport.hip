@@ -1666,6 +1666,99 @@ __global__ void __launch_bounds__(kBlock) photon_kernel(const StepArgs* __restri
     reduce_tallies(tally, A.tallies);
 }
 
+// The photon fast path.  At small dt nearly every photon is valid, weighted,
+// above the photon cutoff and draws a depth beyond its chord: it survives at
+// (or escapes through) the end of the clipped chord from the captured point to
+// the pushed point.  That branch of the complete step is taken here with its
+// own arithmetic, in the same order; any other outcome -- an interaction, a
+// sub-cutoff or weightless photon, an error -- is detected before the first
+// write and the photon is handed over to the complete step.
+__global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __restrict__ args,
+                                                           int32_t* __restrict__ slow_list,
+                                                           int32_t* __restrict__ slow_count)
+{
+    StepArgs const& A = *args;
+    double tally[2] = {0.0, 0.0};  // survivors, escapes
+    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    bool slow = false;
+    if (ip < A.n) {
+        slow = true;
+        DevTables const& T = A.T;
+        Domain const& D = A.D;
+        rrea_step_config const& cfg = A.cfg;
+        int64_t const id = A.id[ip];
+        if (id == RREA_INVALID_ID) {
+            if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SKIPPED_INVALID);
+            if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
+            if (A.substeps) A.substeps[ip] = 0;
+            slow = false;
+        } else do {
+            double const w = A.w[ip];
+            if (!(w > 0.0)) break;
+            double const ux = A.ux[ip], uy = A.uy[ip], uz = A.uz[ip];
+            double const energy = sqrt(ux * ux + uy * uy + uz * uz);
+            if (!(energy >= cfg.photon_cutoff_eV)) break;
+            double const pr = A.prev_r[ip], pth = A.prev_th[ip], pz = A.prev_z[ip];
+            if (pr > D.r_hi + 1.0e-9 * D.r_hi || pz < D.z_lo - 1.0e-9 || pz > D.z_hi + 1.0e-9 * D.z_hi) break;
+            int32_t err = kOk;
+            Path path;
+            if (!clip_rz_path(pr, pth, pz, A.r[ip], A.th[ip], A.z[ip], D, path, err) || err) break;
+            bool const escapes = clipped_escapes(path, D);
+            Interval const iv = build_interval(A.F, path, D, A.density_ratio, err);
+            double const chord_dp = density_path(iv, path, 0.0, path.end_f);
+            if (err) break;
+            if (path.in_len > 0.0) {
+                PhotonRates rates;
+                if (!photon_rates(T, energy, rates, err) || err) break;
+                double const total = rates.compton + rates.pe + rates.pair_n + rates.pair_t;
+                if (!finite(total) || total < 0.0) break;
+                double const mfp = total > 0.0 ? 1.0 / total : 1.0e99;
+                int32_t const cpu = A.cpu ? A.cpu[ip] : 0;
+                uint64_t const pkey = static_cast<uint64_t>(id < 0 ? -id : id)
+                    ^ (static_cast<uint64_t>(static_cast<uint32_t>(cpu)) << 40);
+                uint64_t const step = static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0);
+                // the first pass's live index is segment 0, event 0: index 0
+                double const draw = rng_uniform(rng_prefix(cfg.rng_seed, pkey, step), 0U, kChPhotonInteraction);
+                if (select_hazard(chord_dp / mfp, 0.0, draw, 0.0).event) break;
+            }
+            // ---- commit: nothing above has written anything ----
+            slow = false;
+            double x, y, z;
+            point_at_full(path, path.end_f, x, y, z);
+            A.r[ip] = hypot(x, y);
+            A.th[ip] = atan2(y, x);
+            A.z[ip] = z;
+            if (escapes) {
+                A.id[ip] = RREA_INVALID_ID;
+                tally[1] += 1.0;
+            } else {
+                tally[0] += 1.0;
+            }
+            if (A.status) A.status[ip] = static_cast<uint8_t>(escapes ? RREA_ESCAPE : RREA_SURVIVE);
+            if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
+            if (A.substeps) A.substeps[ip] = 0;
+        } while (false);
+    }
+    append_slow(slow, ip, slow_list, slow_count);
+    reduce_tallies<2>(tally, A.tallies);
+}
+
+// The photons the fast path handed over, strided over a fixed grid.
+__global__ void __launch_bounds__(kBlock) photon_slow_kernel(const StepArgs* __restrict__ args,
+                                                           const int32_t* __restrict__ list,
+                                                           const int32_t* __restrict__ count)
+{
+    StepArgs const& A = *args;
+    double tally[kTallyUsed];
+    for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
+    int64_t const n = *count;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kBlock) {
+        photon_particle(A, list[i], tally);
+    }
+    reduce_tallies(tally, A.tallies);
+}
+
 // Every positron through the complete path (used when the fast path is off).
 __global__ void __launch_bounds__(kBlock) positron_full_kernel(const StepArgs* __restrict__ args)
 {
@@ -2732,7 +2825,7 @@ rrea_status rrea_kernel_timing(rrea_fields_t F, int32_t enable)
 
 rrea_status rrea_kernel_timing_read(rrea_fields_t F, int32_t kernel, double* total_ms, int64_t* launches)
 {
-    if (!F || !total_ms || !launches || kernel < -1 || kernel > 4) return RREA_ERR_ARGUMENT;
+    if (!F || !total_ms || !launches || kernel < -1 || kernel > 6) return RREA_ERR_ARGUMENT;
     double sum = 0.0;
     int64_t count = 0;
     for (size_t i = 0; i < F->events_used; ++i) {
@@ -2864,6 +2957,23 @@ rrea_status order_staged(rrea_fields_t F, int64_t n, rrea_secondary* dst, hipStr
     secondary_gather_kernel<<<dim3(static_cast<unsigned>(blocks_for(n, 256))), dim3(256), 0, s>>>(
         F->d_stage_sec, F->d_idx_out, F->d_count, n, dst);
     HIP_TRY(hipGetLastError());
+    return RREA_OK;
+}
+
+// The fast paths' hand-over list, sized for n particles, with its count
+// cleared on `s`.
+rrea_status slow_list_workspace(rrea_fields_t F, int64_t n, hipStream_t s)
+{
+    if (F->slow_cap < n) {
+        if (F->d_slow_list) (void)hipFree(F->d_slow_list);
+        F->d_slow_list = nullptr;
+        F->slow_cap = 0;
+        if (n > INT32_MAX) return RREA_ERR_ARGUMENT;
+        HIP_TRY(hipMalloc(&F->d_slow_list, n * sizeof(int32_t)));
+        F->slow_cap = n;
+    }
+    if (!F->d_slow_count) HIP_TRY(hipMalloc(&F->d_slow_count, sizeof(int32_t)));
+    HIP_TRY(hipMemsetAsync(F->d_slow_count, 0, sizeof(int32_t), s));
     return RREA_OK;
 }
 
@@ -3008,23 +3118,29 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     const StepArgs* const d_args = static_cast<const StepArgs*>(F->d_args);
     rrea_status st = RREA_OK;
     int32_t const kid = positron ? 2 : 0;   // timing ids: main kernel, then its hand-over kernel
-    if (photon) {
+    if (photon && cfg->complete_path_only) {
         st = timed(4, [&] { photon_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args); });
+    } else if (photon) {
+        rrea_status const ls = slow_list_workspace(F, tile->n, s);
+        if (ls != RREA_OK) return ls;
+        st = timed(4, [&] {
+            photon_fast_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args, F->d_slow_list,
+                                                                                      F->d_slow_count);
+        });
+        if (st != RREA_OK) return st;
+        int64_t const slow_blocks = nb < 2048 ? nb : 2048;
+        st = timed(5, [&] {
+            photon_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
+                d_args, F->d_slow_list, F->d_slow_count);
+        });
     } else if (cfg->complete_path_only) {
         st = timed(kid, [&] {
             if (positron) positron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
             else electron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
         });
     } else {
-        if (F->slow_cap < tile->n) {
-            if (F->d_slow_list) (void)hipFree(F->d_slow_list);
-            F->d_slow_list = nullptr;
-            if (tile->n > INT32_MAX) return RREA_ERR_ARGUMENT;
-            HIP_TRY(hipMalloc(&F->d_slow_list, tile->n * sizeof(int32_t)));
-            F->slow_cap = tile->n;
-        }
-        if (!F->d_slow_count) HIP_TRY(hipMalloc(&F->d_slow_count, sizeof(int32_t)));
-        HIP_TRY(hipMemsetAsync(F->d_slow_count, 0, sizeof(int32_t), s));
+        rrea_status const ls = slow_list_workspace(F, tile->n, s);
+        if (ls != RREA_OK) return ls;
         st = timed(kid, [&] {
             if (positron) {
                 charged_fast_kernel<true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
@@ -3169,9 +3285,25 @@ rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step
         NewbornArgs N{A, io->work, gb, ge, F->d_fin, F->d_fst, F->d_fion, F->d_fsub};
         // pageable source: the copy has consumed N when hipMemcpyAsync returns
         HIP_TRY(hipMemcpyAsync(F->d_nargs, &N, sizeof(NewbornArgs), hipMemcpyHostToDevice, s));
-        newborn_kernel<<<dim3(static_cast<unsigned>(blocks_for(ge - gb, kBlock))), dim3(kBlock), 0, s>>>(
-            static_cast<const NewbornArgs*>(F->d_nargs));
-        HIP_TRY(hipGetLastError());
+        {
+            hipEvent_t stop = nullptr;
+            if (F->timing) {
+                if (F->events_used == F->events.size()) {
+                    hipEvent_t a, b;
+                    HIP_TRY(hipEventCreate(&a));
+                    HIP_TRY(hipEventCreate(&b));
+                    F->events.push_back({a, b, 0});
+                }
+                rrea_fields_s::Timed& t = F->events[F->events_used++];
+                t.kernel = 6;
+                HIP_TRY(hipEventRecord(t.start, s));
+                stop = t.stop;
+            }
+            newborn_kernel<<<dim3(static_cast<unsigned>(blocks_for(ge - gb, kBlock))), dim3(kBlock), 0, s>>>(
+                static_cast<const NewbornArgs*>(F->d_nargs));
+            HIP_TRY(hipGetLastError());
+            if (stop) HIP_TRY(hipEventRecord(stop, s));
+        }
         int64_t c = 0;
         HIP_TRY(hipMemcpyAsync(&c, F->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(err_host, io->error, sizeof(err_host), hipMemcpyDeviceToHost, s));

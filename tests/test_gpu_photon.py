@@ -92,3 +92,35 @@ def test_photon_push_and_determinism(tables):
     r, th, z = (t.cpu().numpy() for t in (dev.cur.r, dev.cur.theta, dev.cur.z))
     step = np.sqrt((r * np.cos(th) - pr * np.cos(pth)) ** 2 + (r * np.sin(th) - pr * np.sin(pth)) ** 2 + (z - pz) ** 2)
     assert np.allclose(step, 299792458.0 * 1e-7, rtol=1e-9)
+
+
+@pytest.mark.parametrize("dt", [1e-12, 1e-8])
+def test_photon_fast_path_matches_complete_path(tables, otables, dt):
+    """The fast path (no interaction on the chord) and the complete step give
+    bit-identical results on the same photons; the complete path alone also
+    matches the oracle."""
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import PhotonTile, RreaParticleInteraction, StepOutputs
+    fields = wl.c2_fields()
+    cp = coupling_for(fields, tables)
+    tile = wl.c2_photons(1 << 14, fields, dt_s=dt, seed=43)
+    results = []
+    for complete in (False, True):
+        cfg = StepConfig(rng_seed=57, dt_s=dt, complete_path_only=complete)
+        dev = PhotonTile.from_host(tile)
+        out = StepOutputs(len(tile), fields, secondary_capacity=1 << 14)
+        RreaParticleInteraction.apply_photons(tables, cp, dev, cfg, out)
+        torch.cuda.synchronize()
+        results.append((dev, out))
+    (a, oa), (b, ob_) = results
+    for k in ("r", "z", "theta", "ux", "uy", "uz", "id"):
+        assert torch.equal(getattr(a.cur, k), getattr(b.cur, k)), k
+    assert torch.equal(oa.status, ob_.status) and torch.equal(oa.substeps, ob_.substeps)
+    assert torch.equal(oa.ion_pairs, ob_.ion_pairs)
+    sa, sb = oa.secondaries_host(), ob_.secondaries_host()
+    assert len(sa) == len(sb) and (sa == sb).all()
+    assert torch.allclose(oa.tallies, ob_.tallies, rtol=1e-12, atol=0.0)
+    print(f"photon fast == complete at dt={dt}: {int((oa.status == 0).sum())} survivors, {len(sa)} secondaries",
+          flush=True)
+    run_photons_and_compare(otables, tables, cp, fields, tile,
+                            StepConfig(rng_seed=57, dt_s=dt, complete_path_only=True), steps=1,
+                            label=f"photon complete dt={dt}")
