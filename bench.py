@@ -8,6 +8,10 @@ are transported: 70 % electrons (11,744,051) and 5 % positrons (838,861),
 K ~ Exp(7.3 MeV) clipped to [10 keV, 100 MeV] in a 30-degree cone about the
 force; 25 % photons (4,194,304), isotropic, same energy law.
 
+Every --sort-every steps (default 25) the three tiles are reordered by cell
+and energy band (rrea_tile_sort, the particle sort WarpX applies to its
+tiles) inside the timed region.
+
 One "step" is one RreaParticleInteraction::Apply (RreaParticleInteraction.cpp:57-60):
 the electron kernels, the photon kernel, the positron kernels, each followed
 by the deterministic secondary ordering, then the newborn residual advance of
@@ -118,6 +122,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--particles", type=int, default=None, help="total macroparticles (default 2^24)")
+    ap.add_argument("--sort-every", type=int, default=25,
+                    help="reorder the tiles by cell every this many steps (0: never), inside the timed region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
     ap.add_argument("--cpu-steps", type=int, default=200)
@@ -170,6 +176,11 @@ def main():
     apply = RreaParticleInteraction
 
     def run_step(s):
+        if args.sort_every > 0 and s % args.sort_every == 0:
+            # WarpX's periodic particle sort: cell-ordered tiles
+            dev_e.sort_by_cell(coupling)
+            dev_p.sort_by_cell(coupling)
+            dev_g.sort_by_cell(coupling)
         for o in (out_e, out_g, out_p):
             o.secondary_count.zero_()
         out_e.jr.zero_()
@@ -249,6 +260,7 @@ def main():
                                "dt=1 ps, RZ 256x1024, deposit on, secondaries ordered and their newborn residual "
                                "advance drained, photon push on device",
                    "electrons": n_e, "photons": n_g, "positrons": n_p, "path": "plain",
+                   "tile_sort_every": args.sort_every,
                    "kernel_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in kms.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,

@@ -291,6 +291,18 @@ rrea_status rrea_transport_step_photons(rrea_tables_t tables, rrea_fields_t fiel
  * tile's const pointers. */
 rrea_status rrea_photon_push(const rrea_tile* tile, double dt_s, rrea_stream_t stream);
 
+/* Reorders one tile's particles by the cell of their captured position (r
+ * fastest) and, within a cell, by kinetic-energy band (photon energy for
+ * photon tiles, photon != 0); invalid ids go last.  Every distinct SoA array
+ * of the tile (prev_*, w, id, cpu and r..uz) is permuted together, written
+ * through the tile's pointers.  No counterpart in the overlay: this is the
+ * particle sort WarpX applies to its tiles, offered so that a caller can keep
+ * a tile cell-ordered, which lets the transport kernels accumulate a
+ * workgroup's current in LDS and gather fields from cache.  Results of a
+ * step do not depend on the order except through the order of the staged
+ * secondaries (by parent index) and floating-point summation order. */
+rrea_status rrea_tile_sort(rrea_fields_t fields, const rrea_tile* tile, int32_t photon, rrea_stream_t stream);
+
 /* The newborn residual advance of one step.  Replaces the transient FIFO of
  * TransportStep::DrainAndCommitSecondaries
  * (warpx_overlay/Source/Rrea/RreaSecondaryCommit.cpp:338-468) with its

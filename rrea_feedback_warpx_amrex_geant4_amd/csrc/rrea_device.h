@@ -1148,8 +1148,18 @@ __device__ void deposit_cic(DevGrid const& g, double q, double r, double z)
     }
 }
 
+// Where the face currents of a deposit go: straight to the grid with device
+// atomics, or into a workgroup's LDS window of it (LdsWindow below).
+struct GlobalSink {
+    DevGrid const& g;
+    __device__ void jr(int32_t i, int32_t j, double v) const { atomic_add(&g.jr[i + (g.nr + 1) * j], v); }
+    __device__ void jz(int32_t i, int32_t j, double v) const { atomic_add(&g.jz[i + g.nr * j], v); }
+};
+
 // detail::deposit_piece :96-152
-__device__ void deposit_piece(DevGrid const& g, double qdt, double xr0, double xz0, double xr1, double xz1)
+template <class Sink>
+__device__ void deposit_piece(DevGrid const& g, Sink const& sink, double qdt, double xr0, double xz0, double xr1,
+                              double xz1)
 {
     double fr_flux = qdt * (xr1 - xr0);
     double const fz_flux = qdt * (xz1 - xz0);
@@ -1167,7 +1177,7 @@ __device__ void deposit_piece(DevGrid const& g, double qdt, double xr0, double x
                 int32_t j = jf + b;
                 if (j < 0) j = 0;
                 if (j >= g.nz) j = g.nz - 1;
-                atomic_add(&g.jr[i + (g.nr + 1) * j], fr_flux * (b == 0 ? wz0 : fz));
+                sink.jr(i, j, fr_flux * (b == 0 ? wz0 : fz));
             }
         }
     }
@@ -1183,7 +1193,7 @@ __device__ void deposit_piece(DevGrid const& g, double qdt, double xr0, double x
             if (i < 0) i = -1 - i;
             if (i >= g.nr) i = g.nr - 1;
             if (i < 0) i = 0;  // only reachable for a non-finite position
-            atomic_add(&g.jz[i + g.nr * fj], fz_flux * (a == 0 ? wr0 : fr));
+            sink.jz(i, fj, fz_flux * (a == 0 ? wr0 : fr));
         }
     }
 }
@@ -1220,7 +1230,9 @@ struct CutStream {
 
 // DepositSegmentRZ :159-192, merging the two sorted cut streams in place of
 // the reference's collect-and-sort.
-__device__ bool deposit_segment(DevGrid const& g, double q, double dt, double r0, double z0, double r1, double z1)
+template <class Sink>
+__device__ bool deposit_segment_to(DevGrid const& g, Sink const& sink, double q, double dt, double r0, double z0,
+                                   double r1, double z1)
 {
     double const qdt = q / dt;
     double const xr0 = (r0 - g.r_lo) / g.dr - 0.5;
@@ -1243,13 +1255,46 @@ __device__ bool deposit_segment(DevGrid const& g, double q, double dt, double r0
         else if (b < 2.0) { s1 = b; cz.pop(); }
         else { s1 = 1.0; }
         if (s1 > s0) {
-            deposit_piece(g, qdt, xr0 + s0 * (xr1 - xr0), xz0 + s0 * (xz1 - xz0), xr0 + s1 * (xr1 - xr0),
+            deposit_piece(g, sink, qdt, xr0 + s0 * (xr1 - xr0), xz0 + s0 * (xz1 - xz0), xr0 + s1 * (xr1 - xr0),
                           xz0 + s1 * (xz1 - xz0));
         }
         s0 = s1;
         if (a >= 2.0 && b >= 2.0) break;
     }
     return true;
+}
+
+__device__ __forceinline__ bool deposit_segment(DevGrid const& g, double q, double dt, double r0, double z0, double r1,
+                                                double z1)
+{
+    return deposit_segment_to(g, GlobalSink{g}, q, dt, r0, z0, r1, z1);
+}
+
+// A workgroup's window of the face grids in LDS: faces (i, j) with
+// i0 <= i < i0 + wi and j0 <= j < j0 + wj, for jr and for jz.  Every face a
+// segment between two points can touch lies within [floor(x) - 1, floor(x) + 2]
+// of its end points' index coordinates (clamped to the grid), which is how
+// the window is sized.
+struct LdsWindow {
+    double* jr_w;
+    double* jz_w;
+    int32_t i0, j0, wi;
+    __device__ void jr(int32_t i, int32_t j, double v) const { atomicAdd(&jr_w[(i - i0) + wi * (j - j0)], v); }
+    __device__ void jz(int32_t i, int32_t j, double v) const { atomicAdd(&jz_w[(i - i0) + wi * (j - j0)], v); }
+};
+
+// The index-space face range [lo, hi] one segment end point can reach.
+__device__ __forceinline__ void segment_face_range(DevGrid const& g, double r, double z, int32_t& ilo, int32_t& ihi,
+                                                   int32_t& jlo, int32_t& jhi)
+{
+    double const xr = (r - g.r_lo) / g.dr - 0.5;
+    double const xz = (z - g.z_lo) / g.dz - 0.5;
+    int32_t const fi = static_cast<int32_t>(floor(xr));
+    int32_t const fj = static_cast<int32_t>(floor(xz));
+    ilo = fi - 1 < 0 ? 0 : fi - 1;
+    ihi = fi + 2 > g.nr ? g.nr : fi + 2;
+    jlo = fj - 1 < 0 ? 0 : fj - 1;
+    jhi = fj + 2 > g.nz ? g.nz : fj + 2;
 }
 
 // The range test of deposit_segment alone, so that a caller can decide

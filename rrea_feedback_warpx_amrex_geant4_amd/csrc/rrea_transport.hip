@@ -193,6 +193,15 @@ struct rrea_fields_s {
     int32_t* d_slow_list = nullptr;  // fast-path hand-over list
     int32_t* d_slow_count = nullptr;
     int64_t slow_cap = 0;
+    // tile ordering workspace
+    uint32_t* d_tkeys = nullptr;
+    uint32_t* d_tkeys_out = nullptr;
+    int32_t* d_tidx = nullptr;
+    int32_t* d_tidx_out = nullptr;
+    void* d_tsort_tmp = nullptr;
+    size_t tsort_tmp_bytes = 0;
+    void* d_tbuf = nullptr;
+    int64_t tsort_cap = 0;
     // newborn FIFO workspace
     void* d_nargs = nullptr;
     rrea_secondary* d_fin = nullptr;
@@ -1397,6 +1406,73 @@ __device__ __forceinline__ void reduce_tallies(double* tally, double* out)
     }
 }
 
+// The workgroup's net-chord deposits.  When the chords of the workgroup's
+// survivors lie close together (a cell-ordered tile), the face currents are
+// accumulated in an LDS window of the grid and flushed with one device atomic
+// per touched face; otherwise every lane deposits with device atomics.  Every
+// thread of the workgroup calls this.
+constexpr int kWindowFaces = 2048;
+
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) { int32_t const o = __shfl_xor(v, off, 64); v = o < v ? o : v; }
+    return v;
+}
+
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) { int32_t const o = __shfl_xor(v, off, 64); v = o > v ? o : v; }
+    return v;
+}
+
+__device__ void deposit_block(DevGrid const& g, bool dep, double q, double dt, double r0, double z0, double r1,
+                              double z1)
+{
+    __shared__ double jr_w[kWindowFaces];
+    __shared__ double jz_w[kWindowFaces];
+    __shared__ int32_t win[4];
+    int32_t ilo = INT32_MAX, ihi = INT32_MIN, jlo = INT32_MAX, jhi = INT32_MIN;
+    if (dep) {
+        int32_t a0, a1, b0, b1, c0, c1, d0, d1;
+        segment_face_range(g, r0, z0, a0, a1, b0, b1);
+        segment_face_range(g, r1, z1, c0, c1, d0, d1);
+        ilo = a0 < c0 ? a0 : c0;
+        ihi = a1 > c1 ? a1 : c1;
+        jlo = b0 < d0 ? b0 : d0;
+        jhi = b1 > d1 ? b1 : d1;
+    }
+    if (threadIdx.x == 0) { win[0] = INT32_MAX; win[1] = INT32_MIN; win[2] = INT32_MAX; win[3] = INT32_MIN; }
+    ilo = wave_min_i32(ilo);
+    ihi = wave_max_i32(ihi);
+    jlo = wave_min_i32(jlo);
+    jhi = wave_max_i32(jhi);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&win[0], ilo);
+        atomicMax(&win[1], ihi);
+        atomicMin(&win[2], jlo);
+        atomicMax(&win[3], jhi);
+    }
+    __syncthreads();
+    int32_t const i0 = win[0], i1 = win[1], j0 = win[2], j1 = win[3];
+    if (i0 > i1) return;  // no survivor in this workgroup
+    int32_t const wi = i1 - i0 + 1, nw = wi * (j1 - j0 + 1);
+    if (nw > kWindowFaces) {
+        if (dep) deposit_segment(g, q, dt, r0, z0, r1, z1);
+        return;
+    }
+    for (int c = threadIdx.x; c < nw; c += kBlock) { jr_w[c] = 0.0; jz_w[c] = 0.0; }
+    __syncthreads();
+    if (dep) deposit_segment_to(g, LdsWindow{jr_w, jz_w, i0, j0, wi}, q, dt, r0, z0, r1, z1);
+    __syncthreads();
+    for (int c = threadIdx.x; c < nw; c += kBlock) {
+        int32_t const i = i0 + c % wi, j = j0 + c / wi;
+        double const vr = jr_w[c], vz = jz_w[c];
+        if (vr != 0.0) atomic_add(&g.jr[i + (g.nr + 1) * j], vr);
+        if (vz != 0.0) atomic_add(&g.jz[i + g.nr * j], vz);
+    }
+}
+
 // The particles the fast path handed over, strided over a fixed grid.
 template <bool kPos>
 __global__ void __launch_bounds__(kBlock) charged_slow_kernel(const StepArgs* __restrict__ args,
@@ -1451,6 +1527,9 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     bool slow = false;
+    // the survivor's net chord, deposited by the whole workgroup at the end
+    bool dep = false;
+    double dep_q = 0.0, dep_r0 = 0.0, dep_z0 = 0.0, dep_r1 = 0.0, dep_z1 = 0.0;
     if (ip < A.n) {
         slow = true;  // cleared only when the whole fast branch completes
         DevTables const& T = A.T;
@@ -1602,7 +1681,9 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
             if (!deposit_segment_check(A.G, pr, pz, end_r, z)) break;
             // ---- commit: nothing above has written anything ----
             slow = false;
-            deposit_segment(A.G, S::kCharge * w, dt, pr, pz, end_r, z);
+            dep = true;
+            dep_q = S::kCharge * w;
+            dep_r0 = pr; dep_z0 = pz; dep_r1 = end_r; dep_z1 = z;
             double const um = proper_velocity(k2);
             A.ux[ip] = um * d1x; A.uy[ip] = um * d1y; A.uz[ip] = um * d1z;
             A.r[ip] = end_r;
@@ -1620,6 +1701,7 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
         } while (false);
     }
     append_slow(slow, ip, slow_list, slow_count);
+    deposit_block(A.G, dep, dep_q, A.cfg.dt_s, dep_r0, dep_z0, dep_r1, dep_z1);
     reduce_tallies(tally, A.tallies);
 }
 
@@ -2553,6 +2635,38 @@ __global__ void survivor_flag_kernel(const uint8_t* __restrict__ status, int64_t
     if (i < n) flag[i] = status[i] == RREA_SURVIVE ? 1 : 0;
 }
 
+// Tile ordering keys: cell of the captured position (r fastest), then a
+// kinetic-energy band within the cell; invalid particles last.
+__global__ void tile_key_kernel(int64_t n, const double* __restrict__ r, const double* __restrict__ z,
+                                const double* __restrict__ ux, const double* __restrict__ uy,
+                                const double* __restrict__ uz, const int64_t* __restrict__ id, int32_t photon,
+                                int32_t nr, int32_t nz, double r_lo, double z_lo, double dr, double dz, int32_t ebits,
+                                uint32_t* __restrict__ keys, int32_t* __restrict__ idx)
+{
+    int64_t const p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    idx[p] = static_cast<int32_t>(p);
+    if (id[p] == RREA_INVALID_ID) { keys[p] = 0xffffffffU; return; }
+    double const fr = (r[p] - r_lo) / dr, fz = (z[p] - z_lo) / dz;
+    int32_t i = finite(fr) ? static_cast<int32_t>(dmin(dmax(fr, 0.0), static_cast<double>(nr - 1))) : nr - 1;
+    int32_t j = finite(fz) ? static_cast<int32_t>(dmin(dmax(fz, 0.0), static_cast<double>(nz - 1))) : nz - 1;
+    double const u2 = ux[p] * ux[p] + uy[p] * uy[p] + uz[p] * uz[p];
+    double const e = photon ? sqrt(u2) : kinetic_from_u(ux[p], uy[p], uz[p]);
+    // four bands per octave above 1 keV
+    double const band = finite(e) && e > 1.0e3 ? 4.0 * log2(e * 1.0e-3) : 0.0;
+    uint32_t const emax = (1U << ebits) - 1U;
+    uint32_t const eb = ebits > 0 ? static_cast<uint32_t>(dmin(band, static_cast<double>(emax))) : 0U;
+    uint32_t const cell = static_cast<uint32_t>(i) + static_cast<uint32_t>(nr) * static_cast<uint32_t>(j);
+    keys[p] = (cell << ebits) | eb;
+}
+
+template <class V>
+__global__ void gather_kernel(int64_t n, const int32_t* __restrict__ idx, const V* __restrict__ src, V* __restrict__ dst)
+{
+    int64_t const p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[idx[p]];
+}
+
 __global__ void secondary_keys_kernel(const rrea_secondary* __restrict__ s, const int64_t* __restrict__ count,
                                       int64_t cap, uint64_t* __restrict__ keys, int64_t* __restrict__ idx)
 {
@@ -2808,7 +2922,8 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
     void* ptrs[] = {F->d_cell_er, F->d_cell_ez, F->d_btheta, F->d_stage, F->d_prof, F->d_stage_sec, F->d_keys,
                     F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args,
                     F->d_slow_list, F->d_slow_count, F->d_prof_guide, F->d_nargs, F->d_fin, F->d_fst,
-                    F->d_fion, F->d_fsub, F->d_anchor, F->d_next, F->d_flag, F->d_nsel, F->d_sel_tmp};
+                    F->d_fion, F->d_fsub, F->d_anchor, F->d_next, F->d_flag, F->d_nsel, F->d_sel_tmp,
+                    F->d_tkeys, F->d_tkeys_out, F->d_tidx, F->d_tidx_out, F->d_tsort_tmp, F->d_tbuf};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : F->events) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
     delete F;
@@ -3355,6 +3470,85 @@ rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step
     HIP_TRY(hipMemcpyAsync(io->work_count, &total, sizeof(int64_t), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(io->survivor_count, &nsurv, sizeof(int64_t), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return RREA_OK;
+}
+
+rrea_status rrea_tile_sort(rrea_fields_t F, const rrea_tile* tile, int32_t photon, rrea_stream_t stream)
+{
+    if (!F || !tile || tile->n < 0) return RREA_ERR_ARGUMENT;
+    int64_t const n = tile->n;
+    if (n == 0) return RREA_OK;
+    if (n > INT32_MAX || !tile->prev_r || !tile->prev_z || !tile->prev_theta || !tile->w || !tile->id
+        || !tile->r || !tile->z || !tile->theta || !tile->ux || !tile->uy || !tile->uz
+        || (!photon && (!tile->prev_ux || !tile->prev_uy || !tile->prev_uz))) {
+        return RREA_ERR_ARGUMENT;
+    }
+    rrea_rz_geometry const& g = F->geom;
+    uint64_t const cells = static_cast<uint64_t>(g.nr) * static_cast<uint64_t>(g.nz);
+    int32_t cbits = 0;
+    while ((uint64_t{1} << cbits) < cells) ++cbits;
+    if (cbits > 31) return RREA_ERR_ARGUMENT;
+    int32_t const ebits = 32 - cbits < 6 ? 32 - cbits : 6;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    if (F->tsort_cap < n) {
+        void* ptrs[] = {F->d_tkeys, F->d_tkeys_out, F->d_tidx, F->d_tidx_out, F->d_tsort_tmp, F->d_tbuf};
+        for (void* p : ptrs) if (p) (void)hipFree(p);
+        F->d_tkeys = nullptr; F->d_tkeys_out = nullptr; F->d_tidx = nullptr; F->d_tidx_out = nullptr;
+        F->d_tsort_tmp = nullptr; F->d_tbuf = nullptr;
+        F->tsort_cap = 0;
+        HIP_TRY(hipMalloc(&F->d_tkeys, n * sizeof(uint32_t)));
+        HIP_TRY(hipMalloc(&F->d_tkeys_out, n * sizeof(uint32_t)));
+        HIP_TRY(hipMalloc(&F->d_tidx, n * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&F->d_tidx_out, n * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&F->d_tbuf, n * sizeof(double)));
+        size_t bytes = 0;
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, F->d_tkeys, F->d_tkeys_out, F->d_tidx,
+                                                   F->d_tidx_out, static_cast<int>(n)));
+        HIP_TRY(hipMalloc(&F->d_tsort_tmp, bytes));
+        F->tsort_tmp_bytes = bytes;
+        F->tsort_cap = n;
+    }
+    // charged tiles key on the captured state; photons on their current point
+    const double* ku_x = photon ? tile->ux : tile->prev_ux;
+    const double* ku_y = photon ? tile->uy : tile->prev_uy;
+    const double* ku_z = photon ? tile->uz : tile->prev_uz;
+    unsigned const nb = static_cast<unsigned>(blocks_for(n, 256));
+    tile_key_kernel<<<dim3(nb), dim3(256), 0, s>>>(n, tile->prev_r, tile->prev_z, ku_x, ku_y, ku_z, tile->id,
+                                                   photon ? 1 : 0, g.nr, g.nz, g.r_lo, g.z_lo, g.dr, g.dz, ebits,
+                                                   F->d_tkeys, F->d_tidx);
+    HIP_TRY(hipGetLastError());
+    size_t bytes = F->tsort_tmp_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(F->d_tsort_tmp, bytes, F->d_tkeys, F->d_tkeys_out, F->d_tidx,
+                                               F->d_tidx_out, static_cast<int>(n), 0, cbits + ebits, s));
+    // permute every distinct array of the tile through the staging buffer
+    std::vector<double*> f64 = {const_cast<double*>(tile->prev_r), const_cast<double*>(tile->prev_z),
+                                const_cast<double*>(tile->prev_theta), const_cast<double*>(tile->w)};
+    if (!photon) {
+        f64.push_back(const_cast<double*>(tile->prev_ux));
+        f64.push_back(const_cast<double*>(tile->prev_uy));
+        f64.push_back(const_cast<double*>(tile->prev_uz));
+    }
+    for (double* p : {tile->r, tile->z, tile->theta, tile->ux, tile->uy, tile->uz}) {
+        if (std::find(f64.begin(), f64.end(), p) == f64.end()) f64.push_back(p);
+    }
+    double* const buf = static_cast<double*>(F->d_tbuf);
+    for (double* p : f64) {
+        gather_kernel<double><<<dim3(nb), dim3(256), 0, s>>>(n, F->d_tidx_out, p, buf);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(p, buf, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    {
+        int64_t* const ib = static_cast<int64_t*>(F->d_tbuf);
+        gather_kernel<int64_t><<<dim3(nb), dim3(256), 0, s>>>(n, F->d_tidx_out, tile->id, ib);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(tile->id, ib, n * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    }
+    if (tile->cpu) {
+        int32_t* const cb = static_cast<int32_t*>(F->d_tbuf);
+        gather_kernel<int32_t><<<dim3(nb), dim3(256), 0, s>>>(n, F->d_tidx_out, tile->cpu, cb);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(const_cast<int32_t*>(tile->cpu), cb, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
     return RREA_OK;
 }
 

@@ -205,6 +205,18 @@ class DeviceTile:
         return DeviceTile(*(getattr(self, k).clone() for k in
                             ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")))
 
+    def struct(self) -> _lib.Tile:
+        """The fused in-place tile (r..uz are both the captured state and the writeback)."""
+        p = [getattr(self, k).data_ptr() for k in ("r", "z", "theta", "ux", "uy", "uz")]
+        return _lib.Tile(len(self), *p, self.w.data_ptr(), self.id.data_ptr(), self.cpu.data_ptr(), *p)
+
+    def sort_by_cell(self, coupling: "Coupling", stream: int | None = None) -> None:
+        """Reorder the tile by cell and energy band (rrea_tile_sort), in place."""
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        t = self.struct()
+        _lib.check(_lib.lib().rrea_tile_sort(coupling.h, C.byref(t), 0, stream), "rrea_tile_sort")
+
 
 @dataclass
 class PhotonTile:
@@ -234,6 +246,13 @@ class PhotonTile:
         return _lib.Tile(len(c), self.prev_r.data_ptr(), self.prev_z.data_ptr(), self.prev_theta.data_ptr(), None,
                          None, None, c.w.data_ptr(), c.id.data_ptr(), c.cpu.data_ptr(), c.r.data_ptr(), c.z.data_ptr(),
                          c.theta.data_ptr(), c.ux.data_ptr(), c.uy.data_ptr(), c.uz.data_ptr())
+
+    def sort_by_cell(self, coupling: "Coupling", stream: int | None = None) -> None:
+        """Reorder the photon tile by cell and energy (rrea_tile_sort), in place."""
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        t = self.struct()
+        _lib.check(_lib.lib().rrea_tile_sort(coupling.h, C.byref(t), 1, stream), "rrea_tile_sort")
 
     def push(self, dt_s: float, stream: int | None = None) -> None:
         """WarpX's photon push (rrea_photon_push): prev <- current, current += c dt u/|u|."""
