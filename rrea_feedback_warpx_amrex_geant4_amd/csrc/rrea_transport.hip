@@ -190,6 +190,7 @@ struct rrea_fields_s {
     size_t sort_tmp_bytes = 0;
     int64_t* d_count = nullptr;
     void* d_args = nullptr;      // device copy of the kernel's argument block
+    int64_t* d_base = nullptr;       // the caller's secondary count at the start of a call
     int32_t* d_slow_list = nullptr;  // fast-path hand-over list
     int32_t* d_slow_count = nullptr;
     int64_t slow_cap = 0;
@@ -2537,7 +2538,10 @@ __global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __re
     double tally[kTallyNewborn];
     for (int q = 0; q < kTallyNewborn; ++q) tally[q] = 0.0;
     int64_t const i = N.gb + static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (i < N.ge) {
+    // an error recorded before the launch (the step's, or a rejected birth
+    // payload) stops the drain: nothing is advanced
+    bool const stopped = __hip_atomic_load(N.S.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    if (i < N.ge && !stopped) {
         rrea_secondary const& p = N.work[i];
         bool const photon = p.species == 2;
         // transient_energy_is_certified OV/RreaSecondaryCommit.cpp:148-180
@@ -2687,6 +2691,34 @@ __global__ void secondary_gather_kernel(const rrea_secondary* __restrict__ in, c
     int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     int64_t const n = *count < cap ? *count : cap;
     if (i < n) out[i] = in[idx[i]];
+}
+
+// The ordered records appended behind the caller's count (read on the
+// device); records beyond the caller's capacity are dropped and reported.
+__global__ void secondary_append_kernel(const rrea_secondary* __restrict__ in, const int64_t* __restrict__ idx,
+                                        int64_t n, const int64_t* __restrict__ base, int64_t cap,
+                                        rrea_secondary* __restrict__ out, int32_t* __restrict__ error)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t const at = *base + i;
+    if (at < cap) out[at] = in[idx[i]];
+    else if (error) report(error, kErrCapacity, -1);
+}
+
+__global__ void append_count_kernel(int64_t* __restrict__ count, const int64_t* __restrict__ base, int64_t n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *count = *base + n;
+}
+
+__global__ void set_i64_kernel(int64_t* __restrict__ p, int64_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+__global__ void report_kernel(int32_t* __restrict__ error, int32_t code, int64_t index)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) report(error, code, index);
 }
 
 int64_t blocks_for(int64_t n, int b) { return (n + b - 1) / b; }
@@ -2923,7 +2955,7 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
                     F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args,
                     F->d_slow_list, F->d_slow_count, F->d_prof_guide, F->d_nargs, F->d_fin, F->d_fst,
                     F->d_fion, F->d_fsub, F->d_anchor, F->d_next, F->d_flag, F->d_nsel, F->d_sel_tmp,
-                    F->d_tkeys, F->d_tkeys_out, F->d_tidx, F->d_tidx_out, F->d_tsort_tmp, F->d_tbuf};
+                    F->d_tkeys, F->d_tkeys_out, F->d_tidx, F->d_tidx_out, F->d_tsort_tmp, F->d_tbuf, F->d_base};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : F->events) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
     delete F;
@@ -3060,7 +3092,8 @@ rrea_status order_workspace(rrea_fields_t F, int64_t cap)
 
 // Sorts the n records staged in F->d_stage_sec (count at F->d_count) by
 // (parent index, lineage) into `dst`.
-rrea_status order_staged(rrea_fields_t F, int64_t n, rrea_secondary* dst, hipStream_t s)
+rrea_status order_staged(rrea_fields_t F, int64_t n, rrea_secondary* dst, hipStream_t s,
+                         const int64_t* base = nullptr, int64_t cap = 0, int32_t* error = nullptr)
 {
     if (n <= 0) return RREA_OK;
     secondary_keys_kernel<<<dim3(static_cast<unsigned>(blocks_for(n, 256))), dim3(256), 0, s>>>(
@@ -3069,8 +3102,13 @@ rrea_status order_staged(rrea_fields_t F, int64_t n, rrea_secondary* dst, hipStr
     size_t bytes = F->sort_tmp_bytes;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(F->d_sort_tmp, bytes, F->d_keys, F->d_keys_out, F->d_idx, F->d_idx_out,
                                                static_cast<int>(n), 0, 64, s));
-    secondary_gather_kernel<<<dim3(static_cast<unsigned>(blocks_for(n, 256))), dim3(256), 0, s>>>(
-        F->d_stage_sec, F->d_idx_out, F->d_count, n, dst);
+    if (base) {
+        secondary_append_kernel<<<dim3(static_cast<unsigned>(blocks_for(n, 256))), dim3(256), 0, s>>>(
+            F->d_stage_sec, F->d_idx_out, n, base, cap, dst, error);
+    } else {
+        secondary_gather_kernel<<<dim3(static_cast<unsigned>(blocks_for(n, 256))), dim3(256), 0, s>>>(
+            F->d_stage_sec, F->d_idx_out, F->d_count, n, dst);
+    }
     HIP_TRY(hipGetLastError());
     return RREA_OK;
 }
@@ -3183,12 +3221,16 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     bool const order = out->order_secondaries != 0 && out->secondary_capacity > 0;
     int64_t const cap = out->secondary_capacity;
     if (order) {
-        // stage into the library's workspace, then sort into the caller's buffer
+        // stage into the library's workspace (count from 0), then sort into the
+        // caller's buffer behind its current count, which stays on the device
         {
             rrea_status const ws = order_workspace(F, cap);
             if (ws != RREA_OK) return ws;
         }
-        HIP_TRY(hipMemcpyAsync(F->d_count, out->secondary_count, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+        if (!F->d_base) HIP_TRY(hipMalloc(&F->d_base, sizeof(int64_t)));
+        HIP_TRY(hipMemcpyAsync(F->d_base, out->secondary_count, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(F->d_count), 1, 0ULL);
+        HIP_TRY(hipGetLastError());
         A.sec = F->d_stage_sec;
         A.sec_count = F->d_count;
     } else {
@@ -3196,16 +3238,6 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
         A.sec_count = out->secondary_count;
     }
     A.sec_cap = cap;
-    // The staged region starts at the caller's current count.
-    int64_t base = 0;
-    if (order) {
-        HIP_TRY(hipMemcpyAsync(&base, out->secondary_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        int64_t const zero = 0;
-        HIP_TRY(hipMemcpyAsync(F->d_count, &zero, sizeof(int64_t), hipMemcpyHostToDevice, s));
-        A.sec_cap = cap - base;
-        if (A.sec_cap < 0) A.sec_cap = 0;
-    }
 
     int64_t const nb = blocks_for(tile->n, kBlock);
     // pageable source: the copy has consumed A when hipMemcpyAsync returns
@@ -3280,18 +3312,17 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     if (st != RREA_OK) return st;
 
     if (order) {
+        // the one host round trip of the call: the sort's item count
         int64_t nstaged = 0;
         HIP_TRY(hipMemcpyAsync(&nstaged, F->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        int64_t const room = A.sec_cap;
-        int64_t const nsort = nstaged < room ? nstaged : room;
+        int64_t const nsort = nstaged < cap ? nstaged : cap;
         {
-            rrea_status const os = order_staged(F, nsort, out->secondaries + base, s);
+            rrea_status const os = order_staged(F, nsort, out->secondaries, s, F->d_base, cap, out->error);
             if (os != RREA_OK) return os;
         }
-        int64_t const total = base + nstaged;
-        HIP_TRY(hipMemcpyAsync(out->secondary_count, &total, sizeof(int64_t), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        append_count_kernel<<<dim3(1), dim3(64), 0, s>>>(out->secondary_count, F->d_base, nstaged);
+        HIP_TRY(hipGetLastError());
     }
     return RREA_OK;
 }
@@ -3334,12 +3365,11 @@ rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     if (dev != T->device) return RREA_ERR_ARGUMENT;
-    static int64_t const kZero = 0;
     // RreaParticleInteraction::Apply :38-40 -- a non-positive dt does nothing.
     if (!(cfg->dt_s > 0.0) || io->n_initial == 0) {
-        HIP_TRY(hipMemcpyAsync(io->work_count, &kZero, sizeof(int64_t), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(io->survivor_count, &kZero, sizeof(int64_t), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->work_count, 0);
+        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->survivor_count, 0);
+        HIP_TRY(hipGetLastError());
         return RREA_OK;
     }
     if (!(cfg->max_optical_depth_per_substep > 0.0) || cfg->max_subcycles == 0 || cfg->max_subcycles > 16383) {
@@ -3385,12 +3415,11 @@ rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step
     fifo_init_kernel<<<dim3(static_cast<unsigned>(blocks_for(n0, 256))), dim3(256), 0, s>>>(
         io->work, n0, F->d_anchor, F->d_next, io->error);
     HIP_TRY(hipGetLastError());
-    int32_t err_host[2] = {0, 0};
-    HIP_TRY(hipMemcpyAsync(err_host, io->error, sizeof(err_host), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
     // One launch per generation; the host reads the child count in between.
-    // A rejected birth payload advances nothing.
-    int64_t gb = 0, ge = err_host[0] != 0 ? 0 : n0;
+    // The generation kernel advances nothing once an error is recorded (a
+    // rejected birth payload included).
+    int32_t err_host[2] = {0, 0};
+    int64_t gb = 0, ge = n0;
     while (gb < ge) {
         fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(F->d_count), 1, 0ULL);
         HIP_TRY(hipGetLastError());
@@ -3440,36 +3469,46 @@ rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step
         ge += c;
     }
     int64_t const total = ge;
-    int64_t nsurv = 0;
-    if (err_host[0] == 0) {
-        survivor_flag_kernel<<<dim3(static_cast<unsigned>(blocks_for(total, 256))), dim3(256), 0, s>>>(F->d_fst, total,
-                                                                                                      F->d_flag);
+    set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->work_count, total);
+    HIP_TRY(hipGetLastError());
+    if (err_host[0] != 0) {
+        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->survivor_count, 0);
         HIP_TRY(hipGetLastError());
-        size_t bytes = F->sel_tmp_bytes;
+        return RREA_OK;
+    }
+    survivor_flag_kernel<<<dim3(static_cast<unsigned>(blocks_for(total, 256))), dim3(256), 0, s>>>(F->d_fst, total,
+                                                                                                  F->d_flag);
+    HIP_TRY(hipGetLastError());
+    size_t bytes = F->sel_tmp_bytes;
+    if (io->survivor_capacity >= total) {
+        // every entry fits: select straight into the caller's buffer, no round trip
+        HIP_TRY(hipcub::DeviceSelect::Flagged(F->d_sel_tmp, bytes, F->d_fin, F->d_flag, io->survivors,
+                                              io->survivor_count, static_cast<int>(total), s));
+    } else {
+        int64_t nsurv = 0;
         HIP_TRY(hipcub::DeviceSelect::Flagged(F->d_sel_tmp, bytes, F->d_fin, F->d_flag, F->d_stage_sec, F->d_nsel,
                                               static_cast<int>(total), s));
         HIP_TRY(hipMemcpyAsync(&nsurv, F->d_nsel, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (nsurv > io->survivor_capacity) {
-            int32_t const e2[2] = {kErrCapacity, -1};
-            HIP_TRY(hipMemcpyAsync(io->error, e2, sizeof(e2), hipMemcpyHostToDevice, s));
+            report_kernel<<<dim3(1), dim3(64), 0, s>>>(io->error, kErrCapacity, -1);
+            HIP_TRY(hipGetLastError());
             nsurv = io->survivor_capacity;
         }
         if (nsurv > 0) {
             HIP_TRY(hipMemcpyAsync(io->survivors, F->d_stage_sec, nsurv * sizeof(rrea_secondary),
                                    hipMemcpyDeviceToDevice, s));
         }
-        if (io->status) HIP_TRY(hipMemcpyAsync(io->status, F->d_fst, total, hipMemcpyDeviceToDevice, s));
-        if (io->ion_pairs) {
-            HIP_TRY(hipMemcpyAsync(io->ion_pairs, F->d_fion, total * sizeof(double), hipMemcpyDeviceToDevice, s));
-        }
-        if (io->substeps) {
-            HIP_TRY(hipMemcpyAsync(io->substeps, F->d_fsub, total * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
-        }
+        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->survivor_count, nsurv);
+        HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipMemcpyAsync(io->work_count, &total, sizeof(int64_t), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(io->survivor_count, &nsurv, sizeof(int64_t), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    if (io->status) HIP_TRY(hipMemcpyAsync(io->status, F->d_fst, total, hipMemcpyDeviceToDevice, s));
+    if (io->ion_pairs) {
+        HIP_TRY(hipMemcpyAsync(io->ion_pairs, F->d_fion, total * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    if (io->substeps) {
+        HIP_TRY(hipMemcpyAsync(io->substeps, F->d_fsub, total * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
     return RREA_OK;
 }
 

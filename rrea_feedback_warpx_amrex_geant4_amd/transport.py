@@ -342,8 +342,9 @@ class DrainOutputs:
                 self.work[: n * size].copy_(torch.from_numpy(raw.copy()))
             return n
         at = 0
-        for out in staged:
-            n = min(int(out.secondary_count.item()), out.secondary_capacity)
+        counts = torch.cat([out.secondary_count for out in staged]).cpu().tolist()  # one round trip
+        for out, count in zip(staged, counts):
+            n = min(int(count), out.secondary_capacity)
             if at + n > self.capacity:
                 raise ValueError("the step's secondaries exceed the FIFO capacity")
             if n:

@@ -199,3 +199,27 @@ def test_tile_sort(tables):
     for k in ("r", "z", "theta", "ux", "uy", "uz", "w", "cpu"):
         assert np.array_equal(getattr(host, k)[order_h], got[k][order_g]), k
     assert np.array_equal(np.sort(host.id[host.id != np.iinfo(np.int64).min]), np.sort(got["id"][valid]))
+
+
+def test_secondaries_append_behind_count_and_capacity(tables):
+    """Two steps into one output append their ordered secondaries behind the
+    count the caller holds; a capacity too small for the second is reported."""
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile, RreaParticleInteraction, StepOutputs
+    fields = wl.c2_fields()
+    cp = coupling_for(fields, tables)
+    host = wl.c2_electrons(1 << 12, fields, seed=35)
+    cfg = StepConfig(rng_seed=36, dt_s=1e-9)
+    one = StepOutputs(len(host), fields, secondary_capacity=1 << 12)
+    RreaParticleInteraction.apply_electrons(tables, cp, DeviceTile.from_host(host), cfg, one)
+    first = one.secondaries_host().copy()
+    n1 = len(first)
+    assert n1 > 4
+    both = StepOutputs(len(host), fields, secondary_capacity=1 << 12)
+    RreaParticleInteraction.apply_electrons(tables, cp, DeviceTile.from_host(host), cfg, both)
+    RreaParticleInteraction.apply_electrons(tables, cp, DeviceTile.from_host(host), cfg, both)
+    s = both.secondaries_host()
+    assert len(s) == 2 * n1 and (s[:n1] == first).all() and (s[n1:] == first).all()
+    small = StepOutputs(len(host), fields, secondary_capacity=n1 + 2)
+    RreaParticleInteraction.apply_electrons(tables, cp, DeviceTile.from_host(host), cfg, small)
+    RreaParticleInteraction.apply_electrons(tables, cp, DeviceTile.from_host(host), cfg, small, check=False)
+    assert small.error.cpu().tolist()[0] == 5
