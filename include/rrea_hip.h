@@ -215,6 +215,8 @@ typedef struct rrea_secondary {
                                the root's next free lineage for a newborn's child */
     int32_t root_species;   /* species of the tracked root: 0 e-, 1 e+, 2 photon */
     int32_t generation;     /* 0: staged by a live step; g: made by a generation-g newborn */
+    int32_t instance;       /* the simulation instance of a batched step; -1 for a plain step */
+    int32_t reserved;
 } rrea_secondary;
 
 enum {
@@ -340,6 +342,33 @@ typedef struct rrea_drain_io {
 
 rrea_status rrea_newborn_drain(rrea_tables_t tables, rrea_fields_t fields, const rrea_step_config* config,
                                const rrea_drain_io* io, rrea_stream_t stream);
+
+/* Batched transport: independent simulation instances stepped together, one
+ * launch per species over the particles of every instance (no counterpart in
+ * the overlay, which runs one simulation per process).  Each instance keeps
+ * its own fields handle (geometry, fields, density), its own padded J grid
+ * (jr, jz, rho device pointers, as in rrea_step_out) and its own RNG seed; a
+ * batched tile holds the particles of all instances with an instance index
+ * per particle (device int32).  An instance's particle-steps are those of a
+ * plain step of that instance alone with rng_seed = its seed, except for the
+ * summation order of its J grid, the tallies (summed over the batch) and the
+ * parent indices of the staged secondaries (indices into the batched tile;
+ * each record carries its instance).  The batch's workspace handle
+ * (rrea_batch_workspace) is where kernel timing is enabled and read. */
+typedef struct rrea_batch_s* rrea_batch_t;
+rrea_status rrea_batch_create(rrea_fields_t const* fields, int32_t n, double* const* jr, double* const* jz,
+                              double* const* rho, const uint64_t* rng_seeds, rrea_batch_t* out);
+rrea_status rrea_batch_destroy(rrea_batch_t batch);
+rrea_status rrea_batch_workspace(rrea_batch_t batch, rrea_fields_t* out);
+/* species: 0 electrons, 1 positrons, 2 photons; out->jr/jz/rho are not used. */
+rrea_status rrea_transport_step_batched(rrea_tables_t tables, rrea_batch_t batch, int32_t species,
+                                        const rrea_tile* tile, const int32_t* instance,
+                                        const rrea_step_config* config, const rrea_step_out* out,
+                                        rrea_stream_t stream);
+/* The newborn drain of a batched step: each FIFO record advances in its own
+ * instance (rrea_secondary.instance); io->jr/jz/rho are not used. */
+rrea_status rrea_newborn_drain_batched(rrea_tables_t tables, rrea_batch_t batch, const rrea_step_config* config,
+                                       const rrea_drain_io* io, rrea_stream_t stream);
 
 /* Kernel timing with HIP events recorded around every transport-kernel
  * launch made through `fields` while enabled (no counterpart in the reference:

@@ -47,7 +47,7 @@ class Secondary(C.Structure):
         ("dir_x", C.c_double), ("dir_y", C.c_double), ("dir_z", C.c_double),
         ("weight", C.c_double), ("energy_eV", C.c_double), ("birth_fraction", C.c_double),
         ("root_key", C.c_uint64), ("lineage", C.c_uint64),
-        ("root_species", C.c_int32), ("generation", C.c_int32),
+        ("root_species", C.c_int32), ("generation", C.c_int32), ("instance", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -57,6 +57,7 @@ SECONDARY_DTYPE = np.dtype([
     ("dir_x", "<f8"), ("dir_y", "<f8"), ("dir_z", "<f8"),
     ("weight", "<f8"), ("energy_eV", "<f8"), ("birth_fraction", "<f8"),
     ("root_key", "<u8"), ("lineage", "<u8"), ("root_species", "<i4"), ("generation", "<i4"),
+    ("instance", "<i4"), ("reserved", "<i4"),
 ])
 assert SECONDARY_DTYPE.itemsize == C.sizeof(Secondary)
 

@@ -1507,7 +1507,7 @@ struct Emitter {
                    double energy, double birth)
     {
         orc_secondary s{ip, sp, process, x, y, z, sdx, sdy, sdz, w, energy, birth, pkey, lineage++,
-                        root_species, generation};
+                        root_species, generation, -1, 0};
         fx.secondaries.push_back(s);
     }
 };

@@ -69,6 +69,8 @@ typedef struct orc_secondary {
     uint64_t lineage;
     int32_t root_species; /* species of the tracked root particle: 0 e-, 1 e+, 2 photon */
     int32_t generation;   /* 0: staged by a live step; g: created by a generation-g newborn */
+    int32_t instance;     /* -1: one simulation (the oracle runs instances one at a time) */
+    int32_t reserved;
 } orc_secondary;
 
 /* Per-particle status written by the step. */

@@ -95,7 +95,7 @@ class Secondary(C.Structure):
         ("dir_x", C.c_double), ("dir_y", C.c_double), ("dir_z", C.c_double),
         ("weight", C.c_double), ("energy_eV", C.c_double), ("birth_fraction", C.c_double),
         ("root_key", C.c_uint64), ("lineage", C.c_uint64),
-        ("root_species", C.c_int32), ("generation", C.c_int32),
+        ("root_species", C.c_int32), ("generation", C.c_int32), ("instance", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -143,6 +143,15 @@ SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
     ("rrea_photon_push", C.c_int32, [C.POINTER(Tile), C.c_double, C.c_void_p]),
     ("rrea_tile_sort", C.c_int32, [C.c_void_p, C.POINTER(Tile), C.c_int32, C.c_void_p]),
+    ("rrea_batch_create", C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.POINTER(C.c_void_p)]),
+    ("rrea_batch_destroy", C.c_int32, [C.c_void_p]),
+    ("rrea_batch_workspace", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("rrea_transport_step_batched", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(Tile), C.c_void_p, C.POINTER(StepConfigC), C.POINTER(StepOut),
+      C.c_void_p]),
+    ("rrea_newborn_drain_batched", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.POINTER(StepConfigC), C.POINTER(DrainIO), C.c_void_p]),
     ("rrea_newborn_drain", C.c_int32,
      [C.c_void_p, C.c_void_p, C.POINTER(StepConfigC), C.POINTER(DrainIO), C.c_void_p]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),

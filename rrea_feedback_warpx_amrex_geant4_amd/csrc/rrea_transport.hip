@@ -222,6 +222,17 @@ struct rrea_fields_s {
     size_t events_used = 0;
 };
 
+// A batch of independent simulation instances (one fields handle, J grid and
+// RNG seed each) stepped together.
+struct rrea_batch_s {
+    std::vector<rrea_fields_t> fields;
+    std::vector<double*> jr, jz, rho;
+    std::vector<uint64_t> seeds;
+    rrea_fields_t work = nullptr;   // workspace of the batched calls (kernel arguments, lists, timing)
+    void* d_envs = nullptr;         // one environment per instance
+    std::vector<unsigned char> h_envs;
+};
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -261,13 +272,25 @@ __device__ __forceinline__ double wave_sum(double v)
 // ---------------------------------------------------------------------------
 // The electron step kernel
 // ---------------------------------------------------------------------------
-struct StepArgs {
-    DevTables T;
+// One simulation's environment: its fields, domain, padded J grid, the
+// reference-z density fallback and its RNG seed.  A plain step has one; a
+// batched step one per instance, selected by each particle's instance id.
+struct Env {
     DevFields F;
     Domain D;
     DevGrid G;
-    rrea_step_config cfg;
     double density_ratio;     // reference-z fallback ratio
+    int32_t pnr, pzb, pnz;
+    int32_t instance;         // -1 for a plain step
+    uint64_t rng_seed;
+};
+
+struct StepArgs {
+    DevTables T;
+    Env env;                  // the plain step's environment
+    const Env* batch;         // batched step: one environment per instance
+    const int32_t* inst;      // batched step: each particle's instance
+    rrea_step_config cfg;
     int64_t n;
     const double* prev_r; const double* prev_z; const double* prev_th;
     const double* prev_ux; const double* prev_uy; const double* prev_uz;
@@ -279,8 +302,18 @@ struct StepArgs {
     rrea_secondary* sec; int64_t sec_cap; int64_t* sec_count;
     double* tallies;
     int32_t* error;
-    int32_t pnr, pzb, pnz;
 };
+
+__device__ __forceinline__ Env const& env_of(StepArgs const& A, int64_t ip)
+{
+    return A.inst ? A.batch[A.inst[ip]] : A.env;
+}
+
+// A FIFO entry's environment: its instance's in a batched drain
+__device__ __forceinline__ Env const& env_of_record(StepArgs const& A, int32_t instance)
+{
+    return (A.batch && instance >= 0) ? A.batch[instance] : A.env;
+}
 
 __device__ __forceinline__ void report(int32_t* error, int32_t code, int64_t ip)
 {
@@ -308,6 +341,7 @@ struct Emitter {
     int32_t& err;
     int32_t root_species;   // species of the tracked root of every record made here
     int32_t generation;     // FIFO generation of the records made here
+    int32_t instance;       // the simulation instance (-1: a plain step)
 
     __device__ void deposit_local(double energy)
     {
@@ -330,6 +364,7 @@ struct Emitter {
         s.weight = w; s.energy_eV = energy; s.birth_fraction = birth;
         s.root_key = pkey; s.lineage = lineage++;
         s.root_species = root_species; s.generation = generation;
+        s.instance = instance; s.reserved = 0;
         push_secondary(A, s, err);
     }
 };
@@ -556,8 +591,9 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
     int32_t err = kOk;
     {
         DevTables const& T = A.T;
-        DevFields const& F = A.F;
-        Domain const& D = A.D;
+        Env const& E = env_of(A, ip);
+        DevFields const& F = E.F;
+        Domain const& D = E.D;
         rrea_step_config const& cfg = A.cfg;
         double const cutoff = cfg.low_energy_cutoff_eV;
         double const gcut = cfg.photon_cutoff_eV;
@@ -595,7 +631,7 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
         } else if (pr > D.r_hi + 1.0e-9 * D.r_hi) {
             err = kErrDomain;
         } else {
-            uint64_t const rng = rng_prefix(cfg.rng_seed, pkey, step);
+            uint64_t const rng = rng_prefix(E.rng_seed, pkey, step);
             double dx = pux, dy = puy, dz = puz;
             double k = kinetic_from_u(dx, dy, dz);
             normalize(dx, dy, dz);
@@ -611,17 +647,17 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
             // whose zero domain puts every endpoint outside: it escapes.
             bool escapes = driver_valid ? clipped_escapes(path, D) : true;
             double const ds = driver_valid ? path.in_len : 0.0;
-            Interval iv = build_interval(F, path, D, A.density_ratio, err);
+            Interval iv = build_interval(F, path, D, E.density_ratio, err);
             double const span = dmax(path.end_f, 1.0e-6);
             double const field_work = path_field_work(F, path, D, S::kCharge, 0.0, 1.0, err).work / span;
             double const chord_dp = density_path(iv, path, 0.0, path.end_f);
             double particle_ratio = 0.0;
             if constexpr (!kPos) {
-                particle_ratio = ds > 0.0 ? chord_dp / ds : positive_or(density_ratio_at_z(F, z_particle, err), A.density_ratio);
+                particle_ratio = ds > 0.0 ? chord_dp / ds : positive_or(density_ratio_at_z(F, z_particle, err), E.density_ratio);
             }
             double tr_dp = 0.0, tr_len = 0.0;
 
-            Emitter em{A, ip, pkey, w, ion, lineage, err, kPos ? 1 : 0, 0};
+            Emitter em{A, ip, pkey, w, ion, lineage, err, kPos ? 1 : 0, 0, E.instance};
             auto deposit_local = [&](double energy) { em.deposit_local(energy); };
             auto push = [&](int32_t species, int32_t process, double x, double y, double z, double sdx, double sdy,
                             double sdz, double energy, double birth) {
@@ -806,7 +842,7 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                             path = next;
                         }
                         if (!(path.full > 0.0) || !(path.in_len > 0.0)) { exited = path.exits; break; }
-                        iv = build_interval(F, path, D, A.density_ratio, err);
+                        iv = build_interval(F, path, D, E.density_ratio, err);
                         if (!iv.valid) { if (!err) err = kErrInvalid; break; }
                         tr_dp += iv.length * iv.ratio;
                         tr_len += iv.length;
@@ -1072,7 +1108,7 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                         o_th = atan2(run_y, run_x);
                         o_z = run_z;
                         terminal = kSurvive;
-                        if (w != 0.0 && !deposit_segment(A.G, S::kCharge * w, dt, r_particle, z_particle, end_r, run_z)) {
+                        if (w != 0.0 && !deposit_segment(E.G, S::kCharge * w, dt, r_particle, z_particle, end_r, run_z)) {
                             err = kErrInvalid;
                         }
                     }
@@ -1087,9 +1123,9 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                     double const q = S::kCharge * w;
                     if (!bc.valid) {
                         err = kErrInvalid;
-                    } else if (q != 0.0 && !deposit_escaping_segment(A.G, q, dt, r_particle, z_particle, bc.start_r,
-                                                                     bc.start_z, bc.cross_r, bc.cross_z, A.pnr, A.pzb,
-                                                                     A.pnz, bc.faces)) {
+                    } else if (q != 0.0 && !deposit_escaping_segment(E.G, q, dt, r_particle, z_particle, bc.start_r,
+                                                                     bc.start_z, bc.cross_r, bc.cross_z, E.pnr, E.pzb,
+                                                                     E.pnz, bc.faces)) {
                         err = kErrInvalid;
                     }
                     status = RREA_ESCAPE;
@@ -1099,12 +1135,12 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                     // (AccumulateLowEnergyElectron, OV/RreaElectronTransport.cpp:152, 857)
                     tally[16] += w;
                     deposit_local(k);
-                    if (w != 0.0 && !deposit_segment(A.G, -w, dt, r_particle, z_particle, demote_r, demote_z)) err = kErrInvalid;
+                    if (w != 0.0 && !deposit_segment(E.G, -w, dt, r_particle, z_particle, demote_r, demote_z)) err = kErrInvalid;
                     status = RREA_DEMOTED;
                     tally[2] += 1.0;
                 } else if (terminal == kAnnihilated) {
                     // net chord from the captured start to the annihilation point
-                    if (w != 0.0 && !deposit_segment(A.G, w, dt, r_particle, z_particle, demote_r, demote_z)) err = kErrInvalid;
+                    if (w != 0.0 && !deposit_segment(E.G, w, dt, r_particle, z_particle, demote_r, demote_z)) err = kErrInvalid;
                     status = RREA_ANNIHILATED;
                 } else if (terminal == kSurvive) {
                     status = RREA_SURVIVE;
@@ -1239,8 +1275,9 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
 {
     int32_t err = kOk;
     DevTables const& T = A.T;
-    DevFields const& F = A.F;
-    Domain const& D = A.D;
+    Env const& E = env_of(A, ip);
+    DevFields const& F = E.F;
+    Domain const& D = E.D;
     rrea_step_config const& cfg = A.cfg;
     double const cutoff = cfg.low_energy_cutoff_eV;
     double const gcut = cfg.photon_cutoff_eV;
@@ -1261,7 +1298,7 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
     double const w = A.w[ip];
     int32_t const cpu = A.cpu ? A.cpu[ip] : 0;
     uint64_t const pkey = static_cast<uint64_t>(id < 0 ? -id : id) ^ (static_cast<uint64_t>(static_cast<uint32_t>(cpu)) << 40);
-    uint64_t const rng = rng_prefix(cfg.rng_seed, pkey, step);
+    uint64_t const rng = rng_prefix(E.rng_seed, pkey, step);
 
     double o_r = cr, o_th = cth, o_z = cz, o_ux = ux, o_uy = uy, o_uz = uz;
     bool write_pos = false, write_u = false;
@@ -1269,7 +1306,7 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
     double ion = 0.0;
     uint64_t lineage = 0;
     uint64_t events = 0;
-    Emitter em{A, ip, pkey, w, ion, lineage, err, 2, 0};
+    Emitter em{A, ip, pkey, w, ion, lineage, err, 2, 0, E.instance};
 
     double energy = sqrt(ux * ux + uy * uy + uz * uz);
     double dx = ux, dy = uy, dz = uz;
@@ -1285,7 +1322,7 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
         if (ordinal >= kEventCap) { err = kErrInvalid; break; }
         bool const escapes = clipped_escapes(path, D);
         double const ds = path.in_len;
-        Interval const iv = build_interval(F, path, D, A.density_ratio, err);
+        Interval const iv = build_interval(F, path, D, E.density_ratio, err);
         double const chord_dp = density_path(iv, path, 0.0, path.end_f);
         if (err) break;
         if (w <= 0.0 || energy < gcut) {
@@ -1426,13 +1463,16 @@ __device__ __forceinline__ int32_t wave_max_i32(int32_t v)
     return v;
 }
 
-__device__ void deposit_block(DevGrid const& g, bool dep, double q, double dt, double r0, double z0, double r1,
-                              double z1)
+__device__ void deposit_block(StepArgs const& A, bool dep, int32_t instance, double q, double dt, double r0,
+                              double z0, double r1, double z1)
 {
     __shared__ double jr_w[kWindowFaces];
     __shared__ double jz_w[kWindowFaces];
-    __shared__ int32_t win[4];
+    __shared__ int32_t win[6];
+    // a batched workgroup can hold several instances: the window needs one grid
+    DevGrid const& g = instance >= 0 ? A.batch[instance].G : A.env.G;
     int32_t ilo = INT32_MAX, ihi = INT32_MIN, jlo = INT32_MAX, jhi = INT32_MIN;
+    int32_t klo = dep ? instance : INT32_MAX, khi = dep ? instance : INT32_MIN;
     if (dep) {
         int32_t a0, a1, b0, b1, c0, c1, d0, d1;
         segment_face_range(g, r0, z0, a0, a1, b0, b1);
@@ -1442,35 +1482,44 @@ __device__ void deposit_block(DevGrid const& g, bool dep, double q, double dt, d
         jlo = b0 < d0 ? b0 : d0;
         jhi = b1 > d1 ? b1 : d1;
     }
-    if (threadIdx.x == 0) { win[0] = INT32_MAX; win[1] = INT32_MIN; win[2] = INT32_MAX; win[3] = INT32_MIN; }
+    if (threadIdx.x == 0) {
+        win[0] = INT32_MAX; win[1] = INT32_MIN; win[2] = INT32_MAX; win[3] = INT32_MIN;
+        win[4] = INT32_MAX; win[5] = INT32_MIN;
+    }
     ilo = wave_min_i32(ilo);
     ihi = wave_max_i32(ihi);
     jlo = wave_min_i32(jlo);
     jhi = wave_max_i32(jhi);
+    klo = wave_min_i32(klo);
+    khi = wave_max_i32(khi);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) {
         atomicMin(&win[0], ilo);
         atomicMax(&win[1], ihi);
         atomicMin(&win[2], jlo);
         atomicMax(&win[3], jhi);
+        atomicMin(&win[4], klo);
+        atomicMax(&win[5], khi);
     }
     __syncthreads();
     int32_t const i0 = win[0], i1 = win[1], j0 = win[2], j1 = win[3];
     if (i0 > i1) return;  // no survivor in this workgroup
     int32_t const wi = i1 - i0 + 1, nw = wi * (j1 - j0 + 1);
-    if (nw > kWindowFaces) {
+    if (nw > kWindowFaces || win[4] != win[5]) {
         if (dep) deposit_segment(g, q, dt, r0, z0, r1, z1);
         return;
     }
+    // the window's one grid: every lane flushes into it, depositing or not
+    DevGrid const& gw = win[4] >= 0 ? A.batch[win[4]].G : A.env.G;
     for (int c = threadIdx.x; c < nw; c += kBlock) { jr_w[c] = 0.0; jz_w[c] = 0.0; }
     __syncthreads();
-    if (dep) deposit_segment_to(g, LdsWindow{jr_w, jz_w, i0, j0, wi}, q, dt, r0, z0, r1, z1);
+    if (dep) deposit_segment_to(gw, LdsWindow{jr_w, jz_w, i0, j0, wi}, q, dt, r0, z0, r1, z1);
     __syncthreads();
     for (int c = threadIdx.x; c < nw; c += kBlock) {
         int32_t const i = i0 + c % wi, j = j0 + c / wi;
         double const vr = jr_w[c], vz = jz_w[c];
-        if (vr != 0.0) atomic_add(&g.jr[i + (g.nr + 1) * j], vr);
-        if (vz != 0.0) atomic_add(&g.jz[i + g.nr * j], vz);
+        if (vr != 0.0) atomic_add(&gw.jr[i + (gw.nr + 1) * j], vr);
+        if (vz != 0.0) atomic_add(&gw.jz[i + gw.nr * j], vz);
     }
 }
 
@@ -1517,7 +1566,7 @@ __device__ __forceinline__ void append_slow(bool slow, int64_t ip, int32_t* list
     }
 }
 
-template <bool kPos>
+template <bool kPos, bool kBatched>
 __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __restrict__ args,
                                                             int32_t* __restrict__ slow_list,
                                                             int32_t* __restrict__ slow_count)
@@ -1530,12 +1579,26 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
     bool slow = false;
     // the survivor's net chord, deposited by the whole workgroup at the end
     bool dep = false;
+    int32_t dep_inst = -1;
     double dep_q = 0.0, dep_r0 = 0.0, dep_z0 = 0.0, dep_r1 = 0.0, dep_z1 = 0.0;
+    // a batched wave takes the fast path only when all its particles share one
+    // instance, whose environment is then wave-uniform; a wave that straddles
+    // an instance boundary goes to the complete path
+    int32_t wave_inst = 0;
+    bool mixed = false;
+    if constexpr (kBatched) {
+        int32_t const mine = ip < A.n ? A.inst[ip] : -1;
+        int32_t const first = __builtin_amdgcn_readfirstlane(mine);
+        mixed = __ballot(ip < A.n && mine != first) != 0;
+        wave_inst = first < 0 ? 0 : first;
+    }
     if (ip < A.n) {
         slow = true;  // cleared only when the whole fast branch completes
         DevTables const& T = A.T;
-        DevFields const& F = A.F;
-        Domain const& D = A.D;
+        Env const* __restrict__ const envs = A.batch;
+        Env const& E = kBatched ? envs[wave_inst] : A.env;
+        DevFields const& F = E.F;
+        Domain const& D = E.D;
         rrea_step_config const& cfg = A.cfg;
         double const cutoff = cfg.low_energy_cutoff_eV;
         double const gcut = cfg.photon_cutoff_eV;
@@ -1545,7 +1608,9 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
         DevElastic const& EL = kPos ? T.el_p : T.el_e;
         int32_t err = kOk;
         int64_t const id = A.id[ip];
-        if (id == RREA_INVALID_ID) {
+        if (kBatched && mixed) {
+            // handed over as a whole wave
+        } else if (id == RREA_INVALID_ID) {
             if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SKIPPED_INVALID);
             if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
             if (A.substeps) A.substeps[ip] = 0;
@@ -1565,7 +1630,7 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
             Path path;
             if (!build_chord(run_x, run_y, run_z, dx, dy, dz, k, dt, dt, D, path, err) || err) break;
             if (clipped_escapes(path, D) || !(path.in_len > 0.0)) break;
-            Interval const iv = build_interval(F, path, D, A.density_ratio, err);
+            Interval const iv = build_interval(F, path, D, E.density_ratio, err);
             if (!iv.valid || err) break;
             double const span = dmax(path.end_f, 1.0e-6);
             FieldRange const fr = path_field_work(F, path, D, S::kCharge, 0.0, 1.0, err);
@@ -1613,7 +1678,7 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
             Bracket b1;
             if (!charged_bracket(T, k1, b1)) break;
             double const L = density_path(iv, path, 0.0, se);
-            uint64_t const rng = rng_prefix(cfg.rng_seed, static_cast<uint64_t>(id < 0 ? -id : id)
+            uint64_t const rng = rng_prefix(E.rng_seed, static_cast<uint64_t>(id < 0 ? -id : id)
                                                 ^ (static_cast<uint64_t>(static_cast<uint32_t>(A.cpu ? A.cpu[ip] : 0)) << 40),
                                             static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0));
             Hazard h;
@@ -1679,10 +1744,11 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
             // terminal: a survivor (for e-, above the demotion gate's energy)
             if (!kPos && thr > 0.0 && k2 <= thr) break;
             double const end_r = hypot(x, y);
-            if (!deposit_segment_check(A.G, pr, pz, end_r, z)) break;
+            if (!deposit_segment_check(E.G, pr, pz, end_r, z)) break;
             // ---- commit: nothing above has written anything ----
             slow = false;
             dep = true;
+            dep_inst = E.instance;
             dep_q = S::kCharge * w;
             dep_r0 = pr; dep_z0 = pz; dep_r1 = end_r; dep_z1 = z;
             double const um = proper_velocity(k2);
@@ -1702,7 +1768,7 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
         } while (false);
     }
     append_slow(slow, ip, slow_list, slow_count);
-    deposit_block(A.G, dep, dep_q, A.cfg.dt_s, dep_r0, dep_z0, dep_r1, dep_z1);
+    deposit_block(A, dep, dep_inst, dep_q, A.cfg.dt_s, dep_r0, dep_z0, dep_r1, dep_z1);
     reduce_tallies(tally, A.tallies);
 }
 
@@ -1756,6 +1822,7 @@ __global__ void __launch_bounds__(kBlock) photon_kernel(const StepArgs* __restri
 // own arithmetic, in the same order; any other outcome -- an interaction, a
 // sub-cutoff or weightless photon, an error -- is detected before the first
 // write and the photon is handed over to the complete step.
+template <bool kBatched>
 __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __restrict__ args,
                                                            int32_t* __restrict__ slow_list,
                                                            int32_t* __restrict__ slow_count)
@@ -1764,13 +1831,28 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
     double tally[2] = {0.0, 0.0};  // survivors, escapes
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     bool slow = false;
+    // a batched wave takes the fast path only when all its particles share one
+    // instance, whose environment is then wave-uniform; a wave that straddles
+    // an instance boundary goes to the complete path
+    int32_t wave_inst = 0;
+    bool mixed = false;
+    if constexpr (kBatched) {
+        int32_t const mine = ip < A.n ? A.inst[ip] : -1;
+        int32_t const first = __builtin_amdgcn_readfirstlane(mine);
+        mixed = __ballot(ip < A.n && mine != first) != 0;
+        wave_inst = first < 0 ? 0 : first;
+    }
     if (ip < A.n) {
         slow = true;
         DevTables const& T = A.T;
-        Domain const& D = A.D;
+        Env const* __restrict__ const envs = A.batch;
+        Env const& E = kBatched ? envs[wave_inst] : A.env;
+        Domain const& D = E.D;
         rrea_step_config const& cfg = A.cfg;
         int64_t const id = A.id[ip];
-        if (id == RREA_INVALID_ID) {
+        if (kBatched && mixed) {
+            // handed over as a whole wave
+        } else if (id == RREA_INVALID_ID) {
             if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SKIPPED_INVALID);
             if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
             if (A.substeps) A.substeps[ip] = 0;
@@ -1787,7 +1869,7 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
             Path path;
             if (!clip_rz_path(pr, pth, pz, A.r[ip], A.th[ip], A.z[ip], D, path, err) || err) break;
             bool const escapes = clipped_escapes(path, D);
-            Interval const iv = build_interval(A.F, path, D, A.density_ratio, err);
+            Interval const iv = build_interval(E.F, path, D, E.density_ratio, err);
             double const chord_dp = density_path(iv, path, 0.0, path.end_f);
             if (err) break;
             if (path.in_len > 0.0) {
@@ -1801,7 +1883,7 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
                     ^ (static_cast<uint64_t>(static_cast<uint32_t>(cpu)) << 40);
                 uint64_t const step = static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0);
                 // the first pass's live index is segment 0, event 0: index 0
-                double const draw = rng_uniform(rng_prefix(cfg.rng_seed, pkey, step), 0U, kChPhotonInteraction);
+                double const draw = rng_uniform(rng_prefix(E.rng_seed, pkey, step), 0U, kChPhotonInteraction);
                 if (select_hazard(chord_dp / mfp, 0.0, draw, 0.0).event) break;
             }
             // ---- commit: nothing above has written anything ----
@@ -1941,13 +2023,14 @@ __device__ bool newborn_final_point(Path const& p, Domain const& D, double fract
 // The newborn's electron_state_policy_at (OV/RreaNewbornAdvance.cpp:79-96):
 // the demotion gate with the field gathered at the point and the density
 // ratio at its z
-__device__ bool newborn_electron_demotes(StepArgs const& A, double x, double y, double z, double k, int32_t& err)
+__device__ bool newborn_electron_demotes(StepArgs const& A, Env const& E, double x, double y, double z, double k,
+                                         int32_t& err)
 {
     double const cutoff = A.cfg.low_energy_cutoff_eV;
     double const thr = A.T.hard_moller_threshold;
     double er, ez, bt;
-    gather(A.F, hypot(x, y), z, er, ez, bt);
-    double const ratio = positive_or(density_ratio_at_z(A.F, z, err), 1.0);
+    gather(E.F, hypot(x, y), z, er, ez, bt);
+    double const ratio = positive_or(density_ratio_at_z(E.F, z, err), 1.0);
     double const drag = ensemble_drag(A.T, dmax(k, cutoff), ratio, A.cfg.disable_hard_moller == 0, err);
     return k < cutoff || (thr > 0.0 && k <= thr && hypot(er, ez) <= drag);
 }
@@ -1959,8 +2042,9 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
     using S = Species<kPos>;
     StepArgs const& A = N.S;
     DevTables const& T = A.T;
-    DevFields const& F = A.F;
-    Domain const& D = A.D;
+    Env const& E = env_of_record(A, N.work[i].instance);
+    DevFields const& F = E.F;
+    Domain const& D = E.D;
     rrea_step_config const& cfg = A.cfg;
     double const cutoff = cfg.low_energy_cutoff_eV;
     double const gcut = cfg.photon_cutoff_eV;
@@ -1984,23 +2068,23 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
     uint64_t const rsp = static_cast<uint64_t>(p.root_species);
     uint64_t const lin = p.lineage;
     double const w = p.weight;
-    uint64_t const rng = rng_prefix(cfg.rng_seed, p.root_key, step);
-    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1};
+    uint64_t const rng = rng_prefix(E.rng_seed, p.root_key, step);
+    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1, E.instance};
     double const birth_r = hypot(p.x, p.y), birth_z = p.z;
     double const residual = newborn_residual(p.birth_fraction);
     double k = p.energy_eV;
     double dx = p.dir_x, dy = p.dir_y, dz = p.dir_z;
     normalize(dx, dy, dz);
     auto segment_to = [&](double r1, double z1) {
-        if (w != 0.0 && !deposit_segment(A.G, S::kCharge * w, dt, birth_r, birth_z, r1, z1) && !err) err = kErrInvalid;
+        if (w != 0.0 && !deposit_segment(E.G, S::kCharge * w, dt, birth_r, birth_z, r1, z1) && !err) err = kErrInvalid;
     };
     auto escape_along = [&](Path const& path) {
         Crossing const bc = boundary_crossing(path, D);
         double const q = S::kCharge * w;
         if (!bc.valid) {
             if (!err) err = kErrInvalid;
-        } else if (q != 0.0 && !deposit_escaping_segment(A.G, q, dt, birth_r, birth_z, bc.start_r, bc.start_z,
-                                                         bc.cross_r, bc.cross_z, A.pnr, A.pzb, A.pnz, bc.faces)) {
+        } else if (q != 0.0 && !deposit_escaping_segment(E.G, q, dt, birth_r, birth_z, bc.start_r, bc.start_z,
+                                                         bc.cross_r, bc.cross_z, E.pnr, E.pzb, E.pnz, bc.faces)) {
             if (!err) err = kErrInvalid;
         }
         status = RREA_ESCAPE;
@@ -2022,7 +2106,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
     if (!in_half_open(p.x, p.y, p.z, D)) {
         err = kErrDomain;
     } else {
-        bool const birth_demote = kPos ? k < cutoff : newborn_electron_demotes(A, p.x, p.y, p.z, k, err);
+        bool const birth_demote = kPos ? k < cutoff : newborn_electron_demotes(A, E, p.x, p.y, p.z, k, err);
         if (err) {
         } else if (birth_demote) {
             terminate_at(p.x, p.y, p.z, newborn_index(rsp, lin, 0U, 0U, 0U, err), p.birth_fraction);
@@ -2047,7 +2131,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                 }
             }
             if (driver_ok && !err) {
-                Interval iv = build_interval(F, driver, D, A.density_ratio, err);
+                Interval iv = build_interval(F, driver, D, E.density_ratio, err);
                 if (!iv.valid && !err) err = kErrInvalid;
                 Path path = driver;
                 uint64_t substeps = 1;
@@ -2148,7 +2232,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                             final_f = 0.0;
                             break;
                         }
-                        iv = build_interval(F, path, D, A.density_ratio, err);
+                        iv = build_interval(F, path, D, E.density_ratio, err);
                         if (!iv.valid) { if (!err) err = kErrInvalid; break; }
                         sb = 0.0;
                         se = path.end_f;
@@ -2369,7 +2453,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                         } else {
                             double const final_r = hypot(fx_, fy_);
                             double const final_el = elapsed_at(final_f);
-                            bool const demote = kPos ? k < cutoff : newborn_electron_demotes(A, fx_, fy_, fz_, k, err);
+                            bool const demote = kPos ? k < cutoff : newborn_electron_demotes(A, E, fx_, fy_, fz_, k, err);
                             if (err) {
                             } else if (terminal_at_point || demote) {
                                 terminate_at(fx_, fy_, fz_,
@@ -2409,8 +2493,9 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
 {
     StepArgs const& A = N.S;
     DevTables const& T = A.T;
-    DevFields const& F = A.F;
-    Domain const& D = A.D;
+    Env const& E = env_of_record(A, N.work[i].instance);
+    DevFields const& F = E.F;
+    Domain const& D = E.D;
     rrea_step_config const& cfg = A.cfg;
     double const cutoff = cfg.low_energy_cutoff_eV;
     double const gcut = cfg.photon_cutoff_eV;
@@ -2425,8 +2510,8 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
     int status = RREA_SURVIVE;
     uint64_t const rsp = static_cast<uint64_t>(p.root_species);
     double const w = p.weight;
-    uint64_t const rng = rng_prefix(cfg.rng_seed, p.root_key, step);
-    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1};
+    uint64_t const rng = rng_prefix(E.rng_seed, p.root_key, step);
+    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1, E.instance};
     double const residual = newborn_residual(p.birth_fraction);
     double dx = p.dir_x, dy = p.dir_y, dz = p.dir_z;
     normalize(dx, dy, dz);
@@ -2462,7 +2547,7 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
                 while (!err) {
                     uint64_t const idx = newborn_index(rsp, p.lineage, 0U, ord, 0U, err);
                     if (err) break;
-                    Interval const iv = build_interval(F, path, D, A.density_ratio, err);
+                    Interval const iv = build_interval(F, path, D, E.density_ratio, err);
                     PhotonRates rates;
                     if (!photon_rates(T, energy, rates, err)) break;
                     double const total = rates.compton + rates.pe + rates.pair_n + rates.pair_t;
@@ -3130,21 +3215,19 @@ rrea_status slow_list_workspace(rrea_fields_t F, int64_t n, hipStream_t s)
     return RREA_OK;
 }
 
-// The argument block every transport kernel shares: tables, fields, domain,
-// the reference-z density ratio, the J grid and the step configuration.
-rrea_status common_args(rrea_tables_t T, rrea_fields_t F, const rrea_step_config* cfg, double* jr, double* jz,
-                        double* rho, StepArgs& A)
+// One simulation's environment from its fields handle: fields, domain, the
+// reference-z density ratio, the padded J grid and the RNG seed.
+rrea_status make_env(rrea_fields_t F, double* jr, double* jz, double* rho, int32_t instance, uint64_t seed, Env& E)
 {
-    A.T = T->view;
     rrea_rz_geometry const& g = F->geom;
-    A.F.nr = g.nr; A.F.nz = g.nz; A.F.dr = g.dr; A.F.dz = g.dz; A.F.r_lo = g.r_lo; A.F.z_lo = g.z_lo;
-    A.F.cell_er = F->d_cell_er; A.F.cell_ez = F->d_cell_ez; A.F.btheta = F->d_btheta;
-    A.F.density_mode = F->density_mode; A.F.density_ratio = F->density_ratio;
-    A.F.prof_alt = F->d_prof; A.F.prof_rho = F->d_prof ? F->d_prof + F->prof_n : nullptr;
-    A.F.prof_n = static_cast<int32_t>(F->prof_n); A.F.altitude_at_z0 = F->altitude_at_z0;
-    A.F.prof_guide = Guide{F->d_prof_guide, kProfileGuideBins, F->prof_guide_lo, F->prof_guide_scale};
-    A.F.reference_density = F->reference_density;
-    A.D = Domain{g.r_lo, g.r_lo + g.nr * g.dr, g.z_lo, g.z_lo + g.nz * g.dz};
+    E.F.nr = g.nr; E.F.nz = g.nz; E.F.dr = g.dr; E.F.dz = g.dz; E.F.r_lo = g.r_lo; E.F.z_lo = g.z_lo;
+    E.F.cell_er = F->d_cell_er; E.F.cell_ez = F->d_cell_ez; E.F.btheta = F->d_btheta;
+    E.F.density_mode = F->density_mode; E.F.density_ratio = F->density_ratio;
+    E.F.prof_alt = F->d_prof; E.F.prof_rho = F->d_prof ? F->d_prof + F->prof_n : nullptr;
+    E.F.prof_n = static_cast<int32_t>(F->prof_n); E.F.altitude_at_z0 = F->altitude_at_z0;
+    E.F.prof_guide = Guide{F->d_prof_guide, kProfileGuideBins, F->prof_guide_lo, F->prof_guide_scale};
+    E.F.reference_density = F->reference_density;
+    E.D = Domain{g.r_lo, g.r_lo + g.nr * g.dr, g.z_lo, g.z_lo + g.nz * g.dz};
     // RreaParticleInteraction::Apply :29-35: the ratio at the domain's mid z.
     {
         double ratio = F->density_ratio;
@@ -3152,7 +3235,7 @@ rrea_status common_args(rrea_tables_t T, rrea_fields_t F, const rrea_step_config
             // the host copy of the profile kept by rrea_density_set_profile
             std::vector<double> const& alt = F->h_prof_alt;
             std::vector<double> const& rhov = F->h_prof_rho;
-            double const a = F->altitude_at_z0 + 0.5 * (A.D.z_lo + A.D.z_hi);
+            double const a = F->altitude_at_z0 + 0.5 * (E.D.z_lo + E.D.z_hi);
             if (a < alt.front() || a > alt.back()) return RREA_ERR_TABLE_RANGE;
             auto const up = std::upper_bound(alt.begin(), alt.end(), a);
             double v;
@@ -3166,26 +3249,68 @@ rrea_status common_args(rrea_tables_t T, rrea_fields_t F, const rrea_step_config
             }
             ratio = v / F->reference_density;
         }
-        A.density_ratio = ratio > 0.0 ? ratio : 1.0;
+        E.density_ratio = ratio > 0.0 ? ratio : 1.0;
     }
     int32_t const pad = 2;
-    A.G = DevGrid{g.nr + pad, g.nz + 2 * pad, g.dr, g.dz, g.r_lo, g.z_lo - pad * g.dz, rho, jr, jz};
-    A.pnr = g.nr; A.pzb = pad; A.pnz = g.nz;
-    A.cfg = *cfg;
+    E.G = DevGrid{g.nr + pad, g.nz + 2 * pad, g.dr, g.dz, g.r_lo, g.z_lo - pad * g.dz, rho, jr, jz};
+    E.pnr = g.nr; E.pzb = pad; E.pnz = g.nz;
+    E.instance = instance;
+    E.rng_seed = seed;
     return RREA_OK;
 }
 
-rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
-                         const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
+// The argument block every transport kernel shares: tables, the plain step's
+// environment and the step configuration.
+rrea_status common_args(rrea_tables_t T, rrea_fields_t F, const rrea_step_config* cfg, double* jr, double* jz,
+                        double* rho, StepArgs& A)
 {
+    A.T = T->view;
+    A.batch = nullptr;
+    A.inst = nullptr;
+    A.cfg = *cfg;
+    return make_env(F, jr, jz, rho, -1, cfg->rng_seed, A.env);
+}
+
+// Every instance's environment, formed on the host from its fields handle
+// and copied to the device (fields may have changed since the last call).
+rrea_status upload_envs(rrea_batch_t B, const rrea_step_config* cfg, hipStream_t s)
+{
+    (void)cfg;
+    size_t const n = B->fields.size();
+    Env* h = reinterpret_cast<Env*>(B->h_envs.data());
+    for (size_t i = 0; i < n; ++i) {
+        if (!B->fields[i]->fields_set) return RREA_ERR_ARGUMENT;
+        rrea_status const st = make_env(B->fields[i], B->jr[i], B->jz[i], B->rho[i], static_cast<int32_t>(i),
+                                        B->seeds[i], h[i]);
+        if (st != RREA_OK) return st;
+    }
+    // pageable source: the copy has consumed the host array when hipMemcpyAsync returns
+    HIP_TRY(hipMemcpyAsync(B->d_envs, h, n * sizeof(Env), hipMemcpyHostToDevice, s));
+    return RREA_OK;
+}
+
+// A batched step's instances: their environments on the device, each
+// particle's instance, and the workspace handle the step runs in.
+struct BatchCtx {
+    rrea_fields_t work;
+    const Env* d_envs;
+    const int32_t* inst;
+};
+
+rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+                         const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream,
+                         const BatchCtx* batch = nullptr)
+{
+    bool const batched = batch != nullptr;
     bool const positron = species == kPositron;
     bool const photon = species == kPhoton;
     if (!T || !F || !tile || !cfg || !out || !F->fields_set) return RREA_ERR_ARGUMENT;
     if (!out->secondaries || !out->secondary_count || !out->tallies || !out->error || out->secondary_capacity < 0) {
         return RREA_ERR_ARGUMENT;
     }
-    // the photon step deposits no current
-    if (!photon && (!out->jr || !out->jz || !out->rho)) return RREA_ERR_ARGUMENT;
+    // the photon step deposits no current; a batched step deposits into its instances' grids
+    if (!photon && !batched && (!out->jr || !out->jz || !out->rho)) return RREA_ERR_ARGUMENT;
+    if (batched && (!batch->work || !batch->d_envs || (tile->n > 0 && !batch->inst))) return RREA_ERR_ARGUMENT;
     if (tile->n < 0 || (tile->n > 0 && (!tile->prev_r || !tile->prev_z || !tile->prev_theta || !tile->w || !tile->id
                                          || !tile->r || !tile->z || !tile->theta || !tile->ux || !tile->uy || !tile->uz
                                          || (!photon && (!tile->prev_ux || !tile->prev_uy || !tile->prev_uz))))) {
@@ -3205,10 +3330,15 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     HIP_TRY(hipGetDevice(&dev));
     if (dev != T->device) return RREA_ERR_ARGUMENT;
 
+    rrea_fields_t const W = batched ? batch->work : F;  // the call's workspace
     StepArgs A;
     {
         rrea_status const cs = common_args(T, F, cfg, out->jr, out->jz, out->rho, A);
         if (cs != RREA_OK) return cs;
+    }
+    if (batched) {
+        A.batch = batch->d_envs;
+        A.inst = batch->inst;
     }
     A.n = tile->n;
     A.prev_r = tile->prev_r; A.prev_z = tile->prev_z; A.prev_th = tile->prev_theta;
@@ -3224,15 +3354,15 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
         // stage into the library's workspace (count from 0), then sort into the
         // caller's buffer behind its current count, which stays on the device
         {
-            rrea_status const ws = order_workspace(F, cap);
+            rrea_status const ws = order_workspace(W, cap);
             if (ws != RREA_OK) return ws;
         }
-        if (!F->d_base) HIP_TRY(hipMalloc(&F->d_base, sizeof(int64_t)));
-        HIP_TRY(hipMemcpyAsync(F->d_base, out->secondary_count, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
-        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(F->d_count), 1, 0ULL);
+        if (!W->d_base) HIP_TRY(hipMalloc(&W->d_base, sizeof(int64_t)));
+        HIP_TRY(hipMemcpyAsync(W->d_base, out->secondary_count, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(W->d_count), 1, 0ULL);
         HIP_TRY(hipGetLastError());
-        A.sec = F->d_stage_sec;
-        A.sec_count = F->d_count;
+        A.sec = W->d_stage_sec;
+        A.sec_count = W->d_count;
     } else {
         A.sec = out->secondaries;
         A.sec_count = out->secondary_count;
@@ -3241,18 +3371,18 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
 
     int64_t const nb = blocks_for(tile->n, kBlock);
     // pageable source: the copy has consumed A when hipMemcpyAsync returns
-    HIP_TRY(hipMemcpyAsync(F->d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(W->d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
     // HIP events around each transport kernel while timing is enabled
     auto timed = [&](int32_t kernel, auto&& launch) -> rrea_status {
         hipEvent_t stop = nullptr;
-        if (F->timing) {
-            if (F->events_used == F->events.size()) {
+        if (W->timing) {
+            if (W->events_used == W->events.size()) {
                 hipEvent_t a, b;
                 HIP_TRY(hipEventCreate(&a));
                 HIP_TRY(hipEventCreate(&b));
-                F->events.push_back({a, b, 0});
+                W->events.push_back({a, b, 0});
             }
-            rrea_fields_s::Timed& t = F->events[F->events_used++];
+            rrea_fields_s::Timed& t = W->events[W->events_used++];
             t.kernel = kernel;
             HIP_TRY(hipEventRecord(t.start, s));
             stop = t.stop;
@@ -3262,23 +3392,28 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
         if (stop) HIP_TRY(hipEventRecord(stop, s));
         return RREA_OK;
     };
-    const StepArgs* const d_args = static_cast<const StepArgs*>(F->d_args);
+    const StepArgs* const d_args = static_cast<const StepArgs*>(W->d_args);
     rrea_status st = RREA_OK;
     int32_t const kid = positron ? 2 : 0;   // timing ids: main kernel, then its hand-over kernel
     if (photon && cfg->complete_path_only) {
         st = timed(4, [&] { photon_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args); });
     } else if (photon) {
-        rrea_status const ls = slow_list_workspace(F, tile->n, s);
+        rrea_status const ls = slow_list_workspace(W, tile->n, s);
         if (ls != RREA_OK) return ls;
         st = timed(4, [&] {
-            photon_fast_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args, F->d_slow_list,
-                                                                                      F->d_slow_count);
+            if (batched) {
+                photon_fast_kernel<true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
+                    d_args, W->d_slow_list, W->d_slow_count);
+            } else {
+                photon_fast_kernel<false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
+                    d_args, W->d_slow_list, W->d_slow_count);
+            }
         });
         if (st != RREA_OK) return st;
         int64_t const slow_blocks = nb < 2048 ? nb : 2048;
         st = timed(5, [&] {
             photon_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
-                d_args, F->d_slow_list, F->d_slow_count);
+                d_args, W->d_slow_list, W->d_slow_count);
         });
     } else if (cfg->complete_path_only) {
         st = timed(kid, [&] {
@@ -3286,15 +3421,25 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
             else electron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
         });
     } else {
-        rrea_status const ls = slow_list_workspace(F, tile->n, s);
+        rrea_status const ls = slow_list_workspace(W, tile->n, s);
         if (ls != RREA_OK) return ls;
         st = timed(kid, [&] {
             if (positron) {
-                charged_fast_kernel<true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                    d_args, F->d_slow_list, F->d_slow_count);
+                if (batched) {
+                    charged_fast_kernel<true, true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
+                        d_args, W->d_slow_list, W->d_slow_count);
+                } else {
+                    charged_fast_kernel<true, false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
+                        d_args, W->d_slow_list, W->d_slow_count);
+                }
             } else {
-                charged_fast_kernel<false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                    d_args, F->d_slow_list, F->d_slow_count);
+                if (batched) {
+                    charged_fast_kernel<false, true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
+                        d_args, W->d_slow_list, W->d_slow_count);
+                } else {
+                    charged_fast_kernel<false, false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
+                        d_args, W->d_slow_list, W->d_slow_count);
+                }
             }
         });
         if (st != RREA_OK) return st;
@@ -3302,10 +3447,10 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
         st = timed(kid + 1, [&] {
             if (positron) {
                 charged_slow_kernel<true><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
-                    d_args, F->d_slow_list, F->d_slow_count);
+                    d_args, W->d_slow_list, W->d_slow_count);
             } else {
                 charged_slow_kernel<false><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
-                    d_args, F->d_slow_list, F->d_slow_count);
+                    d_args, W->d_slow_list, W->d_slow_count);
             }
         });
     }
@@ -3314,15 +3459,182 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     if (order) {
         // the one host round trip of the call: the sort's item count
         int64_t nstaged = 0;
-        HIP_TRY(hipMemcpyAsync(&nstaged, F->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&nstaged, W->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         int64_t const nsort = nstaged < cap ? nstaged : cap;
         {
-            rrea_status const os = order_staged(F, nsort, out->secondaries, s, F->d_base, cap, out->error);
+            rrea_status const os = order_staged(W, nsort, out->secondaries, s, W->d_base, cap, out->error);
             if (os != RREA_OK) return os;
         }
-        append_count_kernel<<<dim3(1), dim3(64), 0, s>>>(out->secondary_count, F->d_base, nstaged);
+        append_count_kernel<<<dim3(1), dim3(64), 0, s>>>(out->secondary_count, W->d_base, nstaged);
         HIP_TRY(hipGetLastError());
+    }
+    return RREA_OK;
+}
+
+// The drain of one plain step (batch == nullptr) or of a batched step, whose
+// records carry their instance.
+rrea_status drain_impl(rrea_tables_t T, rrea_fields_t F, const rrea_step_config* cfg, const rrea_drain_io* io,
+                       rrea_stream_t stream, const BatchCtx* batch)
+{
+    if (!T || !F || !cfg || !io || !F->fields_set) return RREA_ERR_ARGUMENT;
+    if (!io->work || !io->work_count || !io->survivors || !io->survivor_count || !io->tallies || !io->error
+        || (!batch && (!io->jr || !io->jz || !io->rho))) {
+        return RREA_ERR_ARGUMENT;
+    }
+    int64_t const cap = io->work_capacity;
+    if (cap < 0 || cap > INT32_MAX || io->n_initial < 0 || io->n_initial > cap || io->survivor_capacity < 0) {
+        return RREA_ERR_ARGUMENT;
+    }
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != T->device) return RREA_ERR_ARGUMENT;
+    // RreaParticleInteraction::Apply :38-40 -- a non-positive dt does nothing.
+    if (!(cfg->dt_s > 0.0) || io->n_initial == 0) {
+        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->work_count, 0);
+        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->survivor_count, 0);
+        HIP_TRY(hipGetLastError());
+        return RREA_OK;
+    }
+    if (!(cfg->max_optical_depth_per_substep > 0.0) || cfg->max_subcycles == 0 || cfg->max_subcycles > 16383) {
+        return RREA_ERR_ARGUMENT;
+    }
+    rrea_fields_t const W = batch ? batch->work : F;  // the call's workspace
+    StepArgs A{};
+    {
+        rrea_status const cs = common_args(T, F, cfg, io->jr, io->jz, io->rho, A);
+        if (cs != RREA_OK) return cs;
+    }
+    A.tallies = io->tallies;
+    A.error = io->error;
+    if (batch) A.batch = batch->d_envs;
+    {
+        rrea_status const ws = order_workspace(W, cap);
+        if (ws != RREA_OK) return ws;
+    }
+    if (W->fifo_cap < cap) {
+        void* ptrs[] = {W->d_fin, W->d_fst, W->d_fion, W->d_fsub, W->d_anchor, W->d_next, W->d_flag, W->d_sel_tmp};
+        for (void* p : ptrs) if (p) (void)hipFree(p);
+        W->d_fin = nullptr; W->d_fst = nullptr; W->d_fion = nullptr; W->d_fsub = nullptr;
+        W->d_anchor = nullptr; W->d_next = nullptr;
+        W->d_flag = nullptr; W->d_sel_tmp = nullptr;
+        W->fifo_cap = 0;
+        HIP_TRY(hipMalloc(&W->d_fin, cap * sizeof(rrea_secondary)));
+        HIP_TRY(hipMalloc(&W->d_fst, cap * sizeof(uint8_t)));
+        HIP_TRY(hipMalloc(&W->d_fion, cap * sizeof(double)));
+        HIP_TRY(hipMalloc(&W->d_fsub, cap * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&W->d_anchor, cap * sizeof(int64_t)));
+        HIP_TRY(hipMalloc(&W->d_next, cap * sizeof(unsigned long long)));
+        HIP_TRY(hipMalloc(&W->d_flag, cap * sizeof(int32_t)));
+        size_t bytes = 0;
+        HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, W->d_fin, W->d_flag, W->d_stage_sec, W->d_nsel,
+                                              static_cast<int>(cap), s));
+        HIP_TRY(hipMalloc(&W->d_sel_tmp, bytes));
+        W->sel_tmp_bytes = bytes;
+        W->fifo_cap = cap;
+    }
+    if (!W->d_nsel) HIP_TRY(hipMalloc(&W->d_nsel, sizeof(int64_t)));
+    if (!W->d_nargs) HIP_TRY(hipMalloc(&W->d_nargs, sizeof(NewbornArgs)));
+    int64_t const n0 = io->n_initial;
+    fill_u64_kernel<<<dim3(static_cast<unsigned>(blocks_for(n0, 256))), dim3(256), 0, s>>>(W->d_next, n0, 0ULL);
+    HIP_TRY(hipGetLastError());
+    fifo_init_kernel<<<dim3(static_cast<unsigned>(blocks_for(n0, 256))), dim3(256), 0, s>>>(
+        io->work, n0, W->d_anchor, W->d_next, io->error);
+    HIP_TRY(hipGetLastError());
+    // One launch per generation; the host reads the child count in between.
+    // The generation kernel advances nothing once an error is recorded (a
+    // rejected birth payload included).
+    int32_t err_host[2] = {0, 0};
+    int64_t gb = 0, ge = n0;
+    while (gb < ge) {
+        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(W->d_count), 1, 0ULL);
+        HIP_TRY(hipGetLastError());
+        A.sec = W->d_stage_sec;
+        A.sec_cap = cap - ge;
+        A.sec_count = W->d_count;
+        NewbornArgs N{A, io->work, gb, ge, W->d_fin, W->d_fst, W->d_fion, W->d_fsub};
+        // pageable source: the copy has consumed N when hipMemcpyAsync returns
+        HIP_TRY(hipMemcpyAsync(W->d_nargs, &N, sizeof(NewbornArgs), hipMemcpyHostToDevice, s));
+        {
+            hipEvent_t stop = nullptr;
+            if (W->timing) {
+                if (W->events_used == W->events.size()) {
+                    hipEvent_t a, b;
+                    HIP_TRY(hipEventCreate(&a));
+                    HIP_TRY(hipEventCreate(&b));
+                    W->events.push_back({a, b, 0});
+                }
+                rrea_fields_s::Timed& t = W->events[W->events_used++];
+                t.kernel = 6;
+                HIP_TRY(hipEventRecord(t.start, s));
+                stop = t.stop;
+            }
+            newborn_kernel<<<dim3(static_cast<unsigned>(blocks_for(ge - gb, kBlock))), dim3(kBlock), 0, s>>>(
+                static_cast<const NewbornArgs*>(W->d_nargs));
+            HIP_TRY(hipGetLastError());
+            if (stop) HIP_TRY(hipEventRecord(stop, s));
+        }
+        int64_t c = 0;
+        HIP_TRY(hipMemcpyAsync(&c, W->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(err_host, io->error, sizeof(err_host), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (err_host[0] != 0 || c <= 0) break;
+        if (c > cap - ge) break;  // the kernel reported the capacity error
+        {
+            rrea_status const os = order_staged(W, c, io->work + ge, s);
+            if (os != RREA_OK) return os;
+        }
+        unsigned const cb = static_cast<unsigned>(blocks_for(c, 256));
+        fifo_anchor_kernel<<<dim3(cb), dim3(256), 0, s>>>(io->work, ge, ge + c, W->d_anchor);
+        HIP_TRY(hipGetLastError());
+        fifo_lineage_kernel<<<dim3(cb), dim3(256), 0, s>>>(io->work, ge, ge + c, W->d_anchor, W->d_next, io->error);
+        HIP_TRY(hipGetLastError());
+        fifo_advance_kernel<<<dim3(cb), dim3(256), 0, s>>>(ge, ge + c, W->d_anchor, W->d_next);
+        HIP_TRY(hipGetLastError());
+        gb = ge;
+        ge += c;
+    }
+    int64_t const total = ge;
+    set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->work_count, total);
+    HIP_TRY(hipGetLastError());
+    if (err_host[0] != 0) {
+        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->survivor_count, 0);
+        HIP_TRY(hipGetLastError());
+        return RREA_OK;
+    }
+    survivor_flag_kernel<<<dim3(static_cast<unsigned>(blocks_for(total, 256))), dim3(256), 0, s>>>(W->d_fst, total,
+                                                                                                  W->d_flag);
+    HIP_TRY(hipGetLastError());
+    size_t bytes = W->sel_tmp_bytes;
+    if (io->survivor_capacity >= total) {
+        // every entry fits: select straight into the caller's buffer, no round trip
+        HIP_TRY(hipcub::DeviceSelect::Flagged(W->d_sel_tmp, bytes, W->d_fin, W->d_flag, io->survivors,
+                                              io->survivor_count, static_cast<int>(total), s));
+    } else {
+        int64_t nsurv = 0;
+        HIP_TRY(hipcub::DeviceSelect::Flagged(W->d_sel_tmp, bytes, W->d_fin, W->d_flag, W->d_stage_sec, W->d_nsel,
+                                              static_cast<int>(total), s));
+        HIP_TRY(hipMemcpyAsync(&nsurv, W->d_nsel, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (nsurv > io->survivor_capacity) {
+            report_kernel<<<dim3(1), dim3(64), 0, s>>>(io->error, kErrCapacity, -1);
+            HIP_TRY(hipGetLastError());
+            nsurv = io->survivor_capacity;
+        }
+        if (nsurv > 0) {
+            HIP_TRY(hipMemcpyAsync(io->survivors, W->d_stage_sec, nsurv * sizeof(rrea_secondary),
+                                   hipMemcpyDeviceToDevice, s));
+        }
+        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->survivor_count, nsurv);
+        HIP_TRY(hipGetLastError());
+    }
+    if (io->status) HIP_TRY(hipMemcpyAsync(io->status, W->d_fst, total, hipMemcpyDeviceToDevice, s));
+    if (io->ion_pairs) {
+        HIP_TRY(hipMemcpyAsync(io->ion_pairs, W->d_fion, total * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    if (io->substeps) {
+        HIP_TRY(hipMemcpyAsync(io->substeps, W->d_fsub, total * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
     }
     return RREA_OK;
 }
@@ -3330,6 +3642,68 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
 }  // namespace
 
 extern "C" {
+
+rrea_status rrea_batch_create(rrea_fields_t const* fields, int32_t n, double* const* jr, double* const* jz,
+                              double* const* rho, const uint64_t* rng_seeds, rrea_batch_t* out)
+{
+    if (!fields || n <= 0 || !jr || !jz || !rho || !rng_seeds || !out) return RREA_ERR_ARGUMENT;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!fields[i] || !jr[i] || !jz[i] || !rho[i]) return RREA_ERR_ARGUMENT;
+    }
+    auto* B = new (std::nothrow) rrea_batch_s();
+    if (!B) return RREA_ERR_HIP;
+    B->fields.assign(fields, fields + n);
+    B->jr.assign(jr, jr + n);
+    B->jz.assign(jz, jz + n);
+    B->rho.assign(rho, rho + n);
+    B->seeds.assign(rng_seeds, rng_seeds + n);
+    rrea_status st = rrea_fields_create(&fields[0]->geom, &B->work);
+    if (st == RREA_OK && hipMalloc(&B->d_envs, static_cast<size_t>(n) * sizeof(Env)) != hipSuccess) st = RREA_ERR_HIP;
+    if (st != RREA_OK) {
+        rrea_batch_destroy(B);
+        return st;
+    }
+    B->h_envs.resize(static_cast<size_t>(n) * sizeof(Env));
+    *out = B;
+    return RREA_OK;
+}
+
+rrea_status rrea_batch_destroy(rrea_batch_t B)
+{
+    if (!B) return RREA_OK;
+    if (B->d_envs) (void)hipFree(B->d_envs);
+    if (B->work) rrea_fields_destroy(B->work);
+    delete B;
+    return RREA_OK;
+}
+
+rrea_status rrea_batch_workspace(rrea_batch_t B, rrea_fields_t* out)
+{
+    if (!B || !out) return RREA_ERR_ARGUMENT;
+    *out = B->work;
+    return RREA_OK;
+}
+
+rrea_status rrea_transport_step_batched(rrea_tables_t T, rrea_batch_t B, int32_t species, const rrea_tile* tile,
+                                        const int32_t* instance, const rrea_step_config* cfg, const rrea_step_out* out,
+                                        rrea_stream_t stream)
+{
+    if (!T || !B || !cfg || species < kElectron || species > kPhoton) return RREA_ERR_ARGUMENT;
+    rrea_status const es = upload_envs(B, cfg, static_cast<hipStream_t>(stream));
+    if (es != RREA_OK) return es;
+    BatchCtx const ctx{B->work, static_cast<const Env*>(B->d_envs), instance};
+    return species_step(species, T, B->fields[0], tile, cfg, out, stream, &ctx);
+}
+
+rrea_status rrea_newborn_drain_batched(rrea_tables_t T, rrea_batch_t B, const rrea_step_config* cfg,
+                                       const rrea_drain_io* io, rrea_stream_t stream)
+{
+    if (!T || !B || !cfg) return RREA_ERR_ARGUMENT;
+    rrea_status const es = upload_envs(B, cfg, static_cast<hipStream_t>(stream));
+    if (es != RREA_OK) return es;
+    BatchCtx const ctx{B->work, static_cast<const Env*>(B->d_envs), nullptr};
+    return drain_impl(T, B->fields[0], cfg, io, stream, &ctx);
+}
 
 rrea_status rrea_transport_step_electrons(rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
                                           const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
@@ -3352,164 +3726,7 @@ rrea_status rrea_transport_step_photons(rrea_tables_t T, rrea_fields_t F, const 
 rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step_config* cfg, const rrea_drain_io* io,
                                rrea_stream_t stream)
 {
-    if (!T || !F || !cfg || !io || !F->fields_set) return RREA_ERR_ARGUMENT;
-    if (!io->work || !io->work_count || !io->survivors || !io->survivor_count || !io->jr || !io->jz || !io->rho
-        || !io->tallies || !io->error) {
-        return RREA_ERR_ARGUMENT;
-    }
-    int64_t const cap = io->work_capacity;
-    if (cap < 0 || cap > INT32_MAX || io->n_initial < 0 || io->n_initial > cap || io->survivor_capacity < 0) {
-        return RREA_ERR_ARGUMENT;
-    }
-    hipStream_t const s = static_cast<hipStream_t>(stream);
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    if (dev != T->device) return RREA_ERR_ARGUMENT;
-    // RreaParticleInteraction::Apply :38-40 -- a non-positive dt does nothing.
-    if (!(cfg->dt_s > 0.0) || io->n_initial == 0) {
-        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->work_count, 0);
-        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->survivor_count, 0);
-        HIP_TRY(hipGetLastError());
-        return RREA_OK;
-    }
-    if (!(cfg->max_optical_depth_per_substep > 0.0) || cfg->max_subcycles == 0 || cfg->max_subcycles > 16383) {
-        return RREA_ERR_ARGUMENT;
-    }
-    StepArgs A{};
-    {
-        rrea_status const cs = common_args(T, F, cfg, io->jr, io->jz, io->rho, A);
-        if (cs != RREA_OK) return cs;
-    }
-    A.tallies = io->tallies;
-    A.error = io->error;
-    {
-        rrea_status const ws = order_workspace(F, cap);
-        if (ws != RREA_OK) return ws;
-    }
-    if (F->fifo_cap < cap) {
-        void* ptrs[] = {F->d_fin, F->d_fst, F->d_fion, F->d_fsub, F->d_anchor, F->d_next, F->d_flag, F->d_sel_tmp};
-        for (void* p : ptrs) if (p) (void)hipFree(p);
-        F->d_fin = nullptr; F->d_fst = nullptr; F->d_fion = nullptr; F->d_fsub = nullptr;
-        F->d_anchor = nullptr; F->d_next = nullptr;
-        F->d_flag = nullptr; F->d_sel_tmp = nullptr;
-        F->fifo_cap = 0;
-        HIP_TRY(hipMalloc(&F->d_fin, cap * sizeof(rrea_secondary)));
-        HIP_TRY(hipMalloc(&F->d_fst, cap * sizeof(uint8_t)));
-        HIP_TRY(hipMalloc(&F->d_fion, cap * sizeof(double)));
-        HIP_TRY(hipMalloc(&F->d_fsub, cap * sizeof(int32_t)));
-        HIP_TRY(hipMalloc(&F->d_anchor, cap * sizeof(int64_t)));
-        HIP_TRY(hipMalloc(&F->d_next, cap * sizeof(unsigned long long)));
-        HIP_TRY(hipMalloc(&F->d_flag, cap * sizeof(int32_t)));
-        size_t bytes = 0;
-        HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, F->d_fin, F->d_flag, F->d_stage_sec, F->d_nsel,
-                                              static_cast<int>(cap), s));
-        HIP_TRY(hipMalloc(&F->d_sel_tmp, bytes));
-        F->sel_tmp_bytes = bytes;
-        F->fifo_cap = cap;
-    }
-    if (!F->d_nsel) HIP_TRY(hipMalloc(&F->d_nsel, sizeof(int64_t)));
-    if (!F->d_nargs) HIP_TRY(hipMalloc(&F->d_nargs, sizeof(NewbornArgs)));
-    int64_t const n0 = io->n_initial;
-    fill_u64_kernel<<<dim3(static_cast<unsigned>(blocks_for(n0, 256))), dim3(256), 0, s>>>(F->d_next, n0, 0ULL);
-    HIP_TRY(hipGetLastError());
-    fifo_init_kernel<<<dim3(static_cast<unsigned>(blocks_for(n0, 256))), dim3(256), 0, s>>>(
-        io->work, n0, F->d_anchor, F->d_next, io->error);
-    HIP_TRY(hipGetLastError());
-    // One launch per generation; the host reads the child count in between.
-    // The generation kernel advances nothing once an error is recorded (a
-    // rejected birth payload included).
-    int32_t err_host[2] = {0, 0};
-    int64_t gb = 0, ge = n0;
-    while (gb < ge) {
-        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(F->d_count), 1, 0ULL);
-        HIP_TRY(hipGetLastError());
-        A.sec = F->d_stage_sec;
-        A.sec_cap = cap - ge;
-        A.sec_count = F->d_count;
-        NewbornArgs N{A, io->work, gb, ge, F->d_fin, F->d_fst, F->d_fion, F->d_fsub};
-        // pageable source: the copy has consumed N when hipMemcpyAsync returns
-        HIP_TRY(hipMemcpyAsync(F->d_nargs, &N, sizeof(NewbornArgs), hipMemcpyHostToDevice, s));
-        {
-            hipEvent_t stop = nullptr;
-            if (F->timing) {
-                if (F->events_used == F->events.size()) {
-                    hipEvent_t a, b;
-                    HIP_TRY(hipEventCreate(&a));
-                    HIP_TRY(hipEventCreate(&b));
-                    F->events.push_back({a, b, 0});
-                }
-                rrea_fields_s::Timed& t = F->events[F->events_used++];
-                t.kernel = 6;
-                HIP_TRY(hipEventRecord(t.start, s));
-                stop = t.stop;
-            }
-            newborn_kernel<<<dim3(static_cast<unsigned>(blocks_for(ge - gb, kBlock))), dim3(kBlock), 0, s>>>(
-                static_cast<const NewbornArgs*>(F->d_nargs));
-            HIP_TRY(hipGetLastError());
-            if (stop) HIP_TRY(hipEventRecord(stop, s));
-        }
-        int64_t c = 0;
-        HIP_TRY(hipMemcpyAsync(&c, F->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(err_host, io->error, sizeof(err_host), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (err_host[0] != 0 || c <= 0) break;
-        if (c > cap - ge) break;  // the kernel reported the capacity error
-        {
-            rrea_status const os = order_staged(F, c, io->work + ge, s);
-            if (os != RREA_OK) return os;
-        }
-        unsigned const cb = static_cast<unsigned>(blocks_for(c, 256));
-        fifo_anchor_kernel<<<dim3(cb), dim3(256), 0, s>>>(io->work, ge, ge + c, F->d_anchor);
-        HIP_TRY(hipGetLastError());
-        fifo_lineage_kernel<<<dim3(cb), dim3(256), 0, s>>>(io->work, ge, ge + c, F->d_anchor, F->d_next, io->error);
-        HIP_TRY(hipGetLastError());
-        fifo_advance_kernel<<<dim3(cb), dim3(256), 0, s>>>(ge, ge + c, F->d_anchor, F->d_next);
-        HIP_TRY(hipGetLastError());
-        gb = ge;
-        ge += c;
-    }
-    int64_t const total = ge;
-    set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->work_count, total);
-    HIP_TRY(hipGetLastError());
-    if (err_host[0] != 0) {
-        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->survivor_count, 0);
-        HIP_TRY(hipGetLastError());
-        return RREA_OK;
-    }
-    survivor_flag_kernel<<<dim3(static_cast<unsigned>(blocks_for(total, 256))), dim3(256), 0, s>>>(F->d_fst, total,
-                                                                                                  F->d_flag);
-    HIP_TRY(hipGetLastError());
-    size_t bytes = F->sel_tmp_bytes;
-    if (io->survivor_capacity >= total) {
-        // every entry fits: select straight into the caller's buffer, no round trip
-        HIP_TRY(hipcub::DeviceSelect::Flagged(F->d_sel_tmp, bytes, F->d_fin, F->d_flag, io->survivors,
-                                              io->survivor_count, static_cast<int>(total), s));
-    } else {
-        int64_t nsurv = 0;
-        HIP_TRY(hipcub::DeviceSelect::Flagged(F->d_sel_tmp, bytes, F->d_fin, F->d_flag, F->d_stage_sec, F->d_nsel,
-                                              static_cast<int>(total), s));
-        HIP_TRY(hipMemcpyAsync(&nsurv, F->d_nsel, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (nsurv > io->survivor_capacity) {
-            report_kernel<<<dim3(1), dim3(64), 0, s>>>(io->error, kErrCapacity, -1);
-            HIP_TRY(hipGetLastError());
-            nsurv = io->survivor_capacity;
-        }
-        if (nsurv > 0) {
-            HIP_TRY(hipMemcpyAsync(io->survivors, F->d_stage_sec, nsurv * sizeof(rrea_secondary),
-                                   hipMemcpyDeviceToDevice, s));
-        }
-        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->survivor_count, nsurv);
-        HIP_TRY(hipGetLastError());
-    }
-    if (io->status) HIP_TRY(hipMemcpyAsync(io->status, F->d_fst, total, hipMemcpyDeviceToDevice, s));
-    if (io->ion_pairs) {
-        HIP_TRY(hipMemcpyAsync(io->ion_pairs, F->d_fion, total * sizeof(double), hipMemcpyDeviceToDevice, s));
-    }
-    if (io->substeps) {
-        HIP_TRY(hipMemcpyAsync(io->substeps, F->d_fsub, total * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
-    }
-    return RREA_OK;
+    return drain_impl(T, F, cfg, io, stream, nullptr);
 }
 
 rrea_status rrea_tile_sort(rrea_fields_t F, const rrea_tile* tile, int32_t photon, rrea_stream_t stream)

@@ -40,6 +40,7 @@ SECONDARY_DTYPE = np.dtype([
     ("dir_x", "<f8"), ("dir_y", "<f8"), ("dir_z", "<f8"),
     ("weight", "<f8"), ("energy_eV", "<f8"), ("birth_fraction", "<f8"),
     ("root_key", "<u8"), ("lineage", "<u8"), ("root_species", "<i4"), ("generation", "<i4"),
+    ("instance", "<i4"), ("reserved", "<i4"),
 ])
 assert SECONDARY_DTYPE.itemsize == C.sizeof(_lib.Secondary)
 
@@ -369,6 +370,73 @@ class DrainOutputs:
         n = int(self.survivor_count.item())
         raw = self.survivors[: n * SECONDARY_DTYPE.itemsize].cpu().numpy()
         return np.frombuffer(raw.tobytes(), dtype=SECONDARY_DTYPE, count=n)
+
+
+class Batch:
+    """Independent simulation instances stepped together (rrea_batch_*): one
+    Coupling (fields, density), one J grid (a StepOutputs) and one RNG seed
+    per instance.  A batched tile is a DeviceTile holding every instance's
+    particles plus ``inst``, each particle's instance index (int32)."""
+
+    SPECIES = {"electron": 0, "positron": 1, "photon": 2}
+
+    def __init__(self, couplings, grids, seeds):
+        n = len(couplings)
+        if not (n == len(grids) == len(seeds)) or n == 0:
+            raise ValueError("one coupling, grid and seed per instance")
+        self.couplings, self.grids = list(couplings), list(grids)
+        arr = lambda vals: (C.c_void_p * n)(*vals)  # noqa: E731
+        fh = arr([c.h.value if isinstance(c.h, C.c_void_p) else c.h for c in couplings])
+        jr = arr([g.jr.data_ptr() for g in grids])
+        jz = arr([g.jz.data_ptr() for g in grids])
+        rho = arr([g.rho.data_ptr() for g in grids])
+        sd = (C.c_uint64 * n)(*[int(x) for x in seeds])
+        h = C.c_void_p()
+        _lib.check(_lib.lib().rrea_batch_create(fh, n, jr, jz, rho, sd, C.byref(h)), "rrea_batch_create")
+        self.h = h
+        w = C.c_void_p()
+        _lib.check(_lib.lib().rrea_batch_workspace(self.h, C.byref(w)), "rrea_batch_workspace")
+        self.workspace = w
+
+    def step(self, tables: InteractionTables, species: str, tile: DeviceTile, inst: torch.Tensor, cfg: StepConfig,
+             out: StepOutputs, order_secondaries: bool = True, stream: int | None = None, check: bool = True):
+        """One batched step of one species over the instances' particles (in place)."""
+        if inst.dtype != torch.int32 or inst.numel() != len(tile):
+            raise ValueError("inst must be int32, one per particle")
+        t = tile.struct()
+        c = _step_config(cfg)
+        o = out.struct(order_secondaries)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().rrea_transport_step_batched(tables.h, self.h, self.SPECIES[species], C.byref(t),
+                                                          inst.data_ptr(), C.byref(c), C.byref(o), stream),
+                   "rrea_transport_step_batched")
+        if check:
+            out.check_error()
+
+    def drain(self, tables: InteractionTables, cfg: StepConfig, staged, out: "DrainOutputs",
+              stream: int | None = None, check: bool = True):
+        """The newborn drain of a batched step: every record advances in its instance."""
+        n0 = out.stage(staged)
+        io = out.struct(n0)
+        c = _step_config(cfg)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().rrea_newborn_drain_batched(tables.h, self.h, C.byref(c), C.byref(io), stream),
+                   "rrea_newborn_drain_batched")
+        if check:
+            out.grid.check_error()
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().rrea_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _step_config(cfg: StepConfig) -> _lib.StepConfigC:

@@ -104,6 +104,36 @@ def c3_instances(n_instances: int = 256, seed: int = 3):
     return e_mag, rho, n0
 
 
+def c3_batch(n_instances: int = 256, seed: int = 3, scale: float = 1.0, nr: int = 64, nz: int = 256,
+             cell_m: float = 1.0):
+    """The batched configuration: per instance a uniform ambient E_z = -|E|
+    (the force on e- along +z) over air of constant density rho on its own
+    nr x nz RZ grid of cell_m cells, and N0 * scale seed electrons (at least
+    one) with K ~ U[1, 10] MeV in a 30-degree cone about +z, started in a
+    5 m disk at z in [1, 5] m.  Returns (fields list, tiles list, seeds)."""
+    e_mag, rho, n0 = c3_instances(n_instances, seed)
+    rng = np.random.default_rng(seed + 1)
+    fields, tiles, seeds = [], [], []
+    next_id = 1
+    for i in range(n_instances):
+        f = RzFields.uniform(nr, nz, cell_m, cell_m, z_lo=0.0, ez_v_per_m=-float(e_mag[i]),
+                             density_ratio=float(rho[i]) / REFERENCE_DENSITY)
+        n = max(1, int(round(n0[i] * scale)))
+        k = rng.uniform(1.0e6, 1.0e7, n)
+        dx, dy, dz = _cone_directions(rng, n, 1.0, np.deg2rad(30.0))
+        u = proper_velocity(k)
+        r = 5.0 * np.sqrt(rng.random(n))
+        z = 1.0 + 4.0 * rng.random(n)
+        theta = 2.0 * np.pi * rng.random(n) - np.pi
+        tiles.append(ParticleTile(r=r, z=z, theta=theta, ux=u * dx, uy=u * dy, uz=u * dz, w=np.ones(n),
+                                  id=np.arange(next_id, next_id + n, dtype=np.int64),
+                                  cpu=np.zeros(n, dtype=np.int32)))
+        next_id += n
+        fields.append(f)
+        seeds.append(20240623 + 7919 * i)
+    return fields, tiles, seeds
+
+
 def push_photons(r, theta, z, ux, uy, uz, dt_s):
     """Straight-line photon push at c along u / |u| in Cartesian, back to
     (r, theta, z): the pushed position the photon step reads as its chord end
