@@ -106,6 +106,122 @@ def cpu_baseline(n_particles: int, steps: int, threads: int):
     return done / dt, dt, done
 
 
+def batched_leg(args, tables):
+    """The batched configuration on the same device: 256 independent
+    avalanche instances (|E| ~ U[250, 1000] kV/m, rho ~ U[0.3, 1.225] kg/m^3,
+    N0 ~ log-U[1e3, 1e6] electrons at K ~ U[1, 10] MeV, 64 x 256 RZ grids of
+    1 m cells), dt = 1 ps, each step the electron step of every instance then
+    the newborn drain of its secondaries, J deposited per instance.  Timed
+    twice on the same inputs: the batched path (rrea_transport_step_batched /
+    rrea_newborn_drain_batched: one launch set for all instances) and the
+    plain per-instance path (the drop-in Apply run instance after instance,
+    one launch set and drain each).  Returns the report object."""
+    import ctypes as C
+
+    import torch
+
+    from rrea_feedback_warpx_amrex_geant4_amd import _lib
+    from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
+    from rrea_feedback_warpx_amrex_geant4_amd.config import StepConfig
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import (Batch, Coupling, DeviceTile, DrainOutputs,
+                                                                InstanceGrids, RreaParticleInteraction, StepOutputs)
+
+    keys = ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")
+    n_inst = args.batched_instances
+    fields, tiles, seeds = wl.c3_batch(n_inst, seed=3, scale=args.batched_scale)
+    sizes = np.array([len(t.r) for t in tiles], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(offs[-1])
+    log(f"batched: {n_inst} instances, {n} electrons (smallest {sizes.min()}, largest {sizes.max()})")
+    dev_b = DeviceTile.from_host(type(tiles[0])(*(np.concatenate([getattr(t, k) for t in tiles]) for k in keys)))
+    del tiles
+    dev_p = dev_b.clone()
+    inst = torch.as_tensor(np.repeat(np.arange(n_inst, dtype=np.int32), sizes), device="cuda")
+    couplings = [Coupling(f, tables) for f in fields]
+    grids_b, grids_p = InstanceGrids(fields), InstanceGrids(fields)
+    batch = Batch(couplings, grids_b.grids, seeds)
+    out_b = StepOutputs(n, fields[0], secondary_capacity=max(1 << 14, n >> 6), per_particle=False)
+    drain_b = DrainOutputs(max(1 << 15, n >> 5), out_b)
+    # the plain path: instance i's tile is a view of its rows, its J grid its own
+    plain_tiles = [DeviceTile(*(getattr(dev_p, k)[offs[i]:offs[i + 1]] for k in keys)) for i in range(n_inst)]
+    counts = torch.zeros(n_inst, dtype=torch.int64, device="cuda")
+    plain_outs, plain_drains = [], []
+    for i, f in enumerate(fields):
+        g = grids_p.grids[i]
+        o = StepOutputs(int(sizes[i]), f, secondary_capacity=max(1 << 10, int(sizes[i]) >> 6), per_particle=False,
+                        grid_buffers=(g.jr, g.jz, g.rho))
+        o.secondary_count = counts[i:i + 1]
+        plain_outs.append(o)
+        plain_drains.append(DrainOutputs(max(1 << 11, int(sizes[i]) >> 5), o))
+    apply = RreaParticleInteraction
+    dt = 1e-12
+
+    def step_batched(s):
+        out_b.secondary_count.zero_()
+        grids_b.zero_()
+        cfg = StepConfig(rng_seed=0, step=s, dt_s=dt)
+        batch.step(tables, "electron", dev_b, inst, cfg, out_b, check=False)
+        batch.drain(tables, cfg, [out_b], drain_b, check=False)
+
+    def step_plain(s):
+        counts.zero_()
+        grids_p.zero_()
+        for i in range(n_inst):
+            cfg = StepConfig(rng_seed=seeds[i], step=s, dt_s=dt)
+            apply.apply_electrons(tables, couplings[i], plain_tiles[i], cfg, plain_outs[i], order_secondaries=True,
+                                  check=False)
+            apply.drain_secondaries(tables, couplings[i], cfg, [plain_outs[i]], plain_drains[i], check=False)
+
+    def live(t):
+        return int((t.id != _lib.INVALID_ID).sum().item())
+
+    def timed(run, tile):
+        for s in range(args.batched_warmup):
+            run(s)
+        torch.cuda.synchronize()
+        l0 = live(tile)
+        t0 = time.perf_counter()
+        for s in range(args.batched_steps):
+            run(args.batched_warmup + s)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        l1 = live(tile)
+        return 0.5 * (l0 + l1) * args.batched_steps / el, el, l0, l1
+
+    L = _lib.lib()
+    L.rrea_kernel_timing(batch.workspace, 1)
+    rate_b, el_b, b0, b1 = timed(step_batched, dev_b)
+    ms = C.c_double(0.0)
+    cnt = C.c_int64(0)
+    _lib.check(L.rrea_kernel_timing_read(batch.workspace, 0, C.byref(ms), C.byref(cnt)), "timing read")
+    L.rrea_kernel_timing(batch.workspace, 0)
+    out_b.check_error()
+    mean_launch_s = ms.value * 1e-3 / max(cnt.value, 1)
+    achieved = 0.5 * (b0 + b1) * B_ALG / mean_launch_s
+    log(f"batched: {args.batched_steps} steps in {el_b:.3f} s, {rate_b:.4e} particle-steps/s; electron kernel "
+        f"{mean_launch_s * 1e3:.3f} ms/launch")
+    rate_p, el_p, p0, p1 = timed(step_plain, dev_p)
+    for o in plain_outs:
+        o.check_error()
+    log(f"plain per instance: {args.batched_steps} steps in {el_p:.3f} s, {rate_p:.4e} particle-steps/s")
+    # the two paths must have advanced the same particles to the same state
+    same = all(torch.equal(getattr(dev_b, k), getattr(dev_p, k)) for k in keys)
+    batch.close()
+    return {
+        "workload": f"{n_inst} independent instances, {n} electrons (N0 log-uniform 1e3..1e6 x {args.batched_scale}),"
+                    " |E| U[250,1000] kV/m, rho U[0.3,1.225] kg/m^3, 64x256 RZ grids, dt=1 ps, electron step + "
+                    "newborn drain per step, J per instance",
+        "instances": n_inst, "electrons": n, "steps": args.batched_steps, "warmup": args.batched_warmup,
+        "batched": {"value": rate_b, "ms_per_step": el_b / args.batched_steps * 1e3,
+                    "electron_kernel_ms": mean_launch_s * 1e3, "hbm_frac": achieved / HBM_PEAK,
+                    "achieved_GBps": achieved / 1e9},
+        "plain_per_instance": {"value": rate_p, "ms_per_step": el_p / args.batched_steps * 1e3},
+        "ratio": rate_b / rate_p,
+        "unit": "particle-steps/s",
+        "states_identical": bool(same),
+    }
+
+
 def traffic_record():
     """Measured HBM bytes per launch of the electron kernel, or None."""
     try:
@@ -127,6 +243,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
     ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--batched-instances", type=int, default=256, help="instances of the batched leg (0: skip it)")
+    ap.add_argument("--batched-scale", type=float, default=1.0, help="scale of the batched leg's populations")
+    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--batched-warmup", type=int, default=2)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run only a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -233,6 +353,8 @@ def main():
         ", ".join(f"{k} {v[0]:.2f} ms / {v[1]}" for k, v in kms.items()))
     traffic, rec = traffic_record()
 
+    batched = batched_leg(args, tables) if args.batched_instances > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -267,6 +389,7 @@ def main():
                      "kernel": "electron_fast_kernel", "alg_bytes_per_launch": alg_bytes,
                      "mean_launch_ms": mean_launch_s * 1e3},
         "cpu_baseline": cpu,
+        "batched_vs_plain": batched,
     }
     print(json.dumps(line), flush=True)
 

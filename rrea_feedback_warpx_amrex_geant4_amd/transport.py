@@ -267,10 +267,11 @@ class StepOutputs:
     """Device outputs of one step (allocated once, reused)."""
 
     def __init__(self, n: int, fields: RzFields, secondary_capacity: int, device="cuda", per_particle: bool = True,
-                 share_grid: "StepOutputs | None" = None):
+                 share_grid: "StepOutputs | None" = None, grid_buffers: "tuple | None" = None):
         """``share_grid``: deposit into another output's J grid and tallies, as
         every species of one step deposits into the coupling's one net-chord
-        grid (RreaWarpXCoupling.cpp:417-444)."""
+        grid (RreaWarpXCoupling.cpp:417-444).  ``grid_buffers``: (jr, jz, rho)
+        device views to deposit into instead of grids of its own."""
         nrp, nzp = fields.padded_shape()
         self.status = torch.zeros(n, dtype=torch.uint8, device=device) if per_particle else None
         self.ion_pairs = torch.zeros(n, dtype=torch.float64, device=device) if per_particle else None
@@ -282,6 +283,12 @@ class StepOutputs:
         if share_grid is not None:
             self.jr, self.jz, self.rho = share_grid.jr, share_grid.jz, share_grid.rho
             self.tallies, self.error = share_grid.tallies, share_grid.error
+        elif grid_buffers is not None:
+            self.jr, self.jz, self.rho = grid_buffers
+            if (self.jr.numel(), self.jz.numel(), self.rho.numel()) != ((nrp + 1) * nzp, nrp * (nzp + 1), nrp * nzp):
+                raise ValueError("grid buffers do not match the padded grid")
+            self.tallies = torch.zeros(_lib.TALLY_COUNT, dtype=torch.float64, device=device)
+            self.error = torch.zeros(2, dtype=torch.int32, device=device)
         else:
             self.jr = torch.zeros((nrp + 1) * nzp, dtype=torch.float64, device=device)
             self.jz = torch.zeros(nrp * (nzp + 1), dtype=torch.float64, device=device)
@@ -370,6 +377,29 @@ class DrainOutputs:
         n = int(self.survivor_count.item())
         raw = self.survivors[: n * SECONDARY_DTYPE.itemsize].cpu().numpy()
         return np.frombuffer(raw.tobytes(), dtype=SECONDARY_DTYPE, count=n)
+
+
+class InstanceGrids:
+    """The J grids of many instances in three contiguous device buffers, so
+    one fill clears them all; ``grids[i]`` is instance i's StepOutputs (no
+    per-particle outputs) over views of them."""
+
+    def __init__(self, fields_list, device="cuda"):
+        shapes = [f.padded_shape() for f in fields_list]
+        sizes = [((a + 1) * b, a * (b + 1), a * b) for a, b in shapes]
+        tot = [sum(s[k] for s in sizes) for k in range(3)]
+        self.jr, self.jz, self.rho = (torch.zeros(t, dtype=torch.float64, device=device) for t in tot)
+        self.grids = []
+        at = [0, 0, 0]
+        for f, sz in zip(fields_list, sizes):
+            views = tuple(buf[at[k]:at[k] + sz[k]] for k, buf in enumerate((self.jr, self.jz, self.rho)))
+            self.grids.append(StepOutputs(0, f, secondary_capacity=0, device=device, per_particle=False,
+                                          grid_buffers=views))
+            at = [at[k] + sz[k] for k in range(3)]
+
+    def zero_(self):
+        for t in (self.jr, self.jz, self.rho):
+            t.zero_()
 
 
 class Batch:
