@@ -370,6 +370,45 @@ rrea_status rrea_transport_step_batched(rrea_tables_t tables, rrea_batch_t batch
 rrea_status rrea_newborn_drain_batched(rrea_tables_t tables, rrea_batch_t batch, const rrea_step_config* config,
                                        const rrea_drain_io* io, rrea_stream_t stream);
 
+/* Device glue on either side of the hook (the WarpX calls the reference
+ * makes around Apply, done here without leaving HBM). */
+
+/* One species' particle store: tile arrays with room for `capacity`
+ * particles (tile.n is not used; charged stores may alias prev_* and r..uz,
+ * photon stores need no prev_u*), and, for a batched store, each particle's
+ * instance (NULL otherwise). */
+typedef struct rrea_particle_store {
+    rrea_tile tile;
+    int32_t* instance;
+    int64_t capacity;
+} rrea_particle_store;
+
+/* Commit of the drain's survivors: CreateLocal{KineticElectrons,Photons,
+ * KineticPositrons} -> CommitLocalParticleBatch -> AddNParticles
+ * (RreaSecondaryCommit.cpp:594-629, RreaCouplingSources.cpp:239-370).
+ * stores[0..2] are the electron, positron and photon stores (the species
+ * codes of rrea_secondary).  Each species' survivors, in FIFO order, are
+ * appended behind counts[species] (device int64[3], updated); a survivor of
+ * non-positive weight, or a photon of non-positive energy, is skipped.
+ * Charged momentum u = u(K) dir/|dir| (+z for a zero direction; K <= 0 keeps
+ * the raw direction), photon u = E dir/|dir|; r = hypot(x, y), theta =
+ * atan2(y, x); previous state = the new state.  Ids are consecutive from
+ * *next_id (device int64, advanced) in the reference's commit order:
+ * electrons, then photons, then positrons; cpu is the caller's rank.  If a
+ * store would overflow, nothing is written and error[0] = 5 (capacity). */
+rrea_status rrea_commit_survivors(const rrea_secondary* survivors, const int64_t* survivor_count,
+                                  const rrea_particle_store* stores, int64_t* counts, int64_t* next_id,
+                                  int32_t cpu, int32_t* error, rrea_stream_t stream);
+
+/* deleteInvalidParticles (RreaSecondaryCommit.cpp:578-592): a stable
+ * partition of the tile, live particles (id != RREA_INVALID_ID) first in
+ * their order, invalidated ones behind; `instance` (may be NULL) is permuted
+ * with the tile.  *n_live (host) receives the live count; the call
+ * synchronises its stream once to return it.  Transport results do not
+ * depend on it: steps skip invalidated particles wherever they are. */
+rrea_status rrea_tile_compact(rrea_fields_t fields, const rrea_tile* tile, int32_t photon, int32_t* instance,
+                              int64_t* n_live, rrea_stream_t stream);
+
 /* Kernel timing with HIP events recorded around every transport-kernel
  * launch made through `fields` while enabled (no counterpart in the reference:
  * measurement support).  Enabling or disabling clears the record.  _read

@@ -99,6 +99,10 @@ class Secondary(C.Structure):
     ]
 
 
+class ParticleStore(C.Structure):
+    _fields_ = [("tile", Tile), ("instance", C.c_void_p), ("capacity", C.c_int64)]
+
+
 class StepOut(C.Structure):
     _fields_ = [
         ("status", C.c_void_p), ("ion_pairs", C.c_void_p), ("substeps", C.c_void_p),
@@ -154,6 +158,9 @@ SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.POINTER(StepConfigC), C.POINTER(DrainIO), C.c_void_p]),
     ("rrea_newborn_drain", C.c_int32,
      [C.c_void_p, C.c_void_p, C.POINTER(StepConfigC), C.POINTER(DrainIO), C.c_void_p]),
+    ("rrea_commit_survivors", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(ParticleStore), C.c_void_p, C.c_void_p,
+                                          C.c_int32, C.c_void_p, C.c_void_p]),
+    ("rrea_tile_compact", C.c_int32, [C.c_void_p, C.POINTER(Tile), C.c_int32, C.c_void_p, P_I64, C.c_void_p]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),

@@ -203,6 +203,7 @@ struct rrea_fields_s {
     size_t tsort_tmp_bytes = 0;
     void* d_tbuf = nullptr;
     int64_t tsort_cap = 0;
+    int64_t* d_live = nullptr;       // rrea_tile_compact's live count
     // newborn FIFO workspace
     void* d_nargs = nullptr;
     rrea_secondary* d_fin = nullptr;
@@ -2749,6 +2750,145 @@ __global__ void tile_key_kernel(int64_t n, const double* __restrict__ r, const d
     keys[p] = (cell << ebits) | eb;
 }
 
+// rrea_tile_compact's key: live particles (0) before invalidated ones (1),
+// and the live count (wave-aggregated).
+__global__ void live_key_kernel(int64_t n, const int64_t* __restrict__ id, uint32_t* __restrict__ keys,
+                                int32_t* __restrict__ idx, unsigned long long* __restrict__ live)
+{
+    int64_t const p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool const ok = p < n && id[p] != RREA_INVALID_ID;
+    if (p < n) {
+        idx[p] = static_cast<int32_t>(p);
+        keys[p] = ok ? 0U : 1U;
+    }
+    uint64_t const m = __ballot(ok);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(live, static_cast<unsigned long long>(__popcll(m)));
+}
+
+// One species' particle store for rrea_commit_survivors (device arrays with
+// room for cap particles; prev_u* null for photons; inst null for a plain run).
+struct CommitStore {
+    double *prev_r, *prev_z, *prev_th, *prev_ux, *prev_uy, *prev_uz;
+    double *r, *z, *th, *ux, *uy, *uz, *w;
+    int64_t* id;
+    int32_t* cpu;
+    int32_t* inst;
+    int64_t cap;
+};
+struct CommitStores { CommitStore s[3]; };
+
+// The species slot a survivor commits to, or -1 when CommitLocalParticleBatch
+// skips it (RreaCouplingSources.cpp:280-283: non-positive weight; a photon
+// with non-positive energy).
+__device__ __forceinline__ int32_t commit_slot(rrea_secondary const& q)
+{
+    if (q.species < 0 || q.species > 2) return -1;
+    if (!(q.weight > 0.0)) return -1;
+    if (q.species == 2 && !(q.energy_eV > 0.0)) return -1;
+    return q.species;
+}
+
+// AddNParticles of the drain's survivors, one workgroup: each species'
+// accepted survivors are appended behind its count in FIFO order, and ids
+// are taken from *next_id in the order the reference commits the batches --
+// electrons, then photons, then positrons (RreaSecondaryCommit.cpp:594-629).
+// Momentum and position as CommitLocalParticleBatch and
+// append_rrea_previous_position_attributes form them
+// (RreaCouplingSources.cpp:284-307, 68-74).  Nothing is written when a store
+// would overflow.
+__global__ void __launch_bounds__(1024) commit_kernel(const rrea_secondary* __restrict__ sv,
+                                                      const int64_t* __restrict__ sv_count, CommitStores S,
+                                                      int64_t* __restrict__ counts, int64_t* __restrict__ next_id,
+                                                      int32_t cpu, int32_t* __restrict__ error)
+{
+    __shared__ unsigned long long tot[3];
+    __shared__ int32_t wave_n[16][3];
+    __shared__ int64_t running[3];
+    __shared__ int32_t abort_flag;
+    int32_t const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int64_t const n = *sv_count;
+    if (tid < 3) { tot[tid] = 0ULL; running[tid] = 0; }
+    if (tid == 0) abort_flag = 0;
+    __syncthreads();
+    {
+        unsigned long long c[3] = {0ULL, 0ULL, 0ULL};
+        for (int64_t i = tid; i < n; i += 1024) {
+            int32_t const q = commit_slot(sv[i]);
+            if (q >= 0) c[q] += 1ULL;
+        }
+        for (int q = 0; q < 3; ++q) if (c[q]) atomicAdd(&tot[q], c[q]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int q = 0; q < 3; ++q) {
+            if (counts[q] + static_cast<int64_t>(tot[q]) > S.s[q].cap) abort_flag = 1;
+        }
+        if (abort_flag && atomicCAS(&error[0], 0, kErrCapacity) == 0) error[1] = -1;
+    }
+    __syncthreads();
+    if (abort_flag) return;
+    int64_t const id0 = *next_id;
+    // commit order electrons (0), photons (2), positrons (1)
+    int64_t const id_base[3] = {id0, id0 + static_cast<int64_t>(tot[0] + tot[2]), id0 + static_cast<int64_t>(tot[0])};
+    for (int64_t start = 0; start < n; start += 1024) {
+        int64_t const i = start + tid;
+        int32_t const q = i < n ? commit_slot(sv[i]) : -1;
+        uint64_t const below = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+        int32_t rank_in_wave = 0;
+        for (int k = 0; k < 3; ++k) {
+            uint64_t const m = __ballot(q == k);
+            if (q == k) rank_in_wave = __popcll(m & below);
+            if (lane == 0) wave_n[wave][k] = __popcll(m);
+        }
+        __syncthreads();
+        if (q >= 0) {
+            int64_t rank = rank_in_wave;
+            for (int w = 0; w < wave; ++w) rank += wave_n[w][q];
+            CommitStore const& D = S.s[q];
+            int64_t const slot = counts[q] + running[q] + rank;
+            rrea_secondary const& t = sv[i];
+            double const px = t.dir_x, py = t.dir_y, pz = t.dir_z;
+            double const norm = sqrt(px * px + py * py + pz * pz);
+            double ux = px, uy = py, uz = pz;
+            if (q == 2) {
+                double const scale = norm > 0.0 ? t.energy_eV / norm : t.energy_eV;
+                ux = px * scale;
+                uy = py * scale;
+                uz = norm > 0.0 ? pz * scale : t.energy_eV;
+            } else if (t.energy_eV > 0.0) {
+                double const um = proper_velocity(t.energy_eV);
+                if (norm > 0.0) {
+                    ux = um * px / norm;
+                    uy = um * py / norm;
+                    uz = um * pz / norm;
+                } else {
+                    ux = 0.0;
+                    uy = 0.0;
+                    uz = um;
+                }
+            }
+            double const r = hypot(t.x, t.y), th = atan2(t.y, t.x);
+            D.prev_r[slot] = r; D.prev_z[slot] = t.z; D.prev_th[slot] = th;
+            if (D.prev_ux) { D.prev_ux[slot] = ux; D.prev_uy[slot] = uy; D.prev_uz[slot] = uz; }
+            D.r[slot] = r; D.z[slot] = t.z; D.th[slot] = th;
+            D.ux[slot] = ux; D.uy[slot] = uy; D.uz[slot] = uz;
+            D.w[slot] = t.weight;
+            D.id[slot] = id_base[q] + running[q] + rank;
+            if (D.cpu) D.cpu[slot] = cpu;
+            if (D.inst) D.inst[slot] = t.instance;
+        }
+        __syncthreads();
+        if (tid < 3) {
+            int64_t add = 0;
+            for (int w = 0; w < 16; ++w) add += wave_n[w][tid];
+            running[tid] += add;
+        }
+        __syncthreads();
+    }
+    if (tid < 3) counts[tid] += static_cast<int64_t>(tot[tid]);
+    if (tid == 0) *next_id = id0 + static_cast<int64_t>(tot[0] + tot[1] + tot[2]);
+}
+
 template <class V>
 __global__ void gather_kernel(int64_t n, const int32_t* __restrict__ idx, const V* __restrict__ src, V* __restrict__ dst)
 {
@@ -3040,7 +3180,8 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
                     F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args,
                     F->d_slow_list, F->d_slow_count, F->d_prof_guide, F->d_nargs, F->d_fin, F->d_fst,
                     F->d_fion, F->d_fsub, F->d_anchor, F->d_next, F->d_flag, F->d_nsel, F->d_sel_tmp,
-                    F->d_tkeys, F->d_tkeys_out, F->d_tidx, F->d_tidx_out, F->d_tsort_tmp, F->d_tbuf, F->d_base};
+                    F->d_tkeys, F->d_tkeys_out, F->d_tidx, F->d_tidx_out, F->d_tsort_tmp, F->d_tbuf, F->d_base,
+                    F->d_live};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : F->events) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
     delete F;
@@ -3639,6 +3780,74 @@ rrea_status drain_impl(rrea_tables_t T, rrea_fields_t F, const rrea_step_config*
     return RREA_OK;
 }
 
+// The tile ordering workspace for n particles (keys, permutation, sort
+// temporary, one staging array).
+rrea_status tile_sort_workspace(rrea_fields_t F, int64_t n)
+{
+    if (F->tsort_cap < n) {
+        void* ptrs[] = {F->d_tkeys, F->d_tkeys_out, F->d_tidx, F->d_tidx_out, F->d_tsort_tmp, F->d_tbuf};
+        for (void* p : ptrs) if (p) (void)hipFree(p);
+        F->d_tkeys = nullptr; F->d_tkeys_out = nullptr; F->d_tidx = nullptr; F->d_tidx_out = nullptr;
+        F->d_tsort_tmp = nullptr; F->d_tbuf = nullptr;
+        F->tsort_cap = 0;
+        HIP_TRY(hipMalloc(&F->d_tkeys, n * sizeof(uint32_t)));
+        HIP_TRY(hipMalloc(&F->d_tkeys_out, n * sizeof(uint32_t)));
+        HIP_TRY(hipMalloc(&F->d_tidx, n * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&F->d_tidx_out, n * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&F->d_tbuf, n * sizeof(double)));
+        size_t bytes = 0;
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, F->d_tkeys, F->d_tkeys_out, F->d_tidx,
+                                                   F->d_tidx_out, static_cast<int>(n)));
+        HIP_TRY(hipMalloc(&F->d_tsort_tmp, bytes));
+        F->tsort_tmp_bytes = bytes;
+        F->tsort_cap = n;
+    }
+    return RREA_OK;
+}
+
+// Gather every distinct array of the tile (and the instance array, when
+// given) through F->d_tidx_out, the permutation a sort left there.
+rrea_status permute_tile(rrea_fields_t F, const rrea_tile* tile, int32_t photon, int32_t* instance, hipStream_t s)
+{
+    int64_t const n = tile->n;
+    unsigned const nb = static_cast<unsigned>(blocks_for(n, 256));
+    std::vector<double*> f64 = {const_cast<double*>(tile->prev_r), const_cast<double*>(tile->prev_z),
+                                const_cast<double*>(tile->prev_theta), const_cast<double*>(tile->w)};
+    if (!photon) {
+        f64.push_back(const_cast<double*>(tile->prev_ux));
+        f64.push_back(const_cast<double*>(tile->prev_uy));
+        f64.push_back(const_cast<double*>(tile->prev_uz));
+    }
+    for (double* p : {tile->r, tile->z, tile->theta, tile->ux, tile->uy, tile->uz}) {
+        if (std::find(f64.begin(), f64.end(), p) == f64.end()) f64.push_back(p);
+    }
+    double* const buf = static_cast<double*>(F->d_tbuf);
+    for (double* p : f64) {
+        gather_kernel<double><<<dim3(nb), dim3(256), 0, s>>>(n, F->d_tidx_out, p, buf);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(p, buf, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    {
+        int64_t* const ib = static_cast<int64_t*>(F->d_tbuf);
+        gather_kernel<int64_t><<<dim3(nb), dim3(256), 0, s>>>(n, F->d_tidx_out, tile->id, ib);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(tile->id, ib, n * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    }
+    if (tile->cpu) {
+        int32_t* const cb = static_cast<int32_t*>(F->d_tbuf);
+        gather_kernel<int32_t><<<dim3(nb), dim3(256), 0, s>>>(n, F->d_tidx_out, tile->cpu, cb);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(const_cast<int32_t*>(tile->cpu), cb, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
+    if (instance) {
+        int32_t* const ib = static_cast<int32_t*>(F->d_tbuf);
+        gather_kernel<int32_t><<<dim3(nb), dim3(256), 0, s>>>(n, F->d_tidx_out, instance, ib);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(instance, ib, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
+    return RREA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -3746,23 +3955,9 @@ rrea_status rrea_tile_sort(rrea_fields_t F, const rrea_tile* tile, int32_t photo
     if (cbits > 31) return RREA_ERR_ARGUMENT;
     int32_t const ebits = 32 - cbits < 6 ? 32 - cbits : 6;
     hipStream_t const s = static_cast<hipStream_t>(stream);
-    if (F->tsort_cap < n) {
-        void* ptrs[] = {F->d_tkeys, F->d_tkeys_out, F->d_tidx, F->d_tidx_out, F->d_tsort_tmp, F->d_tbuf};
-        for (void* p : ptrs) if (p) (void)hipFree(p);
-        F->d_tkeys = nullptr; F->d_tkeys_out = nullptr; F->d_tidx = nullptr; F->d_tidx_out = nullptr;
-        F->d_tsort_tmp = nullptr; F->d_tbuf = nullptr;
-        F->tsort_cap = 0;
-        HIP_TRY(hipMalloc(&F->d_tkeys, n * sizeof(uint32_t)));
-        HIP_TRY(hipMalloc(&F->d_tkeys_out, n * sizeof(uint32_t)));
-        HIP_TRY(hipMalloc(&F->d_tidx, n * sizeof(int32_t)));
-        HIP_TRY(hipMalloc(&F->d_tidx_out, n * sizeof(int32_t)));
-        HIP_TRY(hipMalloc(&F->d_tbuf, n * sizeof(double)));
-        size_t bytes = 0;
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, F->d_tkeys, F->d_tkeys_out, F->d_tidx,
-                                                   F->d_tidx_out, static_cast<int>(n)));
-        HIP_TRY(hipMalloc(&F->d_tsort_tmp, bytes));
-        F->tsort_tmp_bytes = bytes;
-        F->tsort_cap = n;
+    {
+        rrea_status const ws = tile_sort_workspace(F, n);
+        if (ws != RREA_OK) return ws;
     }
     // charged tiles key on the captured state; photons on their current point
     const double* ku_x = photon ? tile->ux : tile->prev_ux;
@@ -3776,35 +3971,83 @@ rrea_status rrea_tile_sort(rrea_fields_t F, const rrea_tile* tile, int32_t photo
     size_t bytes = F->tsort_tmp_bytes;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(F->d_tsort_tmp, bytes, F->d_tkeys, F->d_tkeys_out, F->d_tidx,
                                                F->d_tidx_out, static_cast<int>(n), 0, cbits + ebits, s));
-    // permute every distinct array of the tile through the staging buffer
-    std::vector<double*> f64 = {const_cast<double*>(tile->prev_r), const_cast<double*>(tile->prev_z),
-                                const_cast<double*>(tile->prev_theta), const_cast<double*>(tile->w)};
-    if (!photon) {
-        f64.push_back(const_cast<double*>(tile->prev_ux));
-        f64.push_back(const_cast<double*>(tile->prev_uy));
-        f64.push_back(const_cast<double*>(tile->prev_uz));
+    return permute_tile(F, tile, photon, nullptr, s);
+}
+
+rrea_status rrea_tile_compact(rrea_fields_t F, const rrea_tile* tile, int32_t photon, int32_t* instance,
+                              int64_t* n_live, rrea_stream_t stream)
+{
+    if (!F || !tile || tile->n < 0 || !n_live) return RREA_ERR_ARGUMENT;
+    int64_t const n = tile->n;
+    *n_live = 0;
+    if (n == 0) return RREA_OK;
+    if (n > INT32_MAX || !tile->prev_r || !tile->prev_z || !tile->prev_theta || !tile->w || !tile->id
+        || !tile->r || !tile->z || !tile->theta || !tile->ux || !tile->uy || !tile->uz
+        || (!photon && (!tile->prev_ux || !tile->prev_uy || !tile->prev_uz))) {
+        return RREA_ERR_ARGUMENT;
     }
-    for (double* p : {tile->r, tile->z, tile->theta, tile->ux, tile->uy, tile->uz}) {
-        if (std::find(f64.begin(), f64.end(), p) == f64.end()) f64.push_back(p);
-    }
-    double* const buf = static_cast<double*>(F->d_tbuf);
-    for (double* p : f64) {
-        gather_kernel<double><<<dim3(nb), dim3(256), 0, s>>>(n, F->d_tidx_out, p, buf);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(p, buf, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    }
+    hipStream_t const s = static_cast<hipStream_t>(stream);
     {
-        int64_t* const ib = static_cast<int64_t*>(F->d_tbuf);
-        gather_kernel<int64_t><<<dim3(nb), dim3(256), 0, s>>>(n, F->d_tidx_out, tile->id, ib);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(tile->id, ib, n * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+        rrea_status const ws = tile_sort_workspace(F, n);
+        if (ws != RREA_OK) return ws;
     }
-    if (tile->cpu) {
-        int32_t* const cb = static_cast<int32_t*>(F->d_tbuf);
-        gather_kernel<int32_t><<<dim3(nb), dim3(256), 0, s>>>(n, F->d_tidx_out, tile->cpu, cb);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(const_cast<int32_t*>(tile->cpu), cb, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    if (!F->d_live) HIP_TRY(hipMalloc(&F->d_live, sizeof(int64_t)));
+    fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(F->d_live), 1, 0ULL);
+    HIP_TRY(hipGetLastError());
+    unsigned const nb = static_cast<unsigned>(blocks_for(n, 256));
+    live_key_kernel<<<dim3(nb), dim3(256), 0, s>>>(n, tile->id, F->d_tkeys, F->d_tidx,
+                                                   reinterpret_cast<unsigned long long*>(F->d_live));
+    HIP_TRY(hipGetLastError());
+    size_t bytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, F->d_tkeys, F->d_tkeys_out, F->d_tidx,
+                                               F->d_tidx_out, static_cast<int>(n), 0, 1));
+    if (bytes > F->tsort_tmp_bytes) {
+        if (F->d_tsort_tmp) (void)hipFree(F->d_tsort_tmp);
+        F->d_tsort_tmp = nullptr;
+        F->tsort_tmp_bytes = 0;
+        HIP_TRY(hipMalloc(&F->d_tsort_tmp, bytes));
+        F->tsort_tmp_bytes = bytes;
     }
+    bytes = F->tsort_tmp_bytes;
+    // one key bit: a stable partition, live particles keep their order
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(F->d_tsort_tmp, bytes, F->d_tkeys, F->d_tkeys_out, F->d_tidx,
+                                               F->d_tidx_out, static_cast<int>(n), 0, 1, s));
+    rrea_status const ps = permute_tile(F, tile, photon, instance, s);
+    if (ps != RREA_OK) return ps;
+    HIP_TRY(hipMemcpyAsync(n_live, F->d_live, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return RREA_OK;
+}
+
+rrea_status rrea_commit_survivors(const rrea_secondary* survivors, const int64_t* survivor_count,
+                                  const rrea_particle_store* stores, int64_t* counts, int64_t* next_id, int32_t cpu,
+                                  int32_t* error, rrea_stream_t stream)
+{
+    if (!survivors || !survivor_count || !stores || !counts || !next_id || !error) return RREA_ERR_ARGUMENT;
+    CommitStores S{};
+    for (int q = 0; q < 3; ++q) {
+        rrea_particle_store const& P = stores[q];
+        rrea_tile const& t = P.tile;
+        bool const photon = q == 2;
+        if (P.capacity < 0) return RREA_ERR_ARGUMENT;
+        if (P.capacity > 0 && (!t.prev_r || !t.prev_z || !t.prev_theta || !t.w || !t.id || !t.r || !t.z || !t.theta
+                               || !t.ux || !t.uy || !t.uz || (!photon && (!t.prev_ux || !t.prev_uy || !t.prev_uz)))) {
+            return RREA_ERR_ARGUMENT;
+        }
+        CommitStore& D = S.s[q];
+        D.prev_r = const_cast<double*>(t.prev_r); D.prev_z = const_cast<double*>(t.prev_z);
+        D.prev_th = const_cast<double*>(t.prev_theta);
+        D.prev_ux = photon ? nullptr : const_cast<double*>(t.prev_ux);
+        D.prev_uy = photon ? nullptr : const_cast<double*>(t.prev_uy);
+        D.prev_uz = photon ? nullptr : const_cast<double*>(t.prev_uz);
+        D.r = t.r; D.z = t.z; D.th = t.theta; D.ux = t.ux; D.uy = t.uy; D.uz = t.uz;
+        D.w = const_cast<double*>(t.w); D.id = t.id; D.cpu = const_cast<int32_t*>(t.cpu);
+        D.inst = P.instance;
+        D.cap = P.capacity;
+    }
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    commit_kernel<<<dim3(1), dim3(1024), 0, s>>>(survivors, survivor_count, S, counts, next_id, cpu, error);
+    HIP_TRY(hipGetLastError());
     return RREA_OK;
 }
 

@@ -27,6 +27,8 @@ STRUCTS = {
     "rrea_tile": _lib.Tile,
     "rrea_secondary": _lib.Secondary,
     "rrea_step_out": _lib.StepOut,
+    "rrea_drain_io": _lib.DrainIO,
+    "rrea_particle_store": _lib.ParticleStore,
 }
 
 
