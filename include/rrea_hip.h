@@ -409,6 +409,43 @@ rrea_status rrea_commit_survivors(const rrea_secondary* survivors, const int64_t
 rrea_status rrea_tile_compact(rrea_fields_t fields, const rrea_tile* tile, int32_t photon, int32_t* instance,
                               int64_t* n_live, rrea_stream_t stream);
 
+/* Differentiation of the drift segment (no counterpart in the reference,
+ * which has no adjoint).  The drift step is the charged step's one-substep
+ * branch -- chord, GL2 field work, Strang half kicks, CSDA drag, net-chord
+ * deposit -- with the discrete events and the elastic deflection frozen
+ * out; a particle whose chord reaches the domain boundary or whose energy
+ * falls below the cutoff ends its drift there (removed, no deposit for that
+ * step).  rrea_drift_forward runs `steps` drift steps over the tile in place
+ * (r..uz), depositing into jr/jz (the padded net-chord grid of rrea_step_out,
+ * accumulated over all steps; may be NULL) and, with loss weights gjr/gjz
+ * (dL/dJ on the same faces), each particle's sum of J x dL/dJ into loss[n]
+ * (an order-independent per-particle loss, for finite differences).  It
+ * writes the state every checkpoint_every steps into checkpoints
+ * ([n_ck][6][n] doubles: r, theta, z, ux, uy, uz; n_ck = ceil(steps /
+ * checkpoint_every)) and the live flags into checkpoint_alive ([n_ck][n]).
+ * rrea_drift_adjoint then forms dL/dE for L = sum(J x dL/dJ) over the
+ * fields' Maxwell Er (r-faces, (nr+1)*nz) and Ez (z-faces, nr*(nz+1)),
+ * accumulated into grad_er / grad_ez: a reverse sweep over the checkpoint
+ * segments with recompute, each step's local Jacobian from a tangent-linear
+ * (dual-number) evaluation of the same step. */
+typedef struct rrea_drift_config {
+    int32_t species;            /* 0 electrons, 1 positrons */
+    int32_t steps;
+    int32_t checkpoint_every;   /* 1..16 */
+    int32_t reserved;
+    double dt_s;
+    double low_energy_cutoff_eV;
+} rrea_drift_config;
+
+rrea_status rrea_drift_forward(rrea_tables_t tables, rrea_fields_t fields, const rrea_tile* tile,
+                               const rrea_drift_config* config, double* checkpoints, uint8_t* checkpoint_alive,
+                               uint8_t* alive, double* jr, double* jz, const double* gjr, const double* gjz,
+                               double* loss, rrea_stream_t stream);
+rrea_status rrea_drift_adjoint(rrea_tables_t tables, rrea_fields_t fields, const rrea_tile* tile,
+                               const rrea_drift_config* config, const double* checkpoints,
+                               const uint8_t* checkpoint_alive, const double* gjr, const double* gjz,
+                               double* grad_er, double* grad_ez, rrea_stream_t stream);
+
 /* Kernel timing with HIP events recorded around every transport-kernel
  * launch made through `fields` while enabled (no counterpart in the reference:
  * measurement support).  Enabling or disabling clears the record.  _read

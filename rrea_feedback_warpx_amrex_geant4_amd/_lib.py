@@ -103,6 +103,11 @@ class ParticleStore(C.Structure):
     _fields_ = [("tile", Tile), ("instance", C.c_void_p), ("capacity", C.c_int64)]
 
 
+class DriftConfig(C.Structure):
+    _fields_ = [("species", C.c_int32), ("steps", C.c_int32), ("checkpoint_every", C.c_int32),
+                ("reserved", C.c_int32), ("dt_s", C.c_double), ("low_energy_cutoff_eV", C.c_double)]
+
+
 class StepOut(C.Structure):
     _fields_ = [
         ("status", C.c_void_p), ("ion_pairs", C.c_void_p), ("substeps", C.c_void_p),
@@ -161,6 +166,10 @@ SIGNATURES = [
     ("rrea_commit_survivors", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(ParticleStore), C.c_void_p, C.c_void_p,
                                           C.c_int32, C.c_void_p, C.c_void_p]),
     ("rrea_tile_compact", C.c_int32, [C.c_void_p, C.POINTER(Tile), C.c_int32, C.c_void_p, P_I64, C.c_void_p]),
+    ("rrea_drift_forward", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(DriftConfig)] + [C.c_void_p] * 9),
+    ("rrea_drift_adjoint", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(DriftConfig)] + [C.c_void_p] * 7),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),

@@ -15,6 +15,7 @@
 // multiply-add anywhere: the step must agree with the reference to 1e-12).
 
 #include "rrea_device.h"
+#include "rrea_drift.h"
 #include "../../include/rrea_hip.h"
 
 #include <hip/hip_runtime.h>
@@ -204,6 +205,9 @@ struct rrea_fields_s {
     void* d_tbuf = nullptr;
     int64_t tsort_cap = 0;
     int64_t* d_live = nullptr;       // rrea_tile_compact's live count
+    // drift adjoint workspace: the argument block, the cell-centred gradients
+    void* d_dargs = nullptr;
+    double* d_gcell = nullptr;
     // newborn FIFO workspace
     void* d_nargs = nullptr;
     rrea_secondary* d_fin = nullptr;
@@ -3181,7 +3185,7 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
                     F->d_slow_list, F->d_slow_count, F->d_prof_guide, F->d_nargs, F->d_fin, F->d_fst,
                     F->d_fion, F->d_fsub, F->d_anchor, F->d_next, F->d_flag, F->d_nsel, F->d_sel_tmp,
                     F->d_tkeys, F->d_tkeys_out, F->d_tidx, F->d_tidx_out, F->d_tsort_tmp, F->d_tbuf, F->d_base,
-                    F->d_live};
+                    F->d_live, F->d_dargs, F->d_gcell};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : F->events) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
     delete F;
@@ -3848,6 +3852,163 @@ rrea_status permute_tile(rrea_fields_t F, const rrea_tile* tile, int32_t photon,
     return RREA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The drift segment (forward model, checkpoints, reverse sweep)
+// ---------------------------------------------------------------------------
+constexpr int32_t kDriftMaxEvery = 16;
+
+struct DriftArgs {
+    DevTables T;
+    DriftEnv E;
+    int64_t n;
+    int32_t steps, every, n_ck;
+    double *r, *z, *th, *ux, *uy, *uz;
+    const double* w;
+    const int64_t* id;
+    double* ck;              // [checkpoint][component][particle], components r, theta, z, ux, uy, uz
+    uint8_t* ck_alive;       // [checkpoint][particle]
+    uint8_t* alive;          // [particle], after the last step
+    DriftSinks sinks;        // forward: J grid and/or loss weights; adjoint: loss weights
+    double* loss;            // [particle]: the particle's sum over its deposits of J x dL/dJ
+    double* g_er;            // adjoint: dL/d(cell-centred Er), nr*nz
+    double* g_ez;
+};
+
+__global__ void __launch_bounds__(256) drift_forward_kernel(const DriftArgs* __restrict__ args)
+{
+    DriftArgs const& A = *args;
+    int64_t const ip = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= A.n) return;
+    DState<double> s{A.r[ip], A.th[ip], A.z[ip], A.ux[ip], A.uy[ip], A.uz[ip]};
+    bool alive = A.id[ip] != RREA_INVALID_ID;
+    double const w = A.w[ip];
+    DriftSink<double> sink{A.E.G, A.sinks, 0.0};
+    for (int32_t t = 0; t < A.steps; ++t) {
+        if (A.ck && t % A.every == 0) {
+            int64_t const c = t / A.every;
+            double* const o = A.ck + c * 6 * A.n + ip;
+            o[0] = s.r; o[A.n] = s.th; o[2 * A.n] = s.z; o[3 * A.n] = s.ux; o[4 * A.n] = s.uy; o[5 * A.n] = s.uz;
+            A.ck_alive[c * A.n + ip] = alive ? 1 : 0;
+        }
+        if (alive) alive = drift_step(A.E, A.T, w, s, sink, nullptr);
+    }
+    A.r[ip] = s.r; A.th[ip] = s.th; A.z[ip] = s.z; A.ux[ip] = s.ux; A.uy[ip] = s.uy; A.uz[ip] = s.uz;
+    if (A.alive) A.alive[ip] = alive ? 1 : 0;
+    if (A.loss) A.loss[ip] = sink.acc;
+}
+
+// The reverse sweep, one particle per lane: per checkpoint segment (last
+// first), the segment's states are recomputed from its checkpoint, then each
+// step is evaluated once more in dual numbers over (state, gathered fields)
+// and the adjoint of its input state is formed from the local Jacobian, the
+// field adjoints scattered to the cells of the gather stencils.
+__global__ void __launch_bounds__(64) drift_adjoint_kernel(const DriftArgs* __restrict__ args)
+{
+    DriftArgs const& A = *args;
+    int64_t const ip = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (ip >= A.n) return;
+    double const w = A.w[ip];
+    DriftSinks const none{nullptr, nullptr, nullptr, nullptr};
+    DriftSinks const weights{nullptr, nullptr, A.sinks.gjr, A.sinks.gjz};
+    double adj[kStateSlots] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int32_t c = A.n_ck - 1; c >= 0; --c) {
+        int32_t const t0 = c * A.every;
+        int32_t const m = min(A.steps - t0, A.every);
+        DState<double> st[kDriftMaxEvery];
+        bool al[kDriftMaxEvery];
+        {
+            const double* const o = A.ck + static_cast<int64_t>(c) * 6 * A.n + ip;
+            st[0] = DState<double>{o[0], o[A.n], o[2 * A.n], o[3 * A.n], o[4 * A.n], o[5 * A.n]};
+            al[0] = A.ck_alive[static_cast<int64_t>(c) * A.n + ip] != 0;
+        }
+        for (int32_t q = 0; q + 1 < m; ++q) {
+            st[q + 1] = st[q];
+            DriftSink<double> sk{A.E.G, none, 0.0};
+            al[q + 1] = al[q] && drift_step(A.E, A.T, w, st[q + 1], sk, nullptr);
+        }
+        for (int32_t q = m - 1; q >= 0; --q) {
+            if (!al[q]) continue;
+            DState<D10> x{D10::seed(st[q].r, 0), D10::seed(st[q].th, 1), D10::seed(st[q].z, 2),
+                          D10::seed(st[q].ux, 3), D10::seed(st[q].uy, 4), D10::seed(st[q].uz, 5)};
+            DriftSink<D10> sk{A.E.G, weights, D10(0.0)};
+            GatherStencil gs[2];
+            if (!drift_step(A.E, A.T, w, x, sk, gs)) continue;
+            D10 const* out[kStateSlots] = {&x.r, &x.th, &x.z, &x.ux, &x.uy, &x.uz};
+            double g[kDriftSlots];
+            for (int j = 0; j < kDriftSlots; ++j) {
+                double v = sk.acc.d[j];
+                for (int i = 0; i < kStateSlots; ++i) v += adj[i] * out[i]->d[j];
+                g[j] = v;
+            }
+            for (int i = 0; i < kStateSlots; ++i) adj[i] = g[i];
+            for (int node = 0; node < 2; ++node) {
+                double const ger = g[kStateSlots + 2 * node], gez = g[kStateSlots + 2 * node + 1];
+                for (int k = 0; k < 4; ++k) {
+                    int32_t const cell = gs[node].c[k];
+                    if (ger != 0.0) atomic_add(&A.g_er[cell], ger * gs[node].w_er[k]);
+                    if (gez != 0.0) atomic_add(&A.g_ez[cell], gez * gs[node].w_ez[k]);
+                }
+            }
+        }
+    }
+}
+
+// dL/d(face) from dL/d(cell): each cell-centred value is the average of its
+// two faces (cell_fields_kernel), so a face gets half of each adjacent
+// cell's gradient.  Accumulates into the caller's face arrays.
+__global__ void face_gradient_kernel(int32_t nr, int32_t nz, const double* __restrict__ g_er,
+                                     const double* __restrict__ g_ez, double* __restrict__ f_er,
+                                     double* __restrict__ f_ez)
+{
+    int64_t const f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t const n_er = static_cast<int64_t>(nr + 1) * nz, n_ez = static_cast<int64_t>(nr) * (nz + 1);
+    if (f < n_er) {
+        int32_t const i = static_cast<int32_t>(f % (nr + 1)), j = static_cast<int32_t>(f / (nr + 1));
+        double v = 0.0;
+        if (i >= 1) v += 0.5 * g_er[(i - 1) + static_cast<int64_t>(nr) * j];
+        if (i < nr) v += 0.5 * g_er[i + static_cast<int64_t>(nr) * j];
+        f_er[f] += v;
+    }
+    if (f < n_ez) {
+        int32_t const i = static_cast<int32_t>(f % nr), j = static_cast<int32_t>(f / nr);
+        double v = 0.0;
+        if (j >= 1) v += 0.5 * g_ez[i + static_cast<int64_t>(nr) * (j - 1)];
+        if (j < nz) v += 0.5 * g_ez[i + static_cast<int64_t>(nr) * j];
+        f_ez[f] += v;
+    }
+}
+
+rrea_status drift_args(rrea_tables_t T, rrea_fields_t F, const rrea_drift_config* cfg, DriftArgs& A)
+{
+    if (!T || !F || !cfg || !F->fields_set) return RREA_ERR_ARGUMENT;
+    if (cfg->species != 0 && cfg->species != 1) return RREA_ERR_ARGUMENT;
+    if (cfg->steps < 0 || cfg->checkpoint_every < 1 || cfg->checkpoint_every > kDriftMaxEvery) {
+        return RREA_ERR_ARGUMENT;
+    }
+    if (!(cfg->dt_s > 0.0) || !std::isfinite(cfg->dt_s) || !(cfg->low_energy_cutoff_eV > 0.0)) {
+        return RREA_ERR_ARGUMENT;
+    }
+    Env env;
+    rrea_status const st = make_env(F, nullptr, nullptr, nullptr, -1, 0, env);
+    if (st != RREA_OK) return st;
+    A = DriftArgs{};
+    A.T = T->view;
+    A.E.F = env.F;
+    A.E.D = env.D;
+    A.E.G = env.G;
+    A.E.fallback_ratio = env.density_ratio;
+    bool const pos = cfg->species == 1;
+    A.E.q = pos ? 1.0 : -1.0;
+    A.E.col = pos ? kPCol : kECol;
+    A.E.soft = pos ? kPSoft : kESoft;
+    A.E.dt = cfg->dt_s;
+    A.E.cutoff = cfg->low_energy_cutoff_eV;
+    A.steps = cfg->steps;
+    A.every = cfg->checkpoint_every;
+    A.n_ck = (cfg->steps + cfg->checkpoint_every - 1) / cfg->checkpoint_every;
+    return RREA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -4064,6 +4225,77 @@ rrea_status rrea_photon_push(const rrea_tile* tile, double dt_s, rrea_stream_t s
         tile->n, tile->id, const_cast<double*>(tile->prev_r), const_cast<double*>(tile->prev_z),
         const_cast<double*>(tile->prev_theta), tile->r, tile->z, tile->theta, tile->ux, tile->uy, tile->uz,
         kC * dt_s);
+    HIP_TRY(hipGetLastError());
+    return RREA_OK;
+}
+
+
+rrea_status rrea_drift_forward(rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile, const rrea_drift_config* cfg,
+                               double* checkpoints, uint8_t* checkpoint_alive, uint8_t* alive, double* jr,
+                               double* jz, const double* gjr, const double* gjz, double* loss, rrea_stream_t stream)
+{
+    DriftArgs A;
+    {
+        rrea_status const st = drift_args(T, F, cfg, A);
+        if (st != RREA_OK) return st;
+    }
+    if (!tile || tile->n < 0 || (!checkpoints) != (!checkpoint_alive) || (!gjr) != (!gjz) || (loss && !gjr)
+        || (!jr) != (!jz)) {
+        return RREA_ERR_ARGUMENT;
+    }
+    if (tile->n == 0) return RREA_OK;
+    if (!tile->r || !tile->z || !tile->theta || !tile->ux || !tile->uy || !tile->uz || !tile->w || !tile->id) {
+        return RREA_ERR_ARGUMENT;
+    }
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    A.n = tile->n;
+    A.r = tile->r; A.z = tile->z; A.th = tile->theta; A.ux = tile->ux; A.uy = tile->uy; A.uz = tile->uz;
+    A.w = tile->w; A.id = tile->id;
+    A.ck = checkpoints; A.ck_alive = checkpoint_alive; A.alive = alive;
+    A.sinks = DriftSinks{jr, jz, gjr, gjz};
+    A.loss = loss;
+    if (!F->d_dargs) HIP_TRY(hipMalloc(&F->d_dargs, sizeof(DriftArgs)));
+    HIP_TRY(hipMemcpyAsync(F->d_dargs, &A, sizeof(DriftArgs), hipMemcpyHostToDevice, s));
+    drift_forward_kernel<<<dim3(static_cast<unsigned>(blocks_for(A.n, 256))), dim3(256), 0, s>>>(
+        static_cast<const DriftArgs*>(F->d_dargs));
+    HIP_TRY(hipGetLastError());
+    return RREA_OK;
+}
+
+rrea_status rrea_drift_adjoint(rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile, const rrea_drift_config* cfg,
+                               const double* checkpoints, const uint8_t* checkpoint_alive, const double* gjr,
+                               const double* gjz, double* grad_er, double* grad_ez, rrea_stream_t stream)
+{
+    DriftArgs A;
+    {
+        rrea_status const st = drift_args(T, F, cfg, A);
+        if (st != RREA_OK) return st;
+    }
+    if (!tile || tile->n < 0 || !gjr || !gjz || !grad_er || !grad_ez) return RREA_ERR_ARGUMENT;
+    if (tile->n == 0 || cfg->steps == 0) return RREA_OK;
+    if (!checkpoints || !checkpoint_alive || !tile->w) return RREA_ERR_ARGUMENT;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    int32_t const nr = F->geom.nr, nz = F->geom.nz;
+    int64_t const cells = static_cast<int64_t>(nr) * nz;
+    if (!F->d_gcell) HIP_TRY(hipMalloc(&F->d_gcell, 2 * cells * sizeof(double)));
+    fill_u64_kernel<<<dim3(static_cast<unsigned>(blocks_for(2 * cells, 256))), dim3(256), 0, s>>>(
+        reinterpret_cast<unsigned long long*>(F->d_gcell), 2 * cells, 0ULL);
+    HIP_TRY(hipGetLastError());
+    A.n = tile->n;
+    A.w = tile->w;
+    A.ck = const_cast<double*>(checkpoints);
+    A.ck_alive = const_cast<uint8_t*>(checkpoint_alive);
+    A.sinks = DriftSinks{nullptr, nullptr, gjr, gjz};
+    A.g_er = F->d_gcell;
+    A.g_ez = F->d_gcell + cells;
+    if (!F->d_dargs) HIP_TRY(hipMalloc(&F->d_dargs, sizeof(DriftArgs)));
+    HIP_TRY(hipMemcpyAsync(F->d_dargs, &A, sizeof(DriftArgs), hipMemcpyHostToDevice, s));
+    drift_adjoint_kernel<<<dim3(static_cast<unsigned>(blocks_for(A.n, 64))), dim3(64), 0, s>>>(
+        static_cast<const DriftArgs*>(F->d_dargs));
+    HIP_TRY(hipGetLastError());
+    int64_t const faces = static_cast<int64_t>(nr + 1) * (nz + 1);
+    face_gradient_kernel<<<dim3(static_cast<unsigned>(blocks_for(faces, 256))), dim3(256), 0, s>>>(
+        nr, nz, A.g_er, A.g_ez, grad_er, grad_ez);
     HIP_TRY(hipGetLastError());
     return RREA_OK;
 }

@@ -134,6 +134,40 @@ def c3_batch(n_instances: int = 256, seed: int = 3, scale: float = 1.0, nr: int 
     return fields, tiles, seeds
 
 
+def c4_drift(n: int = 1 << 20, nr: int = 128, nz: int = 512, cell_m: float = 0.25, seed: int = 4):
+    """The differentiation configuration: a 128 x 512 RZ grid of 0.25 m cells
+    over air at 1.225 kg/m^3 with an ambient E_z = -400 kV/m, the Maxwell
+    E_r, E_z a smooth seeded perturbation of up to 50 kV/m (the parameters),
+    and n electrons with K ~ U[1, 20] MeV in a 30-degree cone about the force
+    (+z), spread over r < 0.8 R and the middle 80 % of z.  Returns
+    (fields, electrons)."""
+    rng = np.random.default_rng(seed)
+    f = RzFields.uniform(nr, nz, cell_m, cell_m, z_lo=0.0, ez_v_per_m=-4.0e5,
+                         density_ratio=1.225 / REFERENCE_DENSITY)
+    r_max, z_max = nr * cell_m, nz * cell_m
+    # a few smooth modes on the face centres
+    def smooth(rr, zz):
+        v = np.zeros_like(rr)
+        for _ in range(4):
+            kr, kz = rng.uniform(0.5, 3.0) * np.pi / r_max, rng.uniform(0.5, 6.0) * np.pi / z_max
+            v += rng.uniform(-1.0, 1.0) * np.cos(kr * rr + rng.uniform(0, 2 * np.pi)) * np.sin(
+                kz * zz + rng.uniform(0, 2 * np.pi))
+        return 5.0e4 * v / 4.0
+    ir, jr_ = np.meshgrid(np.arange(nr + 1), np.arange(nz), indexing="xy")
+    f.er = smooth(ir * cell_m, (jr_ + 0.5) * cell_m).ravel()
+    iz, jz_ = np.meshgrid(np.arange(nr), np.arange(nz + 1), indexing="xy")
+    f.ez = smooth((iz + 0.5) * cell_m, jz_ * cell_m).ravel()
+    k = rng.uniform(1.0e6, 2.0e7, n)
+    dx, dy, dz = _cone_directions(rng, n, 1.0, np.deg2rad(30.0))
+    u = proper_velocity(k)
+    r = 0.8 * r_max * np.sqrt(rng.random(n))
+    z = z_max * (0.1 + 0.8 * rng.random(n))
+    theta = 2.0 * np.pi * rng.random(n) - np.pi
+    tile = ParticleTile(r=r, z=z, theta=theta, ux=u * dx, uy=u * dy, uz=u * dz, w=np.ones(n),
+                        id=np.arange(1, n + 1, dtype=np.int64), cpu=np.zeros(n, dtype=np.int32))
+    return f, tile
+
+
 def push_photons(r, theta, z, ux, uy, uz, dt_s):
     """Straight-line photon push at c along u / |u| in Cartesian, back to
     (r, theta, z): the pushed position the photon step reads as its chord end

@@ -29,6 +29,7 @@ STRUCTS = {
     "rrea_step_out": _lib.StepOut,
     "rrea_drain_io": _lib.DrainIO,
     "rrea_particle_store": _lib.ParticleStore,
+    "rrea_drift_config": _lib.DriftConfig,
 }
 
 
