@@ -253,6 +253,8 @@ typedef struct rrea_step_out {
     double* tallies;          /* device double[RREA_TALLY_COUNT], accumulated */
     int32_t* error;           /* device int32[2]: first error code, particle index */
     int32_t order_secondaries;/* nonzero: sort the staged secondaries by (parent, lineage) */
+    double* sources;          /* material-source grids (RREA_SOURCE_FIELDS x nr x nz, cell c = i + nr*j),
+                                 accumulated raw (see rrea_sources_normalize); NULL: not accumulated */
 } rrea_step_out;
 
 rrea_status rrea_transport_step_electrons(rrea_tables_t tables, rrea_fields_t fields, const rrea_tile* tile,
@@ -338,6 +340,7 @@ typedef struct rrea_drain_io {
     double* jr; double* jz; double* rho;  /* the step's padded net-chord grid, accumulated */
     double* tallies;             /* device double[RREA_TALLY_COUNT], accumulated */
     int32_t* error;              /* device int32[2]: first error code, FIFO index */
+    double* sources;             /* material-source grids as rrea_step_out.sources (NULL: none) */
 } rrea_drain_io;
 
 rrea_status rrea_newborn_drain(rrea_tables_t tables, rrea_fields_t fields, const rrea_step_config* config,
@@ -445,6 +448,35 @@ rrea_status rrea_drift_adjoint(rrea_tables_t tables, rrea_fields_t fields, const
                                const rrea_drift_config* config, const double* checkpoints,
                                const uint8_t* checkpoint_alive, const double* gjr, const double* gjz,
                                double* grad_er, double* grad_ez, rrea_stream_t stream);
+
+/* Material sources (TransportSideEffects::Accumulate* replayed into
+ * RreaWarpXCoupling::Accumulate{IonizationEvent,PositiveIonSourceOnly,
+ * LowEnergyElectron}, OV/RreaCouplingSources.cpp:435-516, into
+ * RreaEventAccumulator, NA/src/RreaEventAccumulator.cpp).  With
+ * rrea_step_out.sources / rrea_drain_io.sources set, every ionization,
+ * positive-ion and low-energy-electron record of the step -- continuous ion
+ * pairs at their interval midpoints, hard Moller and Bhabha events, local
+ * deposits as E / W_air pairs, demotions, annihilations, Compton,
+ * photoelectric and pair events -- is deposited where it occurs with the
+ * accumulator's order-1 cell-centred shape (AddToField :386-512: a radial
+ * ghost below the axis folds onto the first cell; support past the other
+ * boundaries is dropped) into the fields below, as raw weights.  A record
+ * outside the physical domain is reported as error 2 (the reference aborts). */
+#define RREA_SOURCE_FIELDS 5
+enum {
+    RREA_SOURCE_LOW_ELECTRON = 0,   /* low-energy electron source */
+    RREA_SOURCE_POSITIVE_ION = 1,   /* positive-ion source */
+    RREA_SOURCE_NEGATIVE_ION = 2,   /* direct negative-ion source (no transport record feeds it) */
+    RREA_SOURCE_EVENT_COUNT = 3,    /* ionization event count (diagnostic) */
+    RREA_SOURCE_ENERGY_LOSS = 4,    /* energy loss, weight x eV (diagnostic) */
+};
+/* NormalizeAndSync (:280-345) on one grid: every field over its RZ shell
+ * cell volume pi (r_o^2 - r_i^2) dz, the three source fields also over dt;
+ * with diagnostics == 0 the two diagnostic fields are left raw. */
+rrea_status rrea_sources_normalize(rrea_fields_t fields, double* sources, double dt_s, int32_t diagnostics,
+                                   rrea_stream_t stream);
+/* Each instance's source grids for batched steps and drains (NULL: none). */
+rrea_status rrea_batch_bind_sources(rrea_batch_t batch, double* const* sources);
 
 /* Kernel timing with HIP events recorded around every transport-kernel
  * launch made through `fields` while enabled (no counterpart in the reference:

@@ -26,6 +26,7 @@ class Geometry(C.Structure):
         ("er", P_D), ("ez", P_D), ("amb_er", P_D), ("amb_ez", P_D), ("btheta", P_D),
         ("density_mode", C.c_int32), ("profile_n", C.c_int32), ("density_ratio", C.c_double),
         ("profile_alt_m", P_D), ("profile_rho", P_D), ("altitude_at_z0_m", C.c_double),
+        ("sources", P_D),
     ]
 
 
@@ -137,6 +138,10 @@ def lib():
                                       + [P_D] * 10 + [P_I64, P_I32] + [P_D] * 6 + [P_I32, P_D, P_I64]
                                       + [C.c_int64, C.POINTER(Secondary), P_I64, P_D, C.c_int])
         L.orc_newborn_drain.restype = C.c_int
+        L.orc_accumulate_source.restype = C.c_int
+        L.orc_accumulate_source.argtypes = [C.POINTER(Geometry), C.c_double, C.c_int] + [C.c_double] * 5
+        L.orc_sources_normalize.restype = C.c_int
+        L.orc_sources_normalize.argtypes = [C.POINTER(Geometry), C.c_double, C.c_int]
         L.orc_newborn_drain.argtypes = [C.c_void_p, C.POINTER(Geometry), C.POINTER(Config), C.POINTER(Secondary),
                                         C.c_int64, C.c_int64, C.POINTER(Secondary), P_I64, P_I32, P_D, P_I64,
                                         C.POINTER(Secondary), P_I64, P_D, P_D, P_D, P_D]
@@ -318,8 +323,9 @@ class Tables:
         return self._call("orc_hard_moller_energy_loss", e)
 
 
-def make_geometry(fields) -> tuple[Geometry, list]:
-    """Pack a transport field/density description into the oracle struct."""
+def make_geometry(fields, sources: np.ndarray | None = None) -> tuple[Geometry, list]:
+    """Pack a transport field/density description into the oracle struct;
+    ``sources`` (float64, 5 * nr * nz) receives the material sources."""
     keep = []
 
     def arr(a):
@@ -341,7 +347,31 @@ def make_geometry(fields) -> tuple[Geometry, list]:
         g.profile_alt_m = arr(fields.profile_alt_m)
         g.profile_rho = arr(fields.profile_rho)
         g.altitude_at_z0_m = fields.altitude_at_z0_m
+    if sources is not None:
+        assert sources.dtype == np.float64 and sources.flags.c_contiguous and sources.size == 5 * fields.nr * fields.nz
+        g.sources = _ptr(sources)
     return g, keep
+
+
+def new_sources(fields) -> np.ndarray:
+    """Zeroed material-source grids (5 * nr * nz)."""
+    return np.zeros(5 * fields.nr * fields.nz)
+
+
+def accumulate_source(fields, sources: np.ndarray, cutoff: float, kind: int, r: float, z: float, weight: float,
+                      secondary: float = 0.0, loss: float = 0.0) -> int:
+    """One record (0 ionization, 1 positive ion, 2 low-energy electron) into ``sources``."""
+    g, keep = make_geometry(fields, sources)
+    code = lib().orc_accumulate_source(C.byref(g), cutoff, kind, r, z, weight, secondary, loss)
+    del keep
+    return code
+
+
+def sources_normalize(fields, sources: np.ndarray, dt: float, diagnostics: bool = True) -> int:
+    g, keep = make_geometry(fields, sources)
+    code = lib().orc_sources_normalize(C.byref(g), dt, int(diagnostics))
+    del keep
+    return code
 
 
 def make_config(cfg) -> Config:
@@ -362,13 +392,14 @@ def make_config(cfg) -> Config:
     return c
 
 
-def photon_step(tables: Tables, fields, cfg, particles: dict, nthreads: int = 1, sec_cap: int | None = None):
+def photon_step(tables: Tables, fields, cfg, particles: dict, nthreads: int = 1, sec_cap: int | None = None,
+                sources: np.ndarray | None = None):
     """Run the oracle photon step.  ``particles`` holds prev r, z, theta, the
     pushed (current) r_cur, z_cur, theta_cur, u = E dir (ux, uy, uz), w, id,
     cpu.  Returns a dict of outputs (r, z, theta, ux, uy, uz written back)."""
     L = lib()
     n = len(particles["r"])
-    g, keep = make_geometry(fields)
+    g, keep = make_geometry(fields, sources)
     c = make_config(cfg)
     ins = [np.ascontiguousarray(particles[k], dtype=np.float64)
            for k in ("r", "z", "theta", "r_cur", "z_cur", "theta_cur", "ux", "uy", "uz", "w")]
@@ -392,7 +423,8 @@ def photon_step(tables: Tables, fields, cfg, particles: dict, nthreads: int = 1,
             "tallies": tallies}
 
 
-def newborn_drain(tables: Tables, fields, cfg, initial: np.ndarray, cap: int | None = None):
+def newborn_drain(tables: Tables, fields, cfg, initial: np.ndarray, cap: int | None = None,
+                  sources: np.ndarray | None = None):
     """Run the oracle's transient FIFO over the step's staged secondaries
     (SECONDARY_DTYPE records, Apply's order).  Returns the whole FIFO, one
     status and ion-pair weight per entry, the survivors, and the J grid and
@@ -401,7 +433,7 @@ def newborn_drain(tables: Tables, fields, cfg, initial: np.ndarray, cap: int | N
     initial = np.ascontiguousarray(initial, dtype=SECONDARY_DTYPE)
     n0 = len(initial)
     cap = cap if cap is not None else max(1024, 8 * n0)
-    g, keep = make_geometry(fields)
+    g, keep = make_geometry(fields, sources)
     c = make_config(cfg)
     src = (Secondary * max(n0, 1)).from_buffer_copy(initial.tobytes() + bytes(SECONDARY_DTYPE.itemsize * (n0 == 0)))
     work = (Secondary * cap)()
@@ -467,10 +499,10 @@ def positron_step(tables: Tables, fields, cfg, particles: dict, nthreads: int = 
 
 
 def charged_step(species: str, tables: Tables, fields, cfg, particles: dict, nthreads: int = 1,
-                 sec_cap: int | None = None):
+                 sec_cap: int | None = None, sources: np.ndarray | None = None):
     L = lib()
     n = len(particles["r"])
-    g, keep = make_geometry(fields)
+    g, keep = make_geometry(fields, sources)
     c = make_config(cfg)
     ins = [np.ascontiguousarray(particles[k], dtype=np.float64) for k in ("r", "z", "theta", "ux", "uy", "uz", "w")]
     pid = np.ascontiguousarray(particles["id"], dtype=np.int64)

@@ -1429,9 +1429,18 @@ struct DepositOp {
     uint32_t faces;
 };
 
+// One material-source record of the ledger (TransportSideEffects::
+// Accumulate*, OV/RreaTransportLog.H:449-490): kind 0 ionization event,
+// 1 positive ion only, 2 low-energy electron.
+struct SourceOp {
+    int kind;
+    double r, z, w, secondary, loss;
+};
+
 struct ChunkLog {
     std::vector<orc_secondary> secondaries;
     std::vector<DepositOp> deposits;
+    std::vector<SourceOp> sources;
     double tallies[ORC_T_COUNT] = {0};
     int error = 0;
 };
@@ -1443,6 +1452,84 @@ struct StepContext {
     double r_lo, r_hi, z_lo, z_hi;
     double density_ratio;  // reference-z fallback (RreaParticleInteraction.cpp:33-35)
 };
+
+// ---------------------------------------------------------------------------
+// Material sources: RreaEventAccumulator (NA/src/RreaEventAccumulator.cpp)
+// fed by RreaWarpXCoupling::Accumulate* (OV/RreaCouplingSources.cpp:435-516)
+// ---------------------------------------------------------------------------
+// AddToField :386-512 on a non-periodic RZ grid from the axis
+inline void source_add(orc_geometry const* g, int field, double r, double z, double value)
+{
+    if (!std::isfinite(value) || value == 0.0) return;
+    int const ic = static_cast<int>(std::floor((r - g->r_lo) / g->dr));
+    int const jc = static_cast<int>(std::floor((z - g->z_lo) / g->dz));
+    if (ic < 0 || ic >= g->nr || jc < 0 || jc >= g->nz) return;
+    double const r_cell = (r - g->r_lo) / g->dr + static_cast<double>(0) - 0.5;
+    int const i0 = static_cast<int>(std::floor(r_cell));
+    double const wr1 = r_cell - static_cast<double>(i0);
+    double const wr0 = 1.0 - wr1;
+    double const z_cell = (z - g->z_lo) / g->dz + static_cast<double>(0) - 0.5;
+    int const j0 = static_cast<int>(std::floor(z_cell));
+    double const wz1 = z_cell - static_cast<double>(j0);
+    double const wz0 = 1.0 - wz1;
+    double* const dst = g->sources + static_cast<int64_t>(field) * g->nr * g->nz;
+    for (int rz = 0; rz < 2; ++rz) {
+        double const wz = rz == 0 ? wz0 : wz1;
+        if (wz == 0.0) continue;
+        int const j = j0 + rz;
+        if (j < 0 || j > g->nz - 1) continue;  // an absorbing z boundary keeps it in its ghost cell
+        for (int rr = 0; rr < 2; ++rr) {
+            double const wr = rr == 0 ? wr0 : wr1;
+            double const v = value * wr * wz;
+            if (v == 0.0) continue;
+            int i = i0 + rr;
+            if (i < 0 || i > g->nr - 1) {
+                if (i == -1 && g->r_lo == 0.0) i = 0;  // the RZ mode-0 fold across the axis
+                else continue;
+            }
+            dst[i + static_cast<int64_t>(g->nr) * j] += v;
+        }
+    }
+}
+
+// AddIonizationDiagnostic :156-167
+inline void source_diagnostic(orc_geometry const* g, double r, double z, double weight, double loss)
+{
+    source_add(g, 3, r, z, weight);
+    source_add(g, 4, r, z, std::max(weight, 0.0) * std::max(loss, 0.0));
+}
+
+// the coupling's three accumulators; a record outside the half-open physical
+// domain is the reference's abort (EventSourcePositionInDomain)
+inline int accumulate_source(orc_geometry const* g, double cutoff, SourceOp const& op)
+{
+    if (op.w <= 0.0) return 0;
+    double const r_hi = g->r_lo + g->nr * g->dr, z_hi = g->z_lo + g->nz * g->dz;
+    if (!(op.r >= g->r_lo && op.r < r_hi && op.z >= g->z_lo && op.z < z_hi)) return ORC_ERR_OUTSIDE_DOMAIN;
+    if (op.kind == 0) {
+        source_add(g, 1, op.r, op.z, op.w);
+        if (op.secondary < cutoff) source_add(g, 0, op.r, op.z, op.w);
+        source_diagnostic(g, op.r, op.z, op.w, op.loss);
+    } else if (op.kind == 1) {
+        source_add(g, 1, op.r, op.z, op.w);
+        source_diagnostic(g, op.r, op.z, op.w, op.loss);
+    } else {
+        source_add(g, 0, op.r, op.z, op.w);
+        source_diagnostic(g, op.r, op.z, op.w, std::max(op.loss, 0.0));
+    }
+    return 0;
+}
+
+// the replay of one chunk's material-source ledger, in order
+inline int replay_sources(orc_geometry const* g, double cutoff, std::vector<SourceOp> const& ops)
+{
+    if (!g->sources) return 0;
+    for (auto const& op : ops) {
+        int const e = accumulate_source(g, cutoff, op);
+        if (e) return e;
+    }
+    return 0;
+}
 
 // substep_driver_count OV/RreaTransportStep.H:106-110
 inline uint64_t driver_count(double amount, double per)
@@ -1496,12 +1583,28 @@ struct Emitter {
         ion += ev_w;
         fx.tallies[ORC_T_ION_PAIRS] += ev_w;
     }
-    void deposit_local(double energy)
+    // material-source records at (r, z)
+    void ionization(double r, double z, double ew, double secondary, double loss)
+    {
+        fx.sources.push_back(SourceOp{0, r, z, ew, secondary, loss});
+    }
+    void positive_ion(double r, double z, double ew, double loss)
+    {
+        fx.sources.push_back(SourceOp{1, r, z, ew, 0.0, loss});
+    }
+    void low_electron(double r, double z, double ew, double loss)
+    {
+        fx.sources.push_back(SourceOp{2, r, z, ew, 0.0, loss});
+    }
+    // deposit_local_energy_as_ion_pairs: E / W_air pairs, one ionization
+    // event of secondary energy 0 and diagnostic loss W_air at (r, z)
+    void deposit_local(double energy, double r, double z)
     {
         if (w <= 0.0 || energy <= 0.0) return;
         double const pw = w * energy / std::max(kWAir, 1.0);
         if (pw <= 0.0) return;
         ion_event(pw);
+        ionization(r, z, pw, 0.0, kWAir);
     }
     void secondary(int sp, int process, double x, double y, double z, double sdx, double sdy, double sdz,
                    double energy, double birth)
@@ -1518,7 +1621,8 @@ struct Emitter {
 // and the positive ion the annihilated target electron leaves behind.  The
 // photons join the enclosing group, so they carry its process id.
 inline void spawn_annihilation(StepContext const& ctx, Emitter& em, int32_t process, double x, double y, double z,
-                               double ax, double ay, double az, double k, bool at_rest, uint64_t index, double birth)
+                               double r, double ax, double ay, double az, double k, bool at_rest, uint64_t index,
+                               double birth)
 {
     Tables const& t = *ctx.t;
     double const gcut = ctx.cfg.photon_cutoff_eV;
@@ -1549,8 +1653,10 @@ inline void spawn_annihilation(StepContext const& ctx, Emitter& em, int32_t proc
     else local += e1;
     if (e2 >= gcut) em.secondary(2, process, x, y, z, g2x, g2y, g2z, e2, birth);
     else local += e2;
-    if (at_rest && k > 0.0) em.deposit_local(k);
-    if (local > 0.0) em.deposit_local(local);
+    if (at_rest && k > 0.0) em.deposit_local(k, r, z);
+    if (local > 0.0) em.deposit_local(local, r, z);
+    // the annihilated target electron leaves one positive ion
+    em.positive_ion(r, z, em.w, 0.0);
     em.fx.tallies[ORC_T_POSITIVE_IONS] += em.w;
     em.fx.tallies[ORC_T_ANNIHILATIONS] += 1.0;
 }
@@ -1719,7 +1825,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
 
     Emitter em{fx, ip, pkey, w, out.ion_pairs, pos ? 1 : 0};
     auto ion_event = [&](double ev_w) { em.ion_event(ev_w); };
-    auto deposit_local = [&](double energy) { em.deposit_local(energy); };
+    auto deposit_local = [&](double energy, double r, double z) { em.deposit_local(energy, r, z); };
     // the net chord carries the species' signed charge (-w for e-, +w for e+)
     auto segment = [&](double r0, double z0, double r1, double z1) {
         DepositOp op{0, {r0, z0, r1, z1, charge * w, 0.0, 0.0}, 0U};
@@ -1741,7 +1847,8 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
         // a low-energy electron source of the parent's weight
         // (AccumulateLowEnergyElectron, OV/RreaElectronTransport.cpp:152, 857)
         fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
-        deposit_local(k);
+        em.low_electron(rr, zz, w, 0.0);
+        deposit_local(k, rr, zz);
         segment(r_particle, z_particle, rr, zz);
         out.r = pr; out.z = pz; out.th = pth; out.ux = pux; out.uy = puy; out.uz = puz;
         out.status = ORC_DEMOTED;
@@ -1751,8 +1858,8 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                               double energy, double birth) {
         em.secondary(sp, process, x, y, z, sdx, sdy, sdz, energy, birth);
     };
-    auto annihilate = [&](double x, double y, double z, bool at_rest, uint64_t index, double birth) {
-        spawn_annihilation(ctx, em, kProcAnnihilation, x, y, z, dx, dy, dz, k, at_rest, index, birth);
+    auto annihilate = [&](double x, double y, double z, double r, bool at_rest, uint64_t index, double birth) {
+        spawn_annihilation(ctx, em, kProcAnnihilation, x, y, z, r, dx, dy, dz, k, at_rest, index, birth);
     };
     auto annihilated_at = [&](double r, double z) {
         segment(r_particle, z_particle, r, z);
@@ -1767,7 +1874,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
     if (!escape.escapes && below0) {
         if (pos) {
             // at rest where it stands, before any transport: a zero-length net chord
-            annihilate(x_particle, y_particle, z_particle, true, 0U, 1.0);
+            annihilate(x_particle, y_particle, z_particle, r_particle, true, 0U, 1.0);
             annihilated_at(r_particle, z_particle);
         } else {
             demote(r_particle, z_particle);
@@ -1883,6 +1990,9 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                 double const pw = w * loss_i / std::max(kWAir, 1.0);
                 if (!(pw > 0.0)) continue;
                 ion_event(pw);
+                double mx, my, mz;
+                path.point_at_full(0.5 * (lb + le), mx, my, mz);
+                em.ionization(std::hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
             }
         }
         fx.tallies[ORC_T_DRAG_LOSS_WEV] += w * drag;
@@ -1974,6 +2084,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                     dx = pdx; dy = pdy; dz = pdz;
                     push_secondary(0, kProcHardMoller, ex_, ey_, ez_, sdx, sdy, sdz, T, efrac);
                     ion_event(w);
+                    em.ionization(std::hypot(ex_, ey_), ez_, w, T, T);
                     fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
                 } else if (pos && h.channel == 0) {
                     // RreaApplyBhabhaFinalState OV/RreaChargedFinalStates.H:238-297: a
@@ -1989,6 +2100,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                         k = std::max(0.0, before - T);
                         dx = pdx; dy = pdy; dz = pdz;
                         push_secondary(0, kProcHardBhabha, ex_, ey_, ez_, sdx, sdy, sdz, T, efrac);
+                        em.positive_ion(std::hypot(ex_, ey_), ez_, w, T);
                         fx.tallies[ORC_T_POSITIVE_IONS] += w;
                         fx.tallies[ORC_T_BHABHA_EVENTS] += 1.0;
                     }
@@ -2014,13 +2126,13 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                         push_secondary(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
                         fx.tallies[ORC_T_BREMS_PHOTONS] += 1.0;
                     } else if (local > 0.0) {
-                        deposit_local(local);
+                        deposit_local(local, std::hypot(ex_, ey_), ez_);
                     }
                     fx.tallies[ORC_T_BREMS_EVENTS] += 1.0;
                 } else if (pos && h.channel == 2) {
                     // in-flight annihilation, OV/RreaPositronTransport.cpp:710-745: the
                     // parent is consumed at the event point
-                    annihilate(ex_, ey_, ez_, false, idx, efrac);
+                    annihilate(ex_, ey_, ez_, std::hypot(ex_, ey_), false, idx, efrac);
                     segment(r_particle, z_particle, er_, ez_);
                     annihilated = true;
                     break;
@@ -2085,7 +2197,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
             double const ay = terminated ? mid_y : run_y;
             double const az = terminated ? mid_z : run_z;
             double const ar = terminated ? mid_r : std::hypot(run_x, run_y);
-            annihilate(ax, ay, az, true, subcycles, terminated ? stop_elapsed : 1.0);
+            annihilate(ax, ay, az, std::hypot(ax, ay), true, subcycles, terminated ? stop_elapsed : 1.0);
             annihilated_at(ar, az);
             return;
         }
@@ -2432,6 +2544,7 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
     double const w = em.w;
     uint64_t const step = static_cast<uint64_t>(std::max<int64_t>(ctx.cfg.step, 0));
     auto U = [&](uint64_t chan) { return rng_uniform(ctx.cfg.rng_seed, em.pkey, step, idx, chan); };
+    double const sr = std::hypot(x, y);  // RreaInteractionSite::r_m of the event
     if (channel == kCompton) {
         double const incident = energy;
         ComptonState const st = sample_compton(t, incident, U(kChComptonRecoil), U(kChComptonScatter));
@@ -2444,14 +2557,16 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
         if (st.recoil >= cutoff) {
             em.secondary(0, kProcCompton, x, y, z, rx, ry, rz, st.recoil, birth);
             em.ion_event(w);
+            em.ionization(sr, z, w, st.recoil, st.recoil);
         } else {
             em.ion_event(w);
-            em.deposit_local(st.recoil);
+            em.ionization(sr, z, w, st.recoil, 0.0);
+            em.deposit_local(st.recoil, sr, z);
         }
         fx.tallies[ORC_T_COMPTON_EVENTS] += 1.0;
         energy = st.scattered;
         if (energy < gcut) {
-            em.deposit_local(energy);
+            em.deposit_local(energy, sr, z);
             return false;
         }
         dx = sx; dy = sy; dz = sz;
@@ -2466,11 +2581,13 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
                            U(kChPhotoelectricAzimuth), ex, ey, ez);
             em.secondary(0, kProcPhotoelectric, x, y, z, ex, ey, ez, ke, birth);
             em.ion_event(w);
+            em.ionization(sr, z, w, ke, ke);
         } else {
             em.ion_event(w);
-            em.deposit_local(ke);
+            em.ionization(sr, z, w, ke, 0.0);
+            em.deposit_local(ke, sr, z);
         }
-        if (st.binding > 0.0) em.deposit_local(st.binding);
+        if (st.binding > 0.0) em.deposit_local(st.binding, sr, z);
         fx.tallies[ORC_T_PHOTOELECTRIC_EVENTS] += 1.0;
         return false;
     }
@@ -2486,21 +2603,23 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
         em.secondary(0, process, x, y, z, ex, ey, ez, pair.electron, birth);
     } else {
         fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
-        em.deposit_local(pair.electron);
+        em.low_electron(sr, z, w, 0.0);
+        em.deposit_local(pair.electron, sr, z);
     }
     if (pair.positron >= cutoff) {
         em.secondary(1, process, x, y, z, px, py, pz2, pair.positron, birth);
     } else {
-        spawn_annihilation(ctx, em, process, x, y, z, px, py, pz2, pair.positron, true, idx, birth);
+        spawn_annihilation(ctx, em, process, x, y, z, sr, px, py, pz2, pair.positron, true, idx, birth);
     }
     if (pair.has_recoil) {
         double qx, qy, qz;
         rotate_to_axis(dx, dy, dz, pair.rdx, pair.rdy, pair.rdz, qx, qy, qz);
         if (pair.recoil >= cutoff) em.secondary(0, process, x, y, z, qx, qy, qz, pair.recoil, birth);
         em.ion_event(w);
-        if (pair.recoil < cutoff) em.deposit_local(pair.recoil);
+        em.ionization(sr, z, w, pair.recoil, 0.0);
+        if (pair.recoil < cutoff) em.deposit_local(pair.recoil, sr, z);
     }
-    if (pair.local > 0.0) em.deposit_local(pair.local);
+    if (pair.local > 0.0) em.deposit_local(pair.local, sr, z);
     fx.tallies[ORC_T_PAIR_EVENTS] += 1.0;
     return false;
 }
@@ -2545,7 +2664,7 @@ inline void photon_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, do
             // a photon below the production cutoff deposits at its path start
             double x, y, z;
             path.point_at_full(path.begin_f, x, y, z);
-            em.deposit_local(energy);
+            em.deposit_local(energy, std::hypot(x, y), z);
             out.status = w <= 0.0 ? ORC_ZERO_WEIGHT : ORC_DEMOTED;
             if (w > 0.0) fx.tallies[ORC_T_DEMOTIONS] += 1.0;
             return;
@@ -2743,12 +2862,13 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
     bool const birth_demote = pos ? k < cutoff : newborn_electron_demotes(ctx, p.x, p.y, p.z, k, hm_on);
     if (birth_demote) {
         if (pos) {
-            spawn_annihilation(ctx, em, kProcAnnihilation, p.x, p.y, p.z, dx, dy, dz, k, true, pack(0, 0, 0),
-                               p.birth_fraction);
+            spawn_annihilation(ctx, em, kProcAnnihilation, p.x, p.y, p.z, std::hypot(p.x, p.y), dx, dy, dz, k, true,
+                               pack(0, 0, 0), p.birth_fraction);
             out.status = ORC_ANNIHILATED;
         } else {
             fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
-            em.deposit_local(k);
+            em.low_electron(std::hypot(p.x, p.y), p.z, w, 0.0);
+            em.deposit_local(k, std::hypot(p.x, p.y), p.z);
             out.status = ORC_DEMOTED;
         }
         fx.tallies[ORC_T_NEWBORN_TERMINATED] += 1.0;
@@ -2853,6 +2973,9 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                 double const pw = w * loss_i / std::max(kWAir, 1.0);
                 if (!(pw > 0.0)) continue;
                 em.ion_event(pw);
+                double mx, my, mz;
+                path.point_at_full(0.5 * (lb + le), mx, my, mz);
+                em.ionization(std::hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
             }
         }
         fx.tallies[ORC_T_DRAG_LOSS_WEV] += w * drag;
@@ -2945,6 +3068,7 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                     dx = pdx; dy = pdy; dz = pdz;
                     em.secondary(0, kProcHardMoller, ex_, ey_, ez_, sdx, sdy, sdz, T, efrac);
                     em.ion_event(w);
+                    em.ionization(std::hypot(ex_, ey_), ez_, w, T, T);
                     fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
                 } else if (pos && h.channel == 0) {
                     double const before = k;
@@ -2958,6 +3082,7 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                         k = std::max(0.0, before - T);
                         dx = pdx; dy = pdy; dz = pdz;
                         em.secondary(0, kProcHardBhabha, ex_, ey_, ez_, sdx, sdy, sdz, T, efrac);
+                        em.positive_ion(std::hypot(ex_, ey_), ez_, w, T);
                         fx.tallies[ORC_T_POSITIVE_IONS] += w;
                         fx.tallies[ORC_T_BHABHA_EVENTS] += 1.0;
                     }
@@ -2981,12 +3106,13 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                         em.secondary(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
                         fx.tallies[ORC_T_BREMS_PHOTONS] += 1.0;
                     } else if (local > 0.0) {
-                        em.deposit_local(local);
+                        em.deposit_local(local, std::hypot(ex_, ey_), ez_);
                     }
                     fx.tallies[ORC_T_BREMS_EVENTS] += 1.0;
                 } else if (pos && h.channel == 2) {
                     // in flight: the positron is consumed at the event point
-                    spawn_annihilation(ctx, em, kProcAnnihilation, ex_, ey_, ez_, dx, dy, dz, k, false, idx, efrac);
+                    spawn_annihilation(ctx, em, kProcAnnihilation, ex_, ey_, ez_, std::hypot(ex_, ey_), dx, dy, dz, k,
+                                       false, idx, efrac);
                     segment(er_, ez_);
                     terminal_interaction = true;
                     final_f = ef;
@@ -3055,12 +3181,13 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
     bool const demote = pos ? k < cutoff : newborn_electron_demotes(ctx, fx_, fy_, fz_, k, hm_on);
     if (terminal_at_point || demote) {
         if (pos) {
-            spawn_annihilation(ctx, em, kProcAnnihilation, fx_, fy_, fz_, dx, dy, dz, k, true,
+            spawn_annihilation(ctx, em, kProcAnnihilation, fx_, fy_, fz_, final_r, dx, dy, dz, k, true,
                                pack(std::min<uint64_t>(substeps, kSegmentCap - 1U), 0U, 0U), final_el);
             out.status = ORC_ANNIHILATED;
         } else {
             fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
-            em.deposit_local(k);
+            em.low_electron(final_r, fz_, w, 0.0);
+            em.deposit_local(k, final_r, fz_);
             out.status = ORC_DEMOTED;
         }
         segment(final_r, fz_);
@@ -3103,7 +3230,7 @@ inline void photon_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary& 
     normalize(dx, dy, dz);
     double energy = p.energy_eV;
     if (energy < gcut) {
-        em.deposit_local(energy);
+        em.deposit_local(energy, std::hypot(p.x, p.y), p.z);
         out.status = ORC_DEMOTED;
         fx.tallies[ORC_T_NEWBORN_TERMINATED] += 1.0;
         return;
@@ -3467,6 +3594,8 @@ static int charged_step(bool positron, void* h, const orc_geometry* g, const orc
                 }
             }
         }
+        int const se = replay_sources(g, cfg->low_energy_cutoff_eV, fx.sources);
+        if (se) { *n_sec = ns; return se; }
     }
     *n_sec = ns;
     return 0;
@@ -3572,6 +3701,8 @@ int orc_photon_step(void* h, const orc_geometry* g, const orc_config* cfg, int64
             sec[ns++] = s;
         }
         for (int q = 0; q < ORC_T_COUNT; ++q) tallies[q] += fx.tallies[q];
+        int const se = replay_sources(g, cfg->low_energy_cutoff_eV, fx.sources);
+        if (se) { *n_sec = ns; return se; }
     }
     *n_sec = ns;
     return 0;
@@ -3763,6 +3894,9 @@ extern "C" int orc_newborn_drain(void* h, const orc_geometry* g, const orc_confi
                     fail(ORC_ERR_INVALID, "newborn escape deposit rejected");
                 }
             }
+            if (int const se = replay_sources(g, cfg->low_energy_cutoff_eV, fx.sources)) {
+                fail(se, "newborn material source outside the half-open domain");
+            }
             st.push_back(o.status);
             ion.push_back(o.ion_pairs);
             sub.push_back(static_cast<int64_t>(o.substeps));
@@ -3814,4 +3948,37 @@ extern "C" int orc_electron_state_demotes(double energy, double cutoff, double t
 extern "C" double orc_newborn_elapsed(double birth, uint64_t completed, uint64_t substeps, double within)
 {
     return newborn_elapsed(birth, newborn_residual_fraction(birth), completed, substeps, within);
+}
+
+extern "C" int orc_accumulate_source(const orc_geometry* g, double cutoff, int kind, double r, double z, double weight,
+                                     double secondary, double loss)
+{
+    if (!g || !g->sources || kind < 0 || kind > 2) return ORC_ERR_INVALID;
+    return accumulate_source(g, cutoff, SourceOp{kind, r, z, weight, secondary, loss});
+}
+
+// NormalizeAndSync :280-345: every field over its RZ shell cell volume
+// (rz_shell_volume, NA/include/rrea/RreaConstants.H:37-43), the three
+// source rates also over dt
+extern "C" int orc_sources_normalize(const orc_geometry* g, double dt, int diagnostics)
+{
+    if (!g || !g->sources || !(dt > 0.0)) return ORC_ERR_INVALID;
+    int64_t const cells = static_cast<int64_t>(g->nr) * g->nz;
+    double const pi = 3.141592653589793238462643383279502884;
+    for (int j = 0; j < g->nz; ++j) {
+        for (int i = 0; i < g->nr; ++i) {
+            double const r_inner = g->r_lo + static_cast<double>(i - 0) * g->dr;
+            double const inner = r_inner > 0.0 ? r_inner : 0.0;
+            double const outer = r_inner + g->dr > 0.0 ? r_inner + g->dr : 0.0;
+            double const volume = pi * (outer * outer - inner * inner) * g->dz;
+            for (int f = 0; f < 5; ++f) {
+                bool const rate = f <= 2;
+                if (!rate && !diagnostics) continue;
+                double const denom = rate ? volume * dt : volume;
+                double& v = g->sources[f * cells + i + static_cast<int64_t>(g->nr) * j];
+                v = (denom > 0.0) ? v / denom : 0.0;
+            }
+        }
+    }
+    return 0;
 }

@@ -38,6 +38,10 @@ typedef struct orc_geometry {
     const double* profile_alt_m;
     const double* profile_rho;
     double altitude_at_z0_m;
+    /* material-source grids (5 x nr x nz: low-energy electron, positive ion,
+       direct negative ion, event count, energy loss; c = i + nr*j), raw
+       weights accumulated by the steps and the drain; NULL: none */
+    double* sources;
 } orc_geometry;
 
 typedef struct orc_config {
@@ -121,6 +125,15 @@ enum {
     ORC_ERR_INVALID = 4,
     ORC_ERR_SECONDARY_CAPACITY = 5,
 };
+
+/* One material-source record into g->sources: kind 0 ionization event,
+   1 positive ion only, 2 low-energy electron (RreaWarpXCoupling::
+   Accumulate*, OV/RreaCouplingSources.cpp:435-516).  0, or the error code of
+   a record outside the physical domain. */
+int orc_accumulate_source(const orc_geometry* g, double cutoff, int kind, double r, double z, double weight,
+                          double secondary, double loss);
+/* RreaEventAccumulator::NormalizeAndSync on g->sources. */
+int orc_sources_normalize(const orc_geometry* g, double dt, int diagnostics);
 
 #ifdef __cplusplus
 }

@@ -113,7 +113,7 @@ class StepOut(C.Structure):
         ("status", C.c_void_p), ("ion_pairs", C.c_void_p), ("substeps", C.c_void_p),
         ("secondaries", C.c_void_p), ("secondary_capacity", C.c_int64), ("secondary_count", C.c_void_p),
         ("jr", C.c_void_p), ("jz", C.c_void_p), ("rho", C.c_void_p), ("tallies", C.c_void_p),
-        ("error", C.c_void_p), ("order_secondaries", C.c_int32),
+        ("error", C.c_void_p), ("order_secondaries", C.c_int32), ("sources", C.c_void_p),
     ]
 
 
@@ -123,11 +123,12 @@ class DrainIO(C.Structure):
         ("status", C.c_void_p), ("ion_pairs", C.c_void_p), ("substeps", C.c_void_p), ("survivors", C.c_void_p),
         ("survivor_capacity", C.c_int64), ("survivor_count", C.c_void_p),
         ("jr", C.c_void_p), ("jz", C.c_void_p), ("rho", C.c_void_p), ("tallies", C.c_void_p),
-        ("error", C.c_void_p),
+        ("error", C.c_void_p), ("sources", C.c_void_p),
     ]
 
 
 TALLY_COUNT = 24
+SOURCE_FIELDS = 5
 STATUS = {0: "survive", 1: "escape", 2: "demoted", 3: "zero_weight", 4: "no_writeback", 5: "skipped_invalid",
           6: "annihilated", 7: "absorbed"}
 INVALID_ID = -(2**63)
@@ -170,6 +171,8 @@ SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(DriftConfig)] + [C.c_void_p] * 9),
     ("rrea_drift_adjoint", C.c_int32,
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(DriftConfig)] + [C.c_void_p] * 7),
+    ("rrea_sources_normalize", C.c_int32, [C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_void_p]),
+    ("rrea_batch_bind_sources", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),

@@ -23,6 +23,7 @@ constexpr double kMeC2 = 510998.95000;           // NA/include/rrea/RreaConstant
 constexpr double kC = 299792458.0;               // RreaConstants.H:10
 constexpr double kWAir = 33.97;                  // RreaConstants.H:30
 constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kPi = 3.141592653589793238462643383279502884;  // NA/include/rrea/RreaConstants.H:8
 constexpr double kEps = 2.220446049250313e-16;   // DBL_EPSILON
 constexpr double kDblMin = 2.2250738585072014e-308;
 // Two-point Gauss-Legendre node 1/sqrt(3) (NA/include/rrea/RreaGaussLegendre8.H:29-30),

@@ -233,6 +233,7 @@ struct rrea_batch_s {
     std::vector<rrea_fields_t> fields;
     std::vector<double*> jr, jz, rho;
     std::vector<uint64_t> seeds;
+    std::vector<double*> src;       // each instance's material-source grids (empty: none)
     rrea_fields_t work = nullptr;   // workspace of the batched calls (kernel arguments, lists, timing)
     void* d_envs = nullptr;         // one environment per instance
     std::vector<unsigned char> h_envs;
@@ -288,7 +289,99 @@ struct Env {
     int32_t pnr, pzb, pnz;
     int32_t instance;         // -1 for a plain step
     uint64_t rng_seed;
+    double* src;              // material-source grids (5 x nr x nz), or null: none accumulated
 };
+
+// ---------------------------------------------------------------------------
+// Material sources: the coupling's AccumulateIonizationEvent /
+// AccumulatePositiveIonSourceOnly / AccumulateLowEnergyElectron
+// (OV/RreaCouplingSources.cpp:435-516) into the event accumulator's fields
+// (RreaEventAccumulator, NA/src/RreaEventAccumulator.cpp): 0 low-energy
+// electron source, 1 positive-ion source, 2 direct negative-ion source,
+// 3 ionization event count, 4 energy loss.  Raw weights here; the division
+// by cell volume (and dt) is rrea_sources_normalize.
+// ---------------------------------------------------------------------------
+enum : int32_t { kSrcLowElectron = 0, kSrcPositiveIon, kSrcNegativeIon, kSrcEventCount, kSrcEnergyLoss, kSrcFields };
+
+// AddToField :386-512 on a non-periodic RZ grid from the axis: the order-1
+// shape at cell centres, a radial ghost below the axis folded onto the first
+// cell, support beyond the other boundaries dropped.
+__device__ void source_add(Env const& E, int32_t field, double r, double z, double value)
+{
+    if (!finite(value) || value == 0.0) return;
+    DevFields const& F = E.F;
+    int32_t const ic = static_cast<int32_t>(floor((r - F.r_lo) / F.dr));
+    int32_t const jc = static_cast<int32_t>(floor((z - F.z_lo) / F.dz));
+    if (ic < 0 || ic >= F.nr || jc < 0 || jc >= F.nz) return;
+    double const r_cell = (r - F.r_lo) / F.dr + 0.0 - 0.5;
+    int32_t const i0 = static_cast<int32_t>(floor(r_cell));
+    double const wr1 = r_cell - static_cast<double>(i0);
+    double const wr0 = 1.0 - wr1;
+    double const z_cell = (z - F.z_lo) / F.dz + 0.0 - 0.5;
+    int32_t const j0 = static_cast<int32_t>(floor(z_cell));
+    double const wz1 = z_cell - static_cast<double>(j0);
+    double const wz0 = 1.0 - wz1;
+    double* const dst = E.src + static_cast<int64_t>(field) * F.nr * F.nz;
+    for (int rz = 0; rz < 2; ++rz) {
+        double const wz = rz == 0 ? wz0 : wz1;
+        if (wz == 0.0) continue;
+        int32_t const j = j0 + rz;
+        if (j < 0 || j > F.nz - 1) continue;
+        for (int rr = 0; rr < 2; ++rr) {
+            double const wr = rr == 0 ? wr0 : wr1;
+            double const v = value * wr * wz;
+            if (v == 0.0) continue;
+            int32_t i = i0 + rr;
+            if (i < 0 || i > F.nr - 1) {
+                if (i == -1 && F.r_lo == 0.0) i = 0;
+                else continue;
+            }
+            atomic_add(&dst[i + static_cast<int64_t>(F.nr) * j], v);
+        }
+    }
+}
+
+// RreaRzPositionInHalfOpenDomain: a source outside the physical domain is
+// the reference's abort, reported as an outside-domain error
+__device__ __forceinline__ bool source_in_domain(Env const& E, double r, double z)
+{
+    return r >= E.D.r_lo && r < E.D.r_hi && z >= E.D.z_lo && z < E.D.z_hi;
+}
+
+__device__ void source_diagnostic(Env const& E, double r, double z, double weight, double loss)
+{
+    source_add(E, kSrcEventCount, r, z, weight);
+    source_add(E, kSrcEnergyLoss, r, z, dmax(weight, 0.0) * dmax(loss, 0.0));
+}
+
+// AccumulateIonizationEvent :435-464
+__device__ void source_ionization(Env const& E, double cutoff, double r, double z, double weight, double secondary,
+                                  double loss, int32_t& err)
+{
+    if (!E.src || weight <= 0.0) return;
+    if (!source_in_domain(E, r, z)) { if (!err) err = kErrDomain; return; }
+    source_add(E, kSrcPositiveIon, r, z, weight);
+    if (secondary < cutoff) source_add(E, kSrcLowElectron, r, z, weight);
+    source_diagnostic(E, r, z, weight, loss);
+}
+
+// AccumulatePositiveIonSourceOnly :466-487
+__device__ void source_positive_ion(Env const& E, double r, double z, double weight, double loss, int32_t& err)
+{
+    if (!E.src || weight <= 0.0) return;
+    if (!source_in_domain(E, r, z)) { if (!err) err = kErrDomain; return; }
+    source_add(E, kSrcPositiveIon, r, z, weight);
+    source_diagnostic(E, r, z, weight, loss);
+}
+
+// AccumulateLowEnergyElectron :489-516
+__device__ void source_low_electron(Env const& E, double r, double z, double weight, double loss, int32_t& err)
+{
+    if (!E.src || weight <= 0.0) return;
+    if (!source_in_domain(E, r, z)) { if (!err) err = kErrDomain; return; }
+    source_add(E, kSrcLowElectron, r, z, weight);
+    source_diagnostic(E, r, z, weight, dmax(loss, 0.0));
+}
 
 struct StepArgs {
     DevTables T;
@@ -347,13 +440,29 @@ struct Emitter {
     int32_t root_species;   // species of the tracked root of every record made here
     int32_t generation;     // FIFO generation of the records made here
     int32_t instance;       // the simulation instance (-1: a plain step)
+    Env const* env;         // where material sources go
 
-    __device__ void deposit_local(double energy)
+    // deposit_local_energy_as_ion_pairs: E / W_air pairs booked as one
+    // ionization event of diagnostic loss W_air at (r, z)
+    __device__ void deposit_local(double energy, double r, double z)
     {
         if (w <= 0.0 || energy <= 0.0) return;
         double const pw = w * energy / dmax(kWAir, 1.0);
         if (pw <= 0.0) return;
         ion += pw;
+        source_ionization(*env, A.cfg.low_energy_cutoff_eV, r, z, pw, 0.0, kWAir, err);
+    }
+    __device__ void ionization(double r, double z, double weight, double secondary, double loss)
+    {
+        source_ionization(*env, A.cfg.low_energy_cutoff_eV, r, z, weight, secondary, loss, err);
+    }
+    __device__ void positive_ion(double r, double z, double weight, double loss)
+    {
+        source_positive_ion(*env, r, z, weight, loss, err);
+    }
+    __device__ void low_electron(double r, double z, double weight, double loss)
+    {
+        source_low_electron(*env, r, z, weight, loss, err);
     }
     // Out of line: inlined into the photon step, the species word of the
     // record was taken from a register shared across the call sites and came
@@ -409,8 +518,11 @@ __device__ void spawn_annihilation(DevTables const& T, double gcut, uint64_t rng
     else local += e1;
     if (e2 >= gcut) em.secondary(2, process, x, y, z, g2x, g2y, g2z, e2, birth);
     else local += e2;
-    if (at_rest && k > 0.0) em.deposit_local(k);
-    if (local > 0.0) em.deposit_local(local);
+    double const sr = hypot(x, y);
+    if (at_rest && k > 0.0) em.deposit_local(k, sr, z);
+    if (local > 0.0) em.deposit_local(local, sr, z);
+    // the annihilated target electron leaves one positive ion
+    em.positive_ion(sr, z, em.w, 0.0);
     tally[12] += em.w;
     tally[10] += 1.0;
 }
@@ -662,8 +774,8 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
             }
             double tr_dp = 0.0, tr_len = 0.0;
 
-            Emitter em{A, ip, pkey, w, ion, lineage, err, kPos ? 1 : 0, 0, E.instance};
-            auto deposit_local = [&](double energy) { em.deposit_local(energy); };
+            Emitter em{A, ip, pkey, w, ion, lineage, err, kPos ? 1 : 0, 0, E.instance, &E};
+            auto deposit_local = [&](double energy, double r, double z) { em.deposit_local(energy, r, z); };
             auto push = [&](int32_t species, int32_t process, double x, double y, double z, double sdx, double sdy,
                             double sdz, double energy, double birth) {
                 em.secondary(species, process, x, y, z, sdx, sdy, sdz, energy, birth);
@@ -823,7 +935,12 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                             double const raw_i = path.full * (le - lb) * iv.ratio * col;
                             double const loss_i = col_loss * raw_i / raw_col;
                             double const pw = w * loss_i / dmax(kWAir, 1.0);
-                            if (pw > 0.0) ion += pw;
+                            if (pw > 0.0) {
+                                ion += pw;
+                                double mx, my, mz;
+                                point_at_full(path, 0.5 * (lb + le), mx, my, mz);
+                                em.ionization(hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
+                            }
                         }
                     }
                     tally[7] += w * drag;
@@ -938,6 +1055,7 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                                         k = dmax(0.0, before - T2);
                                         dx = ndx; dy = ndy; dz = ndz;
                                         push(0, kProcHardBhabha, ex_, ey_, ez_, sdx, sdy, sdz, T2, efrac);
+                                        em.positive_ion(hypot(ex_, ey_), ez_, w, T2);
                                         tally[12] += w;
                                         tally[11] += 1.0;
                                     }
@@ -952,6 +1070,7 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                                     dx = ndx; dy = ndy; dz = ndz;
                                     push(0, kProcHardMoller, ex_, ey_, ez_, sdx, sdy, sdz, T2, efrac);
                                     ion += w;
+                                    em.ionization(hypot(ex_, ey_), ez_, w, T2, T2);
                                     tally[3] += 1.0;
                                 }
                             } else if (h.channel == 1) {
@@ -986,7 +1105,7 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                                     push(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
                                     tally[5] += 1.0;
                                 } else if (pe > 0.0) {
-                                    deposit_local(pe);
+                                    deposit_local(pe, hypot(ex_, ey_), ez_);
                                 }
                                 tally[4] += 1.0;
                             } else if (kPos && h.channel == 2) {
@@ -1139,7 +1258,8 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                     // a low-energy electron source of the parent's weight
                     // (AccumulateLowEnergyElectron, OV/RreaElectronTransport.cpp:152, 857)
                     tally[16] += w;
-                    deposit_local(k);
+                    em.low_electron(demote_r, demote_z, w, 0.0);
+                    deposit_local(k, demote_r, demote_z);
                     if (w != 0.0 && !deposit_segment(E.G, -w, dt, r_particle, z_particle, demote_r, demote_z)) err = kErrInvalid;
                     status = RREA_DEMOTED;
                     tally[2] += 1.0;
@@ -1189,6 +1309,7 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
     int32_t& err = em.err;
     double const w = em.w;
     double& ion = em.ion;
+    double const sr = hypot(x, y);  // the event site's radius (material sources)
     if (channel == kCompton) {
         double const incident = energy;
         double scattered, recoil, ct;
@@ -1203,14 +1324,16 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
         if (recoil >= cutoff) {
             em.secondary(0, kProcCompton, x, y, z, rx, ry, rz, recoil, birth);
             ion += w;
+            em.ionization(sr, z, w, recoil, recoil);
         } else {
             ion += w;
-            em.deposit_local(recoil);
+            em.ionization(sr, z, w, recoil, 0.0);
+            em.deposit_local(recoil, sr, z);
         }
         tally[13] += 1.0;
         energy = scattered;
         if (energy < gcut) {
-            em.deposit_local(energy);
+            em.deposit_local(energy, sr, z);
             return false;
         }
         dx = sx; dy = sy; dz = sz;
@@ -1230,11 +1353,13 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
                            rng_uniform(rng, idx, kChPhotoelectricAzimuth), ex, ey, ez);
             em.secondary(0, kProcPhotoelectric, x, y, z, ex, ey, ez, ke, birth);
             ion += w;
+            em.ionization(sr, z, w, ke, ke);
         } else {
             ion += w;
-            em.deposit_local(ke);
+            em.ionization(sr, z, w, ke, 0.0);
+            em.deposit_local(ke, sr, z);
         }
-        if (binding > 0.0) em.deposit_local(binding);
+        if (binding > 0.0) em.deposit_local(binding, sr, z);
         tally[14] += 1.0;
         return false;
     }
@@ -1252,7 +1377,8 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
         em.secondary(0, process, x, y, z, ex, ey, ez, pair.electron, birth);
     } else {
         tally[16] += w;
-        em.deposit_local(pair.electron);
+        em.low_electron(sr, z, w, 0.0);
+        em.deposit_local(pair.electron, sr, z);
     }
     if (pair.positron >= cutoff) {
         em.secondary(1, process, x, y, z, px, py, pz2, pair.positron, birth);
@@ -1264,9 +1390,10 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
         rotate_to_axis(dx, dy, dz, pair.rdx, pair.rdy, pair.rdz, qx, qy, qz);
         if (pair.recoil >= cutoff) em.secondary(0, process, x, y, z, qx, qy, qz, pair.recoil, birth);
         ion += w;
-        if (pair.recoil < cutoff) em.deposit_local(pair.recoil);
+        em.ionization(sr, z, w, pair.recoil, 0.0);
+        if (pair.recoil < cutoff) em.deposit_local(pair.recoil, sr, z);
     }
-    if (pair.local > 0.0) em.deposit_local(pair.local);
+    if (pair.local > 0.0) em.deposit_local(pair.local, sr, z);
     tally[15] += 1.0;
     return false;
 }
@@ -1311,7 +1438,7 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
     double ion = 0.0;
     uint64_t lineage = 0;
     uint64_t events = 0;
-    Emitter em{A, ip, pkey, w, ion, lineage, err, 2, 0, E.instance};
+    Emitter em{A, ip, pkey, w, ion, lineage, err, 2, 0, E.instance, &E};
 
     double energy = sqrt(ux * ux + uy * uy + uz * uz);
     double dx = ux, dy = uy, dz = uz;
@@ -1332,7 +1459,7 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
         if (err) break;
         if (w <= 0.0 || energy < gcut) {
             // below the production cutoff: deposited where the path starts
-            em.deposit_local(energy);
+            em.deposit_local(energy, hypot(path.sx, path.sy), path.sz);
             status = w <= 0.0 ? RREA_ZERO_WEIGHT : RREA_DEMOTED;
             if (w > 0.0) tally[2] += 1.0;
             break;
@@ -1753,6 +1880,15 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
             // ---- commit: nothing above has written anything ----
             slow = false;
             dep = true;
+            if (ion > 0.0 && E.src) {
+                // the ion pairs' record at the interval midpoint; the one
+                // interval ends where the substep does (iv.end == se here)
+                double mx, my, mz;
+                point_at_full(path, 0.5 * (0.0 + se), mx, my, mz);
+                int32_t serr = kOk;
+                source_ionization(E, cutoff, hypot(mx, my), mz, ion, 0.5 * cutoff, kWAir, serr);
+                if (serr) report(A.error, serr, ip);
+            }
             dep_inst = E.instance;
             dep_q = S::kCharge * w;
             dep_r0 = pr; dep_z0 = pz; dep_r1 = end_r; dep_z1 = z;
@@ -2074,7 +2210,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
     uint64_t const lin = p.lineage;
     double const w = p.weight;
     uint64_t const rng = rng_prefix(E.rng_seed, p.root_key, step);
-    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1, E.instance};
+    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1, E.instance, &E};
     double const birth_r = hypot(p.x, p.y), birth_z = p.z;
     double const residual = newborn_residual(p.birth_fraction);
     double k = p.energy_eV;
@@ -2102,7 +2238,8 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
             status = RREA_ANNIHILATED;
         } else {
             tally[16] += w;
-            em.deposit_local(k);
+            em.low_electron(hypot(x, y), z, w, 0.0);
+            em.deposit_local(k, hypot(x, y), z);
             status = RREA_DEMOTED;
         }
         tally[20] += 1.0;
@@ -2214,7 +2351,12 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                             double const raw_i = path.full * (le - lb) * iv.ratio * col;
                             double const loss_i = col_loss * raw_i / raw_col;
                             double const pw = w * loss_i / dmax(kWAir, 1.0);
-                            if (pw > 0.0) ion += pw;
+                            if (pw > 0.0) {
+                                ion += pw;
+                                double mx, my, mz;
+                                point_at_full(path, 0.5 * (lb + le), mx, my, mz);
+                                em.ionization(hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
+                            }
                         }
                     }
                     tally[7] += w * drag;
@@ -2327,6 +2469,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                                         k = dmax(0.0, before - T2);
                                         dx = ndx; dy = ndy; dz = ndz;
                                         em.secondary(0, kProcHardBhabha, ex_, ey_, ez_, sdx, sdy, sdz, T2, efrac);
+                                        em.positive_ion(hypot(ex_, ey_), ez_, w, T2);
                                         tally[12] += w;
                                         tally[11] += 1.0;
                                     }
@@ -2341,6 +2484,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                                     dx = ndx; dy = ndy; dz = ndz;
                                     em.secondary(0, kProcHardMoller, ex_, ey_, ez_, sdx, sdy, sdz, T2, efrac);
                                     ion += w;
+                                    em.ionization(hypot(ex_, ey_), ez_, w, T2, T2);
                                     tally[3] += 1.0;
                                 }
                             } else if (h.channel == 1) {
@@ -2375,7 +2519,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                                     em.secondary(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
                                     tally[5] += 1.0;
                                 } else if (pe > 0.0) {
-                                    em.deposit_local(pe);
+                                    em.deposit_local(pe, hypot(ex_, ey_), ez_);
                                 }
                                 tally[4] += 1.0;
                             } else if (kPos && h.channel == 2) {
@@ -2516,7 +2660,7 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
     uint64_t const rsp = static_cast<uint64_t>(p.root_species);
     double const w = p.weight;
     uint64_t const rng = rng_prefix(E.rng_seed, p.root_key, step);
-    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1, E.instance};
+    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1, E.instance, &E};
     double const residual = newborn_residual(p.birth_fraction);
     double dx = p.dir_x, dy = p.dir_y, dz = p.dir_z;
     normalize(dx, dy, dz);
@@ -2535,7 +2679,7 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
     } else if (!in_half_open(p.x, p.y, p.z, D)) {
         err = kErrDomain;
     } else if (energy < gcut) {
-        em.deposit_local(energy);
+        em.deposit_local(energy, hypot(p.x, p.y), p.z);
         status = RREA_DEMOTED;
         tally[20] += 1.0;
     } else {
@@ -3362,7 +3506,8 @@ rrea_status slow_list_workspace(rrea_fields_t F, int64_t n, hipStream_t s)
 
 // One simulation's environment from its fields handle: fields, domain, the
 // reference-z density ratio, the padded J grid and the RNG seed.
-rrea_status make_env(rrea_fields_t F, double* jr, double* jz, double* rho, int32_t instance, uint64_t seed, Env& E)
+rrea_status make_env(rrea_fields_t F, double* jr, double* jz, double* rho, int32_t instance, uint64_t seed, Env& E,
+                     double* src = nullptr)
 {
     rrea_rz_geometry const& g = F->geom;
     E.F.nr = g.nr; E.F.nz = g.nz; E.F.dr = g.dr; E.F.dz = g.dz; E.F.r_lo = g.r_lo; E.F.z_lo = g.z_lo;
@@ -3401,19 +3546,20 @@ rrea_status make_env(rrea_fields_t F, double* jr, double* jz, double* rho, int32
     E.pnr = g.nr; E.pzb = pad; E.pnz = g.nz;
     E.instance = instance;
     E.rng_seed = seed;
+    E.src = src;
     return RREA_OK;
 }
 
 // The argument block every transport kernel shares: tables, the plain step's
 // environment and the step configuration.
 rrea_status common_args(rrea_tables_t T, rrea_fields_t F, const rrea_step_config* cfg, double* jr, double* jz,
-                        double* rho, StepArgs& A)
+                        double* rho, StepArgs& A, double* src = nullptr)
 {
     A.T = T->view;
     A.batch = nullptr;
     A.inst = nullptr;
     A.cfg = *cfg;
-    return make_env(F, jr, jz, rho, -1, cfg->rng_seed, A.env);
+    return make_env(F, jr, jz, rho, -1, cfg->rng_seed, A.env, src);
 }
 
 // Every instance's environment, formed on the host from its fields handle
@@ -3426,7 +3572,7 @@ rrea_status upload_envs(rrea_batch_t B, const rrea_step_config* cfg, hipStream_t
     for (size_t i = 0; i < n; ++i) {
         if (!B->fields[i]->fields_set) return RREA_ERR_ARGUMENT;
         rrea_status const st = make_env(B->fields[i], B->jr[i], B->jz[i], B->rho[i], static_cast<int32_t>(i),
-                                        B->seeds[i], h[i]);
+                                        B->seeds[i], h[i], B->src.empty() ? nullptr : B->src[i]);
         if (st != RREA_OK) return st;
     }
     // pageable source: the copy has consumed the host array when hipMemcpyAsync returns
@@ -3478,7 +3624,7 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     rrea_fields_t const W = batched ? batch->work : F;  // the call's workspace
     StepArgs A;
     {
-        rrea_status const cs = common_args(T, F, cfg, out->jr, out->jz, out->rho, A);
+        rrea_status const cs = common_args(T, F, cfg, out->jr, out->jz, out->rho, A, batched ? nullptr : out->sources);
         if (cs != RREA_OK) return cs;
     }
     if (batched) {
@@ -3648,7 +3794,7 @@ rrea_status drain_impl(rrea_tables_t T, rrea_fields_t F, const rrea_step_config*
     rrea_fields_t const W = batch ? batch->work : F;  // the call's workspace
     StepArgs A{};
     {
-        rrea_status const cs = common_args(T, F, cfg, io->jr, io->jz, io->rho, A);
+        rrea_status const cs = common_args(T, F, cfg, io->jr, io->jz, io->rho, A, batch ? nullptr : io->sources);
         if (cs != RREA_OK) return cs;
     }
     A.tallies = io->tallies;
@@ -3978,6 +4124,29 @@ __global__ void face_gradient_kernel(int32_t nr, int32_t nz, const double* __res
     }
 }
 
+// RreaEventAccumulator::NormalizeAndSync :280-345: each source field over
+// its RZ shell cell volume pi (r_o^2 - r_i^2) dz (rz_shell_volume,
+// NA/include/rrea/RreaConstants.H:37-43), the three source rates also over dt.
+__global__ void sources_normalize_kernel(int32_t nr, int32_t nz, double r_lo, double dr, double dz, double dt,
+                                         int32_t diagnostics, double* __restrict__ src)
+{
+    int64_t const c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t const cells = static_cast<int64_t>(nr) * nz;
+    if (c >= cells) return;
+    int32_t const i = static_cast<int32_t>(c % nr);
+    double const r_inner = r_lo + static_cast<double>(i - 0) * dr;
+    double const inner = r_inner > 0.0 ? r_inner : 0.0;
+    double const outer = r_inner + dr > 0.0 ? r_inner + dr : 0.0;
+    double const volume = kPi * (outer * outer - inner * inner) * dz;
+    for (int32_t f = 0; f < kSrcFields; ++f) {
+        bool const rate = f <= kSrcNegativeIon;
+        if (!rate && !diagnostics) continue;
+        double const denom = rate ? volume * dt : volume;
+        double& v = src[f * cells + c];
+        v = (denom > 0.0) ? v / denom : 0.0;
+    }
+}
+
 rrea_status drift_args(rrea_tables_t T, rrea_fields_t F, const rrea_drift_config* cfg, DriftArgs& A)
 {
     if (!T || !F || !cfg || !F->fields_set) return RREA_ERR_ARGUMENT;
@@ -4296,6 +4465,29 @@ rrea_status rrea_drift_adjoint(rrea_tables_t T, rrea_fields_t F, const rrea_tile
     int64_t const faces = static_cast<int64_t>(nr + 1) * (nz + 1);
     face_gradient_kernel<<<dim3(static_cast<unsigned>(blocks_for(faces, 256))), dim3(256), 0, s>>>(
         nr, nz, A.g_er, A.g_ez, grad_er, grad_ez);
+    HIP_TRY(hipGetLastError());
+    return RREA_OK;
+}
+
+
+rrea_status rrea_batch_bind_sources(rrea_batch_t B, double* const* sources)
+{
+    if (!B) return RREA_ERR_ARGUMENT;
+    if (!sources) { B->src.clear(); return RREA_OK; }
+    for (size_t i = 0; i < B->fields.size(); ++i) if (!sources[i]) return RREA_ERR_ARGUMENT;
+    B->src.assign(sources, sources + B->fields.size());
+    return RREA_OK;
+}
+
+rrea_status rrea_sources_normalize(rrea_fields_t F, double* sources, double dt_s, int32_t diagnostics,
+                                   rrea_stream_t stream)
+{
+    if (!F || !sources || !(dt_s > 0.0) || !std::isfinite(dt_s)) return RREA_ERR_ARGUMENT;
+    rrea_rz_geometry const& g = F->geom;
+    int64_t const cells = static_cast<int64_t>(g.nr) * g.nz;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    sources_normalize_kernel<<<dim3(static_cast<unsigned>(blocks_for(cells, 256))), dim3(256), 0, s>>>(
+        g.nr, g.nz, g.r_lo, g.dr, g.dz, dt_s, diagnostics, sources);
     HIP_TRY(hipGetLastError());
     return RREA_OK;
 }

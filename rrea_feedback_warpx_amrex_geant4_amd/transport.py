@@ -167,6 +167,14 @@ class Coupling:
         _lib.check(_lib.lib().rrea_fields_update(self.h, *[_dptr(a) for a in arrs], 0, stream),
                    "rrea_fields_update")
 
+    def normalize_sources(self, sources: torch.Tensor, dt_s: float, diagnostics: bool = True,
+                          stream: int | None = None) -> None:
+        """RreaEventAccumulator::NormalizeAndSync on raw source grids, in place."""
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().rrea_sources_normalize(self.h, sources.data_ptr(), dt_s, int(diagnostics), stream),
+                   "rrea_sources_normalize")
+
     def close(self):
         if getattr(self, "h", None):
             _lib.lib().rrea_fields_destroy(self.h)
@@ -267,11 +275,14 @@ class StepOutputs:
     """Device outputs of one step (allocated once, reused)."""
 
     def __init__(self, n: int, fields: RzFields, secondary_capacity: int, device="cuda", per_particle: bool = True,
-                 share_grid: "StepOutputs | None" = None, grid_buffers: "tuple | None" = None):
-        """``share_grid``: deposit into another output's J grid and tallies, as
-        every species of one step deposits into the coupling's one net-chord
-        grid (RreaWarpXCoupling.cpp:417-444).  ``grid_buffers``: (jr, jz, rho)
-        device views to deposit into instead of grids of its own."""
+                 share_grid: "StepOutputs | None" = None, grid_buffers: "tuple | None" = None,
+                 sources: bool = False):
+        """``share_grid``: deposit into another output's J grid, tallies and
+        material sources, as every species of one step deposits into the
+        coupling's one net-chord grid (RreaWarpXCoupling.cpp:417-444).
+        ``grid_buffers``: (jr, jz, rho) device views to deposit into instead of
+        grids of its own.  ``sources``: accumulate the material sources
+        (RREA_SOURCE_FIELDS x nr x nz raw weights) into ``self.sources``."""
         nrp, nzp = fields.padded_shape()
         self.status = torch.zeros(n, dtype=torch.uint8, device=device) if per_particle else None
         self.ion_pairs = torch.zeros(n, dtype=torch.float64, device=device) if per_particle else None
@@ -280,9 +291,13 @@ class StepOutputs:
                                        device=device)
         self.secondary_capacity = secondary_capacity
         self.secondary_count = torch.zeros(1, dtype=torch.int64, device=device)
+        self.sources = None
+        if sources and share_grid is None:
+            self.sources = torch.zeros(_lib.SOURCE_FIELDS * fields.nr * fields.nz, dtype=torch.float64, device=device)
         if share_grid is not None:
             self.jr, self.jz, self.rho = share_grid.jr, share_grid.jz, share_grid.rho
             self.tallies, self.error = share_grid.tallies, share_grid.error
+            self.sources = share_grid.sources
         elif grid_buffers is not None:
             self.jr, self.jz, self.rho = grid_buffers
             if (self.jr.numel(), self.jz.numel(), self.rho.numel()) != ((nrp + 1) * nzp, nrp * (nzp + 1), nrp * nzp):
@@ -297,14 +312,15 @@ class StepOutputs:
             self.error = torch.zeros(2, dtype=torch.int32, device=device)
 
     def reset(self):
-        for t in (self.secondary_count, self.jr, self.jz, self.rho, self.tallies, self.error):
-            t.zero_()
+        for t in (self.secondary_count, self.jr, self.jz, self.rho, self.tallies, self.error, self.sources):
+            if t is not None:
+                t.zero_()
 
     def struct(self, order: bool) -> _lib.StepOut:
         p = lambda t: None if t is None else t.data_ptr()  # noqa: E731
         return _lib.StepOut(p(self.status), p(self.ion_pairs), p(self.substeps), p(self.secondaries),
                             self.secondary_capacity, p(self.secondary_count), p(self.jr), p(self.jz), p(self.rho),
-                            p(self.tallies), p(self.error), 1 if order else 0)
+                            p(self.tallies), p(self.error), 1 if order else 0, p(self.sources))
 
     def secondaries_host(self) -> np.ndarray:
         n = min(int(self.secondary_count.item()), self.secondary_capacity)
@@ -366,7 +382,8 @@ class DrainOutputs:
                             self.status.data_ptr(), self.ion_pairs.data_ptr(), self.substeps.data_ptr(),
                             self.survivors.data_ptr(),
                             self.capacity, self.survivor_count.data_ptr(), g.jr.data_ptr(), g.jz.data_ptr(),
-                            g.rho.data_ptr(), g.tallies.data_ptr(), g.error.data_ptr())
+                            g.rho.data_ptr(), g.tallies.data_ptr(), g.error.data_ptr(),
+                            g.sources.data_ptr() if g.sources is not None else None)
 
     def work_host(self) -> np.ndarray:
         n = int(self.work_count.item())
