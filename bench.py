@@ -3,7 +3,8 @@
 Workload: the at-scale configuration -- 2^24 macroparticles on a 256 x 1024 RZ
 mesh over 2.56 km x 10 km, rho = 1.225 exp(-h / 8.4 km) with h = 4 km + z,
 |E_z| piecewise-linear 0 -> 500 kV/m, dt = 1 ps, FP64, net-chord current
-deposition on, secondaries staged and ordered each step.  The charged shares
+deposition and material-source accumulation on, secondaries staged and
+ordered each step.  The charged shares
 are transported: 70 % electrons (11,744,051) and 5 % positrons (838,861),
 K ~ Exp(7.3 MeV) clipped to [10 keV, 100 MeV] in a 30-degree cone about the
 force; 25 % photons (4,194,304), isotropic, same energy law.
@@ -57,8 +58,9 @@ def cpu_baseline(n_particles: int, steps: int, threads: int):
     """The CPU oracle (a restatement of the reference step, OpenMP over
     64-particle chunks) timed on this host on a bounded sample of the same
     workload: electrons, photons and positrons in the 70:25:5 ratio, then
-    the newborn drain of the step's secondaries.  Only the oracle's calls are
-    timed (not the numpy photon push between steps)."""
+    the newborn drain of the step's secondaries, material sources
+    accumulated.  Only the oracle's calls are timed (not the numpy photon
+    push between steps)."""
     from oracle import binding as ob
     from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
     from rrea_feedback_warpx_amrex_geant4_amd.config import StepConfig
@@ -75,15 +77,17 @@ def cpu_baseline(n_particles: int, steps: int, threads: int):
              "positron": wl.c2_electrons(n_p, fields, seed=3, first_id=1 << 40, charge_sign=1.0).as_dict()}
     done = 0
     dt = 0.0
+    src = ob.new_sources(fields)
     for s in range(steps):
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
+        src[:] = 0.0
         staged = []
         for species, p in tiles.items():
             t0 = time.perf_counter()
             if species == "photon":
-                o = ob.photon_step(tables, fields, cfg, p, nthreads=threads, sec_cap=1 << 16)
+                o = ob.photon_step(tables, fields, cfg, p, nthreads=threads, sec_cap=1 << 16, sources=src)
             else:
-                o = ob.charged_step(species, tables, fields, cfg, p, nthreads=threads, sec_cap=1 << 16)
+                o = ob.charged_step(species, tables, fields, cfg, p, nthreads=threads, sec_cap=1 << 16, sources=src)
             dt += time.perf_counter() - t0
             if o["code"] != 0:
                 raise RuntimeError(f"oracle error {o['code']}")
@@ -99,7 +103,7 @@ def cpu_baseline(n_particles: int, steps: int, threads: int):
             tiles[species] = p
         # the newborn residual advance of the step's secondaries
         t0 = time.perf_counter()
-        d = ob.newborn_drain(tables, fields, cfg, np.concatenate(staged), cap=1 << 16)
+        d = ob.newborn_drain(tables, fields, cfg, np.concatenate(staged), cap=1 << 16, sources=src)
         dt += time.perf_counter() - t0
         if d["code"] != 0:
             raise RuntimeError(f"oracle drain error {d['code']}")
@@ -138,7 +142,7 @@ def batched_leg(args, tables):
     dev_p = dev_b.clone()
     inst = torch.as_tensor(np.repeat(np.arange(n_inst, dtype=np.int32), sizes), device="cuda")
     couplings = [Coupling(f, tables) for f in fields]
-    grids_b, grids_p = InstanceGrids(fields), InstanceGrids(fields)
+    grids_b, grids_p = InstanceGrids(fields, sources=True), InstanceGrids(fields, sources=True)
     batch = Batch(couplings, grids_b.grids, seeds)
     out_b = StepOutputs(n, fields[0], secondary_capacity=max(1 << 14, n >> 6), per_particle=False)
     drain_b = DrainOutputs(max(1 << 15, n >> 5), out_b)
@@ -149,7 +153,7 @@ def batched_leg(args, tables):
     for i, f in enumerate(fields):
         g = grids_p.grids[i]
         o = StepOutputs(int(sizes[i]), f, secondary_capacity=max(1 << 10, int(sizes[i]) >> 6), per_particle=False,
-                        grid_buffers=(g.jr, g.jz, g.rho))
+                        grid_buffers=(g.jr, g.jz, g.rho, g.sources))
         o.secondary_count = counts[i:i + 1]
         plain_outs.append(o)
         plain_drains.append(DrainOutputs(max(1 << 11, int(sizes[i]) >> 5), o))
@@ -210,7 +214,7 @@ def batched_leg(args, tables):
     return {
         "workload": f"{n_inst} independent instances, {n} electrons (N0 log-uniform 1e3..1e6 x {args.batched_scale}),"
                     " |E| U[250,1000] kV/m, rho U[0.3,1.225] kg/m^3, 64x256 RZ grids, dt=1 ps, electron step + "
-                    "newborn drain per step, J per instance",
+                    "newborn drain per step, J and material sources per instance",
         "instances": n_inst, "electrons": n, "steps": args.batched_steps, "warmup": args.batched_warmup,
         "batched": {"value": rate_b, "ms_per_step": el_b / args.batched_steps * 1e3,
                     "electron_kernel_ms": mean_launch_s * 1e3, "hbm_frac": achieved / HBM_PEAK,
@@ -242,7 +246,7 @@ def main():
                     help="reorder the tiles by cell every this many steps (0: never), inside the timed region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
-    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--batched-instances", type=int, default=256, help="instances of the batched leg (0: skip it)")
     ap.add_argument("--batched-scale", type=float, default=1.0, help="scale of the batched leg's populations")
     ap.add_argument("--batched-steps", type=int, default=10)
@@ -288,7 +292,7 @@ def main():
     dev_e = DeviceTile.from_host(wl.c2_electrons(n_e, fields, seed=2))
     dev_p = DeviceTile.from_host(wl.c2_electrons(n_p, fields, seed=3, first_id=1 << 40, charge_sign=1.0))
     dev_g = PhotonTile.from_host(wl.c2_photons(n_g, fields, dt_s=1e-12, seed=4, first_id=1 << 41))
-    out_e = StepOutputs(n_e, fields, secondary_capacity=1 << 20, per_particle=False)
+    out_e = StepOutputs(n_e, fields, secondary_capacity=1 << 20, per_particle=False, sources=True)
     out_p = StepOutputs(n_p, fields, secondary_capacity=1 << 18, per_particle=False, share_grid=out_e)
     out_g = StepOutputs(n_g, fields, secondary_capacity=1 << 18, per_particle=False, share_grid=out_e)
     drain = DrainOutputs(1 << 21, out_e)
@@ -306,6 +310,7 @@ def main():
         out_e.jr.zero_()
         out_e.jz.zero_()
         out_e.rho.zero_()
+        out_e.sources.zero_()
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
         apply.apply_electrons(tables, coupling, dev_e, cfg, out_e, order_secondaries=True, check=False)
         apply.apply_photons(tables, coupling, dev_g, cfg, out_g, order_secondaries=True, check=False)
@@ -379,7 +384,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"C2 plain at scale, {total} macroparticles: {n_e} e- + {n_g} gamma + {n_p} e+, "
-                               "dt=1 ps, RZ 256x1024, deposit on, secondaries ordered and their newborn residual "
+                               "dt=1 ps, RZ 256x1024, deposit on, material sources accumulated, secondaries ordered and their newborn residual "
                                "advance drained, photon push on device",
                    "electrons": n_e, "photons": n_g, "positrons": n_p, "path": "plain",
                    "tile_sort_every": args.sort_every,

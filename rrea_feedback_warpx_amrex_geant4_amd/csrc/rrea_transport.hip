@@ -1595,63 +1595,189 @@ __device__ __forceinline__ int32_t wave_max_i32(int32_t v)
     return v;
 }
 
+// A fast-path lane's ion-pair record located on its grid (AddToField's
+// order-1 shape: the left cell index and upper weight in r and z); w = 0: none.
+struct SourceCell {
+    double w, wr1, wz1;
+    int32_t i0, j0;
+};
+
+__device__ __forceinline__ SourceCell source_cell(Env const& E, double r, double z, double w)
+{
+    SourceCell c{0.0, 0.0, 0.0, 0, 0};
+    DevFields const& F = E.F;
+    int32_t const ic = static_cast<int32_t>(floor((r - F.r_lo) / F.dr));
+    int32_t const jc = static_cast<int32_t>(floor((z - F.z_lo) / F.dz));
+    if (ic < 0 || ic >= F.nr || jc < 0 || jc >= F.nz) return c;  // AddToField keeps nothing outside
+    double const r_cell = (r - F.r_lo) / F.dr + 0.0 - 0.5;
+    c.i0 = static_cast<int32_t>(floor(r_cell));
+    c.wr1 = r_cell - static_cast<double>(c.i0);
+    double const z_cell = (z - F.z_lo) / F.dz + 0.0 - 0.5;
+    c.j0 = static_cast<int32_t>(floor(z_cell));
+    c.wz1 = z_cell - static_cast<double>(c.j0);
+    c.w = w;
+    return c;
+}
+
+// The record's shape into the source fields with device atomics: the low-
+// energy electron, positive-ion and event-count fields get w x shape, the
+// energy-loss field W_air w x shape (AddToField :386-512 per field).
+__device__ void source_cell_add(Env const& E, SourceCell const& c)
+{
+    int32_t const nr = E.F.nr, nz = E.F.nz;
+    int64_t const cells = static_cast<int64_t>(nr) * nz;
+    double const v4 = dmax(c.w, 0.0) * dmax(kWAir, 0.0);
+    for (int rz = 0; rz < 2; ++rz) {
+        double const wz = rz == 0 ? 1.0 - c.wz1 : c.wz1;
+        if (wz == 0.0) continue;
+        int32_t const j = c.j0 + rz;
+        if (j < 0 || j > nz - 1) continue;
+        for (int rr = 0; rr < 2; ++rr) {
+            double const wr = rr == 0 ? 1.0 - c.wr1 : c.wr1;
+            double const v = c.w * wr * wz;
+            if (v == 0.0) continue;
+            int32_t i = c.i0 + rr;
+            if (i < 0 || i > nr - 1) {
+                if (i == -1) i = 0;  // the axis fold (the grid starts at r = 0)
+                else continue;
+            }
+            int64_t const cell = i + static_cast<int64_t>(nr) * j;
+            atomic_add(&E.src[kSrcLowElectron * cells + cell], v);
+            atomic_add(&E.src[kSrcPositiveIon * cells + cell], v);
+            atomic_add(&E.src[kSrcEventCount * cells + cell], v);
+            atomic_add(&E.src[kSrcEnergyLoss * cells + cell], v4 * wr * wz);
+        }
+    }
+}
+
+// The workgroup's deferred side effects of the fast path, through one LDS
+// window at a time: first every survivor's net chord into the J faces, then
+// every survivor's continuous ion-pair record (weight sw at (sr, sz); sw = 0:
+// none) into the material-source cells, each window flushed with one device
+// atomic per touched face or cell.  A window that is too large, or whose
+// lanes span several batched instances, falls back to per-lane atomics.
 __device__ void deposit_block(StepArgs const& A, bool dep, int32_t instance, double q, double dt, double r0,
-                              double z0, double r1, double z1)
+                              double z0, double r1, double z1, SourceCell const& sc)
 {
     __shared__ double jr_w[kWindowFaces];
     __shared__ double jz_w[kWindowFaces];
     __shared__ int32_t win[6];
-    // a batched workgroup can hold several instances: the window needs one grid
-    DevGrid const& g = instance >= 0 ? A.batch[instance].G : A.env.G;
-    int32_t ilo = INT32_MAX, ihi = INT32_MIN, jlo = INT32_MAX, jhi = INT32_MIN;
-    int32_t klo = dep ? instance : INT32_MAX, khi = dep ? instance : INT32_MIN;
-    if (dep) {
-        int32_t a0, a1, b0, b1, c0, c1, d0, d1;
-        segment_face_range(g, r0, z0, a0, a1, b0, b1);
-        segment_face_range(g, r1, z1, c0, c1, d0, d1);
-        ilo = a0 < c0 ? a0 : c0;
-        ihi = a1 > c1 ? a1 : c1;
-        jlo = b0 < d0 ? b0 : d0;
-        jhi = b1 > d1 ? b1 : d1;
+    auto reduce_window = [&](int32_t ilo, int32_t ihi, int32_t jlo, int32_t jhi, int32_t klo, int32_t khi) {
+        __syncthreads();  // the previous window's readers are done
+        if (threadIdx.x == 0) {
+            win[0] = INT32_MAX; win[1] = INT32_MIN; win[2] = INT32_MAX; win[3] = INT32_MIN;
+            win[4] = INT32_MAX; win[5] = INT32_MIN;
+        }
+        ilo = wave_min_i32(ilo);
+        ihi = wave_max_i32(ihi);
+        jlo = wave_min_i32(jlo);
+        jhi = wave_max_i32(jhi);
+        klo = wave_min_i32(klo);
+        khi = wave_max_i32(khi);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&win[0], ilo);
+            atomicMax(&win[1], ihi);
+            atomicMin(&win[2], jlo);
+            atomicMax(&win[3], jhi);
+            atomicMin(&win[4], klo);
+            atomicMax(&win[5], khi);
+        }
+        __syncthreads();
+    };
+    // ---- the net chords ----
+    {
+        // a batched workgroup can hold several instances: the window needs one grid
+        DevGrid const& g = instance >= 0 ? A.batch[instance].G : A.env.G;
+        int32_t ilo = INT32_MAX, ihi = INT32_MIN, jlo = INT32_MAX, jhi = INT32_MIN;
+        if (dep) {
+            int32_t a0, a1, b0, b1, c0, c1, d0, d1;
+            segment_face_range(g, r0, z0, a0, a1, b0, b1);
+            segment_face_range(g, r1, z1, c0, c1, d0, d1);
+            ilo = a0 < c0 ? a0 : c0;
+            ihi = a1 > c1 ? a1 : c1;
+            jlo = b0 < d0 ? b0 : d0;
+            jhi = b1 > d1 ? b1 : d1;
+        }
+        reduce_window(ilo, ihi, jlo, jhi, dep ? instance : INT32_MAX, dep ? instance : INT32_MIN);
+        int32_t const i0 = win[0], i1 = win[1], j0 = win[2], j1 = win[3];
+        int32_t const wi = i1 - i0 + 1, nw = i0 > i1 ? 0 : wi * (j1 - j0 + 1);
+        if (nw > kWindowFaces || (nw > 0 && win[4] != win[5])) {
+            if (dep) deposit_segment(g, q, dt, r0, z0, r1, z1);
+        } else if (nw > 0) {
+            // the window's one grid: every lane flushes into it, depositing or not
+            DevGrid const& gw = win[4] >= 0 ? A.batch[win[4]].G : A.env.G;
+            for (int c = threadIdx.x; c < nw; c += kBlock) { jr_w[c] = 0.0; jz_w[c] = 0.0; }
+            __syncthreads();
+            if (dep) deposit_segment_to(gw, LdsWindow{jr_w, jz_w, i0, j0, wi}, q, dt, r0, z0, r1, z1);
+            __syncthreads();
+            for (int c = threadIdx.x; c < nw; c += kBlock) {
+                int32_t const i = i0 + c % wi, j = j0 + c / wi;
+                double const vr = jr_w[c], vz = jz_w[c];
+                if (vr != 0.0) atomic_add(&gw.jr[i + (gw.nr + 1) * j], vr);
+                if (vz != 0.0) atomic_add(&gw.jz[i + gw.nr * j], vz);
+            }
+        }
     }
-    if (threadIdx.x == 0) {
-        win[0] = INT32_MAX; win[1] = INT32_MIN; win[2] = INT32_MAX; win[3] = INT32_MIN;
-        win[4] = INT32_MAX; win[5] = INT32_MIN;
-    }
-    ilo = wave_min_i32(ilo);
-    ihi = wave_max_i32(ihi);
-    jlo = wave_min_i32(jlo);
-    jhi = wave_max_i32(jhi);
-    klo = wave_min_i32(klo);
-    khi = wave_max_i32(khi);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) {
-        atomicMin(&win[0], ilo);
-        atomicMax(&win[1], ihi);
-        atomicMin(&win[2], jlo);
-        atomicMax(&win[3], jhi);
-        atomicMin(&win[4], klo);
-        atomicMax(&win[5], khi);
-    }
-    __syncthreads();
-    int32_t const i0 = win[0], i1 = win[1], j0 = win[2], j1 = win[3];
-    if (i0 > i1) return;  // no survivor in this workgroup
-    int32_t const wi = i1 - i0 + 1, nw = wi * (j1 - j0 + 1);
-    if (nw > kWindowFaces || win[4] != win[5]) {
-        if (dep) deposit_segment(g, q, dt, r0, z0, r1, z1);
-        return;
-    }
-    // the window's one grid: every lane flushes into it, depositing or not
-    DevGrid const& gw = win[4] >= 0 ? A.batch[win[4]].G : A.env.G;
-    for (int c = threadIdx.x; c < nw; c += kBlock) { jr_w[c] = 0.0; jz_w[c] = 0.0; }
-    __syncthreads();
-    if (dep) deposit_segment_to(gw, LdsWindow{jr_w, jz_w, i0, j0, wi}, q, dt, r0, z0, r1, z1);
-    __syncthreads();
-    for (int c = threadIdx.x; c < nw; c += kBlock) {
-        int32_t const i = i0 + c % wi, j = j0 + c / wi;
-        double const vr = jr_w[c], vz = jz_w[c];
-        if (vr != 0.0) atomic_add(&gw.jr[i + (gw.nr + 1) * j], vr);
-        if (vz != 0.0) atomic_add(&gw.jz[i + gw.nr * j], vz);
+    // ---- the ion-pair records: an ionization event of secondary energy
+    // cutoff / 2 and diagnostic loss W_air, so the low-energy electron,
+    // positive-ion and event-count fields receive the same shape-weighted
+    // value and the energy-loss field W_air times it ----
+    {
+        bool const has = dep && sc.w > 0.0;
+        int32_t const ci0 = sc.i0, cj0 = sc.j0;
+        Env const& El = instance >= 0 ? A.batch[instance] : A.env;
+        int32_t ilo = INT32_MAX, ihi = INT32_MIN, jlo = INT32_MAX, jhi = INT32_MIN;
+        if (has) {
+            ilo = ci0 < 0 ? 0 : ci0;
+            ihi = ci0 + 1 > El.F.nr - 1 ? El.F.nr - 1 : ci0 + 1;
+            jlo = cj0 < 0 ? 0 : cj0;
+            jhi = cj0 + 1 > El.F.nz - 1 ? El.F.nz - 1 : cj0 + 1;
+        }
+        reduce_window(ilo, ihi, jlo, jhi, has ? instance : INT32_MAX, has ? instance : INT32_MIN);
+        int32_t const i0 = win[0], i1 = win[1], j0 = win[2], j1 = win[3];
+        int32_t const wi = i1 - i0 + 1, nw = i0 > i1 ? 0 : wi * (j1 - j0 + 1);
+        if (nw > kWindowFaces || (nw > 0 && win[4] != win[5])) {
+            if (has) source_cell_add(El, sc);
+        } else if (nw > 0) {
+            Env const& Ew = win[4] >= 0 ? A.batch[win[4]] : A.env;
+            for (int c = threadIdx.x; c < nw; c += kBlock) { jr_w[c] = 0.0; jz_w[c] = 0.0; }
+            __syncthreads();
+            if (has) {
+                double const v4 = dmax(sc.w, 0.0) * dmax(kWAir, 0.0);
+                for (int rz = 0; rz < 2; ++rz) {
+                    double const wz = rz == 0 ? 1.0 - sc.wz1 : sc.wz1;
+                    if (wz == 0.0) continue;
+                    int32_t const j = cj0 + rz;
+                    if (j < 0 || j > Ew.F.nz - 1) continue;
+                    for (int rr = 0; rr < 2; ++rr) {
+                        double const wr = rr == 0 ? 1.0 - sc.wr1 : sc.wr1;
+                        double const v = sc.w * wr * wz;
+                        if (v == 0.0) continue;
+                        int32_t i = ci0 + rr;
+                        if (i < 0 || i > Ew.F.nr - 1) {
+                            if (i == -1) i = 0;  // the axis fold (the grid starts at r = 0)
+                            else continue;
+                        }
+                        int32_t const c = (i - i0) + wi * (j - j0);
+                        atomicAdd(&jr_w[c], v);
+                        atomicAdd(&jz_w[c], v4 * wr * wz);
+                    }
+                }
+            }
+            __syncthreads();
+            int64_t const cells = static_cast<int64_t>(Ew.F.nr) * Ew.F.nz;
+            for (int c = threadIdx.x; c < nw; c += kBlock) {
+                int64_t const cell = (i0 + c % wi) + static_cast<int64_t>(Ew.F.nr) * (j0 + c / wi);
+                double const v = jr_w[c], v4 = jz_w[c];
+                if (v != 0.0) {
+                    atomic_add(&Ew.src[kSrcLowElectron * cells + cell], v);
+                    atomic_add(&Ew.src[kSrcPositiveIon * cells + cell], v);
+                    atomic_add(&Ew.src[kSrcEventCount * cells + cell], v);
+                }
+                if (v4 != 0.0) atomic_add(&Ew.src[kSrcEnergyLoss * cells + cell], v4);
+            }
+        }
     }
 }
 
@@ -1698,8 +1824,10 @@ __device__ __forceinline__ void append_slow(bool slow, int64_t ip, int32_t* list
     }
 }
 
+// the plain kernels fit four waves per SIMD (128 VGPRs) without spilling
 template <bool kPos, bool kBatched>
-__global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __restrict__ args,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBatched ? 3 : 4)))
+charged_fast_kernel(const StepArgs* __restrict__ args,
                                                             int32_t* __restrict__ slow_list,
                                                             int32_t* __restrict__ slow_count)
 {
@@ -1713,6 +1841,7 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
     bool dep = false;
     int32_t dep_inst = -1;
     double dep_q = 0.0, dep_r0 = 0.0, dep_z0 = 0.0, dep_r1 = 0.0, dep_z1 = 0.0;
+    SourceCell src{0.0, 0.0, 0.0, 0, 0};  // its ion-pair source record
     // a batched wave takes the fast path only when all its particles share one
     // instance, whose environment is then wave-uniform; a wave that straddles
     // an instance boundary goes to the complete path
@@ -1881,13 +2010,14 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
             slow = false;
             dep = true;
             if (ion > 0.0 && E.src) {
-                // the ion pairs' record at the interval midpoint; the one
-                // interval ends where the substep does (iv.end == se here)
+                // the ion pairs' record at the interval midpoint (the one
+                // interval ends where the substep does: iv.end == se here),
+                // deposited with the workgroup's other records at the end
                 double mx, my, mz;
                 point_at_full(path, 0.5 * (0.0 + se), mx, my, mz);
-                int32_t serr = kOk;
-                source_ionization(E, cutoff, hypot(mx, my), mz, ion, 0.5 * cutoff, kWAir, serr);
-                if (serr) report(A.error, serr, ip);
+                double const mr = hypot(mx, my);
+                if (source_in_domain(E, mr, mz)) src = source_cell(E, mr, mz, ion);
+                else report(A.error, kErrDomain, ip);
             }
             dep_inst = E.instance;
             dep_q = S::kCharge * w;
@@ -1909,8 +2039,8 @@ __global__ void __launch_bounds__(kBlock) charged_fast_kernel(const StepArgs* __
         } while (false);
     }
     append_slow(slow, ip, slow_list, slow_count);
-    deposit_block(A, dep, dep_inst, dep_q, A.cfg.dt_s, dep_r0, dep_z0, dep_r1, dep_z1);
     reduce_tallies(tally, A.tallies);
+    deposit_block(A, dep, dep_inst, dep_q, A.cfg.dt_s, dep_r0, dep_z0, dep_r1, dep_z1, src);
 }
 
 // Every particle through the complete path (used when the fast path is off).

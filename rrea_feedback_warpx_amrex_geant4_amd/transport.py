@@ -299,9 +299,13 @@ class StepOutputs:
             self.tallies, self.error = share_grid.tallies, share_grid.error
             self.sources = share_grid.sources
         elif grid_buffers is not None:
-            self.jr, self.jz, self.rho = grid_buffers
+            self.jr, self.jz, self.rho = grid_buffers[:3]
             if (self.jr.numel(), self.jz.numel(), self.rho.numel()) != ((nrp + 1) * nzp, nrp * (nzp + 1), nrp * nzp):
                 raise ValueError("grid buffers do not match the padded grid")
+            if len(grid_buffers) > 3 and grid_buffers[3] is not None:
+                if grid_buffers[3].numel() != _lib.SOURCE_FIELDS * fields.nr * fields.nz:
+                    raise ValueError("source buffer does not match the grid")
+                self.sources = grid_buffers[3]
             self.tallies = torch.zeros(_lib.TALLY_COUNT, dtype=torch.float64, device=device)
             self.error = torch.zeros(2, dtype=torch.int32, device=device)
         else:
@@ -397,25 +401,30 @@ class DrainOutputs:
 
 
 class InstanceGrids:
-    """The J grids of many instances in three contiguous device buffers, so
-    one fill clears them all; ``grids[i]`` is instance i's StepOutputs (no
-    per-particle outputs) over views of them."""
+    """The J grids (and, with ``sources``, the material-source grids) of many
+    instances in contiguous device buffers, so one fill clears them all;
+    ``grids[i]`` is instance i's StepOutputs (no per-particle outputs) over
+    views of them."""
 
-    def __init__(self, fields_list, device="cuda"):
+    def __init__(self, fields_list, device="cuda", sources: bool = False):
         shapes = [f.padded_shape() for f in fields_list]
-        sizes = [((a + 1) * b, a * (b + 1), a * b) for a, b in shapes]
-        tot = [sum(s[k] for s in sizes) for k in range(3)]
-        self.jr, self.jz, self.rho = (torch.zeros(t, dtype=torch.float64, device=device) for t in tot)
+        sizes = [((a + 1) * b, a * (b + 1), a * b, _lib.SOURCE_FIELDS * f.nr * f.nz if sources else 0)
+                 for (a, b), f in zip(shapes, fields_list)]
+        tot = [sum(s[k] for s in sizes) for k in range(4)]
+        self.jr, self.jz, self.rho, self.sources = (torch.zeros(t, dtype=torch.float64, device=device) for t in tot)
         self.grids = []
-        at = [0, 0, 0]
+        at = [0, 0, 0, 0]
+        bufs = (self.jr, self.jz, self.rho, self.sources)
         for f, sz in zip(fields_list, sizes):
-            views = tuple(buf[at[k]:at[k] + sz[k]] for k, buf in enumerate((self.jr, self.jz, self.rho)))
+            views = [buf[at[k]:at[k] + sz[k]] for k, buf in enumerate(bufs)]
+            if not sources:
+                views[3] = None
             self.grids.append(StepOutputs(0, f, secondary_capacity=0, device=device, per_particle=False,
-                                          grid_buffers=views))
-            at = [at[k] + sz[k] for k in range(3)]
+                                          grid_buffers=tuple(views)))
+            at = [at[k] + sz[k] for k in range(4)]
 
     def zero_(self):
-        for t in (self.jr, self.jz, self.rho):
+        for t in (self.jr, self.jz, self.rho, self.sources):
             t.zero_()
 
 
@@ -441,6 +450,9 @@ class Batch:
         h = C.c_void_p()
         _lib.check(_lib.lib().rrea_batch_create(fh, n, jr, jz, rho, sd, C.byref(h)), "rrea_batch_create")
         self.h = h
+        if all(g.sources is not None for g in grids):
+            src = arr([g.sources.data_ptr() for g in grids])
+            _lib.check(_lib.lib().rrea_batch_bind_sources(self.h, src), "rrea_batch_bind_sources")
         w = C.c_void_p()
         _lib.check(_lib.lib().rrea_batch_workspace(self.h, C.byref(w)), "rrea_batch_workspace")
         self.workspace = w
