@@ -226,6 +226,45 @@ def batched_leg(args, tables):
     }
 
 
+def drift_leg(args, tables):
+    """The differentiation configuration: 2^20 electrons (K ~ U[1, 20] MeV),
+    64 drift steps of 100 ps (push + CSDA loss + deposition, events frozen)
+    on a 128 x 512 grid, checkpoints every 8; L = sum(J_z W), W ~ N(0, 1).
+    Times the forward (with checkpoints) and the adjoint (reverse sweep with
+    recompute) separately, HIP events around each on the current stream."""
+    import torch
+
+    from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
+    from rrea_feedback_warpx_amrex_geant4_amd.adjoint import DriftConfig, DriftProblem
+
+    n = args.drift_particles
+    fields, parts = wl.c4_drift(n)
+    prob = DriftProblem(tables, fields, parts, DriftConfig(steps=64, dt_s=1e-10, checkpoint_every=8))
+    g = torch.Generator(device="cpu").manual_seed(11)
+    gjr = torch.zeros(prob.jr_size, dtype=torch.float64).cuda()
+    gjz = torch.randn(prob.jz_size, generator=g, dtype=torch.float64).cuda()
+    fw, bw = [], []
+    for it in range(4):
+        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a.record()
+        out = prob.forward(prob.er0, prob.ez0, gjr, gjz)
+        b.record()
+        prob.adjoint(prob.er0, prob.ez0, out["ck"], out["ck_alive"], gjr, gjz)
+        c.record()
+        torch.cuda.synchronize()
+        if it > 0:
+            fw.append(a.elapsed_time(b))
+            bw.append(b.elapsed_time(c))
+    steps = n * 64
+    log(f"drift: forward {np.mean(fw):.2f} ms, adjoint {np.mean(bw):.2f} ms for {n} electrons x 64 steps")
+    return {"workload": f"{n} electrons x 64 drift steps of 100 ps, 128x512 grid, checkpoints every 8, "
+                        "L = sum(J_z W)",
+            "forward_ms": float(np.mean(fw)), "adjoint_ms": float(np.mean(bw)),
+            "forward_particle_steps_per_s": steps / (np.mean(fw) * 1e-3),
+            "adjoint_particle_steps_per_s": steps / (np.mean(bw) * 1e-3),
+            "checkpoint_bytes": int(out["ck"].numel() * 8 + out["ck_alive"].numel())}
+
+
 def traffic_record():
     """Measured HBM bytes per launch of the electron kernel, or None."""
     try:
@@ -251,6 +290,8 @@ def main():
     ap.add_argument("--batched-scale", type=float, default=1.0, help="scale of the batched leg's populations")
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batched-warmup", type=int, default=2)
+    ap.add_argument("--drift-particles", type=int, default=1 << 20,
+                    help="electrons of the drift-adjoint leg (0: skip it)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run only a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -359,6 +400,7 @@ def main():
     traffic, rec = traffic_record()
 
     batched = batched_leg(args, tables) if args.batched_instances > 0 else None
+    drift = drift_leg(args, tables) if args.drift_particles > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -395,6 +437,7 @@ def main():
                      "mean_launch_ms": mean_launch_s * 1e3},
         "cpu_baseline": cpu,
         "batched_vs_plain": batched,
+        "drift_adjoint": drift,
     }
     print(json.dumps(line), flush=True)
 
