@@ -15,7 +15,9 @@ tiles) inside the timed region.
 
 One "step" is one RreaParticleInteraction::Apply (RreaParticleInteraction.cpp:57-60):
 the electron kernels, the photon kernel, the positron kernels, each followed
-by the deterministic secondary ordering, then the newborn residual advance of
+by the deterministic secondary ordering -- the three passes run concurrently
+on three streams (ConcurrentApply; --sequential runs them one after
+another; the results are the same) -- then the newborn residual advance of
 every secondary the step made (the transient FIFO of
 DrainAndCommitSecondaries), all charged chords depositing into one J grid.  Charged tiles step in place (the fused pre-push snapshot); the photon
 tile is pushed on the device after its step (prev <- current, current += c dt
@@ -26,9 +28,13 @@ until its redistribute.
 
 Prints one JSON line (rank 0, N=1).  `roofline` is for the dominant kernel,
 the electron kernel: algorithmic bytes per launch (112 B per live electron)
-over its mean launch time from HIP events on its own stream; `traffic` is that
-kernel's measured HBM bytes per launch from the rocprofv3 FETCH_SIZE /
-WRITE_SIZE record in profiles/electron_kernel_traffic.json when present.
+over its mean launch time from HIP events on its own stream over the timed
+region (where it shares the CUs with the concurrent photon and positron
+passes), and `roofline.isolated` the same for the kernel timed alone on a
+few steps after the timed region; `traffic` is that kernel's measured HBM
+bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE record in
+profiles/electron_kernel_traffic.json when present.  The batched leg, the
+drift-adjoint leg and the CPU baseline follow (see their functions).
 """
 
 from __future__ import annotations
@@ -284,6 +290,8 @@ def main():
     ap.add_argument("--sort-every", type=int, default=25,
                     help="reorder the tiles by cell every this many steps (0: never), inside the timed region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sequential", action="store_true",
+                    help="run the three species passes one after another on one stream (default: concurrently)")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--batched-instances", type=int, default=256, help="instances of the batched leg (0: skip it)")
@@ -319,8 +327,9 @@ def main():
     from rrea_feedback_warpx_amrex_geant4_amd import _lib
     from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
     from rrea_feedback_warpx_amrex_geant4_amd.config import StepConfig
-    from rrea_feedback_warpx_amrex_geant4_amd.transport import (Coupling, DeviceTile, DrainOutputs, InteractionTables,
-                                                                PhotonTile, RreaParticleInteraction, StepOutputs)
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import (ConcurrentApply, Coupling, DeviceTile, DrainOutputs,
+                                                                InteractionTables, PhotonTile, RreaParticleInteraction,
+                                                                StepOutputs)
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     torch.cuda.set_device(0)
@@ -339,6 +348,9 @@ def main():
     drain = DrainOutputs(1 << 21, out_e)
     L = _lib.lib()
     apply = RreaParticleInteraction
+    conc = None if args.sequential else ConcurrentApply(tables, fields)
+    # the handle each species' kernels are timed on
+    h_e, h_g, h_p = (coupling.h,) * 3 if conc is None else tuple(c.h for c in conc.couplings)
 
     def run_step(s):
         if args.sort_every > 0 and s % args.sort_every == 0:
@@ -353,10 +365,13 @@ def main():
         out_e.rho.zero_()
         out_e.sources.zero_()
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
-        apply.apply_electrons(tables, coupling, dev_e, cfg, out_e, order_secondaries=True, check=False)
-        apply.apply_photons(tables, coupling, dev_g, cfg, out_g, order_secondaries=True, check=False)
-        apply.apply_positrons(tables, coupling, dev_p, cfg, out_p, order_secondaries=True, check=False)
-        apply.drain_secondaries(tables, coupling, cfg, [out_e, out_g, out_p], drain, check=False)
+        if conc is not None:
+            conc.run(dev_e, dev_g, dev_p, cfg, out_e, out_g, out_p, drain)
+        else:
+            apply.apply_electrons(tables, coupling, dev_e, cfg, out_e, order_secondaries=True, check=False)
+            apply.apply_photons(tables, coupling, dev_g, cfg, out_g, order_secondaries=True, check=False)
+            apply.apply_positrons(tables, coupling, dev_p, cfg, out_p, order_secondaries=True, check=False)
+            apply.drain_secondaries(tables, coupling, cfg, [out_e, out_g, out_p], drain, check=False)
         dev_g.push(cfg.dt_s)
 
     def live():
@@ -369,7 +384,8 @@ def main():
     out_e.check_error()
     e0, g0, p0 = live()
 
-    L.rrea_kernel_timing(coupling.h, 1)
+    for h in {h_e.value, h_g.value, h_p.value}:
+        L.rrea_kernel_timing(h, 1)
     log(f"timed {args.steps} steps")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -378,13 +394,15 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     kms = {}
-    for name, kid in (("electron", 0), ("electron_handover", 1), ("positron", 2), ("positron_handover", 3),
-                      ("photon", 4), ("photon_handover", 5), ("newborn", 6)):
+    for name, kid, h in (("electron", 0, h_e), ("electron_handover", 1, h_e), ("positron", 2, h_p),
+                         ("positron_handover", 3, h_p), ("photon", 4, h_g), ("photon_handover", 5, h_g),
+                         ("newborn", 6, h_e)):
         ms = C.c_double(0.0)
         cnt = C.c_int64(0)
-        _lib.check(L.rrea_kernel_timing_read(coupling.h, kid, C.byref(ms), C.byref(cnt)), "timing read")
+        _lib.check(L.rrea_kernel_timing_read(h, kid, C.byref(ms), C.byref(cnt)), "timing read")
         kms[name] = (ms.value, cnt.value)
-    L.rrea_kernel_timing(coupling.h, 0)
+    for h in {h_e.value, h_g.value, h_p.value}:
+        L.rrea_kernel_timing(h, 0)
     out_e.check_error()
     e1, g1, p1 = live()
     # live counts decline monotonically and by a tiny fraction per step; the
@@ -398,6 +416,28 @@ def main():
     log(f"live e- {e0} -> {e1}, gamma {g0} -> {g1}, e+ {p0} -> {p1}; wall {elapsed:.3f}s; kernels " +
         ", ".join(f"{k} {v[0]:.2f} ms / {v[1]}" for k, v in kms.items()))
     traffic, rec = traffic_record()
+
+    # the electron kernel alone: a few sequential Applies after the timed
+    # region (the timed region overlaps the species passes, which stretches
+    # the kernel's own launch time while it shares the CUs)
+    iso = None
+    if conc is not None:
+        L.rrea_kernel_timing(coupling.h, 1)
+        e_before = live()[0]
+        for s in range(3):
+            cfg = StepConfig(rng_seed=20240623, step=args.warmup + args.steps + s, dt_s=1e-12)
+            for o in (out_e, out_g, out_p):
+                o.secondary_count.zero_()
+            apply.apply_electrons(tables, coupling, dev_e, cfg, out_e, order_secondaries=True, check=False)
+        ms = C.c_double(0.0)
+        cnt = C.c_int64(0)
+        _lib.check(L.rrea_kernel_timing_read(coupling.h, 0, C.byref(ms), C.byref(cnt)), "timing read")
+        L.rrea_kernel_timing(coupling.h, 0)
+        iso_s = ms.value * 1e-3 / max(cnt.value, 1)
+        iso_ach = e_before * B_ALG / iso_s
+        iso = {"mean_launch_ms": iso_s * 1e3, "achieved": iso_ach / 1e9, "frac": iso_ach / HBM_PEAK,
+               "note": "the electron kernel timed alone (3 electron steps after the timed region)"}
+        log(f"electron kernel alone: {iso_s * 1e3:.3f} ms/launch, {iso_ach / 1e9:.0f} GB/s")
 
     batched = batched_leg(args, tables) if args.batched_instances > 0 else None
     drift = drift_leg(args, tables) if args.drift_particles > 0 else None
@@ -429,12 +469,13 @@ def main():
                                "dt=1 ps, RZ 256x1024, deposit on, material sources accumulated, secondaries ordered and their newborn residual "
                                "advance drained, photon push on device",
                    "electrons": n_e, "photons": n_g, "positrons": n_p, "path": "plain",
+                   "species_passes": "sequential" if args.sequential else "concurrent (three streams)",
                    "tile_sort_every": args.sort_every,
                    "kernel_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in kms.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "electron_fast_kernel", "alg_bytes_per_launch": alg_bytes,
-                     "mean_launch_ms": mean_launch_s * 1e3},
+                     "mean_launch_ms": mean_launch_s * 1e3, "isolated": iso},
         "cpu_baseline": cpu,
         "batched_vs_plain": batched,
         "drift_adjoint": drift,

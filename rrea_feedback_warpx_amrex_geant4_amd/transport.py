@@ -608,3 +608,56 @@ class RreaParticleInteraction:
                    entry)
         if check:
             out.check_error()
+
+
+class ConcurrentApply:
+    """One Apply with its three species passes on three streams at once.
+
+    The electron, photon and positron steps of one Apply read only their own
+    tile and the (unchanging) fields, and write their own secondaries and,
+    through device atomics, the shared J grid, sources and tallies; none
+    reads another's output before the drain.  Run concurrently they give the
+    sequential Apply's results (the J and source sums up to FP64 summation
+    order).  Each pass gets its own fields handle (the argument block and the
+    ordering workspace belong to a handle) and its own stream, and is driven
+    from its own host thread, so the one host round trip each ordered pass
+    makes does not hold back the others.  The drain then runs on the caller's
+    stream after all three."""
+
+    def __init__(self, tables: InteractionTables, fields: RzFields):
+        from concurrent.futures import ThreadPoolExecutor
+        self.tables = tables
+        self.couplings = [Coupling(fields, tables) for _ in range(3)]  # electrons, photons, positrons
+        self.streams = [torch.cuda.Stream(), torch.cuda.Stream()]      # default priority
+        self.pool = ThreadPoolExecutor(max_workers=2)
+
+    def run(self, electrons: DeviceTile, photons: "PhotonTile", positrons: DeviceTile, cfg: StepConfig,
+            out_e: StepOutputs, out_g: StepOutputs, out_p: StepOutputs, drain: "DrainOutputs | None" = None,
+            check: bool = False) -> None:
+        main = torch.cuda.current_stream()
+        ready = torch.cuda.Event()
+        ready.record(main)
+        for s in self.streams:
+            s.wait_event(ready)
+        apply = RreaParticleInteraction
+        cg, cp = self.couplings[1], self.couplings[2]
+        fg = self.pool.submit(apply.apply_photons, self.tables, cg, photons, cfg, out_g, True,
+                              self.streams[0].cuda_stream, False)
+        fp = self.pool.submit(apply.apply_positrons, self.tables, cp, positrons, cfg, out_p, None, True,
+                              self.streams[1].cuda_stream, False)
+        apply.apply_electrons(self.tables, self.couplings[0], electrons, cfg, out_e, stream=main.cuda_stream,
+                              check=False)
+        fg.result()
+        fp.result()
+        for s in self.streams:
+            done = torch.cuda.Event()
+            done.record(s)
+            main.wait_event(done)
+        if drain is not None:
+            apply.drain_secondaries(self.tables, self.couplings[0], cfg, [out_e, out_g, out_p], drain,
+                                    stream=main.cuda_stream, check=False)
+        if check:
+            out_e.check_error()
+
+    def close(self):
+        self.pool.shutdown(wait=True)
