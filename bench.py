@@ -289,6 +289,8 @@ def main():
     ap.add_argument("--particles", type=int, default=None, help="total macroparticles (default 2^24)")
     ap.add_argument("--sort-every", type=int, default=25,
                     help="reorder the tiles by cell every this many steps (0: never), inside the timed region")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="pick the sort interval and energy-band bits by timing candidates before the warm-up")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sequential", action="store_true",
                     help="run the three species passes one after another on one stream (default: concurrently)")
@@ -352,12 +354,14 @@ def main():
     # the handle each species' kernels are timed on
     h_e, h_g, h_p = (coupling.h,) * 3 if conc is None else tuple(c.h for c in conc.couplings)
 
-    def run_step(s):
-        if args.sort_every > 0 and s % args.sort_every == 0:
+    knobs = {"sort_every": args.sort_every, "energy_bits": -1, "first_step": 0}
+
+    def one_step(s, sort_now, energy_bits):
+        if sort_now:
             # WarpX's periodic particle sort: cell-ordered tiles
-            dev_e.sort_by_cell(coupling)
-            dev_p.sort_by_cell(coupling)
-            dev_g.sort_by_cell(coupling)
+            dev_e.sort_by_cell(coupling, energy_bits=energy_bits)
+            dev_p.sort_by_cell(coupling, energy_bits=energy_bits)
+            dev_g.sort_by_cell(coupling, energy_bits=energy_bits)
         for o in (out_e, out_g, out_p):
             o.secondary_count.zero_()
         out_e.jr.zero_()
@@ -374,9 +378,24 @@ def main():
             apply.drain_secondaries(tables, coupling, cfg, [out_e, out_g, out_p], drain, check=False)
         dev_g.push(cfg.dt_s)
 
+    def run_step(s):
+        # s counts from 0 (the first sort); the RNG step index continues past the tuning steps
+        every = knobs["sort_every"]
+        one_step(knobs["first_step"] + s, every > 0 and s % every == 0, knobs["energy_bits"])
+
     def live():
         return tuple(int((t.id != _lib.INVALID_ID).sum().item()) for t in (dev_e, dev_g.cur, dev_p))
 
+    tuned = None
+    if args.autotune:
+        from rrea_feedback_warpx_amrex_geant4_amd.autotune import tune
+        log("autotune: tile-sort interval x energy-band bits")
+        choice = tune(one_step, first_step=0)
+        knobs["sort_every"], knobs["energy_bits"] = choice.sort_every, choice.energy_bits
+        knobs["first_step"] = choice.steps_run
+        tuned = choice.as_dict()
+        log(f"autotune: sort every {choice.sort_every} steps, {choice.energy_bits} band bits "
+            f"({choice.ms_per_step:.3f} ms/step)")
     log(f"warmup {args.warmup}")
     for s in range(args.warmup):
         run_step(s)
@@ -470,7 +489,8 @@ def main():
                                "advance drained, photon push on device",
                    "electrons": n_e, "photons": n_g, "positrons": n_p, "path": "plain",
                    "species_passes": "sequential" if args.sequential else "concurrent (three streams)",
-                   "tile_sort_every": args.sort_every,
+                   "tile_sort_every": knobs["sort_every"], "tile_sort_energy_bits": knobs["energy_bits"],
+                   "autotune": tuned,
                    "kernel_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in kms.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,

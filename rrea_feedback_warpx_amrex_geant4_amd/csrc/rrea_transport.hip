@@ -4400,7 +4400,13 @@ rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step
 
 rrea_status rrea_tile_sort(rrea_fields_t F, const rrea_tile* tile, int32_t photon, rrea_stream_t stream)
 {
-    if (!F || !tile || tile->n < 0) return RREA_ERR_ARGUMENT;
+    return rrea_tile_sort_banded(F, tile, photon, -1, stream);
+}
+
+rrea_status rrea_tile_sort_banded(rrea_fields_t F, const rrea_tile* tile, int32_t photon, int32_t energy_bits,
+                                  rrea_stream_t stream)
+{
+    if (!F || !tile || tile->n < 0 || energy_bits < -1 || energy_bits > 6) return RREA_ERR_ARGUMENT;
     int64_t const n = tile->n;
     if (n == 0) return RREA_OK;
     if (n > INT32_MAX || !tile->prev_r || !tile->prev_z || !tile->prev_theta || !tile->w || !tile->id
@@ -4413,7 +4419,8 @@ rrea_status rrea_tile_sort(rrea_fields_t F, const rrea_tile* tile, int32_t photo
     int32_t cbits = 0;
     while ((uint64_t{1} << cbits) < cells) ++cbits;
     if (cbits > 31) return RREA_ERR_ARGUMENT;
-    int32_t const ebits = 32 - cbits < 6 ? 32 - cbits : 6;
+    int32_t const emax = 32 - cbits < 6 ? 32 - cbits : 6;
+    int32_t const ebits = energy_bits < 0 || energy_bits > emax ? emax : energy_bits;
     hipStream_t const s = static_cast<hipStream_t>(stream);
     {
         rrea_status const ws = tile_sort_workspace(F, n);

@@ -219,12 +219,12 @@ class DeviceTile:
         p = [getattr(self, k).data_ptr() for k in ("r", "z", "theta", "ux", "uy", "uz")]
         return _lib.Tile(len(self), *p, self.w.data_ptr(), self.id.data_ptr(), self.cpu.data_ptr(), *p)
 
-    def sort_by_cell(self, coupling: "Coupling", stream: int | None = None) -> None:
-        """Reorder the tile by cell and energy band (rrea_tile_sort), in place."""
+    def sort_by_cell(self, coupling: "Coupling", stream: int | None = None, energy_bits: int = -1) -> None:
+        """Reorder the tile by cell and energy band (rrea_tile_sort_banded), in place."""
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         t = self.struct()
-        _lib.check(_lib.lib().rrea_tile_sort(coupling.h, C.byref(t), 0, stream), "rrea_tile_sort")
+        _lib.check(_lib.lib().rrea_tile_sort_banded(coupling.h, C.byref(t), 0, energy_bits, stream), "rrea_tile_sort")
 
 
 @dataclass
@@ -256,12 +256,12 @@ class PhotonTile:
                          None, None, c.w.data_ptr(), c.id.data_ptr(), c.cpu.data_ptr(), c.r.data_ptr(), c.z.data_ptr(),
                          c.theta.data_ptr(), c.ux.data_ptr(), c.uy.data_ptr(), c.uz.data_ptr())
 
-    def sort_by_cell(self, coupling: "Coupling", stream: int | None = None) -> None:
-        """Reorder the photon tile by cell and energy (rrea_tile_sort), in place."""
+    def sort_by_cell(self, coupling: "Coupling", stream: int | None = None, energy_bits: int = -1) -> None:
+        """Reorder the photon tile by cell and energy (rrea_tile_sort_banded), in place."""
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         t = self.struct()
-        _lib.check(_lib.lib().rrea_tile_sort(coupling.h, C.byref(t), 1, stream), "rrea_tile_sort")
+        _lib.check(_lib.lib().rrea_tile_sort_banded(coupling.h, C.byref(t), 1, energy_bits, stream), "rrea_tile_sort")
 
     def push(self, dt_s: float, stream: int | None = None) -> None:
         """WarpX's photon push (rrea_photon_push): prev <- current, current += c dt u/|u|."""

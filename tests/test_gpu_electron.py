@@ -174,15 +174,20 @@ def test_cell_ordered_tile_lds_deposit(tables, otables):
     run_and_compare(otables, tables, cp, fields, tile, cfg, steps=2, label="cell-ordered")
 
 
-def test_tile_sort(tables):
-    """rrea_tile_sort permutes every array of the tile together into cell
-    order (r fastest) with invalid particles last."""
+@pytest.mark.parametrize("energy_bits", [-1, 0, 3])
+def test_tile_sort(tables, energy_bits):
+    """rrea_tile_sort_banded permutes every array of the tile together into
+    cell order (r fastest) with invalid particles last, and within a cell into
+    quarter-octave kinetic-energy bands above 1 keV (capped at
+    2^energy_bits - 1; -1 takes the widest band field the key has room for,
+    6 bits on this grid)."""
     from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile
     fields = wl.c2_fields()
     host = wl.c2_electrons(1 << 14, fields, seed=33)
     host.id[::7] = np.iinfo(np.int64).min
     dev = DeviceTile.from_host(host)
-    dev.sort_by_cell(coupling_for(fields, tables))
+    print(f"case tile sort: 2^14 C2 e-, energy_bits {energy_bits}", flush=True)
+    dev.sort_by_cell(coupling_for(fields, tables), energy_bits=energy_bits)
     torch.cuda.synchronize()
     got = {k: getattr(dev, k).cpu().numpy() for k in ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")}
     invalid = got["id"] == np.iinfo(np.int64).min
@@ -192,7 +197,20 @@ def test_tile_sort(tables):
     valid = ~invalid
     i = (got["r"][valid] / fields.dr).astype(np.int64)
     j = ((got["z"][valid] - fields.z_lo) / fields.dz).astype(np.int64)
-    assert np.all(np.diff(i + fields.nr * j) >= 0)
+    cell = i + fields.nr * j
+    assert np.all(np.diff(cell) >= 0)
+    bits = 6 if energy_bits < 0 else energy_bits
+    if bits > 0:
+        c = 299792458.0
+        ux, uy, uz = (got[k][valid] for k in ("ux", "uy", "uz"))
+        u2 = ux * ux + uy * uy + uz * uz
+        e = u2 * 510998.95 / ((c * c) * (np.sqrt(1.0 + u2 / (c * c)) + 1.0))
+        band = np.minimum(np.where(e > 1e3, np.floor(4.0 * np.log2(np.maximum(e, 1e3) * 1e-3)), 0), 2 ** bits - 1)
+        same = np.diff(cell) == 0
+        d = np.diff(band)[same]
+        # float log2 on either side of a band edge may differ by one ulp
+        assert np.mean(d >= 0) > 0.999 and np.all(d >= -1), (bits, int((d < 0).sum()))
+        assert len(np.unique(band)) > 2
     # the same particles, each with all of its components
     order_h = np.argsort(host.r)
     order_g = np.argsort(got["r"])
