@@ -306,9 +306,11 @@ rrea_status rrea_photon_push(const rrea_tile* tile, double dt_s, rrea_stream_t s
  * step do not depend on the order except through the order of the staged
  * secondaries (by parent index) and floating-point summation order. */
 rrea_status rrea_tile_sort(rrea_fields_t fields, const rrea_tile* tile, int32_t photon, rrea_stream_t stream);
-/* The same with the energy key's width chosen: energy_bits bits of quarter-
- * octave bands above 1 keV (0: cell order only; -1: the default, 6, or what
- * the cell key leaves of 32 bits).  Tuning knob (no reference counterpart). */
+/* The same with the energy key's width chosen: energy_bits bits of energy
+ * bands above 1 keV, 2^(6 - energy_bits) quarter-octaves wide, so the bands
+ * span 16 octaves at every width (0: cell order only; -1: the default, 6, or
+ * what the cell key leaves of 32 bits).  Tuning knob (no reference
+ * counterpart). */
 rrea_status rrea_tile_sort_banded(rrea_fields_t fields, const rrea_tile* tile, int32_t photon, int32_t energy_bits,
                                   rrea_stream_t stream);
 

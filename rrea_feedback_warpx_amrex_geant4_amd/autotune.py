@@ -10,7 +10,8 @@ configuration), but the speed does:
   gathers cache-resident; the sort itself costs a radix sort and a gather of
   every array of the tile.
 - ``energy_bits``: the width of the energy-band part of the sort key (0 =
-  cell order only, up to 6 = quarter-octave bands over 16 octaves).  Bands
+  cell order only, up to 6 = quarter-octave bands; a width b merges
+  2^(6 - b) quarter-octaves, so every width spans 16 octaves).  Bands
   keep a wavefront's lanes on similar branches and table brackets.
 
 ``tune`` times every candidate over one sort period of the real workload

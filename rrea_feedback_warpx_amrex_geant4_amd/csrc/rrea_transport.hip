@@ -3020,10 +3020,12 @@ __global__ void tile_key_kernel(int64_t n, const double* __restrict__ r, const d
     int32_t j = finite(fz) ? static_cast<int32_t>(dmin(dmax(fz, 0.0), static_cast<double>(nz - 1))) : nz - 1;
     double const u2 = ux[p] * ux[p] + uy[p] * uy[p] + uz[p] * uz[p];
     double const e = photon ? sqrt(u2) : kinetic_from_u(ux[p], uy[p], uz[p]);
-    // four bands per octave above 1 keV
+    // four bands per octave above 1 keV at 6 bits; a narrower field merges
+    // 2^(6 - ebits) neighbouring bands, so the bands still span 16 octaves
     double const band = finite(e) && e > 1.0e3 ? 4.0 * log2(e * 1.0e-3) : 0.0;
     uint32_t const emax = (1U << ebits) - 1U;
-    uint32_t const eb = ebits > 0 ? static_cast<uint32_t>(dmin(band, static_cast<double>(emax))) : 0U;
+    uint32_t const q = static_cast<uint32_t>(dmin(band, 63.0)) >> (6 - ebits);
+    uint32_t const eb = ebits > 0 ? (q < emax ? q : emax) : 0U;
     uint32_t const cell = static_cast<uint32_t>(i) + static_cast<uint32_t>(nr) * static_cast<uint32_t>(j);
     keys[p] = (cell << ebits) | eb;
 }

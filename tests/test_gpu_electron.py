@@ -179,8 +179,9 @@ def test_tile_sort(tables, energy_bits):
     """rrea_tile_sort_banded permutes every array of the tile together into
     cell order (r fastest) with invalid particles last, and within a cell into
     quarter-octave kinetic-energy bands above 1 keV (capped at
-    2^energy_bits - 1; -1 takes the widest band field the key has room for,
-    6 bits on this grid)."""
+    2^(6 - energy_bits) quarter-octaves wide, capped at 2^energy_bits - 1;
+    -1 takes the widest band field the key has room for, 6 bits on this
+    grid)."""
     from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile
     fields = wl.c2_fields()
     host = wl.c2_electrons(1 << 14, fields, seed=33)
@@ -205,7 +206,8 @@ def test_tile_sort(tables, energy_bits):
         ux, uy, uz = (got[k][valid] for k in ("ux", "uy", "uz"))
         u2 = ux * ux + uy * uy + uz * uz
         e = u2 * 510998.95 / ((c * c) * (np.sqrt(1.0 + u2 / (c * c)) + 1.0))
-        band = np.minimum(np.where(e > 1e3, np.floor(4.0 * np.log2(np.maximum(e, 1e3) * 1e-3)), 0), 2 ** bits - 1)
+        quarter = np.minimum(np.where(e > 1e3, np.floor(4.0 * np.log2(np.maximum(e, 1e3) * 1e-3)), 0), 63)
+        band = np.minimum(quarter.astype(np.int64) >> (6 - bits), 2 ** bits - 1)
         same = np.diff(cell) == 0
         d = np.diff(band)[same]
         # float log2 on either side of a band edge may differ by one ulp
