@@ -380,6 +380,53 @@ rrea_status rrea_transport_step_batched(rrea_tables_t tables, rrea_batch_t batch
 rrea_status rrea_newborn_drain_batched(rrea_tables_t tables, rrea_batch_t batch, const rrea_step_config* config,
                                        const rrea_drain_io* io, rrea_stream_t stream);
 
+/* Graph-captured batched steps (no counterpart in the overlay, which steps one
+ * simulation per process and returns to the host between phases).  One graph
+ * holds `steps` whole steps of a batch: per step, the listed device ranges
+ * cleared (the instances' J and source grids), the record count zeroed, the
+ * optional photon push, the species steps in Apply's order (electrons,
+ * photons, positrons; RreaParticleInteraction.cpp:12-62) with their
+ * secondaries ordered into `records`, then the newborn drain of those records
+ * as the FIFO (rrea_newborn_drain_batched), then the next step number.
+ * Nothing in a step reads a count on the host: the records are ordered with
+ * their count on the device, and the drain's generation bounds live on the
+ * device, up to max_generations generations per step (a FIFO still holding
+ * an unadvanced generation after the last is RREA_ERR_SECONDARY_CAPACITY,
+ * index -2).  The particle states, records, FIFO and survivors equal those of
+ * rrea_transport_step_batched + rrea_newborn_drain_batched on the same inputs
+ * (the J grids to summation order).  Tiles keep their size: invalidated
+ * particles stay in place and are skipped, as in a plain step.  Per-particle
+ * and per-entry outputs (status, ion pairs, substeps) are not written.  The
+ * tiles, instance arrays and buffers named here are bound at creation and
+ * must outlive the graph; every launch uploads the instances' environments
+ * from their fields handles, so field and density updates between launches
+ * are seen.  Errors accumulate in `error` and are read by the caller. */
+typedef struct rrea_batch_graph_desc {
+    const rrea_tile* tiles[3];       /* electrons, positrons, photons (index = species); NULL or n == 0: absent */
+    const int32_t* instance[3];      /* each tile's per-particle instance index (device int32) */
+    rrea_step_config config;         /* step is replaced by the launch's step numbers */
+    rrea_secondary* records;         /* device: the step's ordered secondaries, then the whole FIFO */
+    int64_t record_capacity;
+    int64_t* record_count;           /* device int64: FIFO length at the end of each step */
+    rrea_secondary* survivors;       /* device: the drain's survivors, FIFO order */
+    int64_t survivor_capacity;
+    int64_t* survivor_count;         /* device int64 */
+    double* tallies;                 /* device double[RREA_TALLY_COUNT], accumulated */
+    int32_t* error;                  /* device int32[2] */
+    double* clear[4];                /* device double ranges zeroed at the start of every step */
+    int64_t clear_n[4];
+    int32_t max_generations;         /* newborn FIFO generations per step, 1..64 */
+    int32_t steps;                   /* steps per launch, 1..1024 */
+    int32_t push_photons;            /* nonzero: the straight photon push (rrea_photon_push) opens each step */
+    int32_t reserved;
+} rrea_batch_graph_desc;
+typedef struct rrea_batch_graph_s* rrea_batch_graph_t;
+rrea_status rrea_batch_graph_create(rrea_tables_t tables, rrea_batch_t batch, const rrea_batch_graph_desc* desc,
+                                    rrea_batch_graph_t* out);
+/* Enqueues the graph's steps first_step, first_step + 1, ... on `stream`. */
+rrea_status rrea_batch_graph_launch(rrea_batch_graph_t graph, int64_t first_step, rrea_stream_t stream);
+rrea_status rrea_batch_graph_destroy(rrea_batch_graph_t graph);
+
 /* Device glue on either side of the hook (the WarpX calls the reference
  * makes around Apply, done here without leaving HBM). */
 

@@ -127,6 +127,16 @@ class DrainIO(C.Structure):
     ]
 
 
+class BatchGraphDesc(C.Structure):
+    _fields_ = [
+        ("tiles", C.c_void_p * 3), ("instance", C.c_void_p * 3), ("config", StepConfigC),
+        ("records", C.c_void_p), ("record_capacity", C.c_int64), ("record_count", C.c_void_p),
+        ("survivors", C.c_void_p), ("survivor_capacity", C.c_int64), ("survivor_count", C.c_void_p),
+        ("tallies", C.c_void_p), ("error", C.c_void_p), ("clear", C.c_void_p * 4), ("clear_n", C.c_int64 * 4),
+        ("max_generations", C.c_int32), ("steps", C.c_int32), ("push_photons", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
 TALLY_COUNT = 24
 SOURCE_FIELDS = 5
 STATUS = {0: "survive", 1: "escape", 2: "demoted", 3: "zero_weight", 4: "no_writeback", 5: "skipped_invalid",
@@ -174,6 +184,10 @@ SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(DriftConfig)] + [C.c_void_p] * 7),
     ("rrea_sources_normalize", C.c_int32, [C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_void_p]),
     ("rrea_batch_bind_sources", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("rrea_batch_graph_create", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.POINTER(BatchGraphDesc), C.POINTER(C.c_void_p)]),
+    ("rrea_batch_graph_launch", C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p]),
+    ("rrea_batch_graph_destroy", C.c_int32, [C.c_void_p]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),

@@ -2901,11 +2901,14 @@ __global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __re
     NewbornArgs const& N = *args;
     double tally[kTallyNewborn];
     for (int q = 0; q < kTallyNewborn; ++q) tally[q] = 0.0;
-    int64_t const i = N.gb + static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     // an error recorded before the launch (the step's, or a rejected birth
     // payload) stops the drain: nothing is advanced
     bool const stopped = __hip_atomic_load(N.S.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-    if (i < N.ge && !stopped) {
+    // grid-stride: a launch sized for the FIFO's capacity covers a generation
+    // whose bounds only the device knows
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    int64_t const ge = stopped ? N.gb : N.ge;
+    for (int64_t i = N.gb + static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < ge; i += stride) {
         rrea_secondary const& p = N.work[i];
         bool const photon = p.species == 2;
         // transient_energy_is_certified OV/RreaSecondaryCommit.cpp:148-180
@@ -2931,11 +2934,15 @@ __global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __re
 // The FIFO's birth-payload validation (OV/RreaSecondaryCommit.cpp:338-376),
 // each entry's root anchor (the first entry of its root's contiguous run) and
 // the root's next free lineage (:390-394).
-__global__ void fifo_init_kernel(const rrea_secondary* __restrict__ work, int64_t n, int64_t* __restrict__ anchor,
-                                 unsigned long long* __restrict__ next, int32_t* __restrict__ error)
+// `dn` (device, may be NULL) replaces n.  Grid-stride, so a launch sized for
+// the capacity covers a count known only on the device.
+__global__ void fifo_init_kernel(const rrea_secondary* __restrict__ work, int64_t n, const int64_t* __restrict__ dn,
+                                 int64_t* __restrict__ anchor, unsigned long long* __restrict__ next,
+                                 int32_t* __restrict__ error)
 {
-    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (dn) n = *dn;
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     rrea_secondary const& p = work[i];
     bool const finite_payload = finite(p.x) && finite(p.y) && finite(p.z) && finite(p.dir_x) && finite(p.dir_y)
         && finite(p.dir_z) && finite(p.weight) && finite(p.energy_eV) && finite(p.birth_fraction);
@@ -2949,51 +2956,62 @@ __global__ void fifo_init_kernel(const rrea_secondary* __restrict__ work, int64_
     while (a > 0 && work[a - 1].root_key == p.root_key && work[a - 1].root_species == p.root_species) --a;
     anchor[i] = a;
     atomicMax(&next[a], static_cast<unsigned long long>(p.lineage + 1U));
+    }
 }
 
 // Children of one generation, already in FIFO order: inherit the parent's
-// anchor
+// anchor.  The generation's FIFO range is [b, e), or be[0..1] on the device
+// when be is given; grid-stride as fifo_init_kernel.
 __global__ void fifo_anchor_kernel(const rrea_secondary* __restrict__ work, int64_t b, int64_t e,
-                                   int64_t* __restrict__ anchor)
+                                   const int64_t* __restrict__ be, int64_t* __restrict__ anchor)
 {
-    int64_t const j = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= e) return;
-    anchor[j] = anchor[work[j].parent_index];
+    if (be) { b = be[0]; e = be[1]; }
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t j = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < e; j += stride) {
+        anchor[j] = anchor[work[j].parent_index];
+    }
 }
 
 // allocate_lineage (OV/RreaSecondaryCommit.cpp:395-406) for one generation:
 // the root's next free lineage plus the child's rank within its root's run
 __global__ void fifo_lineage_kernel(rrea_secondary* __restrict__ work, int64_t b, int64_t e,
-                                    const int64_t* __restrict__ anchor, const unsigned long long* __restrict__ next,
-                                    int32_t* __restrict__ error)
+                                    const int64_t* __restrict__ be, const int64_t* __restrict__ anchor,
+                                    const unsigned long long* __restrict__ next, int32_t* __restrict__ error)
 {
-    int64_t const j = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= e) return;
-    int64_t const a = anchor[j];
-    int64_t s = j;
-    while (s > b && anchor[s - 1] == a) --s;
-    uint64_t const lin = static_cast<uint64_t>(next[a]) + static_cast<uint64_t>(j - s);
-    if (lin >= kLineageCap) report(error, kErrInvalid, j);
-    work[j].lineage = lin;
+    if (be) { b = be[0]; e = be[1]; }
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t j = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < e; j += stride) {
+        int64_t const a = anchor[j];
+        int64_t s = j;
+        while (s > b && anchor[s - 1] == a) --s;
+        uint64_t const lin = static_cast<uint64_t>(next[a]) + static_cast<uint64_t>(j - s);
+        if (lin >= kLineageCap) report(error, kErrInvalid, j);
+        work[j].lineage = lin;
+    }
 }
 
 // The last child of each root's run advances the root's next free lineage.
-__global__ void fifo_advance_kernel(int64_t b, int64_t e, const int64_t* __restrict__ anchor,
-                                    unsigned long long* __restrict__ next)
+__global__ void fifo_advance_kernel(int64_t b, int64_t e, const int64_t* __restrict__ be,
+                                    const int64_t* __restrict__ anchor, unsigned long long* __restrict__ next)
 {
-    int64_t const j = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= e) return;
-    int64_t const a = anchor[j];
-    if (j + 1 < e && anchor[j + 1] == a) return;
-    int64_t s = j;
-    while (s > b && anchor[s - 1] == a) --s;
-    next[a] += static_cast<unsigned long long>(j - s + 1);
+    if (be) { b = be[0]; e = be[1]; }
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t j = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < e; j += stride) {
+        int64_t const a = anchor[j];
+        if (j + 1 < e && anchor[j + 1] == a) continue;
+        int64_t s = j;
+        while (s > b && anchor[s - 1] == a) --s;
+        next[a] += static_cast<unsigned long long>(j - s + 1);
+    }
 }
 
-__global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t n, unsigned long long v)
+// `dn` (device, may be NULL) replaces n; grid-stride.
+__global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t n, unsigned long long v,
+                                const int64_t* __restrict__ dn = nullptr)
 {
-    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    if (dn) n = *dn;
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
 // Survivors in FIFO order: the end-of-step states of the entries that survive.
@@ -3224,6 +3242,306 @@ __global__ void set_i64_kernel(int64_t* __restrict__ p, int64_t v)
 __global__ void report_kernel(int32_t* __restrict__ error, int32_t code, int64_t index)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) report(error, code, index);
+}
+
+__global__ void copy_i64_kernel(int64_t* __restrict__ dst, const int64_t* __restrict__ src)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+
+__global__ void set_i32_kernel(int32_t* __restrict__ p, int32_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// The caller's count after an ordered append: its value at the start of the
+// call plus the records staged (the staged count is not clipped, as in the
+// host-ordered path).
+__global__ void append_count_dev_kernel(int64_t* __restrict__ count, const int64_t* __restrict__ base,
+                                        const int64_t* __restrict__ staged)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *count = *base + *staged;
+}
+
+// Clears `n` doubles (graph-safe: a kernel, not a memset node).
+__global__ void fill_f64_kernel(double* __restrict__ p, int64_t n)
+{
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Record ordering with the count on the device (graph-captured steps).  One
+// workgroup sorts the staged records by the key the host-ordered path sorts
+// on, (parent_index << 16) | (lineage & 0xffff): a stable LSD radix sort
+// over 8-bit digits, as many digits as the largest key has, a pass whose
+// digit is the same for every record skipped.  Within each 1024-record tile
+// a record's rank among equal digits comes from wavefront ballots over the
+// digit's bits and per-wavefront digit counts in LDS, so the scatter keeps
+// the input order of equal keys (the order hipcub's radix sort also keeps).
+// Up to kSortLds records are sorted in LDS; more go through global ping-pong
+// buffers of the workspace.  Writes the permutation to `order`.
+// ---------------------------------------------------------------------------
+constexpr int kSortThreads = 1024;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortLds = 2048;
+
+__device__ __forceinline__ uint64_t record_key(rrea_secondary const& r)
+{
+    return (static_cast<uint64_t>(r.parent_index) << 16) | (r.lineage & 0xffffULL);
+}
+
+// The number of records to order: the device count clipped to the staging
+// capacity (host value, or the device value at cap_dev when given).
+__device__ __forceinline__ int64_t staged_records(const int64_t* count, const int64_t* cap_dev, int64_t cap)
+{
+    int64_t const c = cap_dev ? *cap_dev : cap;
+    int64_t const n = *count < c ? *count : c;
+    return n > 0 ? n : 0;
+}
+
+__global__ void __launch_bounds__(kSortThreads) sort_records_kernel(
+    const rrea_secondary* __restrict__ in, const int64_t* __restrict__ count, const int64_t* __restrict__ cap_dev,
+    int64_t cap, uint64_t* __restrict__ gk0, uint64_t* __restrict__ gk1, int32_t* __restrict__ gi0,
+    int32_t* __restrict__ gi1, int32_t* __restrict__ order)
+{
+    __shared__ uint64_t lk[2][kSortLds];
+    __shared__ int32_t li[2][kSortLds];
+    __shared__ uint16_t wc[kSortWaves][256];
+    __shared__ uint32_t hist[256], dbase[256], tot[256];
+    __shared__ uint64_t wmax[kSortWaves];
+    __shared__ int32_t skip[8];  // per pass, so no reset races a slower wavefront's read
+    int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int64_t const n = staged_records(count, cap_dev, cap);
+    if (n == 0) return;
+    bool const in_lds = n <= kSortLds;
+    uint64_t* ks = in_lds ? lk[0] : gk0;
+    uint64_t* kd = in_lds ? lk[1] : gk1;
+    int32_t* is = in_lds ? li[0] : gi0;
+    int32_t* id = in_lds ? li[1] : gi1;
+    uint64_t mx = 0;
+    for (int64_t i = tid; i < n; i += kSortThreads) {
+        uint64_t const k = record_key(in[i]);
+        ks[i] = k;
+        is[i] = static_cast<int32_t>(i);
+        mx = k > mx ? k : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        uint64_t const o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if (lane == 0) wmax[wave] = mx;
+    if (tid < 8) skip[tid] = 0;
+    for (int j = tid; j < kSortWaves * 256; j += kSortThreads) (&wc[0][0])[j] = 0;
+    __syncthreads();
+    mx = 0;
+    for (int w = 0; w < kSortWaves; ++w) mx = wmax[w] > mx ? wmax[w] : mx;
+    int const bits = mx ? 64 - __clzll(static_cast<long long>(mx)) : 0;
+    int const passes = (bits + 7) / 8;
+    uint64_t const lt = (1ULL << lane) - 1ULL;
+    for (int pass = 0; pass < passes; ++pass) {
+        int const shift = 8 * pass;
+        if (tid < 256) hist[tid] = 0;
+        __syncthreads();
+        for (int64_t i = tid; i < n; i += kSortThreads) atomicAdd(&hist[(ks[i] >> shift) & 255U], 1U);
+        __syncthreads();
+        if (tid < 256 && static_cast<int64_t>(hist[tid]) == n) skip[pass] = 1;
+        if (tid < 64) {
+            // exclusive scan of the 256 digit counts, four per lane
+            uint32_t const h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2], h3 = hist[4 * tid + 3];
+            uint32_t const own = h0 + h1 + h2 + h3;
+            uint32_t inc = own;
+            for (int off = 1; off < 64; off <<= 1) {
+                uint32_t const o = __shfl_up(inc, off, 64);
+                if (lane >= off) inc += o;
+            }
+            uint32_t const ex = inc - own;
+            dbase[4 * tid] = ex;
+            dbase[4 * tid + 1] = ex + h0;
+            dbase[4 * tid + 2] = ex + h0 + h1;
+            dbase[4 * tid + 3] = ex + h0 + h1 + h2;
+        }
+        __syncthreads();
+        if (skip[pass]) continue;  // every record has this digit: the order stands
+        for (int64_t b0 = 0; b0 < n; b0 += kSortThreads) {
+            int64_t const i = b0 + tid;
+            bool const valid = i < n;
+            uint64_t const key = valid ? ks[i] : 0ULL;
+            int32_t const idx = valid ? is[i] : 0;
+            uint32_t const d = static_cast<uint32_t>(key >> shift) & 255U;
+            uint64_t m = __ballot(valid);
+            for (int b = 0; b < 8; ++b) {
+                bool const bit = (d >> b) & 1U;
+                uint64_t const bb = __ballot(bit);
+                m &= bit ? bb : ~bb;
+            }
+            uint32_t const rank = static_cast<uint32_t>(__popcll(m & lt));
+            if (valid && (m & lt) == 0ULL) wc[wave][d] = static_cast<uint16_t>(__popcll(m));
+            __syncthreads();
+            if (tid < 256) {
+                uint32_t run = 0;
+                for (int w = 0; w < kSortWaves; ++w) {
+                    uint32_t const c = wc[w][tid];
+                    wc[w][tid] = static_cast<uint16_t>(run);
+                    run += c;
+                }
+                tot[tid] = run;
+            }
+            __syncthreads();
+            if (valid) {
+                uint32_t const pos = dbase[d] + wc[wave][d] + rank;
+                kd[pos] = key;
+                id[pos] = idx;
+            }
+            __syncthreads();
+            if (tid < 256) {
+                dbase[tid] += tot[tid];
+                for (int w = 0; w < kSortWaves; ++w) wc[w][tid] = 0;
+            }
+            __syncthreads();
+        }
+        uint64_t* const tk = ks; ks = kd; kd = tk;
+        int32_t* const ti = is; is = id; id = ti;
+    }
+    for (int64_t i = tid; i < n; i += kSortThreads) order[i] = is[i];
+}
+
+// The ordered records appended behind the count at `base` (device; NULL: 0),
+// within the destination's capacity; records beyond it are dropped and
+// reported.  With `be` (device) the destination range is [be[0], be[1])
+// instead: a drain generation's children, empty when the drain stopped.
+__global__ void gather_records_kernel(const rrea_secondary* __restrict__ in, const int32_t* __restrict__ order,
+                                      const int64_t* __restrict__ count, const int64_t* __restrict__ cap_dev,
+                                      int64_t cap, const int64_t* __restrict__ base, int64_t dst_cap,
+                                      rrea_secondary* __restrict__ dst, int32_t* __restrict__ error,
+                                      const int64_t* __restrict__ be = nullptr)
+{
+    int64_t const n = be ? be[1] - be[0] : staged_records(count, cap_dev, cap);
+    int64_t const b = be ? be[0] : (base ? *base : 0);
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        int64_t const at = b + i;
+        if (at < dst_cap) dst[at] = in[order[i]];
+        else report(error, kErrCapacity, -1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The device-driven newborn drain (graph-captured steps).  The host-driven
+// drain reads each generation's child count to size the next launches; here
+// the generation bounds live on the device (DrainState) and every launch is
+// sized for the FIFO's capacity, finishing early when a generation is empty.
+// ---------------------------------------------------------------------------
+struct DrainState {
+    int64_t gb, ge;         // the generation being advanced
+    int64_t stage_cap;      // room for its children: capacity - ge
+    int64_t be[2];          // its children's FIFO range once ordered
+    int64_t cap;            // FIFO capacity
+};
+
+// The step's ordered records are generation 0.
+__global__ void drain_begin_kernel(NewbornArgs* __restrict__ N, DrainState* __restrict__ g,
+                                   const int64_t* __restrict__ n_initial, int64_t cap)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t n0 = *n_initial;
+    n0 = n0 < 0 ? 0 : (n0 > cap ? cap : n0);
+    g->gb = 0; g->ge = n0; g->stage_cap = cap - n0; g->cap = cap;
+    g->be[0] = n0; g->be[1] = n0;
+    N->gb = 0; N->ge = n0; N->S.sec_cap = cap - n0;
+}
+
+// After a generation: its children's range, empty when it made none, an
+// error is recorded, or they overflowed the FIFO (the newborn kernel
+// reported that).
+__global__ void drain_bounds_kernel(DrainState* __restrict__ g, const int64_t* __restrict__ children,
+                                    const int32_t* __restrict__ error)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t const c = *children;
+    bool const stop = error[0] != 0 || c <= 0 || c > g->stage_cap;
+    g->be[0] = g->ge;
+    g->be[1] = stop ? g->ge : g->ge + c;
+}
+
+// The children become the next generation.
+__global__ void drain_advance_kernel(NewbornArgs* __restrict__ N, DrainState* __restrict__ g)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    g->gb = g->be[0];
+    g->ge = g->be[1];
+    g->stage_cap = g->cap - g->ge;
+    N->gb = g->gb; N->ge = g->ge; N->S.sec_cap = g->stage_cap;
+}
+
+// The FIFO's length; a generation left unadvanced after the last captured
+// one is a capacity error (index -2).
+__global__ void drain_end_kernel(const DrainState* __restrict__ g, int64_t* __restrict__ work_count,
+                                 int32_t* __restrict__ error)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *work_count = g->ge;
+    if (g->gb < g->ge) report(error, kErrCapacity, -2);
+}
+
+// Survivors in FIFO order (one workgroup): the end-of-step states of the
+// entries with RREA_SURVIVE, up to the capacity (beyond it: reported, the
+// count clipped); none after an error, as in the host-driven drain.
+__global__ void __launch_bounds__(1024) survivors_kernel(const rrea_secondary* __restrict__ fin,
+                                                         const uint8_t* __restrict__ status,
+                                                         const DrainState* __restrict__ g,
+                                                         rrea_secondary* __restrict__ out, int64_t out_cap,
+                                                         int64_t* __restrict__ out_count, int32_t* __restrict__ error)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ int64_t base;
+    int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    bool const failed = __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    int64_t const total = failed ? 0 : g->ge;
+    if (tid == 0) base = 0;
+    __syncthreads();
+    uint64_t const lt = (1ULL << lane) - 1ULL;
+    for (int64_t b0 = 0; b0 < total; b0 += 1024) {
+        int64_t const i = b0 + tid;
+        bool const keep = i < total && status[i] == RREA_SURVIVE;
+        uint64_t const m = __ballot(keep);
+        if (lane == 0) wsum[wave] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        int64_t at = base;
+        for (int w = 0; w < wave; ++w) at += wsum[w];
+        at += __popcll(m & lt);
+        if (keep) {
+            if (at < out_cap) out[at] = fin[i];
+            else report(error, kErrCapacity, -1);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int64_t t = 0;
+            for (int w = 0; w < 16; ++w) t += wsum[w];
+            base += t;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *out_count = base < out_cap ? base : out_cap;
+}
+
+// The step number of every argument block a captured step reads.
+__global__ void set_step_kernel(StepArgs* a0, StepArgs* a1, StepArgs* a2, NewbornArgs* nb, int64_t step)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (a0) a0->cfg.step = step;
+    if (a1) a1->cfg.step = step;
+    if (a2) a2->cfg.step = step;
+    if (nb) nb->S.cfg.step = step;
+}
+
+__global__ void next_step_kernel(StepArgs* a0, StepArgs* a1, StepArgs* a2, NewbornArgs* nb)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (a0) a0->cfg.step += 1;
+    if (a1) a1->cfg.step += 1;
+    if (a2) a2->cfg.step += 1;
+    if (nb) nb->S.cfg.step += 1;
 }
 
 int64_t blocks_for(int64_t n, int b) { return (n + b - 1) / b; }
@@ -3619,9 +3937,9 @@ rrea_status order_staged(rrea_fields_t F, int64_t n, rrea_secondary* dst, hipStr
     return RREA_OK;
 }
 
-// The fast paths' hand-over list, sized for n particles, with its count
-// cleared on `s`.
-rrea_status slow_list_workspace(rrea_fields_t F, int64_t n, hipStream_t s)
+// The fast paths' hand-over list and its count, sized for n particles (the
+// count is cleared by the launch sequence).
+rrea_status slow_list_workspace(rrea_fields_t F, int64_t n)
 {
     if (F->slow_cap < n) {
         if (F->d_slow_list) (void)hipFree(F->d_slow_list);
@@ -3632,7 +3950,6 @@ rrea_status slow_list_workspace(rrea_fields_t F, int64_t n, hipStream_t s)
         F->slow_cap = n;
     }
     if (!F->d_slow_count) HIP_TRY(hipMalloc(&F->d_slow_count, sizeof(int32_t)));
-    HIP_TRY(hipMemsetAsync(F->d_slow_count, 0, sizeof(int32_t), s));
     return RREA_OK;
 }
 
@@ -3720,9 +4037,23 @@ struct BatchCtx {
     const int32_t* inst;
 };
 
-rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
-                         const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream,
-                         const BatchCtx* batch = nullptr)
+// What a species step's launches need beyond its device argument block.
+struct SpeciesLaunch {
+    int species = 0;
+    bool batched = false;
+    bool run = false;       // false: nothing to launch (dt <= 0 or an empty tile)
+    bool order = false;     // the staged secondaries are ordered into the caller's buffer
+    bool complete = false;  // every particle through the complete step
+    int64_t n = 0;
+    int64_t cap = 0;        // the caller's secondary capacity
+};
+
+// A species step's host half: argument checks, the argument block A and the
+// workspace it needs (grown here, before any launch, so that the launches
+// can be captured into a graph).
+rrea_status species_prepare(int species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+                            const rrea_step_config* cfg, const rrea_step_out* out, const BatchCtx* batch,
+                            StepArgs& A, SpeciesLaunch& L)
 {
     bool const batched = batch != nullptr;
     bool const positron = species == kPositron;
@@ -3743,18 +4074,19 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     if (photon && tile->n > 0 && (tile->prev_r == tile->r || tile->prev_z == tile->z || tile->prev_theta == tile->theta)) {
         return RREA_ERR_ARGUMENT;
     }
+    L = SpeciesLaunch{};
+    L.species = species;
+    L.batched = batched;
     // RreaParticleInteraction::Apply :38-40 -- a non-positive dt does nothing.
     if (!(cfg->dt_s > 0.0) || tile->n == 0) return RREA_OK;
     if (!(cfg->max_optical_depth_per_substep > 0.0) || cfg->max_subcycles == 0 || cfg->max_subcycles > 16383) {
         return RREA_ERR_ARGUMENT;
     }
-    hipStream_t const s = static_cast<hipStream_t>(stream);
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     if (dev != T->device) return RREA_ERR_ARGUMENT;
 
     rrea_fields_t const W = batched ? batch->work : F;  // the call's workspace
-    StepArgs A;
     {
         rrea_status const cs = common_args(T, F, cfg, out->jr, out->jz, out->rho, A, batched ? nullptr : out->sources);
         if (cs != RREA_OK) return cs;
@@ -3781,9 +4113,6 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
             if (ws != RREA_OK) return ws;
         }
         if (!W->d_base) HIP_TRY(hipMalloc(&W->d_base, sizeof(int64_t)));
-        HIP_TRY(hipMemcpyAsync(W->d_base, out->secondary_count, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
-        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(W->d_count), 1, 0ULL);
-        HIP_TRY(hipGetLastError());
         A.sec = W->d_stage_sec;
         A.sec_count = W->d_count;
     } else {
@@ -3791,10 +4120,42 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
         A.sec_count = out->secondary_count;
     }
     A.sec_cap = cap;
+    if (!cfg->complete_path_only) {
+        rrea_status const ls = slow_list_workspace(W, tile->n);
+        if (ls != RREA_OK) return ls;
+    }
+    L.run = true;
+    L.order = order;
+    L.complete = cfg->complete_path_only != 0;
+    L.n = tile->n;
+    L.cap = cap;
+    return RREA_OK;
+}
 
-    int64_t const nb = blocks_for(tile->n, kBlock);
-    // pageable source: the copy has consumed A when hipMemcpyAsync returns
-    HIP_TRY(hipMemcpyAsync(W->d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
+// A species step's launches on `s`, reading the argument block at d_args.
+// With device_order the staged secondaries are ordered with their count on
+// the device (sort_records_kernel), so the sequence makes no host round trip
+// and can be captured into a graph; otherwise the count is read back once and
+// hipcub sorts exactly that many records.
+rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunch const& L, const rrea_step_out* out,
+                           hipStream_t s, bool device_order)
+{
+    if (!L.run) return RREA_OK;
+    bool const positron = L.species == kPositron;
+    bool const photon = L.species == kPhoton;
+    bool const batched = L.batched;
+    bool const order = L.order;
+    int64_t const cap = L.cap;
+    if (order) {
+        copy_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(W->d_base, out->secondary_count);
+        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(W->d_count), 1, 0ULL);
+        HIP_TRY(hipGetLastError());
+    }
+    if (!L.complete) {
+        set_i32_kernel<<<dim3(1), dim3(64), 0, s>>>(W->d_slow_count, 0);
+        HIP_TRY(hipGetLastError());
+    }
+    int64_t const nb = blocks_for(L.n, kBlock);
     // HIP events around each transport kernel while timing is enabled
     auto timed = [&](int32_t kernel, auto&& launch) -> rrea_status {
         hipEvent_t stop = nullptr;
@@ -3815,14 +4176,11 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
         if (stop) HIP_TRY(hipEventRecord(stop, s));
         return RREA_OK;
     };
-    const StepArgs* const d_args = static_cast<const StepArgs*>(W->d_args);
     rrea_status st = RREA_OK;
     int32_t const kid = positron ? 2 : 0;   // timing ids: main kernel, then its hand-over kernel
-    if (photon && cfg->complete_path_only) {
+    if (photon && L.complete) {
         st = timed(4, [&] { photon_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args); });
     } else if (photon) {
-        rrea_status const ls = slow_list_workspace(W, tile->n, s);
-        if (ls != RREA_OK) return ls;
         st = timed(4, [&] {
             if (batched) {
                 photon_fast_kernel<true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
@@ -3838,14 +4196,12 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
             photon_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
                 d_args, W->d_slow_list, W->d_slow_count);
         });
-    } else if (cfg->complete_path_only) {
+    } else if (L.complete) {
         st = timed(kid, [&] {
             if (positron) positron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
             else electron_full_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args);
         });
     } else {
-        rrea_status const ls = slow_list_workspace(W, tile->n, s);
-        if (ls != RREA_OK) return ls;
         st = timed(kid, [&] {
             if (positron) {
                 if (batched) {
@@ -3879,7 +4235,18 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     }
     if (st != RREA_OK) return st;
 
-    if (order) {
+    if (order && device_order) {
+        sort_records_kernel<<<dim3(1), dim3(kSortThreads), 0, s>>>(
+            W->d_stage_sec, W->d_count, nullptr, cap, W->d_keys, W->d_keys_out, reinterpret_cast<int32_t*>(W->d_idx),
+            reinterpret_cast<int32_t*>(W->d_idx) + cap, reinterpret_cast<int32_t*>(W->d_idx_out));
+        HIP_TRY(hipGetLastError());
+        gather_records_kernel<<<dim3(static_cast<unsigned>(std::min<int64_t>(blocks_for(cap, 256), 1024))), dim3(256),
+                                0, s>>>(W->d_stage_sec, reinterpret_cast<const int32_t*>(W->d_idx_out), W->d_count,
+                                        nullptr, cap, W->d_base, cap, out->secondaries, out->error);
+        HIP_TRY(hipGetLastError());
+        append_count_dev_kernel<<<dim3(1), dim3(64), 0, s>>>(out->secondary_count, W->d_base, W->d_count);
+        HIP_TRY(hipGetLastError());
+    } else if (order) {
         // the one host round trip of the call: the sort's item count
         int64_t nstaged = 0;
         HIP_TRY(hipMemcpyAsync(&nstaged, W->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -3892,6 +4259,98 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
         append_count_kernel<<<dim3(1), dim3(64), 0, s>>>(out->secondary_count, W->d_base, nstaged);
         HIP_TRY(hipGetLastError());
     }
+    return RREA_OK;
+}
+
+rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+                         const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream,
+                         const BatchCtx* batch = nullptr)
+{
+    StepArgs A;
+    SpeciesLaunch L;
+    rrea_status const ps = species_prepare(species, T, F, tile, cfg, out, batch, A, L);
+    if (ps != RREA_OK || !L.run) return ps;
+    rrea_fields_t const W = batch ? batch->work : F;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    // pageable source: the copy has consumed A when hipMemcpyAsync returns
+    HIP_TRY(hipMemcpyAsync(W->d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
+    return species_launch(W, static_cast<const StepArgs*>(W->d_args), L, out, s, false);
+}
+
+
+// The drain's workspace for a FIFO of `cap` records: the ordering workspace,
+// the per-entry end states, statuses, ion pairs and substeps, the lineage
+// anchors and counters, the survivor selection and the argument block.
+rrea_status fifo_workspace(rrea_fields_t W, int64_t cap)
+{
+    {
+        rrea_status const ws = order_workspace(W, cap);
+        if (ws != RREA_OK) return ws;
+    }
+    if (W->fifo_cap < cap) {
+        void* ptrs[] = {W->d_fin, W->d_fst, W->d_fion, W->d_fsub, W->d_anchor, W->d_next, W->d_flag, W->d_sel_tmp};
+        for (void* p : ptrs) if (p) (void)hipFree(p);
+        W->d_fin = nullptr; W->d_fst = nullptr; W->d_fion = nullptr; W->d_fsub = nullptr;
+        W->d_anchor = nullptr; W->d_next = nullptr;
+        W->d_flag = nullptr; W->d_sel_tmp = nullptr;
+        W->fifo_cap = 0;
+        HIP_TRY(hipMalloc(&W->d_fin, cap * sizeof(rrea_secondary)));
+        HIP_TRY(hipMalloc(&W->d_fst, cap * sizeof(uint8_t)));
+        HIP_TRY(hipMalloc(&W->d_fion, cap * sizeof(double)));
+        HIP_TRY(hipMalloc(&W->d_fsub, cap * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&W->d_anchor, cap * sizeof(int64_t)));
+        HIP_TRY(hipMalloc(&W->d_next, cap * sizeof(unsigned long long)));
+        HIP_TRY(hipMalloc(&W->d_flag, cap * sizeof(int32_t)));
+        size_t bytes = 0;
+        HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, W->d_fin, W->d_flag, W->d_stage_sec, W->d_nsel,
+                                              static_cast<int>(cap)));
+        HIP_TRY(hipMalloc(&W->d_sel_tmp, bytes));
+        W->sel_tmp_bytes = bytes;
+        W->fifo_cap = cap;
+    }
+    if (!W->d_nsel) HIP_TRY(hipMalloc(&W->d_nsel, sizeof(int64_t)));
+    if (!W->d_nargs) HIP_TRY(hipMalloc(&W->d_nargs, sizeof(NewbornArgs)));
+    return RREA_OK;
+}
+
+// The device-driven drain's launches: no host round trip, so the sequence
+// can be captured into a graph.  The argument block at d_nargs holds all but
+// the generation bounds, which drain_begin/advance_kernel write on the
+// device; max_gen generations are launched, each sized for the FIFO's
+// capacity and finishing at once when its generation is empty.  The FIFO's
+// records, order, lineages and survivors are those of drain_impl; the
+// per-entry status, ion pairs and substeps stay in the workspace.
+rrea_status drain_device_launch(rrea_fields_t W, NewbornArgs* d_nargs, DrainState* d_state, rrea_secondary* work,
+                                int64_t cap, const int64_t* n_initial, int64_t* work_count,
+                                rrea_secondary* survivors, int64_t survivor_cap, int64_t* survivor_count,
+                                int32_t* error, int32_t max_gen, hipStream_t s)
+{
+    unsigned const g256 = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks_for(cap, 256), 1024)));
+    unsigned const gnb = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks_for(cap, kBlock), 1024)));
+    int32_t* const order = reinterpret_cast<int32_t*>(W->d_idx_out);
+    drain_begin_kernel<<<dim3(1), dim3(64), 0, s>>>(d_nargs, d_state, n_initial, cap);
+    fill_u64_kernel<<<dim3(g256), dim3(256), 0, s>>>(W->d_next, 0, 0ULL, &d_state->ge);
+    fifo_init_kernel<<<dim3(g256), dim3(256), 0, s>>>(work, 0, &d_state->ge, W->d_anchor, W->d_next, error);
+    HIP_TRY(hipGetLastError());
+    for (int32_t gen = 0; gen < max_gen; ++gen) {
+        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(W->d_count), 1, 0ULL);
+        newborn_kernel<<<dim3(gnb), dim3(kBlock), 0, s>>>(d_nargs);
+        sort_records_kernel<<<dim3(1), dim3(kSortThreads), 0, s>>>(
+            W->d_stage_sec, W->d_count, &d_state->stage_cap, 0, W->d_keys, W->d_keys_out,
+            reinterpret_cast<int32_t*>(W->d_idx), reinterpret_cast<int32_t*>(W->d_idx) + cap, order);
+        drain_bounds_kernel<<<dim3(1), dim3(64), 0, s>>>(d_state, W->d_count, error);
+        gather_records_kernel<<<dim3(g256), dim3(256), 0, s>>>(W->d_stage_sec, order, W->d_count, nullptr, 0, nullptr,
+                                                               cap, work, error, d_state->be);
+        fifo_anchor_kernel<<<dim3(g256), dim3(256), 0, s>>>(work, 0, 0, d_state->be, W->d_anchor);
+        fifo_lineage_kernel<<<dim3(g256), dim3(256), 0, s>>>(work, 0, 0, d_state->be, W->d_anchor, W->d_next, error);
+        fifo_advance_kernel<<<dim3(g256), dim3(256), 0, s>>>(0, 0, d_state->be, W->d_anchor, W->d_next);
+        drain_advance_kernel<<<dim3(1), dim3(64), 0, s>>>(d_nargs, d_state);
+        HIP_TRY(hipGetLastError());
+    }
+    drain_end_kernel<<<dim3(1), dim3(64), 0, s>>>(d_state, work_count, error);
+    survivors_kernel<<<dim3(1), dim3(1024), 0, s>>>(W->d_fin, W->d_fst, d_state, survivors, survivor_cap,
+                                                    survivor_count, error);
+    HIP_TRY(hipGetLastError());
     return RREA_OK;
 }
 
@@ -3933,37 +4392,14 @@ rrea_status drain_impl(rrea_tables_t T, rrea_fields_t F, const rrea_step_config*
     A.error = io->error;
     if (batch) A.batch = batch->d_envs;
     {
-        rrea_status const ws = order_workspace(W, cap);
+        rrea_status const ws = fifo_workspace(W, cap);
         if (ws != RREA_OK) return ws;
     }
-    if (W->fifo_cap < cap) {
-        void* ptrs[] = {W->d_fin, W->d_fst, W->d_fion, W->d_fsub, W->d_anchor, W->d_next, W->d_flag, W->d_sel_tmp};
-        for (void* p : ptrs) if (p) (void)hipFree(p);
-        W->d_fin = nullptr; W->d_fst = nullptr; W->d_fion = nullptr; W->d_fsub = nullptr;
-        W->d_anchor = nullptr; W->d_next = nullptr;
-        W->d_flag = nullptr; W->d_sel_tmp = nullptr;
-        W->fifo_cap = 0;
-        HIP_TRY(hipMalloc(&W->d_fin, cap * sizeof(rrea_secondary)));
-        HIP_TRY(hipMalloc(&W->d_fst, cap * sizeof(uint8_t)));
-        HIP_TRY(hipMalloc(&W->d_fion, cap * sizeof(double)));
-        HIP_TRY(hipMalloc(&W->d_fsub, cap * sizeof(int32_t)));
-        HIP_TRY(hipMalloc(&W->d_anchor, cap * sizeof(int64_t)));
-        HIP_TRY(hipMalloc(&W->d_next, cap * sizeof(unsigned long long)));
-        HIP_TRY(hipMalloc(&W->d_flag, cap * sizeof(int32_t)));
-        size_t bytes = 0;
-        HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, W->d_fin, W->d_flag, W->d_stage_sec, W->d_nsel,
-                                              static_cast<int>(cap), s));
-        HIP_TRY(hipMalloc(&W->d_sel_tmp, bytes));
-        W->sel_tmp_bytes = bytes;
-        W->fifo_cap = cap;
-    }
-    if (!W->d_nsel) HIP_TRY(hipMalloc(&W->d_nsel, sizeof(int64_t)));
-    if (!W->d_nargs) HIP_TRY(hipMalloc(&W->d_nargs, sizeof(NewbornArgs)));
     int64_t const n0 = io->n_initial;
     fill_u64_kernel<<<dim3(static_cast<unsigned>(blocks_for(n0, 256))), dim3(256), 0, s>>>(W->d_next, n0, 0ULL);
     HIP_TRY(hipGetLastError());
     fifo_init_kernel<<<dim3(static_cast<unsigned>(blocks_for(n0, 256))), dim3(256), 0, s>>>(
-        io->work, n0, W->d_anchor, W->d_next, io->error);
+        io->work, n0, nullptr, W->d_anchor, W->d_next, io->error);
     HIP_TRY(hipGetLastError());
     // One launch per generation; the host reads the child count in between.
     // The generation kernel advances nothing once an error is recorded (a
@@ -4009,11 +4445,12 @@ rrea_status drain_impl(rrea_tables_t T, rrea_fields_t F, const rrea_step_config*
             if (os != RREA_OK) return os;
         }
         unsigned const cb = static_cast<unsigned>(blocks_for(c, 256));
-        fifo_anchor_kernel<<<dim3(cb), dim3(256), 0, s>>>(io->work, ge, ge + c, W->d_anchor);
+        fifo_anchor_kernel<<<dim3(cb), dim3(256), 0, s>>>(io->work, ge, ge + c, nullptr, W->d_anchor);
         HIP_TRY(hipGetLastError());
-        fifo_lineage_kernel<<<dim3(cb), dim3(256), 0, s>>>(io->work, ge, ge + c, W->d_anchor, W->d_next, io->error);
+        fifo_lineage_kernel<<<dim3(cb), dim3(256), 0, s>>>(io->work, ge, ge + c, nullptr, W->d_anchor, W->d_next,
+                                                           io->error);
         HIP_TRY(hipGetLastError());
-        fifo_advance_kernel<<<dim3(cb), dim3(256), 0, s>>>(ge, ge + c, W->d_anchor, W->d_next);
+        fifo_advance_kernel<<<dim3(cb), dim3(256), 0, s>>>(ge, ge + c, nullptr, W->d_anchor, W->d_next);
         HIP_TRY(hipGetLastError());
         gb = ge;
         ge += c;
@@ -4628,6 +5065,189 @@ rrea_status rrea_sources_normalize(rrea_fields_t F, double* sources, double dt_s
     sources_normalize_kernel<<<dim3(static_cast<unsigned>(blocks_for(cells, 256))), dim3(256), 0, s>>>(
         g.nr, g.nz, g.r_lo, g.dr, g.dz, dt_s, diagnostics, sources);
     HIP_TRY(hipGetLastError());
+    return RREA_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Graph-captured batched steps
+// ---------------------------------------------------------------------------
+struct rrea_batch_graph_s {
+    rrea_batch_t batch = nullptr;
+    rrea_tile tiles[3] = {};
+    bool present[3] = {false, false, false};
+    void* d_sargs[3] = {nullptr, nullptr, nullptr};  // StepArgs per species
+    void* d_nargs = nullptr;                         // NewbornArgs of the drain
+    void* d_state = nullptr;                         // DrainState
+    rrea_fields_t work = nullptr;                    // the graph's own workspace: its buffers are baked in
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
+};
+
+rrea_status rrea_batch_graph_destroy(rrea_batch_graph_t G)
+{
+    if (!G) return RREA_OK;
+    if (G->exec) (void)hipGraphExecDestroy(G->exec);
+    if (G->graph) (void)hipGraphDestroy(G->graph);
+    for (void* p : G->d_sargs) if (p) (void)hipFree(p);
+    if (G->d_nargs) (void)hipFree(G->d_nargs);
+    if (G->d_state) (void)hipFree(G->d_state);
+    if (G->work) rrea_fields_destroy(G->work);
+    delete G;
+    return RREA_OK;
+}
+
+rrea_status rrea_batch_graph_create(rrea_tables_t T, rrea_batch_t B, const rrea_batch_graph_desc* d,
+                                    rrea_batch_graph_t* out)
+{
+    if (!T || !B || !d || !out) return RREA_ERR_ARGUMENT;
+    *out = nullptr;
+    if (!d->records || !d->record_count || d->record_capacity <= 0 || d->record_capacity > INT32_MAX
+        || !d->survivors || !d->survivor_count || d->survivor_capacity < 0 || !d->tallies || !d->error
+        || d->max_generations < 1 || d->max_generations > 64 || d->steps < 1 || d->steps > 1024) {
+        return RREA_ERR_ARGUMENT;
+    }
+    for (int k = 0; k < 4; ++k) {
+        if (d->clear_n[k] < 0 || (d->clear_n[k] > 0 && !d->clear[k])) return RREA_ERR_ARGUMENT;
+    }
+    bool any = false;
+    for (int k = 0; k < 3; ++k) {
+        if (d->tiles[k] && d->tiles[k]->n > 0) {
+            if (!d->instance[k]) return RREA_ERR_ARGUMENT;
+            any = true;
+        }
+    }
+    if (!any || !(d->config.dt_s > 0.0)) return RREA_ERR_ARGUMENT;
+    if (d->push_photons && !(d->tiles[kPhoton] && d->tiles[kPhoton]->n > 0)) return RREA_ERR_ARGUMENT;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != T->device) return RREA_ERR_ARGUMENT;
+
+    auto* G = new (std::nothrow) rrea_batch_graph_s();
+    if (!G) return RREA_ERR_HIP;
+    G->batch = B;
+    G->device = dev;
+    rrea_status st = rrea_fields_create(&B->fields[0]->geom, &G->work);
+    if (st != RREA_OK) {
+        rrea_batch_graph_destroy(G);
+        return st;
+    }
+    // a workspace no other call grows or frees: the captured launches hold its pointers
+    rrea_fields_t const W = G->work;
+    hipStream_t cs = nullptr;
+    auto fail = [&](rrea_status e) {
+        if (cs) (void)hipStreamDestroy(cs);
+        rrea_batch_graph_destroy(G);
+        return e;
+    };
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return fail(RREA_ERR_HIP);
+    st = upload_envs(B, &d->config, cs);
+    if (st != RREA_OK) return fail(st);
+
+    // every species' argument block and launch plan, formed on the host once
+    rrea_step_out so{};
+    so.secondaries = d->records;
+    so.secondary_capacity = d->record_capacity;
+    so.secondary_count = d->record_count;
+    so.tallies = d->tallies;
+    so.error = d->error;
+    so.order_secondaries = 1;
+    SpeciesLaunch L[3];
+    for (int k = 0; k < 3; ++k) {
+        if (!(d->tiles[k] && d->tiles[k]->n > 0)) continue;
+        G->tiles[k] = *d->tiles[k];
+        G->present[k] = true;
+        BatchCtx const ctx{W, static_cast<const Env*>(B->d_envs), d->instance[k]};
+        StepArgs A;
+        st = species_prepare(k, T, B->fields[0], &G->tiles[k], &d->config, &so, &ctx, A, L[k]);
+        if (st != RREA_OK) return fail(st);
+        if (hipMalloc(&G->d_sargs[k], sizeof(StepArgs)) != hipSuccess) return fail(RREA_ERR_HIP);
+        if (hipMemcpy(G->d_sargs[k], &A, sizeof(StepArgs), hipMemcpyHostToDevice) != hipSuccess) {
+            return fail(RREA_ERR_HIP);
+        }
+    }
+    // the drain's argument block: the step's records are the FIFO
+    {
+        st = fifo_workspace(W, d->record_capacity);
+        if (st != RREA_OK) return fail(st);
+        StepArgs A{};
+        st = common_args(T, B->fields[0], &d->config, nullptr, nullptr, nullptr, A, nullptr);
+        if (st != RREA_OK) return fail(st);
+        A.tallies = d->tallies;
+        A.error = d->error;
+        A.batch = static_cast<const Env*>(B->d_envs);
+        A.sec = W->d_stage_sec;
+        A.sec_cap = 0;  // written on the device per generation
+        A.sec_count = W->d_count;
+        NewbornArgs N{A, d->records, 0, 0, W->d_fin, W->d_fst, W->d_fion, W->d_fsub};
+        if (hipMalloc(&G->d_nargs, sizeof(NewbornArgs)) != hipSuccess
+            || hipMalloc(&G->d_state, sizeof(DrainState)) != hipSuccess
+            || hipMemcpy(G->d_nargs, &N, sizeof(NewbornArgs), hipMemcpyHostToDevice) != hipSuccess) {
+            return fail(RREA_ERR_HIP);
+        }
+    }
+    if (hipStreamSynchronize(cs) != hipSuccess) return fail(RREA_ERR_HIP);
+
+    // capture `steps` steps: clear, (photon push), Apply's species order, the
+    // drain, the next step number
+    StepArgs* const sa[3] = {static_cast<StepArgs*>(G->d_sargs[0]), static_cast<StepArgs*>(G->d_sargs[1]),
+                             static_cast<StepArgs*>(G->d_sargs[2])};
+    NewbornArgs* const na = static_cast<NewbornArgs*>(G->d_nargs);
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail(RREA_ERR_HIP);
+    rrea_status cst = RREA_OK;
+    for (int32_t step = 0; step < d->steps && cst == RREA_OK; ++step) {
+        set_i64_kernel<<<dim3(1), dim3(64), 0, cs>>>(d->record_count, 0);
+        for (int k = 0; k < 4; ++k) {
+            if (d->clear_n[k] == 0) continue;
+            unsigned const b = static_cast<unsigned>(std::min<int64_t>(blocks_for(d->clear_n[k], 256), 2048));
+            fill_f64_kernel<<<dim3(b), dim3(256), 0, cs>>>(d->clear[k], d->clear_n[k]);
+        }
+        if (hipGetLastError() != hipSuccess) { cst = RREA_ERR_HIP; break; }
+        if (d->push_photons) {
+            cst = rrea_photon_push(&G->tiles[kPhoton], d->config.dt_s, cs);
+            if (cst != RREA_OK) break;
+        }
+        for (int k : {static_cast<int>(kElectron), static_cast<int>(kPhoton), static_cast<int>(kPositron)}) {
+            if (!G->present[k]) continue;
+            cst = species_launch(W, sa[k], L[k], &so, cs, true);
+            if (cst != RREA_OK) break;
+        }
+        if (cst != RREA_OK) break;
+        cst = drain_device_launch(W, na, static_cast<DrainState*>(G->d_state), d->records, d->record_capacity,
+                                  d->record_count, d->record_count, d->survivors, d->survivor_capacity,
+                                  d->survivor_count, d->error, d->max_generations, cs);
+        if (cst != RREA_OK) break;
+        next_step_kernel<<<dim3(1), dim3(64), 0, cs>>>(sa[0], sa[1], sa[2], na);
+        if (hipGetLastError() != hipSuccess) cst = RREA_ERR_HIP;
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t const ce = hipStreamEndCapture(cs, &graph);
+    G->graph = graph;
+    if (cst != RREA_OK) return fail(cst);
+    if (ce != hipSuccess || !graph) return fail(RREA_ERR_HIP);
+    if (hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0) != hipSuccess) return fail(RREA_ERR_HIP);
+    (void)hipStreamDestroy(cs);
+    *out = G;
+    return RREA_OK;
+}
+
+rrea_status rrea_batch_graph_launch(rrea_batch_graph_t G, int64_t first_step, rrea_stream_t stream)
+{
+    if (!G || !G->exec) return RREA_ERR_ARGUMENT;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != G->device) return RREA_ERR_ARGUMENT;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    // the instances' environments as their fields handles stand now
+    rrea_status const es = upload_envs(G->batch, nullptr, s);
+    if (es != RREA_OK) return es;
+    set_step_kernel<<<dim3(1), dim3(64), 0, s>>>(static_cast<StepArgs*>(G->d_sargs[0]),
+                                                 static_cast<StepArgs*>(G->d_sargs[1]),
+                                                 static_cast<StepArgs*>(G->d_sargs[2]),
+                                                 static_cast<NewbornArgs*>(G->d_nargs), first_step);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipGraphLaunch(G->exec, s));
     return RREA_OK;
 }
 

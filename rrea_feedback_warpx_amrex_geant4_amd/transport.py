@@ -498,6 +498,90 @@ class Batch:
             pass
 
 
+class BatchGraph:
+    """Whole batched steps captured in one HIP graph (rrea_batch_graph_*): per
+    step the listed grids cleared, the optional photon push, the species
+    steps in Apply's order and the device-driven newborn drain, with no
+    host round trip inside.  ``tiles`` maps "electron", "positron" and
+    "photon" to (tile, inst) pairs of the batch's layout.  The records
+    buffer holds each step's ordered secondaries, then the whole FIFO; the
+    survivors are the drain's.  ``launch(first_step)`` enqueues the graph's
+    ``steps`` steps; outputs are read after a synchronisation."""
+
+    def __init__(self, tables: InteractionTables, batch: Batch, tiles: dict, cfg: StepConfig, record_capacity: int,
+                 clear=(), steps: int = 1, max_generations: int = 8, push_photons: bool = False,
+                 survivor_capacity: int | None = None, device="cuda"):
+        size = SECONDARY_DTYPE.itemsize
+        self.record_capacity = int(record_capacity)
+        self.survivor_capacity = int(survivor_capacity if survivor_capacity is not None else record_capacity)
+        self.records = torch.zeros(max(self.record_capacity, 1) * size, dtype=torch.uint8, device=device)
+        self.record_count = torch.zeros(1, dtype=torch.int64, device=device)
+        self.survivors = torch.zeros(max(self.survivor_capacity, 1) * size, dtype=torch.uint8, device=device)
+        self.survivor_count = torch.zeros(1, dtype=torch.int64, device=device)
+        self.tallies = torch.zeros(_lib.TALLY_COUNT, dtype=torch.float64, device=device)
+        self.error = torch.zeros(2, dtype=torch.int32, device=device)
+        self.steps = int(steps)
+        clear = list(clear)
+        if len(clear) > 4 or any(t.dtype != torch.float64 or not t.is_contiguous() for t in clear):
+            raise ValueError("at most four contiguous float64 buffers to clear")
+        d = _lib.BatchGraphDesc()
+        structs = []
+        for name, (tile, inst) in tiles.items():
+            k = Batch.SPECIES[name]
+            if inst.dtype != torch.int32 or inst.numel() != len(tile):
+                raise ValueError("inst must be int32, one per particle")
+            t = tile.struct()
+            structs.append(t)
+            d.tiles[k] = C.cast(C.pointer(t), C.c_void_p)
+            d.instance[k] = inst.data_ptr()
+        d.config = _step_config(cfg)
+        d.records, d.record_capacity, d.record_count = (self.records.data_ptr(), self.record_capacity,
+                                                        self.record_count.data_ptr())
+        d.survivors, d.survivor_capacity, d.survivor_count = (self.survivors.data_ptr(), self.survivor_capacity,
+                                                              self.survivor_count.data_ptr())
+        d.tallies, d.error = self.tallies.data_ptr(), self.error.data_ptr()
+        for k, t in enumerate(clear):
+            d.clear[k] = t.data_ptr()
+            d.clear_n[k] = t.numel()
+        d.max_generations, d.steps, d.push_photons = int(max_generations), self.steps, 1 if push_photons else 0
+        self._keep = (tiles, clear)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().rrea_batch_graph_create(tables.h, batch.h, C.byref(d), C.byref(h)),
+                   "rrea_batch_graph_create")
+        self.h = h
+
+    def launch(self, first_step: int, stream: int | None = None):
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().rrea_batch_graph_launch(self.h, int(first_step), stream), "rrea_batch_graph_launch")
+
+    def records_host(self) -> np.ndarray:
+        n = min(int(self.record_count.item()), self.record_capacity)
+        raw = self.records[: n * SECONDARY_DTYPE.itemsize].cpu().numpy()
+        return np.frombuffer(raw.tobytes(), dtype=SECONDARY_DTYPE, count=n)
+
+    def survivors_host(self) -> np.ndarray:
+        n = int(self.survivor_count.item())
+        raw = self.survivors[: n * SECONDARY_DTYPE.itemsize].cpu().numpy()
+        return np.frombuffer(raw.tobytes(), dtype=SECONDARY_DTYPE, count=n)
+
+    def check_error(self):
+        code, index = (int(v) for v in self.error.cpu().tolist())
+        if code:
+            raise _lib.RreaHipError(f"device transport error: {_lib.ERRORS.get(code, code)} at index {index}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().rrea_batch_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _step_config(cfg: StepConfig) -> _lib.StepConfigC:
     return _lib.StepConfigC(
         cfg.rng_seed, cfg.step, cfg.dt_s, cfg.low_energy_cutoff_eV, cfg.photon_cutoff_eV,

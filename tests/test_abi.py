@@ -30,6 +30,7 @@ STRUCTS = {
     "rrea_drain_io": _lib.DrainIO,
     "rrea_particle_store": _lib.ParticleStore,
     "rrea_drift_config": _lib.DriftConfig,
+    "rrea_batch_graph_desc": _lib.BatchGraphDesc,
 }
 
 
