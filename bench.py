@@ -122,10 +122,12 @@ def batched_leg(args, tables):
     N0 ~ log-U[1e3, 1e6] electrons at K ~ U[1, 10] MeV, 64 x 256 RZ grids of
     1 m cells), dt = 1 ps, each step the electron step of every instance then
     the newborn drain of its secondaries, J deposited per instance.  Timed
-    twice on the same inputs: the batched path (rrea_transport_step_batched /
-    rrea_newborn_drain_batched: one launch set for all instances) and the
-    plain per-instance path (the drop-in Apply run instance after instance,
-    one launch set and drain each).  Returns the report object."""
+    three times on the same inputs: the batched path
+    (rrea_transport_step_batched / rrea_newborn_drain_batched: one launch set
+    for all instances), the plain per-instance path (the drop-in Apply run
+    instance after instance, one launch set and drain each), and the batched
+    steps captured in one HIP graph (rrea_batch_graph_*).  Returns the report
+    object."""
     import ctypes as C
 
     import torch
@@ -133,8 +135,9 @@ def batched_leg(args, tables):
     from rrea_feedback_warpx_amrex_geant4_amd import _lib
     from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
     from rrea_feedback_warpx_amrex_geant4_amd.config import StepConfig
-    from rrea_feedback_warpx_amrex_geant4_amd.transport import (Batch, Coupling, DeviceTile, DrainOutputs,
-                                                                InstanceGrids, RreaParticleInteraction, StepOutputs)
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import (Batch, BatchGraph, Coupling, DeviceTile,
+                                                                DrainOutputs, InstanceGrids, RreaParticleInteraction,
+                                                                StepOutputs)
 
     keys = ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")
     n_inst = args.batched_instances
@@ -146,6 +149,7 @@ def batched_leg(args, tables):
     dev_b = DeviceTile.from_host(type(tiles[0])(*(np.concatenate([getattr(t, k) for t in tiles]) for k in keys)))
     del tiles
     dev_p = dev_b.clone()
+    dev_g = dev_b.clone()
     inst = torch.as_tensor(np.repeat(np.arange(n_inst, dtype=np.int32), sizes), device="cuda")
     couplings = [Coupling(f, tables) for f in fields]
     grids_b, grids_p = InstanceGrids(fields, sources=True), InstanceGrids(fields, sources=True)
@@ -214,8 +218,50 @@ def batched_leg(args, tables):
     for o in plain_outs:
         o.check_error()
     log(f"plain per instance: {args.batched_steps} steps in {el_p:.3f} s, {rate_p:.4e} particle-steps/s")
-    # the two paths must have advanced the same particles to the same state
+    # the graph-captured batched steps on a third copy: the warm-up steps and
+    # the timed steps each one graph (rrea_batch_graph_*), no host round trip
+    # inside a step
+    grids_g = InstanceGrids(fields, sources=True)
+    batch_g = Batch(couplings, grids_g.grids, seeds)
+    clear = (grids_g.jr, grids_g.jz, grids_g.rho, grids_g.sources)
+    gcfg = StepConfig(rng_seed=0, dt_s=dt)
+    t0 = time.perf_counter()
+    graphs = [BatchGraph(tables, batch_g, {"electron": (dev_g, inst)}, gcfg, record_capacity=drain_b.capacity,
+                         clear=clear, steps=k, max_generations=args.graph_generations)
+              for k in (args.batched_warmup, args.batched_steps)]
+    capture_s = time.perf_counter() - t0
+    graphs[0].launch(0)
+    torch.cuda.synchronize()
+    g0 = live(dev_g)
+    t0 = time.perf_counter()
+    graphs[1].launch(args.batched_warmup)
+    torch.cuda.synchronize()
+    el_g = time.perf_counter() - t0
+    g1 = live(dev_g)
+    for g in graphs:
+        g.check_error()
+    rate_g = 0.5 * (g0 + g1) * args.batched_steps / el_g
+    log(f"graph-captured batched: {args.batched_steps} steps in {el_g:.3f} s, {rate_g:.4e} particle-steps/s "
+        f"(capture + instantiate {capture_s:.2f} s)")
+    # the three paths must have advanced the same particles to the same state
     same = all(torch.equal(getattr(dev_b, k), getattr(dev_p, k)) for k in keys)
+    same_g = all(torch.equal(getattr(dev_b, k), getattr(dev_g, k)) for k in keys)
+    # the graph's loop continued to the configuration's step count, relaunched
+    # back to back
+    launches = max(1, args.graph_loop_steps // args.batched_steps)
+    t0 = time.perf_counter()
+    for i in range(1, launches):
+        graphs[1].launch(args.batched_warmup + i * args.batched_steps)
+    torch.cuda.synchronize()
+    el_loop = el_g + time.perf_counter() - t0
+    g2 = live(dev_g)
+    graphs[1].check_error()
+    loop_steps = launches * args.batched_steps
+    rate_loop = 0.5 * (g0 + g2) * loop_steps / el_loop
+    log(f"graph-captured loop: {loop_steps} steps in {el_loop:.3f} s, {rate_loop:.4e} particle-steps/s")
+    for g in graphs:
+        g.close()
+    batch_g.close()
     batch.close()
     return {
         "workload": f"{n_inst} independent instances, {n} electrons (N0 log-uniform 1e3..1e6 x {args.batched_scale}),"
@@ -226,9 +272,16 @@ def batched_leg(args, tables):
                     "electron_kernel_ms": mean_launch_s * 1e3, "hbm_frac": achieved / HBM_PEAK,
                     "achieved_GBps": achieved / 1e9},
         "plain_per_instance": {"value": rate_p, "ms_per_step": el_p / args.batched_steps * 1e3},
+        "graph_captured": {"value": rate_g, "ms_per_step": el_g / args.batched_steps * 1e3,
+                           "steps_per_graph": args.batched_steps, "max_generations": args.graph_generations,
+                           "capture_and_instantiate_s": capture_s,
+                           "loop": {"steps": loop_steps, "launches": launches, "value": rate_loop,
+                                    "ms_per_step": el_loop / loop_steps * 1e3}},
         "ratio": rate_b / rate_p,
+        "ratio_graph": rate_g / rate_p,
         "unit": "particle-steps/s",
         "states_identical": bool(same),
+        "graph_states_identical": bool(same_g),
     }
 
 
@@ -300,6 +353,10 @@ def main():
     ap.add_argument("--batched-scale", type=float, default=1.0, help="scale of the batched leg's populations")
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batched-warmup", type=int, default=2)
+    ap.add_argument("--graph-loop-steps", type=int, default=500,
+                    help="steps of the batched leg's graph-captured loop (the timed graph relaunched)")
+    ap.add_argument("--graph-generations", type=int, default=4,
+                    help="newborn FIFO generations captured per graph step of the batched leg")
     ap.add_argument("--drift-particles", type=int, default=1 << 20,
                     help="electrons of the drift-adjoint leg (0: skip it)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
