@@ -427,6 +427,51 @@ rrea_status rrea_batch_graph_create(rrea_tables_t tables, rrea_batch_t batch, co
 rrea_status rrea_batch_graph_launch(rrea_batch_graph_t graph, int64_t first_step, rrea_stream_t stream);
 rrea_status rrea_batch_graph_destroy(rrea_batch_graph_t graph);
 
+/* ------------------------------------------------------------------------
+ * The field side of the feedback loop: the step's net-chord deposit as
+ * physical face currents, and the axisymmetric TM Maxwell leapfrog those
+ * currents drive.  Layouts are the reference's (rrea_fields_update's): Er on
+ * radial faces (nr+1)*nz, Ez on axial faces nr*(nz+1), Btheta on corners
+ * (nr+1)*(nz+1); all pointers are device memory; geometry r_lo must be 0.
+ * Every field value is bit-identical to the reference's serial loops.
+ * ------------------------------------------------------------------------ */
+/* Replaces ConvertPaddedIndexCurrentToPhysicalRZ
+ * (native_amrex_rrea/include/rrea/RreaSegmentDepositionRZ.H:399-453), called
+ * from RreaAmrexAdvance::AssembleTotalFaceCurrent
+ * (native_amrex_rrea/src/RreaAmrexAdvance.cpp:860-863) with pad_cells 2 and
+ * charge_per_index_flux_c = qe (RreaWarpXCoupling.cpp:715-717): the padded
+ * index-space jr/jz of rrea_step_out to A/m^2 on the physical faces (the pad
+ * support past absorbing faces is dropped; the axis face carries none). */
+rrea_status rrea_current_to_physical(const rrea_rz_geometry* geometry, int32_t pad_cells,
+                                     double charge_per_index_flux_c, const double* jr_index, const double* jz_index,
+                                     double* jr, double* jz, rrea_stream_t stream);
+/* Replace MaxwellAdvanceB, MaxwellSilverMueller, MaxwellAdvanceE and
+ * MaxwellStep (native_amrex_rrea/include/rrea/RreaMaxwellTMRZ.H:147-230):
+ * the Faraday update of the interior corners, the first-order Silver-Mueller
+ * boundary corners (and the axis), the semi-implicit Ampere update (sigma_r/
+ * sigma_z conductivity, jr/jz current, amb_er/amb_ez the static ambient the
+ * scattered field rides on; each may be NULL), and the three in order. */
+rrea_status rrea_maxwell_advance_b(const rrea_rz_geometry* geometry, const double* er, const double* ez,
+                                   double* btheta, double dt, rrea_stream_t stream);
+rrea_status rrea_maxwell_silver_mueller(const rrea_rz_geometry* geometry, const double* er, const double* ez,
+                                        double* btheta, rrea_stream_t stream);
+rrea_status rrea_maxwell_advance_e(const rrea_rz_geometry* geometry, double* er, double* ez, const double* btheta,
+                                   double dt, const double* sigma_r, const double* sigma_z, const double* jr,
+                                   const double* jz, const double* amb_er, const double* amb_ez,
+                                   rrea_stream_t stream);
+rrea_status rrea_maxwell_step(const rrea_rz_geometry* geometry, double* er, double* ez, double* btheta, double dt,
+                              const double* sigma_r, const double* sigma_z, const double* jr, const double* jz,
+                              const double* amb_er, const double* amb_ez, rrea_stream_t stream);
+/* The energy ledger of RreaAmrexAdvance::AdvanceOneStep (:694-846) on the
+ * device: out[0] MaxwellFieldEnergy (:241-268; ambient-relative when amb_er
+ * and amb_ez are given), out[1] MaxwellJoulePower (:291-316; jr and jz
+ * given, else 0), out[2] MaxwellBoundaryPoyntingPower (:326-357), out[3] the
+ * Jz moment sum Jz dV.  `out` is device double[4], overwritten; the sums are
+ * workgroup-reduced, so they agree with the serial loops to round-off. */
+rrea_status rrea_maxwell_ledger(const rrea_rz_geometry* geometry, const double* er, const double* ez,
+                                const double* btheta, const double* amb_er, const double* amb_ez, const double* jr,
+                                const double* jz, double* out, rrea_stream_t stream);
+
 /* Device glue on either side of the hook (the WarpX calls the reference
  * makes around Apply, done here without leaving HBM). */
 

@@ -3,16 +3,20 @@
 // Test infrastructure only: used to pin the oracle's restatement of the
 // counter RNG (OV/RreaRng.cpp), the uniform stream
 // (NA/include/rrea/RreaUniformStream.H), the two-point Gauss-Legendre rule
-// (NA/include/rrea/RreaGaussLegendre8.H) and the RZ segment deposition
-// (NA/include/rrea/RreaSegmentDepositionRZ.H).
+// (NA/include/rrea/RreaGaussLegendre8.H), the RZ segment deposition and its
+// conversion to physical face currents (NA/include/rrea/
+// RreaSegmentDepositionRZ.H) and the TM Maxwell leapfrog
+// (NA/include/rrea/RreaMaxwellTMRZ.H).
 #include "RreaRng.H"
 #include "rrea/RreaGaussLegendre8.H"
+#include "rrea/RreaMaxwellTMRZ.H"
 #include "rrea/RreaSegmentDepositionRZ.H"
 #include "rrea/RreaUniformStream.H"
 
 #include <array>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 extern "C" {
 
@@ -77,6 +81,92 @@ int ref_deposit_escaping_segment(int nr, int nz, double dr, double dz, double r_
                                                     pnz, faces);
     copy_back(g, rho, jr, jz);
     return res.valid ? 1 : 0;
+}
+
+void ref_current_to_physical(int nr, int nz, double dr, double dz, int pad, double q, const double* jr_index,
+                             const double* jz_index, double* jr, double* jz)
+{
+    std::vector<double> const a(jr_index, jr_index + static_cast<size_t>(nr + pad + 1) * (nz + 2 * pad));
+    std::vector<double> const b(jz_index, jz_index + static_cast<size_t>(nr + pad) * (nz + 2 * pad + 1));
+    std::vector<double> out_r, out_z;
+    rrea::ConvertPaddedIndexCurrentToPhysicalRZ(nr, nz, dr, dz, pad, q, a, b, out_r, out_z);
+    std::memcpy(jr, out_r.data(), out_r.size() * sizeof(double));
+    std::memcpy(jz, out_z.data(), out_z.size() * sizeof(double));
+}
+
+static rrea::MaxwellTMRZ make_maxwell(int nr, int nz, double dr, double dz, const double* er, const double* ez,
+                                      const double* bt)
+{
+    rrea::MaxwellTMRZ f;
+    f.nr = nr; f.nz = nz; f.dr = dr; f.dz = dz;
+    f.resize();
+    std::memcpy(f.er.data(), er, f.er.size() * sizeof(double));
+    std::memcpy(f.ez.data(), ez, f.ez.size() * sizeof(double));
+    std::memcpy(f.btheta.data(), bt, f.btheta.size() * sizeof(double));
+    return f;
+}
+
+static const std::vector<double>* opt(std::vector<double>& v, const double* p, size_t n)
+{
+    if (!p) return nullptr;
+    v.assign(p, p + n);
+    return &v;
+}
+
+// op: 0 MaxwellStep, 1 MaxwellAdvanceB, 2 MaxwellSilverMueller, 3 MaxwellAdvanceE
+void ref_maxwell(int op, int nr, int nz, double dr, double dz, double* er, double* ez, double* bt, double dt,
+                 const double* sr, const double* sz, const double* jr, const double* jz, const double* aer,
+                 const double* aez)
+{
+    auto f = make_maxwell(nr, nz, dr, dz, er, ez, bt);
+    size_t const ner = f.er.size(), nez = f.ez.size();
+    std::vector<double> vsr, vsz, vjr, vjz, vaer, vaez;
+    auto const* psr = opt(vsr, sr, ner);
+    auto const* psz = opt(vsz, sz, nez);
+    auto const* pjr = opt(vjr, jr, ner);
+    auto const* pjz = opt(vjz, jz, nez);
+    auto const* paer = opt(vaer, aer, ner);
+    auto const* paez = opt(vaez, aez, nez);
+    if (op == 0) rrea::MaxwellStep(f, dt, psr, psz, pjr, pjz, paer, paez);
+    else if (op == 1) rrea::MaxwellAdvanceB(f, dt);
+    else if (op == 2) rrea::MaxwellSilverMueller(f);
+    else rrea::MaxwellAdvanceE(f, dt, psr, psz, pjr, pjz, paer, paez);
+    std::memcpy(er, f.er.data(), ner * sizeof(double));
+    std::memcpy(ez, f.ez.data(), nez * sizeof(double));
+    std::memcpy(bt, f.btheta.data(), f.btheta.size() * sizeof(double));
+}
+
+// out: field energy (ambient-relative when aer/aez given), Joule power (jr/jz
+// given), boundary Poynting power, and the Jz moment
+void ref_maxwell_diagnostics(int nr, int nz, double dr, double dz, const double* er, const double* ez,
+                             const double* bt, const double* aer, const double* aez, const double* jr,
+                             const double* jz, double* out)
+{
+    auto f = make_maxwell(nr, nz, dr, dz, er, ez, bt);
+    rrea::MaxwellTMRZ amb = f;
+    bool const has_amb = aer && aez;
+    if (has_amb) {
+        std::memcpy(amb.er.data(), aer, amb.er.size() * sizeof(double));
+        std::memcpy(amb.ez.data(), aez, amb.ez.size() * sizeof(double));
+        std::fill(amb.btheta.begin(), amb.btheta.end(), 0.0);
+    }
+    out[0] = rrea::MaxwellFieldEnergy(f, has_amb ? &amb : nullptr);
+    out[1] = 0.0;
+    out[3] = 0.0;
+    if (jr && jz) {
+        std::vector<double> const vjr(jr, jr + f.er.size()), vjz(jz, jz + f.ez.size());
+        out[1] = rrea::MaxwellJoulePower(f, has_amb ? &amb : nullptr, vjr, vjz, &out[3]);
+    }
+    out[2] = rrea::MaxwellBoundaryPoyntingPower(f, has_amb ? &amb : nullptr);
+}
+
+double ref_maxwell_mixed_magnetic_energy(int nr, int nz, double dr, double dz, const double* bt,
+                                         const double* bt_previous)
+{
+    std::vector<double> zr(static_cast<size_t>(nr + 1) * nz, 0.0), zz(static_cast<size_t>(nr) * (nz + 1), 0.0);
+    auto f = make_maxwell(nr, nz, dr, dz, zr.data(), zz.data(), bt);
+    std::vector<double> const prev(bt_previous, bt_previous + f.btheta.size());
+    return rrea::MaxwellMixedMagneticEnergy(f, prev);
 }
 
 }  // extern "C"

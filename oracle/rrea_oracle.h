@@ -135,6 +135,31 @@ int orc_accumulate_source(const orc_geometry* g, double cutoff, int kind, double
 /* RreaEventAccumulator::NormalizeAndSync on g->sources. */
 int orc_sources_normalize(const orc_geometry* g, double dt, int diagnostics);
 
+/* The field side of the feedback loop (rrea_maxwell_oracle.cpp):
+ * ConvertPaddedIndexCurrentToPhysicalRZ and the TM Maxwell leapfrog of
+ * NA/include/rrea/RreaMaxwellTMRZ.H, in the reference's layouts.  Optional
+ * arrays may be NULL. */
+void orc_current_to_physical(int nr, int nz, double dr, double dz, int pad, double charge_per_index_flux,
+                             const double* jr_index, const double* jz_index, double* jr, double* jz);
+void orc_maxwell_advance_b(int nr, int nz, double dr, double dz, const double* er, const double* ez, double* bt,
+                           double dt);
+void orc_maxwell_silver_mueller(int nr, int nz, const double* er, const double* ez, double* bt);
+void orc_maxwell_advance_e(int nr, int nz, double dr, double dz, double* er, double* ez, const double* bt, double dt,
+                           const double* sigma_r, const double* sigma_z, const double* jr, const double* jz,
+                           const double* amb_er, const double* amb_ez);
+void orc_maxwell_step(int nr, int nz, double dr, double dz, double* er, double* ez, double* bt, double dt,
+                      const double* sigma_r, const double* sigma_z, const double* jr, const double* jz,
+                      const double* amb_er, const double* amb_ez);
+double orc_maxwell_field_energy(int nr, int nz, double dr, double dz, const double* er, const double* ez,
+                                const double* bt, const double* amb_er, const double* amb_ez);
+double orc_maxwell_mixed_magnetic_energy(int nr, int nz, double dr, double dz, const double* bt,
+                                         const double* bt_previous);
+double orc_maxwell_joule_power(int nr, int nz, double dr, double dz, const double* er, const double* ez,
+                               const double* amb_er, const double* amb_ez, const double* jr, const double* jz,
+                               double* jz_moment);
+double orc_maxwell_poynting_power(int nr, int nz, double dr, double dz, const double* er, const double* ez,
+                                  const double* bt, const double* amb_er, const double* amb_ez);
+
 #ifdef __cplusplus
 }
 #endif

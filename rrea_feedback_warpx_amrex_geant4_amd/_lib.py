@@ -188,6 +188,16 @@ SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.POINTER(BatchGraphDesc), C.POINTER(C.c_void_p)]),
     ("rrea_batch_graph_launch", C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p]),
     ("rrea_batch_graph_destroy", C.c_int32, [C.c_void_p]),
+    ("rrea_current_to_physical", C.c_int32,
+     [C.POINTER(RzGeometry), C.c_int32, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rrea_maxwell_advance_b", C.c_int32,
+     [C.POINTER(RzGeometry), C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
+    ("rrea_maxwell_silver_mueller", C.c_int32, [C.POINTER(RzGeometry)] + [C.c_void_p] * 4),
+    ("rrea_maxwell_advance_e", C.c_int32,
+     [C.POINTER(RzGeometry), C.c_void_p, C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 7),
+    ("rrea_maxwell_step", C.c_int32,
+     [C.POINTER(RzGeometry), C.c_void_p, C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 7),
+    ("rrea_maxwell_ledger", C.c_int32, [C.POINTER(RzGeometry)] + [C.c_void_p] * 9),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),
