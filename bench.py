@@ -324,6 +324,56 @@ def drift_leg(args, tables):
             "checkpoint_bytes": int(out["ck"].numel() * 8 + out["ck_alive"].numel())}
 
 
+def field_leg(args, tables):
+    """The field side of the feedback loop at the plain configuration's grid
+    (256 x 1024 RZ, dr = 10 m, dz = 9.77 m): per step the net-chord deposit
+    (a synthetic padded index-space grid, half its faces non-zero) converted
+    to physical face currents, one TM Maxwell leapfrog step driven by them,
+    and the total field handed back to the transport's gather
+    (rrea_current_to_physical, rrea_maxwell_step, rrea_fields_update).  HIP
+    events around the timed steps on the current stream."""
+    import torch
+
+    from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
+    from rrea_feedback_warpx_amrex_geant4_amd.maxwell import MaxwellField
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import Coupling
+
+    fields = wl.c2_fields()
+    nrp, nzp = fields.padded_shape()
+    g = torch.Generator(device="cpu").manual_seed(5)
+    jr_i = torch.randn((nrp + 1) * nzp, generator=g, dtype=torch.float64)
+    jz_i = torch.randn(nrp * (nzp + 1), generator=g, dtype=torch.float64)
+    jr_i[torch.rand(jr_i.numel(), generator=g) < 0.5] = 0.0
+    jz_i[torch.rand(jz_i.numel(), generator=g) < 0.5] = 0.0
+    jr_i, jz_i = jr_i.cuda(), jz_i.cuda()
+    cp = Coupling(fields, tables)
+    m = MaxwellField(fields)
+    for _ in range(5):
+        m.feedback_step(jr_i, jz_i, 1e-12, coupling=cp)
+    steps = args.field_steps
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        m.feedback_step(jr_i, jz_i, 1e-12, coupling=cp)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    nr, nz = fields.nr, fields.nz
+    n_er, n_ez, n_bt, cells = (nr + 1) * nz, nr * (nz + 1), (nr + 1) * (nz + 1), nr * nz
+    # algorithmic bytes: read both index grids, write jr/jz; Faraday + absorber
+    # read Er, Ez and read/write Btheta; Ampere reads Btheta, jr, jz and
+    # reads/writes Er, Ez; the gather's cell totals read Er, Ez, both ambients,
+    # write two cell arrays, and copy Btheta
+    alg = 8 * ((nrp + 1) * nzp + nrp * (nzp + 1) + n_er + n_ez
+               + n_er + n_ez + 2 * n_bt
+               + n_bt + n_er + n_ez + 2 * (n_er + n_ez)
+               + 2 * (n_er + n_ez) + 2 * cells + 2 * n_bt)
+    log(f"field side: {ms * 1e3:.1f} us per feedback step on {nr}x{nz} ({alg / (ms * 1e-3) / 1e9:.0f} GB/s algorithmic)")
+    return {"workload": f"{nr}x{nz} RZ grid: deposit -> physical current -> TM Maxwell step -> gather field totals",
+            "us_per_step": ms * 1e3, "alg_bytes_per_step": alg, "achieved_GBps": alg / (ms * 1e-3) / 1e9,
+            "steps": steps}
+
+
 def traffic_record():
     """Measured HBM bytes per launch of the electron kernel, or None."""
     try:
@@ -353,6 +403,7 @@ def main():
     ap.add_argument("--batched-scale", type=float, default=1.0, help="scale of the batched leg's populations")
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batched-warmup", type=int, default=2)
+    ap.add_argument("--field-steps", type=int, default=200, help="timed feedback steps of the field leg (0: skip)")
     ap.add_argument("--graph-loop-steps", type=int, default=500,
                     help="steps of the batched leg's graph-captured loop (the timed graph relaunched)")
     ap.add_argument("--graph-generations", type=int, default=4,
@@ -517,6 +568,7 @@ def main():
 
     batched = batched_leg(args, tables) if args.batched_instances > 0 else None
     drift = drift_leg(args, tables) if args.drift_particles > 0 else None
+    field = field_leg(args, tables) if args.field_steps > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -555,6 +607,7 @@ def main():
                      "mean_launch_ms": mean_launch_s * 1e3, "isolated": iso},
         "cpu_baseline": cpu,
         "batched_vs_plain": batched,
+        "field_side": field,
         "drift_adjoint": drift,
     }
     print(json.dumps(line), flush=True)
