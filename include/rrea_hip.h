@@ -472,6 +472,54 @@ rrea_status rrea_maxwell_ledger(const rrea_rz_geometry* geometry, const double* 
                                 const double* btheta, const double* amb_er, const double* amb_ez, const double* jr,
                                 const double* jz, double* out, rrea_stream_t stream);
 
+/* ------------------------------------------------------------------------
+ * Population control: the charge-exact thinning of the overlay's
+ * adaptive_resample_v1 controller for one charged species on one rank.
+ * Replaces the thinning plan of
+ * RreaWarpXCoupling::PreflightAndApplySecondaryPopulationControl
+ * (warpx_overlay/Source/Rrea/RreaCouplingPopulation.cpp:175-520, ranks = 1)
+ * over RreaPlanGlobalCicThinning / RreaThinCicWeights
+ * (native_amrex_rrea/include/rrea/RreaCicConservingResample.H:236-617).
+ * Each record's order-1 cell-centred CIC support and phase bin are formed
+ * as the coupling's record lambda does (:233-277); records are grouped by
+ * support in (support, phase, candidate, id) order; thinning engages above
+ * 1.25 x target and removes min(count - target, exact capacity) records in
+ * support order, each support's weights moved along null directions of its
+ * four CIC columns with the controller's counter draws
+ * (RreaAdaptiveResampleCicUniform, RreaRng.H:112-129), so every support's
+ * four deposited charges are preserved.  Weights are rewritten in place;
+ * with `id` given, a record whose weight reached zero gets RREA_INVALID_ID.
+ * All pointers are device memory; every record's weight must be positive.
+ * Synchronises `stream` (a host-planned pass, not part of the step).
+ * ------------------------------------------------------------------------ */
+typedef struct rrea_population_records {
+    int64_t n;
+    const double* r; const double* z; const double* theta;   /* position (z_lo-relative domain) */
+    const double* ux; const double* uy; const double* uz;    /* u = gamma v [m/s] */
+    double* w;                       /* in/out: macro weights */
+    const uint64_t* stable_id;       /* the record's stable id (low word) */
+    const uint64_t* stable_id_high;  /* may be NULL: 0 (live particles) */
+    const uint8_t* candidate;        /* may be NULL: 0 (live particles; 1 = a staged secondary) */
+    int64_t* id;                     /* may be NULL: invalidated where the weight reaches zero */
+} rrea_population_records;
+
+typedef struct rrea_population_config {
+    double z_lo, dr, dz;             /* the mesh (r_lo = 0, non-periodic) */
+    int64_t target;                  /* the species' target macro count */
+    uint64_t rng_seed;               /* rrea.rng_seed */
+    uint64_t rng_salt;               /* population_controller_rng_salt */
+    int64_t step;
+    int32_t species;                 /* 0 electrons, 1 positrons (the draw channel and support stream) */
+    int32_t reserved;
+} rrea_population_config;
+
+/* counts (host int64[4]): exact removal capacity, geometric floor, the
+ * nominal request, records removed.  Device errors (an invalid record or
+ * support plan, insufficient progress) go to `error` (device int32[2]) and
+ * leave every weight unchanged, as the reference aborts before applying. */
+rrea_status rrea_population_thin(const rrea_population_records* records, const rrea_population_config* config,
+                                 int64_t* counts, int32_t* error, rrea_stream_t stream);
+
 /* Device glue on either side of the hook (the WarpX calls the reference
  * makes around Apply, done here without leaving HBM). */
 

@@ -8,6 +8,7 @@
 // RreaSegmentDepositionRZ.H) and the TM Maxwell leapfrog
 // (NA/include/rrea/RreaMaxwellTMRZ.H).
 #include "RreaRng.H"
+#include "rrea/RreaCicConservingResample.H"
 #include "rrea/RreaGaussLegendre8.H"
 #include "rrea/RreaMaxwellTMRZ.H"
 #include "rrea/RreaSegmentDepositionRZ.H"
@@ -158,6 +159,98 @@ void ref_maxwell_diagnostics(int nr, int nz, double dr, double dz, const double*
         out[1] = rrea::MaxwellJoulePower(f, has_amb ? &amb : nullptr, vjr, vjz, &out[3]);
     }
     out[2] = rrea::MaxwellBoundaryPoyntingPower(f, has_amb ? &amb : nullptr);
+}
+
+// The single-rank flow of the adaptive_resample_v1 thinning plan
+// (RreaCouplingPopulation.cpp:321-520 with ranks = 1) over records whose
+// support, phase and fractions the caller formed: groups by support in
+// (support_r, support_z, phase, candidate, id) order, exact removal capacity
+// per support, the greedy request per support in support order, and
+// RreaPlanGlobalCicThinning per requested support with the controller's
+// counter draw.  out_w receives every record's new weight; counts: global
+// capacity, geometric floor, nominal request, removed.  Returns 0, or the
+// failing step (1 invalid plan, 2 insufficient progress).
+int ref_population_thin(int64_t n, const int32_t* sr, const int32_t* sz, const int32_t* phase, const double* fr,
+                        const double* fz, const double* w, const uint8_t* candidate, const uint64_t* high,
+                        const uint64_t* low, int64_t target, uint64_t seed, uint64_t salt, uint64_t step,
+                        int species, double* out_w, int64_t* counts)
+{
+    std::vector<rrea::RreaGlobalCicParticle> owned(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) {
+        auto& p = owned[static_cast<size_t>(i)];
+        p.support_r = sr[i]; p.support_z = sz[i]; p.phase = phase[i];
+        p.source_rank = 0;
+        p.stable_id = low[i];
+        p.stable_id_high = high ? high[i] : 0;
+        p.candidate = candidate ? candidate[i] != 0 : false;
+        p.fraction_r = fr[i]; p.fraction_z = fz[i]; p.weight = w[i];
+        out_w[i] = w[i];
+    }
+    std::vector<size_t> order(owned.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    auto less = [&](size_t a, size_t b) {
+        auto const& l = owned[a];
+        auto const& r = owned[b];
+        if (l.support_r != r.support_r) return l.support_r < r.support_r;
+        if (l.support_z != r.support_z) return l.support_z < r.support_z;
+        if (l.phase != r.phase) return l.phase < r.phase;
+        if (l.candidate != r.candidate) return l.candidate < r.candidate;
+        if (l.stable_id_high != r.stable_id_high) return l.stable_id_high < r.stable_id_high;
+        return l.stable_id < r.stable_id;
+    };
+    std::stable_sort(order.begin(), order.end(), less);
+    uint64_t const stream_key = species == 0 ? 0x656c656374726f6eULL : 0x706f736974726f6eULL;
+    struct G { size_t begin, end, capacity, floor; };
+    std::vector<G> groups;
+    int64_t capacity = 0, floor = 0;
+    for (size_t b = 0; b < order.size();) {
+        size_t e = b + 1;
+        while (e < order.size() && owned[order[e]].support_r == owned[order[b]].support_r
+               && owned[order[e]].support_z == owned[order[b]].support_z) ++e;
+        std::vector<rrea::RreaCicResampleParticle> support;
+        for (size_t i = b; i < e; ++i) {
+            auto const& p = owned[order[i]];
+            support.push_back({p.fraction_r, p.fraction_z, p.weight});
+        }
+        size_t const cap = rrea::RreaCicResampleRemovalCapacity(support);
+        groups.push_back(G{b, e, cap, support.size() - cap});
+        capacity += static_cast<int64_t>(cap);
+        floor += static_cast<int64_t>(support.size() - cap);
+        b = e;
+    }
+    bool const due = n > target + target / 4;
+    int64_t const nominal = due ? std::min<int64_t>(n - target, capacity) : 0;
+    int64_t remaining = nominal, removed = 0;
+    rrea::warpx::RreaAdaptiveResampleSpecies const sp =
+        species == 0 ? rrea::warpx::RreaAdaptiveResampleSpecies::Electron
+                     : rrea::warpx::RreaAdaptiveResampleSpecies::Positron;
+    int status = 0;
+    for (auto const& g : groups) {
+        int64_t const request = std::min<int64_t>(remaining, static_cast<int64_t>(g.capacity));
+        remaining -= request;
+        if (request == 0) continue;
+        std::vector<rrea::RreaGlobalCicParticle> support;
+        for (size_t i = g.begin; i < g.end; ++i) support.push_back(owned[order[i]]);
+        auto plan = rrea::RreaPlanGlobalCicThinning<double>(
+            support, support.size() - static_cast<size_t>(request), 1, stream_key,
+            [&](std::uint64_t group_id, std::size_t reduction) {
+                return rrea::warpx::RreaAdaptiveResampleCicUniform(seed, group_id ^ salt, step, reduction, sp);
+            });
+        if (!plan.valid() || plan.removed == 0) { status = 1; break; }
+        removed += static_cast<int64_t>(plan.removed);
+        for (auto const& u : plan.updates) {
+            for (size_t i = g.begin; i < g.end; ++i) {
+                auto const& p = owned[order[i]];
+                if (p.stable_id == u.stable_id && p.stable_id_high == u.stable_id_high && p.candidate == u.candidate) {
+                    out_w[order[i]] = u.stored_weight;
+                    break;
+                }
+            }
+        }
+    }
+    if (status == 0 && nominal > 0 && removed < nominal) status = 2;
+    counts[0] = capacity; counts[1] = floor; counts[2] = nominal; counts[3] = removed;
+    return status;
 }
 
 double ref_maxwell_mixed_magnetic_energy(int nr, int nz, double dr, double dz, const double* bt,

@@ -137,6 +137,17 @@ class BatchGraphDesc(C.Structure):
     ]
 
 
+class PopulationRecords(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in (
+        "r", "z", "theta", "ux", "uy", "uz", "w", "stable_id", "stable_id_high", "candidate", "id")]
+
+
+class PopulationConfig(C.Structure):
+    _fields_ = [("z_lo", C.c_double), ("dr", C.c_double), ("dz", C.c_double), ("target", C.c_int64),
+                ("rng_seed", C.c_uint64), ("rng_salt", C.c_uint64), ("step", C.c_int64), ("species", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 TALLY_COUNT = 24
 SOURCE_FIELDS = 5
 STATUS = {0: "survive", 1: "escape", 2: "demoted", 3: "zero_weight", 4: "no_writeback", 5: "skipped_invalid",
@@ -198,6 +209,8 @@ SIGNATURES = [
     ("rrea_maxwell_step", C.c_int32,
      [C.POINTER(RzGeometry), C.c_void_p, C.c_void_p, C.c_void_p, C.c_double] + [C.c_void_p] * 7),
     ("rrea_maxwell_ledger", C.c_int32, [C.POINTER(RzGeometry)] + [C.c_void_p] * 9),
+    ("rrea_population_thin", C.c_int32,
+     [C.POINTER(PopulationRecords), C.POINTER(PopulationConfig), P_I64, C.c_void_p, C.c_void_p]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),

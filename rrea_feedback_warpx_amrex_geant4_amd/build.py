@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librrea_hip.so")
-SOURCES = [os.path.join(HERE, "csrc", "rrea_transport.hip"), os.path.join(HERE, "csrc", "rrea_maxwell.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("rrea_transport.hip", "rrea_maxwell.hip", "rrea_population.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

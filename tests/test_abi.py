@@ -31,6 +31,8 @@ STRUCTS = {
     "rrea_particle_store": _lib.ParticleStore,
     "rrea_drift_config": _lib.DriftConfig,
     "rrea_batch_graph_desc": _lib.BatchGraphDesc,
+    "rrea_population_records": _lib.PopulationRecords,
+    "rrea_population_config": _lib.PopulationConfig,
 }
 
 
