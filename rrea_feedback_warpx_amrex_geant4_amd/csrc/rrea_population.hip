@@ -674,7 +674,8 @@ rrea_status rrea_population_thin(const rrea_population_records* rec, const rrea_
                cfg->species == 0 ? 0x656c656374726f6eULL : 0x706f736974726f6eULL, removed, error};
     // pageable source: the copy has consumed A when hipMemcpyAsync returns
     HIP_TRY(hipMemcpyAsync(d_args, &A, sizeof(PlanArgs), hipMemcpyHostToDevice, s));
-    thin_kernel<<<dim3(blocks(n_groups)), dim3(64), 0, s>>>(d_args);
+    // one lane per support, 64-lane workgroups
+    thin_kernel<<<dim3(static_cast<unsigned>((n_groups + 63) / 64)), dim3(64), 0, s>>>(d_args);
     HIP_TRY(hipGetLastError());
     unsigned long long removed_host = 0;
     HIP_TRY(hipMemcpyAsync(&removed_host, removed, sizeof(removed_host), hipMemcpyDeviceToHost, s));

@@ -92,6 +92,11 @@ def cases():
                           (np.arange(n_cand, dtype=np.uint64) << np.uint64(2)) | np.uint64(0)])
     out.append(dict(name="live_and_candidates", species=0, target=200, p=p, w=rng.uniform(0.2, 3.0, n_live + n_cand),
                     low=low, high=high, cand=cand))
+    # many supports (more than one 64-lane workgroup of support plans)
+    n = 3000
+    p = particles(rng, n, 20, 10, dr, dz, z_lo)
+    out.append(dict(name="many_supports", species=0, target=1500, p=p, w=rng.uniform(0.5, 2.0, n),
+                    low=np.arange(1, n + 1, dtype=np.uint64), high=None, cand=None))
     return out, (z_lo, dr, dz)
 
 

@@ -76,6 +76,37 @@ def test_thinning_matches_reference_plan(gpu, case):
           f"max weight difference {np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)):.2e} relative", flush=True)
 
 
+def test_thinning_at_scale_conserves_and_meets_the_request(gpu):
+    """2^20 electrons over the 256 x 1024 grid (~260,000 supports), thinned
+    to half: the nominal request is met, the removed count is the zeroed
+    count, and every support's CIC charges are unchanged (size-independent
+    properties; the reference's own plan completes this input too)."""
+    from golden.make_resample_fixtures import records
+    from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
+    from rrea_feedback_warpx_amrex_geant4_amd.population import thin_tile
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import DeviceTile
+    fields = wl.c2_fields()
+    n = 1 << 20
+    print(f"case population at scale: {n} electrons", flush=True)
+    host = wl.c2_electrons(n, fields, seed=31)
+    dev = DeviceTile.from_host(host)
+    res = thin_tile(dev, fields, target=n // 2, rng_seed=20240623, step=0)
+    w = dev.w.cpu().numpy()
+    assert res.requested > 0 and res.removed >= res.requested
+    assert int((w == 0.0).sum()) == res.removed
+    sr, sz, _, fr, fz = records(host.r, host.z, host.theta, host.ux, host.uy, host.uz, fields.z_lo, fields.dr,
+                                fields.dz)
+    key = (sr.astype(np.int64) + 1) * (fields.nz + 2) + (sz + 1)
+    for wt, name in ((host.w, "before"), (w, "after")):
+        cells = np.stack([np.bincount(key, wt * (1 - fr) * (1 - fz)), np.bincount(key, wt * fr * (1 - fz)),
+                          np.bincount(key, wt * (1 - fr) * fz), np.bincount(key, wt * fr * fz)])
+        if name == "before":
+            c0 = cells
+    assert np.allclose(cells, c0, rtol=1e-12, atol=1e-12)
+    print(f"population at scale: removed {res.removed} (requested {res.requested}, capacity {res.capacity})",
+          flush=True)
+
+
 def test_thin_tile_invalidates_and_conserves(gpu):
     """A live tile thinned in place: zero-weight particles invalidated, the
     survivors' CIC charges per support unchanged, invalid particles untouched."""
