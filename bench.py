@@ -406,6 +406,39 @@ def population_leg(args, tables):
             "records_per_s": n / (ms * 1e-3)}
 
 
+def column_leg(args, tables):
+    """The feedback column (one instance of the feedback configuration):
+    every step injects electrons at the column top (power law E^-1.5 over
+    1-100 MeV, r < 500 m, 1e5 electrons/ns), runs the photon push, Apply,
+    the drain, the commit and compaction, converts the deposit to current,
+    advances the TM Maxwell field and hands it back to the gather; every 10
+    steps population control thins electrons and positrons above 1.25 x
+    their targets.  dt = 1 ns on the 256 x 1024 column grid.  Times the whole
+    loop; particle-steps counted are the live particles at each step."""
+    import torch
+
+    from rrea_feedback_warpx_amrex_geant4_amd.column import FeedbackColumn
+
+    steps = args.column_steps
+    col = FeedbackColumn(tables, dt_s=1e-9, inject_macros=args.column_inject, targets=(1 << 18, 1 << 15),
+                         control_every=10, capacity=1 << 22)
+    col.run(10)  # warm-up: the first injections and control pass
+    torch.cuda.synchronize()
+    live = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        col.step()
+        live += sum(s.n for s in col.sim.stores)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    obs = col.observe()
+    log(f"feedback column: {steps} steps in {el:.2f} s ({el / steps * 1e3:.1f} ms/step), "
+        f"{live / el:.3e} particle-steps/s; {obs}")
+    return {"workload": "feedback column, 256x1024 grid, dt = 1 ns, injection + transport + drain + commit + "
+                        "Maxwell feedback + population control", "steps": steps, "ms_per_step": el / steps * 1e3,
+            "particle_steps_per_s": live / el, "final": obs}
+
+
 def traffic_record():
     """Measured HBM bytes per launch of the electron kernel, or None."""
     try:
@@ -436,6 +469,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batched-warmup", type=int, default=2)
     ap.add_argument("--field-steps", type=int, default=200, help="timed feedback steps of the field leg (0: skip)")
+    ap.add_argument("--column-steps", type=int, default=100, help="timed steps of the feedback-column leg (0: skip)")
+    ap.add_argument("--column-inject", type=int, default=2048, help="macroparticles injected per column step")
     ap.add_argument("--population-particles", type=int, default=1 << 22,
                     help="electrons of the population-control leg (0: skip)")
     ap.add_argument("--graph-loop-steps", type=int, default=500,
@@ -604,6 +639,7 @@ def main():
     drift = drift_leg(args, tables) if args.drift_particles > 0 else None
     field = field_leg(args, tables) if args.field_steps > 0 else None
     population = population_leg(args, tables) if args.population_particles > 0 else None
+    column = column_leg(args, tables) if args.column_steps > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -644,6 +680,7 @@ def main():
         "batched_vs_plain": batched,
         "field_side": field,
         "population_control": population,
+        "feedback_column": column,
         "drift_adjoint": drift,
     }
     print(json.dumps(line), flush=True)

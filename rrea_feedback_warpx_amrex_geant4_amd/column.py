@@ -1,0 +1,145 @@
+"""The feedback column: one avalanche instance with every part of the
+coupling's step on the device, in the reference's order.
+
+Per step (RreaWarpXCoupling::AfterParticlePushBeforeBoundaries,
+warpx_overlay/Source/Rrea/RreaWarpXCoupling.cpp:210-252, and
+RreaAmrexAdvance::AdvanceOneStep for the kinetic current):
+
+1. injection at the column top: ``rate_per_s * dt`` physical electrons as
+   ``inject_macros`` macroparticles, kinetic energies from a power law
+   E^-index over [e_min, e_max], uniform over r < ``inject_radius_m``, in a
+   cone about the force (the seed-injection of the production decks; the
+   overlay's own injector is WarpX input, not part of the hook);
+2. ``Simulation.step``: the photon push, ``Apply`` (electrons, photons,
+   positrons), the newborn drain, the commit of its survivors, compaction;
+3. the field side: the step's net-chord deposit as physical current, one TM
+   Maxwell step, the total field back to the gather (``MaxwellField``);
+4. every ``control_every`` steps, population control of the charged species
+   above 1.25 x their target (``population.thin_tile``), as the controller's
+   interval does (RreaCouplingPopulation.cpp:649-684).
+
+The atmosphere is exponential (rho = 1.225 exp(-h / 8.4 km), h = 4 km + z
+over a 10 km column); the ambient E_z ramps to ``peak_v_per_m`` and is
+signed so the force on electrons points down the column (-z), from the
+injection plane toward the ground.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import population
+from .config import ParticleTile, RzFields, StepConfig
+from .maxwell import MaxwellField
+from .simulation import ELECTRON, ME_C2_EV, POSITRON, Simulation
+from .transport import InteractionTables
+
+C_LIGHT = 299792458.0
+
+
+def column_fields(peak_v_per_m: float = 4.5e5, nr: int = 256, nz: int = 1024, r_max_m: float = 2560.0,
+                  z_max_m: float = 10000.0) -> RzFields:
+    """The column's mesh, density profile and ambient field: E_z > 0 (force
+    on electrons -z), ramping over the top and bottom 2 km."""
+    dr, dz = r_max_m / nr, z_max_m / nz
+    zf = np.arange(nz + 1) * dz
+    prof = peak_v_per_m * np.minimum(np.clip(zf / 2000.0, 0, 1), np.clip((z_max_m - zf) / 2000.0, 0, 1))
+    ez = np.repeat(prof[:, None], nr, axis=1).reshape(-1)
+    alt = np.linspace(4000.0, 4000.0 + z_max_m, 1001)
+    rho = 1.225 * np.exp(-alt / 8400.0)
+    return RzFields(nr, nz, dr, dz, 0.0, np.zeros((nr + 1) * nz), np.zeros(nr * (nz + 1)),
+                    np.zeros((nr + 1) * nz), ez, np.zeros((nr + 1) * (nz + 1)), density_ratio=1.0,
+                    profile_alt_m=alt, profile_rho=rho, altitude_at_z0_m=4000.0)
+
+
+def power_law_energies(rng, n: int, index: float, e_min: float, e_max: float) -> np.ndarray:
+    """Inverse-CDF samples of E^-index on [e_min, e_max] (index != 1)."""
+    a = 1.0 - index
+    u = rng.random(n)
+    return (e_min ** a + u * (e_max ** a - e_min ** a)) ** (1.0 / a)
+
+
+class FeedbackColumn:
+    def __init__(self, tables: InteractionTables, fields: RzFields | None = None, dt_s: float = 1.0e-9,
+                 rate_per_s: float = 1.0e14, inject_macros: int = 256, inject_radius_m: float = 500.0,
+                 inject_top_offset_m: float = 50.0, spectrum_index: float = 1.5, e_min_eV: float = 1.0e6,
+                 e_max_eV: float = 1.0e8, targets=(1 << 20, 1 << 18), control_every: int = 10,
+                 capacity: int = 1 << 22, rng_seed: int = 20240623, feedback: bool = True, seed: int = 1):
+        self.fields = fields if fields is not None else column_fields()
+        self.cfg = StepConfig(rng_seed=rng_seed, dt_s=dt_s)
+        self.rng = np.random.default_rng(seed)
+        self.rate, self.inject_macros = rate_per_s, inject_macros
+        self.inject_radius, self.inject_z = inject_radius_m, self.fields.z_hi - inject_top_offset_m
+        self.index, self.e_min, self.e_max = spectrum_index, e_min_eV, e_max_eV
+        self.targets = targets
+        self.control_every = control_every
+        empty = ParticleTile(*(np.zeros(0) for _ in range(7)), id=np.zeros(0, dtype=np.int64),
+                             cpu=np.zeros(0, dtype=np.int32))
+        self.sim = Simulation(tables, self.fields, self.cfg, empty, capacity=capacity, next_id=1,
+                              secondary_capacity=max(1 << 14, capacity // 16), compact_every=1)
+        self.maxwell = MaxwellField(self.fields) if feedback else None
+        self.injected = 0
+        self.removed = np.zeros(2, dtype=np.int64)
+        self.steps = 0
+
+    def _inject(self) -> None:
+        n = self.inject_macros
+        if n <= 0:
+            return
+        w = self.rate * self.cfg.dt_s / n
+        k = power_law_energies(self.rng, n, self.index, self.e_min, self.e_max)
+        g = 1.0 + k / ME_C2_EV
+        u = C_LIGHT * np.sqrt((g - 1.0) * (g + 1.0))
+        # a 30-degree cone about -z (the force on electrons)
+        mu = -(1.0 - (1.0 - np.cos(np.deg2rad(30.0))) * self.rng.random(n))
+        phi = 2.0 * np.pi * self.rng.random(n)
+        st = np.sqrt(1.0 - mu * mu)
+        r = self.inject_radius * np.sqrt(self.rng.random(n))
+        theta = 2.0 * np.pi * self.rng.random(n) - np.pi
+        store = self.sim.electrons
+        first = int(self.sim.next_id.item())
+        tile = ParticleTile(r=r, z=np.full(n, self.inject_z), theta=theta, ux=u * st * np.cos(phi),
+                            uy=u * st * np.sin(phi), uz=u * mu, w=np.full(n, w),
+                            id=np.arange(first, first + n, dtype=np.int64), cpu=np.zeros(n, dtype=np.int32))
+        store.load(tile)
+        self.sim.next_id += n
+        self.sim.counts[ELECTRON] = store.n
+        self.injected += n
+
+    def step(self) -> None:
+        self._inject()
+        self.sim.step()
+        if self.maxwell is not None:
+            g = self.sim.out[0]
+            self.maxwell.feedback_step(g.jr, g.jz, self.cfg.dt_s, coupling=self.sim.coupling)
+        self.steps += 1
+        if self.control_every > 0 and self.steps % self.control_every == 0:
+            self.control()
+
+    def control(self) -> None:
+        """Population control of electrons and positrons above 1.25 x target."""
+        for k, (sp, target) in enumerate(((ELECTRON, self.targets[0]), (POSITRON, self.targets[1]))):
+            store = self.sim.stores[sp]
+            live = store.live()
+            if population.action(live, target, 0.0) != "thin":
+                continue
+            res = population.thin_tile(store.tile(), self.fields, target, self.cfg.rng_seed, self.sim.step_index,
+                                       species=0 if sp == ELECTRON else 1)
+            self.removed[k] += res.removed
+        self.sim.compact()
+
+    def run(self, steps: int) -> None:
+        for _ in range(steps):
+            self.step()
+
+    def observe(self) -> dict:
+        e, p, g = (self.sim.stores[s] for s in (0, 1, 2))
+        out = {"steps": self.steps, "electrons": e.live(), "positrons": p.live(), "photons": g.live(),
+               "injected": self.injected, "removed_electrons": int(self.removed[0]),
+               "removed_positrons": int(self.removed[1])}
+        if self.maxwell is not None:
+            out["max_scattered_ez_v_per_m"] = float(self.maxwell.ez.abs().max().item())
+        w = e.w[: e.n][e.id[: e.n] != -(2 ** 63)]
+        out["electron_weight"] = float(w.sum().item()) if w.numel() else 0.0
+        return out
