@@ -434,9 +434,31 @@ def column_leg(args, tables):
     obs = col.observe()
     log(f"feedback column: {steps} steps in {el:.2f} s ({el / steps * 1e3:.1f} ms/step), "
         f"{live / el:.3e} particle-steps/s; {obs}")
-    return {"workload": "feedback column, 256x1024 grid, dt = 1 ns, injection + transport + drain + commit + "
-                        "Maxwell feedback + population control", "steps": steps, "ms_per_step": el / steps * 1e3,
-            "particle_steps_per_s": live / el, "final": obs}
+    single = {"workload": "feedback column, 256x1024 grid, dt = 1 ns, injection + transport + drain + commit + "
+                          "Maxwell feedback + population control", "steps": steps, "ms_per_step": el / steps * 1e3,
+              "particle_steps_per_s": live / el, "final": obs}
+    # the feedback configuration's batch: 16 columns, E_z peaks 300-600 kV/m
+    from rrea_feedback_warpx_amrex_geant4_amd.column import BatchedColumn
+    peaks = np.linspace(3.0e5, 6.0e5, 16)
+    bc = BatchedColumn(tables, peaks, inject_macros=args.column_inject // 8, targets=(1 << 16, 1 << 13),
+                       control_every=10, capacity=1 << 23)
+    bc.run(10)
+    torch.cuda.synchronize()
+    live = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bc.step()
+        live += sum(s.n for s in bc.stores)
+    torch.cuda.synchronize()
+    el_b = time.perf_counter() - t0
+    obs_b = bc.observe()
+    bc.close()
+    log(f"batched columns: 16 instances, {steps} steps in {el_b:.2f} s ({el_b / steps * 1e3:.1f} ms/step), "
+        f"{live / el_b:.3e} particle-steps/s; electrons per instance {obs_b['electrons']}")
+    single["batched_16"] = {"steps": steps, "ms_per_step": el_b / steps * 1e3, "particle_steps_per_s": live / el_b,
+                            "electrons_per_instance": obs_b["electrons"],
+                            "removed_electrons": obs_b["removed_electrons"]}
+    return single
 
 
 def traffic_record():

@@ -143,3 +143,172 @@ class FeedbackColumn:
         w = e.w[: e.n][e.id[: e.n] != -(2 ** 63)]
         out["electron_weight"] = float(w.sum().item()) if w.numel() else 0.0
         return out
+
+
+class BatchedColumn:
+    """The feedback configuration's batch: ``len(peaks)`` independent columns
+    of differing E_z peaks stepped together.  Per step: injection into every
+    instance, the photon push, the batched species steps and drain
+    (rrea_transport_step_batched, rrea_newborn_drain_batched), the commit of
+    the survivors into the batched stores (each particle keeps its instance),
+    compaction, then each instance's deposit drives its own Maxwell field,
+    handed back to its own fields handle; every ``control_every`` steps each
+    instance's charged species are thinned above 1.25 x target on its own
+    mesh.  Ids are drawn from one counter for the whole batch, so an
+    instance's particle ids (the RNG keys) are not those a separate run of
+    that instance would assign."""
+
+    def __init__(self, tables: InteractionTables, peaks_v_per_m, nr: int = 256, nz: int = 1024,
+                 dt_s: float = 1.0e-9, rate_per_s: float = 1.0e14, inject_macros: int = 256,
+                 targets=(1 << 18, 1 << 15), control_every: int = 10, capacity: int = 1 << 22,
+                 rng_seed: int = 20240623, seed: int = 1, inject_radius_m: float = 500.0,
+                 spectrum_index: float = 1.5, e_min_eV: float = 1.0e6, e_max_eV: float = 1.0e8):
+        from .simulation import PHOTON, ParticleStore
+        from .transport import Batch, Coupling, DrainOutputs, InstanceGrids, StepOutputs
+        self.tables = tables
+        self.fields = [column_fields(float(p), nr=nr, nz=nz) for p in peaks_v_per_m]
+        self.n_inst = len(self.fields)
+        self.cfg = StepConfig(rng_seed=0, dt_s=dt_s)
+        self.seeds = [rng_seed + 7919 * i for i in range(self.n_inst)]
+        self.couplings = [Coupling(f, tables) for f in self.fields]
+        self.grids = InstanceGrids(self.fields)
+        self.batch = Batch(self.couplings, self.grids.grids, self.seeds)
+        self.maxwell = [MaxwellField(f) for f in self.fields]
+        self.stores = [ParticleStore(s, capacity, batched=True) for s in (ELECTRON, POSITRON, PHOTON)]
+        sec_cap = max(1 << 14, capacity // 16)
+        self.out = [StepOutputs(0, self.fields[0], secondary_capacity=sec_cap, per_particle=False)]
+        for _ in range(2):
+            self.out.append(StepOutputs(0, self.fields[0], secondary_capacity=sec_cap, per_particle=False,
+                                        share_grid=self.out[0]))
+        self.drain = DrainOutputs(4 * sec_cap, self.out[0])
+        self.next_id = torch.ones(1, dtype=torch.int64, device="cuda")
+        self.counts = torch.zeros(3, dtype=torch.int64, device="cuda")
+        self.rng = np.random.default_rng(seed)
+        self.rate, self.inject_macros, self.radius = rate_per_s, inject_macros, inject_radius_m
+        self.index, self.e_min, self.e_max = spectrum_index, e_min_eV, e_max_eV
+        self.targets, self.control_every = targets, control_every
+        self.steps = 0
+        self.step_index = 0
+        self.removed = np.zeros(2, dtype=np.int64)
+        self.injected = 0
+        self._photon = PHOTON
+
+    def _inject(self):
+        n = self.inject_macros
+        if n <= 0:
+            return
+        w = self.rate * self.cfg.dt_s / n
+        m = n * self.n_inst
+        k = power_law_energies(self.rng, m, self.index, self.e_min, self.e_max)
+        g = 1.0 + k / ME_C2_EV
+        u = C_LIGHT * np.sqrt((g - 1.0) * (g + 1.0))
+        mu = -(1.0 - (1.0 - np.cos(np.deg2rad(30.0))) * self.rng.random(m))
+        phi = 2.0 * np.pi * self.rng.random(m)
+        st = np.sqrt(1.0 - mu * mu)
+        first = int(self.next_id.item())
+        z_top = np.repeat([f.z_hi - 50.0 for f in self.fields], n)
+        tile = ParticleTile(r=self.radius * np.sqrt(self.rng.random(m)), z=z_top,
+                            theta=2.0 * np.pi * self.rng.random(m) - np.pi, ux=u * st * np.cos(phi),
+                            uy=u * st * np.sin(phi), uz=u * mu, w=np.full(m, w),
+                            id=np.arange(first, first + m, dtype=np.int64), cpu=np.zeros(m, dtype=np.int32))
+        e = self.stores[ELECTRON]
+        e.load(tile, np.repeat(np.arange(self.n_inst, dtype=np.int32), n))
+        self.next_id += m
+        self.counts[ELECTRON] = e.n
+        self.injected += m
+
+    def step(self):
+        from . import _lib
+        self._inject()
+        self.grids.zero_()
+        for o in self.out:
+            o.secondary_count.zero_()
+        cfg = StepConfig(rng_seed=0, step=self.step_index, dt_s=self.cfg.dt_s)
+        e, p, g = (self.stores[s] for s in (ELECTRON, POSITRON, self._photon))
+        if g.n:
+            g.photon_tile().push(cfg.dt_s)
+        for name, store, out in (("electron", e, self.out[0]), ("photon", g, self.out[1]),
+                                 ("positron", p, self.out[2])):
+            if store.n == 0:
+                continue
+            tile = store.photon_tile() if name == "photon" else store.tile()
+            self.batch.step(self.tables, name, tile, store.inst[: store.n], cfg, out, check=False)
+        self.batch.drain(self.tables, cfg, self.out, self.drain, check=False)
+        structs = (_lib.ParticleStore * 3)(*[s.struct() for s in self.stores])
+        err = self.out[0].error
+        _lib.check(_lib.lib().rrea_commit_survivors(self.drain.survivors.data_ptr(),
+                                                    self.drain.survivor_count.data_ptr(), structs,
+                                                    self.counts.data_ptr(), self.next_id.data_ptr(), 0,
+                                                    err.data_ptr(), torch.cuda.current_stream().cuda_stream),
+                   "rrea_commit_survivors")
+        counts = self.counts.cpu().numpy()
+        self.out[0].check_error()
+        for s, c in zip(self.stores, counts):
+            s.n = int(c)
+        self.compact()
+        for i, m in enumerate(self.maxwell):
+            gi = self.grids.grids[i]
+            m.feedback_step(gi.jr, gi.jz, cfg.dt_s, coupling=self.couplings[i])
+        self.step_index += 1
+        self.steps += 1
+        if self.control_every > 0 and self.steps % self.control_every == 0:
+            self.control()
+
+    def compact(self):
+        import ctypes as C
+
+        from . import _lib
+        L = _lib.lib()
+        for s in self.stores:
+            if s.n == 0:
+                continue
+            t = s.photon_tile().struct() if s.photon else s.tile().struct()
+            live = C.c_int64(0)
+            _lib.check(L.rrea_tile_compact(self.couplings[0].h, C.byref(t), 1 if s.photon else 0, s.inst.data_ptr(),
+                                           C.byref(live), torch.cuda.current_stream().cuda_stream),
+                       "rrea_tile_compact")
+            s.n = int(live.value)
+        self.counts.copy_(torch.tensor([s.n for s in self.stores], dtype=torch.int64))
+
+    def control(self):
+        """Each instance's electrons and positrons thinned on its own mesh."""
+        from . import _lib
+        for k, (sp, target) in enumerate(((ELECTRON, self.targets[0]), (POSITRON, self.targets[1]))):
+            store = self.stores[sp]
+            if store.n == 0:
+                continue
+            inst = store.inst[: store.n]
+            valid = store.id[: store.n] != _lib.INVALID_ID
+            per = torch.bincount(inst[valid].long(), minlength=self.n_inst).cpu().numpy()
+            for i in range(self.n_inst):
+                if population.action(int(per[i]), target, 0.0) != "thin":
+                    continue
+                idx = torch.nonzero(valid & (inst == i)).flatten()
+                sub = {q: getattr(store, q)[: store.n][idx].contiguous()
+                       for q in ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")}
+                sid = (torch.abs(sub["id"]) ^ (sub["cpu"].to(torch.int64) << 40)).contiguous()
+                res = population.thin(sub["r"], sub["z"], sub["theta"], sub["ux"], sub["uy"], sub["uz"], sub["w"],
+                                      sid, self.fields[i], target, self.seeds[i], self.step_index,
+                                      species=0 if sp == ELECTRON else 1, id=sub["id"])
+                store.w[: store.n][idx] = sub["w"]
+                store.id[: store.n][idx] = sub["id"]
+                self.removed[k] += res.removed
+        self.compact()
+
+    def run(self, steps: int):
+        for _ in range(steps):
+            self.step()
+
+    def observe(self) -> dict:
+        from . import _lib
+        out = {"steps": self.steps, "injected": self.injected, "removed_electrons": int(self.removed[0]),
+               "removed_positrons": int(self.removed[1])}
+        for name, sp in (("electrons", ELECTRON), ("positrons", POSITRON), ("photons", self._photon)):
+            s = self.stores[sp]
+            valid = s.id[: s.n] != _lib.INVALID_ID
+            out[name] = torch.bincount(s.inst[: s.n][valid].long(), minlength=self.n_inst).cpu().tolist()
+        out["max_scattered_ez_v_per_m"] = [float(m.ez.abs().max().item()) for m in self.maxwell]
+        return out
+
+    def close(self):
+        self.batch.close()

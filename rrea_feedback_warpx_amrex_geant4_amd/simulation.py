@@ -45,23 +45,30 @@ class ParticleStore:
     arrays of ``capacity`` particles, the first ``n`` in use.  Photon stores
     also hold the pre-push point (prev_r, prev_z, prev_theta)."""
 
-    def __init__(self, species: int, capacity: int, device="cuda"):
+    def __init__(self, species: int, capacity: int, device="cuda", batched: bool = False):
         self.species = species
         self.capacity = int(capacity)
         f = lambda: torch.zeros(self.capacity, dtype=torch.float64, device=device)  # noqa: E731
         self.r, self.z, self.theta, self.ux, self.uy, self.uz, self.w = (f() for _ in range(7))
         self.id = torch.full((self.capacity,), _lib.INVALID_ID, dtype=torch.int64, device=device)
         self.cpu = torch.zeros(self.capacity, dtype=torch.int32, device=device)
+        # a batched store also holds each particle's instance (rrea_particle_store.instance)
+        self.inst = torch.zeros(self.capacity, dtype=torch.int32, device=device) if batched else None
         self.photon = species == PHOTON
         if self.photon:
             self.prev_r, self.prev_z, self.prev_theta = f(), f(), f()
         self.n = 0
 
-    def load(self, tile: ParticleTile) -> None:
-        """Append host particles (r, z, theta, u, w, id, cpu) behind the store's count."""
+    def load(self, tile: ParticleTile, instance: np.ndarray | None = None) -> None:
+        """Append host particles (r, z, theta, u, w, id, cpu; and their
+        instances in a batched store) behind the store's count."""
         m = len(tile.r)
         if self.n + m > self.capacity:
             raise ValueError("particle store capacity exceeded")
+        if self.inst is not None:
+            if instance is None or len(instance) != m:
+                raise ValueError("a batched store needs each particle's instance")
+            self.inst[self.n:self.n + m].copy_(torch.as_tensor(np.ascontiguousarray(instance, dtype=np.int32)))
         for k in ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu"):
             dst = getattr(self, k)
             dst[self.n:self.n + m].copy_(torch.as_tensor(np.ascontiguousarray(getattr(tile, k)), dtype=dst.dtype))
@@ -87,7 +94,7 @@ class ParticleStore:
         else:
             cur = [p(getattr(self, k)) for k in ("r", "z", "theta", "ux", "uy", "uz")]
             t = _lib.Tile(0, *cur, p(self.w), p(self.id), p(self.cpu), *cur)
-        return _lib.ParticleStore(t, None, self.capacity)
+        return _lib.ParticleStore(t, None if self.inst is None else p(self.inst), self.capacity)
 
     def live(self) -> int:
         return int((self.id[: self.n] != _lib.INVALID_ID).sum().item())
