@@ -133,6 +133,36 @@ class FeedbackColumn:
         for _ in range(steps):
             self.step()
 
+    def final_window_sensitivity(self, weights: torch.Tensor, steps: int = 8, checkpoint_every: int = 4,
+                                 dt_s: float | None = None):
+        """The adjoint of the final window's current with respect to the
+        Maxwell field: the live electrons' drift segment (push, CSDA loss,
+        deposit; discrete events frozen) over ``steps`` steps from the column's
+        present state and field, L = sum(J_z * weights) over the padded jz
+        grid, and dL/dE_r, dL/dE_z on the Maxwell faces (adjoint.DriftCurrent).
+        Returns (L, dL/dEr, dL/dEz)."""
+        import dataclasses
+
+        from . import _lib
+        from .adjoint import DriftConfig, DriftProblem, drift_current
+        e = self.sim.electrons
+        h = e.host()
+        live = h["id"] != _lib.INVALID_ID
+        keys = ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")
+        parts = ParticleTile(*(h[k][live] for k in keys))
+        f = self.fields
+        if self.maxwell is not None:
+            f = dataclasses.replace(f, er=self.maxwell.er.cpu().numpy(), ez=self.maxwell.ez.cpu().numpy(),
+                                    btheta=self.maxwell.bt.cpu().numpy())
+        prob = DriftProblem(self.sim.tables, f, parts,
+                            DriftConfig(steps=steps, dt_s=dt_s or self.cfg.dt_s, checkpoint_every=checkpoint_every))
+        er = prob.er0.clone().requires_grad_(True)
+        ez = prob.ez0.clone().requires_grad_(True)
+        _, jz = drift_current(er, ez, prob)
+        loss = (jz * weights).sum()
+        loss.backward()
+        return loss.detach(), er.grad, ez.grad
+
     def observe(self) -> dict:
         e, p, g = (self.sim.stores[s] for s in (0, 1, 2))
         out = {"steps": self.steps, "electrons": e.live(), "positrons": p.live(), "photons": g.live(),

@@ -75,3 +75,49 @@ def test_batched_column_steps_every_instance(tables):
     w = e.w[: e.n].cpu().numpy()
     assert np.all(np.isfinite(w)) and np.all(w > 0)
     col.close()
+
+
+def test_final_window_sensitivity(tables):
+    """The adjoint of the column's final-window current w.r.t. its Maxwell
+    field: finite, non-zero where the electrons drift, and equal to a
+    Richardson-extrapolated central difference of the same drift along a
+    random field direction.  This checks the wiring (a wrong field, sign or
+    weight gives an O(1) mismatch); the difference of a kinked loss (cell
+    crossings, the cutoff) over every face at once carries ~1e-4 noise, so
+    the bar is 2e-3.  The drift adjoint itself is pinned face by face against
+    an FP64 reference gradient to 3e-15 in test_gpu_adjoint.py."""
+    from rrea_feedback_warpx_amrex_geant4_amd.adjoint import DriftConfig, DriftProblem, drift_current
+    from rrea_feedback_warpx_amrex_geant4_amd.column import FeedbackColumn, column_fields
+    fields = column_fields(nr=64, nz=256)
+    col = FeedbackColumn(tables, fields, dt_s=1e-9, inject_macros=256, targets=(1 << 20, 1 << 18), capacity=1 << 16)
+    col.run(5)
+    print("case final-window sensitivity: 5 column steps, then 4 drift steps", flush=True)
+    nrp, nzp = fields.padded_shape()
+    g = torch.Generator(device="cpu").manual_seed(3)
+    wts = torch.randn(nrp * (nzp + 1), generator=g, dtype=torch.float64).cuda()
+    L, ger, gez = col.final_window_sensitivity(wts, steps=4, checkpoint_every=2, dt_s=1e-10)
+    assert torch.isfinite(ger).all() and torch.isfinite(gez).all()
+    assert float(gez.abs().max()) > 0.0
+    # directional derivative check along a random direction of E_z
+    import dataclasses
+    f = dataclasses.replace(fields, er=col.maxwell.er.cpu().numpy(), ez=col.maxwell.ez.cpu().numpy(),
+                            btheta=col.maxwell.bt.cpu().numpy())
+    e = col.sim.electrons
+    h = e.host()
+    live = h["id"] != np.iinfo(np.int64).min
+    from rrea_feedback_warpx_amrex_geant4_amd.config import ParticleTile
+    parts = ParticleTile(*(h[k][live] for k in ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")))
+    prob = DriftProblem(tables, f, parts, DriftConfig(steps=4, dt_s=1e-10, checkpoint_every=2))
+    d = torch.randn(prob.ez0.numel(), generator=g, dtype=torch.float64).cuda()
+    h_step = 1e-3 * float(prob.ez0.abs().max() + 1.0) + 1.0
+
+    def loss(ez):
+        with torch.no_grad():
+            return float((drift_current(prob.er0, ez, prob)[1] * wts).sum())
+    def central(hh):
+        return (loss(prob.ez0 + hh * d) - loss(prob.ez0 - hh * d)) / (2 * hh)
+    fd = (4.0 * central(h_step) - central(2.0 * h_step)) / 3.0
+    ad = float((gez * d).sum())
+    assert abs(fd - ad) <= 2e-3 * abs(ad), (fd, ad)
+    print(f"final-window sensitivity: L = {float(L):.4e}, directional adjoint {ad:.6e} vs difference {fd:.6e}",
+          flush=True)
