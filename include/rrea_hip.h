@@ -252,7 +252,9 @@ typedef struct rrea_step_out {
     double* jr; double* jz; double* rho; /* padded net-chord grid, accumulated */
     double* tallies;          /* device double[RREA_TALLY_COUNT], accumulated */
     int32_t* error;           /* device int32[2]: first error code, particle index */
-    int32_t order_secondaries;/* nonzero: sort the staged secondaries by (parent, lineage) */
+    int32_t order_secondaries;/* nonzero: sort the staged secondaries by (parent, lineage); 1: the call
+                                 reads the staged count back once to size the sort; 2: the count stays
+                                 on the device (one-workgroup sort, no host round trip) */
     double* sources;          /* material-source grids (RREA_SOURCE_FIELDS x nr x nz, cell c = i + nr*j),
                                  accumulated raw (see rrea_sources_normalize); NULL: not accumulated */
 } rrea_step_out;
@@ -426,6 +428,17 @@ rrea_status rrea_batch_graph_create(rrea_tables_t tables, rrea_batch_t batch, co
 /* Enqueues the graph's steps first_step, first_step + 1, ... on `stream`. */
 rrea_status rrea_batch_graph_launch(rrea_batch_graph_t graph, int64_t first_step, rrea_stream_t stream);
 rrea_status rrea_batch_graph_destroy(rrea_batch_graph_t graph);
+
+/* Scheduling support (no counterpart in the overlay, which runs the species
+ * loops one after another on the host).  With `enable`, every species step
+ * on this handle records the handle's event on its stream right after the
+ * fast-path kernel, before the hand-over kernel and the ordering;
+ * rrea_stream_wait_fast_path makes `stream` wait for the last record, so
+ * other work (another species' pass) can start once the bulk of a step is
+ * done and run under the latency-bound hand-over.  Enqueue the wait after
+ * the step call that records it. */
+rrea_status rrea_fields_signal_fast_path(rrea_fields_t fields, int32_t enable);
+rrea_status rrea_stream_wait_fast_path(rrea_fields_t fields, rrea_stream_t stream);
 
 /* ------------------------------------------------------------------------
  * The field side of the feedback loop: the step's net-chord deposit as

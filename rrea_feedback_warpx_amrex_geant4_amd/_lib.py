@@ -211,6 +211,8 @@ SIGNATURES = [
     ("rrea_maxwell_ledger", C.c_int32, [C.POINTER(RzGeometry)] + [C.c_void_p] * 9),
     ("rrea_population_thin", C.c_int32,
      [C.POINTER(PopulationRecords), C.POINTER(PopulationConfig), P_I64, C.c_void_p, C.c_void_p]),
+    ("rrea_fields_signal_fast_path", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("rrea_stream_wait_fast_path", C.c_int32, [C.c_void_p, C.c_void_p]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),

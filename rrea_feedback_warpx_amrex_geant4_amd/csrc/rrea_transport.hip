@@ -222,6 +222,9 @@ struct rrea_fields_s {
     size_t sel_tmp_bytes = 0;
     int64_t fifo_cap = 0;
     bool timing = false;
+    // signalled after each fast-path kernel when enabled (rrea_fields_signal_fast_path)
+    hipEvent_t fast_event = nullptr;
+    bool fast_signal = false;
     struct Timed { hipEvent_t start, stop; int32_t kernel; };
     std::vector<Timed> events;   // pool; the first events_used are recorded launches
     size_t events_used = 0;
@@ -3785,7 +3788,23 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
                     F->d_live, F->d_dargs, F->d_gcell};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : F->events) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
+    if (F->fast_event) (void)hipEventDestroy(F->fast_event);
     delete F;
+    return RREA_OK;
+}
+
+rrea_status rrea_fields_signal_fast_path(rrea_fields_t F, int32_t enable)
+{
+    if (!F) return RREA_ERR_ARGUMENT;
+    if (enable && !F->fast_event) HIP_TRY(hipEventCreateWithFlags(&F->fast_event, hipEventDisableTiming));
+    F->fast_signal = enable != 0;
+    return RREA_OK;
+}
+
+rrea_status rrea_stream_wait_fast_path(rrea_fields_t F, rrea_stream_t stream)
+{
+    if (!F || !F->fast_event) return RREA_ERR_ARGUMENT;
+    HIP_TRY(hipStreamWaitEvent(static_cast<hipStream_t>(stream), F->fast_event, 0));
     return RREA_OK;
 }
 
@@ -4194,6 +4213,7 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
             }
         });
         if (st != RREA_OK) return st;
+        if (W->fast_signal) HIP_TRY(hipEventRecord(W->fast_event, s));
         int64_t const slow_blocks = nb < 2048 ? nb : 2048;
         st = timed(5, [&] {
             photon_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
@@ -4225,6 +4245,7 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
             }
         });
         if (st != RREA_OK) return st;
+        if (W->fast_signal) HIP_TRY(hipEventRecord(W->fast_event, s));
         int64_t const slow_blocks = nb < 2048 ? nb : 2048;
         st = timed(kid + 1, [&] {
             if (positron) {
@@ -4277,7 +4298,8 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     hipStream_t const s = static_cast<hipStream_t>(stream);
     // pageable source: the copy has consumed A when hipMemcpyAsync returns
     HIP_TRY(hipMemcpyAsync(W->d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
-    return species_launch(W, static_cast<const StepArgs*>(W->d_args), L, out, s, false);
+    // order_secondaries 2: the count stays on the device (no host round trip)
+    return species_launch(W, static_cast<const StepArgs*>(W->d_args), L, out, s, out->order_secondaries == 2);
 }
 
 

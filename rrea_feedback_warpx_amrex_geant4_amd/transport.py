@@ -324,7 +324,7 @@ class StepOutputs:
         p = lambda t: None if t is None else t.data_ptr()  # noqa: E731
         return _lib.StepOut(p(self.status), p(self.ion_pairs), p(self.substeps), p(self.secondaries),
                             self.secondary_capacity, p(self.secondary_count), p(self.jr), p(self.jz), p(self.rho),
-                            p(self.tallies), p(self.error), 1 if order else 0, p(self.sources))
+                            p(self.tallies), p(self.error), int(order), p(self.sources))
 
     def secondaries_host(self) -> np.ndarray:
         n = min(int(self.secondary_count.item()), self.secondary_capacity)
@@ -703,17 +703,19 @@ class ConcurrentApply:
     reads another's output before the drain.  Run concurrently they give the
     sequential Apply's results (the J and source sums up to FP64 summation
     order).  Each pass gets its own fields handle (the argument block and the
-    ordering workspace belong to a handle) and its own stream, and is driven
-    from its own host thread, so the one host round trip each ordered pass
-    makes does not hold back the others.  The drain then runs on the caller's
-    stream after all three."""
+    ordering workspace belong to a handle) and its own stream; every pass
+    orders its secondaries with the count on the device, so one host thread
+    enqueues all three without a round trip.  The photon and positron passes
+    wait for the end of the electron fast-path kernel
+    (rrea_fields_signal_fast_path), so they run under the latency-bound
+    electron hand-over rather than competing with the electron kernel.  The
+    drain then runs on the caller's stream after all three."""
 
     def __init__(self, tables: InteractionTables, fields: RzFields):
-        from concurrent.futures import ThreadPoolExecutor
         self.tables = tables
         self.couplings = [Coupling(fields, tables) for _ in range(3)]  # electrons, photons, positrons
         self.streams = [torch.cuda.Stream(), torch.cuda.Stream()]      # default priority
-        self.pool = ThreadPoolExecutor(max_workers=2)
+        _lib.check(_lib.lib().rrea_fields_signal_fast_path(self.couplings[0].h, 1), "rrea_fields_signal_fast_path")
 
     def run(self, electrons: DeviceTile, photons: "PhotonTile", positrons: DeviceTile, cfg: StepConfig,
             out_e: StepOutputs, out_g: StepOutputs, out_p: StepOutputs, drain: "DrainOutputs | None" = None,
@@ -725,14 +727,13 @@ class ConcurrentApply:
             s.wait_event(ready)
         apply = RreaParticleInteraction
         cg, cp = self.couplings[1], self.couplings[2]
-        fg = self.pool.submit(apply.apply_photons, self.tables, cg, photons, cfg, out_g, True,
-                              self.streams[0].cuda_stream, False)
-        fp = self.pool.submit(apply.apply_positrons, self.tables, cp, positrons, cfg, out_p, None, True,
-                              self.streams[1].cuda_stream, False)
-        apply.apply_electrons(self.tables, self.couplings[0], electrons, cfg, out_e, stream=main.cuda_stream,
-                              check=False)
-        fg.result()
-        fp.result()
+        apply.apply_electrons(self.tables, self.couplings[0], electrons, cfg, out_e, order_secondaries=2,
+                              stream=main.cuda_stream, check=False)
+        L = _lib.lib()
+        for s in self.streams:
+            _lib.check(L.rrea_stream_wait_fast_path(self.couplings[0].h, s.cuda_stream), "rrea_stream_wait_fast_path")
+        apply.apply_photons(self.tables, cg, photons, cfg, out_g, 2, self.streams[0].cuda_stream, False)
+        apply.apply_positrons(self.tables, cp, positrons, cfg, out_p, None, 2, self.streams[1].cuda_stream, False)
         for s in self.streams:
             done = torch.cuda.Event()
             done.record(s)
@@ -744,4 +745,4 @@ class ConcurrentApply:
             out_e.check_error()
 
     def close(self):
-        self.pool.shutdown(wait=True)
+        pass
