@@ -15,9 +15,11 @@ tiles) inside the timed region.
 
 One "step" is one RreaParticleInteraction::Apply (RreaParticleInteraction.cpp:57-60):
 the electron kernels, the photon kernel, the positron kernels, each followed
-by the deterministic secondary ordering -- the three passes run concurrently
-on three streams (ConcurrentApply; --sequential runs them one after
-another; the results are the same) -- then the newborn residual advance of
+by the deterministic secondary ordering -- the three passes run on three
+streams, the photon and positron passes starting when the electron fast-path
+kernel ends, under the electron hand-over (ConcurrentApply; --sequential runs
+them one after another; the results are the same) -- then the newborn
+residual advance of
 every secondary the step made (the transient FIFO of
 DrainAndCommitSecondaries), all charged chords depositing into one J grid.  Charged tiles step in place (the fused pre-push snapshot); the photon
 tile is pushed on the device after its step (prev <- current, current += c dt
@@ -29,12 +31,12 @@ until its redistribute.
 Prints one JSON line (rank 0, N=1).  `roofline` is for the dominant kernel,
 the electron kernel: algorithmic bytes per launch (112 B per live electron)
 over its mean launch time from HIP events on its own stream over the timed
-region (where it shares the CUs with the concurrent photon and positron
-passes), and `roofline.isolated` the same for the kernel timed alone on a
+region, and `roofline.isolated` the same for the kernel timed alone on a
 few steps after the timed region; `traffic` is that kernel's measured HBM
 bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE record in
-profiles/electron_kernel_traffic.json when present.  The batched leg, the
-drift-adjoint leg and the CPU baseline follow (see their functions).
+profiles/electron_kernel_traffic.json when present.  The batched (plain per
+instance, batched, graph-captured), drift-adjoint, field-side, population
+and feedback-column legs and the CPU baseline follow (see their functions).
 """
 
 from __future__ import annotations
