@@ -719,7 +719,11 @@ class ConcurrentApply:
 
     def run(self, electrons: DeviceTile, photons: "PhotonTile", positrons: DeviceTile, cfg: StepConfig,
             out_e: StepOutputs, out_g: StepOutputs, out_p: StepOutputs, drain: "DrainOutputs | None" = None,
-            check: bool = False) -> None:
+            check: bool = False, push_photons: bool = False) -> None:
+        """``push_photons``: WarpX's photon push for the next step
+        (rrea_photon_push) right after the photon pass, on its stream, so it
+        runs under the other passes instead of after the drain (nothing
+        after the photon pass reads the photon tile)."""
         main = torch.cuda.current_stream()
         ready = torch.cuda.Event()
         ready.record(main)
@@ -733,6 +737,8 @@ class ConcurrentApply:
         for s in self.streams:
             _lib.check(L.rrea_stream_wait_fast_path(self.couplings[0].h, s.cuda_stream), "rrea_stream_wait_fast_path")
         apply.apply_photons(self.tables, cg, photons, cfg, out_g, 2, self.streams[0].cuda_stream, False)
+        if push_photons:
+            photons.push(cfg.dt_s, self.streams[0].cuda_stream)
         apply.apply_positrons(self.tables, cp, positrons, cfg, out_p, None, 2, self.streams[1].cuda_stream, False)
         for s in self.streams:
             done = torch.cuda.Event()

@@ -55,7 +55,11 @@ def test_concurrent_apply_matches_sequential(tables):
         apply.apply_photons(tables, coupling, seq_t[1], cfg, seq_o[1])
         apply.apply_positrons(tables, coupling, seq_t[2], cfg, seq_o[2])
         apply.drain_secondaries(tables, coupling, cfg, list(seq_o), seq_d)
-        conc.run(con_t[0], con_t[1], con_t[2], cfg, con_o[0], con_o[1], con_o[2], con_d, check=True)
+        # the concurrent step pushes its photons on the photon stream; the
+        # sequential one pushes after its drain
+        conc.run(con_t[0], con_t[1], con_t[2], cfg, con_o[0], con_o[1], con_o[2], con_d, check=True,
+                 push_photons=True)
+        seq_t[1].push(cfg.dt_s)
         torch.cuda.synchronize()
         for a, b in ((seq_t[0], con_t[0]), (seq_t[1].cur, con_t[1].cur), (seq_t[2], con_t[2])):
             for k in ("r", "z", "theta", "ux", "uy", "uz", "w", "id"):
@@ -71,7 +75,7 @@ def test_concurrent_apply_matches_sequential(tables):
             a, b = getattr(seq_o[0], k), getattr(con_o[0], k)
             assert torch.allclose(a, b, rtol=0.0, atol=1e-12 * float(a.abs().max())), k
         assert torch.allclose(seq_o[0].tallies, con_o[0].tallies, rtol=1e-12, atol=0.0)
-        seq_t[1].push(cfg.dt_s)
-        con_t[1].push(cfg.dt_s)
+        for k in ("prev_r", "prev_z", "prev_theta"):
+            assert torch.equal(getattr(seq_t[1], k), getattr(con_t[1], k)), (step, k)
     print(f"concurrent apply: 3 steps identical ({len(wa)} FIFO entries in the last)", flush=True)
     conc.close()
