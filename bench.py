@@ -493,6 +493,9 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batched-warmup", type=int, default=2)
     ap.add_argument("--field-steps", type=int, default=200, help="timed feedback steps of the field leg (0: skip)")
+    ap.add_argument("--device-drain", type=int, default=0,
+                    help="concurrent step: drain with the counts on the device, up to this many generations "
+                         "(0: the host-driven drain)")
     ap.add_argument("--push-on-photon-stream", action=argparse.BooleanOptionalAction, default=False,
                     help="concurrent step: push the photons on their own stream right after their pass "
                          "(default: on the main stream after the drain, measured 1-3 %% faster)")
@@ -574,7 +577,8 @@ def main():
         out_e.sources.zero_()
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
         if conc is not None:
-            conc.run(dev_e, dev_g, dev_p, cfg, out_e, out_g, out_p, drain, push_photons=args.push_on_photon_stream)
+            conc.run(dev_e, dev_g, dev_p, cfg, out_e, out_g, out_p, drain, push_photons=args.push_on_photon_stream,
+                     device_drain=args.device_drain)
             if not args.push_on_photon_stream:
                 dev_g.push(cfg.dt_s)
         else:

@@ -429,6 +429,21 @@ rrea_status rrea_batch_graph_create(rrea_tables_t tables, rrea_batch_t batch, co
 rrea_status rrea_batch_graph_launch(rrea_batch_graph_t graph, int64_t first_step, rrea_stream_t stream);
 rrea_status rrea_batch_graph_destroy(rrea_batch_graph_t graph);
 
+/* The newborn drain of one plain step without a host round trip: the
+ * species' ordered records (staged[k], staged_counts[k] device int64,
+ * staged_caps[k]; Apply's order) are concatenated into io->work on the
+ * device, then the FIFO is advanced with its generation bounds on the
+ * device, up to max_generations generations (a FIFO still holding an
+ * unadvanced generation is RREA_ERR_SECONDARY_CAPACITY, index -2); the FIFO,
+ * its lineages and the survivors are rrea_newborn_drain's; io->n_initial is
+ * not read, and the per-entry status, ion pairs and substeps are not
+ * written.  Replaces the same FIFO as rrea_newborn_drain
+ * (RreaSecondaryCommit.cpp:338-468). */
+rrea_status rrea_newborn_drain_staged(rrea_tables_t tables, rrea_fields_t fields, const rrea_step_config* config,
+                                      const rrea_secondary* const* staged, const int64_t* const* staged_counts,
+                                      const int64_t* staged_caps, int32_t n_staged, const rrea_drain_io* io,
+                                      int32_t max_generations, rrea_stream_t stream);
+
 /* Scheduling support (no counterpart in the overlay, which runs the species
  * loops one after another on the host).  With `enable`, every species step
  * on this handle records the handle's event on its stream right after the

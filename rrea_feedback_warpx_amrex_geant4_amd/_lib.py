@@ -211,6 +211,9 @@ SIGNATURES = [
     ("rrea_maxwell_ledger", C.c_int32, [C.POINTER(RzGeometry)] + [C.c_void_p] * 9),
     ("rrea_population_thin", C.c_int32,
      [C.POINTER(PopulationRecords), C.POINTER(PopulationConfig), P_I64, C.c_void_p, C.c_void_p]),
+    ("rrea_newborn_drain_staged", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.POINTER(StepConfigC), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), P_I64,
+      C.c_int32, C.POINTER(DrainIO), C.c_int32, C.c_void_p]),
     ("rrea_fields_signal_fast_path", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_stream_wait_fast_path", C.c_int32, [C.c_void_p, C.c_void_p]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),

@@ -225,6 +225,7 @@ struct rrea_fields_s {
     // signalled after each fast-path kernel when enabled (rrea_fields_signal_fast_path)
     hipEvent_t fast_event = nullptr;
     bool fast_signal = false;
+    void* d_dstate = nullptr;        // the device-driven drain's generation state
     struct Timed { hipEvent_t start, stop; int32_t kernel; };
     std::vector<Timed> events;   // pool; the first events_used are recorded launches
     size_t events_used = 0;
@@ -3789,6 +3790,7 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& ev : F->events) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
     if (F->fast_event) (void)hipEventDestroy(F->fast_event);
+    if (F->d_dstate) (void)hipFree(F->d_dstate);
     delete F;
     return RREA_OK;
 }
@@ -4338,6 +4340,39 @@ rrea_status fifo_workspace(rrea_fields_t W, int64_t cap)
     return RREA_OK;
 }
 
+// Up to three species' ordered records, concatenated on the device in the
+// order given (Apply's: electrons, photons, positrons) -- what the host-side
+// staging does with counts read back (DrainOutputs.stage).
+struct StagedLists {
+    const rrea_secondary* rec[3];
+    const int64_t* count[3];
+    int64_t cap[3];
+    int32_t n;
+};
+
+__global__ void concat_staged_kernel(StagedLists S, rrea_secondary* __restrict__ work, int64_t work_cap,
+                                     int64_t* __restrict__ total, int32_t* __restrict__ error)
+{
+    int64_t off[4] = {0, 0, 0, 0};
+    for (int k = 0; k < S.n; ++k) {
+        int64_t c = *S.count[k];
+        c = c < 0 ? 0 : (c > S.cap[k] ? S.cap[k] : c);
+        off[k + 1] = off[k] + c;
+    }
+    int64_t const all = off[S.n];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *total = all < work_cap ? all : work_cap;
+        if (all > work_cap) report(error, kErrCapacity, -1);
+    }
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < all && i < work_cap;
+         i += stride) {
+        int k = 0;
+        while (k + 1 < S.n && i >= off[k + 1]) ++k;
+        work[i] = S.rec[k][i - off[k]];
+    }
+}
+
 // The device-driven drain's launches: no host round trip, so the sequence
 // can be captured into a graph.  The argument block at d_nargs holds all but
 // the generation bounds, which drain_begin/advance_kernel write on the
@@ -4861,6 +4896,77 @@ rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step
 {
     return drain_impl(T, F, cfg, io, stream, nullptr);
 }
+
+rrea_status rrea_newborn_drain_staged(rrea_tables_t T, rrea_fields_t F, const rrea_step_config* cfg,
+                                      const rrea_secondary* const* staged, const int64_t* const* staged_counts,
+                                      const int64_t* staged_caps, int32_t n_staged, const rrea_drain_io* io,
+                                      int32_t max_generations, rrea_stream_t stream)
+{
+    if (!T || !F || !cfg || !io || !F->fields_set || !staged || !staged_counts || !staged_caps || n_staged < 1
+        || n_staged > 3 || max_generations < 1 || max_generations > 64) {
+        return RREA_ERR_ARGUMENT;
+    }
+    for (int32_t k = 0; k < n_staged; ++k) {
+        if (!staged[k] || !staged_counts[k] || staged_caps[k] < 0) return RREA_ERR_ARGUMENT;
+    }
+    if (!io->work || !io->work_count || !io->survivors || !io->survivor_count || !io->tallies || !io->error
+        || !io->jr || !io->jz || !io->rho) {
+        return RREA_ERR_ARGUMENT;
+    }
+    int64_t const cap = io->work_capacity;
+    if (cap <= 0 || cap > INT32_MAX || io->survivor_capacity < 0) return RREA_ERR_ARGUMENT;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != T->device) return RREA_ERR_ARGUMENT;
+    // RreaParticleInteraction::Apply :38-40 -- a non-positive dt does nothing.
+    if (!(cfg->dt_s > 0.0)) {
+        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->work_count, 0);
+        set_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(io->survivor_count, 0);
+        HIP_TRY(hipGetLastError());
+        return RREA_OK;
+    }
+    if (!(cfg->max_optical_depth_per_substep > 0.0) || cfg->max_subcycles == 0 || cfg->max_subcycles > 16383) {
+        return RREA_ERR_ARGUMENT;
+    }
+    rrea_fields_t const W = F;
+    {
+        rrea_status const ws = fifo_workspace(W, cap);
+        if (ws != RREA_OK) return ws;
+    }
+    if (!W->d_dstate) HIP_TRY(hipMalloc(&W->d_dstate, sizeof(DrainState)));
+    StepArgs A{};
+    {
+        rrea_status const cs = common_args(T, F, cfg, io->jr, io->jz, io->rho, A, io->sources);
+        if (cs != RREA_OK) return cs;
+    }
+    A.tallies = io->tallies;
+    A.error = io->error;
+    A.sec = W->d_stage_sec;
+    A.sec_cap = 0;  // written on the device per generation
+    A.sec_count = W->d_count;
+    NewbornArgs N{A, io->work, 0, 0, W->d_fin, W->d_fst, W->d_fion, W->d_fsub};
+    // pageable source: the copy has consumed N when hipMemcpyAsync returns
+    HIP_TRY(hipMemcpyAsync(W->d_nargs, &N, sizeof(NewbornArgs), hipMemcpyHostToDevice, s));
+    StagedLists S{};
+    S.n = n_staged;
+    int64_t most = 0;
+    for (int32_t k = 0; k < n_staged; ++k) {
+        S.rec[k] = staged[k];
+        S.count[k] = staged_counts[k];
+        S.cap[k] = staged_caps[k];
+        most += staged_caps[k];
+    }
+    unsigned const cb = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks_for(std::min(most, cap), 256),
+                                                                                      1024)));
+    // the FIFO's initial length goes through work_count, which the drain's end overwrites
+    concat_staged_kernel<<<dim3(cb), dim3(256), 0, s>>>(S, io->work, cap, io->work_count, io->error);
+    HIP_TRY(hipGetLastError());
+    return drain_device_launch(W, static_cast<NewbornArgs*>(W->d_nargs), static_cast<DrainState*>(W->d_dstate),
+                               io->work, cap, io->work_count, io->work_count, io->survivors, io->survivor_capacity,
+                               io->survivor_count, io->error, max_generations, s);
+}
+
 
 rrea_status rrea_tile_sort(rrea_fields_t F, const rrea_tile* tile, int32_t photon, rrea_stream_t stream)
 {

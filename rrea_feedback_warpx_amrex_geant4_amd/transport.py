@@ -719,7 +719,7 @@ class ConcurrentApply:
 
     def run(self, electrons: DeviceTile, photons: "PhotonTile", positrons: DeviceTile, cfg: StepConfig,
             out_e: StepOutputs, out_g: StepOutputs, out_p: StepOutputs, drain: "DrainOutputs | None" = None,
-            check: bool = False, push_photons: bool = False) -> None:
+            check: bool = False, push_photons: bool = False, device_drain: int = 0) -> None:
         """``push_photons``: WarpX's photon push for the next step
         (rrea_photon_push) right after the photon pass, on its stream, so it
         runs under the other passes instead of after the drain (nothing
@@ -744,7 +744,18 @@ class ConcurrentApply:
             done = torch.cuda.Event()
             done.record(s)
             main.wait_event(done)
-        if drain is not None:
+        if drain is not None and device_drain > 0:
+            # the drain with its counts on the device, up to device_drain generations
+            outs = (out_e, out_g, out_p)
+            recs = (C.c_void_p * 3)(*[o.secondaries.data_ptr() for o in outs])
+            cnts = (C.c_void_p * 3)(*[o.secondary_count.data_ptr() for o in outs])
+            caps = (C.c_int64 * 3)(*[o.secondary_capacity for o in outs])
+            io = drain.struct(0)
+            c = _step_config(cfg)
+            _lib.check(_lib.lib().rrea_newborn_drain_staged(self.tables.h, self.couplings[0].h, C.byref(c), recs,
+                                                            cnts, caps, 3, C.byref(io), int(device_drain),
+                                                            main.cuda_stream), "rrea_newborn_drain_staged")
+        elif drain is not None:
             apply.drain_secondaries(self.tables, self.couplings[0], cfg, [out_e, out_g, out_p], drain,
                                     stream=main.cuda_stream, check=False)
         if check:

@@ -57,8 +57,9 @@ def test_concurrent_apply_matches_sequential(tables):
         apply.drain_secondaries(tables, coupling, cfg, list(seq_o), seq_d)
         # the concurrent step pushes its photons on the photon stream; the
         # sequential one pushes after its drain
+        # and drains with its counts on the device (rrea_newborn_drain_staged)
         conc.run(con_t[0], con_t[1], con_t[2], cfg, con_o[0], con_o[1], con_o[2], con_d, check=True,
-                 push_photons=True)
+                 push_photons=True, device_drain=16)
         seq_t[1].push(cfg.dt_s)
         torch.cuda.synchronize()
         for a, b in ((seq_t[0], con_t[0]), (seq_t[1].cur, con_t[1].cur), (seq_t[2], con_t[2])):
