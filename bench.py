@@ -228,15 +228,18 @@ def batched_leg(args, tables):
     clear = (grids_g.jr, grids_g.jz, grids_g.rho, grids_g.sources)
     gcfg = StepConfig(rng_seed=0, dt_s=dt)
     t0 = time.perf_counter()
+    # a graph holds at least one step: with no warm-up steps only the timed
+    # graph is captured
     graphs = [BatchGraph(tables, batch_g, {"electron": (dev_g, inst)}, gcfg, record_capacity=drain_b.capacity,
                          clear=clear, steps=k, max_generations=args.graph_generations)
-              for k in (args.batched_warmup, args.batched_steps)]
+              for k in (args.batched_warmup, args.batched_steps) if k > 0]
     capture_s = time.perf_counter() - t0
-    graphs[0].launch(0)
+    if args.batched_warmup > 0:
+        graphs[0].launch(0)
     torch.cuda.synchronize()
     g0 = live(dev_g)
     t0 = time.perf_counter()
-    graphs[1].launch(args.batched_warmup)
+    graphs[-1].launch(args.batched_warmup)
     torch.cuda.synchronize()
     el_g = time.perf_counter() - t0
     g1 = live(dev_g)
@@ -253,11 +256,11 @@ def batched_leg(args, tables):
     launches = max(1, args.graph_loop_steps // args.batched_steps)
     t0 = time.perf_counter()
     for i in range(1, launches):
-        graphs[1].launch(args.batched_warmup + i * args.batched_steps)
+        graphs[-1].launch(args.batched_warmup + i * args.batched_steps)
     torch.cuda.synchronize()
     el_loop = el_g + time.perf_counter() - t0
     g2 = live(dev_g)
-    graphs[1].check_error()
+    graphs[-1].check_error()
     loop_steps = launches * args.batched_steps
     rate_loop = 0.5 * (g0 + g2) * loop_steps / el_loop
     log(f"graph-captured loop: {loop_steps} steps in {el_loop:.3f} s, {rate_loop:.4e} particle-steps/s")
