@@ -173,6 +173,12 @@ def lib():
     return _lib
 
 
+def ref_available() -> bool:
+    """Whether the reference-compiled pins exist or can be built here,
+    without loading them."""
+    return os.path.exists(REF_PATH) or os.path.isdir("/root/reference/native_amrex_rrea")
+
+
 def ref():
     """The reference-compiled pins, or None where the reference is absent."""
     global _ref

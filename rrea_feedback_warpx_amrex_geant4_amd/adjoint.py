@@ -151,7 +151,8 @@ class DriftCurrent(torch.autograd.Function):
         if g_jz is None:
             g_jz = torch.zeros(p.jz_size, dtype=torch.float64, device=er.device)
         g_er, g_ez = p.adjoint(er, ez, ctx.ck, ctx.ck_alive, g_jr.to(torch.float64), g_jz.to(torch.float64))
-        return g_er, g_ez, None
+        # the gradients take the shapes of the fields the caller passed (flat or gridded)
+        return g_er.view_as(er), g_ez.view_as(ez), None
 
 
 def drift_current(er: torch.Tensor, ez: torch.Tensor, problem: DriftProblem):

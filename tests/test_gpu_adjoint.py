@@ -139,6 +139,25 @@ def test_autograd_binding(tables):
     assert abs(float(L) - float(out["loss"].sum())) <= 1e-9 * abs(float(L))
 
 
+def test_autograd_binding_gridded_fields(tables):
+    """Fields passed as (nz, nr + 1) / (nz + 1, nr) grids: the gradients come
+    back in the same shapes, equal to the flat call's."""
+    from rrea_feedback_warpx_amrex_geant4_amd.adjoint import drift_current
+    fields, parts, prob = small_problem(tables, steps=6, every=2, n=128, seed=35)
+    gjr, gjz = loss_weights(prob, 8)
+    nr, nz = fields.nr, fields.nz
+    er = prob.er0.clone().view(nz, nr + 1).requires_grad_(True)
+    ez = prob.ez0.clone().view(nz + 1, nr).requires_grad_(True)
+    jr, jz = drift_current(er, ez, prob)
+    ((jr * gjr.cuda()).sum() + (jz * gjz.cuda()).sum()).backward()
+    assert er.grad.shape == er.shape and ez.grad.shape == ez.shape
+    er1 = prob.er0.clone().requires_grad_(True)
+    ez1 = prob.ez0.clone().requires_grad_(True)
+    jr1, jz1 = drift_current(er1, ez1, prob)
+    ((jr1 * gjr.cuda()).sum() + (jz1 * gjz.cuda()).sum()).backward()
+    assert torch.equal(er.grad.reshape(-1), er1.grad) and torch.equal(ez.grad.reshape(-1), ez1.grad)
+
+
 def test_adjoint_matches_finite_differences(tables):
     """The differentiation configuration: 2^20 electrons, 64 drift steps of
     100 ps, 128 x 512 grid, checkpoints every 8; L = sum(J_z W) with

@@ -22,7 +22,12 @@ def tables(bundle):
     return InteractionTables(bundle)
 
 
-def test_concurrent_apply_matches_sequential(tables):
+# (0, False) is the bench's default: the host-driven drain on the electron
+# pass's handle (whose ordering workspace the electron pass also uses), then
+# the photon push on the main stream after the drain; (16, True) drains with
+# the counts on the device and pushes the photons on the photon stream.
+@pytest.mark.parametrize("device_drain,push_photons", [(0, False), (16, True)])
+def test_concurrent_apply_matches_sequential(tables, device_drain, push_photons):
     from rrea_feedback_warpx_amrex_geant4_amd.transport import (ConcurrentApply, Coupling, DeviceTile, DrainOutputs,
                                                                 PhotonTile, RreaParticleInteraction, StepOutputs)
     fields = wl.c2_fields()
@@ -30,7 +35,8 @@ def test_concurrent_apply_matches_sequential(tables):
     e = wl.c2_electrons(n, fields, seed=2)
     g = wl.c2_photons(n // 2, fields, dt_s=1e-9, seed=4, first_id=1 << 41)
     p = wl.c2_electrons(n // 4, fields, seed=3, first_id=1 << 40, charge_sign=1.0)
-    print(f"case concurrent apply: {n} e-, {n // 2} gamma, {n // 4} e+, 3 steps of 1 ns", flush=True)
+    print(f"case concurrent apply: {n} e-, {n // 2} gamma, {n // 4} e+, 3 steps of 1 ns, "
+          f"device_drain={device_drain}, push_photons={push_photons}", flush=True)
 
     def make():
         tiles = (DeviceTile.from_host(e), PhotonTile.from_host(g), DeviceTile.from_host(p))
@@ -55,11 +61,10 @@ def test_concurrent_apply_matches_sequential(tables):
         apply.apply_photons(tables, coupling, seq_t[1], cfg, seq_o[1])
         apply.apply_positrons(tables, coupling, seq_t[2], cfg, seq_o[2])
         apply.drain_secondaries(tables, coupling, cfg, list(seq_o), seq_d)
-        # the concurrent step pushes its photons on the photon stream; the
-        # sequential one pushes after its drain
-        # and drains with its counts on the device (rrea_newborn_drain_staged)
         conc.run(con_t[0], con_t[1], con_t[2], cfg, con_o[0], con_o[1], con_o[2], con_d, check=True,
-                 push_photons=True, device_drain=16)
+                 push_photons=push_photons, device_drain=device_drain)
+        if not push_photons:
+            con_t[1].push(cfg.dt_s)
         seq_t[1].push(cfg.dt_s)
         torch.cuda.synchronize()
         for a, b in ((seq_t[0], con_t[0]), (seq_t[1].cur, con_t[1].cur), (seq_t[2], con_t[2])):

@@ -35,7 +35,7 @@ def padded_index_current(rng, nr, nz, pad=2, density=0.3):
     return jr, jz
 
 
-needs_ref = pytest.mark.skipif(ob.ref() is None, reason="reference checkout absent")
+needs_ref = pytest.mark.skipif(not ob.ref_available(), reason="reference checkout absent")
 
 
 @needs_ref
