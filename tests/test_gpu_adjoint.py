@@ -141,7 +141,8 @@ def test_autograd_binding(tables):
 
 def test_autograd_binding_gridded_fields(tables):
     """Fields passed as (nz, nr + 1) / (nz + 1, nr) grids: the gradients come
-    back in the same shapes, equal to the flat call's."""
+    back in the same shapes, equal to the flat call's up to the summation
+    order of the atomic scatter."""
     from rrea_feedback_warpx_amrex_geant4_amd.adjoint import drift_current
     fields, parts, prob = small_problem(tables, steps=6, every=2, n=128, seed=35)
     gjr, gjz = loss_weights(prob, 8)
@@ -155,7 +156,9 @@ def test_autograd_binding_gridded_fields(tables):
     ez1 = prob.ez0.clone().requires_grad_(True)
     jr1, jz1 = drift_current(er1, ez1, prob)
     ((jr1 * gjr.cuda()).sum() + (jz1 * gjz.cuda()).sum()).backward()
-    assert torch.equal(er.grad.reshape(-1), er1.grad) and torch.equal(ez.grad.reshape(-1), ez1.grad)
+    # the field adjoints are scattered with FP64 atomics: equal to summation order
+    for a, b in ((er.grad.reshape(-1), er1.grad), (ez.grad.reshape(-1), ez1.grad)):
+        assert torch.allclose(a, b, rtol=0.0, atol=1e-12 * float(b.abs().max()))
 
 
 def test_adjoint_matches_finite_differences(tables):
