@@ -289,6 +289,26 @@ rrea_status rrea_transport_step_photons(rrea_tables_t tables, rrea_fields_t fiel
                                         const rrea_step_config* config, const rrea_step_out* out,
                                         rrea_stream_t stream);
 
+/* The species step without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory during a step (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same step as rrea_transport_step_electrons (species 0),
+ * rrea_transport_step_positrons (1) or rrea_transport_step_photons (2), with
+ * the same arguments, results and error behaviour, except:
+ *   - the call allocates nothing: its workspace (argument block, hand-over
+ *     list, staging and ordering buffers) is one fixed device array per
+ *     device, about 8.5 MiB, held until the process ends;
+ *   - tile->n is at most 1,048,576 and, when ordering, secondary_capacity is
+ *     at most 32,768 (RREA_ERR_ARGUMENT beyond either);
+ *   - a nonzero order_secondaries always orders with the count on the device
+ *     (the order_secondaries = 2 mode); kernel timing and the fast-path signal
+ *     of the fields handle do not apply;
+ *   - every call on a device shares that device's workspace, so calls that
+ *     overlap on one device (other streams, other threads) race. */
+rrea_status rrea_transport_step_noalloc(int32_t species, rrea_tables_t tables, rrea_fields_t fields,
+                                        const rrea_tile* tile, const rrea_step_config* config,
+                                        const rrea_step_out* out, rrea_stream_t stream);
+
 /* The straight-line photon push at c along u / |u| that WarpX applies before
  * the hook (no counterpart in the transport path: a stand-in for the caller's
  * own push so that a photon tile arrives as rrea_transport_step_photons

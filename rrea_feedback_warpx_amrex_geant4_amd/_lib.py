@@ -172,6 +172,8 @@ SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
     ("rrea_transport_step_photons", C.c_int32,
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
+    ("rrea_transport_step_noalloc", C.c_int32,
+     [C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
     ("rrea_photon_push", C.c_int32, [C.POINTER(Tile), C.c_double, C.c_void_p]),
     ("rrea_tile_sort", C.c_int32, [C.c_void_p, C.POINTER(Tile), C.c_int32, C.c_void_p]),
     ("rrea_tile_sort_banded", C.c_int32, [C.c_void_p, C.POINTER(Tile), C.c_int32, C.c_int32, C.c_void_p]),

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -4075,9 +4076,38 @@ struct SpeciesLaunch {
 // A species step's host half: argument checks, the argument block A and the
 // workspace it needs (grown here, before any launch, so that the launches
 // can be captured into a graph).
+// The device buffers a species step works in beyond its inputs and outputs:
+// the staging buffer of the secondaries with its count, the caller's count at
+// the start of the call, the ordering's keys and indices (and hipcub's
+// temporary storage for the host-counted ordering), and the fast path's
+// hand-over list with its count.
+struct StepWork {
+    rrea_secondary* stage = nullptr;
+    int64_t* count = nullptr;
+    int64_t* base = nullptr;
+    uint64_t* keys = nullptr;
+    uint64_t* keys_out = nullptr;
+    int64_t* idx = nullptr;
+    int64_t* idx_out = nullptr;
+    int32_t* slow_list = nullptr;
+    int32_t* slow_count = nullptr;
+};
+
+StepWork work_of(rrea_fields_t W)
+{
+    StepWork K;
+    K.stage = W->d_stage_sec; K.count = W->d_count; K.base = W->d_base;
+    K.keys = W->d_keys; K.keys_out = W->d_keys_out; K.idx = W->d_idx; K.idx_out = W->d_idx_out;
+    K.slow_list = W->d_slow_list; K.slow_count = W->d_slow_count;
+    return K;
+}
+
+// `fixed`: the step works in these buffers, which hold room for the tile and
+// the caller's secondary capacity (checked by the caller); nothing is
+// allocated.  Otherwise the handle's workspace is grown as needed.
 rrea_status species_prepare(int species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
                             const rrea_step_config* cfg, const rrea_step_out* out, const BatchCtx* batch,
-                            StepArgs& A, SpeciesLaunch& L)
+                            StepArgs& A, SpeciesLaunch& L, const StepWork* fixed = nullptr)
 {
     bool const batched = batch != nullptr;
     bool const positron = species == kPositron;
@@ -4129,25 +4159,28 @@ rrea_status species_prepare(int species, rrea_tables_t T, rrea_fields_t F, const
 
     bool const order = out->order_secondaries != 0 && out->secondary_capacity > 0;
     int64_t const cap = out->secondary_capacity;
-    if (order) {
-        // stage into the library's workspace (count from 0), then sort into the
-        // caller's buffer behind its current count, which stays on the device
-        {
+    if (!fixed) {
+        if (order) {
             rrea_status const ws = order_workspace(W, cap);
             if (ws != RREA_OK) return ws;
+            if (!W->d_base) HIP_TRY(hipMalloc(&W->d_base, sizeof(int64_t)));
         }
-        if (!W->d_base) HIP_TRY(hipMalloc(&W->d_base, sizeof(int64_t)));
-        A.sec = W->d_stage_sec;
-        A.sec_count = W->d_count;
+        if (!cfg->complete_path_only) {
+            rrea_status const ls = slow_list_workspace(W, tile->n);
+            if (ls != RREA_OK) return ls;
+        }
+    }
+    StepWork const K = fixed ? *fixed : work_of(W);
+    if (order) {
+        // stage into the workspace (count from 0), then sort into the caller's
+        // buffer behind its current count, which stays on the device
+        A.sec = K.stage;
+        A.sec_count = K.count;
     } else {
         A.sec = out->secondaries;
         A.sec_count = out->secondary_count;
     }
     A.sec_cap = cap;
-    if (!cfg->complete_path_only) {
-        rrea_status const ls = slow_list_workspace(W, tile->n);
-        if (ls != RREA_OK) return ls;
-    }
     L.run = true;
     L.order = order;
     L.complete = cfg->complete_path_only != 0;
@@ -4161,9 +4194,14 @@ rrea_status species_prepare(int species, rrea_tables_t T, rrea_fields_t F, const
 // the device (sort_records_kernel), so the sequence makes no host round trip
 // and can be captured into a graph; otherwise the count is read back once and
 // hipcub sorts exactly that many records.
+// W: the handle whose workspace the launches use, and whose kernel timing and
+// fast-path signal apply; nullptr with `fixed` (no timing, no signal).
 rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunch const& L, const rrea_step_out* out,
-                           hipStream_t s, bool device_order)
+                           hipStream_t s, bool device_order, const StepWork* fixed = nullptr)
 {
+    StepWork const K = fixed ? *fixed : work_of(W);
+    bool const timing = W && W->timing;
+    bool const signal = W && W->fast_signal;
     if (!L.run) return RREA_OK;
     bool const positron = L.species == kPositron;
     bool const photon = L.species == kPhoton;
@@ -4171,19 +4209,19 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
     bool const order = L.order;
     int64_t const cap = L.cap;
     if (order) {
-        copy_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(W->d_base, out->secondary_count);
-        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(W->d_count), 1, 0ULL);
+        copy_i64_kernel<<<dim3(1), dim3(64), 0, s>>>(K.base, out->secondary_count);
+        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(K.count), 1, 0ULL);
         HIP_TRY(hipGetLastError());
     }
     if (!L.complete) {
-        set_i32_kernel<<<dim3(1), dim3(64), 0, s>>>(W->d_slow_count, 0);
+        set_i32_kernel<<<dim3(1), dim3(64), 0, s>>>(K.slow_count, 0);
         HIP_TRY(hipGetLastError());
     }
     int64_t const nb = blocks_for(L.n, kBlock);
     // HIP events around each transport kernel while timing is enabled
     auto timed = [&](int32_t kernel, auto&& launch) -> rrea_status {
         hipEvent_t stop = nullptr;
-        if (W->timing) {
+        if (timing) {
             if (W->events_used == W->events.size()) {
                 hipEvent_t a, b;
                 HIP_TRY(hipEventCreate(&a));
@@ -4208,18 +4246,18 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
         st = timed(4, [&] {
             if (batched) {
                 photon_fast_kernel<true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                    d_args, W->d_slow_list, W->d_slow_count);
+                    d_args, K.slow_list, K.slow_count);
             } else {
                 photon_fast_kernel<false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                    d_args, W->d_slow_list, W->d_slow_count);
+                    d_args, K.slow_list, K.slow_count);
             }
         });
         if (st != RREA_OK) return st;
-        if (W->fast_signal) HIP_TRY(hipEventRecord(W->fast_event, s));
+        if (signal) HIP_TRY(hipEventRecord(W->fast_event, s));
         int64_t const slow_blocks = nb < 2048 ? nb : 2048;
         st = timed(5, [&] {
             photon_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
-                d_args, W->d_slow_list, W->d_slow_count);
+                d_args, K.slow_list, K.slow_count);
         });
     } else if (L.complete) {
         st = timed(kid, [&] {
@@ -4231,31 +4269,31 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
             if (positron) {
                 if (batched) {
                     charged_fast_kernel<true, true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, W->d_slow_list, W->d_slow_count);
+                        d_args, K.slow_list, K.slow_count);
                 } else {
                     charged_fast_kernel<true, false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, W->d_slow_list, W->d_slow_count);
+                        d_args, K.slow_list, K.slow_count);
                 }
             } else {
                 if (batched) {
                     charged_fast_kernel<false, true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, W->d_slow_list, W->d_slow_count);
+                        d_args, K.slow_list, K.slow_count);
                 } else {
                     charged_fast_kernel<false, false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, W->d_slow_list, W->d_slow_count);
+                        d_args, K.slow_list, K.slow_count);
                 }
             }
         });
         if (st != RREA_OK) return st;
-        if (W->fast_signal) HIP_TRY(hipEventRecord(W->fast_event, s));
+        if (signal) HIP_TRY(hipEventRecord(W->fast_event, s));
         int64_t const slow_blocks = nb < 2048 ? nb : 2048;
         st = timed(kid + 1, [&] {
             if (positron) {
                 charged_slow_kernel<true><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
-                    d_args, W->d_slow_list, W->d_slow_count);
+                    d_args, K.slow_list, K.slow_count);
             } else {
                 charged_slow_kernel<false><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
-                    d_args, W->d_slow_list, W->d_slow_count);
+                    d_args, K.slow_list, K.slow_count);
             }
         });
     }
@@ -4263,16 +4301,17 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
 
     if (order && device_order) {
         sort_records_kernel<<<dim3(1), dim3(kSortThreads), 0, s>>>(
-            W->d_stage_sec, W->d_count, nullptr, cap, W->d_keys, W->d_keys_out, reinterpret_cast<int32_t*>(W->d_idx),
-            reinterpret_cast<int32_t*>(W->d_idx) + cap, reinterpret_cast<int32_t*>(W->d_idx_out));
+            K.stage, K.count, nullptr, cap, K.keys, K.keys_out, reinterpret_cast<int32_t*>(K.idx),
+            reinterpret_cast<int32_t*>(K.idx) + cap, reinterpret_cast<int32_t*>(K.idx_out));
         HIP_TRY(hipGetLastError());
         gather_records_kernel<<<dim3(static_cast<unsigned>(std::min<int64_t>(blocks_for(cap, 256), 1024))), dim3(256),
-                                0, s>>>(W->d_stage_sec, reinterpret_cast<const int32_t*>(W->d_idx_out), W->d_count,
-                                        nullptr, cap, W->d_base, cap, out->secondaries, out->error);
+                                0, s>>>(K.stage, reinterpret_cast<const int32_t*>(K.idx_out), K.count,
+                                        nullptr, cap, K.base, cap, out->secondaries, out->error);
         HIP_TRY(hipGetLastError());
-        append_count_dev_kernel<<<dim3(1), dim3(64), 0, s>>>(out->secondary_count, W->d_base, W->d_count);
+        append_count_dev_kernel<<<dim3(1), dim3(64), 0, s>>>(out->secondary_count, K.base, K.count);
         HIP_TRY(hipGetLastError());
     } else if (order) {
+        if (!W || fixed) return RREA_ERR_ARGUMENT;  // the host-counted ordering sorts in the handle's workspace
         // the one host round trip of the call: the sort's item count
         int64_t nstaged = 0;
         HIP_TRY(hipMemcpyAsync(&nstaged, W->d_count, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -4302,6 +4341,85 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     HIP_TRY(hipMemcpyAsync(W->d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
     // order_secondaries 2: the count stays on the device (no host round trip)
     return species_launch(W, static_cast<const StepArgs*>(W->d_args), L, out, s, out->order_secondaries == 2);
+}
+
+// ---------------------------------------------------------------------------
+// The species step without allocation (rrea_transport_step_noalloc).  Its
+// workspace -- the argument block, the counts, the hand-over list and the
+// ordering buffers -- lives in one file-scope device array of fixed size,
+// `scratch`, so the call makes no hipMalloc.  Each device holds its own copy
+// of the array; its address is looked up once per device with
+// hipGetSymbolAddress and only that address is handed to kernels and copies.
+// Every call on a device shares that device's array: calls that overlap on
+// one device race.  The counts that must start at zero are cleared by kernels
+// at the start of each call's launch sequence.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocMaxParticles = int64_t(1) << 20;   // particles per tile
+constexpr int64_t kNoAllocMaxSecondaries = int64_t(1) << 15; // secondary_capacity when ordering
+constexpr size_t kNoAllocMaxDevices = 64;
+
+constexpr size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+constexpr size_t kNaArgs = 0;
+constexpr size_t kNaCount = kNaArgs + round256(sizeof(StepArgs));
+constexpr size_t kNaBase = kNaCount + 256;
+constexpr size_t kNaSlowCount = kNaBase + 256;
+constexpr size_t kNaSlowList = kNaSlowCount + 256;
+constexpr size_t kNaStage = kNaSlowList + round256(kNoAllocMaxParticles * sizeof(int32_t));
+constexpr size_t kNaKeys = kNaStage + round256(kNoAllocMaxSecondaries * sizeof(rrea_secondary));
+constexpr size_t kNaKeysOut = kNaKeys + round256(kNoAllocMaxSecondaries * sizeof(uint64_t));
+constexpr size_t kNaIdx = kNaKeysOut + round256(kNoAllocMaxSecondaries * sizeof(uint64_t));
+constexpr size_t kNaIdxOut = kNaIdx + round256(kNoAllocMaxSecondaries * sizeof(int64_t));
+constexpr size_t kNoAllocBytes = kNaIdxOut + round256(kNoAllocMaxSecondaries * sizeof(int64_t));  // ~8.5 MiB
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// The device address of this device's `scratch`, looked up on the first call
+// on the device (with that device current) and kept; nullptr on failure.
+unsigned char* noalloc_scratch(int dev)
+{
+    static std::mutex mu;
+    static unsigned char* ptr[kNoAllocMaxDevices] = {};
+    if (dev < 0 || static_cast<size_t>(dev) >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return ptr[dev];
+}
+
+rrea_status noalloc_step(int species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+                         const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
+{
+    if (species < kElectron || species > kPhoton || !tile || !out || !cfg) return RREA_ERR_ARGUMENT;
+    if (tile->n > kNoAllocMaxParticles) return RREA_ERR_ARGUMENT;
+    // the ordering always keeps its count on the device (order_secondaries 2)
+    bool const order = out->order_secondaries != 0 && out->secondary_capacity > 0;
+    if (order && out->secondary_capacity > kNoAllocMaxSecondaries) return RREA_ERR_ARGUMENT;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    unsigned char* const base = noalloc_scratch(dev);
+    if (!base) return RREA_ERR_HIP;
+    StepWork K;
+    K.count = reinterpret_cast<int64_t*>(base + kNaCount);
+    K.base = reinterpret_cast<int64_t*>(base + kNaBase);
+    K.slow_count = reinterpret_cast<int32_t*>(base + kNaSlowCount);
+    K.slow_list = reinterpret_cast<int32_t*>(base + kNaSlowList);
+    K.stage = reinterpret_cast<rrea_secondary*>(base + kNaStage);
+    K.keys = reinterpret_cast<uint64_t*>(base + kNaKeys);
+    K.keys_out = reinterpret_cast<uint64_t*>(base + kNaKeysOut);
+    K.idx = reinterpret_cast<int64_t*>(base + kNaIdx);
+    K.idx_out = reinterpret_cast<int64_t*>(base + kNaIdxOut);
+    StepArgs A;
+    SpeciesLaunch L;
+    rrea_status const ps = species_prepare(species, T, F, tile, cfg, out, nullptr, A, L, &K);
+    if (ps != RREA_OK || !L.run) return ps;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    StepArgs* const d_args = reinterpret_cast<StepArgs*>(base + kNaArgs);
+    // pageable source: the copy has consumed A when hipMemcpyAsync returns
+    HIP_TRY(hipMemcpyAsync(d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
+    return species_launch(nullptr, d_args, L, out, s, true, &K);
 }
 
 
@@ -4889,6 +5007,12 @@ rrea_status rrea_transport_step_photons(rrea_tables_t T, rrea_fields_t F, const 
                                         const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
 {
     return species_step(kPhoton, T, F, tile, cfg, out, stream);
+}
+
+rrea_status rrea_transport_step_noalloc(int32_t species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,
+                                        const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
+{
+    return noalloc_step(species, T, F, tile, cfg, out, stream);
 }
 
 rrea_status rrea_newborn_drain(rrea_tables_t T, rrea_fields_t F, const rrea_step_config* cfg, const rrea_drain_io* io,

@@ -81,3 +81,4 @@ def test_argument_checks_need_no_device():
     h = C.c_void_p()
     assert lib.rrea_fields_create(C.byref(bad), C.byref(h)) == 16
     assert lib.rrea_transport_step_electrons(None, None, None, None, None, None) == 16
+    assert lib.rrea_transport_step_noalloc(0, None, None, None, None, None, None) == 16
