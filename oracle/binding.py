@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+# RREA_ORACLE_LIB names another build of the oracle (the planted-mistake
+# checks load deliberately broken copies through it)
+LIB_PATH = os.environ.get("RREA_ORACLE_LIB") or os.path.join(HERE, "build", "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref.so")
 
 P_D = C.POINTER(C.c_double)
@@ -68,6 +70,16 @@ RATE_KEYS = ["e_col", "e_soft", "e_moller", "e_brems", "e_eltr",
 CDF_KEYS = ["moller", "bhabha", "ebrems", "pbrems"]
 ANGLE_KEYS = ["gs_e", "gs_p", "wz_e", "wz_p"]
 N_TALLIES = 24
+
+
+class InterleaveRecord(C.Structure):
+    """orc_interleave_record: what the event interleave handed its policies."""
+    _fields_ = [("n_frozen", C.c_int32), ("n_segments", C.c_int32), ("n_draws", C.c_int32), ("n_open", C.c_int32),
+                ("n_final", C.c_int32), ("n_stops", C.c_int32),
+                ("frozen", C.c_double * 16), ("taus", C.c_double * 16),
+                ("seg_begin", C.c_double * 16), ("seg_end", C.c_double * 16), ("seg_frozen", C.c_double * 16),
+                ("draw_ordinals", C.c_uint64 * 16), ("open_energies", C.c_double * 16),
+                ("final_fractions", C.c_double * 16), ("stops", C.c_double * 16), ("kinetic_end", C.c_double)]
 
 
 def build(quiet: bool = True) -> None:
@@ -169,6 +181,43 @@ def lib():
             f = getattr(L, name)
             f.restype = C.c_double
             f.argtypes = [C.c_void_p, C.c_double, C.c_double, P_I32]
+        # probes of the step's primitives
+        P_U64 = C.POINTER(C.c_uint64)
+        L.orc_unit_cycle.restype = C.c_double
+        L.orc_unit_cycle.argtypes = [C.c_double]
+        L.orc_normalize.argtypes = [P_D]
+        L.orc_boris_rotate.argtypes = [P_D, C.c_double, P_D]
+        L.orc_cone_direction.argtypes = [P_D, C.c_double, C.c_double, P_D]
+        L.orc_moller_cos.restype = C.c_double
+        L.orc_moller_cos.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.orc_moller_lab_directions.argtypes = [P_D, C.c_double, C.c_double, C.c_double, P_D, P_D]
+        L.orc_photon_recoil.argtypes = [C.c_double, C.c_double, P_D, P_D]
+        L.orc_rotate_to_axis.argtypes = [P_D, P_D, P_D]
+        L.orc_poisson_capped.restype = C.c_int
+        L.orc_poisson_capped.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.orc_select_hazard.argtypes = [P_D, C.c_int, C.c_double, C.c_double, P_D]
+        L.orc_half_kick.restype = C.c_double
+        L.orc_half_kick.argtypes = [P_D, C.c_double, C.c_double, P_D, P_D]
+        L.orc_gated_inverse_length.restype = C.c_double
+        L.orc_gated_inverse_length.argtypes = [C.c_int, C.c_double, P_I32]
+        L.orc_stop_is_escape.restype = C.c_int
+        L.orc_stop_is_escape.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.orc_limit_at_cutoff_linear.restype = C.c_int
+        L.orc_limit_at_cutoff_linear.argtypes = [C.c_double] * 5 + [P_D]
+        L.orc_partition_ion_pairs.restype = C.c_int
+        L.orc_partition_ion_pairs.argtypes = [P_D, P_D, P_D, C.c_int] + [C.c_double] * 8 + [P_D, P_D, C.c_int]
+        L.orc_interleave_probe.restype = C.c_int
+        L.orc_interleave_probe.argtypes = [C.c_double] * 3 + [P_D, C.c_int, P_D, C.c_int, C.c_int,
+                                                              C.POINTER(InterleaveRecord)]
+        L.orc_substep_probe.restype = C.c_int
+        L.orc_substep_probe.argtypes = [C.c_uint64, C.c_double, C.c_int, C.c_uint64, C.c_uint64, C.c_int, P_I32,
+                                        C.c_int, P_U64, P_I32]
+        L.orc_boundary_crossing.restype = C.c_int
+        L.orc_boundary_crossing.argtypes = [C.c_double] * 10 + [P_D]
+        L.orc_advance_running_position.restype = C.c_int
+        L.orc_advance_running_position.argtypes = [C.c_double] * 11 + [P_D]
+        L.orc_outside_certified.restype = C.c_int
+        L.orc_outside_certified.argtypes = [C.c_double] * 6
         _lib = L
     return _lib
 

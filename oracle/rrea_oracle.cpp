@@ -34,6 +34,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #ifdef _OPENMP
@@ -1662,51 +1663,50 @@ inline void spawn_annihilation(StepContext const& ctx, Emitter& em, int32_t proc
 }
 
 // mean_free_path_from_inverse_length NA/src/RreaInteractionTables.cpp:110-119
-// followed by RreaChannelInverseLength OV/RreaChargedInterleave.H:69-78
-inline double channel_inverse_length(double inverse_per_m)
+inline double mfp_from_inverse(double inverse_per_m)
 {
     if (!std::isfinite(inverse_per_m) || inverse_per_m < 0.0) fail(ORC_ERR_INVALID, "inverse length invalid");
-    double const mfp = inverse_per_m > 0.0 ? 1.0 / inverse_per_m : 1.0e99;
+    return inverse_per_m > 0.0 ? 1.0 / inverse_per_m : 1.0e99;
+}
+
+// RreaChannelInverseLength OV/RreaChargedInterleave.H:69-78: a closed channel
+// is exactly zero and never evaluates its mean free path (the gate keeps the
+// lookup inside the channel's energy range); a non-positive mean free path
+// is a table hole, zero.
+template <class Mfp>
+inline double gated_inverse_length(bool open, Mfp&& mfp_m)
+{
+    if (!open) return 0.0;
+    double const mfp = mfp_m();
     return mfp > 0.0 ? 1.0 / mfp : 0.0;
 }
 
-// RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:107-152 (electron)
+// RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:107-152 (electron):
+// hard Moller (ElectronHardMollerMeanFreePath :2394-2412, open above 2 T_cut)
+// and tracked brems (ElectronBremsstrahlungMeanFreePath :2666-2676, open
+// above the photon cutoff).
 inline void electron_stp_rates(Tables const& t, double thr, double gcut, double e, bool hm_on, double out[3])
 {
     bool const hm_open = hm_on && thr > 0.0 && e > 2.0 * thr;
-    double hm = 0.0;
-    if (hm_open) {
-        // ElectronHardMollerMeanFreePath :2394-2412 (its K <= 2 T_cut branch is
-        // behind the closed gate)
-        hm = channel_inverse_length(t.rate[kEMoller].interpolate(e) * 1.0);
-    }
-    double br = 0.0;
-    if (std::isfinite(e) && std::isfinite(gcut) && e > gcut) {
-        // ElectronBremsstrahlungMeanFreePath :2666-2676
-        br = channel_inverse_length(t.rate[kEBrems].interpolate(e) * 1.0);
-    }
+    bool const br_open = std::isfinite(e) && std::isfinite(gcut) && e > gcut;
+    double const hm = gated_inverse_length(hm_open, [&] { return mfp_from_inverse(t.rate[kEMoller].interpolate(e) * 1.0); });
+    double const br = gated_inverse_length(br_open, [&] { return mfp_from_inverse(t.rate[kEBrems].interpolate(e) * 1.0); });
     out[0] = frozen(hm);
     out[1] = frozen(br);
     out[2] = 0.0;
 }
 
 // RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:114-133 (positron):
-// hard Bhabha (open unless K <= T_cut), tracked brems (K > photon cutoff),
-// in-flight annihilation (always open).
+// hard Bhabha (PositronHardBhabhaMeanFreePath :3163-3178, open unless
+// K <= T_cut), tracked brems (PositronBremsstrahlungMeanFreePath :3151-3161,
+// K > photon cutoff), in-flight annihilation
+// (PositronAnnihilationMeanFreePath :3196-3206, always open).
 inline void positron_stp_rates(Tables const& t, double thr, double gcut, double e, double out[3])
 {
-    double bh = 0.0;
-    if (!(thr > 0.0 && e <= thr)) {
-        // PositronHardBhabhaMeanFreePath :3163-3178
-        bh = channel_inverse_length(t.rate[kPBhabha].interpolate(e) * 1.0);
-    }
-    double br = 0.0;
-    if (e > gcut) {
-        // PositronBremsstrahlungMeanFreePath :3151-3161
-        br = channel_inverse_length(t.rate[kPBrems].interpolate(e) * 1.0);
-    }
-    // PositronAnnihilationMeanFreePath :3196-3206
-    double const an = channel_inverse_length(t.rate[kPAnnih].interpolate(e) * 1.0);
+    double const bh = gated_inverse_length(!(thr > 0.0 && e <= thr),
+                                           [&] { return mfp_from_inverse(t.rate[kPBhabha].interpolate(e) * 1.0); });
+    double const br = gated_inverse_length(e > gcut, [&] { return mfp_from_inverse(t.rate[kPBrems].interpolate(e) * 1.0); });
+    double const an = gated_inverse_length(true, [&] { return mfp_from_inverse(t.rate[kPAnnih].interpolate(e) * 1.0); });
     out[0] = frozen(bh);
     out[1] = frozen(br);
     out[2] = frozen(an);
@@ -1762,6 +1762,166 @@ inline double ensemble_drag(Tables const& t, double e, double ratio, bool hm_on)
     if (!(e > 0.0) || !(ratio > 0.0)) return 0.0;
     double const dd = hm_on ? t.hard_moller_energy_loss(e, ratio) : 0.0;
     return realized_loss(t, e, ratio) + dd + t.tracked_brems_energy_loss(e, ratio);
+}
+
+// ---------------------------------------------------------------------------
+// The charged interleave core shared by the live and the newborn loops
+// (OV/RreaChargedInterleave.H, OV/RreaParticleInteraction.H).  The policies
+// are callables, so the exported probes (orc_interleave_probe,
+// orc_substep_probe, orc_partition_ion_pairs, orc_limit_at_cutoff) run the
+// very code the steps run on the reference's own behavioural cases.
+// ---------------------------------------------------------------------------
+struct SegLimit { double end; bool cutoff; };
+
+// RreaChargedStopIsEscape OV/RreaChargedInterleave.H:140-146: a stop at or past
+// the clip root of an exiting chord is an escape.
+inline bool stop_is_escape(bool path_exits, double f, double end_f) { return path_exits && f >= end_f; }
+
+// RreaLimitContinuousSegmentAtCutoff OV/RreaParticleInteraction.H:203-265:
+// where the loss accumulated from b, loss_to(f), first reaches k - cutoff
+// (56 bisection steps), or e when it does not.
+template <class LossTo>
+inline SegLimit limit_at_cutoff(double b, double e, double k, double cutoff, LossTo&& loss_to)
+{
+    if (!std::isfinite(b) || !std::isfinite(e) || !std::isfinite(k) || !std::isfinite(cutoff) || !(e >= b)
+        || !(k >= cutoff) || !(cutoff >= 0.0)) {
+        fail(ORC_ERR_INVALID, "continuous stopping could not locate a cutoff point");
+    }
+    if (e == b) return {b, k == cutoff};
+    double const to_cut = k - cutoff;
+    double const req = loss_to(e);
+    if (!std::isfinite(req) || req < 0.0) fail(ORC_ERR_INVALID, "continuous stopping invalid");
+    if (req < to_cut) return {e, false};
+    if (to_cut == 0.0) return {b, true};
+    double lo = b, hi = e;
+    for (int it = 0; it < 56; ++it) {
+        double const mid = 0.5 * (lo + hi);
+        double const l = loss_to(mid);
+        if (!std::isfinite(l) || l < 0.0) fail(ORC_ERR_INVALID, "continuous stopping invalid");
+        if (l < to_cut) lo = mid; else hi = mid;
+    }
+    return {0.5 * (lo + hi), true};
+}
+
+// RreaPartitionContinuousIonPairs OV/RreaChargedInterleave.H:157-196: the
+// segment's collision loss prorated over the material intervals it overlaps,
+// each share emitted as ion pairs at the overlap's midpoint fraction.
+template <class Emit>
+inline void partition_ion_pairs(std::vector<Interval> const& iv, double full_length, double b, double e,
+                                double segment_loss, double raw_loss, double w, double w_air, double stp, Emit&& emit)
+{
+    if (!(segment_loss > 0.0) || !(raw_loss > 0.0)) return;
+    for (auto const& it : iv) {
+        double const lb = std::max(b, it.begin), le = std::min(e, it.end);
+        if (!(le > lb)) continue;
+        double const raw_i = full_length * (le - lb) * it.ratio * stp;
+        double const loss_i = segment_loss * raw_i / raw_loss;
+        double const pw = w * loss_i / std::max(w_air, 1.0);
+        if (!(pw > 0.0)) continue;
+        emit(0.5 * (lb + le), pw);
+    }
+}
+
+// RreaPlanContinuousDragSegment OV/RreaChargedInterleave.H:239-292 and its
+// application in apply_electron_continuous_segment
+// (OV/RreaElectronTransport.cpp:294-400) / the positron and newborn
+// equivalents: collision + soft radiative stopping frozen at stop_e over
+// [b, e], limited at the cutoff, its collision share emitted as ion pairs;
+// k is reduced by the drag (set to the cutoff when it is reached).
+template <class Emit>
+inline SegLimit continuous_drag(Tables const& t, bool pos, Path const& path, std::vector<Interval> const& iv,
+                                double b, double e, double stop_e, double cutoff, double w, double& k, double& drag_out,
+                                Emit&& emit)
+{
+    drag_out = 0.0;
+    if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return {b, false};
+    double const col = frozen(t.rate[pos ? kPCol : kECol].interpolate(stop_e) * 1.0);
+    double const soft = frozen(t.rate[pos ? kPSoft : kESoft].interpolate(stop_e) * 1.0);
+    double const total = col + soft;
+    SegLimit const lim = limit_at_cutoff(b, e, k, cutoff, [&](double f) { return total * density_path(iv, path, b, f); });
+    double const pm = density_path(iv, path, b, lim.end);
+    double const raw_col = col * pm;
+    double const raw_soft = soft * pm;
+    double const avail = std::max(0.0, k - cutoff);
+    double const drag = lim.cutoff ? avail : std::min(raw_col + raw_soft, avail);
+    double const col_loss = std::min(raw_col, drag);
+    partition_ion_pairs(iv, path.full_length, b, lim.end, col_loss, raw_col, w, kWAir, col, emit);
+    k = std::max(0.0, k - drag);
+    if (lim.cutoff) k = cutoff;
+    drag_out = drag;
+    return lim;
+}
+
+// RreaRunChargedEventInterleave OV/RreaChargedInterleave.H:546-664 with
+// RreaResolveRemainingSegmentHazard OV/RreaParticleInteraction.H:1467-1492.
+// Per iteration: rates refrozen at the survivor energy, one competing-hazard
+// draw; no event: the continuous tail to se; an event: continuous loss to its
+// exact depth, the channel revalidated at the post-drag energy (a channel
+// closed by the drag is a null event with a new ordinal), the final state
+// (false: the parent was consumed).
+template <class Taus, class Draws, class DepthToFraction, class Continuous, class MarkStopped, class EventPoint,
+          class ChannelOpen, class FinalState>
+inline void run_event_interleave(double sb, double se, double const& k, double cutoff, int nch, uint64_t max_ordinal,
+                                 Taus&& taus_at, Draws&& draws, DepthToFraction&& depth_to_fraction,
+                                 Continuous&& continuous, MarkStopped&& mark_stopped, EventPoint&& event_point,
+                                 ChannelOpen&& channel_open, FinalState&& final_state)
+{
+    double rem = sb;
+    uint64_t ord = 0;
+    while (rem < se && k >= cutoff) {
+        if (ord >= max_ordinal) fail(ORC_ERR_INVALID, "event ordinal overflow");
+        double const frozen_e = k;
+        std::array<double, 3> const taus = taus_at(frozen_e, rem, se);
+        std::pair<double, double> const u = draws(ord);
+        Hazard const h = select_hazard(taus.data(), nch, u.first, u.second);
+        if (!h.valid) fail(ORC_ERR_INVALID, "competing-hazard selection invalid");
+        if (!h.event) {
+            SegLimit const sl = continuous(rem, se, frozen_e);
+            if (sl.cutoff) mark_stopped(sl.end);
+            break;
+        }
+        double const ef = depth_to_fraction(h.sampled, rem, se);
+        if (!(std::isfinite(rem) && std::isfinite(ef) && std::isfinite(se) && se > rem && ef >= rem && ef <= se)) {
+            fail(ORC_ERR_INVALID, "sampled event depth outside its subcycle");
+        }
+        event_point(ef);
+        SegLimit const sl = continuous(rem, ef, frozen_e);
+        if (sl.cutoff) { mark_stopped(sl.end); break; }
+        bool const open_ev = k >= cutoff && channel_open(h.channel, k);
+        if (!open_ev) {
+            bool const open_fr = frozen_e >= cutoff && channel_open(h.channel, frozen_e);
+            if (!open_fr) fail(ORC_ERR_INVALID, "selected channel closed before its sampled depth");
+            rem = ef;
+            ++ord;
+            continue;
+        }
+        if (!final_state(h.channel, ef)) return;
+        if (k <= cutoff) { mark_stopped(ef); break; }
+        rem = ef;
+        ++ord;
+    }
+}
+
+// RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:686-732: per
+// substep build -> events -> finish -> advance; a stop skips finish and the
+// commit; a refused endpoint or an exiting chord sets `exited` and ends the
+// loop; `completed` counts only committed substeps.
+template <class Build, class Events, class Stopped, class Finish, class Advance, class Exits>
+inline void run_substep_interleave(uint64_t subcycles, double const& k, double cutoff, uint64_t& completed,
+                                   bool& exited, Build&& build, Events&& events, Stopped&& stopped, Finish&& finish,
+                                   Advance&& advance, Exits&& exits)
+{
+    for (uint64_t sc = 0; sc < subcycles; ++sc) {
+        if (k < cutoff) break;
+        double sb = 0.0, se = 0.0;
+        if (!build(sc, sb, se)) break;
+        events(sc, sb, se);
+        if (stopped()) break;
+        if (finish(sc, sb, se)) break;
+        if (!advance(se)) { exited = true; break; }
+        if (exits()) { exited = true; break; }
+        completed = sc + 1U;
+    }
 }
 
 // One charged particle: the body of the particle loop of
@@ -1928,7 +2088,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
         return std::min(1.0, (static_cast<double>(completed) + within) / static_cast<double>(subcycles));
     };
     auto mark_stopped = [&](double f) {
-        if (path.exits && f >= path.end_f) { exited = true; return; }
+        if (stop_is_escape(path.exits, f, path.end_f)) { exited = true; return; }
         double x, y, z;
         path.point_at_full(f, x, y, z);
         double const r = std::hypot(x, y);
@@ -1940,139 +2100,77 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
     // apply_electron_continuous_segment OV/RreaElectronTransport.cpp:294-400 /
     // apply_positron_continuous_segment OV/RreaPositronTransport.cpp:278-360:
     // collision + soft radiative stopping frozen at the segment's energy.
-    struct SegLimit { double end; bool cutoff; };
     auto continuous = [&](double b, double e, double stop_e) -> SegLimit {
-        if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return {b, false};
-        double const col = frozen(t.rate[pos ? kPCol : kECol].interpolate(stop_e) * 1.0);
-        double const soft = frozen(t.rate[pos ? kPSoft : kESoft].interpolate(stop_e) * 1.0);
-        double const total = col + soft;
-        // RreaLimitContinuousSegmentAtCutoff OV/RreaParticleInteraction.H:203-265
-        auto loss_to = [&](double f) { return total * density_path(iv, path, b, f); };
-        double eff = b;
-        bool reaches = false;
-        {
-            if (!std::isfinite(b) || !std::isfinite(e) || !std::isfinite(k) || !std::isfinite(cutoff) || !(e >= b)
-                || !(k >= cutoff) || !(cutoff >= 0.0)) {
-                fail(ORC_ERR_INVALID, "continuous stopping could not locate a cutoff point");
-            }
-            double const to_cut = k - cutoff;
-            double const req = loss_to(e);
-            if (!std::isfinite(req) || req < 0.0) fail(ORC_ERR_INVALID, "continuous stopping invalid");
-            if (req < to_cut) {
-                eff = e;
-            } else {
-                reaches = true;
-                if (to_cut != 0.0) {
-                    double lo = b, hi = e;
-                    for (int it = 0; it < 56; ++it) {
-                        double const mid = 0.5 * (lo + hi);
-                        double const l = loss_to(mid);
-                        if (!std::isfinite(l) || l < 0.0) fail(ORC_ERR_INVALID, "continuous stopping invalid");
-                        if (l < to_cut) lo = mid; else hi = mid;
-                    }
-                    eff = 0.5 * (lo + hi);
-                }
-            }
-        }
-        double const pm = density_path(iv, path, b, eff);
-        double const raw_col = col * pm;
-        double const raw_soft = soft * pm;
-        double const avail = std::max(0.0, k - cutoff);
-        double const drag = reaches ? avail : std::min(raw_col + raw_soft, avail);
-        double const col_loss = std::min(raw_col, drag);
-        // RreaPartitionContinuousIonPairs OV/RreaChargedInterleave.H:157-196
-        if (col_loss > 0.0 && raw_col > 0.0) {
-            for (auto const& it : iv) {
-                double const lb = std::max(b, it.begin), le = std::min(eff, it.end);
-                if (!(le > lb)) continue;
-                double const raw_i = path.full_length * (le - lb) * it.ratio * col;
-                double const loss_i = col_loss * raw_i / raw_col;
-                double const pw = w * loss_i / std::max(kWAir, 1.0);
-                if (!(pw > 0.0)) continue;
-                ion_event(pw);
-                double mx, my, mz;
-                path.point_at_full(0.5 * (lb + le), mx, my, mz);
-                em.ionization(std::hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
-            }
-        }
+        double drag = 0.0;
+        SegLimit const sl = continuous_drag(t, pos, path, iv, b, e, stop_e, cutoff, w, k, drag,
+                                            [&](double mid_f, double pw) {
+                                                ion_event(pw);
+                                                double mx, my, mz;
+                                                path.point_at_full(mid_f, mx, my, mz);
+                                                em.ionization(std::hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
+                                            });
         fx.tallies[ORC_T_DRAG_LOSS_WEV] += w * drag;
-        k = std::max(0.0, k - drag);
-        if (reaches) k = cutoff;
-        return {eff, reaches};
+        return sl;
     };
 
     double const sub_dt = dt / static_cast<double>(subcycles);
     FieldRange sfield;
     double substep_dt = 0.0;
-    double ex_ = 0.0, ey_ = 0.0, ez_ = 0.0, er_ = 0.0;
-    uint64_t idx = 0;
-    double stp[3] = {0, 0, 0};
     int const nch = pos ? 3 : 2;
     uint64_t const ch_depth0 = pos ? kChPositronCompetingOpticalDepth : kChCompetingOpticalDepth;
     uint64_t const ch_pick0 = pos ? kChPositronCompetingChannel : kChCompetingChannel;
     uint64_t const ch_depth1 = pos ? kChPositronContinuationOpticalDepth : kChContinuationOpticalDepth;
     uint64_t const ch_pick1 = pos ? kChPositronContinuationChannel : kChContinuationChannel;
 
-    // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:686-732
-    for (uint64_t sc = 0; sc < subcycles; ++sc) {
-        if (k < cutoff) break;
-        double sb = 0.0, se = 0.0;
-        // build_substep
-        {
-            Chord const ch = build_chord(run_x, run_y, run_z, dx, dy, dz, k, sub_dt, dt, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
-            if (!ch.valid) break;
-            path = ch.path;
-            if (!(path.full_length > 0.0) || !(path.in_length > 0.0)) { exited = path.exits; break; }
-            iv = build_intervals(ctx.cp, path, ctx.density_ratio);
-            if (iv.empty()) fail(ORC_ERR_INVALID, "microchord without material intervals");
-            for (auto const& it : iv) { tr_dp += it.length * it.ratio; tr_len += it.length; }
-            sb = path.begin_f;
-            se = path.end_f;
-            sfield = path_field_work(ctx.cp, path, charge, sb, se);
-            substep_dt = sub_dt * (se - sb);
-            fx.tallies[ORC_T_FIELD_WORK_WEV] += w * half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
-            if (k < cutoff) { mark_stopped(sb); break; }
-        }
-        // run_events via RreaRunChargedEventInterleave :546-664
-        {
-            double rem = sb;
-            uint64_t ord = 0;
-            while (rem < se && k >= cutoff) {
-                if (ord >= kEventCap) fail(ORC_ERR_INVALID, "event ordinal overflow");
-                double const frozen_e = k;
+    // build_substep: the microchord from the running point, its material
+    // intervals and field sub-range, the leading half kick
+    auto build_substep = [&](uint64_t, double& sb, double& se) -> bool {
+        Chord const ch = build_chord(run_x, run_y, run_z, dx, dy, dz, k, sub_dt, dt, ctx.r_lo, ctx.r_hi, ctx.z_lo, ctx.z_hi);
+        if (!ch.valid) return false;
+        path = ch.path;
+        if (!(path.full_length > 0.0) || !(path.in_length > 0.0)) { exited = path.exits; return false; }
+        iv = build_intervals(ctx.cp, path, ctx.density_ratio);
+        if (iv.empty()) fail(ORC_ERR_INVALID, "microchord without material intervals");
+        for (auto const& it : iv) { tr_dp += it.length * it.ratio; tr_len += it.length; }
+        sb = path.begin_f;
+        se = path.end_f;
+        sfield = path_field_work(ctx.cp, path, charge, sb, se);
+        substep_dt = sub_dt * (se - sb);
+        fx.tallies[ORC_T_FIELD_WORK_WEV] += w * half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
+        if (k < cutoff) { mark_stopped(sb); return false; }
+        return true;
+    };
+    // run_events: the competing hazards of the substep
+    auto run_events = [&](uint64_t sc, double sb, double se) {
+        uint64_t idx = 0;
+        double stp[3] = {0, 0, 0};
+        double ex_ = 0.0, ey_ = 0.0, ez_ = 0.0, er_ = 0.0;
+        run_event_interleave(
+            sb, se, k, cutoff, nch, kEventCap,
+            [&](double frozen_e, double rem, double end) {
                 if (pos) positron_stp_rates(t, thr, gcut, frozen_e, stp);
                 else electron_stp_rates(t, thr, gcut, frozen_e, hm_on, stp);
-                double const L = density_path(iv, path, rem, se);
-                double const taus[3] = {stp[0] * L, stp[1] * L, stp[2] * L};
+                double const L = density_path(iv, path, rem, end);
+                return std::array<double, 3>{stp[0] * L, stp[1] * L, stp[2] * L};
+            },
+            [&](uint64_t ord) {
                 if (!live_hazard_index(species, sc, ord, idx)) fail(ORC_ERR_INVALID, "hazard RNG index overflow");
-                double const u1 = rng_uniform(seed, pkey, step, idx, ord == 0U ? ch_depth0 : ch_depth1);
-                double const u2 = rng_uniform(seed, pkey, step, idx, ord == 0U ? ch_pick0 : ch_pick1);
-                Hazard const h = select_hazard(taus, nch, u1, u2);
-                if (!h.valid) fail(ORC_ERR_INVALID, "competing-hazard selection invalid");
-                if (!h.event) {
-                    SegLimit const sl = continuous(rem, se, frozen_e);
-                    if (sl.cutoff) mark_stopped(sl.end);
-                    break;
-                }
+                return std::pair<double, double>{rng_uniform(seed, pkey, step, idx, ord == 0U ? ch_depth0 : ch_depth1),
+                                                 rng_uniform(seed, pkey, step, idx, ord == 0U ? ch_pick0 : ch_pick1)};
+            },
+            [&](double tau, double rem, double end) {
                 double const total_stp = pos ? stp[0] + stp[1] + stp[2] : stp[0] + stp[1];
-                double const ef = fraction_at_density_path(iv, path, rem, se, h.sampled / total_stp);
-                if (!(std::isfinite(rem) && std::isfinite(ef) && std::isfinite(se) && se > rem && ef >= rem && ef <= se)) {
-                    fail(ORC_ERR_INVALID, "sampled event depth outside its subcycle");
-                }
+                return fraction_at_density_path(iv, path, rem, end, tau / total_stp);
+            },
+            continuous, mark_stopped,
+            [&](double ef) {
                 path.point_at_full(ef, ex_, ey_, ez_);
                 er_ = std::hypot(ex_, ey_);
-                SegLimit const sl = continuous(rem, ef, frozen_e);
-                if (sl.cutoff) { mark_stopped(sl.end); break; }
-                bool const open_ev = k >= cutoff && charged_channel_open(t, pos, thr, gcut, h.channel, k, hm_on);
-                if (!open_ev) {
-                    bool const open_fr = frozen_e >= cutoff && charged_channel_open(t, pos, thr, gcut, h.channel, frozen_e, hm_on);
-                    if (!open_fr) fail(ORC_ERR_INVALID, "selected channel closed before its sampled depth");
-                    rem = ef;
-                    ++ord;
-                    continue;
-                }
+            },
+            [&](int channel, double e) { return charged_channel_open(t, pos, thr, gcut, channel, e, hm_on); },
+            [&](int channel, double ef) -> bool {
                 double const efrac = elapsed_at(ef);
-                if (!pos && h.channel == 0) {
+                if (!pos && channel == 0) {
                     // RreaApplyHardMollerFinalState OV/RreaChargedFinalStates.H:164-225
                     double const before = k;
                     double T = t.sample_moller(before, rng_uniform(seed, pkey, step, idx, kChHardMollerEnergy));
@@ -2086,7 +2184,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                     ion_event(w);
                     em.ionization(std::hypot(ex_, ey_), ez_, w, T, T);
                     fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
-                } else if (pos && h.channel == 0) {
+                } else if (pos && channel == 0) {
                     // RreaApplyBhabhaFinalState OV/RreaChargedFinalStates.H:238-297: a
                     // secondary below the production threshold discards this event only
                     double const before = k;
@@ -2104,7 +2202,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                         fx.tallies[ORC_T_POSITIVE_IONS] += w;
                         fx.tallies[ORC_T_BHABHA_EVENTS] += 1.0;
                     }
-                } else if (h.channel == 1) {
+                } else if (channel == 1) {
                     // RreaApplyBremsFinalState OV/RreaChargedFinalStates.H:329-394
                     double const before = k;
                     double const eu = rng_uniform(seed, pkey, step, idx, pos ? kChPositronBremsPhotonEnergy : kChBremsPhotonEnergy);
@@ -2129,57 +2227,58 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                         deposit_local(local, std::hypot(ex_, ey_), ez_);
                     }
                     fx.tallies[ORC_T_BREMS_EVENTS] += 1.0;
-                } else if (pos && h.channel == 2) {
+                } else if (pos && channel == 2) {
                     // in-flight annihilation, OV/RreaPositronTransport.cpp:710-745: the
                     // parent is consumed at the event point
                     annihilate(ex_, ey_, ez_, std::hypot(ex_, ey_), false, idx, efrac);
                     segment(r_particle, z_particle, er_, ez_);
                     annihilated = true;
-                    break;
+                    return false;
                 }
-                if (k <= cutoff) { mark_stopped(ef); break; }
-                rem = ef;
-                ++ord;
-            }
-        }
-        if (terminated || annihilated) break;
-        // finish_interleaved_substep
-        {
-            if (k >= cutoff) {
-                // apply_gs_elastic_scatter OV/RreaTransportStep.H:372-423
-                double const gs_n = elastic_inverse_length(t, pos, k) * density_path(iv, path, sb, se);
-                if (gs_n > 0.0) {
-                    auto U = [&](uint64_t chan, uint64_t inner) {
-                        return rng_uniform(seed, pkey, step, (sc << 8) | inner, chan);
-                    };
-                    auto deflect = [&](double ct, double uphi) {
-                        double nx, ny, nz;
-                        cone_direction(dx, dy, dz, ct, uphi, nx, ny, nz);
-                        dx = nx; dy = ny; dz = nz;
-                    };
-                    if (gs_n < t.gs_few) {
-                        int const cap = static_cast<int>(std::ceil(3.0 * t.gs_few));
-                        if (cap >= 255) fail(ORC_ERR_INVALID, "GS few-collision cap does not pack");
-                        int const cnt = poisson_capped(gs_n, U(pos ? kChPositronGsFewCount : kChGsFewCount, 0), cap);
-                        for (int q = 0; q < cnt; ++q) {
-                            uint64_t const inner = static_cast<uint64_t>(q + 1);
-                            double const ct = t.sample_single_elastic_cos(
-                                pos, k, U(pos ? kChPositronGsFewScatter : kChGsFewScatter, inner));
-                            deflect(ct, U(pos ? kChPositronGsFewAzimuth : kChGsFewAzimuth, inner));
-                        }
-                    } else {
-                        double const ct = t.sample_gs_cos(pos, k, gs_n, U(pos ? kChPositronGsAngle : kChGsAngle, 0));
-                        deflect(ct, U(pos ? kChPositronGsAzimuth : kChGsAzimuth, 0));
+                return true;
+            });
+    };
+    // finish_interleaved_substep: elastic scattering, then the trailing half kick
+    auto finish_substep = [&](uint64_t sc, double sb, double se) -> bool {
+        if (k >= cutoff) {
+            // apply_gs_elastic_scatter OV/RreaTransportStep.H:372-423
+            double const gs_n = elastic_inverse_length(t, pos, k) * density_path(iv, path, sb, se);
+            if (gs_n > 0.0) {
+                auto U = [&](uint64_t chan, uint64_t inner) {
+                    return rng_uniform(seed, pkey, step, (sc << 8) | inner, chan);
+                };
+                auto deflect = [&](double ct, double uphi) {
+                    double nx, ny, nz;
+                    cone_direction(dx, dy, dz, ct, uphi, nx, ny, nz);
+                    dx = nx; dy = ny; dz = nz;
+                };
+                if (gs_n < t.gs_few) {
+                    int const cap = static_cast<int>(std::ceil(3.0 * t.gs_few));
+                    if (cap >= 255) fail(ORC_ERR_INVALID, "GS few-collision cap does not pack");
+                    int const cnt = poisson_capped(gs_n, U(pos ? kChPositronGsFewCount : kChGsFewCount, 0), cap);
+                    for (int q = 0; q < cnt; ++q) {
+                        uint64_t const inner = static_cast<uint64_t>(q + 1);
+                        double const ct = t.sample_single_elastic_cos(
+                            pos, k, U(pos ? kChPositronGsFewScatter : kChGsFewScatter, inner));
+                        deflect(ct, U(pos ? kChPositronGsFewAzimuth : kChGsFewAzimuth, inner));
                     }
+                } else {
+                    double const ct = t.sample_gs_cos(pos, k, gs_n, U(pos ? kChPositronGsAngle : kChGsAngle, 0));
+                    deflect(ct, U(pos ? kChPositronGsAzimuth : kChGsAzimuth, 0));
                 }
             }
-            fx.tallies[ORC_T_FIELD_WORK_WEV] += w * half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
-            if (k < cutoff) { mark_stopped(se); break; }
         }
-        if (!advance_running_position(path, se, run_x, run_y, run_z)) { exited = true; break; }
-        if (path.exits) { exited = true; break; }
-        completed = sc + 1U;
-    }
+        fx.tallies[ORC_T_FIELD_WORK_WEV] += w * half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
+        if (k < cutoff) { mark_stopped(se); return true; }
+        return false;
+    };
+
+    // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:686-732
+    run_substep_interleave(
+        subcycles, k, cutoff, completed, exited, build_substep, run_events,
+        [&] { return terminated || annihilated; }, finish_substep,
+        [&](double se) { return advance_running_position(path, se, run_x, run_y, run_z); },
+        [&] { return path.exits; });
 
     if (exited) escape = clipped_escape_policy(path, std::hypot(path.ex, path.ey), path.ez);
     if (pos) {
@@ -2925,139 +3024,78 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
     };
     // RreaPlanContinuousDragSegment OV/RreaChargedInterleave.H:239-292 with the
     // cutoff limiter :203-265 and the ion-pair partition :157-196
-    struct SegLimit { double end; bool cutoff; };
     auto continuous = [&](double b, double e, double stop_e) -> SegLimit {
-        if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return {b, false};
-        double const col = frozen(t.rate[pos ? kPCol : kECol].interpolate(stop_e) * 1.0);
-        double const soft = frozen(t.rate[pos ? kPSoft : kESoft].interpolate(stop_e) * 1.0);
-        double const total = col + soft;
-        auto loss_to = [&](double f) { return total * density_path(iv, path, b, f); };
-        double eff = b;
-        bool reaches = false;
-        {
-            if (!std::isfinite(b) || !std::isfinite(e) || !std::isfinite(k) || !std::isfinite(cutoff) || !(e >= b)
-                || !(k >= cutoff) || !(cutoff >= 0.0)) {
-                fail(ORC_ERR_INVALID, "newborn continuous stopping could not locate a cutoff point");
-            }
-            double const to_cut = k - cutoff;
-            double const req = loss_to(e);
-            if (!std::isfinite(req) || req < 0.0) fail(ORC_ERR_INVALID, "newborn continuous stopping invalid");
-            if (req < to_cut) {
-                eff = e;
-            } else {
-                reaches = true;
-                if (to_cut != 0.0) {
-                    double lo = b, hi = e;
-                    for (int it = 0; it < 56; ++it) {
-                        double const mid = 0.5 * (lo + hi);
-                        double const l = loss_to(mid);
-                        if (!std::isfinite(l) || l < 0.0) fail(ORC_ERR_INVALID, "newborn continuous stopping invalid");
-                        if (l < to_cut) lo = mid; else hi = mid;
-                    }
-                    eff = 0.5 * (lo + hi);
-                }
-            }
-        }
-        double const pm = density_path(iv, path, b, eff);
-        double const raw_col = col * pm;
-        double const raw_soft = soft * pm;
-        double const avail = std::max(0.0, k - cutoff);
-        double const drag = reaches ? avail : std::min(raw_col + raw_soft, avail);
-        double const col_loss = std::min(raw_col, drag);
-        if (col_loss > 0.0 && raw_col > 0.0) {
-            for (auto const& it : iv) {
-                double const lb = std::max(b, it.begin), le = std::min(eff, it.end);
-                if (!(le > lb)) continue;
-                double const raw_i = path.full_length * (le - lb) * it.ratio * col;
-                double const loss_i = col_loss * raw_i / raw_col;
-                double const pw = w * loss_i / std::max(kWAir, 1.0);
-                if (!(pw > 0.0)) continue;
-                em.ion_event(pw);
-                double mx, my, mz;
-                path.point_at_full(0.5 * (lb + le), mx, my, mz);
-                em.ionization(std::hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
-            }
-        }
+        double drag = 0.0;
+        SegLimit const sl = continuous_drag(t, pos, path, iv, b, e, stop_e, cutoff, w, k, drag,
+                                            [&](double mid_f, double pw) {
+                                                em.ion_event(pw);
+                                                double mx, my, mz;
+                                                path.point_at_full(mid_f, mx, my, mz);
+                                                em.ionization(std::hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
+                                            });
         fx.tallies[ORC_T_DRAG_LOSS_WEV] += w * drag;
-        k = std::max(0.0, k - drag);
-        if (reaches) k = cutoff;
-        return {eff, reaches};
+        return sl;
     };
 
     FieldRange sfield;
     double substep_dt = 0.0;
     int const nch = pos ? 3 : 2;
-    // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:693-732
-    for (uint64_t sc = 0; sc < substeps; ++sc) {
-        if (k < cutoff) break;
-        double sb = 0.0, se = 0.0;
-        {
-            Chord const ch = build_chord(run_x, run_y, run_z, dx, dy, dz, k, sub_dt, residual_dt, ctx.r_lo, ctx.r_hi,
-                                         ctx.z_lo, ctx.z_hi);
-            if (!ch.valid) break;
-            path = ch.path;
-            if (!(path.full_length > 0.0) || !(path.in_length > 0.0)) {
-                exited = path.exits;
-                final_f = 0.0;
-                break;
-            }
-            iv = build_intervals(ctx.cp, path, ctx.density_ratio);
-            if (iv.empty()) fail(ORC_ERR_INVALID, "newborn microchord without material intervals");
-            sb = path.begin_f;
-            se = path.end_f;
-            final_f = se;
-            sfield = path_field_work(ctx.cp, path, charge, sb, se);
-            substep_dt = sub_dt * (se - sb);
-            (void)half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
-            if (k < cutoff) {
-                final_f = sb;
-                terminal_at_point = true;
-                break;
-            }
-            in_range();
+    auto build_substep = [&](uint64_t, double& sb, double& se) -> bool {
+        Chord const ch = build_chord(run_x, run_y, run_z, dx, dy, dz, k, sub_dt, residual_dt, ctx.r_lo, ctx.r_hi,
+                                     ctx.z_lo, ctx.z_hi);
+        if (!ch.valid) return false;
+        path = ch.path;
+        if (!(path.full_length > 0.0) || !(path.in_length > 0.0)) {
+            exited = path.exits;
+            final_f = 0.0;
+            return false;
         }
-        // RreaRunChargedEventInterleave OV/RreaChargedInterleave.H:546-664
-        {
-            double rem = sb;
-            uint64_t ord = 0;
-            double ex_ = 0.0, ey_ = 0.0, ez_ = 0.0, er_ = 0.0, efrac = 1.0;
-            while (rem < se && k >= cutoff) {
-                if (ord >= kEventCap) fail(ORC_ERR_INVALID, "newborn event ordinal overflow");
-                double const frozen_e = k;
-                double stp[3];
+        iv = build_intervals(ctx.cp, path, ctx.density_ratio);
+        if (iv.empty()) fail(ORC_ERR_INVALID, "newborn microchord without material intervals");
+        sb = path.begin_f;
+        se = path.end_f;
+        final_f = se;
+        sfield = path_field_work(ctx.cp, path, charge, sb, se);
+        substep_dt = sub_dt * (se - sb);
+        (void)half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
+        if (k < cutoff) {
+            final_f = sb;
+            terminal_at_point = true;
+            return false;
+        }
+        in_range();
+        return true;
+    };
+    auto run_events = [&](uint64_t sc, double sb, double se) {
+        uint64_t idx = 0;
+        double stp[3] = {0, 0, 0};
+        double ex_ = 0.0, ey_ = 0.0, ez_ = 0.0, er_ = 0.0;
+        run_event_interleave(
+            sb, se, k, cutoff, nch, kEventCap,
+            [&](double frozen_e, double rem, double end) {
                 if (pos) positron_stp_rates(t, thr, gcut, frozen_e, stp);
                 else electron_stp_rates(t, thr, gcut, frozen_e, hm_on, stp);
+                double const L = density_path(iv, path, rem, end);
+                return std::array<double, 3>{stp[0] * L, stp[1] * L, stp[2] * L};
+            },
+            [&](uint64_t ord) {
+                idx = pack(sc, ord, 0U);
+                return std::pair<double, double>{rng_uniform(seed, root, step, idx, kChNewbornCompetingOpticalDepth),
+                                                 rng_uniform(seed, root, step, idx, kChNewbornCompetingChannel)};
+            },
+            [&](double tau, double rem, double end) {
                 double const total_stp = stp[0] + stp[1] + stp[2];
-                double const L = density_path(iv, path, rem, se);
-                double const taus[3] = {stp[0] * L, stp[1] * L, stp[2] * L};
-                uint64_t const idx = pack(sc, ord, 0U);
-                double const u1 = rng_uniform(seed, root, step, idx, kChNewbornCompetingOpticalDepth);
-                double const u2 = rng_uniform(seed, root, step, idx, kChNewbornCompetingChannel);
-                Hazard const h = select_hazard(taus, nch, u1, u2);
-                if (!h.valid) fail(ORC_ERR_INVALID, "newborn competing-hazard selection invalid");
-                if (!h.event) {
-                    SegLimit const sl = continuous(rem, se, frozen_e);
-                    if (sl.cutoff) mark_stopped(sl.end);
-                    break;
-                }
-                double const ef = fraction_at_density_path(iv, path, rem, se, h.sampled / total_stp);
-                if (!(std::isfinite(rem) && std::isfinite(ef) && std::isfinite(se) && se > rem && ef >= rem && ef <= se)) {
-                    fail(ORC_ERR_INVALID, "newborn sampled event depth outside its subcycle");
-                }
+                return fraction_at_density_path(iv, path, rem, end, tau / total_stp);
+            },
+            continuous, mark_stopped,
+            [&](double ef) {
                 path.point_at_full(ef, ex_, ey_, ez_);
                 er_ = std::hypot(ex_, ey_);
-                efrac = elapsed_at(ef);
-                SegLimit const sl = continuous(rem, ef, frozen_e);
-                if (sl.cutoff) { mark_stopped(sl.end); break; }
-                bool const open_ev = k >= cutoff && charged_channel_open(t, pos, thr, gcut, h.channel, k, hm_on);
-                if (!open_ev) {
-                    bool const open_fr = frozen_e >= cutoff && charged_channel_open(t, pos, thr, gcut, h.channel, frozen_e, hm_on);
-                    if (!open_fr) fail(ORC_ERR_INVALID, "newborn selected channel closed before its sampled depth");
-                    rem = ef;
-                    ++ord;
-                    continue;
-                }
-                if (!pos && h.channel == 0) {
+            },
+            [&](int channel, double e) { return charged_channel_open(t, pos, thr, gcut, channel, e, hm_on); },
+            [&](int channel, double ef) -> bool {
+                double const efrac = elapsed_at(ef);
+                if (!pos && channel == 0) {
                     double const before = k;
                     double T = t.sample_moller(before, rng_uniform(seed, root, step, idx, kChHardMollerEnergy));
                     T = std::max(thr, std::min(T, 0.5 * before));
@@ -3070,7 +3108,7 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                     em.ion_event(w);
                     em.ionization(std::hypot(ex_, ey_), ez_, w, T, T);
                     fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
-                } else if (pos && h.channel == 0) {
+                } else if (pos && channel == 0) {
                     double const before = k;
                     double const T = std::min(t.sample_bhabha(before, rng_uniform(seed, root, step, idx, kChPositronBhabhaEnergy)),
                                               std::max(0.0, before));
@@ -3086,7 +3124,7 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                         fx.tallies[ORC_T_POSITIVE_IONS] += w;
                         fx.tallies[ORC_T_BHABHA_EVENTS] += 1.0;
                     }
-                } else if (h.channel == 1) {
+                } else if (channel == 1) {
                     double const before = k;
                     double const eu = rng_uniform(seed, root, step, idx, pos ? kChPositronBremsPhotonEnergy : kChBremsPhotonEnergy);
                     double const ks = pos ? t.sample_positron_brems_energy(before, eu) : t.sample_brems_energy(before, eu);
@@ -3109,64 +3147,64 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                         em.deposit_local(local, std::hypot(ex_, ey_), ez_);
                     }
                     fx.tallies[ORC_T_BREMS_EVENTS] += 1.0;
-                } else if (pos && h.channel == 2) {
+                } else if (pos && channel == 2) {
                     // in flight: the positron is consumed at the event point
                     spawn_annihilation(ctx, em, kProcAnnihilation, ex_, ey_, ez_, std::hypot(ex_, ey_), dx, dy, dz, k,
                                        false, idx, efrac);
                     segment(er_, ez_);
                     terminal_interaction = true;
                     final_f = ef;
-                    break;
+                    return false;
                 }
-                if (k <= cutoff) { mark_stopped(ef); break; }
-                rem = ef;
-                ++ord;
-            }
-        }
-        if (terminal_interaction || terminal_at_point) break;
-        {
-            if (k >= cutoff) {
-                // apply_gs_elastic_scatter OV/RreaTransportStep.H:372-423 on the newborn channels
-                double const gs_n = elastic_inverse_length(t, pos, k) * density_path(iv, path, sb, se);
-                if (gs_n > 0.0) {
-                    auto U = [&](uint64_t chan, uint64_t inner) {
-                        return rng_uniform(seed, root, step, pack(sc, 0U, inner), chan);
-                    };
-                    auto deflect = [&](double ct, double uphi) {
-                        double nx, ny, nz;
-                        cone_direction(dx, dy, dz, ct, uphi, nx, ny, nz);
-                        dx = nx; dy = ny; dz = nz;
-                    };
-                    if (gs_n < t.gs_few) {
-                        int const cap = static_cast<int>(std::ceil(3.0 * t.gs_few));
-                        if (cap >= 255) fail(ORC_ERR_INVALID, "GS few-collision cap does not pack");
-                        int const cnt = poisson_capped(
-                            gs_n, U(pos ? kChNewbornPositronGsFewCount : kChNewbornElectronGsFewCount, 0), cap);
-                        for (int q = 0; q < cnt; ++q) {
-                            uint64_t const inner = static_cast<uint64_t>(q + 1);
-                            double const ct = t.sample_single_elastic_cos(
-                                pos, k, U(pos ? kChNewbornPositronGsFewScatter : kChNewbornElectronGsFewScatter, inner));
-                            deflect(ct, U(pos ? kChNewbornPositronGsFewAzimuth : kChNewbornElectronGsFewAzimuth, inner));
-                        }
-                    } else {
-                        double const ct = t.sample_gs_cos(pos, k, gs_n,
-                                                          U(pos ? kChNewbornPositronGsAngle : kChNewbornElectronGsAngle, 0));
-                        deflect(ct, U(pos ? kChNewbornPositronGsAzimuth : kChNewbornElectronGsAzimuth, 0));
+                return true;
+            });
+    };
+    auto finish_substep = [&](uint64_t sc, double sb, double se) -> bool {
+        if (k >= cutoff) {
+            // apply_gs_elastic_scatter OV/RreaTransportStep.H:372-423 on the newborn channels
+            double const gs_n = elastic_inverse_length(t, pos, k) * density_path(iv, path, sb, se);
+            if (gs_n > 0.0) {
+                auto U = [&](uint64_t chan, uint64_t inner) {
+                    return rng_uniform(seed, root, step, pack(sc, 0U, inner), chan);
+                };
+                auto deflect = [&](double ct, double uphi) {
+                    double nx, ny, nz;
+                    cone_direction(dx, dy, dz, ct, uphi, nx, ny, nz);
+                    dx = nx; dy = ny; dz = nz;
+                };
+                if (gs_n < t.gs_few) {
+                    int const cap = static_cast<int>(std::ceil(3.0 * t.gs_few));
+                    if (cap >= 255) fail(ORC_ERR_INVALID, "GS few-collision cap does not pack");
+                    int const cnt = poisson_capped(
+                        gs_n, U(pos ? kChNewbornPositronGsFewCount : kChNewbornElectronGsFewCount, 0), cap);
+                    for (int q = 0; q < cnt; ++q) {
+                        uint64_t const inner = static_cast<uint64_t>(q + 1);
+                        double const ct = t.sample_single_elastic_cos(
+                            pos, k, U(pos ? kChNewbornPositronGsFewScatter : kChNewbornElectronGsFewScatter, inner));
+                        deflect(ct, U(pos ? kChNewbornPositronGsFewAzimuth : kChNewbornElectronGsFewAzimuth, inner));
                     }
+                } else {
+                    double const ct = t.sample_gs_cos(pos, k, gs_n,
+                                                      U(pos ? kChNewbornPositronGsAngle : kChNewbornElectronGsAngle, 0));
+                    deflect(ct, U(pos ? kChNewbornPositronGsAzimuth : kChNewbornElectronGsAzimuth, 0));
                 }
             }
-            (void)half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
-            if (k < cutoff) {
-                final_f = se;
-                terminal_at_point = true;
-                break;
-            }
-            in_range();
         }
-        if (!advance_running_position(path, se, run_x, run_y, run_z)) { exited = true; break; }
-        if (path.exits) { exited = true; break; }
-        completed = sc + 1U;
-    }
+        (void)half_kick(sfield, 0.5 * substep_dt, charge, k, dx, dy, dz);
+        if (k < cutoff) {
+            final_f = se;
+            terminal_at_point = true;
+            return true;
+        }
+        in_range();
+        return false;
+    };
+    // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:693-732
+    run_substep_interleave(
+        substeps, k, cutoff, completed, exited, build_substep, run_events,
+        [&] { return terminal_interaction || terminal_at_point; }, finish_substep,
+        [&](double se) { return advance_running_position(path, se, run_x, run_y, run_z); },
+        [&] { return path.exits; });
     if (terminal_interaction) {
         out.status = ORC_ANNIHILATED;
         fx.tallies[ORC_T_NEWBORN_TERMINATED] += 1.0;
@@ -3982,3 +4020,256 @@ extern "C" int orc_sources_normalize(const orc_geometry* g, double dt, int diagn
     }
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Probes of the step's primitives: the functions above, called on explicit
+// inputs, so the reference's behavioural cases (its charged-interleave,
+// transport-path and sampler smokes) and closed forms can be checked against
+// the code the oracle's steps run.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+double orc_unit_cycle(double u) { return unit_cycle(u); }
+
+void orc_normalize(double* v) { normalize(v[0], v[1], v[2]); }
+
+void orc_boris_rotate(double* u, double qmdt, const double* b) { boris_rotate(u[0], u[1], u[2], qmdt, b[0], b[1], b[2]); }
+
+void orc_cone_direction(const double* axis, double ct, double uphi, double* out)
+{
+    cone_direction(axis[0], axis[1], axis[2], ct, uphi, out[0], out[1], out[2]);
+}
+
+double orc_moller_cos(double kp, double ks, int secondary)
+{
+    return secondary ? moller_secondary_cos(kp, ks) : moller_primary_cos(kp, ks);
+}
+
+void orc_moller_lab_directions(const double* d, double kp, double ks, double az, double* parent, double* secondary)
+{
+    moller_lab_directions(d[0], d[1], d[2], kp, ks, az, parent[0], parent[1], parent[2], secondary[0], secondary[1],
+                          secondary[2]);
+}
+
+void orc_photon_recoil(double k0, double eg, const double* g, double* c)
+{
+    photon_recoil(k0, eg, g[0], g[1], g[2], c[0], c[1], c[2]);
+}
+
+void orc_rotate_to_axis(const double* axis, const double* local, double* out)
+{
+    rotate_to_axis(axis[0], axis[1], axis[2], local[0], local[1], local[2], out[0], out[1], out[2]);
+}
+
+int orc_poisson_capped(double mean, double u, int kmax) { return poisson_capped(mean, u, kmax); }
+
+// out = [valid, event, channel, sampled, total]
+void orc_select_hazard(const double* taus, int nch, double u1, double u2, double* out)
+{
+    Hazard const h = select_hazard(taus, nch, u1, u2);
+    out[0] = h.valid ? 1.0 : 0.0;
+    out[1] = h.event ? 1.0 : 0.0;
+    out[2] = h.channel;
+    out[3] = h.sampled;
+    out[4] = h.total;
+}
+
+// field = [work, ex, ey, ez, bx, by, bz]; returns the half work
+double orc_half_kick(const double* field, double half_dt, double q, double* k, double* d)
+{
+    FieldRange f;
+    f.work = field[0]; f.ex = field[1]; f.ey = field[2]; f.ez = field[3];
+    f.bx = field[4]; f.by = field[5]; f.bz = field[6];
+    return half_kick(f, half_dt, q, *k, d[0], d[1], d[2]);
+}
+
+// The gate of a channel's inverse length; *calls counts the mean free path's
+// evaluations.
+double orc_gated_inverse_length(int open, double mfp, int* calls)
+{
+    return gated_inverse_length(open != 0, [&] { ++*calls; return mfp; });
+}
+
+int orc_stop_is_escape(int exits, double f, double end_f) { return stop_is_escape(exits != 0, f, end_f) ? 1 : 0; }
+
+// The cutoff limiter on a linear loss loss_to(f) = rate (f - b): 1 when the
+// cutoff is reached, 0 when not, -code on a fail-closed input.
+int orc_limit_at_cutoff_linear(double b, double e, double k, double cutoff, double rate, double* end)
+{
+    try {
+        SegLimit const l = limit_at_cutoff(b, e, k, cutoff, [&](double f) { return rate * (f - b); });
+        *end = l.end;
+        return l.cutoff ? 1 : 0;
+    } catch (OracleError const& err) {
+        return -err.code;
+    }
+}
+
+// The ion-pair partition over n intervals [begin, end) of density ratio
+// `ratio`; writes the emitted (midpoint fraction, pair weight) and returns
+// their count.
+int orc_partition_ion_pairs(const double* ib, const double* ie, const double* ir, int n, double full_length, double b,
+                            double e, double segment_loss, double raw_loss, double w, double w_air, double stp,
+                            double* out_f, double* out_w, int cap)
+{
+    std::vector<Interval> iv;
+    for (int i = 0; i < n; ++i) iv.push_back(Interval{ib[i], ie[i], full_length * (ie[i] - ib[i]), ir[i]});
+    int m = 0;
+    partition_ion_pairs(iv, full_length, b, e, segment_loss, raw_loss, w, w_air, stp, [&](double f, double pw) {
+        if (m < cap) { out_f[m] = f; out_w[m] = pw; }
+        ++m;
+    });
+    return m;
+}
+
+// The event interleave on the analytic medium of the reference's recorder:
+// rate sigma(E) = E / 1e6 per unit chord fraction, stopping 0.5 E, a cutoff
+// stop at the segment midpoint, a channel open above open_above, scripted
+// draw pairs and per-event losses.  Records what the core handed each
+// policy.  0, or the error code of a fail-closed exit.
+typedef struct orc_interleave_record {
+    int32_t n_frozen, n_segments, n_draws, n_open, n_final, n_stops;
+    double frozen[16], taus[16];
+    double seg_begin[16], seg_end[16], seg_frozen[16];
+    uint64_t draw_ordinals[16];
+    double open_energies[16];
+    double final_fractions[16];
+    double stops[16];
+    double kinetic_end;
+} orc_interleave_record;
+
+int orc_interleave_probe(double kinetic, double cutoff, double open_above, const double* draws, int n_draws,
+                         const double* losses, int n_losses, int consume, orc_interleave_record* rec)
+{
+    std::memset(rec, 0, sizeof(*rec));
+    double k = kinetic;
+    int dc = 0, lc = 0;
+    auto put = [](double* a, int32_t& n, double v) { if (n < 16) a[n] = v; ++n; };
+    try {
+        run_event_interleave(
+            0.0, 1.0, k, cutoff, 2, 64,
+            [&](double frozen_e, double b, double e) {
+                double const tau = (e - b) * frozen_e / 1.0e6;
+                int32_t nf = rec->n_frozen;
+                put(rec->frozen, nf, frozen_e);
+                put(rec->taus, rec->n_frozen, tau);
+                return std::array<double, 3>{tau, 0.0, 0.0};
+            },
+            [&](uint64_t ord) {
+                if (rec->n_draws < 16) rec->draw_ordinals[rec->n_draws] = ord;
+                ++rec->n_draws;
+                if (dc >= n_draws) fail(ORC_ERR_INVALID, "the script holds no more draws");
+                std::pair<double, double> const u{draws[2 * dc], draws[2 * dc + 1]};
+                ++dc;
+                return u;
+            },
+            [&](double tau, double b, double) { return b + tau * 1.0e6 / k; },
+            [&](double b, double e, double frozen_e) -> SegLimit {
+                int32_t ns = rec->n_segments, ns2 = rec->n_segments;
+                put(rec->seg_begin, ns, b);
+                put(rec->seg_end, ns2, e);
+                put(rec->seg_frozen, rec->n_segments, frozen_e);
+                k -= 0.5 * (e - b) * frozen_e;
+                if (k < cutoff) return {0.5 * (b + e), true};
+                return {e, false};
+            },
+            [&](double f) { put(rec->stops, rec->n_stops, f); },
+            [](double) {},
+            [&](int, double e) {
+                put(rec->open_energies, rec->n_open, e);
+                return e > open_above;
+            },
+            [&](int channel, double f) -> bool {
+                if (channel != 0) fail(ORC_ERR_INVALID, "the single-rate medium selected another channel");
+                put(rec->final_fractions, rec->n_final, f);
+                if (lc >= n_losses) fail(ORC_ERR_INVALID, "the script holds no more event losses");
+                k -= losses[lc++];
+                return consume == 0;
+            });
+    } catch (OracleError const& err) {
+        rec->kinetic_end = k;
+        return err.code;
+    }
+    rec->kinetic_end = k;
+    return 0;
+}
+
+// The substep skeleton with recording policies: tags kind * 1000 + substep
+// (1 build, 2 events, 3 finish, 4 advance); a build that fails when
+// build_ok == 0, an endpoint refused at substep refuse_at, an exiting chord
+// at substep exit_at, a stop after the events when stop_after != 0.
+int orc_substep_probe(uint64_t subcycles, double kinetic, int build_ok, uint64_t refuse_at, uint64_t exit_at,
+                      int stop_after, int32_t* tags, int cap, uint64_t* completed, int* exited)
+{
+    int n = 0;
+    auto tag = [&](int kind, uint64_t sc) {
+        if (n < cap) tags[n] = kind * 1000 + static_cast<int32_t>(sc);
+        ++n;
+    };
+    uint64_t done = 0;
+    bool ex = false;
+    double const k = kinetic;
+    run_substep_interleave(
+        subcycles, k, 1.0e3, done, ex,
+        [&](uint64_t sc, double& b, double& e) {
+            tag(1, sc);
+            b = 0.0;
+            e = 1.0;
+            return build_ok != 0;
+        },
+        [&](uint64_t sc, double, double) { tag(2, sc); },
+        [&] { return stop_after != 0; },
+        [&](uint64_t sc, double, double) {
+            tag(3, sc);
+            return false;
+        },
+        [&](double) {
+            uint64_t const sc = done;
+            tag(4, sc);
+            return sc != refuse_at;
+        },
+        [&] { return done == exit_at; });
+    *completed = done;
+    *exited = ex ? 1 : 0;
+    return n;
+}
+
+// The clipped chord's boundary crossing: out = [valid, faces, start_r,
+// start_z, cross_r, cross_z, exits, end_f]; 0 or an error code.
+int orc_boundary_crossing(double sr, double st, double sz, double er_, double et, double ez_, double r_lo, double r_hi,
+                          double z_lo, double z_hi, double* out)
+{
+    try {
+        Path const p = clip_rz_path(sr, st, sz, er_, et, ez_, r_lo, r_hi, z_lo, z_hi);
+        BoundaryCrossing const bc = boundary_crossing(p);
+        double const v[8] = {bc.valid ? 1.0 : 0.0, static_cast<double>(bc.faces), bc.start_r, bc.start_z,
+                             bc.cross_r, bc.cross_z, p.exits ? 1.0 : 0.0, p.end_f};
+        std::memcpy(out, v, sizeof(v));
+        return 0;
+    } catch (OracleError const& err) {
+        return err.code;
+    }
+}
+
+// Commit of the running point at fraction f of the clipped chord: 1 when the
+// point is in the half-open domain (run <- point), 0 when refused (run kept).
+int orc_advance_running_position(double sr, double st, double sz, double er_, double et, double ez_, double r_lo,
+                                 double r_hi, double z_lo, double z_hi, double f, double* run)
+{
+    try {
+        Path const p = clip_rz_path(sr, st, sz, er_, et, ez_, r_lo, r_hi, z_lo, z_hi);
+        return advance_running_position(p, f, run[0], run[1], run[2]) ? 1 : 0;
+    } catch (OracleError const& err) {
+        return -err.code;
+    }
+}
+
+// OutsideCertifiedDomain of a chord clipped to the given domain.
+int orc_outside_certified(double r_lo, double r_hi, double z_lo, double z_hi, double r, double z)
+{
+    Path p;
+    p.r_lo = r_lo; p.r_hi = r_hi; p.z_lo = z_lo; p.z_hi = z_hi;
+    return p.outside_certified(r, z) ? 1 : 0;
+}
+
+}  // extern "C"
