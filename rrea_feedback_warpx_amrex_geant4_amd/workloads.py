@@ -105,13 +105,16 @@ def c3_instances(n_instances: int = 256, seed: int = 3):
 
 
 def c3_batch(n_instances: int = 256, seed: int = 3, scale: float = 1.0, nr: int = 64, nz: int = 256,
-             cell_m: float = 1.0):
+             cell_m: float = 1.0, params: tuple | None = None):
     """The batched configuration: per instance a uniform ambient E_z = -|E|
     (the force on e- along +z) over air of constant density rho on its own
     nr x nz RZ grid of cell_m cells, and N0 * scale seed electrons (at least
     one) with K ~ U[1, 10] MeV in a 30-degree cone about +z, started in a
-    5 m disk at z in [1, 5] m.  Returns (fields list, tiles list, seeds)."""
-    e_mag, rho, n0 = c3_instances(n_instances, seed)
+    5 m disk at z in [1, 5] m.  ``params`` = (|E|, rho, N0) arrays replaces
+    the seeded draw of the instances' parameters.  Returns (fields list,
+    tiles list, seeds)."""
+    e_mag, rho, n0 = c3_instances(n_instances, seed) if params is None else params
+    n_instances = len(e_mag)
     rng = np.random.default_rng(seed + 1)
     fields, tiles, seeds = [], [], []
     next_id = 1
