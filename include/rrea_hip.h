@@ -422,7 +422,9 @@ rrea_status rrea_newborn_drain_batched(rrea_tables_t tables, rrea_batch_t batch,
  * tiles, instance arrays and buffers named here are bound at creation and
  * must outlive the graph; every launch uploads the instances' environments
  * from their fields handles, so field and density updates between launches
- * are seen.  Errors accumulate in `error` and are read by the caller. */
+ * are seen.  Errors accumulate in `error` and are read by the caller.  With
+ * `counts` set (below) the tiles grow instead: each step commits its
+ * survivors, the avalanche of the reference's Apply + AddNParticles. */
 typedef struct rrea_batch_graph_desc {
     const rrea_tile* tiles[3];       /* electrons, positrons, photons (index = species); NULL or n == 0: absent */
     const int32_t* instance[3];      /* each tile's per-particle instance index (device int32) */
@@ -441,6 +443,19 @@ typedef struct rrea_batch_graph_desc {
     int32_t steps;                   /* steps per launch, 1..1024 */
     int32_t push_photons;            /* nonzero: the straight photon push (rrea_photon_push) opens each step */
     int32_t reserved;
+    /* The avalanche.  With counts != NULL every step ends with the commit of
+     * the drain's survivors into the tiles (rrea_commit_survivors, the tiles
+     * as its stores, instance arrays included), so each instance's newborns
+     * are transported from the next step on: tiles[k]->n is then tile k's
+     * capacity and counts[k] (device int64[3]) its length, the particles a
+     * step transports; a tile of capacity 0 is absent.  Ids continue from
+     * *next_id (device int64); committed particles get the cpu word `cpu`.
+     * Survivors beyond a tile's capacity: RREA_ERR_SECONDARY_CAPACITY and
+     * nothing committed that step. */
+    int64_t* counts;
+    int64_t* next_id;
+    int32_t cpu;
+    int32_t reserved2;
 } rrea_batch_graph_desc;
 typedef struct rrea_batch_graph_s* rrea_batch_graph_t;
 rrea_status rrea_batch_graph_create(rrea_tables_t tables, rrea_batch_t batch, const rrea_batch_graph_desc* desc,

@@ -134,6 +134,7 @@ class BatchGraphDesc(C.Structure):
         ("survivors", C.c_void_p), ("survivor_capacity", C.c_int64), ("survivor_count", C.c_void_p),
         ("tallies", C.c_void_p), ("error", C.c_void_p), ("clear", C.c_void_p * 4), ("clear_n", C.c_int64 * 4),
         ("max_generations", C.c_int32), ("steps", C.c_int32), ("push_photons", C.c_int32), ("reserved", C.c_int32),
+        ("counts", C.c_void_p), ("next_id", C.c_void_p), ("cpu", C.c_int32), ("reserved2", C.c_int32),
     ]
 
 

@@ -2065,13 +2065,17 @@ __global__ void __launch_bounds__(kBlock) electron_full_kernel(const StepArgs* _
 // The straight-line photon push at c along u / |u| that WarpX applies before
 // the hook: the pushed point becomes the chord end, the old point the
 // captured pre-push snapshot.
+// n_dev (device, may be NULL): the live length, read instead of n (a tile
+// growing inside a captured graph).
 __global__ void photon_push_kernel(int64_t n, const int64_t* __restrict__ id, double* __restrict__ prev_r,
                                    double* __restrict__ prev_z, double* __restrict__ prev_th, double* __restrict__ r,
                                    double* __restrict__ z, double* __restrict__ th, const double* __restrict__ ux,
-                                   const double* __restrict__ uy, const double* __restrict__ uz, double c_dt)
+                                   const double* __restrict__ uy, const double* __restrict__ uz, double c_dt,
+                                   const int64_t* __restrict__ n_dev = nullptr)
 {
     int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n || id[i] == RREA_INVALID_ID) return;
+    int64_t const len = n_dev ? (*n_dev < n ? *n_dev : n) : n;
+    if (i >= len || id[i] == RREA_INVALID_ID) return;
     double const r0 = r[i], t0 = th[i], z0 = z[i];
     prev_r[i] = r0; prev_th[i] = t0; prev_z[i] = z0;
     double const un = sqrt(ux[i] * ux[i] + uy[i] * uy[i] + uz[i] * uz[i]);
@@ -3543,6 +3547,21 @@ __global__ void set_step_kernel(StepArgs* a0, StepArgs* a1, StepArgs* a2, Newbor
     if (nb) nb->S.cfg.step = step;
 }
 
+// The live length of each growing tile at the start of a captured step:
+// counts[k] clipped to the tile's capacity, into species k's argument block.
+__global__ void set_lengths_kernel(StepArgs* a0, StepArgs* a1, StepArgs* a2, const int64_t* counts,
+                                   int64_t cap0, int64_t cap1, int64_t cap2)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    StepArgs* const a[3] = {a0, a1, a2};
+    int64_t const cap[3] = {cap0, cap1, cap2};
+    for (int k = 0; k < 3; ++k) {
+        if (!a[k]) continue;
+        int64_t const c = counts[k];
+        a[k]->n = c < 0 ? 0 : (c < cap[k] ? c : cap[k]);
+    }
+}
+
 __global__ void next_step_kernel(StepArgs* a0, StepArgs* a1, StepArgs* a2, NewbornArgs* nb)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -4704,6 +4723,33 @@ rrea_status tile_sort_workspace(rrea_fields_t F, int64_t n)
 
 // Gather every distinct array of the tile (and the instance array, when
 // given) through F->d_tidx_out, the permutation a sort left there.
+// The commit kernel's view of three particle stores (electrons, positrons,
+// photons), checked.
+rrea_status commit_stores(const rrea_particle_store* stores, CommitStores& S)
+{
+    for (int q = 0; q < 3; ++q) {
+        rrea_particle_store const& P = stores[q];
+        rrea_tile const& t = P.tile;
+        bool const photon = q == 2;
+        if (P.capacity < 0) return RREA_ERR_ARGUMENT;
+        if (P.capacity > 0 && (!t.prev_r || !t.prev_z || !t.prev_theta || !t.w || !t.id || !t.r || !t.z || !t.theta
+                               || !t.ux || !t.uy || !t.uz || (!photon && (!t.prev_ux || !t.prev_uy || !t.prev_uz)))) {
+            return RREA_ERR_ARGUMENT;
+        }
+        CommitStore& D = S.s[q];
+        D.prev_r = const_cast<double*>(t.prev_r); D.prev_z = const_cast<double*>(t.prev_z);
+        D.prev_th = const_cast<double*>(t.prev_theta);
+        D.prev_ux = photon ? nullptr : const_cast<double*>(t.prev_ux);
+        D.prev_uy = photon ? nullptr : const_cast<double*>(t.prev_uy);
+        D.prev_uz = photon ? nullptr : const_cast<double*>(t.prev_uz);
+        D.r = t.r; D.z = t.z; D.th = t.theta; D.ux = t.ux; D.uy = t.uy; D.uz = t.uz;
+        D.w = const_cast<double*>(t.w); D.id = t.id; D.cpu = const_cast<int32_t*>(t.cpu);
+        D.inst = P.instance;
+        D.cap = P.capacity;
+    }
+    return RREA_OK;
+}
+
 rrea_status permute_tile(rrea_fields_t F, const rrea_tile* tile, int32_t photon, int32_t* instance, hipStream_t s)
 {
     int64_t const n = tile->n;
@@ -5186,26 +5232,8 @@ rrea_status rrea_commit_survivors(const rrea_secondary* survivors, const int64_t
 {
     if (!survivors || !survivor_count || !stores || !counts || !next_id || !error) return RREA_ERR_ARGUMENT;
     CommitStores S{};
-    for (int q = 0; q < 3; ++q) {
-        rrea_particle_store const& P = stores[q];
-        rrea_tile const& t = P.tile;
-        bool const photon = q == 2;
-        if (P.capacity < 0) return RREA_ERR_ARGUMENT;
-        if (P.capacity > 0 && (!t.prev_r || !t.prev_z || !t.prev_theta || !t.w || !t.id || !t.r || !t.z || !t.theta
-                               || !t.ux || !t.uy || !t.uz || (!photon && (!t.prev_ux || !t.prev_uy || !t.prev_uz)))) {
-            return RREA_ERR_ARGUMENT;
-        }
-        CommitStore& D = S.s[q];
-        D.prev_r = const_cast<double*>(t.prev_r); D.prev_z = const_cast<double*>(t.prev_z);
-        D.prev_th = const_cast<double*>(t.prev_theta);
-        D.prev_ux = photon ? nullptr : const_cast<double*>(t.prev_ux);
-        D.prev_uy = photon ? nullptr : const_cast<double*>(t.prev_uy);
-        D.prev_uz = photon ? nullptr : const_cast<double*>(t.prev_uz);
-        D.r = t.r; D.z = t.z; D.th = t.theta; D.ux = t.ux; D.uy = t.uy; D.uz = t.uz;
-        D.w = const_cast<double*>(t.w); D.id = t.id; D.cpu = const_cast<int32_t*>(t.cpu);
-        D.inst = P.instance;
-        D.cap = P.capacity;
-    }
+    rrea_status const cs = commit_stores(stores, S);
+    if (cs != RREA_OK) return cs;
     hipStream_t const s = static_cast<hipStream_t>(stream);
     commit_kernel<<<dim3(1), dim3(1024), 0, s>>>(survivors, survivor_count, S, counts, next_id, cpu, error);
     HIP_TRY(hipGetLastError());
@@ -5338,6 +5366,13 @@ struct rrea_batch_graph_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int device = 0;
+    // the avalanche: each step ends with the commit of its survivors into the tiles
+    bool grow = false;
+    CommitStores commit{};
+    int64_t* counts = nullptr;
+    int64_t* next_id = nullptr;
+    int32_t cpu = 0;
+    int64_t cap[3] = {0, 0, 0};
 };
 
 rrea_status rrea_batch_graph_destroy(rrea_batch_graph_t G)
@@ -5374,6 +5409,7 @@ rrea_status rrea_batch_graph_create(rrea_tables_t T, rrea_batch_t B, const rrea_
         }
     }
     if (!any || !(d->config.dt_s > 0.0)) return RREA_ERR_ARGUMENT;
+    if (d->counts && !d->next_id) return RREA_ERR_ARGUMENT;
     if (d->push_photons && !(d->tiles[kPhoton] && d->tiles[kPhoton]->n > 0)) return RREA_ERR_ARGUMENT;
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
@@ -5383,6 +5419,26 @@ rrea_status rrea_batch_graph_create(rrea_tables_t T, rrea_batch_t B, const rrea_
     if (!G) return RREA_ERR_HIP;
     G->batch = B;
     G->device = dev;
+    if (d->counts) {
+        // the tiles are stores: n is their capacity, counts[k] their length
+        G->grow = true;
+        G->counts = d->counts;
+        G->next_id = d->next_id;
+        G->cpu = d->cpu;
+        rrea_particle_store stores[3] = {};
+        for (int k = 0; k < 3; ++k) {
+            if (!(d->tiles[k] && d->tiles[k]->n > 0)) continue;
+            stores[k].tile = *d->tiles[k];
+            stores[k].instance = const_cast<int32_t*>(d->instance[k]);
+            stores[k].capacity = d->tiles[k]->n;
+            G->cap[k] = d->tiles[k]->n;
+        }
+        rrea_status const cs = commit_stores(stores, G->commit);
+        if (cs != RREA_OK) {
+            delete G;
+            return cs;
+        }
+    }
     rrea_status st = rrea_fields_create(&B->fields[0]->geom, &G->work);
     if (st != RREA_OK) {
         rrea_batch_graph_destroy(G);
@@ -5453,13 +5509,23 @@ rrea_status rrea_batch_graph_create(rrea_tables_t T, rrea_batch_t B, const rrea_
     rrea_status cst = RREA_OK;
     for (int32_t step = 0; step < d->steps && cst == RREA_OK; ++step) {
         set_i64_kernel<<<dim3(1), dim3(64), 0, cs>>>(d->record_count, 0);
+        if (G->grow) {
+            set_lengths_kernel<<<dim3(1), dim3(64), 0, cs>>>(sa[0], sa[1], sa[2], G->counts, G->cap[0], G->cap[1],
+                                                             G->cap[2]);
+        }
         for (int k = 0; k < 4; ++k) {
             if (d->clear_n[k] == 0) continue;
             unsigned const b = static_cast<unsigned>(std::min<int64_t>(blocks_for(d->clear_n[k], 256), 2048));
             fill_f64_kernel<<<dim3(b), dim3(256), 0, cs>>>(d->clear[k], d->clear_n[k]);
         }
         if (hipGetLastError() != hipSuccess) { cst = RREA_ERR_HIP; break; }
-        if (d->push_photons) {
+        if (d->push_photons && G->grow) {
+            rrea_tile const& t = G->tiles[kPhoton];
+            photon_push_kernel<<<dim3(static_cast<unsigned>(blocks_for(t.n, 256))), dim3(256), 0, cs>>>(
+                t.n, t.id, const_cast<double*>(t.prev_r), const_cast<double*>(t.prev_z),
+                const_cast<double*>(t.prev_theta), t.r, t.z, t.theta, t.ux, t.uy, t.uz, kC * d->config.dt_s,
+                G->counts + kPhoton);
+        } else if (d->push_photons) {
             cst = rrea_photon_push(&G->tiles[kPhoton], d->config.dt_s, cs);
             if (cst != RREA_OK) break;
         }
@@ -5473,6 +5539,11 @@ rrea_status rrea_batch_graph_create(rrea_tables_t T, rrea_batch_t B, const rrea_
                                   d->record_count, d->record_count, d->survivors, d->survivor_capacity,
                                   d->survivor_count, d->error, d->max_generations, cs);
         if (cst != RREA_OK) break;
+        if (G->grow) {
+            // AddNParticles of the drain's survivors (RreaSecondaryCommit.cpp:594-629)
+            commit_kernel<<<dim3(1), dim3(1024), 0, cs>>>(d->survivors, d->survivor_count, G->commit, G->counts,
+                                                          G->next_id, G->cpu, d->error);
+        }
         next_step_kernel<<<dim3(1), dim3(64), 0, cs>>>(sa[0], sa[1], sa[2], na);
         if (hipGetLastError() != hipSuccess) cst = RREA_ERR_HIP;
     }

@@ -85,6 +85,13 @@ class ParticleStore:
     def photon_tile(self) -> PhotonTile:
         return PhotonTile(self.prev_r[: self.n], self.prev_z[: self.n], self.prev_theta[: self.n], self.tile())
 
+    def full_tile(self) -> DeviceTile:
+        """All ``capacity`` rows as a DeviceTile (a graph-captured store)."""
+        return DeviceTile(*(getattr(self, k) for k in ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")))
+
+    def full_photon_tile(self) -> PhotonTile:
+        return PhotonTile(self.prev_r, self.prev_z, self.prev_theta, self.full_tile())
+
     def struct(self) -> _lib.ParticleStore:
         p = lambda t: t.data_ptr()  # noqa: E731
         if self.photon:
@@ -237,3 +244,103 @@ def observables(particles: dict, runaway_eV: float = 1.0e6) -> dict:
     return {"live": int(live.sum()), "runaways": int(run.sum()),
             "mean_runaway_eV": float(k[run].mean()) if run.any() else 0.0,
             "mean_runaway_z": float(z[run].mean()) if run.any() else 0.0}
+
+
+class BatchedAvalanche:
+    """Independent avalanche instances stepped together: batched stores of
+    the three species, each particle keeping its instance, each instance with
+    its own fields handle, J grid and RNG seed.  A step is what
+    ``Simulation.step`` does for one instance -- the photon push, the species
+    steps in Apply's order, the newborn drain, the commit of the survivors --
+    for every instance at once (rrea_transport_step_batched,
+    rrea_newborn_drain_batched, rrea_commit_survivors).  ``step()`` runs one
+    from the host; ``graph(steps)`` captures ``steps`` whole steps, commit
+    included, into one HIP graph (rrea_batch_graph_* with the stores'
+    counts), whose launches grow the stores on the device."""
+
+    def __init__(self, tables: InteractionTables, fields_list, electrons_list, seeds, cfg: StepConfig,
+                 capacity=(1 << 20, 1 << 16, 1 << 18), secondary_capacity: int = 1 << 16,
+                 fifo_capacity: int | None = None, device="cuda"):
+        from .transport import Batch, InstanceGrids
+        self.tables = tables
+        self.fields = list(fields_list)
+        self.cfg = cfg
+        self.couplings = [Coupling(f, tables) for f in self.fields]
+        self.grids = InstanceGrids(self.fields)
+        self.batch = Batch(self.couplings, self.grids.grids, seeds)
+        self.stores = [ParticleStore(s, c, device, batched=True) for s, c in zip((ELECTRON, POSITRON, PHOTON),
+                                                                                  capacity)]
+        sizes = [len(t.r) for t in electrons_list]
+        cat = ParticleTile(*(np.concatenate([getattr(t, k) for t in electrons_list]) for k in
+                             ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")))
+        self.stores[ELECTRON].load(cat, np.repeat(np.arange(len(sizes), dtype=np.int32), sizes))
+        self.next_id = torch.tensor([int(cat.id.max()) + 1 if len(cat.id) else 1], dtype=torch.int64, device=device)
+        self.counts = torch.tensor([s.n for s in self.stores], dtype=torch.int64, device=device)
+        f0 = self.fields[0]
+        self.out = [StepOutputs(0, f0, secondary_capacity=secondary_capacity, per_particle=False, device=device)]
+        for _ in range(2):
+            self.out.append(StepOutputs(0, f0, secondary_capacity=secondary_capacity, per_particle=False,
+                                        share_grid=self.out[0], device=device))
+        self.drain = DrainOutputs(fifo_capacity or 4 * secondary_capacity, self.out[0], device=device)
+        self.step_index = cfg.step
+
+    @property
+    def electrons(self) -> ParticleStore:
+        return self.stores[ELECTRON]
+
+    @property
+    def photons(self) -> ParticleStore:
+        return self.stores[PHOTON]
+
+    @property
+    def positrons(self) -> ParticleStore:
+        return self.stores[POSITRON]
+
+    def step(self, stream: int | None = None) -> None:
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        cfg = dataclasses.replace(self.cfg, step=self.step_index)
+        self.grids.zero_()
+        for o in self.out:
+            o.secondary_count.zero_()
+        ph, el, po = self.photons, self.electrons, self.positrons
+        if ph.n:
+            ph.photon_tile().push(cfg.dt_s, stream)
+        e_out, g_out, p_out = self.out
+        for name, store, o in (("electron", el, e_out), ("photon", ph, g_out), ("positron", po, p_out)):
+            if store.n:
+                tile = store.photon_tile() if store.photon else store.tile()
+                self.batch.step(self.tables, name, tile, store.inst[: store.n], cfg, o, stream=stream, check=False)
+        self.batch.drain(self.tables, cfg, [e_out, g_out, p_out], self.drain, stream=stream, check=False)
+        stores = (_lib.ParticleStore * 3)(*[s.struct() for s in self.stores])
+        L = _lib.lib()
+        _lib.check(L.rrea_commit_survivors(self.drain.survivors.data_ptr(), self.drain.survivor_count.data_ptr(),
+                                           stores, self.counts.data_ptr(), self.next_id.data_ptr(), 0,
+                                           e_out.error.data_ptr(), stream), "rrea_commit_survivors")
+        counts = self.counts.cpu().numpy()
+        e_out.check_error()
+        for s, c in zip(self.stores, counts):
+            s.n = int(c)
+        self.step_index += 1
+
+    def graph(self, steps: int, record_capacity: int | None = None, max_generations: int = 4):
+        """``steps`` whole steps, commit included, captured into one graph;
+        ``launch(first_step)`` runs them and grows the stores on the device
+        (read ``counts`` after a synchronisation: ``sync_counts``)."""
+        from .transport import BatchGraph
+        tiles = {"electron": (self.electrons.full_tile(), self.electrons.inst),
+                 "positron": (self.positrons.full_tile(), self.positrons.inst),
+                 "photon": (self.photons.full_photon_tile(), self.photons.inst)}
+        cap = record_capacity or self.drain.capacity
+        return BatchGraph(self.tables, self.batch, tiles, self.cfg, cap,
+                          clear=(self.grids.jr, self.grids.jz, self.grids.rho), steps=steps,
+                          max_generations=max_generations, push_photons=True,
+                          survivor_capacity=cap, counts=self.counts, next_id=self.next_id)
+
+    def sync_counts(self) -> None:
+        """The stores' lengths from the device (after graph launches)."""
+        for s, c in zip(self.stores, self.counts.cpu().numpy()):
+            s.n = int(c)
+
+    def live(self) -> list:
+        return [s.live() for s in self.stores]

@@ -505,12 +505,16 @@ class BatchGraph:
     host round trip inside.  ``tiles`` maps "electron", "positron" and
     "photon" to (tile, inst) pairs of the batch's layout.  The records
     buffer holds each step's ordered secondaries, then the whole FIFO; the
-    survivors are the drain's.  ``launch(first_step)`` enqueues the graph's
+    survivors are the drain's.  With ``counts`` (device int64[3], the tiles'
+    lengths) and ``next_id`` (device int64) each step also commits the
+    survivors into the tiles, which are then stores whose length is the
+    capacity (the avalanche).  ``launch(first_step)`` enqueues the graph's
     ``steps`` steps; outputs are read after a synchronisation."""
 
     def __init__(self, tables: InteractionTables, batch: Batch, tiles: dict, cfg: StepConfig, record_capacity: int,
                  clear=(), steps: int = 1, max_generations: int = 8, push_photons: bool = False,
-                 survivor_capacity: int | None = None, device="cuda"):
+                 survivor_capacity: int | None = None, counts: torch.Tensor | None = None,
+                 next_id: torch.Tensor | None = None, cpu: int = 0, device="cuda"):
         size = SECONDARY_DTYPE.itemsize
         self.record_capacity = int(record_capacity)
         self.survivor_capacity = int(survivor_capacity if survivor_capacity is not None else record_capacity)
@@ -544,7 +548,11 @@ class BatchGraph:
             d.clear[k] = t.data_ptr()
             d.clear_n[k] = t.numel()
         d.max_generations, d.steps, d.push_photons = int(max_generations), self.steps, 1 if push_photons else 0
-        self._keep = (tiles, clear)
+        if counts is not None:
+            if next_id is None or counts.dtype != torch.int64 or counts.numel() != 3 or next_id.dtype != torch.int64:
+                raise ValueError("counts: device int64[3] and next_id: device int64")
+            d.counts, d.next_id, d.cpu = counts.data_ptr(), next_id.data_ptr(), int(cpu)
+        self._keep = (tiles, clear, counts, next_id)
         h = C.c_void_p()
         _lib.check(_lib.lib().rrea_batch_graph_create(tables.h, batch.h, C.byref(d), C.byref(h)),
                    "rrea_batch_graph_create")

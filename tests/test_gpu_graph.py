@@ -5,7 +5,9 @@ steps made by the host-driven batched entry points on the same inputs.  The
 particle states, the FIFO (ordered step records, then every generation) and
 the survivors must be identical, record for record; the J grids equal to
 FP64 summation order.  The graph is launched twice, so a replay after the
-first is checked too."""
+first is checked too.  In the avalanche mode every captured step also
+commits its survivors into the batched stores, as a WarpX step adds the
+newborns after the hook."""
 import numpy as np
 import pytest
 
@@ -173,3 +175,43 @@ def test_graph_generation_limit_is_reported(tables):
     assert int(graph.survivor_count.item()) == 0
     graph.close()
     batch.close()
+
+
+@pytest.mark.parametrize("dt", [1e-9, 1e-8])
+def test_graph_avalanche_matches_host_driven_steps(tables, dt):
+    """The avalanche: each captured step ends with the commit of the drain's
+    survivors into the batched stores (rrea_batch_graph_desc.counts), so
+    newborn electrons, positrons and photons are transported from the next
+    step on.  Two launches of a 3-step graph against six host-driven steps
+    (BatchedAvalanche.step: the same batched entry points plus
+    rrea_commit_survivors): the stores' lengths, the next id and every row of
+    every store (state, weight, id, instance) identical; the last step's J
+    grids equal to FP64 summation order."""
+    from rrea_feedback_warpx_amrex_geant4_amd.simulation import BatchedAvalanche
+    fields, etiles, seeds = wl.c3_batch(6, seed=21, scale=2e-3)
+    cfg = StepConfig(rng_seed=0, dt_s=dt)
+    caps = (1 << 15, 1 << 12, 1 << 14)
+    host = BatchedAvalanche(tables, fields, etiles, seeds, cfg, capacity=caps, secondary_capacity=1 << 14)
+    dev = BatchedAvalanche(tables, fields, etiles, seeds, cfg, capacity=caps, secondary_capacity=1 << 14)
+    print(f"case graph avalanche dt={dt}: instances of {[len(t.r) for t in etiles]} electrons", flush=True)
+    g = dev.graph(3, max_generations=8)
+    for launch in range(2):
+        for _ in range(3):
+            host.step()
+        g.launch(3 * launch)
+        torch.cuda.synchronize()
+        g.check_error()
+        dev.sync_counts()
+        assert torch.equal(host.counts, dev.counts), (launch, host.counts, dev.counts)
+        assert torch.equal(host.next_id, dev.next_id)
+        for a, b in zip(host.stores, dev.stores):
+            n = a.n
+            for k in KEYS + (("prev_r", "prev_z", "prev_theta") if a.photon else ()):
+                assert torch.equal(getattr(a, k)[:n], getattr(b, k)[:n]), (launch, a.species, k)
+            assert torch.equal(a.inst[:n], b.inst[:n]), (launch, a.species)
+        for k in ("jr", "jz", "rho"):
+            x, y = getattr(host.grids, k), getattr(dev.grids, k)
+            assert torch.allclose(x, y, rtol=0.0, atol=1e-12 * float(x.abs().max()) + 1e-300), (launch, k)
+    grown = [int(c) for c in host.counts.cpu()]
+    assert grown[1] + grown[2] > 0 and grown[0] > sum(len(t.r) for t in etiles), grown
+    print(f"graph avalanche dt={dt}: stores grew to {grown} (e-, e+, gamma) over 6 steps -- identical", flush=True)
