@@ -9,32 +9,38 @@ are transported: 70 % electrons (11,744,051) and 5 % positrons (838,861),
 K ~ Exp(7.3 MeV) clipped to [10 keV, 100 MeV] in a 30-degree cone about the
 force; 25 % photons (4,194,304), isotropic, same energy law.
 
-Every --sort-every steps (default 25) the three tiles are reordered by cell
-and energy band (rrea_tile_sort, the particle sort WarpX applies to its
-tiles) inside the timed region.
+The three tiles are reordered by cell and energy band (rrea_tile_sort, the
+particle sort WarpX applies to its tiles) at the first timed step and every
+--sort-every steps after it (default 25; the autotuner picks the interval
+and the band width before the warm-up), inside the timed region, so it holds
+at least its amortised share of sorts.
 
-One "step" is one RreaParticleInteraction::Apply (RreaParticleInteraction.cpp:57-60):
-the electron kernels, the photon kernel, the positron kernels, each followed
-by the deterministic secondary ordering -- the three passes run on three
-streams, the photon and positron passes starting when the electron fast-path
-kernel ends, under the electron hand-over (ConcurrentApply; --sequential runs
-them one after another; the results are the same) -- then the newborn
-residual advance of
-every secondary the step made (the transient FIFO of
-DrainAndCommitSecondaries), all charged chords depositing into one J grid.  Charged tiles step in place (the fused pre-push snapshot); the photon
-tile is pushed on the device after its step (prev <- current, current += c dt
-u/|u|, WarpX's photon push), so each step reads a freshly pushed chord.  The
-push is inside the timed region.  Particles a step invalidates stay in their
-tile with the invalid id and are skipped by later steps, as WarpX keeps them
-until its redistribute.
+One "step" is one RreaParticleInteraction::Apply (RreaParticleInteraction.cpp:57-60)
+and the WarpX calls around it: the electron kernels, the photon kernel, the
+positron kernels, each followed by the deterministic secondary ordering --
+the three passes run on three streams, the photon and positron passes
+starting when the electron fast-path kernel ends, under the electron
+hand-over (ConcurrentApply; --sequential runs them one after another; the
+results are the same) -- then the newborn residual advance of every
+secondary the step made (the transient FIFO of DrainAndCommitSecondaries),
+all charged chords depositing into one J grid; then the commit of the
+drain's survivors into the species' stores (AddNParticles,
+RreaSecondaryCommit.cpp:594-629), so newborns are transported from the next
+step on; then WarpX's photon push on the device (prev <- current, current
++= c dt u/|u|), so each step reads a freshly pushed chord.  Charged tiles
+step in place (the fused pre-push snapshot).  Everything is inside the timed
+region.  Particles a step invalidates stay in their store with the invalid
+id and are skipped by later steps, as WarpX keeps them until its
+redistribute.
 
 Prints one JSON line (rank 0, N=1).  `roofline` is for the dominant kernel,
 the electron kernel: algorithmic bytes per launch (112 B per live electron)
 over its mean launch time from HIP events on its own stream over the timed
 region, and `roofline.isolated` the same for the kernel timed alone on a
 few steps after the timed region; `traffic` is that kernel's measured HBM
-bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE record in
-profiles/electron_kernel_traffic.json when present.  The batched (plain per
+bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+record in profiles/electron_kernel_traffic.json when present (labelled as
+such), and `valu` its VALU issue from profiles/electron_kernel_valu.json.  The batched (plain per
 instance, batched, graph-captured), drift-adjoint, field-side, population
 and feedback-column legs and the CPU baseline follow (see their functions).
 """
@@ -55,6 +61,7 @@ sys.path.insert(0, ROOT)
 B_ALG = 112.0  # bytes per particle-step: read r,z,theta,ux,uy,uz,w,id (64), write r,z,theta,ux,uy,uz (48)
 HBM_PEAK = 8.0e12
 TRAFFIC_RECORD = os.path.join(ROOT, "profiles", "electron_kernel_traffic.json")
+VALU_RECORD = os.path.join(ROOT, "profiles", "electron_kernel_valu.json")
 INVALID = np.iinfo(np.int64).min
 
 
@@ -62,20 +69,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_particles: int, steps: int, threads: int):
+def cpu_c2_sample(tables, n_particles: int, steps: int, threads: int):
     """The CPU oracle (a restatement of the reference step, OpenMP over
-    64-particle chunks) timed on this host on a bounded sample of the same
-    workload: electrons, photons and positrons in the 70:25:5 ratio, then
-    the newborn drain of the step's secondaries, material sources
-    accumulated.  Only the oracle's calls are timed (not the numpy photon
-    push between steps)."""
+    64-particle chunks) on a bounded sample of the at-scale workload:
+    electrons, photons and positrons in the 70:25:5 ratio, each step followed
+    by the newborn drain of its secondaries, material sources accumulated.
+    Only the oracle's calls are timed (not the numpy photon push between
+    steps).  Returns (particle-steps/s, seconds, particle-steps)."""
     from oracle import binding as ob
     from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
     from rrea_feedback_warpx_amrex_geant4_amd.config import StepConfig
-    from rrea_feedback_warpx_amrex_geant4_amd.transport import load_bundle
 
-    ob.build()
-    tables = ob.Tables(load_bundle())
     fields = wl.c2_fields()
     n_e, n_g, n_p = wl.c2_species_counts(n_particles)
     g = wl.c2_photons(n_g, fields, dt_s=1e-12, seed=4, first_id=1 << 41)
@@ -118,175 +122,212 @@ def cpu_baseline(n_particles: int, steps: int, threads: int):
     return done / dt, dt, done
 
 
+def cpu_c1_run(tables, steps: int, threads: int):
+    """The plain CPU configuration on the oracle's multi-step loop
+    (oracle/driver.py: photon push, the three species steps, the drain, the
+    commit of its survivors): 1,000 e- at 7.2 MeV along the force in
+    E_z = -400 kV/m, air at 1.225 kg/m^3, 10 keV cutoff, dt = 1 ps, seed
+    20240623.  The whole loop is timed (it is the CPU path's own cost).
+    Returns (particle-steps/s, seconds, particle-steps, final live counts)."""
+    from oracle import driver as od
+    from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
+    from rrea_feedback_warpx_amrex_geant4_amd.config import StepConfig
+
+    fields, tile = wl.c1_plain(1000)
+    sim = od.OracleSimulation(tables, fields, StepConfig(rng_seed=20240623, dt_s=1e-12), tile, nthreads=threads)
+    done = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        done += sum(int((sim.sp[k]["id"] != INVALID).sum()) for k in sim.sp)
+        sim.step()
+    el = time.perf_counter() - t0
+    live = {name: int((sim.sp[k]["id"] != INVALID).sum()) for name, k in
+            (("electrons", od.ELECTRON), ("photons", od.PHOTON), ("positrons", od.POSITRON))}
+    return done / el, el, done, live
+
+
+def cpu_baseline(args):
+    """The CPU baselines on this host's cores: the at-scale sample (the
+    headline, same per-particle work as the GPU metric) with every host
+    thread this process may use and with one thread, and the plain CPU
+    configuration -- in full (20,000 steps) with every thread, a bounded
+    2,000 steps with one thread."""
+    from oracle import binding as ob
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import load_bundle
+
+    ob.build()
+    tables = ob.Tables(load_bundle())
+    nproc = os.cpu_count() or 1
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    threads = max(1, min(16, allowed))
+    rate, secs, done = cpu_c2_sample(tables, args.cpu_particles, args.cpu_steps, threads)
+    log(f"cpu baseline: {rate:.4e} particle-steps/s on {threads} threads ({secs:.1f} s)")
+    rate1, secs1, done1 = cpu_c2_sample(tables, args.cpu_particles_1t, args.cpu_steps_1t, 1)
+    log(f"cpu baseline, 1 thread: {rate1:.4e} particle-steps/s ({secs1:.1f} s)")
+    c1 = c1_1 = None
+    if args.cpu_c1_steps > 0:
+        r, el, dn, live = cpu_c1_run(tables, args.cpu_c1_steps, threads)
+        c1 = {"value": r, "seconds": el, "particle_steps": dn, "steps": args.cpu_c1_steps, "threads": threads,
+              "live_at_end": live}
+        log(f"cpu plain configuration: {args.cpu_c1_steps} steps, {r:.4e} particle-steps/s on {threads} threads "
+            f"({el:.1f} s)")
+        r1, el1, dn1, _ = cpu_c1_run(tables, min(args.cpu_c1_steps, 2000), 1)
+        c1_1 = {"value": r1, "seconds": el1, "particle_steps": dn1, "steps": min(args.cpu_c1_steps, 2000),
+                "threads": 1}
+        log(f"cpu plain configuration, 1 thread: {r1:.4e} particle-steps/s ({el1:.1f} s)")
+    return {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_particles} at-scale particles (e-:gamma:e+ = 70:25:5) x {args.cpu_steps} steps "
+                      f"({done} particle-steps, {secs:.1f} s), oracle restatement, OpenMP dynamic over 64-particle "
+                      f"chunks, each step's newborn drain and material sources included",
+            "nproc": nproc, "threads_allowed": allowed,
+            "single_thread": {"value": rate1, "particles": args.cpu_particles_1t, "steps": args.cpu_steps_1t,
+                              "seconds": secs1, "particle_steps": done1},
+            "plain_cpu_configuration": c1, "plain_cpu_configuration_single_thread": c1_1}
+
+
 def batched_leg(args, tables):
     """The batched configuration on the same device: 256 independent
     avalanche instances (|E| ~ U[250, 1000] kV/m, rho ~ U[0.3, 1.225] kg/m^3,
     N0 ~ log-U[1e3, 1e6] electrons at K ~ U[1, 10] MeV, 64 x 256 RZ grids of
-    1 m cells), dt = 1 ps, each step the electron step of every instance then
-    the newborn drain of its secondaries, J deposited per instance.  Timed
-    three times on the same inputs: the batched path
-    (rrea_transport_step_batched / rrea_newborn_drain_batched: one launch set
-    for all instances), the plain per-instance path (the drop-in Apply run
-    instance after instance, one launch set and drain each), and the batched
-    steps captured in one HIP graph (rrea_batch_graph_*).  Returns the report
-    object."""
-    import ctypes as C
-
+    1 m cells), dt = 1 ps.  A step is the whole Apply of every instance --
+    the photon push, the electron, photon and positron steps, the newborn
+    drain -- then the commit of the drain's survivors into the stores
+    (AddNParticles), so each instance avalanches: its newborn electrons,
+    positrons and photons are transported from the next step on.  Timed on
+    the same inputs three ways: the batched entry points driven from the
+    host (simulation.BatchedAvalanche.step: one launch set per species and
+    one drain for all instances), the same steps captured into one HIP graph
+    (rrea_batch_graph_* with the stores' counts: commit inside the graph, no
+    host round trip in a step), relaunched to the configuration's 500 steps,
+    and the plain per-instance path (simulation.Simulation: the drop-in
+    Apply + commit run instance after instance).  Particle-steps are the
+    stores' live particles per step (the mean of the live counts before and
+    after the timed steps, times the steps).  Returns the report object."""
     import torch
 
-    from rrea_feedback_warpx_amrex_geant4_amd import _lib
     from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl
     from rrea_feedback_warpx_amrex_geant4_amd.config import StepConfig
-    from rrea_feedback_warpx_amrex_geant4_amd.transport import (Batch, BatchGraph, Coupling, DeviceTile,
-                                                                DrainOutputs, InstanceGrids, RreaParticleInteraction,
-                                                                StepOutputs)
+    from rrea_feedback_warpx_amrex_geant4_amd.simulation import BatchedAvalanche, Simulation
 
-    keys = ("r", "z", "theta", "ux", "uy", "uz", "w", "id", "cpu")
     n_inst = args.batched_instances
     fields, tiles, seeds = wl.c3_batch(n_inst, seed=3, scale=args.batched_scale)
     sizes = np.array([len(t.r) for t in tiles], dtype=np.int64)
-    offs = np.concatenate([[0], np.cumsum(sizes)])
-    n = int(offs[-1])
+    n = int(sizes.sum())
     log(f"batched: {n_inst} instances, {n} electrons (smallest {sizes.min()}, largest {sizes.max()})")
-    dev_b = DeviceTile.from_host(type(tiles[0])(*(np.concatenate([getattr(t, k) for t in tiles]) for k in keys)))
-    del tiles
-    dev_p = dev_b.clone()
-    dev_g = dev_b.clone()
-    inst = torch.as_tensor(np.repeat(np.arange(n_inst, dtype=np.int32), sizes), device="cuda")
-    couplings = [Coupling(f, tables) for f in fields]
-    grids_b, grids_p = InstanceGrids(fields, sources=True), InstanceGrids(fields, sources=True)
-    batch = Batch(couplings, grids_b.grids, seeds)
-    out_b = StepOutputs(n, fields[0], secondary_capacity=max(1 << 14, n >> 6), per_particle=False)
-    drain_b = DrainOutputs(max(1 << 15, n >> 5), out_b)
-    # the plain path: instance i's tile is a view of its rows, its J grid its own
-    plain_tiles = [DeviceTile(*(getattr(dev_p, k)[offs[i]:offs[i + 1]] for k in keys)) for i in range(n_inst)]
-    counts = torch.zeros(n_inst, dtype=torch.int64, device="cuda")
-    plain_outs, plain_drains = [], []
-    for i, f in enumerate(fields):
-        g = grids_p.grids[i]
-        o = StepOutputs(int(sizes[i]), f, secondary_capacity=max(1 << 10, int(sizes[i]) >> 6), per_particle=False,
-                        grid_buffers=(g.jr, g.jz, g.rho, g.sources))
-        o.secondary_count = counts[i:i + 1]
-        plain_outs.append(o)
-        plain_drains.append(DrainOutputs(max(1 << 11, int(sizes[i]) >> 5), o))
-    apply = RreaParticleInteraction
     dt = 1e-12
+    cfg = StepConfig(rng_seed=0, dt_s=dt)
+    steps_total = args.batched_warmup + args.batched_steps + args.graph_loop_steps
+    # room for every newborn of the run: well above the ~1e-4 secondaries per
+    # particle-step at 1 ps
+    caps = (n + max(1 << 20, n // 8), max(1 << 18, n // 64), max(1 << 20, n // 16))
+    sec_cap = max(1 << 15, n >> 6)
 
-    def step_batched(s):
-        out_b.secondary_count.zero_()
-        grids_b.zero_()
-        cfg = StepConfig(rng_seed=0, step=s, dt_s=dt)
-        batch.step(tables, "electron", dev_b, inst, cfg, out_b, check=False)
-        batch.drain(tables, cfg, [out_b], drain_b, check=False)
+    def live(av):
+        return sum(av.live())
 
-    def step_plain(s):
-        counts.zero_()
-        grids_p.zero_()
-        for i in range(n_inst):
-            cfg = StepConfig(rng_seed=seeds[i], step=s, dt_s=dt)
-            apply.apply_electrons(tables, couplings[i], plain_tiles[i], cfg, plain_outs[i], order_secondaries=True,
-                                  check=False)
-            apply.drain_secondaries(tables, couplings[i], cfg, [plain_outs[i]], plain_drains[i], check=False)
-
-    def live(t):
-        return int((t.id != _lib.INVALID_ID).sum().item())
-
-    def timed(run, tile):
-        for s in range(args.batched_warmup):
-            run(s)
-        torch.cuda.synchronize()
-        l0 = live(tile)
-        t0 = time.perf_counter()
-        for s in range(args.batched_steps):
-            run(args.batched_warmup + s)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        l1 = live(tile)
-        return 0.5 * (l0 + l1) * args.batched_steps / el, el, l0, l1
-
-    L = _lib.lib()
-    L.rrea_kernel_timing(batch.workspace, 1)
-    rate_b, el_b, b0, b1 = timed(step_batched, dev_b)
-    ms = C.c_double(0.0)
-    cnt = C.c_int64(0)
-    _lib.check(L.rrea_kernel_timing_read(batch.workspace, 0, C.byref(ms), C.byref(cnt)), "timing read")
-    L.rrea_kernel_timing(batch.workspace, 0)
-    out_b.check_error()
-    mean_launch_s = ms.value * 1e-3 / max(cnt.value, 1)
-    achieved = 0.5 * (b0 + b1) * B_ALG / mean_launch_s
-    log(f"batched: {args.batched_steps} steps in {el_b:.3f} s, {rate_b:.4e} particle-steps/s; electron kernel "
-        f"{mean_launch_s * 1e3:.3f} ms/launch")
-    rate_p, el_p, p0, p1 = timed(step_plain, dev_p)
-    for o in plain_outs:
-        o.check_error()
-    log(f"plain per instance: {args.batched_steps} steps in {el_p:.3f} s, {rate_p:.4e} particle-steps/s")
-    # the graph-captured batched steps on a third copy: the warm-up steps and
-    # the timed steps each one graph (rrea_batch_graph_*), no host round trip
-    # inside a step
-    grids_g = InstanceGrids(fields, sources=True)
-    batch_g = Batch(couplings, grids_g.grids, seeds)
-    clear = (grids_g.jr, grids_g.jz, grids_g.rho, grids_g.sources)
-    gcfg = StepConfig(rng_seed=0, dt_s=dt)
-    t0 = time.perf_counter()
-    # a graph holds at least one step: with no warm-up steps only the timed
-    # graph is captured
-    graphs = [BatchGraph(tables, batch_g, {"electron": (dev_g, inst)}, gcfg, record_capacity=drain_b.capacity,
-                         clear=clear, steps=k, max_generations=args.graph_generations)
-              for k in (args.batched_warmup, args.batched_steps) if k > 0]
-    capture_s = time.perf_counter() - t0
-    if args.batched_warmup > 0:
-        graphs[0].launch(0)
+    # host-driven batched steps
+    bav = BatchedAvalanche(tables, fields, tiles, seeds, cfg, capacity=caps, secondary_capacity=sec_cap)
+    for _ in range(args.batched_warmup):
+        bav.step()
     torch.cuda.synchronize()
-    g0 = live(dev_g)
+    b0 = live(bav)
     t0 = time.perf_counter()
-    graphs[-1].launch(args.batched_warmup)
+    for _ in range(args.batched_steps):
+        bav.step()
+    torch.cuda.synchronize()
+    el_b = time.perf_counter() - t0
+    b1 = live(bav)
+    rate_b = 0.5 * (b0 + b1) * args.batched_steps / el_b
+    log(f"batched: {args.batched_steps} steps in {el_b:.3f} s, {rate_b:.4e} particle-steps/s "
+        f"(live {b0} -> {b1}, stores {[s.n for s in bav.stores]})")
+
+    # the same steps captured in one graph: its first launch (graph upload
+    # included) is the warm-up, untimed
+    gav = BatchedAvalanche(tables, fields, tiles, seeds, cfg, capacity=caps, secondary_capacity=sec_cap)
+    del tiles
+    t0 = time.perf_counter()
+    graph = gav.graph(args.batched_steps, max_generations=args.graph_generations)
+    capture_s = time.perf_counter() - t0
+    graph.launch(0)
+    torch.cuda.synchronize()
+    graph.check_error()
+    gav.sync_counts()
+    g0 = live(gav)
+    t0 = time.perf_counter()
+    graph.launch(args.batched_steps)
     torch.cuda.synchronize()
     el_g = time.perf_counter() - t0
-    g1 = live(dev_g)
-    for g in graphs:
-        g.check_error()
+    graph.check_error()
+    gav.sync_counts()
+    g1 = live(gav)
     rate_g = 0.5 * (g0 + g1) * args.batched_steps / el_g
     log(f"graph-captured batched: {args.batched_steps} steps in {el_g:.3f} s, {rate_g:.4e} particle-steps/s "
         f"(capture + instantiate {capture_s:.2f} s)")
-    # the three paths must have advanced the same particles to the same state
-    same = all(torch.equal(getattr(dev_b, k), getattr(dev_p, k)) for k in keys)
-    same_g = all(torch.equal(getattr(dev_b, k), getattr(dev_g, k)) for k in keys)
-    # the graph's loop continued to the configuration's step count, relaunched
-    # back to back
+    # the graph's loop continued to the configuration's 500 steps, relaunched back to back
     launches = max(1, args.graph_loop_steps // args.batched_steps)
     t0 = time.perf_counter()
     for i in range(1, launches):
-        graphs[-1].launch(args.batched_warmup + i * args.batched_steps)
+        graph.launch((i + 1) * args.batched_steps)
     torch.cuda.synchronize()
     el_loop = el_g + time.perf_counter() - t0
-    g2 = live(dev_g)
-    graphs[-1].check_error()
+    graph.check_error()
+    gav.sync_counts()
+    g2 = live(gav)
     loop_steps = launches * args.batched_steps
     rate_loop = 0.5 * (g0 + g2) * loop_steps / el_loop
-    log(f"graph-captured loop: {loop_steps} steps in {el_loop:.3f} s, {rate_loop:.4e} particle-steps/s")
-    for g in graphs:
-        g.close()
-    batch_g.close()
-    batch.close()
+    log(f"graph-captured loop: {loop_steps} steps in {el_loop:.3f} s, {rate_loop:.4e} particle-steps/s "
+        f"(live {g0} -> {g2}, stores {[s.n for s in gav.stores]})")
+    grown = [s.n for s in gav.stores]
+    graph.close()
+    del gav
+    torch.cuda.empty_cache()
+
+    # the plain per-instance path: one Simulation per instance, stepped one after another
+    _, tiles, _ = wl.c3_batch(n_inst, seed=3, scale=args.batched_scale)
+    sims = [Simulation(tables, f, StepConfig(rng_seed=sd, dt_s=dt), t, capacity=int(len(t.r)) + 4096,
+                       secondary_capacity=max(1 << 10, len(t.r) >> 6), compact_every=0)
+            for f, t, sd in zip(fields, tiles, seeds)]
+    del tiles
+
+    def live_plain():
+        return sum(sum(s.live() for s in sim.stores) for sim in sims)
+
+    for _ in range(args.batched_warmup):
+        for sim in sims:
+            sim.step()
+    torch.cuda.synchronize()
+    p0 = live_plain()
+    t0 = time.perf_counter()
+    for _ in range(args.batched_steps):
+        for sim in sims:
+            sim.step()
+    torch.cuda.synchronize()
+    el_p = time.perf_counter() - t0
+    p1 = live_plain()
+    rate_p = 0.5 * (p0 + p1) * args.batched_steps / el_p
+    log(f"plain per instance: {args.batched_steps} steps in {el_p:.3f} s, {rate_p:.4e} particle-steps/s")
+    # the host-driven batch and the plain runs agree on the seed electrons' first-step
+    # outcomes only up to the ids of committed newborns (one counter per batch,
+    # one per run), so the report compares live counts
+    del sims
+    torch.cuda.empty_cache()
     return {
-        "workload": f"{n_inst} independent instances, {n} electrons (N0 log-uniform 1e3..1e6 x {args.batched_scale}),"
-                    " |E| U[250,1000] kV/m, rho U[0.3,1.225] kg/m^3, 64x256 RZ grids, dt=1 ps, electron step + "
-                    "newborn drain per step, J and material sources per instance",
-        "instances": n_inst, "electrons": n, "steps": args.batched_steps, "warmup": args.batched_warmup,
-        "batched": {"value": rate_b, "ms_per_step": el_b / args.batched_steps * 1e3,
-                    "electron_kernel_ms": mean_launch_s * 1e3, "hbm_frac": achieved / HBM_PEAK,
-                    "achieved_GBps": achieved / 1e9},
-        "plain_per_instance": {"value": rate_p, "ms_per_step": el_p / args.batched_steps * 1e3},
+        "workload": f"{n_inst} independent avalanche instances, {n} seed electrons (N0 log-uniform 1e3..1e6 x "
+                    f"{args.batched_scale}), |E| U[250,1000] kV/m, rho U[0.3,1.225] kg/m^3, 64x256 RZ grids, "
+                    "dt=1 ps; per step the photon push, the electron, photon and positron steps, the newborn "
+                    "drain and the commit of its survivors (the avalanche), J per instance",
+        "instances": n_inst, "seed_electrons": n, "steps": args.batched_steps, "warmup": args.batched_warmup,
+        "batched": {"value": rate_b, "ms_per_step": el_b / args.batched_steps * 1e3, "live": [b0, b1]},
+        "plain_per_instance": {"value": rate_p, "ms_per_step": el_p / args.batched_steps * 1e3, "live": [p0, p1]},
         "graph_captured": {"value": rate_g, "ms_per_step": el_g / args.batched_steps * 1e3,
                            "steps_per_graph": args.batched_steps, "max_generations": args.graph_generations,
                            "capture_and_instantiate_s": capture_s,
                            "loop": {"steps": loop_steps, "launches": launches, "value": rate_loop,
-                                    "ms_per_step": el_loop / loop_steps * 1e3}},
+                                    "ms_per_step": el_loop / loop_steps * 1e3, "live": [g0, g2],
+                                    "store_rows_end": grown}},
         "ratio": rate_b / rate_p,
         "ratio_graph": rate_g / rate_p,
         "unit": "particle-steps/s",
-        "states_identical": bool(same),
-        "graph_states_identical": bool(same_g),
     }
 
 
@@ -466,6 +507,20 @@ def column_leg(args, tables):
     return single
 
 
+def valu_record():
+    """The electron kernel's VALU issue from the committed rocprofv3 SQ
+    counter record (profiles/electron_kernel_valu.json), or None: VALU
+    instructions per electron-step and the share of the launch's SIMD
+    cycles they occupy -- the ceiling that binds this kernel."""
+    try:
+        with open(VALU_RECORD) as f:
+            rec = json.load(f)
+        return {k: rec[k] for k in ("valu_insts_per_particle_step", "valu_issue_frac", "waves_per_simd",
+                                    "vgprs", "source") if k in rec}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def traffic_record():
     """Measured HBM bytes per launch of the electron kernel, or None."""
     try:
@@ -490,18 +545,19 @@ def main():
     ap.add_argument("--sequential", action="store_true",
                     help="run the three species passes one after another on one stream (default: concurrently)")
     ap.add_argument("--cpu-particles", type=int, default=1 << 20)
-    ap.add_argument("--cpu-steps", type=int, default=150)
+    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-particles-1t", type=int, default=1 << 16)
+    ap.add_argument("--cpu-steps-1t", type=int, default=20)
+    ap.add_argument("--cpu-c1-steps", type=int, default=20000,
+                    help="steps of the plain CPU configuration timed on the CPU (0: skip)")
     ap.add_argument("--batched-instances", type=int, default=256, help="instances of the batched leg (0: skip it)")
     ap.add_argument("--batched-scale", type=float, default=1.0, help="scale of the batched leg's populations")
-    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--batched-steps", type=int, default=10, help="timed steps of the batched leg (>= 1)")
     ap.add_argument("--batched-warmup", type=int, default=2)
     ap.add_argument("--field-steps", type=int, default=200, help="timed feedback steps of the field leg (0: skip)")
     ap.add_argument("--device-drain", type=int, default=0,
                     help="concurrent step: drain with the counts on the device, up to this many generations "
                          "(0: the host-driven drain)")
-    ap.add_argument("--push-on-photon-stream", action=argparse.BooleanOptionalAction, default=False,
-                    help="concurrent step: push the photons on their own stream right after their pass "
-                         "(default: on the main stream after the drain, measured 1-3 %% faster)")
     ap.add_argument("--column-steps", type=int, default=100, help="timed steps of the feedback-column leg (0: skip)")
     ap.add_argument("--column-inject", type=int, default=2048, help="macroparticles injected per column step")
     ap.add_argument("--population-particles", type=int, default=1 << 22,
@@ -515,6 +571,8 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run only a device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
+    if args.batched_steps < 1 or args.batched_warmup < 0:
+        ap.error("--batched-steps must be at least 1 and --batched-warmup not negative")
 
     if args.copy_calibration_mib:
         import torch
@@ -551,9 +609,25 @@ def main():
     tables = InteractionTables()
     fields = wl.c2_fields()
     coupling = Coupling(fields, tables)
-    dev_e = DeviceTile.from_host(wl.c2_electrons(n_e, fields, seed=2))
-    dev_p = DeviceTile.from_host(wl.c2_electrons(n_p, fields, seed=3, first_id=1 << 40, charge_sign=1.0))
-    dev_g = PhotonTile.from_host(wl.c2_photons(n_g, fields, dt_s=1e-12, seed=4, first_id=1 << 41))
+    # the three species in growing stores (room for the newborns committed
+    # every step); the tiles a step transports are views of their rows
+    from rrea_feedback_warpx_amrex_geant4_amd.simulation import ELECTRON, PHOTON, POSITRON, ParticleStore
+    stores = [ParticleStore(ELECTRON, n_e + (1 << 20)), ParticleStore(POSITRON, n_p + (1 << 18)),
+              ParticleStore(PHOTON, n_g + (1 << 20))]
+    stores[ELECTRON].load(wl.c2_electrons(n_e, fields, seed=2))
+    stores[POSITRON].load(wl.c2_electrons(n_p, fields, seed=3, first_id=1 << 40, charge_sign=1.0))
+    g_host = wl.c2_photons(n_g, fields, dt_s=1e-12, seed=4, first_id=1 << 41)
+    stores[PHOTON].load(g_host)
+    for k in ("r", "z", "theta"):  # the first step's chords run to the pushed points
+        getattr(stores[PHOTON], k)[:n_g].copy_(torch.as_tensor(g_host.extra[k + "_cur"]))
+    del g_host
+    counts = torch.tensor([s.n for s in stores], dtype=torch.int64, device="cuda")
+    next_id = torch.tensor([(1 << 42)], dtype=torch.int64, device="cuda")
+    store_structs = (_lib.ParticleStore * 3)(*[s.struct() for s in stores])
+    # every step transports the stores' whole capacity: rows past a store's
+    # length hold the invalid id and are skipped (8 B read each), so the
+    # commit grows the stores on the device with no host round trip
+    dev_e, dev_g, dev_p = stores[ELECTRON].full_tile(), stores[PHOTON].full_photon_tile(), stores[POSITRON].full_tile()
     out_e = StepOutputs(n_e, fields, secondary_capacity=1 << 20, per_particle=False, sources=True)
     out_p = StepOutputs(n_p, fields, secondary_capacity=1 << 18, per_particle=False, share_grid=out_e)
     out_g = StepOutputs(n_g, fields, secondary_capacity=1 << 18, per_particle=False, share_grid=out_e)
@@ -580,21 +654,23 @@ def main():
         out_e.sources.zero_()
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
         if conc is not None:
-            conc.run(dev_e, dev_g, dev_p, cfg, out_e, out_g, out_p, drain, push_photons=args.push_on_photon_stream,
-                     device_drain=args.device_drain)
-            if not args.push_on_photon_stream:
-                dev_g.push(cfg.dt_s)
+            conc.run(dev_e, dev_g, dev_p, cfg, out_e, out_g, out_p, drain, device_drain=args.device_drain)
         else:
             apply.apply_electrons(tables, coupling, dev_e, cfg, out_e, order_secondaries=True, check=False)
             apply.apply_photons(tables, coupling, dev_g, cfg, out_g, order_secondaries=True, check=False)
             apply.apply_positrons(tables, coupling, dev_p, cfg, out_p, order_secondaries=True, check=False)
             apply.drain_secondaries(tables, coupling, cfg, [out_e, out_g, out_p], drain, check=False)
-            dev_g.push(cfg.dt_s)
+        # AddNParticles of the drain's survivors (RreaSecondaryCommit.cpp:594-629),
+        # then WarpX's photon push for the next step, newborn photons included
+        _lib.check(L.rrea_commit_survivors(drain.survivors.data_ptr(), drain.survivor_count.data_ptr(),
+                                           store_structs, counts.data_ptr(), next_id.data_ptr(), 0,
+                                           out_e.error.data_ptr(), torch.cuda.current_stream().cuda_stream),
+                   "rrea_commit_survivors")
+        dev_g.push(cfg.dt_s)
 
-    def run_step(s):
-        # s counts from 0 (the first sort); the RNG step index continues past the tuning steps
-        every = knobs["sort_every"]
-        one_step(knobs["first_step"] + s, every > 0 and s % every == 0, knobs["energy_bits"])
+    def run_step(s, sort_now):
+        # the RNG step index continues past the tuning steps
+        one_step(knobs["first_step"] + s, sort_now, knobs["energy_bits"])
 
     def live():
         return tuple(int((t.id != _lib.INVALID_ID).sum().item()) for t in (dev_e, dev_g.cur, dev_p))
@@ -610,8 +686,9 @@ def main():
         log(f"autotune: sort every {choice.sort_every} steps, {choice.energy_bits} band bits "
             f"({choice.ms_per_step:.3f} ms/step)")
     log(f"warmup {args.warmup}")
+    every = knobs["sort_every"]
     for s in range(args.warmup):
-        run_step(s)
+        run_step(s, every > 0 and s == 0)
     torch.cuda.synchronize()
     out_e.check_error()
     e0, g0, p0 = live()
@@ -621,8 +698,14 @@ def main():
     log(f"timed {args.steps} steps")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    # the timed region opens with a tile sort and sorts every `every` steps
+    # after it, so it holds ceil(steps / every) whole sorts: at least its
+    # amortised share of them
+    sorts_timed = 0
     for s in range(args.steps):
-        run_step(args.warmup + s)
+        sort_now = every > 0 and s % every == 0
+        sorts_timed += int(sort_now)
+        run_step(args.warmup + s, sort_now)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     kms = {}
@@ -679,13 +762,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle, {threads} threads, {args.cpu_particles} particles x {args.cpu_steps} steps")
-        rate, secs, done = cpu_baseline(args.cpu_particles, args.cpu_steps, threads)
-        cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_particles} C2 particles (e-:gamma:e+ = 70:25:5) x {args.cpu_steps} steps "
-                         f"({done} particle-steps, {secs:.1f} s), oracle restatement, OpenMP dynamic over "
-                         f"64-particle chunks"}
+        cpu = cpu_baseline(args)
 
     line = {
         "metric": "particle-steps/s on 1 MI355X, plain vs batched, at 1e-12 step match; % HBM peak",
@@ -706,10 +783,15 @@ def main():
                    "electrons": n_e, "photons": n_g, "positrons": n_p, "path": "plain",
                    "species_passes": "sequential" if args.sequential else "concurrent (three streams)",
                    "tile_sort_every": knobs["sort_every"], "tile_sort_energy_bits": knobs["energy_bits"],
+                   "tile_sorts_in_timed_region": sorts_timed,
                    "autotune": tuned,
                    "kernel_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in kms.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": ("committed rocprofv3 FETCH_SIZE / WRITE_SIZE record of this kernel on this "
+                                        "workload (profiles/electron_kernel_traffic.json), not measured in this run"
+                                        if traffic is not None else None),
+                     "valu": valu_record(),
                      "kernel": "electron_fast_kernel", "alg_bytes_per_launch": alg_bytes,
                      "mean_launch_ms": mean_launch_s * 1e3, "isolated": iso},
         "cpu_baseline": cpu,

@@ -256,7 +256,8 @@ class BatchedAvalanche:
     rrea_newborn_drain_batched, rrea_commit_survivors).  ``step()`` runs one
     from the host; ``graph(steps)`` captures ``steps`` whole steps, commit
     included, into one HIP graph (rrea_batch_graph_* with the stores'
-    counts), whose launches grow the stores on the device."""
+    counts).  Either way the stores grow on the device; ``sync_counts()``
+    reads their lengths back."""
 
     def __init__(self, tables: InteractionTables, fields_list, electrons_list, seeds, cfg: StepConfig,
                  capacity=(1 << 20, 1 << 16, 1 << 18), secondary_capacity: int = 1 << 16,
@@ -282,7 +283,11 @@ class BatchedAvalanche:
             self.out.append(StepOutputs(0, f0, secondary_capacity=secondary_capacity, per_particle=False,
                                         share_grid=self.out[0], device=device))
         self.drain = DrainOutputs(fifo_capacity or 4 * secondary_capacity, self.out[0], device=device)
+        self._stores = (_lib.ParticleStore * 3)(*[s.struct() for s in self.stores])
         self.step_index = cfg.step
+
+    def check_error(self) -> None:
+        self.out[0].check_error()
 
     @property
     def electrons(self) -> ParticleStore:
@@ -303,24 +308,21 @@ class BatchedAvalanche:
         self.grids.zero_()
         for o in self.out:
             o.secondary_count.zero_()
+        # every step runs over the stores' whole capacity: rows past a store's
+        # length hold the invalid id and are skipped, so the commit grows the
+        # stores on the device and the step needs no host copy of their lengths
         ph, el, po = self.photons, self.electrons, self.positrons
-        if ph.n:
-            ph.photon_tile().push(cfg.dt_s, stream)
+        ph.full_photon_tile().push(cfg.dt_s, stream)
         e_out, g_out, p_out = self.out
         for name, store, o in (("electron", el, e_out), ("photon", ph, g_out), ("positron", po, p_out)):
-            if store.n:
-                tile = store.photon_tile() if store.photon else store.tile()
-                self.batch.step(self.tables, name, tile, store.inst[: store.n], cfg, o, stream=stream, check=False)
+            if store.capacity:
+                tile = store.full_photon_tile() if store.photon else store.full_tile()
+                self.batch.step(self.tables, name, tile, store.inst, cfg, o, stream=stream, check=False)
         self.batch.drain(self.tables, cfg, [e_out, g_out, p_out], self.drain, stream=stream, check=False)
-        stores = (_lib.ParticleStore * 3)(*[s.struct() for s in self.stores])
         L = _lib.lib()
         _lib.check(L.rrea_commit_survivors(self.drain.survivors.data_ptr(), self.drain.survivor_count.data_ptr(),
-                                           stores, self.counts.data_ptr(), self.next_id.data_ptr(), 0,
+                                           self._stores, self.counts.data_ptr(), self.next_id.data_ptr(), 0,
                                            e_out.error.data_ptr(), stream), "rrea_commit_survivors")
-        counts = self.counts.cpu().numpy()
-        e_out.check_error()
-        for s, c in zip(self.stores, counts):
-            s.n = int(c)
         self.step_index += 1
 
     def graph(self, steps: int, record_capacity: int | None = None, max_generations: int = 4):
@@ -343,4 +345,6 @@ class BatchedAvalanche:
             s.n = int(c)
 
     def live(self) -> list:
+        """Live particles per store (electrons, positrons, photons), lengths read back first."""
+        self.sync_counts()
         return [s.live() for s in self.stores]

@@ -201,6 +201,8 @@ def test_graph_avalanche_matches_host_driven_steps(tables, dt):
         g.launch(3 * launch)
         torch.cuda.synchronize()
         g.check_error()
+        host.check_error()
+        host.sync_counts()
         dev.sync_counts()
         assert torch.equal(host.counts, dev.counts), (launch, host.counts, dev.counts)
         assert torch.equal(host.next_id, dev.next_id)
