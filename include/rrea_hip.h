@@ -289,6 +289,50 @@ rrea_status rrea_transport_step_photons(rrea_tables_t tables, rrea_fields_t fiel
                                         const rrea_step_config* config, const rrea_step_out* out,
                                         rrea_stream_t stream);
 
+/* Secondary-group conservation.  Replaces RreaValidateSecondaryGroupConservation
+ * (warpx_overlay/Source/Rrea/RreaParticleInteraction.H:1099-1251) over the
+ * RreaSecondaryGroup of :777-806 and its closure (:709-717).  The transport
+ * kernels run the same check on every interaction they make (hard Moller,
+ * hard Bhabha, brems, annihilation, Compton, photoelectric, pair) and report
+ * a violation as RREA_ERR_INVALID, which stops the commit of the step's
+ * survivors (rrea_commit_survivors commits nothing once the error word is
+ * set), as the reference refuses to commit (RreaSecondaryCommit.cpp:240-265).
+ * This entry point checks caller-built groups, one result per group. */
+typedef struct rrea_group_entry {
+    int32_t kind;        /* a particle: species 0 e-, 1 e+, 2 photon; a material source:
+                            0 ionization event, 1 positive ion, 2 low-energy electron */
+    int32_t reserved;
+    double weight;       /* macro weight (a source's event weight) */
+    double energy_eV;    /* kinetic (photon) energy; a source's secondary energy */
+} rrea_group_entry;
+
+typedef struct rrea_secondary_group {
+    int32_t process;            /* RreaSecondaryProcess id: 0 continuous transport, 1 seed (no closure
+                                   required), 2 hard Moller .. 12 low-energy demotion */
+    int32_t parent_species;     /* 0 e-, 1 e+, 2 photon */
+    int32_t remove_parent;
+    int32_t parent_still_valid; /* the parent was still valid at preflight */
+    int32_t closure_mode;       /* 0 missing, 1 detailed, 2 explicit zero */
+    int32_t parent_survives;
+    double parent_weight;
+    double parent_energy_before_eV;
+    double parent_energy_after_eV;
+    double local_energy_eV;     /* weighted local, recoil, binding and sub-cutoff energy */
+    double escaped_energy_eV;   /* weighted */
+    double escaped_charge_e;
+    int32_t n_particles;        /* 0..4 */
+    int32_t n_sources;          /* 0..4 */
+    rrea_group_entry particles[4];
+    rrea_group_entry sources[4];
+} rrea_secondary_group;
+
+/* result (device int32 per group): bit 0 closure required, bit 1 present,
+ * bit 2 finite, bit 3 valid; residuals (device double[2] per group, may be
+ * NULL): charge (e) and energy (eV) residuals of a detailed closure. */
+rrea_status rrea_validate_secondary_groups(const rrea_secondary_group* groups, int64_t n,
+                                           double low_energy_cutoff_eV, int32_t* result, double* residuals,
+                                           rrea_stream_t stream);
+
 /* The species step without allocation.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory during a step (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
@@ -608,7 +652,10 @@ typedef struct rrea_particle_store {
  * atan2(y, x); previous state = the new state.  Ids are consecutive from
  * *next_id (device int64, advanced) in the reference's commit order:
  * electrons, then photons, then positrons; cpu is the caller's rank.  If a
- * store would overflow, nothing is written and error[0] = 5 (capacity). */
+ * store would overflow, nothing is written and error[0] = 5 (capacity).
+ * Fail-closed: when error[0] is already set (the step or the drain reported
+ * an error, a group that did not conserve among them), nothing is committed
+ * (RreaSecondaryCommit.cpp:240-265). */
 rrea_status rrea_commit_survivors(const rrea_secondary* survivors, const int64_t* survivor_count,
                                   const rrea_particle_store* stores, int64_t* counts, int64_t* next_id,
                                   int32_t cpu, int32_t* error, rrea_stream_t stream);

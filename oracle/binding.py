@@ -82,6 +82,30 @@ class InterleaveRecord(C.Structure):
                 ("final_fractions", C.c_double * 16), ("stops", C.c_double * 16), ("kinetic_end", C.c_double)]
 
 
+class GroupEntry(C.Structure):
+    """orc_group_entry: a particle (kind = species) or a material source."""
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("weight", C.c_double), ("energy_eV", C.c_double)]
+
+
+class Group(C.Structure):
+    """orc_group: one interaction group and its conservation closure."""
+    _fields_ = [("process", C.c_int32), ("parent_species", C.c_int32), ("remove_parent", C.c_int32),
+                ("parent_still_valid", C.c_int32), ("closure_mode", C.c_int32), ("parent_survives", C.c_int32),
+                ("parent_weight", C.c_double), ("parent_energy_before_eV", C.c_double),
+                ("parent_energy_after_eV", C.c_double), ("local_energy_eV", C.c_double),
+                ("escaped_energy_eV", C.c_double), ("escaped_charge_e", C.c_double),
+                ("n_particles", C.c_int32), ("n_sources", C.c_int32),
+                ("particles", GroupEntry * 4), ("sources", GroupEntry * 4)]
+
+
+def validate_group(g: Group, low_energy_cutoff_eV: float):
+    """orc_validate_group: (result bits 1 required, 2 present, 4 finite,
+    8 valid; charge residual; energy residual)."""
+    res = np.zeros(2)
+    bits = lib().orc_validate_group(C.byref(g), low_energy_cutoff_eV, _ptr(res))
+    return int(bits), float(res[0]), float(res[1])
+
+
 def build(quiet: bool = True) -> None:
     """Compile the oracle (and the reference pins when the checkout exists)."""
     out = subprocess.DEVNULL if quiet else None
@@ -171,6 +195,8 @@ def lib():
         L.orc_sample_compton.restype = C.c_int
         L.orc_sample_photoelectric.argtypes = [C.c_void_p, C.c_double, C.c_double, P_D]
         L.orc_sample_photoelectric.restype = C.c_int
+        L.orc_validate_group.argtypes = [C.POINTER(Group), C.c_double, P_D]
+        L.orc_validate_group.restype = C.c_int
         L.orc_sample_photoelectron_cos.argtypes = [C.c_void_p, C.c_double, C.c_double, P_I32]
         L.orc_sample_photoelectron_cos.restype = C.c_double
         L.orc_sample_photon_channel.argtypes = [C.c_void_p, C.c_double, C.c_double]

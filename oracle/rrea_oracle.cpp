@@ -1140,6 +1140,91 @@ inline void photon_recoil(double k0, double eg, double gx, double gy, double gz,
 }
 
 // ---------------------------------------------------------------------------
+// Secondary-group conservation -- RreaValidateSecondaryGroupConservation
+// OV/RreaParticleInteraction.H:1099-1251: charge (e x weight) and total
+// energy (kinetic + rest, weighted eV) of one interaction, input against
+// output, each within 64 eps of its throughput.  A neutral target electron
+// consumed by an ionization event or a positive ion brings its rest energy
+// in and leaves +e; a sub-cutoff ionization, or a low-energy electron
+// source, releases -e and that rest energy into the fluid.
+// ---------------------------------------------------------------------------
+inline double species_q(int s) { return s == 0 ? -1.0 : s == 1 ? 1.0 : 0.0; }
+inline double species_m(int s) { return s == 2 ? 0.0 : kMeC2; }
+
+struct Closure {
+    // 'through' holds the particles' and sources' charge magnitudes; the
+    // parent enters as |q_in| + |q_out + q_escaped|
+    double in_q = 0, out_q = 0, in_e = 0, out_e = 0, through = 0;
+    double parent_in_q = 0, parent_out_q = 0, escaped_q = 0;
+    bool finite = true;
+    void parent(int s, double w, double before, double after, bool survives)
+    {
+        finite = finite && std::isfinite(w) && std::isfinite(before) && std::isfinite(after) && w > 0.0
+            && before >= 0.0 && after >= 0.0;
+        double const q = species_q(s) * w;
+        in_q += q;
+        out_q += survives ? q : 0.0;
+        in_e += w * (before + species_m(s));
+        out_e += survives ? w * (after + species_m(s)) : 0.0;
+        parent_in_q = q;
+        parent_out_q = survives ? q : 0.0;
+    }
+    void local(double e) { finite = finite && std::isfinite(e) && e >= 0.0; out_e += e; }
+    void escaped(double e, double q)
+    {
+        finite = finite && std::isfinite(e) && std::isfinite(q) && e >= 0.0;
+        out_e += e;
+        out_q += q;
+        escaped_q += q;
+    }
+    // a group record's local and escaped terms, summed as one output
+    // (RreaParticleInteraction.H:1172-1179)
+    void residual_terms(double local, double esc_e, double esc_q)
+    {
+        finite = finite && std::isfinite(local) && std::isfinite(esc_e) && std::isfinite(esc_q) && local >= 0.0
+            && esc_e >= 0.0;
+        out_q += esc_q;
+        escaped_q += esc_q;
+        out_e += local + esc_e;
+    }
+    void particle(int s, double w, double e)
+    {
+        finite = finite && std::isfinite(w) && std::isfinite(e) && w > 0.0 && e >= 0.0;
+        out_q += species_q(s) * w;
+        through += std::abs(species_q(s) * w);
+        out_e += w * (e + species_m(s));
+    }
+    void ionization(double w, double secondary, double cutoff)
+    {
+        finite = finite && std::isfinite(w) && std::isfinite(secondary) && w > 0.0;
+        in_e += w * kMeC2;
+        out_q += w;
+        through += w;
+        if (secondary < cutoff) { out_q -= w; through += w; out_e += w * kMeC2; }
+    }
+    void positive_ion(double w) { finite = finite && std::isfinite(w) && w > 0.0; in_e += w * kMeC2; out_q += w; through += w; }
+    void low_electron(double w) { finite = finite && std::isfinite(w) && w > 0.0; out_q -= w; through += w; out_e += w * kMeC2; }
+    // 0 valid, 1 non-finite or out of range, 2 charge, 3 energy
+    int verdict(double cutoff, double* res = nullptr) const
+    {
+        if (!(finite && std::isfinite(cutoff) && cutoff > 0.0 && std::isfinite(in_q) && std::isfinite(out_q)
+              && std::isfinite(in_e) && std::isfinite(out_e))) return 1;
+        double const eps = std::numeric_limits<double>::epsilon();
+        if (res) { res[0] = out_q - in_q; res[1] = out_e - in_e; }
+        double const q_through = std::abs(parent_in_q) + std::abs(parent_out_q + escaped_q) + through;
+        if (std::abs(out_q - in_q) > 64.0 * eps * std::max(1.0, q_through)) return 2;
+        if (std::abs(out_e - in_e) > 64.0 * eps * std::max({1.0, std::abs(in_e), std::abs(out_e)})) return 3;
+        return 0;
+    }
+};
+
+// the transport's own check of a group it just made (fails the step)
+inline void require_conserved(Closure const& c, double cutoff)
+{
+    if (c.verdict(cutoff) != 0) fail(ORC_ERR_INVALID, "interaction group does not conserve charge and energy");
+}
+
+// ---------------------------------------------------------------------------
 // Two-photon positron annihilation -- NA/src/RreaInteractionTables.cpp
 // ---------------------------------------------------------------------------
 // clamp_unit / wrap_unit_interval :43-61
@@ -1660,6 +1745,15 @@ inline void spawn_annihilation(StepContext const& ctx, Emitter& em, int32_t proc
     em.positive_ion(r, z, em.w, 0.0);
     em.fx.tallies[ORC_T_POSITIVE_IONS] += em.w;
     em.fx.tallies[ORC_T_ANNIHILATIONS] += 1.0;
+    if (em.w > 0.0) {
+        Closure c;
+        c.parent(1, em.w, std::max(k, 0.0), 0.0, false);
+        if (e1 >= ctx.cfg.photon_cutoff_eV) c.particle(2, em.w, e1);
+        if (e2 >= ctx.cfg.photon_cutoff_eV) c.particle(2, em.w, e2);
+        c.local(em.w * ((at_rest && k > 0.0 ? k : 0.0) + local));
+        c.positive_ion(em.w);
+        require_conserved(c, ctx.cfg.low_energy_cutoff_eV);
+    }
 }
 
 // mean_free_path_from_inverse_length NA/src/RreaInteractionTables.cpp:110-119
@@ -2184,6 +2278,11 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                     ion_event(w);
                     em.ionization(std::hypot(ex_, ey_), ez_, w, T, T);
                     fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
+                    Closure c;
+                    c.parent(0, w, before, k, true);
+                    c.particle(0, w, T);
+                    c.ionization(w, T, cutoff);
+                    require_conserved(c, cutoff);
                 } else if (pos && channel == 0) {
                     // RreaApplyBhabhaFinalState OV/RreaChargedFinalStates.H:238-297: a
                     // secondary below the production threshold discards this event only
@@ -2201,6 +2300,11 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                         em.positive_ion(std::hypot(ex_, ey_), ez_, w, T);
                         fx.tallies[ORC_T_POSITIVE_IONS] += w;
                         fx.tallies[ORC_T_BHABHA_EVENTS] += 1.0;
+                        Closure c;
+                        c.parent(1, w, before, k, true);
+                        c.particle(0, w, T);
+                        c.positive_ion(w);
+                        require_conserved(c, cutoff);
                     }
                 } else if (channel == 1) {
                     // RreaApplyBremsFinalState OV/RreaChargedFinalStates.H:329-394
@@ -2227,6 +2331,13 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                         deposit_local(local, std::hypot(ex_, ey_), ez_);
                     }
                     fx.tallies[ORC_T_BREMS_EVENTS] += 1.0;
+                    {
+                        Closure c;
+                        c.parent(pos ? 1 : 0, w, before, after, true);
+                        if (tracked) c.particle(2, w, pe);
+                        else c.local(w * local);
+                        require_conserved(c, cutoff);
+                    }
                 } else if (pos && channel == 2) {
                     // in-flight annihilation, OV/RreaPositronTransport.cpp:710-745: the
                     // parent is consumed at the event point
@@ -2663,6 +2774,16 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
             em.deposit_local(st.recoil, sr, z);
         }
         fx.tallies[ORC_T_COMPTON_EVENTS] += 1.0;
+        {
+            Closure c;
+            bool const survives = !(st.scattered < gcut);
+            c.parent(2, w, incident, st.scattered, survives);
+            if (!survives) c.local(w * st.scattered);
+            if (st.recoil >= cutoff) c.particle(0, w, st.recoil);
+            else c.local(w * st.recoil);
+            c.ionization(w, st.recoil, cutoff);
+            require_conserved(c, cutoff);
+        }
         energy = st.scattered;
         if (energy < gcut) {
             em.deposit_local(energy, sr, z);
@@ -2688,6 +2809,13 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
         }
         if (st.binding > 0.0) em.deposit_local(st.binding, sr, z);
         fx.tallies[ORC_T_PHOTOELECTRIC_EVENTS] += 1.0;
+        Closure c;
+        c.parent(2, w, energy, 0.0, false);
+        if (ke >= cutoff) c.particle(0, w, ke);
+        else c.local(w * ke);
+        c.ionization(w, ke, cutoff);
+        c.local(w * std::max(st.binding, 0.0));
+        require_conserved(c, cutoff);
         return false;
     }
     PairState const pair = sample_pair(t, energy, channel, U(kChPairInvariant), U(kChPairAngle), U(kChPairAzimuth));
@@ -2720,6 +2848,22 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
     }
     if (pair.local > 0.0) em.deposit_local(pair.local, sr, z);
     fx.tallies[ORC_T_PAIR_EVENTS] += 1.0;
+    {
+        // a sub-cutoff positron leaves as the particle its at-rest
+        // annihilation (a group checked of its own) consumes
+        Closure c;
+        c.parent(2, w, energy, 0.0, false);
+        if (pair.electron >= cutoff) c.particle(0, w, pair.electron);
+        else { c.low_electron(w); c.local(w * pair.electron); }
+        c.particle(1, w, pair.positron);
+        if (pair.has_recoil) {
+            if (pair.recoil >= cutoff) c.particle(0, w, pair.recoil);
+            else c.local(w * pair.recoil);
+            c.ionization(w, pair.recoil, cutoff);
+        }
+        c.local(w * std::max(pair.local, 0.0));
+        require_conserved(c, cutoff);
+    }
     return false;
 }
 
@@ -3108,6 +3252,11 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                     em.ion_event(w);
                     em.ionization(std::hypot(ex_, ey_), ez_, w, T, T);
                     fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
+                    Closure c;
+                    c.parent(0, w, before, k, true);
+                    c.particle(0, w, T);
+                    c.ionization(w, T, cutoff);
+                    require_conserved(c, cutoff);
                 } else if (pos && channel == 0) {
                     double const before = k;
                     double const T = std::min(t.sample_bhabha(before, rng_uniform(seed, root, step, idx, kChPositronBhabhaEnergy)),
@@ -3123,6 +3272,11 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                         em.positive_ion(std::hypot(ex_, ey_), ez_, w, T);
                         fx.tallies[ORC_T_POSITIVE_IONS] += w;
                         fx.tallies[ORC_T_BHABHA_EVENTS] += 1.0;
+                        Closure c;
+                        c.parent(1, w, before, k, true);
+                        c.particle(0, w, T);
+                        c.positive_ion(w);
+                        require_conserved(c, cutoff);
                     }
                 } else if (channel == 1) {
                     double const before = k;
@@ -3147,6 +3301,13 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                         em.deposit_local(local, std::hypot(ex_, ey_), ez_);
                     }
                     fx.tallies[ORC_T_BREMS_EVENTS] += 1.0;
+                    {
+                        Closure c;
+                        c.parent(pos ? 1 : 0, w, before, after, true);
+                        if (tracked) c.particle(2, w, pe);
+                        else c.local(w * local);
+                        require_conserved(c, cutoff);
+                    }
                 } else if (pos && channel == 2) {
                     // in flight: the positron is consumed at the event point
                     spawn_annihilation(ctx, em, kProcAnnihilation, ex_, ey_, ez_, std::hypot(ex_, ey_), dx, dy, dz, k,
@@ -4273,3 +4434,46 @@ int orc_outside_certified(double r_lo, double r_hi, double z_lo, double z_hi, do
 }
 
 }  // extern "C"
+
+// RreaValidateSecondaryGroupConservation, OV/RreaParticleInteraction.H:1099-1252
+extern "C" int orc_validate_group(const orc_group* g, double cutoff, double* res)
+{
+    if (res) res[0] = res[1] = 0.0;
+    bool const required = !(g->process == 0 || g->process == 1);
+    if (!required) return 1 | 2 | 4 | 8;
+    int out = 1;
+    if (g->closure_mode == 0) return out;
+    out |= 2;
+    if (g->closure_mode == 2) {
+        bool const zero = g->n_particles == 0 && g->n_sources == 0 && g->remove_parent && !g->parent_still_valid
+            && !g->parent_survives && g->parent_weight == 0.0 && g->parent_energy_before_eV == 0.0
+            && g->parent_energy_after_eV == 0.0 && g->local_energy_eV == 0.0 && g->escaped_energy_eV == 0.0
+            && g->escaped_charge_e == 0.0;
+        return zero ? (out | 4 | 8) : out;
+    }
+    if (g->closure_mode != 1 || g->n_particles < 0 || g->n_particles > 4 || g->n_sources < 0 || g->n_sources > 4) {
+        return out;
+    }
+    Closure c;
+    c.finite = (g->parent_survives != 0) == (g->remove_parent == 0) && g->parent_species >= 0 && g->parent_species <= 2;
+    c.parent(g->parent_species, g->parent_weight, g->parent_energy_before_eV, g->parent_energy_after_eV,
+             g->parent_survives != 0);
+    c.residual_terms(g->local_energy_eV, g->escaped_energy_eV, g->escaped_charge_e);
+    for (int i = 0; i < g->n_particles; ++i) {
+        orc_group_entry const& p = g->particles[i];
+        c.finite = c.finite && p.kind >= 0 && p.kind <= 2;
+        c.particle(p.kind, p.weight, p.energy_eV);
+    }
+    for (int i = 0; i < g->n_sources; ++i) {
+        orc_group_entry const& e = g->sources[i];
+        c.finite = c.finite && std::isfinite(e.energy_eV);
+        if (e.kind == 0) c.ionization(e.weight, e.energy_eV, cutoff);
+        else if (e.kind == 1) c.positive_ion(e.weight);
+        else if (e.kind == 2) c.low_electron(e.weight);
+        else c.finite = false;
+    }
+    int const v = c.verdict(cutoff, res);
+    if (v == 1) return out;
+    out |= 4;
+    return v == 0 ? (out | 8) : out;
+}

@@ -126,6 +126,30 @@ enum {
     ORC_ERR_SECONDARY_CAPACITY = 5,
 };
 
+/* One interaction group and its conservation closure (RreaSecondaryGroup,
+   OV/RreaParticleInteraction.H:709-717, 777-806): entries are particles
+   (kind = species 0 e-, 1 e+, 2 photon) or material sources (kind 0
+   ionization, 1 positive ion, 2 low-energy electron). */
+typedef struct orc_group_entry {
+    int32_t kind;
+    int32_t reserved;
+    double weight;
+    double energy_eV;
+} orc_group_entry;
+
+typedef struct orc_group {
+    int32_t process, parent_species, remove_parent, parent_still_valid, closure_mode, parent_survives;
+    double parent_weight, parent_energy_before_eV, parent_energy_after_eV;
+    double local_energy_eV, escaped_energy_eV, escaped_charge_e;
+    int32_t n_particles, n_sources;
+    orc_group_entry particles[4];
+    orc_group_entry sources[4];
+} orc_group;
+
+/* RreaValidateSecondaryGroupConservation: bit 0 required, 1 present, 2 finite,
+   3 valid; res[2] = charge, energy residuals (may be NULL). */
+int orc_validate_group(const orc_group* g, double low_energy_cutoff_eV, double* res);
+
 /* One material-source record into g->sources: kind 0 ionization event,
    1 positive ion only, 2 low-energy electron (RreaWarpXCoupling::
    Accumulate*, OV/RreaCouplingSources.cpp:435-516).  0, or the error code of

@@ -138,6 +138,20 @@ class BatchGraphDesc(C.Structure):
     ]
 
 
+class GroupEntry(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("weight", C.c_double), ("energy_eV", C.c_double)]
+
+
+class SecondaryGroup(C.Structure):
+    _fields_ = [("process", C.c_int32), ("parent_species", C.c_int32), ("remove_parent", C.c_int32),
+                ("parent_still_valid", C.c_int32), ("closure_mode", C.c_int32), ("parent_survives", C.c_int32),
+                ("parent_weight", C.c_double), ("parent_energy_before_eV", C.c_double),
+                ("parent_energy_after_eV", C.c_double), ("local_energy_eV", C.c_double),
+                ("escaped_energy_eV", C.c_double), ("escaped_charge_e", C.c_double),
+                ("n_particles", C.c_int32), ("n_sources", C.c_int32),
+                ("particles", GroupEntry * 4), ("sources", GroupEntry * 4)]
+
+
 class PopulationRecords(C.Structure):
     _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in (
         "r", "z", "theta", "ux", "uy", "uz", "w", "stable_id", "stable_id_high", "candidate", "id")]
@@ -175,6 +189,8 @@ SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
     ("rrea_transport_step_noalloc", C.c_int32,
      [C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(Tile), C.POINTER(StepConfigC), C.POINTER(StepOut), C.c_void_p]),
+    ("rrea_validate_secondary_groups", C.c_int32,
+     [C.c_void_p, C.c_int64, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rrea_photon_push", C.c_int32, [C.POINTER(Tile), C.c_double, C.c_void_p]),
     ("rrea_tile_sort", C.c_int32, [C.c_void_p, C.POINTER(Tile), C.c_int32, C.c_void_p]),
     ("rrea_tile_sort_banded", C.c_int32, [C.c_void_p, C.POINTER(Tile), C.c_int32, C.c_int32, C.c_void_p]),

@@ -90,6 +90,118 @@ enum : int32_t {
     kProcPairNuclear = 7, kProcPairTriplet = 8, kProcAnnihilation = 9,
 };
 
+// ---------------------------------------------------------------------------
+// Secondary-group conservation -- RreaValidateSecondaryGroupConservation
+// OV/RreaParticleInteraction.H:1099-1251.  One interaction's charge (in e x
+// macro weight) and total energy (kinetic + rest, macro-weighted eV) before
+// and after: the parent before and, if it survives, after; its particles;
+// its material sources (an ionization event or a positive ion consumes one
+// neutral target electron: its rest energy in, +e out; a sub-cutoff
+// ionization also releases that electron into the fluid: -e and its rest
+// energy out; a low-energy electron source: -e and its rest energy out);
+// the weighted local deposit and escaped energy and charge.  Valid when
+// every input is finite and in range and both residuals are within
+// 64 eps x max(1, throughput).  Species: 0 e-, 1 e+, 2 photon.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline double species_charge(int species) { return species == 0 ? -1.0 : species == 1 ? 1.0 : 0.0; }
+__host__ __device__ inline double species_rest(int species) { return species == 2 ? 0.0 : kMeC2; }
+
+struct GroupClosure {
+    // q_through holds the particles' and sources' charge magnitudes; the
+    // parent's input and its output plus any escaped charge enter as the two
+    // magnitudes |q_in| + |q_out + q_escaped|.
+    double in_q = 0.0, out_q = 0.0, in_e = 0.0, out_e = 0.0, q_through = 0.0;
+    double parent_in_q = 0.0, parent_out_q = 0.0, escaped_q = 0.0;
+    bool ok = true;
+
+    __host__ __device__ static bool fin(double v) { return v == v && v - v == 0.0; }
+
+    __host__ __device__ void parent(int species, double w, double before, double after, bool survives)
+    {
+        ok = ok && fin(w) && fin(before) && fin(after) && w > 0.0 && before >= 0.0 && after >= 0.0;
+        double const q = species_charge(species) * w;
+        double const m = species_rest(species);
+        in_q += q;
+        double const oq = survives ? q : 0.0;
+        out_q += oq;
+        in_e += w * (before + m);
+        out_e += survives ? w * (after + m) : 0.0;
+        parent_in_q = q;
+        parent_out_q = oq;
+    }
+    __host__ __device__ void local(double weighted_eV)
+    {
+        ok = ok && fin(weighted_eV) && weighted_eV >= 0.0;
+        out_e += weighted_eV;
+    }
+    __host__ __device__ void escaped(double weighted_eV, double charge_e)
+    {
+        ok = ok && fin(weighted_eV) && fin(charge_e) && weighted_eV >= 0.0;
+        out_e += weighted_eV;
+        out_q += charge_e;
+        escaped_q += charge_e;
+    }
+    // a group record's local and escaped terms, summed as one output
+    __host__ __device__ void residual_terms(double local_eV, double escaped_eV, double charge_e)
+    {
+        ok = ok && fin(local_eV) && fin(escaped_eV) && fin(charge_e) && local_eV >= 0.0 && escaped_eV >= 0.0;
+        out_q += charge_e;
+        escaped_q += charge_e;
+        out_e += local_eV + escaped_eV;
+    }
+    __host__ __device__ void particle(int species, double w, double e)
+    {
+        ok = ok && fin(w) && fin(e) && w > 0.0 && e >= 0.0;
+        double const q = species_charge(species) * w;
+        out_q += q;
+        q_through += q < 0.0 ? -q : q;
+        out_e += w * (e + species_rest(species));
+    }
+    __host__ __device__ void ionization(double w, double secondary_eV, double cutoff)
+    {
+        ok = ok && fin(w) && fin(secondary_eV) && w > 0.0;
+        in_e += w * kMeC2;
+        out_q += w;
+        q_through += w;
+        if (secondary_eV < cutoff) {
+            out_q -= w;
+            q_through += w;
+            out_e += w * kMeC2;
+        }
+    }
+    __host__ __device__ void positive_ion(double w)
+    {
+        ok = ok && fin(w) && w > 0.0;
+        in_e += w * kMeC2;
+        out_q += w;
+        q_through += w;
+    }
+    __host__ __device__ void low_electron(double w)
+    {
+        ok = ok && fin(w) && w > 0.0;
+        out_q -= w;
+        q_through += w;
+        out_e += w * kMeC2;
+    }
+    // 0: valid; otherwise why not (1 non-finite or out of range, 2 charge, 3 energy)
+    __host__ __device__ int verdict(double cutoff, double* q_res = nullptr, double* e_res = nullptr) const
+    {
+        if (!(ok && fin(cutoff) && cutoff > 0.0 && fin(in_q) && fin(out_q) && fin(in_e) && fin(out_e))) return 1;
+        double const dq = out_q - in_q;
+        double const de = out_e - in_e;
+        if (q_res) *q_res = dq;
+        if (e_res) *e_res = de;
+        double const po = parent_out_q + escaped_q;
+        double const through = (parent_in_q < 0.0 ? -parent_in_q : parent_in_q) + (po < 0.0 ? -po : po) + q_through;
+        double const tq = 64.0 * kEps * (through > 1.0 ? through : 1.0);
+        double const ai = in_e < 0.0 ? -in_e : in_e, ao = out_e < 0.0 ? -out_e : out_e;
+        double const te = 64.0 * kEps * (ai > ao ? (ai > 1.0 ? ai : 1.0) : (ao > 1.0 ? ao : 1.0));
+        if ((dq < 0.0 ? -dq : dq) > tq) return 2;
+        if ((de < 0.0 ? -de : de) > te) return 3;
+        return 0;
+    }
+};
+
 constexpr uint64_t kSegmentCap = uint64_t{1} << 14;  // OV/RreaParticleInteraction.H:642-650
 constexpr uint64_t kEventCap = uint64_t{1} << 13;
 

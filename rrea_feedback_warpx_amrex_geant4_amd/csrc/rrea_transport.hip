@@ -430,6 +430,15 @@ __device__ __forceinline__ void push_secondary(StepArgs const& A, rrea_secondary
     A.sec[slot] = s;
 }
 
+// The transport's own run of RreaValidateSecondaryGroupConservation
+// (OV/RreaParticleInteraction.H:1099-1251) on an interaction it just made: a
+// group whose charge or energy does not close fails the step with
+// RREA_ERR_INVALID, and nothing of the step is committed.
+__device__ __forceinline__ void check_group(GroupClosure const& c, double cutoff, int32_t& err)
+{
+    if (c.verdict(cutoff) != 0 && !err) err = kErrInvalid;
+}
+
 // The per-particle side-effect sink of one transaction: local deposits as
 // E / W_air ion pairs (deposit_local_energy_as_ion_pairs
 // OV/RreaTransportStep.H:537-561), ion-pair events, and secondaries with the
@@ -530,6 +539,15 @@ __device__ void spawn_annihilation(DevTables const& T, double gcut, uint64_t rng
     em.positive_ion(sr, z, em.w, 0.0);
     tally[12] += em.w;
     tally[10] += 1.0;
+    {
+        GroupClosure c;
+        c.parent(1, em.w, dmax(k, 0.0), 0.0, false);
+        if (e1 >= gcut) c.particle(2, em.w, e1);
+        if (e2 >= gcut) c.particle(2, em.w, e2);
+        c.local(em.w * ((at_rest && k > 0.0 ? k : 0.0) + local));
+        c.positive_ion(em.w);
+        check_group(c, em.A.cfg.low_energy_cutoff_eV, em.err);
+    }
 }
 
 // RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:107-152 (electron)
@@ -1063,6 +1081,11 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                                         em.positive_ion(hypot(ex_, ey_), ez_, w, T2);
                                         tally[12] += w;
                                         tally[11] += 1.0;
+                                        GroupClosure c;
+                                        c.parent(1, w, before, k, true);
+                                        c.particle(0, w, T2);
+                                        c.positive_ion(w);
+                                        check_group(c, cutoff, err);
                                     }
                                 } else {
                                     // RreaApplyHardMollerFinalState OV/RreaChargedFinalStates.H:164-225
@@ -1077,6 +1100,11 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                                     ion += w;
                                     em.ionization(hypot(ex_, ey_), ez_, w, T2, T2);
                                     tally[3] += 1.0;
+                                    GroupClosure c;
+                                    c.parent(0, w, before, k, true);
+                                    c.particle(0, w, T2);
+                                    c.ionization(w, T2, cutoff);
+                                    check_group(c, cutoff, err);
                                 }
                             } else if (h.channel == 1) {
                                 // RreaApplyBremsFinalState OV/RreaChargedFinalStates.H:329-394
@@ -1113,6 +1141,13 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                                     deposit_local(pe, hypot(ex_, ey_), ez_);
                                 }
                                 tally[4] += 1.0;
+                                {
+                                    GroupClosure c;
+                                    c.parent(kPos ? 1 : 0, w, before, after, true);
+                                    if (tracked) c.particle(2, w, pe);
+                                    else c.local(w * pe);
+                                    check_group(c, cutoff, err);
+                                }
                             } else if (kPos && h.channel == 2) {
                                 // in-flight annihilation, OV/RreaPositronTransport.cpp:710-745:
                                 // the parent is consumed at the event point
@@ -1336,6 +1371,16 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
             em.deposit_local(recoil, sr, z);
         }
         tally[13] += 1.0;
+        {
+            GroupClosure c;
+            bool const survives = !(scattered < gcut);
+            c.parent(2, w, incident, scattered, survives);
+            if (!survives) c.local(w * scattered);
+            if (recoil >= cutoff) c.particle(0, w, recoil);
+            else c.local(w * recoil);
+            c.ionization(w, recoil, cutoff);
+            check_group(c, cutoff, err);
+        }
         energy = scattered;
         if (energy < gcut) {
             em.deposit_local(energy, sr, z);
@@ -1366,6 +1411,13 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
         }
         if (binding > 0.0) em.deposit_local(binding, sr, z);
         tally[14] += 1.0;
+        GroupClosure c;
+        c.parent(2, w, energy, 0.0, false);
+        if (ke >= cutoff) c.particle(0, w, ke);
+        else c.local(w * ke);
+        c.ionization(w, ke, cutoff);
+        c.local(w * dmax(binding, 0.0));
+        check_group(c, cutoff, err);
         return false;
     }
     PairState const pair = sample_pair(T, energy, channel, rng_uniform(rng, idx, kChPairInvariant),
@@ -1400,6 +1452,22 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
     }
     if (pair.local > 0.0) em.deposit_local(pair.local, sr, z);
     tally[15] += 1.0;
+    {
+        // the pair group; a sub-cutoff positron goes on as the particle its
+        // at-rest annihilation (checked as its own group) consumes
+        GroupClosure c;
+        c.parent(2, w, energy, 0.0, false);
+        if (pair.electron >= cutoff) c.particle(0, w, pair.electron);
+        else { c.low_electron(w); c.local(w * pair.electron); }
+        c.particle(1, w, pair.positron);
+        if (pair.has_recoil) {
+            if (pair.recoil >= cutoff) c.particle(0, w, pair.recoil);
+            else c.local(w * pair.recoil);
+            c.ionization(w, pair.recoil, cutoff);
+        }
+        c.local(w * dmax(pair.local, 0.0));
+        check_group(c, cutoff, err);
+    }
     return false;
 }
 
@@ -2614,6 +2682,11 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                                         em.positive_ion(hypot(ex_, ey_), ez_, w, T2);
                                         tally[12] += w;
                                         tally[11] += 1.0;
+                                        GroupClosure c;
+                                        c.parent(1, w, before, k, true);
+                                        c.particle(0, w, T2);
+                                        c.positive_ion(w);
+                                        check_group(c, cutoff, err);
                                     }
                                 } else {
                                     // RreaApplyHardMollerFinalState OV/RreaChargedFinalStates.H:164-225
@@ -2628,6 +2701,11 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                                     ion += w;
                                     em.ionization(hypot(ex_, ey_), ez_, w, T2, T2);
                                     tally[3] += 1.0;
+                                    GroupClosure c;
+                                    c.parent(0, w, before, k, true);
+                                    c.particle(0, w, T2);
+                                    c.ionization(w, T2, cutoff);
+                                    check_group(c, cutoff, err);
                                 }
                             } else if (h.channel == 1) {
                                 // RreaApplyBremsFinalState OV/RreaChargedFinalStates.H:329-394
@@ -2664,6 +2742,13 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                                     em.deposit_local(pe, hypot(ex_, ey_), ez_);
                                 }
                                 tally[4] += 1.0;
+                                {
+                                    GroupClosure c;
+                                    c.parent(kPos ? 1 : 0, w, before, after, true);
+                                    if (tracked) c.particle(2, w, pe);
+                                    else c.local(w * pe);
+                                    check_group(c, cutoff, err);
+                                }
                             } else if (kPos && h.channel == 2) {
                                 // in flight: the positron is consumed at the event point
                                 spawn_annihilation(T, gcut, rng, em, kProcAnnihilation, ex_, ey_, ez_, dx, dy, dz, k, false,
@@ -3116,6 +3201,9 @@ __global__ void __launch_bounds__(1024) commit_kernel(const rrea_secondary* __re
     __shared__ int64_t running[3];
     __shared__ int32_t abort_flag;
     int32_t const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // fail closed: a step or drain that reported an error commits nothing
+    // (RreaSecondaryCommit.cpp:240-265 refuses the transaction)
+    if (__hip_atomic_load(&error[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     int64_t const n = *sv_count;
     if (tid < 3) { tot[tid] = 0ULL; running[tid] = 0; }
     if (tid == 0) abort_flag = 0;
@@ -3569,6 +3657,59 @@ __global__ void next_step_kernel(StepArgs* a0, StepArgs* a1, StepArgs* a2, Newbo
     if (a1) a1->cfg.step += 1;
     if (a2) a2->cfg.step += 1;
     if (nb) nb->S.cfg.step += 1;
+}
+
+// RreaValidateSecondaryGroupConservation over caller-built groups.
+__device__ int32_t validate_group(rrea_secondary_group const& g, double cutoff, double* res)
+{
+    bool const required = !(g.process == 0 || g.process == 1);  // continuous transport, seed
+    if (!required) return 1 | 2 | 4 | 8;
+    int32_t out = 1;
+    if (g.closure_mode == 0) return out;  // missing
+    out |= 2;
+    if (g.closure_mode == 2) {
+        // an explicit zero: a payload-free removal of a parent already invalid
+        bool const zero = g.n_particles == 0 && g.n_sources == 0 && g.remove_parent && !g.parent_still_valid
+            && !g.parent_survives && g.parent_weight == 0.0 && g.parent_energy_before_eV == 0.0
+            && g.parent_energy_after_eV == 0.0 && g.local_energy_eV == 0.0 && g.escaped_energy_eV == 0.0
+            && g.escaped_charge_e == 0.0;
+        return zero ? (out | 4 | 8) : out;
+    }
+    if (g.closure_mode != 1 || g.n_particles < 0 || g.n_particles > 4 || g.n_sources < 0 || g.n_sources > 4) {
+        return out;
+    }
+    GroupClosure c;
+    c.ok = (g.parent_survives != 0) == (g.remove_parent == 0) && g.parent_species >= 0 && g.parent_species <= 2;
+    c.parent(g.parent_species, g.parent_weight, g.parent_energy_before_eV, g.parent_energy_after_eV,
+             g.parent_survives != 0);
+    c.residual_terms(g.local_energy_eV, g.escaped_energy_eV, g.escaped_charge_e);
+    for (int i = 0; i < g.n_particles; ++i) {
+        rrea_group_entry const& p = g.particles[i];
+        c.ok = c.ok && p.kind >= 0 && p.kind <= 2;
+        c.particle(p.kind, p.weight, p.energy_eV);
+    }
+    for (int i = 0; i < g.n_sources; ++i) {
+        rrea_group_entry const& s = g.sources[i];
+        c.ok = c.ok && GroupClosure::fin(s.energy_eV);
+        if (s.kind == 0) c.ionization(s.weight, s.energy_eV, cutoff);
+        else if (s.kind == 1) c.positive_ion(s.weight);
+        else if (s.kind == 2) c.low_electron(s.weight);
+        else c.ok = false;
+    }
+    double qr = 0.0, er = 0.0;
+    int const v = c.verdict(cutoff, &qr, &er);
+    if (res) { res[0] = qr; res[1] = er; }
+    if (v == 1) return out;
+    out |= 4;
+    return v == 0 ? (out | 8) : out;
+}
+
+__global__ void validate_groups_kernel(const rrea_secondary_group* __restrict__ g, int64_t n, double cutoff,
+                                       int32_t* __restrict__ result, double* __restrict__ residuals)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    result[i] = validate_group(g[i], cutoff, residuals ? residuals + 2 * i : nullptr);
 }
 
 int64_t blocks_for(int64_t n, int b) { return (n + b - 1) / b; }
@@ -5053,6 +5194,17 @@ rrea_status rrea_transport_step_photons(rrea_tables_t T, rrea_fields_t F, const 
                                         const rrea_step_config* cfg, const rrea_step_out* out, rrea_stream_t stream)
 {
     return species_step(kPhoton, T, F, tile, cfg, out, stream);
+}
+
+rrea_status rrea_validate_secondary_groups(const rrea_secondary_group* groups, int64_t n, double cutoff,
+                                           int32_t* result, double* residuals, rrea_stream_t stream)
+{
+    if (n < 0 || (n > 0 && (!groups || !result))) return RREA_ERR_ARGUMENT;
+    if (n == 0) return RREA_OK;
+    validate_groups_kernel<<<dim3(static_cast<unsigned>(blocks_for(n, 256))), dim3(256), 0,
+                             static_cast<hipStream_t>(stream)>>>(groups, n, cutoff, result, residuals);
+    HIP_TRY(hipGetLastError());
+    return RREA_OK;
 }
 
 rrea_status rrea_transport_step_noalloc(int32_t species, rrea_tables_t T, rrea_fields_t F, const rrea_tile* tile,

@@ -771,3 +771,28 @@ class ConcurrentApply:
 
     def close(self):
         pass
+
+
+GROUP_RESULT_BITS = {"required": 1, "present": 2, "finite": 4, "valid": 8}
+
+
+def validate_secondary_groups(groups, low_energy_cutoff_eV: float, stream: int | None = None):
+    """RreaValidateSecondaryGroupConservation over an array of group records
+    (OV/RreaParticleInteraction.H:1099-1252), evaluated on the device.
+
+    ``groups`` is a ctypes array of ``_lib.SecondaryGroup``; returns the
+    result words (bit 1 required, 2 present, 4 finite, 8 valid) and the
+    (charge, energy) residuals per group, both as numpy arrays."""
+    n = len(groups)
+    buf = torch.empty(max(n, 1) * C.sizeof(_lib.SecondaryGroup), dtype=torch.uint8)
+    if n:
+        C.memmove(buf.data_ptr(), C.addressof(groups), n * C.sizeof(_lib.SecondaryGroup))
+    dev = buf.cuda()
+    result = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    residuals = torch.zeros(2 * max(n, 1), dtype=torch.float64, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    _lib.check(_lib.lib().rrea_validate_secondary_groups(dev.data_ptr(), n, low_energy_cutoff_eV,
+                                                         result.data_ptr(), residuals.data_ptr(), s),
+               "rrea_validate_secondary_groups")
+    torch.cuda.synchronize()
+    return result[:n].cpu().numpy(), residuals[:2 * n].cpu().numpy().reshape(n, 2)

@@ -33,6 +33,8 @@ STRUCTS = {
     "rrea_batch_graph_desc": _lib.BatchGraphDesc,
     "rrea_population_records": _lib.PopulationRecords,
     "rrea_population_config": _lib.PopulationConfig,
+    "rrea_group_entry": _lib.GroupEntry,
+    "rrea_secondary_group": _lib.SecondaryGroup,
 }
 
 
