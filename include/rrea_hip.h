@@ -232,7 +232,7 @@ enum {
 
 #define RREA_INVALID_ID INT64_MIN
 
-#define RREA_TALLY_COUNT 24
+#define RREA_TALLY_COUNT (RREA_LEDGER_OFFSET + RREA_LEDGER_COUNT)
 /* Tally slots: 0 survivors, 1 escapes, 2 demotions, 3 hard-Moller events,
  * 4 brems events, 5 tracked brems photons, 6 substeps, 7 continuous drag
  * (weight x eV), 8 ion pairs, 9 field work (weight x eV), 10 annihilations,
@@ -241,7 +241,96 @@ enum {
  * sources (weight: demoted electrons and sub-cutoff pair electrons);
  * rrea_newborn_drain adds 17 newborns advanced, 18 newborn survivors,
  * 19 newborn escapes, 20 newborns demoted, annihilated or absorbed;
- * 21-23 reserved. */
+ * 21-23 reserved; then the event ledger. */
+
+/* The event ledger (RreaEventLedger, native_amrex_rrea/include/rrea/
+ * RreaEventLedger.H:18-96), in the reference's field order, from
+ * tallies[RREA_LEDGER_OFFSET] on.  Counts are kept as exact double sums.
+ * The RREA_LEDGER_IS_MAX slots are running maxima of non-negative values
+ * (folded with max, as RreaFoldEventLedger, OV/RreaTransportLog.H:45-127);
+ * every other slot is a sum.  A zeroed tally array is the empty ledger. */
+#define RREA_LEDGER_OFFSET 24
+enum {
+    RREA_L_PHOTON_INTERACTION_COUNT = 0,
+    RREA_L_ELECTRON_BREMS_PHOTON_COUNT,
+    RREA_L_TOTAL_BREMS_PHOTON_COUNT,
+    RREA_L_PHOTON_INTERACTION_WEIGHT,
+    RREA_L_ELECTRON_BREMS_PHOTON_WEIGHT,
+    RREA_L_ELECTRON_BREMS_PHOTON_ENERGY_EV,
+    RREA_L_TOTAL_BREMS_PHOTON_WEIGHT,
+    RREA_L_TOTAL_BREMS_PHOTON_ENERGY_EV,
+    RREA_L_PHOTON_CONTINUOUS_SOFT_BREMS_ENERGY_EV,
+    RREA_L_COMPTON_INTERACTION_COUNT,
+    RREA_L_PHOTOELECTRIC_INTERACTION_COUNT,          /* 10 */
+    RREA_L_PAIR_PRODUCTION_COUNT,
+    RREA_L_PAIR_NUCLEAR_COUNT,
+    RREA_L_PAIR_TRIPLET_COUNT,
+    RREA_L_COMPTON_INTERACTION_WEIGHT,
+    RREA_L_PHOTOELECTRIC_INTERACTION_WEIGHT,
+    RREA_L_PAIR_PRODUCTION_WEIGHT,
+    RREA_L_PAIR_NUCLEAR_WEIGHT,
+    RREA_L_PAIR_TRIPLET_WEIGHT,
+    RREA_L_PAIR_RECOIL_ELECTRON_ENERGY_EV,
+    RREA_L_PAIR_LOCAL_DEPOSIT_ENERGY_EV,             /* 20 */
+    RREA_L_PHOTON_SEEDED_ELECTRON_WEIGHT,
+    RREA_L_PHOTON_SEEDED_POSITRON_WEIGHT,
+    RREA_L_PHOTON_SEEDED_KINETIC_ELECTRON_WEIGHT,
+    RREA_L_PHOTON_SEEDED_LOW_ELECTRON_WEIGHT,
+    RREA_L_PHOTON_SEEDED_KINETIC_POSITRON_WEIGHT,
+    RREA_L_PHOTON_SEEDED_BELOW_CUTOFF_POSITRON_WEIGHT,
+    RREA_L_COMPTON_KINETIC_ELECTRON_WEIGHT,
+    RREA_L_COMPTON_LOW_ELECTRON_WEIGHT,
+    RREA_L_PHOTOELECTRIC_KINETIC_ELECTRON_WEIGHT,
+    RREA_L_PHOTOELECTRIC_LOW_ELECTRON_WEIGHT,        /* 30 */
+    RREA_L_PAIR_KINETIC_ELECTRON_WEIGHT,
+    RREA_L_PAIR_LOW_ELECTRON_WEIGHT,
+    RREA_L_PAIR_KINETIC_POSITRON_WEIGHT,
+    RREA_L_PAIR_BELOW_CUTOFF_POSITRON_WEIGHT,
+    RREA_L_PAIR_TRIPLET_RECOIL_KINETIC_ELECTRON_WEIGHT,
+    RREA_L_PAIR_TRIPLET_RECOIL_LOW_ELECTRON_WEIGHT,
+    RREA_L_PHOTON_ABSORBED_ENERGY_EV,
+    RREA_L_PHOTON_ESCAPED_ENERGY_EV,
+    RREA_L_PHOTON_TO_CHARGED_ENERGY_EV,
+    RREA_L_PHOTON_LOCAL_DEPOSIT_ENERGY_EV,           /* 40 */
+    RREA_L_POSITRON_TRANSPORT_ENERGY_LOSS_EV,
+    RREA_L_POSITRON_THERMALIZED_COUNT,
+    RREA_L_POSITRON_BHABHA_EVENT_COUNT,
+    RREA_L_POSITRON_BHABHA_EVENT_WEIGHT,
+    RREA_L_POSITRON_BHABHA_SECONDARY_ENERGY_EV,
+    RREA_L_POSITRON_BHABHA_KINETIC_SECONDARY_WEIGHT,
+    RREA_L_POSITRON_BREMS_EVENT_COUNT,
+    RREA_L_POSITRON_BREMS_EVENT_WEIGHT,
+    RREA_L_POSITRON_BREMS_PHOTON_ENERGY_EV,
+    RREA_L_TRANSPORT_IONIZATION_WEIGHT,              /* 50 */
+    RREA_L_TRANSPORT_LOW_SECONDARY_WEIGHT,
+    RREA_L_TRANSPORT_KINETIC_SECONDARY_WEIGHT,
+    RREA_L_TRANSPORT_ENERGY_LOSS_EV,
+    RREA_L_HARD_MOLLER_EVENT_COUNT,
+    RREA_L_HARD_MOLLER_EVENT_WEIGHT,
+    RREA_L_DIRECT_HARD_MOLLER_ELECTRON_WEIGHT,
+    RREA_L_HARD_MOLLER_EXPECTED_EVENT_WEIGHT,
+    RREA_L_HARD_MOLLER_SECONDARY_ENERGY_WEIGHTED_SUM_EV,
+    RREA_L_HARD_MOLLER_SECONDARY_ENERGY_WEIGHT_SUM,
+    RREA_L_HARD_MOLLER_SECONDARY_ENERGY_MAX_EV,      /* 60, max */
+    RREA_L_PARENT_DRAG_LOSS_EV,
+    RREA_L_ION_PAIR_SOURCE_LOSS_EV,
+    RREA_L_RESTRICTED_COLLISION_LOSS_EV,
+    RREA_L_EXPLICIT_HARD_MOLLER_ENERGY_LOSS_EV,
+    RREA_L_MAX_TRANSPORT_HARD_MOLLER_TAU,            /* max */
+    RREA_L_MAX_TRANSPORT_BREMS_TAU,                  /* max */
+    RREA_L_MAX_TRANSPORT_ELECTRON_TOTAL_COMPETING_TAU, /* max */
+    RREA_L_MAX_TRANSPORT_PHOTON_TAU,                 /* max */
+    RREA_L_MAX_TRANSPORT_POSITRON_TAU,               /* max */
+    RREA_L_MAX_TRANSPORT_SUBCYCLES,                  /* 70, max */
+    RREA_L_TRANSPORT_OPTICAL_DEPTH_VIOLATION_COUNT,
+    RREA_L_MAX_TRANSPORT_ELECTRON_ENERGY_EV,         /* max */
+    RREA_L_MAX_TRANSPORT_PHOTON_ENERGY_SEEN_EV,      /* max */
+    RREA_L_MAX_TRANSPORT_POSITRON_ENERGY_EV,         /* max */
+    RREA_LEDGER_COUNT                                /* 75 */
+};
+#define RREA_LEDGER_IS_MAX(slot) ((slot) == RREA_L_HARD_MOLLER_SECONDARY_ENERGY_MAX_EV \
+    || ((slot) >= RREA_L_MAX_TRANSPORT_HARD_MOLLER_TAU && (slot) <= RREA_L_MAX_TRANSPORT_SUBCYCLES) \
+    || (slot) >= RREA_L_MAX_TRANSPORT_ELECTRON_ENERGY_EV)
 
 typedef struct rrea_step_out {
     uint8_t* status;          /* per particle, RREA_SURVIVE.. (may be NULL) */

@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -69,7 +70,25 @@ RATE_KEYS = ["e_col", "e_soft", "e_moller", "e_brems", "e_eltr",
              "g_compton", "g_pe", "g_pairn", "g_pairt"]
 CDF_KEYS = ["moller", "bhabha", "ebrems", "pbrems"]
 ANGLE_KEYS = ["gs_e", "gs_p", "wz_e", "wz_p"]
-N_TALLIES = 24
+N_TALLIES = 24 + 75
+LEDGER_OFFSET = 24
+
+
+def _ledger_names():
+    """The event-ledger slot names, in order, from the oracle header."""
+    txt = open(os.path.join(HERE, "rrea_oracle.h")).read()
+    blk = txt[txt.index("ORC_L_PHOTON_INTERACTION_COUNT = 0"):txt.index("    ORC_LEDGER_COUNT")]
+    return [n.lower() for n in re.findall(r"ORC_L_([A-Z0-9_]+)", blk)]
+
+
+LEDGER_NAMES = _ledger_names()
+LEDGER_MAX = {n for n in LEDGER_NAMES if n.startswith("max_")} | {"hard_moller_secondary_energy_max_ev"}
+assert len(LEDGER_NAMES) == N_TALLIES - LEDGER_OFFSET
+
+
+def ledger(tallies) -> dict:
+    """The event ledger of a tally array, by field name."""
+    return {n: float(tallies[LEDGER_OFFSET + i]) for i, n in enumerate(LEDGER_NAMES)}
 
 
 class InterleaveRecord(C.Structure):
@@ -104,6 +123,15 @@ def validate_group(g: Group, low_energy_cutoff_eV: float):
     res = np.zeros(2)
     bits = lib().orc_validate_group(C.byref(g), low_energy_cutoff_eV, _ptr(res))
     return int(bits), float(res[0]), float(res[1])
+
+
+def fold_tallies(into: np.ndarray, frm: np.ndarray) -> np.ndarray:
+    """orc_fold_tallies on copies: the fold of two tally arrays."""
+    a = np.ascontiguousarray(into, dtype=np.float64).copy()
+    b = np.ascontiguousarray(frm, dtype=np.float64)
+    assert a.shape == b.shape == (N_TALLIES,)
+    lib().orc_fold_tallies(_ptr(a), _ptr(b))
+    return a
 
 
 def build(quiet: bool = True) -> None:
@@ -195,6 +223,8 @@ def lib():
         L.orc_sample_compton.restype = C.c_int
         L.orc_sample_photoelectric.argtypes = [C.c_void_p, C.c_double, C.c_double, P_D]
         L.orc_sample_photoelectric.restype = C.c_int
+        L.orc_fold_tallies.argtypes = [P_D, P_D]
+        L.orc_fold_tallies.restype = None
         L.orc_validate_group.argtypes = [C.POINTER(Group), C.c_double, P_D]
         L.orc_validate_group.restype = C.c_int
         L.orc_sample_photoelectron_cos.argtypes = [C.c_void_p, C.c_double, C.c_double, P_I32]

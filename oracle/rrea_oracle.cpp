@@ -1531,6 +1531,247 @@ struct ChunkLog {
     int error = 0;
 };
 
+// RreaFoldEventLedger OV/RreaTransportLog.H:45-127: sums add, the running
+// maxima take the larger value
+inline void fold_tallies(double* into, double const* from)
+{
+    for (int q = 0; q < ORC_T_COUNT; ++q) {
+        int const l = q - ORC_LEDGER_OFFSET;
+        if (l >= 0 && ORC_LEDGER_IS_MAX(l)) into[q] = std::max(into[q], from[q]);
+        else into[q] += from[q];
+    }
+}
+
+// The record operations of RreaEventLedger (NA/include/rrea/RreaEventLedger.H:
+// 98-413) on a chunk's ledger slots; 'cutoff' is the ledger's
+// low_energy_cutoff_eV.
+struct Ledger {
+    double* t;
+    double cutoff;
+    double& at(int q) { return t[ORC_LEDGER_OFFSET + q]; }
+    void add(int q, double v) { at(q) += v; }
+    void up(int q, double v) { at(q) = (at(q) < v) ? v : at(q); }
+    static double pos(double v) { return std::max(v, 0.0); }
+
+    void transport_ionization(double w, double loss, bool low, bool kinetic)  // :98-116
+    {
+        if (w <= 0.0) return;
+        add(ORC_L_TRANSPORT_IONIZATION_WEIGHT, w);
+        add(ORC_L_TRANSPORT_ENERGY_LOSS_EV, w * pos(loss));
+        if (low) add(ORC_L_TRANSPORT_LOW_SECONDARY_WEIGHT, w);
+        if (kinetic) add(ORC_L_TRANSPORT_KINETIC_SECONDARY_WEIGHT, w);
+    }
+    void loss_partition(double w, double drag, double col, double hard)  // :118-134
+    {
+        if (w <= 0.0) return;
+        add(ORC_L_PARENT_DRAG_LOSS_EV, w * pos(drag));
+        add(ORC_L_ION_PAIR_SOURCE_LOSS_EV, w * pos(col));
+        add(ORC_L_RESTRICTED_COLLISION_LOSS_EV, w * pos(col));
+        add(ORC_L_EXPLICIT_HARD_MOLLER_ENERGY_LOSS_EV, w * pos(hard));
+    }
+    void soft_brems(double w, double e)  // :136-143
+    {
+        if (w > 0.0 && e > 0.0) add(ORC_L_PHOTON_CONTINUOUS_SOFT_BREMS_ENERGY_EV, w * e);
+    }
+    void hm_expectation(double x)  // :145-150
+    {
+        if (x > 0.0) add(ORC_L_HARD_MOLLER_EXPECTED_EVENT_WEIGHT, x);
+    }
+    void hm_event(double w, double sec)  // :152-168
+    {
+        if (w <= 0.0) return;
+        add(ORC_L_HARD_MOLLER_EVENT_COUNT, 1.0);
+        add(ORC_L_HARD_MOLLER_EVENT_WEIGHT, w);
+        if (sec >= cutoff) add(ORC_L_DIRECT_HARD_MOLLER_ELECTRON_WEIGHT, w);
+        add(ORC_L_HARD_MOLLER_SECONDARY_ENERGY_WEIGHTED_SUM_EV, w * pos(sec));
+        add(ORC_L_HARD_MOLLER_SECONDARY_ENERGY_WEIGHT_SUM, w);
+        up(ORC_L_HARD_MOLLER_SECONDARY_ENERGY_MAX_EV, sec);
+    }
+    void photon_interaction(double w)  // :170-180
+    {
+        if (w <= 0.0) return;
+        add(ORC_L_PHOTON_INTERACTION_COUNT, 1.0);
+        add(ORC_L_PHOTON_INTERACTION_WEIGHT, w);
+    }
+    void photon_escape(double w, double e)  // :182-189
+    {
+        if (w > 0.0 && e > 0.0) add(ORC_L_PHOTON_ESCAPED_ENERGY_EV, w * e);
+    }
+    void brems_photon(double w, double e)  // :191-204
+    {
+        if (w <= 0.0 || e <= 0.0) return;
+        add(ORC_L_ELECTRON_BREMS_PHOTON_COUNT, 1.0);
+        add(ORC_L_TOTAL_BREMS_PHOTON_COUNT, 1.0);
+        add(ORC_L_ELECTRON_BREMS_PHOTON_WEIGHT, w);
+        add(ORC_L_ELECTRON_BREMS_PHOTON_ENERGY_EV, w * e);
+        add(ORC_L_TOTAL_BREMS_PHOTON_WEIGHT, w);
+        add(ORC_L_TOTAL_BREMS_PHOTON_ENERGY_EV, w * e);
+    }
+    // a photon-seeded electron of kinetic energy e, kinetic or low-energy
+    void seeded_electron(double w, double e, int kinetic_slot, int low_slot)
+    {
+        if (e >= 0.0) {
+            add(ORC_L_PHOTON_SEEDED_ELECTRON_WEIGHT, w);
+            if (e >= cutoff) {
+                add(ORC_L_PHOTON_SEEDED_KINETIC_ELECTRON_WEIGHT, w);
+                add(kinetic_slot, w);
+            } else {
+                add(ORC_L_PHOTON_SEEDED_LOW_ELECTRON_WEIGHT, w);
+                add(low_slot, w);
+            }
+        }
+    }
+    void compton(double w, double recoil, double charged, double local)  // :206-235
+    {
+        photon_interaction(w);
+        add(ORC_L_COMPTON_INTERACTION_COUNT, 1.0);
+        add(ORC_L_COMPTON_INTERACTION_WEIGHT, w);
+        seeded_electron(w, recoil, ORC_L_COMPTON_KINETIC_ELECTRON_WEIGHT, ORC_L_COMPTON_LOW_ELECTRON_WEIGHT);
+        add(ORC_L_PHOTON_TO_CHARGED_ENERGY_EV, w * pos(charged));
+        add(ORC_L_PHOTON_LOCAL_DEPOSIT_ENERGY_EV, w * pos(local));
+        add(ORC_L_TRANSPORT_ENERGY_LOSS_EV, w * (pos(charged) + pos(local)));
+    }
+    void photoelectric(double w, double photon, double electron, double charged, double local)  // :237-269
+    {
+        photon_interaction(w);
+        add(ORC_L_PHOTOELECTRIC_INTERACTION_COUNT, 1.0);
+        add(ORC_L_PHOTOELECTRIC_INTERACTION_WEIGHT, w);
+        seeded_electron(w, electron, ORC_L_PHOTOELECTRIC_KINETIC_ELECTRON_WEIGHT,
+                        ORC_L_PHOTOELECTRIC_LOW_ELECTRON_WEIGHT);
+        add(ORC_L_PHOTON_ABSORBED_ENERGY_EV, w * pos(photon));
+        add(ORC_L_PHOTON_TO_CHARGED_ENERGY_EV, w * pos(charged));
+        add(ORC_L_PHOTON_LOCAL_DEPOSIT_ENERGY_EV, w * pos(local));
+        add(ORC_L_TRANSPORT_ENERGY_LOSS_EV, w * pos(photon));
+    }
+    void pair(double w, double photon, double electron, double positron, double recoil, double charged,
+              double local, bool triplet)  // :271-340
+    {
+        photon_interaction(w);
+        add(ORC_L_PAIR_PRODUCTION_COUNT, 1.0);
+        add(ORC_L_PAIR_PRODUCTION_WEIGHT, w);
+        add(triplet ? ORC_L_PAIR_TRIPLET_COUNT : ORC_L_PAIR_NUCLEAR_COUNT, 1.0);
+        add(triplet ? ORC_L_PAIR_TRIPLET_WEIGHT : ORC_L_PAIR_NUCLEAR_WEIGHT, w);
+        seeded_electron(w, electron, ORC_L_PAIR_KINETIC_ELECTRON_WEIGHT, ORC_L_PAIR_LOW_ELECTRON_WEIGHT);
+        add(ORC_L_PHOTON_SEEDED_POSITRON_WEIGHT, w);
+        if (positron >= cutoff) {
+            add(ORC_L_PHOTON_SEEDED_KINETIC_POSITRON_WEIGHT, w);
+            add(ORC_L_PAIR_KINETIC_POSITRON_WEIGHT, w);
+        } else {
+            add(ORC_L_PHOTON_SEEDED_BELOW_CUTOFF_POSITRON_WEIGHT, w);
+            add(ORC_L_PAIR_BELOW_CUTOFF_POSITRON_WEIGHT, w);
+        }
+        if (triplet) {
+            add(ORC_L_PHOTON_SEEDED_ELECTRON_WEIGHT, w);
+            if (recoil >= cutoff) {
+                add(ORC_L_PHOTON_SEEDED_KINETIC_ELECTRON_WEIGHT, w);
+                add(ORC_L_PAIR_TRIPLET_RECOIL_KINETIC_ELECTRON_WEIGHT, w);
+            } else {
+                add(ORC_L_PHOTON_SEEDED_LOW_ELECTRON_WEIGHT, w);
+                add(ORC_L_PAIR_TRIPLET_RECOIL_LOW_ELECTRON_WEIGHT, w);
+            }
+        }
+        add(ORC_L_PAIR_RECOIL_ELECTRON_ENERGY_EV, w * pos(recoil));
+        add(ORC_L_PAIR_LOCAL_DEPOSIT_ENERGY_EV, w * pos(local));
+        add(ORC_L_PHOTON_ABSORBED_ENERGY_EV, w * pos(photon));
+        add(ORC_L_PHOTON_LOCAL_DEPOSIT_ENERGY_EV, w * pos(local));
+        add(ORC_L_PHOTON_TO_CHARGED_ENERGY_EV, w * pos(charged));
+        add(ORC_L_TRANSPORT_ENERGY_LOSS_EV, w * pos(charged + local - 2.0 * kMeC2));
+    }
+    void positron_transport(double w, double loss, bool thermalized)  // :342-354
+    {
+        if (w <= 0.0) return;
+        add(ORC_L_POSITRON_TRANSPORT_ENERGY_LOSS_EV, w * pos(loss));
+        if (thermalized) add(ORC_L_POSITRON_THERMALIZED_COUNT, 1.0);
+    }
+    void bhabha(double w, double sec, bool kinetic)  // :356-370
+    {
+        if (w <= 0.0) return;
+        add(ORC_L_POSITRON_BHABHA_EVENT_COUNT, 1.0);
+        add(ORC_L_POSITRON_BHABHA_EVENT_WEIGHT, w);
+        add(ORC_L_POSITRON_BHABHA_SECONDARY_ENERGY_EV, w * pos(sec));
+        if (kinetic) add(ORC_L_POSITRON_BHABHA_KINETIC_SECONDARY_WEIGHT, w);
+    }
+    void positron_brems(double w, double e)  // :372-385
+    {
+        if (w <= 0.0 || e <= 0.0) return;
+        add(ORC_L_POSITRON_BREMS_EVENT_COUNT, 1.0);
+        add(ORC_L_TOTAL_BREMS_PHOTON_COUNT, 1.0);
+        add(ORC_L_POSITRON_BREMS_EVENT_WEIGHT, w);
+        add(ORC_L_POSITRON_BREMS_PHOTON_ENERGY_EV, w * e);
+        add(ORC_L_TOTAL_BREMS_PHOTON_WEIGHT, w);
+        add(ORC_L_TOTAL_BREMS_PHOTON_ENERGY_EV, w * e);
+    }
+    void guard(double hm_tau, double brems_tau, double total_tau, double photon_tau, double positron_tau,
+               uint64_t subcycles, double electron_e, double photon_e, double positron_e, bool violation)  // :387-413
+    {
+        up(ORC_L_MAX_TRANSPORT_HARD_MOLLER_TAU, hm_tau);
+        up(ORC_L_MAX_TRANSPORT_BREMS_TAU, brems_tau);
+        up(ORC_L_MAX_TRANSPORT_ELECTRON_TOTAL_COMPETING_TAU, total_tau);
+        up(ORC_L_MAX_TRANSPORT_PHOTON_TAU, photon_tau);
+        up(ORC_L_MAX_TRANSPORT_POSITRON_TAU, positron_tau);
+        up(ORC_L_MAX_TRANSPORT_SUBCYCLES, static_cast<double>(subcycles));
+        up(ORC_L_MAX_TRANSPORT_ELECTRON_ENERGY_EV, electron_e);
+        up(ORC_L_MAX_TRANSPORT_PHOTON_ENERGY_SEEN_EV, photon_e);
+        up(ORC_L_MAX_TRANSPORT_POSITRON_ENERGY_EV, positron_e);
+        if (violation) add(ORC_L_TRANSPORT_OPTICAL_DEPTH_VIOLATION_COUNT, 1.0);
+    }
+    // RreaWarpXCoupling::RecordPhotonLocalDeposit OV/RreaCouplingSources.cpp:518-525,
+    // the ledger part of RecordPositronAnnihilation :527-553
+    void photon_local_deposit(double w, double e)
+    {
+        if (w > 0.0 && e > 0.0) add(ORC_L_PHOTON_LOCAL_DEPOSIT_ENERGY_EV, w * e);
+    }
+};
+
+// Probability of a competing-risk optical depth, OV/RreaMaterialPath.H:203-210
+inline double probability_from_optical_depth(double tau)
+{
+    if (!(tau > 0.0)) return 0.0;
+    return std::min(1.0, -std::expm1(-tau));
+}
+
+// The photon energy flows the interaction records carry (charged kinetic plus
+// rest mass, local), RreaComptonPhotonEnergyFlow / RreaPhotoelectricPhoton
+// EnergyFlow / RreaPairPhotonEnergyFlow, OV/RreaParticleInteraction.H:1014-1092
+struct EnergyFlow { double charged = 0.0, local = 0.0; };
+
+inline EnergyFlow compton_flow(double recoil, double scattered, double cutoff, double gcut)
+{
+    EnergyFlow f;
+    double const r = std::max(recoil, 0.0), s = std::max(scattered, 0.0);
+    if (r >= cutoff) f.charged = r;
+    else f.local = r;
+    if (s < gcut) f.local += s;
+    return f;
+}
+
+inline EnergyFlow photoelectric_flow(double electron, double binding, double cutoff)
+{
+    EnergyFlow f;
+    double const e = std::max(electron, 0.0);
+    f.local = std::max(binding, 0.0);
+    if (e >= cutoff) f.charged = e;
+    else f.local += e;
+    return f;
+}
+
+inline EnergyFlow pair_flow(double electron, double positron, double recoil, double local, double cutoff,
+                            bool has_recoil)
+{
+    EnergyFlow f;
+    f.charged = 2.0 * kMeC2;
+    f.local = std::max(local, 0.0);
+    auto part = [&](double k) {
+        double const e = std::max(k, 0.0);
+        if (e >= cutoff) f.charged += e;
+        else f.local += e;
+    };
+    part(electron);
+    part(positron);
+    if (has_recoil) part(recoil);
+    return f;
+}
+
 struct StepContext {
     Tables const* t;
     Coupling cp;
@@ -1691,6 +1932,7 @@ struct Emitter {
         if (pw <= 0.0) return;
         ion_event(pw);
         ionization(r, z, pw, 0.0, kWAir);
+        Ledger{fx.tallies, 0.0}.transport_ionization(pw, kWAir, true, false);
     }
     void secondary(int sp, int process, double x, double y, double z, double sdx, double sdy, double sdz,
                    double energy, double birth)
@@ -1745,6 +1987,7 @@ inline void spawn_annihilation(StepContext const& ctx, Emitter& em, int32_t proc
     em.positive_ion(r, z, em.w, 0.0);
     em.fx.tallies[ORC_T_POSITIVE_IONS] += em.w;
     em.fx.tallies[ORC_T_ANNIHILATIONS] += 1.0;
+    Ledger{em.fx.tallies, 0.0}.photon_local_deposit(em.w, local);
     if (em.w > 0.0) {
         Closure c;
         c.parent(1, em.w, std::max(k, 0.0), 0.0, false);
@@ -1922,12 +2165,17 @@ inline void partition_ion_pairs(std::vector<Interval> const& iv, double full_len
 // equivalents: collision + soft radiative stopping frozen at stop_e over
 // [b, e], limited at the cutoff, its collision share emitted as ion pairs;
 // k is reduced by the drag (set to the cutoff when it is reached).
-template <class Emit>
+// The plan's loss partition for the ledger: drag, its collision and soft
+// radiative shares, and the mean hard-Moller transfer over the same density
+// path (zero where the policy passes no hard-transfer rate).
+struct DragPlan { double drag = 0.0, collision = 0.0, soft = 0.0, hard = 0.0; };
+
+template <class HardStp, class Emit>
 inline SegLimit continuous_drag(Tables const& t, bool pos, Path const& path, std::vector<Interval> const& iv,
-                                double b, double e, double stop_e, double cutoff, double w, double& k, double& drag_out,
-                                Emit&& emit)
+                                double b, double e, double stop_e, double cutoff, double w, double& k, DragPlan& plan,
+                                HardStp&& hard_stp, Emit&& emit)
 {
-    drag_out = 0.0;
+    plan = DragPlan{};
     if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return {b, false};
     double const col = frozen(t.rate[pos ? kPCol : kECol].interpolate(stop_e) * 1.0);
     double const soft = frozen(t.rate[pos ? kPSoft : kESoft].interpolate(stop_e) * 1.0);
@@ -1939,10 +2187,13 @@ inline SegLimit continuous_drag(Tables const& t, bool pos, Path const& path, std
     double const avail = std::max(0.0, k - cutoff);
     double const drag = lim.cutoff ? avail : std::min(raw_col + raw_soft, avail);
     double const col_loss = std::min(raw_col, drag);
+    plan.drag = drag;
+    plan.collision = col_loss;
+    plan.soft = std::max(0.0, drag - col_loss);
+    plan.hard = std::min(frozen(hard_stp()) * pm, avail);
     partition_ion_pairs(iv, path.full_length, b, lim.end, col_loss, raw_col, w, kWAir, col, emit);
     k = std::max(0.0, k - drag);
     if (lim.cutoff) k = cutoff;
-    drag_out = drag;
     return lim;
 }
 
@@ -2078,6 +2329,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
     std::vector<Interval> iv = div;
 
     Emitter em{fx, ip, pkey, w, out.ion_pairs, pos ? 1 : 0};
+    Ledger L{fx.tallies, cutoff};
     auto ion_event = [&](double ev_w) { em.ion_event(ev_w); };
     auto deposit_local = [&](double energy, double r, double z) { em.deposit_local(energy, r, z); };
     // the net chord carries the species' signed charge (-w for e-, +w for e+)
@@ -2129,6 +2381,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
         if (pos) {
             // at rest where it stands, before any transport: a zero-length net chord
             annihilate(x_particle, y_particle, z_particle, r_particle, true, 0U, 1.0);
+            L.positron_transport(w, k, true);  // OV/RreaPositronTransport.cpp:152
             annihilated_at(r_particle, z_particle);
         } else {
             demote(r_particle, z_particle);
@@ -2167,9 +2420,20 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
         uint64_t const imd = driver_count(std::abs(field_work) + drag_est, impulse_scale);
         subcycles = std::min<uint64_t>(std::max({subcycles, gsd, imd}), cfg.max_subcycles);
     }
+    // the reported per-substep taus: uniform slices of the driver chord over the
+    // clipped span (OV/RreaElectronTransport.cpp:250-278, OV/RreaPositronTransport.cpp:241-259)
+    if (pos) {
+        double const max_positron_tau = (stp0[0] + stp0[1] + stp0[2]) * max_uniform_slice(div, dpath, subcycles) / span;
+        L.guard(0.0, 0.0, 0.0, 0.0, max_positron_tau, subcycles, 0.0, 0.0, k, violation);
+    } else {
+        double const slice = max_uniform_slice(div, dpath, subcycles) / span;
+        L.guard(stp0[0] * slice, stp0[1] * slice, (stp0[0] + stp0[1]) * slice, 0.0, 0.0, subcycles, k, 0.0, 0.0,
+                violation);
+    }
     if (cfg.strict_transport_guards && violation) fail(ORC_ERR_GUARD, "strict transport guard: optical-depth cap exceeded");
     out.substeps = subcycles;
     fx.tallies[ORC_T_SUBSTEPS] += static_cast<double>(subcycles);
+    double positron_total_loss = 0.0;
 
     bool exited = false;
     bool terminated = false;
@@ -2195,15 +2459,21 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
     // apply_positron_continuous_segment OV/RreaPositronTransport.cpp:278-360:
     // collision + soft radiative stopping frozen at the segment's energy.
     auto continuous = [&](double b, double e, double stop_e) -> SegLimit {
-        double drag = 0.0;
-        SegLimit const sl = continuous_drag(t, pos, path, iv, b, e, stop_e, cutoff, w, k, drag,
-                                            [&](double mid_f, double pw) {
-                                                ion_event(pw);
-                                                double mx, my, mz;
-                                                path.point_at_full(mid_f, mx, my, mz);
-                                                em.ionization(std::hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
-                                            });
-        fx.tallies[ORC_T_DRAG_LOSS_WEV] += w * drag;
+        DragPlan plan;
+        SegLimit const sl = continuous_drag(
+            t, pos, path, iv, b, e, stop_e, cutoff, w, k, plan,
+            [&] { return (!pos && hm_on && thr > 0.0 && stop_e > 2.0 * thr) ? t.hard_moller_energy_loss(stop_e, 1.0) : 0.0; },
+            [&](double mid_f, double pw) {
+                ion_event(pw);
+                double mx, my, mz;
+                path.point_at_full(mid_f, mx, my, mz);
+                em.ionization(std::hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
+                L.transport_ionization(pw, kWAir, true, false);
+            });
+        fx.tallies[ORC_T_DRAG_LOSS_WEV] += w * plan.drag;
+        L.soft_brems(w, plan.soft);
+        if (pos) positron_total_loss += plan.drag;
+        else L.loss_partition(w, plan.drag, plan.collision, plan.hard);
         return sl;
     };
 
@@ -2244,8 +2514,15 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
             [&](double frozen_e, double rem, double end) {
                 if (pos) positron_stp_rates(t, thr, gcut, frozen_e, stp);
                 else electron_stp_rates(t, thr, gcut, frozen_e, hm_on, stp);
-                double const L = density_path(iv, path, rem, end);
-                return std::array<double, 3>{stp[0] * L, stp[1] * L, stp[2] * L};
+                double const dp = density_path(iv, path, rem, end);
+                if (!pos) {
+                    double const hard_tau = stp[0] * dp, brems_tau = stp[1] * dp;
+                    double const total_tau = hard_tau + brems_tau;
+                    double const p_hard = total_tau > 0.0
+                        ? probability_from_optical_depth(total_tau) * hard_tau / total_tau : 0.0;
+                    L.hm_expectation(w * p_hard);
+                }
+                return std::array<double, 3>{stp[0] * dp, stp[1] * dp, stp[2] * dp};
             },
             [&](uint64_t ord) {
                 if (!live_hazard_index(species, sc, ord, idx)) fail(ORC_ERR_INVALID, "hazard RNG index overflow");
@@ -2278,6 +2555,8 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                     ion_event(w);
                     em.ionization(std::hypot(ex_, ey_), ez_, w, T, T);
                     fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
+                    L.hm_event(w, T);
+                    L.transport_ionization(w, T, false, true);
                     Closure c;
                     c.parent(0, w, before, k, true);
                     c.particle(0, w, T);
@@ -2300,6 +2579,8 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                         em.positive_ion(std::hypot(ex_, ey_), ez_, w, T);
                         fx.tallies[ORC_T_POSITIVE_IONS] += w;
                         fx.tallies[ORC_T_BHABHA_EVENTS] += 1.0;
+                        L.transport_ionization(w, T, false, true);
+                        L.bhabha(w, T, true);
                         Closure c;
                         c.parent(1, w, before, k, true);
                         c.particle(0, w, T);
@@ -2324,6 +2605,8 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
                                    gx, gy, gz);
                     photon_recoil(before, pe, gx, gy, gz, dx, dy, dz);
                     k = after;
+                    if (pos) L.positron_brems(w, pe);
+                    else L.brems_photon(w, pe);
                     if (tracked) {
                         push_secondary(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
                         fx.tallies[ORC_T_BREMS_PHOTONS] += 1.0;
@@ -2392,6 +2675,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
         [&] { return path.exits; });
 
     if (exited) escape = clipped_escape_policy(path, std::hypot(path.ex, path.ey), path.ez);
+    if (pos) L.positron_transport(w, positron_total_loss, false);  // OV/RreaPositronTransport.cpp:776
     if (pos) {
         // OV/RreaPositronTransport.cpp:764-846: a consumed parent outranks every
         // other outcome; then escape; then a stop or a below-cutoff endpoint
@@ -2408,6 +2692,7 @@ inline void charged_particle(StepContext const& ctx, ChunkLog& fx, bool const po
             double const az = terminated ? mid_z : run_z;
             double const ar = terminated ? mid_r : std::hypot(run_x, run_y);
             annihilate(ax, ay, az, std::hypot(ax, ay), true, subcycles, terminated ? stop_elapsed : 1.0);
+            L.positron_transport(w, k, true);  // OV/RreaPositronTransport.cpp:839
             annihilated_at(ar, az);
             return;
         }
@@ -2755,6 +3040,7 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
     uint64_t const step = static_cast<uint64_t>(std::max<int64_t>(ctx.cfg.step, 0));
     auto U = [&](uint64_t chan) { return rng_uniform(ctx.cfg.rng_seed, em.pkey, step, idx, chan); };
     double const sr = std::hypot(x, y);  // RreaInteractionSite::r_m of the event
+    Ledger L{fx.tallies, cutoff};
     if (channel == kCompton) {
         double const incident = energy;
         ComptonState const st = sample_compton(t, incident, U(kChComptonRecoil), U(kChComptonScatter));
@@ -2774,6 +3060,11 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
             em.deposit_local(st.recoil, sr, z);
         }
         fx.tallies[ORC_T_COMPTON_EVENTS] += 1.0;
+        L.transport_ionization(w, 0.0, !(st.recoil >= cutoff), st.recoil >= cutoff);
+        {
+            EnergyFlow const f = compton_flow(st.recoil, st.scattered, cutoff, gcut);
+            L.compton(w, st.recoil, f.charged, f.local);
+        }
         {
             Closure c;
             bool const survives = !(st.scattered < gcut);
@@ -2807,8 +3098,13 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
             em.ionization(sr, z, w, ke, 0.0);
             em.deposit_local(ke, sr, z);
         }
+        L.transport_ionization(w, 0.0, !(ke >= cutoff), ke >= cutoff);
         if (st.binding > 0.0) em.deposit_local(st.binding, sr, z);
         fx.tallies[ORC_T_PHOTOELECTRIC_EVENTS] += 1.0;
+        {
+            EnergyFlow const f = photoelectric_flow(ke, st.binding, cutoff);
+            L.photoelectric(w, energy, ke, f.charged, f.local);
+        }
         Closure c;
         c.parent(2, w, energy, 0.0, false);
         if (ke >= cutoff) c.particle(0, w, ke);
@@ -2837,6 +3133,7 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
         em.secondary(1, process, x, y, z, px, py, pz2, pair.positron, birth);
     } else {
         spawn_annihilation(ctx, em, process, x, y, z, sr, px, py, pz2, pair.positron, true, idx, birth);
+        L.positron_transport(w, pair.positron, true);  // OV/RreaPhotonFinalStates.H:320
     }
     if (pair.has_recoil) {
         double qx, qy, qz;
@@ -2848,6 +3145,10 @@ inline bool photon_final_state(StepContext const& ctx, Emitter& em, int channel,
     }
     if (pair.local > 0.0) em.deposit_local(pair.local, sr, z);
     fx.tallies[ORC_T_PAIR_EVENTS] += 1.0;
+    {
+        EnergyFlow const f = pair_flow(pair.electron, pair.positron, pair.recoil, pair.local, cutoff, pair.has_recoil);
+        L.pair(w, energy, pair.electron, pair.positron, pair.recoil, f.charged, f.local, triplet);
+    }
     {
         // a sub-cutoff positron leaves as the particle its at-rest
         // annihilation (a group checked of its own) consumes
@@ -2921,6 +3222,7 @@ inline void photon_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, do
             if (escape.escapes) {
                 out.status = ORC_ESCAPE;
                 fx.tallies[ORC_T_ESCAPES] += 1.0;
+                Ledger{fx.tallies, 0.0}.photon_escape(w, energy);  // OV/RreaPhotonTransport.cpp:192
             } else {
                 out.status = ORC_SURVIVE;
                 fx.tallies[ORC_T_SURVIVORS] += 1.0;
@@ -2929,6 +3231,8 @@ inline void photon_particle(StepContext const& ctx, ChunkLog& fx, int64_t ip, do
         if (ds <= 0.0) { commit_no_event(); return; }
         double const mfp = photon_mfp(t, energy);
         double const tau = chord_dp / mfp;
+        // one exponential depth per pass (OV/RreaPhotonTransport.cpp:222-233)
+        Ledger{fx.tallies, 0.0}.guard(0.0, 0.0, 0.0, tau, 0.0, 1U, 0.0, energy, 0.0, false);
         uint64_t idx = 0;
         if (!live_hazard_index(2U, 0U, ordinal, idx)) fail(ORC_ERR_INVALID, "photon hazard RNG index overflow");
         double const draw = rng_uniform(seed, pkey, step, idx,
@@ -3068,6 +3372,7 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
     uint64_t const lin = p.lineage;
     double const w = p.weight;
     Emitter em{fx, ip, root, w, out.ion_pairs, p.root_species, p.generation + 1};
+    Ledger L{fx.tallies, cutoff};
     auto pack = [&](uint64_t segment, uint64_t ordinal, uint64_t inner) {
         uint64_t v = 0;
         if (!pack_transient_index(true, rsp, lin, segment, ordinal, inner, v)) {
@@ -3107,6 +3412,7 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
         if (pos) {
             spawn_annihilation(ctx, em, kProcAnnihilation, p.x, p.y, p.z, std::hypot(p.x, p.y), dx, dy, dz, k, true,
                                pack(0, 0, 0), p.birth_fraction);
+            L.positron_transport(w, k, true);  // OV/RreaNewbornAdvance.cpp:123
             out.status = ORC_ANNIHILATED;
         } else {
             fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
@@ -3168,16 +3474,24 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
     };
     // RreaPlanContinuousDragSegment OV/RreaChargedInterleave.H:239-292 with the
     // cutoff limiter :203-265 and the ion-pair partition :157-196
+    double positron_total_loss = 0.0;
     auto continuous = [&](double b, double e, double stop_e) -> SegLimit {
-        double drag = 0.0;
-        SegLimit const sl = continuous_drag(t, pos, path, iv, b, e, stop_e, cutoff, w, k, drag,
-                                            [&](double mid_f, double pw) {
-                                                em.ion_event(pw);
-                                                double mx, my, mz;
-                                                path.point_at_full(mid_f, mx, my, mz);
-                                                em.ionization(std::hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
-                                            });
-        fx.tallies[ORC_T_DRAG_LOSS_WEV] += w * drag;
+        DragPlan plan;
+        SegLimit const sl = continuous_drag(
+            t, pos, path, iv, b, e, stop_e, cutoff, w, k, plan,
+            [&] { return (!pos && hm_on && thr > 0.0 && stop_e > 2.0 * thr) ? t.hard_moller_energy_loss(stop_e, 1.0) : 0.0; },
+            [&](double mid_f, double pw) {
+                em.ion_event(pw);
+                double mx, my, mz;
+                path.point_at_full(mid_f, mx, my, mz);
+                em.ionization(std::hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
+                L.transport_ionization(pw, kWAir, true, false);
+            });
+        fx.tallies[ORC_T_DRAG_LOSS_WEV] += w * plan.drag;
+        // OV/RreaNewbornAdvance.cpp:309-320
+        L.soft_brems(w, plan.soft);
+        if (pos) positron_total_loss += plan.drag;
+        else L.loss_partition(w, plan.drag, plan.collision, plan.hard);
         return sl;
     };
 
@@ -3219,8 +3533,15 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
             [&](double frozen_e, double rem, double end) {
                 if (pos) positron_stp_rates(t, thr, gcut, frozen_e, stp);
                 else electron_stp_rates(t, thr, gcut, frozen_e, hm_on, stp);
-                double const L = density_path(iv, path, rem, end);
-                return std::array<double, 3>{stp[0] * L, stp[1] * L, stp[2] * L};
+                double const dp = density_path(iv, path, rem, end);
+                std::array<double, 3> const taus{stp[0] * dp, stp[1] * dp, stp[2] * dp};
+                if (!pos) {
+                    // OV/RreaNewbornAdvance.cpp:536-548
+                    double const total_tau = taus[0] + taus[1];
+                    L.hm_expectation(w * (total_tau > 0.0 ? probability_from_optical_depth(total_tau) * taus[0] / total_tau
+                                                          : 0.0));
+                }
+                return taus;
             },
             [&](uint64_t ord) {
                 idx = pack(sc, ord, 0U);
@@ -3252,6 +3573,8 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                     em.ion_event(w);
                     em.ionization(std::hypot(ex_, ey_), ez_, w, T, T);
                     fx.tallies[ORC_T_MOLLER_EVENTS] += 1.0;
+                    L.hm_event(w, T);
+                    L.transport_ionization(w, T, false, true);
                     Closure c;
                     c.parent(0, w, before, k, true);
                     c.particle(0, w, T);
@@ -3272,6 +3595,8 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                         em.positive_ion(std::hypot(ex_, ey_), ez_, w, T);
                         fx.tallies[ORC_T_POSITIVE_IONS] += w;
                         fx.tallies[ORC_T_BHABHA_EVENTS] += 1.0;
+                        L.transport_ionization(w, T, false, true);
+                        L.bhabha(w, T, true);
                         Closure c;
                         c.parent(1, w, before, k, true);
                         c.particle(0, w, T);
@@ -3294,6 +3619,8 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
                                    gx, gy, gz);
                     photon_recoil(before, pe, gx, gy, gz, dx, dy, dz);
                     k = after;
+                    if (pos) L.positron_brems(w, pe);
+                    else L.brems_photon(w, pe);
                     if (tracked) {
                         em.secondary(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
                         fx.tallies[ORC_T_BREMS_PHOTONS] += 1.0;
@@ -3366,6 +3693,7 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
         [&] { return terminal_interaction || terminal_at_point; }, finish_substep,
         [&](double se) { return advance_running_position(path, se, run_x, run_y, run_z); },
         [&] { return path.exits; });
+    if (pos && positron_total_loss > 0.0) L.positron_transport(w, positron_total_loss, false);  // :686-689
     if (terminal_interaction) {
         out.status = ORC_ANNIHILATED;
         fx.tallies[ORC_T_NEWBORN_TERMINATED] += 1.0;
@@ -3382,6 +3710,7 @@ inline void charged_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary&
         if (pos) {
             spawn_annihilation(ctx, em, kProcAnnihilation, fx_, fy_, fz_, final_r, dx, dy, dz, k, true,
                                pack(std::min<uint64_t>(substeps, kSegmentCap - 1U), 0U, 0U), final_el);
+            L.positron_transport(w, k, true);  // OV/RreaNewbornAdvance.cpp:729
             out.status = ORC_ANNIHILATED;
         } else {
             fx.tallies[ORC_T_LOW_ENERGY_ELECTRONS] += w;
@@ -3437,6 +3766,7 @@ inline void photon_newborn(StepContext const& ctx, ChunkLog& fx, orc_secondary& 
     auto escape = [&]() {
         out.status = ORC_ESCAPE;
         fx.tallies[ORC_T_NEWBORN_ESCAPES] += 1.0;
+        Ledger{fx.tallies, 0.0}.photon_escape(w, energy);
     };
     auto survive_at = [&](double x, double y, double z) {
         p.x = x; p.y = y; p.z = z;
@@ -3780,7 +4110,7 @@ static int charged_step(bool positron, void* h, const orc_geometry* g, const orc
             if (ns >= sec_cap) { *n_sec = ns; return ORC_ERR_SECONDARY_CAPACITY; }
             sec[ns++] = s;
         }
-        for (int q = 0; q < ORC_T_COUNT; ++q) tallies[q] += fx.tallies[q];
+        fold_tallies(tallies, fx.tallies);
         for (auto const& op : fx.deposits) {
             if (op.kind == 0) {
                 if (op.a[4] != 0.0) deposit_segment(grid, op.a[4], cfg->dt_s, op.a[0], op.a[1], op.a[2], op.a[3]);
@@ -3899,7 +4229,7 @@ int orc_photon_step(void* h, const orc_geometry* g, const orc_config* cfg, int64
             if (ns >= sec_cap) { *n_sec = ns; return ORC_ERR_SECONDARY_CAPACITY; }
             sec[ns++] = s;
         }
-        for (int q = 0; q < ORC_T_COUNT; ++q) tallies[q] += fx.tallies[q];
+        fold_tallies(tallies, fx.tallies);
         int const se = replay_sources(g, cfg->low_energy_cutoff_eV, fx.sources);
         if (se) { *n_sec = ns; return se; }
     }
@@ -4083,7 +4413,7 @@ extern "C" int orc_newborn_drain(void* h, const orc_geometry* g, const orc_confi
                 c.lineage = nx++;
                 fifo.push_back(c);
             }
-            for (int q = 0; q < ORC_T_COUNT; ++q) tallies[q] += fx.tallies[q];
+            fold_tallies(tallies, fx.tallies);
             for (auto const& op : fx.deposits) {
                 if (op.kind == 0) {
                     if (op.a[4] != 0.0) deposit_segment(grid, op.a[4], cfg->dt_s, op.a[0], op.a[1], op.a[2], op.a[3]);
@@ -4477,3 +4807,5 @@ extern "C" int orc_validate_group(const orc_group* g, double cutoff, double* res
     out |= 4;
     return v == 0 ? (out | 8) : out;
 }
+
+extern "C" void orc_fold_tallies(double* into, const double* from) { fold_tallies(into, from); }

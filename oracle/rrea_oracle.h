@@ -113,8 +113,94 @@ enum {
     ORC_T_NEWBORN_SURVIVORS,    /* newborns that reach the end of the step */
     ORC_T_NEWBORN_ESCAPES,      /* newborns that leave the domain */
     ORC_T_NEWBORN_TERMINATED,   /* newborns demoted, annihilated or absorbed */
-    ORC_T_COUNT = 24,
+    ORC_T_COUNT = 24 + 75,      /* 21-23 reserved, then the event ledger */
 };
+
+/* The event ledger (RreaEventLedger, NA/include/rrea/RreaEventLedger.H:18-96)
+   from tallies[ORC_LEDGER_OFFSET] on, in the reference's field order; the
+   ORC_LEDGER_IS_MAX slots are running maxima, the rest sums. */
+#define ORC_LEDGER_OFFSET 24
+enum {
+    ORC_L_PHOTON_INTERACTION_COUNT = 0,
+    ORC_L_ELECTRON_BREMS_PHOTON_COUNT,
+    ORC_L_TOTAL_BREMS_PHOTON_COUNT,
+    ORC_L_PHOTON_INTERACTION_WEIGHT,
+    ORC_L_ELECTRON_BREMS_PHOTON_WEIGHT,
+    ORC_L_ELECTRON_BREMS_PHOTON_ENERGY_EV,
+    ORC_L_TOTAL_BREMS_PHOTON_WEIGHT,
+    ORC_L_TOTAL_BREMS_PHOTON_ENERGY_EV,
+    ORC_L_PHOTON_CONTINUOUS_SOFT_BREMS_ENERGY_EV,
+    ORC_L_COMPTON_INTERACTION_COUNT,
+    ORC_L_PHOTOELECTRIC_INTERACTION_COUNT,
+    ORC_L_PAIR_PRODUCTION_COUNT,
+    ORC_L_PAIR_NUCLEAR_COUNT,
+    ORC_L_PAIR_TRIPLET_COUNT,
+    ORC_L_COMPTON_INTERACTION_WEIGHT,
+    ORC_L_PHOTOELECTRIC_INTERACTION_WEIGHT,
+    ORC_L_PAIR_PRODUCTION_WEIGHT,
+    ORC_L_PAIR_NUCLEAR_WEIGHT,
+    ORC_L_PAIR_TRIPLET_WEIGHT,
+    ORC_L_PAIR_RECOIL_ELECTRON_ENERGY_EV,
+    ORC_L_PAIR_LOCAL_DEPOSIT_ENERGY_EV,
+    ORC_L_PHOTON_SEEDED_ELECTRON_WEIGHT,
+    ORC_L_PHOTON_SEEDED_POSITRON_WEIGHT,
+    ORC_L_PHOTON_SEEDED_KINETIC_ELECTRON_WEIGHT,
+    ORC_L_PHOTON_SEEDED_LOW_ELECTRON_WEIGHT,
+    ORC_L_PHOTON_SEEDED_KINETIC_POSITRON_WEIGHT,
+    ORC_L_PHOTON_SEEDED_BELOW_CUTOFF_POSITRON_WEIGHT,
+    ORC_L_COMPTON_KINETIC_ELECTRON_WEIGHT,
+    ORC_L_COMPTON_LOW_ELECTRON_WEIGHT,
+    ORC_L_PHOTOELECTRIC_KINETIC_ELECTRON_WEIGHT,
+    ORC_L_PHOTOELECTRIC_LOW_ELECTRON_WEIGHT,
+    ORC_L_PAIR_KINETIC_ELECTRON_WEIGHT,
+    ORC_L_PAIR_LOW_ELECTRON_WEIGHT,
+    ORC_L_PAIR_KINETIC_POSITRON_WEIGHT,
+    ORC_L_PAIR_BELOW_CUTOFF_POSITRON_WEIGHT,
+    ORC_L_PAIR_TRIPLET_RECOIL_KINETIC_ELECTRON_WEIGHT,
+    ORC_L_PAIR_TRIPLET_RECOIL_LOW_ELECTRON_WEIGHT,
+    ORC_L_PHOTON_ABSORBED_ENERGY_EV,
+    ORC_L_PHOTON_ESCAPED_ENERGY_EV,
+    ORC_L_PHOTON_TO_CHARGED_ENERGY_EV,
+    ORC_L_PHOTON_LOCAL_DEPOSIT_ENERGY_EV,
+    ORC_L_POSITRON_TRANSPORT_ENERGY_LOSS_EV,
+    ORC_L_POSITRON_THERMALIZED_COUNT,
+    ORC_L_POSITRON_BHABHA_EVENT_COUNT,
+    ORC_L_POSITRON_BHABHA_EVENT_WEIGHT,
+    ORC_L_POSITRON_BHABHA_SECONDARY_ENERGY_EV,
+    ORC_L_POSITRON_BHABHA_KINETIC_SECONDARY_WEIGHT,
+    ORC_L_POSITRON_BREMS_EVENT_COUNT,
+    ORC_L_POSITRON_BREMS_EVENT_WEIGHT,
+    ORC_L_POSITRON_BREMS_PHOTON_ENERGY_EV,
+    ORC_L_TRANSPORT_IONIZATION_WEIGHT,
+    ORC_L_TRANSPORT_LOW_SECONDARY_WEIGHT,
+    ORC_L_TRANSPORT_KINETIC_SECONDARY_WEIGHT,
+    ORC_L_TRANSPORT_ENERGY_LOSS_EV,
+    ORC_L_HARD_MOLLER_EVENT_COUNT,
+    ORC_L_HARD_MOLLER_EVENT_WEIGHT,
+    ORC_L_DIRECT_HARD_MOLLER_ELECTRON_WEIGHT,
+    ORC_L_HARD_MOLLER_EXPECTED_EVENT_WEIGHT,
+    ORC_L_HARD_MOLLER_SECONDARY_ENERGY_WEIGHTED_SUM_EV,
+    ORC_L_HARD_MOLLER_SECONDARY_ENERGY_WEIGHT_SUM,
+    ORC_L_HARD_MOLLER_SECONDARY_ENERGY_MAX_EV,
+    ORC_L_PARENT_DRAG_LOSS_EV,
+    ORC_L_ION_PAIR_SOURCE_LOSS_EV,
+    ORC_L_RESTRICTED_COLLISION_LOSS_EV,
+    ORC_L_EXPLICIT_HARD_MOLLER_ENERGY_LOSS_EV,
+    ORC_L_MAX_TRANSPORT_HARD_MOLLER_TAU,
+    ORC_L_MAX_TRANSPORT_BREMS_TAU,
+    ORC_L_MAX_TRANSPORT_ELECTRON_TOTAL_COMPETING_TAU,
+    ORC_L_MAX_TRANSPORT_PHOTON_TAU,
+    ORC_L_MAX_TRANSPORT_POSITRON_TAU,
+    ORC_L_MAX_TRANSPORT_SUBCYCLES,
+    ORC_L_TRANSPORT_OPTICAL_DEPTH_VIOLATION_COUNT,
+    ORC_L_MAX_TRANSPORT_ELECTRON_ENERGY_EV,
+    ORC_L_MAX_TRANSPORT_PHOTON_ENERGY_SEEN_EV,
+    ORC_L_MAX_TRANSPORT_POSITRON_ENERGY_EV,
+    ORC_LEDGER_COUNT
+};
+#define ORC_LEDGER_IS_MAX(q) ((q) == ORC_L_HARD_MOLLER_SECONDARY_ENERGY_MAX_EV \
+    || ((q) >= ORC_L_MAX_TRANSPORT_HARD_MOLLER_TAU && (q) <= ORC_L_MAX_TRANSPORT_SUBCYCLES) \
+    || (q) >= ORC_L_MAX_TRANSPORT_ELECTRON_ENERGY_EV)
 
 /* Error codes; 0 means success. */
 enum {
@@ -149,6 +235,10 @@ typedef struct orc_group {
 /* RreaValidateSecondaryGroupConservation: bit 0 required, 1 present, 2 finite,
    3 valid; res[2] = charge, energy residuals (may be NULL). */
 int orc_validate_group(const orc_group* g, double low_energy_cutoff_eV, double* res);
+
+/* RreaFoldEventLedger: into += from, the ledger's running maxima by max
+   (ORC_T_COUNT doubles each). */
+void orc_fold_tallies(double* into, const double* from);
 
 /* One material-source record into g->sources: kind 0 ionization event,
    1 positive ion only, 2 low-energy electron (RreaWarpXCoupling::

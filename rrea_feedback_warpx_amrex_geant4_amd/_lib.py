@@ -163,7 +163,8 @@ class PopulationConfig(C.Structure):
                 ("reserved", C.c_int32)]
 
 
-TALLY_COUNT = 24
+TALLY_COUNT = 24 + 75  # RREA_TALLY_COUNT: the counts, then the event ledger
+LEDGER_OFFSET = 24
 SOURCE_FIELDS = 5
 STATUS = {0: "survive", 1: "escape", 2: "demoted", 3: "zero_weight", 4: "no_writeback", 5: "skipped_invalid",
           6: "annihilated", 7: "absorbed"}

@@ -280,6 +280,286 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        double const o = __shfl_xor(v, off, 64);
+        v = v < o ? o : v;
+    }
+    return v;
+}
+
+// A running maximum of non-negative doubles, whose bit patterns order as
+// unsigned integers (a zeroed slot is the empty maximum).
+__device__ __forceinline__ void atomic_max_nonneg(double* p, double v)
+{
+    if (v > 0.0) {
+        atomicMax(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(__double_as_longlong(v)));
+    }
+}
+
+__host__ __device__ constexpr bool tally_is_max(int q)
+{
+    return q >= RREA_LEDGER_OFFSET && RREA_LEDGER_IS_MAX(q - RREA_LEDGER_OFFSET);
+}
+
+// The workgroup's event ledger in LDS: zeroed before any record ...
+__device__ __forceinline__ void ledger_begin(double* ledger)
+{
+    for (int q = threadIdx.x; q < RREA_LEDGER_COUNT; q += blockDim.x) ledger[q] = 0.0;
+    __syncthreads();
+}
+
+// ... and folded into the device tallies once every record is in (call after a
+// workgroup barrier that follows the last record)
+__device__ __forceinline__ void ledger_flush(double const* ledger, double* tallies)
+{
+    for (int q = threadIdx.x; q < RREA_LEDGER_COUNT; q += blockDim.x) {
+        double const v = ledger[q];
+        double* const out = tallies + RREA_LEDGER_OFFSET + q;
+        if (RREA_LEDGER_IS_MAX(q)) atomic_max_nonneg(out, v);
+        else if (v != 0.0) atomic_add(out, v);
+    }
+}
+
+// The record operations of RreaEventLedger (NA/include/rrea/RreaEventLedger.H:
+// 98-413) on the workgroup's ledger in LDS (RREA_LEDGER_COUNT doubles, zeroed
+// at the start of the kernel and folded into the device tallies at its end);
+// cutoff is the ledger's low_energy_cutoff_eV.  Sums add atomically; the
+// running maxima keep the larger value (atomic max on the bit pattern of a
+// non-negative double; a zero, negative or NaN value leaves the maximum as
+// it is, as the reference's max does for a ledger that starts at zero).
+struct Ledger {
+    double* t;
+    double cutoff;
+    __device__ void add(int q, double v)
+    {
+        if (v != 0.0) atomicAdd(t + q, v);
+    }
+    __device__ void up(int q, double v)
+    {
+        if (v > 0.0) {
+            atomicMax(reinterpret_cast<unsigned long long*>(t + q), static_cast<unsigned long long>(__double_as_longlong(v)));
+        }
+    }
+    __device__ static double pos(double v) { return dmax(v, 0.0); }
+
+    __device__ void transport_ionization(double w, double loss, bool low, bool kinetic)  // :98-116
+    {
+        if (w <= 0.0) return;
+        add(RREA_L_TRANSPORT_IONIZATION_WEIGHT, w);
+        add(RREA_L_TRANSPORT_ENERGY_LOSS_EV, w * pos(loss));
+        if (low) add(RREA_L_TRANSPORT_LOW_SECONDARY_WEIGHT, w);
+        if (kinetic) add(RREA_L_TRANSPORT_KINETIC_SECONDARY_WEIGHT, w);
+    }
+    __device__ void loss_partition(double w, double drag, double col, double hard)  // :118-134
+    {
+        if (w <= 0.0) return;
+        add(RREA_L_PARENT_DRAG_LOSS_EV, w * pos(drag));
+        add(RREA_L_ION_PAIR_SOURCE_LOSS_EV, w * pos(col));
+        add(RREA_L_RESTRICTED_COLLISION_LOSS_EV, w * pos(col));
+        add(RREA_L_EXPLICIT_HARD_MOLLER_ENERGY_LOSS_EV, w * pos(hard));
+    }
+    __device__ void soft_brems(double w, double e)  // :136-143
+    {
+        if (w > 0.0 && e > 0.0) add(RREA_L_PHOTON_CONTINUOUS_SOFT_BREMS_ENERGY_EV, w * e);
+    }
+    __device__ void hm_expectation(double x)  // :145-150
+    {
+        if (x > 0.0) add(RREA_L_HARD_MOLLER_EXPECTED_EVENT_WEIGHT, x);
+    }
+    __device__ void hm_event(double w, double sec)  // :152-168
+    {
+        if (w <= 0.0) return;
+        add(RREA_L_HARD_MOLLER_EVENT_COUNT, 1.0);
+        add(RREA_L_HARD_MOLLER_EVENT_WEIGHT, w);
+        if (sec >= cutoff) add(RREA_L_DIRECT_HARD_MOLLER_ELECTRON_WEIGHT, w);
+        add(RREA_L_HARD_MOLLER_SECONDARY_ENERGY_WEIGHTED_SUM_EV, w * pos(sec));
+        add(RREA_L_HARD_MOLLER_SECONDARY_ENERGY_WEIGHT_SUM, w);
+        up(RREA_L_HARD_MOLLER_SECONDARY_ENERGY_MAX_EV, sec);
+    }
+    __device__ void photon_interaction(double w)  // :170-180
+    {
+        if (w <= 0.0) return;
+        add(RREA_L_PHOTON_INTERACTION_COUNT, 1.0);
+        add(RREA_L_PHOTON_INTERACTION_WEIGHT, w);
+    }
+    __device__ void photon_escape(double w, double e)  // :182-189
+    {
+        if (w > 0.0 && e > 0.0) add(RREA_L_PHOTON_ESCAPED_ENERGY_EV, w * e);
+    }
+    __device__ void brems_photon(double w, double e)  // :191-204
+    {
+        if (w <= 0.0 || e <= 0.0) return;
+        add(RREA_L_ELECTRON_BREMS_PHOTON_COUNT, 1.0);
+        add(RREA_L_TOTAL_BREMS_PHOTON_COUNT, 1.0);
+        add(RREA_L_ELECTRON_BREMS_PHOTON_WEIGHT, w);
+        add(RREA_L_ELECTRON_BREMS_PHOTON_ENERGY_EV, w * e);
+        add(RREA_L_TOTAL_BREMS_PHOTON_WEIGHT, w);
+        add(RREA_L_TOTAL_BREMS_PHOTON_ENERGY_EV, w * e);
+    }
+    // a photon-seeded electron of kinetic energy e, kinetic or low-energy
+    __device__ void seeded_electron(double w, double e, int kinetic_slot, int low_slot)
+    {
+        if (e >= 0.0) {
+            add(RREA_L_PHOTON_SEEDED_ELECTRON_WEIGHT, w);
+            if (e >= cutoff) {
+                add(RREA_L_PHOTON_SEEDED_KINETIC_ELECTRON_WEIGHT, w);
+                add(kinetic_slot, w);
+            } else {
+                add(RREA_L_PHOTON_SEEDED_LOW_ELECTRON_WEIGHT, w);
+                add(low_slot, w);
+            }
+        }
+    }
+    __device__ void compton(double w, double recoil, double charged, double local)  // :206-235
+    {
+        photon_interaction(w);
+        add(RREA_L_COMPTON_INTERACTION_COUNT, 1.0);
+        add(RREA_L_COMPTON_INTERACTION_WEIGHT, w);
+        seeded_electron(w, recoil, RREA_L_COMPTON_KINETIC_ELECTRON_WEIGHT, RREA_L_COMPTON_LOW_ELECTRON_WEIGHT);
+        add(RREA_L_PHOTON_TO_CHARGED_ENERGY_EV, w * pos(charged));
+        add(RREA_L_PHOTON_LOCAL_DEPOSIT_ENERGY_EV, w * pos(local));
+        add(RREA_L_TRANSPORT_ENERGY_LOSS_EV, w * (pos(charged) + pos(local)));
+    }
+    __device__ void photoelectric(double w, double photon, double electron, double charged, double local)  // :237-269
+    {
+        photon_interaction(w);
+        add(RREA_L_PHOTOELECTRIC_INTERACTION_COUNT, 1.0);
+        add(RREA_L_PHOTOELECTRIC_INTERACTION_WEIGHT, w);
+        seeded_electron(w, electron, RREA_L_PHOTOELECTRIC_KINETIC_ELECTRON_WEIGHT,
+                        RREA_L_PHOTOELECTRIC_LOW_ELECTRON_WEIGHT);
+        add(RREA_L_PHOTON_ABSORBED_ENERGY_EV, w * pos(photon));
+        add(RREA_L_PHOTON_TO_CHARGED_ENERGY_EV, w * pos(charged));
+        add(RREA_L_PHOTON_LOCAL_DEPOSIT_ENERGY_EV, w * pos(local));
+        add(RREA_L_TRANSPORT_ENERGY_LOSS_EV, w * pos(photon));
+    }
+    __device__ void pair(double w, double photon, double electron, double positron, double recoil, double charged,
+                         double local, bool triplet)  // :271-340
+    {
+        photon_interaction(w);
+        add(RREA_L_PAIR_PRODUCTION_COUNT, 1.0);
+        add(RREA_L_PAIR_PRODUCTION_WEIGHT, w);
+        add(triplet ? RREA_L_PAIR_TRIPLET_COUNT : RREA_L_PAIR_NUCLEAR_COUNT, 1.0);
+        add(triplet ? RREA_L_PAIR_TRIPLET_WEIGHT : RREA_L_PAIR_NUCLEAR_WEIGHT, w);
+        seeded_electron(w, electron, RREA_L_PAIR_KINETIC_ELECTRON_WEIGHT, RREA_L_PAIR_LOW_ELECTRON_WEIGHT);
+        add(RREA_L_PHOTON_SEEDED_POSITRON_WEIGHT, w);
+        if (positron >= cutoff) {
+            add(RREA_L_PHOTON_SEEDED_KINETIC_POSITRON_WEIGHT, w);
+            add(RREA_L_PAIR_KINETIC_POSITRON_WEIGHT, w);
+        } else {
+            add(RREA_L_PHOTON_SEEDED_BELOW_CUTOFF_POSITRON_WEIGHT, w);
+            add(RREA_L_PAIR_BELOW_CUTOFF_POSITRON_WEIGHT, w);
+        }
+        if (triplet) {
+            add(RREA_L_PHOTON_SEEDED_ELECTRON_WEIGHT, w);
+            if (recoil >= cutoff) {
+                add(RREA_L_PHOTON_SEEDED_KINETIC_ELECTRON_WEIGHT, w);
+                add(RREA_L_PAIR_TRIPLET_RECOIL_KINETIC_ELECTRON_WEIGHT, w);
+            } else {
+                add(RREA_L_PHOTON_SEEDED_LOW_ELECTRON_WEIGHT, w);
+                add(RREA_L_PAIR_TRIPLET_RECOIL_LOW_ELECTRON_WEIGHT, w);
+            }
+        }
+        add(RREA_L_PAIR_RECOIL_ELECTRON_ENERGY_EV, w * pos(recoil));
+        add(RREA_L_PAIR_LOCAL_DEPOSIT_ENERGY_EV, w * pos(local));
+        add(RREA_L_PHOTON_ABSORBED_ENERGY_EV, w * pos(photon));
+        add(RREA_L_PHOTON_LOCAL_DEPOSIT_ENERGY_EV, w * pos(local));
+        add(RREA_L_PHOTON_TO_CHARGED_ENERGY_EV, w * pos(charged));
+        add(RREA_L_TRANSPORT_ENERGY_LOSS_EV, w * pos(charged + local - 2.0 * kMeC2));
+    }
+    __device__ void positron_transport(double w, double loss, bool thermalized)  // :342-354
+    {
+        if (w <= 0.0) return;
+        add(RREA_L_POSITRON_TRANSPORT_ENERGY_LOSS_EV, w * pos(loss));
+        if (thermalized) add(RREA_L_POSITRON_THERMALIZED_COUNT, 1.0);
+    }
+    __device__ void bhabha(double w, double sec, bool kinetic)  // :356-370
+    {
+        if (w <= 0.0) return;
+        add(RREA_L_POSITRON_BHABHA_EVENT_COUNT, 1.0);
+        add(RREA_L_POSITRON_BHABHA_EVENT_WEIGHT, w);
+        add(RREA_L_POSITRON_BHABHA_SECONDARY_ENERGY_EV, w * pos(sec));
+        if (kinetic) add(RREA_L_POSITRON_BHABHA_KINETIC_SECONDARY_WEIGHT, w);
+    }
+    __device__ void positron_brems(double w, double e)  // :372-385
+    {
+        if (w <= 0.0 || e <= 0.0) return;
+        add(RREA_L_POSITRON_BREMS_EVENT_COUNT, 1.0);
+        add(RREA_L_TOTAL_BREMS_PHOTON_COUNT, 1.0);
+        add(RREA_L_POSITRON_BREMS_EVENT_WEIGHT, w);
+        add(RREA_L_POSITRON_BREMS_PHOTON_ENERGY_EV, w * e);
+        add(RREA_L_TOTAL_BREMS_PHOTON_WEIGHT, w);
+        add(RREA_L_TOTAL_BREMS_PHOTON_ENERGY_EV, w * e);
+    }
+    __device__ void guard(double hm_tau, double brems_tau, double total_tau, double photon_tau, double positron_tau,
+                          uint64_t subcycles, double electron_e, double photon_e, double positron_e,
+                          bool violation)  // :387-413
+    {
+        up(RREA_L_MAX_TRANSPORT_HARD_MOLLER_TAU, hm_tau);
+        up(RREA_L_MAX_TRANSPORT_BREMS_TAU, brems_tau);
+        up(RREA_L_MAX_TRANSPORT_ELECTRON_TOTAL_COMPETING_TAU, total_tau);
+        up(RREA_L_MAX_TRANSPORT_PHOTON_TAU, photon_tau);
+        up(RREA_L_MAX_TRANSPORT_POSITRON_TAU, positron_tau);
+        up(RREA_L_MAX_TRANSPORT_SUBCYCLES, static_cast<double>(subcycles));
+        up(RREA_L_MAX_TRANSPORT_ELECTRON_ENERGY_EV, electron_e);
+        up(RREA_L_MAX_TRANSPORT_PHOTON_ENERGY_SEEN_EV, photon_e);
+        up(RREA_L_MAX_TRANSPORT_POSITRON_ENERGY_EV, positron_e);
+        if (violation) add(RREA_L_TRANSPORT_OPTICAL_DEPTH_VIOLATION_COUNT, 1.0);
+    }
+    // the ledger part of RreaWarpXCoupling::RecordPositronAnnihilation
+    // (RecordPhotonLocalDeposit), OV/RreaCouplingSources.cpp:518-553
+    __device__ void photon_local_deposit(double w, double e)
+    {
+        if (w > 0.0 && e > 0.0) add(RREA_L_PHOTON_LOCAL_DEPOSIT_ENERGY_EV, w * e);
+    }
+};
+
+// Probability of a competing-risk optical depth, OV/RreaMaterialPath.H:203-210
+__device__ __forceinline__ double probability_from_optical_depth(double tau)
+{
+    if (!(tau > 0.0)) return 0.0;
+    return dmin(1.0, -expm1(-tau));
+}
+
+// The photon energy flows the interaction records carry: charged (kinetic
+// plus created rest mass) and local, RreaComptonPhotonEnergyFlow /
+// RreaPhotoelectricPhotonEnergyFlow / RreaPairPhotonEnergyFlow,
+// OV/RreaParticleInteraction.H:1014-1092
+struct EnergyFlow { double charged, local; };
+
+__device__ __forceinline__ EnergyFlow compton_flow(double recoil, double scattered, double cutoff, double gcut)
+{
+    EnergyFlow f{0.0, 0.0};
+    double const r = dmax(recoil, 0.0), sc = dmax(scattered, 0.0);
+    if (r >= cutoff) f.charged = r;
+    else f.local = r;
+    if (sc < gcut) f.local += sc;
+    return f;
+}
+
+__device__ __forceinline__ EnergyFlow photoelectric_flow(double electron, double binding, double cutoff)
+{
+    EnergyFlow f{0.0, dmax(binding, 0.0)};
+    double const e = dmax(electron, 0.0);
+    if (e >= cutoff) f.charged = e;
+    else f.local += e;
+    return f;
+}
+
+__device__ __forceinline__ EnergyFlow pair_flow(double electron, double positron, double recoil, double local,
+                                                double cutoff, bool has_recoil)
+{
+    EnergyFlow f{2.0 * kMeC2, dmax(local, 0.0)};
+    double const e[3] = {dmax(electron, 0.0), dmax(positron, 0.0), has_recoil ? dmax(recoil, 0.0) : -1.0};
+    for (int q = 0; q < 3; ++q) {
+        if (e[q] < 0.0) continue;
+        if (e[q] >= cutoff) f.charged += e[q];
+        else f.local += e[q];
+    }
+    return f;
+}
+
 // ---------------------------------------------------------------------------
 // The electron step kernel
 // ---------------------------------------------------------------------------
@@ -455,6 +735,7 @@ struct Emitter {
     int32_t generation;     // FIFO generation of the records made here
     int32_t instance;       // the simulation instance (-1: a plain step)
     Env const* env;         // where material sources go
+    double* ledger;         // the workgroup's event ledger (LDS)
 
     // deposit_local_energy_as_ion_pairs: E / W_air pairs booked as one
     // ionization event of diagnostic loss W_air at (r, z)
@@ -465,6 +746,7 @@ struct Emitter {
         if (pw <= 0.0) return;
         ion += pw;
         source_ionization(*env, A.cfg.low_energy_cutoff_eV, r, z, pw, 0.0, kWAir, err);
+        Ledger{ledger, 0.0}.transport_ionization(pw, kWAir, true, false);
     }
     __device__ void ionization(double r, double z, double weight, double secondary, double loss)
     {
@@ -539,6 +821,7 @@ __device__ void spawn_annihilation(DevTables const& T, double gcut, uint64_t rng
     em.positive_ion(sr, z, em.w, 0.0);
     tally[12] += em.w;
     tally[10] += 1.0;
+    Ledger{em.ledger, 0.0}.photon_local_deposit(em.w, local);
     {
         GroupClosure c;
         c.parent(1, em.w, dmax(k, 0.0), 0.0, false);
@@ -552,13 +835,15 @@ __device__ void spawn_annihilation(DevTables const& T, double gcut, uint64_t rng
 
 // RreaChargedCompetingStpRates OV/RreaChargedFinalStates.H:107-152 (electron)
 __device__ __forceinline__ void electron_stp_rates(DevTables const& T, Bracket const& b, double thr, double gcut,
-                                                   double e, bool hm_on, double& hm, double& br, int32_t& err)
+                                                   double e, bool hm_on, double& hm, double& br, int32_t& err,
+                                                   double* hm_inv = nullptr)
 {
     hm = 0.0;
     br = 0.0;
     if (hm_on && thr > 0.0 && e > 2.0 * thr) {
         // ElectronHardMollerMeanFreePath :2394-2412, then 1/mfp
         double const v = charged_value(T, kEMoller, b, err) * 1.0;
+        if (hm_inv) *hm_inv = v;
         double const mfp = v > 0.0 ? 1.0 / v : 1.0e99;
         hm = mfp > 0.0 ? 1.0 / mfp : 0.0;
     }
@@ -626,6 +911,28 @@ __device__ __forceinline__ bool positron_channel_open(DevTables const& T, double
     return false;
 }
 
+// ElectronHardMollerEnergyLoss NA/src/RreaInteractionTables.cpp:2414-2440 at
+// unit density ratio: the mean drag of the discrete delta-ray channel
+// given the channel's inverse length there (inv); the mean secondary is the
+// semilog interpolation on the charged grid, whose t the bracket holds
+__device__ __forceinline__ double hard_moller_loss_at(DevTables const& T, Bracket const& b, double inv)
+{
+    if (!(inv > 0.0)) return 0.0;
+    double mean;
+    if (b.kind == 1) mean = T.hm_mean[0];
+    else if (b.kind == 2) mean = T.hm_mean[T.nc - 1];
+    else if (b.kind == 3) mean = T.hm_mean[b.hi];
+    else mean = T.hm_mean[b.lo] + b.t * (T.hm_mean[b.hi] - T.hm_mean[b.lo]);
+    if (!(mean > 0.0)) return 0.0;
+    return mean * inv;
+}
+
+__device__ double hard_moller_energy_loss(DevTables const& T, Bracket const& b, double e, int32_t& err)
+{
+    (void)e;
+    return hard_moller_loss_at(T, b, charged_value(T, kEMoller, b, err));
+}
+
 // electron_ensemble_mean_drag_ev_per_m OV/RreaTransportStep.H:226-247
 __device__ double ensemble_drag(DevTables const& T, double e, double ratio, bool hm_on, int32_t& err)
 {
@@ -633,14 +940,7 @@ __device__ double ensemble_drag(DevTables const& T, double e, double ratio, bool
     Bracket b;
     if (!charged_bracket(T, e, b)) { if (!err) err = kErrRange; return 0.0; }
     double dd = 0.0;
-    if (hm_on) {
-        // ElectronHardMollerEnergyLoss :2414-2440
-        double const inv = charged_value(T, kEMoller, b, err);
-        if (inv > 0.0) {
-            double const mean = charged_semilog(T, T.hm_mean, b, e);
-            if (mean > 0.0) dd = mean * inv * ratio;
-        }
-    }
+    if (hm_on) dd = hard_moller_energy_loss(T, b, e, err) * ratio;
     double const realized = charged_value(T, kECol, b, err) * ratio + charged_value(T, kESoft, b, err) * ratio;
     // ElectronTrackedBremsEnergyLoss :2678-2696
     double tb = 0.0;
@@ -725,7 +1025,7 @@ template <> struct Species<true> {
 // (OV/RreaTransportStep.H:720-820) and the substep and event skeletons of
 // OV/RreaChargedInterleave.H:546-732.
 template <bool kPos>
-__device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip, double* tally)
+__device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip, double* tally, double* ledger)
 {
     using S = Species<kPos>;
     int32_t err = kOk;
@@ -797,7 +1097,7 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
             }
             double tr_dp = 0.0, tr_len = 0.0;
 
-            Emitter em{A, ip, pkey, w, ion, lineage, err, kPos ? 1 : 0, 0, E.instance, &E};
+            Emitter em{A, ip, pkey, w, ion, lineage, err, kPos ? 1 : 0, 0, E.instance, &E, ledger};
             auto deposit_local = [&](double energy, double r, double z) { em.deposit_local(energy, r, z); };
             auto push = [&](int32_t species, int32_t process, double x, double y, double z, double sdx, double sdy,
                             double sdz, double energy, double birth) {
@@ -819,6 +1119,7 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                 if constexpr (kPos) {
                     // at rest where it stands, before any transport: a zero-length net chord
                     annihilate(x_particle, y_particle, z_particle, true, 0U, 1.0);
+                    Ledger{ledger, cutoff}.positron_transport(w, k, true);  // OV/RreaPositronTransport.cpp:152
                     terminal = kAnnihilated;
                 } else {
                     terminal = kDemote;
@@ -894,8 +1195,19 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                     if (imd > m) m = imd;
                     subcycles = m < cfg.max_subcycles ? m : cfg.max_subcycles;
                 }
+                // the reported per-substep taus: uniform slices of the driver chord over its
+                // clipped span (OV/RreaElectronTransport.cpp:250-278, OV/RreaPositronTransport.cpp:241-259)
+                if constexpr (kPos) {
+                    double const max_positron_tau = (s00 + s01 + s02) * max_uniform_slice(iv, path, subcycles) / span;
+                    Ledger{ledger, cutoff}.guard(0.0, 0.0, 0.0, 0.0, max_positron_tau, subcycles, 0.0, 0.0, k, violation);
+                } else {
+                    double const slice = max_uniform_slice(iv, path, subcycles) / span;
+                    Ledger{ledger, cutoff}.guard(s00 * slice, s01 * slice, (s00 + s01) * slice, 0.0, 0.0, subcycles, k, 0.0,
+                                                0.0, violation);
+                }
                 if (cfg.strict_transport_guards && violation && !err) err = kErrGuard;
                 tally[6] += static_cast<double>(subcycles);
+                double positron_total_loss = 0.0;
 
                 bool exited = false, terminated = false, annihilated = false;
                 double mid_x = x_particle, mid_y = y_particle, mid_r = r_particle, mid_z = z_particle;
@@ -963,10 +1275,24 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                                 double mx, my, mz;
                                 point_at_full(path, 0.5 * (lb + le), mx, my, mz);
                                 em.ionization(hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
+                                Ledger{ledger, cutoff}.transport_ionization(pw, kWAir, true, false);
                             }
                         }
                     }
                     tally[7] += w * drag;
+                    {
+                        // the plan's partition: soft radiative share, and the mean hard-Moller
+                        // transfer over the same density path (electrons; OV/RreaElectronTransport.cpp:306-338)
+                        Ledger led{ledger, cutoff};
+                        led.soft_brems(w, dmax(0.0, drag - col_loss));
+                        if constexpr (kPos) {
+                            positron_total_loss += drag;
+                        } else {
+                            double const hstp = (hm_on && thr > 0.0 && stop_e > 2.0 * thr)
+                                ? hard_moller_energy_loss(T, bs, stop_e, err) : 0.0;
+                            led.loss_partition(w, drag, col_loss, dmin(hstp * pm, avail));
+                        }
+                    }
                     k = dmax(0.0, k - drag);
                     if (reaches) k = cutoff;
                     return reaches;
@@ -1011,6 +1337,13 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                             if constexpr (kPos) positron_stp_rates(T, bf, thr, gcut, frozen_e, s0, s1, s2, err);
                             else electron_stp_rates(T, bf, thr, gcut, frozen_e, hm_on, s0, s1, err);
                             double const L = density_path(iv, path, rem, se);
+                            if constexpr (!kPos) {
+                                double const hard_tau = s0 * L, brems_tau = s1 * L;
+                                double const total_tau = hard_tau + brems_tau;
+                                Ledger{ledger, cutoff}.hm_expectation(
+                                    w * (total_tau > 0.0 ? probability_from_optical_depth(total_tau) * hard_tau / total_tau
+                                                         : 0.0));
+                            }
                             uint64_t idx = 0;
                             if (!live_hazard_index(sc, ord, idx, S::kId)) { err = kErrInvalid; break; }
                             double const u1 = rng_uniform(rng, idx, ord == 0U ? S::kDepth0 : S::kDepth1);
@@ -1081,6 +1414,11 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                                         em.positive_ion(hypot(ex_, ey_), ez_, w, T2);
                                         tally[12] += w;
                                         tally[11] += 1.0;
+                                        {
+                                            Ledger led{ledger, cutoff};
+                                            led.transport_ionization(w, T2, false, true);
+                                            led.bhabha(w, T2, true);
+                                        }
                                         GroupClosure c;
                                         c.parent(1, w, before, k, true);
                                         c.particle(0, w, T2);
@@ -1100,6 +1438,11 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                                     ion += w;
                                     em.ionization(hypot(ex_, ey_), ez_, w, T2, T2);
                                     tally[3] += 1.0;
+                                    {
+                                        Ledger led{ledger, cutoff};
+                                        led.hm_event(w, T2);
+                                        led.transport_ionization(w, T2, false, true);
+                                    }
                                     GroupClosure c;
                                     c.parent(0, w, before, k, true);
                                     c.particle(0, w, T2);
@@ -1134,6 +1477,8 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                                     }
                                 }
                                 k = after;
+                                if constexpr (kPos) Ledger{ledger, cutoff}.positron_brems(w, pe);
+                                else Ledger{ledger, cutoff}.brems_photon(w, pe);
                                 if (tracked) {
                                     push(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
                                     tally[5] += 1.0;
@@ -1216,6 +1561,8 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                     completed = sc + 1U;
                 }
 
+                // OV/RreaPositronTransport.cpp:776
+                if constexpr (kPos) Ledger{ledger, cutoff}.positron_transport(w, positron_total_loss, false);
                 if (!err) {
                     if (exited) escapes = clipped_escapes(path, D);
                     double const end_r = hypot(run_x, run_y);
@@ -1235,6 +1582,7 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                             double const ay = terminated ? mid_y : run_y;
                             double const az = terminated ? mid_z : run_z;
                             annihilate(ax, ay, az, true, subcycles, terminated ? stop_elapsed : 1.0);
+                            Ledger{ledger, cutoff}.positron_transport(w, k, true);  // OV/RreaPositronTransport.cpp:839
                             demote_r = terminated ? mid_r : end_r;
                             demote_z = az;
                             terminal = kAnnihilated;
@@ -1372,6 +1720,12 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
         }
         tally[13] += 1.0;
         {
+            Ledger led{em.ledger, cutoff};
+            led.transport_ionization(w, 0.0, !(recoil >= cutoff), recoil >= cutoff);
+            EnergyFlow const f = compton_flow(recoil, scattered, cutoff, gcut);
+            led.compton(w, recoil, f.charged, f.local);
+        }
+        {
             GroupClosure c;
             bool const survives = !(scattered < gcut);
             c.parent(2, w, incident, scattered, survives);
@@ -1409,8 +1763,13 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
             em.ionization(sr, z, w, ke, 0.0);
             em.deposit_local(ke, sr, z);
         }
+        Ledger{em.ledger, cutoff}.transport_ionization(w, 0.0, !(ke >= cutoff), ke >= cutoff);
         if (binding > 0.0) em.deposit_local(binding, sr, z);
         tally[14] += 1.0;
+        {
+            EnergyFlow const f = photoelectric_flow(ke, binding, cutoff);
+            Ledger{em.ledger, cutoff}.photoelectric(w, energy, ke, f.charged, f.local);
+        }
         GroupClosure c;
         c.parent(2, w, energy, 0.0, false);
         if (ke >= cutoff) c.particle(0, w, ke);
@@ -1441,6 +1800,7 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
         em.secondary(1, process, x, y, z, px, py, pz2, pair.positron, birth);
     } else {
         spawn_annihilation(T, gcut, rng, em, process, x, y, z, px, py, pz2, pair.positron, true, idx, birth, tally);
+        Ledger{em.ledger, cutoff}.positron_transport(w, pair.positron, true);  // OV/RreaPhotonFinalStates.H:320
     }
     if (pair.has_recoil) {
         double qx, qy, qz;
@@ -1452,6 +1812,10 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
     }
     if (pair.local > 0.0) em.deposit_local(pair.local, sr, z);
     tally[15] += 1.0;
+    {
+        EnergyFlow const f = pair_flow(pair.electron, pair.positron, pair.recoil, pair.local, cutoff, pair.has_recoil);
+        Ledger{em.ledger, cutoff}.pair(w, energy, pair.electron, pair.positron, pair.recoil, f.charged, f.local, triplet);
+    }
     {
         // the pair group; a sub-cutoff positron goes on as the particle its
         // at-rest annihilation (checked as its own group) consumes
@@ -1476,7 +1840,7 @@ __device__ __noinline__ bool photon_final_state(DevTables const& T, double cutof
 // exponential depth is sampled over the remaining chord per pass, and a
 // Compton survivor continues from its interaction point along the scattered
 // direction for the unused path length.
-__device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, double* tally)
+__device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, double* tally, double* ledger)
 {
     int32_t err = kOk;
     DevTables const& T = A.T;
@@ -1511,7 +1875,7 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
     double ion = 0.0;
     uint64_t lineage = 0;
     uint64_t events = 0;
-    Emitter em{A, ip, pkey, w, ion, lineage, err, 2, 0, E.instance, &E};
+    Emitter em{A, ip, pkey, w, ion, lineage, err, 2, 0, E.instance, &E, ledger};
 
     double energy = sqrt(ux * ux + uy * uy + uz * uz);
     double dx = ux, dy = uy, dz = uz;
@@ -1547,6 +1911,7 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
             if (escapes) {
                 status = RREA_ESCAPE;
                 tally[1] += 1.0;
+                Ledger{ledger, 0.0}.photon_escape(w, energy);  // OV/RreaPhotonTransport.cpp:192
             } else {
                 status = RREA_SURVIVE;
                 tally[0] += 1.0;
@@ -1560,6 +1925,8 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
         if (!finite(total) || total < 0.0) { err = kErrInvalid; break; }
         double const mfp = total > 0.0 ? 1.0 / total : 1.0e99;
         double const tau = chord_dp / mfp;
+        // one exponential depth per pass (OV/RreaPhotonTransport.cpp:222-233)
+        Ledger{ledger, 0.0}.guard(0.0, 0.0, 0.0, tau, 0.0, 1U, 0.0, energy, 0.0, false);
         uint64_t idx = 0;
         if (!live_hazard_index(0U, ordinal, idx, 2U)) { err = kErrInvalid; break; }
         double const draw = rng_uniform(rng, idx, ordinal == 0U ? kChPhotonInteraction : kChPhotonContinuationOpticalDepth);
@@ -1631,22 +1998,33 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
 }
 
 // workgroup reduction of the tallies, one atomic per tally per workgroup
+// Returns thread q's workgroup total of slot q (zero for threads past the slots).
 template <int kSlots = kTallyUsed>
-__device__ __forceinline__ void reduce_tallies(double* tally, double* out)
+__device__ __forceinline__ double reduce_tallies(double* tally, double* out)
 {
     __shared__ double red[kSlots][kBlock / 64];
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
     for (int q = 0; q < kSlots; ++q) {
-        double const s = wave_sum(tally[q]);
+        double const v = tally[q];
+        double s = 0.0;
+        if (__ballot(v != 0.0) != 0) s = tally_is_max(q) ? wave_max(v) : wave_sum(v);
         if (lane == 0) red[q][wave] = s;
     }
     __syncthreads();
-    if (threadIdx.x < kSlots) {
+    double mine = 0.0;
+    for (int q = threadIdx.x; q < kSlots; q += kBlock) {
+        bool const mx = tally_is_max(q);
         double s = 0.0;
-        for (int wv = 0; wv < kBlock / 64; ++wv) s += red[threadIdx.x][wv];
-        if (s != 0.0) atomic_add(&out[threadIdx.x], s);
+        for (int wv = 0; wv < kBlock / 64; ++wv) {
+            double const r = red[q][wv];
+            s = mx ? (s < r ? r : s) : s + r;
+        }
+        if (mx) atomic_max_nonneg(&out[q], s);
+        else if (s != 0.0) atomic_add(&out[q], s);
+        mine = s;
     }
+    return mine;
 }
 
 // The workgroup's net-chord deposits.  When the chords of the workgroup's
@@ -1729,11 +2107,11 @@ __device__ void source_cell_add(Env const& E, SourceCell const& c)
 // none) into the material-source cells, each window flushed with one device
 // atomic per touched face or cell.  A window that is too large, or whose
 // lanes span several batched instances, falls back to per-lane atomics.
-__device__ void deposit_block(StepArgs const& A, bool dep, int32_t instance, double q, double dt, double r0,
-                              double z0, double r1, double z1, SourceCell const& sc)
+// jr_w, jz_w: the kernel's LDS window (kWindowFaces doubles each), which it
+// may use for other per-lane data as long as that is read before this call.
+__device__ __forceinline__ void deposit_block(StepArgs const& A, double* jr_w, double* jz_w, bool dep, int32_t instance, double q,
+                              double dt, double r0, double z0, double r1, double z1, SourceCell const& sc)
 {
-    __shared__ double jr_w[kWindowFaces];
-    __shared__ double jz_w[kWindowFaces];
     __shared__ int32_t win[6];
     auto reduce_window = [&](int32_t ilo, int32_t ihi, int32_t jlo, int32_t jhi, int32_t klo, int32_t khi) {
         __syncthreads();  // the previous window's readers are done
@@ -1861,14 +2239,47 @@ __global__ void __launch_bounds__(kBlock) charged_slow_kernel(const StepArgs* __
                                                             const int32_t* __restrict__ count)
 {
     StepArgs const& A = *args;
+    __shared__ double ledger[RREA_LEDGER_COUNT];
+    ledger_begin(ledger);
     double tally[kTallyUsed];
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
     int64_t const n = *count;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kBlock) {
-        charged_full_particle<kPos>(A, list[i], tally);
+        charged_full_particle<kPos>(A, list[i], tally, ledger);
     }
     reduce_tallies(tally, A.tallies);
+    ledger_flush(ledger, A.tallies);
+}
+
+// The workgroup fold of a fast path's per-lane values, held in LDS columns
+// (value q of lane t at col[q * kBlock + t], kQ <= 16 columns): the first
+// kSums columns are sums, the rest running maxima, and emit(q, total) books
+// column q's workgroup total in one thread.  Sixteen lanes reduce a column,
+// sixteen entries each (consecutive lanes read consecutive entries), then
+// combine within their group of sixteen.  The whole workgroup calls this once
+// its columns are written.
+template <int kQ, int kSums, class Emit>
+__device__ __forceinline__ void fold_columns(double const* col, Emit&& emit)
+{
+    static_assert(kQ <= kBlock / 16, "one group of sixteen lanes per column");
+    __syncthreads();
+    int const q = threadIdx.x >> 4;
+    int const part = threadIdx.x & 15;
+    bool const mx = q >= kSums;
+    double s = 0.0;
+    if (q < kQ) {
+        double const* c = col + q * kBlock + part;
+        for (int i = 0; i < 16; ++i) {
+            double const v = c[i * 16];
+            s = mx ? (s < v ? v : s) : s + v;
+        }
+    }
+    for (int off = 8; off > 0; off >>= 1) {
+        double const o = __shfl_xor(s, off, 64);
+        s = mx ? (s < o ? o : s) : s + o;
+    }
+    if (q < kQ && part == 0) emit(q, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1909,8 +2320,6 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
 {
     using S = Species<kPos>;
     StepArgs const& A = *args;
-    double tally[kTallyUsed];
-    for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     bool slow = false;
     // the survivor's net chord, deposited by the whole workgroup at the end
@@ -1918,6 +2327,33 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
     int32_t dep_inst = -1;
     double dep_q = 0.0, dep_r0 = 0.0, dep_z0 = 0.0, dep_r1 = 0.0, dep_z1 = 0.0;
     SourceCell src{0.0, 0.0, 0.0, 0, 0};  // its ion-pair source record
+    // The LDS window of the deposits at the end; until then it holds each
+    // lane's tallies and ledger records in columns of its own (column q of
+    // the lane at window[q * kBlock + lane]), folded per workgroup at the end:
+    //   sums -- the tallies of a committed step (survivor, substep, drag, ion
+    //     pairs, field work), and its soft radiative loss, collision loss,
+    //     mean hard-Moller transfer and expected hard-Moller weight, written as
+    //     they are formed and cleared for a lane handed over;
+    //   maxima -- the guard's per-substep taus and the transported energy of a
+    //     lane whose drivers all asked for one subcycle (a running maximum
+    //     takes the same value again if the complete path later steps the
+    //     particle, which it sizes the same way, so recording them before the
+    //     commit is exact).
+    __shared__ double window[2 * kWindowFaces];
+    enum {
+        kColSurvivor, kColSubstep, kColDrag, kColIon, kColFieldWork, kColSoft,
+        kColCol = kColSoft + 1, kColHard = kColSoft + 2, kColHmx = kColSoft + 3,  // electrons only
+        kColSums = kPos ? kColSoft + 1 : kColSoft + 4,
+        kColTauA = kColSums, kColTauB = kColSums + 1, kColTauTot = kColSums + 2,  // positrons: A only
+        kColEnergy = kPos ? kColSums + 1 : kColSums + 3,
+        kCols = kColEnergy + 1,
+        // not folded: the hazard's total tau, with its hard-Moller tau in
+        // kColHmx until the epilogue forms the expected weight (electrons)
+        kColHazardTotal = kCols,
+    };
+    static_assert((kCols + 1) * kBlock <= 2 * kWindowFaces, "the columns exceed the deposit window");
+    double* const lane_col = window + threadIdx.x;
+    for (int q = 0; q < kCols + 1; ++q) lane_col[q * kBlock] = 0.0;
     // a batched wave takes the fast path only when all its particles share one
     // instance, whose environment is then wave-uniform; a wave that straddles
     // an instance boundary goes to the complete path
@@ -1980,6 +2416,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
             double const ddp = chord_dp / span;
             double const target = cfg.max_optical_depth_per_substep;
             double tau0, total0;
+            double g_a = 0.0, g_b = 0.0;
             if constexpr (kPos) {
                 double bh0, br0, an0;
                 positron_stp_rates(T, b0, thr, gcut, k, bh0, br0, an0, err);
@@ -1990,9 +2427,22 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
                 electron_stp_rates(T, b0, thr, gcut, k, hm_on, hm0, br0, err);
                 total0 = hm0 + br0;
                 tau0 = hm0 * ddp + br0 * ddp;
+                g_a = hm0;
+                g_b = br0;
             }
             if (dmax(1.0, ceil(tau0 / target)) != 1.0) break;
-            if (!(total0 * max_uniform_slice(iv, path, 1) <= target + 64.0 * kEps * dmax(1.0, target))) break;
+            double const mus = max_uniform_slice(iv, path, 1);
+            if (!(total0 * mus <= target + 64.0 * kEps * dmax(1.0, target))) break;
+            // the guard's taus for one subcycle; they count only if every driver
+            // below asks for one subcycle too, which the energy column marks
+            if constexpr (kPos) {
+                lane_col[kColTauA * kBlock] = total0 * mus / span;
+            } else {
+                double const slice = mus / span;
+                lane_col[kColTauA * kBlock] = g_a * slice;
+                lane_col[kColTauB * kBlock] = g_b * slice;
+                lane_col[kColTauTot * kBlock] = (g_a + g_b) * slice;
+            }
             EBracket eb0;
             if (!elastic_bracket(EL, k, eb0)) break;
             double const mfp0 = elastic_value(EL, EL.mfp, eb0) / 1.0;
@@ -2005,6 +2455,8 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
             if (dmax(1.0, ceil(dmin(gs_n_total, cfg.gs_total_collision_cap) / dmax(cfg.max_gs_collisions_per_substep, 1.0))) != 1.0) break;
             if (dmax(1.0, ceil((fabs(field_work) + drag_est) / impulse_scale)) != 1.0) break;
             if (err) break;
+            // every driver asked for one subcycle: the guard's record of this step
+            lane_col[kColEnergy * kBlock] = k;
             // the single substep: the driver chord again (sub_dt = dt / 1)
             double const se = path.end_f;
             double const substep_dt = dt * (se - 0.0);
@@ -2019,19 +2471,26 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
                                                 ^ (static_cast<uint64_t>(static_cast<uint32_t>(A.cpu ? A.cpu[ip] : 0)) << 40),
                                             static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0));
             Hazard h;
+            double hm_inv1 = 0.0;
             if constexpr (kPos) {
                 double s0, s1, s2;
                 positron_stp_rates(T, b1, thr, gcut, k1, s0, s1, s2, err);
                 h = select_hazard3(s0 * L, s1 * L, s2 * L, rng_uniform(rng, 0, S::kDepth0), rng_uniform(rng, 0, S::kPick0));
             } else {
                 double s0, s1;
-                electron_stp_rates(T, b1, thr, gcut, k1, hm_on, s0, s1, err);
-                h = select_hazard(s0 * L, s1 * L, rng_uniform(rng, 0, S::kDepth0), rng_uniform(rng, 0, S::kPick0));
+                electron_stp_rates(T, b1, thr, gcut, k1, hm_on, s0, s1, err, &hm_inv1);
+                double const hard_tau = s0 * L, brems_tau = s1 * L;
+                h = select_hazard(hard_tau, brems_tau, rng_uniform(rng, 0, S::kDepth0), rng_uniform(rng, 0, S::kPick0));
+                lane_col[kColHmx * kBlock] = hard_tau;
+                lane_col[kColHazardTotal * kBlock] = hard_tau + brems_tau;
             }
             if (!h.valid || h.event || err) break;
             // continuous drag over [0, se] at the frozen energy k1, not reaching the cutoff
             double const col = charged_value(T, S::kCol, b1, err) * 1.0;
             double const soft = charged_value(T, S::kSoft, b1, err) * 1.0;
+            // the plan's mean hard-Moller transfer rate at the frozen energy (its
+            // channel gate is the rate's own, so the inverse length is at hand)
+            double const hstp = kPos ? 0.0 : hard_moller_loss_at(T, b1, hm_inv1);
             double const total = col + soft;
             double const to_cut = k1 - cutoff;
             double const req = total * L;
@@ -2041,6 +2500,11 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
             double const avail = dmax(0.0, k1 - cutoff);
             double const drag = dmin(raw_col + raw_soft, avail);
             double const col_loss = dmin(raw_col, drag);
+            lane_col[kColSoft * kBlock] = w * dmax(0.0, drag - col_loss);
+            if constexpr (!kPos) {
+                lane_col[kColCol * kBlock] = w * col_loss;
+                lane_col[kColHard * kBlock] = w * dmin(hstp * L, avail);
+            }
             double ion = 0.0;
             if (col_loss > 0.0 && raw_col > 0.0) {
                 double const le = dmin(se, iv.end);
@@ -2106,28 +2570,98 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
             if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SURVIVE);
             if (A.ion_pairs) A.ion_pairs[ip] = ion;
             if (A.substeps) A.substeps[ip] = 1;
-            tally[0] += 1.0;
-            tally[6] += 1.0;
-            tally[7] += w * drag;
-            tally[8] += ion;
-            tally[9] += w * hw1;
-            tally[9] += w * hw2;
+            lane_col[kColSurvivor * kBlock] = 1.0;
+            lane_col[kColSubstep * kBlock] = 1.0;
+            lane_col[kColDrag * kBlock] = w * drag;
+            lane_col[kColIon * kBlock] = ion;
+            lane_col[kColFieldWork * kBlock] = w * hw1 + w * hw2;
         } while (false);
     }
     append_slow(slow, ip, slow_list, slow_count);
-    reduce_tallies(tally, A.tallies);
-    deposit_block(A, dep, dep_inst, dep_q, A.cfg.dt_s, dep_r0, dep_z0, dep_r1, dep_z1, src);
+    if (!(lane_col[kColEnergy * kBlock] > 0.0)) {
+        // a lane stopped by a substep driver records no guard taus
+        for (int q = kColSums; q < kColEnergy; ++q) lane_col[q * kBlock] = 0.0;
+    }
+    if (slow) {
+        // a lane handed over counts none of the sums it formed
+        for (int q = kColSoft; q < kColSums; ++q) lane_col[q * kBlock] = 0.0;
+    } else if constexpr (!kPos) {
+        if (ip < A.n) {
+            // the expected hard-Moller weight of the committed step's one hazard
+            // draw, w P(total) hard / total (OV/RreaElectronTransport.cpp:630-646)
+            double const hard_tau = lane_col[kColHmx * kBlock];
+            double const total_tau = lane_col[kColHazardTotal * kBlock];
+            lane_col[kColHmx * kBlock] = A.w[ip] * (total_tau > 0.0
+                ? probability_from_optical_depth(total_tau) * hard_tau / total_tau : 0.0);
+        }
+    }
+    {
+        // the tallies and the committed steps' ledger records
+        // (OV/RreaElectronTransport.cpp:250-400, OV/RreaPositronTransport.cpp:241-360, 776)
+        double* const led = A.tallies + RREA_LEDGER_OFFSET;
+        fold_columns<kCols, kColSums>(window, [&](int q, double v) {
+            if (q < kColSums) {
+                if (v == 0.0) return;
+                switch (q) {
+                case kColSurvivor: atomic_add(&A.tallies[0], v); break;
+                case kColSubstep: atomic_add(&A.tallies[6], v); break;
+                case kColDrag:
+                    atomic_add(&A.tallies[7], v);
+                    atomic_add(led + (kPos ? RREA_L_POSITRON_TRANSPORT_ENERGY_LOSS_EV : RREA_L_PARENT_DRAG_LOSS_EV), v);
+                    break;
+                case kColIon:
+                    // every ion-pair piece is a low-energy ionization record of diagnostic loss W_air
+                    atomic_add(&A.tallies[8], v);
+                    atomic_add(led + RREA_L_TRANSPORT_IONIZATION_WEIGHT, v);
+                    atomic_add(led + RREA_L_TRANSPORT_LOW_SECONDARY_WEIGHT, v);
+                    atomic_add(led + RREA_L_TRANSPORT_ENERGY_LOSS_EV, v * kWAir);
+                    break;
+                case kColFieldWork: atomic_add(&A.tallies[9], v); break;
+                case kColSoft: atomic_add(led + RREA_L_PHOTON_CONTINUOUS_SOFT_BREMS_ENERGY_EV, v); break;
+                default:
+                    if constexpr (!kPos) {
+                        if (q == kColCol) {
+                            atomic_add(led + RREA_L_ION_PAIR_SOURCE_LOSS_EV, v);
+                            atomic_add(led + RREA_L_RESTRICTED_COLLISION_LOSS_EV, v);
+                        } else if (q == kColHard) {
+                            atomic_add(led + RREA_L_EXPLICIT_HARD_MOLLER_ENERGY_LOSS_EV, v);
+                        } else if (q == kColHmx) {
+                            atomic_add(led + RREA_L_HARD_MOLLER_EXPECTED_EVENT_WEIGHT, v);
+                        }
+                    }
+                    break;
+                }
+                return;
+            }
+            if (q == kColEnergy) {
+                atomic_max_nonneg(led + (kPos ? RREA_L_MAX_TRANSPORT_POSITRON_ENERGY_EV
+                                              : RREA_L_MAX_TRANSPORT_ELECTRON_ENERGY_EV), v);
+                if (v > 0.0) atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_SUBCYCLES, 1.0);
+            } else if constexpr (kPos) {
+                atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_POSITRON_TAU, v);
+            } else {
+                atomic_max_nonneg(led + (q == kColTauA ? RREA_L_MAX_TRANSPORT_HARD_MOLLER_TAU
+                                         : q == kColTauB ? RREA_L_MAX_TRANSPORT_BREMS_TAU
+                                                         : RREA_L_MAX_TRANSPORT_ELECTRON_TOTAL_COMPETING_TAU), v);
+            }
+        });
+    }
+    deposit_block(A, window, window + kWindowFaces, dep, dep_inst, dep_q, A.cfg.dt_s, dep_r0, dep_z0, dep_r1, dep_z1,
+                  src);
 }
 
 // Every particle through the complete path (used when the fast path is off).
 __global__ void __launch_bounds__(kBlock) electron_full_kernel(const StepArgs* __restrict__ args)
 {
     StepArgs const& A = *args;
+    __shared__ double ledger[RREA_LEDGER_COUNT];
+    ledger_begin(ledger);
     double tally[kTallyUsed];
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (ip < A.n) charged_full_particle<false>(A, ip, tally);
+    if (ip < A.n) charged_full_particle<false>(A, ip, tally, ledger);
     reduce_tallies(tally, A.tallies);
+    ledger_flush(ledger, A.tallies);
 }
 
 // The straight-line photon push at c along u / |u| that WarpX applies before
@@ -2159,11 +2693,14 @@ __global__ void photon_push_kernel(int64_t n, const int64_t* __restrict__ id, do
 __global__ void __launch_bounds__(kBlock) photon_kernel(const StepArgs* __restrict__ args)
 {
     StepArgs const& A = *args;
+    __shared__ double ledger[RREA_LEDGER_COUNT];
+    ledger_begin(ledger);
     double tally[kTallyUsed];
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (ip < A.n) photon_particle(A, ip, tally);
+    if (ip < A.n) photon_particle(A, ip, tally, ledger);
     reduce_tallies(tally, A.tallies);
+    ledger_flush(ledger, A.tallies);
 }
 
 // The photon fast path.  At small dt nearly every photon is valid, weighted,
@@ -2179,7 +2716,13 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
                                                            int32_t* __restrict__ slow_count)
 {
     StepArgs const& A = *args;
-    double tally[2] = {0.0, 0.0};  // survivors, escapes
+    // each lane's tallies and ledger records in LDS columns of its own, folded
+    // per workgroup at the end: survivors, escapes, escaped energy (sums); the
+    // pass's tau and energy (maxima) -- OV/RreaPhotonTransport.cpp:192, 222-233
+    enum { kColSurvivor, kColEscape, kColEscapedEnergy, kColTau, kColEnergy, kCols };
+    __shared__ double cols[kCols * kBlock];
+    double* const lane_col = cols + threadIdx.x;
+    for (int q = 0; q < kCols; ++q) lane_col[q * kBlock] = 0.0;
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     bool slow = false;
     // a batched wave takes the fast path only when all its particles share one
@@ -2223,6 +2766,7 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
             Interval const iv = build_interval(E.F, path, D, E.density_ratio, err);
             double const chord_dp = density_path(iv, path, 0.0, path.end_f);
             if (err) break;
+            double pass_tau = 0.0;
             if (path.in_len > 0.0) {
                 PhotonRates rates;
                 if (!photon_rates(T, energy, rates, err) || err) break;
@@ -2235,7 +2779,8 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
                 uint64_t const step = static_cast<uint64_t>(cfg.step > 0 ? cfg.step : 0);
                 // the first pass's live index is segment 0, event 0: index 0
                 double const draw = rng_uniform(rng_prefix(E.rng_seed, pkey, step), 0U, kChPhotonInteraction);
-                if (select_hazard(chord_dp / mfp, 0.0, draw, 0.0).event) break;
+                pass_tau = chord_dp / mfp;
+                if (select_hazard(pass_tau, 0.0, draw, 0.0).event) break;
             }
             // ---- commit: nothing above has written anything ----
             slow = false;
@@ -2246,9 +2791,14 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
             A.z[ip] = z;
             if (escapes) {
                 A.id[ip] = RREA_INVALID_ID;
-                tally[1] += 1.0;
+                lane_col[kColEscape * kBlock] = 1.0;
+                lane_col[kColEscapedEnergy * kBlock] = w * energy;
             } else {
-                tally[0] += 1.0;
+                lane_col[kColSurvivor * kBlock] = 1.0;
+            }
+            if (path.in_len > 0.0) {
+                lane_col[kColTau * kBlock] = pass_tau;
+                lane_col[kColEnergy * kBlock] = energy;
             }
             if (A.status) A.status[ip] = static_cast<uint8_t>(escapes ? RREA_ESCAPE : RREA_SURVIVE);
             if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
@@ -2256,7 +2806,19 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
         } while (false);
     }
     append_slow(slow, ip, slow_list, slow_count);
-    reduce_tallies<2>(tally, A.tallies);
+    {
+        double* const led = A.tallies + RREA_LEDGER_OFFSET;
+        fold_columns<kCols, kColTau>(cols, [&](int q, double v) {
+            if (q == kColSurvivor && v != 0.0) atomic_add(&A.tallies[0], v);
+            if (q == kColEscape && v != 0.0) atomic_add(&A.tallies[1], v);
+            if (q == kColEscapedEnergy && v != 0.0) atomic_add(led + RREA_L_PHOTON_ESCAPED_ENERGY_EV, v);
+            if (q == kColTau) atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_PHOTON_TAU, v);
+            if (q == kColEnergy) {
+                atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_PHOTON_ENERGY_SEEN_EV, v);
+                if (v > 0.0) atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_SUBCYCLES, 1.0);
+            }
+        });
+    }
 }
 
 // The photons the fast path handed over, strided over a fixed grid.
@@ -2265,25 +2827,31 @@ __global__ void __launch_bounds__(kBlock) photon_slow_kernel(const StepArgs* __r
                                                            const int32_t* __restrict__ count)
 {
     StepArgs const& A = *args;
+    __shared__ double ledger[RREA_LEDGER_COUNT];
+    ledger_begin(ledger);
     double tally[kTallyUsed];
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
     int64_t const n = *count;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * kBlock) {
-        photon_particle(A, list[i], tally);
+        photon_particle(A, list[i], tally, ledger);
     }
     reduce_tallies(tally, A.tallies);
+    ledger_flush(ledger, A.tallies);
 }
 
 // Every positron through the complete path (used when the fast path is off).
 __global__ void __launch_bounds__(kBlock) positron_full_kernel(const StepArgs* __restrict__ args)
 {
     StepArgs const& A = *args;
+    __shared__ double ledger[RREA_LEDGER_COUNT];
+    ledger_begin(ledger);
     double tally[kTallyUsed];
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (ip < A.n) charged_full_particle<true>(A, ip, tally);
+    if (ip < A.n) charged_full_particle<true>(A, ip, tally, ledger);
     reduce_tallies(tally, A.tallies);
+    ledger_flush(ledger, A.tallies);
 }
 
 // ---------------------------------------------------------------------------
@@ -2388,7 +2956,7 @@ __device__ bool newborn_electron_demotes(StepArgs const& A, Env const& E, double
 
 // TransportStep::AdvanceChargedNewborn OV/RreaNewbornAdvance.cpp:50-777
 template <bool kPos>
-__device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, double* tally)
+__device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, double* tally, double* ledger)
 {
     using S = Species<kPos>;
     StepArgs const& A = N.S;
@@ -2420,7 +2988,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
     uint64_t const lin = p.lineage;
     double const w = p.weight;
     uint64_t const rng = rng_prefix(E.rng_seed, p.root_key, step);
-    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1, E.instance, &E};
+    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1, E.instance, &E, ledger};
     double const birth_r = hypot(p.x, p.y), birth_z = p.z;
     double const residual = newborn_residual(p.birth_fraction);
     double k = p.energy_eV;
@@ -2445,6 +3013,8 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
         if constexpr (kPos) {
             spawn_annihilation(T, gcut, rng, em, kProcAnnihilation, x, y, z, dx, dy, dz, k, true, index, elapsed,
                                tally);
+            // OV/RreaNewbornAdvance.cpp:123, 729
+            Ledger{ledger, cutoff}.positron_transport(w, k, true);
             status = RREA_ANNIHILATED;
         } else {
             tally[16] += w;
@@ -2522,6 +3092,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                 auto in_range = [&]() {
                     if (!(finite(k) && k >= T.charged_min && k <= T.charged_max) && !err) err = kErrRange;
                 };
+                double positron_total_loss = 0.0;
                 // RreaPlanContinuousDragSegment OV/RreaChargedInterleave.H:239-292
                 auto continuous = [&](double b, double e, double stop_e, double& eff) -> bool {
                     eff = b;
@@ -2566,10 +3137,24 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                                 double mx, my, mz;
                                 point_at_full(path, 0.5 * (lb + le), mx, my, mz);
                                 em.ionization(hypot(mx, my), mz, pw, 0.5 * cutoff, kWAir);
+                                Ledger{ledger, cutoff}.transport_ionization(pw, kWAir, true, false);
                             }
                         }
                     }
                     tally[7] += w * drag;
+                    {
+                        // the plan's partition: soft radiative share, and the mean hard-Moller
+                        // transfer over the same density path (electrons; OV/RreaElectronTransport.cpp:306-338)
+                        Ledger led{ledger, cutoff};
+                        led.soft_brems(w, dmax(0.0, drag - col_loss));
+                        if constexpr (kPos) {
+                            positron_total_loss += drag;
+                        } else {
+                            double const hstp = (hm_on && thr > 0.0 && stop_e > 2.0 * thr)
+                                ? hard_moller_energy_loss(T, bs, stop_e, err) : 0.0;
+                            led.loss_partition(w, drag, col_loss, dmin(hstp * pm, avail));
+                        }
+                    }
                     k = dmax(0.0, k - drag);
                     if (reaches) k = cutoff;
                     return reaches;
@@ -2619,6 +3204,14 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                             else electron_stp_rates(T, bf, thr, gcut, frozen_e, hm_on, s0, s1, err);
                             double const total_stp = s0 + s1 + s2;
                             double const L = density_path(iv, path, rem, se);
+                            if constexpr (!kPos) {
+                                // OV/RreaNewbornAdvance.cpp:536-548
+                                double const hard_tau = s0 * L, brems_tau = s1 * L;
+                                double const total_tau = hard_tau + brems_tau;
+                                Ledger{ledger, cutoff}.hm_expectation(
+                                    w * (total_tau > 0.0 ? probability_from_optical_depth(total_tau) * hard_tau / total_tau
+                                                         : 0.0));
+                            }
                             uint64_t const idx = newborn_index(rsp, lin, sc, ord, 0U, err);
                             if (err) break;
                             double const u1 = rng_uniform(rng, idx, kChNewbornCompetingOpticalDepth);
@@ -2682,6 +3275,11 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                                         em.positive_ion(hypot(ex_, ey_), ez_, w, T2);
                                         tally[12] += w;
                                         tally[11] += 1.0;
+                                        {
+                                            Ledger led{ledger, cutoff};
+                                            led.transport_ionization(w, T2, false, true);
+                                            led.bhabha(w, T2, true);
+                                        }
                                         GroupClosure c;
                                         c.parent(1, w, before, k, true);
                                         c.particle(0, w, T2);
@@ -2701,6 +3299,11 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                                     ion += w;
                                     em.ionization(hypot(ex_, ey_), ez_, w, T2, T2);
                                     tally[3] += 1.0;
+                                    {
+                                        Ledger led{ledger, cutoff};
+                                        led.hm_event(w, T2);
+                                        led.transport_ionization(w, T2, false, true);
+                                    }
                                     GroupClosure c;
                                     c.parent(0, w, before, k, true);
                                     c.particle(0, w, T2);
@@ -2735,6 +3338,8 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                                     }
                                 }
                                 k = after;
+                                if constexpr (kPos) Ledger{ledger, cutoff}.positron_brems(w, pe);
+                                else Ledger{ledger, cutoff}.brems_photon(w, pe);
                                 if (tracked) {
                                     em.secondary(2, kProcBrems, ex_, ey_, ez_, gx, gy, gz, pe, efrac);
                                     tally[5] += 1.0;
@@ -2818,6 +3423,10 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                     if (path.exits) { exited = true; break; }
                     completed = sc + 1U;
                 }
+                // OV/RreaNewbornAdvance.cpp:686-689
+                if constexpr (kPos) {
+                    if (positron_total_loss > 0.0) Ledger{ledger, cutoff}.positron_transport(w, positron_total_loss, false);
+                }
                 if (!err) {
                     if (terminal_interaction) {
                         status = RREA_ANNIHILATED;
@@ -2865,7 +3474,7 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
 }
 
 // TransportStep::AdvancePhotonNewborn OV/RreaNewbornAdvance.cpp:779-994
-__device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, double* tally)
+__device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, double* tally, double* ledger)
 {
     StepArgs const& A = N.S;
     DevTables const& T = A.T;
@@ -2887,7 +3496,7 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
     uint64_t const rsp = static_cast<uint64_t>(p.root_species);
     double const w = p.weight;
     uint64_t const rng = rng_prefix(E.rng_seed, p.root_key, step);
-    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1, E.instance, &E};
+    Emitter em{A, i, p.root_key, w, ion, lineage, err, p.root_species, p.generation + 1, E.instance, &E, ledger};
     double const residual = newborn_residual(p.birth_fraction);
     double dx = p.dir_x, dy = p.dir_y, dz = p.dir_z;
     normalize(dx, dy, dz);
@@ -2895,6 +3504,7 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
     auto escape = [&]() {
         status = RREA_ESCAPE;
         tally[19] += 1.0;
+        Ledger{ledger, 0.0}.photon_escape(w, energy);
     };
     auto survive_at = [&](double x, double y, double z) {
         p.x = x; p.y = y; p.z = z;
@@ -2996,6 +3606,8 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
 __global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __restrict__ args)
 {
     NewbornArgs const& N = *args;
+    __shared__ double ledger[RREA_LEDGER_COUNT];
+    ledger_begin(ledger);
     double tally[kTallyNewborn];
     for (int q = 0; q < kTallyNewborn; ++q) tally[q] = 0.0;
     // an error recorded before the launch (the step's, or a rejected birth
@@ -3015,9 +3627,9 @@ __global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __re
             report(N.S.error, kErrRange, i);
         } else if (p.birth_fraction < 1.0) {
             tally[17] += 1.0;
-            if (photon) photon_newborn(N, i, tally);
-            else if (p.species == 1) charged_newborn<true>(N, i, tally);
-            else charged_newborn<false>(N, i, tally);
+            if (photon) photon_newborn(N, i, tally, ledger);
+            else if (p.species == 1) charged_newborn<true>(N, i, tally, ledger);
+            else charged_newborn<false>(N, i, tally, ledger);
         } else {
             N.fin[i] = p;
             N.status[i] = static_cast<uint8_t>(RREA_SURVIVE);
@@ -3026,6 +3638,7 @@ __global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __re
         }
     }
     reduce_tallies<kTallyNewborn>(tally, N.S.tallies);
+    ledger_flush(ledger, N.S.tallies);
 }
 
 // The FIFO's birth-payload validation (OV/RreaSecondaryCommit.cpp:338-376),

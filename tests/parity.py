@@ -136,9 +136,27 @@ def compare_step(fields, tile_host_prev, oracle_out, tile_dev, out, label="", dt
         scale = max(np.abs(o).max(), 1e-300)
         tol = 1e-11 * scale + (64.0 * EPS * coord * wmax / dt if k != "rho" else 64.0 * EPS * coord * wmax)
         assert np.abs(g - o).max() <= tol, f"{label} {k} differs: {np.abs(g - o).max()} of {scale} (tol {tol})"
-    t_g = out.tallies.cpu().numpy()[:17]
-    t_o = oracle_out["tallies"][:17]
-    assert np.allclose(t_g, t_o, rtol=1e-10, atol=1e-300), f"{label} tallies {t_g} vs {t_o}"
+    t_g = out.tallies.cpu().numpy()
+    t_o = oracle_out["tallies"]
+    assert np.allclose(t_g[:17], t_o[:17], rtol=1e-10, atol=1e-300), f"{label} tallies {t_g[:17]} vs {t_o[:17]}"
+    compare_ledger(t_g, t_o, label, kappa=float(np.max(kappa, initial=1.0)))
+
+
+def compare_ledger(t_g, t_o, label, rtol=1e-10, kappa=1.0):
+    """The event ledger (every RreaEventLedger field): counts exactly; sums
+    and running maxima to rtol (the device adds in another order) plus the
+    worst particle's round-trip conditioning kappa, as they are sums and
+    maxima of per-particle values (path lengths, taus, losses) that agree to
+    the state tolerance, 1e-12 + 16 eps kappa."""
+    from oracle import binding as ob
+    bad = []
+    tol = rtol + 16.0 * EPS * kappa
+    for i, n in enumerate(ob.LEDGER_NAMES):
+        g, o = float(t_g[ob.LEDGER_OFFSET + i]), float(t_o[ob.LEDGER_OFFSET + i])
+        ok = g == o if n.endswith("_count") else abs(g - o) <= tol * max(abs(g), abs(o)) + 1e-300
+        if not ok:
+            bad.append((n, g, o))
+    assert not bad, f"{label} event ledger differs: {bad}"
 
 
 def host_tile_from_dev(tile_dev):
@@ -223,9 +241,10 @@ def compare_photon_step(fields, p_in, oracle_out, tile_dev, out, label="", dt=No
         tol = (REL + 16.0 * EPS * kp) * np.maximum(np.maximum(np.abs(sg[f]), np.abs(so[f])), 1.0)
         bad = np.flatnonzero(np.abs(sg[f] - so[f]) > tol)
         assert bad.size == 0, f"{label} secondary {f} differs at {bad[:5]}: {sg[f][bad[:5]]} vs {so[f][bad[:5]]}"
-    t_g = out.tallies.cpu().numpy()[:17]
-    t_o = o["tallies"][:17]
-    assert np.allclose(t_g, t_o, rtol=1e-10, atol=1e-300), f"{label} tallies {t_g} vs {t_o}"
+    t_g = out.tallies.cpu().numpy()
+    t_o = o["tallies"]
+    assert np.allclose(t_g[:17], t_o[:17], rtol=1e-10, atol=1e-300), f"{label} tallies {t_g[:17]} vs {t_o[:17]}"
+    compare_ledger(t_g, t_o, label, kappa=float(np.max(kappa, initial=1.0)))
 
 
 def run_photons_and_compare(otables, tables, coupling, fields, tile_host, cfg, steps=1, label="", nthreads=8):
@@ -357,6 +376,7 @@ def compare_drain(fields, cfg, oracle_out, out, label="", stats=None):
     t_o = oracle_out["tallies"]
     assert np.array_equal(t_g[17:21], t_o[17:21]), f"{label} newborn counts {t_g[17:21]} vs {t_o[17:21]}"
     assert np.allclose(t_g[:17], t_o[:17], rtol=1e-9, atol=1e-300), f"{label} tallies {t_g[:17]} vs {t_o[:17]}"
+    compare_ledger(t_g, t_o, label, rtol=1e-9, kappa=float(np.max(kappa, initial=1.0)))
     # a newborn's chord ends where its transport ended, which the comparison
     # above allows to differ by (1e-12 + 64 eps) of the coordinates; a face
     # flux q/dt (x1 - x0)/dx inherits that in index units

@@ -84,3 +84,27 @@ def test_argument_checks_need_no_device():
     assert lib.rrea_fields_create(C.byref(bad), C.byref(h)) == 16
     assert lib.rrea_transport_step_electrons(None, None, None, None, None, None) == 16
     assert lib.rrea_transport_step_noalloc(0, None, None, None, None, None, None) == 16
+
+
+def test_event_ledger_layout_matches_the_oracle():
+    """The header's ledger slots are the reference's RreaEventLedger fields in
+    order (the oracle's list), the tally array holds the counts and then the
+    ledger, and exactly the ten running maxima fold by max."""
+    from oracle import binding as ob
+    text = open(HEADER).read()
+    blk = text[text.index("RREA_L_PHOTON_INTERACTION_COUNT = 0"):text.index("    RREA_LEDGER_COUNT ")]
+    names = [n.lower() for n in re.findall(r"RREA_L_([A-Z0-9_]+)", blk)]
+    assert names == ob.LEDGER_NAMES
+    assert _lib.TALLY_COUNT == ob.N_TALLIES == _lib.LEDGER_OFFSET + len(names)
+    src = ('#include <stdio.h>\n#include "rrea_hip.h"\n'
+           'int main(void) { printf("%d %d\\n", (int)RREA_TALLY_COUNT, (int)RREA_LEDGER_OFFSET);'
+           ' for (int q = 0; q < RREA_LEDGER_COUNT; ++q) printf("%d", RREA_LEDGER_IS_MAX(q) ? 1 : 0);'
+           ' printf("\\n"); return 0; }\n')
+    with tempfile.TemporaryDirectory() as d:
+        c, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+        open(c, "w").write(src)
+        subprocess.check_call(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), "-o", exe, c])
+        out = subprocess.check_output([exe], text=True).split()
+    assert int(out[0]) == _lib.TALLY_COUNT and int(out[1]) == _lib.LEDGER_OFFSET
+    is_max = {n for n, b in zip(names, out[2]) if b == "1"}
+    assert is_max == ob.LEDGER_MAX and len(is_max) == 10

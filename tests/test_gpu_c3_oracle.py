@@ -15,7 +15,7 @@ import pytest
 torch = pytest.importorskip("torch")
 
 from oracle import binding as ob  # noqa: E402
-from parity import compare_drain, compare_step  # noqa: E402
+from parity import compare_drain, compare_ledger, compare_step  # noqa: E402
 from rrea_feedback_warpx_amrex_geant4_amd import workloads as wl  # noqa: E402
 from rrea_feedback_warpx_amrex_geant4_amd.config import ParticleTile, StepConfig  # noqa: E402
 
@@ -116,19 +116,20 @@ def test_c3_regime_batched_against_oracle(tables, otables, dt):
         torch.cuda.synchronize()
         out.check_error()
         oracle = []
-        t_sum = np.zeros(ob.N_TALLIES)
+        t_sum = np.zeros(ob.N_TALLIES)  # the batch's tallies: the instances' folded
         for i, (f, t, sd) in enumerate(zip(fields, hosts, seeds)):
             c = StepConfig(rng_seed=sd, dt_s=dt, step=step)
             o = ob.electron_step(otables, f, c, t.as_dict(), nthreads=8, sec_cap=1 << 16)
             assert o["code"] == 0
             oracle.append(o)
-            t_sum += o["tallies"]
+            t_sum = ob.fold_tallies(t_sum, o["tallies"])
             a, b = offs[i], offs[i + 1]
             compare_step(f, t, o, Rows(dev, a, b, KEYS), InstanceStepOut(out, grids[i], a, b, i, o["tallies"]),
                          label=f"c3 dt={dt} step {step} instance {i}", dt=dt)
             max_sub = max(max_sub, int(o["substeps"].max(initial=0)))
-        t_g = out.tallies.cpu().numpy()[:17]
-        assert np.allclose(t_g, t_sum[:17], rtol=1e-10, atol=1e-300), (t_g, t_sum[:17])
+        t_g = out.tallies.cpu().numpy()
+        assert np.allclose(t_g[:17], t_sum[:17], rtol=1e-10, atol=1e-300), (t_g[:17], t_sum[:17])
+        compare_ledger(t_g, t_sum, f"c3 dt={dt} step {step} batch")
         # the drain of the step's staged records, each FIFO record in its own instance
         for g in grids:
             g.reset()
@@ -143,12 +144,13 @@ def test_c3_regime_batched_against_oracle(tables, otables, dt):
             c = StepConfig(rng_seed=sd, dt_s=dt, step=step)
             od = ob.newborn_drain(otables, f, c, o["secondaries"], cap=1 << 17)
             assert od["code"] == 0
-            d_sum += od["tallies"]
+            d_sum = ob.fold_tallies(d_sum, od["tallies"])
             entries += len(od["work"])
             compare_drain(f, c, od, InstanceDrainOut(drain, grids[i], i, offs[i], od["tallies"]),
                           label=f"c3 drain dt={dt} step {step} instance {i}")
         t_g = dgrid.tallies.cpu().numpy()
         assert np.array_equal(t_g[17:21], d_sum[17:21]) and np.allclose(t_g[:17], d_sum[:17], rtol=1e-9, atol=1e-300)
+        compare_ledger(t_g, d_sum, f"c3 drain dt={dt} step {step} batch", rtol=1e-9)
         print(f"c3 dt={dt} step {step}: {len(host.r)} electrons, {len(out.secondaries_host())} staged, "
               f"{entries} FIFO entries -- every instance equal to its oracle run", flush=True)
         # both continue from the oracle's survivors
