@@ -30,6 +30,26 @@ FLAGS = [
 ]
 
 
+# A compiled C++ caller of the C-ABI (tests/cpp/rrea_caller.cpp), linked
+# against the in-tree library by a relative run path.
+CALLER_SRC = os.path.join(ROOT, "tests", "cpp", "rrea_caller.cpp")
+CALLER = os.path.join(ROOT, "tests", "cpp", "rrea_caller")
+
+
+def build_caller(verbose: bool = False) -> str:
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-Wno-unused-command-line-argument",
+           "-I" + os.path.join(ROOT, "include"), "-o", CALLER + ".tmp", CALLER_SRC, "-L" + HERE, "-lrrea_hip",
+           "-Wl,-rpath,$ORIGIN/../../rrea_feedback_warpx_amrex_geant4_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building the C++ caller")
+    os.replace(CALLER + ".tmp", CALLER)
+    return CALLER
+
+
 def build(verbose: bool = False) -> str:
     cmd = [HIPCC, *FLAGS, "-o", LIB + ".tmp", *SOURCES]
     if verbose:
@@ -39,6 +59,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building librrea_hip.so")
     os.replace(LIB + ".tmp", LIB)
+    build_caller(verbose)
     return LIB
 
 
