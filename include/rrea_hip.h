@@ -429,8 +429,8 @@ rrea_status rrea_validate_secondary_groups(const rrea_secondary_group* groups, i
  * rrea_transport_step_positrons (1) or rrea_transport_step_photons (2), with
  * the same arguments, results and error behaviour, except:
  *   - the call allocates nothing: its workspace (argument block, hand-over
- *     list, staging and ordering buffers) is one fixed device array per
- *     device, about 8.5 MiB, held until the process ends;
+ *     list, partial rows, staging and ordering buffers) is one fixed device
+ *     array per device, about 9 MiB, held until the process ends;
  *   - tile->n is at most 1,048,576 and, when ordering, secondary_capacity is
  *     at most 32,768 (RREA_ERR_ARGUMENT beyond either);
  *   - a nonzero order_secondaries always orders with the count on the device

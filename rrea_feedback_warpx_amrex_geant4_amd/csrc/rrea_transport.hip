@@ -196,6 +196,7 @@ struct rrea_fields_s {
     int32_t* d_slow_list = nullptr;  // fast-path hand-over list
     int32_t* d_slow_count = nullptr;
     int64_t slow_cap = 0;
+    double* d_partials = nullptr;    // the fast paths' per-workgroup partial rows (slow_cap's workgroups)
     // tile ordering workspace
     uint32_t* d_tkeys = nullptr;
     uint32_t* d_tkeys_out = nullptr;
@@ -2282,6 +2283,157 @@ __device__ __forceinline__ void fold_columns(double const* col, Emit&& emit)
     if (q < kQ && part == 0) emit(q, s);
 }
 
+// The columns of the fast paths' per-lane values.  Charged: the first
+// kColSums are sums, then the guard taus and the transported energy (maxima);
+// kColHazardTotal is a lane's scratch column, not folded.
+template <bool kPos>
+struct ChargedColumns {
+    enum {
+        kColSurvivor, kColSubstep, kColDrag, kColIon, kColFieldWork, kColSoft,
+        kColCol = kColSoft + 1, kColHard = kColSoft + 2, kColHmx = kColSoft + 3,  // electrons only
+        kColSums = kPos ? kColSoft + 1 : kColSoft + 4,
+        kColTauA = kColSums, kColTauB = kColSums + 1, kColTauTot = kColSums + 2,  // positrons: A only
+        kColEnergy = kPos ? kColSums + 1 : kColSums + 3,
+        kCols = kColEnergy + 1,
+        // the hazard's total tau, with its hard-Moller tau in kColHmx until
+        // the epilogue forms the expected weight (electrons)
+        kColHazardTotal = kCols,
+    };
+};
+struct PhotonColumns {
+    enum { kColSurvivor, kColEscape, kColEscapedEnergy, kColTau, kColEnergy, kCols, kColSums = kColTau };
+};
+
+// Books column q's total v of a charged fast-path fold: the tallies and the
+// committed steps' ledger records (OV/RreaElectronTransport.cpp:250-400,
+// OV/RreaPositronTransport.cpp:241-360, 776).
+template <bool kPos>
+__device__ void book_charged_column(double* tallies, int q, double v)
+{
+    using Col = ChargedColumns<kPos>;
+    double* const led = tallies + RREA_LEDGER_OFFSET;
+    if (q < Col::kColSums) {
+        if (v == 0.0) return;
+        switch (q) {
+        case Col::kColSurvivor: atomic_add(&tallies[0], v); break;
+        case Col::kColSubstep: atomic_add(&tallies[6], v); break;
+        case Col::kColDrag:
+            atomic_add(&tallies[7], v);
+            atomic_add(led + (kPos ? RREA_L_POSITRON_TRANSPORT_ENERGY_LOSS_EV : RREA_L_PARENT_DRAG_LOSS_EV), v);
+            break;
+        case Col::kColIon:
+            // every ion-pair piece is a low-energy ionization record of diagnostic loss W_air
+            atomic_add(&tallies[8], v);
+            atomic_add(led + RREA_L_TRANSPORT_IONIZATION_WEIGHT, v);
+            atomic_add(led + RREA_L_TRANSPORT_LOW_SECONDARY_WEIGHT, v);
+            atomic_add(led + RREA_L_TRANSPORT_ENERGY_LOSS_EV, v * kWAir);
+            break;
+        case Col::kColFieldWork: atomic_add(&tallies[9], v); break;
+        case Col::kColSoft: atomic_add(led + RREA_L_PHOTON_CONTINUOUS_SOFT_BREMS_ENERGY_EV, v); break;
+        default:
+            if constexpr (!kPos) {
+                if (q == Col::kColCol) {
+                    atomic_add(led + RREA_L_ION_PAIR_SOURCE_LOSS_EV, v);
+                    atomic_add(led + RREA_L_RESTRICTED_COLLISION_LOSS_EV, v);
+                } else if (q == Col::kColHard) {
+                    atomic_add(led + RREA_L_EXPLICIT_HARD_MOLLER_ENERGY_LOSS_EV, v);
+                } else if (q == Col::kColHmx) {
+                    atomic_add(led + RREA_L_HARD_MOLLER_EXPECTED_EVENT_WEIGHT, v);
+                }
+            }
+            break;
+        }
+        return;
+    }
+    if (q == Col::kColEnergy) {
+        atomic_max_nonneg(led + (kPos ? RREA_L_MAX_TRANSPORT_POSITRON_ENERGY_EV
+                                      : RREA_L_MAX_TRANSPORT_ELECTRON_ENERGY_EV), v);
+        if (v > 0.0) atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_SUBCYCLES, 1.0);
+    } else if constexpr (kPos) {
+        atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_POSITRON_TAU, v);
+    } else {
+        atomic_max_nonneg(led + (q == Col::kColTauA ? RREA_L_MAX_TRANSPORT_HARD_MOLLER_TAU
+                                 : q == Col::kColTauB ? RREA_L_MAX_TRANSPORT_BREMS_TAU
+                                                 : RREA_L_MAX_TRANSPORT_ELECTRON_TOTAL_COMPETING_TAU), v);
+    }
+}
+
+// Books column q's total v of a photon fast-path fold (OV/RreaPhotonTransport.cpp:192, 222-233).
+__device__ void book_photon_column(double* tallies, int q, double v)
+{
+    using Col = PhotonColumns;
+    double* const led = tallies + RREA_LEDGER_OFFSET;
+    if (q == Col::kColSurvivor && v != 0.0) atomic_add(&tallies[0], v);
+    if (q == Col::kColEscape && v != 0.0) atomic_add(&tallies[1], v);
+    if (q == Col::kColEscapedEnergy && v != 0.0) atomic_add(led + RREA_L_PHOTON_ESCAPED_ENERGY_EV, v);
+    if (q == Col::kColTau) atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_PHOTON_TAU, v);
+    if (q == Col::kColEnergy) {
+        atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_PHOTON_ENERGY_SEEN_EV, v);
+        if (v > 0.0) atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_SUBCYCLES, 1.0);
+    }
+}
+
+// A fast-path workgroup's folded columns go to row blockIdx.x of the
+// partials (one 128-byte line per workgroup, every workgroup of the launch
+// writing its columns), and a fold kernel books the rows into the tallies and
+// ledger: same-address atomics from every workgroup of a launch would
+// serialise on the tally lines, where the rows take plain stores.
+constexpr int kPartialStride = 16;
+static_assert(ChargedColumns<false>::kCols <= kPartialStride && PhotonColumns::kCols <= kPartialStride,
+              "a workgroup's columns exceed its partial row");
+
+// The fold of `rows` partial rows: sixteen rows per workgroup pass over a grid
+// stride (sixteen lanes a row, consecutive lanes consecutive columns), the
+// wave's row groups and then the workgroup's waves combined, and book(q, v)
+// called once per nonzero column total of the workgroup.
+template <class Col, class Book>
+__device__ __forceinline__ void fold_partial_rows(const double* __restrict__ partials, int64_t rows, Book&& book)
+{
+    constexpr int kRowsPerPass = kBlock / kPartialStride;
+    int const q = threadIdx.x & (kPartialStride - 1);
+    bool const mx = q >= Col::kColSums;
+    double s = 0.0;
+    if (q < Col::kCols) {
+        int64_t const stride = static_cast<int64_t>(gridDim.x) * kRowsPerPass;
+        for (int64_t r = static_cast<int64_t>(blockIdx.x) * kRowsPerPass + threadIdx.x / kPartialStride; r < rows;
+             r += stride) {
+            double const v = partials[r * kPartialStride + q];
+            s = mx ? (s < v ? v : s) : s + v;
+        }
+    }
+    for (int off = kPartialStride; off < 64; off <<= 1) {
+        double const o = __shfl_xor(s, off, 64);
+        s = mx ? (s < o ? o : s) : s + o;
+    }
+    __shared__ double wave_total[kBlock / 64][kPartialStride];
+    if ((threadIdx.x & 63) < kPartialStride) wave_total[threadIdx.x >> 6][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (threadIdx.x < Col::kCols) {
+        double t = wave_total[0][threadIdx.x];
+        for (int w = 1; w < kBlock / 64; ++w) {
+            double const o = wave_total[w][threadIdx.x];
+            t = mx ? (t < o ? o : t) : t + o;
+        }
+        if (t != 0.0) book(static_cast<int>(threadIdx.x), t);
+    }
+}
+
+template <bool kPos>
+__global__ void __launch_bounds__(kBlock) charged_fold_kernel(const StepArgs* __restrict__ args,
+                                                              const double* __restrict__ partials, int64_t rows)
+{
+    double* const tallies = args->tallies;
+    fold_partial_rows<ChargedColumns<kPos>>(partials, rows,
+                                            [&](int q, double v) { book_charged_column<kPos>(tallies, q, v); });
+}
+
+__global__ void __launch_bounds__(kBlock) photon_fold_kernel(const StepArgs* __restrict__ args,
+                                                             const double* __restrict__ partials, int64_t rows)
+{
+    double* const tallies = args->tallies;
+    fold_partial_rows<PhotonColumns>(partials, rows, [&](int q, double v) { book_photon_column(tallies, q, v); });
+}
+
 // ---------------------------------------------------------------------------
 // The fast path.  The dominant particle-step of the benchmark configurations
 // -- one substep, no discrete event, at most two single elastic scatters, no
@@ -2314,9 +2466,8 @@ __device__ __forceinline__ void append_slow(bool slow, int64_t ip, int32_t* list
 // 256-instance batch
 template <bool kPos, bool kBatched>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
-charged_fast_kernel(const StepArgs* __restrict__ args,
-                                                            int32_t* __restrict__ slow_list,
-                                                            int32_t* __restrict__ slow_count)
+charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slow_list,
+                    int32_t* __restrict__ slow_count, double* __restrict__ partials)
 {
     using S = Species<kPos>;
     StepArgs const& A = *args;
@@ -2340,20 +2491,10 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
     //     particle, which it sizes the same way, so recording them before the
     //     commit is exact).
     __shared__ double window[2 * kWindowFaces];
-    enum {
-        kColSurvivor, kColSubstep, kColDrag, kColIon, kColFieldWork, kColSoft,
-        kColCol = kColSoft + 1, kColHard = kColSoft + 2, kColHmx = kColSoft + 3,  // electrons only
-        kColSums = kPos ? kColSoft + 1 : kColSoft + 4,
-        kColTauA = kColSums, kColTauB = kColSums + 1, kColTauTot = kColSums + 2,  // positrons: A only
-        kColEnergy = kPos ? kColSums + 1 : kColSums + 3,
-        kCols = kColEnergy + 1,
-        // not folded: the hazard's total tau, with its hard-Moller tau in
-        // kColHmx until the epilogue forms the expected weight (electrons)
-        kColHazardTotal = kCols,
-    };
-    static_assert((kCols + 1) * kBlock <= 2 * kWindowFaces, "the columns exceed the deposit window");
+    using Col = ChargedColumns<kPos>;
+    static_assert((Col::kCols + 1) * kBlock <= 2 * kWindowFaces, "the columns exceed the deposit window");
     double* const lane_col = window + threadIdx.x;
-    for (int q = 0; q < kCols + 1; ++q) lane_col[q * kBlock] = 0.0;
+    for (int q = 0; q < Col::kCols + 1; ++q) lane_col[q * kBlock] = 0.0;
     // a batched wave takes the fast path only when all its particles share one
     // instance, whose environment is then wave-uniform; a wave that straddles
     // an instance boundary goes to the complete path
@@ -2436,12 +2577,12 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
             // the guard's taus for one subcycle; they count only if every driver
             // below asks for one subcycle too, which the energy column marks
             if constexpr (kPos) {
-                lane_col[kColTauA * kBlock] = total0 * mus / span;
+                lane_col[Col::kColTauA * kBlock] = total0 * mus / span;
             } else {
                 double const slice = mus / span;
-                lane_col[kColTauA * kBlock] = g_a * slice;
-                lane_col[kColTauB * kBlock] = g_b * slice;
-                lane_col[kColTauTot * kBlock] = (g_a + g_b) * slice;
+                lane_col[Col::kColTauA * kBlock] = g_a * slice;
+                lane_col[Col::kColTauB * kBlock] = g_b * slice;
+                lane_col[Col::kColTauTot * kBlock] = (g_a + g_b) * slice;
             }
             EBracket eb0;
             if (!elastic_bracket(EL, k, eb0)) break;
@@ -2456,7 +2597,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
             if (dmax(1.0, ceil((fabs(field_work) + drag_est) / impulse_scale)) != 1.0) break;
             if (err) break;
             // every driver asked for one subcycle: the guard's record of this step
-            lane_col[kColEnergy * kBlock] = k;
+            lane_col[Col::kColEnergy * kBlock] = k;
             // the single substep: the driver chord again (sub_dt = dt / 1)
             double const se = path.end_f;
             double const substep_dt = dt * (se - 0.0);
@@ -2481,8 +2622,8 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
                 electron_stp_rates(T, b1, thr, gcut, k1, hm_on, s0, s1, err, &hm_inv1);
                 double const hard_tau = s0 * L, brems_tau = s1 * L;
                 h = select_hazard(hard_tau, brems_tau, rng_uniform(rng, 0, S::kDepth0), rng_uniform(rng, 0, S::kPick0));
-                lane_col[kColHmx * kBlock] = hard_tau;
-                lane_col[kColHazardTotal * kBlock] = hard_tau + brems_tau;
+                lane_col[Col::kColHmx * kBlock] = hard_tau;
+                lane_col[Col::kColHazardTotal * kBlock] = hard_tau + brems_tau;
             }
             if (!h.valid || h.event || err) break;
             // continuous drag over [0, se] at the frozen energy k1, not reaching the cutoff
@@ -2500,10 +2641,10 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
             double const avail = dmax(0.0, k1 - cutoff);
             double const drag = dmin(raw_col + raw_soft, avail);
             double const col_loss = dmin(raw_col, drag);
-            lane_col[kColSoft * kBlock] = w * dmax(0.0, drag - col_loss);
+            lane_col[Col::kColSoft * kBlock] = w * dmax(0.0, drag - col_loss);
             if constexpr (!kPos) {
-                lane_col[kColCol * kBlock] = w * col_loss;
-                lane_col[kColHard * kBlock] = w * dmin(hstp * L, avail);
+                lane_col[Col::kColCol * kBlock] = w * col_loss;
+                lane_col[Col::kColHard * kBlock] = w * dmin(hstp * L, avail);
             }
             double ion = 0.0;
             if (col_loss > 0.0 && raw_col > 0.0) {
@@ -2570,82 +2711,35 @@ charged_fast_kernel(const StepArgs* __restrict__ args,
             if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SURVIVE);
             if (A.ion_pairs) A.ion_pairs[ip] = ion;
             if (A.substeps) A.substeps[ip] = 1;
-            lane_col[kColSurvivor * kBlock] = 1.0;
-            lane_col[kColSubstep * kBlock] = 1.0;
-            lane_col[kColDrag * kBlock] = w * drag;
-            lane_col[kColIon * kBlock] = ion;
-            lane_col[kColFieldWork * kBlock] = w * hw1 + w * hw2;
+            lane_col[Col::kColSurvivor * kBlock] = 1.0;
+            lane_col[Col::kColSubstep * kBlock] = 1.0;
+            lane_col[Col::kColDrag * kBlock] = w * drag;
+            lane_col[Col::kColIon * kBlock] = ion;
+            lane_col[Col::kColFieldWork * kBlock] = w * hw1 + w * hw2;
         } while (false);
     }
     append_slow(slow, ip, slow_list, slow_count);
-    if (!(lane_col[kColEnergy * kBlock] > 0.0)) {
+    if (!(lane_col[Col::kColEnergy * kBlock] > 0.0)) {
         // a lane stopped by a substep driver records no guard taus
-        for (int q = kColSums; q < kColEnergy; ++q) lane_col[q * kBlock] = 0.0;
+        for (int q = Col::kColSums; q < Col::kColEnergy; ++q) lane_col[q * kBlock] = 0.0;
     }
     if (slow) {
         // a lane handed over counts none of the sums it formed
-        for (int q = kColSoft; q < kColSums; ++q) lane_col[q * kBlock] = 0.0;
+        for (int q = Col::kColSoft; q < Col::kColSums; ++q) lane_col[q * kBlock] = 0.0;
     } else if constexpr (!kPos) {
         if (ip < A.n) {
             // the expected hard-Moller weight of the committed step's one hazard
             // draw, w P(total) hard / total (OV/RreaElectronTransport.cpp:630-646)
-            double const hard_tau = lane_col[kColHmx * kBlock];
-            double const total_tau = lane_col[kColHazardTotal * kBlock];
-            lane_col[kColHmx * kBlock] = A.w[ip] * (total_tau > 0.0
+            double const hard_tau = lane_col[Col::kColHmx * kBlock];
+            double const total_tau = lane_col[Col::kColHazardTotal * kBlock];
+            lane_col[Col::kColHmx * kBlock] = A.w[ip] * (total_tau > 0.0
                 ? probability_from_optical_depth(total_tau) * hard_tau / total_tau : 0.0);
         }
     }
-    {
-        // the tallies and the committed steps' ledger records
-        // (OV/RreaElectronTransport.cpp:250-400, OV/RreaPositronTransport.cpp:241-360, 776)
-        double* const led = A.tallies + RREA_LEDGER_OFFSET;
-        fold_columns<kCols, kColSums>(window, [&](int q, double v) {
-            if (q < kColSums) {
-                if (v == 0.0) return;
-                switch (q) {
-                case kColSurvivor: atomic_add(&A.tallies[0], v); break;
-                case kColSubstep: atomic_add(&A.tallies[6], v); break;
-                case kColDrag:
-                    atomic_add(&A.tallies[7], v);
-                    atomic_add(led + (kPos ? RREA_L_POSITRON_TRANSPORT_ENERGY_LOSS_EV : RREA_L_PARENT_DRAG_LOSS_EV), v);
-                    break;
-                case kColIon:
-                    // every ion-pair piece is a low-energy ionization record of diagnostic loss W_air
-                    atomic_add(&A.tallies[8], v);
-                    atomic_add(led + RREA_L_TRANSPORT_IONIZATION_WEIGHT, v);
-                    atomic_add(led + RREA_L_TRANSPORT_LOW_SECONDARY_WEIGHT, v);
-                    atomic_add(led + RREA_L_TRANSPORT_ENERGY_LOSS_EV, v * kWAir);
-                    break;
-                case kColFieldWork: atomic_add(&A.tallies[9], v); break;
-                case kColSoft: atomic_add(led + RREA_L_PHOTON_CONTINUOUS_SOFT_BREMS_ENERGY_EV, v); break;
-                default:
-                    if constexpr (!kPos) {
-                        if (q == kColCol) {
-                            atomic_add(led + RREA_L_ION_PAIR_SOURCE_LOSS_EV, v);
-                            atomic_add(led + RREA_L_RESTRICTED_COLLISION_LOSS_EV, v);
-                        } else if (q == kColHard) {
-                            atomic_add(led + RREA_L_EXPLICIT_HARD_MOLLER_ENERGY_LOSS_EV, v);
-                        } else if (q == kColHmx) {
-                            atomic_add(led + RREA_L_HARD_MOLLER_EXPECTED_EVENT_WEIGHT, v);
-                        }
-                    }
-                    break;
-                }
-                return;
-            }
-            if (q == kColEnergy) {
-                atomic_max_nonneg(led + (kPos ? RREA_L_MAX_TRANSPORT_POSITRON_ENERGY_EV
-                                              : RREA_L_MAX_TRANSPORT_ELECTRON_ENERGY_EV), v);
-                if (v > 0.0) atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_SUBCYCLES, 1.0);
-            } else if constexpr (kPos) {
-                atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_POSITRON_TAU, v);
-            } else {
-                atomic_max_nonneg(led + (q == kColTauA ? RREA_L_MAX_TRANSPORT_HARD_MOLLER_TAU
-                                         : q == kColTauB ? RREA_L_MAX_TRANSPORT_BREMS_TAU
-                                                         : RREA_L_MAX_TRANSPORT_ELECTRON_TOTAL_COMPETING_TAU), v);
-            }
-        });
-    }
+    // the workgroup's tallies and ledger records, booked by the fold kernel
+    fold_columns<Col::kCols, Col::kColSums>(window, [&](int q, double v) {
+        partials[static_cast<int64_t>(blockIdx.x) * kPartialStride + q] = v;
+    });
     deposit_block(A, window, window + kWindowFaces, dep, dep_inst, dep_q, A.cfg.dt_s, dep_r0, dep_z0, dep_r1, dep_z1,
                   src);
 }
@@ -2713,16 +2807,17 @@ __global__ void __launch_bounds__(kBlock) photon_kernel(const StepArgs* __restri
 template <bool kBatched>
 __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __restrict__ args,
                                                            int32_t* __restrict__ slow_list,
-                                                           int32_t* __restrict__ slow_count)
+                                                           int32_t* __restrict__ slow_count,
+                                                           double* __restrict__ partials)
 {
     StepArgs const& A = *args;
     // each lane's tallies and ledger records in LDS columns of its own, folded
     // per workgroup at the end: survivors, escapes, escaped energy (sums); the
     // pass's tau and energy (maxima) -- OV/RreaPhotonTransport.cpp:192, 222-233
-    enum { kColSurvivor, kColEscape, kColEscapedEnergy, kColTau, kColEnergy, kCols };
-    __shared__ double cols[kCols * kBlock];
+    using Col = PhotonColumns;
+    __shared__ double cols[Col::kCols * kBlock];
     double* const lane_col = cols + threadIdx.x;
-    for (int q = 0; q < kCols; ++q) lane_col[q * kBlock] = 0.0;
+    for (int q = 0; q < Col::kCols; ++q) lane_col[q * kBlock] = 0.0;
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     bool slow = false;
     // a batched wave takes the fast path only when all its particles share one
@@ -2791,14 +2886,14 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
             A.z[ip] = z;
             if (escapes) {
                 A.id[ip] = RREA_INVALID_ID;
-                lane_col[kColEscape * kBlock] = 1.0;
-                lane_col[kColEscapedEnergy * kBlock] = w * energy;
+                lane_col[Col::kColEscape * kBlock] = 1.0;
+                lane_col[Col::kColEscapedEnergy * kBlock] = w * energy;
             } else {
-                lane_col[kColSurvivor * kBlock] = 1.0;
+                lane_col[Col::kColSurvivor * kBlock] = 1.0;
             }
             if (path.in_len > 0.0) {
-                lane_col[kColTau * kBlock] = pass_tau;
-                lane_col[kColEnergy * kBlock] = energy;
+                lane_col[Col::kColTau * kBlock] = pass_tau;
+                lane_col[Col::kColEnergy * kBlock] = energy;
             }
             if (A.status) A.status[ip] = static_cast<uint8_t>(escapes ? RREA_ESCAPE : RREA_SURVIVE);
             if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
@@ -2806,19 +2901,9 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
         } while (false);
     }
     append_slow(slow, ip, slow_list, slow_count);
-    {
-        double* const led = A.tallies + RREA_LEDGER_OFFSET;
-        fold_columns<kCols, kColTau>(cols, [&](int q, double v) {
-            if (q == kColSurvivor && v != 0.0) atomic_add(&A.tallies[0], v);
-            if (q == kColEscape && v != 0.0) atomic_add(&A.tallies[1], v);
-            if (q == kColEscapedEnergy && v != 0.0) atomic_add(led + RREA_L_PHOTON_ESCAPED_ENERGY_EV, v);
-            if (q == kColTau) atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_PHOTON_TAU, v);
-            if (q == kColEnergy) {
-                atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_PHOTON_ENERGY_SEEN_EV, v);
-                if (v > 0.0) atomic_max_nonneg(led + RREA_L_MAX_TRANSPORT_SUBCYCLES, 1.0);
-            }
-        });
-    }
+    fold_columns<Col::kCols, Col::kColSums>(cols, [&](int q, double v) {
+        partials[static_cast<int64_t>(blockIdx.x) * kPartialStride + q] = v;
+    });
 }
 
 // The photons the fast path handed over, strided over a fixed grid.
@@ -4557,7 +4642,7 @@ rrea_status rrea_fields_destroy(rrea_fields_t F)
     if (!F) return RREA_OK;
     void* ptrs[] = {F->d_cell_er, F->d_cell_ez, F->d_btheta, F->d_stage, F->d_prof, F->d_stage_sec, F->d_keys,
                     F->d_keys_out, F->d_idx, F->d_idx_out, F->d_sort_tmp, F->d_count, F->d_args,
-                    F->d_slow_list, F->d_slow_count, F->d_prof_guide, F->d_nargs, F->d_fin, F->d_fst,
+                    F->d_slow_list, F->d_slow_count, F->d_partials, F->d_prof_guide, F->d_nargs, F->d_fin, F->d_fst,
                     F->d_fion, F->d_fsub, F->d_anchor, F->d_next, F->d_flag, F->d_nsel, F->d_sel_tmp,
                     F->d_tkeys, F->d_tkeys_out, F->d_tidx, F->d_tidx_out, F->d_tsort_tmp, F->d_tbuf, F->d_base,
                     F->d_live, F->d_dargs, F->d_gcell};
@@ -4741,10 +4826,13 @@ rrea_status slow_list_workspace(rrea_fields_t F, int64_t n)
 {
     if (F->slow_cap < n) {
         if (F->d_slow_list) (void)hipFree(F->d_slow_list);
+        if (F->d_partials) (void)hipFree(F->d_partials);
         F->d_slow_list = nullptr;
+        F->d_partials = nullptr;
         F->slow_cap = 0;
         if (n > INT32_MAX) return RREA_ERR_ARGUMENT;
         HIP_TRY(hipMalloc(&F->d_slow_list, n * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&F->d_partials, blocks_for(n, kBlock) * kPartialStride * sizeof(double)));
         F->slow_cap = n;
     }
     if (!F->d_slow_count) HIP_TRY(hipMalloc(&F->d_slow_count, sizeof(int32_t)));
@@ -4864,6 +4952,7 @@ struct StepWork {
     int64_t* idx_out = nullptr;
     int32_t* slow_list = nullptr;
     int32_t* slow_count = nullptr;
+    double* partials = nullptr;
 };
 
 StepWork work_of(rrea_fields_t W)
@@ -4871,7 +4960,7 @@ StepWork work_of(rrea_fields_t W)
     StepWork K;
     K.stage = W->d_stage_sec; K.count = W->d_count; K.base = W->d_base;
     K.keys = W->d_keys; K.keys_out = W->d_keys_out; K.idx = W->d_idx; K.idx_out = W->d_idx_out;
-    K.slow_list = W->d_slow_list; K.slow_count = W->d_slow_count;
+    K.slow_list = W->d_slow_list; K.slow_count = W->d_slow_count; K.partials = W->d_partials;
     return K;
 }
 
@@ -4962,6 +5051,14 @@ rrea_status species_prepare(int species, rrea_tables_t T, rrea_fields_t F, const
     return RREA_OK;
 }
 
+// The fold kernel's grid over `rows` partial rows: four passes of sixteen
+// rows per workgroup, at most 512 workgroups.
+int64_t fold_blocks(int64_t rows)
+{
+    int64_t const b = blocks_for(rows, 4 * (kBlock / kPartialStride));
+    return b < 512 ? b : 512;
+}
+
 // A species step's launches on `s`, reading the argument block at d_args.
 // With device_order the staged secondaries are ordered with their count on
 // the device (sort_records_kernel), so the sequence makes no host round trip
@@ -5019,16 +5116,18 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
         st = timed(4, [&] {
             if (batched) {
                 photon_fast_kernel<true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                    d_args, K.slow_list, K.slow_count);
+                    d_args, K.slow_list, K.slow_count, K.partials);
             } else {
                 photon_fast_kernel<false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                    d_args, K.slow_list, K.slow_count);
+                    d_args, K.slow_list, K.slow_count, K.partials);
             }
         });
         if (st != RREA_OK) return st;
         if (signal) HIP_TRY(hipEventRecord(W->fast_event, s));
         int64_t const slow_blocks = nb < 2048 ? nb : 2048;
         st = timed(5, [&] {
+            photon_fold_kernel<<<dim3(static_cast<unsigned>(fold_blocks(nb))), dim3(kBlock), 0, s>>>(
+                d_args, K.partials, nb);
             photon_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
                 d_args, K.slow_list, K.slow_count);
         });
@@ -5042,18 +5141,18 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
             if (positron) {
                 if (batched) {
                     charged_fast_kernel<true, true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, K.slow_list, K.slow_count);
+                        d_args, K.slow_list, K.slow_count, K.partials);
                 } else {
                     charged_fast_kernel<true, false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, K.slow_list, K.slow_count);
+                        d_args, K.slow_list, K.slow_count, K.partials);
                 }
             } else {
                 if (batched) {
                     charged_fast_kernel<false, true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, K.slow_list, K.slow_count);
+                        d_args, K.slow_list, K.slow_count, K.partials);
                 } else {
                     charged_fast_kernel<false, false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, K.slow_list, K.slow_count);
+                        d_args, K.slow_list, K.slow_count, K.partials);
                 }
             }
         });
@@ -5062,9 +5161,13 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
         int64_t const slow_blocks = nb < 2048 ? nb : 2048;
         st = timed(kid + 1, [&] {
             if (positron) {
+                charged_fold_kernel<true><<<dim3(static_cast<unsigned>(fold_blocks(nb))), dim3(kBlock), 0, s>>>(
+                    d_args, K.partials, nb);
                 charged_slow_kernel<true><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
                     d_args, K.slow_list, K.slow_count);
             } else {
+                charged_fold_kernel<false><<<dim3(static_cast<unsigned>(fold_blocks(nb))), dim3(kBlock), 0, s>>>(
+                    d_args, K.partials, nb);
                 charged_slow_kernel<false><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
                     d_args, K.slow_list, K.slow_count);
             }
@@ -5118,8 +5221,8 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
 
 // ---------------------------------------------------------------------------
 // The species step without allocation (rrea_transport_step_noalloc).  Its
-// workspace -- the argument block, the counts, the hand-over list and the
-// ordering buffers -- lives in one file-scope device array of fixed size,
+// workspace -- the argument block, the counts, the hand-over list, the fast
+// path's partial rows and the ordering buffers -- lives in one file-scope device array of fixed size,
 // `scratch`, so the call makes no hipMalloc.  Each device holds its own copy
 // of the array; its address is looked up once per device with
 // hipGetSymbolAddress and only that address is handed to kernels and copies.
@@ -5137,12 +5240,13 @@ constexpr size_t kNaCount = kNaArgs + round256(sizeof(StepArgs));
 constexpr size_t kNaBase = kNaCount + 256;
 constexpr size_t kNaSlowCount = kNaBase + 256;
 constexpr size_t kNaSlowList = kNaSlowCount + 256;
-constexpr size_t kNaStage = kNaSlowList + round256(kNoAllocMaxParticles * sizeof(int32_t));
+constexpr size_t kNaPartials = kNaSlowList + round256(kNoAllocMaxParticles * sizeof(int32_t));
+constexpr size_t kNaStage = kNaPartials + round256((kNoAllocMaxParticles / kBlock) * kPartialStride * sizeof(double));
 constexpr size_t kNaKeys = kNaStage + round256(kNoAllocMaxSecondaries * sizeof(rrea_secondary));
 constexpr size_t kNaKeysOut = kNaKeys + round256(kNoAllocMaxSecondaries * sizeof(uint64_t));
 constexpr size_t kNaIdx = kNaKeysOut + round256(kNoAllocMaxSecondaries * sizeof(uint64_t));
 constexpr size_t kNaIdxOut = kNaIdx + round256(kNoAllocMaxSecondaries * sizeof(int64_t));
-constexpr size_t kNoAllocBytes = kNaIdxOut + round256(kNoAllocMaxSecondaries * sizeof(int64_t));  // ~8.5 MiB
+constexpr size_t kNoAllocBytes = kNaIdxOut + round256(kNoAllocMaxSecondaries * sizeof(int64_t));  // ~9 MiB
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
@@ -5179,6 +5283,7 @@ rrea_status noalloc_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     K.base = reinterpret_cast<int64_t*>(base + kNaBase);
     K.slow_count = reinterpret_cast<int32_t*>(base + kNaSlowCount);
     K.slow_list = reinterpret_cast<int32_t*>(base + kNaSlowList);
+    K.partials = reinterpret_cast<double*>(base + kNaPartials);
     K.stage = reinterpret_cast<rrea_secondary*>(base + kNaStage);
     K.keys = reinterpret_cast<uint64_t*>(base + kNaKeys);
     K.keys_out = reinterpret_cast<uint64_t*>(base + kNaKeysOut);
