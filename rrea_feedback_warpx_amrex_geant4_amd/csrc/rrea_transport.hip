@@ -1076,8 +1076,10 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
             double dx = pux, dy = puy, dz = puz;
             double k = kinetic_from_u(dx, dy, dz);
             normalize(dx, dy, dz);
-            double run_x = pr * cos(pth);
-            double run_y = pr * sin(pth);
+            double s_th, c_th;
+            sin_cos(pth, s_th, c_th);
+            double run_x = pr * c_th;
+            double run_y = pr * s_th;
             double run_z = pz;
             double const x_particle = run_x, y_particle = run_y;
             double const r_particle = pr, z_particle = pz;
@@ -2490,10 +2492,16 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
     //     takes the same value again if the complete path later steps the
     //     particle, which it sizes the same way, so recording them before the
     //     commit is exact).
-    __shared__ double window[2 * kWindowFaces];
+    // Past the window, three more columns hold the lane's start point and
+    // weight while the chord's field work and the kicks need the registers;
+    // they are read back (relaxed LDS loads, which the compiler does not fold
+    // into the stores) where the drag, the commit and the deposit use them.
+    __shared__ double window[2 * kWindowFaces + 3 * kBlock];
     using Col = ChargedColumns<kPos>;
     static_assert((Col::kCols + 1) * kBlock <= 2 * kWindowFaces, "the columns exceed the deposit window");
     double* const lane_col = window + threadIdx.x;
+    double* const stash = window + 2 * kWindowFaces + threadIdx.x;
+    auto unstash = [&](int q) { return __hip_atomic_load(stash + q * kBlock, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
     for (int q = 0; q < Col::kCols + 1; ++q) lane_col[q * kBlock] = 0.0;
     // a batched wave takes the fast path only when all its particles share one
     // instance, whose environment is then wave-uniform; a wave that straddles
@@ -2534,12 +2542,17 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             double const pr = A.prev_r[ip];
             if (!(w > 0.0) || pr > D.r_hi + 1.0e-9 * D.r_hi) break;
             double const pz = A.prev_z[ip], pth = A.prev_th[ip];
+            stash[0] = pr;
+            stash[kBlock] = pz;
+            stash[2 * kBlock] = w;
             double dx = A.prev_ux[ip], dy = A.prev_uy[ip], dz = A.prev_uz[ip];
             double k = kinetic_from_u(dx, dy, dz);
             normalize(dx, dy, dz);
             if (!(k >= cutoff)) break;
-            double run_x = pr * cos(pth);
-            double run_y = pr * sin(pth);
+            double s_th, c_th;
+            sin_cos(pth, s_th, c_th);
+            double run_x = pr * c_th;
+            double run_y = pr * s_th;
             double run_z = pz;
             Path path;
             if (!build_chord(run_x, run_y, run_z, dx, dy, dz, k, dt, dt, D, path, err) || err) break;
@@ -2636,15 +2649,16 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             double const to_cut = k1 - cutoff;
             double const req = total * L;
             if (!finite(req) || req < 0.0 || !(req < to_cut) || !(k1 > 0.0) || err) break;
+            double const wq = unstash(2);
             double const raw_col = col * L;
             double const raw_soft = soft * L;
             double const avail = dmax(0.0, k1 - cutoff);
             double const drag = dmin(raw_col + raw_soft, avail);
             double const col_loss = dmin(raw_col, drag);
-            lane_col[Col::kColSoft * kBlock] = w * dmax(0.0, drag - col_loss);
+            lane_col[Col::kColSoft * kBlock] = wq * dmax(0.0, drag - col_loss);
             if constexpr (!kPos) {
-                lane_col[Col::kColCol * kBlock] = w * col_loss;
-                lane_col[Col::kColHard * kBlock] = w * dmin(hstp * L, avail);
+                lane_col[Col::kColCol * kBlock] = wq * col_loss;
+                lane_col[Col::kColHard * kBlock] = wq * dmin(hstp * L, avail);
             }
             double ion = 0.0;
             if (col_loss > 0.0 && raw_col > 0.0) {
@@ -2652,7 +2666,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
                 if (le > 0.0) {
                     double const raw_i = path.full * (le - 0.0) * iv.ratio * col;
                     double const loss_i = col_loss * raw_i / raw_col;
-                    double const pw = w * loss_i / dmax(kWAir, 1.0);
+                    double const pw = wq * loss_i / dmax(kWAir, 1.0);
                     if (pw > 0.0) ion += pw;
                 }
             }
@@ -2686,7 +2700,8 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             // terminal: a survivor (for e-, above the demotion gate's energy)
             if (!kPos && thr > 0.0 && k2 <= thr) break;
             double const end_r = hypot(x, y);
-            if (!deposit_segment_check(E.G, pr, pz, end_r, z)) break;
+            double const pr0 = unstash(0), pz0 = unstash(1);
+            if (!deposit_segment_check(E.G, pr0, pz0, end_r, z)) break;
             // ---- commit: nothing above has written anything ----
             slow = false;
             dep = true;
@@ -2701,8 +2716,8 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
                 else report(A.error, kErrDomain, ip);
             }
             dep_inst = E.instance;
-            dep_q = S::kCharge * w;
-            dep_r0 = pr; dep_z0 = pz; dep_r1 = end_r; dep_z1 = z;
+            dep_q = S::kCharge * wq;
+            dep_r0 = pr0; dep_z0 = pz0; dep_r1 = end_r; dep_z1 = z;
             double const um = proper_velocity(k2);
             A.ux[ip] = um * d1x; A.uy[ip] = um * d1y; A.uz[ip] = um * d1z;
             A.r[ip] = end_r;
@@ -2713,9 +2728,9 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             if (A.substeps) A.substeps[ip] = 1;
             lane_col[Col::kColSurvivor * kBlock] = 1.0;
             lane_col[Col::kColSubstep * kBlock] = 1.0;
-            lane_col[Col::kColDrag * kBlock] = w * drag;
+            lane_col[Col::kColDrag * kBlock] = wq * drag;
             lane_col[Col::kColIon * kBlock] = ion;
-            lane_col[Col::kColFieldWork * kBlock] = w * hw1 + w * hw2;
+            lane_col[Col::kColFieldWork * kBlock] = wq * hw1 + wq * hw2;
         } while (false);
     }
     append_slow(slow, ip, slow_list, slow_count);
@@ -2776,8 +2791,10 @@ __global__ void photon_push_kernel(int64_t n, const int64_t* __restrict__ id, do
     prev_r[i] = r0; prev_th[i] = t0; prev_z[i] = z0;
     double const un = sqrt(ux[i] * ux[i] + uy[i] * uy[i] + uz[i] * uz[i]);
     double const s = c_dt / (un > 0.0 ? un : 1.0);
-    double const x = r0 * cos(t0) + ux[i] * s;
-    double const y = r0 * sin(t0) + uy[i] * s;
+    double s_t0, c_t0;
+    sin_cos(t0, s_t0, c_t0);
+    double const x = r0 * c_t0 + ux[i] * s;
+    double const y = r0 * s_t0 + uy[i] * s;
     r[i] = hypot(x, y);
     th[i] = atan2(y, x);
     z[i] = z0 + uz[i] * s;
