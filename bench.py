@@ -18,11 +18,14 @@ at least its amortised share of sorts.
 One "step" is one RreaParticleInteraction::Apply (RreaParticleInteraction.cpp:57-60)
 and the WarpX calls around it: the electron kernels, the photon kernel, the
 positron kernels, each followed by the deterministic secondary ordering --
-the three passes run on three streams, the photon and positron passes
-starting when the electron fast-path kernel ends, under the electron
-hand-over (ConcurrentApply; --sequential runs them one after another; the
+the three passes run on three streams, the positron pass starting with
+the electron pass (its latency-bound hand-over runs under the electron fast
+kernel) and the photon pass when the electron fast-path kernel ends, under
+the electron hand-over (ConcurrentApply; --early chooses the passes that
+start with the electron pass; --sequential runs them one after another; the
 results are the same) -- then the newborn residual advance of every
-secondary the step made (the transient FIFO of DrainAndCommitSecondaries),
+secondary the step made (the transient FIFO of DrainAndCommitSecondaries,
+with its generation counts on the device: no host round trip in the step),
 all charged chords depositing into one J grid; then the commit of the
 drain's survivors into the species' stores (AddNParticles,
 RreaSecondaryCommit.cpp:594-629), so newborns are transported from the next
@@ -555,9 +558,11 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=10, help="timed steps of the batched leg (>= 1)")
     ap.add_argument("--batched-warmup", type=int, default=2)
     ap.add_argument("--field-steps", type=int, default=200, help="timed feedback steps of the field leg (0: skip)")
-    ap.add_argument("--device-drain", type=int, default=0,
+    ap.add_argument("--early", nargs="*", default=["positron"], choices=["photon", "positron"],
+                    help="species passes that start with the electron pass instead of after its fast kernel")
+    ap.add_argument("--device-drain", type=int, default=4,
                     help="concurrent step: drain with the counts on the device, up to this many generations "
-                         "(0: the host-driven drain)")
+                         "(0: the host-driven drain); a FIFO still holding a generation after them is an error")
     ap.add_argument("--column-steps", type=int, default=100, help="timed steps of the feedback-column leg (0: skip)")
     ap.add_argument("--column-inject", type=int, default=2048, help="macroparticles injected per column step")
     ap.add_argument("--population-particles", type=int, default=1 << 22,
@@ -634,7 +639,8 @@ def main():
     drain = DrainOutputs(1 << 21, out_e)
     L = _lib.lib()
     apply = RreaParticleInteraction
-    conc = None if args.sequential else ConcurrentApply(tables, fields)
+    conc = None if args.sequential else ConcurrentApply(tables, fields, early=tuple(args.early))
+    step_clear = [o.secondary_count for o in (out_e, out_g, out_p)] + [out_e.jr, out_e.jz, out_e.rho, out_e.sources]
     # the handle each species' kernels are timed on
     h_e, h_g, h_p = (coupling.h,) * 3 if conc is None else tuple(c.h for c in conc.couplings)
 
@@ -646,12 +652,8 @@ def main():
             dev_e.sort_by_cell(coupling, energy_bits=energy_bits)
             dev_p.sort_by_cell(coupling, energy_bits=energy_bits)
             dev_g.sort_by_cell(coupling, energy_bits=energy_bits)
-        for o in (out_e, out_g, out_p):
-            o.secondary_count.zero_()
-        out_e.jr.zero_()
-        out_e.jz.zero_()
-        out_e.rho.zero_()
-        out_e.sources.zero_()
+        # the step's counters and grids cleared (one launch per dtype)
+        torch._foreach_zero_(step_clear)
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
         if conc is not None:
             conc.run(dev_e, dev_g, dev_p, cfg, out_e, out_g, out_p, drain, device_drain=args.device_drain)
@@ -781,7 +783,11 @@ def main():
                                "dt=1 ps, RZ 256x1024, deposit on, material sources accumulated, secondaries ordered and their newborn residual "
                                "advance drained, photon push on device",
                    "electrons": n_e, "photons": n_g, "positrons": n_p, "path": "plain",
-                   "species_passes": "sequential" if args.sequential else "concurrent (three streams)",
+                   "species_passes": "sequential" if args.sequential else
+                   ("concurrent (three streams): " + (", ".join(args.early) or "no pass") + " starting with the "
+                    "electron pass, the others after the electron fast kernel"),
+                   "drain": (f"device-driven, {args.device_drain} generations" if args.device_drain > 0 and not args.sequential
+                             else "host-driven"),
                    "tile_sort_every": knobs["sort_every"], "tile_sort_energy_bits": knobs["energy_bits"],
                    "tile_sorts_in_timed_region": sorts_timed,
                    "autotune": tuned,

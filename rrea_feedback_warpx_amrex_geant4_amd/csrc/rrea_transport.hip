@@ -3708,10 +3708,6 @@ __device__ __noinline__ void photon_newborn(NewbornArgs const& N, int64_t i, dou
 __global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __restrict__ args)
 {
     NewbornArgs const& N = *args;
-    __shared__ double ledger[RREA_LEDGER_COUNT];
-    ledger_begin(ledger);
-    double tally[kTallyNewborn];
-    for (int q = 0; q < kTallyNewborn; ++q) tally[q] = 0.0;
     // an error recorded before the launch (the step's, or a rejected birth
     // payload) stops the drain: nothing is advanced
     bool const stopped = __hip_atomic_load(N.S.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
@@ -3719,6 +3715,13 @@ __global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __re
     // whose bounds only the device knows
     int64_t const stride = static_cast<int64_t>(gridDim.x) * kBlock;
     int64_t const ge = stopped ? N.gb : N.ge;
+    // a workgroup with no entry (an empty generation of the device-driven
+    // drain) leaves at once; the test is the same for all its lanes
+    if (N.gb + static_cast<int64_t>(blockIdx.x) * kBlock >= ge) return;
+    __shared__ double ledger[RREA_LEDGER_COUNT];
+    ledger_begin(ledger);
+    double tally[kTallyNewborn];
+    for (int q = 0; q < kTallyNewborn; ++q) tally[q] = 0.0;
     for (int64_t i = N.gb + static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < ge; i += stride) {
         rrea_secondary const& p = N.work[i];
         bool const photon = p.species == 2;
@@ -4115,10 +4118,12 @@ __device__ __forceinline__ int64_t staged_records(const int64_t* count, const in
     return n > 0 ? n : 0;
 }
 
-__global__ void __launch_bounds__(kSortThreads) sort_records_kernel(
-    const rrea_secondary* __restrict__ in, const int64_t* __restrict__ count, const int64_t* __restrict__ cap_dev,
-    int64_t cap, uint64_t* __restrict__ gk0, uint64_t* __restrict__ gk1, int32_t* __restrict__ gi0,
-    int32_t* __restrict__ gi1, int32_t* __restrict__ order)
+// The sort of n records by one workgroup of kSortThreads lanes; every lane
+// returns after its last write to `order` (no barrier after it).
+__device__ __forceinline__ void sort_records_block(const rrea_secondary* __restrict__ in, int64_t n,
+                                                   uint64_t* __restrict__ gk0, uint64_t* __restrict__ gk1,
+                                                   int32_t* __restrict__ gi0, int32_t* __restrict__ gi1,
+                                                   int32_t* __restrict__ order)
 {
     __shared__ uint64_t lk[2][kSortLds];
     __shared__ int32_t li[2][kSortLds];
@@ -4127,7 +4132,6 @@ __global__ void __launch_bounds__(kSortThreads) sort_records_kernel(
     __shared__ uint64_t wmax[kSortWaves];
     __shared__ int32_t skip[8];  // per pass, so no reset races a slower wavefront's read
     int const tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    int64_t const n = staged_records(count, cap_dev, cap);
     if (n == 0) return;
     bool const in_lds = n <= kSortLds;
     uint64_t* ks = in_lds ? lk[0] : gk0;
@@ -4221,6 +4225,14 @@ __global__ void __launch_bounds__(kSortThreads) sort_records_kernel(
     for (int64_t i = tid; i < n; i += kSortThreads) order[i] = is[i];
 }
 
+__global__ void __launch_bounds__(kSortThreads) sort_records_kernel(
+    const rrea_secondary* __restrict__ in, const int64_t* __restrict__ count, const int64_t* __restrict__ cap_dev,
+    int64_t cap, uint64_t* __restrict__ gk0, uint64_t* __restrict__ gk1, int32_t* __restrict__ gi0,
+    int32_t* __restrict__ gi1, int32_t* __restrict__ order)
+{
+    sort_records_block(in, staged_records(count, cap_dev, cap), gk0, gk1, gi0, gi1, order);
+}
+
 // The ordered records appended behind the count at `base` (device; NULL: 0),
 // within the destination's capacity; records beyond it are dropped and
 // reported.  With `be` (device) the destination range is [be[0], be[1])
@@ -4266,29 +4278,6 @@ __global__ void drain_begin_kernel(NewbornArgs* __restrict__ N, DrainState* __re
     N->gb = 0; N->ge = n0; N->S.sec_cap = cap - n0;
 }
 
-// After a generation: its children's range, empty when it made none, an
-// error is recorded, or they overflowed the FIFO (the newborn kernel
-// reported that).
-__global__ void drain_bounds_kernel(DrainState* __restrict__ g, const int64_t* __restrict__ children,
-                                    const int32_t* __restrict__ error)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int64_t const c = *children;
-    bool const stop = error[0] != 0 || c <= 0 || c > g->stage_cap;
-    g->be[0] = g->ge;
-    g->be[1] = stop ? g->ge : g->ge + c;
-}
-
-// The children become the next generation.
-__global__ void drain_advance_kernel(NewbornArgs* __restrict__ N, DrainState* __restrict__ g)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    g->gb = g->be[0];
-    g->ge = g->be[1];
-    g->stage_cap = g->cap - g->ge;
-    N->gb = g->gb; N->ge = g->ge; N->S.sec_cap = g->stage_cap;
-}
-
 // The FIFO's length; a generation left unadvanced after the last captured
 // one is a capacity error (index -2).
 __global__ void drain_end_kernel(const DrainState* __restrict__ g, int64_t* __restrict__ work_count,
@@ -4297,6 +4286,67 @@ __global__ void drain_end_kernel(const DrainState* __restrict__ g, int64_t* __re
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     *work_count = g->ge;
     if (g->gb < g->ge) report(error, kErrCapacity, -2);
+}
+
+// One generation's children linked into the FIFO by one workgroup, after the
+// generation's newborn kernel: the work of sort_records, drain_bounds,
+// gather_records, fifo_anchor, fifo_lineage, fifo_advance and drain_advance
+// (each phase as those kernels do it, a barrier between phases), then the
+// child count cleared for the next generation.  Two launches per captured
+// generation (this and the newborn kernel) instead of nine.
+__global__ void __launch_bounds__(kSortThreads) link_generation_kernel(
+    NewbornArgs* __restrict__ N, DrainState* __restrict__ g, const rrea_secondary* __restrict__ stage,
+    int64_t* __restrict__ count, uint64_t* __restrict__ gk0, uint64_t* __restrict__ gk1, int32_t* __restrict__ gi0,
+    int32_t* __restrict__ gi1, int32_t* __restrict__ order, rrea_secondary* __restrict__ work,
+    int64_t* __restrict__ anchor, unsigned long long* __restrict__ next, int32_t* __restrict__ error)
+{
+    __shared__ int64_t range[2];
+    int const tid = threadIdx.x;
+    int64_t const stage_cap = g->stage_cap, cap = g->cap, ge = g->ge;
+    int64_t const c = *count;
+    int64_t const n = c < stage_cap ? (c > 0 ? c : 0) : stage_cap;
+    sort_records_block(stage, n, gk0, gk1, gi0, gi1, order);
+    if (tid == 0) {
+        // drain_bounds: empty when the generation made no children, an error
+        // is recorded, or they overflowed the FIFO (the newborn kernel
+        // reported that)
+        bool const stop = __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || c <= 0
+            || c > stage_cap;
+        range[0] = ge;
+        range[1] = stop ? ge : ge + c;
+    }
+    __syncthreads();
+    int64_t const b = range[0], e = range[1];
+    for (int64_t j = b + tid; j < e; j += kSortThreads) {
+        if (j < cap) work[j] = stage[order[j - b]];
+        else report(error, kErrCapacity, -1);
+    }
+    __syncthreads();
+    for (int64_t j = b + tid; j < e; j += kSortThreads) anchor[j] = anchor[work[j].parent_index];
+    __syncthreads();
+    for (int64_t j = b + tid; j < e; j += kSortThreads) {
+        int64_t const a = anchor[j];
+        int64_t s = j;
+        while (s > b && anchor[s - 1] == a) --s;
+        uint64_t const lin = static_cast<uint64_t>(next[a]) + static_cast<uint64_t>(j - s);
+        if (lin >= kLineageCap) report(error, kErrInvalid, j);
+        work[j].lineage = lin;
+    }
+    __syncthreads();
+    for (int64_t j = b + tid; j < e; j += kSortThreads) {
+        int64_t const a = anchor[j];
+        if (j + 1 < e && anchor[j + 1] == a) continue;
+        int64_t s = j;
+        while (s > b && anchor[s - 1] == a) --s;
+        next[a] += static_cast<unsigned long long>(j - s + 1);
+    }
+    if (tid == 0) {
+        g->be[0] = b; g->be[1] = e;
+        g->gb = b; g->ge = e;
+        g->stage_cap = cap - e;
+        N->gb = b; N->ge = e; N->S.sec_cap = cap - e;
+        *count = 0;
+    }
 }
 
 // Survivors in FIFO order (one workgroup): the end-of-step states of the
@@ -5399,25 +5449,22 @@ rrea_status drain_device_launch(rrea_fields_t W, NewbornArgs* d_nargs, DrainStat
                                 int32_t* error, int32_t max_gen, hipStream_t s)
 {
     unsigned const g256 = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks_for(cap, 256), 1024)));
-    unsigned const gnb = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks_for(cap, kBlock), 1024)));
+    // 256 workgroups stride over a generation of any size (the drain's
+    // generations are thousands of entries at most in the configurations);
+    // an empty generation's launch then costs a few microseconds
+    unsigned const gnb = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks_for(cap, kBlock), 256)));
     int32_t* const order = reinterpret_cast<int32_t*>(W->d_idx_out);
     drain_begin_kernel<<<dim3(1), dim3(64), 0, s>>>(d_nargs, d_state, n_initial, cap);
     fill_u64_kernel<<<dim3(g256), dim3(256), 0, s>>>(W->d_next, 0, 0ULL, &d_state->ge);
     fifo_init_kernel<<<dim3(g256), dim3(256), 0, s>>>(work, 0, &d_state->ge, W->d_anchor, W->d_next, error);
     HIP_TRY(hipGetLastError());
+    fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(W->d_count), 1, 0ULL);
     for (int32_t gen = 0; gen < max_gen; ++gen) {
-        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(W->d_count), 1, 0ULL);
         newborn_kernel<<<dim3(gnb), dim3(kBlock), 0, s>>>(d_nargs);
-        sort_records_kernel<<<dim3(1), dim3(kSortThreads), 0, s>>>(
-            W->d_stage_sec, W->d_count, &d_state->stage_cap, 0, W->d_keys, W->d_keys_out,
-            reinterpret_cast<int32_t*>(W->d_idx), reinterpret_cast<int32_t*>(W->d_idx) + cap, order);
-        drain_bounds_kernel<<<dim3(1), dim3(64), 0, s>>>(d_state, W->d_count, error);
-        gather_records_kernel<<<dim3(g256), dim3(256), 0, s>>>(W->d_stage_sec, order, W->d_count, nullptr, 0, nullptr,
-                                                               cap, work, error, d_state->be);
-        fifo_anchor_kernel<<<dim3(g256), dim3(256), 0, s>>>(work, 0, 0, d_state->be, W->d_anchor);
-        fifo_lineage_kernel<<<dim3(g256), dim3(256), 0, s>>>(work, 0, 0, d_state->be, W->d_anchor, W->d_next, error);
-        fifo_advance_kernel<<<dim3(g256), dim3(256), 0, s>>>(0, 0, d_state->be, W->d_anchor, W->d_next);
-        drain_advance_kernel<<<dim3(1), dim3(64), 0, s>>>(d_nargs, d_state);
+        link_generation_kernel<<<dim3(1), dim3(kSortThreads), 0, s>>>(
+            d_nargs, d_state, W->d_stage_sec, W->d_count, W->d_keys, W->d_keys_out,
+            reinterpret_cast<int32_t*>(W->d_idx), reinterpret_cast<int32_t*>(W->d_idx) + cap, order, work,
+            W->d_anchor, W->d_next, error);
         HIP_TRY(hipGetLastError());
     }
     drain_end_kernel<<<dim3(1), dim3(64), 0, s>>>(d_state, work_count, error);

@@ -716,11 +716,18 @@ class ConcurrentApply:
     enqueues all three without a round trip.  The photon and positron passes
     wait for the end of the electron fast-path kernel
     (rrea_fields_signal_fast_path), so they run under the latency-bound
-    electron hand-over rather than competing with the electron kernel.  The
-    drain then runs on the caller's stream after all three."""
+    electron hand-over rather than competing with the electron kernel;
+    ``early`` names the passes that start with the electron pass instead
+    (``("positron",)``: the small positron pass and its own hand-over run
+    under the electron fast kernel, and the tail after it holds only the
+    electron and photon hand-overs).  The drain then runs on the caller's
+    stream after all three."""
 
-    def __init__(self, tables: InteractionTables, fields: RzFields):
+    def __init__(self, tables: InteractionTables, fields: RzFields, early: tuple = ()):
+        if not set(early) <= {"photon", "positron"}:
+            raise ValueError(f"early passes must be 'photon' and/or 'positron', not {early}")
         self.tables = tables
+        self.early = tuple(early)
         self.couplings = [Coupling(fields, tables) for _ in range(3)]  # electrons, photons, positrons
         self.streams = [torch.cuda.Stream(), torch.cuda.Stream()]      # default priority
         _lib.check(_lib.lib().rrea_fields_signal_fast_path(self.couplings[0].h, 1), "rrea_fields_signal_fast_path")
@@ -739,15 +746,27 @@ class ConcurrentApply:
             s.wait_event(ready)
         apply = RreaParticleInteraction
         cg, cp = self.couplings[1], self.couplings[2]
+        L = _lib.lib()
+
+        def photon_pass():
+            apply.apply_photons(self.tables, cg, photons, cfg, out_g, 2, self.streams[0].cuda_stream, False)
+            if push_photons:
+                photons.push(cfg.dt_s, self.streams[0].cuda_stream)
+
+        def positron_pass():
+            apply.apply_positrons(self.tables, cp, positrons, cfg, out_p, None, 2, self.streams[1].cuda_stream, False)
+
+        passes = (("photon", photon_pass, self.streams[0]), ("positron", positron_pass, self.streams[1]))
+        for name, run, _ in passes:
+            if name in self.early:
+                run()
         apply.apply_electrons(self.tables, self.couplings[0], electrons, cfg, out_e, order_secondaries=2,
                               stream=main.cuda_stream, check=False)
-        L = _lib.lib()
-        for s in self.streams:
-            _lib.check(L.rrea_stream_wait_fast_path(self.couplings[0].h, s.cuda_stream), "rrea_stream_wait_fast_path")
-        apply.apply_photons(self.tables, cg, photons, cfg, out_g, 2, self.streams[0].cuda_stream, False)
-        if push_photons:
-            photons.push(cfg.dt_s, self.streams[0].cuda_stream)
-        apply.apply_positrons(self.tables, cp, positrons, cfg, out_p, None, 2, self.streams[1].cuda_stream, False)
+        for name, run, s in passes:
+            if name not in self.early:
+                _lib.check(L.rrea_stream_wait_fast_path(self.couplings[0].h, s.cuda_stream),
+                           "rrea_stream_wait_fast_path")
+                run()
         for s in self.streams:
             done = torch.cuda.Event()
             done.record(s)

@@ -22,12 +22,17 @@ def tables(bundle):
     return InteractionTables(bundle)
 
 
-# (0, False) is the bench's default: the host-driven drain on the electron
-# pass's handle (whose ordering workspace the electron pass also uses), then
-# the photon push on the main stream after the drain; (16, True) drains with
-# the counts on the device and pushes the photons on the photon stream.
-@pytest.mark.parametrize("device_drain,push_photons", [(0, False), (16, True)])
-def test_concurrent_apply_matches_sequential(tables, device_drain, push_photons):
+# (4, False, positron early) is the bench's default: the positron pass starts
+# with the electron pass, the photon pass after the electron fast kernel, the
+# drain with its counts on the device (four generations), then the photon
+# push on the main stream after the drain; (0, False, none) is the
+# host-driven drain on the electron pass's handle (whose ordering workspace
+# the electron pass also uses) with both passes after the electron fast
+# kernel; (16, True, both early) drains on the device and pushes the photons
+# on the photon stream.
+@pytest.mark.parametrize("device_drain,push_photons,early", [(4, False, ("positron",)), (0, False, ()),
+                                                             (16, True, ("positron", "photon"))])
+def test_concurrent_apply_matches_sequential(tables, device_drain, push_photons, early):
     from rrea_feedback_warpx_amrex_geant4_amd.transport import (ConcurrentApply, Coupling, DeviceTile, DrainOutputs,
                                                                 PhotonTile, RreaParticleInteraction, StepOutputs)
     fields = wl.c2_fields()
@@ -36,7 +41,7 @@ def test_concurrent_apply_matches_sequential(tables, device_drain, push_photons)
     g = wl.c2_photons(n // 2, fields, dt_s=1e-9, seed=4, first_id=1 << 41)
     p = wl.c2_electrons(n // 4, fields, seed=3, first_id=1 << 40, charge_sign=1.0)
     print(f"case concurrent apply: {n} e-, {n // 2} gamma, {n // 4} e+, 3 steps of 1 ns, "
-          f"device_drain={device_drain}, push_photons={push_photons}", flush=True)
+          f"device_drain={device_drain}, push_photons={push_photons}, early={early}", flush=True)
 
     def make():
         tiles = (DeviceTile.from_host(e), PhotonTile.from_host(g), DeviceTile.from_host(p))
@@ -48,7 +53,7 @@ def test_concurrent_apply_matches_sequential(tables, device_drain, push_photons)
     seq_t, seq_o, seq_d = make()
     con_t, con_o, con_d = make()
     coupling = Coupling(fields, tables)
-    conc = ConcurrentApply(tables, fields)
+    conc = ConcurrentApply(tables, fields, early=early)
     apply = RreaParticleInteraction
     for step in range(3):
         cfg = StepConfig(rng_seed=81, step=step, dt_s=1e-9)
