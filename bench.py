@@ -463,7 +463,9 @@ def column_leg(args, tables):
     advances the TM Maxwell field and hands it back to the gather; every 10
     steps population control thins electrons and positrons above 1.25 x
     their targets.  dt = 1 ns on the 256 x 1024 column grid.  Times the whole
-    loop; particle-steps counted are the live particles at each step."""
+    loop; particle-steps counted are the live particles at each step.  Then
+    the configuration itself (``configuration_16``): 16 columns of E_z peaks
+    300-600 kV/m holding 5e7 live macroparticles, stepped as a batch."""
     import torch
 
     from rrea_feedback_warpx_amrex_geant4_amd.column import FeedbackColumn
@@ -486,27 +488,45 @@ def column_leg(args, tables):
     single = {"workload": "feedback column, 256x1024 grid, dt = 1 ns, injection + transport + drain + commit + "
                           "Maxwell feedback + population control", "steps": steps, "ms_per_step": el / steps * 1e3,
               "particle_steps_per_s": live / el, "final": obs}
-    # the feedback configuration's batch: 16 columns, E_z peaks 300-600 kV/m
+    # the feedback configuration's batch at its size: 16 columns (E_z peaks
+    # 300-600 kV/m) seeded with 5e7 live electrons in all, then injection
+    # (--column-inject per instance per step), the batched species steps,
+    # drain, commit, compaction, each instance's Maxwell feedback, and
+    # population control every 10 steps at 1.25 x 3,125,000 electrons per
+    # instance (the seeded level)
     from rrea_feedback_warpx_amrex_geant4_amd.column import BatchedColumn
     peaks = np.linspace(3.0e5, 6.0e5, 16)
-    bc = BatchedColumn(tables, peaks, inject_macros=args.column_inject // 8, targets=(1 << 16, 1 << 13),
-                       control_every=10, capacity=1 << 23)
-    bc.run(10)
+    per = args.column_seed // 16
+    bc = BatchedColumn(tables, peaks, inject_macros=args.column_inject, targets=(per, 1 << 18), control_every=10,
+                       capacity=16 * per + (1 << 23))
+    bc.seed_population(per)
+    bc.run(2)
     torch.cuda.synchronize()
+    live0 = bc.live()
     live = 0
+    n_b = args.column_batched_steps
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(n_b):
+        live += bc.live()
         bc.step()
-        live += sum(s.n for s in bc.stores)
     torch.cuda.synchronize()
     el_b = time.perf_counter() - t0
+    live1 = bc.live()
     obs_b = bc.observe()
+    bc.out[0].check_error()
     bc.close()
-    log(f"batched columns: 16 instances, {steps} steps in {el_b:.2f} s ({el_b / steps * 1e3:.1f} ms/step), "
-        f"{live / el_b:.3e} particle-steps/s; electrons per instance {obs_b['electrons']}")
-    single["batched_16"] = {"steps": steps, "ms_per_step": el_b / steps * 1e3, "particle_steps_per_s": live / el_b,
-                            "electrons_per_instance": obs_b["electrons"],
-                            "removed_electrons": obs_b["removed_electrons"]}
+    del bc
+    torch.cuda.empty_cache()
+    log(f"batched columns: 16 instances, live {live0} -> {live1}, {n_b} steps in {el_b:.2f} s "
+        f"({el_b / n_b * 1e3:.1f} ms/step), {live / el_b:.3e} particle-steps/s; electrons per instance "
+        f"{obs_b['electrons']}")
+    single["configuration_16"] = {
+        "workload": f"16 columns (E_z peaks 300-600 kV/m, 256x1024, 4-14 km), {16 * per} seeded electrons, "
+                    f"{args.column_inject} injected per instance per step, dt = 1 ns, batched e-/e+/gamma steps + "
+                    "drain + commit + compaction + per-instance Maxwell feedback, control every 10 steps",
+        "steps": n_b, "ms_per_step": el_b / n_b * 1e3, "particle_steps_per_s": live / el_b,
+        "live": [live0, live1], "electrons_per_instance": obs_b["electrons"],
+        "removed_electrons": obs_b["removed_electrons"]}
     return single
 
 
@@ -565,6 +585,10 @@ def main():
                          "(0: the host-driven drain); a FIFO still holding a generation after them is an error")
     ap.add_argument("--column-steps", type=int, default=100, help="timed steps of the feedback-column leg (0: skip)")
     ap.add_argument("--column-inject", type=int, default=2048, help="macroparticles injected per column step")
+    ap.add_argument("--column-seed", type=int, default=50_000_000,
+                    help="live electrons seeded over the 16 columns of the feedback configuration")
+    ap.add_argument("--column-batched-steps", type=int, default=10,
+                    help="timed steps of the 16-column feedback configuration")
     ap.add_argument("--population-particles", type=int, default=1 << 22,
                     help="electrons of the population-control leg (0: skip)")
     ap.add_argument("--graph-loop-steps", type=int, default=500,

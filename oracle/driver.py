@@ -96,6 +96,8 @@ class OracleSimulation:
         self.next_id = int(next_id) if next_id is not None else (int(ids.max()) + 1 if len(ids) else 1)
         self.step_index = cfg.step
         self.created = np.zeros(3, dtype=np.int64)
+        # the last step's net-chord deposit (index space), species steps and drain together
+        self.jr = self.jz = None
 
     def _push_photons(self, dt):
         g = self.sp[PHOTON]
@@ -117,15 +119,26 @@ class OracleSimulation:
             d[k] = np.where(ok, o[k], d[k])
         d["id"] = np.where(np.isin(st, kill), INVALID_ID, d["id"])
 
+    def _deposit(self, o):
+        if o.get("jr") is None:
+            return
+        if self.jr is None:
+            self.jr, self.jz = o["jr"].copy(), o["jz"].copy()
+        else:
+            self.jr += o["jr"]
+            self.jz += o["jz"]
+
     def step(self) -> dict:
         cfg = dataclasses.replace(self.cfg, step=self.step_index)
         self._push_photons(cfg.dt_s)
+        self.jr = self.jz = None
         staged = []
         e = self.sp[ELECTRON]
         if len(e["r"]):
             o = ob.charged_step("electron", self.t, self.fields, cfg, e, self.nthreads, sec_cap=max(1 << 12, len(e["r"])))
             if o["code"]:
                 raise ob.OracleError(f"electron step {o['code']}")
+            self._deposit(o)
             self._writeback(e, o, CHARGED_KILL)
             staged.append(o["secondaries"])
         g = self.sp[PHOTON]
@@ -144,12 +157,14 @@ class OracleSimulation:
                                 sec_cap=max(1 << 12, 4 * len(p["r"])))
             if o["code"]:
                 raise ob.OracleError(f"positron step {o['code']}")
+            self._deposit(o)
             self._writeback(p, o, CHARGED_KILL)
             staged.append(o["secondaries"])
         ini = np.concatenate(staged) if staged else np.zeros(0, dtype=ob.SECONDARY_DTYPE)
         d = ob.newborn_drain(self.t, self.fields, cfg, ini, cap=max(1 << 12, 16 * len(ini)))
         if d["code"]:
             raise ob.OracleError(f"drain {d['code']}")
+        self._deposit(d)
         # deleteInvalidParticles, then AddNParticles of each species' batch
         for s, arr in self.sp.items():
             keep = arr["id"] != INVALID_ID

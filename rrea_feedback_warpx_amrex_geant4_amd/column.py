@@ -60,6 +60,25 @@ def power_law_energies(rng, n: int, index: float, e_min: float, e_max: float) ->
     return (e_min ** a + u * (e_max ** a - e_min ** a)) ** (1.0 / a)
 
 
+def injection_tile(rng, n: int, weight: float, radius_m: float, z_m, first_id: int, index: float = 1.5,
+                   e_min_eV: float = 1.0e6, e_max_eV: float = 1.0e8) -> ParticleTile:
+    """One step's injected electrons at the column top: kinetic energies from
+    E^-index over [e_min, e_max], uniform over r < radius, in a 30-degree cone
+    about -z (the force on electrons), ids from first_id on.  Every draw comes
+    from ``rng`` in a fixed order, so a replica generator replays it."""
+    k = power_law_energies(rng, n, index, e_min_eV, e_max_eV)
+    g = 1.0 + k / ME_C2_EV
+    u = C_LIGHT * np.sqrt((g - 1.0) * (g + 1.0))
+    mu = -(1.0 - (1.0 - np.cos(np.deg2rad(30.0))) * rng.random(n))
+    phi = 2.0 * np.pi * rng.random(n)
+    st = np.sqrt(1.0 - mu * mu)
+    r = radius_m * np.sqrt(rng.random(n))
+    theta = 2.0 * np.pi * rng.random(n) - np.pi
+    return ParticleTile(r=r, z=np.broadcast_to(np.asarray(z_m, dtype=np.float64), (n,)).copy(), theta=theta,
+                        ux=u * st * np.cos(phi), uy=u * st * np.sin(phi), uz=u * mu, w=np.full(n, weight),
+                        id=np.arange(first_id, first_id + n, dtype=np.int64), cpu=np.zeros(n, dtype=np.int32))
+
+
 class FeedbackColumn:
     def __init__(self, tables: InteractionTables, fields: RzFields | None = None, dt_s: float = 1.0e-9,
                  rate_per_s: float = 1.0e14, inject_macros: int = 256, inject_radius_m: float = 500.0,
@@ -87,21 +106,10 @@ class FeedbackColumn:
         n = self.inject_macros
         if n <= 0:
             return
-        w = self.rate * self.cfg.dt_s / n
-        k = power_law_energies(self.rng, n, self.index, self.e_min, self.e_max)
-        g = 1.0 + k / ME_C2_EV
-        u = C_LIGHT * np.sqrt((g - 1.0) * (g + 1.0))
-        # a 30-degree cone about -z (the force on electrons)
-        mu = -(1.0 - (1.0 - np.cos(np.deg2rad(30.0))) * self.rng.random(n))
-        phi = 2.0 * np.pi * self.rng.random(n)
-        st = np.sqrt(1.0 - mu * mu)
-        r = self.inject_radius * np.sqrt(self.rng.random(n))
-        theta = 2.0 * np.pi * self.rng.random(n) - np.pi
         store = self.sim.electrons
         first = int(self.sim.next_id.item())
-        tile = ParticleTile(r=r, z=np.full(n, self.inject_z), theta=theta, ux=u * st * np.cos(phi),
-                            uy=u * st * np.sin(phi), uz=u * mu, w=np.full(n, w),
-                            id=np.arange(first, first + n, dtype=np.int64), cpu=np.zeros(n, dtype=np.int32))
+        tile = injection_tile(self.rng, n, self.rate * self.cfg.dt_s / n, self.inject_radius, self.inject_z, first,
+                              self.index, self.e_min, self.e_max)
         store.load(tile)
         self.sim.next_id += n
         self.sim.counts[ELECTRON] = store.n
@@ -227,25 +235,61 @@ class BatchedColumn:
         n = self.inject_macros
         if n <= 0:
             return
-        w = self.rate * self.cfg.dt_s / n
         m = n * self.n_inst
-        k = power_law_energies(self.rng, m, self.index, self.e_min, self.e_max)
-        g = 1.0 + k / ME_C2_EV
-        u = C_LIGHT * np.sqrt((g - 1.0) * (g + 1.0))
-        mu = -(1.0 - (1.0 - np.cos(np.deg2rad(30.0))) * self.rng.random(m))
-        phi = 2.0 * np.pi * self.rng.random(m)
-        st = np.sqrt(1.0 - mu * mu)
         first = int(self.next_id.item())
         z_top = np.repeat([f.z_hi - 50.0 for f in self.fields], n)
-        tile = ParticleTile(r=self.radius * np.sqrt(self.rng.random(m)), z=z_top,
-                            theta=2.0 * np.pi * self.rng.random(m) - np.pi, ux=u * st * np.cos(phi),
-                            uy=u * st * np.sin(phi), uz=u * mu, w=np.full(m, w),
-                            id=np.arange(first, first + m, dtype=np.int64), cpu=np.zeros(m, dtype=np.int32))
+        tile = injection_tile(self.rng, m, self.rate * self.cfg.dt_s / n, self.radius, z_top, first, self.index,
+                              self.e_min, self.e_max)
         e = self.stores[ELECTRON]
         e.load(tile, np.repeat(np.arange(self.n_inst, dtype=np.int32), n))
         self.next_id += m
         self.counts[ELECTRON] = e.n
         self.injected += m
+
+    def seed_population(self, per_instance: int, seed: int = 7) -> None:
+        """A start population at the configuration's scale: ``per_instance``
+        electrons in every instance, drawn on the device (torch's generator)
+        from the injection's spectrum, cone and radius, spread uniformly over
+        the column's height, with the injection's macro weight; appended in
+        instance order behind the store's count, ids from the batch's
+        counter."""
+        import math
+        e = self.stores[ELECTRON]
+        m = per_instance * self.n_inst
+        if e.n + m > e.capacity:
+            raise ValueError("particle store capacity exceeded")
+        g = torch.Generator(device="cuda").manual_seed(seed)
+
+        def rnd():
+            return torch.rand(m, generator=g, dtype=torch.float64, device="cuda")
+        a = 1.0 - self.index
+        k = (self.e_min ** a + rnd() * (self.e_max ** a - self.e_min ** a)) ** (1.0 / a)
+        gam = 1.0 + k / ME_C2_EV
+        u = C_LIGHT * torch.sqrt((gam - 1.0) * (gam + 1.0))
+        mu = -(1.0 - (1.0 - math.cos(math.radians(30.0))) * rnd())
+        phi = 2.0 * math.pi * rnd()
+        st = torch.sqrt(1.0 - mu * mu)
+        f0 = self.fields[0]
+        sl = slice(e.n, e.n + m)
+        e.r[sl] = self.radius * torch.sqrt(rnd())
+        e.theta[sl] = 2.0 * math.pi * rnd() - math.pi
+        e.z[sl] = f0.z_lo + (f0.z_hi - f0.z_lo) * (1.0 - 1.0e-9) * rnd()
+        e.ux[sl] = u * st * torch.cos(phi)
+        e.uy[sl] = u * st * torch.sin(phi)
+        e.uz[sl] = u * mu
+        e.w[sl] = self.rate * self.cfg.dt_s / max(self.inject_macros, 1)
+        first = int(self.next_id.item())
+        e.id[sl] = torch.arange(first, first + m, dtype=torch.int64, device="cuda")
+        e.cpu[sl] = 0
+        e.inst[sl] = torch.arange(self.n_inst, dtype=torch.int32, device="cuda").repeat_interleave(per_instance)
+        e.n += m
+        self.next_id += m
+        self.counts[ELECTRON] = e.n
+
+    def live(self) -> int:
+        """Live macroparticles of all species and instances."""
+        from . import _lib
+        return sum(int((s.id[: s.n] != _lib.INVALID_ID).sum().item()) for s in self.stores)
 
     def step(self):
         from . import _lib
