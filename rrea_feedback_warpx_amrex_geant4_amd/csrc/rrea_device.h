@@ -943,9 +943,15 @@ __device__ __forceinline__ double max_uniform_slice(Interval const& iv, Path con
 // ---------------------------------------------------------------------------
 struct FieldRange { double work, ex, ey, ez, bx, by, bz, length; };
 
-// path_field_work_and_average OV/RreaPathFieldWork.H:57-150 (GL2 per piece)
+// path_field_work_and_average OV/RreaPathFieldWork.H:57-150 (GL2 per piece).
+// With a `hold` pointer (kHold) the first node's six products wait in LDS
+// (hold[c * kHoldStride], zero again on return) while the second node is
+// gathered, so a caller short of registers does not spill them; the values
+// and the arithmetic are the same.
+constexpr int kHoldStride = 256;
+template <bool kHold = false>
 __device__ FieldRange path_field_work(DevFields const& F, Path const& p, Domain const& d, double q, double b, double e,
-                                      int32_t& err)
+                                      int32_t& err, double* hold = nullptr)
 {
     FieldRange res{0, 0, 0, 0, 0, 0, 0, 0};
     if (!(p.full > 0.0)) return res;
@@ -973,6 +979,18 @@ __device__ FieldRange path_field_work(DevFields const& F, Path const& p, Domain 
         if (!finite(wk) || !finite(er) || !finite(ez) || !finite(bt)) { if (!err) err = kErrInvalid; }
         s[node][0] = wk; s[node][1] = er * rc; s[node][2] = er * rs; s[node][3] = ez;
         s[node][4] = -bt * rs; s[node][5] = bt * rc; s[node][6] = 0.0;
+        if constexpr (kHold) {
+            if (node == 0) {
+                for (int c = 0; c < 6; ++c) hold[c * kHoldStride] = s[0][c];
+            }
+        }
+    }
+    if constexpr (kHold) {
+        // relaxed LDS loads, which the compiler does not fold into the stores
+        for (int c = 0; c < 6; ++c) {
+            s[0][c] = __hip_atomic_load(hold + c * kHoldStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            hold[c * kHoldStride] = 0.0;
+        }
     }
     double integ[7];
     for (int c = 0; c < 7; ++c) integ[c] = half * (s[0][c] + s[1][c]);

@@ -2560,7 +2560,10 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             Interval const iv = build_interval(F, path, D, E.density_ratio, err);
             if (!iv.valid || err) break;
             double const span = dmax(path.end_f, 1.0e-6);
-            FieldRange const fr = path_field_work(F, path, D, S::kCharge, 0.0, 1.0, err);
+            // batched: the first GL node's products wait in the (still empty)
+            // ledger columns, which keeps that kernel from spilling them
+            static_assert(kHoldStride == kBlock && Col::kCols >= 6, "the hold columns are the ledger's");
+            FieldRange const fr = path_field_work<kBatched>(F, path, D, S::kCharge, 0.0, 1.0, err, lane_col);
             if (err) break;
             double const field_work = fr.work / span;
             double const chord_dp = density_path(iv, path, 0.0, path.end_f);
