@@ -228,14 +228,36 @@ def batched_leg(args, tables):
     def live(av):
         return sum(av.live())
 
-    # host-driven batched steps
+    # host-driven batched steps; the batch's particle sort (instance, cell,
+    # energy band) tuned on this workload first: its interval and band width
+    tuned = None
+    every, bits = 0, -1
+    if args.batched_autotune:
+        # on a copy of the batch, so that the timed legs below all start from
+        # the seeded state
+        from rrea_feedback_warpx_amrex_geant4_amd.autotune import tune
+        tav = BatchedAvalanche(tables, fields, tiles, seeds, cfg, capacity=caps, secondary_capacity=sec_cap)
+
+        def tune_step(s, sort_now, energy_bits):
+            if sort_now:
+                tav.sort(energy_bits)
+            tav.step()
+        choice = tune(tune_step, sort_every=(0, 10, 20, 50), energy_bits=(0, 3, 6))
+        every, bits = choice.sort_every, choice.energy_bits
+        tuned = choice.as_dict()
+        tuned["block"] = 256
+        del tav
+        torch.cuda.empty_cache()
+        log(f"batched autotune: sort every {every} steps, {bits} band bits ({choice.ms_per_step:.3f} ms/step)")
     bav = BatchedAvalanche(tables, fields, tiles, seeds, cfg, capacity=caps, secondary_capacity=sec_cap)
     for _ in range(args.batched_warmup):
         bav.step()
     torch.cuda.synchronize()
     b0 = live(bav)
     t0 = time.perf_counter()
-    for _ in range(args.batched_steps):
+    for k in range(args.batched_steps):
+        if every > 0 and k % every == 0:
+            bav.sort(bits)
         bav.step()
     torch.cuda.synchronize()
     el_b = time.perf_counter() - t0
@@ -256,7 +278,12 @@ def batched_leg(args, tables):
     graph.check_error()
     gav.sync_counts()
     g0 = live(gav)
+    # the sort (outside the graph: hipcub clears its scratch with memsets)
+    # before the launches its interval falls on, counted in the time
+    sort_launches = max(1, -(-every // args.batched_steps)) if every > 0 else 0
     t0 = time.perf_counter()
+    if sort_launches:
+        gav.sort(bits)
     graph.launch(args.batched_steps)
     torch.cuda.synchronize()
     el_g = time.perf_counter() - t0
@@ -270,6 +297,8 @@ def batched_leg(args, tables):
     launches = max(1, args.graph_loop_steps // args.batched_steps)
     t0 = time.perf_counter()
     for i in range(1, launches):
+        if sort_launches and i % sort_launches == 0:
+            gav.sort(bits)
         graph.launch((i + 1) * args.batched_steps)
     torch.cuda.synchronize()
     el_loop = el_g + time.perf_counter() - t0
@@ -320,6 +349,7 @@ def batched_leg(args, tables):
                     "dt=1 ps; per step the photon push, the electron, photon and positron steps, the newborn "
                     "drain and the commit of its survivors (the avalanche), J per instance",
         "instances": n_inst, "seed_electrons": n, "steps": args.batched_steps, "warmup": args.batched_warmup,
+        "autotune": tuned, "sort_every": every, "sort_energy_bits": bits,
         "batched": {"value": rate_b, "ms_per_step": el_b / args.batched_steps * 1e3, "live": [b0, b1]},
         "plain_per_instance": {"value": rate_p, "ms_per_step": el_p / args.batched_steps * 1e3, "live": [p0, p1]},
         "graph_captured": {"value": rate_g, "ms_per_step": el_g / args.batched_steps * 1e3,
@@ -562,6 +592,8 @@ def main():
     ap.add_argument("--particles", type=int, default=None, help="total macroparticles (default 2^24)")
     ap.add_argument("--sort-every", type=int, default=25,
                     help="reorder the tiles by cell every this many steps (0: never), inside the timed region")
+    ap.add_argument("--batched-autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="tune the batched leg's particle-sort interval and band bits first")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="pick the sort interval and energy-band bits by timing candidates before the warm-up")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -534,6 +534,19 @@ rrea_status rrea_transport_step_batched(rrea_tables_t tables, rrea_batch_t batch
  * instance (rrea_secondary.instance); io->jr/jz/rho are not used. */
 rrea_status rrea_newborn_drain_batched(rrea_tables_t tables, rrea_batch_t batch, const rrea_step_config* config,
                                        const rrea_drain_io* io, rrea_stream_t stream);
+/* rrea_tile_sort_banded for a batched tile: particles ordered by instance,
+ * then by the cell of their captured position on that instance's own mesh
+ * (r fastest), then by energy band; invalid ids and instances out of range
+ * go last.  Every distinct SoA array of the tile and the instance array
+ * (device int32, written) are permuted together.  Keeps a batched
+ * wavefront's lanes in one instance (the fast path's uniform environment)
+ * and a workgroup's chords in a few cells (its LDS deposit window) as an
+ * avalanche appends newborns of every instance behind the seeds.  No
+ * counterpart in the overlay (WarpX's particle sort, per instance);
+ * energy_bits as rrea_tile_sort_banded, within what the instance and cell
+ * keys leave of 31 bits. */
+rrea_status rrea_batch_tile_sort(rrea_batch_t batch, const rrea_tile* tile, int32_t photon, int32_t* instance,
+                                 int32_t energy_bits, rrea_stream_t stream);
 
 /* Graph-captured batched steps (no counterpart in the overlay, which steps one
  * simulation per process and returns to the host between phases).  One graph

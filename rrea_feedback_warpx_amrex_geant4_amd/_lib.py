@@ -198,6 +198,7 @@ SIGNATURES = [
     ("rrea_batch_create", C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.POINTER(C.c_void_p)]),
     ("rrea_batch_destroy", C.c_int32, [C.c_void_p]),
+    ("rrea_batch_tile_sort", C.c_int32, [C.c_void_p, C.POINTER(Tile), C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
     ("rrea_batch_workspace", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("rrea_transport_step_batched", C.c_int32,
      [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(Tile), C.c_void_p, C.POINTER(StepConfigC), C.POINTER(StepOut),

@@ -3866,6 +3866,36 @@ __global__ void tile_key_kernel(int64_t n, const double* __restrict__ r, const d
     keys[p] = (cell << ebits) | eb;
 }
 
+// rrea_batch_tile_sort's key: the particle's instance, then its cell on
+// that instance's own mesh, then the energy band (as tile_key_kernel);
+// invalidated particles last.  The instances' meshes come from their
+// environments.
+__global__ void batch_tile_key_kernel(int64_t n, const double* __restrict__ r, const double* __restrict__ z,
+                                      const double* __restrict__ ux, const double* __restrict__ uy,
+                                      const double* __restrict__ uz, const int64_t* __restrict__ id,
+                                      const int32_t* __restrict__ inst, int32_t photon,
+                                      const Env* __restrict__ envs, int32_t n_inst, int32_t cbits, int32_t ebits,
+                                      uint32_t* __restrict__ keys, int32_t* __restrict__ idx)
+{
+    int64_t const p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    idx[p] = static_cast<int32_t>(p);
+    int32_t const k = inst[p];
+    if (id[p] == RREA_INVALID_ID || k < 0 || k >= n_inst) { keys[p] = 0xffffffffU; return; }
+    DevFields const& F = envs[k].F;
+    double const fr = (r[p] - F.r_lo) / F.dr, fz = (z[p] - F.z_lo) / F.dz;
+    int32_t const i = finite(fr) ? static_cast<int32_t>(dmin(dmax(fr, 0.0), static_cast<double>(F.nr - 1))) : F.nr - 1;
+    int32_t const j = finite(fz) ? static_cast<int32_t>(dmin(dmax(fz, 0.0), static_cast<double>(F.nz - 1))) : F.nz - 1;
+    double const u2 = ux[p] * ux[p] + uy[p] * uy[p] + uz[p] * uz[p];
+    double const e = photon ? sqrt(u2) : kinetic_from_u(ux[p], uy[p], uz[p]);
+    double const band = finite(e) && e > 1.0e3 ? 4.0 * log2(e * 1.0e-3) : 0.0;
+    uint32_t const emax = (1U << ebits) - 1U;
+    uint32_t const q = static_cast<uint32_t>(dmin(band, 63.0)) >> (6 - ebits);
+    uint32_t const eb = ebits > 0 ? (q < emax ? q : emax) : 0U;
+    uint32_t const cell = static_cast<uint32_t>(i) + static_cast<uint32_t>(F.nr) * static_cast<uint32_t>(j);
+    keys[p] = (((static_cast<uint32_t>(k) << cbits) | cell) << ebits) | eb;
+}
+
 // rrea_tile_compact's key: live particles (0) before invalidated ones (1),
 // and the live count (wave-aggregated).
 __global__ void live_key_kernel(int64_t n, const int64_t* __restrict__ id, uint32_t* __restrict__ keys,
@@ -6116,6 +6146,52 @@ rrea_status rrea_tile_sort_banded(rrea_fields_t F, const rrea_tile* tile, int32_
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(F->d_tsort_tmp, bytes, F->d_tkeys, F->d_tkeys_out, F->d_tidx,
                                                F->d_tidx_out, static_cast<int>(n), 0, cbits + ebits, s));
     return permute_tile(F, tile, photon, nullptr, s);
+}
+
+rrea_status rrea_batch_tile_sort(rrea_batch_t B, const rrea_tile* tile, int32_t photon, int32_t* instance,
+                                 int32_t energy_bits, rrea_stream_t stream)
+{
+    if (!B || !tile || tile->n < 0 || energy_bits < -1 || energy_bits > 6) return RREA_ERR_ARGUMENT;
+    int64_t const n = tile->n;
+    if (n == 0) return RREA_OK;
+    if (n > INT32_MAX || !instance || !tile->prev_r || !tile->prev_z || !tile->prev_theta || !tile->w || !tile->id
+        || !tile->r || !tile->z || !tile->theta || !tile->ux || !tile->uy || !tile->uz
+        || (!photon && (!tile->prev_ux || !tile->prev_uy || !tile->prev_uz))) {
+        return RREA_ERR_ARGUMENT;
+    }
+    int32_t const n_inst = static_cast<int32_t>(B->fields.size());
+    int32_t ibits = 0;
+    while ((int64_t{1} << ibits) < n_inst) ++ibits;
+    int32_t cbits = 0;
+    for (rrea_fields_t f : B->fields) {
+        uint64_t const cells = static_cast<uint64_t>(f->geom.nr) * static_cast<uint64_t>(f->geom.nz);
+        while ((uint64_t{1} << cbits) < cells) ++cbits;
+    }
+    // the invalid key (all ones) must stay above every live key
+    if (ibits + cbits > 31) return RREA_ERR_ARGUMENT;
+    int32_t const emax = 31 - ibits - cbits < 6 ? 31 - ibits - cbits : 6;
+    int32_t const ebits = energy_bits < 0 || energy_bits > emax ? emax : energy_bits;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    rrea_status const es = upload_envs(B, nullptr, s);
+    if (es != RREA_OK) return es;
+    rrea_fields_t const W = B->work;
+    {
+        rrea_status const ws = tile_sort_workspace(W, n);
+        if (ws != RREA_OK) return ws;
+    }
+    // charged tiles key on the captured state; photons on their current point
+    const double* ku_x = photon ? tile->ux : tile->prev_ux;
+    const double* ku_y = photon ? tile->uy : tile->prev_uy;
+    const double* ku_z = photon ? tile->uz : tile->prev_uz;
+    unsigned const nb = static_cast<unsigned>(blocks_for(n, 256));
+    batch_tile_key_kernel<<<dim3(nb), dim3(256), 0, s>>>(n, tile->prev_r, tile->prev_z, ku_x, ku_y, ku_z, tile->id,
+                                                         instance, photon ? 1 : 0, static_cast<const Env*>(B->d_envs),
+                                                         n_inst, cbits, ebits, W->d_tkeys, W->d_tidx);
+    HIP_TRY(hipGetLastError());
+    size_t bytes = W->tsort_tmp_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(W->d_tsort_tmp, bytes, W->d_tkeys, W->d_tkeys_out, W->d_tidx,
+                                               W->d_tidx_out, static_cast<int>(n), 0, 32, s));
+    return permute_tile(W, tile, photon, instance, s);
 }
 
 rrea_status rrea_tile_compact(rrea_fields_t F, const rrea_tile* tile, int32_t photon, int32_t* instance,

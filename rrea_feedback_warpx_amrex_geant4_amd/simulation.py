@@ -339,6 +339,18 @@ class BatchedAvalanche:
                           max_generations=max_generations, push_photons=True,
                           survivor_capacity=cap, counts=self.counts, next_id=self.next_id)
 
+    def sort(self, energy_bits: int = -1, stream: int | None = None) -> None:
+        """WarpX's particle sort for the batch (rrea_batch_tile_sort): each
+        store's rows reordered by instance, cell on the instance's mesh and
+        energy band, invalidated particles behind the live ones (the
+        stores keep their lengths).  Reads the lengths back first."""
+        self.sync_counts()
+        for st in self.stores:
+            if st.n == 0:
+                continue
+            tile = st.photon_tile() if st.photon else st.tile()
+            self.batch.sort(tile, st.inst[: st.n], st.photon, energy_bits, stream)
+
     def sync_counts(self) -> None:
         """The stores' lengths from the device (after graph launches)."""
         for s, c in zip(self.stores, self.counts.cpu().numpy()):

@@ -473,6 +473,19 @@ class Batch:
         if check:
             out.check_error()
 
+    def sort(self, tile, inst: torch.Tensor, photon: bool = False, energy_bits: int = -1,
+             stream: int | None = None) -> None:
+        """rrea_batch_tile_sort: the batched tile (a DeviceTile, or a
+        PhotonTile with ``photon``) and its instance array reordered by
+        instance, cell on the instance's mesh and energy band, in place."""
+        if inst.dtype != torch.int32 or inst.numel() != len(tile):
+            raise ValueError("inst must be int32, one per particle")
+        t = tile.struct()
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().rrea_batch_tile_sort(self.h, C.byref(t), 1 if photon else 0, inst.data_ptr(),
+                                                   int(energy_bits), stream), "rrea_batch_tile_sort")
+
     def drain(self, tables: InteractionTables, cfg: StepConfig, staged, out: "DrainOutputs",
               stream: int | None = None, check: bool = True):
         """The newborn drain of a batched step: every record advances in its instance."""
