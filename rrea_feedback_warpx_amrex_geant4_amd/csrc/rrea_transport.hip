@@ -5151,12 +5151,21 @@ rrea_status species_prepare(int species, rrea_tables_t T, rrea_fields_t F, const
     return RREA_OK;
 }
 
-// The fold kernel's grid over `rows` partial rows: four passes of sixteen
-// rows per workgroup, at most 512 workgroups.
+// The hand-over kernels run at one wave per SIMD (they hold ~290 registers),
+// so 256 workgroups of four waves are all the chip runs of them at once;
+// they stride over the hand-over list, and a larger grid would only queue
+// empty workgroups behind the busy ones.
+constexpr int64_t kSlowBlocks = 256;
+
+// The fold kernel's grid over `rows` partial rows: sixteen passes of sixteen
+// rows per workgroup, at most 64 workgroups.  Each fold workgroup books its
+// column totals with one atomic per column, all workgroups on the same
+// addresses, so few workgroups (each striding over many rows) keep that
+// contention low.
 int64_t fold_blocks(int64_t rows)
 {
-    int64_t const b = blocks_for(rows, 4 * (kBlock / kPartialStride));
-    return b < 512 ? b : 512;
+    int64_t const b = blocks_for(rows, 16 * (kBlock / kPartialStride));
+    return b < 64 ? b : 64;
 }
 
 // A species step's launches on `s`, reading the argument block at d_args.
@@ -5224,7 +5233,7 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
         });
         if (st != RREA_OK) return st;
         if (signal) HIP_TRY(hipEventRecord(W->fast_event, s));
-        int64_t const slow_blocks = nb < 2048 ? nb : 2048;
+        int64_t const slow_blocks = nb < kSlowBlocks ? nb : kSlowBlocks;
         st = timed(5, [&] {
             photon_fold_kernel<<<dim3(static_cast<unsigned>(fold_blocks(nb))), dim3(kBlock), 0, s>>>(
                 d_args, K.partials, nb);
@@ -5258,7 +5267,7 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
         });
         if (st != RREA_OK) return st;
         if (signal) HIP_TRY(hipEventRecord(W->fast_event, s));
-        int64_t const slow_blocks = nb < 2048 ? nb : 2048;
+        int64_t const slow_blocks = nb < kSlowBlocks ? nb : kSlowBlocks;
         st = timed(kid + 1, [&] {
             if (positron) {
                 charged_fold_kernel<true><<<dim3(static_cast<unsigned>(fold_blocks(nb))), dim3(kBlock), 0, s>>>(
