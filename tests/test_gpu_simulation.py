@@ -242,3 +242,45 @@ def test_ensemble_observables(tables, otables):
     lam = [r["efold_m"] for r in rec["gpu"]]
     print(f"ensemble runaway e-folding length over 6 m: {np.mean(lam):.1f} m (per seed {np.round(lam, 1).tolist()})",
           flush=True)
+
+
+def test_avalanche_efolding_length_matches_coleman_dwyer(tables):
+    """A physical anchor for the avalanche: the first configuration's physics
+    (1000 e- at 7.2 MeV along the force in E = 400 kV/m of air at
+    1.225 kg/m^3, 10 keV cutoff) run on the device for 400 ns (4000 steps of
+    100 ps, 120 m of flight).  Past the seeds' transient, the runaway count
+    (K > 1 MeV) grows as exp(z / lambda); the fit of ln N over the second half
+    gives lambda within 20 % of the Coleman-Dwyer form
+    lambda = 7.3 MeV / (e E - 276 keV/m (rho / 1.225 kg/m^3)) = 58.9 m, and
+    the mean runaway energy within 20 % of its 7.3 MeV (Coleman & Dwyer,
+    Geophys. Res. Lett. 33, L11810 (2006)).  The oracle's loop gives
+    62.9 m and 8.0 MeV on this run."""
+    from rrea_feedback_warpx_amrex_geant4_amd import _lib
+    from rrea_feedback_warpx_amrex_geant4_amd.simulation import Simulation
+    fields, e = wl.c1_plain(1000)
+    steps, dt, every = 4000, 1e-10, 100
+    print(f"case avalanche e-folding: 1000 e-, {steps} steps of {dt:.0e} s", flush=True)
+    sim = Simulation(tables, fields, StepConfig(rng_seed=20240623, dt_s=dt), e, capacity=1 << 16,
+                     secondary_capacity=1 << 14)
+    c, me = 299792458.0, 510998.95
+    z, n_run, k_mean = [], [], []
+    for k in range(steps // every):
+        sim.run(every)
+        st = sim.electrons
+        live = st.id[: st.n] != _lib.INVALID_ID
+        u2 = (st.ux[: st.n] ** 2 + st.uy[: st.n] ** 2 + st.uz[: st.n] ** 2)[live]
+        kin = (torch.sqrt(1.0 + u2 / c ** 2) - 1.0) * me
+        run = kin > 1.0e6
+        z.append((k + 1) * every * dt * c)
+        n_run.append(int(run.sum()))
+        k_mean.append(float(kin[run].mean()))
+    sim.out[0].check_error()
+    z, n_run = np.array(z), np.array(n_run, dtype=np.float64)
+    h = len(z) // 2
+    lam = 1.0 / np.polyfit(z[h:], np.log(n_run[h:]), 1)[0]
+    cd = 7.3e6 / (4.0e5 - 2.76e5 * (1.225 / 1.225))
+    print(f"avalanche: runaway count {int(n_run[0])} -> {int(n_run[-1])} over {z[-1]:.0f} m, e-folding length "
+          f"{lam:.1f} m (Coleman-Dwyer {cd:.1f} m), mean runaway energy {np.mean(k_mean[h:]) / 1e6:.2f} MeV", flush=True)
+    assert abs(lam - cd) <= 0.2 * cd, (lam, cd)
+    assert abs(np.mean(k_mean[h:]) - 7.3e6) <= 0.2 * 7.3e6
+    assert n_run[-1] > 3.0 * 1000
