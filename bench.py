@@ -231,7 +231,7 @@ def batched_leg(args, tables):
     # host-driven batched steps; the batch's particle sort (instance, cell,
     # energy band) tuned on this workload first: its interval and band width
     tuned = None
-    every, bits = 0, -1
+    every, bits = args.batched_sort_every, -1
     if args.batched_autotune:
         # on a copy of the batch, so that the timed legs below all start from
         # the seeded state
@@ -242,7 +242,10 @@ def batched_leg(args, tables):
             if sort_now:
                 tav.sort(energy_bits)
             tav.step()
-        choice = tune(tune_step, sort_every=(0, 10, 20, 50), energy_bits=(0, 3, 6))
+        # intervals up to a fifth of the graph loop; the candidates are timed
+        # one after another on the evolving avalanche, the never-sort one
+        # first, after 100 steps of warm-up
+        choice = tune(tune_step, sort_every=(0, 50, 100), energy_bits=(0, 6), unsorted_steps=100, warmup=100)
         every, bits = choice.sort_every, choice.energy_bits
         tuned = choice.as_dict()
         tuned["block"] = 256
@@ -594,6 +597,8 @@ def main():
                     help="reorder the tiles by cell every this many steps (0: never), inside the timed region")
     ap.add_argument("--batched-autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="tune the batched leg's particle-sort interval and band bits first")
+    ap.add_argument("--batched-sort-every", type=int, default=0,
+                    help="the batched leg's particle-sort interval without the autotuner (0: never)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="pick the sort interval and energy-band bits by timing candidates before the warm-up")
     ap.add_argument("--no-cpu-baseline", action="store_true")
