@@ -318,6 +318,10 @@ def batched_leg(args, tables):
     torch.cuda.empty_cache()
 
     # the plain per-instance path: one Simulation per instance, stepped one after another
+    if not args.batched_plain:
+        return {"batched": {"value": rate_b, "ms_per_step": el_b / args.batched_steps * 1e3, "live": [b0, b1]},
+                "graph_captured": {"value": rate_g, "ms_per_step": el_g / args.batched_steps * 1e3},
+                "plain_per_instance": None}
     _, tiles, _ = wl.c3_batch(n_inst, seed=3, scale=args.batched_scale)
     sims = [Simulation(tables, f, StepConfig(rng_seed=sd, dt_s=dt), t, capacity=int(len(t.r)) + 4096,
                        secondary_capacity=max(1 << 10, len(t.r) >> 6), compact_every=0)
@@ -597,6 +601,8 @@ def main():
                     help="reorder the tiles by cell every this many steps (0: never), inside the timed region")
     ap.add_argument("--batched-autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="tune the batched leg's particle-sort interval and band bits first")
+    ap.add_argument("--batched-plain", action=argparse.BooleanOptionalAction, default=True,
+                    help="time the plain per-instance path on the batched inputs (the batched/plain ratio)")
     ap.add_argument("--batched-sort-every", type=int, default=0,
                     help="the batched leg's particle-sort interval without the autotuner (0: never)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
