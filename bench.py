@@ -232,6 +232,7 @@ def batched_leg(args, tables):
     # energy band) tuned on this workload first: its interval and band width
     tuned = None
     every, bits = args.batched_sort_every, -1
+    block = 256
     if args.batched_autotune:
         # on a copy of the batch, so that the timed legs below all start from
         # the seeded state
@@ -248,11 +249,16 @@ def batched_leg(args, tables):
         choice = tune(tune_step, sort_every=(0, 50, 100), energy_bits=(0, 6), unsorted_steps=100, warmup=100)
         every, bits = choice.sort_every, choice.energy_bits
         tuned = choice.as_dict()
-        tuned["block"] = 256
+        from rrea_feedback_warpx_amrex_geant4_amd.autotune import tune_block
+        bchoice = tune_block(lambda s: tav.step(), tav.batch.set_block, steps=10)
+        block = bchoice.block
+        tuned["block"] = bchoice.as_dict()
         del tav
         torch.cuda.empty_cache()
-        log(f"batched autotune: sort every {every} steps, {bits} band bits ({choice.ms_per_step:.3f} ms/step)")
+        log(f"batched autotune: sort every {every} steps, {bits} band bits ({choice.ms_per_step:.3f} ms/step), "
+            f"fast-kernel workgroups of {block} lanes ({', '.join(f'{b}: {t:.3f}' for b, t in bchoice.trials)})")
     bav = BatchedAvalanche(tables, fields, tiles, seeds, cfg, capacity=caps, secondary_capacity=sec_cap)
+    bav.batch.set_block(block)
     for _ in range(args.batched_warmup):
         bav.step()
     torch.cuda.synchronize()
@@ -272,6 +278,7 @@ def batched_leg(args, tables):
     # the same steps captured in one graph: its first launch (graph upload
     # included) is the warm-up, untimed
     gav = BatchedAvalanche(tables, fields, tiles, seeds, cfg, capacity=caps, secondary_capacity=sec_cap)
+    gav.batch.set_block(block)
     del tiles
     t0 = time.perf_counter()
     graph = gav.graph(args.batched_steps, max_generations=args.graph_generations)
@@ -356,7 +363,7 @@ def batched_leg(args, tables):
                     "dt=1 ps; per step the photon push, the electron, photon and positron steps, the newborn "
                     "drain and the commit of its survivors (the avalanche), J per instance",
         "instances": n_inst, "seed_electrons": n, "steps": args.batched_steps, "warmup": args.batched_warmup,
-        "autotune": tuned, "sort_every": every, "sort_energy_bits": bits,
+        "autotune": tuned, "sort_every": every, "sort_energy_bits": bits, "block": block,
         "batched": {"value": rate_b, "ms_per_step": el_b / args.batched_steps * 1e3, "live": [b0, b1]},
         "plain_per_instance": {"value": rate_p, "ms_per_step": el_p / args.batched_steps * 1e3, "live": [p0, p1]},
         "graph_captured": {"value": rate_g, "ms_per_step": el_g / args.batched_steps * 1e3,
@@ -597,6 +604,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--particles", type=int, default=None, help="total macroparticles (default 2^24)")
+    ap.add_argument("--block", type=int, default=0, choices=[0, 128, 256, 512],
+                    help="fast kernels' workgroup size without the autotuner (0: 256, or the autotuner's choice)")
     ap.add_argument("--sort-every", type=int, default=25,
                     help="reorder the tiles by cell every this many steps (0: never), inside the timed region")
     ap.add_argument("--batched-autotune", action=argparse.BooleanOptionalAction, default=True,
@@ -711,7 +720,7 @@ def main():
     # the handle each species' kernels are timed on
     h_e, h_g, h_p = (coupling.h,) * 3 if conc is None else tuple(c.h for c in conc.couplings)
 
-    knobs = {"sort_every": args.sort_every, "energy_bits": -1, "first_step": 0}
+    knobs = {"sort_every": args.sort_every, "energy_bits": -1, "first_step": 0, "block": args.block or 256}
 
     def one_step(s, sort_now, energy_bits):
         if sort_now:
@@ -750,10 +759,25 @@ def main():
         log("autotune: tile-sort interval x energy-band bits")
         choice = tune(one_step, first_step=0)
         knobs["sort_every"], knobs["energy_bits"] = choice.sort_every, choice.energy_bits
-        knobs["first_step"] = choice.steps_run
         tuned = choice.as_dict()
         log(f"autotune: sort every {choice.sort_every} steps, {choice.energy_bits} band bits "
             f"({choice.ms_per_step:.3f} ms/step)")
+        # then the fast kernels' workgroup size, on tiles freshly sorted with
+        # the chosen band width
+        from rrea_feedback_warpx_amrex_geant4_amd.autotune import tune_block
+        s0 = choice.steps_run
+        one_step(s0, True, choice.energy_bits)
+        bchoice = tune_block(lambda s: one_step(s, False, choice.energy_bits),
+                             conc.set_block if conc is not None else coupling.set_block, first_step=s0 + 1)
+        knobs["first_step"] = s0 + 1 + bchoice.steps_run
+        knobs["block"] = bchoice.block
+        tuned["block"] = bchoice.as_dict()
+        log(f"autotune: fast-kernel workgroups of {bchoice.block} lanes "
+            f"({', '.join(f'{b}: {t:.3f}' for b, t in bchoice.trials)} ms/step)")
+    # the workgroup size on every handle the step and the isolated timing use
+    coupling.set_block(knobs["block"])
+    if conc is not None:
+        conc.set_block(knobs["block"])
     log(f"warmup {args.warmup}")
     every = knobs["sort_every"]
     for s in range(args.warmup):
@@ -856,6 +880,7 @@ def main():
                    "drain": (f"device-driven, {args.device_drain} generations" if args.device_drain > 0 and not args.sequential
                              else "host-driven"),
                    "tile_sort_every": knobs["sort_every"], "tile_sort_energy_bits": knobs["energy_bits"],
+                   "fast_kernel_block": knobs["block"],
                    "tile_sorts_in_timed_region": sorts_timed,
                    "autotune": tuned,
                    "kernel_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in kms.items()}},

@@ -635,6 +635,15 @@ rrea_status rrea_newborn_drain_staged(rrea_tables_t tables, rrea_fields_t fields
  * the step call that records it. */
 rrea_status rrea_fields_signal_fast_path(rrea_fields_t fields, int32_t enable);
 rrea_status rrea_stream_wait_fast_path(rrea_fields_t fields, rrea_stream_t stream);
+/* Tuning knob (no counterpart in the overlay): the workgroup size of the
+ * fast-path kernels of the species steps on this handle -- 128, 256 (the
+ * default) or 512 lanes.  Each workgroup keeps an LDS deposit window of 8
+ * faces per lane and folds its tallies into one partial row, so the size
+ * trades window flushes and fold rows against scheduling granularity.
+ * Results do not depend on it (the J grid and tallies to summation order).
+ * For a batch, set it on the batch's workspace handle
+ * (rrea_batch_workspace). */
+rrea_status rrea_fields_set_block(rrea_fields_t fields, int32_t threads);
 
 /* ------------------------------------------------------------------------
  * The field side of the feedback loop: the step's net-chord deposit as

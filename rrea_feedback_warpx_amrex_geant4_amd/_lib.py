@@ -237,6 +237,7 @@ SIGNATURES = [
       C.c_int32, C.POINTER(DrainIO), C.c_int32, C.c_void_p]),
     ("rrea_fields_signal_fast_path", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_stream_wait_fast_path", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("rrea_fields_set_block", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rrea_kernel_timing_read", C.c_int32, [C.c_void_p, C.c_int32, P_D, P_I64]),
     ("rrea_version", C.c_char_p, []),

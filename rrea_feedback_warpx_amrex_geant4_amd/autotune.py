@@ -14,9 +14,15 @@ configuration), but the speed does:
   2^(6 - b) quarter-octaves, so every width spans 16 octaves).  Bands
   keep a wavefront's lanes on similar branches and table brackets.
 
-``tune`` times every candidate over one sort period of the real workload
+- ``block``: the fast kernels' workgroup size (128, 256 or 512 lanes;
+  ``rrea_fields_set_block``): a workgroup keeps one LDS deposit window and
+  one partial row of tallies, so the size trades window flushes and fold
+  rows against scheduling granularity.
+
+``tune`` times every sort candidate over one sort period of the real
+workload, ``tune_block`` every workgroup size over a fixed number of steps
 (the steps are real: the caller's simulation advances through them, as
-through a warm-up) and keeps the fastest per step.
+through a warm-up); each keeps the fastest per step.
 """
 
 from __future__ import annotations
@@ -76,3 +82,41 @@ def tune(step: Callable[[int, bool, int], None], sort_every: Iterable[int] = (0,
             trials.append((every, bits, (time.perf_counter() - t0) / n * 1e3))
     best = min(trials, key=lambda t: t[2])
     return Choice(best[0], best[1], best[2], trials, s - first_step)
+
+
+@dataclass
+class BlockChoice:
+    block: int
+    ms_per_step: float
+    trials: list = field(default_factory=list)  # (block, ms_per_step) of every candidate
+    steps_run: int = 0
+
+    def as_dict(self) -> dict:
+        return {"block": self.block, "ms_per_step": self.ms_per_step, "trials": [list(t) for t in self.trials],
+                "steps_run": self.steps_run}
+
+
+def tune_block(step: Callable[[int], None], set_block: Callable[[int], None], blocks: Iterable[int] = (128, 256, 512),
+               steps: int = 10, warmup: int = 1, first_step: int = 0) -> BlockChoice:
+    """The fast kernels' workgroup size (rrea_fields_set_block): each
+    candidate is set, run for ``warmup`` untimed steps, then timed over
+    ``steps`` steps of the workload (``step(s)`` runs step s, in whatever
+    tile order the caller keeps).  The fastest per step wins, ties to the
+    first timed, and is left set."""
+    s = first_step
+    trials = []
+    for b in blocks:
+        set_block(int(b))
+        for _ in range(warmup):
+            step(s)
+            s += 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(s)
+            s += 1
+        torch.cuda.synchronize()
+        trials.append((int(b), (time.perf_counter() - t0) / steps * 1e3))
+    best = min(trials, key=lambda t: t[1])
+    set_block(best[0])
+    return BlockChoice(best[0], best[1], trials, s - first_step)

@@ -948,8 +948,7 @@ struct FieldRange { double work, ex, ey, ez, bx, by, bz, length; };
 // (hold[c * kHoldStride], zero again on return) while the second node is
 // gathered, so a caller short of registers does not spill them; the values
 // and the arithmetic are the same.
-constexpr int kHoldStride = 256;
-template <bool kHold = false>
+template <bool kHold = false, int kHoldStride = 256>
 __device__ FieldRange path_field_work(DevFields const& F, Path const& p, Domain const& d, double q, double b, double e,
                                       int32_t& err, double* hold = nullptr)
 {

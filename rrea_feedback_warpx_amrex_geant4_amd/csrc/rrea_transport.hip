@@ -227,6 +227,7 @@ struct rrea_fields_s {
     // signalled after each fast-path kernel when enabled (rrea_fields_signal_fast_path)
     hipEvent_t fast_event = nullptr;
     bool fast_signal = false;
+    int32_t block = 256;             // the fast kernels' workgroup size (rrea_fields_set_block)
     void* d_dstate = nullptr;        // the device-driven drain's generation state
     struct Timed { hipEvent_t start, stop; int32_t kernel; };
     std::vector<Timed> events;   // pool; the first events_used are recorded launches
@@ -2035,7 +2036,12 @@ __device__ __forceinline__ double reduce_tallies(double* tally, double* out)
 // accumulated in an LDS window of the grid and flushed with one device atomic
 // per touched face; otherwise every lane deposits with device atomics.  Every
 // thread of the workgroup calls this.
-constexpr int kWindowFaces = 2048;
+// The LDS deposit window of a fast-path workgroup of kB lanes: 8 faces per
+// lane (2048 at 256 lanes), so the LDS per wave is the same at every size.
+template <int kB> constexpr int window_faces() { return 8 * kB; }
+constexpr int kWindowFaces = window_faces<256>();
+// the fast-path workgroup sizes the library is built for
+constexpr bool fast_block_ok(int b) { return b == 128 || b == 256 || b == 512; }
 
 __device__ __forceinline__ int32_t wave_min_i32(int32_t v)
 {
@@ -2112,6 +2118,7 @@ __device__ void source_cell_add(Env const& E, SourceCell const& c)
 // lanes span several batched instances, falls back to per-lane atomics.
 // jr_w, jz_w: the kernel's LDS window (kWindowFaces doubles each), which it
 // may use for other per-lane data as long as that is read before this call.
+template <int kB>
 __device__ __forceinline__ void deposit_block(StepArgs const& A, double* jr_w, double* jz_w, bool dep, int32_t instance, double q,
                               double dt, double r0, double z0, double r1, double z1, SourceCell const& sc)
 {
@@ -2156,16 +2163,16 @@ __device__ __forceinline__ void deposit_block(StepArgs const& A, double* jr_w, d
         reduce_window(ilo, ihi, jlo, jhi, dep ? instance : INT32_MAX, dep ? instance : INT32_MIN);
         int32_t const i0 = win[0], i1 = win[1], j0 = win[2], j1 = win[3];
         int32_t const wi = i1 - i0 + 1, nw = i0 > i1 ? 0 : wi * (j1 - j0 + 1);
-        if (nw > kWindowFaces || (nw > 0 && win[4] != win[5])) {
+        if (nw > window_faces<kB>() || (nw > 0 && win[4] != win[5])) {
             if (dep) deposit_segment(g, q, dt, r0, z0, r1, z1);
         } else if (nw > 0) {
             // the window's one grid: every lane flushes into it, depositing or not
             DevGrid const& gw = win[4] >= 0 ? A.batch[win[4]].G : A.env.G;
-            for (int c = threadIdx.x; c < nw; c += kBlock) { jr_w[c] = 0.0; jz_w[c] = 0.0; }
+            for (int c = threadIdx.x; c < nw; c += kB) { jr_w[c] = 0.0; jz_w[c] = 0.0; }
             __syncthreads();
             if (dep) deposit_segment_to(gw, LdsWindow{jr_w, jz_w, i0, j0, wi}, q, dt, r0, z0, r1, z1);
             __syncthreads();
-            for (int c = threadIdx.x; c < nw; c += kBlock) {
+            for (int c = threadIdx.x; c < nw; c += kB) {
                 int32_t const i = i0 + c % wi, j = j0 + c / wi;
                 double const vr = jr_w[c], vz = jz_w[c];
                 if (vr != 0.0) atomic_add(&gw.jr[i + (gw.nr + 1) * j], vr);
@@ -2191,11 +2198,11 @@ __device__ __forceinline__ void deposit_block(StepArgs const& A, double* jr_w, d
         reduce_window(ilo, ihi, jlo, jhi, has ? instance : INT32_MAX, has ? instance : INT32_MIN);
         int32_t const i0 = win[0], i1 = win[1], j0 = win[2], j1 = win[3];
         int32_t const wi = i1 - i0 + 1, nw = i0 > i1 ? 0 : wi * (j1 - j0 + 1);
-        if (nw > kWindowFaces || (nw > 0 && win[4] != win[5])) {
+        if (nw > window_faces<kB>() || (nw > 0 && win[4] != win[5])) {
             if (has) source_cell_add(El, sc);
         } else if (nw > 0) {
             Env const& Ew = win[4] >= 0 ? A.batch[win[4]] : A.env;
-            for (int c = threadIdx.x; c < nw; c += kBlock) { jr_w[c] = 0.0; jz_w[c] = 0.0; }
+            for (int c = threadIdx.x; c < nw; c += kB) { jr_w[c] = 0.0; jz_w[c] = 0.0; }
             __syncthreads();
             if (has) {
                 double const v4 = dmax(sc.w, 0.0) * dmax(kWAir, 0.0);
@@ -2221,7 +2228,7 @@ __device__ __forceinline__ void deposit_block(StepArgs const& A, double* jr_w, d
             }
             __syncthreads();
             int64_t const cells = static_cast<int64_t>(Ew.F.nr) * Ew.F.nz;
-            for (int c = threadIdx.x; c < nw; c += kBlock) {
+            for (int c = threadIdx.x; c < nw; c += kB) {
                 int64_t const cell = (i0 + c % wi) + static_cast<int64_t>(Ew.F.nr) * (j0 + c / wi);
                 double const v = jr_w[c], v4 = jz_w[c];
                 if (v != 0.0) {
@@ -2262,23 +2269,26 @@ __global__ void __launch_bounds__(kBlock) charged_slow_kernel(const StepArgs* __
 // sixteen entries each (consecutive lanes read consecutive entries), then
 // combine within their group of sixteen.  The whole workgroup calls this once
 // its columns are written.
-template <int kQ, int kSums, class Emit>
+template <int kQ, int kSums, int kB, class Emit>
 __device__ __forceinline__ void fold_columns(double const* col, Emit&& emit)
 {
-    static_assert(kQ <= kBlock / 16, "one group of sixteen lanes per column");
+    // kB / 16 lanes per column (a group within one wavefront), each summing
+    // sixteen entries
+    constexpr int kL = kB / 16;
+    static_assert(kQ <= 16 && kL <= 64 && (kL & (kL - 1)) == 0, "sixteen column groups of a power of two");
     __syncthreads();
-    int const q = threadIdx.x >> 4;
-    int const part = threadIdx.x & 15;
+    int const q = threadIdx.x / kL;
+    int const part = threadIdx.x % kL;
     bool const mx = q >= kSums;
     double s = 0.0;
     if (q < kQ) {
-        double const* c = col + q * kBlock + part;
+        double const* c = col + q * kB + part;
         for (int i = 0; i < 16; ++i) {
-            double const v = c[i * 16];
+            double const v = c[i * kL];
             s = mx ? (s < v ? v : s) : s + v;
         }
     }
-    for (int off = 8; off > 0; off >>= 1) {
+    for (int off = kL / 2; off > 0; off >>= 1) {
         double const o = __shfl_xor(s, off, 64);
         s = mx ? (s < o ? o : s) : s + o;
     }
@@ -2466,14 +2476,14 @@ __device__ __forceinline__ void append_slow(bool slow, int64_t ip, int32_t* list
 // the batched ones spill 60 bytes per lane and still run 7 % faster than at
 // three waves (143 VGPRs, no spill): 7.8 against 8.4 ms per launch over the
 // 256-instance batch
-template <bool kPos, bool kBatched>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+template <bool kPos, bool kBatched, int kB = 256>
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4)))
 charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slow_list,
                     int32_t* __restrict__ slow_count, double* __restrict__ partials)
 {
     using S = Species<kPos>;
     StepArgs const& A = *args;
-    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     bool slow = false;
     // the survivor's net chord, deposited by the whole workgroup at the end
     bool dep = false;
@@ -2482,7 +2492,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
     SourceCell src{0.0, 0.0, 0.0, 0, 0};  // its ion-pair source record
     // The LDS window of the deposits at the end; until then it holds each
     // lane's tallies and ledger records in columns of its own (column q of
-    // the lane at window[q * kBlock + lane]), folded per workgroup at the end:
+    // the lane at window[q * kB + lane]), folded per workgroup at the end:
     //   sums -- the tallies of a committed step (survivor, substep, drag, ion
     //     pairs, field work), and its soft radiative loss, collision loss,
     //     mean hard-Moller transfer and expected hard-Moller weight, written as
@@ -2496,13 +2506,13 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
     // weight while the chord's field work and the kicks need the registers;
     // they are read back (relaxed LDS loads, which the compiler does not fold
     // into the stores) where the drag, the commit and the deposit use them.
-    __shared__ double window[2 * kWindowFaces + 3 * kBlock];
+    __shared__ double window[2 * window_faces<kB>() + 3 * kB];
     using Col = ChargedColumns<kPos>;
-    static_assert((Col::kCols + 1) * kBlock <= 2 * kWindowFaces, "the columns exceed the deposit window");
+    static_assert((Col::kCols + 1) * kB <= 2 * window_faces<kB>(), "the columns exceed the deposit window");
     double* const lane_col = window + threadIdx.x;
-    double* const stash = window + 2 * kWindowFaces + threadIdx.x;
-    auto unstash = [&](int q) { return __hip_atomic_load(stash + q * kBlock, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
-    for (int q = 0; q < Col::kCols + 1; ++q) lane_col[q * kBlock] = 0.0;
+    double* const stash = window + 2 * window_faces<kB>() + threadIdx.x;
+    auto unstash = [&](int q) { return __hip_atomic_load(stash + q * kB, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
+    for (int q = 0; q < Col::kCols + 1; ++q) lane_col[q * kB] = 0.0;
     // a batched wave takes the fast path only when all its particles share one
     // instance, whose environment is then wave-uniform; a wave that straddles
     // an instance boundary goes to the complete path
@@ -2543,8 +2553,8 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             if (!(w > 0.0) || pr > D.r_hi + 1.0e-9 * D.r_hi) break;
             double const pz = A.prev_z[ip], pth = A.prev_th[ip];
             stash[0] = pr;
-            stash[kBlock] = pz;
-            stash[2 * kBlock] = w;
+            stash[kB] = pz;
+            stash[2 * kB] = w;
             double dx = A.prev_ux[ip], dy = A.prev_uy[ip], dz = A.prev_uz[ip];
             double k = kinetic_from_u(dx, dy, dz);
             normalize(dx, dy, dz);
@@ -2562,8 +2572,8 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             double const span = dmax(path.end_f, 1.0e-6);
             // batched: the first GL node's products wait in the (still empty)
             // ledger columns, which keeps that kernel from spilling them
-            static_assert(kHoldStride == kBlock && Col::kCols >= 6, "the hold columns are the ledger's");
-            FieldRange const fr = path_field_work<kBatched>(F, path, D, S::kCharge, 0.0, 1.0, err, lane_col);
+            static_assert(Col::kCols >= 6, "the hold columns are the ledger's");
+            FieldRange const fr = path_field_work<kBatched, kB>(F, path, D, S::kCharge, 0.0, 1.0, err, lane_col);
             if (err) break;
             double const field_work = fr.work / span;
             double const chord_dp = density_path(iv, path, 0.0, path.end_f);
@@ -2593,12 +2603,12 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             // the guard's taus for one subcycle; they count only if every driver
             // below asks for one subcycle too, which the energy column marks
             if constexpr (kPos) {
-                lane_col[Col::kColTauA * kBlock] = total0 * mus / span;
+                lane_col[Col::kColTauA * kB] = total0 * mus / span;
             } else {
                 double const slice = mus / span;
-                lane_col[Col::kColTauA * kBlock] = g_a * slice;
-                lane_col[Col::kColTauB * kBlock] = g_b * slice;
-                lane_col[Col::kColTauTot * kBlock] = (g_a + g_b) * slice;
+                lane_col[Col::kColTauA * kB] = g_a * slice;
+                lane_col[Col::kColTauB * kB] = g_b * slice;
+                lane_col[Col::kColTauTot * kB] = (g_a + g_b) * slice;
             }
             EBracket eb0;
             if (!elastic_bracket(EL, k, eb0)) break;
@@ -2613,7 +2623,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             if (dmax(1.0, ceil((fabs(field_work) + drag_est) / impulse_scale)) != 1.0) break;
             if (err) break;
             // every driver asked for one subcycle: the guard's record of this step
-            lane_col[Col::kColEnergy * kBlock] = k;
+            lane_col[Col::kColEnergy * kB] = k;
             // the single substep: the driver chord again (sub_dt = dt / 1)
             double const se = path.end_f;
             double const substep_dt = dt * (se - 0.0);
@@ -2638,8 +2648,8 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
                 electron_stp_rates(T, b1, thr, gcut, k1, hm_on, s0, s1, err, &hm_inv1);
                 double const hard_tau = s0 * L, brems_tau = s1 * L;
                 h = select_hazard(hard_tau, brems_tau, rng_uniform(rng, 0, S::kDepth0), rng_uniform(rng, 0, S::kPick0));
-                lane_col[Col::kColHmx * kBlock] = hard_tau;
-                lane_col[Col::kColHazardTotal * kBlock] = hard_tau + brems_tau;
+                lane_col[Col::kColHmx * kB] = hard_tau;
+                lane_col[Col::kColHazardTotal * kB] = hard_tau + brems_tau;
             }
             if (!h.valid || h.event || err) break;
             // continuous drag over [0, se] at the frozen energy k1, not reaching the cutoff
@@ -2658,10 +2668,10 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             double const avail = dmax(0.0, k1 - cutoff);
             double const drag = dmin(raw_col + raw_soft, avail);
             double const col_loss = dmin(raw_col, drag);
-            lane_col[Col::kColSoft * kBlock] = wq * dmax(0.0, drag - col_loss);
+            lane_col[Col::kColSoft * kB] = wq * dmax(0.0, drag - col_loss);
             if constexpr (!kPos) {
-                lane_col[Col::kColCol * kBlock] = wq * col_loss;
-                lane_col[Col::kColHard * kBlock] = wq * dmin(hstp * L, avail);
+                lane_col[Col::kColCol * kB] = wq * col_loss;
+                lane_col[Col::kColHard * kB] = wq * dmin(hstp * L, avail);
             }
             double ion = 0.0;
             if (col_loss > 0.0 && raw_col > 0.0) {
@@ -2729,36 +2739,36 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             if (A.status) A.status[ip] = static_cast<uint8_t>(RREA_SURVIVE);
             if (A.ion_pairs) A.ion_pairs[ip] = ion;
             if (A.substeps) A.substeps[ip] = 1;
-            lane_col[Col::kColSurvivor * kBlock] = 1.0;
-            lane_col[Col::kColSubstep * kBlock] = 1.0;
-            lane_col[Col::kColDrag * kBlock] = wq * drag;
-            lane_col[Col::kColIon * kBlock] = ion;
-            lane_col[Col::kColFieldWork * kBlock] = wq * hw1 + wq * hw2;
+            lane_col[Col::kColSurvivor * kB] = 1.0;
+            lane_col[Col::kColSubstep * kB] = 1.0;
+            lane_col[Col::kColDrag * kB] = wq * drag;
+            lane_col[Col::kColIon * kB] = ion;
+            lane_col[Col::kColFieldWork * kB] = wq * hw1 + wq * hw2;
         } while (false);
     }
     append_slow(slow, ip, slow_list, slow_count);
-    if (!(lane_col[Col::kColEnergy * kBlock] > 0.0)) {
+    if (!(lane_col[Col::kColEnergy * kB] > 0.0)) {
         // a lane stopped by a substep driver records no guard taus
-        for (int q = Col::kColSums; q < Col::kColEnergy; ++q) lane_col[q * kBlock] = 0.0;
+        for (int q = Col::kColSums; q < Col::kColEnergy; ++q) lane_col[q * kB] = 0.0;
     }
     if (slow) {
         // a lane handed over counts none of the sums it formed
-        for (int q = Col::kColSoft; q < Col::kColSums; ++q) lane_col[q * kBlock] = 0.0;
+        for (int q = Col::kColSoft; q < Col::kColSums; ++q) lane_col[q * kB] = 0.0;
     } else if constexpr (!kPos) {
         if (ip < A.n) {
             // the expected hard-Moller weight of the committed step's one hazard
             // draw, w P(total) hard / total (OV/RreaElectronTransport.cpp:630-646)
-            double const hard_tau = lane_col[Col::kColHmx * kBlock];
-            double const total_tau = lane_col[Col::kColHazardTotal * kBlock];
-            lane_col[Col::kColHmx * kBlock] = A.w[ip] * (total_tau > 0.0
+            double const hard_tau = lane_col[Col::kColHmx * kB];
+            double const total_tau = lane_col[Col::kColHazardTotal * kB];
+            lane_col[Col::kColHmx * kB] = A.w[ip] * (total_tau > 0.0
                 ? probability_from_optical_depth(total_tau) * hard_tau / total_tau : 0.0);
         }
     }
     // the workgroup's tallies and ledger records, booked by the fold kernel
-    fold_columns<Col::kCols, Col::kColSums>(window, [&](int q, double v) {
+    fold_columns<Col::kCols, Col::kColSums, kB>(window, [&](int q, double v) {
         partials[static_cast<int64_t>(blockIdx.x) * kPartialStride + q] = v;
     });
-    deposit_block(A, window, window + kWindowFaces, dep, dep_inst, dep_q, A.cfg.dt_s, dep_r0, dep_z0, dep_r1, dep_z1,
+    deposit_block<kB>(A, window, window + window_faces<kB>(), dep, dep_inst, dep_q, A.cfg.dt_s, dep_r0, dep_z0, dep_r1, dep_z1,
                   src);
 }
 
@@ -2824,8 +2834,8 @@ __global__ void __launch_bounds__(kBlock) photon_kernel(const StepArgs* __restri
 // own arithmetic, in the same order; any other outcome -- an interaction, a
 // sub-cutoff or weightless photon, an error -- is detected before the first
 // write and the photon is handed over to the complete step.
-template <bool kBatched>
-__global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __restrict__ args,
+template <bool kBatched, int kB = 256>
+__global__ void __launch_bounds__(kB) photon_fast_kernel(const StepArgs* __restrict__ args,
                                                            int32_t* __restrict__ slow_list,
                                                            int32_t* __restrict__ slow_count,
                                                            double* __restrict__ partials)
@@ -2835,10 +2845,10 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
     // per workgroup at the end: survivors, escapes, escaped energy (sums); the
     // pass's tau and energy (maxima) -- OV/RreaPhotonTransport.cpp:192, 222-233
     using Col = PhotonColumns;
-    __shared__ double cols[Col::kCols * kBlock];
+    __shared__ double cols[Col::kCols * kB];
     double* const lane_col = cols + threadIdx.x;
-    for (int q = 0; q < Col::kCols; ++q) lane_col[q * kBlock] = 0.0;
-    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    for (int q = 0; q < Col::kCols; ++q) lane_col[q * kB] = 0.0;
+    int64_t const ip = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     bool slow = false;
     // a batched wave takes the fast path only when all its particles share one
     // instance, whose environment is then wave-uniform; a wave that straddles
@@ -2906,14 +2916,14 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
             A.z[ip] = z;
             if (escapes) {
                 A.id[ip] = RREA_INVALID_ID;
-                lane_col[Col::kColEscape * kBlock] = 1.0;
-                lane_col[Col::kColEscapedEnergy * kBlock] = w * energy;
+                lane_col[Col::kColEscape * kB] = 1.0;
+                lane_col[Col::kColEscapedEnergy * kB] = w * energy;
             } else {
-                lane_col[Col::kColSurvivor * kBlock] = 1.0;
+                lane_col[Col::kColSurvivor * kB] = 1.0;
             }
             if (path.in_len > 0.0) {
-                lane_col[Col::kColTau * kBlock] = pass_tau;
-                lane_col[Col::kColEnergy * kBlock] = energy;
+                lane_col[Col::kColTau * kB] = pass_tau;
+                lane_col[Col::kColEnergy * kB] = energy;
             }
             if (A.status) A.status[ip] = static_cast<uint8_t>(escapes ? RREA_ESCAPE : RREA_SURVIVE);
             if (A.ion_pairs) A.ion_pairs[ip] = 0.0;
@@ -2921,7 +2931,7 @@ __global__ void __launch_bounds__(kBlock) photon_fast_kernel(const StepArgs* __r
         } while (false);
     }
     append_slow(slow, ip, slow_list, slow_count);
-    fold_columns<Col::kCols, Col::kColSums>(cols, [&](int q, double v) {
+    fold_columns<Col::kCols, Col::kColSums, kB>(cols, [&](int q, double v) {
         partials[static_cast<int64_t>(blockIdx.x) * kPartialStride + q] = v;
     });
 }
@@ -4762,6 +4772,13 @@ rrea_status rrea_fields_signal_fast_path(rrea_fields_t F, int32_t enable)
     return RREA_OK;
 }
 
+rrea_status rrea_fields_set_block(rrea_fields_t F, int32_t threads)
+{
+    if (!F || !fast_block_ok(threads)) return RREA_ERR_ARGUMENT;
+    F->block = threads;
+    return RREA_OK;
+}
+
 rrea_status rrea_stream_wait_fast_path(rrea_fields_t F, rrea_stream_t stream)
 {
     if (!F || !F->fast_event) return RREA_ERR_ARGUMENT;
@@ -4932,7 +4949,8 @@ rrea_status slow_list_workspace(rrea_fields_t F, int64_t n)
         F->slow_cap = 0;
         if (n > INT32_MAX) return RREA_ERR_ARGUMENT;
         HIP_TRY(hipMalloc(&F->d_slow_list, n * sizeof(int32_t)));
-        HIP_TRY(hipMalloc(&F->d_partials, blocks_for(n, kBlock) * kPartialStride * sizeof(double)));
+        // one partial row per fast-path workgroup, at the smallest workgroup size
+        HIP_TRY(hipMalloc(&F->d_partials, blocks_for(n, 128) * kPartialStride * sizeof(double)));
         F->slow_cap = n;
     }
     if (!F->d_slow_count) HIP_TRY(hipMalloc(&F->d_slow_count, sizeof(int32_t)));
@@ -5168,6 +5186,34 @@ int64_t fold_blocks(int64_t rows)
     return b < 64 ? b : 64;
 }
 
+// The fast kernels at the workgroup size kb (128, 256 or 512 lanes) over nb
+// workgroups.
+template <bool kPos, bool kBatched>
+void launch_charged_fast(int32_t kb, int64_t nb, hipStream_t s, const StepArgs* d_args, StepWork const& K)
+{
+    dim3 const g(static_cast<unsigned>(nb));
+    if (kb == 128) {
+        charged_fast_kernel<kPos, kBatched, 128><<<g, dim3(128), 0, s>>>(d_args, K.slow_list, K.slow_count, K.partials);
+    } else if (kb == 512) {
+        charged_fast_kernel<kPos, kBatched, 512><<<g, dim3(512), 0, s>>>(d_args, K.slow_list, K.slow_count, K.partials);
+    } else {
+        charged_fast_kernel<kPos, kBatched, 256><<<g, dim3(256), 0, s>>>(d_args, K.slow_list, K.slow_count, K.partials);
+    }
+}
+
+template <bool kBatched>
+void launch_photon_fast(int32_t kb, int64_t nb, hipStream_t s, const StepArgs* d_args, StepWork const& K)
+{
+    dim3 const g(static_cast<unsigned>(nb));
+    if (kb == 128) {
+        photon_fast_kernel<kBatched, 128><<<g, dim3(128), 0, s>>>(d_args, K.slow_list, K.slow_count, K.partials);
+    } else if (kb == 512) {
+        photon_fast_kernel<kBatched, 512><<<g, dim3(512), 0, s>>>(d_args, K.slow_list, K.slow_count, K.partials);
+    } else {
+        photon_fast_kernel<kBatched, 256><<<g, dim3(256), 0, s>>>(d_args, K.slow_list, K.slow_count, K.partials);
+    }
+}
+
 // A species step's launches on `s`, reading the argument block at d_args.
 // With device_order the staged secondaries are ordered with their count on
 // the device (sort_records_kernel), so the sequence makes no host round trip
@@ -5196,7 +5242,10 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
         set_i32_kernel<<<dim3(1), dim3(64), 0, s>>>(K.slow_count, 0);
         HIP_TRY(hipGetLastError());
     }
-    int64_t const nb = blocks_for(L.n, kBlock);
+    // the fast kernels' workgroup size: the handle's setting (the fixed
+    // workspace of the no-allocation step is laid out for 256)
+    int32_t const kb = (W && !fixed) ? W->block : 256;
+    int64_t const nb = blocks_for(L.n, L.complete ? kBlock : kb);
     // HIP events around each transport kernel while timing is enabled
     auto timed = [&](int32_t kernel, auto&& launch) -> rrea_status {
         hipEvent_t stop = nullptr;
@@ -5223,13 +5272,8 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
         st = timed(4, [&] { photon_kernel<<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(d_args); });
     } else if (photon) {
         st = timed(4, [&] {
-            if (batched) {
-                photon_fast_kernel<true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                    d_args, K.slow_list, K.slow_count, K.partials);
-            } else {
-                photon_fast_kernel<false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                    d_args, K.slow_list, K.slow_count, K.partials);
-            }
+            if (batched) launch_photon_fast<true>(kb, nb, s, d_args, K);
+            else launch_photon_fast<false>(kb, nb, s, d_args, K);
         });
         if (st != RREA_OK) return st;
         if (signal) HIP_TRY(hipEventRecord(W->fast_event, s));
@@ -5248,21 +5292,11 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
     } else {
         st = timed(kid, [&] {
             if (positron) {
-                if (batched) {
-                    charged_fast_kernel<true, true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, K.slow_list, K.slow_count, K.partials);
-                } else {
-                    charged_fast_kernel<true, false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, K.slow_list, K.slow_count, K.partials);
-                }
+                if (batched) launch_charged_fast<true, true>(kb, nb, s, d_args, K);
+                else launch_charged_fast<true, false>(kb, nb, s, d_args, K);
             } else {
-                if (batched) {
-                    charged_fast_kernel<false, true><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, K.slow_list, K.slow_count, K.partials);
-                } else {
-                    charged_fast_kernel<false, false><<<dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s>>>(
-                        d_args, K.slow_list, K.slow_count, K.partials);
-                }
+                if (batched) launch_charged_fast<false, true>(kb, nb, s, d_args, K);
+                else launch_charged_fast<false, false>(kb, nb, s, d_args, K);
             }
         });
         if (st != RREA_OK) return st;
@@ -6468,6 +6502,8 @@ rrea_status rrea_batch_graph_create(rrea_tables_t T, rrea_batch_t B, const rrea_
     }
     // a workspace no other call grows or frees: the captured launches hold its pointers
     rrea_fields_t const W = G->work;
+    // the fast kernels' workgroup size the batch's workspace is set to
+    W->block = B->work->block;
     hipStream_t cs = nullptr;
     auto fail = [&](rrea_status e) {
         if (cs) (void)hipStreamDestroy(cs);

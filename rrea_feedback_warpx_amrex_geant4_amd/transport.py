@@ -161,6 +161,11 @@ class Coupling:
         else:
             _lib.check(L.rrea_density_set_constant(self.h, fields.density_ratio), "rrea_density_set_constant")
 
+    def set_block(self, threads: int) -> None:
+        """The fast kernels' workgroup size on this handle (128, 256 or 512;
+        rrea_fields_set_block): a tuning knob, results unchanged."""
+        _lib.check(_lib.lib().rrea_fields_set_block(self.h, int(threads)), "rrea_fields_set_block")
+
     def update(self, fields: RzFields, stream: int | None = None):
         arrs = [np.ascontiguousarray(a, dtype=np.float64)
                 for a in (fields.er, fields.ez, fields.amb_er, fields.amb_ez, fields.btheta)]
@@ -472,6 +477,11 @@ class Batch:
                    "rrea_transport_step_batched")
         if check:
             out.check_error()
+
+    def set_block(self, threads: int) -> None:
+        """The batched fast kernels' workgroup size (on the batch's workspace
+        handle; graphs created afterwards take it)."""
+        _lib.check(_lib.lib().rrea_fields_set_block(self.workspace, int(threads)), "rrea_fields_set_block")
 
     def sort(self, tile, inst: torch.Tensor, photon: bool = False, energy_bits: int = -1,
              stream: int | None = None) -> None:
@@ -800,6 +810,11 @@ class ConcurrentApply:
                                     stream=main.cuda_stream, check=False)
         if check:
             out_e.check_error()
+
+    def set_block(self, threads: int) -> None:
+        """The fast kernels' workgroup size for all three passes."""
+        for c in self.couplings:
+            c.set_block(threads)
 
     def close(self):
         pass

@@ -59,3 +59,23 @@ def test_tune_amortises_the_sort_and_its_decay():
 def test_tune_rejects_negative_interval():
     with pytest.raises(ValueError):
         autotune.tune(FakeWorkload(), sort_every=(-1,))
+
+
+def test_tune_block_picks_the_cheapest_and_leaves_it_set():
+    """tune_block times each workgroup size over the same number of steps
+    after its warm-up and leaves the fastest set."""
+    cost = {128: 3.0, 256: 1.0, 512: 2.0}
+    state = {"block": None, "sets": []}
+
+    def set_block(b):
+        state["block"] = b
+        state["sets"].append(b)
+
+    def step(s):
+        time.sleep(cost[state["block"]] * 2e-3)
+
+    c = autotune.tune_block(step, set_block, blocks=(128, 256, 512), steps=4, warmup=1, first_step=7)
+    assert c.block == 256 and state["block"] == 256
+    assert [t[0] for t in c.trials] == [128, 256, 512]
+    assert state["sets"] == [128, 256, 512, 256]
+    assert c.steps_run == 15

@@ -243,3 +243,45 @@ def test_secondaries_append_behind_count_and_capacity(tables):
     RreaParticleInteraction.apply_electrons(tables, cp, DeviceTile.from_host(host), cfg, small)
     RreaParticleInteraction.apply_electrons(tables, cp, DeviceTile.from_host(host), cfg, small, check=False)
     assert small.error.cpu().tolist()[0] == 5
+
+
+@pytest.mark.parametrize("species", ["electron", "positron", "photon"])
+def test_fast_kernel_workgroup_sizes_agree(tables, species):
+    """The fast kernels at 128 and 512 lanes per workgroup
+    (rrea_fields_set_block) give the 256-lane results on a cell-ordered
+    at-scale sample: states, statuses and ids bit for bit, the staged
+    secondaries record for record, the tallies and the J grid to FP64
+    summation order (each workgroup folds and flushes its own share)."""
+    from rrea_feedback_warpx_amrex_geant4_amd.transport import (Coupling, DeviceTile, PhotonTile,
+                                                                RreaParticleInteraction, StepOutputs)
+    fields = wl.c2_fields()
+    n = 1 << 16
+    if species == "photon":
+        host = wl.c2_photons(n, fields, dt_s=1e-12, seed=31, first_id=1 << 41)
+    else:
+        host = wl.c2_electrons(n, fields, seed=32, charge_sign=1.0 if species == "positron" else -1.0)
+    cfg = StepConfig(rng_seed=5, dt_s=1e-12)
+    print(f"case fast-kernel workgroup sizes: {n} {species}s at 128, 256, 512 lanes", flush=True)
+    res = {}
+    for block in (256, 128, 512):
+        cp = Coupling(fields, tables)
+        cp.set_block(block)
+        tile = PhotonTile.from_host(host) if species == "photon" else DeviceTile.from_host(host)
+        tile.sort_by_cell(cp)
+        out = StepOutputs(n, fields, secondary_capacity=1 << 15, sources=True)
+        getattr(RreaParticleInteraction, f"apply_{species}s")(tables, cp, tile, cfg, out)
+        torch.cuda.synchronize()
+        t = tile.cur if species == "photon" else tile
+        res[block] = ({k: getattr(t, k).clone() for k in ("r", "z", "theta", "ux", "uy", "uz", "w", "id")},
+                      out.status.clone(), out.secondaries_host(), out.tallies.clone(), out.jr.clone(), out.jz.clone())
+    ref = res[256]
+    for block in (128, 512):
+        got = res[block]
+        for k in ref[0]:
+            assert torch.equal(got[0][k], ref[0][k]), (block, k)
+        assert torch.equal(got[1], ref[1]), block
+        assert len(got[2]) == len(ref[2]) and (got[2] == ref[2]).all(), block
+        scale = ref[3].abs().clamp_min(1e-300)
+        assert bool(((got[3] - ref[3]).abs() <= 1e-12 * scale + 1e-300).all()), block
+        for a, b in ((got[4], ref[4]), (got[5], ref[5])):
+            assert torch.allclose(a, b, rtol=0, atol=1e-12 * float(b.abs().max()) + 1e-300), block
