@@ -2003,10 +2003,10 @@ __device__ __noinline__ void photon_particle(StepArgs const& A, int64_t ip, doub
 
 // workgroup reduction of the tallies, one atomic per tally per workgroup
 // Returns thread q's workgroup total of slot q (zero for threads past the slots).
-template <int kSlots = kTallyUsed>
+template <int kSlots = kTallyUsed, int kThreads = kBlock>
 __device__ __forceinline__ double reduce_tallies(double* tally, double* out)
 {
-    __shared__ double red[kSlots][kBlock / 64];
+    __shared__ double red[kSlots][kThreads / 64];
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
     for (int q = 0; q < kSlots; ++q) {
@@ -2017,10 +2017,10 @@ __device__ __forceinline__ double reduce_tallies(double* tally, double* out)
     }
     __syncthreads();
     double mine = 0.0;
-    for (int q = threadIdx.x; q < kSlots; q += kBlock) {
+    for (int q = threadIdx.x; q < kSlots; q += kThreads) {
         bool const mx = tally_is_max(q);
         double s = 0.0;
-        for (int wv = 0; wv < kBlock / 64; ++wv) {
+        for (int wv = 0; wv < kThreads / 64; ++wv) {
             double const r = red[q][wv];
             s = mx ? (s < r ? r : s) : s + r;
         }
@@ -2243,8 +2243,13 @@ __device__ __forceinline__ void deposit_block(StepArgs const& A, double* jr_w, d
 }
 
 // The particles the fast path handed over, strided over a fixed grid.
+// One wavefront per workgroup: a hand-over wave (~290 registers) is placed
+// on any SIMD with room for it, not only where a whole CU's four SIMDs are
+// free, while the other species' passes fill the chip.
+constexpr int kSlowThreads = 64;
+
 template <bool kPos>
-__global__ void __launch_bounds__(kBlock) charged_slow_kernel(const StepArgs* __restrict__ args,
+__global__ void __launch_bounds__(kSlowThreads) charged_slow_kernel(const StepArgs* __restrict__ args,
                                                             const int32_t* __restrict__ list,
                                                             const int32_t* __restrict__ count)
 {
@@ -2254,11 +2259,11 @@ __global__ void __launch_bounds__(kBlock) charged_slow_kernel(const StepArgs* __
     double tally[kTallyUsed];
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
     int64_t const n = *count;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * kBlock) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kSlowThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kSlowThreads) {
         charged_full_particle<kPos>(A, list[i], tally, ledger);
     }
-    reduce_tallies(tally, A.tallies);
+    reduce_tallies<kTallyUsed, kSlowThreads>(tally, A.tallies);
     ledger_flush(ledger, A.tallies);
 }
 
@@ -2937,7 +2942,7 @@ __global__ void __launch_bounds__(kB) photon_fast_kernel(const StepArgs* __restr
 }
 
 // The photons the fast path handed over, strided over a fixed grid.
-__global__ void __launch_bounds__(kBlock) photon_slow_kernel(const StepArgs* __restrict__ args,
+__global__ void __launch_bounds__(kSlowThreads) photon_slow_kernel(const StepArgs* __restrict__ args,
                                                            const int32_t* __restrict__ list,
                                                            const int32_t* __restrict__ count)
 {
@@ -2947,11 +2952,11 @@ __global__ void __launch_bounds__(kBlock) photon_slow_kernel(const StepArgs* __r
     double tally[kTallyUsed];
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
     int64_t const n = *count;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * kBlock) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kSlowThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kSlowThreads) {
         photon_particle(A, list[i], tally, ledger);
     }
-    reduce_tallies(tally, A.tallies);
+    reduce_tallies<kTallyUsed, kSlowThreads>(tally, A.tallies);
     ledger_flush(ledger, A.tallies);
 }
 
@@ -5170,10 +5175,10 @@ rrea_status species_prepare(int species, rrea_tables_t T, rrea_fields_t F, const
 }
 
 // The hand-over kernels run at one wave per SIMD (they hold ~290 registers),
-// so 256 workgroups of four waves are all the chip runs of them at once;
-// they stride over the hand-over list, and a larger grid would only queue
-// empty workgroups behind the busy ones.
-constexpr int64_t kSlowBlocks = 256;
+// so 1024 one-wave workgroups are all the chip runs of them at once; they
+// stride over the hand-over list, and a larger grid would only queue empty
+// workgroups behind the busy ones.
+constexpr int64_t kSlowBlocks = 1024;
 
 // The fold kernel's grid over `rows` partial rows: sixteen passes of sixteen
 // rows per workgroup, at most 64 workgroups.  Each fold workgroup books its
@@ -5281,7 +5286,7 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
         st = timed(5, [&] {
             photon_fold_kernel<<<dim3(static_cast<unsigned>(fold_blocks(nb))), dim3(kBlock), 0, s>>>(
                 d_args, K.partials, nb);
-            photon_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
+            photon_slow_kernel<<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kSlowThreads), 0, s>>>(
                 d_args, K.slow_list, K.slow_count);
         });
     } else if (L.complete) {
@@ -5306,12 +5311,12 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
             if (positron) {
                 charged_fold_kernel<true><<<dim3(static_cast<unsigned>(fold_blocks(nb))), dim3(kBlock), 0, s>>>(
                     d_args, K.partials, nb);
-                charged_slow_kernel<true><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
+                charged_slow_kernel<true><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kSlowThreads), 0, s>>>(
                     d_args, K.slow_list, K.slow_count);
             } else {
                 charged_fold_kernel<false><<<dim3(static_cast<unsigned>(fold_blocks(nb))), dim3(kBlock), 0, s>>>(
                     d_args, K.partials, nb);
-                charged_slow_kernel<false><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kBlock), 0, s>>>(
+                charged_slow_kernel<false><<<dim3(static_cast<unsigned>(slow_blocks)), dim3(kSlowThreads), 0, s>>>(
                     d_args, K.slow_list, K.slow_count);
             }
         });
