@@ -3730,17 +3730,28 @@ __global__ void __launch_bounds__(kBlock) newborn_kernel(const NewbornArgs* __re
     // payload) stops the drain: nothing is advanced
     bool const stopped = __hip_atomic_load(N.S.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
     // grid-stride: a launch sized for the FIFO's capacity covers a generation
-    // whose bounds only the device knows
-    int64_t const stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    // whose bounds only the device knows.  A generation smaller than the
+    // launch's lanes is spread over every wavefront, `lpw` lanes of each
+    // (one entry per wavefront when there are no more entries than
+    // wavefronts), so a wavefront runs its own few entries' paths rather
+    // than the union of 64 divergent ones; the other lanes idle.
     int64_t const ge = stopped ? N.gb : N.ge;
+    constexpr int kWaves = kBlock / 64;
+    int64_t const waves = static_cast<int64_t>(gridDim.x) * kWaves;
+    int64_t lpw = (ge - N.gb + waves - 1) / waves;
+    lpw = lpw < 1 ? 1 : (lpw > 64 ? 64 : lpw);
+    int64_t const wave = static_cast<int64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+    int64_t const lane = threadIdx.x & 63;
     // a workgroup with no entry (an empty generation of the device-driven
     // drain) leaves at once; the test is the same for all its lanes
-    if (N.gb + static_cast<int64_t>(blockIdx.x) * kBlock >= ge) return;
+    if (N.gb + static_cast<int64_t>(blockIdx.x) * kWaves * lpw >= ge) return;
     __shared__ double ledger[RREA_LEDGER_COUNT];
     ledger_begin(ledger);
     double tally[kTallyNewborn];
     for (int q = 0; q < kTallyNewborn; ++q) tally[q] = 0.0;
-    for (int64_t i = N.gb + static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < ge; i += stride) {
+    for (int64_t base = N.gb + wave * lpw; base < ge; base += waves * lpw) {
+        int64_t const i = base + lane;
+        if (lane >= lpw || i >= ge) continue;
         rrea_secondary const& p = N.work[i];
         bool const photon = p.species == 2;
         // transient_energy_is_certified OV/RreaSecondaryCommit.cpp:148-180
