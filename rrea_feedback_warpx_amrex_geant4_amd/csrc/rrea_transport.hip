@@ -2251,17 +2251,38 @@ constexpr int kSlowThreads = 64;
 template <bool kPos>
 __global__ void __launch_bounds__(kSlowThreads) charged_slow_kernel(const StepArgs* __restrict__ args,
                                                             const int32_t* __restrict__ list,
-                                                            const int32_t* __restrict__ count)
+                                                            int32_t* __restrict__ count)
 {
     StepArgs const& A = *args;
     __shared__ double ledger[RREA_LEDGER_COUNT];
     ledger_begin(ledger);
     double tally[kTallyUsed];
     for (int q = 0; q < kTallyUsed; ++q) tally[q] = 0.0;
-    int64_t const n = *count;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kSlowThreads + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * kSlowThreads) {
-        charged_full_particle<kPos>(A, list[i], tally, ledger);
+    // a work queue over the two classes of append_slow_split.  Each class is
+    // cut into chunks of about half a wavefront's share of it (so a heavy
+    // chunk holds a few divergent paths, not 64), heavy chunks numbered first
+    // so that the long paths start earliest; count[2] hands out the chunks
+    // after each wavefront's first, and every wavefront ends when they run
+    // out.
+    static_assert(kSlowThreads == 64, "one wavefront per workgroup");
+    int64_t const cl = count[0], ch = count[1];
+    int64_t const waves = gridDim.x, lane = threadIdx.x;
+    int64_t lh = (2 * ch + waves - 1) / waves, ll = (2 * cl + waves - 1) / waves;
+    lh = lh < 1 ? 1 : (lh > 64 ? 64 : lh);
+    ll = ll < 1 ? 1 : (ll > 64 ? 64 : ll);
+    int64_t const hc = (ch + lh - 1) / lh, lc = (cl + ll - 1) / ll;
+    // the first chunk is the wavefront's own index, later ones from the queue
+    for (int64_t c = blockIdx.x; c < hc + lc;) {
+        if (c < hc) {
+            int64_t const e = c * lh + lane;
+            if (lane < lh && e < ch) charged_full_particle<kPos>(A, list[A.n - 1 - e], tally, ledger);
+        } else {
+            int64_t const e = (c - hc) * ll + lane;
+            if (lane < ll && e < cl) charged_full_particle<kPos>(A, list[e], tally, ledger);
+        }
+        int32_t q = 0;
+        if (lane == 0) q = atomicAdd(count + 2, 1);
+        c = waves + __shfl(q, 0, 64);
     }
     reduce_tallies<kTallyUsed, kSlowThreads>(tally, A.tallies);
     ledger_flush(ledger, A.tallies);
@@ -2462,6 +2483,31 @@ __global__ void __launch_bounds__(kBlock) photon_fold_kernel(const StepArgs* __r
 // the slow list, which the complete path then finishes from the unchanged
 // inputs.
 // ---------------------------------------------------------------------------
+// The charged fast paths hand over in two classes: light lanes (a particle
+// below the cutoff from the start, an electron ending the step at or below
+// the demotion gate: a short complete path) from the front of the list with
+// count[0], the rest from the back (list[cap - 1 - j]) with count[1], so the
+// hand-over kernel can give each class its own wavefronts.
+__device__ __forceinline__ void append_slow_split(bool slow, bool light, int64_t ip, int32_t* list, int32_t* count,
+                                                  int64_t cap)
+{
+    int const lane = threadIdx.x & 63;
+    uint64_t const below = (uint64_t{1} << lane) - 1U;
+    for (int cls = 0; cls < 2; ++cls) {
+        bool const mine = slow && (light == (cls == 0));
+        uint64_t const mask = __ballot(mine);
+        if (mask == 0) continue;
+        int const leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        int32_t base = 0;
+        if (lane == leader) base = atomicAdd(count + cls, static_cast<int32_t>(__popcll(mask)));
+        base = __shfl(base, leader, 64);
+        if (mine) {
+            int64_t const j = base + __popcll(mask & below);
+            list[cls == 0 ? j : cap - 1 - j] = static_cast<int32_t>(ip);
+        }
+    }
+}
+
 __device__ __forceinline__ void append_slow(bool slow, int64_t ip, int32_t* list, int32_t* count)
 {
     uint64_t const mask = __ballot(slow);
@@ -2490,6 +2536,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
     StepArgs const& A = *args;
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     bool slow = false;
+    bool light = false;  // a short complete path: below the cutoff from the start, or at the demotion gate
     // the survivor's net chord, deposited by the whole workgroup at the end
     bool dep = false;
     int32_t dep_inst = -1;
@@ -2563,7 +2610,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             double dx = A.prev_ux[ip], dy = A.prev_uy[ip], dz = A.prev_uz[ip];
             double k = kinetic_from_u(dx, dy, dz);
             normalize(dx, dy, dz);
-            if (!(k >= cutoff)) break;
+            if (!(k >= cutoff)) { light = true; break; }
             double s_th, c_th;
             sin_cos(pth, s_th, c_th);
             double run_x = pr * c_th;
@@ -2716,7 +2763,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             point_at_full(path, se, x, y, z);
             if (!in_half_open(x, y, z, D) || path.exits) break;
             // terminal: a survivor (for e-, above the demotion gate's energy)
-            if (!kPos && thr > 0.0 && k2 <= thr) break;
+            if (!kPos && thr > 0.0 && k2 <= thr) { light = true; break; }
             double const end_r = hypot(x, y);
             double const pr0 = unstash(0), pz0 = unstash(1);
             if (!deposit_segment_check(E.G, pr0, pz0, end_r, z)) break;
@@ -2751,7 +2798,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             lane_col[Col::kColFieldWork * kB] = wq * hw1 + wq * hw2;
         } while (false);
     }
-    append_slow(slow, ip, slow_list, slow_count);
+    append_slow_split(slow, light, ip, slow_list, slow_count, A.n);
     if (!(lane_col[Col::kColEnergy * kB] > 0.0)) {
         // a lane stopped by a substep driver records no guard taus
         for (int q = Col::kColSums; q < Col::kColEnergy; ++q) lane_col[q * kB] = 0.0;
@@ -4953,8 +5000,10 @@ rrea_status order_staged(rrea_fields_t F, int64_t n, rrea_secondary* dst, hipStr
     return RREA_OK;
 }
 
-// The fast paths' hand-over list and its count, sized for n particles (the
-// count is cleared by the launch sequence).
+// The fast paths' hand-over list and its counts (int32: the light and the
+// heavy class of append_slow_split, or one count for the photon path, then the
+// hand-over kernel's chunk cursor), sized for n particles (the counts are
+// cleared by the launch sequence).
 rrea_status slow_list_workspace(rrea_fields_t F, int64_t n)
 {
     if (F->slow_cap < n) {
@@ -4969,7 +5018,7 @@ rrea_status slow_list_workspace(rrea_fields_t F, int64_t n)
         HIP_TRY(hipMalloc(&F->d_partials, blocks_for(n, 128) * kPartialStride * sizeof(double)));
         F->slow_cap = n;
     }
-    if (!F->d_slow_count) HIP_TRY(hipMalloc(&F->d_slow_count, sizeof(int32_t)));
+    if (!F->d_slow_count) HIP_TRY(hipMalloc(&F->d_slow_count, 4 * sizeof(int32_t)));
     return RREA_OK;
 }
 
@@ -5255,7 +5304,8 @@ rrea_status species_launch(rrea_fields_t W, const StepArgs* d_args, SpeciesLaunc
         HIP_TRY(hipGetLastError());
     }
     if (!L.complete) {
-        set_i32_kernel<<<dim3(1), dim3(64), 0, s>>>(K.slow_count, 0);
+        // both classes' counts and the chunk cursor (int32 in two aligned words)
+        fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(K.slow_count), 2, 0ULL);
         HIP_TRY(hipGetLastError());
     }
     // the fast kernels' workgroup size: the handle's setting (the fixed
