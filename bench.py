@@ -699,8 +699,12 @@ def main():
     stores[POSITRON].load(wl.c2_electrons(n_p, fields, seed=3, first_id=1 << 40, charge_sign=1.0))
     g_host = wl.c2_photons(n_g, fields, dt_s=1e-12, seed=4, first_id=1 << 41)
     stores[PHOTON].load(g_host)
-    for k in ("r", "z", "theta"):  # the first step's chords run to the pushed points
-        getattr(stores[PHOTON], k)[:n_g].copy_(torch.as_tensor(g_host.extra[k + "_cur"]))
+    if args.sequential:
+        # the first step's chords run to the pushed points
+        for k in ("r", "z", "theta"):
+            getattr(stores[PHOTON], k)[:n_g].copy_(torch.as_tensor(g_host.extra[k + "_cur"]))
+    # (otherwise the step's own push, before its photon pass, moves the
+    # photons there)
     del g_host
     counts = torch.tensor([s.n for s in stores], dtype=torch.int64, device="cuda")
     next_id = torch.tensor([(1 << 42)], dtype=torch.int64, device="cuda")
@@ -716,7 +720,8 @@ def main():
     L = _lib.lib()
     apply = RreaParticleInteraction
     conc = None if args.sequential else ConcurrentApply(tables, fields, early=tuple(args.early))
-    step_clear = [o.secondary_count for o in (out_e, out_g, out_p)] + [out_e.jr, out_e.jz, out_e.rho, out_e.sources]
+    # one list per dtype: torch's multi-tensor zero then takes one launch each
+    step_clear = ([o.secondary_count for o in (out_e, out_g, out_p)], [out_e.jr, out_e.jz, out_e.rho, out_e.sources])
     # the handle each species' kernels are timed on
     h_e, h_g, h_p = (coupling.h,) * 3 if conc is None else tuple(c.h for c in conc.couplings)
 
@@ -729,22 +734,28 @@ def main():
             dev_p.sort_by_cell(coupling, energy_bits=energy_bits)
             dev_g.sort_by_cell(coupling, energy_bits=energy_bits)
         # the step's counters and grids cleared (one launch per dtype)
-        torch._foreach_zero_(step_clear)
+        for group in step_clear:
+            torch._foreach_zero_(group)
         cfg = StepConfig(rng_seed=20240623, step=s, dt_s=1e-12)
         if conc is not None:
-            conc.run(dev_e, dev_g, dev_p, cfg, out_e, out_g, out_p, drain, device_drain=args.device_drain)
+            # WarpX's photon push of this step on the photon stream, before
+            # its pass, under the electron hand-over
+            conc.run(dev_e, dev_g, dev_p, cfg, out_e, out_g, out_p, drain, device_drain=args.device_drain,
+                     pre_push=True)
         else:
             apply.apply_electrons(tables, coupling, dev_e, cfg, out_e, order_secondaries=True, check=False)
             apply.apply_photons(tables, coupling, dev_g, cfg, out_g, order_secondaries=True, check=False)
             apply.apply_positrons(tables, coupling, dev_p, cfg, out_p, order_secondaries=True, check=False)
             apply.drain_secondaries(tables, coupling, cfg, [out_e, out_g, out_p], drain, check=False)
-        # AddNParticles of the drain's survivors (RreaSecondaryCommit.cpp:594-629),
-        # then WarpX's photon push for the next step, newborn photons included
+        # AddNParticles of the drain's survivors (RreaSecondaryCommit.cpp:594-629);
+        # the sequential step then pushes the photons for the next step,
+        # newborn photons included (the concurrent one pushes them before its photon pass)
         _lib.check(L.rrea_commit_survivors(drain.survivors.data_ptr(), drain.survivor_count.data_ptr(),
                                            store_structs, counts.data_ptr(), next_id.data_ptr(), 0,
                                            out_e.error.data_ptr(), torch.cuda.current_stream().cuda_stream),
                    "rrea_commit_survivors")
-        dev_g.push(cfg.dt_s)
+        if conc is None:
+            dev_g.push(cfg.dt_s)
 
     def run_step(s, sort_now):
         # the RNG step index continues past the tuning steps

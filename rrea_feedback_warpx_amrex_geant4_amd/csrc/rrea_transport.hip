@@ -4158,6 +4158,19 @@ __global__ void append_count_kernel(int64_t* __restrict__ count, const int64_t* 
     if (threadIdx.x == 0 && blockIdx.x == 0) *count = *base + n;
 }
 
+// A host argument block put into device memory by a one-workgroup kernel
+// that takes it by value: the launch copies it into the kernel's argument
+// segment, so the host block is consumed when the launch returns and no
+// staging copy of pageable memory runs on the stream.
+template <typename T>
+__global__ void put_args_kernel(T* __restrict__ dst, T const value)
+{
+    static_assert(sizeof(T) % sizeof(unsigned long long) == 0 && sizeof(T) <= 4000, "argument block shape");
+    auto const* src = reinterpret_cast<const unsigned long long*>(&value);
+    auto* out = reinterpret_cast<unsigned long long*>(dst);
+    for (unsigned i = threadIdx.x; i < sizeof(T) / sizeof(unsigned long long); i += blockDim.x) out[i] = src[i];
+}
+
 __global__ void set_i64_kernel(int64_t* __restrict__ p, int64_t v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
@@ -4373,15 +4386,22 @@ struct DrainState {
 };
 
 // The step's ordered records are generation 0.
+// One workgroup: the drain's first generation bounds, the lineage counters
+// of the initial entries cleared (fifo_init_kernel takes their maxima) and
+// the staging count cleared for the first generation's children.
 __global__ void drain_begin_kernel(NewbornArgs* __restrict__ N, DrainState* __restrict__ g,
-                                   const int64_t* __restrict__ n_initial, int64_t cap)
+                                   const int64_t* __restrict__ n_initial, int64_t cap,
+                                   unsigned long long* __restrict__ next, int64_t* __restrict__ count)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (blockIdx.x != 0) return;
     int64_t n0 = *n_initial;
     n0 = n0 < 0 ? 0 : (n0 > cap ? cap : n0);
+    for (int64_t i = threadIdx.x; i < n0; i += blockDim.x) next[i] = 0ULL;
+    if (threadIdx.x != 0) return;
     g->gb = 0; g->ge = n0; g->stage_cap = cap - n0; g->cap = cap;
     g->be[0] = n0; g->be[1] = n0;
     N->gb = 0; N->ge = n0; N->S.sec_cap = cap - n0;
+    *count = 0;
 }
 
 // The FIFO's length; a generation left unadvanced after the last captured
@@ -5422,8 +5442,8 @@ rrea_status species_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     if (ps != RREA_OK || !L.run) return ps;
     rrea_fields_t const W = batch ? batch->work : F;
     hipStream_t const s = static_cast<hipStream_t>(stream);
-    // pageable source: the copy has consumed A when hipMemcpyAsync returns
-    HIP_TRY(hipMemcpyAsync(W->d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
+    put_args_kernel<StepArgs><<<dim3(1), dim3(256), 0, s>>>(static_cast<StepArgs*>(W->d_args), A);
+    HIP_TRY(hipGetLastError());
     // order_secondaries 2: the count stays on the device (no host round trip)
     return species_launch(W, static_cast<const StepArgs*>(W->d_args), L, out, s, out->order_secondaries == 2);
 }
@@ -5504,8 +5524,8 @@ rrea_status noalloc_step(int species, rrea_tables_t T, rrea_fields_t F, const rr
     if (ps != RREA_OK || !L.run) return ps;
     hipStream_t const s = static_cast<hipStream_t>(stream);
     StepArgs* const d_args = reinterpret_cast<StepArgs*>(base + kNaArgs);
-    // pageable source: the copy has consumed A when hipMemcpyAsync returns
-    HIP_TRY(hipMemcpyAsync(d_args, &A, sizeof(StepArgs), hipMemcpyHostToDevice, s));
+    put_args_kernel<StepArgs><<<dim3(1), dim3(256), 0, s>>>(d_args, A);
+    HIP_TRY(hipGetLastError());
     return species_launch(nullptr, d_args, L, out, s, true, &K);
 }
 
@@ -5596,11 +5616,9 @@ rrea_status drain_device_launch(rrea_fields_t W, NewbornArgs* d_nargs, DrainStat
     // an empty generation's launch then costs a few microseconds
     unsigned const gnb = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks_for(cap, kBlock), 256)));
     int32_t* const order = reinterpret_cast<int32_t*>(W->d_idx_out);
-    drain_begin_kernel<<<dim3(1), dim3(64), 0, s>>>(d_nargs, d_state, n_initial, cap);
-    fill_u64_kernel<<<dim3(g256), dim3(256), 0, s>>>(W->d_next, 0, 0ULL, &d_state->ge);
+    drain_begin_kernel<<<dim3(1), dim3(1024), 0, s>>>(d_nargs, d_state, n_initial, cap, W->d_next, W->d_count);
     fifo_init_kernel<<<dim3(g256), dim3(256), 0, s>>>(work, 0, &d_state->ge, W->d_anchor, W->d_next, error);
     HIP_TRY(hipGetLastError());
-    fill_u64_kernel<<<dim3(1), dim3(64), 0, s>>>(reinterpret_cast<unsigned long long*>(W->d_count), 1, 0ULL);
     for (int32_t gen = 0; gen < max_gen; ++gen) {
         newborn_kernel<<<dim3(gnb), dim3(kBlock), 0, s>>>(d_nargs);
         link_generation_kernel<<<dim3(1), dim3(kSortThreads), 0, s>>>(
@@ -5675,8 +5693,8 @@ rrea_status drain_impl(rrea_tables_t T, rrea_fields_t F, const rrea_step_config*
         A.sec_cap = cap - ge;
         A.sec_count = W->d_count;
         NewbornArgs N{A, io->work, gb, ge, W->d_fin, W->d_fst, W->d_fion, W->d_fsub};
-        // pageable source: the copy has consumed N when hipMemcpyAsync returns
-        HIP_TRY(hipMemcpyAsync(W->d_nargs, &N, sizeof(NewbornArgs), hipMemcpyHostToDevice, s));
+        put_args_kernel<NewbornArgs><<<dim3(1), dim3(256), 0, s>>>(static_cast<NewbornArgs*>(W->d_nargs), N);
+        HIP_TRY(hipGetLastError());
         {
             hipEvent_t stop = nullptr;
             if (W->timing) {
@@ -6192,8 +6210,8 @@ rrea_status rrea_newborn_drain_staged(rrea_tables_t T, rrea_fields_t F, const rr
     A.sec_cap = 0;  // written on the device per generation
     A.sec_count = W->d_count;
     NewbornArgs N{A, io->work, 0, 0, W->d_fin, W->d_fst, W->d_fion, W->d_fsub};
-    // pageable source: the copy has consumed N when hipMemcpyAsync returns
-    HIP_TRY(hipMemcpyAsync(W->d_nargs, &N, sizeof(NewbornArgs), hipMemcpyHostToDevice, s));
+    put_args_kernel<NewbornArgs><<<dim3(1), dim3(256), 0, s>>>(static_cast<NewbornArgs*>(W->d_nargs), N);
+    HIP_TRY(hipGetLastError());
     StagedLists S{};
     S.n = n_staged;
     int64_t most = 0;

@@ -757,11 +757,18 @@ class ConcurrentApply:
 
     def run(self, electrons: DeviceTile, photons: "PhotonTile", positrons: DeviceTile, cfg: StepConfig,
             out_e: StepOutputs, out_g: StepOutputs, out_p: StepOutputs, drain: "DrainOutputs | None" = None,
-            check: bool = False, push_photons: bool = False, device_drain: int = 0) -> None:
+            check: bool = False, push_photons: bool = False, device_drain: int = 0, pre_push: bool = False) -> None:
         """``push_photons``: WarpX's photon push for the next step
         (rrea_photon_push) right after the photon pass, on its stream, so it
         runs under the other passes instead of after the drain (nothing
-        after the photon pass reads the photon tile)."""
+        after the photon pass reads the photon tile).  ``pre_push``: the push
+        of this step instead, on the photon stream right before the photon
+        pass, so it runs with that pass (under the electron hand-over unless
+        the photon pass is early; nothing before the photon pass reads the
+        photon tile); photons the caller commits
+        after the drain are then pushed by the next step's run."""
+        if push_photons and pre_push:
+            raise ValueError("push_photons and pre_push each push the photons once per step: choose one")
         main = torch.cuda.current_stream()
         ready = torch.cuda.Event()
         ready.record(main)
@@ -772,6 +779,8 @@ class ConcurrentApply:
         L = _lib.lib()
 
         def photon_pass():
+            if pre_push:
+                photons.push(cfg.dt_s, self.streams[0].cuda_stream)
             apply.apply_photons(self.tables, cg, photons, cfg, out_g, 2, self.streams[0].cuda_stream, False)
             if push_photons:
                 photons.push(cfg.dt_s, self.streams[0].cuda_stream)

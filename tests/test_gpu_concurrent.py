@@ -22,17 +22,18 @@ def tables(bundle):
     return InteractionTables(bundle)
 
 
-# (4, False, positron early) is the bench's default: the positron pass starts
-# with the electron pass, the photon pass after the electron fast kernel, the
-# drain with its counts on the device (four generations), then the photon
-# push on the main stream after the drain; (0, False, none) is the
-# host-driven drain on the electron pass's handle (whose ordering workspace
-# the electron pass also uses) with both passes after the electron fast
-# kernel; (16, True, both early) drains on the device and pushes the photons
-# on the photon stream.
-@pytest.mark.parametrize("device_drain,push_photons,early", [(4, False, ("positron",)), (0, False, ()),
-                                                             (16, True, ("positron", "photon"))])
-def test_concurrent_apply_matches_sequential(tables, device_drain, push_photons, early):
+# (4, "pre", positron early) is the bench's default: the photon push of the
+# step at the head of the photon stream, the positron pass starting with the
+# electron pass, the photon pass after the electron fast kernel, the drain
+# with its counts on the device (four generations); (4, "after", positron
+# early) pushes the photons on the main stream after the drain; (0, "after",
+# none) is the host-driven drain on the electron pass's handle (whose
+# ordering workspace the electron pass also uses) with both passes after the
+# electron fast kernel; (16, "stream", both early) drains on the device and
+# pushes the photons for the next step on the photon stream after its pass.
+@pytest.mark.parametrize("device_drain,push,early", [(4, "pre", ("positron",)), (4, "after", ("positron",)),
+                                                     (0, "after", ()), (16, "stream", ("positron", "photon"))])
+def test_concurrent_apply_matches_sequential(tables, device_drain, push, early):
     from rrea_feedback_warpx_amrex_geant4_amd.transport import (ConcurrentApply, Coupling, DeviceTile, DrainOutputs,
                                                                 PhotonTile, RreaParticleInteraction, StepOutputs)
     fields = wl.c2_fields()
@@ -41,7 +42,7 @@ def test_concurrent_apply_matches_sequential(tables, device_drain, push_photons,
     g = wl.c2_photons(n // 2, fields, dt_s=1e-9, seed=4, first_id=1 << 41)
     p = wl.c2_electrons(n // 4, fields, seed=3, first_id=1 << 40, charge_sign=1.0)
     print(f"case concurrent apply: {n} e-, {n // 2} gamma, {n // 4} e+, 3 steps of 1 ns, "
-          f"device_drain={device_drain}, push_photons={push_photons}, early={early}", flush=True)
+          f"device_drain={device_drain}, push={push}, early={early}", flush=True)
 
     def make():
         tiles = (DeviceTile.from_host(e), PhotonTile.from_host(g), DeviceTile.from_host(p))
@@ -62,15 +63,18 @@ def test_concurrent_apply_matches_sequential(tables, device_drain, push_photons,
                 o.secondary_count.zero_()
             for t in (outs[0].jr, outs[0].jz, outs[0].rho, outs[0].sources):
                 t.zero_()
+        if push == "pre":
+            seq_t[1].push(cfg.dt_s)
         apply.apply_electrons(tables, coupling, seq_t[0], cfg, seq_o[0])
         apply.apply_photons(tables, coupling, seq_t[1], cfg, seq_o[1])
         apply.apply_positrons(tables, coupling, seq_t[2], cfg, seq_o[2])
         apply.drain_secondaries(tables, coupling, cfg, list(seq_o), seq_d)
         conc.run(con_t[0], con_t[1], con_t[2], cfg, con_o[0], con_o[1], con_o[2], con_d, check=True,
-                 push_photons=push_photons, device_drain=device_drain)
-        if not push_photons:
+                 push_photons=push == "stream", pre_push=push == "pre", device_drain=device_drain)
+        if push == "after":
             con_t[1].push(cfg.dt_s)
-        seq_t[1].push(cfg.dt_s)
+        if push != "pre":
+            seq_t[1].push(cfg.dt_s)
         torch.cuda.synchronize()
         for a, b in ((seq_t[0], con_t[0]), (seq_t[1].cur, con_t[1].cur), (seq_t[2], con_t[2])):
             for k in ("r", "z", "theta", "ux", "uy", "uz", "w", "id"):
