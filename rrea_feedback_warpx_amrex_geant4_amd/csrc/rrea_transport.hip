@@ -2537,11 +2537,14 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
     int64_t const ip = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
     bool slow = false;
     bool light = false;  // a short complete path: below the cutoff from the start, or at the demotion gate
-    // the survivor's net chord, deposited by the whole workgroup at the end
+    // the survivor's net chord, deposited by the whole workgroup at the end;
+    // the chord and its ion-pair source record are set with dep and read only
+    // where it is set, so they carry no value on the paths that leave the
+    // fast branch early (no zeroing of them at each of its exits)
     bool dep = false;
     int32_t dep_inst = -1;
-    double dep_q = 0.0, dep_r0 = 0.0, dep_z0 = 0.0, dep_r1 = 0.0, dep_z1 = 0.0;
-    SourceCell src{0.0, 0.0, 0.0, 0, 0};  // its ion-pair source record
+    double dep_q, dep_r0, dep_z0, dep_r1, dep_z1;
+    SourceCell src;
     // The LDS window of the deposits at the end; until then it holds each
     // lane's tallies and ledger records in columns of its own (column q of
     // the lane at window[q * kB + lane]), folded per workgroup at the end:
@@ -2770,6 +2773,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             // ---- commit: nothing above has written anything ----
             slow = false;
             dep = true;
+            src.w = 0.0;  // no record unless one is formed below
             if (ion > 0.0 && E.src) {
                 // the ion pairs' record at the interval midpoint (the one
                 // interval ends where the substep does: iv.end == se here),
