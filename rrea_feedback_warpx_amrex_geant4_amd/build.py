@@ -25,6 +25,11 @@ FLAGS = [
     "-ffp-contract=off",
     # packed FP32 instructions stay off
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no two-address FP64 fmac: its addend must sit in the destination VGPR,
+    # so every Horner step of the inlined libm polynomials paid two VALU
+    # moves for its constant; the three-address fma reads it from SGPRs
+    # (the same fused operation, rounded once: results do not change)
+    "-Xclang", "-target-feature", "-Xclang", "-fmacf64-inst",
     "-Wno-unused-command-line-argument",
     "-I" + os.path.join(ROOT, "include"),
 ]
