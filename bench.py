@@ -577,13 +577,14 @@ def column_leg(args, tables):
 def valu_record():
     """The electron kernel's VALU issue from the committed rocprofv3 SQ
     counter record (profiles/electron_kernel_valu.json), or None: VALU
-    instructions per electron-step and the share of the launch's SIMD
-    cycles they occupy -- the ceiling that binds this kernel."""
+    instructions per electron-step, the share of the launch's SIMD cycles
+    they occupy, and how a wave's cycles split between waiting on memory or
+    LDS, waiting for issue and issuing."""
     try:
         with open(VALU_RECORD) as f:
             rec = json.load(f)
         return {k: rec[k] for k in ("valu_insts_per_particle_step", "valu_issue_frac", "waves_per_simd",
-                                    "vgprs", "source") if k in rec}
+                                    "vgprs_compiler", "wave_cycle_split", "source") if k in rec}
     except (OSError, KeyError, ValueError):
         return None
 
