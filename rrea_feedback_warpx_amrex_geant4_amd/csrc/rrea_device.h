@@ -383,6 +383,33 @@ __device__ __forceinline__ bool charged_bracket(DevTables const& T, double q, Br
     return true;
 }
 
+// charged_bracket for an energy q near one already bracketed: when q lies in
+// (x[lo], x[hi]] of `near` (an interior or exact-node bracket), lower_bound
+// is near's hi, so the bracket follows from those two nodes with the same
+// arithmetic and no guided search (one round of loads instead of the
+// guide's and the search's); otherwise the full search.
+__device__ __forceinline__ bool charged_bracket_near(DevTables const& T, double q, Bracket const& near, Bracket& b)
+{
+    if (near.kind == 0 || near.kind == 3) {
+        int32_t const lo = near.lo, hi = near.hi;
+        double const xlo = T.cx[lo], xhi = T.cx[hi];
+        double const llo = T.clogx[lo], lhi = T.clogx[hi];
+        if (xlo < q && q <= xhi) {
+            b.lo = lo;
+            b.hi = hi;
+            b.t = 0.0;
+            b.lin_t = 0.0;
+            if (xhi == q) { b.kind = 3; return true; }
+            double const lq = log(q);
+            b.kind = 0;
+            b.t = (lq - llo) / (lhi - llo);
+            b.lin_t = (q - xlo) / (xhi - xlo);
+            return true;
+        }
+    }
+    return charged_bracket(T, q, b);
+}
+
 // The same bracket on the photon rate grid, whose duplicated absorption-edge
 // nodes resolve to the low side at an exact edge and to the high side just
 // above it (lower_bound + node-exact test).
@@ -483,6 +510,31 @@ __device__ __forceinline__ bool elastic_bracket(DevElastic const& E, double q, E
     b.kind = 0;
     b.t = (lq - E.log_e[b.lo]) / (E.log_e[b.hi] - E.log_e[b.lo]);
     return true;
+}
+
+// elastic_bracket for an energy q near one already bracketed, as
+// charged_bracket_near: q in (e[lo], e[hi]) of an interior or exact-node
+// `near` gives its nodes; q == e[hi] is that node (the last node's own kind
+// when hi is the last); otherwise the full search.
+__device__ __forceinline__ bool elastic_bracket_near(DevElastic const& E, double q, EBracket const& near, EBracket& b)
+{
+    if (near.kind == 0 || near.kind == 3) {
+        int32_t const lo = near.lo, hi = near.hi;
+        double const xlo = E.e[lo], xhi = E.e[hi];
+        double const llo = E.log_e[lo], lhi = E.log_e[hi];
+        if (xlo < q && q <= xhi) {
+            b.lo = 0; b.hi = 0; b.kind = 1; b.t = 0.0;
+            if (q == xhi && hi == E.n - 1) { b.kind = 2; return true; }
+            b.hi = hi;
+            b.lo = lo;
+            if (q == xhi) { b.kind = 3; return true; }
+            double const lq = log(q);
+            b.kind = 0;
+            b.t = (lq - llo) / (lhi - llo);
+            return true;
+        }
+    }
+    return elastic_bracket(E, q, b);
 }
 
 __device__ __forceinline__ double elastic_value(DevElastic const& E, const double* y, EBracket const& b)

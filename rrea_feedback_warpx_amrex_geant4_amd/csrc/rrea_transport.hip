@@ -2687,7 +2687,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             if (!(k1 >= cutoff) || !(0.0 < se)) break;
             // the first hazard attempt: no event
             Bracket b1;
-            if (!charged_bracket(T, k1, b1)) break;
+            if (!charged_bracket_near(T, k1, b0, b1)) break;
             double const L = density_path(iv, path, 0.0, se);
             uint64_t const rng = rng_prefix(E.rng_seed, static_cast<uint64_t>(id < 0 ? -id : id)
                                                 ^ (static_cast<uint64_t>(static_cast<uint32_t>(A.cpu ? A.cpu[ip] : 0)) << 40),
@@ -2742,7 +2742,7 @@ charged_fast_kernel(const StepArgs* __restrict__ args, int32_t* __restrict__ slo
             if (!(k2 >= cutoff)) break;
             // elastic scattering: the few-collision branch with at most two scatters
             EBracket eb2;
-            if (!elastic_bracket(EL, k2, eb2)) break;
+            if (!elastic_bracket_near(EL, k2, eb0, eb2)) break;
             double const mfp2 = elastic_value(EL, EL.mfp, eb2) / 1.0;
             double const gs_n = (mfp2 > 0.0 ? 1.0 / mfp2 : 0.0) * L;
             if (gs_n > 0.0) {
