@@ -1136,7 +1136,8 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                 // Substep sizing, OV/RreaElectronTransport.cpp:187-279 /
                 // OV/RreaPositronTransport.cpp:177-245
                 Bracket b0;
-                if (!charged_bracket(T, k, b0)) err = kErrRange;
+                bool const b0_ok = charged_bracket(T, k, b0);
+                if (!b0_ok) err = kErrRange;
                 double s00, s01, s02 = 0.0;
                 if constexpr (kPos) positron_stp_rates(T, b0, thr, gcut, k, s00, s01, s02, err);
                 else electron_stp_rates(T, b0, thr, gcut, k, hm_on, s00, s01, err);
@@ -1179,8 +1180,10 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                 {
                     double const gs_n_total = elastic_inverse_length(EL, k, err) * ddp;
                     double const kc = dmax(k, cutoff);
-                    Bracket bc;
-                    if (!charged_bracket(T, kc, bc)) err = kErrRange;
+                    // kc is k unless k is below the cutoff: then b0 is its
+                    // bracket (the same search on the same energy)
+                    Bracket bc = b0;
+                    if (!(kc == k ? b0_ok : charged_bracket(T, kc, bc))) err = kErrRange;
                     double drag_est;
                     if constexpr (kPos) {
                         // PositronCollisionLoss only
