@@ -1308,6 +1308,9 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                 FieldRange sfield{0, 0, 0, 0, 0, 0, 0, 0};
                 double substep_dt = 0.0;
                 // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:686-732
+                // the last event-interleave bracket: the next one (the energy
+                // after a kick, drag or event) is usually in its interval
+                Bracket hint{0, 0, 1, 0.0, 0.0};
                 for (uint64_t sc = 0; sc < subcycles && !err; ++sc) {
                     if (k < cutoff) break;
                     double sb = 0.0, se = 0.0;
@@ -1339,7 +1342,8 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                             if (ord >= kEventCap) { err = kErrInvalid; break; }
                             double const frozen_e = k;
                             Bracket bf;
-                            if (!charged_bracket(T, frozen_e, bf)) { err = kErrRange; break; }
+                            if (!charged_bracket_near(T, frozen_e, hint, bf)) { err = kErrRange; break; }
+                            hint = bf;
                             double s0, s1, s2 = 0.0;
                             if constexpr (kPos) positron_stp_rates(T, bf, thr, gcut, frozen_e, s0, s1, s2, err);
                             else electron_stp_rates(T, bf, thr, gcut, frozen_e, hm_on, s0, s1, err);
@@ -3336,6 +3340,9 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                 FieldRange sfield{0, 0, 0, 0, 0, 0, 0, 0};
                 double substep_dt = 0.0;
                 // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:693-732
+                // the last event-interleave bracket: the next one (the energy
+                // after a kick, drag or event) is usually in its interval
+                Bracket hint{0, 0, 1, 0.0, 0.0};
                 for (uint64_t sc = 0; sc < substeps && !err; ++sc) {
                     if (k < cutoff) break;
                     double sb = 0.0, se = 0.0;
@@ -3372,7 +3379,8 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                             if (ord >= kEventCap) { err = kErrInvalid; break; }
                             double const frozen_e = k;
                             Bracket bf;
-                            if (!charged_bracket(T, frozen_e, bf)) { err = kErrRange; break; }
+                            if (!charged_bracket_near(T, frozen_e, hint, bf)) { err = kErrRange; break; }
+                            hint = bf;
                             double s0, s1, s2 = 0.0;
                             if constexpr (kPos) positron_stp_rates(T, bf, thr, gcut, frozen_e, s0, s1, s2, err);
                             else electron_stp_rates(T, bf, thr, gcut, frozen_e, hm_on, s0, s1, err);
