@@ -1239,11 +1239,14 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                 // apply_electron_continuous_segment OV/RreaElectronTransport.cpp:294-400 /
                 // apply_positron_continuous_segment OV/RreaPositronTransport.cpp:278-360 via
                 // RreaPlanContinuousDragSegment OV/RreaChargedInterleave.H:239-292
+                // the last event-interleave bracket: the next one (the energy
+                // after a kick, drag or event) is usually in its interval; the drag segment brackets the same energy
+                Bracket hint{0, 0, 1, 0.0, 0.0};
                 auto continuous = [&](double b, double e, double stop_e, double& eff) -> bool {
                     eff = b;
                     if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return false;
                     Bracket bs;
-                    if (!charged_bracket(T, stop_e, bs)) { if (!err) err = kErrRange; return false; }
+                    if (!charged_bracket_near(T, stop_e, hint, bs)) { if (!err) err = kErrRange; return false; }
                     double const col = charged_value(T, S::kCol, bs, err) * 1.0;
                     double const soft = charged_value(T, S::kSoft, bs, err) * 1.0;
                     double const total = col + soft;
@@ -1308,9 +1311,6 @@ __device__ __noinline__ void charged_full_particle(StepArgs const& A, int64_t ip
                 FieldRange sfield{0, 0, 0, 0, 0, 0, 0, 0};
                 double substep_dt = 0.0;
                 // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:686-732
-                // the last event-interleave bracket: the next one (the energy
-                // after a kick, drag or event) is usually in its interval
-                Bracket hint{0, 0, 1, 0.0, 0.0};
                 for (uint64_t sc = 0; sc < subcycles && !err; ++sc) {
                     if (k < cutoff) break;
                     double sb = 0.0, se = 0.0;
@@ -3272,11 +3272,14 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                 };
                 double positron_total_loss = 0.0;
                 // RreaPlanContinuousDragSegment OV/RreaChargedInterleave.H:239-292
+                // the last event-interleave bracket: the next one (the energy
+                // after a kick, drag or event) is usually in its interval; the drag segment brackets the same energy
+                Bracket hint{0, 0, 1, 0.0, 0.0};
                 auto continuous = [&](double b, double e, double stop_e, double& eff) -> bool {
                     eff = b;
                     if (!(e > b) || !(k > 0.0) || stop_e < cutoff) return false;
                     Bracket bs;
-                    if (!charged_bracket(T, stop_e, bs)) { if (!err) err = kErrRange; return false; }
+                    if (!charged_bracket_near(T, stop_e, hint, bs)) { if (!err) err = kErrRange; return false; }
                     double const col = charged_value(T, S::kCol, bs, err) * 1.0;
                     double const soft = charged_value(T, S::kSoft, bs, err) * 1.0;
                     double const total = col + soft;
@@ -3340,9 +3343,6 @@ __device__ __noinline__ void charged_newborn(NewbornArgs const& N, int64_t i, do
                 FieldRange sfield{0, 0, 0, 0, 0, 0, 0, 0};
                 double substep_dt = 0.0;
                 // RreaRunChargedSubstepInterleave OV/RreaChargedInterleave.H:693-732
-                // the last event-interleave bracket: the next one (the energy
-                // after a kick, drag or event) is usually in its interval
-                Bracket hint{0, 0, 1, 0.0, 0.0};
                 for (uint64_t sc = 0; sc < substeps && !err; ++sc) {
                     if (k < cutoff) break;
                     double sb = 0.0, se = 0.0;
