@@ -79,3 +79,21 @@ def test_tune_block_picks_the_cheapest_and_leaves_it_set():
     assert [t[0] for t in c.trials] == [128, 256, 512]
     assert state["sets"] == [128, 256, 512, 256]
     assert c.steps_run == 15
+
+
+def test_tune_block_keeps_the_default_within_the_margin():
+    # 128 is timed faster than 256 by far less than the margin
+    state = {"block": 0}
+
+    def set_block(b):
+        state["block"] = b
+
+    times = iter([0.0, 1.0, 1.0, 2.0 + 1e-9, 2.0 + 1e-9, 4.0])
+    orig = autotune.time.perf_counter
+    autotune.time.perf_counter = lambda: next(times)
+    try:
+        c = autotune.tune_block(lambda s: None, set_block, blocks=(128, 256, 512), steps=1, warmup=0)
+    finally:
+        autotune.time.perf_counter = orig
+    assert c.block == 256 and state["block"] == 256
+    assert c.trials[0][1] < c.trials[1][1]
